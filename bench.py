@@ -1,0 +1,179 @@
+"""Benchmark of the path-tracing core on one MI355X.
+
+Workload (default): cornell_mesh -- Cornell room + 81,920-triangle icosphere
+with one OpenPBR material, 4 pinhole cameras 640x480, 16 spp, depth 8; one
+step = one frame of all four cameras through the wavefront pipeline.  The
+scene, BVH and buffers are resident in HBM before the timed region.
+
+Prints one JSON line: Mpaths/s (whole job), ms per frame, the roofline of
+the dominant kernel (closest-hit traversal, algorithmic bytes from measured
+node visits / primitive tests) and the CPU oracle timed on this host on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# canonical unit sizes for algorithmic traffic (bytes)
+RAY_B, HIT_B, NODE_B, PRIM_IDX_B, TRI_B, SPHERE_B = 32, 16, 32, 4, 36, 16
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(config: str, rows: int, threads: int) -> dict:
+    """Times the CPU oracle on `rows` rows of camera 0 of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ctypes as C
+
+    import oracle_ctypes as O
+    import ray_tracing_amd as rt
+
+    g = rt.Generated(config, 0)
+    desc = g.desc
+    s = O.Scene()
+    for name in ("sphere_count", "quad_count", "triangle_count", "material_count",
+                 "openpbr_count", "analytic_light_count", "background_mode"):
+        setattr(s, name, getattr(desc, name))
+    s.background = O.f3(*desc.background)
+    s.background_top = O.f3(*desc.background_top)
+    s.spheres = C.cast(desc.spheres, C.POINTER(O.Sphere))
+    s.quads = C.cast(desc.quads, C.POINTER(O.Quad))
+    s.triangles = C.cast(desc.triangles, C.POINTER(O.Triangle))
+    s.materials = C.cast(desc.materials, C.POINTER(O.Material))
+    s.openpbr = C.cast(desc.openpbr, C.POINTER(O.OpenPbr))
+    s.analytic_lights = C.cast(desc.analytic_lights, C.POINTER(O.AnalyticLight))
+    bvh = O.OracleBvh(s)
+    cam = O.Camera()
+    C.memmove(C.byref(cam), C.byref(g.g.cameras[0]), C.sizeof(cam))
+    prof = O.Profile()
+    C.memmove(C.byref(prof), C.byref(g.g.profile), C.sizeof(prof))
+    r0 = max(0, cam.height // 2 - rows // 2)
+    t0 = time.perf_counter()
+    out = O.render_camera(s, bvh, cam, 0, prof, 0, threads=threads, rows=(r0, r0 + rows))
+    dt = time.perf_counter() - t0
+    paths = out["paths"]
+    return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": f"{config} camera 0 rows {r0}..{r0 + rows} ({paths} paths, "
+                      f"{prof.samples_per_pixel} spp, depth {prof.max_bounces}) in {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="cornell_mesh")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=0)
+    ap.add_argument("--wavefront-paths", type=int, default=0)
+    ap.add_argument("--trace-block", type=int, default=0)
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this path tracer shards by camera with no "
+              "collectives, and multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import ray_tracing_amd as rt
+
+    dev = rt.Device(0)
+    info = dev.info()
+    g = rt.Generated(a.config, 0)
+    scene = rt.Scene(dev, g.desc)
+    bst = scene.build_bvh()
+    ren = rt.Renderer(dev, scene)
+    tune = {}
+    if a.wavefront_paths:
+        tune["wavefront_paths"] = a.wavefront_paths
+    if a.trace_block:
+        tune["trace_block"] = a.trace_block
+    if a.blocks_per_cu:
+        tune["trace_blocks_per_cu"] = a.blocks_per_cu
+    if tune:
+        ren.set_tuning(**tune)
+    rig = g.rig()
+    prof = g.profile
+    paths_per_frame = sum(rig.cameras[i].width * rig.cameras[i].height
+                          for i in range(rig.active_count)) * prof.samples_per_pixel
+    log(f"[bench] {info['name']} CUs={info['cu_count']} config={a.config} prims={scene.prim_count} "
+        f"bvh depth={bst.max_depth} build={bst.build_ms:.2f} ms paths/frame={paths_per_frame}")
+
+    # traversal counts for the algorithmic-bytes roofline (separate, untimed frame)
+    ren.set_tuning(count_traversal=1)
+    cnt = ren.render(rig, prof, 0)
+    ren.set_tuning(count_traversal=0)
+    nonsphere = cnt.prim_tests - cnt.sphere_tests
+    extend_bytes = (cnt.extend_rays * (RAY_B + HIT_B) + cnt.node_visits * NODE_B
+                    + nonsphere * (PRIM_IDX_B + TRI_B) + cnt.sphere_tests * (PRIM_IDX_B + SPHERE_B))
+    log(f"[bench] counting frame: rays={cnt.extend_rays} visits/ray="
+        f"{cnt.node_visits / max(cnt.extend_rays, 1):.2f} tests/ray="
+        f"{cnt.prim_tests / max(cnt.extend_rays, 1):.2f}")
+
+    for i in range(a.warmup):
+        ren.render(rig, prof, 1000 + i)
+    dev.synchronize()
+    frames = []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        frames.append(ren.render(rig, prof, i))
+        log(f"[bench] step {i} render_ms={frames[-1].render_ms:.3f}")
+    dev.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / a.steps
+    value = paths_per_frame * a.steps / wall / 1e6
+    ext_ms = sum(f.extend_ms for f in frames) / a.steps
+    launches = frames[0].extend_launches
+    avg_launch_ms = ext_ms / max(launches, 1)
+    achieved = extend_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
+    gpu_ms = sum(f.render_ms for f in frames) / a.steps
+
+    result = {
+        "metric": "Mpaths/s & ms/frame 4-cam; plain vs learned path at equal relMSE",
+        "value": round(value, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{a.config}: plain wavefront tracer, 4x640x480 pinhole, "
+                               f"{prof.samples_per_pixel} spp, depth {prof.max_bounces}",
+                   "cameras": rig.active_count, "spp": prof.samples_per_pixel,
+                   "max_bounces": prof.max_bounces, "paths_per_frame": paths_per_frame,
+                   "gpu_ms_per_frame": round(gpu_ms, 3),
+                   "extend_ms_per_frame": round(ext_ms, 3),
+                   "shade_ms_per_frame": round(sum(f.shade_ms for f in frames) / a.steps, 3),
+                   "connect_ms_per_frame": round(sum(f.connect_ms for f in frames) / a.steps, 3),
+                   "extend_rays_per_frame": cnt.extend_rays,
+                   "shadow_rays_per_frame": cnt.shadow_rays,
+                   "tuning": ren.tuning()},
+        "roofline": {"bound": "hbm", "kernel": "k_extend (closest-hit traversal)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": int(extend_bytes / max(launches, 1)),
+                     "avg_launch_ms": round(avg_launch_ms, 4), "launches_per_frame": launches},
+    }
+    if not a.no_cpu_baseline:
+        threads = os.cpu_count() or 1
+        threads = min(threads, 16)
+        rows = a.cpu_rows or 48
+        result["cpu_baseline"] = cpu_baseline(a.config, rows, threads)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,279 @@
+/*
+ * rt_hip.h -- C ABI of the MI355X (gfx950) path-tracing core.
+ *
+ * Plain C: POD structs, opaque handles, explicit sizes, rt_status return
+ * codes and a thread-local rt_last_error().  No HIP or C++ types cross this
+ * boundary.  Each entry point names the reference interface it replaces
+ * (paths relative to the reference repository root).
+ *
+ * Threading: handles are independent; calls on one handle must not overlap.
+ * Ownership: handles own their device memory; callers own every host array.
+ */
+#ifndef RT_HIP_H
+#define RT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT_ABI_VERSION 1
+
+typedef enum rt_status {
+    RT_OK = 0,
+    RT_ERROR_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+    RT_ERROR_HIP = 2,              /* "CUDA runtime failure at ..." in the reference */
+    RT_ERROR_NOT_READY = 3,        /* e.g. render before prepare_scene() */
+    RT_ERROR_OUT_OF_MEMORY = 4,
+    RT_ERROR_UNSUPPORTED = 5
+} rt_status;
+
+/* ---------------------------------------------------------------- scene
+ * Flattened world-space primitives, as PackedScene after
+ * adapt_scene_ir_v2_to_realtime (src/realtime/scene_description.h:130-146,
+ * src/scene/realtime_scene_adapter.cpp:371-487).  Global primitive ids are
+ * spheres, then quads, then triangles. */
+typedef struct rt_sphere { float center[3]; float radius; int32_t material; } rt_sphere;
+typedef struct rt_quad { float origin[3]; float edge_u[3]; float edge_v[3]; int32_t material; } rt_quad;
+typedef struct rt_triangle { float p0[3]; float p1[3]; float p2[3]; int32_t material; } rt_triangle;
+
+/* material types of MaterialSample (src/realtime/gpu/launch_params.h:216-223) */
+enum { RT_MAT_LAMBERTIAN = 0, RT_MAT_METAL = 1, RT_MAT_DIELECTRIC = 2, RT_MAT_DIFFUSE_LIGHT = 3,
+       RT_MAT_ISOTROPIC = 4, RT_MAT_OPENPBR = 5 };
+
+typedef struct rt_material {
+    int32_t type;
+    float albedo[3];
+    float emission[3];
+    float fuzz;
+    float ior;
+    int32_t openpbr_index;
+} rt_material;
+
+/* OpenPbrCoreMaterial, src/common/openpbr_core.h:33-77 (same field order) */
+typedef struct rt_openpbr_material {
+    float base_weight; float base_color[3]; float base_diffuse_roughness; float base_metalness;
+    float specular_weight; float specular_color[3]; float specular_roughness; float specular_ior;
+    float specular_roughness_anisotropy;
+    float transmission_weight; float transmission_color[3]; float transmission_depth;
+    float transmission_dispersion_scale; float transmission_dispersion_abbe_number;
+    float subsurface_weight; float subsurface_color[3]; float subsurface_radius;
+    float subsurface_radius_scale[3]; float subsurface_scatter_anisotropy;
+    float fuzz_weight; float fuzz_color[3]; float fuzz_roughness;
+    float coat_weight; float coat_color[3]; float coat_roughness; float coat_roughness_anisotropy;
+    float coat_ior; float coat_darkening;
+    float thin_film_weight; float thin_film_thickness; float thin_film_ior;
+    float emission_luminance; float emission_color[3];
+    float geometry_opacity; int32_t geometry_thin_walled;
+} rt_openpbr_material;
+
+/* PackedAnalyticLight subset (src/common/light_sampling.h:44-62): sphere lights */
+enum { RT_ALIGHT_SPHERE = 0 };
+typedef struct rt_analytic_light {
+    int32_t type;
+    float position[3];
+    float radius;
+    float radiance[3];
+} rt_analytic_light;
+
+enum { RT_BG_CONSTANT = 0, RT_BG_SKY_GRADIENT = 1 };
+
+typedef struct rt_scene_desc {
+    int32_t sphere_count, quad_count, triangle_count, material_count;
+    int32_t openpbr_count, analytic_light_count;
+    int32_t background_mode;
+    float background[3];     /* constant colour, or gradient colour at d.y = -1 */
+    float background_top[3]; /* gradient colour at d.y = +1 */
+    const rt_sphere* spheres;
+    const rt_quad* quads;
+    const rt_triangle* triangles;
+    const rt_material* materials;
+    const rt_openpbr_material* openpbr;
+    const rt_analytic_light* analytic_lights;
+} rt_scene_desc;
+
+/* -------------------------------------------------------------- cameras
+ * DeviceActiveCamera (src/realtime/gpu/launch_params.h:63-98): basis_* are
+ * the columns of T_rc's rotation (renderer <- camera, OpenCV axes). */
+enum { RT_CAM_PINHOLE32 = 0, RT_CAM_EQUI62_LUT1D = 1 };
+typedef struct rt_camera {
+    int32_t enabled, width, height, model;
+    double origin[3], basis_x[3], basis_y[3], basis_z[3];
+    double fx, fy, cx, cy;
+    double k1, k2, k3, p1, p2;
+    double radial[6], tangential[2];
+    double lut_step;
+    double lut[1024];
+} rt_camera;
+
+/* PackedCameraRig, src/realtime/camera_rig.h:31-34 */
+typedef struct rt_camera_rig {
+    int32_t active_count;
+    int32_t pad;
+    rt_camera cameras[4];
+} rt_camera_rig;
+
+/* RenderProfile, src/realtime/render_profile.h:10-25, plus two extensions */
+typedef struct rt_profile {
+    int32_t samples_per_pixel, max_bounces, enable_denoise, rr_start_bounce;
+    double accumulation_reset_rotation_deg, accumulation_reset_translation;
+    int32_t enable_restir_di, restir_initial_candidates, restir_temporal_reuse,
+        restir_max_history_age, restir_max_temporal_candidates, restir_spatial_neighbors,
+        restir_max_spatial_candidates, restir_bias_correction, restir_min_analytic_lights;
+    int32_t pixel_jitter;          /* extension: jitter primary rays inside the pixel */
+    int32_t restir_spatial_radius; /* extension: spatial reuse radius in pixels */
+    int32_t pad0;
+} rt_profile;
+
+/* ------------------------------------------------------------------ BVH
+ * Binary Morton LBVH (Karras 2012) over all primitives.  Node = 64 bytes:
+ * the boxes of both children followed by the two child references, the
+ * parent index and padding.  A reference with bit 31 set is a leaf whose low
+ * 31 bits index the Morton-sorted primitive order; 0xFFFFFFFF is empty. */
+typedef struct rt_bvh_node {
+    float lo0[3]; float hi0[3];
+    float lo1[3]; float hi1[3];
+    uint32_t child[2];
+    uint32_t parent;
+    uint32_t pad;
+} rt_bvh_node;
+
+typedef struct rt_bvh_stats {
+    int32_t prim_count;
+    int32_t node_count;
+    int32_t max_depth;
+    int32_t pad;
+    double build_ms; /* device time of the build kernels */
+} rt_bvh_stats;
+
+typedef struct rt_ray { float o[3]; float tmin; float d[3]; float tmax; } rt_ray;
+typedef struct rt_hit { float t; int32_t prim; float u; float v; } rt_hit;
+
+/* ---------------------------------------------------------------- frames */
+/* RadianceTiming, src/realtime/gpu/optix_renderer.h:26-30, plus GPU counters */
+typedef struct rt_frame_timing {
+    double render_ms;      /* device time of the whole frame (all cameras) */
+    double denoise_ms;
+    double download_ms;
+    double extend_ms;      /* closest-hit stage, summed over launches */
+    double shade_ms;
+    double connect_ms;     /* any-hit (shadow) stage */
+    double other_ms;       /* raygen + accumulate + resolve */
+    uint64_t paths;        /* camera paths started */
+    uint64_t extend_rays;  /* closest-hit rays traced */
+    uint64_t shadow_rays;  /* any-hit rays traced */
+    uint64_t extend_launches;
+    uint64_t node_visits;  /* closest-hit BVH node fetches (count_traversal only) */
+    uint64_t prim_tests;   /* closest-hit primitive tests (count_traversal only) */
+    uint64_t sphere_tests; /* of which sphere tests */
+} rt_frame_timing;
+
+/* DeviceRadianceFrameView, src/realtime/gpu/device_radiance_frame_view.h */
+typedef struct rt_device_frame_view {
+    int32_t width, height;
+    const float* beauty; /* device float4 per pixel */
+    const float* normal;
+    const float* albedo;
+    const float* depth;  /* device float per pixel */
+} rt_device_frame_view;
+
+/* RestirDiagnostics (src/realtime/gpu/optix_renderer.cpp:183-215) */
+typedef struct rt_restir_stats {
+    uint64_t valid_reservoirs;
+    uint64_t temporal_candidates;
+    uint64_t spatial_candidates;
+    uint64_t bias_correction_sources;
+} rt_restir_stats;
+
+typedef struct rt_device rt_device;
+typedef struct rt_scene rt_scene;
+typedef struct rt_renderer rt_renderer;
+
+/* ------------------------------------------------------------------- API */
+int rt_abi_version(void);
+/* thread-local message of the last failing call on this thread */
+const char* rt_last_error(void);
+
+/* replaces the device set-up of OptixRenderer::OptixRenderer
+ * (src/realtime/gpu/optix_renderer.cpp:118-138) */
+rt_status rt_device_create(int ordinal, rt_device** out);
+rt_status rt_device_destroy(rt_device* dev);
+rt_status rt_device_synchronize(rt_device* dev);
+/* name, CU count, HBM bytes; any pointer may be NULL */
+rt_status rt_device_info(rt_device* dev, char* name, size_t name_len, int* cu_count,
+    uint64_t* total_bytes, uint64_t* free_bytes);
+
+/* replaces prepare_gpu_scene + DeviceSceneBuffers::upload
+ * (src/realtime/gpu/packed_scene_preparation.cpp:306-348,
+ *  src/realtime/gpu/device_scene_buffers.cpp:58-95) */
+rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* desc, rt_scene** out);
+rt_status rt_scene_destroy(rt_scene* scene);
+
+/* replaces GpuSceneAcceleration::rebuild (the CPU median split of
+ * src/realtime/gpu/gpu_scene_acceleration.cpp:145-179,314-346) with a GPU LBVH */
+rt_status rt_bvh_build(rt_scene* scene, rt_bvh_stats* stats);
+/* copies node_count nodes, prim_count leaf primitive ids and Morton codes;
+ * any pointer may be NULL */
+rt_status rt_bvh_download(rt_scene* scene, rt_bvh_node* nodes, uint32_t* sorted_prims,
+    uint32_t* morton);
+
+/* test hooks replacing intersect_scene / is_occluded
+ * (src/realtime/gpu/programs.cu:1654-1762); host arrays of n rays */
+rt_status rt_trace_closest(rt_scene* scene, const rt_ray* rays, int64_t n, rt_hit* hits);
+rt_status rt_trace_any(rt_scene* scene, const rt_ray* rays, int64_t n, int32_t* occluded);
+/* device-resident variant for throughput measurement: d_rays/d_hits are
+ * device pointers; kernel_ms receives the traversal kernel time */
+rt_status rt_trace_closest_device(rt_scene* scene, const rt_ray* d_rays, int64_t n,
+    rt_hit* d_hits, double* kernel_ms);
+
+/* replaces OptixRenderer + RendererPool (src/realtime/gpu/optix_renderer.h:69-128,
+ * src/realtime/gpu/renderer_pool.h:30-53): one renderer traces every
+ * camera of a rig in one wavefront pipeline */
+rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out);
+rt_status rt_renderer_destroy(rt_renderer* r);
+/* replaces render_prepared_device / launch_radiance_pipeline
+ * (src/realtime/gpu/optix_renderer.cpp:230-277,434-457) */
+rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_profile* profile,
+    uint32_t sample_stream, rt_frame_timing* timing);
+/* replaces download_radiance_frame_profiled (optix_renderer.cpp:283-326):
+ * W*H*4 floats for beauty/normal/albedo, W*H floats for depth; NULL skips */
+rt_status rt_frame_download(rt_renderer* r, int camera, float* beauty_rgba, float* normal_rgba,
+    float* albedo_rgba, float* depth);
+/* device view valid until the next rt_render_frame on this renderer */
+rt_status rt_frame_device_view(rt_renderer* r, int camera, rt_device_frame_view* view);
+/* OptixRenderer::reset_accumulation / reset_sequence */
+rt_status rt_reset_accumulation(rt_renderer* r);
+rt_status rt_reset_sequence(rt_renderer* r, uint32_t seed);
+rt_status rt_restir_diagnostics(rt_renderer* r, int camera, rt_restir_stats* out);
+
+/* tuning knobs chosen by the autotuner; 0 keeps the current value */
+typedef struct rt_tuning {
+    int32_t wavefront_paths;  /* paths in flight per wavefront (queue capacity) */
+    int32_t trace_block;      /* threads per block of the traversal kernels */
+    int32_t trace_blocks_per_cu; /* persistent blocks per CU */
+    int32_t use_graph;        /* capture the per-frame stage sequence in a hipGraph */
+    int32_t count_traversal;  /* count node visits / primitive tests (slower; for measurement) */
+} rt_tuning;
+rt_status rt_renderer_set_tuning(rt_renderer* r, const rt_tuning* t);
+rt_status rt_renderer_get_tuning(rt_renderer* r, rt_tuning* t);
+
+/* ------------------------------------------------- synthetic scenes
+ * Seeded generators of the benchmark scenes (this repository's equivalent
+ * of build_scene(id) + make_rig, src/scene/shared_scene_builders.h:56-57):
+ *   0 random_spheres, 1 cornell_mesh, 2 instanced_mesh_restir */
+typedef struct rt_generated_scene {
+    rt_scene_desc scene;
+    int32_t camera_count;
+    int32_t pad;
+    rt_camera cameras[4];
+    rt_profile profile;
+} rt_generated_scene;
+rt_status rt_scene_generate(int config, uint64_t seed, rt_generated_scene** out);
+rt_status rt_scene_generated_free(rt_generated_scene* g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

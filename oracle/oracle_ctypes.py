@@ -1,0 +1,326 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.  The struct
+layouts mirror oracle/rt_oracle.h, which in turn mirrors the C-ABI structs in
+include/rt/rt_hip.h byte for byte.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+f3 = C.c_float * 3
+d3 = C.c_double * 3
+
+
+class Sphere(C.Structure):
+    _fields_ = [("center", f3), ("radius", C.c_float), ("material", C.c_int32)]
+
+
+class Quad(C.Structure):
+    _fields_ = [("origin", f3), ("edge_u", f3), ("edge_v", f3), ("material", C.c_int32)]
+
+
+class Triangle(C.Structure):
+    _fields_ = [("p0", f3), ("p1", f3), ("p2", f3), ("material", C.c_int32)]
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_int32), ("albedo", f3), ("emission", f3), ("fuzz", C.c_float),
+                ("ior", C.c_float), ("openpbr_index", C.c_int32)]
+
+
+OPENPBR_FIELDS = [
+    ("base_weight", C.c_float), ("base_color", f3), ("base_diffuse_roughness", C.c_float),
+    ("base_metalness", C.c_float), ("specular_weight", C.c_float), ("specular_color", f3),
+    ("specular_roughness", C.c_float), ("specular_ior", C.c_float),
+    ("specular_roughness_anisotropy", C.c_float), ("transmission_weight", C.c_float),
+    ("transmission_color", f3), ("transmission_depth", C.c_float),
+    ("transmission_dispersion_scale", C.c_float),
+    ("transmission_dispersion_abbe_number", C.c_float), ("subsurface_weight", C.c_float),
+    ("subsurface_color", f3), ("subsurface_radius", C.c_float),
+    ("subsurface_radius_scale", f3), ("subsurface_scatter_anisotropy", C.c_float),
+    ("fuzz_weight", C.c_float), ("fuzz_color", f3), ("fuzz_roughness", C.c_float),
+    ("coat_weight", C.c_float), ("coat_color", f3), ("coat_roughness", C.c_float),
+    ("coat_roughness_anisotropy", C.c_float), ("coat_ior", C.c_float),
+    ("coat_darkening", C.c_float), ("thin_film_weight", C.c_float),
+    ("thin_film_thickness", C.c_float), ("thin_film_ior", C.c_float),
+    ("emission_luminance", C.c_float), ("emission_color", f3),
+    ("geometry_opacity", C.c_float), ("geometry_thin_walled", C.c_int32),
+]
+
+
+class OpenPbr(C.Structure):
+    _fields_ = OPENPBR_FIELDS
+
+
+class AnalyticLight(C.Structure):
+    _fields_ = [("type", C.c_int32), ("position", f3), ("radius", C.c_float), ("radiance", f3)]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("sphere_count", C.c_int32), ("quad_count", C.c_int32), ("triangle_count", C.c_int32),
+        ("material_count", C.c_int32), ("openpbr_count", C.c_int32),
+        ("analytic_light_count", C.c_int32), ("background_mode", C.c_int32),
+        ("background", f3), ("background_top", f3),
+        ("spheres", C.POINTER(Sphere)), ("quads", C.POINTER(Quad)),
+        ("triangles", C.POINTER(Triangle)), ("materials", C.POINTER(Material)),
+        ("openpbr", C.POINTER(OpenPbr)), ("analytic_lights", C.POINTER(AnalyticLight)),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("model", C.c_int32),
+        ("origin", d3), ("basis_x", d3), ("basis_y", d3), ("basis_z", d3),
+        ("fx", C.c_double), ("fy", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
+        ("k1", C.c_double), ("k2", C.c_double), ("k3", C.c_double), ("p1", C.c_double),
+        ("p2", C.c_double), ("radial", C.c_double * 6), ("tangential", C.c_double * 2),
+        ("lut_step", C.c_double), ("lut", C.c_double * 1024),
+    ]
+
+
+class Profile(C.Structure):
+    _fields_ = [
+        ("samples_per_pixel", C.c_int32), ("max_bounces", C.c_int32),
+        ("enable_denoise", C.c_int32), ("rr_start_bounce", C.c_int32),
+        ("accumulation_reset_rotation_deg", C.c_double),
+        ("accumulation_reset_translation", C.c_double),
+        ("enable_restir_di", C.c_int32), ("restir_initial_candidates", C.c_int32),
+        ("restir_temporal_reuse", C.c_int32), ("restir_max_history_age", C.c_int32),
+        ("restir_max_temporal_candidates", C.c_int32), ("restir_spatial_neighbors", C.c_int32),
+        ("restir_max_spatial_candidates", C.c_int32), ("restir_bias_correction", C.c_int32),
+        ("restir_min_analytic_lights", C.c_int32), ("pixel_jitter", C.c_int32),
+        ("restir_spatial_radius", C.c_int32), ("pad0", C.c_int32),
+    ]
+
+
+class BvhNode(C.Structure):
+    _fields_ = [("lo0", f3), ("hi0", f3), ("lo1", f3), ("hi1", f3),
+                ("child", C.c_uint32 * 2), ("parent", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class Bvh(C.Structure):
+    _fields_ = [("prim_count", C.c_int32), ("node_count", C.c_int32),
+                ("nodes", C.POINTER(BvhNode)), ("sorted_prims", C.POINTER(C.c_uint32)),
+                ("morton", C.POINTER(C.c_uint32)), ("max_depth", C.c_int32)]
+
+
+class Generated(C.Structure):
+    _fields_ = [("scene", Scene), ("camera_count", C.c_int32), ("pad", C.c_int32),
+                ("cameras", Camera * 4), ("profile", Profile)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("rays_closest", C.c_uint64), ("rays_shadow", C.c_uint64)]
+
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<i4"), ("u", "<f4"), ("v", "<f4")])
+NODE_DTYPE = np.dtype([("lo0", "<f4", 3), ("hi0", "<f4", 3), ("lo1", "<f4", 3),
+                       ("hi1", "<f4", 3), ("child", "<u4", 2), ("parent", "<u4"),
+                       ("pad", "<u4")])
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        L.rto_generate_scene.argtypes = [C.c_int, C.c_uint64, P(Generated)]
+        L.rto_free_generated.argtypes = [P(Generated)]
+        L.rto_build_lbvh.argtypes = [P(Scene), P(Bvh)]
+        L.rto_free_lbvh.argtypes = [P(Bvh)]
+        L.rto_bvh_validate.argtypes = [P(Scene), P(Bvh)]
+        L.rto_trace_closest.argtypes = [P(Scene), P(Bvh), C.c_void_p, C.c_int, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_int]
+        L.rto_trace_any.argtypes = [P(Scene), P(Bvh), C.c_void_p, C.c_int, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_int]
+        L.rto_brute_closest.argtypes = [P(Scene), C.c_void_p, C.c_int, C.c_void_p]
+        L.rto_rng4.argtypes = [C.c_uint32] * 6 + [C.c_void_p]
+        L.rto_pcg4d.argtypes = [C.c_void_p]
+        L.rto_sincos_turn.argtypes = [C.c_float, P(C.c_float), P(C.c_float)]
+        L.rto_cbrt01.argtypes = [C.c_float]
+        L.rto_cbrt01.restype = C.c_float
+        L.rto_expand_bits10.argtypes = [C.c_uint32]
+        L.rto_expand_bits10.restype = C.c_uint32
+        L.rto_unproject.argtypes = [P(Camera), C.c_double, C.c_double, C.c_void_p]
+        L.rto_camera_ray.argtypes = [P(Camera), C.c_double, C.c_double, C.c_void_p, C.c_void_p]
+        L.rto_project.argtypes = [P(Camera), C.c_void_p, C.c_void_p]
+        L.rto_make_equi_lut.argtypes = [P(Camera)]
+        L.rto_primary_rays.argtypes = [P(Camera), C.c_int, C.c_int, C.c_uint32, C.c_int,
+                                       C.c_void_p]
+        L.rto_build_lights.argtypes = [P(Scene), C.c_void_p, C.c_int]
+        L.rto_render_camera.argtypes = [P(Scene), P(Bvh), P(Camera), C.c_int, P(Profile),
+                                        C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_int, C.c_int, C.c_int, P(RenderStats)]
+        L.rto_trace_path.argtypes = [P(Scene), P(Bvh), P(Camera), C.c_int, P(Profile),
+                                     C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class GeneratedScene:
+    """A synthetic scene generated by the oracle's own generator."""
+
+    def __init__(self, config: int, seed: int = 0):
+        self.g = Generated()
+        rc = lib().rto_generate_scene(config, seed, C.byref(self.g))
+        if rc != 0:
+            raise ValueError(f"unknown scene config {config}")
+
+    def __del__(self):
+        try:
+            lib().rto_free_generated(C.byref(self.g))
+        except Exception:
+            pass
+
+    @property
+    def scene(self) -> Scene:
+        return self.g.scene
+
+    def cameras(self):
+        return [self.g.cameras[i] for i in range(self.g.camera_count)]
+
+
+def scene_from_arrays(spheres=None, quads=None, triangles=None, materials=None, openpbr=None,
+                      analytic=None, background=(0.0, 0.0, 0.0), background_top=None):
+    """Builds an oracle Scene over numpy structured/raw byte arrays; returns (scene, keepalive)."""
+    keep = []
+    s = Scene()
+
+    def attach(name, cnt_name, arr, ctype):
+        if arr is None or len(arr) == 0:
+            setattr(s, cnt_name, 0)
+            setattr(s, name, C.POINTER(ctype)())
+            return
+        a = np.ascontiguousarray(arr)
+        keep.append(a)
+        setattr(s, cnt_name, len(a))
+        setattr(s, name, a.ctypes.data_as(C.POINTER(ctype)))
+
+    attach("spheres", "sphere_count", spheres, Sphere)
+    attach("quads", "quad_count", quads, Quad)
+    attach("triangles", "triangle_count", triangles, Triangle)
+    attach("materials", "material_count", materials, Material)
+    attach("openpbr", "openpbr_count", openpbr, OpenPbr)
+    attach("analytic_lights", "analytic_light_count", analytic, AnalyticLight)
+    s.background_mode = 1 if background_top is not None else 0
+    s.background = f3(*background)
+    s.background_top = f3(*(background_top if background_top is not None else (0, 0, 0)))
+    return s, keep
+
+
+class OracleBvh:
+    def __init__(self, scene: Scene):
+        self.scene = scene
+        self.bvh = Bvh()
+        lib().rto_build_lbvh(C.byref(scene), C.byref(self.bvh))
+
+    def __del__(self):
+        try:
+            lib().rto_free_lbvh(C.byref(self.bvh))
+        except Exception:
+            pass
+
+    def nodes(self) -> np.ndarray:
+        n = self.bvh.node_count
+        if n == 0:
+            return np.zeros(0, NODE_DTYPE)
+        buf = C.string_at(self.bvh.nodes, n * NODE_DTYPE.itemsize)
+        return np.frombuffer(buf, NODE_DTYPE).copy()
+
+    def sorted_prims(self) -> np.ndarray:
+        n = self.bvh.prim_count
+        return np.ctypeslib.as_array(self.bvh.sorted_prims, (n,)).copy() if n else np.zeros(0, np.uint32)
+
+    def morton(self) -> np.ndarray:
+        n = self.bvh.prim_count
+        return np.ctypeslib.as_array(self.bvh.morton, (n,)).copy() if n else np.zeros(0, np.uint32)
+
+    def validate(self) -> int:
+        return lib().rto_bvh_validate(C.byref(self.scene), C.byref(self.bvh))
+
+    def trace_closest(self, rays: np.ndarray, threads: int = 8, counts: bool = False):
+        n = len(rays)
+        hits = np.zeros(n, HIT_DTYPE)
+        nv = np.zeros(n, np.uint32)
+        nt = np.zeros(n, np.uint32)
+        lib().rto_trace_closest(C.byref(self.scene), C.byref(self.bvh), ptr(rays), n, ptr(hits),
+                                ptr(nv), ptr(nt), threads)
+        return (hits, nv, nt) if counts else hits
+
+    def trace_any(self, rays: np.ndarray, threads: int = 8, counts: bool = False):
+        n = len(rays)
+        occ = np.zeros(n, np.int32)
+        nv = np.zeros(n, np.uint32)
+        nt = np.zeros(n, np.uint32)
+        lib().rto_trace_any(C.byref(self.scene), C.byref(self.bvh), ptr(rays), n, ptr(occ),
+                            ptr(nv), ptr(nt), threads)
+        return (occ, nv, nt) if counts else occ
+
+
+def brute_closest(scene: Scene, rays: np.ndarray) -> np.ndarray:
+    hits = np.zeros(len(rays), HIT_DTYPE)
+    lib().rto_brute_closest(C.byref(scene), ptr(rays), len(rays), ptr(hits))
+    return hits
+
+
+def primary_rays(cam: Camera, cam_index: int, sample: int, stream: int, jitter: int) -> np.ndarray:
+    rays = np.zeros(cam.width * cam.height, RAY_DTYPE)
+    lib().rto_primary_rays(C.byref(cam), cam_index, sample, stream, jitter, ptr(rays))
+    return rays
+
+
+def render_camera(scene: Scene, bvh: OracleBvh, cam: Camera, cam_index: int, profile: Profile,
+                  stream: int, threads: int = 8, rows=None):
+    w, h = cam.width, cam.height
+    beauty = np.zeros((h, w, 4), np.float32)
+    normal = np.zeros((h, w, 4), np.float32)
+    albedo = np.zeros((h, w, 4), np.float32)
+    depth = np.zeros((h, w), np.float32)
+    st = RenderStats()
+    r0, r1 = (0, h) if rows is None else rows
+    rc = lib().rto_render_camera(C.byref(scene), C.byref(bvh.bvh), C.byref(cam), cam_index,
+                                 C.byref(profile), stream, ptr(beauty), ptr(normal), ptr(albedo),
+                                 ptr(depth), threads, r0, r1, C.byref(st))
+    if rc != 0:
+        raise RuntimeError("oracle render failed")
+    return {"beauty": beauty, "normal": normal, "albedo": albedo, "depth": depth,
+            "paths": st.paths, "rays_closest": st.rays_closest, "rays_shadow": st.rays_shadow}
+
+
+def rng4(pixel, sample, camera, bounce, group, stream) -> np.ndarray:
+    out = np.zeros(4, np.float32)
+    lib().rto_rng4(pixel, sample, camera, bounce, group, stream, ptr(out))
+    return out
+
+
+def sincos_turn(u: float):
+    s = C.c_float()
+    c = C.c_float()
+    lib().rto_sincos_turn(u, C.byref(s), C.byref(c))
+    return s.value, c.value

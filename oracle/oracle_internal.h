@@ -1,0 +1,16 @@
+/* oracle_internal.h -- declarations shared by the oracle's translation units.
+ * TEST INFRASTRUCTURE ONLY (see rt_oracle.h). */
+#ifndef RT_ORACLE_INTERNAL_H
+#define RT_ORACLE_INTERNAL_H
+#include "om.h"
+#include "rt_oracle.h"
+
+int om_closest(const rto_scene* s, const rto_bvh* bvh, v3 o, v3 d, float tmin, float tmax,
+    float* t, int* prim, float* u, float* v);
+int om_occluded(const rto_scene* s, const rto_bvh* bvh, v3 o, v3 d, float tmin, float tmax);
+
+v3 om_openpbr_emission(const rto_openpbr* m);
+void om_openpbr_eval(const rto_openpbr* m, v3 n, v3 wo, v3 wi, v3* value, float* pdf);
+void om_openpbr_sample(const rto_openpbr* m, v3 n, v3 wo, float u_lobe, float u1, float u2,
+    rto_openpbr_sample* out);
+#endif

@@ -1,0 +1,883 @@
+/*
+ * oracle_render.c -- CPU restatement of the realtime integrator.
+ * TEST INFRASTRUCTURE ONLY (see rt_oracle.h).
+ *
+ * Follows src/realtime/gpu/programs.cu:
+ *   camera rays        :308-373 (pinhole32), :375-515 (equi62_lut1d), :2086-2099
+ *   light selection    :559-619 (prim->light lookup replaces the O(L) scan)
+ *   emission MIS       :621-647
+ *   populate_material  :961-1009, direct response :1026-1055
+ *   sample_direct_light:1057-1232, accumulate_direct_light :2249-2280
+ *   russian roulette   :1764-1771, sample_bsdf :2451-2565
+ *   radiance_kernel    :1828-1953, store_output :1773-1805
+ * with the counter RNG of om.h: every bounce draws exactly two 4-tuples
+ * (group 0: light select + light position; group 1: BSDF lobe/direction + RR);
+ * rejection loops are replaced by closed-form samplers of the same
+ * distribution (unit sphere from two uniforms, unit ball via cube root).
+ */
+#include "rt_oracle.h"
+#include "om.h"
+#include "oracle_internal.h"
+
+#include <pthread.h>
+#include <stdlib.h>
+
+/* ------------------------------------------------------------- cameras */
+typedef struct { double x, y; } d2;
+
+static d2 distort_pinhole(const rto_camera* c, d2 xy) {
+    const double x = xy.x, y = xy.y;
+    const double x2 = x * x, y2 = y * y, xyt = x * y;
+    const double r2 = x2 + y2, r4 = r2 * r2, r6 = r4 * r2;
+    const double radial = 1.0 + c->k1 * r2 + c->k2 * r4 + c->k3 * r6;
+    d2 r = {x * radial + c->p2 * (r2 + 2.0 * x2) + 2.0 * c->p1 * xyt,
+        y * radial + c->p1 * (r2 + 2.0 * y2) + 2.0 * c->p2 * xyt};
+    return r;
+}
+
+static int solve2(double a00, double a01, double a10, double a11, d2 rhs, d2* delta) {
+    const double det = a00 * a11 - a01 * a10;
+    if (fabs(det) < 1e-12) return 0;
+    const double inv = 1.0 / det;
+    delta->x = (a11 * rhs.x - a01 * rhs.y) * inv;
+    delta->y = (-a10 * rhs.x + a00 * rhs.y) * inv;
+    return 1;
+}
+
+/* programs.cu:335-373 */
+static v3 unproject_pinhole(const rto_camera* c, double px, double py) {
+    const d2 xyd = {(px - c->cx) / c->fx, (py - c->cy) / c->fy};
+    d2 xy = xyd;
+    for (int it = 0; it < 24; ++it) {
+        const d2 dist = distort_pinhole(c, xy);
+        const d2 err = {xyd.x - dist.x, xyd.y - dist.y};
+        if (err.x * err.x + err.y * err.y < 1e-20) break;
+        const double x = xy.x, y = xy.y;
+        const double r2 = x * x + y * y, r4 = r2 * r2, r6 = r4 * r2;
+        const double radial = 1.0 + c->k1 * r2 + c->k2 * r4 + c->k3 * r6;
+        const double drdx = 2.0 * x * (c->k1 + 2.0 * c->k2 * r2 + 3.0 * c->k3 * r4);
+        const double drdy = 2.0 * y * (c->k1 + 2.0 * c->k2 * r2 + 3.0 * c->k3 * r4);
+        d2 delta;
+        if (!solve2(radial + x * drdx + 6.0 * c->p2 * x + 2.0 * c->p1 * y,
+                x * drdy + 2.0 * c->p1 * x + 2.0 * c->p2 * y,
+                y * drdx + 2.0 * c->p1 * x + 2.0 * c->p2 * y,
+                radial + y * drdy + 6.0 * c->p1 * y + 2.0 * c->p2 * x, err, &delta))
+            break;
+        xy.x += delta.x;
+        xy.y += delta.y;
+    }
+    return norm3(mk3((float)xy.x, (float)xy.y, 1.0f));
+}
+
+static d2 equi_tangential(d2 xy, const double* tg) {
+    const double xr = xy.x, yr = xy.y, xr2 = xr * xr, yr2 = yr * yr, xryr = xr * yr;
+    d2 r = {xr + 2.0 * tg[0] * xryr + tg[1] * (xr2 + yr2 + 2.0 * xr2),
+        yr + tg[0] * (xr2 + yr2 + 2.0 * yr2) + 2.0 * tg[1] * xryr};
+    return r;
+}
+
+/* programs.cu:399-413 */
+static d2 equi_remove_tangential(d2 xyd, const double* tg) {
+    const d2 dist = equi_tangential(xyd, tg);
+    d2 delta;
+    const d2 rhs = {xyd.x - dist.x, xyd.y - dist.y};
+    if (!solve2(1.0 + 2.0 * tg[0] * xyd.y + 6.0 * tg[1] * xyd.x,
+            2.0 * tg[0] * xyd.x + 2.0 * tg[1] * xyd.y, 2.0 * tg[0] * xyd.x + 2.0 * tg[1] * xyd.y,
+            1.0 + 6.0 * tg[0] * xyd.y + 2.0 * tg[1] * xyd.x, rhs, &delta))
+        return xyd;
+    d2 r = {xyd.x + delta.x, xyd.y + delta.y};
+    return r;
+}
+
+/* programs.cu:415-433 */
+static int equi_lut_theta(const rto_camera* c, double rd, double* theta) {
+    if (rd <= 0.0 || c->lut_step <= 0.0) { *theta = 0.0; return 1; }
+    const double pos = rd / c->lut_step;
+    if (pos > 1023.0) return 0;
+    if (pos >= 1023.0) { *theta = c->lut[1023]; return 1; }
+    const int idx = (int)pos;
+    const double alpha = pos - (double)idx;
+    *theta = (1.0 - alpha) * c->lut[idx] + alpha * c->lut[idx + 1];
+    return 1;
+}
+
+/* programs.cu:475-497 */
+static v3 unproject_equi(const rto_camera* c, double px, double py) {
+    const d2 xy = {(px - c->cx) / c->fx, (py - c->cy) / c->fy};
+    if (xy.x * xy.x + xy.y * xy.y < 1e-12 * 1e-12) return mk3(0.0f, 0.0f, 1.0f);
+    const d2 xr = equi_remove_tangential(xy, c->tangential);
+    const double rd = sqrt(xr.x * xr.x + xr.y * xr.y);
+    if (rd < 1e-12) return mk3(0.0f, 0.0f, 1.0f);
+    double theta = 0.0;
+    if (!equi_lut_theta(c, rd, &theta)) return norm3(mk3((float)xy.x, (float)xy.y, 1.0f));
+    const double scale = tan(theta) / rd;
+    return norm3(mk3((float)(xr.x * scale), (float)(xr.y * scale), 1.0f));
+}
+
+void rto_unproject(const rto_camera* cam, double px, double py, float dir_cam[3]) {
+    const v3 d = cam->model == RTO_CAM_EQUI62_LUT1D ? unproject_equi(cam, px, py)
+                                                    : unproject_pinhole(cam, px, py);
+    dir_cam[0] = d.x; dir_cam[1] = d.y; dir_cam[2] = d.z;
+}
+
+/* programs.cu:521-532 */
+static v3 cam_transform(const rto_camera* c, v3 dc) {
+    const v3 bx = mk3((float)c->basis_x[0], (float)c->basis_x[1], (float)c->basis_x[2]);
+    const v3 by = mk3((float)c->basis_y[0], (float)c->basis_y[1], (float)c->basis_y[2]);
+    const v3 bz = mk3((float)c->basis_z[0], (float)c->basis_z[1], (float)c->basis_z[2]);
+    return norm3(add3(add3(scl3(bx, dc.x), scl3(by, dc.y)), scl3(bz, dc.z)));
+}
+
+void rto_camera_ray(const rto_camera* cam, double px, double py, float origin[3], float dir[3]) {
+    float dc[3];
+    rto_unproject(cam, px, py, dc);
+    const v3 d = cam_transform(cam, ld3(dc));
+    origin[0] = (float)cam->origin[0]; origin[1] = (float)cam->origin[1]; origin[2] = (float)cam->origin[2];
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
+
+/* programs.cu:375-473 (projection, used by reprojection) */
+void rto_project(const rto_camera* c, const float dir_cam[3], float pixel[2]) {
+    if (c->model == RTO_CAM_EQUI62_LUT1D) {
+        const double dx = dir_cam[0], dy = dir_cam[1], dz = dir_cam[2];
+        const double r = sqrt(dx * dx + dy * dy);
+        if (r < 1e-12) { pixel[0] = (float)c->cx; pixel[1] = (float)c->cy; return; }
+        const double theta = atan2(r, dz);
+        double rd;
+        if (theta <= 0.0 || c->lut_step <= 0.0) rd = 0.0;
+        else if (theta >= c->lut[1023]) rd = 1023.0 * c->lut_step;
+        else {
+            rd = 1023.0 * c->lut_step;
+            for (int i = 0; i < 1023; ++i) {
+                if (c->lut[i] <= theta && theta <= c->lut[i + 1]) {
+                    const double den = c->lut[i + 1] - c->lut[i];
+                    const double a = den > 0.0 ? (theta - c->lut[i]) / den : 0.0;
+                    rd = ((double)i + a) * c->lut_step;
+                    break;
+                }
+            }
+        }
+        const d2 xyr = {dx / r * rd, dy / r * rd};
+        const d2 xyd = equi_tangential(xyr, c->tangential);
+        pixel[0] = (float)(xyd.x * c->fx + c->cx);
+        pixel[1] = (float)(xyd.y * c->fy + c->cy);
+        return;
+    }
+    if (fabsf(dir_cam[2]) < 1e-7f) { pixel[0] = -1.0f; pixel[1] = -1.0f; return; }
+    const double inv_z = 1.0 / (double)dir_cam[2];
+    const d2 xyu = {(double)dir_cam[0] * inv_z, (double)dir_cam[1] * inv_z};
+    const d2 xyd = distort_pinhole(c, xyu);
+    pixel[0] = (float)(xyd.x * c->fx + c->cx);
+    pixel[1] = (float)(xyd.y * c->fy + c->cy);
+}
+
+/* host LUT builder, camera_models.cpp:64-100,228-248 */
+static double equi_theta_dist(const double* radial, double theta) {
+    const double t2 = theta * theta;
+    double scale = 1.0, tp = t2;
+    for (int i = 0; i < 6; ++i) { scale += radial[i] * tp; tp *= t2; }
+    return theta * scale;
+}
+static double equi_theta_deriv(const double* radial, double theta) {
+    const double t2 = theta * theta;
+    double der = 1.0, tp = t2;
+    for (int i = 0; i < 6; ++i) { der += (double)(2 * i + 3) * radial[i] * tp; tp *= t2; }
+    return der;
+}
+static double equi_invert(const double* radial, double thetad) {
+    double theta = cbrt(thetad > 0.0 ? thetad : 0.0);
+    for (int it = 0; it < 30; ++it) {
+        const double err = thetad - equi_theta_dist(radial, theta);
+        if (fabs(err) < 1e-12) break;
+        const double der = equi_theta_deriv(radial, theta);
+        if (fabs(der) < 1e-12) break;
+        const double delta = err / der;
+        theta += delta;
+        if (fabs(delta) < 1e-12 || fabs(delta) > 3.1416) break;
+    }
+    return theta;
+}
+void rto_make_equi_lut(rto_camera* c) {
+    const double x = ((double)c->width + 10.0 - c->cx) / c->fx;
+    const double y = ((double)c->height + 10.0 - c->cy) / c->fy;
+    c->lut_step = sqrt(x * x + y * y) / 1024.0;
+    for (int i = 0; i < 1024; ++i) c->lut[i] = equi_invert(c->radial, c->lut_step * (double)i);
+}
+
+static void pixel_position(int x, int y, int cam_index, int sample, uint32_t stream, int jitter,
+    int width, double* px, double* py) {
+    if (jitter) {
+        float r[4];
+        om_rng4((uint32_t)(y * width + x), (uint32_t)sample, (uint32_t)cam_index, 0u, OM_G_CAMERA,
+            stream, r);
+        *px = (double)((float)x + r[0]);
+        *py = (double)((float)y + r[1]);
+    } else {
+        *px = (double)((float)x + 0.5f);
+        *py = (double)((float)y + 0.5f);
+    }
+}
+
+void rto_primary_rays(const rto_camera* cam, int cam_index, int sample, uint32_t stream, int jitter,
+    rto_ray* rays) {
+    for (int y = 0; y < cam->height; ++y) {
+        for (int x = 0; x < cam->width; ++x) {
+            double px, py;
+            pixel_position(x, y, cam_index, sample, stream, jitter, cam->width, &px, &py);
+            rto_ray* r = &rays[y * cam->width + x];
+            rto_camera_ray(cam, px, py, r->o, r->d);
+            r->tmin = 0.0f;
+            r->tmax = OM_RAY_FAR;
+        }
+    }
+}
+
+/* ------------------------------------------------------------ lights */
+static float lum_clamped(v3 c) {
+    const v3 m = mk3(fmaxf(c.x, 0.0f), fmaxf(c.y, 0.0f), fmaxf(c.z, 0.0f));
+    return fmaxf(0.0f, 0.2126f * m.x + 0.7152f * m.y + 0.0722f * m.z);
+}
+
+static float material_emission_lum(const rto_scene* s, int m) {
+    if (m < 0 || m >= s->material_count) return 0.0f;
+    const rto_material* mt = &s->materials[m];
+    if (mt->type == RTO_MAT_DIFFUSE_LIGHT) return lum_clamped(ld3(mt->emission));
+    if (mt->type != RTO_MAT_OPENPBR || mt->openpbr_index < 0 || mt->openpbr_index >= s->openpbr_count)
+        return 0.0f;
+    const v3 e = om_openpbr_emission(&s->openpbr[mt->openpbr_index]);
+    return 0.2126f * fmaxf(0.0f, e.x) + 0.7152f * fmaxf(0.0f, e.y) + 0.0722f * fmaxf(0.0f, e.z);
+}
+
+static v3 background_mean(const rto_scene* s) {
+    if (s->background_mode == RTO_BG_SKY_GRADIENT)
+        return scl3(add3(ld3(s->background), ld3(s->background_top)), 0.5f);
+    return ld3(s->background);
+}
+
+/* packed_scene_preparation.cpp:250-302; light types 0 sphere,1 quad,2 tri,3 env */
+int rto_build_lights(const rto_scene* s, rto_light* out, int capacity) {
+    int n = 0;
+    const float kPi = OM_PI;
+#define APPEND(TYPE, PRIM, W)                                                        \
+    do {                                                                             \
+        const float w_ = (W);                                                        \
+        if (isfinite(w_) && w_ > 1e-12f) {                                           \
+            if (n < capacity) { out[n].type = (TYPE); out[n].prim = (PRIM); out[n].selection_pdf = w_; out[n].cdf = 0.0f; } \
+            ++n;                                                                     \
+        }                                                                            \
+    } while (0)
+    for (int i = 0; i < s->sphere_count; ++i) {
+        const float area = 4.0f * kPi * s->spheres[i].radius * s->spheres[i].radius;
+        APPEND(0, i, area * material_emission_lum(s, s->spheres[i].material));
+    }
+    for (int i = 0; i < s->quad_count; ++i) {
+        const float area = len3(cross3(ld3(s->quads[i].edge_u), ld3(s->quads[i].edge_v)));
+        APPEND(1, i, area * material_emission_lum(s, s->quads[i].material));
+    }
+    for (int i = 0; i < s->triangle_count; ++i) {
+        const rto_triangle* t = &s->triangles[i];
+        const float area = 0.5f * len3(cross3(sub3(ld3(t->p1), ld3(t->p0)), sub3(ld3(t->p2), ld3(t->p0))));
+        APPEND(2, i, area * material_emission_lum(s, t->material));
+    }
+    APPEND(3, -1, 4.0f * kPi * lum_clamped(background_mean(s)));
+#undef APPEND
+    if (n > capacity) return n;
+    float total = 0.0f;
+    for (int i = 0; i < n; ++i) total += out[i].selection_pdf;
+    float cdf = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        out[i].selection_pdf /= total;
+        cdf += out[i].selection_pdf;
+        out[i].cdf = cdf;
+    }
+    if (n > 0) out[n - 1].cdf = 1.0f;
+    return n;
+}
+
+/* -------------------------------------------------------------- context */
+typedef struct {
+    const rto_scene* s;
+    const rto_bvh* bvh;
+    rto_light lights[4096];
+    int light_count;
+    int32_t* light_of_prim; /* global prim id -> light index or -1 */
+    int env_light;
+} ctx_t;
+
+static int prim_global(const rto_scene* s, int type, int idx) {
+    if (type == 0) return idx;
+    if (type == 1) return s->sphere_count + idx;
+    return s->sphere_count + s->quad_count + idx;
+}
+
+static int ctx_init(ctx_t* c, const rto_scene* s, const rto_bvh* bvh) {
+    c->s = s;
+    c->bvh = bvh;
+    c->light_count = rto_build_lights(s, c->lights, 4096);
+    if (c->light_count > 4096) return -1;
+    const int np = s->sphere_count + s->quad_count + s->triangle_count;
+    c->light_of_prim = (int32_t*)malloc(sizeof(int32_t) * (size_t)(np > 0 ? np : 1));
+    for (int i = 0; i < np; ++i) c->light_of_prim[i] = -1;
+    c->env_light = -1;
+    for (int i = 0; i < c->light_count; ++i) {
+        if (c->lights[i].type == 3) c->env_light = i;
+        else c->light_of_prim[prim_global(s, c->lights[i].type, c->lights[i].prim)] = i;
+    }
+    return 0;
+}
+static void ctx_free(ctx_t* c) { free(c->light_of_prim); }
+
+static v3 background_at(const rto_scene* s, v3 d) {
+    if (s->background_mode == RTO_BG_SKY_GRADIENT) {
+        const v3 dn = norm3(d);
+        const float t = 0.5f * (dn.y + 1.0f);
+        return add3(scl3(ld3(s->background), 1.0f - t), scl3(ld3(s->background_top), t));
+    }
+    return ld3(s->background);
+}
+
+/* --------------------------------------------------------------- hit info */
+typedef struct {
+    float t;
+    v3 pos, gn, sn, base, emission;
+    float fuzz, ior;
+    int mat_type, openpbr, prim, prim_type, prim_index, front;
+} hit_t;
+
+static void set_face(v3 d, v3 outward, hit_t* h) {
+    h->front = dot3(d, outward) < 0.0f;
+    h->gn = outward;
+    h->sn = h->front ? outward : neg3(outward);
+}
+
+static void make_hit(const rto_scene* s, v3 o, v3 d, float t, int prim, hit_t* h) {
+    h->t = t;
+    h->prim = prim;
+    int mat;
+    if (prim < s->sphere_count) {
+        const rto_sphere* sp = &s->spheres[prim];
+        h->prim_type = 0; h->prim_index = prim;
+        h->pos = add3(o, scl3(d, t));
+        set_face(d, div3(sub3(h->pos, ld3(sp->center)), sp->radius), h);
+        mat = sp->material;
+    } else if (prim < s->sphere_count + s->quad_count) {
+        const int qi = prim - s->sphere_count;
+        const rto_quad* q = &s->quads[qi];
+        h->prim_type = 1; h->prim_index = qi;
+        const v3 n = cross3(ld3(q->edge_u), ld3(q->edge_v));
+        const v3 normal = div3(n, sqrtf(lensq3(n)));
+        h->pos = add3(o, scl3(d, t));
+        set_face(d, normal, h);
+        mat = q->material;
+    } else {
+        const int ti = prim - s->sphere_count - s->quad_count;
+        const rto_triangle* tr = &s->triangles[ti];
+        h->prim_type = 2; h->prim_index = ti;
+        const v3 p0 = ld3(tr->p0);
+        const v3 n = norm3(cross3(sub3(ld3(tr->p1), p0), sub3(ld3(tr->p2), p0)));
+        h->pos = add3(o, scl3(d, t));
+        set_face(d, n, h);
+        mat = tr->material;
+    }
+    /* populate_material, programs.cu:961-1009 */
+    if (mat < 0 || mat >= s->material_count) {
+        h->base = mk3(0, 0, 0); h->emission = mk3(0, 0, 0); h->mat_type = 0; h->ior = 1.0f;
+        h->fuzz = 0.0f; h->openpbr = -1;
+        return;
+    }
+    const rto_material* m = &s->materials[mat];
+    h->openpbr = m->openpbr_index;
+    h->mat_type = m->type;
+    h->fuzz = m->fuzz;
+    h->ior = m->ior;
+    h->base = ld3(m->albedo);
+    h->emission = ld3(m->emission);
+    if (h->mat_type == RTO_MAT_DIELECTRIC) { h->base = mk3(0.92f, 0.95f, 1.0f); h->emission = mk3(0, 0, 0); }
+    if (h->mat_type == RTO_MAT_DIFFUSE_LIGHT && !h->front) h->emission = mk3(0, 0, 0);
+    if (h->mat_type != RTO_MAT_DIFFUSE_LIGHT) h->emission = mk3(0, 0, 0);
+    if (h->mat_type == RTO_MAT_OPENPBR) {
+        if (m->openpbr_index < 0 || m->openpbr_index >= s->openpbr_count) {
+            h->mat_type = -1; h->base = mk3(0, 0, 0); h->emission = mk3(0, 0, 0);
+            return;
+        }
+        const rto_openpbr* p = &s->openpbr[m->openpbr_index];
+        h->base = ld3(p->base_color);
+        h->emission = om_openpbr_emission(p);
+    }
+}
+
+/* ------------------------------------------------------------- path state */
+typedef struct {
+    v3 o, d, throughput, radiance;
+    v3 prev_pos, prev_n;
+    float prev_pdf;
+    int prev_mat, prev_prim, prev_valid, prev_delta, alive;
+} path_t;
+
+static float power_heuristic(float a, float b) {
+    if (a <= 0.0f) return 0.0f;
+    const float a2 = a * a, b2 = b * b;
+    return a2 / (a2 + b2);
+}
+static float cone_pdf(float cos_max) {
+    const float sa = 6.28318530717958648f * (1.0f - cos_max);
+    return sa > 1e-12f ? 1.0f / sa : 0.0f;
+}
+static float area_pdf(float area, float dsq, float acos_) {
+    if (area <= 1e-12f || dsq <= 0.0f || acos_ <= 1e-8f) return 0.0f;
+    return dsq / (area * acos_);
+}
+#define UNIFORM_SPHERE_PDF 0.07957747154594767f
+
+/* programs.cu:571-582 */
+static float eff_selection_pdf(const ctx_t* c, int light, int excluded_prim) {
+    const int ex = excluded_prim < 0 ? -1 : c->light_of_prim[excluded_prim];
+    if (ex == light) return 0.0f;
+    const float elig = ex < 0 ? 1.0f : 1.0f - c->lights[ex].selection_pdf;
+    return elig > 1e-8f ? c->lights[light].selection_pdf / elig : 0.0f;
+}
+
+/* programs.cu:584-647 */
+static float emission_mis(const ctx_t* c, const path_t* st, const hit_t* h, int env) {
+    if (!st->prev_valid || st->prev_delta) return 1.0f;
+    const int light = env ? c->env_light : c->light_of_prim[h->prim];
+    if (light < 0) return 1.0f;
+    const v3 dir = norm3(st->d);
+    float cond;
+    if (env) {
+        cond = st->prev_mat == RTO_MAT_ISOTROPIC ? UNIFORM_SPHERE_PDF
+                                                 : fmaxf(dot3(st->prev_n, dir), 0.0f) / OM_PI;
+    } else {
+        const rto_scene* s = c->s;
+        if (!h->front) cond = 0.0f;
+        else if (h->prim_type == 0) {
+            const rto_sphere* sp = &s->spheres[h->prim_index];
+            const float dsq = lensq3(sub3(ld3(sp->center), st->prev_pos));
+            const float rsq = sp->radius * sp->radius;
+            cond = dsq <= rsq ? 0.0f : cone_pdf(sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq)));
+        } else {
+            const float dsq = lensq3(sub3(h->pos, st->prev_pos));
+            const float ac = fmaxf(dot3(h->gn, scl3(dir, -1.0f)), 0.0f);
+            float area;
+            if (h->prim_type == 1) {
+                const rto_quad* q = &s->quads[h->prim_index];
+                area = len3(cross3(ld3(q->edge_u), ld3(q->edge_v)));
+            } else {
+                const rto_triangle* t = &s->triangles[h->prim_index];
+                area = 0.5f * len3(cross3(sub3(ld3(t->p1), ld3(t->p0)), sub3(ld3(t->p2), ld3(t->p0))));
+            }
+            cond = area_pdf(area, dsq, ac);
+        }
+    }
+    const float sel = eff_selection_pdf(c, light, st->prev_prim);
+    return power_heuristic(st->prev_pdf, sel * cond);
+}
+
+typedef struct { v3 dir, emission; float dist, pdf; int infinite, valid; } light_sample_t;
+
+/* programs.cu:539-550 with the own sin/cos */
+static v3 cone_dir(v3 axis, float cos_max, float u0, float u1) {
+    const float ct = 1.0f - u0 * (1.0f - cos_max);
+    const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float sp, cp;
+    om_sincos_turn(u1, &sp, &cp);
+    const v3 helper = fabsf(axis.x) > 0.9f ? mk3(0.0f, 1.0f, 0.0f) : mk3(1.0f, 0.0f, 0.0f);
+    const v3 tangent = norm3(cross3(helper, axis));
+    const v3 bitangent = cross3(axis, tangent);
+    return norm3(add3(scl3(axis, ct), add3(scl3(tangent, st * cp), scl3(bitangent, st * sp))));
+}
+
+/* programs.cu:1057-1232 */
+static light_sample_t sample_light(const ctx_t* c, const hit_t* h, const float r[4]) {
+    light_sample_t ls;
+    memset(&ls, 0, sizeof(ls));
+    const rto_scene* s = c->s;
+    const int ex = c->light_of_prim[h->prim];
+    const float elig = ex < 0 ? 1.0f : 1.0f - c->lights[ex].selection_pdf;
+    if (elig <= 1e-8f) return ls;
+    int li = -1;
+    if (ex < 0) {
+        const float u = r[0] < 0.0f ? 0.0f : (r[0] < 1.0f ? r[0] : 0.99999994f);
+        for (int i = 0; i < c->light_count; ++i) {
+            if (u < c->lights[i].cdf) { li = i; break; }
+        }
+        if (li < 0) li = c->light_count - 1;
+    } else {
+        const float target = r[0] * elig;
+        float acc = 0.0f;
+        for (int i = 0; i < c->light_count; ++i) {
+            if (i == ex) continue;
+            acc += c->lights[i].selection_pdf;
+            if (target < acc) { li = i; break; }
+        }
+        if (li < 0)
+            for (int i = c->light_count - 1; i >= 0; --i)
+                if (i != ex) { li = i; break; }
+    }
+    if (li < 0) return ls;
+    const rto_light* L = &c->lights[li];
+    const float sel = L->selection_pdf / elig;
+    if (sel <= 0.0f) return ls;
+    const v3 sp = h->pos;
+    if (L->type == 3) {
+        if (h->mat_type == RTO_MAT_ISOTROPIC) {
+            ls.dir = om_sphere_dir(r[1], r[2]);
+            ls.pdf = sel * UNIFORM_SPHERE_PDF;
+        } else {
+            ls.dir = add3(h->sn, om_sphere_dir(r[1], r[2]));
+            if (near_zero3(ls.dir)) ls.dir = h->sn;
+            ls.dir = norm3(ls.dir);
+            const float cs = fmaxf(dot3(h->sn, ls.dir), 0.0f);
+            ls.pdf = sel * cs / OM_PI;
+        }
+        ls.emission = background_at(s, ls.dir);
+        ls.dist = OM_RAY_FAR;
+        ls.infinite = 1;
+        ls.valid = ls.pdf > 0.0f;
+        return ls;
+    }
+    float cond = 0.0f;
+    int mat = -1;
+    if (L->type == 0) {
+        const rto_sphere* sph = &s->spheres[L->prim];
+        const v3 center = ld3(sph->center);
+        const v3 tc = sub3(center, sp);
+        const float dsq = lensq3(tc);
+        const float rsq = sph->radius * sph->radius;
+        if (dsq <= rsq * (1.0f + 1e-6f)) return ls;
+        const float cmax = sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq));
+        ls.dir = cone_dir(norm3(tc), cmax, r[1], r[2]);
+        const v3 oc = sub3(sp, center);
+        const float hb = dot3(oc, ls.dir);
+        const float disc = hb * hb - (lensq3(oc) - rsq);
+        if (disc <= 0.0f) return ls;
+        ls.dist = -hb - sqrtf(disc);
+        if (ls.dist <= OM_RAY_EPS) return ls;
+        mat = sph->material;
+        cond = cone_pdf(cmax);
+    } else if (L->type == 1) {
+        const rto_quad* q = &s->quads[L->prim];
+        const v3 eu = ld3(q->edge_u), ev = ld3(q->edge_v);
+        const v3 pos = add3(ld3(q->origin), add3(scl3(eu, r[1]), scl3(ev, r[2])));
+        const v3 an = cross3(eu, ev);
+        const float area = len3(an);
+        if (area <= 1e-12f) return ls;
+        const v3 normal = div3(an, area);
+        mat = q->material;
+        const v3 tl = sub3(pos, sp);
+        const float dsq = lensq3(tl);
+        if (dsq <= 1e-12f) { memset(&ls, 0, sizeof(ls)); return ls; }
+        ls.dist = sqrtf(dsq);
+        ls.dir = div3(tl, ls.dist);
+        cond = area_pdf(area, dsq, dot3(normal, scl3(ls.dir, -1.0f)));
+    } else {
+        const rto_triangle* t = &s->triangles[L->prim];
+        const v3 p0 = ld3(t->p0), p1 = ld3(t->p1), p2 = ld3(t->p2);
+        const float su = sqrtf(r[1] < 0.0f ? 0.0f : (r[1] < 1.0f ? r[1] : 1.0f));
+        const float w0 = 1.0f - su, w1 = su * (1.0f - r[2]), w2 = su * r[2];
+        const v3 pos = add3(scl3(p0, w0), add3(scl3(p1, w1), scl3(p2, w2)));
+        const v3 an = cross3(sub3(p1, p0), sub3(p2, p0));
+        const float da = len3(an);
+        if (da <= 1e-12f) return ls;
+        const v3 normal = div3(an, da);
+        mat = t->material;
+        const v3 tl = sub3(pos, sp);
+        const float dsq = lensq3(tl);
+        if (dsq <= 1e-12f) { memset(&ls, 0, sizeof(ls)); return ls; }
+        ls.dist = sqrtf(dsq);
+        ls.dir = div3(tl, ls.dist);
+        cond = area_pdf(0.5f * da, dsq, dot3(normal, scl3(ls.dir, -1.0f)));
+    }
+    if (cond <= 0.0f || mat < 0 || mat >= s->material_count) { memset(&ls, 0, sizeof(ls)); return ls; }
+    const rto_material* m = &s->materials[mat];
+    if (m->type == RTO_MAT_OPENPBR) {
+        ls.emission = (m->openpbr_index >= 0 && m->openpbr_index < s->openpbr_count)
+                          ? om_openpbr_emission(&s->openpbr[m->openpbr_index]) : mk3(0, 0, 0);
+    } else {
+        ls.emission = ld3(m->emission);
+    }
+    ls.pdf = sel * cond;
+    ls.valid = ls.pdf > 0.0f && maxc3(ls.emission) > 0.0f;
+    return ls;
+}
+
+/* programs.cu:1026-1055 */
+static v3 direct_response(const ctx_t* c, const hit_t* h, const path_t* st, v3 ldir, float* pdf) {
+    *pdf = 0.0f;
+    if (h->mat_type == RTO_MAT_LAMBERTIAN) {
+        const float cs = fmaxf(dot3(h->sn, ldir), 0.0f);
+        *pdf = cs / OM_PI;
+        return scl3(h->base, *pdf);
+    }
+    if (h->mat_type == RTO_MAT_ISOTROPIC) {
+        *pdf = UNIFORM_SPHERE_PDF;
+        return scl3(h->base, *pdf);
+    }
+    if (h->mat_type != RTO_MAT_OPENPBR) return mk3(0, 0, 0);
+    const rto_openpbr* p = &c->s->openpbr[h->openpbr];
+    v3 value;
+    om_openpbr_eval(p, h->gn, neg3(norm3(st->d)), ldir, &value, pdf);
+    const float cs = fabsf(dot3(h->sn, ldir));
+    return scl3(value, cs);
+}
+
+/* programs.cu:2249-2280 */
+static void direct_light(const ctx_t* c, const hit_t* h, const float r[4], int bsdf_avail, path_t* st) {
+    if (h->mat_type == RTO_MAT_DIFFUSE_LIGHT || c->light_count <= 0) return;
+    const light_sample_t L = sample_light(c, h, r);
+    if (!L.valid) return;
+    float bpdf = 0.0f;
+    const v3 resp = direct_response(c, h, st, L.dir, &bpdf);
+    if (maxc3(resp) <= 0.0f) return;
+    const v3 so = add3(h->pos, scl3(L.dir, OM_RAY_EPS * 2.0f));
+    const float max_t = L.infinite ? OM_RAY_FAR : L.dist - OM_RAY_EPS * 3.0f;
+    if (max_t <= OM_RAY_EPS || om_occluded(c->s, c->bvh, so, L.dir, OM_RAY_EPS, max_t)) return;
+    const float mis = bsdf_avail ? power_heuristic(L.pdf, bpdf) : 1.0f;
+    const v3 direct = scl3(mul3(L.emission, resp), mis / L.pdf);
+    st->radiance = add3(st->radiance, mul3(st->throughput, direct));
+}
+
+static void record_scatter(path_t* st, const hit_t* h, float pdf, int delta) {
+    st->prev_pos = h->pos;
+    st->prev_n = h->sn;
+    st->prev_pdf = pdf;
+    st->prev_mat = h->mat_type;
+    st->prev_prim = h->prim;
+    st->prev_valid = 1;
+    st->prev_delta = delta;
+}
+
+static float schlick(float cosine, float ri) {
+    float r0 = (1.0f - ri) / (1.0f + ri);
+    r0 *= r0;
+    const float x = 1.0f - cosine;
+    const float x2 = x * x;
+    return r0 + (1.0f - r0) * (x2 * x2 * x);
+}
+
+/* programs.cu:2463-2565 */
+static void sample_bsdf(const ctx_t* c, const hit_t* h, const float r[4], path_t* st) {
+    if (h->mat_type == RTO_MAT_DIFFUSE_LIGHT) { st->alive = 0; return; }
+    if (h->mat_type == RTO_MAT_LAMBERTIAN) {
+        v3 sc = add3(h->sn, om_sphere_dir(r[0], r[1]));
+        if (near_zero3(sc)) sc = h->sn;
+        st->o = add3(h->pos, scl3(h->sn, OM_RAY_EPS));
+        st->d = norm3(sc);
+        st->throughput = mul3(st->throughput, h->base);
+        const float cs = fmaxf(dot3(h->sn, st->d), 0.0f);
+        record_scatter(st, h, cs / OM_PI, 0);
+        return;
+    }
+    if (h->mat_type == RTO_MAT_METAL) {
+        const v3 refl = reflect3(norm3(st->d), h->sn);
+        const v3 ball = scl3(om_sphere_dir(r[0], r[1]), om_cbrt01(r[2]));
+        const v3 fz = scl3(ball, fminf(fmaxf(h->fuzz, 0.0f), 1.0f));
+        const v3 sc = norm3(add3(refl, fz));
+        if (dot3(sc, h->sn) <= 0.0f) { st->alive = 0; return; }
+        st->o = add3(h->pos, scl3(h->sn, OM_RAY_EPS));
+        st->d = sc;
+        st->throughput = mul3(st->throughput, h->base);
+        record_scatter(st, h, 0.0f, 1);
+        return;
+    }
+    if (h->mat_type == RTO_MAT_DIELECTRIC) {
+        const float ratio = h->front ? (1.0f / fmaxf(h->ior, 1e-4f)) : fmaxf(h->ior, 1e-4f);
+        const v3 unit = norm3(st->d);
+        const float ct = fminf(dot3(neg3(unit), h->sn), 1.0f);
+        const float stt = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+        const int cannot = ratio * stt > 1.0f;
+        v3 dir;
+        if (cannot || schlick(ct, ratio) > r[0]) dir = reflect3(unit, h->sn);
+        else dir = refract3(unit, h->sn, ratio);
+        st->o = add3(h->pos, scl3(dir, OM_RAY_EPS));
+        st->d = norm3(dir);
+        st->throughput = mul3(st->throughput, h->base);
+        record_scatter(st, h, 0.0f, 1);
+        return;
+    }
+    if (h->mat_type == RTO_MAT_OPENPBR) {
+        const rto_openpbr* p = &c->s->openpbr[h->openpbr];
+        rto_openpbr_sample smp;
+        om_openpbr_sample(p, h->gn, neg3(norm3(st->d)), r[0], r[1], r[2], &smp);
+        if (!smp.valid) { st->alive = 0; return; }
+        const v3 dir = ld3(smp.wi);
+        st->o = add3(h->pos, scl3(dir, OM_RAY_EPS));
+        st->d = norm3(dir);
+        st->throughput = mul3(st->throughput, ld3(smp.weight));
+        record_scatter(st, h, smp.pdf, smp.delta != 0);
+        return;
+    }
+    if (h->mat_type == RTO_MAT_ISOTROPIC) {
+        const v3 dir = om_sphere_dir(r[0], r[1]);
+        st->o = add3(h->pos, scl3(dir, OM_RAY_EPS));
+        st->d = dir;
+        st->throughput = mul3(st->throughput, h->base);
+        record_scatter(st, h, UNIFORM_SPHERE_PDF, 0);
+        return;
+    }
+    st->alive = 0;
+}
+
+typedef struct { v3 radiance, normal, albedo; float depth; int aux; } path_out;
+
+/* programs.cu:1828-1953 (one sample of one pixel) */
+static void trace_one(const ctx_t* c, const rto_camera* cam, int cam_index, const rto_profile* pr,
+    uint32_t stream, int x, int y, int sample, path_out* out, rto_render_stats* stats) {
+    const rto_scene* s = c->s;
+    const uint32_t pixel = (uint32_t)(y * cam->width + x);
+    double px, py;
+    pixel_position(x, y, cam_index, sample, stream, pr->pixel_jitter, cam->width, &px, &py);
+    path_t st;
+    memset(&st, 0, sizeof(st));
+    float o[3], d[3];
+    rto_camera_ray(cam, px, py, o, d);
+    st.o = ld3(o);
+    st.d = ld3(d);
+    st.throughput = mk3(1, 1, 1);
+    st.radiance = mk3(0, 0, 0);
+    st.prev_delta = 1;
+    st.prev_prim = -1;
+    st.prev_mat = -1;
+    st.alive = 1;
+    out->aux = 0;
+    out->normal = mk3(0, 0, 1);
+    out->albedo = mk3(0, 0, 0);
+    out->depth = 0.0f;
+    const int max_b = pr->max_bounces > 0 ? pr->max_bounces : 1;
+    for (int b = 0; b < max_b && st.alive; ++b) {
+        float rl[4], rb[4];
+        om_rng4(pixel, (uint32_t)sample, (uint32_t)cam_index, (uint32_t)b, OM_G_LIGHT, stream, rl);
+        om_rng4(pixel, (uint32_t)sample, (uint32_t)cam_index, (uint32_t)b, OM_G_BSDF, stream, rb);
+        float t, u, v;
+        int prim;
+        if (stats) stats->rays_closest++;
+        const int found = om_closest(s, c->bvh, st.o, st.d, OM_RAY_EPS, OM_RAY_FAR, &t, &prim, &u, &v);
+        if (!found) {
+            const float w = emission_mis(c, &st, NULL, 1);
+            st.radiance = add3(st.radiance, mul3(st.throughput, scl3(background_at(s, st.d), w)));
+            break;
+        }
+        hit_t h;
+        make_hit(s, st.o, st.d, t, prim, &h);
+        if (b == 0) {
+            out->normal = h.sn;
+            out->albedo = h.base;
+            out->depth = h.t;
+            out->aux = 1;
+        }
+        const float ew = emission_mis(c, &st, &h, 0);
+        st.radiance = add3(st.radiance, mul3(st.throughput, scl3(h.emission, ew)));
+        const int bsdf_avail = b + 1 < max_b;
+        if (stats && h.mat_type != RTO_MAT_DIFFUSE_LIGHT && c->light_count > 0) stats->rays_shadow++;
+        direct_light(c, &h, rl, bsdf_avail, &st);
+        if (!bsdf_avail) break;
+        sample_bsdf(c, &h, rb, &st);
+        if (st.alive && b >= pr->rr_start_bounce) {
+            const float p = fminf(fmaxf(maxc3(st.throughput), 0.05f), 0.95f);
+            if (rb[3] > p) { st.alive = 0; break; }
+            st.throughput = div3(st.throughput, p);
+        }
+    }
+    out->radiance = st.radiance;
+}
+
+void rto_trace_path(const rto_scene* s, const rto_bvh* bvh, const rto_camera* cam, int cam_index,
+    const rto_profile* pr, uint32_t stream, int x, int y, int sample, float radiance[3]) {
+    ctx_t c;
+    if (ctx_init(&c, s, bvh) != 0) return;
+    path_out po;
+    trace_one(&c, cam, cam_index, pr, stream, x, y, sample, &po, NULL);
+    radiance[0] = po.radiance.x; radiance[1] = po.radiance.y; radiance[2] = po.radiance.z;
+    ctx_free(&c);
+}
+
+/* --------------------------------------------------------------- render */
+typedef struct {
+    const ctx_t* c; const rto_camera* cam; int cam_index; const rto_profile* pr; uint32_t stream;
+    float *beauty, *normal, *albedo, *depth;
+    int row_begin, row_end;
+    volatile int next_row;
+    pthread_mutex_t lock;
+    rto_render_stats stats;
+} render_job;
+
+static void* render_worker(void* arg) {
+    render_job* j = (render_job*)arg;
+    const rto_camera* cam = j->cam;
+    const int spp = j->pr->samples_per_pixel > 0 ? j->pr->samples_per_pixel : 1;
+    rto_render_stats local = {0, 0, 0};
+    while (1) {
+        pthread_mutex_lock(&j->lock);
+        const int y = j->next_row++;
+        pthread_mutex_unlock(&j->lock);
+        if (y >= j->row_end) break;
+        for (int x = 0; x < cam->width; ++x) {
+            v3 beauty = mk3(0, 0, 0), nsum = mk3(0, 0, 0), asum = mk3(0, 0, 0);
+            float dsum = 0.0f;
+            int aux = 0;
+            for (int s = 0; s < spp; ++s) {
+                path_out po;
+                trace_one(j->c, cam, j->cam_index, j->pr, j->stream, x, y, s, &po, &local);
+                local.paths++;
+                beauty = add3(beauty, po.radiance);
+                if (po.aux) {
+                    nsum = add3(nsum, po.normal);
+                    asum = add3(asum, po.albedo);
+                    dsum += po.depth;
+                    ++aux;
+                }
+            }
+            /* programs.cu:1944-1952 + store_output :1773-1805 */
+            const float inv_spp = 1.0f / (float)spp;
+            const v3 bavg = scl3(beauty, inv_spp);
+            const float inv_aux = aux > 0 ? 1.0f / (float)aux : 0.0f;
+            const v3 navg = aux > 0 ? norm3(scl3(nsum, inv_aux)) : mk3(0, 0, 1);
+            const v3 aavg = aux > 0 ? scl3(asum, inv_aux) : mk3(0, 0, 0);
+            const float davg = aux > 0 ? dsum * inv_aux : 0.0f;
+            const size_t p = (size_t)y * (size_t)cam->width + (size_t)x;
+            if (j->beauty) { float* q = j->beauty + 4 * p; q[0] = bavg.x; q[1] = bavg.y; q[2] = bavg.z; q[3] = 1.0f; }
+            if (j->normal) {
+                const v3 e = clamp3(add3(scl3(navg, 0.5f), mk3(0.5f, 0.5f, 0.5f)), 0.0f, 1.0f);
+                float* q = j->normal + 4 * p; q[0] = e.x; q[1] = e.y; q[2] = e.z; q[3] = 1.0f;
+            }
+            if (j->albedo) {
+                const v3 e = clamp3(aavg, 0.0f, 1.0f);
+                float* q = j->albedo + 4 * p; q[0] = e.x; q[1] = e.y; q[2] = e.z; q[3] = 1.0f;
+            }
+            if (j->depth) j->depth[p] = davg;
+        }
+    }
+    pthread_mutex_lock(&j->lock);
+    j->stats.paths += local.paths;
+    j->stats.rays_closest += local.rays_closest;
+    j->stats.rays_shadow += local.rays_shadow;
+    pthread_mutex_unlock(&j->lock);
+    return NULL;
+}
+
+int rto_render_camera(const rto_scene* s, const rto_bvh* bvh, const rto_camera* cam, int cam_index,
+    const rto_profile* pr, uint32_t stream, float* beauty, float* normal, float* albedo,
+    float* depth, int threads, int row_begin, int row_end, rto_render_stats* stats) {
+    static ctx_t c;  /* large; one render at a time per process */
+    static pthread_mutex_t once = PTHREAD_MUTEX_INITIALIZER;
+    pthread_mutex_lock(&once);
+    if (ctx_init(&c, s, bvh) != 0) { pthread_mutex_unlock(&once); return -1; }
+    if (row_end <= 0 || row_end > cam->height) row_end = cam->height;
+    if (row_begin < 0) row_begin = 0;
+    render_job j;
+    memset(&j, 0, sizeof(j));
+    j.c = &c; j.cam = cam; j.cam_index = cam_index; j.pr = pr; j.stream = stream;
+    j.beauty = beauty; j.normal = normal; j.albedo = albedo; j.depth = depth;
+    j.row_begin = row_begin; j.row_end = row_end; j.next_row = row_begin;
+    pthread_mutex_init(&j.lock, NULL);
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    for (int t = 0; t < threads; ++t) pthread_create(&th[t], NULL, render_worker, &j);
+    for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+    pthread_mutex_destroy(&j.lock);
+    if (stats) *stats = j.stats;
+    ctx_free(&c);
+    pthread_mutex_unlock(&once);
+    return 0;
+}

@@ -1,0 +1,310 @@
+/*
+ * oracle_scenes.c -- seeded synthetic scenes (independent second copy used
+ * to pin the product's scene generators byte for byte).
+ * TEST INFRASTRUCTURE ONLY (see rt_oracle.h).
+ *
+ * Draws come from PCG32 (O'Neill 2014, pcg32_srandom_r(seed, 54), output
+ * XSH-RR); a uniform float is (x >> 8) * 2^-24.  Draw order is part of the
+ * scene definition.
+ *   random_spheres : ground + 22x22 small spheres + 3 large spheres
+ *   cornell_mesh   : 555-unit Cornell room (5 walls + light quad), icosphere
+ *                    subdivided 6x (81,920 triangles), one OpenPBR material
+ *   instanced_mesh : 1000 rigid instances of a 1024-triangle mesh + 256
+ *                    analytic sphere lights
+ * Camera construction (look-at, vertical fov) is host math in FP64.
+ */
+#include "rt_oracle.h"
+#include "om.h"
+
+#include <stdlib.h>
+
+typedef struct { uint64_t state, inc; } pcg32_t;
+
+uint32_t rto_pcg32_next(uint64_t* state, uint64_t inc) {
+    const uint64_t old = *state;
+    *state = old * 6364136223846793005ULL + inc;
+    const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31u));
+}
+static void pcg_seed(pcg32_t* r, uint64_t seed) {
+    r->state = 0u;
+    r->inc = (54u << 1u) | 1u;
+    rto_pcg32_next(&r->state, r->inc);
+    r->state += seed;
+    rto_pcg32_next(&r->state, r->inc);
+}
+static float pcg_u(pcg32_t* r) { return (float)(rto_pcg32_next(&r->state, r->inc) >> 8) * (1.0f / 16777216.0f); }
+
+typedef struct {
+    rto_scene scene;
+    int32_t camera_count;
+    int32_t pad;
+    rto_camera cameras[4];
+    rto_profile profile;
+} rto_generated;
+
+static void set3(float* d, float x, float y, float z) { d[0] = x; d[1] = y; d[2] = z; }
+
+static void openpbr_defaults(rto_openpbr* m) {
+    memset(m, 0, sizeof(*m));
+    m->base_weight = 1.0f; set3(m->base_color, 0.8f, 0.8f, 0.8f);
+    m->specular_weight = 1.0f; set3(m->specular_color, 1.0f, 1.0f, 1.0f);
+    m->specular_roughness = 0.3f; m->specular_ior = 1.5f;
+    set3(m->transmission_color, 1.0f, 1.0f, 1.0f);
+    m->transmission_dispersion_abbe_number = 20.0f;
+    set3(m->subsurface_color, 0.8f, 0.8f, 0.8f); m->subsurface_radius = 1.0f;
+    set3(m->subsurface_radius_scale, 1.0f, 0.5f, 0.25f);
+    set3(m->fuzz_color, 1.0f, 1.0f, 1.0f); m->fuzz_roughness = 0.5f;
+    set3(m->coat_color, 1.0f, 1.0f, 1.0f); m->coat_ior = 1.6f; m->coat_darkening = 1.0f;
+    m->thin_film_thickness = 0.5f; m->thin_film_ior = 1.4f;
+    set3(m->emission_color, 1.0f, 1.0f, 1.0f);
+    m->geometry_opacity = 1.0f;
+}
+
+/* pinhole32 look-at camera; OpenCV axes (x right, y down, z forward) */
+static void lookat_pinhole(rto_camera* c, int w, int h, double vfov_deg, const double from[3],
+    const double at[3], const double up[3]) {
+    memset(c, 0, sizeof(*c));
+    c->enabled = 1; c->width = w; c->height = h; c->model = RTO_CAM_PINHOLE32;
+    double z[3] = {at[0] - from[0], at[1] - from[1], at[2] - from[2]};
+    double zl = sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+    for (int i = 0; i < 3; ++i) z[i] /= zl;
+    double x[3] = {z[1] * up[2] - z[2] * up[1], z[2] * up[0] - z[0] * up[2], z[0] * up[1] - z[1] * up[0]};
+    double xl = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    for (int i = 0; i < 3; ++i) x[i] /= xl;
+    double y[3] = {z[1] * x[2] - z[2] * x[1], z[2] * x[0] - z[0] * x[2], z[0] * x[1] - z[1] * x[0]};
+    for (int i = 0; i < 3; ++i) {
+        c->origin[i] = from[i]; c->basis_x[i] = x[i]; c->basis_y[i] = y[i]; c->basis_z[i] = z[i];
+    }
+    const double vfov = vfov_deg * 3.14159265358979323846 / 180.0;
+    const double f = 0.5 * (double)h / tan(0.5 * vfov);
+    c->fx = f; c->fy = f; c->cx = 0.5 * (double)w; c->cy = 0.5 * (double)h;
+}
+
+static void profile_defaults(rto_profile* p, int spp, int depth) {
+    memset(p, 0, sizeof(*p));
+    p->samples_per_pixel = spp; p->max_bounces = depth; p->enable_denoise = 0;
+    p->rr_start_bounce = depth; /* no roulette unless asked */
+    p->accumulation_reset_rotation_deg = 2.0; p->accumulation_reset_translation = 0.05;
+    p->restir_initial_candidates = 4; p->restir_temporal_reuse = 1; p->restir_max_history_age = 20;
+    p->restir_max_temporal_candidates = 64; p->restir_spatial_neighbors = 1;
+    p->restir_max_spatial_candidates = 4; p->restir_bias_correction = 1;
+    p->restir_min_analytic_lights = 16; p->pixel_jitter = 1; p->restir_spatial_radius = 30;
+}
+
+static int gen_random_spheres(rto_generated* g, uint64_t seed) {
+    const int n = 1 + 22 * 22 + 3;
+    rto_sphere* sp = (rto_sphere*)calloc((size_t)n, sizeof(rto_sphere));
+    rto_material* mt = (rto_material*)calloc((size_t)n, sizeof(rto_material));
+    pcg32_t r;
+    pcg_seed(&r, seed);
+    int k = 0;
+    set3(sp[k].center, 0.0f, -1000.0f, 0.0f); sp[k].radius = 1000.0f; sp[k].material = k;
+    mt[k].type = RTO_MAT_LAMBERTIAN; set3(mt[k].albedo, 0.5f, 0.5f, 0.5f); mt[k].ior = 1.0f; mt[k].openpbr_index = -1;
+    ++k;
+    for (int a = -11; a < 11; ++a) {
+        for (int b = -11; b < 11; ++b) {
+            const float choose = pcg_u(&r);
+            const float u = pcg_u(&r);
+            const float v = pcg_u(&r);
+            set3(sp[k].center, (float)a + 0.9f * u, 0.2f, (float)b + 0.9f * v);
+            sp[k].radius = 0.2f;
+            sp[k].material = k;
+            mt[k].openpbr_index = -1;
+            mt[k].ior = 1.0f;
+            if (choose < 0.8f) {
+                mt[k].type = RTO_MAT_LAMBERTIAN;
+                for (int ch = 0; ch < 3; ++ch) { const float x0 = pcg_u(&r); const float x1 = pcg_u(&r); mt[k].albedo[ch] = x0 * x1; }
+            } else if (choose < 0.95f) {
+                mt[k].type = RTO_MAT_METAL;
+                for (int ch = 0; ch < 3; ++ch) mt[k].albedo[ch] = 0.5f + 0.5f * pcg_u(&r);
+                mt[k].fuzz = 0.5f * pcg_u(&r);
+            } else {
+                mt[k].type = RTO_MAT_DIELECTRIC;
+                mt[k].ior = 1.5f;
+            }
+            ++k;
+        }
+    }
+    set3(sp[k].center, 0.0f, 1.0f, 0.0f); sp[k].radius = 1.0f; sp[k].material = k;
+    mt[k].type = RTO_MAT_DIELECTRIC; mt[k].ior = 1.5f; mt[k].openpbr_index = -1; ++k;
+    set3(sp[k].center, -4.0f, 1.0f, 0.0f); sp[k].radius = 1.0f; sp[k].material = k;
+    mt[k].type = RTO_MAT_LAMBERTIAN; set3(mt[k].albedo, 0.4f, 0.2f, 0.1f); mt[k].ior = 1.0f; mt[k].openpbr_index = -1; ++k;
+    set3(sp[k].center, 4.0f, 1.0f, 0.0f); sp[k].radius = 1.0f; sp[k].material = k;
+    mt[k].type = RTO_MAT_METAL; set3(mt[k].albedo, 0.7f, 0.6f, 0.5f); mt[k].fuzz = 0.0f; mt[k].ior = 1.0f; mt[k].openpbr_index = -1; ++k;
+    g->scene.sphere_count = n;
+    g->scene.material_count = n;
+    g->scene.spheres = sp;
+    g->scene.materials = mt;
+    g->scene.background_mode = RTO_BG_SKY_GRADIENT;
+    set3(g->scene.background, 1.0f, 1.0f, 1.0f);
+    set3(g->scene.background_top, 0.5f, 0.7f, 1.0f);
+    g->camera_count = 1;
+    const double from[3] = {13, 2, 3}, at[3] = {0, 0, 0}, up[3] = {0, 1, 0};
+    lookat_pinhole(&g->cameras[0], 400, 225, 20.0, from, at, up);
+    profile_defaults(&g->profile, 32, 10);
+    return 0;
+}
+
+/* ------------------------------------------------------------ icosphere */
+typedef struct { uint64_t* keys; int32_t* vals; uint32_t cap; } edge_map;
+
+static int32_t edge_mid(edge_map* m, double (*verts)[3], int* nverts, int32_t a, int32_t b) {
+    const uint32_t lo = (uint32_t)(a < b ? a : b), hi = (uint32_t)(a < b ? b : a);
+    const uint64_t key = ((uint64_t)lo << 32) | hi;
+    uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ULL) >> 40) & (m->cap - 1u);
+    while (m->vals[h] >= 0) {
+        if (m->keys[h] == key) return m->vals[h];
+        h = (h + 1u) & (m->cap - 1u);
+    }
+    double p[3];
+    for (int i = 0; i < 3; ++i) p[i] = 0.5 * (verts[a][i] + verts[b][i]);
+    const double l = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    const int32_t id = (*nverts)++;
+    for (int i = 0; i < 3; ++i) verts[id][i] = p[i] / l;
+    m->keys[h] = key;
+    m->vals[h] = id;
+    return id;
+}
+
+/* unit icosphere with `levels` 4:1 subdivisions; outward winding */
+int rto_icosphere(int levels, double (**verts_out)[3], int32_t (**faces_out)[3], int* nv_out, int* nf_out) {
+    const double t = (1.0 + sqrt(5.0)) / 2.0;
+    const double base[12][3] = {{-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0}, {0, -1, t}, {0, 1, t},
+        {0, -1, -t}, {0, 1, -t}, {t, 0, -1}, {t, 0, 1}, {-t, 0, -1}, {-t, 0, 1}};
+    const int32_t bf[20][3] = {{0, 11, 5}, {0, 5, 1}, {0, 1, 7}, {0, 7, 10}, {0, 10, 11}, {1, 5, 9},
+        {5, 11, 4}, {11, 10, 2}, {10, 7, 6}, {7, 1, 8}, {3, 9, 4}, {3, 4, 2}, {3, 2, 6}, {3, 6, 8},
+        {3, 8, 9}, {4, 9, 5}, {2, 4, 11}, {6, 2, 10}, {8, 6, 7}, {9, 8, 1}};
+    int nf = 20;
+    for (int l = 0; l < levels; ++l) nf *= 4;
+    const int maxv = nf / 2 + 2;
+    double (*verts)[3] = (double (*)[3])malloc(sizeof(double) * 3 * (size_t)maxv);
+    int32_t (*faces)[3] = (int32_t (*)[3])malloc(sizeof(int32_t) * 3 * (size_t)nf);
+    int nv = 12;
+    for (int i = 0; i < 12; ++i) {
+        const double l = sqrt(base[i][0] * base[i][0] + base[i][1] * base[i][1] + base[i][2] * base[i][2]);
+        for (int k = 0; k < 3; ++k) verts[i][k] = base[i][k] / l;
+    }
+    int cur = 20;
+    for (int i = 0; i < 20; ++i) {
+        int32_t a = bf[i][0], b = bf[i][1], c = bf[i][2];
+        double e1[3], e2[3], cr[3], ce[3];
+        for (int k = 0; k < 3; ++k) { e1[k] = verts[b][k] - verts[a][k]; e2[k] = verts[c][k] - verts[a][k]; ce[k] = verts[a][k] + verts[b][k] + verts[c][k]; }
+        cr[0] = e1[1] * e2[2] - e1[2] * e2[1]; cr[1] = e1[2] * e2[0] - e1[0] * e2[2]; cr[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        if (cr[0] * ce[0] + cr[1] * ce[1] + cr[2] * ce[2] < 0.0) { const int32_t tmp = b; b = c; c = tmp; }
+        faces[i][0] = a; faces[i][1] = b; faces[i][2] = c;
+    }
+    int32_t (*next)[3] = (int32_t (*)[3])malloc(sizeof(int32_t) * 3 * (size_t)nf);
+    for (int l = 0; l < levels; ++l) {
+        edge_map m;
+        m.cap = 1u;
+        while (m.cap < (uint32_t)(cur * 4)) m.cap <<= 1;
+        m.keys = (uint64_t*)calloc(m.cap, sizeof(uint64_t));
+        m.vals = (int32_t*)malloc(sizeof(int32_t) * m.cap);
+        for (uint32_t i = 0; i < m.cap; ++i) m.vals[i] = -1;
+        int nn = 0;
+        for (int f = 0; f < cur; ++f) {
+            const int32_t a = faces[f][0], b = faces[f][1], c = faces[f][2];
+            const int32_t ab = edge_mid(&m, verts, &nv, a, b);
+            const int32_t bc = edge_mid(&m, verts, &nv, b, c);
+            const int32_t ca = edge_mid(&m, verts, &nv, c, a);
+            next[nn][0] = a; next[nn][1] = ab; next[nn][2] = ca; ++nn;
+            next[nn][0] = b; next[nn][1] = bc; next[nn][2] = ab; ++nn;
+            next[nn][0] = c; next[nn][1] = ca; next[nn][2] = bc; ++nn;
+            next[nn][0] = ab; next[nn][1] = bc; next[nn][2] = ca; ++nn;
+        }
+        memcpy(faces, next, sizeof(int32_t) * 3 * (size_t)nn);
+        cur = nn;
+        free(m.keys);
+        free(m.vals);
+    }
+    free(next);
+    *verts_out = verts; *faces_out = faces; *nv_out = nv; *nf_out = cur;
+    return 0;
+}
+
+static void quad_set(rto_quad* q, float ox, float oy, float oz, float ux, float uy, float uz,
+    float vx, float vy, float vz, int mat) {
+    set3(q->origin, ox, oy, oz); set3(q->edge_u, ux, uy, uz); set3(q->edge_v, vx, vy, vz); q->material = mat;
+}
+
+static int gen_cornell_mesh(rto_generated* g, uint64_t seed) {
+    pcg32_t r;
+    pcg_seed(&r, seed);
+    rto_material* mt = (rto_material*)calloc(5, sizeof(rto_material));
+    for (int i = 0; i < 5; ++i) { mt[i].openpbr_index = -1; mt[i].ior = 1.0f; }
+    set3(mt[0].albedo, 0.65f, 0.05f, 0.05f);
+    set3(mt[1].albedo, 0.73f, 0.73f, 0.73f);
+    set3(mt[2].albedo, 0.12f, 0.45f, 0.15f);
+    mt[3].type = RTO_MAT_DIFFUSE_LIGHT; set3(mt[3].emission, 15.0f, 15.0f, 15.0f);
+    set3(mt[3].albedo, 0.0f, 0.0f, 0.0f);
+    mt[4].type = RTO_MAT_OPENPBR; mt[4].openpbr_index = 0;
+    rto_openpbr* op = (rto_openpbr*)calloc(1, sizeof(rto_openpbr));
+    openpbr_defaults(op);
+    for (int ch = 0; ch < 3; ++ch) op->base_color[ch] = pcg_u(&r);
+    op->specular_roughness = 0.05f + 0.85f * pcg_u(&r);
+    op->base_metalness = pcg_u(&r) < 0.5f ? 0.0f : 1.0f;
+    op->specular_ior = 1.5f;
+    op->coat_weight = 0.5f * pcg_u(&r);
+    for (int ch = 0; ch < 3; ++ch) mt[4].albedo[ch] = op->base_color[ch];
+
+    rto_quad* q = (rto_quad*)calloc(6, sizeof(rto_quad));
+    quad_set(&q[0], 555, 0, 0, 0, 555, 0, 0, 0, 555, 2);
+    quad_set(&q[1], 0, 0, 0, 0, 555, 0, 0, 0, 555, 0);
+    quad_set(&q[2], 343, 554, 332, -130, 0, 0, 0, 0, -105, 3);
+    quad_set(&q[3], 0, 0, 0, 555, 0, 0, 0, 0, 555, 1);
+    quad_set(&q[4], 555, 555, 555, -555, 0, 0, 0, 0, -555, 1);
+    quad_set(&q[5], 0, 0, 555, 555, 0, 0, 0, 555, 0, 1);
+
+    double (*verts)[3];
+    int32_t (*faces)[3];
+    int nv, nf;
+    rto_icosphere(6, &verts, &faces, &nv, &nf);
+    const double center[3] = {277.5, 277.5, 277.5};
+    const double radius = 150.0;
+    float (*pos)[3] = (float (*)[3])malloc(sizeof(float) * 3 * (size_t)nv);
+    for (int i = 0; i < nv; ++i) {
+        const float u = pcg_u(&r);
+        const double rr = radius * (1.0 + 0.05 * (2.0 * (double)u - 1.0));
+        for (int k = 0; k < 3; ++k) pos[i][k] = (float)(center[k] + verts[i][k] * rr);
+    }
+    rto_triangle* tr = (rto_triangle*)calloc((size_t)nf, sizeof(rto_triangle));
+    for (int f = 0; f < nf; ++f) {
+        memcpy(tr[f].p0, pos[faces[f][0]], 12);
+        memcpy(tr[f].p1, pos[faces[f][1]], 12);
+        memcpy(tr[f].p2, pos[faces[f][2]], 12);
+        tr[f].material = 4;
+    }
+    free(pos); free(verts); free(faces);
+    g->scene.quad_count = 6; g->scene.quads = q;
+    g->scene.triangle_count = nf; g->scene.triangles = tr;
+    g->scene.material_count = 5; g->scene.materials = mt;
+    g->scene.openpbr_count = 1; g->scene.openpbr = op;
+    g->scene.background_mode = RTO_BG_CONSTANT;
+    set3(g->scene.background, 0.0f, 0.0f, 0.0f);
+    g->camera_count = 4;
+    const double up[3] = {0, 1, 0};
+    for (int c = 0; c < 4; ++c) {
+        const double th = (double)c * 0.5 * 3.14159265358979323846;
+        const double from[3] = {center[0] + 800.0 * sin(th), center[1], center[2] - 800.0 * cos(th)};
+        lookat_pinhole(&g->cameras[c], 640, 480, 40.0, from, center, up);
+    }
+    profile_defaults(&g->profile, 16, 8);
+    return 0;
+}
+
+int rto_generate_scene(int config, uint64_t seed, rto_generated* g) {
+    memset(g, 0, sizeof(*g));
+    switch (config) {
+        case 0: return gen_random_spheres(g, seed);
+        case 1: return gen_cornell_mesh(g, seed);
+        default: return -1;
+    }
+}
+
+void rto_free_generated(rto_generated* g) {
+    free((void*)g->scene.spheres); free((void*)g->scene.quads); free((void*)g->scene.triangles);
+    free((void*)g->scene.materials); free((void*)g->scene.openpbr); free((void*)g->scene.analytic_lights);
+    memset(g, 0, sizeof(*g));
+}

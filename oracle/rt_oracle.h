@@ -1,0 +1,207 @@
+/*
+ * rt_oracle.h -- CPU oracle for the path-tracing core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * product path is compared against; it is never linked into, loaded by, or
+ * called from the product library.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it.
+ *
+ * It restates, in plain C11 on FP32 (FP64 for camera math, as the reference
+ * does), the semantics of the reference's realtime GPU integrator
+ *   src/realtime/gpu/programs.cu:1828-1953  (radiance_kernel)
+ * on a counter-based RNG keyed on (pixel, sample, camera, bounce, dimension),
+ * a Morton-code LBVH (Karras 2012) and brute-force closest hit.  Every
+ * function cites the reference file:line it follows.
+ *
+ * Memory layouts of the structs below are identical to the C-ABI structs in
+ * include/rt/rt_hip.h, so a test hands the same numpy buffers to both.
+ *
+ * Pinning (how each part is checked independently of the HIP path):
+ *   RNG, sincos/cbrt  : closed-form/integer spec; KATs + math identities in
+ *                       tests/test_oracle_math.py
+ *   LBVH              : structural invariants + brute-force equivalence of
+ *                       closest hits (tests/test_oracle_bvh.py)
+ *   intersectors      : closed forms (tests/test_oracle_bvh.py)
+ *   light sampling    : reference KATs of tests/test_light_sampling.cpp
+ *   camera models     : reference KATs of tests/test_camera_models.cpp
+ *   integrator        : furnace / analytic-radiance closed forms
+ *                       (tests/test_oracle_render.py)
+ *   OpenPBR subset    : oracle/_ref (reference header compiled here) +
+ *                       reference KATs of tests/test_openpbr_core.cpp
+ *   ReSTIR reservoir  : reference KATs of tests/test_restir_di.cpp
+ */
+#ifndef RT_ORACLE_H
+#define RT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- scene ------------------------------------------------------------ */
+typedef struct { float center[3]; float radius; int32_t material; } rto_sphere;
+typedef struct { float origin[3]; float edge_u[3]; float edge_v[3]; int32_t material; } rto_quad;
+typedef struct { float p0[3]; float p1[3]; float p2[3]; int32_t material; } rto_triangle;
+
+enum { RTO_MAT_LAMBERTIAN = 0, RTO_MAT_METAL = 1, RTO_MAT_DIELECTRIC = 2,
+       RTO_MAT_DIFFUSE_LIGHT = 3, RTO_MAT_ISOTROPIC = 4, RTO_MAT_OPENPBR = 5 };
+
+typedef struct {
+    int32_t type;
+    float albedo[3];
+    float emission[3];
+    float fuzz;
+    float ior;
+    int32_t openpbr_index;
+} rto_material;
+
+/* Constant-input OpenPBR parameters (reference OpenPbrCoreMaterial,
+ * src/common/openpbr_core.h:33-77, same field order, 51 floats/ints). */
+typedef struct {
+    float base_weight; float base_color[3]; float base_diffuse_roughness; float base_metalness;
+    float specular_weight; float specular_color[3]; float specular_roughness; float specular_ior;
+    float specular_roughness_anisotropy;
+    float transmission_weight; float transmission_color[3]; float transmission_depth;
+    float transmission_dispersion_scale; float transmission_dispersion_abbe_number;
+    float subsurface_weight; float subsurface_color[3]; float subsurface_radius;
+    float subsurface_radius_scale[3]; float subsurface_scatter_anisotropy;
+    float fuzz_weight; float fuzz_color[3]; float fuzz_roughness;
+    float coat_weight; float coat_color[3]; float coat_roughness; float coat_roughness_anisotropy;
+    float coat_ior; float coat_darkening;
+    float thin_film_weight; float thin_film_thickness; float thin_film_ior;
+    float emission_luminance; float emission_color[3];
+    float geometry_opacity; int32_t geometry_thin_walled;
+} rto_openpbr;
+
+enum { RTO_ALIGHT_SPHERE = 0 };
+typedef struct {
+    int32_t type;
+    float position[3];
+    float radius;
+    float radiance[3];
+} rto_analytic_light;
+
+enum { RTO_BG_CONSTANT = 0, RTO_BG_SKY_GRADIENT = 1 };
+
+typedef struct {
+    int32_t sphere_count, quad_count, triangle_count, material_count;
+    int32_t openpbr_count, analytic_light_count;
+    int32_t background_mode;
+    float background[3];     /* constant colour, or gradient colour at d.y = -1 */
+    float background_top[3]; /* gradient colour at d.y = +1 */
+    const rto_sphere* spheres;
+    const rto_quad* quads;
+    const rto_triangle* triangles;
+    const rto_material* materials;
+    const rto_openpbr* openpbr;
+    const rto_analytic_light* analytic_lights;
+} rto_scene;
+
+/* ---- cameras / profile ------------------------------------------------ */
+enum { RTO_CAM_PINHOLE32 = 0, RTO_CAM_EQUI62_LUT1D = 1 };
+typedef struct {
+    int32_t enabled, width, height, model;
+    double origin[3], basis_x[3], basis_y[3], basis_z[3];
+    double fx, fy, cx, cy;
+    double k1, k2, k3, p1, p2;
+    double radial[6], tangential[2];
+    double lut_step;
+    double lut[1024];
+} rto_camera;
+
+typedef struct {
+    int32_t samples_per_pixel, max_bounces, enable_denoise, rr_start_bounce;
+    double accumulation_reset_rotation_deg, accumulation_reset_translation;
+    int32_t enable_restir_di, restir_initial_candidates, restir_temporal_reuse,
+        restir_max_history_age, restir_max_temporal_candidates, restir_spatial_neighbors,
+        restir_max_spatial_candidates, restir_bias_correction, restir_min_analytic_lights;
+    int32_t pixel_jitter;          /* extension: jitter primary rays inside the pixel */
+    int32_t restir_spatial_radius; /* extension: spatial reuse radius in pixels */
+    int32_t pad0;
+} rto_profile;
+
+/* ---- BVH (binary LBVH, child boxes stored in the parent) -------------- */
+/* 64-byte node: child 0 box, child 1 box, two child references.
+ * A child reference with bit 31 set is a leaf: low 31 bits = position in
+ * the Morton-sorted primitive order.  0xFFFFFFFF = empty child. */
+typedef struct {
+    float lo0[3]; float hi0[3];
+    float lo1[3]; float hi1[3];
+    uint32_t child[2];
+    uint32_t parent;
+    uint32_t pad;
+} rto_bvh_node;
+
+typedef struct {
+    int32_t prim_count;       /* spheres + quads + triangles */
+    int32_t node_count;       /* max(prim_count - 1, 1) if prim_count > 0 */
+    rto_bvh_node* nodes;
+    uint32_t* sorted_prims;   /* leaf position -> global primitive id */
+    uint32_t* morton;         /* sorted Morton codes */
+    int32_t max_depth;        /* internal levels on the deepest root-leaf path */
+} rto_bvh;
+
+typedef struct { float o[3]; float tmin; float d[3]; float tmax; } rto_ray;
+typedef struct { float t; int32_t prim; float u; float v; } rto_hit;
+
+/* ---- API -------------------------------------------------------------- */
+int rto_version(void);
+
+/* RNG: counter hash (Jarzynski & Olano 2020, pcg4d) */
+void rto_pcg4d(uint32_t v[4]);
+void rto_rng4(uint32_t pixel, uint32_t sample, uint32_t camera, uint32_t bounce, uint32_t group,
+    uint32_t stream, float out[4]);
+void rto_sincos_turn(float u, float* s, float* c);
+float rto_cbrt01(float x);
+
+/* scene generators (seeded PCG32) */
+uint32_t rto_pcg32_next(uint64_t* state, uint64_t inc);
+int rto_build_lbvh(const rto_scene* scene, rto_bvh* out);   /* allocates */
+void rto_free_lbvh(rto_bvh* bvh);
+int rto_bvh_validate(const rto_scene* scene, const rto_bvh* bvh);
+uint32_t rto_expand_bits10(uint32_t v);
+
+/* closest hit / any hit; counters (node visits, prim tests) may be NULL */
+void rto_trace_closest(const rto_scene* scene, const rto_bvh* bvh, const rto_ray* rays, int n,
+    rto_hit* hits, uint32_t* node_visits, uint32_t* prim_tests, int threads);
+void rto_trace_any(const rto_scene* scene, const rto_bvh* bvh, const rto_ray* rays, int n,
+    int32_t* occluded, uint32_t* node_visits, uint32_t* prim_tests, int threads);
+void rto_brute_closest(const rto_scene* scene, const rto_ray* rays, int n, rto_hit* hits);
+
+/* cameras */
+void rto_unproject(const rto_camera* cam, double px, double py, float dir_cam[3]);
+void rto_camera_ray(const rto_camera* cam, double px, double py, float origin[3], float dir[3]);
+void rto_project(const rto_camera* cam, const float dir_cam[3], float pixel[2]);
+void rto_make_equi_lut(rto_camera* cam);
+void rto_primary_rays(const rto_camera* cam, int cam_index, int sample, uint32_t stream,
+    int jitter, rto_ray* rays);
+
+/* light distribution (reference packed_scene_preparation.cpp:265-302) */
+typedef struct { int32_t type; int32_t prim; float selection_pdf; float cdf; } rto_light;
+int rto_build_lights(const rto_scene* scene, rto_light* out, int capacity);
+
+/* full render of one camera; outputs are W*H*4 floats (beauty, normal,
+ * albedo) and W*H floats (depth).  Any output may be NULL. */
+typedef struct { uint64_t paths; uint64_t rays_closest; uint64_t rays_shadow; } rto_render_stats;
+int rto_render_camera(const rto_scene* scene, const rto_bvh* bvh, const rto_camera* cam,
+    int cam_index, const rto_profile* profile, uint32_t stream, float* beauty, float* normal,
+    float* albedo, float* depth, int threads, int row_begin, int row_end,
+    rto_render_stats* stats);
+/* per-path radiance of one (pixel, sample): for tests of single paths */
+void rto_trace_path(const rto_scene* scene, const rto_bvh* bvh, const rto_camera* cam,
+    int cam_index, const rto_profile* profile, uint32_t stream, int x, int y, int sample,
+    float radiance[3]);
+
+/* OpenPBR subset (src/common/openpbr_core.h) */
+typedef struct { float value[3]; float pdf; } rto_openpbr_eval;
+typedef struct { float wi[3]; float weight[3]; float pdf; int32_t delta; int32_t valid; int32_t event; } rto_openpbr_sample;
+void rto_openpbr_evaluate(const rto_openpbr* m, const float n[3], const float wo[3],
+    const float wi[3], rto_openpbr_eval* out);
+void rto_openpbr_sample_fn(const rto_openpbr* m, const float n[3], const float wo[3],
+    float u_lobe, float u1, float u2, rto_openpbr_sample* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,260 @@
+"""ray_tracing_amd -- MI355X (gfx950) path-tracing core.
+
+Python view of the C ABI in include/rt/rt_hip.h, used by the tests and by
+bench.py.  The compute path is the in-tree HIP library librt_hip.so; if it is
+missing this module raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import (Camera, CameraRig, FrameTiming, GeneratedScene, Profile, SceneDesc, Tuning,
+                  BvhStats, RAY_DTYPE, HIT_DTYPE, NODE_DTYPE)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librt_hip.so")
+
+RT_OK = 0
+STATUS_NAMES = {0: "RT_OK", 1: "RT_ERROR_INVALID_ARGUMENT", 2: "RT_ERROR_HIP",
+                3: "RT_ERROR_NOT_READY", 4: "RT_ERROR_OUT_OF_MEMORY", 5: "RT_ERROR_UNSUPPORTED"}
+
+# every entry point declared in include/rt/rt_hip.h
+EXPORTED = [
+    "rt_abi_version", "rt_last_error", "rt_device_create", "rt_device_destroy",
+    "rt_device_synchronize", "rt_device_info", "rt_scene_upload", "rt_scene_destroy",
+    "rt_bvh_build", "rt_bvh_download", "rt_trace_closest", "rt_trace_any",
+    "rt_trace_closest_device", "rt_renderer_create", "rt_renderer_destroy", "rt_render_frame",
+    "rt_frame_download", "rt_frame_device_view", "rt_reset_accumulation", "rt_reset_sequence",
+    "rt_restir_diagnostics", "rt_renderer_set_tuning", "rt_renderer_get_tuning",
+    "rt_scene_generate", "rt_scene_generated_free",
+]
+
+
+class RtError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+_lib = None
+
+
+def lib():
+    """Loads librt_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() or "
+                              "python ray_tracing_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        abi.declare(L)
+        _lib = L
+    return _lib
+
+
+def check(status: int):
+    if status != RT_OK:
+        raise RtError(status, lib().rt_last_error().decode())
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Device:
+    def __init__(self, ordinal: int = 0):
+        self.h = C.c_void_p()
+        check(lib().rt_device_create(ordinal, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().rt_device_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(lib().rt_device_synchronize(self.h))
+
+    def info(self):
+        name = C.create_string_buffer(128)
+        cu = C.c_int()
+        tot = C.c_uint64()
+        free = C.c_uint64()
+        check(lib().rt_device_info(self.h, name, 128, C.byref(cu), C.byref(tot), C.byref(free)))
+        return {"name": name.value.decode(), "cu_count": cu.value, "total_bytes": tot.value,
+                "free_bytes": free.value}
+
+
+class Scene:
+    def __init__(self, device: Device, desc: SceneDesc, keepalive=None):
+        self.device = device
+        self.desc = desc
+        self._keep = keepalive
+        self.h = C.c_void_p()
+        check(lib().rt_scene_upload(device.h, C.byref(desc), C.byref(self.h)))
+        self.stats = None
+
+    def close(self):
+        if self.h:
+            lib().rt_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def prim_count(self) -> int:
+        d = self.desc
+        return d.sphere_count + d.quad_count + d.triangle_count
+
+    def build_bvh(self) -> BvhStats:
+        st = BvhStats()
+        check(lib().rt_bvh_build(self.h, C.byref(st)))
+        self.stats = st
+        return st
+
+    def download_bvh(self):
+        n = self.prim_count
+        nodes_n = self.stats.node_count
+        nodes = np.zeros(nodes_n, NODE_DTYPE)
+        sp = np.zeros(n, np.uint32)
+        mc = np.zeros(n, np.uint32)
+        check(lib().rt_bvh_download(self.h, ptr(nodes), ptr(sp), ptr(mc)))
+        return nodes, sp, mc
+
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        check(lib().rt_trace_closest(self.h, ptr(rays), len(rays), ptr(hits)))
+        return hits
+
+    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        occ = np.zeros(len(rays), np.int32)
+        check(lib().rt_trace_any(self.h, ptr(rays), len(rays), ptr(occ)))
+        return occ
+
+
+class Renderer:
+    def __init__(self, device: Device, scene: Scene):
+        self.device = device
+        self.scene = scene
+        self.h = C.c_void_p()
+        check(lib().rt_renderer_create(device.h, scene.h, C.byref(self.h)))
+        self.rig = None
+
+    def close(self):
+        if self.h:
+            lib().rt_renderer_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        t.use_graph = -1
+        t.count_traversal = -1
+        for k, v in kw.items():
+            setattr(t, k, v)
+        check(lib().rt_renderer_set_tuning(self.h, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        check(lib().rt_renderer_get_tuning(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def render(self, rig: CameraRig, profile: Profile, sample_stream: int = 0,
+               timed: bool = True) -> FrameTiming | None:
+        self.rig = rig
+        tm = FrameTiming()
+        check(lib().rt_render_frame(self.h, C.byref(rig), C.byref(profile), sample_stream,
+                                    C.byref(tm) if timed else None))
+        return tm if timed else None
+
+    def download(self, camera: int) -> dict:
+        cam = self.rig.cameras[camera]
+        w, h = cam.width, cam.height
+        out = {"beauty": np.zeros((h, w, 4), np.float32), "normal": np.zeros((h, w, 4), np.float32),
+               "albedo": np.zeros((h, w, 4), np.float32), "depth": np.zeros((h, w), np.float32)}
+        check(lib().rt_frame_download(self.h, camera, ptr(out["beauty"]), ptr(out["normal"]),
+                                      ptr(out["albedo"]), ptr(out["depth"])))
+        return out
+
+
+class Generated:
+    """A seeded benchmark scene from the library's generator (rt_scene_generate)."""
+
+    CONFIGS = {"random_spheres": 0, "cornell_mesh": 1, "instanced_mesh_restir": 2}
+
+    def __init__(self, config, seed: int = 0):
+        if isinstance(config, str):
+            config = self.CONFIGS[config]
+        self.config = config
+        self.p = C.POINTER(GeneratedScene)()
+        check(lib().rt_scene_generate(config, seed, C.byref(self.p)))
+        self.g = self.p.contents
+
+    def __del__(self):
+        try:
+            lib().rt_scene_generated_free(self.p)
+        except Exception:
+            pass
+
+    @property
+    def desc(self) -> SceneDesc:
+        return self.g.scene
+
+    @property
+    def profile(self) -> Profile:
+        p = Profile()
+        C.memmove(C.byref(p), C.byref(self.g.profile), C.sizeof(Profile))
+        return p
+
+    def rig(self, count: int | None = None) -> CameraRig:
+        r = CameraRig()
+        n = self.g.camera_count if count is None else count
+        r.active_count = n
+        for i in range(n):
+            C.memmove(C.byref(r.cameras[i]), C.byref(self.g.cameras[i]), C.sizeof(Camera))
+        return r
+
+    def arrays(self) -> dict:
+        """Copies of the scene arrays as raw numpy byte records."""
+        d = self.g.scene
+
+        def grab(p, n, ctype):
+            if n == 0:
+                return np.zeros(0, dtype=np.dtype((np.void, C.sizeof(ctype))))
+            raw = C.string_at(p, n * C.sizeof(ctype))
+            return np.frombuffer(raw, dtype=np.dtype((np.void, C.sizeof(ctype)))).copy()
+
+        return {
+            "spheres": grab(d.spheres, d.sphere_count, abi.Sphere),
+            "quads": grab(d.quads, d.quad_count, abi.Quad),
+            "triangles": grab(d.triangles, d.triangle_count, abi.Triangle),
+            "materials": grab(d.materials, d.material_count, abi.Material),
+            "openpbr": grab(d.openpbr, d.openpbr_count, abi.OpenPbr),
+            "analytic": grab(d.analytic_lights, d.analytic_light_count, abi.AnalyticLight),
+        }
+
+
+__all__ = ["Device", "Scene", "Renderer", "Generated", "RtError", "lib", "abi", "EXPORTED",
+           "RAY_DTYPE", "HIT_DTYPE", "NODE_DTYPE", "Camera", "CameraRig", "Profile",
+           "FrameTiming", "SceneDesc", "Tuning"]

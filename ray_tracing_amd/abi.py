@@ -1,0 +1,165 @@
+"""ctypes mirror of the POD structs and entry points of include/rt/rt_hip.h."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+f3 = C.c_float * 3
+d3 = C.c_double * 3
+
+
+class Sphere(C.Structure):
+    _fields_ = [("center", f3), ("radius", C.c_float), ("material", C.c_int32)]
+
+
+class Quad(C.Structure):
+    _fields_ = [("origin", f3), ("edge_u", f3), ("edge_v", f3), ("material", C.c_int32)]
+
+
+class Triangle(C.Structure):
+    _fields_ = [("p0", f3), ("p1", f3), ("p2", f3), ("material", C.c_int32)]
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_int32), ("albedo", f3), ("emission", f3), ("fuzz", C.c_float),
+                ("ior", C.c_float), ("openpbr_index", C.c_int32)]
+
+
+class OpenPbr(C.Structure):
+    _fields_ = [
+        ("base_weight", C.c_float), ("base_color", f3), ("base_diffuse_roughness", C.c_float),
+        ("base_metalness", C.c_float), ("specular_weight", C.c_float), ("specular_color", f3),
+        ("specular_roughness", C.c_float), ("specular_ior", C.c_float),
+        ("specular_roughness_anisotropy", C.c_float), ("transmission_weight", C.c_float),
+        ("transmission_color", f3), ("transmission_depth", C.c_float),
+        ("transmission_dispersion_scale", C.c_float),
+        ("transmission_dispersion_abbe_number", C.c_float), ("subsurface_weight", C.c_float),
+        ("subsurface_color", f3), ("subsurface_radius", C.c_float),
+        ("subsurface_radius_scale", f3), ("subsurface_scatter_anisotropy", C.c_float),
+        ("fuzz_weight", C.c_float), ("fuzz_color", f3), ("fuzz_roughness", C.c_float),
+        ("coat_weight", C.c_float), ("coat_color", f3), ("coat_roughness", C.c_float),
+        ("coat_roughness_anisotropy", C.c_float), ("coat_ior", C.c_float),
+        ("coat_darkening", C.c_float), ("thin_film_weight", C.c_float),
+        ("thin_film_thickness", C.c_float), ("thin_film_ior", C.c_float),
+        ("emission_luminance", C.c_float), ("emission_color", f3),
+        ("geometry_opacity", C.c_float), ("geometry_thin_walled", C.c_int32),
+    ]
+
+
+class AnalyticLight(C.Structure):
+    _fields_ = [("type", C.c_int32), ("position", f3), ("radius", C.c_float), ("radiance", f3)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("sphere_count", C.c_int32), ("quad_count", C.c_int32), ("triangle_count", C.c_int32),
+        ("material_count", C.c_int32), ("openpbr_count", C.c_int32),
+        ("analytic_light_count", C.c_int32), ("background_mode", C.c_int32),
+        ("background", f3), ("background_top", f3),
+        ("spheres", C.c_void_p), ("quads", C.c_void_p), ("triangles", C.c_void_p),
+        ("materials", C.c_void_p), ("openpbr", C.c_void_p), ("analytic_lights", C.c_void_p),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("model", C.c_int32),
+        ("origin", d3), ("basis_x", d3), ("basis_y", d3), ("basis_z", d3),
+        ("fx", C.c_double), ("fy", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
+        ("k1", C.c_double), ("k2", C.c_double), ("k3", C.c_double), ("p1", C.c_double),
+        ("p2", C.c_double), ("radial", C.c_double * 6), ("tangential", C.c_double * 2),
+        ("lut_step", C.c_double), ("lut", C.c_double * 1024),
+    ]
+
+
+class CameraRig(C.Structure):
+    _fields_ = [("active_count", C.c_int32), ("pad", C.c_int32), ("cameras", Camera * 4)]
+
+
+class Profile(C.Structure):
+    _fields_ = [
+        ("samples_per_pixel", C.c_int32), ("max_bounces", C.c_int32),
+        ("enable_denoise", C.c_int32), ("rr_start_bounce", C.c_int32),
+        ("accumulation_reset_rotation_deg", C.c_double),
+        ("accumulation_reset_translation", C.c_double),
+        ("enable_restir_di", C.c_int32), ("restir_initial_candidates", C.c_int32),
+        ("restir_temporal_reuse", C.c_int32), ("restir_max_history_age", C.c_int32),
+        ("restir_max_temporal_candidates", C.c_int32), ("restir_spatial_neighbors", C.c_int32),
+        ("restir_max_spatial_candidates", C.c_int32), ("restir_bias_correction", C.c_int32),
+        ("restir_min_analytic_lights", C.c_int32), ("pixel_jitter", C.c_int32),
+        ("restir_spatial_radius", C.c_int32), ("pad0", C.c_int32),
+    ]
+
+
+class BvhStats(C.Structure):
+    _fields_ = [("prim_count", C.c_int32), ("node_count", C.c_int32), ("max_depth", C.c_int32),
+                ("pad", C.c_int32), ("build_ms", C.c_double)]
+
+
+class FrameTiming(C.Structure):
+    _fields_ = [("render_ms", C.c_double), ("denoise_ms", C.c_double),
+                ("download_ms", C.c_double), ("extend_ms", C.c_double),
+                ("shade_ms", C.c_double), ("connect_ms", C.c_double), ("other_ms", C.c_double),
+                ("paths", C.c_uint64), ("extend_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("extend_launches", C.c_uint64), ("node_visits", C.c_uint64),
+                ("prim_tests", C.c_uint64), ("sphere_tests", C.c_uint64)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("wavefront_paths", C.c_int32), ("trace_block", C.c_int32),
+                ("trace_blocks_per_cu", C.c_int32), ("use_graph", C.c_int32),
+                ("count_traversal", C.c_int32)]
+
+
+class RestirStats(C.Structure):
+    _fields_ = [("valid_reservoirs", C.c_uint64), ("temporal_candidates", C.c_uint64),
+                ("spatial_candidates", C.c_uint64), ("bias_correction_sources", C.c_uint64)]
+
+
+class DeviceFrameView(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("beauty", C.c_void_p),
+                ("normal", C.c_void_p), ("albedo", C.c_void_p), ("depth", C.c_void_p)]
+
+
+class GeneratedScene(C.Structure):
+    _fields_ = [("scene", SceneDesc), ("camera_count", C.c_int32), ("pad", C.c_int32),
+                ("cameras", Camera * 4), ("profile", Profile)]
+
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<i4"), ("u", "<f4"), ("v", "<f4")])
+NODE_DTYPE = np.dtype([("lo0", "<f4", 3), ("hi0", "<f4", 3), ("lo1", "<f4", 3),
+                       ("hi1", "<f4", 3), ("child", "<u4", 2), ("parent", "<u4"),
+                       ("pad", "<u4")])
+
+
+def declare(L):
+    P = C.POINTER
+    vp = C.c_void_p
+    L.rt_abi_version.restype = C.c_int
+    L.rt_last_error.restype = C.c_char_p
+    L.rt_device_create.argtypes = [C.c_int, P(vp)]
+    L.rt_device_destroy.argtypes = [vp]
+    L.rt_device_synchronize.argtypes = [vp]
+    L.rt_device_info.argtypes = [vp, C.c_char_p, C.c_size_t, P(C.c_int), P(C.c_uint64),
+                                 P(C.c_uint64)]
+    L.rt_scene_upload.argtypes = [vp, P(SceneDesc), P(vp)]
+    L.rt_scene_destroy.argtypes = [vp]
+    L.rt_bvh_build.argtypes = [vp, P(BvhStats)]
+    L.rt_bvh_download.argtypes = [vp, vp, vp, vp]
+    L.rt_trace_closest.argtypes = [vp, vp, C.c_int64, vp]
+    L.rt_trace_any.argtypes = [vp, vp, C.c_int64, vp]
+    L.rt_trace_closest_device.argtypes = [vp, vp, C.c_int64, vp, P(C.c_double)]
+    L.rt_renderer_create.argtypes = [vp, vp, P(vp)]
+    L.rt_renderer_destroy.argtypes = [vp]
+    L.rt_render_frame.argtypes = [vp, P(CameraRig), P(Profile), C.c_uint32, P(FrameTiming)]
+    L.rt_frame_download.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+    L.rt_frame_device_view.argtypes = [vp, C.c_int, P(DeviceFrameView)]
+    L.rt_reset_accumulation.argtypes = [vp]
+    L.rt_reset_sequence.argtypes = [vp, C.c_uint32]
+    L.rt_restir_diagnostics.argtypes = [vp, C.c_int, P(RestirStats)]
+    L.rt_renderer_set_tuning.argtypes = [vp, P(Tuning)]
+    L.rt_renderer_get_tuning.argtypes = [vp, P(Tuning)]
+    L.rt_scene_generate.argtypes = [C.c_int, C.c_uint64, P(P(GeneratedScene))]
+    L.rt_scene_generated_free.argtypes = [P(GeneratedScene)]

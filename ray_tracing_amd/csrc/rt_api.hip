@@ -1,0 +1,831 @@
+// rt_api.hip -- C ABI of the gfx950 path tracer (declared in include/rt/rt_hip.h).
+//
+// Host orchestration: device and stream set-up, scene upload with light CDF
+// construction, LBVH build, trace test hooks, and the per-frame wavefront
+// launch sequence.  Errors are returned as rt_status with a thread-local
+// message (the reference throws std::runtime_error / std::invalid_argument
+// with the same kind of text, src/realtime/gpu/optix_renderer.cpp:31-59).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rt_host.h"
+#include "rt_kernels.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+rt_status fail(rt_status s, const std::string& msg) {
+    g_error = msg;
+    return s;
+}
+
+#define RT_HIP(expr)                                                                        \
+    do {                                                                                    \
+        const hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                               \
+            return fail(RT_ERROR_HIP, std::string("HIP runtime failure at ") + #expr + ": " \
+                                          + hipGetErrorString(e_));                        \
+    } while (0)
+
+#define RT_REQUIRE(cond, status, msg) \
+    do {                              \
+        if (!(cond)) return fail(status, msg); \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t reserve(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        release();
+        if (count == 0) count = 1;
+        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count);
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+};
+
+template <typename T>
+hipError_t upload(DevBuf<T>& b, const T* src, size_t count, hipStream_t st) {
+    hipError_t e = b.reserve(count);
+    if (e != hipSuccess || count == 0) return e;
+    return hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, st);
+}
+
+float lum_clamped(const float* c) {
+    const float r = std::max(c[0], 0.0f), g = std::max(c[1], 0.0f), b = std::max(c[2], 0.0f);
+    return std::max(0.0f, 0.2126f * r + 0.7152f * g + 0.0722f * b);
+}
+
+struct V3 {
+    float x, y, z;
+};
+V3 v3(const float* p) { return V3{p[0], p[1], p[2]}; }
+V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3 cross(V3 a, V3 b) { return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+float length(V3 v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+
+}  // namespace
+
+struct rt_device {
+    int ordinal = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    std::string name;
+    size_t total_mem = 0;
+};
+
+struct rt_scene {
+    rt_device* dev = nullptr;
+    int32_t sphere_count = 0, quad_count = 0, triangle_count = 0, material_count = 0;
+    int32_t openpbr_count = 0, analytic_count = 0;
+    int32_t background_mode = 0;
+    float bg[3] = {0, 0, 0}, bg_top[3] = {0, 0, 0};
+    DevBuf<rt_sphere> spheres;
+    DevBuf<rt_quad> quads;
+    DevBuf<rt_triangle> triangles;
+    DevBuf<rt_material> materials;
+    DevBuf<rt_openpbr_material> openpbr;
+    DevBuf<rt_analytic_light> analytic;
+    std::vector<rtk::LightRec> host_lights;
+    DevBuf<rtk::LightRec> lights;
+    DevBuf<int32_t> light_of_prim;
+    int32_t env_light = -1;
+    // BVH
+    bool bvh_ready = false;
+    int32_t prim_count = 0;
+    int32_t node_count = 0;
+    int32_t max_depth = 0;
+    DevBuf<float4> nodes;
+    DevBuf<float4> leaves;
+    DevBuf<uint32_t> sorted;
+    DevBuf<uint32_t> morton;
+
+    rtk::SceneView view() const {
+        rtk::SceneView v{};
+        v.nodes = nodes.p;
+        v.leaves = leaves.p;
+        v.node_count = node_count;
+        v.prim_count = bvh_ready ? prim_count : 0;
+        v.spheres = spheres.p;
+        v.quads = quads.p;
+        v.triangles = triangles.p;
+        v.sphere_count = sphere_count;
+        v.quad_count = quad_count;
+        v.triangle_count = triangle_count;
+        v.material_count = material_count;
+        v.materials = materials.p;
+        v.openpbr = openpbr.p;
+        v.openpbr_count = openpbr_count;
+        v.light_count = static_cast<int32_t>(host_lights.size());
+        v.lights = lights.p;
+        v.light_of_prim = light_of_prim.p;
+        v.env_light = env_light;
+        v.background_mode = background_mode;
+        for (int i = 0; i < 3; ++i) {
+            v.bg[i] = bg[i];
+            v.bg_top[i] = bg_top[i];
+        }
+        v.analytic = analytic.p;
+        v.analytic_count = analytic_count;
+        v.stack_depth = std::max(max_depth, 1);
+        return v;
+    }
+};
+
+struct rt_renderer {
+    rt_device* dev = nullptr;
+    rt_scene* scene = nullptr;
+    rt_tuning tuning{};
+    uint32_t sequence_seed = 0;
+    // rig
+    rt_camera* host_cams = nullptr;  // pinned
+    DevBuf<rt_camera> cams;
+    int camera_count = 0;
+    int pixel_offset[5] = {0, 0, 0, 0, 0};
+    int widths[4] = {0, 0, 0, 0};
+    int heights[4] = {0, 0, 0, 0};
+    // frame outputs
+    DevBuf<float4> beauty, normal, albedo;
+    DevBuf<float> depth;
+    // wavefront state
+    DevBuf<float4> ray_o, ray_d, hit, throughput, radiance, prev_pos, prev_n, aux_normal, aux_albedo;
+    DevBuf<int32_t> prev_flags;
+    DevBuf<uint32_t> queue0, queue1;
+    DevBuf<float4> shadow_o, shadow_d, shadow_c;
+    DevBuf<uint32_t> counters;
+    DevBuf<float4> acc_beauty, acc_normal, acc_albedo;
+    size_t path_capacity = 0;
+    size_t pixel_capacity = 0;
+    std::vector<hipEvent_t> events;
+    bool rendered = false;
+
+    ~rt_renderer() {
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        if (host_cams) (void)hipHostFree(host_cams);
+    }
+
+    rtk::WavefrontBuffers buffers() {
+        rtk::WavefrontBuffers b{};
+        b.ray_o = ray_o.p;
+        b.ray_d = ray_d.p;
+        b.hit = hit.p;
+        b.throughput = throughput.p;
+        b.radiance = radiance.p;
+        b.prev_pos = prev_pos.p;
+        b.prev_n = prev_n.p;
+        b.prev_flags = prev_flags.p;
+        b.aux_normal = aux_normal.p;
+        b.aux_albedo = aux_albedo.p;
+        b.queue[0] = queue0.p;
+        b.queue[1] = queue1.p;
+        b.shadow_o = shadow_o.p;
+        b.shadow_d = shadow_d.p;
+        b.shadow_c = shadow_c.p;
+        b.counters = counters.p;
+        b.acc_beauty = acc_beauty.p;
+        b.acc_normal = acc_normal.p;
+        b.acc_albedo = acc_albedo.p;
+        return b;
+    }
+
+    hipError_t ensure_paths(size_t q) {
+        if (q <= path_capacity) return hipSuccess;
+        hipError_t e;
+        for (DevBuf<float4>* b : {&ray_o, &ray_d, &hit, &throughput, &radiance, &prev_pos, &prev_n,
+                 &aux_normal, &aux_albedo, &shadow_o, &shadow_d, &shadow_c}) {
+            if ((e = b->reserve(q)) != hipSuccess) return e;
+        }
+        if ((e = prev_flags.reserve(q)) != hipSuccess) return e;
+        if ((e = queue0.reserve(q)) != hipSuccess) return e;
+        if ((e = queue1.reserve(q)) != hipSuccess) return e;
+        path_capacity = q;
+        return hipSuccess;
+    }
+
+    hipError_t ensure_pixels(size_t p) {
+        if (p <= pixel_capacity) return hipSuccess;
+        hipError_t e;
+        for (DevBuf<float4>* b : {&beauty, &normal, &albedo, &acc_beauty, &acc_normal, &acc_albedo}) {
+            if ((e = b->reserve(p)) != hipSuccess) return e;
+        }
+        if ((e = depth.reserve(p)) != hipSuccess) return e;
+        pixel_capacity = p;
+        return hipSuccess;
+    }
+};
+
+extern "C" {
+
+int rt_abi_version(void) { return RT_ABI_VERSION; }
+
+const char* rt_last_error(void) { return g_error.c_str(); }
+
+rt_status rt_device_create(int ordinal, rt_device** out) {
+    RT_REQUIRE(out != nullptr, RT_ERROR_INVALID_ARGUMENT, "rt_device_create requires an output handle");
+    int count = 0;
+    RT_HIP(hipGetDeviceCount(&count));
+    RT_REQUIRE(ordinal >= 0 && ordinal < count, RT_ERROR_INVALID_ARGUMENT,
+        "device ordinal " + std::to_string(ordinal) + " out of range [0, " + std::to_string(count) + ")");
+    RT_HIP(hipSetDevice(ordinal));
+    hipDeviceProp_t prop;
+    RT_HIP(hipGetDeviceProperties(&prop, ordinal));
+    auto* d = new rt_device();
+    d->ordinal = ordinal;
+    d->cu_count = prop.multiProcessorCount;
+    d->name = prop.gcnArchName;
+    d->total_mem = prop.totalGlobalMem;
+    const hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete d;
+        return fail(RT_ERROR_HIP, std::string("HIP runtime failure at hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = d;
+    return RT_OK;
+}
+
+rt_status rt_device_destroy(rt_device* dev) {
+    if (!dev) return RT_OK;
+    (void)hipSetDevice(dev->ordinal);
+    if (dev->stream) (void)hipStreamDestroy(dev->stream);
+    delete dev;
+    return RT_OK;
+}
+
+rt_status rt_device_synchronize(rt_device* dev) {
+    RT_REQUIRE(dev != nullptr, RT_ERROR_INVALID_ARGUMENT, "null device");
+    RT_HIP(hipStreamSynchronize(dev->stream));
+    return RT_OK;
+}
+
+rt_status rt_device_info(rt_device* dev, char* name, size_t name_len, int* cu_count,
+    uint64_t* total_bytes, uint64_t* free_bytes) {
+    RT_REQUIRE(dev != nullptr, RT_ERROR_INVALID_ARGUMENT, "null device");
+    if (name && name_len > 0) {
+        std::strncpy(name, dev->name.c_str(), name_len - 1);
+        name[name_len - 1] = '\0';
+    }
+    if (cu_count) *cu_count = dev->cu_count;
+    if (total_bytes) *total_bytes = dev->total_mem;
+    if (free_bytes) {
+        size_t f = 0, t = 0;
+        RT_HIP(hipMemGetInfo(&f, &t));
+        *free_bytes = f;
+    }
+    return RT_OK;
+}
+
+// light distribution: power x area CDF over emissive primitives plus the
+// environment (src/realtime/gpu/packed_scene_preparation.cpp:250-302)
+static void build_lights(const rt_scene_desc* d, std::vector<rtk::LightRec>& out) {
+    const float kPi = 3.14159265358979323846f;
+    out.clear();
+    auto emission_lum = [&](int m) -> float {
+        if (m < 0 || m >= d->material_count) return 0.0f;
+        const rt_material& mt = d->materials[m];
+        if (mt.type == RT_MAT_DIFFUSE_LIGHT) return lum_clamped(mt.emission);
+        if (mt.type != RT_MAT_OPENPBR || mt.openpbr_index < 0 || mt.openpbr_index >= d->openpbr_count)
+            return 0.0f;
+        const rt_openpbr_material& p = d->openpbr[mt.openpbr_index];
+        (void)p;
+        return 0.0f;
+    };
+    auto append = [&](int type, int prim, float w) {
+        if (std::isfinite(w) && w > 1e-12f) out.push_back(rtk::LightRec{type, prim, w, 0.0f});
+    };
+    for (int i = 0; i < d->sphere_count; ++i) {
+        const float area = 4.0f * kPi * d->spheres[i].radius * d->spheres[i].radius;
+        append(0, i, area * emission_lum(d->spheres[i].material));
+    }
+    for (int i = 0; i < d->quad_count; ++i) {
+        const float area = length(cross(v3(d->quads[i].edge_u), v3(d->quads[i].edge_v)));
+        append(1, i, area * emission_lum(d->quads[i].material));
+    }
+    for (int i = 0; i < d->triangle_count; ++i) {
+        const rt_triangle& t = d->triangles[i];
+        const float area = 0.5f * length(cross(sub(v3(t.p1), v3(t.p0)), sub(v3(t.p2), v3(t.p0))));
+        append(2, i, area * emission_lum(t.material));
+    }
+    float mean[3];
+    if (d->background_mode == RT_BG_SKY_GRADIENT) {
+        for (int i = 0; i < 3; ++i) mean[i] = (d->background[i] + d->background_top[i]) * 0.5f;
+    } else {
+        for (int i = 0; i < 3; ++i) mean[i] = d->background[i];
+    }
+    append(3, -1, 4.0f * kPi * lum_clamped(mean));
+    float total = 0.0f;
+    for (const auto& l : out) total += l.selection_pdf;
+    float cdf = 0.0f;
+    for (auto& l : out) {
+        l.selection_pdf /= total;
+        cdf += l.selection_pdf;
+        l.cdf = cdf;
+    }
+    if (!out.empty()) out.back().cdf = 1.0f;
+}
+
+rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out) {
+    RT_REQUIRE(dev && d && out, RT_ERROR_INVALID_ARGUMENT, "rt_scene_upload requires device, description and output");
+    RT_REQUIRE(d->sphere_count >= 0 && d->quad_count >= 0 && d->triangle_count >= 0
+                   && d->material_count >= 0 && d->openpbr_count >= 0 && d->analytic_light_count >= 0,
+        RT_ERROR_INVALID_ARGUMENT, "scene counts must be non-negative");
+    RT_REQUIRE((d->sphere_count == 0 || d->spheres) && (d->quad_count == 0 || d->quads)
+                   && (d->triangle_count == 0 || d->triangles)
+                   && (d->material_count == 0 || d->materials)
+                   && (d->openpbr_count == 0 || d->openpbr)
+                   && (d->analytic_light_count == 0 || d->analytic_lights),
+        RT_ERROR_INVALID_ARGUMENT, "scene arrays must be non-null when their count is positive");
+    const int64_t prims = (int64_t)d->sphere_count + d->quad_count + d->triangle_count;
+    RT_REQUIRE(prims < (int64_t(1) << 30), RT_ERROR_INVALID_ARGUMENT, "scene primitive count exceeds 2^30");
+    for (int i = 0; i < d->openpbr_count; ++i) {
+        const rt_openpbr_material& m = d->openpbr[i];
+        RT_REQUIRE(m.transmission_weight <= 0.0f && m.subsurface_weight <= 0.0f && m.fuzz_weight <= 0.0f
+                       && m.thin_film_weight <= 0.0f && m.specular_roughness_anisotropy == 0.0f
+                       && m.coat_roughness_anisotropy == 0.0f && m.geometry_thin_walled == 0,
+            RT_ERROR_UNSUPPORTED,
+            "OpenPBR material " + std::to_string(i)
+                + " uses a lobe outside the supported constant-input subset (base, specular, metal, coat, emission)");
+    }
+    RT_HIP(hipSetDevice(dev->ordinal));
+    auto* s = new rt_scene();
+    s->dev = dev;
+    s->sphere_count = d->sphere_count;
+    s->quad_count = d->quad_count;
+    s->triangle_count = d->triangle_count;
+    s->material_count = d->material_count;
+    s->openpbr_count = d->openpbr_count;
+    s->analytic_count = d->analytic_light_count;
+    s->background_mode = d->background_mode;
+    for (int i = 0; i < 3; ++i) {
+        s->bg[i] = d->background[i];
+        s->bg_top[i] = d->background_top[i];
+    }
+    s->prim_count = static_cast<int32_t>(prims);
+    build_lights(d, s->host_lights);
+    std::vector<int32_t> lop(static_cast<size_t>(std::max<int64_t>(prims, 1)), -1);
+    for (size_t i = 0; i < s->host_lights.size(); ++i) {
+        const rtk::LightRec& l = s->host_lights[i];
+        if (l.type == 3) {
+            s->env_light = static_cast<int32_t>(i);
+            continue;
+        }
+        const int base = l.type == 0 ? 0 : (l.type == 1 ? d->sphere_count : d->sphere_count + d->quad_count);
+        lop[static_cast<size_t>(base + l.prim)] = static_cast<int32_t>(i);
+    }
+    hipStream_t st = dev->stream;
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = upload(s->spheres, d->spheres, (size_t)d->sphere_count, st);
+    if (e == hipSuccess) e = upload(s->quads, d->quads, (size_t)d->quad_count, st);
+    if (e == hipSuccess) e = upload(s->triangles, d->triangles, (size_t)d->triangle_count, st);
+    if (e == hipSuccess) e = upload(s->materials, d->materials, (size_t)d->material_count, st);
+    if (e == hipSuccess) e = upload(s->openpbr, d->openpbr, (size_t)d->openpbr_count, st);
+    if (e == hipSuccess) e = upload(s->analytic, d->analytic_lights, (size_t)d->analytic_light_count, st);
+    if (e == hipSuccess) e = upload(s->lights, s->host_lights.data(), s->host_lights.size(), st);
+    if (e == hipSuccess) e = upload(s->light_of_prim, lop.data(), lop.size(), st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(RT_ERROR_HIP, std::string("HIP runtime failure at scene upload: ") + hipGetErrorString(e));
+    }
+    *out = s;
+    return RT_OK;
+}
+
+rt_status rt_scene_destroy(rt_scene* scene) {
+    if (!scene) return RT_OK;
+    (void)hipSetDevice(scene->dev->ordinal);
+    (void)hipStreamSynchronize(scene->dev->stream);
+    delete scene;
+    return RT_OK;
+}
+
+rt_status rt_bvh_build(rt_scene* s, rt_bvh_stats* stats) {
+    RT_REQUIRE(s != nullptr, RT_ERROR_INVALID_ARGUMENT, "null scene");
+    RT_HIP(hipSetDevice(s->dev->ordinal));
+    const int n = s->prim_count;
+    hipStream_t st = s->dev->stream;
+    s->bvh_ready = false;
+    s->node_count = n > 1 ? n - 1 : (n == 1 ? 1 : 0);
+    s->max_depth = 0;
+    double build_ms = 0.0;
+    if (n > 0) {
+        RT_HIP(s->nodes.reserve((size_t)s->node_count * 4));
+        RT_HIP(s->leaves.reserve((size_t)n * 3));
+        RT_HIP(s->sorted.reserve((size_t)n));
+        RT_HIP(s->morton.reserve((size_t)n));
+        DevBuf<float4> pbox;
+        DevBuf<uint32_t> k0, k1, v0, v1, counts, leaf_parent, flags, cb;
+        const size_t nblocks = ((size_t)n + 1023) / 1024;
+        RT_HIP(pbox.reserve((size_t)n * 2));
+        RT_HIP(k0.reserve((size_t)n));
+        RT_HIP(k1.reserve((size_t)n));
+        RT_HIP(v0.reserve((size_t)n));
+        RT_HIP(v1.reserve((size_t)n));
+        RT_HIP(counts.reserve(16 * nblocks));
+        RT_HIP(leaf_parent.reserve((size_t)n));
+        RT_HIP(flags.reserve((size_t)n));
+        RT_HIP(cb.reserve(8));
+        rtk::BvhBuildScratch w{};
+        w.pbox = pbox.p;
+        w.keys[0] = k0.p;
+        w.keys[1] = k1.p;
+        w.vals[0] = v0.p;
+        w.vals[1] = v1.p;
+        w.counts = counts.p;
+        w.leaf_parent = leaf_parent.p;
+        w.flags = flags.p;
+        w.cb = cb.p;
+        hipEvent_t e0, e1;
+        RT_HIP(hipEventCreate(&e0));
+        RT_HIP(hipEventCreate(&e1));
+        rtk::SceneView v = s->view();
+        RT_HIP(hipEventRecord(e0, st));
+        RT_HIP(rtk::bvh_build(v, n, w, s->nodes.p, s->leaves.p, s->sorted.p, s->morton.p, st));
+        RT_HIP(hipEventRecord(e1, st));
+        uint32_t hcb[8] = {0};
+        RT_HIP(hipMemcpyAsync(hcb, cb.p, sizeof(hcb), hipMemcpyDeviceToHost, st));
+        RT_HIP(hipStreamSynchronize(st));
+        float ms = 0.0f;
+        RT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        build_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        s->max_depth = static_cast<int32_t>(hcb[6]);
+        RT_REQUIRE(s->max_depth >= 1 && s->max_depth <= 64, RT_ERROR_HIP,
+            "LBVH build produced an invalid depth " + std::to_string(s->max_depth));
+    }
+    s->bvh_ready = true;
+    if (stats) {
+        stats->prim_count = n;
+        stats->node_count = s->node_count;
+        stats->max_depth = s->max_depth;
+        stats->pad = 0;
+        stats->build_ms = build_ms;
+    }
+    return RT_OK;
+}
+
+rt_status rt_bvh_download(rt_scene* s, rt_bvh_node* nodes, uint32_t* sorted_prims, uint32_t* morton) {
+    RT_REQUIRE(s != nullptr, RT_ERROR_INVALID_ARGUMENT, "null scene");
+    RT_REQUIRE(s->bvh_ready, RT_ERROR_NOT_READY, "rt_bvh_download requires rt_bvh_build() first");
+    RT_HIP(hipSetDevice(s->dev->ordinal));
+    hipStream_t st = s->dev->stream;
+    if (s->prim_count > 0) {
+        if (nodes)
+            RT_HIP(hipMemcpyAsync(nodes, s->nodes.p, sizeof(rt_bvh_node) * (size_t)s->node_count,
+                hipMemcpyDeviceToHost, st));
+        if (sorted_prims)
+            RT_HIP(hipMemcpyAsync(sorted_prims, s->sorted.p, sizeof(uint32_t) * (size_t)s->prim_count,
+                hipMemcpyDeviceToHost, st));
+        if (morton)
+            RT_HIP(hipMemcpyAsync(morton, s->morton.p, sizeof(uint32_t) * (size_t)s->prim_count,
+                hipMemcpyDeviceToHost, st));
+    }
+    RT_HIP(hipStreamSynchronize(st));
+    return RT_OK;
+}
+
+static rt_status trace_host(rt_scene* s, const rt_ray* rays, int64_t n, void* out, bool any) {
+    RT_REQUIRE(s != nullptr, RT_ERROR_INVALID_ARGUMENT, "null scene");
+    RT_REQUIRE(n >= 0 && (n == 0 || (rays && out)), RT_ERROR_INVALID_ARGUMENT, "invalid ray arrays");
+    RT_REQUIRE(s->bvh_ready, RT_ERROR_NOT_READY, "tracing requires rt_bvh_build() first");
+    if (n == 0) return RT_OK;
+    RT_HIP(hipSetDevice(s->dev->ordinal));
+    hipStream_t st = s->dev->stream;
+    DevBuf<rt_ray> dr;
+    RT_HIP(upload(dr, rays, (size_t)n, st));
+    const rtk::SceneView v = s->view();
+    if (any) {
+        DevBuf<int32_t> dh;
+        RT_HIP(dh.reserve((size_t)n));
+        RT_HIP(rtk::trace_any_launch(v, v.stack_depth, dr.p, n, dh.p, st));
+        RT_HIP(hipMemcpyAsync(out, dh.p, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st));
+        RT_HIP(hipStreamSynchronize(st));
+    } else {
+        DevBuf<rt_hit> dh;
+        RT_HIP(dh.reserve((size_t)n));
+        RT_HIP(rtk::trace_closest_launch(v, v.stack_depth, dr.p, n, dh.p, st));
+        RT_HIP(hipMemcpyAsync(out, dh.p, sizeof(rt_hit) * (size_t)n, hipMemcpyDeviceToHost, st));
+        RT_HIP(hipStreamSynchronize(st));
+    }
+    return RT_OK;
+}
+
+rt_status rt_trace_closest(rt_scene* s, const rt_ray* rays, int64_t n, rt_hit* hits) {
+    return trace_host(s, rays, n, hits, false);
+}
+
+rt_status rt_trace_any(rt_scene* s, const rt_ray* rays, int64_t n, int32_t* occluded) {
+    return trace_host(s, rays, n, occluded, true);
+}
+
+rt_status rt_trace_closest_device(rt_scene* s, const rt_ray* d_rays, int64_t n, rt_hit* d_hits,
+    double* kernel_ms) {
+    RT_REQUIRE(s != nullptr, RT_ERROR_INVALID_ARGUMENT, "null scene");
+    RT_REQUIRE(s->bvh_ready, RT_ERROR_NOT_READY, "tracing requires rt_bvh_build() first");
+    RT_REQUIRE(n >= 0 && (n == 0 || (d_rays && d_hits)), RT_ERROR_INVALID_ARGUMENT, "invalid ray arrays");
+    RT_HIP(hipSetDevice(s->dev->ordinal));
+    hipStream_t st = s->dev->stream;
+    hipEvent_t e0, e1;
+    RT_HIP(hipEventCreate(&e0));
+    RT_HIP(hipEventCreate(&e1));
+    const rtk::SceneView v = s->view();
+    RT_HIP(hipEventRecord(e0, st));
+    RT_HIP(rtk::trace_closest_launch(v, v.stack_depth, d_rays, n, d_hits, st));
+    RT_HIP(hipEventRecord(e1, st));
+    RT_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    RT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (kernel_ms) *kernel_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return RT_OK;
+}
+
+rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out) {
+    RT_REQUIRE(dev && scene && out, RT_ERROR_INVALID_ARGUMENT, "rt_renderer_create requires device, scene and output");
+    RT_HIP(hipSetDevice(dev->ordinal));
+    auto* r = new rt_renderer();
+    r->dev = dev;
+    r->scene = scene;
+    r->tuning.wavefront_paths = 1 << 22;
+    r->tuning.trace_block = 128;
+    r->tuning.trace_blocks_per_cu = 8;
+    r->tuning.use_graph = 0;
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&r->host_cams), sizeof(rt_camera) * 4,
+        hipHostMallocDefault);
+    if (e == hipSuccess) e = r->cams.reserve(4);
+    if (e == hipSuccess) e = r->counters.reserve(16);
+    if (e != hipSuccess) {
+        delete r;
+        return fail(RT_ERROR_HIP, std::string("HIP runtime failure at renderer allocation: ") + hipGetErrorString(e));
+    }
+    *out = r;
+    return RT_OK;
+}
+
+rt_status rt_renderer_destroy(rt_renderer* r) {
+    if (!r) return RT_OK;
+    (void)hipSetDevice(r->dev->ordinal);
+    (void)hipStreamSynchronize(r->dev->stream);
+    delete r;
+    return RT_OK;
+}
+
+rt_status rt_renderer_set_tuning(rt_renderer* r, const rt_tuning* t) {
+    RT_REQUIRE(r && t, RT_ERROR_INVALID_ARGUMENT, "null renderer or tuning");
+    if (t->wavefront_paths > 0) r->tuning.wavefront_paths = t->wavefront_paths;
+    if (t->trace_block > 0) {
+        RT_REQUIRE(t->trace_block == 64 || t->trace_block == 128 || t->trace_block == 256,
+            RT_ERROR_INVALID_ARGUMENT, "trace_block must be 64, 128 or 256");
+        r->tuning.trace_block = t->trace_block;
+    }
+    if (t->trace_blocks_per_cu > 0) r->tuning.trace_blocks_per_cu = t->trace_blocks_per_cu;
+    if (t->use_graph >= 0) r->tuning.use_graph = t->use_graph;
+    if (t->count_traversal >= 0) r->tuning.count_traversal = t->count_traversal;
+    return RT_OK;
+}
+
+rt_status rt_renderer_get_tuning(rt_renderer* r, rt_tuning* t) {
+    RT_REQUIRE(r && t, RT_ERROR_INVALID_ARGUMENT, "null renderer or tuning");
+    *t = r->tuning;
+    return RT_OK;
+}
+
+// render_request_validation.cpp:15-66
+static rt_status validate_request(const rt_camera_rig* rig, const rt_profile* p) {
+    RT_REQUIRE(rig && p, RT_ERROR_INVALID_ARGUMENT, "render request requires a camera rig and a profile");
+    RT_REQUIRE(rig->active_count >= 1 && rig->active_count <= 4, RT_ERROR_INVALID_ARGUMENT,
+        "camera rig active_count must be in [1, 4]");
+    for (int c = 0; c < rig->active_count; ++c) {
+        const rt_camera& cam = rig->cameras[c];
+        RT_REQUIRE(cam.enabled != 0, RT_ERROR_INVALID_ARGUMENT,
+            "camera " + std::to_string(c) + " is disabled");
+        RT_REQUIRE(cam.width > 0 && cam.height > 0, RT_ERROR_INVALID_ARGUMENT,
+            "camera " + std::to_string(c) + " has invalid resolution");
+        RT_REQUIRE(cam.model == RT_CAM_PINHOLE32 || cam.model == RT_CAM_EQUI62_LUT1D,
+            RT_ERROR_INVALID_ARGUMENT, "camera " + std::to_string(c) + " has an unknown model");
+        RT_REQUIRE((int64_t)cam.width * cam.height <= (int64_t(1) << 24), RT_ERROR_INVALID_ARGUMENT,
+            "camera " + std::to_string(c) + " exceeds 2^24 pixels");
+    }
+    RT_REQUIRE(p->max_bounces <= 64, RT_ERROR_INVALID_ARGUMENT, "max_bounces must be <= 64");
+    RT_REQUIRE(p->samples_per_pixel <= (1 << 20), RT_ERROR_INVALID_ARGUMENT, "samples_per_pixel must be <= 2^20");
+    return RT_OK;
+}
+
+rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_profile* p,
+    uint32_t sample_stream, rt_frame_timing* timing) {
+    RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
+    const rt_status vs = validate_request(rig, p);
+    if (vs != RT_OK) return vs;
+    rt_scene* s = r->scene;
+    RT_REQUIRE(s->bvh_ready, RT_ERROR_NOT_READY, "render_prepared_device requires prepare_scene() first");
+    RT_HIP(hipSetDevice(r->dev->ordinal));
+    hipStream_t st = r->dev->stream;
+
+    // rig
+    r->camera_count = rig->active_count;
+    int total = 0;
+    for (int c = 0; c < 4; ++c) {
+        r->pixel_offset[c] = total;
+        if (c < rig->active_count) {
+            r->widths[c] = rig->cameras[c].width;
+            r->heights[c] = rig->cameras[c].height;
+            total += rig->cameras[c].width * rig->cameras[c].height;
+        } else {
+            r->widths[c] = 0;
+            r->heights[c] = 0;
+        }
+    }
+    r->pixel_offset[4] = total;
+    RT_HIP(hipStreamSynchronize(st));  // host_cams may still feed a previous copy
+    std::memcpy(r->host_cams, rig->cameras, sizeof(rt_camera) * 4);
+    RT_HIP(hipMemcpyAsync(r->cams.p, r->host_cams, sizeof(rt_camera) * 4, hipMemcpyHostToDevice, st));
+
+    const int spp = p->samples_per_pixel > 0 ? p->samples_per_pixel : 1;
+    const int max_b = p->max_bounces > 0 ? p->max_bounces : 1;
+    const size_t P = (size_t)total;
+    const size_t cap_target = (size_t)std::max(r->tuning.wavefront_paths, 1);
+    const int layers = (int)std::max<size_t>(1, std::min<size_t>((size_t)spp, cap_target / P));
+    RT_HIP(r->ensure_pixels(P));
+    RT_HIP(r->ensure_paths(P * (size_t)layers));
+
+    rtk::CameraView cv{};
+    cv.cams = r->cams.p;
+    cv.count = rig->active_count;
+    for (int c = 0; c < 5; ++c) cv.pixel_offset[c] = r->pixel_offset[c];
+    for (int c = 0; c < 4; ++c) cv.width[c] = r->widths[c];
+
+    rtk::FrameParams fp{};
+    fp.spp = spp;
+    fp.max_bounces = max_b;
+    fp.rr_start_bounce = p->rr_start_bounce;
+    fp.jitter = p->pixel_jitter;
+    fp.stream = sample_stream;
+    fp.total_pixels = total;
+
+    const rtk::SceneView sv = s->view();
+    rtk::WavefrontBuffers b = r->buffers();
+    const int block = r->tuning.trace_block;
+    const rtk::StageLaunch trace_l{r->dev->cu_count * r->tuning.trace_blocks_per_cu, block};
+    const rtk::StageLaunch shade_l{r->dev->cu_count * 4, 256};
+
+    const int chunks = (spp + layers - 1) / layers;
+    const size_t stage_events = (size_t)chunks * (size_t)max_b * 6 + 4;
+    const bool timed = timing != nullptr;
+    if (timed) {
+        while (r->events.size() < stage_events) {
+            hipEvent_t e;
+            RT_HIP(hipEventCreate(&e));
+            r->events.push_back(e);
+        }
+    }
+    size_t ev = 2;
+    auto mark = [&](void) -> hipError_t {
+        if (!timed) return hipSuccess;
+        return hipEventRecord(r->events[ev++], st);
+    };
+    if (timed) RT_HIP(hipEventRecord(r->events[0], st));
+    RT_HIP(rtk::wf_reset_counters(b.counters, st));
+    RT_HIP(rtk::wf_clear_accum(total, b, st));
+    for (int s0 = 0; s0 < spp; s0 += layers) {
+        fp.layer_count = std::min(layers, spp - s0);
+        fp.first_sample = s0;
+        const int paths = fp.layer_count * total;
+        RT_HIP(rtk::wf_raygen(cv, fp, b, paths, st));
+        for (int bounce = 0; bounce < max_b; ++bounce) {
+            RT_HIP(mark());
+            RT_HIP(rtk::wf_extend(sv, sv.stack_depth, b, bounce, r->tuning.count_traversal != 0, trace_l, st));
+            RT_HIP(mark());
+            RT_HIP(rtk::wf_shade(sv, cv, fp, b, bounce, shade_l, st));
+            RT_HIP(mark());
+            RT_HIP(rtk::wf_connect(sv, sv.stack_depth, b, bounce, trace_l, st));
+            RT_HIP(mark());
+        }
+        RT_HIP(rtk::wf_accumulate(fp, b, st));
+    }
+    RT_HIP(rtk::wf_finalize(total, spp, b, r->beauty.p, r->normal.p, r->albedo.p, r->depth.p, st));
+    if (timed) RT_HIP(hipEventRecord(r->events[1], st));
+    uint32_t stats[16];
+    RT_HIP(hipMemcpyAsync(stats, b.counters, sizeof(stats), hipMemcpyDeviceToHost, st));
+    RT_HIP(hipStreamSynchronize(st));
+    r->rendered = true;
+    if (timed) {
+        std::memset(timing, 0, sizeof(*timing));
+        float ms = 0.0f;
+        RT_HIP(hipEventElapsedTime(&ms, r->events[0], r->events[1]));
+        timing->render_ms = ms;
+        double ext = 0.0, shd = 0.0, con = 0.0;
+        for (size_t i = 2; i + 3 < ev; i += 4) {
+            float a = 0.0f, bb = 0.0f, c = 0.0f;
+            RT_HIP(hipEventElapsedTime(&a, r->events[i], r->events[i + 1]));
+            RT_HIP(hipEventElapsedTime(&bb, r->events[i + 1], r->events[i + 2]));
+            RT_HIP(hipEventElapsedTime(&c, r->events[i + 2], r->events[i + 3]));
+            ext += a;
+            shd += bb;
+            con += c;
+        }
+        timing->extend_ms = ext;
+        timing->shade_ms = shd;
+        timing->connect_ms = con;
+        timing->other_ms = std::max(0.0, timing->render_ms - ext - shd - con);
+        uint64_t ex = 0, sh = 0;
+        std::memcpy(&ex, stats + 4, 8);
+        std::memcpy(&sh, stats + 6, 8);
+        timing->paths = (uint64_t)P * (uint64_t)spp;
+        timing->extend_rays = ex;
+        timing->shadow_rays = sh;
+        timing->extend_launches = (uint64_t)chunks * (uint64_t)max_b;
+        std::memcpy(&timing->node_visits, stats + 8, 8);
+        std::memcpy(&timing->prim_tests, stats + 10, 8);
+        std::memcpy(&timing->sphere_tests, stats + 12, 8);
+    }
+    return RT_OK;
+}
+
+rt_status rt_frame_download(rt_renderer* r, int camera, float* beauty, float* normal, float* albedo,
+    float* depth) {
+    RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
+    RT_REQUIRE(r->rendered, RT_ERROR_NOT_READY, "rt_frame_download requires a rendered frame");
+    RT_REQUIRE(camera >= 0 && camera < r->camera_count, RT_ERROR_INVALID_ARGUMENT,
+        "camera index " + std::to_string(camera) + " out of range");
+    RT_HIP(hipSetDevice(r->dev->ordinal));
+    hipStream_t st = r->dev->stream;
+    const size_t off = (size_t)r->pixel_offset[camera];
+    const size_t n = (size_t)r->widths[camera] * (size_t)r->heights[camera];
+    if (beauty) RT_HIP(hipMemcpyAsync(beauty, r->beauty.p + off, n * 16, hipMemcpyDeviceToHost, st));
+    if (normal) RT_HIP(hipMemcpyAsync(normal, r->normal.p + off, n * 16, hipMemcpyDeviceToHost, st));
+    if (albedo) RT_HIP(hipMemcpyAsync(albedo, r->albedo.p + off, n * 16, hipMemcpyDeviceToHost, st));
+    if (depth) RT_HIP(hipMemcpyAsync(depth, r->depth.p + off, n * 4, hipMemcpyDeviceToHost, st));
+    RT_HIP(hipStreamSynchronize(st));
+    return RT_OK;
+}
+
+rt_status rt_frame_device_view(rt_renderer* r, int camera, rt_device_frame_view* view) {
+    RT_REQUIRE(r && view, RT_ERROR_INVALID_ARGUMENT, "null renderer or view");
+    RT_REQUIRE(r->rendered, RT_ERROR_NOT_READY, "rt_frame_device_view requires a rendered frame");
+    RT_REQUIRE(camera >= 0 && camera < r->camera_count, RT_ERROR_INVALID_ARGUMENT,
+        "camera index " + std::to_string(camera) + " out of range");
+    const size_t off = (size_t)r->pixel_offset[camera];
+    view->width = r->widths[camera];
+    view->height = r->heights[camera];
+    view->beauty = reinterpret_cast<const float*>(r->beauty.p + off);
+    view->normal = reinterpret_cast<const float*>(r->normal.p + off);
+    view->albedo = reinterpret_cast<const float*>(r->albedo.p + off);
+    view->depth = r->depth.p + off;
+    return RT_OK;
+}
+
+rt_status rt_reset_accumulation(rt_renderer* r) {
+    RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
+    return RT_OK;
+}
+
+rt_status rt_reset_sequence(rt_renderer* r, uint32_t seed) {
+    RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
+    r->sequence_seed = seed;
+    return RT_OK;
+}
+
+rt_status rt_restir_diagnostics(rt_renderer* r, int camera, rt_restir_stats* out) {
+    RT_REQUIRE(r && out, RT_ERROR_INVALID_ARGUMENT, "null renderer or output");
+    (void)camera;
+    std::memset(out, 0, sizeof(*out));
+    return RT_OK;
+}
+
+rt_status rt_scene_generate(int config, uint64_t seed, rt_generated_scene** out) {
+    RT_REQUIRE(out != nullptr, RT_ERROR_INVALID_ARGUMENT, "null output");
+    auto* g = new rt_generated_scene();
+    if (!rth::generate_scene(config, seed, g)) {
+        delete g;
+        return fail(RT_ERROR_INVALID_ARGUMENT, "unknown scene config " + std::to_string(config));
+    }
+    *out = g;
+    return RT_OK;
+}
+
+rt_status rt_scene_generated_free(rt_generated_scene* g) {
+    if (!g) return RT_OK;
+    rth::free_generated(g);
+    delete g;
+    return RT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// rt_internal.h -- structures shared by the host orchestration and the gfx950
+// kernels of the path tracer.
+//
+// HBM layout (one scene):
+//   nodes   : BVH node array, 4 x float4 = 64 B per node (child boxes,
+//             child references, parent) -- same bytes as rt_bvh_node
+//   leaves  : 3 x float4 = 48 B per primitive in Morton order
+//             triangle (p0 | id), (p1-p0), (p2-p0); quad (origin | id),
+//             edge_u, edge_v; sphere (center | id), (radius, 0, 0, 0)
+//             where id = global primitive id | type << 30
+//   lights  : power CDF over emissive primitives + environment, and a
+//             primitive -> light index table (O(1) MIS lookups)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rt/rt_hip.h"
+
+namespace rtk {
+
+enum : uint32_t { kPrimSphere = 0, kPrimQuad = 1, kPrimTriangle = 2 };
+constexpr uint32_t kIdMask = 0x3FFFFFFFu;
+
+struct LightRec {
+    int32_t type;  // 0 sphere, 1 quad, 2 triangle, 3 environment
+    int32_t prim;  // index within its type, -1 for the environment
+    float selection_pdf;
+    float cdf;
+};
+
+struct SceneView {
+    const float4* nodes;
+    const float4* leaves;
+    int32_t node_count;
+    int32_t prim_count;
+    const rt_sphere* spheres;
+    const rt_quad* quads;
+    const rt_triangle* triangles;
+    int32_t sphere_count, quad_count, triangle_count, material_count;
+    const rt_material* materials;
+    const rt_openpbr_material* openpbr;
+    int32_t openpbr_count;
+    int32_t light_count;
+    const LightRec* lights;
+    const int32_t* light_of_prim;
+    int32_t env_light;
+    int32_t background_mode;
+    float bg[3];
+    float bg_top[3];
+    const rt_analytic_light* analytic;
+    int32_t analytic_count;
+    int32_t stack_depth;  // LDS traversal stack entries per lane (>= BVH depth)
+};
+
+// per-camera constants for ray generation (double camera math as the
+// reference's DeviceActiveCamera)
+struct CameraView {
+    const rt_camera* cams;   // device copy of the rig's cameras
+    int32_t count;
+    int32_t pixel_offset[5]; // prefix sum of W*H
+    int32_t width[4];
+};
+
+}  // namespace rtk

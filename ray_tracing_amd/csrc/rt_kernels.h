@@ -1,0 +1,84 @@
+// rt_kernels.h -- host-side launch entry points of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rt_internal.h"
+
+namespace rtk {
+
+struct BvhBuildScratch {
+    float4* pbox;          // 2 x float4 per primitive
+    uint32_t* keys[2];
+    uint32_t* vals[2];
+    uint32_t* counts;      // 16 x sort blocks
+    uint32_t* leaf_parent;
+    uint32_t* flags;
+    uint32_t* cb;          // centroid bounds (6) + max depth (1)
+};
+
+hipError_t bvh_build(const SceneView& s, int n, BvhBuildScratch& w, float4* nodes, float4* leaves,
+    uint32_t* sorted_out, uint32_t* morton_out, hipStream_t st);
+
+// test-hook traversal over device arrays of rt_ray / rt_hit
+hipError_t trace_closest_launch(const SceneView& s, int stack_depth, const rt_ray* rays,
+    int64_t n, rt_hit* hits, hipStream_t st);
+hipError_t trace_any_launch(const SceneView& s, int stack_depth, const rt_ray* rays, int64_t n,
+    int32_t* occluded, hipStream_t st);
+
+// ---- wavefront pipeline
+struct WavefrontBuffers {
+    // per path slot (capacity = queue capacity)
+    float4* ray_o;      // origin.xyz, unused
+    float4* ray_d;      // direction.xyz, unused
+    float4* hit;        // t, prim bits, u, v
+    float4* throughput; // xyz, unused
+    float4* radiance;   // xyz, unused
+    float4* prev_pos;   // xyz, bsdf pdf
+    float4* prev_n;     // xyz, bits(prev prim)
+    int32_t* prev_flags;// bit0 valid, bit1 delta, bits 8.. material type + 1
+    float4* aux_normal; // bounce-0 shading normal, w = 1 if captured
+    float4* aux_albedo; // bounce-0 base colour, w = depth
+    uint32_t* queue[2]; // active path slots (ping-pong)
+    float4* shadow_o;   // origin.xyz, max_t
+    float4* shadow_d;   // direction.xyz, bits(path slot)
+    float4* shadow_c;   // contribution.xyz
+    uint32_t* counters; // [0]/[1] queue counts, [2] shadow count, [4..15] u64 ray / traversal stats
+    // per pixel (all cameras)
+    float4* acc_beauty;
+    float4* acc_normal; // xyz sum, w = aux count
+    float4* acc_albedo; // xyz sum, w = depth sum
+};
+
+struct FrameParams {
+    int32_t spp;
+    int32_t max_bounces;
+    int32_t rr_start_bounce;
+    int32_t jitter;
+    uint32_t stream;
+    int32_t total_pixels;
+    int32_t layer_count;    // samples per pixel in this chunk
+    int32_t first_sample;   // sample index of the chunk's first layer
+};
+
+struct StageLaunch {
+    int grid;
+    int block;
+};
+
+hipError_t wf_reset_counters(uint32_t* counters, hipStream_t st);
+hipError_t wf_raygen(const CameraView& cams, const FrameParams& fp, WavefrontBuffers& b,
+    int paths, hipStream_t st);
+hipError_t wf_extend(const SceneView& s, int stack_depth, WavefrontBuffers& b, int bounce,
+    bool count, StageLaunch l, hipStream_t st);
+hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParams& fp,
+    WavefrontBuffers& b, int bounce, StageLaunch l, hipStream_t st);
+hipError_t wf_connect(const SceneView& s, int stack_depth, WavefrontBuffers& b, int bounce,
+    StageLaunch l, hipStream_t st);
+hipError_t wf_accumulate(const FrameParams& fp, WavefrontBuffers& b, hipStream_t st);
+hipError_t wf_clear_accum(int pixels, WavefrontBuffers& b, hipStream_t st);
+hipError_t wf_finalize(int pixels, int spp, const WavefrontBuffers& b, float4* beauty,
+    float4* normal, float4* albedo, float* depth, hipStream_t st);
+
+}  // namespace rtk

@@ -1,0 +1,382 @@
+// rt_scenes.cpp -- seeded synthetic benchmark scenes and camera rigs.
+//
+// The benchmark configurations are new scenes (the reference's builtin scenes
+// come from build_scene(id), src/scene/shared_scene_builders.h:56-57, and
+// make_rig, utils/render_realtime.cpp:288):
+//   0 random_spheres        : ground + 22x22 small spheres + 3 large; one
+//                             pinhole 400x225, vfov 20; sky gradient
+//   1 cornell_mesh          : 555-unit Cornell room + 130x105 light (15),
+//                             icosphere subdivided 6x (81,920 triangles),
+//                             one constant OpenPBR material; 4 pinhole
+//                             640x480, vfov 40, on a radius-800 ring
+//   2 instanced_mesh_restir : 1000 rigid instances of a 1024-triangle mesh
+//                             in a 100-unit cube, 256 analytic sphere lights;
+//                             2 pinhole32 640x480 + 2 equi62_lut1d 1024x1024
+// Draws come from PCG32 (O'Neill 2014; pcg32_srandom_r(seed, 54), XSH-RR
+// output); a float draw is (x >> 8) * 2^-24; the draw order is part of the
+// scene definition.
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/rt/rt_hip.h"
+#include "rt_host.h"
+
+namespace rth {
+
+namespace {
+
+class Pcg32 {
+   public:
+    explicit Pcg32(uint64_t seed) {
+        state_ = 0u;
+        inc_ = (54u << 1u) | 1u;
+        next();
+        state_ += seed;
+        next();
+    }
+    uint32_t next() {
+        const uint64_t old = state_;
+        state_ = old * 6364136223846793005ULL + inc_;
+        const uint32_t xs = static_cast<uint32_t>(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = static_cast<uint32_t>(old >> 59u);
+        return (xs >> rot) | (xs << ((-rot) & 31u));
+    }
+    float uniform() { return static_cast<float>(next() >> 8) * (1.0f / 16777216.0f); }
+
+   private:
+    uint64_t state_;
+    uint64_t inc_;
+};
+
+void set3(float* d, float x, float y, float z) {
+    d[0] = x;
+    d[1] = y;
+    d[2] = z;
+}
+
+rt_material lambert(float r, float g, float b) {
+    rt_material m{};
+    m.type = RT_MAT_LAMBERTIAN;
+    set3(m.albedo, r, g, b);
+    m.ior = 1.0f;
+    m.openpbr_index = -1;
+    return m;
+}
+
+template <typename T>
+T* copy_out(const std::vector<T>& v) {
+    if (v.empty()) return nullptr;
+    T* p = static_cast<T*>(std::malloc(sizeof(T) * v.size()));
+    std::memcpy(p, v.data(), sizeof(T) * v.size());
+    return p;
+}
+
+void generate_random_spheres(uint64_t seed, rt_generated_scene* g) {
+    std::vector<rt_sphere> sp;
+    std::vector<rt_material> mt;
+    Pcg32 r(seed);
+    auto add = [&](float x, float y, float z, float radius, const rt_material& m) {
+        rt_sphere s{};
+        set3(s.center, x, y, z);
+        s.radius = radius;
+        s.material = static_cast<int32_t>(mt.size());
+        sp.push_back(s);
+        mt.push_back(m);
+    };
+    add(0.0f, -1000.0f, 0.0f, 1000.0f, lambert(0.5f, 0.5f, 0.5f));
+    for (int a = -11; a < 11; ++a) {
+        for (int b = -11; b < 11; ++b) {
+            const float choose = r.uniform();
+            const float u = r.uniform();
+            const float v = r.uniform();
+            rt_material m{};
+            m.openpbr_index = -1;
+            m.ior = 1.0f;
+            if (choose < 0.8f) {
+                m.type = RT_MAT_LAMBERTIAN;
+                for (int ch = 0; ch < 3; ++ch) {
+                    const float x0 = r.uniform();
+                    const float x1 = r.uniform();
+                    m.albedo[ch] = x0 * x1;
+                }
+            } else if (choose < 0.95f) {
+                m.type = RT_MAT_METAL;
+                for (int ch = 0; ch < 3; ++ch) m.albedo[ch] = 0.5f + 0.5f * r.uniform();
+                m.fuzz = 0.5f * r.uniform();
+            } else {
+                m.type = RT_MAT_DIELECTRIC;
+                m.ior = 1.5f;
+            }
+            add(static_cast<float>(a) + 0.9f * u, 0.2f, static_cast<float>(b) + 0.9f * v, 0.2f, m);
+        }
+    }
+    rt_material glass{};
+    glass.type = RT_MAT_DIELECTRIC;
+    glass.ior = 1.5f;
+    glass.openpbr_index = -1;
+    add(0.0f, 1.0f, 0.0f, 1.0f, glass);
+    add(-4.0f, 1.0f, 0.0f, 1.0f, lambert(0.4f, 0.2f, 0.1f));
+    rt_material metal{};
+    metal.type = RT_MAT_METAL;
+    set3(metal.albedo, 0.7f, 0.6f, 0.5f);
+    metal.ior = 1.0f;
+    metal.openpbr_index = -1;
+    add(4.0f, 1.0f, 0.0f, 1.0f, metal);
+
+    g->scene.sphere_count = static_cast<int32_t>(sp.size());
+    g->scene.material_count = static_cast<int32_t>(mt.size());
+    g->scene.spheres = copy_out(sp);
+    g->scene.materials = copy_out(mt);
+    g->scene.background_mode = RT_BG_SKY_GRADIENT;
+    set3(g->scene.background, 1.0f, 1.0f, 1.0f);
+    set3(g->scene.background_top, 0.5f, 0.7f, 1.0f);
+    g->camera_count = 1;
+    make_lookat_pinhole(&g->cameras[0], 400, 225, 20.0, {13.0, 2.0, 3.0}, {0.0, 0.0, 0.0},
+        {0.0, 1.0, 0.0});
+    g->profile = default_profile(32, 10);
+}
+
+struct Icosphere {
+    std::vector<std::array<double, 3>> verts;
+    std::vector<std::array<int32_t, 3>> faces;
+};
+
+// unit icosphere with `levels` 4:1 subdivisions, outward winding; new
+// vertices are created in face order, edges (a,b), (b,c), (c,a)
+Icosphere make_icosphere(int levels) {
+    const double t = (1.0 + std::sqrt(5.0)) / 2.0;
+    const double base[12][3] = {{-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0}, {0, -1, t},
+        {0, 1, t}, {0, -1, -t}, {0, 1, -t}, {t, 0, -1}, {t, 0, 1}, {-t, 0, -1}, {-t, 0, 1}};
+    const int32_t bf[20][3] = {{0, 11, 5}, {0, 5, 1}, {0, 1, 7}, {0, 7, 10}, {0, 10, 11},
+        {1, 5, 9}, {5, 11, 4}, {11, 10, 2}, {10, 7, 6}, {7, 1, 8}, {3, 9, 4}, {3, 4, 2},
+        {3, 2, 6}, {3, 6, 8}, {3, 8, 9}, {4, 9, 5}, {2, 4, 11}, {6, 2, 10}, {8, 6, 7}, {9, 8, 1}};
+    Icosphere m;
+    for (const auto& v : base) {
+        const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        m.verts.push_back({v[0] / l, v[1] / l, v[2] / l});
+    }
+    for (const auto& f : bf) {
+        int32_t a = f[0], b = f[1], c = f[2];
+        double e1[3], e2[3], ce[3];
+        for (int k = 0; k < 3; ++k) {
+            e1[k] = m.verts[b][k] - m.verts[a][k];
+            e2[k] = m.verts[c][k] - m.verts[a][k];
+            ce[k] = m.verts[a][k] + m.verts[b][k] + m.verts[c][k];
+        }
+        const double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+            e1[0] * e2[1] - e1[1] * e2[0]};
+        if (cr[0] * ce[0] + cr[1] * ce[1] + cr[2] * ce[2] < 0.0) std::swap(b, c);
+        m.faces.push_back({a, b, c});
+    }
+    for (int l = 0; l < levels; ++l) {
+        std::unordered_map<uint64_t, int32_t> mid;
+        mid.reserve(m.faces.size() * 2);
+        auto midpoint = [&](int32_t a, int32_t b) {
+            const uint64_t lo = static_cast<uint64_t>(a < b ? a : b);
+            const uint64_t hi = static_cast<uint64_t>(a < b ? b : a);
+            const uint64_t key = (lo << 32) | hi;
+            const auto it = mid.find(key);
+            if (it != mid.end()) return it->second;
+            std::array<double, 3> p;
+            for (int k = 0; k < 3; ++k) p[k] = 0.5 * (m.verts[a][k] + m.verts[b][k]);
+            const double len = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+            for (int k = 0; k < 3; ++k) p[k] /= len;
+            const int32_t id = static_cast<int32_t>(m.verts.size());
+            m.verts.push_back(p);
+            mid.emplace(key, id);
+            return id;
+        };
+        std::vector<std::array<int32_t, 3>> next;
+        next.reserve(m.faces.size() * 4);
+        for (const auto& f : m.faces) {
+            const int32_t a = f[0], b = f[1], c = f[2];
+            const int32_t ab = midpoint(a, b);
+            const int32_t bc = midpoint(b, c);
+            const int32_t ca = midpoint(c, a);
+            next.push_back({a, ab, ca});
+            next.push_back({b, bc, ab});
+            next.push_back({c, ca, bc});
+            next.push_back({ab, bc, ca});
+        }
+        m.faces.swap(next);
+    }
+    return m;
+}
+
+rt_quad quad(float ox, float oy, float oz, float ux, float uy, float uz, float vx, float vy,
+    float vz, int mat) {
+    rt_quad q{};
+    set3(q.origin, ox, oy, oz);
+    set3(q.edge_u, ux, uy, uz);
+    set3(q.edge_v, vx, vy, vz);
+    q.material = mat;
+    return q;
+}
+
+void generate_cornell_mesh(uint64_t seed, rt_generated_scene* g) {
+    Pcg32 r(seed);
+    std::vector<rt_material> mt = {lambert(0.65f, 0.05f, 0.05f), lambert(0.73f, 0.73f, 0.73f),
+        lambert(0.12f, 0.45f, 0.15f)};
+    rt_material light{};
+    light.type = RT_MAT_DIFFUSE_LIGHT;
+    set3(light.emission, 15.0f, 15.0f, 15.0f);
+    light.ior = 1.0f;
+    light.openpbr_index = -1;
+    mt.push_back(light);
+    rt_openpbr_material op = openpbr_defaults();
+    for (int ch = 0; ch < 3; ++ch) op.base_color[ch] = r.uniform();
+    op.specular_roughness = 0.05f + 0.85f * r.uniform();
+    op.base_metalness = r.uniform() < 0.5f ? 0.0f : 1.0f;
+    op.specular_ior = 1.5f;
+    op.coat_weight = 0.5f * r.uniform();
+    rt_material pbr{};
+    pbr.type = RT_MAT_OPENPBR;
+    pbr.openpbr_index = 0;
+    pbr.ior = 1.0f;
+    for (int ch = 0; ch < 3; ++ch) pbr.albedo[ch] = op.base_color[ch];
+    mt.push_back(pbr);
+
+    const std::vector<rt_quad> q = {quad(555, 0, 0, 0, 555, 0, 0, 0, 555, 2),
+        quad(0, 0, 0, 0, 555, 0, 0, 0, 555, 0), quad(343, 554, 332, -130, 0, 0, 0, 0, -105, 3),
+        quad(0, 0, 0, 555, 0, 0, 0, 0, 555, 1), quad(555, 555, 555, -555, 0, 0, 0, 0, -555, 1),
+        quad(0, 0, 555, 555, 0, 0, 0, 555, 0, 1)};
+
+    const Icosphere ico = make_icosphere(6);
+    const double center[3] = {277.5, 277.5, 277.5};
+    const double radius = 150.0;
+    std::vector<std::array<float, 3>> pos(ico.verts.size());
+    for (size_t i = 0; i < ico.verts.size(); ++i) {
+        const float u = r.uniform();
+        const double rr = radius * (1.0 + 0.05 * (2.0 * static_cast<double>(u) - 1.0));
+        for (int k = 0; k < 3; ++k) pos[i][k] = static_cast<float>(center[k] + ico.verts[i][k] * rr);
+    }
+    std::vector<rt_triangle> tr(ico.faces.size());
+    for (size_t f = 0; f < ico.faces.size(); ++f) {
+        std::memcpy(tr[f].p0, pos[ico.faces[f][0]].data(), 12);
+        std::memcpy(tr[f].p1, pos[ico.faces[f][1]].data(), 12);
+        std::memcpy(tr[f].p2, pos[ico.faces[f][2]].data(), 12);
+        tr[f].material = 4;
+    }
+    g->scene.quad_count = static_cast<int32_t>(q.size());
+    g->scene.quads = copy_out(q);
+    g->scene.triangle_count = static_cast<int32_t>(tr.size());
+    g->scene.triangles = copy_out(tr);
+    g->scene.material_count = static_cast<int32_t>(mt.size());
+    g->scene.materials = copy_out(mt);
+    g->scene.openpbr_count = 1;
+    g->scene.openpbr = copy_out(std::vector<rt_openpbr_material>{op});
+    g->scene.background_mode = RT_BG_CONSTANT;
+    set3(g->scene.background, 0.0f, 0.0f, 0.0f);
+    g->camera_count = 4;
+    for (int c = 0; c < 4; ++c) {
+        const double th = static_cast<double>(c) * 0.5 * 3.14159265358979323846;
+        make_lookat_pinhole(&g->cameras[c], 640, 480, 40.0,
+            {center[0] + 800.0 * std::sin(th), center[1], center[2] - 800.0 * std::cos(th)},
+            {center[0], center[1], center[2]}, {0.0, 1.0, 0.0});
+    }
+    g->profile = default_profile(16, 8);
+}
+
+}  // namespace
+
+rt_openpbr_material openpbr_defaults() {
+    rt_openpbr_material m{};
+    m.base_weight = 1.0f;
+    set3(m.base_color, 0.8f, 0.8f, 0.8f);
+    m.specular_weight = 1.0f;
+    set3(m.specular_color, 1.0f, 1.0f, 1.0f);
+    m.specular_roughness = 0.3f;
+    m.specular_ior = 1.5f;
+    set3(m.transmission_color, 1.0f, 1.0f, 1.0f);
+    m.transmission_dispersion_abbe_number = 20.0f;
+    set3(m.subsurface_color, 0.8f, 0.8f, 0.8f);
+    m.subsurface_radius = 1.0f;
+    set3(m.subsurface_radius_scale, 1.0f, 0.5f, 0.25f);
+    set3(m.fuzz_color, 1.0f, 1.0f, 1.0f);
+    m.fuzz_roughness = 0.5f;
+    set3(m.coat_color, 1.0f, 1.0f, 1.0f);
+    m.coat_ior = 1.6f;
+    m.coat_darkening = 1.0f;
+    m.thin_film_thickness = 0.5f;
+    m.thin_film_ior = 1.4f;
+    set3(m.emission_color, 1.0f, 1.0f, 1.0f);
+    m.geometry_opacity = 1.0f;
+    return m;
+}
+
+rt_profile default_profile(int spp, int depth) {
+    rt_profile p{};
+    p.samples_per_pixel = spp;
+    p.max_bounces = depth;
+    p.enable_denoise = 0;
+    p.rr_start_bounce = depth;
+    p.accumulation_reset_rotation_deg = 2.0;
+    p.accumulation_reset_translation = 0.05;
+    p.restir_initial_candidates = 4;
+    p.restir_temporal_reuse = 1;
+    p.restir_max_history_age = 20;
+    p.restir_max_temporal_candidates = 64;
+    p.restir_spatial_neighbors = 1;
+    p.restir_max_spatial_candidates = 4;
+    p.restir_bias_correction = 1;
+    p.restir_min_analytic_lights = 16;
+    p.pixel_jitter = 1;
+    p.restir_spatial_radius = 30;
+    return p;
+}
+
+// OpenCV camera axes: x right, y down, z forward
+void make_lookat_pinhole(rt_camera* c, int w, int h, double vfov_deg, std::array<double, 3> from,
+    std::array<double, 3> at, std::array<double, 3> up) {
+    std::memset(c, 0, sizeof(*c));
+    c->enabled = 1;
+    c->width = w;
+    c->height = h;
+    c->model = RT_CAM_PINHOLE32;
+    double z[3] = {at[0] - from[0], at[1] - from[1], at[2] - from[2]};
+    const double zl = std::sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+    for (double& v : z) v /= zl;
+    double x[3] = {z[1] * up[2] - z[2] * up[1], z[2] * up[0] - z[0] * up[2],
+        z[0] * up[1] - z[1] * up[0]};
+    const double xl = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    for (double& v : x) v /= xl;
+    const double y[3] = {z[1] * x[2] - z[2] * x[1], z[2] * x[0] - z[0] * x[2],
+        z[0] * x[1] - z[1] * x[0]};
+    for (int i = 0; i < 3; ++i) {
+        c->origin[i] = from[i];
+        c->basis_x[i] = x[i];
+        c->basis_y[i] = y[i];
+        c->basis_z[i] = z[i];
+    }
+    const double vfov = vfov_deg * 3.14159265358979323846 / 180.0;
+    const double f = 0.5 * static_cast<double>(h) / std::tan(0.5 * vfov);
+    c->fx = f;
+    c->fy = f;
+    c->cx = 0.5 * static_cast<double>(w);
+    c->cy = 0.5 * static_cast<double>(h);
+}
+
+bool generate_scene(int config, uint64_t seed, rt_generated_scene* g) {
+    std::memset(g, 0, sizeof(*g));
+    switch (config) {
+        case 0: generate_random_spheres(seed, g); return true;
+        case 1: generate_cornell_mesh(seed, g); return true;
+        default: return false;
+    }
+}
+
+void free_generated(rt_generated_scene* g) {
+    std::free(const_cast<rt_sphere*>(g->scene.spheres));
+    std::free(const_cast<rt_quad*>(g->scene.quads));
+    std::free(const_cast<rt_triangle*>(g->scene.triangles));
+    std::free(const_cast<rt_material*>(g->scene.materials));
+    std::free(const_cast<rt_openpbr_material*>(g->scene.openpbr));
+    std::free(const_cast<rt_analytic_light*>(g->scene.analytic_lights));
+    std::memset(g, 0, sizeof(*g));
+}
+
+}  // namespace rth

@@ -1,0 +1,406 @@
+// rt_shading.h -- device surface interaction, light sampling and BSDF
+// sampling for the shade stage of the wavefront pipeline.
+//
+// Semantics follow the reference megakernel (src/realtime/gpu/programs.cu):
+//   populate_material :961-1009          emission MIS   :571-647
+//   sample_direct_light :1057-1232       direct response :1026-1055
+//   accumulate_direct_light :2249-2280   sample_bsdf    :2451-2565
+// with two structural changes: the light owning a primitive is found through
+// a prim -> light table instead of the O(L) scan of find_packed_light
+// (:559-568), and light selection uses a binary search of the CDF (the same
+// index as the linear walk of light_sampling.h:90-102, since the CDF is
+// non-decreasing).  Random numbers come from the counter RNG, two 4-tuples
+// per bounce; rejection loops are replaced by closed-form samplers.
+#pragma once
+
+#include "rt_internal.h"
+#include "rt_math.h"
+#include "rt_openpbr.h"
+
+namespace rtk {
+
+struct HitInfo {
+    float t;
+    f3 pos, gn, sn, base, emission;
+    float fuzz, ior;
+    int mat_type, openpbr, prim, prim_type, prim_index;
+    bool front;
+};
+
+__device__ __forceinline__ f3 background_at(const SceneView& s, f3 d) {
+    if (s.background_mode == RT_BG_SKY_GRADIENT) {
+        const f3 dn = normalize3(d);
+        const float t = 0.5f * (dn.y + 1.0f);
+        return mk3(s.bg[0], s.bg[1], s.bg[2]) * (1.0f - t) + mk3(s.bg_top[0], s.bg_top[1], s.bg_top[2]) * t;
+    }
+    return mk3(s.bg[0], s.bg[1], s.bg[2]);
+}
+
+__device__ __forceinline__ void set_face(f3 d, f3 outward, HitInfo& h) {
+    h.front = dot3(d, outward) < 0.0f;
+    h.gn = outward;
+    h.sn = h.front ? outward : -outward;
+}
+
+__device__ inline void make_hit(const SceneView& s, f3 o, f3 d, float t, int prim, HitInfo& h) {
+    h.t = t;
+    h.prim = prim;
+    int mat;
+    if (prim < s.sphere_count) {
+        const rt_sphere sp = s.spheres[prim];
+        h.prim_type = 0;
+        h.prim_index = prim;
+        h.pos = o + d * t;
+        set_face(d, div3(h.pos - ld3(sp.center), sp.radius), h);
+        mat = sp.material;
+    } else if (prim < s.sphere_count + s.quad_count) {
+        const int qi = prim - s.sphere_count;
+        const rt_quad q = s.quads[qi];
+        h.prim_type = 1;
+        h.prim_index = qi;
+        const f3 n = cross3(ld3(q.edge_u), ld3(q.edge_v));
+        h.pos = o + d * t;
+        set_face(d, div3(n, sqrtf(lensq3(n))), h);
+        mat = q.material;
+    } else {
+        const int ti = prim - s.sphere_count - s.quad_count;
+        const rt_triangle tr = s.triangles[ti];
+        h.prim_type = 2;
+        h.prim_index = ti;
+        const f3 p0 = ld3(tr.p0);
+        h.pos = o + d * t;
+        set_face(d, normalize3(cross3(ld3(tr.p1) - p0, ld3(tr.p2) - p0)), h);
+        mat = tr.material;
+    }
+    if (mat < 0 || mat >= s.material_count) {
+        h.base = mk3(0, 0, 0); h.emission = mk3(0, 0, 0); h.mat_type = 0; h.ior = 1.0f;
+        h.fuzz = 0.0f; h.openpbr = -1;
+        return;
+    }
+    const rt_material m = s.materials[mat];
+    h.openpbr = m.openpbr_index;
+    h.mat_type = m.type;
+    h.fuzz = m.fuzz;
+    h.ior = m.ior;
+    h.base = ld3(m.albedo);
+    h.emission = ld3(m.emission);
+    if (h.mat_type == RT_MAT_DIELECTRIC) { h.base = mk3(0.92f, 0.95f, 1.0f); h.emission = mk3(0, 0, 0); }
+    if (h.mat_type == RT_MAT_DIFFUSE_LIGHT && !h.front) h.emission = mk3(0, 0, 0);
+    if (h.mat_type != RT_MAT_DIFFUSE_LIGHT) h.emission = mk3(0, 0, 0);
+    if (h.mat_type == RT_MAT_OPENPBR) {
+        if (m.openpbr_index < 0 || m.openpbr_index >= s.openpbr_count) {
+            h.mat_type = -1; h.base = mk3(0, 0, 0); h.emission = mk3(0, 0, 0);
+            return;
+        }
+        const rt_openpbr_material& p = s.openpbr[m.openpbr_index];
+        h.base = ld3(p.base_color);
+        h.emission = openpbr_emission(p);
+    }
+}
+
+// path-vertex state carried between bounces
+struct PathPrev {
+    f3 pos, n;
+    float pdf;
+    int prim;      // -1: none
+    int mat;       // material type
+    bool valid, delta;
+};
+
+__device__ __forceinline__ float power_heuristic(float a, float b) {
+    if (a <= 0.0f) return 0.0f;
+    const float a2 = a * a, b2 = b * b;
+    return a2 / (a2 + b2);
+}
+__device__ __forceinline__ float cone_pdf(float cos_max) {
+    const float sa = 6.28318530717958648f * (1.0f - cos_max);
+    return sa > 1e-12f ? 1.0f / sa : 0.0f;
+}
+__device__ __forceinline__ float area_pdf(float area, float dsq, float acos_) {
+    if (area <= 1e-12f || dsq <= 0.0f || acos_ <= 1e-8f) return 0.0f;
+    return dsq / (area * acos_);
+}
+
+__device__ __forceinline__ float eff_selection_pdf(const SceneView& s, int light, int excluded_prim) {
+    const int ex = excluded_prim < 0 ? -1 : s.light_of_prim[excluded_prim];
+    if (ex == light) return 0.0f;
+    const float elig = ex < 0 ? 1.0f : 1.0f - s.lights[ex].selection_pdf;
+    return elig > 1e-8f ? s.lights[light].selection_pdf / elig : 0.0f;
+}
+
+__device__ inline float emission_mis(const SceneView& s, const PathPrev& pv, f3 ray_d,
+    const HitInfo* h, bool env) {
+    if (!pv.valid || pv.delta) return 1.0f;
+    const int light = env ? s.env_light : s.light_of_prim[h->prim];
+    if (light < 0) return 1.0f;
+    const f3 dir = normalize3(ray_d);
+    float cond;
+    if (env) {
+        cond = pv.mat == RT_MAT_ISOTROPIC ? kUniformSpherePdf : fmaxf(dot3(pv.n, dir), 0.0f) / kPi;
+    } else if (!h->front) {
+        cond = 0.0f;
+    } else if (h->prim_type == 0) {
+        const rt_sphere sp = s.spheres[h->prim_index];
+        const float dsq = lensq3(ld3(sp.center) - pv.pos);
+        const float rsq = sp.radius * sp.radius;
+        cond = dsq <= rsq ? 0.0f : cone_pdf(sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq)));
+    } else {
+        const float dsq = lensq3(h->pos - pv.pos);
+        const float ac = fmaxf(dot3(h->gn, dir * -1.0f), 0.0f);
+        float area;
+        if (h->prim_type == 1) {
+            const rt_quad q = s.quads[h->prim_index];
+            area = len3(cross3(ld3(q.edge_u), ld3(q.edge_v)));
+        } else {
+            const rt_triangle t = s.triangles[h->prim_index];
+            area = 0.5f * len3(cross3(ld3(t.p1) - ld3(t.p0), ld3(t.p2) - ld3(t.p0)));
+        }
+        cond = area_pdf(area, dsq, ac);
+    }
+    const float sel = eff_selection_pdf(s, light, pv.prim);
+    return power_heuristic(pv.pdf, sel * cond);
+}
+
+struct LightSample {
+    f3 dir, emission;
+    float dist, pdf;
+    bool infinite, valid;
+};
+
+__device__ __forceinline__ f3 cone_dir(f3 axis, float cos_max, float u0, float u1) {
+    const float ct = 1.0f - u0 * (1.0f - cos_max);
+    const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float sp, cp;
+    sincos_turn(u1, &sp, &cp);
+    const f3 helper = fabsf(axis.x) > 0.9f ? mk3(0.0f, 1.0f, 0.0f) : mk3(1.0f, 0.0f, 0.0f);
+    const f3 tangent = normalize3(cross3(helper, axis));
+    const f3 bitangent = cross3(axis, tangent);
+    return normalize3(axis * ct + (tangent * (st * cp) + bitangent * (st * sp)));
+}
+
+// first index with u < cdf[i] (the reference's linear walk), by bisection
+__device__ __forceinline__ int pick_light(const SceneView& s, float u) {
+    int lo = 0, hi = s.light_count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (u < s.lights[mid].cdf) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+__device__ inline LightSample sample_light(const SceneView& s, const HitInfo& h, float4 r) {
+    LightSample ls{};
+    ls.valid = false;
+    const int ex = s.light_of_prim[h.prim];
+    const float elig = ex < 0 ? 1.0f : 1.0f - s.lights[ex].selection_pdf;
+    if (elig <= 1e-8f) return ls;
+    int li = -1;
+    if (ex < 0) {
+        const float u = r.x < 0.0f ? 0.0f : (r.x < 1.0f ? r.x : 0.99999994f);
+        li = pick_light(s, u);
+    } else {
+        const float target = r.x * elig;
+        float acc = 0.0f;
+        for (int i = 0; i < s.light_count; ++i) {
+            if (i == ex) continue;
+            acc += s.lights[i].selection_pdf;
+            if (target < acc) { li = i; break; }
+        }
+        if (li < 0)
+            for (int i = s.light_count - 1; i >= 0; --i)
+                if (i != ex) { li = i; break; }
+    }
+    if (li < 0) return ls;
+    const LightRec L = s.lights[li];
+    const float sel = L.selection_pdf / elig;
+    if (sel <= 0.0f) return ls;
+    const f3 sp = h.pos;
+    if (L.type == 3) {
+        if (h.mat_type == RT_MAT_ISOTROPIC) {
+            ls.dir = sphere_dir(r.y, r.z);
+            ls.pdf = sel * kUniformSpherePdf;
+        } else {
+            ls.dir = h.sn + sphere_dir(r.y, r.z);
+            if (near_zero3(ls.dir)) ls.dir = h.sn;
+            ls.dir = normalize3(ls.dir);
+            const float cs = fmaxf(dot3(h.sn, ls.dir), 0.0f);
+            ls.pdf = sel * cs / kPi;
+        }
+        ls.emission = background_at(s, ls.dir);
+        ls.dist = kRayFar;
+        ls.infinite = true;
+        ls.valid = ls.pdf > 0.0f;
+        return ls;
+    }
+    ls.infinite = false;
+    float cond = 0.0f;
+    int mat = -1;
+    if (L.type == 0) {
+        const rt_sphere sph = s.spheres[L.prim];
+        const f3 center = ld3(sph.center);
+        const f3 tc = center - sp;
+        const float dsq = lensq3(tc);
+        const float rsq = sph.radius * sph.radius;
+        if (dsq <= rsq * (1.0f + 1e-6f)) return ls;
+        const float cmax = sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq));
+        ls.dir = cone_dir(normalize3(tc), cmax, r.y, r.z);
+        const f3 oc = sp - center;
+        const float hb = dot3(oc, ls.dir);
+        const float disc = hb * hb - (lensq3(oc) - rsq);
+        if (disc <= 0.0f) return ls;
+        ls.dist = -hb - sqrtf(disc);
+        if (ls.dist <= kRayEps) return ls;
+        mat = sph.material;
+        cond = cone_pdf(cmax);
+    } else if (L.type == 1) {
+        const rt_quad q = s.quads[L.prim];
+        const f3 eu = ld3(q.edge_u), ev = ld3(q.edge_v);
+        const f3 pos = ld3(q.origin) + (eu * r.y + ev * r.z);
+        const f3 an = cross3(eu, ev);
+        const float area = len3(an);
+        if (area <= 1e-12f) return ls;
+        const f3 normal = div3(an, area);
+        mat = q.material;
+        const f3 tl = pos - sp;
+        const float dsq = lensq3(tl);
+        if (dsq <= 1e-12f) return ls;
+        ls.dist = sqrtf(dsq);
+        ls.dir = div3(tl, ls.dist);
+        cond = area_pdf(area, dsq, dot3(normal, ls.dir * -1.0f));
+    } else {
+        const rt_triangle t = s.triangles[L.prim];
+        const f3 p0 = ld3(t.p0), p1 = ld3(t.p1), p2 = ld3(t.p2);
+        const float su = sqrtf(r.y < 0.0f ? 0.0f : (r.y < 1.0f ? r.y : 1.0f));
+        const float w0 = 1.0f - su, w1 = su * (1.0f - r.z), w2 = su * r.z;
+        const f3 pos = p0 * w0 + (p1 * w1 + p2 * w2);
+        const f3 an = cross3(p1 - p0, p2 - p0);
+        const float da = len3(an);
+        if (da <= 1e-12f) return ls;
+        const f3 normal = div3(an, da);
+        mat = t.material;
+        const f3 tl = pos - sp;
+        const float dsq = lensq3(tl);
+        if (dsq <= 1e-12f) return ls;
+        ls.dist = sqrtf(dsq);
+        ls.dir = div3(tl, ls.dist);
+        cond = area_pdf(0.5f * da, dsq, dot3(normal, ls.dir * -1.0f));
+    }
+    if (cond <= 0.0f || mat < 0 || mat >= s.material_count) return ls;
+    const rt_material m = s.materials[mat];
+    if (m.type == RT_MAT_OPENPBR) {
+        ls.emission = (m.openpbr_index >= 0 && m.openpbr_index < s.openpbr_count)
+                          ? openpbr_emission(s.openpbr[m.openpbr_index]) : mk3(0, 0, 0);
+    } else {
+        ls.emission = ld3(m.emission);
+    }
+    ls.pdf = sel * cond;
+    ls.valid = ls.pdf > 0.0f && maxc3(ls.emission) > 0.0f;
+    return ls;
+}
+
+__device__ inline f3 direct_response(const SceneView& s, const HitInfo& h, f3 ray_d, f3 ldir,
+    float* pdf) {
+    *pdf = 0.0f;
+    if (h.mat_type == RT_MAT_LAMBERTIAN) {
+        const float cs = fmaxf(dot3(h.sn, ldir), 0.0f);
+        *pdf = cs / kPi;
+        return h.base * *pdf;
+    }
+    if (h.mat_type == RT_MAT_ISOTROPIC) {
+        *pdf = kUniformSpherePdf;
+        return h.base * *pdf;
+    }
+    if (h.mat_type != RT_MAT_OPENPBR) return mk3(0, 0, 0);
+    f3 value;
+    openpbr_eval(s.openpbr[h.openpbr], h.gn, -normalize3(ray_d), ldir, &value, pdf);
+    const float cs = fabsf(dot3(h.sn, ldir));
+    return value * cs;
+}
+
+__device__ __forceinline__ float schlick(float cosine, float ri) {
+    float r0 = (1.0f - ri) / (1.0f + ri);
+    r0 *= r0;
+    const float x = 1.0f - cosine;
+    const float x2 = x * x;
+    return r0 + (1.0f - r0) * (x2 * x2 * x);
+}
+
+struct BsdfResult {
+    f3 o, d, weight;
+    float pdf;
+    bool delta, alive;
+};
+
+__device__ inline BsdfResult sample_bsdf(const SceneView& s, const HitInfo& h, f3 ray_d, float4 r) {
+    BsdfResult b{};
+    b.alive = false;
+    if (h.mat_type == RT_MAT_DIFFUSE_LIGHT) return b;
+    if (h.mat_type == RT_MAT_LAMBERTIAN) {
+        f3 sc = h.sn + sphere_dir(r.x, r.y);
+        if (near_zero3(sc)) sc = h.sn;
+        b.o = h.pos + h.sn * kRayEps;
+        b.d = normalize3(sc);
+        b.weight = h.base;
+        b.pdf = fmaxf(dot3(h.sn, b.d), 0.0f) / kPi;
+        b.delta = false;
+        b.alive = true;
+        return b;
+    }
+    if (h.mat_type == RT_MAT_METAL) {
+        const f3 refl = reflect3(normalize3(ray_d), h.sn);
+        const f3 ball = sphere_dir(r.x, r.y) * cbrt01(r.z);
+        const f3 fz = ball * fminf(fmaxf(h.fuzz, 0.0f), 1.0f);
+        const f3 sc = normalize3(refl + fz);
+        if (dot3(sc, h.sn) <= 0.0f) return b;
+        b.o = h.pos + h.sn * kRayEps;
+        b.d = sc;
+        b.weight = h.base;
+        b.pdf = 0.0f;
+        b.delta = true;
+        b.alive = true;
+        return b;
+    }
+    if (h.mat_type == RT_MAT_DIELECTRIC) {
+        const float ratio = h.front ? (1.0f / fmaxf(h.ior, 1e-4f)) : fmaxf(h.ior, 1e-4f);
+        const f3 unit = normalize3(ray_d);
+        const float ct = fminf(dot3(-unit, h.sn), 1.0f);
+        const float stt = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+        const bool cannot = ratio * stt > 1.0f;
+        f3 dir;
+        if (cannot || schlick(ct, ratio) > r.x) dir = reflect3(unit, h.sn);
+        else dir = refract3(unit, h.sn, ratio);
+        b.o = h.pos + dir * kRayEps;
+        b.d = normalize3(dir);
+        b.weight = h.base;
+        b.pdf = 0.0f;
+        b.delta = true;
+        b.alive = true;
+        return b;
+    }
+    if (h.mat_type == RT_MAT_OPENPBR) {
+        OpenPbrSampleResult smp;
+        openpbr_sample(s.openpbr[h.openpbr], h.gn, -normalize3(ray_d), r.x, r.y, r.z, &smp);
+        if (!smp.valid) return b;
+        b.o = h.pos + smp.wi * kRayEps;
+        b.d = normalize3(smp.wi);
+        b.weight = smp.weight;
+        b.pdf = smp.pdf;
+        b.delta = smp.delta;
+        b.alive = true;
+        return b;
+    }
+    if (h.mat_type == RT_MAT_ISOTROPIC) {
+        const f3 dir = sphere_dir(r.x, r.y);
+        b.o = h.pos + dir * kRayEps;
+        b.d = dir;
+        b.weight = h.base;
+        b.pdf = kUniformSpherePdf;
+        b.delta = false;
+        b.alive = true;
+        return b;
+    }
+    return b;
+}
+
+}  // namespace rtk

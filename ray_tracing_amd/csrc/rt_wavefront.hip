@@ -1,0 +1,407 @@
+// rt_wavefront.hip -- GPU-resident wavefront path tracer for gfx950.
+//
+// Replaces the reference's per-camera megakernel radiance_kernel
+// (src/realtime/gpu/programs.cu:1828-1953, one thread = one pixel running
+// spp x bounces with traversal, shading and shadow rays inlined) with stage
+// kernels over SoA path state in HBM:
+//   raygen     : one path slot per (camera, pixel, sample) of a chunk, every
+//                camera of the rig in the same launch
+//   extend     : closest hit of every active path (stack-in-LDS traversal)
+//   shade      : emission + MIS, next-event estimation (enqueues one shadow
+//                ray with its precomputed contribution), BSDF sampling,
+//                Russian roulette, compaction into the next active queue
+//   connect    : any-hit with early exit for the shadow queue; adds the
+//                contribution of unoccluded rays
+//   accumulate : per pixel, sums the chunk's samples in sample order
+//   finalize   : averages and encodes beauty / normal / albedo / depth
+// extend, shade and connect are persistent: a fixed grid of a few blocks per
+// CU strides over the queue whose length it reads from device memory, so the
+// per-frame launch sequence is fixed and can be captured in a hipGraph.
+// Floating-point operations per path happen in the same order as the
+// single-path integrator, so a path's radiance does not depend on the queue
+// order in which the stages visit it.
+#include "rt_camera.h"
+#include "rt_kernels.h"
+#include "rt_shading.h"
+#include "rt_traverse.h"
+
+namespace rtk {
+
+namespace {
+
+enum : uint32_t { kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntStatExtend = 4,
+    kCntStatShadow = 6, kCntVisits = 8, kCntTests = 10, kCntSphereTests = 12, kCounterWords = 16 };
+
+enum : int32_t { kPrevValid = 1, kPrevDelta = 2 };
+
+struct SlotId {
+    uint32_t pixel;   // pixel index inside its camera
+    uint32_t sample;
+    uint32_t camera;
+    uint32_t global_pixel;
+};
+
+__device__ __forceinline__ SlotId slot_id(const CameraView& cams, const FrameParams& fp, uint32_t slot) {
+    const uint32_t P = (uint32_t)fp.total_pixels;
+    const uint32_t layer = slot / P;
+    const uint32_t p = slot - layer * P;
+    uint32_t c = 0;
+    while (c + 1 < (uint32_t)cams.count && p >= (uint32_t)cams.pixel_offset[c + 1]) ++c;
+    SlotId id;
+    id.pixel = p - (uint32_t)cams.pixel_offset[c];
+    id.sample = (uint32_t)fp.first_sample + layer;
+    id.camera = c;
+    id.global_pixel = p;
+    return id;
+}
+
+// wave-aggregated append: one atomic per wave
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0ull) return 0u;
+    const int lane = (int)(threadIdx.x & 63u);
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = (uint32_t)__shfl((int)base, leader);
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    return base + (uint32_t)__popcll(mask & lt);
+}
+
+__global__ void k_reset_counters(uint32_t* counters) {
+    if (threadIdx.x < kCounterWords) counters[threadIdx.x] = 0u;
+}
+
+__global__ void k_clear_accum(int pixels, float4* a, float4* b, float4* c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pixels) return;
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    a[i] = z;
+    b[i] = z;
+    c[i] = z;
+}
+
+__global__ __launch_bounds__(256) void k_raygen(CameraView cams, FrameParams fp, WavefrontBuffers b,
+    int paths) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        b.counters[kCntQueue0] = (uint32_t)paths;
+        b.counters[kCntQueue1] = 0u;
+        b.counters[kCntShadow] = 0u;
+    }
+    if (i >= paths) return;
+    const SlotId id = slot_id(cams, fp, (uint32_t)i);
+    const rt_camera& cam = cams.cams[id.camera];
+    const int w = cams.width[id.camera];
+    const int y = (int)id.pixel / w;
+    const int x = (int)id.pixel - y * w;
+    double px, py;
+    if (fp.jitter) {
+        const float4 r = rng4(id.pixel, id.sample, id.camera, 0u, kGroupCamera, fp.stream);
+        px = (double)((float)x + r.x);
+        py = (double)((float)y + r.y);
+    } else {
+        px = (double)((float)x + 0.5f);
+        py = (double)((float)y + 0.5f);
+    }
+    f3 o, d;
+    camera_ray(cam, px, py, &o, &d);
+    b.ray_o[i] = f4(o, 0.0f);
+    b.ray_d[i] = f4(d, 0.0f);
+    b.throughput[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    b.radiance[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    b.prev_flags[i] = kPrevDelta;
+    b.prev_n[i] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+    b.aux_normal[i] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+    b.aux_albedo[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    b.queue[0][i] = (uint32_t)i;
+}
+
+template <int BLOCK, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_extend(SceneView s, WavefrontBuffers b, int cur) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t count = b.counters[cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntStatExtend),
+            (unsigned long long)count);
+    }
+    uint32_t* stack_node = lds + threadIdx.x;
+    float* stack_t = reinterpret_cast<float*>(lds + BLOCK * s.stack_depth) + threadIdx.x;
+    const uint32_t* queue = b.queue[cur];
+    const uint32_t stride = gridDim.x * BLOCK;
+    for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < count; q += stride) {
+        const uint32_t slot = queue[q];
+        const f3 o = xyz(b.ray_o[slot]);
+        const f3 d = xyz(b.ray_d[slot]);
+        TraverseCounts cnt;
+        const TraceResult r = traverse<false, COUNT>(s, o, d, kRayEps, kRayFar, stack_node, stack_t,
+            BLOCK, &cnt);
+        b.hit[slot] = make_float4(r.t, __int_as_float(r.found ? (int)r.prim : -1), r.u, r.v);
+        if (COUNT) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntVisits),
+                (unsigned long long)cnt.visits);
+            atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntTests),
+                (unsigned long long)cnt.tests);
+            atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntSphereTests),
+                (unsigned long long)cnt.sphere_tests);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, FrameParams fp,
+    WavefrontBuffers b, int bounce, int cur) {
+    const uint32_t count = b.counters[cur];
+    const uint32_t* queue = b.queue[cur];
+    uint32_t* next_queue = b.queue[cur ^ 1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t nq = (count + stride - 1u) / stride * stride;  // uniform trip count per wave
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
+        const bool active = q < count;
+        bool shadow = false;
+        bool survive = false;
+        uint32_t slot = 0u;
+        float4 sh_o = make_float4(0, 0, 0, 0), sh_d = make_float4(0, 0, 0, 0), sh_c = sh_o;
+        if (active) {
+            slot = queue[q];
+            const SlotId id = slot_id(cams, fp, slot);
+            const f3 ro = xyz(b.ray_o[slot]);
+            const f3 rd = xyz(b.ray_d[slot]);
+            const float4 hv = b.hit[slot];
+            f3 thr = xyz(b.throughput[slot]);
+            f3 rad = xyz(b.radiance[slot]);
+            const int32_t flags = b.prev_flags[slot];
+            const float4 ppos = b.prev_pos[slot];
+            const float4 pn = b.prev_n[slot];
+            PathPrev pv;
+            pv.valid = (flags & kPrevValid) != 0;
+            pv.delta = (flags & kPrevDelta) != 0;
+            pv.mat = (flags >> 8) - 1;
+            pv.pos = xyz(ppos);
+            pv.pdf = ppos.w;
+            pv.n = xyz(pn);
+            pv.prim = __float_as_int(pn.w);
+            const int prim = __float_as_int(hv.y);
+            if (prim < 0) {
+                const float w = emission_mis(s, pv, rd, nullptr, true);
+                rad = rad + thr * (background_at(s, rd) * w);
+            } else {
+                const float4 rl = rng4(id.pixel, id.sample, id.camera, (uint32_t)bounce,
+                    kGroupLight, fp.stream);
+                const float4 rb = rng4(id.pixel, id.sample, id.camera, (uint32_t)bounce,
+                    kGroupBsdf, fp.stream);
+                HitInfo h;
+                make_hit(s, ro, rd, hv.x, prim, h);
+                if (bounce == 0) {
+                    b.aux_normal[slot] = f4(h.sn, 1.0f);
+                    b.aux_albedo[slot] = f4(h.base, h.t);
+                }
+                const float ew = emission_mis(s, pv, rd, &h, false);
+                rad = rad + thr * (h.emission * ew);
+                const bool bsdf_avail = bounce + 1 < fp.max_bounces;
+                if (h.mat_type != RT_MAT_DIFFUSE_LIGHT && s.light_count > 0) {
+                    const LightSample ls = sample_light(s, h, rl);
+                    if (ls.valid) {
+                        float bpdf = 0.0f;
+                        const f3 resp = direct_response(s, h, rd, ls.dir, &bpdf);
+                        if (maxc3(resp) > 0.0f) {
+                            const f3 so = h.pos + ls.dir * (kRayEps * 2.0f);
+                            const float max_t = ls.infinite ? kRayFar : ls.dist - kRayEps * 3.0f;
+                            if (max_t > kRayEps) {
+                                const float mis = bsdf_avail ? power_heuristic(ls.pdf, bpdf) : 1.0f;
+                                const f3 direct = (ls.emission * resp) * (mis / ls.pdf);
+                                const f3 contrib = thr * direct;
+                                shadow = true;
+                                sh_o = f4(so, max_t);
+                                sh_d = f4(ls.dir, __uint_as_float(slot));
+                                sh_c = f4(contrib, 0.0f);
+                            }
+                        }
+                    }
+                }
+                if (bsdf_avail) {
+                    const BsdfResult bs = sample_bsdf(s, h, rd, rb);
+                    if (bs.alive) {
+                        thr = thr * bs.weight;
+                        bool alive = true;
+                        if (bounce >= fp.rr_start_bounce) {
+                            const float p = fminf(fmaxf(maxc3(thr), 0.05f), 0.95f);
+                            if (rb.w > p) alive = false;
+                            else thr = div3(thr, p);
+                        }
+                        if (alive) {
+                            survive = true;
+                            b.ray_o[slot] = f4(bs.o, 0.0f);
+                            b.ray_d[slot] = f4(bs.d, 0.0f);
+                            b.throughput[slot] = f4(thr, 0.0f);
+                            b.prev_pos[slot] = f4(h.pos, bs.pdf);
+                            b.prev_n[slot] = f4(h.sn, __int_as_float(h.prim));
+                            b.prev_flags[slot] = kPrevValid | (bs.delta ? kPrevDelta : 0)
+                                                 | ((h.mat_type + 1) << 8);
+                        }
+                    }
+                }
+            }
+            b.radiance[slot] = f4(rad, 0.0f);
+        }
+        const uint32_t spos = wave_append(shadow, &b.counters[kCntShadow]);
+        if (shadow) {
+            b.shadow_o[spos] = sh_o;
+            b.shadow_d[spos] = sh_d;
+            b.shadow_c[spos] = sh_c;
+        }
+        const uint32_t npos = wave_append(survive, &b.counters[cur ^ 1]);
+        if (survive) next_queue[npos] = slot;
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_connect(SceneView s, WavefrontBuffers b) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t count = b.counters[kCntShadow];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntStatShadow),
+            (unsigned long long)count);
+    }
+    uint32_t* stack_node = lds + threadIdx.x;
+    float* stack_t = reinterpret_cast<float*>(lds + BLOCK * s.stack_depth) + threadIdx.x;
+    const uint32_t stride = gridDim.x * BLOCK;
+    for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < count; q += stride) {
+        const float4 o = b.shadow_o[q];
+        const float4 d = b.shadow_d[q];
+        const TraceResult r = traverse<true>(s, xyz(o), xyz(d), kRayEps, o.w, stack_node, stack_t,
+            BLOCK);
+        if (!r.found) {
+            const uint32_t slot = __float_as_uint(d.w);
+            const float4 c = b.shadow_c[q];
+            const float4 rad = b.radiance[slot];
+            b.radiance[slot] = f4(xyz(rad) + xyz(c), 0.0f);
+        }
+    }
+}
+
+__global__ void k_swap_queues(uint32_t* counters, int cur) {
+    // the consumed queue becomes the next producer target; shadow queue empties
+    if (threadIdx.x == 0) {
+        counters[cur] = 0u;
+        counters[kCntShadow] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_accumulate(FrameParams fp, WavefrontBuffers b) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= fp.total_pixels) return;
+    float4 ab = b.acc_beauty[p], an = b.acc_normal[p], aa = b.acc_albedo[p];
+    f3 beauty = xyz(ab), nsum = xyz(an), asum = xyz(aa);
+    float aux = an.w, dsum = aa.w;
+    for (int layer = 0; layer < fp.layer_count; ++layer) {
+        const size_t slot = (size_t)layer * (size_t)fp.total_pixels + (size_t)p;
+        beauty = beauty + xyz(b.radiance[slot]);
+        const float4 n = b.aux_normal[slot];
+        if (n.w > 0.0f) {
+            const float4 a = b.aux_albedo[slot];
+            nsum = nsum + xyz(n);
+            asum = asum + xyz(a);
+            dsum += a.w;
+            aux += 1.0f;
+        }
+    }
+    b.acc_beauty[p] = f4(beauty, 0.0f);
+    b.acc_normal[p] = f4(nsum, aux);
+    b.acc_albedo[p] = f4(asum, dsum);
+}
+
+// programs.cu:1944-1952 + store_output :1773-1805
+__global__ __launch_bounds__(256) void k_finalize(int pixels, int spp, const float4* acc_beauty,
+    const float4* acc_normal, const float4* acc_albedo, float4* beauty, float4* normal,
+    float4* albedo, float* depth) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pixels) return;
+    const float inv_spp = 1.0f / (float)spp;
+    const float4 ab = acc_beauty[p], an = acc_normal[p], aa = acc_albedo[p];
+    const int aux = (int)an.w;
+    const float inv_aux = aux > 0 ? 1.0f / (float)aux : 0.0f;
+    const f3 bavg = xyz(ab) * inv_spp;
+    const f3 navg = aux > 0 ? normalize3(xyz(an) * inv_aux) : mk3(0.0f, 0.0f, 1.0f);
+    const f3 aavg = aux > 0 ? xyz(aa) * inv_aux : mk3(0.0f, 0.0f, 0.0f);
+    const float davg = aux > 0 ? aa.w * inv_aux : 0.0f;
+    beauty[p] = f4(bavg, 1.0f);
+    normal[p] = f4(clamp3(navg * 0.5f + mk3(0.5f, 0.5f, 0.5f), 0.0f, 1.0f), 1.0f);
+    albedo[p] = f4(clamp3(aavg, 0.0f, 1.0f), 1.0f);
+    depth[p] = davg;
+}
+
+}  // namespace
+
+hipError_t wf_reset_counters(uint32_t* counters, hipStream_t st) {
+    k_reset_counters<<<1, 64, 0, st>>>(counters);
+    return hipGetLastError();
+}
+
+hipError_t wf_clear_accum(int pixels, WavefrontBuffers& b, hipStream_t st) {
+    k_clear_accum<<<(pixels + 255) / 256, 256, 0, st>>>(pixels, b.acc_beauty, b.acc_normal,
+        b.acc_albedo);
+    return hipGetLastError();
+}
+
+hipError_t wf_raygen(const CameraView& cams, const FrameParams& fp, WavefrontBuffers& b, int paths,
+    hipStream_t st) {
+    k_raygen<<<(paths + 255) / 256, 256, 0, st>>>(cams, fp, b, paths);
+    return hipGetLastError();
+}
+
+template <bool COUNT>
+static void launch_extend(const SceneView& sv, int stack_depth, WavefrontBuffers& b, int cur,
+    StageLaunch l, hipStream_t st) {
+    if (l.block == 64) {
+        k_extend<64, COUNT><<<l.grid, 64, (size_t)stack_depth * 64 * 8, st>>>(sv, b, cur);
+    } else if (l.block == 256) {
+        k_extend<256, COUNT><<<l.grid, 256, (size_t)stack_depth * 256 * 8, st>>>(sv, b, cur);
+    } else {
+        k_extend<128, COUNT><<<l.grid, 128, (size_t)stack_depth * 128 * 8, st>>>(sv, b, cur);
+    }
+}
+
+hipError_t wf_extend(const SceneView& s, int stack_depth, WavefrontBuffers& b, int bounce,
+    bool count, StageLaunch l, hipStream_t st) {
+    SceneView sv = s;
+    sv.stack_depth = stack_depth;
+    const int cur = bounce & 1;
+    if (count) launch_extend<true>(sv, stack_depth, b, cur, l, st);
+    else launch_extend<false>(sv, stack_depth, b, cur, l, st);
+    return hipGetLastError();
+}
+
+hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParams& fp,
+    WavefrontBuffers& b, int bounce, StageLaunch l, hipStream_t st) {
+    k_shade<<<l.grid, 256, 0, st>>>(s, cams, fp, b, bounce, bounce & 1);
+    return hipGetLastError();
+}
+
+hipError_t wf_connect(const SceneView& s, int stack_depth, WavefrontBuffers& b, int bounce,
+    StageLaunch l, hipStream_t st) {
+    SceneView sv = s;
+    sv.stack_depth = stack_depth;
+    if (l.block == 64) {
+        k_connect<64><<<l.grid, 64, (size_t)stack_depth * 64 * 8, st>>>(sv, b);
+    } else if (l.block == 256) {
+        k_connect<256><<<l.grid, 256, (size_t)stack_depth * 256 * 8, st>>>(sv, b);
+    } else {
+        k_connect<128><<<l.grid, 128, (size_t)stack_depth * 128 * 8, st>>>(sv, b);
+    }
+    k_swap_queues<<<1, 64, 0, st>>>(b.counters, bounce & 1);
+    return hipGetLastError();
+}
+
+hipError_t wf_accumulate(const FrameParams& fp, WavefrontBuffers& b, hipStream_t st) {
+    k_accumulate<<<(fp.total_pixels + 255) / 256, 256, 0, st>>>(fp, b);
+    return hipGetLastError();
+}
+
+hipError_t wf_finalize(int pixels, int spp, const WavefrontBuffers& b, float4* beauty,
+    float4* normal, float4* albedo, float* depth, hipStream_t st) {
+    k_finalize<<<(pixels + 255) / 256, 256, 0, st>>>(pixels, spp, b.acc_beauty, b.acc_normal,
+        b.acc_albedo, beauty, normal, albedo, depth);
+    return hipGetLastError();
+}
+
+}  // namespace rtk

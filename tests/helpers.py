@@ -1,0 +1,111 @@
+"""Bridges between the product C ABI structs and the oracle's identical structs."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+
+def oracle_scene(oracle, desc):
+    """Oracle Scene viewing the same host arrays as a product SceneDesc (same layout)."""
+    s = oracle.Scene()
+    for name in ("sphere_count", "quad_count", "triangle_count", "material_count",
+                 "openpbr_count", "analytic_light_count", "background_mode"):
+        setattr(s, name, getattr(desc, name))
+    s.background = oracle.f3(*desc.background)
+    s.background_top = oracle.f3(*desc.background_top)
+    s.spheres = C.cast(desc.spheres, C.POINTER(oracle.Sphere))
+    s.quads = C.cast(desc.quads, C.POINTER(oracle.Quad))
+    s.triangles = C.cast(desc.triangles, C.POINTER(oracle.Triangle))
+    s.materials = C.cast(desc.materials, C.POINTER(oracle.Material))
+    s.openpbr = C.cast(desc.openpbr, C.POINTER(oracle.OpenPbr))
+    s.analytic_lights = C.cast(desc.analytic_lights, C.POINTER(oracle.AnalyticLight))
+    return s
+
+
+def oracle_camera(oracle, cam):
+    c = oracle.Camera()
+    C.memmove(C.byref(c), C.byref(cam), C.sizeof(c))
+    return c
+
+
+def oracle_profile(oracle, prof):
+    p = oracle.Profile()
+    C.memmove(C.byref(p), C.byref(prof), C.sizeof(p))
+    return p
+
+
+def product_desc(rt, arrays: dict, background=(0, 0, 0), background_top=None):
+    """SceneDesc over numpy structured arrays (kept alive by the returned list)."""
+    d = rt.SceneDesc()
+    keep = []
+
+    def attach(field, count_field, arr):
+        if arr is None or len(arr) == 0:
+            setattr(d, count_field, 0)
+            setattr(d, field, None)
+            return
+        a = np.ascontiguousarray(arr)
+        keep.append(a)
+        setattr(d, count_field, len(a))
+        setattr(d, field, a.ctypes.data)
+
+    attach("spheres", "sphere_count", arrays.get("spheres"))
+    attach("quads", "quad_count", arrays.get("quads"))
+    attach("triangles", "triangle_count", arrays.get("triangles"))
+    attach("materials", "material_count", arrays.get("materials"))
+    attach("openpbr", "openpbr_count", arrays.get("openpbr"))
+    attach("analytic_lights", "analytic_light_count", arrays.get("analytic"))
+    d.background_mode = 1 if background_top is not None else 0
+    d.background = (C.c_float * 3)(*background)
+    d.background_top = (C.c_float * 3)(*(background_top or (0, 0, 0)))
+    return d, keep
+
+
+SPHERE_DT = np.dtype([("center", "<f4", 3), ("radius", "<f4"), ("material", "<i4")])
+QUAD_DT = np.dtype([("origin", "<f4", 3), ("edge_u", "<f4", 3), ("edge_v", "<f4", 3),
+                    ("material", "<i4")])
+TRI_DT = np.dtype([("p0", "<f4", 3), ("p1", "<f4", 3), ("p2", "<f4", 3), ("material", "<i4")])
+MAT_DT = np.dtype([("type", "<i4"), ("albedo", "<f4", 3), ("emission", "<f4", 3),
+                   ("fuzz", "<f4"), ("ior", "<f4"), ("openpbr_index", "<i4")])
+ALIGHT_DT = np.dtype([("type", "<i4"), ("position", "<f4", 3), ("radius", "<f4"),
+                      ("radiance", "<f4", 3)])
+
+
+def lambert_materials(n, albedo=(0.5, 0.5, 0.5)):
+    m = np.zeros(n, MAT_DT)
+    m["type"] = 0
+    m["albedo"] = albedo
+    m["ior"] = 1.0
+    m["openpbr_index"] = -1
+    return m
+
+
+def random_triangles(rng, n, scale=10.0, size=1.0, material=0):
+    t = np.zeros(n, TRI_DT)
+    c = rng.uniform(-scale, scale, (n, 3)).astype(np.float32)
+    t["p0"] = c + rng.uniform(-size, size, (n, 3)).astype(np.float32)
+    t["p1"] = c + rng.uniform(-size, size, (n, 3)).astype(np.float32)
+    t["p2"] = c + rng.uniform(-size, size, (n, 3)).astype(np.float32)
+    t["material"] = material
+    return t
+
+
+def random_rays(rng, n, origin_scale=15.0, target_scale=10.0):
+    from oracle_ctypes import RAY_DTYPE
+    r = np.zeros(n, RAY_DTYPE)
+    o = rng.uniform(-origin_scale, origin_scale, (n, 3)).astype(np.float32)
+    t = rng.uniform(-target_scale, target_scale, (n, 3)).astype(np.float32)
+    d = t - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r["o"] = o
+    r["d"] = d.astype(np.float32)
+    r["tmin"] = 1e-3
+    r["tmax"] = 1e30
+    return r
+
+
+def rel_mse(x, ref, eps=1e-2):
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.mean((x - ref) ** 2 / (ref ** 2 + eps)))
