@@ -175,8 +175,86 @@ def lib():
                                         C.c_void_p, C.c_int, C.c_int, C.c_int, P(RenderStats)]
         L.rto_trace_path.argtypes = [P(Scene), P(Bvh), P(Camera), C.c_int, P(Profile),
                                      C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        L.rto_openpbr_evaluate.argtypes = [P(OpenPbr), C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]
+        L.rto_openpbr_sample_fn.argtypes = [P(OpenPbr), C.c_void_p, C.c_void_p, C.c_float,
+                                            C.c_float, C.c_float, C.c_void_p]
+        L.rto_openpbr_emission.argtypes = [P(OpenPbr), C.c_void_p]
         _lib = L
     return _lib
+
+
+REF_PATH = os.path.join(HERE, "_ref", "libref.so")
+_ref = None
+
+
+def ref_lib():
+    """The reference's own headers compiled in place (oracle/_ref), or None when absent."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        R = C.CDLL(REF_PATH)
+        R.ref_openpbr_evaluate.argtypes = [C.c_void_p] * 5
+        R.ref_openpbr_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
+                                         C.c_float, C.c_float, C.c_void_p]
+        R.ref_openpbr_emission.argtypes = [C.c_void_p, C.c_void_p]
+        R.ref_openpbr_material_floats.restype = C.c_int
+        for name in ("ref_light_uniform_cone_pdf", "ref_light_power_heuristic"):
+            getattr(R, name).restype = C.c_float
+        R.ref_light_uniform_cone_pdf.argtypes = [C.c_float]
+        R.ref_light_power_heuristic.argtypes = [C.c_float, C.c_float]
+        R.ref_light_area_to_solid_angle_pdf.argtypes = [C.c_float, C.c_float, C.c_float]
+        R.ref_light_area_to_solid_angle_pdf.restype = C.c_float
+        R.ref_sample_packed_light.argtypes = [C.c_void_p, C.c_int, C.c_float]
+        R.ref_sample_uniform_triangle.argtypes = [C.c_float, C.c_float, C.c_void_p]
+        R.ref_restir_update.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_int,
+                                        C.c_float, C.POINTER(C.c_int)]
+        R.ref_restir_finalize.argtypes = [C.c_void_p]
+        _ref = R
+    return _ref
+
+
+def openpbr_material(**kw) -> "OpenPbr":
+    """OpenPBR parameters with the reference defaults (openpbr_core.h:33-77)."""
+    m = OpenPbr()
+    defaults = dict(base_weight=1.0, base_color=(0.8, 0.8, 0.8), specular_weight=1.0,
+                    specular_color=(1, 1, 1), specular_roughness=0.3, specular_ior=1.5,
+                    transmission_color=(1, 1, 1), transmission_dispersion_abbe_number=20.0,
+                    subsurface_color=(0.8, 0.8, 0.8), subsurface_radius=1.0,
+                    subsurface_radius_scale=(1.0, 0.5, 0.25), fuzz_color=(1, 1, 1),
+                    fuzz_roughness=0.5, coat_color=(1, 1, 1), coat_ior=1.6, coat_darkening=1.0,
+                    thin_film_thickness=0.5, thin_film_ior=1.4, emission_color=(1, 1, 1),
+                    geometry_opacity=1.0)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        if isinstance(v, (tuple, list)):
+            setattr(m, k, f3(*v))
+        else:
+            setattr(m, k, v)
+    return m
+
+
+def openpbr_evaluate(m, n, wo, wi):
+    out = np.zeros(4, np.float32)
+    a = [np.asarray(x, np.float32) for x in (n, wo, wi)]
+    lib().rto_openpbr_evaluate(C.byref(m), ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(out))
+    return out[:3].copy(), float(out[3])
+
+
+def openpbr_sample(m, n, wo, u_lobe, u1, u2):
+    class _S(C.Structure):
+        _fields_ = [("wi", f3), ("weight", f3), ("pdf", C.c_float), ("delta", C.c_int32),
+                    ("valid", C.c_int32), ("event", C.c_int32)]
+    s = _S()
+    a = [np.asarray(x, np.float32) for x in (n, wo)]
+    lib().rto_openpbr_sample_fn(C.byref(m), ptr(a[0]), ptr(a[1]), u_lobe, u1, u2, C.byref(s))
+    return {"wi": np.array(s.wi[:], np.float32), "weight": np.array(s.weight[:], np.float32),
+            "pdf": s.pdf, "delta": s.delta, "valid": s.valid, "event": s.event}
+
+
+def openpbr_emission(m):
+    out = np.zeros(3, np.float32)
+    lib().rto_openpbr_emission(C.byref(m), ptr(out))
+    return out
 
 
 def ptr(a: np.ndarray):

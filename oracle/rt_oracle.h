@@ -198,6 +198,7 @@ typedef struct { float value[3]; float pdf; } rto_openpbr_eval;
 typedef struct { float wi[3]; float weight[3]; float pdf; int32_t delta; int32_t valid; int32_t event; } rto_openpbr_sample;
 void rto_openpbr_evaluate(const rto_openpbr* m, const float n[3], const float wo[3],
     const float wi[3], rto_openpbr_eval* out);
+void rto_openpbr_emission(const rto_openpbr* m, float out[3]);
 void rto_openpbr_sample_fn(const rto_openpbr* m, const float n[3], const float wo[3],
     float u_lobe, float u1, float u2, rto_openpbr_sample* out);
 

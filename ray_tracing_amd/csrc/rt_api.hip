@@ -15,6 +15,7 @@
 
 #include "rt_host.h"
 #include "rt_kernels.h"
+#include "rt_openpbr.h"
 
 namespace {
 
@@ -302,9 +303,9 @@ static void build_lights(const rt_scene_desc* d, std::vector<rtk::LightRec>& out
         if (mt.type == RT_MAT_DIFFUSE_LIGHT) return lum_clamped(mt.emission);
         if (mt.type != RT_MAT_OPENPBR || mt.openpbr_index < 0 || mt.openpbr_index >= d->openpbr_count)
             return 0.0f;
-        const rt_openpbr_material& p = d->openpbr[mt.openpbr_index];
-        (void)p;
-        return 0.0f;
+        const rtk::f3 e = rtk::openpbr_emission(d->openpbr[mt.openpbr_index]);
+        return 0.2126f * std::max(0.0f, e.x) + 0.7152f * std::max(0.0f, e.y)
+               + 0.0722f * std::max(0.0f, e.z);
     };
     auto append = [&](int type, int prim, float w) {
         if (std::isfinite(w) && w > 1e-12f) out.push_back(rtk::LightRec{type, prim, w, 0.0f});
@@ -356,7 +357,8 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
     for (int i = 0; i < d->openpbr_count; ++i) {
         const rt_openpbr_material& m = d->openpbr[i];
         RT_REQUIRE(m.transmission_weight <= 0.0f && m.subsurface_weight <= 0.0f && m.fuzz_weight <= 0.0f
-                       && m.thin_film_weight <= 0.0f && m.specular_roughness_anisotropy == 0.0f
+                       && m.thin_film_weight <= 0.0f && m.transmission_dispersion_scale <= 0.0f
+                       && m.specular_roughness_anisotropy == 0.0f
                        && m.coat_roughness_anisotropy == 0.0f && m.geometry_thin_walled == 0,
             RT_ERROR_UNSUPPORTED,
             "OpenPBR material " + std::to_string(i)

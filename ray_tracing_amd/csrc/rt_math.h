@@ -87,7 +87,7 @@ enum : uint32_t { kGroupLight = 0, kGroupBsdf = 1, kGroupAnalytic = 2, kGroupRes
 
 // sin/cos of a whole turn u (angle 2*pi*u) by quadrant reduction in turns
 // (exact) and Taylor polynomials on [-pi/4, pi/4] (truncation < 3e-11)
-__device__ __forceinline__ void sincos_turn(float u, float* s, float* c) {
+__host__ __device__ __forceinline__ void sincos_turn(float u, float* s, float* c) {
     const float q = u * 4.0f;
     const float k = floorf(q + 0.5f);
     const float f = q - k;
