@@ -302,7 +302,43 @@ typedef struct {
     int light_count;
     int32_t* light_of_prim; /* global prim id -> light index or -1 */
     int env_light;
+    float* a_sel;           /* analytic light selection pdf */
+    float* a_cdf;
+    int a_count;
 } ctx_t;
+
+/* analytic light CDF (src/scene/analytic_light_compiler.cpp:15-39,230-237):
+ * weight = world area x luminance, accumulated in FP64, closed at the last
+ * light with positive probability */
+int rto_build_analytic_cdf(const rto_scene* s, float* sel, float* cdf) {
+    const int n = s->analytic_light_count;
+    double total = 0.0;
+    double* w = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; ++i) {
+        const rto_analytic_light* L = &s->analytic_lights[i];
+        const double r = (double)L->radius;
+        const double area = 4.0 * 3.14159265358979323846 * r * r;
+        const double lum = 0.2126 * (double)L->radiance[0] + 0.7152 * (double)L->radiance[1]
+                           + 0.0722 * (double)L->radiance[2];
+        double x = area * lum;
+        x = isfinite(x) ? (x > 0.0 ? x : 0.0) : 0.0;
+        w[i] = x;
+        total += x;
+    }
+    double acc = 0.0;
+    int last = -1;
+    for (int i = 0; i < n; ++i) {
+        const double p = total > 0.0 ? w[i] / total : 0.0;
+        acc += p;
+        sel[i] = (float)p;
+        cdf[i] = (float)acc;
+        if (p > 0.0) last = i;
+    }
+    if (total > 0.0)
+        for (int i = last; i < n; ++i) cdf[i] = 1.0f;
+    free(w);
+    return n;
+}
 
 static int prim_global(const rto_scene* s, int type, int idx) {
     if (type == 0) return idx;
@@ -323,9 +359,17 @@ static int ctx_init(ctx_t* c, const rto_scene* s, const rto_bvh* bvh) {
         if (c->lights[i].type == 3) c->env_light = i;
         else c->light_of_prim[prim_global(s, c->lights[i].type, c->lights[i].prim)] = i;
     }
+    c->a_count = s->analytic_light_count;
+    c->a_sel = (float*)malloc(sizeof(float) * (size_t)(c->a_count > 0 ? c->a_count : 1));
+    c->a_cdf = (float*)malloc(sizeof(float) * (size_t)(c->a_count > 0 ? c->a_count : 1));
+    rto_build_analytic_cdf(s, c->a_sel, c->a_cdf);
     return 0;
 }
-static void ctx_free(ctx_t* c) { free(c->light_of_prim); }
+static void ctx_free(ctx_t* c) {
+    free(c->light_of_prim);
+    free(c->a_sel);
+    free(c->a_cdf);
+}
 
 static v3 background_at(const rto_scene* s, v3 d) {
     if (s->background_mode == RTO_BG_SKY_GRADIENT) {
@@ -634,6 +678,399 @@ static void direct_light(const ctx_t* c, const hit_t* h, const float r[4], int b
     if (max_t <= OM_RAY_EPS || om_occluded(c->s, c->bvh, so, L.dir, OM_RAY_EPS, max_t)) return;
     const float mis = bsdf_avail ? power_heuristic(L.pdf, bpdf) : 1.0f;
     const v3 direct = scl3(mul3(L.emission, resp), mis / L.pdf);
+    st->radiance = add3(st->radiance, mul3(st->throughput, direct));
+}
+
+/* ------------------------------------------------------ analytic lights */
+/* try_intersect_analytic_light, sphere branch (programs.cu:679-709): the
+ * nearer root in (tmin, tmax), else the farther; front faces only */
+static int alight_hit(const rto_analytic_light* L, v3 o, v3 d, float tmin, float tmax, float* t,
+    v3* pos, v3* n) {
+    const v3 center = ld3(L->position);
+    const float radius = L->radius;
+    const v3 oc = sub3(o, center);
+    const float a = lensq3(d);
+    const float hb = dot3(oc, d);
+    const float c = lensq3(oc) - radius * radius;
+    const float disc = hb * hb - a * c;
+    if (a <= 1e-16f || disc < 0.0f) return 0;
+    const float rt = sqrtf(disc);
+    float root = (-hb - rt) / a;
+    if (root <= tmin || root >= tmax) root = (-hb + rt) / a;
+    if (root <= tmin || root >= tmax) return 0;
+    const v3 p = add3(o, scl3(d, root));
+    const v3 nn = norm3(sub3(p, center));
+    if (dot3(d, nn) >= 0.0f) return 0;
+    *t = root;
+    *pos = p;
+    *n = nn;
+    return 1;
+}
+
+/* intersect_analytic_lights (programs.cu:795-813) */
+static int intersect_analytic(const ctx_t* c, v3 o, v3 d, float tmin, float tmax, int* index) {
+    float closest = tmax;
+    int found = -1;
+    for (int i = 0; i < c->a_count; ++i) {
+        float t;
+        v3 p, n;
+        if (!alight_hit(&c->s->analytic_lights[i], o, d, tmin, closest, &t, &p, &n)) continue;
+        closest = t;
+        found = i;
+    }
+    *index = found;
+    return found >= 0;
+}
+
+/* analytic_emission_mis_weight (programs.cu:838-849) */
+static float analytic_mis(const ctx_t* c, const path_t* st, int li) {
+    if (!st->prev_valid || st->prev_delta || li < 0) return 1.0f;
+    const rto_analytic_light* L = &c->s->analytic_lights[li];
+    const float dsq = lensq3(sub3(ld3(L->position), st->prev_pos));
+    const float rsq = L->radius * L->radius;
+    const float cond = dsq <= rsq ? 0.0f : cone_pdf(sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq)));
+    return power_heuristic(st->prev_pdf, c->a_sel[li] * cond);
+}
+
+/* sample_packed_analytic_light (light_sampling.h:104-116) */
+static int pick_analytic(const ctx_t* c, float u) {
+    if (c->a_count <= 0) return -1;
+    const float s = u < 0.0f ? 0.0f : (u < 1.0f ? u : 0.99999994f);
+    for (int i = 0; i < c->a_count; ++i)
+        if (s < c->a_cdf[i]) return i;
+    return c->a_count - 1;
+}
+
+/* sample_analytic_direct_light_candidate, sphere branch (programs.cu:1234-1300) */
+static light_sample_t analytic_candidate(const ctx_t* c, v3 sp, int li, float u0, float u1) {
+    light_sample_t ls;
+    memset(&ls, 0, sizeof(ls));
+    if (li < 0 || li >= c->a_count) return ls;
+    if (c->a_sel[li] <= 0.0f) return ls;
+    const rto_analytic_light* L = &c->s->analytic_lights[li];
+    ls.emission = ld3(L->radiance);
+    const v3 center = ld3(L->position);
+    const v3 tc = sub3(center, sp);
+    const float dsq = lensq3(tc);
+    const float rsq = L->radius * L->radius;
+    if (dsq <= rsq * (1.0f + 1e-6f)) { memset(&ls, 0, sizeof(ls)); return ls; }
+    const float cmax = sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq));
+    ls.dir = cone_dir(norm3(tc), cmax, u0, u1);
+    float t;
+    v3 p, n;
+    if (!alight_hit(L, sp, ls.dir, OM_RAY_EPS, OM_RAY_FAR, &t, &p, &n)) { memset(&ls, 0, sizeof(ls)); return ls; }
+    ls.dist = t;
+    ls.pdf = c->a_sel[li] * cone_pdf(cmax);
+    ls.valid = ls.pdf > 0.0f && maxc3(ls.emission) > 0.0f;
+    return ls;
+}
+
+/* accumulate_analytic_direct_light (programs.cu:2282-2313) */
+static void analytic_direct_light(const ctx_t* c, const hit_t* h, const float r[4], int bsdf_avail,
+    path_t* st) {
+    if (h->mat_type == RTO_MAT_DIFFUSE_LIGHT || c->a_count <= 0) return;
+    const int li = pick_analytic(c, r[0]);
+    const light_sample_t L = analytic_candidate(c, h->pos, li, r[1], r[2]);
+    if (!L.valid) return;
+    float bpdf = 0.0f;
+    const v3 resp = direct_response(c, h, st, L.dir, &bpdf);
+    if (maxc3(resp) <= 0.0f) return;
+    const v3 so = add3(h->pos, scl3(L.dir, OM_RAY_EPS * 2.0f));
+    const float max_t = L.dist - OM_RAY_EPS * 3.0f;
+    if (max_t <= OM_RAY_EPS || om_occluded(c->s, c->bvh, so, L.dir, OM_RAY_EPS, max_t)) return;
+    const float mis = !bsdf_avail ? 1.0f : power_heuristic(L.pdf, bpdf);
+    const v3 direct = scl3(mul3(L.emission, resp), mis / L.pdf);
+    st->radiance = add3(st->radiance, mul3(st->throughput, direct));
+}
+
+/* --------------------------------------------------------------- ReSTIR DI */
+/* reservoir math: src/common/restir_di.h:54-199 */
+static int rs_finite_pos(float v) { return v > 0.0f && isfinite(v); }
+static float rs_clamp_unit(float v) { return v < 0.0f ? 0.0f : (v < 1.0f ? v : 0.99999994f); }
+
+static void rs_init(rto_reservoir* r) {
+    memset(r, 0, sizeof(*r));
+    r->light_index = -1;
+    r->material_type = -1;
+    r->primitive_type = -1;
+    r->primitive_index = -1;
+}
+
+int rto_restir_update(rto_reservoir* r, int light, float u0, float u1, float target, float weight,
+    int mult, float u) {
+    if (light < 0 || mult <= 0) return 0;
+    r->candidate_count += mult;
+    if (!rs_finite_pos(target) || !rs_finite_pos(weight)) return 0;
+    const float next = r->weight_sum + weight;
+    if (!rs_finite_pos(next)) return 0;
+    const int sel = rs_clamp_unit(u) * next < weight;
+    if (sel) {
+        r->light_index = light;
+        r->sample_u0 = u0;
+        r->sample_u1 = u1;
+        r->selected_target = target;
+        r->valid = 1;
+    }
+    r->weight_sum = next;
+    return sel;
+}
+
+void rto_restir_finalize(rto_reservoir* r) {
+    r->estimator_weight = 0.0f;
+    if (r->valid == 0 || r->candidate_count <= 0 || !rs_finite_pos(r->selected_target)
+        || !rs_finite_pos(r->weight_sum)) {
+        r->valid = 0;
+        return;
+    }
+    r->estimator_weight = r->weight_sum / ((float)r->candidate_count * r->selected_target);
+    if (!rs_finite_pos(r->estimator_weight)) {
+        r->valid = 0;
+        r->estimator_weight = 0.0f;
+    }
+}
+
+void rto_restir_finalize_basic(rto_reservoir* r, float sst, float sum, int count) {
+    r->estimator_weight = 0.0f;
+    r->bias_correction_source_count = count > 0 ? count : 0;
+    if (r->valid == 0 || count <= 0 || !rs_finite_pos(r->selected_target)
+        || !rs_finite_pos(r->weight_sum) || !rs_finite_pos(sst) || !rs_finite_pos(sum)) {
+        r->valid = 0;
+        return;
+    }
+    r->estimator_weight = r->weight_sum * sst / (r->selected_target * sum);
+    if (!rs_finite_pos(r->estimator_weight)) {
+        r->valid = 0;
+        r->estimator_weight = 0.0f;
+    }
+}
+
+int rto_restir_merge(rto_reservoir* cur, const rto_reservoir* prev, float target, int max_count,
+    float u, int temporal, int* selected) {
+    *selected = 0;
+    if (prev->valid == 0 || prev->candidate_count <= 0 || !rs_finite_pos(prev->estimator_weight))
+        return 0;
+    int mult = prev->candidate_count;
+    if (max_count > 0 && mult > max_count) mult = max_count;
+    const float cw = rs_finite_pos(target) ? target * prev->estimator_weight * (float)mult : 0.0f;
+    *selected = rto_restir_update(cur, prev->light_index, prev->sample_u0, prev->sample_u1, target,
+        cw, mult, u);
+    if (temporal) {
+        cur->temporal_candidate_count += mult;
+        cur->age = prev->age + 1;
+    } else {
+        cur->spatial_candidate_count += mult;
+    }
+    return mult;
+}
+
+/* restir_temporal_surface_valid (restir_di.h:137-153) */
+static int rs_surface_valid(const rto_reservoir* cur, const rto_reservoir* prev, float max_dist,
+    float min_dot, int max_age) {
+    if (prev->valid == 0 || prev->light_index < 0 || prev->age < 0 || prev->age >= max_age
+        || cur->material_type != prev->material_type || cur->primitive_type != prev->primitive_type
+        || cur->primitive_index != prev->primitive_index)
+        return 0;
+    const float lim = max_dist > 0.0f ? max_dist : 0.0f;
+    const float x = cur->position[0] - prev->position[0];
+    const float y = cur->position[1] - prev->position[1];
+    const float z = cur->position[2] - prev->position[2];
+    const float d2 = x * x + y * y + z * z;
+    const float nd = cur->normal[0] * prev->normal[0] + cur->normal[1] * prev->normal[1]
+                     + cur->normal[2] * prev->normal[2];
+    return d2 <= lim * lim && nd >= min_dot;
+}
+
+static void rs_surface(const hit_t* h, rto_reservoir* r) {
+    r->position[0] = h->pos.x; r->position[1] = h->pos.y; r->position[2] = h->pos.z;
+    r->normal[0] = h->sn.x; r->normal[1] = h->sn.y; r->normal[2] = h->sn.z;
+    r->material_type = h->mat_type;
+    r->primitive_type = h->prim_type;
+    r->primitive_index = h->prim_index;
+}
+
+/* restir_unshadowed_numerator (programs.cu:2123-2145) */
+static v3 rs_numerator(const ctx_t* c, const hit_t* h, const path_t* st, const light_sample_t* L,
+    int bsdf_avail, float* target) {
+    *target = 0.0f;
+    if (!L->valid || L->pdf <= 0.0f) return mk3(0, 0, 0);
+    float bpdf = 0.0f;
+    const v3 resp = direct_response(c, h, st, L->dir, &bpdf);
+    if (maxc3(resp) <= 0.0f) return mk3(0, 0, 0);
+    const float mis = !bsdf_avail ? 1.0f : power_heuristic(L->pdf, bpdf);
+    const v3 num = scl3(mul3(L->emission, resp), mis);
+    *target = fmaxf(0.0f, 0.2126f * num.x + 0.7152f * num.y + 0.0722f * num.z);
+    return num;
+}
+
+/* restir_previous_pixel (programs.cu:2147-2171) */
+static int rs_previous_pixel(const rto_restir_io* io, const rto_camera* cam, v3 pos, int* index) {
+    *index = -1;
+    if (!io || io->history_length <= 0 || !io->prev_cam || !io->history) return 0;
+    const rto_camera* pc = io->prev_cam;
+    const v3 origin = mk3((float)pc->origin[0], (float)pc->origin[1], (float)pc->origin[2]);
+    const v3 off = sub3(pos, origin);
+    const v3 bx = mk3((float)pc->basis_x[0], (float)pc->basis_x[1], (float)pc->basis_x[2]);
+    const v3 by = mk3((float)pc->basis_y[0], (float)pc->basis_y[1], (float)pc->basis_y[2]);
+    const v3 bz = mk3((float)pc->basis_z[0], (float)pc->basis_z[1], (float)pc->basis_z[2]);
+    const float cp[3] = {dot3(off, bx), dot3(off, by), dot3(off, bz)};
+    float px[2];
+    rto_project(pc, cp, px);
+    if (!isfinite(px[0]) || !isfinite(px[1])) return 0;
+    const int x = (int)floorf(px[0]);
+    const int y = (int)floorf(px[1]);
+    if (x < 0 || x >= cam->width || y < 0 || y >= cam->height) return 0;
+    *index = y * cam->width + x;
+    return 1;
+}
+
+/* restir_target_at_previous_surface (programs.cu:2210-2247) */
+static float rs_target_at_source(const ctx_t* c, const rto_restir_io* io, const rto_reservoir* src,
+    const rto_reservoir* sel, int bsdf_avail) {
+    if (!io || !io->prev_cam) return 0.0f;
+    const rto_camera* pc = io->prev_cam;
+    const v3 sp = ld3(src->position);
+    const v3 origin = mk3((float)pc->origin[0], (float)pc->origin[1], (float)pc->origin[2]);
+    const v3 off = sub3(sp, origin);
+    const float dist = len3(off);
+    if (!isfinite(dist) || dist <= OM_RAY_EPS) return 0.0f;
+    path_t ss;
+    memset(&ss, 0, sizeof(ss));
+    ss.o = origin;
+    ss.d = scl3(off, 1.0f / dist);
+    float t, u, v;
+    int prim;
+    if (!om_closest(c->s, c->bvh, ss.o, ss.d, OM_RAY_EPS, dist + 0.05f, &t, &prim, &u, &v)) return 0.0f;
+    hit_t h;
+    make_hit(c->s, ss.o, ss.d, t, prim, &h);
+    if (h.mat_type != src->material_type || h.prim_type != src->primitive_type
+        || h.prim_index != src->primitive_index)
+        return 0.0f;
+    const float tol = fmaxf(0.01f, 0.02f * dist);
+    if (lensq3(sub3(h.pos, sp)) > tol * tol || dot3(h.sn, ld3(src->normal)) < 0.95f) return 0.0f;
+    const light_sample_t L = analytic_candidate(c, h.pos, sel->light_index, sel->sample_u0, sel->sample_u1);
+    float target = 0.0f;
+    rs_numerator(c, &h, &ss, &L, bsdf_avail, &target);
+    return target;
+}
+
+typedef struct {
+    const rto_restir_io* io;
+    const rto_camera* cam;
+    const rto_profile* pr;
+    uint32_t pixel, sample, cam_index, stream;
+    int x, y;
+} rs_ctx;
+
+/* accumulate_restir_analytic_direct_light (programs.cu:2315-2449); spatial
+ * neighbours come from the fixed 8-ring of the reference, or, when
+ * restir_spatial_radius > 1, from uniform offsets in a disk of that radius */
+static void restir_direct(const ctx_t* c, const hit_t* h, const rs_ctx* rc, int bsdf_avail, path_t* st) {
+    if (h->mat_type == RTO_MAT_DIFFUSE_LIGHT || c->a_count <= 0) return;
+    const rto_profile* pr = rc->pr;
+    rto_reservoir r;
+    rs_init(&r);
+    const int req = pr->restir_initial_candidates > 0 ? pr->restir_initial_candidates : 1;
+    const int ncand = req < 32 ? req : 32;
+    for (int k = 0; k < ncand; ++k) {
+        float q[4];
+        om_rng4(rc->pixel, rc->sample, rc->cam_index, 0u, (uint32_t)(OM_G_RESTIR0 + k), rc->stream, q);
+        const int li = pick_analytic(c, q[0]);
+        const light_sample_t L = analytic_candidate(c, h->pos, li, q[1], q[2]);
+        float target = 0.0f;
+        rs_numerator(c, h, st, &L, bsdf_avail, &target);
+        const float w = L.pdf > 0.0f ? target / L.pdf : 0.0f;
+        rto_restir_update(&r, li, q[1], q[2], target, w, 1, q[3]);
+    }
+    const int canonical = r.candidate_count;
+    rto_reservoir sources[9];
+    int mults[9];
+    int nsrc = 0, selected_source = -1;
+    float sel_u[12], off_u[16];
+    for (int g = 0; g < 3; ++g) om_rng4(rc->pixel, rc->sample, rc->cam_index, 0u, (uint32_t)(OM_G_RESTIR0 + 32 + g), rc->stream, sel_u + 4 * g);
+    for (int g = 0; g < 4; ++g) om_rng4(rc->pixel, rc->sample, rc->cam_index, 0u, (uint32_t)(OM_G_RESTIR0 + 35 + g), rc->stream, off_u + 4 * g);
+    rto_reservoir cur_surface;
+    rs_init(&cur_surface);
+    rs_surface(h, &cur_surface);
+    if (pr->restir_temporal_reuse != 0) {
+        int pidx;
+        if (rs_previous_pixel(rc->io, rc->cam, h->pos, &pidx)) {
+            const rto_reservoir* prev = &rc->io->history[pidx];
+            if (rs_surface_valid(&cur_surface, prev, fmaxf(0.005f, 0.01f * h->t), 0.95f, pr->restir_max_history_age)) {
+                const light_sample_t L = analytic_candidate(c, h->pos, prev->light_index, prev->sample_u0, prev->sample_u1);
+                float target = 0.0f;
+                rs_numerator(c, h, st, &L, bsdf_avail, &target);
+                int sel = 0;
+                const int mult = rto_restir_merge(&r, prev, target, pr->restir_max_temporal_candidates, sel_u[0], 1, &sel);
+                if (mult > 0) {
+                    sources[nsrc] = *prev;
+                    mults[nsrc] = mult;
+                    if (sel) selected_source = nsrc;
+                    ++nsrc;
+                }
+            }
+        }
+    }
+    const int nsp = pr->restir_spatial_neighbors < 8 ? pr->restir_spatial_neighbors : 8;
+    if (nsp > 0) {
+        int ridx;
+        if (rs_previous_pixel(rc->io, rc->cam, h->pos, &ridx)) {
+            static const int ring[8][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}, {-1, -1}, {1, -1}, {-1, 1}, {1, 1}};
+            const int rot = (rc->x * 3 + rc->y * 5 + (int)rc->stream) & 7;
+            const int bx = ridx % rc->cam->width, by = ridx / rc->cam->width;
+            for (int j = 0; j < nsp; ++j) {
+                int dx, dy;
+                if (pr->restir_spatial_radius > 1) {
+                    const float rad = (float)pr->restir_spatial_radius * sqrtf(off_u[2 * j]);
+                    float sp_, cp_;
+                    om_sincos_turn(off_u[2 * j + 1], &sp_, &cp_);
+                    dx = (int)floorf(rad * cp_ + 0.5f);
+                    dy = (int)floorf(rad * sp_ + 0.5f);
+                } else {
+                    dx = ring[(rot + j) & 7][0];
+                    dy = ring[(rot + j) & 7][1];
+                }
+                const int nx = bx + dx, ny = by + dy;
+                if (nx < 0 || nx >= rc->cam->width || ny < 0 || ny >= rc->cam->height) continue;
+                const rto_reservoir* prev = &rc->io->history[ny * rc->cam->width + nx];
+                if (!rs_surface_valid(&cur_surface, prev, fmaxf(0.01f, 0.02f * h->t), 0.95f, pr->restir_max_history_age)) continue;
+                const light_sample_t L = analytic_candidate(c, h->pos, prev->light_index, prev->sample_u0, prev->sample_u1);
+                float target = 0.0f;
+                rs_numerator(c, h, st, &L, bsdf_avail, &target);
+                int sel = 0;
+                const int mult = rto_restir_merge(&r, prev, target, pr->restir_max_spatial_candidates, sel_u[1 + j], 0, &sel);
+                if (mult > 0 && nsrc < 9) {
+                    sources[nsrc] = *prev;
+                    mults[nsrc] = mult;
+                    if (sel) selected_source = nsrc;
+                    ++nsrc;
+                }
+            }
+        }
+    }
+    rs_surface(h, &r);
+    if (pr->restir_bias_correction == 1 && nsrc > 0 && r.valid != 0) {
+        float sst = r.selected_target;
+        float sum = r.selected_target * (float)canonical;
+        for (int i = 0; i < nsrc; ++i) {
+            const float t = rs_target_at_source(c, rc->io, &sources[i], &r, bsdf_avail);
+            sum += t * (float)mults[i];
+            if (selected_source == i) sst = t;
+        }
+        rto_restir_finalize_basic(&r, sst, sum, nsrc);
+    } else {
+        rto_restir_finalize(&r);
+    }
+    if (rc->io && rc->io->out && (int)rc->sample == (pr->samples_per_pixel > 0 ? pr->samples_per_pixel : 1) - 1)
+        rc->io->out[rc->pixel] = r;
+    if (r.valid == 0) return;
+    const light_sample_t L = analytic_candidate(c, h->pos, r.light_index, r.sample_u0, r.sample_u1);
+    float target = 0.0f;
+    const v3 num = rs_numerator(c, h, st, &L, bsdf_avail, &target);
+    if (!L.valid || target <= 0.0f) return;
+    const v3 so = add3(h->pos, scl3(L.dir, OM_RAY_EPS * 2.0f));
+    const float max_t = L.dist - OM_RAY_EPS * 3.0f;
+    if (max_t <= OM_RAY_EPS || om_occluded(c->s, c->bvh, so, L.dir, OM_RAY_EPS, max_t)) return;
+    const v3 direct = scl3(num, r.estimator_weight);
     st->radiance = add3(st->radiance, mul3(st->throughput, direct));
 }
 

@@ -188,6 +188,40 @@ int rto_render_camera(const rto_scene* scene, const rto_bvh* bvh, const rto_came
     int cam_index, const rto_profile* profile, uint32_t stream, float* beauty, float* normal,
     float* albedo, float* depth, int threads, int row_begin, int row_end,
     rto_render_stats* stats);
+/* ReSTIR DI reservoir, field order of RestirReservoir (src/common/restir_di.h:36-48), 84 B */
+typedef struct {
+    int32_t light_index; float sample_u0, sample_u1;
+    float position[3]; float normal[3];
+    int32_t material_type, primitive_type, primitive_index;
+    float weight_sum, selected_target, estimator_weight;
+    int32_t candidate_count, temporal_candidate_count, spatial_candidate_count,
+        bias_correction_source_count, age, valid;
+} rto_reservoir;
+
+/* per-camera frame history for ReSTIR reuse; any pointer may be NULL */
+typedef struct {
+    const rto_camera* prev_cam;     /* camera of the previous frame (NULL: no history) */
+    const rto_reservoir* history;   /* previous frame's reservoirs, W*H */
+    rto_reservoir* out;             /* this frame's reservoirs, W*H (last sample wins) */
+    int32_t history_length;
+    int32_t pad;
+} rto_restir_io;
+
+int rto_render_camera_ex(const rto_scene* scene, const rto_bvh* bvh, const rto_camera* cam,
+    int cam_index, const rto_profile* profile, uint32_t stream, const rto_restir_io* restir,
+    float* beauty, float* normal, float* albedo, float* depth, int threads, int row_begin,
+    int row_end, rto_render_stats* stats);
+int rto_build_analytic_cdf(const rto_scene* scene, float* sel, float* cdf);
+
+/* reservoir math (src/common/restir_di.h:67-199) for known-answer tests */
+int rto_restir_update(rto_reservoir* r, int light, float u0, float u1, float target, float weight,
+    int multiplicity, float u);
+void rto_restir_finalize(rto_reservoir* r);
+void rto_restir_finalize_basic(rto_reservoir* r, float selected_source_target,
+    float source_target_sum, int source_count);
+int rto_restir_merge(rto_reservoir* cur, const rto_reservoir* prev, float target, int max_count,
+    float u, int temporal, int* selected);
+
 /* per-path radiance of one (pixel, sample): for tests of single paths */
 void rto_trace_path(const rto_scene* scene, const rto_bvh* bvh, const rto_camera* cam,
     int cam_index, const rto_profile* profile, uint32_t stream, int x, int y, int sample,
