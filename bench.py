@@ -67,6 +67,10 @@ def cpu_baseline(config: str, rows: int, threads: int) -> dict:
                       f"{prof.samples_per_pixel} spp, depth {prof.max_bounces}) in {dt:.2f} s"}
 
 
+def cam_desc(cam) -> str:
+    return f"{cam.width}x{cam.height} {'equi' if cam.model == 1 else 'pinhole'}"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +107,15 @@ def main():
         ren.set_tuning(**tune)
     rig = g.rig()
     prof = g.profile
+    # scenes with an orbiting rig advance one orbit step per frame (ReSTIR
+    # history then reprojects real camera motion); the others keep a fixed rig
+    try:
+        orbit = [g.rig_at(i, 60) for i in range(60)]
+    except rt.RtError:
+        orbit = None
+
+    def rig_for(i):
+        return orbit[i % 60] if orbit else rig
     paths_per_frame = sum(rig.cameras[i].width * rig.cameras[i].height
                           for i in range(rig.active_count)) * prof.samples_per_pixel
     log(f"[bench] {info['name']} CUs={info['cu_count']} config={a.config} prims={scene.prim_count} "
@@ -120,12 +133,12 @@ def main():
         f"{cnt.prim_tests / max(cnt.extend_rays, 1):.2f}")
 
     for i in range(a.warmup):
-        ren.render(rig, prof, 1000 + i)
+        ren.render(rig_for(i), prof, 1000 + i)
     dev.synchronize()
     frames = []
     t0 = time.perf_counter()
     for i in range(a.steps):
-        frames.append(ren.render(rig, prof, i))
+        frames.append(ren.render(rig_for(a.warmup + i), prof, i))
         log(f"[bench] step {i} render_ms={frames[-1].render_ms:.3f}")
     dev.synchronize()
     wall = time.perf_counter() - t0
@@ -150,8 +163,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{a.config}: plain wavefront tracer, 4x640x480 pinhole, "
-                               f"{prof.samples_per_pixel} spp, depth {prof.max_bounces}",
+        "config": {"workload": f"{a.config}: wavefront path tracer, "
+                               f"{'+'.join(cam_desc(rig.cameras[i]) for i in range(rig.active_count))}, "
+                               f"{prof.samples_per_pixel} spp, depth {prof.max_bounces}"
+                               f"{', ReSTIR DI' if prof.enable_restir_di else ''}",
                    "cameras": rig.active_count, "spp": prof.samples_per_pixel,
                    "max_bounces": prof.max_bounces, "paths_per_frame": paths_per_frame,
                    "gpu_ms_per_frame": round(gpu_ms, 3),

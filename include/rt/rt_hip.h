@@ -247,6 +247,10 @@ rt_status rt_frame_device_view(rt_renderer* r, int camera, rt_device_frame_view*
 rt_status rt_reset_accumulation(rt_renderer* r);
 rt_status rt_reset_sequence(rt_renderer* r, uint32_t seed);
 rt_status rt_restir_diagnostics(rt_renderer* r, int camera, rt_restir_stats* out);
+/* the last frame's RestirReservoir array of one camera (W*H x 84 B,
+ * src/common/restir_di.h:36-48 layout); replaces the reservoir download of
+ * OptixRenderer::restir_diagnostics (optix_renderer.cpp:183-215) */
+rt_status rt_restir_download(rt_renderer* r, int camera, void* reservoirs);
 
 /* tuning knobs chosen by the autotuner; 0 keeps the current value */
 typedef struct rt_tuning {
@@ -272,6 +276,10 @@ typedef struct rt_generated_scene {
 } rt_generated_scene;
 rt_status rt_scene_generate(int config, uint64_t seed, rt_generated_scene** out);
 rt_status rt_scene_generated_free(rt_generated_scene* g);
+/* camera rig of frame `frame` of a `frames`-frame orbit (config 2; the other
+ * configs have static rigs and return their frame-0 rig) */
+rt_status rt_scene_generate_rig(int config, uint64_t seed, int frame, int frames,
+    rt_camera_rig* out);
 
 #ifdef __cplusplus
 }

@@ -123,6 +123,34 @@ class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("rays_closest", C.c_uint64), ("rays_shadow", C.c_uint64)]
 
 
+class Reservoir(C.Structure):
+    """RestirReservoir field order (src/common/restir_di.h:36-48), 84 bytes."""
+    _fields_ = [("light_index", C.c_int32), ("sample_u0", C.c_float), ("sample_u1", C.c_float),
+                ("position", f3), ("normal", f3), ("material_type", C.c_int32),
+                ("primitive_type", C.c_int32), ("primitive_index", C.c_int32),
+                ("weight_sum", C.c_float), ("selected_target", C.c_float),
+                ("estimator_weight", C.c_float), ("candidate_count", C.c_int32),
+                ("temporal_candidate_count", C.c_int32), ("spatial_candidate_count", C.c_int32),
+                ("bias_correction_source_count", C.c_int32), ("age", C.c_int32),
+                ("valid", C.c_int32)]
+
+
+class RestirIO(C.Structure):
+    _fields_ = [("prev_cam", C.c_void_p), ("history", C.c_void_p), ("out", C.c_void_p),
+                ("history_length", C.c_int32), ("pad", C.c_int32)]
+
+
+RESERVOIR_DTYPE = np.dtype([("light_index", "<i4"), ("sample_u0", "<f4"), ("sample_u1", "<f4"),
+                            ("position", "<f4", 3), ("normal", "<f4", 3),
+                            ("material_type", "<i4"), ("primitive_type", "<i4"),
+                            ("primitive_index", "<i4"), ("weight_sum", "<f4"),
+                            ("selected_target", "<f4"), ("estimator_weight", "<f4"),
+                            ("candidate_count", "<i4"), ("temporal_candidate_count", "<i4"),
+                            ("spatial_candidate_count", "<i4"),
+                            ("bias_correction_source_count", "<i4"), ("age", "<i4"),
+                            ("valid", "<i4")])
+
+
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<i4"), ("u", "<f4"), ("v", "<f4")])
 NODE_DTYPE = np.dtype([("lo0", "<f4", 3), ("hi0", "<f4", 3), ("lo1", "<f4", 3),
@@ -180,6 +208,19 @@ def lib():
         L.rto_openpbr_sample_fn.argtypes = [P(OpenPbr), C.c_void_p, C.c_void_p, C.c_float,
                                             C.c_float, C.c_float, C.c_void_p]
         L.rto_openpbr_emission.argtypes = [P(OpenPbr), C.c_void_p]
+        L.rto_generate_rig.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, P(Camera),
+                                       P(C.c_int)]
+        L.rto_render_camera_ex.argtypes = [P(Scene), P(Bvh), P(Camera), C.c_int, P(Profile),
+                                           C.c_uint32, P(RestirIO), C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                           P(RenderStats)]
+        L.rto_restir_update.argtypes = [P(Reservoir), C.c_int, C.c_float, C.c_float, C.c_float,
+                                        C.c_float, C.c_int, C.c_float]
+        L.rto_restir_finalize.argtypes = [P(Reservoir)]
+        L.rto_restir_finalize_basic.argtypes = [P(Reservoir), C.c_float, C.c_float, C.c_int]
+        L.rto_restir_merge.argtypes = [P(Reservoir), P(Reservoir), C.c_float, C.c_int, C.c_float,
+                                       C.c_int, P(C.c_int)]
+        L.rto_build_analytic_cdf.argtypes = [P(Scene), C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -389,6 +430,60 @@ def render_camera(scene: Scene, bvh: OracleBvh, cam: Camera, cam_index: int, pro
         raise RuntimeError("oracle render failed")
     return {"beauty": beauty, "normal": normal, "albedo": albedo, "depth": depth,
             "paths": st.paths, "rays_closest": st.rays_closest, "rays_shadow": st.rays_shadow}
+
+
+def generate_rig(config: int, seed: int, frame: int, frames: int):
+    cams = (Camera * 4)()
+    n = C.c_int()
+    if lib().rto_generate_rig(config, seed, frame, frames, cams, C.byref(n)) != 0:
+        raise ValueError(f"config {config} has no orbit rig")
+    return [cams[i] for i in range(n.value)]
+
+
+class RestirSequence:
+    """Multi-frame oracle render keeping per-camera ReSTIR history, as the
+    renderer does between frames (capture_launch_history)."""
+
+    def __init__(self, scene: Scene, bvh: "OracleBvh", profile: Profile, threads: int = 8):
+        self.scene, self.bvh, self.profile, self.threads = scene, bvh, profile, threads
+        self.prev = {}
+        self.frame = 0
+
+    def render(self, cams, stream: int, rows=None):
+        outs = []
+        new_prev = {}
+        for ci, cam in enumerate(cams):
+            w, h = cam.width, cam.height
+            res = np.zeros(w * h, RESERVOIR_DTYPE)
+            io = RestirIO()
+            keep = []
+            if ci in self.prev and self.frame > 0:
+                pc, hist = self.prev[ci]
+                keep += [pc, hist]
+                io.prev_cam = C.cast(C.pointer(pc), C.c_void_p)
+                io.history = hist.ctypes.data
+                io.history_length = self.frame
+            io.out = res.ctypes.data
+            beauty = np.zeros((h, w, 4), np.float32)
+            normal = np.zeros((h, w, 4), np.float32)
+            albedo = np.zeros((h, w, 4), np.float32)
+            depth = np.zeros((h, w), np.float32)
+            st = RenderStats()
+            r0, r1 = (0, h) if rows is None else rows
+            rc = lib().rto_render_camera_ex(C.byref(self.scene), C.byref(self.bvh.bvh),
+                                            C.byref(cam), ci, C.byref(self.profile), stream,
+                                            C.byref(io), ptr(beauty), ptr(normal), ptr(albedo),
+                                            ptr(depth), self.threads, r0, r1, C.byref(st))
+            if rc != 0:
+                raise RuntimeError("oracle render failed")
+            cam_copy = Camera()
+            C.memmove(C.byref(cam_copy), C.byref(cam), C.sizeof(Camera))
+            new_prev[ci] = (cam_copy, res)
+            outs.append({"beauty": beauty, "normal": normal, "albedo": albedo, "depth": depth,
+                         "reservoirs": res})
+        self.prev = new_prev
+        self.frame += 1
+        return outs
 
 
 def rng4(pixel, sample, camera, bounce, group, stream) -> np.ndarray:

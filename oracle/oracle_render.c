@@ -1159,7 +1159,8 @@ typedef struct { v3 radiance, normal, albedo; float depth; int aux; } path_out;
 
 /* programs.cu:1828-1953 (one sample of one pixel) */
 static void trace_one(const ctx_t* c, const rto_camera* cam, int cam_index, const rto_profile* pr,
-    uint32_t stream, int x, int y, int sample, path_out* out, rto_render_stats* stats) {
+    uint32_t stream, const rto_restir_io* io, int x, int y, int sample, path_out* out,
+    rto_render_stats* stats) {
     const rto_scene* s = c->s;
     const uint32_t pixel = (uint32_t)(y * cam->width + x);
     double px, py;
@@ -1189,6 +1190,13 @@ static void trace_one(const ctx_t* c, const rto_camera* cam, int cam_index, cons
         int prim;
         if (stats) stats->rays_closest++;
         const int found = om_closest(s, c->bvh, st.o, st.d, OM_RAY_EPS, OM_RAY_FAR, &t, &prim, &u, &v);
+        int ali = -1;
+        if (c->a_count > 0 && intersect_analytic(c, st.o, st.d, OM_RAY_EPS, found ? t : OM_RAY_FAR, &ali)) {
+            const float w = analytic_mis(c, &st, ali);
+            st.radiance = add3(st.radiance,
+                mul3(st.throughput, scl3(ld3(s->analytic_lights[ali].radiance), w)));
+            break;
+        }
         if (!found) {
             const float w = emission_mis(c, &st, NULL, 1);
             st.radiance = add3(st.radiance, mul3(st.throughput, scl3(background_at(s, st.d), w)));
@@ -1207,6 +1215,15 @@ static void trace_one(const ctx_t* c, const rto_camera* cam, int cam_index, cons
         const int bsdf_avail = b + 1 < max_b;
         if (stats && h.mat_type != RTO_MAT_DIFFUSE_LIGHT && c->light_count > 0) stats->rays_shadow++;
         direct_light(c, &h, rl, bsdf_avail, &st);
+        const int min_lights = pr->restir_min_analytic_lights > 0 ? pr->restir_min_analytic_lights : 1;
+        if (pr->enable_restir_di && b == 0 && c->a_count >= min_lights) {
+            rs_ctx rc = {io, cam, pr, pixel, (uint32_t)sample, (uint32_t)cam_index, stream, x, y};
+            restir_direct(c, &h, &rc, bsdf_avail, &st);
+        } else {
+            float ra[4];
+            om_rng4(pixel, (uint32_t)sample, (uint32_t)cam_index, (uint32_t)b, OM_G_ALIGHT, stream, ra);
+            analytic_direct_light(c, &h, ra, bsdf_avail, &st);
+        }
         if (!bsdf_avail) break;
         sample_bsdf(c, &h, rb, &st);
         if (st.alive && b >= pr->rr_start_bounce) {
@@ -1223,7 +1240,7 @@ void rto_trace_path(const rto_scene* s, const rto_bvh* bvh, const rto_camera* ca
     ctx_t c;
     if (ctx_init(&c, s, bvh) != 0) return;
     path_out po;
-    trace_one(&c, cam, cam_index, pr, stream, x, y, sample, &po, NULL);
+    trace_one(&c, cam, cam_index, pr, stream, NULL, x, y, sample, &po, NULL);
     radiance[0] = po.radiance.x; radiance[1] = po.radiance.y; radiance[2] = po.radiance.z;
     ctx_free(&c);
 }
@@ -1231,6 +1248,7 @@ void rto_trace_path(const rto_scene* s, const rto_bvh* bvh, const rto_camera* ca
 /* --------------------------------------------------------------- render */
 typedef struct {
     const ctx_t* c; const rto_camera* cam; int cam_index; const rto_profile* pr; uint32_t stream;
+    const rto_restir_io* io;
     float *beauty, *normal, *albedo, *depth;
     int row_begin, row_end;
     volatile int next_row;
@@ -1254,7 +1272,7 @@ static void* render_worker(void* arg) {
             int aux = 0;
             for (int s = 0; s < spp; ++s) {
                 path_out po;
-                trace_one(j->c, cam, j->cam_index, j->pr, j->stream, x, y, s, &po, &local);
+                trace_one(j->c, cam, j->cam_index, j->pr, j->stream, j->io, x, y, s, &po, &local);
                 local.paths++;
                 beauty = add3(beauty, po.radiance);
                 if (po.aux) {
@@ -1295,6 +1313,14 @@ static void* render_worker(void* arg) {
 int rto_render_camera(const rto_scene* s, const rto_bvh* bvh, const rto_camera* cam, int cam_index,
     const rto_profile* pr, uint32_t stream, float* beauty, float* normal, float* albedo,
     float* depth, int threads, int row_begin, int row_end, rto_render_stats* stats) {
+    return rto_render_camera_ex(s, bvh, cam, cam_index, pr, stream, NULL, beauty, normal, albedo,
+        depth, threads, row_begin, row_end, stats);
+}
+
+int rto_render_camera_ex(const rto_scene* s, const rto_bvh* bvh, const rto_camera* cam,
+    int cam_index, const rto_profile* pr, uint32_t stream, const rto_restir_io* io, float* beauty,
+    float* normal, float* albedo, float* depth, int threads, int row_begin, int row_end,
+    rto_render_stats* stats) {
     static ctx_t c;  /* large; one render at a time per process */
     static pthread_mutex_t once = PTHREAD_MUTEX_INITIALIZER;
     pthread_mutex_lock(&once);
@@ -1303,7 +1329,7 @@ int rto_render_camera(const rto_scene* s, const rto_bvh* bvh, const rto_camera* 
     if (row_begin < 0) row_begin = 0;
     render_job j;
     memset(&j, 0, sizeof(j));
-    j.c = &c; j.cam = cam; j.cam_index = cam_index; j.pr = pr; j.stream = stream;
+    j.c = &c; j.cam = cam; j.cam_index = cam_index; j.pr = pr; j.stream = stream; j.io = io;
     j.beauty = beauty; j.normal = normal; j.albedo = albedo; j.depth = depth;
     j.row_begin = row_begin; j.row_end = row_end; j.next_row = row_begin;
     pthread_mutex_init(&j.lock, NULL);

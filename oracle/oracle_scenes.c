@@ -294,11 +294,156 @@ static int gen_cornell_mesh(rto_generated* g, uint64_t seed) {
     return 0;
 }
 
+/* equi62_lut1d camera with the reference's default 120 degree horizontal
+ * field of view (camera_models.cpp:196-226) and a 1024-entry theta LUT */
+static void lookat_equi(rto_camera* c, int w, int h, const double radial[6], const double from[3],
+    const double at[3], const double up[3]) {
+    lookat_pinhole(c, w, h, 60.0, from, at, up);
+    c->model = RTO_CAM_EQUI62_LUT1D;
+    const double hfov = 120.0 * 3.14159265358979323846 / 180.0;
+    const double f = 0.5 * (double)w / (0.5 * hfov);
+    c->fx = f; c->fy = f; c->cx = 0.5 * (double)w; c->cy = 0.5 * (double)h;
+    for (int i = 0; i < 6; ++i) c->radial[i] = radial[i];
+    c->tangential[0] = 0.0; c->tangential[1] = 0.0;
+    rto_make_equi_lut(c);
+}
+
+static double gauss(pcg32_t* r, double sigma) {
+    const double u1 = 1.0 - (double)pcg_u(r);
+    const double u2 = (double)pcg_u(r);
+    return sigma * sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+}
+
+/* four cameras on a radius-70 orbit around the cube centre, 90 degrees apart;
+ * frame f of `frames` advances the orbit by 2*pi*f/frames */
+static void instanced_rig(uint64_t seed, int frame, int frames, rto_camera* cams) {
+    pcg32_t r;
+    pcg_seed(&r, seed ^ 0x5bd1e995ULL);
+    /* k_i ~ N(0, 0.01), redrawn until theta_d(theta) is monotone on
+     * [0, 1.6] rad (derivative >= 0.2) so the LUT inverse is well defined */
+    double radial[2][6];
+    for (int k = 0; k < 2; ++k) {
+        for (int tries = 0; tries < 10000; ++tries) {
+            for (int i = 0; i < 6; ++i) radial[k][i] = gauss(&r, 0.01);
+            int ok = 1;
+            for (int s = 0; s <= 64 && ok; ++s) {
+                const double th = 1.6 * (double)s / 64.0, t2 = th * th;
+                double der = 1.0, tp = t2;
+                for (int i = 0; i < 6; ++i) { der += (double)(2 * i + 3) * radial[k][i] * tp; tp *= t2; }
+                ok = der >= 0.2;
+            }
+            if (ok) break;
+        }
+    }
+    const double center[3] = {50.0, 50.0, 50.0}, up[3] = {0.0, 1.0, 0.0};
+    for (int k = 0; k < 4; ++k) {
+        const double th = 2.0 * 3.14159265358979323846 * (double)frame / (double)(frames > 0 ? frames : 1)
+                          + (double)k * 0.5 * 3.14159265358979323846;
+        const double from[3] = {center[0] + 70.0 * cos(th), center[1] + 15.0, center[2] + 70.0 * sin(th)};
+        if (k < 2) lookat_pinhole(&cams[k], 640, 480, 60.0, from, center, up);
+        else lookat_equi(&cams[k], 1024, 1024, radial[k - 2], from, center, up);
+    }
+}
+
+int rto_generate_rig(int config, uint64_t seed, int frame, int frames, rto_camera* cams, int* count) {
+    if (config != 2) return -1;
+    instanced_rig(seed, frame, frames, cams);
+    *count = 4;
+    return 0;
+}
+
+static int gen_instanced_mesh(rto_generated* g, uint64_t seed) {
+    pcg32_t r;
+    pcg_seed(&r, seed);
+    /* torus 32 x 16 segments = 1024 triangles, tube radius jittered per vertex */
+    enum { NU = 32, NV = 16, NINST = 1000, NMAT = 16, NLIGHT = 256 };
+    double verts[NU * NV][3];
+    for (int i = 0; i < NU; ++i) {
+        for (int j = 0; j < NV; ++j) {
+            const double u = 2.0 * 3.14159265358979323846 * (double)i / NU;
+            const double v = 2.0 * 3.14159265358979323846 * (double)j / NV;
+            const double tube = 0.35 * (1.0 + 0.1 * (2.0 * (double)pcg_u(&r) - 1.0));
+            verts[i * NV + j][0] = (1.0 + tube * cos(v)) * cos(u);
+            verts[i * NV + j][1] = tube * sin(v);
+            verts[i * NV + j][2] = (1.0 + tube * cos(v)) * sin(u);
+        }
+    }
+    int faces[NU * NV * 2][3];
+    int nf = 0;
+    for (int i = 0; i < NU; ++i) {
+        for (int j = 0; j < NV; ++j) {
+            const int a = i * NV + j, b = ((i + 1) % NU) * NV + j;
+            const int c = ((i + 1) % NU) * NV + (j + 1) % NV, d = i * NV + (j + 1) % NV;
+            faces[nf][0] = a; faces[nf][1] = b; faces[nf][2] = c; ++nf;
+            faces[nf][0] = a; faces[nf][1] = c; faces[nf][2] = d; ++nf;
+        }
+    }
+    rto_material* mt = (rto_material*)calloc(NMAT, sizeof(rto_material));
+    for (int m = 0; m < NMAT; ++m) {
+        mt[m].type = RTO_MAT_LAMBERTIAN;
+        mt[m].ior = 1.0f;
+        mt[m].openpbr_index = -1;
+        for (int ch = 0; ch < 3; ++ch) mt[m].albedo[ch] = 0.2f + 0.7f * pcg_u(&r);
+    }
+    rto_triangle* tr = (rto_triangle*)calloc((size_t)NINST * (size_t)nf, sizeof(rto_triangle));
+    for (int k = 0; k < NINST; ++k) {
+        /* uniform rotation from a random unit quaternion (Shoemake 1992) */
+        const double u1 = (double)pcg_u(&r), u2 = (double)pcg_u(&r), u3 = (double)pcg_u(&r);
+        const double s1 = sqrt(1.0 - u1), s2 = sqrt(u1);
+        const double qx = s1 * sin(2.0 * 3.14159265358979323846 * u2);
+        const double qy = s1 * cos(2.0 * 3.14159265358979323846 * u2);
+        const double qz = s2 * sin(2.0 * 3.14159265358979323846 * u3);
+        const double qw = s2 * cos(2.0 * 3.14159265358979323846 * u3);
+        const double R[3][3] = {
+            {1.0 - 2.0 * (qy * qy + qz * qz), 2.0 * (qx * qy - qz * qw), 2.0 * (qx * qz + qy * qw)},
+            {2.0 * (qx * qy + qz * qw), 1.0 - 2.0 * (qx * qx + qz * qz), 2.0 * (qy * qz - qx * qw)},
+            {2.0 * (qx * qz - qy * qw), 2.0 * (qy * qz + qx * qw), 1.0 - 2.0 * (qx * qx + qy * qy)}};
+        const double t[3] = {100.0 * (double)pcg_u(&r), 100.0 * (double)pcg_u(&r), 100.0 * (double)pcg_u(&r)};
+        const int mat = (int)(pcg_u(&r) * (float)NMAT) % NMAT;
+        const double scale = 2.0;
+        for (int f = 0; f < nf; ++f) {
+            rto_triangle* o = &tr[(size_t)k * (size_t)nf + (size_t)f];
+            float* dst[3] = {o->p0, o->p1, o->p2};
+            for (int vi = 0; vi < 3; ++vi) {
+                const double* p = verts[faces[f][vi]];
+                for (int a = 0; a < 3; ++a)
+                    dst[vi][a] = (float)(scale * (R[a][0] * p[0] + R[a][1] * p[1] + R[a][2] * p[2]) + t[a]);
+            }
+            o->material = mat;
+        }
+    }
+    rto_analytic_light* al = (rto_analytic_light*)calloc(NLIGHT, sizeof(rto_analytic_light));
+    for (int i = 0; i < NLIGHT; ++i) {
+        al[i].type = RTO_ALIGHT_SPHERE;
+        for (int a = 0; a < 3; ++a) al[i].position[a] = 100.0f * pcg_u(&r);
+        al[i].radius = 0.1f + 0.4f * pcg_u(&r);
+        const float rad = 1.0f + 19.0f * pcg_u(&r);
+        set3(al[i].radiance, rad, rad, rad);
+    }
+    g->scene.triangle_count = NINST * nf;
+    g->scene.triangles = tr;
+    g->scene.material_count = NMAT;
+    g->scene.materials = mt;
+    g->scene.analytic_light_count = NLIGHT;
+    g->scene.analytic_lights = al;
+    g->scene.background_mode = RTO_BG_CONSTANT;
+    set3(g->scene.background, 0.0f, 0.0f, 0.0f);
+    g->camera_count = 4;
+    instanced_rig(seed, 0, 60, g->cameras);
+    profile_defaults(&g->profile, 1, 2);
+    g->profile.enable_restir_di = 1;
+    g->profile.restir_initial_candidates = 32;
+    g->profile.restir_spatial_neighbors = 5;
+    g->profile.restir_spatial_radius = 30;
+    return 0;
+}
+
 int rto_generate_scene(int config, uint64_t seed, rto_generated* g) {
     memset(g, 0, sizeof(*g));
     switch (config) {
         case 0: return gen_random_spheres(g, seed);
         case 1: return gen_cornell_mesh(g, seed);
+        case 2: return gen_instanced_mesh(g, seed);
         default: return -1;
     }
 }

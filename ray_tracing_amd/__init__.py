@@ -30,7 +30,8 @@ EXPORTED = [
     "rt_trace_closest_device", "rt_renderer_create", "rt_renderer_destroy", "rt_render_frame",
     "rt_frame_download", "rt_frame_device_view", "rt_reset_accumulation", "rt_reset_sequence",
     "rt_restir_diagnostics", "rt_renderer_set_tuning", "rt_renderer_get_tuning",
-    "rt_scene_generate", "rt_scene_generated_free",
+    "rt_scene_generate", "rt_scene_generated_free", "rt_restir_download",
+    "rt_scene_generate_rig",
 ]
 
 
@@ -188,6 +189,20 @@ class Renderer:
                                     C.byref(tm) if timed else None))
         return tm if timed else None
 
+    def reservoirs(self, camera: int) -> np.ndarray:
+        cam = self.rig.cameras[camera]
+        out = np.zeros(cam.width * cam.height, abi.RESERVOIR_DTYPE)
+        check(lib().rt_restir_download(self.h, camera, ptr(out)))
+        return out
+
+    def restir_stats(self, camera: int) -> dict:
+        st = abi.RestirStats()
+        check(lib().rt_restir_diagnostics(self.h, camera, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in abi.RestirStats._fields_}
+
+    def reset_accumulation(self):
+        check(lib().rt_reset_accumulation(self.h))
+
     def download(self, camera: int) -> dict:
         cam = self.rig.cameras[camera]
         w, h = cam.width, cam.height
@@ -207,6 +222,7 @@ class Generated:
         if isinstance(config, str):
             config = self.CONFIGS[config]
         self.config = config
+        self.seed = seed
         self.p = C.POINTER(GeneratedScene)()
         check(lib().rt_scene_generate(config, seed, C.byref(self.p)))
         self.g = self.p.contents
@@ -226,6 +242,11 @@ class Generated:
         p = Profile()
         C.memmove(C.byref(p), C.byref(self.g.profile), C.sizeof(Profile))
         return p
+
+    def rig_at(self, frame: int, frames: int = 60) -> CameraRig:
+        r = CameraRig()
+        check(lib().rt_scene_generate_rig(self.config, self.seed, frame, frames, C.byref(r)))
+        return r
 
     def rig(self, count: int | None = None) -> CameraRig:
         r = CameraRig()

@@ -127,6 +127,16 @@ class GeneratedScene(C.Structure):
                 ("cameras", Camera * 4), ("profile", Profile)]
 
 
+RESERVOIR_DTYPE = np.dtype([("light_index", "<i4"), ("sample_u0", "<f4"), ("sample_u1", "<f4"),
+                            ("position", "<f4", 3), ("normal", "<f4", 3),
+                            ("material_type", "<i4"), ("primitive_type", "<i4"),
+                            ("primitive_index", "<i4"), ("weight_sum", "<f4"),
+                            ("selected_target", "<f4"), ("estimator_weight", "<f4"),
+                            ("candidate_count", "<i4"), ("temporal_candidate_count", "<i4"),
+                            ("spatial_candidate_count", "<i4"),
+                            ("bias_correction_source_count", "<i4"), ("age", "<i4"),
+                            ("valid", "<i4")])
+
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<i4"), ("u", "<f4"), ("v", "<f4")])
 NODE_DTYPE = np.dtype([("lo0", "<f4", 3), ("hi0", "<f4", 3), ("lo1", "<f4", 3),
@@ -163,3 +173,5 @@ def declare(L):
     L.rt_renderer_get_tuning.argtypes = [vp, P(Tuning)]
     L.rt_scene_generate.argtypes = [C.c_int, C.c_uint64, P(P(GeneratedScene))]
     L.rt_scene_generated_free.argtypes = [P(GeneratedScene)]
+    L.rt_restir_download.argtypes = [vp, C.c_int, vp]
+    L.rt_scene_generate_rig.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, P(CameraRig)]

@@ -106,6 +106,7 @@ struct rt_scene {
     DevBuf<rt_analytic_light> analytic;
     std::vector<rtk::LightRec> host_lights;
     DevBuf<rtk::LightRec> lights;
+    DevBuf<float> a_sel, a_cdf;
     DevBuf<int32_t> light_of_prim;
     int32_t env_light = -1;
     // BVH
@@ -146,6 +147,8 @@ struct rt_scene {
         v.analytic = analytic.p;
         v.analytic_count = analytic_count;
         v.stack_depth = std::max(max_depth, 1);
+        v.a_sel = a_sel.p;
+        v.a_cdf = a_cdf.p;
         return v;
     }
 };
@@ -169,7 +172,21 @@ struct rt_renderer {
     DevBuf<float4> ray_o, ray_d, hit, throughput, radiance, prev_pos, prev_n, aux_normal, aux_albedo;
     DevBuf<int32_t> prev_flags;
     DevBuf<uint32_t> queue0, queue1;
-    DevBuf<float4> shadow_o, shadow_d, shadow_c;
+    DevBuf<float4> shadow_o, shadow_d, shadow_c, shadow2_o, shadow2_d, shadow2_c;
+    DevBuf<uint32_t> restir_queue;
+    DevBuf<float4> restir_o, restir_d, restir_thr;
+    // ReSTIR reservoirs (ping-pong, global pixel index) and per-camera history
+    DevBuf<rtk::Reservoir> reservoirs[2];
+    int res_cur = 0;
+    size_t reservoir_pixels = 0;
+    rt_camera prev_cams[4];
+    int history_length[4] = {0, 0, 0, 0};
+    int prev_valid[4] = {0, 0, 0, 0};
+    int prev_width[4] = {0, 0, 0, 0};
+    int prev_height[4] = {0, 0, 0, 0};
+    int prev_offset[4] = {0, 0, 0, 0};
+    bool restir_last = false;
+    int last_reservoirs = -1;
     DevBuf<uint32_t> counters;
     DevBuf<float4> acc_beauty, acc_normal, acc_albedo;
     size_t path_capacity = 0;
@@ -199,6 +216,13 @@ struct rt_renderer {
         b.shadow_o = shadow_o.p;
         b.shadow_d = shadow_d.p;
         b.shadow_c = shadow_c.p;
+        b.shadow2_o = shadow2_o.p;
+        b.shadow2_d = shadow2_d.p;
+        b.shadow2_c = shadow2_c.p;
+        b.restir_queue = restir_queue.p;
+        b.restir_o = restir_o.p;
+        b.restir_d = restir_d.p;
+        b.restir_thr = restir_thr.p;
         b.counters = counters.p;
         b.acc_beauty = acc_beauty.p;
         b.acc_normal = acc_normal.p;
@@ -210,12 +234,14 @@ struct rt_renderer {
         if (q <= path_capacity) return hipSuccess;
         hipError_t e;
         for (DevBuf<float4>* b : {&ray_o, &ray_d, &hit, &throughput, &radiance, &prev_pos, &prev_n,
-                 &aux_normal, &aux_albedo, &shadow_o, &shadow_d, &shadow_c}) {
+                 &aux_normal, &aux_albedo, &shadow_o, &shadow_d, &shadow_c, &shadow2_o, &shadow2_d,
+                 &shadow2_c, &restir_o, &restir_d, &restir_thr}) {
             if ((e = b->reserve(q)) != hipSuccess) return e;
         }
         if ((e = prev_flags.reserve(q)) != hipSuccess) return e;
         if ((e = queue0.reserve(q)) != hipSuccess) return e;
         if ((e = queue1.reserve(q)) != hipSuccess) return e;
+        if ((e = restir_queue.reserve(q)) != hipSuccess) return e;
         path_capacity = q;
         return hipSuccess;
     }
@@ -341,6 +367,39 @@ static void build_lights(const rt_scene_desc* d, std::vector<rtk::LightRec>& out
     if (!out.empty()) out.back().cdf = 1.0f;
 }
 
+// analytic light CDF (src/scene/analytic_light_compiler.cpp:15-39,230-237):
+// world area x luminance, accumulated in FP64, closed at the last positive light
+static void build_analytic_cdf(const rt_scene_desc* d, std::vector<float>& sel,
+    std::vector<float>& cdf) {
+    const int n = d->analytic_light_count;
+    sel.assign((size_t)n, 0.0f);
+    cdf.assign((size_t)n, 0.0f);
+    std::vector<double> w((size_t)n, 0.0);
+    double total = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const rt_analytic_light& L = d->analytic_lights[i];
+        const double r = (double)L.radius;
+        const double area = 4.0 * 3.14159265358979323846 * r * r;
+        const double lum = 0.2126 * (double)L.radiance[0] + 0.7152 * (double)L.radiance[1]
+                           + 0.0722 * (double)L.radiance[2];
+        double x = area * lum;
+        x = std::isfinite(x) ? (x > 0.0 ? x : 0.0) : 0.0;
+        w[(size_t)i] = x;
+        total += x;
+    }
+    double acc = 0.0;
+    int last = -1;
+    for (int i = 0; i < n; ++i) {
+        const double p = total > 0.0 ? w[(size_t)i] / total : 0.0;
+        acc += p;
+        sel[(size_t)i] = (float)p;
+        cdf[(size_t)i] = (float)acc;
+        if (p > 0.0) last = i;
+    }
+    if (total > 0.0)
+        for (int i = last; i < n; ++i) cdf[(size_t)i] = 1.0f;
+}
+
 rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out) {
     RT_REQUIRE(dev && d && out, RT_ERROR_INVALID_ARGUMENT, "rt_scene_upload requires device, description and output");
     RT_REQUIRE(d->sphere_count >= 0 && d->quad_count >= 0 && d->triangle_count >= 0
@@ -363,6 +422,10 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
             RT_ERROR_UNSUPPORTED,
             "OpenPBR material " + std::to_string(i)
                 + " uses a lobe outside the supported constant-input subset (base, specular, metal, coat, emission)");
+    }
+    for (int i = 0; i < d->analytic_light_count; ++i) {
+        RT_REQUIRE(d->analytic_lights[i].type == RT_ALIGHT_SPHERE, RT_ERROR_UNSUPPORTED,
+            "analytic light " + std::to_string(i) + " is not a sphere light");
     }
     RT_HIP(hipSetDevice(dev->ordinal));
     auto* s = new rt_scene();
@@ -400,6 +463,10 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
     if (e == hipSuccess) e = upload(s->analytic, d->analytic_lights, (size_t)d->analytic_light_count, st);
     if (e == hipSuccess) e = upload(s->lights, s->host_lights.data(), s->host_lights.size(), st);
     if (e == hipSuccess) e = upload(s->light_of_prim, lop.data(), lop.size(), st);
+    std::vector<float> asel, acdf;
+    build_analytic_cdf(d, asel, acdf);
+    if (e == hipSuccess) e = upload(s->a_sel, asel.data(), asel.size(), st);
+    if (e == hipSuccess) e = upload(s->a_cdf, acdf.data(), acdf.size(), st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         delete s;
@@ -570,9 +637,9 @@ rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out)
     r->tuning.trace_block = 128;
     r->tuning.trace_blocks_per_cu = 8;
     r->tuning.use_graph = 0;
-    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&r->host_cams), sizeof(rt_camera) * 4,
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&r->host_cams), sizeof(rt_camera) * 8,
         hipHostMallocDefault);
-    if (e == hipSuccess) e = r->cams.reserve(4);
+    if (e == hipSuccess) e = r->cams.reserve(8);
     if (e == hipSuccess) e = r->counters.reserve(16);
     if (e != hipSuccess) {
         delete r;
@@ -656,9 +723,19 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
         }
     }
     r->pixel_offset[4] = total;
+    // per-camera history: a resized or moved-in-the-layout camera starts over
+    // (DeviceFrameBufferSet::resize_history, device_frame_buffers.cpp:88-98)
+    for (int c = 0; c < 4; ++c) {
+        if (c >= rig->active_count || r->prev_width[c] != r->widths[c]
+            || r->prev_height[c] != r->heights[c] || r->prev_offset[c] != r->pixel_offset[c]) {
+            r->history_length[c] = 0;
+            r->prev_valid[c] = 0;
+        }
+    }
     RT_HIP(hipStreamSynchronize(st));  // host_cams may still feed a previous copy
     std::memcpy(r->host_cams, rig->cameras, sizeof(rt_camera) * 4);
-    RT_HIP(hipMemcpyAsync(r->cams.p, r->host_cams, sizeof(rt_camera) * 4, hipMemcpyHostToDevice, st));
+    std::memcpy(r->host_cams + 4, r->prev_cams, sizeof(rt_camera) * 4);
+    RT_HIP(hipMemcpyAsync(r->cams.p, r->host_cams, sizeof(rt_camera) * 8, hipMemcpyHostToDevice, st));
 
     const int spp = p->samples_per_pixel > 0 ? p->samples_per_pixel : 1;
     const int max_b = p->max_bounces > 0 ? p->max_bounces : 1;
@@ -681,6 +758,34 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
     fp.jitter = p->pixel_jitter;
     fp.stream = sample_stream;
     fp.total_pixels = total;
+    const int min_lights = p->restir_min_analytic_lights > 0 ? p->restir_min_analytic_lights : 1;
+    rtk::RestirView rv{};
+    rv.enabled = p->enable_restir_di != 0 && s->analytic_count >= min_lights ? 1 : 0;
+    fp.restir = rv.enabled;
+    if (rv.enabled) {
+        if (r->reservoir_pixels < P) {
+            RT_HIP(r->reservoirs[0].reserve(P));
+            RT_HIP(r->reservoirs[1].reserve(P));
+            r->reservoir_pixels = P;
+            for (int c = 0; c < 4; ++c) r->history_length[c] = 0;
+        }
+        rv.initial_candidates = p->restir_initial_candidates;
+        rv.temporal_reuse = p->restir_temporal_reuse;
+        rv.max_history_age = p->restir_max_history_age;
+        rv.max_temporal_candidates = p->restir_max_temporal_candidates;
+        rv.spatial_neighbors = p->restir_spatial_neighbors;
+        rv.max_spatial_candidates = p->restir_max_spatial_candidates;
+        rv.bias_correction = p->restir_bias_correction;
+        rv.spatial_radius = p->restir_spatial_radius;
+        rv.spp = spp;
+        rv.current = r->reservoirs[r->res_cur].p;
+        rv.history = r->restir_last ? r->reservoirs[r->res_cur ^ 1].p : nullptr;
+        rv.prev_cams = r->cams.p + 4;
+        for (int c = 0; c < 4; ++c) {
+            rv.history_length[c] = r->history_length[c];
+            rv.prev_valid[c] = r->prev_valid[c];
+        }
+    }
 
     const rtk::SceneView sv = s->view();
     rtk::WavefrontBuffers b = r->buffers();
@@ -716,6 +821,8 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
             RT_HIP(rtk::wf_extend(sv, sv.stack_depth, b, bounce, r->tuning.count_traversal != 0, trace_l, st));
             RT_HIP(mark());
             RT_HIP(rtk::wf_shade(sv, cv, fp, b, bounce, shade_l, st));
+            if (fp.restir != 0 && bounce == 0)
+                RT_HIP(rtk::wf_restir(sv, sv.stack_depth, cv, fp, rv, b, trace_l, st));
             RT_HIP(mark());
             RT_HIP(rtk::wf_connect(sv, sv.stack_depth, b, bounce, trace_l, st));
             RT_HIP(mark());
@@ -728,6 +835,18 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
     RT_HIP(hipMemcpyAsync(stats, b.counters, sizeof(stats), hipMemcpyDeviceToHost, st));
     RT_HIP(hipStreamSynchronize(st));
     r->rendered = true;
+    // capture_launch_history (radiance_launch_setup.cpp:122-136)
+    for (int c = 0; c < rig->active_count; ++c) {
+        r->prev_cams[c] = rig->cameras[c];
+        r->history_length[c] = r->history_length[c] == 0 ? 1 : r->history_length[c] + 1;
+        r->prev_valid[c] = 1;
+        r->prev_width[c] = r->widths[c];
+        r->prev_height[c] = r->heights[c];
+        r->prev_offset[c] = r->pixel_offset[c];
+    }
+    r->last_reservoirs = rv.enabled ? r->res_cur : -1;
+    if (rv.enabled) r->res_cur ^= 1;
+    r->restir_last = rv.enabled != 0;
     if (timed) {
         std::memset(timing, 0, sizeof(*timing));
         float ms = 0.0f;
@@ -794,21 +913,55 @@ rt_status rt_frame_device_view(rt_renderer* r, int camera, rt_device_frame_view*
     return RT_OK;
 }
 
+// DeviceFrameBufferSet::reset_accumulation (device_frame_buffers.cpp:146-149)
 rt_status rt_reset_accumulation(rt_renderer* r) {
     RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
+    for (int c = 0; c < 4; ++c) {
+        r->history_length[c] = 0;
+        r->prev_valid[c] = 0;
+    }
     return RT_OK;
 }
 
 rt_status rt_reset_sequence(rt_renderer* r, uint32_t seed) {
     RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
     r->sequence_seed = seed;
+    return rt_reset_accumulation(r);
+}
+
+// RestirDiagnostics over the reservoirs of the last frame
+// (OptixRenderer::restir_diagnostics, optix_renderer.cpp:183-215)
+rt_status rt_restir_diagnostics(rt_renderer* r, int camera, rt_restir_stats* out) {
+    RT_REQUIRE(r && out, RT_ERROR_INVALID_ARGUMENT, "null renderer or output");
+    std::memset(out, 0, sizeof(*out));
+    RT_REQUIRE(camera >= 0 && camera < r->camera_count, RT_ERROR_INVALID_ARGUMENT,
+        "camera index " + std::to_string(camera) + " out of range");
+    if (r->last_reservoirs < 0) return RT_OK;
+    RT_HIP(hipSetDevice(r->dev->ordinal));
+    const size_t n = (size_t)r->widths[camera] * (size_t)r->heights[camera];
+    std::vector<rtk::Reservoir> host(n);
+    RT_HIP(hipMemcpyAsync(host.data(), r->reservoirs[r->last_reservoirs].p + r->pixel_offset[camera],
+        n * sizeof(rtk::Reservoir), hipMemcpyDeviceToHost, r->dev->stream));
+    RT_HIP(hipStreamSynchronize(r->dev->stream));
+    for (const rtk::Reservoir& v : host) {
+        out->valid_reservoirs += v.valid != 0 ? 1u : 0u;
+        out->temporal_candidates += (uint64_t)std::max(v.temporal_candidate_count, 0);
+        out->spatial_candidates += (uint64_t)std::max(v.spatial_candidate_count, 0);
+        out->bias_correction_sources += (uint64_t)std::max(v.bias_correction_source_count, 0);
+    }
     return RT_OK;
 }
 
-rt_status rt_restir_diagnostics(rt_renderer* r, int camera, rt_restir_stats* out) {
-    RT_REQUIRE(r && out, RT_ERROR_INVALID_ARGUMENT, "null renderer or output");
-    (void)camera;
-    std::memset(out, 0, sizeof(*out));
+rt_status rt_restir_download(rt_renderer* r, int camera, void* reservoirs) {
+    RT_REQUIRE(r && reservoirs, RT_ERROR_INVALID_ARGUMENT, "null renderer or output");
+    RT_REQUIRE(camera >= 0 && camera < r->camera_count, RT_ERROR_INVALID_ARGUMENT,
+        "camera index " + std::to_string(camera) + " out of range");
+    RT_REQUIRE(r->last_reservoirs >= 0, RT_ERROR_NOT_READY, "the last frame did not run ReSTIR DI");
+    RT_HIP(hipSetDevice(r->dev->ordinal));
+    const size_t n = (size_t)r->widths[camera] * (size_t)r->heights[camera];
+    RT_HIP(hipMemcpyAsync(reservoirs, r->reservoirs[r->last_reservoirs].p + r->pixel_offset[camera],
+        n * sizeof(rtk::Reservoir), hipMemcpyDeviceToHost, r->dev->stream));
+    RT_HIP(hipStreamSynchronize(r->dev->stream));
     return RT_OK;
 }
 
@@ -820,6 +973,14 @@ rt_status rt_scene_generate(int config, uint64_t seed, rt_generated_scene** out)
         return fail(RT_ERROR_INVALID_ARGUMENT, "unknown scene config " + std::to_string(config));
     }
     *out = g;
+    return RT_OK;
+}
+
+rt_status rt_scene_generate_rig(int config, uint64_t seed, int frame, int frames,
+    rt_camera_rig* out) {
+    RT_REQUIRE(out != nullptr, RT_ERROR_INVALID_ARGUMENT, "null output");
+    if (!rth::generate_rig(config, seed, frame, frames, out))
+        return fail(RT_ERROR_INVALID_ARGUMENT, "unknown scene config " + std::to_string(config));
     return RT_OK;
 }
 

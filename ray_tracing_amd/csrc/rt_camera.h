@@ -95,6 +95,59 @@ __device__ inline f3 unproject_equi(const rt_camera& c, double px, double py) {
     return normalize3(mk3((float)(xr.x * scale), (float)(xr.y * scale), 1.0f));
 }
 
+// project_pinhole32 / project_equi62_lut1d (programs.cu:375-473), camera-space
+// direction -> pixel
+__device__ inline void project_camera(const rt_camera& c, f3 dc, float* px, float* py) {
+    if (c.model == RT_CAM_EQUI62_LUT1D) {
+        const double dx = dc.x, dy = dc.y, dz = dc.z;
+        const double r = sqrt(dx * dx + dy * dy);
+        if (r < 1e-12) {
+            *px = (float)c.cx;
+            *py = (float)c.cy;
+            return;
+        }
+        const double theta = atan2(r, dz);
+        double rd;
+        if (theta <= 0.0 || c.lut_step <= 0.0) {
+            rd = 0.0;
+        } else if (theta >= c.lut[1023]) {
+            rd = 1023.0 * c.lut_step;
+        } else {
+            // first LUT interval containing theta (the LUT is non-decreasing)
+            int lo = 0, hi = 1022;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (c.lut[mid + 1] < theta) lo = mid + 1;
+                else hi = mid;
+            }
+            while (lo > 0 && c.lut[lo] > theta) --lo;
+            rd = 1023.0 * c.lut_step;
+            for (int i = lo; i < 1023; ++i) {
+                if (c.lut[i] <= theta && theta <= c.lut[i + 1]) {
+                    const double den = c.lut[i + 1] - c.lut[i];
+                    const double a = den > 0.0 ? (theta - c.lut[i]) / den : 0.0;
+                    rd = ((double)i + a) * c.lut_step;
+                    break;
+                }
+            }
+        }
+        const d2 xyr{dx / r * rd, dy / r * rd};
+        const d2 xyd = equi_tangential(xyr, c.tangential);
+        *px = (float)(xyd.x * c.fx + c.cx);
+        *py = (float)(xyd.y * c.fy + c.cy);
+        return;
+    }
+    if (fabsf(dc.z) < 1e-7f) {
+        *px = -1.0f;
+        *py = -1.0f;
+        return;
+    }
+    const double inv_z = 1.0 / (double)dc.z;
+    const d2 xyd = distort_pinhole(c, d2{(double)dc.x * inv_z, (double)dc.y * inv_z});
+    *px = (float)(xyd.x * c.fx + c.cx);
+    *py = (float)(xyd.y * c.fy + c.cy);
+}
+
 __device__ __forceinline__ f3 camera_transform(const rt_camera& c, f3 dc) {
     const f3 bx = mk3((float)c.basis_x[0], (float)c.basis_x[1], (float)c.basis_x[2]);
     const f3 by = mk3((float)c.basis_y[0], (float)c.basis_y[1], (float)c.basis_y[2]);

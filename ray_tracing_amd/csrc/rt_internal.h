@@ -51,6 +51,39 @@ struct SceneView {
     const rt_analytic_light* analytic;
     int32_t analytic_count;
     int32_t stack_depth;  // LDS traversal stack entries per lane (>= BVH depth)
+    const float* a_sel;   // analytic light selection pdf
+    const float* a_cdf;   // analytic light CDF
+};
+
+// RestirReservoir (src/common/restir_di.h:36-48): 84 B per pixel
+struct Reservoir {
+    int32_t light_index;
+    float sample_u0, sample_u1;
+    float position[3];
+    float normal[3];
+    int32_t material_type, primitive_type, primitive_index;
+    float weight_sum, selected_target, estimator_weight;
+    int32_t candidate_count, temporal_candidate_count, spatial_candidate_count,
+        bias_correction_source_count, age, valid;
+};
+
+// per-frame ReSTIR DI parameters and per-camera history
+struct RestirView {
+    int32_t enabled;          // profile + light-count rule of radiance_launch_setup.cpp:101-106
+    int32_t initial_candidates;
+    int32_t temporal_reuse;
+    int32_t max_history_age;
+    int32_t max_temporal_candidates;
+    int32_t spatial_neighbors;
+    int32_t max_spatial_candidates;
+    int32_t bias_correction;
+    int32_t spatial_radius;
+    int32_t spp;
+    Reservoir* current;             // all cameras, indexed by global pixel
+    const Reservoir* history;       // previous frame, same indexing
+    const rt_camera* prev_cams;     // previous camera per slot (device)
+    int32_t history_length[4];
+    int32_t prev_valid[4];
 };
 
 // per-camera constants for ray generation (double camera math as the

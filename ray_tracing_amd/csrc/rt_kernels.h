@@ -44,6 +44,15 @@ struct WavefrontBuffers {
     float4* shadow_o;   // origin.xyz, max_t
     float4* shadow_d;   // direction.xyz, bits(path slot)
     float4* shadow_c;   // contribution.xyz
+    // second shadow queue (analytic lights / ReSTIR), traced after the first
+    float4* shadow2_o;
+    float4* shadow2_d;
+    float4* shadow2_c;
+    // bounce-0 paths deferred to the ReSTIR stage
+    uint32_t* restir_queue;
+    float4* restir_o;   // incoming ray origin
+    float4* restir_d;   // incoming ray direction, w = hit t
+    float4* restir_thr; // throughput before BSDF sampling, w = bits(prim)
     uint32_t* counters; // [0]/[1] queue counts, [2] shadow count, [4..15] u64 ray / traversal stats
     // per pixel (all cameras)
     float4* acc_beauty;
@@ -60,6 +69,7 @@ struct FrameParams {
     int32_t total_pixels;
     int32_t layer_count;    // samples per pixel in this chunk
     int32_t first_sample;   // sample index of the chunk's first layer
+    int32_t restir;         // bounce-0 analytic lighting goes through the ReSTIR stage
 };
 
 struct StageLaunch {
@@ -76,6 +86,9 @@ hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParam
     WavefrontBuffers& b, int bounce, StageLaunch l, hipStream_t st);
 hipError_t wf_connect(const SceneView& s, int stack_depth, WavefrontBuffers& b, int bounce,
     StageLaunch l, hipStream_t st);
+hipError_t wf_restir(const SceneView& s, int stack_depth, const CameraView& cams,
+    const FrameParams& fp, const RestirView& rv, WavefrontBuffers& b, StageLaunch l,
+    hipStream_t st);
 hipError_t wf_accumulate(const FrameParams& fp, WavefrontBuffers& b, hipStream_t st);
 hipError_t wf_clear_accum(int pixels, WavefrontBuffers& b, hipStream_t st);
 hipError_t wf_finalize(int pixels, int spp, const WavefrontBuffers& b, float4* beauty,

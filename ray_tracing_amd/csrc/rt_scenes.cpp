@@ -281,7 +281,214 @@ void generate_cornell_mesh(uint64_t seed, rt_generated_scene* g) {
     g->profile = default_profile(16, 8);
 }
 
+double gauss(Pcg32& r, double sigma) {
+    const double u1 = 1.0 - static_cast<double>(r.uniform());
+    const double u2 = static_cast<double>(r.uniform());
+    return sigma * std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * 3.14159265358979323846 * u2);
+}
+
+// four cameras on a radius-70 orbit around the cube centre, 90 degrees apart:
+// two pinhole32 640x480 (vfov 60) and two equi62_lut1d 1024x1024 with
+// distortion k_i ~ N(0, 0.01) redrawn until theta_d is monotone on [0, 1.6]
+void instanced_rig(uint64_t seed, int frame, int frames, rt_camera* cams) {
+    Pcg32 r(seed ^ 0x5bd1e995ULL);
+    double radial[2][6];
+    for (int k = 0; k < 2; ++k) {
+        for (int tries = 0; tries < 10000; ++tries) {
+            for (int i = 0; i < 6; ++i) radial[k][i] = gauss(r, 0.01);
+            bool ok = true;
+            for (int s = 0; s <= 64 && ok; ++s) {
+                const double th = 1.6 * static_cast<double>(s) / 64.0, t2 = th * th;
+                double der = 1.0, tp = t2;
+                for (int i = 0; i < 6; ++i) {
+                    der += static_cast<double>(2 * i + 3) * radial[k][i] * tp;
+                    tp *= t2;
+                }
+                ok = der >= 0.2;
+            }
+            if (ok) break;
+        }
+    }
+    const std::array<double, 3> center = {50.0, 50.0, 50.0};
+    const std::array<double, 3> up = {0.0, 1.0, 0.0};
+    for (int k = 0; k < 4; ++k) {
+        const double th = 2.0 * 3.14159265358979323846 * static_cast<double>(frame)
+                              / static_cast<double>(frames > 0 ? frames : 1)
+                          + static_cast<double>(k) * 0.5 * 3.14159265358979323846;
+        const std::array<double, 3> from = {center[0] + 70.0 * std::cos(th), center[1] + 15.0,
+            center[2] + 70.0 * std::sin(th)};
+        if (k < 2) {
+            make_lookat_pinhole(&cams[k], 640, 480, 60.0, from, center, up);
+        } else {
+            make_lookat_equi(&cams[k], 1024, 1024, radial[k - 2], from, center, up);
+        }
+    }
+}
+
+void generate_instanced_mesh(uint64_t seed, rt_generated_scene* g) {
+    Pcg32 r(seed);
+    constexpr int NU = 32, NV = 16, NINST = 1000, NMAT = 16, NLIGHT = 256;
+    constexpr double kPiD = 3.14159265358979323846;
+    // torus, 32 x 16 segments = 1024 triangles, tube radius jittered per vertex
+    std::vector<std::array<double, 3>> verts(NU * NV);
+    for (int i = 0; i < NU; ++i) {
+        for (int j = 0; j < NV; ++j) {
+            const double u = 2.0 * kPiD * static_cast<double>(i) / NU;
+            const double v = 2.0 * kPiD * static_cast<double>(j) / NV;
+            const double tube = 0.35 * (1.0 + 0.1 * (2.0 * static_cast<double>(r.uniform()) - 1.0));
+            verts[i * NV + j] = {(1.0 + tube * std::cos(v)) * std::cos(u), tube * std::sin(v),
+                (1.0 + tube * std::cos(v)) * std::sin(u)};
+        }
+    }
+    std::vector<std::array<int, 3>> faces;
+    for (int i = 0; i < NU; ++i) {
+        for (int j = 0; j < NV; ++j) {
+            const int a = i * NV + j, b = ((i + 1) % NU) * NV + j;
+            const int c = ((i + 1) % NU) * NV + (j + 1) % NV, d = i * NV + (j + 1) % NV;
+            faces.push_back({a, b, c});
+            faces.push_back({a, c, d});
+        }
+    }
+    std::vector<rt_material> mt(NMAT);
+    for (auto& m : mt) {
+        m = rt_material{};
+        m.type = RT_MAT_LAMBERTIAN;
+        m.ior = 1.0f;
+        m.openpbr_index = -1;
+        for (int ch = 0; ch < 3; ++ch) m.albedo[ch] = 0.2f + 0.7f * r.uniform();
+    }
+    std::vector<rt_triangle> tr((size_t)NINST * faces.size());
+    for (int k = 0; k < NINST; ++k) {
+        // uniform rotation from a random unit quaternion (Shoemake 1992)
+        const double u1 = r.uniform(), u2 = r.uniform(), u3 = r.uniform();
+        const double s1 = std::sqrt(1.0 - u1), s2 = std::sqrt(u1);
+        const double qx = s1 * std::sin(2.0 * kPiD * u2);
+        const double qy = s1 * std::cos(2.0 * kPiD * u2);
+        const double qz = s2 * std::sin(2.0 * kPiD * u3);
+        const double qw = s2 * std::cos(2.0 * kPiD * u3);
+        const double R[3][3] = {
+            {1.0 - 2.0 * (qy * qy + qz * qz), 2.0 * (qx * qy - qz * qw), 2.0 * (qx * qz + qy * qw)},
+            {2.0 * (qx * qy + qz * qw), 1.0 - 2.0 * (qx * qx + qz * qz), 2.0 * (qy * qz - qx * qw)},
+            {2.0 * (qx * qz - qy * qw), 2.0 * (qy * qz + qx * qw), 1.0 - 2.0 * (qx * qx + qy * qy)}};
+        const double t[3] = {100.0 * static_cast<double>(r.uniform()),
+            100.0 * static_cast<double>(r.uniform()), 100.0 * static_cast<double>(r.uniform())};
+        const int mat = static_cast<int>(r.uniform() * static_cast<float>(NMAT)) % NMAT;
+        const double scale = 2.0;
+        for (size_t f = 0; f < faces.size(); ++f) {
+            rt_triangle& o = tr[(size_t)k * faces.size() + f];
+            float* dst[3] = {o.p0, o.p1, o.p2};
+            for (int vi = 0; vi < 3; ++vi) {
+                const auto& p = verts[faces[f][vi]];
+                for (int a = 0; a < 3; ++a)
+                    dst[vi][a] = static_cast<float>(
+                        scale * (R[a][0] * p[0] + R[a][1] * p[1] + R[a][2] * p[2]) + t[a]);
+            }
+            o.material = mat;
+        }
+    }
+    std::vector<rt_analytic_light> al(NLIGHT);
+    for (auto& L : al) {
+        L = rt_analytic_light{};
+        L.type = RT_ALIGHT_SPHERE;
+        for (int a = 0; a < 3; ++a) L.position[a] = 100.0f * r.uniform();
+        L.radius = 0.1f + 0.4f * r.uniform();
+        const float rad = 1.0f + 19.0f * r.uniform();
+        set3(L.radiance, rad, rad, rad);
+    }
+    g->scene.triangle_count = static_cast<int32_t>(tr.size());
+    g->scene.triangles = copy_out(tr);
+    g->scene.material_count = NMAT;
+    g->scene.materials = copy_out(mt);
+    g->scene.analytic_light_count = NLIGHT;
+    g->scene.analytic_lights = copy_out(al);
+    g->scene.background_mode = RT_BG_CONSTANT;
+    set3(g->scene.background, 0.0f, 0.0f, 0.0f);
+    g->camera_count = 4;
+    instanced_rig(seed, 0, 60, g->cameras);
+    g->profile = default_profile(1, 2);
+    g->profile.enable_restir_di = 1;
+    g->profile.restir_initial_candidates = 32;
+    g->profile.restir_spatial_neighbors = 5;
+    g->profile.restir_spatial_radius = 30;
+}
+
 }  // namespace
+
+// equi62 theta distortion and its Newton inverse (camera_models.cpp:64-100)
+static double equi_theta_dist(const double* radial, double theta) {
+    const double t2 = theta * theta;
+    double scale = 1.0, tp = t2;
+    for (int i = 0; i < 6; ++i) {
+        scale += radial[i] * tp;
+        tp *= t2;
+    }
+    return theta * scale;
+}
+
+static double equi_theta_deriv(const double* radial, double theta) {
+    const double t2 = theta * theta;
+    double der = 1.0, tp = t2;
+    for (int i = 0; i < 6; ++i) {
+        der += static_cast<double>(2 * i + 3) * radial[i] * tp;
+        tp *= t2;
+    }
+    return der;
+}
+
+static double equi_invert(const double* radial, double thetad) {
+    double theta = std::cbrt(thetad > 0.0 ? thetad : 0.0);
+    for (int it = 0; it < 30; ++it) {
+        const double err = thetad - equi_theta_dist(radial, theta);
+        if (std::fabs(err) < 1e-12) break;
+        const double der = equi_theta_deriv(radial, theta);
+        if (std::fabs(der) < 1e-12) break;
+        const double delta = err / der;
+        theta += delta;
+        if (std::fabs(delta) < 1e-12 || std::fabs(delta) > 3.1416) break;
+    }
+    return theta;
+}
+
+// make_equi62_lut1d_params (camera_models.cpp:228-248)
+void make_equi_lut(rt_camera* c) {
+    const double x = (static_cast<double>(c->width) + 10.0 - c->cx) / c->fx;
+    const double y = (static_cast<double>(c->height) + 10.0 - c->cy) / c->fy;
+    c->lut_step = std::sqrt(x * x + y * y) / 1024.0;
+    for (int i = 0; i < 1024; ++i) c->lut[i] = equi_invert(c->radial, c->lut_step * static_cast<double>(i));
+}
+
+// equi62_lut1d look-at camera with the default 120 degree horizontal field of
+// view (derive_default_camera_intrinsics, camera_models.cpp:196-226)
+void make_lookat_equi(rt_camera* c, int w, int h, const double radial[6],
+    std::array<double, 3> from, std::array<double, 3> at, std::array<double, 3> up) {
+    make_lookat_pinhole(c, w, h, 60.0, from, at, up);
+    c->model = RT_CAM_EQUI62_LUT1D;
+    const double hfov = 120.0 * 3.14159265358979323846 / 180.0;
+    const double f = 0.5 * static_cast<double>(w) / (0.5 * hfov);
+    c->fx = f;
+    c->fy = f;
+    c->cx = 0.5 * static_cast<double>(w);
+    c->cy = 0.5 * static_cast<double>(h);
+    for (int i = 0; i < 6; ++i) c->radial[i] = radial[i];
+    c->tangential[0] = 0.0;
+    c->tangential[1] = 0.0;
+    make_equi_lut(c);
+}
+
+bool generate_rig(int config, uint64_t seed, int frame, int frames, rt_camera_rig* out) {
+    std::memset(out, 0, sizeof(*out));
+    if (config == 2) {
+        instanced_rig(seed, frame, frames, out->cameras);
+        out->active_count = 4;
+        return true;
+    }
+    rt_generated_scene g;
+    if (!generate_scene(config, seed, &g)) return false;
+    out->active_count = g.camera_count;
+    for (int c = 0; c < g.camera_count; ++c) out->cameras[c] = g.cameras[c];
+    free_generated(&g);
+    return true;
+}
 
 rt_openpbr_material openpbr_defaults() {
     rt_openpbr_material m{};
@@ -365,6 +572,7 @@ bool generate_scene(int config, uint64_t seed, rt_generated_scene* g) {
     switch (config) {
         case 0: generate_random_spheres(seed, g); return true;
         case 1: generate_cornell_mesh(seed, g); return true;
+        case 2: generate_instanced_mesh(seed, g); return true;
         default: return false;
     }
 }

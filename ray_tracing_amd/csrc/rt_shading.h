@@ -318,6 +318,96 @@ __device__ inline f3 direct_response(const SceneView& s, const HitInfo& h, f3 ra
     return value * cs;
 }
 
+// ---- analytic sphere lights
+// try_intersect_analytic_light, sphere branch (programs.cu:679-709)
+__device__ __forceinline__ bool alight_hit(const rt_analytic_light& L, f3 o, f3 d, float tmin,
+    float tmax, float* t, f3* pos, f3* n) {
+    const f3 center = ld3(L.position);
+    const float radius = L.radius;
+    const f3 oc = o - center;
+    const float a = lensq3(d);
+    const float hb = dot3(oc, d);
+    const float c = lensq3(oc) - radius * radius;
+    const float disc = hb * hb - a * c;
+    if (a <= 1e-16f || disc < 0.0f) return false;
+    const float rt = sqrtf(disc);
+    float root = (-hb - rt) / a;
+    if (root <= tmin || root >= tmax) root = (-hb + rt) / a;
+    if (root <= tmin || root >= tmax) return false;
+    const f3 p = o + d * root;
+    const f3 nn = normalize3(p - center);
+    if (dot3(d, nn) >= 0.0f) return false;
+    *t = root;
+    *pos = p;
+    *n = nn;
+    return true;
+}
+
+// intersect_analytic_lights (programs.cu:795-813): nearest front-facing hit,
+// lowest index on equal distance
+__device__ inline int intersect_analytic(const SceneView& s, f3 o, f3 d, float tmin, float tmax) {
+    float closest = tmax;
+    int found = -1;
+    for (int i = 0; i < s.analytic_count; ++i) {
+        float t;
+        f3 p, n;
+        if (!alight_hit(s.analytic[i], o, d, tmin, closest, &t, &p, &n)) continue;
+        closest = t;
+        found = i;
+    }
+    return found;
+}
+
+// analytic_emission_mis_weight (programs.cu:838-849)
+__device__ __forceinline__ float analytic_mis(const SceneView& s, const PathPrev& pv, int li) {
+    if (!pv.valid || pv.delta || li < 0) return 1.0f;
+    const rt_analytic_light L = s.analytic[li];
+    const float dsq = lensq3(ld3(L.position) - pv.pos);
+    const float rsq = L.radius * L.radius;
+    const float cond = dsq <= rsq ? 0.0f : cone_pdf(sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq)));
+    return power_heuristic(pv.pdf, s.a_sel[li] * cond);
+}
+
+// sample_packed_analytic_light (light_sampling.h:104-116) by bisection
+__device__ __forceinline__ int pick_analytic(const SceneView& s, float u) {
+    if (s.analytic_count <= 0) return -1;
+    const float x = u < 0.0f ? 0.0f : (u < 1.0f ? u : 0.99999994f);
+    int lo = 0, hi = s.analytic_count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (x < s.a_cdf[mid]) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// sample_analytic_direct_light_candidate, sphere branch (programs.cu:1234-1300)
+__device__ inline LightSample analytic_candidate(const SceneView& s, f3 sp, int li, float u0,
+    float u1) {
+    LightSample ls{};
+    ls.valid = false;
+    ls.infinite = false;
+    if (li < 0 || li >= s.analytic_count) return ls;
+    if (s.a_sel[li] <= 0.0f) return ls;
+    const rt_analytic_light L = s.analytic[li];
+    const f3 center = ld3(L.position);
+    const f3 tc = center - sp;
+    const float dsq = lensq3(tc);
+    const float rsq = L.radius * L.radius;
+    if (dsq <= rsq * (1.0f + 1e-6f)) return ls;
+    const float cmax = sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq));
+    const f3 dir = cone_dir(normalize3(tc), cmax, u0, u1);
+    float t;
+    f3 p, n;
+    if (!alight_hit(L, sp, dir, kRayEps, kRayFar, &t, &p, &n)) return ls;
+    ls.dir = dir;
+    ls.emission = ld3(L.radiance);
+    ls.dist = t;
+    ls.pdf = s.a_sel[li] * cone_pdf(cmax);
+    ls.valid = ls.pdf > 0.0f && maxc3(ls.emission) > 0.0f;
+    return ls;
+}
+
 __device__ __forceinline__ float schlick(float cosine, float ri) {
     float r0 = (1.0f - ri) / (1.0f + ri);
     r0 *= r0;

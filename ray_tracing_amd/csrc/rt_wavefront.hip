@@ -22,6 +22,7 @@
 // order in which the stages visit it.
 #include "rt_camera.h"
 #include "rt_kernels.h"
+#include "rt_restir.h"
 #include "rt_shading.h"
 #include "rt_traverse.h"
 
@@ -29,8 +30,9 @@ namespace rtk {
 
 namespace {
 
-enum : uint32_t { kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntStatExtend = 4,
-    kCntStatShadow = 6, kCntVisits = 8, kCntTests = 10, kCntSphereTests = 12, kCounterWords = 16 };
+enum : uint32_t { kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntShadow2 = 3,
+    kCntStatExtend = 4, kCntStatShadow = 6, kCntVisits = 8, kCntTests = 10, kCntSphereTests = 12,
+    kCntRestir = 14, kCounterWords = 16 };
 
 enum : int32_t { kPrevValid = 1, kPrevDelta = 2 };
 
@@ -88,6 +90,8 @@ __global__ __launch_bounds__(256) void k_raygen(CameraView cams, FrameParams fp,
         b.counters[kCntQueue0] = (uint32_t)paths;
         b.counters[kCntQueue1] = 0u;
         b.counters[kCntShadow] = 0u;
+        b.counters[kCntShadow2] = 0u;
+        b.counters[kCntRestir] = 0u;
     }
     if (i >= paths) return;
     const SlotId id = slot_id(cams, fp, (uint32_t)i);
@@ -136,7 +140,13 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneView s, WavefrontBuffers 
         TraverseCounts cnt;
         const TraceResult r = traverse<false, COUNT>(s, o, d, kRayEps, kRayFar, stack_node, stack_t,
             BLOCK, &cnt);
-        b.hit[slot] = make_float4(r.t, __int_as_float(r.found ? (int)r.prim : -1), r.u, r.v);
+        // analytic lights are not in the BVH: nearest front-facing one before
+        // the surface hit (programs.cu:1861-1862)
+        const int ali = s.analytic_count > 0
+                            ? intersect_analytic(s, o, d, kRayEps, r.found ? r.t : kRayFar)
+                            : -1;
+        b.hit[slot] = make_float4(r.t, __int_as_float(r.found ? (int)r.prim : -1),
+            __int_as_float(ali), r.u);
         if (COUNT) {
             atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntVisits),
                 (unsigned long long)cnt.visits);
@@ -157,10 +167,10 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
     const uint32_t nq = (count + stride - 1u) / stride * stride;  // uniform trip count per wave
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
         const bool active = q < count;
-        bool shadow = false;
-        bool survive = false;
+        bool shadow = false, shadow2 = false, survive = false, defer = false;
         uint32_t slot = 0u;
-        float4 sh_o = make_float4(0, 0, 0, 0), sh_d = make_float4(0, 0, 0, 0), sh_c = sh_o;
+        float4 sh_o = make_float4(0, 0, 0, 0), sh_d = sh_o, sh_c = sh_o;
+        float4 s2_o = sh_o, s2_d = sh_o, s2_c = sh_o;
         if (active) {
             slot = queue[q];
             const SlotId id = slot_id(cams, fp, slot);
@@ -181,7 +191,12 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
             pv.n = xyz(pn);
             pv.prim = __float_as_int(pn.w);
             const int prim = __float_as_int(hv.y);
-            if (prim < 0) {
+            const int ali = __float_as_int(hv.z);
+            if (ali >= 0) {
+                // path ends on an analytic light (programs.cu:1863-1872)
+                const float w = analytic_mis(s, pv, ali);
+                rad = rad + thr * (ld3(s.analytic[ali].radiance) * w);
+            } else if (prim < 0) {
                 const float w = emission_mis(s, pv, rd, nullptr, true);
                 rad = rad + thr * (background_at(s, rd) * w);
             } else {
@@ -198,6 +213,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                 const float ew = emission_mis(s, pv, rd, &h, false);
                 rad = rad + thr * (h.emission * ew);
                 const bool bsdf_avail = bounce + 1 < fp.max_bounces;
+                // next-event estimation toward emissive geometry (programs.cu:2249-2280)
                 if (h.mat_type != RT_MAT_DIFFUSE_LIGHT && s.light_count > 0) {
                     const LightSample ls = sample_light(s, h, rl);
                     if (ls.valid) {
@@ -209,11 +225,40 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                             if (max_t > kRayEps) {
                                 const float mis = bsdf_avail ? power_heuristic(ls.pdf, bpdf) : 1.0f;
                                 const f3 direct = (ls.emission * resp) * (mis / ls.pdf);
-                                const f3 contrib = thr * direct;
                                 shadow = true;
                                 sh_o = f4(so, max_t);
                                 sh_d = f4(ls.dir, __uint_as_float(slot));
-                                sh_c = f4(contrib, 0.0f);
+                                sh_c = f4(thr * direct, 0.0f);
+                            }
+                        }
+                    }
+                }
+                // analytic lights: ReSTIR stage at the first vertex, otherwise one
+                // light sample (programs.cu:2282-2313)
+                if (fp.restir != 0 && bounce == 0) {
+                    if (h.mat_type != RT_MAT_DIFFUSE_LIGHT && s.analytic_count > 0) {
+                        defer = true;
+                        b.restir_o[slot] = f4(ro, 0.0f);
+                        b.restir_d[slot] = f4(rd, hv.x);
+                        b.restir_thr[slot] = f4(thr, __int_as_float(prim));
+                    }
+                } else if (h.mat_type != RT_MAT_DIFFUSE_LIGHT && s.analytic_count > 0) {
+                    const float4 ra = rng4(id.pixel, id.sample, id.camera, (uint32_t)bounce,
+                        kGroupAnalytic, fp.stream);
+                    const int li = pick_analytic(s, ra.x);
+                    const LightSample ls = analytic_candidate(s, h.pos, li, ra.y, ra.z);
+                    if (ls.valid) {
+                        float bpdf = 0.0f;
+                        const f3 resp = direct_response(s, h, rd, ls.dir, &bpdf);
+                        if (maxc3(resp) > 0.0f) {
+                            const float max_t = ls.dist - kRayEps * 3.0f;
+                            if (max_t > kRayEps) {
+                                const float mis = !bsdf_avail ? 1.0f : power_heuristic(ls.pdf, bpdf);
+                                const f3 direct = (ls.emission * resp) * (mis / ls.pdf);
+                                shadow2 = true;
+                                s2_o = f4(h.pos + ls.dir * (kRayEps * 2.0f), max_t);
+                                s2_d = f4(ls.dir, __uint_as_float(slot));
+                                s2_c = f4(thr * direct, 0.0f);
                             }
                         }
                     }
@@ -249,15 +294,75 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
             b.shadow_d[spos] = sh_d;
             b.shadow_c[spos] = sh_c;
         }
+        const uint32_t s2pos = wave_append(shadow2, &b.counters[kCntShadow2]);
+        if (shadow2) {
+            b.shadow2_o[s2pos] = s2_o;
+            b.shadow2_d[s2pos] = s2_d;
+            b.shadow2_c[s2pos] = s2_c;
+        }
+        const uint32_t rpos = wave_append(defer, &b.counters[kCntRestir]);
+        if (defer) b.restir_queue[rpos] = slot;
         const uint32_t npos = wave_append(survive, &b.counters[cur ^ 1]);
         if (survive) next_queue[npos] = slot;
     }
 }
 
+// ReSTIR DI at the first vertex for the deferred paths; its shadow ray joins
+// the second shadow queue
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_connect(SceneView s, WavefrontBuffers b) {
+__global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, FrameParams fp,
+    RestirView rv, WavefrontBuffers b) {
     extern __shared__ uint32_t lds[];
-    const uint32_t count = b.counters[kCntShadow];
+    const uint32_t count = b.counters[kCntRestir];
+    uint32_t* stack_node = lds + threadIdx.x;
+    float* stack_t = reinterpret_cast<float*>(lds + BLOCK * s.stack_depth) + threadIdx.x;
+    const uint32_t stride = gridDim.x * BLOCK;
+    const uint32_t nq = (count + stride - 1u) / stride * stride;
+    for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < nq; q += stride) {
+        const bool active = q < count;
+        bool shadow = false;
+        float4 s_o = make_float4(0, 0, 0, 0), s_d = s_o, s_c = s_o;
+        if (active) {
+            const uint32_t slot = b.restir_queue[q];
+            const SlotId id = slot_id(cams, fp, slot);
+            const float4 o4 = b.restir_o[slot];
+            const float4 d4 = b.restir_d[slot];
+            const float4 t4 = b.restir_thr[slot];
+            HitInfo h;
+            make_hit(s, xyz(o4), xyz(d4), d4.w, __float_as_int(t4.w), h);
+            const int w = cams.width[id.camera];
+            const int hgt = (cams.pixel_offset[id.camera + 1] - cams.pixel_offset[id.camera]) / w;
+            const int y = (int)id.pixel / w;
+            const int x = (int)id.pixel - y * w;
+            Reservoir* out = (int)id.sample == rv.spp - 1 && rv.current != nullptr
+                                 ? rv.current + id.global_pixel
+                                 : nullptr;
+            const bool bsdf_avail = 1 < fp.max_bounces;
+            const RestirOut ro = restir_shade(s, rv, h, xyz(d4), (int)id.camera, x, y, w, hgt,
+                id.pixel, id.sample, fp.stream, bsdf_avail, out, stack_node, stack_t, BLOCK);
+            if (ro.shadow) {
+                shadow = true;
+                s_o = f4(ro.origin, ro.max_t);
+                s_d = f4(ro.dir, __uint_as_float(slot));
+                s_c = f4(xyz(t4) * (ro.numerator * ro.W), 0.0f);
+            }
+        }
+        const uint32_t spos = wave_append(shadow, &b.counters[kCntShadow2]);
+        if (shadow) {
+            b.shadow2_o[spos] = s_o;
+            b.shadow2_d[spos] = s_d;
+            b.shadow2_c[spos] = s_c;
+        }
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_connect(SceneView s, WavefrontBuffers b, int which) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t count = b.counters[which == 0 ? kCntShadow : kCntShadow2];
+    const float4* so = which == 0 ? b.shadow_o : b.shadow2_o;
+    const float4* sd = which == 0 ? b.shadow_d : b.shadow2_d;
+    const float4* sc = which == 0 ? b.shadow_c : b.shadow2_c;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntStatShadow),
             (unsigned long long)count);
@@ -266,13 +371,13 @@ __global__ __launch_bounds__(BLOCK) void k_connect(SceneView s, WavefrontBuffers
     float* stack_t = reinterpret_cast<float*>(lds + BLOCK * s.stack_depth) + threadIdx.x;
     const uint32_t stride = gridDim.x * BLOCK;
     for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < count; q += stride) {
-        const float4 o = b.shadow_o[q];
-        const float4 d = b.shadow_d[q];
+        const float4 o = so[q];
+        const float4 d = sd[q];
         const TraceResult r = traverse<true>(s, xyz(o), xyz(d), kRayEps, o.w, stack_node, stack_t,
             BLOCK);
         if (!r.found) {
             const uint32_t slot = __float_as_uint(d.w);
-            const float4 c = b.shadow_c[q];
+            const float4 c = sc[q];
             const float4 rad = b.radiance[slot];
             b.radiance[slot] = f4(xyz(rad) + xyz(c), 0.0f);
         }
@@ -280,10 +385,12 @@ __global__ __launch_bounds__(BLOCK) void k_connect(SceneView s, WavefrontBuffers
 }
 
 __global__ void k_swap_queues(uint32_t* counters, int cur) {
-    // the consumed queue becomes the next producer target; shadow queue empties
+    // the consumed queue becomes the next producer target; shadow queues empty
     if (threadIdx.x == 0) {
         counters[cur] = 0u;
         counters[kCntShadow] = 0u;
+        counters[kCntShadow2] = 0u;
+        counters[kCntRestir] = 0u;
     }
 }
 
@@ -377,18 +484,31 @@ hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParam
     return hipGetLastError();
 }
 
+template <int BLOCK>
+static void launch_connect(const SceneView& sv, int stack_depth, WavefrontBuffers& b, int which,
+    StageLaunch l, hipStream_t st) {
+    k_connect<BLOCK><<<l.grid, BLOCK, (size_t)stack_depth * BLOCK * 8, st>>>(sv, b, which);
+}
+
 hipError_t wf_connect(const SceneView& s, int stack_depth, WavefrontBuffers& b, int bounce,
     StageLaunch l, hipStream_t st) {
     SceneView sv = s;
     sv.stack_depth = stack_depth;
-    if (l.block == 64) {
-        k_connect<64><<<l.grid, 64, (size_t)stack_depth * 64 * 8, st>>>(sv, b);
-    } else if (l.block == 256) {
-        k_connect<256><<<l.grid, 256, (size_t)stack_depth * 256 * 8, st>>>(sv, b);
-    } else {
-        k_connect<128><<<l.grid, 128, (size_t)stack_depth * 128 * 8, st>>>(sv, b);
+    for (int which = 0; which < 2; ++which) {
+        if (l.block == 64) launch_connect<64>(sv, stack_depth, b, which, l, st);
+        else if (l.block == 256) launch_connect<256>(sv, stack_depth, b, which, l, st);
+        else launch_connect<128>(sv, stack_depth, b, which, l, st);
     }
     k_swap_queues<<<1, 64, 0, st>>>(b.counters, bounce & 1);
+    return hipGetLastError();
+}
+
+hipError_t wf_restir(const SceneView& s, int stack_depth, const CameraView& cams,
+    const FrameParams& fp, const RestirView& rv, WavefrontBuffers& b, StageLaunch l,
+    hipStream_t st) {
+    SceneView sv = s;
+    sv.stack_depth = stack_depth;
+    k_restir<128><<<l.grid, 128, (size_t)stack_depth * 128 * 8, st>>>(sv, cams, fp, rv, b);
     return hipGetLastError();
 }
 

@@ -46,7 +46,7 @@ def test_invalid_arguments_report_errors(rt):
     assert b"unknown scene config" in rt.lib().rt_last_error()
 
 
-@pytest.mark.parametrize("config", [0, 1])
+@pytest.mark.parametrize("config", [0, 1, 2])
 def test_generators_match_oracle(rt, oracle, config):
     g = rt.Generated(config, 0)
     og = oracle.GeneratedScene(config, 0)
@@ -56,17 +56,30 @@ def test_generators_match_oracle(rt, oracle, config):
                               ("quads", oracle.Quad, "quad_count"),
                               ("triangles", oracle.Triangle, "triangle_count"),
                               ("materials", oracle.Material, "material_count"),
-                              ("openpbr", oracle.OpenPbr, "openpbr_count")):
+                              ("openpbr", oracle.OpenPbr, "openpbr_count"),
+                              ("analytic", oracle.AnalyticLight, "analytic_light_count")):
         n = getattr(od, cnt)
         assert getattr(d, cnt) == n, field
         if n:
-            ref = C.string_at(getattr(od, field), n * C.sizeof(ctype))
+            src = "analytic_lights" if field == "analytic" else field
+            ref = C.string_at(getattr(od, src), n * C.sizeof(ctype))
             assert arr[field].tobytes() == ref, field
     assert g.g.camera_count == og.g.camera_count
     for i in range(g.g.camera_count):
         assert bytes(g.g.cameras[i]) == bytes(og.g.cameras[i])
     assert bytes(g.g.profile) == bytes(og.g.profile)
     assert tuple(d.background) == tuple(od.background)
+
+
+@pytest.mark.parametrize("frame", [0, 7, 59])
+def test_orbit_rigs_match_oracle(rt, oracle, frame):
+    g = rt.Generated("instanced_mesh_restir", 0)
+    rig = g.rig_at(frame, 60)
+    cams = oracle.generate_rig(2, 0, frame, 60)
+    assert rig.active_count == 4
+    for i in range(4):
+        assert bytes(rig.cameras[i]) == bytes(cams[i])
+    assert rig.cameras[2].model == 1 and rig.cameras[2].lut[1023] < 1.7
 
 
 def test_cornell_mesh_size(rt):
