@@ -30,8 +30,9 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(config: str, rows: int, threads: int) -> dict:
-    """Times the CPU oracle on `rows` rows of camera 0 of the workload."""
+def cpu_baseline(config: str, seconds: float, threads: int) -> dict:
+    """Times the CPU oracle on whole cameras of the workload, camera after
+    camera (then again with the next sample stream), until `seconds` of work."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ctypes as C
 
@@ -53,18 +54,26 @@ def cpu_baseline(config: str, rows: int, threads: int) -> dict:
     s.openpbr = C.cast(desc.openpbr, C.POINTER(O.OpenPbr))
     s.analytic_lights = C.cast(desc.analytic_lights, C.POINTER(O.AnalyticLight))
     bvh = O.OracleBvh(s)
-    cam = O.Camera()
-    C.memmove(C.byref(cam), C.byref(g.g.cameras[0]), C.sizeof(cam))
     prof = O.Profile()
     C.memmove(C.byref(prof), C.byref(g.g.profile), C.sizeof(prof))
-    r0 = max(0, cam.height // 2 - rows // 2)
+    ncam = g.g.camera_count
+    paths = 0
+    done = []
     t0 = time.perf_counter()
-    out = O.render_camera(s, bvh, cam, 0, prof, 0, threads=threads, rows=(r0, r0 + rows))
+    k = 0
+    while time.perf_counter() - t0 < seconds and k < 64:
+        ci, stream = k % ncam, k // ncam
+        cam = O.Camera()
+        C.memmove(C.byref(cam), C.byref(g.g.cameras[ci]), C.sizeof(cam))
+        out = O.render_camera(s, bvh, cam, ci, prof, stream, threads=threads)
+        paths += out["paths"]
+        done.append(f"cam{ci}/stream{stream}")
+        k += 1
     dt = time.perf_counter() - t0
-    paths = out["paths"]
     return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"{config} camera 0 rows {r0}..{r0 + rows} ({paths} paths, "
-                      f"{prof.samples_per_pixel} spp, depth {prof.max_bounces}) in {dt:.2f} s"}
+            "sample": f"{config}: {len(done)} whole camera images ({', '.join(done)}; "
+                      f"{paths} paths, {prof.samples_per_pixel} spp, depth {prof.max_bounces}) "
+                      f"in {dt:.2f} s on {threads} threads"}
 
 
 def cam_desc(cam) -> str:
@@ -78,7 +87,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="cornell_mesh")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--wavefront-paths", type=int, default=0)
     ap.add_argument("--trace-block", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
@@ -185,8 +194,7 @@ def main():
     if not a.no_cpu_baseline:
         threads = os.cpu_count() or 1
         threads = min(threads, 16)
-        rows = a.cpu_rows or 48
-        result["cpu_baseline"] = cpu_baseline(a.config, rows, threads)
+        result["cpu_baseline"] = cpu_baseline(a.config, a.cpu_seconds, threads)
     print(json.dumps(result), flush=True)
 
 

@@ -8,6 +8,9 @@
  *
  * Threading: handles are independent; calls on one handle must not overlap.
  * Ownership: handles own their device memory; callers own every host array.
+ * A scene keeps its device alive and a renderer keeps its scene alive, so the
+ * *_destroy calls may come in any order: each object is freed when the last
+ * handle using it is destroyed.
  */
 #ifndef RT_HIP_H
 #define RT_HIP_H

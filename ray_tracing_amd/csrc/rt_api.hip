@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -84,7 +85,12 @@ float length(V3 v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
 
 }  // namespace
 
+// Handles are reference counted: a scene holds its device and a renderer
+// holds its scene, so destroying them in any order frees each object only
+// after the last handle that uses it (the reference shares the prepared
+// scene through shared_ptr<SharedGpuSceneState>, optix_renderer.h:117).
 struct rt_device {
+    std::atomic<int> refs{1};
     int ordinal = 0;
     hipStream_t stream = nullptr;
     int cu_count = 0;
@@ -93,6 +99,7 @@ struct rt_device {
 };
 
 struct rt_scene {
+    std::atomic<int> refs{1};
     rt_device* dev = nullptr;
     int32_t sphere_count = 0, quad_count = 0, triangle_count = 0, material_count = 0;
     int32_t openpbr_count = 0, analytic_count = 0;
@@ -258,6 +265,26 @@ struct rt_renderer {
     }
 };
 
+namespace {
+
+void device_release(rt_device* d) {
+    if (d->refs.fetch_sub(1) != 1) return;
+    (void)hipSetDevice(d->ordinal);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+}
+
+void scene_release(rt_scene* s) {
+    if (s->refs.fetch_sub(1) != 1) return;
+    rt_device* d = s->dev;
+    (void)hipSetDevice(d->ordinal);
+    (void)hipStreamSynchronize(d->stream);
+    delete s;
+    device_release(d);
+}
+
+}  // namespace
+
 extern "C" {
 
 int rt_abi_version(void) { return RT_ABI_VERSION; }
@@ -289,9 +316,7 @@ rt_status rt_device_create(int ordinal, rt_device** out) {
 
 rt_status rt_device_destroy(rt_device* dev) {
     if (!dev) return RT_OK;
-    (void)hipSetDevice(dev->ordinal);
-    if (dev->stream) (void)hipStreamDestroy(dev->stream);
-    delete dev;
+    device_release(dev);
     return RT_OK;
 }
 
@@ -430,6 +455,7 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
     RT_HIP(hipSetDevice(dev->ordinal));
     auto* s = new rt_scene();
     s->dev = dev;
+    dev->refs.fetch_add(1);
     s->sphere_count = d->sphere_count;
     s->quad_count = d->quad_count;
     s->triangle_count = d->triangle_count;
@@ -469,7 +495,7 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
     if (e == hipSuccess) e = upload(s->a_cdf, acdf.data(), acdf.size(), st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
-        delete s;
+        scene_release(s);
         return fail(RT_ERROR_HIP, std::string("HIP runtime failure at scene upload: ") + hipGetErrorString(e));
     }
     *out = s;
@@ -478,9 +504,7 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
 
 rt_status rt_scene_destroy(rt_scene* scene) {
     if (!scene) return RT_OK;
-    (void)hipSetDevice(scene->dev->ordinal);
-    (void)hipStreamSynchronize(scene->dev->stream);
-    delete scene;
+    scene_release(scene);
     return RT_OK;
 }
 
@@ -629,10 +653,12 @@ rt_status rt_trace_closest_device(rt_scene* s, const rt_ray* d_rays, int64_t n, 
 
 rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out) {
     RT_REQUIRE(dev && scene && out, RT_ERROR_INVALID_ARGUMENT, "rt_renderer_create requires device, scene and output");
+    RT_REQUIRE(scene->dev == dev, RT_ERROR_INVALID_ARGUMENT, "the scene was uploaded to another device");
     RT_HIP(hipSetDevice(dev->ordinal));
     auto* r = new rt_renderer();
     r->dev = dev;
     r->scene = scene;
+    scene->refs.fetch_add(1);
     r->tuning.wavefront_paths = 1 << 22;
     r->tuning.trace_block = 128;
     r->tuning.trace_blocks_per_cu = 8;
@@ -643,6 +669,7 @@ rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out)
     if (e == hipSuccess) e = r->counters.reserve(16);
     if (e != hipSuccess) {
         delete r;
+        scene_release(scene);
         return fail(RT_ERROR_HIP, std::string("HIP runtime failure at renderer allocation: ") + hipGetErrorString(e));
     }
     *out = r;
@@ -651,9 +678,11 @@ rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out)
 
 rt_status rt_renderer_destroy(rt_renderer* r) {
     if (!r) return RT_OK;
+    rt_scene* scene = r->scene;
     (void)hipSetDevice(r->dev->ordinal);
     (void)hipStreamSynchronize(r->dev->stream);
     delete r;
+    scene_release(scene);
     return RT_OK;
 }
 

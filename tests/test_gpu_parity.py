@@ -186,3 +186,23 @@ def test_render_single_bounce_cornell(rt, oracle, device):
         b = ref["beauty"][..., :3].astype(np.float64)
         err = np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
         assert err.max() < 1e-4, f"max rel err {err.max()}"
+
+
+def test_handles_destroyed_out_of_order(rt):
+    """A scene keeps its device alive and a renderer its scene: destroying the
+    device first, then the scene, then the renderer frees everything once."""
+    say("[gpu] handle lifetimes")
+    dev = rt.Device(0)
+    desc, keep = product_desc(rt, tiny_scenes()["mixed"])
+    sc = gpu_scene(rt, dev, desc, keep)
+    ren = rt.Renderer(dev, sc)
+    dev.close()
+    sc.close()
+    prof = rt.Generated("random_spheres", 0).profile
+    prof.samples_per_pixel = 1
+    prof.max_bounces = 2
+    rig = rt.Generated("random_spheres", 0).rig()
+    ren.render(rig, prof, 0)
+    out = ren.download(0)
+    assert np.isfinite(out["beauty"]).all()
+    ren.close()
