@@ -216,9 +216,10 @@ struct RestirOut {
 };
 
 // accumulate_restir_analytic_direct_light (programs.cu:2315-2449) up to the
-// shadow ray, which the connect stage traces
+// shadow ray, which the connect stage traces.  `base` is the camera's first
+// pixel in the reservoir buffers, which hold all cameras back to back.
 __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& rv,
-    const HitInfo& h, f3 ray_d, int cam, int x, int y, int width, int height, uint32_t pixel,
+    const HitInfo& h, f3 ray_d, int cam, int base, int x, int y, int width, int height, uint32_t pixel,
     uint32_t sample, uint32_t stream, bool bsdf_avail, Reservoir* out, uint32_t* stack_node,
     float* stack_t, int stride) {
     RestirOut ro{};
@@ -256,6 +257,7 @@ __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& r
     if (rv.temporal_reuse != 0) {
         int pidx;
         if (rs_previous_pixel(rv, cam, width, height, h.pos, &pidx)) {
+            pidx += base;
             const Reservoir prev = rv.history[pidx];
             if (rs_surface_valid(cur, prev, fmaxf(0.005f, 0.01f * h.t), 0.95f, rv.max_history_age)) {
                 const LightSample L = analytic_candidate(s, h.pos, prev.light_index, prev.sample_u0, prev.sample_u1);
@@ -293,7 +295,7 @@ __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& r
                 }
                 const int nx = bx + dx, ny = by + dy;
                 if (nx < 0 || nx >= width || ny < 0 || ny >= height) continue;
-                const int nidx = ny * width + nx;
+                const int nidx = base + ny * width + nx;
                 const Reservoir prev = rv.history[nidx];
                 if (!rs_surface_valid(cur, prev, fmaxf(0.01f, 0.02f * h.t), 0.95f, rv.max_history_age)) continue;
                 const LightSample L = analytic_candidate(s, h.pos, prev.light_index, prev.sample_u0, prev.sample_u1);

@@ -338,7 +338,8 @@ __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, 
                                  ? rv.current + id.global_pixel
                                  : nullptr;
             const bool bsdf_avail = 1 < fp.max_bounces;
-            const RestirOut ro = restir_shade(s, rv, h, xyz(d4), (int)id.camera, x, y, w, hgt,
+            const RestirOut ro = restir_shade(s, rv, h, xyz(d4), (int)id.camera,
+                cams.pixel_offset[id.camera], x, y, w, hgt,
                 id.pixel, id.sample, fp.stream, bsdf_avail, out, stack_node, stack_t, BLOCK);
             if (ro.shadow) {
                 shadow = true;
