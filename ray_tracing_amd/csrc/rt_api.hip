@@ -840,6 +840,9 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
     if (timed) RT_HIP(hipEventRecord(r->events[0], st));
     RT_HIP(rtk::wf_reset_counters(b.counters, st));
     RT_HIP(rtk::wf_clear_accum(total, b, st));
+    // the frame's reservoirs start cleared: pixels without a surface keep an
+    // empty reservoir (optix_renderer.cpp:256-257 clears them per launch)
+    if (rv.enabled) RT_HIP(rtk::wf_zero(rv.current, P * sizeof(rtk::Reservoir), st));
     for (int s0 = 0; s0 < spp; s0 += layers) {
         fp.layer_count = std::min(layers, spp - s0);
         fp.first_sample = s0;

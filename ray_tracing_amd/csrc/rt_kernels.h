@@ -91,6 +91,8 @@ hipError_t wf_restir(const SceneView& s, int stack_depth, const CameraView& cams
     hipStream_t st);
 hipError_t wf_accumulate(const FrameParams& fp, WavefrontBuffers& b, hipStream_t st);
 hipError_t wf_clear_accum(int pixels, WavefrontBuffers& b, hipStream_t st);
+// zero `bytes` (a multiple of 4) of device memory with a fill kernel
+hipError_t wf_zero(void* p, size_t bytes, hipStream_t st);
 hipError_t wf_finalize(int pixels, int spp, const WavefrontBuffers& b, float4* beauty,
     float4* normal, float4* albedo, float* depth, hipStream_t st);
 

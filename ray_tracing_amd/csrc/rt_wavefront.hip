@@ -83,6 +83,13 @@ __global__ void k_clear_accum(int pixels, float4* a, float4* b, float4* c) {
     c[i] = z;
 }
 
+// zero a buffer of n 32-bit words (a fill kernel rather than a memset node,
+// so the frame's stage sequence can be captured in a graph)
+__global__ void k_zero_words(uint32_t* p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 __global__ __launch_bounds__(256) void k_raygen(CameraView cams, FrameParams fp, WavefrontBuffers b,
     int paths) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -448,6 +455,13 @@ hipError_t wf_reset_counters(uint32_t* counters, hipStream_t st) {
 hipError_t wf_clear_accum(int pixels, WavefrontBuffers& b, hipStream_t st) {
     k_clear_accum<<<(pixels + 255) / 256, 256, 0, st>>>(pixels, b.acc_beauty, b.acc_normal,
         b.acc_albedo);
+    return hipGetLastError();
+}
+
+hipError_t wf_zero(void* p, size_t bytes, hipStream_t st) {
+    const size_t n = bytes / 4u;
+    if (n == 0) return hipSuccess;
+    k_zero_words<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(static_cast<uint32_t*>(p), n);
     return hipGetLastError();
 }
 
