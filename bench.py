@@ -90,12 +90,27 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--wavefront-paths", type=int, default=0)
     ap.add_argument("--trace-block", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (3 times), the known-size "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this path tracer shards by camera with no "
               "collectives, and multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
+
+    if a.calibrate_copy:
+        import torch  # device memory for the calibration copy only
+
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: 3 copies of {src.numel() * 4} bytes")
+        return
 
     import ray_tracing_amd as rt
 

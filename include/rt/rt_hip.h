@@ -176,10 +176,13 @@ typedef struct rt_frame_timing {
 /* DeviceRadianceFrameView, src/realtime/gpu/device_radiance_frame_view.h */
 typedef struct rt_device_frame_view {
     int32_t width, height;
-    const float* beauty; /* device float4 per pixel */
-    const float* normal;
+    const float* beauty; /* device float4 per pixel (denoised when the profile asks for it) */
+    const float* normal; /* 0.5 n + 0.5 encoding, temporally resolved */
     const float* albedo;
     const float* depth;  /* device float per pixel */
+    const float* resolved_beauty; /* float4: beauty after reprojection, before denoising */
+    const float* flow;   /* float2: pixel motion to the previous frame */
+    const float* trust;  /* float: 1 where the history texel was accepted */
 } rt_device_frame_view;
 
 /* RestirDiagnostics (src/realtime/gpu/optix_renderer.cpp:183-215) */
@@ -244,6 +247,11 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
  * W*H*4 floats for beauty/normal/albedo, W*H floats for depth; NULL skips */
 rt_status rt_frame_download(rt_renderer* r, int camera, float* beauty_rgba, float* normal_rgba,
     float* albedo_rgba, float* depth);
+/* the denoiser guides of launch_resolve_kernel (programs.cu:1965-2045): flow
+ * (2 floats per pixel), flow trust (1 float) and the resolved beauty before
+ * denoising (4 floats); NULL skips */
+rt_status rt_frame_download_guides(rt_renderer* r, int camera, float* flow_xy, float* trust,
+    float* resolved_beauty_rgba);
 /* device view valid until the next rt_render_frame on this renderer */
 rt_status rt_frame_device_view(rt_renderer* r, int camera, rt_device_frame_view* view);
 /* OptixRenderer::reset_accumulation / reset_sequence */

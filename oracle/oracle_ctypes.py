@@ -195,6 +195,12 @@ def lib():
         L.rto_camera_ray.argtypes = [P(Camera), C.c_double, C.c_double, C.c_void_p, C.c_void_p]
         L.rto_project.argtypes = [P(Camera), C.c_void_p, C.c_void_p]
         L.rto_make_equi_lut.argtypes = [P(Camera)]
+        L.rto_resolve.argtypes = [P(Camera), P(Camera), C.c_int, C.c_int] + [C.c_void_p] * 8
+        L.rto_resolve.restype = None
+        L.rto_denoise_pass.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 5
+        L.rto_denoise_pass.restype = None
+        L.rto_denoise.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 6
+        L.rto_denoise.restype = None
         L.rto_primary_rays.argtypes = [P(Camera), C.c_int, C.c_int, C.c_uint32, C.c_int,
                                        C.c_void_p]
         L.rto_build_lights.argtypes = [P(Scene), C.c_void_p, C.c_int]
@@ -440,14 +446,42 @@ def generate_rig(config: int, seed: int, frame: int, frames: int):
     return [cams[i] for i in range(n.value)]
 
 
+def resolve(cam: Camera, prev_cam, history_length: int, restir_enabled: bool, frame: dict,
+            history: dict):
+    """Temporal reprojection of one camera image in place; returns (flow, trust)."""
+    w, h = cam.width, cam.height
+    flow = np.zeros((h, w, 2), np.float32)
+    trust = np.zeros((h, w), np.float32)
+    lib().rto_resolve(C.byref(cam), C.byref(prev_cam) if prev_cam is not None else None,
+                      history_length, 1 if restir_enabled else 0, ptr(frame["beauty"]),
+                      ptr(frame["normal"]), ptr(frame["depth"]), ptr(history["beauty"]),
+                      ptr(history["normal"]), ptr(history["depth"]), ptr(flow), ptr(trust))
+    return flow, trust
+
+
+def denoise(frame: dict, iterations: int = 4) -> np.ndarray:
+    """The edge-aware a-trous filter over one camera image."""
+    h, w = frame["depth"].shape
+    out = np.zeros((h, w, 4), np.float32)
+    scratch = np.zeros((h, w, 4), np.float32)
+    lib().rto_denoise(w, h, iterations, ptr(np.ascontiguousarray(frame["beauty"])),
+                      ptr(frame["normal"]), ptr(frame["albedo"]), ptr(frame["depth"]), ptr(out),
+                      ptr(scratch))
+    return out
+
+
 class RestirSequence:
-    """Multi-frame oracle render keeping per-camera ReSTIR history, as the
-    renderer does between frames (capture_launch_history)."""
+    """Multi-frame oracle render keeping per-camera history as the renderer
+    does between frames: ReSTIR reservoirs (capture_launch_history) and the
+    resolved beauty / normal / depth for temporal reprojection
+    (launch_resolve_kernel + copy_frame_to_history)."""
 
     def __init__(self, scene: Scene, bvh: "OracleBvh", profile: Profile, threads: int = 8):
         self.scene, self.bvh, self.profile, self.threads = scene, bvh, profile, threads
         self.prev = {}
         self.frame = 0
+        min_l = profile.restir_min_analytic_lights if profile.restir_min_analytic_lights > 0 else 1
+        self.restir = bool(profile.enable_restir_di) and scene.analytic_light_count >= min_l
 
     def render(self, cams, stream: int, rows=None):
         outs = []
@@ -457,8 +491,9 @@ class RestirSequence:
             res = np.zeros(w * h, RESERVOIR_DTYPE)
             io = RestirIO()
             keep = []
-            if ci in self.prev and self.frame > 0:
-                pc, hist = self.prev[ci]
+            prev = self.prev.get(ci) if self.frame > 0 else None
+            if prev is not None:
+                pc, hist, _ = prev
                 keep += [pc, hist]
                 io.prev_cam = C.cast(C.pointer(pc), C.c_void_p)
                 io.history = hist.ctypes.data
@@ -476,11 +511,18 @@ class RestirSequence:
                                             ptr(depth), self.threads, r0, r1, C.byref(st))
             if rc != 0:
                 raise RuntimeError("oracle render failed")
+            out = {"beauty": beauty, "normal": normal, "albedo": albedo, "depth": depth,
+                   "reservoirs": res}
+            if prev is not None:
+                out["flow"], out["trust"] = resolve(cam, prev[0], self.frame, self.restir, out,
+                                                    prev[2])
+            else:
+                out["flow"] = np.zeros((h, w, 2), np.float32)
+                out["trust"] = np.zeros((h, w), np.float32)
             cam_copy = Camera()
             C.memmove(C.byref(cam_copy), C.byref(cam), C.sizeof(Camera))
-            new_prev[ci] = (cam_copy, res)
-            outs.append({"beauty": beauty, "normal": normal, "albedo": albedo, "depth": depth,
-                         "reservoirs": res})
+            new_prev[ci] = (cam_copy, res, {k: out[k].copy() for k in ("beauty", "normal", "depth")})
+            outs.append(out)
         self.prev = new_prev
         self.frame += 1
         return outs

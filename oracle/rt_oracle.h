@@ -174,6 +174,19 @@ void rto_unproject(const rto_camera* cam, double px, double py, float dir_cam[3]
 void rto_camera_ray(const rto_camera* cam, double px, double py, float origin[3], float dir[3]);
 void rto_project(const rto_camera* cam, const float dir_cam[3], float pixel[2]);
 void rto_make_equi_lut(rto_camera* cam);
+/* the orbiting rig of the instanced scene (config 2) at frame `frame` of `frames` */
+int rto_generate_rig(int config, uint64_t seed, int frame, int frames, rto_camera* cams, int* count);
+
+/* ---- per-frame post passes (oracle_post.c) */
+/* temporal reprojection + accumulation of one camera image, in place */
+void rto_resolve(const rto_camera* cam, const rto_camera* prev_cam, int history_length,
+    int restir_enabled, float* beauty, float* normal, float* depth, const float* hbeauty,
+    const float* hnormal, const float* hdepth, float* flow, float* trust);
+/* one edge-aware a-trous pass (spacing `step`) / the full filter */
+void rto_denoise_pass(int w, int h, int step, const float* in, const float* normal,
+    const float* albedo, const float* depth, float* out);
+void rto_denoise(int w, int h, int iterations, const float* beauty, const float* normal,
+    const float* albedo, const float* depth, float* out, float* scratch);
 void rto_primary_rays(const rto_camera* cam, int cam_index, int sample, uint32_t stream,
     int jitter, rto_ray* rays);
 

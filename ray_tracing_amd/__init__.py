@@ -31,7 +31,7 @@ EXPORTED = [
     "rt_frame_download", "rt_frame_device_view", "rt_reset_accumulation", "rt_reset_sequence",
     "rt_restir_diagnostics", "rt_renderer_set_tuning", "rt_renderer_get_tuning",
     "rt_scene_generate", "rt_scene_generated_free", "rt_restir_download",
-    "rt_scene_generate_rig",
+    "rt_scene_generate_rig", "rt_frame_download_guides",
 ]
 
 
@@ -188,6 +188,17 @@ class Renderer:
         check(lib().rt_render_frame(self.h, C.byref(rig), C.byref(profile), sample_stream,
                                     C.byref(tm) if timed else None))
         return tm if timed else None
+
+    def guides(self, camera: int) -> dict:
+        """Flow (pixels to the previous frame), flow trust and the resolved
+        beauty before denoising."""
+        cam = self.rig.cameras[camera]
+        w, h = cam.width, cam.height
+        out = {"flow": np.zeros((h, w, 2), np.float32), "trust": np.zeros((h, w), np.float32),
+               "resolved": np.zeros((h, w, 4), np.float32)}
+        check(lib().rt_frame_download_guides(self.h, camera, ptr(out["flow"]), ptr(out["trust"]),
+                                             ptr(out["resolved"])))
+        return out
 
     def reservoirs(self, camera: int) -> np.ndarray:
         cam = self.rig.cameras[camera]

@@ -119,7 +119,8 @@ class RestirStats(C.Structure):
 
 class DeviceFrameView(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("beauty", C.c_void_p),
-                ("normal", C.c_void_p), ("albedo", C.c_void_p), ("depth", C.c_void_p)]
+                ("normal", C.c_void_p), ("albedo", C.c_void_p), ("depth", C.c_void_p),
+                ("resolved_beauty", C.c_void_p), ("flow", C.c_void_p), ("trust", C.c_void_p)]
 
 
 class GeneratedScene(C.Structure):
@@ -174,4 +175,5 @@ def declare(L):
     L.rt_scene_generate.argtypes = [C.c_int, C.c_uint64, P(P(GeneratedScene))]
     L.rt_scene_generated_free.argtypes = [P(GeneratedScene)]
     L.rt_restir_download.argtypes = [vp, C.c_int, vp]
+    L.rt_frame_download_guides.argtypes = [vp, C.c_int, vp, vp, vp]
     L.rt_scene_generate_rig.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, P(CameraRig)]

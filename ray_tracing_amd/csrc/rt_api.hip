@@ -172,9 +172,20 @@ struct rt_renderer {
     int pixel_offset[5] = {0, 0, 0, 0, 0};
     int widths[4] = {0, 0, 0, 0};
     int heights[4] = {0, 0, 0, 0};
-    // frame outputs
-    DevBuf<float4> beauty, normal, albedo;
-    DevBuf<float> depth;
+    // frame outputs: two sets of beauty / normal / depth, the last frame's
+    // set is the next frame's reprojection history
+    // (DeviceFrameBufferSet::copy_frame_to_history, device_frame_buffers.cpp:151-162,
+    // without the copy)
+    DevBuf<float4> beauty_set[2], normal_set[2];
+    DevBuf<float> depth_set[2];
+    DevBuf<float4> albedo;
+    DevBuf<float2> flow;
+    DevBuf<float> trust;
+    DevBuf<float4> denoised, denoise_scratch;
+    int frame_set = 0;
+    int last_set = 0;
+    bool last_denoised = false;
+    hipEvent_t post_ev[2] = {nullptr, nullptr};
     // wavefront state
     DevBuf<float4> ray_o, ray_d, hit, throughput, radiance, prev_pos, prev_n, aux_normal, aux_albedo;
     DevBuf<int32_t> prev_flags;
@@ -203,6 +214,8 @@ struct rt_renderer {
 
     ~rt_renderer() {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        for (hipEvent_t e : post_ev)
+            if (e) (void)hipEventDestroy(e);
         if (host_cams) (void)hipHostFree(host_cams);
     }
 
@@ -256,10 +269,14 @@ struct rt_renderer {
     hipError_t ensure_pixels(size_t p) {
         if (p <= pixel_capacity) return hipSuccess;
         hipError_t e;
-        for (DevBuf<float4>* b : {&beauty, &normal, &albedo, &acc_beauty, &acc_normal, &acc_albedo}) {
+        for (DevBuf<float4>* b : {&beauty_set[0], &beauty_set[1], &normal_set[0], &normal_set[1],
+                 &albedo, &denoised, &denoise_scratch, &acc_beauty, &acc_normal, &acc_albedo}) {
             if ((e = b->reserve(p)) != hipSuccess) return e;
         }
-        if ((e = depth.reserve(p)) != hipSuccess) return e;
+        if ((e = depth_set[0].reserve(p)) != hipSuccess) return e;
+        if ((e = depth_set[1].reserve(p)) != hipSuccess) return e;
+        if ((e = flow.reserve(p)) != hipSuccess) return e;
+        if ((e = trust.reserve(p)) != hipSuccess) return e;
         pixel_capacity = p;
         return hipSuccess;
     }
@@ -666,6 +683,8 @@ rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out)
     hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&r->host_cams), sizeof(rt_camera) * 8,
         hipHostMallocDefault);
     if (e == hipSuccess) e = r->cams.reserve(8);
+    if (e == hipSuccess) e = hipEventCreate(&r->post_ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&r->post_ev[1]);
     if (e == hipSuccess) e = r->counters.reserve(16);
     if (e != hipSuccess) {
         delete r;
@@ -861,7 +880,27 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
         }
         RT_HIP(rtk::wf_accumulate(fp, b, st));
     }
-    RT_HIP(rtk::wf_finalize(total, spp, b, r->beauty.p, r->normal.p, r->albedo.p, r->depth.p, st));
+    const int fs = r->frame_set, hs = fs ^ 1;
+    RT_HIP(rtk::wf_finalize(total, spp, b, r->beauty_set[fs].p, r->normal_set[fs].p, r->albedo.p,
+        r->depth_set[fs].p, st));
+    // launch_resolve_kernel (optix_renderer.cpp:263): cameras without
+    // history only clear their flow / trust
+    rtk::ResolveParams rp{};
+    for (int c = 0; c < 4; ++c) {
+        rp.history_length[c] = r->history_length[c];
+        rp.prev_valid[c] = r->prev_valid[c];
+    }
+    rp.restir_enabled = rv.enabled;
+    RT_HIP(rtk::wf_resolve(cv, rp, r->beauty_set[fs].p, r->normal_set[fs].p, r->depth_set[fs].p,
+        r->beauty_set[hs].p, r->normal_set[hs].p, r->depth_set[hs].p, r->flow.p, r->trust.p, total, st));
+    const bool denoise = p->enable_denoise != 0;
+    if (denoise) {
+        if (timed) RT_HIP(hipEventRecord(r->post_ev[0], st));
+        RT_HIP(rtk::wf_denoise_filter(cv, rtk::kDenoiseIterations, r->beauty_set[fs].p,
+            r->normal_set[fs].p, r->albedo.p, r->depth_set[fs].p, r->denoised.p,
+            r->denoise_scratch.p, total, st));
+        if (timed) RT_HIP(hipEventRecord(r->post_ev[1], st));
+    }
     if (timed) RT_HIP(hipEventRecord(r->events[1], st));
     uint32_t stats[16];
     RT_HIP(hipMemcpyAsync(stats, b.counters, sizeof(stats), hipMemcpyDeviceToHost, st));
@@ -876,6 +915,9 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
         r->prev_height[c] = r->heights[c];
         r->prev_offset[c] = r->pixel_offset[c];
     }
+    r->last_set = fs;
+    r->frame_set = hs;
+    r->last_denoised = denoise;
     r->last_reservoirs = rv.enabled ? r->res_cur : -1;
     if (rv.enabled) r->res_cur ^= 1;
     r->restir_last = rv.enabled != 0;
@@ -897,7 +939,10 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
         timing->extend_ms = ext;
         timing->shade_ms = shd;
         timing->connect_ms = con;
-        timing->other_ms = std::max(0.0, timing->render_ms - ext - shd - con);
+        float dn = 0.0f;
+        if (denoise) RT_HIP(hipEventElapsedTime(&dn, r->post_ev[0], r->post_ev[1]));
+        timing->denoise_ms = dn;
+        timing->other_ms = std::max(0.0, timing->render_ms - ext - shd - con - (double)dn);
         uint64_t ex = 0, sh = 0;
         std::memcpy(&ex, stats + 4, 8);
         std::memcpy(&sh, stats + 6, 8);
@@ -922,10 +967,11 @@ rt_status rt_frame_download(rt_renderer* r, int camera, float* beauty, float* no
     hipStream_t st = r->dev->stream;
     const size_t off = (size_t)r->pixel_offset[camera];
     const size_t n = (size_t)r->widths[camera] * (size_t)r->heights[camera];
-    if (beauty) RT_HIP(hipMemcpyAsync(beauty, r->beauty.p + off, n * 16, hipMemcpyDeviceToHost, st));
-    if (normal) RT_HIP(hipMemcpyAsync(normal, r->normal.p + off, n * 16, hipMemcpyDeviceToHost, st));
+    const float4* bsrc = r->last_denoised ? r->denoised.p : r->beauty_set[r->last_set].p;
+    if (beauty) RT_HIP(hipMemcpyAsync(beauty, bsrc + off, n * 16, hipMemcpyDeviceToHost, st));
+    if (normal) RT_HIP(hipMemcpyAsync(normal, r->normal_set[r->last_set].p + off, n * 16, hipMemcpyDeviceToHost, st));
     if (albedo) RT_HIP(hipMemcpyAsync(albedo, r->albedo.p + off, n * 16, hipMemcpyDeviceToHost, st));
-    if (depth) RT_HIP(hipMemcpyAsync(depth, r->depth.p + off, n * 4, hipMemcpyDeviceToHost, st));
+    if (depth) RT_HIP(hipMemcpyAsync(depth, r->depth_set[r->last_set].p + off, n * 4, hipMemcpyDeviceToHost, st));
     RT_HIP(hipStreamSynchronize(st));
     return RT_OK;
 }
@@ -938,10 +984,33 @@ rt_status rt_frame_device_view(rt_renderer* r, int camera, rt_device_frame_view*
     const size_t off = (size_t)r->pixel_offset[camera];
     view->width = r->widths[camera];
     view->height = r->heights[camera];
-    view->beauty = reinterpret_cast<const float*>(r->beauty.p + off);
-    view->normal = reinterpret_cast<const float*>(r->normal.p + off);
+    const float4* bsrc = r->last_denoised ? r->denoised.p : r->beauty_set[r->last_set].p;
+    view->beauty = reinterpret_cast<const float*>(bsrc + off);
+    view->normal = reinterpret_cast<const float*>(r->normal_set[r->last_set].p + off);
     view->albedo = reinterpret_cast<const float*>(r->albedo.p + off);
-    view->depth = r->depth.p + off;
+    view->depth = r->depth_set[r->last_set].p + off;
+    view->resolved_beauty = reinterpret_cast<const float*>(r->beauty_set[r->last_set].p + off);
+    view->flow = reinterpret_cast<const float*>(r->flow.p + off);
+    view->trust = r->trust.p + off;
+    return RT_OK;
+}
+
+rt_status rt_frame_download_guides(rt_renderer* r, int camera, float* flow_xy, float* trust,
+    float* resolved_beauty_rgba) {
+    RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
+    RT_REQUIRE(r->rendered, RT_ERROR_NOT_READY, "rt_frame_download_guides requires a rendered frame");
+    RT_REQUIRE(camera >= 0 && camera < r->camera_count, RT_ERROR_INVALID_ARGUMENT,
+        "camera index " + std::to_string(camera) + " out of range");
+    RT_HIP(hipSetDevice(r->dev->ordinal));
+    hipStream_t st = r->dev->stream;
+    const size_t off = (size_t)r->pixel_offset[camera];
+    const size_t n = (size_t)r->widths[camera] * (size_t)r->heights[camera];
+    if (flow_xy) RT_HIP(hipMemcpyAsync(flow_xy, r->flow.p + off, n * 8, hipMemcpyDeviceToHost, st));
+    if (trust) RT_HIP(hipMemcpyAsync(trust, r->trust.p + off, n * 4, hipMemcpyDeviceToHost, st));
+    if (resolved_beauty_rgba)
+        RT_HIP(hipMemcpyAsync(resolved_beauty_rgba, r->beauty_set[r->last_set].p + off, n * 16,
+            hipMemcpyDeviceToHost, st));
+    RT_HIP(hipStreamSynchronize(st));
     return RT_OK;
 }
 

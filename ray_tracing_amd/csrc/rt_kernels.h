@@ -93,6 +93,23 @@ hipError_t wf_accumulate(const FrameParams& fp, WavefrontBuffers& b, hipStream_t
 hipError_t wf_clear_accum(int pixels, WavefrontBuffers& b, hipStream_t st);
 // zero `bytes` (a multiple of 4) of device memory with a fill kernel
 hipError_t wf_zero(void* p, size_t bytes, hipStream_t st);
+// temporal reprojection of every camera with history (per camera: frames of
+// history so far, previous camera valid; previous cameras at cams.cams[4 + c])
+struct ResolveParams {
+    int32_t history_length[4];
+    int32_t prev_valid[4];
+    int32_t restir_enabled;
+};
+hipError_t wf_resolve(const CameraView& cams, const ResolveParams& rp, float4* beauty,
+    float4* normal, float* depth, const float4* hbeauty, const float4* hnormal,
+    const float* hdepth, float2* flow, float* trust, int total, hipStream_t st);
+
+// the edge-aware a-trous denoise filter, spacing 1, 2, 4, ... per pass
+constexpr int kDenoiseIterations = 4;
+hipError_t wf_denoise_filter(const CameraView& cams, int iterations, const float4* beauty,
+    const float4* normal, const float4* albedo, const float* depth, float4* out,
+    float4* scratch, int total, hipStream_t st);
+
 hipError_t wf_finalize(int pixels, int spp, const WavefrontBuffers& b, float4* beauty,
     float4* normal, float4* albedo, float* depth, hipStream_t st);
 

@@ -1,0 +1,186 @@
+// rt_post.hip -- per-frame post passes over all cameras' pixels at once:
+// temporal reprojection / accumulation and the edge-aware denoise filter.
+//
+// Pixels of all cameras lie back to back (CameraView::pixel_offset); one
+// launch covers the whole rig.  Both passes are bandwidth-bound stencils.
+#include <hip/hip_runtime.h>
+
+#include "rt_camera.h"
+#include "rt_kernels.h"
+#include "rt_math.h"
+
+namespace rtk {
+namespace {
+
+struct PixelId {
+    int camera, x, y, local, width, height;
+};
+
+__device__ __forceinline__ PixelId pixel_id(const CameraView& cams, int p) {
+    int c = 0;
+    while (c + 1 < cams.count && p >= cams.pixel_offset[c + 1]) ++c;
+    PixelId id;
+    id.camera = c;
+    id.local = p - cams.pixel_offset[c];
+    id.width = cams.width[c];
+    id.height = (cams.pixel_offset[c + 1] - cams.pixel_offset[c]) / id.width;
+    id.y = id.local / id.width;
+    id.x = id.local - id.y * id.width;
+    return id;
+}
+
+__device__ __forceinline__ f3 decode_normal(float4 e) {
+    return mk3(e.x * 2.0f - 1.0f, e.y * 2.0f - 1.0f, e.z * 2.0f - 1.0f);
+}
+
+// resolve_reprojection_kernel (src/realtime/gpu/programs.cu:1955-2067):
+// reproject the pixel centre's surface into the previous camera, accept the
+// history texel when normals agree (cos > 0.95) and depth matches within
+// max(0.02, 5 %), then blend 1/(n+1) toward the current value.  Beauty is
+// not blended when ReSTIR DI is on.  Writes flow and its trust mask.
+__global__ __launch_bounds__(256) void k_resolve(CameraView cams, ResolveParams rp, float4* beauty,
+    float4* normal, float* depth, const float4* hbeauty, const float4* hnormal,
+    const float* hdepth, float2* flow, float* trust, int total) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= total) return;
+    const PixelId id = pixel_id(cams, p);
+    flow[p] = make_float2(0.0f, 0.0f);
+    trust[p] = 0.0f;
+    const int hl = rp.history_length[id.camera];
+    if (hl <= 0 || rp.prev_valid[id.camera] == 0) return;
+    const rt_camera& cam = cams.cams[id.camera];
+    const rt_camera& pc = cams.cams[4 + id.camera];
+    const float pxf = (float)id.x + 0.5f, pyf = (float)id.y + 0.5f;
+    f3 o, d;
+    camera_ray(cam, (double)pxf, (double)pyf, &o, &d);
+    const float cd = depth[p];
+    const f3 wp = o + d * cd;
+    const f3 off = mk3(wp.x - (float)pc.origin[0], wp.y - (float)pc.origin[1], wp.z - (float)pc.origin[2]);
+    const f3 bx = mk3((float)pc.basis_x[0], (float)pc.basis_x[1], (float)pc.basis_x[2]);
+    const f3 by = mk3((float)pc.basis_y[0], (float)pc.basis_y[1], (float)pc.basis_y[2]);
+    const f3 bz = mk3((float)pc.basis_z[0], (float)pc.basis_z[1], (float)pc.basis_z[2]);
+    const f3 pcs = mk3(dot3(off, bx), dot3(off, by), dot3(off, bz));
+    float ppx, ppy;
+    project_camera(pc, pcs, &ppx, &ppy);
+    const bool finite = isfinite(ppx) && isfinite(ppy);
+    if (finite) flow[p] = make_float2(pxf - ppx, pyf - ppy);
+    const int qx = finite ? (int)floorf(ppx) : -1;
+    const int qy = finite ? (int)floorf(ppy) : -1;
+    if (!finite || qx < 0 || qx >= id.width || qy < 0 || qy >= id.height) return;
+    const int q = cams.pixel_offset[id.camera] + qy * id.width + qx;
+    const float4 hn4 = hnormal[q];
+    const float4 cn4 = normal[p];
+    const float ndot = dot3(decode_normal(cn4), decode_normal(hn4));
+    const float hd = hdepth[q];
+    const float expected = len3(pcs);
+    const float tol = fmaxf(0.02f, 0.05f * fmaxf(expected, hd));
+    if (!(ndot > 0.95f && fabsf(expected - hd) < tol)) return;
+    trust[p] = 1.0f;
+    const float blend = 1.0f / (float)(hl + 1);
+    if (rp.restir_enabled == 0) {
+        const float4 hb = hbeauty[q];
+        const float4 cb = beauty[p];
+        const float cx = fminf(fmaxf(cb.x, 0.0f), 10.0f);
+        const float cy = fminf(fmaxf(cb.y, 0.0f), 10.0f);
+        const float cz = fminf(fmaxf(cb.z, 0.0f), 10.0f);
+        beauty[p] = make_float4(hb.x + (cx - hb.x) * blend, hb.y + (cy - hb.y) * blend,
+            hb.z + (cz - hb.z) * blend, 1.0f);
+    }
+    normal[p] = make_float4(hn4.x + (cn4.x - hn4.x) * blend, hn4.y + (cn4.y - hn4.y) * blend,
+        hn4.z + (cn4.z - hn4.z) * blend, hn4.w + (cn4.w - hn4.w) * blend);
+    depth[p] = hd + (cd - hd) * blend;
+}
+
+// One pass of the edge-aware a-trous filter: 5x5 B3-spline taps at spacing
+// `step`, weighted by normal (cos^16), relative depth, albedo and relative
+// luminance edge stops built from +, *, / only.  Pixels without a surface
+// (depth 0) pass through and never contribute.
+__constant__ float kB3[3] = {0.375f, 0.25f, 0.0625f};
+
+__global__ __launch_bounds__(256) void k_denoise_pass(CameraView cams, int step, const float4* in,
+    const float4* normal, const float4* albedo, const float* depth, float4* out, int total) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= total) return;
+    const float zp = depth[p];
+    const float4 c4 = in[p];
+    if (!(zp > 0.0f)) {
+        out[p] = c4;
+        return;
+    }
+    const PixelId id = pixel_id(cams, p);
+    const int base = cams.pixel_offset[id.camera];
+    const f3 cp = xyz(c4);
+    const f3 np = decode_normal(normal[p]);
+    const f3 ap = xyz(albedo[p]);
+    const float lp = 0.2126f * cp.x + 0.7152f * cp.y + 0.0722f * cp.z;
+    const float zs = 0.1f * (float)step * zp;
+    f3 sum = mk3(0.0f, 0.0f, 0.0f);
+    float wsum = 0.0f;
+    for (int dy = -2; dy <= 2; ++dy) {
+        const int qy = id.y + dy * step;
+        if (qy < 0 || qy >= id.height) continue;
+        for (int dx = -2; dx <= 2; ++dx) {
+            const int qx = id.x + dx * step;
+            if (qx < 0 || qx >= id.width) continue;
+            const int q = base + qy * id.width + qx;
+            const float zq = depth[q];
+            if (!(zq > 0.0f)) continue;
+            const f3 cq = xyz(in[q]);
+            const f3 nq = decode_normal(normal[q]);
+            const f3 aq = xyz(albedo[q]);
+            const float lq = 0.2126f * cq.x + 0.7152f * cq.y + 0.0722f * cq.z;
+            float wn = fmaxf(dot3(np, nq), 0.0f);
+            wn = wn * wn;
+            wn = wn * wn;
+            wn = wn * wn;
+            wn = wn * wn;
+            const float ez = fabsf(zp - zq) / zs;
+            const float wz = 1.0f / (1.0f + ez * ez);
+            const float ea = (fabsf(ap.x - aq.x) + fabsf(ap.y - aq.y) + fabsf(ap.z - aq.z)) / 0.1f;
+            const float wa = 1.0f / (1.0f + ea * ea);
+            const float el = fabsf(lp - lq) / (4.0f * fmaxf(lp, lq) + 1e-3f);
+            const float wl = 1.0f / (1.0f + el * el);
+            const float k = kB3[dx < 0 ? -dx : dx] * kB3[dy < 0 ? -dy : dy];
+            const float wt = k * wn * wz * wa * wl;
+            sum = sum + cq * wt;
+            wsum += wt;
+        }
+    }
+    if (wsum > 0.0f) {
+        const float inv = 1.0f / wsum;
+        out[p] = make_float4(sum.x * inv, sum.y * inv, sum.z * inv, 1.0f);
+    } else {
+        out[p] = make_float4(cp.x, cp.y, cp.z, 1.0f);
+    }
+}
+
+}  // namespace
+
+hipError_t wf_resolve(const CameraView& cams, const ResolveParams& rp, float4* beauty,
+    float4* normal, float* depth, const float4* hbeauty, const float4* hnormal,
+    const float* hdepth, float2* flow, float* trust, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    k_resolve<<<(total + 255) / 256, 256, 0, st>>>(cams, rp, beauty, normal, depth, hbeauty,
+        hnormal, hdepth, flow, trust, total);
+    return hipGetLastError();
+}
+
+hipError_t wf_denoise_filter(const CameraView& cams, int iterations, const float4* beauty,
+    const float4* normal, const float4* albedo, const float* depth, float4* out,
+    float4* scratch, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    // passes alternate between `scratch` and `out`, the last one lands in `out`
+    const float4* src = beauty;
+    for (int it = 0; it < iterations; ++it) {
+        float4* dst = ((iterations - 1 - it) & 1) == 0 ? out : scratch;
+        k_denoise_pass<<<(total + 255) / 256, 256, 0, st>>>(cams, 1 << it, src, normal, albedo,
+            depth, dst, total);
+        src = dst;
+    }
+    if (iterations <= 0) {
+        return hipMemcpyAsync(out, beauty, (size_t)total * sizeof(float4), hipMemcpyDeviceToDevice, st);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace rtk
