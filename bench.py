@@ -143,7 +143,8 @@ def main():
     paths_per_frame = sum(rig.cameras[i].width * rig.cameras[i].height
                           for i in range(rig.active_count)) * prof.samples_per_pixel
     log(f"[bench] {info['name']} CUs={info['cu_count']} config={a.config} prims={scene.prim_count} "
-        f"bvh depth={bst.max_depth} build={bst.build_ms:.2f} ms paths/frame={paths_per_frame}")
+        f"lbvh depth={bst.max_depth} 4-wide nodes={bst.wide_node_count} levels={bst.wide_levels} "
+        f"build={bst.build_ms:.2f} ms paths/frame={paths_per_frame}")
 
     # traversal counts for the algorithmic-bytes roofline (separate, untimed frame)
     ren.set_tuning(count_traversal=1)
@@ -153,7 +154,7 @@ def main():
     extend_bytes = (cnt.extend_rays * (RAY_B + HIT_B) + cnt.node_visits * NODE_B
                     + nonsphere * (PRIM_IDX_B + TRI_B) + cnt.sphere_tests * (PRIM_IDX_B + SPHERE_B))
     log(f"[bench] counting frame: rays={cnt.extend_rays} visits/ray="
-        f"{cnt.node_visits / max(cnt.extend_rays, 1):.2f} tests/ray="
+        f"{cnt.node_visits / max(cnt.extend_rays, 1):.2f} (child boxes) tests/ray="
         f"{cnt.prim_tests / max(cnt.extend_rays, 1):.2f}")
 
     for i in range(a.warmup):

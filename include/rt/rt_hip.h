@@ -145,11 +145,29 @@ typedef struct rt_bvh_node {
 
 typedef struct rt_bvh_stats {
     int32_t prim_count;
-    int32_t node_count;
-    int32_t max_depth;
+    int32_t node_count;      /* binary LBVH internal nodes */
+    int32_t max_depth;       /* binary LBVH depth */
+    int32_t wide_node_count; /* nodes of the 4-wide tree every ray traverses */
+    double build_ms;         /* device time of the build kernels (LBVH + collapse) */
+    int32_t wide_levels;     /* levels of the 4-wide tree */
     int32_t pad;
-    double build_ms; /* device time of the build kernels */
 } rt_bvh_stats;
+
+/* 4-wide quantized BVH node (64 B), collapsed from the binary LBVH: each node
+ * absorbs binary descendants, largest surface area first, until it has 4
+ * children; binary subtrees of at most 4 primitives become leaf ranges.
+ * Child k's box is origin + [qlo, qhi] * 2^(e - 127) per axis (e = byte a of
+ * meta, qlo/qhi = byte k of the axis word), rounded outward.  Child
+ * reference: 0x80000000 | (count - 1) << 28 | first sorted leaf slot for a
+ * leaf range, a node index otherwise, 0xFFFFFFFF when empty. */
+typedef struct rt_bvh_wide_node {
+    float origin[3];
+    uint32_t meta;   /* ex | ey << 8 | ez << 16 | child count << 24 */
+    uint32_t child[4];
+    uint32_t qlo[3];
+    uint32_t qhi[3];
+    uint32_t pad[2];
+} rt_bvh_wide_node;
 
 typedef struct rt_ray { float o[3]; float tmin; float d[3]; float tmax; } rt_ray;
 typedef struct rt_hit { float t; int32_t prim; float u; float v; } rt_hit;
@@ -168,7 +186,8 @@ typedef struct rt_frame_timing {
     uint64_t extend_rays;  /* closest-hit rays traced */
     uint64_t shadow_rays;  /* any-hit rays traced */
     uint64_t extend_launches;
-    uint64_t node_visits;  /* closest-hit BVH node fetches (count_traversal only) */
+    uint64_t node_visits;  /* closest-hit BVH child boxes tested, one canonical node each
+                              (count_traversal only) */
     uint64_t prim_tests;   /* closest-hit primitive tests (count_traversal only) */
     uint64_t sphere_tests; /* of which sphere tests */
 } rt_frame_timing;
@@ -224,6 +243,8 @@ rt_status rt_bvh_build(rt_scene* scene, rt_bvh_stats* stats);
  * any pointer may be NULL */
 rt_status rt_bvh_download(rt_scene* scene, rt_bvh_node* nodes, uint32_t* sorted_prims,
     uint32_t* morton);
+/* copies stats.wide_node_count 4-wide nodes (rt_bvh_wide_node) */
+rt_status rt_bvh_download_wide(rt_scene* scene, void* nodes4);
 
 /* test hooks replacing intersect_scene / is_occluded
  * (src/realtime/gpu/programs.cu:1654-1762); host arrays of n rays */

@@ -138,6 +138,8 @@ static void node_set_child_box(rto_bvh_node* nd, int slot, const box3* b) {
     for (int a = 0; a < 3; ++a) { lo[a] = b->lo[a]; hi[a] = b->hi[a]; }
 }
 
+static int collapse_wide(rto_bvh* bvh);
+
 int rto_build_lbvh(const rto_scene* s, rto_bvh* out) {
     memset(out, 0, sizeof(*out));
     const int n = prim_total(s);
@@ -183,6 +185,7 @@ int rto_build_lbvh(const rto_scene* s, rto_bvh* out) {
         r->child[0] = OM_LEAF_BIT | 0u;
         r->child[1] = OM_EMPTY;
         r->parent = OM_EMPTY;
+        r->pad = 1u;
         node_set_child_box(r, 0, &pb[vals[0]]);
         box3 e;
         box_empty(&e);
@@ -190,7 +193,7 @@ int rto_build_lbvh(const rto_scene* s, rto_bvh* out) {
         out->max_depth = 1;
         free(leaf_parent);
         free(pb);
-        return 0;
+        return collapse_wide(out);
     }
 
     out->nodes[0].parent = OM_EMPTY;
@@ -218,6 +221,7 @@ int rto_build_lbvh(const rto_scene* s, rto_bvh* out) {
         const int first = i < j ? i : j;
         const int last = i < j ? j : i;
         rto_bvh_node* nd = &out->nodes[i];
+        nd->pad = (uint32_t)(last - first + 1);  /* leaves under this node */
         if (first == gamma) {
             nd->child[0] = OM_LEAF_BIT | (uint32_t)gamma;
             leaf_parent[gamma] = (uint32_t)i;
@@ -268,6 +272,7 @@ int rto_build_lbvh(const rto_scene* s, rto_bvh* out) {
         nb[i] = acc;
     }
     out->max_depth = max_depth;
+    if (collapse_wide(out) != 0) { free(depth); free(done); free(stack); free(nb); free(leaf_parent); free(pb); return -1; }
     free(depth);
     free(done);
     free(stack);
@@ -277,7 +282,163 @@ int rto_build_lbvh(const rto_scene* s, rto_bvh* out) {
     return 0;
 }
 
+/* ------------------------------------------------- 4-wide collapse
+ * expand(b): start from binary node b's two children; while fewer than 4,
+ * replace the internal child with the largest surface area (first on ties)
+ * by its two children, in place.  Nodes are numbered by an exclusive scan of
+ * the "roots a 4-wide node" marks in binary node order. */
+typedef struct { uint32_t ref[4]; float lo[4][3]; float hi[4][3]; int count; } wide_t;
+
+static float box_area3(const float* lo, const float* hi) {
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+static void binary_children(const rto_bvh_node* nodes, uint32_t b, uint32_t ref[2], float lo[2][3],
+    float hi[2][3]) {
+    const rto_bvh_node* nd = &nodes[b];
+    ref[0] = nd->child[0];
+    ref[1] = nd->child[1];
+    for (int a = 0; a < 3; ++a) {
+        lo[0][a] = nd->lo0[a]; hi[0][a] = nd->hi0[a];
+        lo[1][a] = nd->lo1[a]; hi[1][a] = nd->hi1[a];
+    }
+}
+
+#define WIDE_LEAF_MAX 4u
+static uint32_t subtree_leaves(const rto_bvh_node* nodes, uint32_t ref) {
+    return (ref & OM_LEAF_BIT) ? 1u : nodes[ref].pad;
+}
+static uint32_t first_leaf(const rto_bvh_node* nodes, uint32_t ref) {
+    while (!(ref & OM_LEAF_BIT)) ref = nodes[ref].child[0];
+    return ref & ~OM_LEAF_BIT;
+}
+
+static void expand_wide(const rto_bvh_node* nodes, uint32_t b, wide_t* w) {
+    uint32_t r2[2];
+    float l2[2][3], h2[2][3];
+    binary_children(nodes, b, r2, l2, h2);
+    w->count = 0;
+    for (int c = 0; c < 2; ++c) {
+        if (r2[c] == OM_EMPTY) continue;
+        w->ref[w->count] = r2[c];
+        for (int a = 0; a < 3; ++a) { w->lo[w->count][a] = l2[c][a]; w->hi[w->count][a] = h2[c][a]; }
+        ++w->count;
+    }
+    while (w->count < 4) {
+        int best = -1;
+        float best_area = -1.0f;
+        for (int j = 0; j < w->count; ++j) {
+            if (subtree_leaves(nodes, w->ref[j]) <= WIDE_LEAF_MAX) continue;
+            const float ar = box_area3(w->lo[j], w->hi[j]);
+            if (ar > best_area) { best_area = ar; best = j; }
+        }
+        if (best < 0) break;
+        binary_children(nodes, w->ref[best], r2, l2, h2);
+        for (int j = w->count; j > best + 1; --j) {
+            w->ref[j] = w->ref[j - 1];
+            for (int a = 0; a < 3; ++a) { w->lo[j][a] = w->lo[j - 1][a]; w->hi[j][a] = w->hi[j - 1][a]; }
+        }
+        for (int c = 0; c < 2; ++c) {
+            w->ref[best + c] = r2[c];
+            for (int a = 0; a < 3; ++a) { w->lo[best + c][a] = l2[c][a]; w->hi[best + c][a] = h2[c][a]; }
+        }
+        ++w->count;
+    }
+}
+
+/* smallest quantum 2^e (from frexp, then fixed up) with 255 * 2^e >= extent */
+static int wide_exponent(double extent) {
+    if (!(extent > 0.0)) return -100;
+    int e;
+    (void)frexp(extent / 255.0, &e);
+    while (255.0 * ldexp(1.0, e) < extent) ++e;
+    if (e < -100) e = -100;
+    if (e > 100) e = 100;
+    return e;
+}
+
+static int collapse_wide(rto_bvh* bvh) {
+    const int nbin = bvh->node_count;
+    bvh->wide = NULL;
+    bvh->wide_count = 0;
+    bvh->wide_levels = 0;
+    if (nbin <= 0) return 0;
+    uint32_t* is_wide = (uint32_t*)calloc((size_t)nbin, sizeof(uint32_t));
+    uint32_t* front = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)nbin);
+    uint32_t* next = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)nbin);
+    uint32_t* index = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)nbin);
+    if (!is_wide || !front || !next || !index) { free(is_wide); free(front); free(next); free(index); return -1; }
+    is_wide[0] = 1u;
+    front[0] = 0u;
+    int nf = 1, levels = 0;
+    while (nf > 0) {
+        ++levels;
+        int nn = 0;
+        for (int i = 0; i < nf; ++i) {
+            wide_t w;
+            expand_wide(bvh->nodes, front[i], &w);
+            for (int k = 0; k < w.count; ++k) {
+                if (subtree_leaves(bvh->nodes, w.ref[k]) <= WIDE_LEAF_MAX) continue;
+                is_wide[w.ref[k]] = 1u;
+                next[nn++] = w.ref[k];
+            }
+        }
+        uint32_t* t = front; front = next; next = t;
+        nf = nn;
+    }
+    uint32_t run = 0;
+    for (int b = 0; b < nbin; ++b) { index[b] = run; run += is_wide[b]; }
+    rto_wide_node* out = (rto_wide_node*)calloc(run, sizeof(rto_wide_node));
+    if (!out) { free(is_wide); free(front); free(next); free(index); return -1; }
+    for (int b = 0; b < nbin; ++b) {
+        if (!is_wide[b]) continue;
+        wide_t w;
+        expand_wide(bvh->nodes, (uint32_t)b, &w);
+        rto_wide_node* o = &out[index[b]];
+        double ext[3];
+        for (int a = 0; a < 3; ++a) {
+            float lo = INFINITY, hi = -INFINITY;
+            for (int k = 0; k < w.count; ++k) { lo = sel_min(lo, w.lo[k][a]); hi = sel_max(hi, w.hi[k][a]); }
+            o->origin[a] = lo;
+            ext[a] = (double)hi - (double)lo;
+        }
+        uint32_t meta = (uint32_t)w.count << 24;
+        for (int a = 0; a < 3; ++a) {
+            const int e = wide_exponent(ext[a]);
+            meta |= (uint32_t)(e + 127) << (8 * a);
+            const double sc = ldexp(1.0, e), inv = ldexp(1.0, -e);
+            for (int k = 0; k < w.count; ++k) {
+                double ql = floor(((double)w.lo[k][a] - (double)o->origin[a]) * inv);
+                if (ql < 0.0) ql = 0.0;
+                if (ql > 255.0) ql = 255.0;
+                while (ql > 0.0 && (double)o->origin[a] + ql * sc > (double)w.lo[k][a]) ql -= 1.0;
+                double qh = ceil(((double)w.hi[k][a] - (double)o->origin[a]) * inv);
+                if (qh < 0.0) qh = 0.0;
+                if (qh > 255.0) qh = 255.0;
+                while (qh < 255.0 && (double)o->origin[a] + qh * sc < (double)w.hi[k][a]) qh += 1.0;
+                o->qlo[a] |= (uint32_t)ql << (8 * k);
+                o->qhi[a] |= (uint32_t)qh << (8 * k);
+            }
+        }
+        o->meta = meta;
+        for (int k = 0; k < 4; ++k) {
+            if (k >= w.count) { o->child[k] = OM_EMPTY; continue; }
+            const uint32_t nl = subtree_leaves(bvh->nodes, w.ref[k]);
+            o->child[k] = nl <= WIDE_LEAF_MAX
+                              ? (OM_LEAF_BIT | ((nl - 1u) << 28) | first_leaf(bvh->nodes, w.ref[k]))
+                              : index[w.ref[k]];
+        }
+    }
+    bvh->wide = out;
+    bvh->wide_count = (int32_t)run;
+    bvh->wide_levels = levels;
+    free(is_wide); free(front); free(next); free(index);
+    return 0;
+}
+
 void rto_free_lbvh(rto_bvh* bvh) {
+    free(bvh->wide);
     free(bvh->nodes);
     free(bvh->sorted_prims);
     free(bvh->morton);
@@ -393,35 +554,6 @@ static float isect_prim(const rto_scene* s, int id, v3 o, v3 d, float tmin, floa
 }
 
 /* ---------------------------------------------------------------- traversal */
-typedef struct { float inv[3]; float oinv[3]; } slab_pre;
-
-static void slab_setup(v3 d, v3 o, slab_pre* p) {
-    const float dd[3] = {d.x, d.y, d.z};
-    const float oo[3] = {o.x, o.y, o.z};
-    for (int a = 0; a < 3; ++a) {
-        const float dv = fabsf(dd[a]) > 1e-12f ? dd[a] : copysignf(1e-12f, dd[a]);
-        p->inv[a] = 1.0f / dv;
-        p->oinv[a] = oo[a] * p->inv[a];
-    }
-}
-
-/* slab test; the far distance is widened by 8e-7 relative so that a box
- * never culls a primitive whose rounded hit distance lies on its surface */
-static int slab_hit(const slab_pre* p, const float* lo, const float* hi, float tmin, float tbest, float* tnear) {
-    float tn = tmin, tf = tbest;
-    float t0[3], t1[3];
-    for (int a = 0; a < 3; ++a) {
-        t0[a] = fmaf(lo[a], p->inv[a], -p->oinv[a]);
-        t1[a] = fmaf(hi[a], p->inv[a], -p->oinv[a]);
-    }
-    const float nx = fminf(t0[0], t1[0]), ny = fminf(t0[1], t1[1]), nz = fminf(t0[2], t1[2]);
-    const float fx = fmaxf(t0[0], t1[0]), fy = fmaxf(t0[1], t1[1]), fz = fmaxf(t0[2], t1[2]);
-    tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, tn));
-    tf = fminf(fminf(fx, fy), fminf(fz, tf)) * 1.0000008f;
-    *tnear = tn;
-    return tn <= tf;
-}
-
 typedef struct { float t; int32_t prim; float u; float v; int found; } best_hit;
 
 static void consider(best_hit* b, int id, float t, float u, float v) {
@@ -431,34 +563,82 @@ static void consider(best_hit* b, int id, float t, float u, float v) {
     }
 }
 
-/* Traversal order (shared definition for the node-visit/prim-test counts):
- * at a node both child boxes are tested against the current best distance;
- * leaf children that pass are intersected (child 0 first); of the internal
- * children that pass, the one with the smaller entry distance (child 0 on a
- * tie) is visited next and the other is pushed with its entry distance;
- * popped entries whose entry distance exceeds the best distance are skipped. */
+/* Traversal of the 4-wide quantized BVH (shared definition for the
+ * box-test / prim-test counts, which the GPU kernels reproduce):
+ *  - at a node every valid child box is tested against the current best
+ *    distance (one counted box test each);
+ *  - leaf-range children that pass are intersected in slot order, their
+ *    primitives in sorted order;
+ *  - internal children that pass are ordered by entry distance (stable: the
+ *    lower slot first on ties); the nearest is visited next and the others
+ *    are pushed farthest first with their entry distance;
+ *  - popped entries whose entry distance exceeds the best distance are
+ *    skipped.
+ * Quantized box test: per axis t = fma(q, inv * 2^e, (origin - o) * inv);
+ * the exit distance is widened by 2.4e-6 relative plus 4e-7 of the summed
+ * |(origin - o) * inv| so rounding never culls a hit on a box face. */
+typedef struct { float inv[3]; } ray_pre;
+
+static int wide_box_tests(const rto_wide_node* nd, const float* o, const ray_pre* p, float tmin,
+    float tbest, float tn_out[4]) {
+    float sc[3], off[3];
+    for (int a = 0; a < 3; ++a) {
+        uint32_t eb = (nd->meta >> (8 * a)) & 255u;
+        uint32_t bits = eb << 23;
+        float pw;
+        memcpy(&pw, &bits, 4);
+        sc[a] = p->inv[a] * pw;
+        off[a] = (nd->origin[a] - o[a]) * p->inv[a];
+    }
+    const float slack = 4e-7f * (fabsf(off[0]) + fabsf(off[1]) + fabsf(off[2]));
+    int mask = 0;
+    for (int k = 0; k < 4; ++k) {
+        tn_out[k] = 0.0f;
+        if (nd->child[k] == OM_EMPTY) continue;
+        float t0[3], t1[3];
+        for (int a = 0; a < 3; ++a) {
+            const float ql = (float)((nd->qlo[a] >> (8 * k)) & 255u);
+            const float qh = (float)((nd->qhi[a] >> (8 * k)) & 255u);
+            t0[a] = fmaf(ql, sc[a], off[a]);
+            t1[a] = fmaf(qh, sc[a], off[a]);
+        }
+        const float nx = fminf(t0[0], t1[0]), ny = fminf(t0[1], t1[1]), nz = fminf(t0[2], t1[2]);
+        const float fx = fmaxf(t0[0], t1[0]), fy = fmaxf(t0[1], t1[1]), fz = fmaxf(t0[2], t1[2]);
+        const float tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
+        const float tf = fminf(fminf(fx, fy), fminf(fz, tbest)) * 1.0000024f + slack;
+        tn_out[k] = tn;
+        if (tn <= tf) mask |= 1 << k;
+    }
+    return mask;
+}
+
 static void traverse(const rto_scene* s, const rto_bvh* bvh, const rto_ray* r, int any,
     best_hit* out, uint32_t* visits, uint32_t* tests) {
     best_hit b = {r->tmax, -1, 0.0f, 0.0f, 0};
     uint32_t nv = 0, nt = 0;
-    if (bvh->prim_count > 0) {
+    if (bvh->prim_count > 0 && bvh->wide_count > 0) {
         const v3 o = ld3(r->o), d = ld3(r->d);
-        slab_pre p;
-        slab_setup(d, o, &p);
-        uint32_t stack_node[96];
-        float stack_t[96];
+        ray_pre p;
+        const float dd[3] = {d.x, d.y, d.z};
+        for (int a = 0; a < 3; ++a) {
+            const float dv = fabsf(dd[a]) > 1e-12f ? dd[a] : copysignf(1e-12f, dd[a]);
+            p.inv[a] = 1.0f / dv;
+        }
+        uint32_t stack_node[192];
+        float stack_t[192];
         int sp = 0;
         uint32_t node = 0;
         while (1) {
-            const rto_bvh_node* nd = &bvh->nodes[node];
-            ++nv;
-            float tn0 = 0.0f, tn1 = 0.0f;
-            const int h0 = nd->child[0] != OM_EMPTY && slab_hit(&p, nd->lo0, nd->hi0, r->tmin, b.t, &tn0);
-            const int h1 = nd->child[1] != OM_EMPTY && slab_hit(&p, nd->lo1, nd->hi1, r->tmin, b.t, &tn1);
-            const int hit[2] = {h0, h1};
-            for (int c = 0; c < 2; ++c) {
-                if (hit[c] && (nd->child[c] & OM_LEAF_BIT)) {
-                    const int id = (int)bvh->sorted_prims[nd->child[c] & ~OM_LEAF_BIT];
+            const rto_wide_node* nd = &bvh->wide[node];
+            nv += nd->meta >> 24;
+            float tn[4];
+            const int mask = wide_box_tests(nd, r->o, &p, r->tmin, b.t, tn);
+            for (int k = 0; k < 4; ++k) {
+                if (!(mask & (1 << k)) || !(nd->child[k] & OM_LEAF_BIT)) continue;
+                const uint32_t first = nd->child[k] & 0x0FFFFFFFu;
+                const uint32_t cnt = ((nd->child[k] >> 28) & 7u) + 1u;
+                for (uint32_t i = 0; i < cnt; ++i) {
+                    const int id = (int)bvh->sorted_prims[first + i];
                     float u, v;
                     ++nt;
                     const float t = isect_prim(s, id, o, d, r->tmin, r->tmax, &u, &v);
@@ -466,19 +646,21 @@ static void traverse(const rto_scene* s, const rto_bvh* bvh, const rto_ray* r, i
                     if (any && b.found) goto done;
                 }
             }
-            const int i0 = h0 && !(nd->child[0] & OM_LEAF_BIT);
-            const int i1 = h1 && !(nd->child[1] & OM_LEAF_BIT);
-            if (i0 && i1) {
-                const int near1 = tn1 < tn0;
-                const uint32_t nn = near1 ? nd->child[1] : nd->child[0];
-                stack_node[sp] = near1 ? nd->child[0] : nd->child[1];
-                stack_t[sp] = near1 ? tn0 : tn1;
-                ++sp;
-                node = nn;
+            float key[4];
+            uint32_t ref[4];
+            int m = 0;
+            for (int k = 0; k < 4; ++k) {
+                if (!(mask & (1 << k)) || (nd->child[k] & OM_LEAF_BIT)) continue;
+                int j = m++;
+                while (j > 0 && tn[k] < key[j - 1]) { key[j] = key[j - 1]; ref[j] = ref[j - 1]; --j; }
+                key[j] = tn[k];
+                ref[j] = nd->child[k];
+            }
+            if (m > 0) {
+                for (int j = m - 1; j >= 1; --j) { stack_node[sp] = ref[j]; stack_t[sp] = key[j]; ++sp; }
+                node = ref[0];
                 continue;
             }
-            if (i0) { node = nd->child[0]; continue; }
-            if (i1) { node = nd->child[1]; continue; }
             int got = 0;
             while (sp > 0) {
                 --sp;

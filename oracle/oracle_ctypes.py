@@ -108,10 +108,16 @@ class BvhNode(C.Structure):
                 ("child", C.c_uint32 * 2), ("parent", C.c_uint32), ("pad", C.c_uint32)]
 
 
+WIDE_DTYPE = np.dtype([("origin", "<f4", 3), ("meta", "<u4"), ("child", "<u4", 4),
+                       ("qlo", "<u4", 3), ("qhi", "<u4", 3), ("pad", "<u4", 2)])
+
+
 class Bvh(C.Structure):
     _fields_ = [("prim_count", C.c_int32), ("node_count", C.c_int32),
                 ("nodes", C.POINTER(BvhNode)), ("sorted_prims", C.POINTER(C.c_uint32)),
-                ("morton", C.POINTER(C.c_uint32)), ("max_depth", C.c_int32)]
+                ("morton", C.POINTER(C.c_uint32)), ("max_depth", C.c_int32),
+                ("wide_count", C.c_int32), ("wide", C.c_void_p), ("wide_levels", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 class Generated(C.Structure):
@@ -382,6 +388,12 @@ class OracleBvh:
         n = self.bvh.prim_count
         return np.ctypeslib.as_array(self.bvh.sorted_prims, (n,)).copy() if n else np.zeros(0, np.uint32)
 
+    def wide(self) -> np.ndarray:
+        n = self.bvh.wide_count
+        if n == 0:
+            return np.zeros(0, WIDE_DTYPE)
+        return np.frombuffer(C.string_at(self.bvh.wide, n * 64), WIDE_DTYPE).copy()
+
     def morton(self) -> np.ndarray:
         n = self.bvh.prim_count
         return np.ctypeslib.as_array(self.bvh.morton, (n,)).copy() if n else np.zeros(0, np.uint32)
@@ -390,6 +402,7 @@ class OracleBvh:
         return lib().rto_bvh_validate(C.byref(self.scene), C.byref(self.bvh))
 
     def trace_closest(self, rays: np.ndarray, threads: int = 8, counts: bool = False):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
         n = len(rays)
         hits = np.zeros(n, HIT_DTYPE)
         nv = np.zeros(n, np.uint32)
@@ -399,6 +412,7 @@ class OracleBvh:
         return (hits, nv, nt) if counts else hits
 
     def trace_any(self, rays: np.ndarray, threads: int = 8, counts: bool = False):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
         n = len(rays)
         occ = np.zeros(n, np.int32)
         nv = np.zeros(n, np.uint32)
@@ -409,6 +423,7 @@ class OracleBvh:
 
 
 def brute_closest(scene: Scene, rays: np.ndarray) -> np.ndarray:
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
     hits = np.zeros(len(rays), HIT_DTYPE)
     lib().rto_brute_closest(C.byref(scene), ptr(rays), len(rays), ptr(hits))
     return hits

@@ -133,6 +133,19 @@ typedef struct {
     uint32_t pad;
 } rto_bvh_node;
 
+/* 4-wide quantized node collapsed from the binary LBVH (64 B): origin and
+ * per-axis quantum exponents (biased by 127) + child count in meta, child
+ * references (leaf bit | sorted leaf slot, wide node index, or empty), and
+ * per-axis child bounds in quanta (byte k of each word = child k) */
+typedef struct {
+    float origin[3];
+    uint32_t meta;            /* ex | ey << 8 | ez << 16 | count << 24 */
+    uint32_t child[4];
+    uint32_t qlo[3];
+    uint32_t qhi[3];
+    uint32_t pad[2];
+} rto_wide_node;
+
 typedef struct {
     int32_t prim_count;       /* spheres + quads + triangles */
     int32_t node_count;       /* max(prim_count - 1, 1) if prim_count > 0 */
@@ -140,6 +153,10 @@ typedef struct {
     uint32_t* sorted_prims;   /* leaf position -> global primitive id */
     uint32_t* morton;         /* sorted Morton codes */
     int32_t max_depth;        /* internal levels on the deepest root-leaf path */
+    int32_t wide_count;       /* 4-wide nodes */
+    rto_wide_node* wide;
+    int32_t wide_levels;      /* levels of the 4-wide tree */
+    int32_t pad;
 } rto_bvh;
 
 typedef struct { float o[3]; float tmin; float d[3]; float tmax; } rto_ray;

@@ -31,7 +31,7 @@ EXPORTED = [
     "rt_frame_download", "rt_frame_device_view", "rt_reset_accumulation", "rt_reset_sequence",
     "rt_restir_diagnostics", "rt_renderer_set_tuning", "rt_renderer_get_tuning",
     "rt_scene_generate", "rt_scene_generated_free", "rt_restir_download",
-    "rt_scene_generate_rig", "rt_frame_download_guides",
+    "rt_scene_generate_rig", "rt_frame_download_guides", "rt_bvh_download_wide",
 ]
 
 
@@ -135,6 +135,12 @@ class Scene:
         mc = np.zeros(n, np.uint32)
         check(lib().rt_bvh_download(self.h, ptr(nodes), ptr(sp), ptr(mc)))
         return nodes, sp, mc
+
+    def download_wide(self) -> np.ndarray:
+        out = np.zeros(self.stats.wide_node_count, abi.WIDE_NODE_DTYPE)
+        if len(out):
+            check(lib().rt_bvh_download_wide(self.h, ptr(out)))
+        return out
 
     def trace_closest(self, rays: np.ndarray) -> np.ndarray:
         rays = np.ascontiguousarray(rays, RAY_DTYPE)

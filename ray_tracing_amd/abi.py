@@ -94,7 +94,12 @@ class Profile(C.Structure):
 
 class BvhStats(C.Structure):
     _fields_ = [("prim_count", C.c_int32), ("node_count", C.c_int32), ("max_depth", C.c_int32),
-                ("pad", C.c_int32), ("build_ms", C.c_double)]
+                ("wide_node_count", C.c_int32), ("build_ms", C.c_double),
+                ("wide_levels", C.c_int32), ("pad", C.c_int32)]
+
+
+WIDE_NODE_DTYPE = np.dtype([("origin", "<f4", 3), ("meta", "<u4"), ("child", "<u4", 4),
+                            ("qlo", "<u4", 3), ("qhi", "<u4", 3), ("pad", "<u4", 2)])
 
 
 class FrameTiming(C.Structure):
@@ -159,6 +164,7 @@ def declare(L):
     L.rt_scene_destroy.argtypes = [vp]
     L.rt_bvh_build.argtypes = [vp, P(BvhStats)]
     L.rt_bvh_download.argtypes = [vp, vp, vp, vp]
+    L.rt_bvh_download_wide.argtypes = [vp, vp]
     L.rt_trace_closest.argtypes = [vp, vp, C.c_int64, vp]
     L.rt_trace_any.argtypes = [vp, vp, C.c_int64, vp]
     L.rt_trace_closest_device.argtypes = [vp, vp, C.c_int64, vp, P(C.c_double)]

@@ -121,6 +121,9 @@ struct rt_scene {
     int32_t prim_count = 0;
     int32_t node_count = 0;
     int32_t max_depth = 0;
+    int32_t wide_count = 0;
+    int32_t wide_levels = 0;
+    DevBuf<float4> nodes4;
     DevBuf<float4> nodes;
     DevBuf<float4> leaves;
     DevBuf<uint32_t> sorted;
@@ -153,7 +156,9 @@ struct rt_scene {
         }
         v.analytic = analytic.p;
         v.analytic_count = analytic_count;
-        v.stack_depth = std::max(max_depth, 1);
+        v.nodes4 = nodes4.p;
+        v.node4_count = bvh_ready ? wide_count : 0;
+        v.stack4_entries = std::max(3 * wide_levels, 1);
         v.a_sel = a_sel.p;
         v.a_cdf = a_cdf.p;
         return v;
@@ -206,6 +211,7 @@ struct rt_renderer {
     bool restir_last = false;
     int last_reservoirs = -1;
     DevBuf<uint32_t> counters;
+    DevBuf<uint2> spill;  // traversal stack entries beyond the LDS part
     DevBuf<float4> acc_beauty, acc_normal, acc_albedo;
     size_t path_capacity = 0;
     size_t pixel_capacity = 0;
@@ -244,6 +250,7 @@ struct rt_renderer {
         b.restir_d = restir_d.p;
         b.restir_thr = restir_thr.p;
         b.counters = counters.p;
+        b.spill = spill.p;
         b.acc_beauty = acc_beauty.p;
         b.acc_normal = acc_normal.p;
         b.acc_albedo = acc_albedo.p;
@@ -454,7 +461,8 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
                    && (d->analytic_light_count == 0 || d->analytic_lights),
         RT_ERROR_INVALID_ARGUMENT, "scene arrays must be non-null when their count is positive");
     const int64_t prims = (int64_t)d->sphere_count + d->quad_count + d->triangle_count;
-    RT_REQUIRE(prims < (int64_t(1) << 30), RT_ERROR_INVALID_ARGUMENT, "scene primitive count exceeds 2^30");
+    // leaf ranges address 28-bit sorted slots
+    RT_REQUIRE(prims < (int64_t(1) << 28), RT_ERROR_INVALID_ARGUMENT, "scene primitive count exceeds 2^28");
     for (int i = 0; i < d->openpbr_count; ++i) {
         const rt_openpbr_material& m = d->openpbr[i];
         RT_REQUIRE(m.transmission_weight <= 0.0f && m.subsurface_weight <= 0.0f && m.fuzz_weight <= 0.0f
@@ -579,12 +587,47 @@ rt_status rt_bvh_build(rt_scene* s, rt_bvh_stats* stats) {
         s->max_depth = static_cast<int32_t>(hcb[6]);
         RT_REQUIRE(s->max_depth >= 1 && s->max_depth <= 64, RT_ERROR_HIP,
             "LBVH build produced an invalid depth " + std::to_string(s->max_depth));
+        // collapse to the 4-wide quantized tree every ray traverses
+        const int nbin = s->node_count;
+        DevBuf<uint32_t> is_wide, index, f0, f1, fc;
+        RT_HIP(is_wide.reserve((size_t)nbin));
+        RT_HIP(index.reserve((size_t)nbin));
+        RT_HIP(f0.reserve((size_t)nbin));
+        RT_HIP(f1.reserve((size_t)nbin));
+        RT_HIP(fc.reserve(2));
+        RT_HIP(s->nodes4.reserve((size_t)nbin * 4));
+        rtk::WideScratch ws{};
+        ws.is_wide = is_wide.p;
+        ws.index = index.p;
+        ws.front[0] = f0.p;
+        ws.front[1] = f1.p;
+        ws.counts = fc.p;
+        hipEvent_t w0, w1;
+        RT_HIP(hipEventCreate(&w0));
+        RT_HIP(hipEventCreate(&w1));
+        RT_HIP(hipEventRecord(w0, st));
+        int wc = 0, wl = 0;
+        RT_HIP(rtk::bvh_collapse_wide(s->nodes.p, nbin, ws, s->nodes4.p, &wc, &wl, st));
+        RT_HIP(hipEventRecord(w1, st));
+        RT_HIP(hipStreamSynchronize(st));
+        float wms = 0.0f;
+        RT_HIP(hipEventElapsedTime(&wms, w0, w1));
+        (void)hipEventDestroy(w0);
+        (void)hipEventDestroy(w1);
+        build_ms += wms;
+        s->wide_count = wc;
+        s->wide_levels = wl;
+        RT_REQUIRE(wc >= 1 && wl >= 1 && wl <= 64, RT_ERROR_HIP,
+            "4-wide collapse produced an invalid tree (" + std::to_string(wc) + " nodes, "
+                + std::to_string(wl) + " levels)");
     }
     s->bvh_ready = true;
     if (stats) {
         stats->prim_count = n;
         stats->node_count = s->node_count;
         stats->max_depth = s->max_depth;
+        stats->wide_node_count = s->wide_count;
+        stats->wide_levels = s->wide_levels;
         stats->pad = 0;
         stats->build_ms = build_ms;
     }
@@ -611,6 +654,39 @@ rt_status rt_bvh_download(rt_scene* s, rt_bvh_node* nodes, uint32_t* sorted_prim
     return RT_OK;
 }
 
+rt_status rt_bvh_download_wide(rt_scene* s, void* nodes4) {
+    RT_REQUIRE(s != nullptr && nodes4 != nullptr, RT_ERROR_INVALID_ARGUMENT, "null scene or output");
+    RT_REQUIRE(s->bvh_ready, RT_ERROR_NOT_READY, "rt_bvh_download_wide requires rt_bvh_build() first");
+    RT_HIP(hipSetDevice(s->dev->ordinal));
+    if (s->prim_count > 0) {
+        RT_HIP(hipMemcpyAsync(nodes4, s->nodes4.p, 64u * (size_t)s->wide_count, hipMemcpyDeviceToHost,
+            s->dev->stream));
+    }
+    RT_HIP(hipStreamSynchronize(s->dev->stream));
+    return RT_OK;
+}
+
+namespace {
+
+// fetch cursor + stack spill of the test-hook trace kernels (one launch)
+struct TraceScratch {
+    DevBuf<uint32_t> fetch;
+    DevBuf<uint2> spill;
+    int grid = 1;
+    hipError_t prepare(const rt_scene* s, int64_t n, hipStream_t st) {
+        const int64_t blocks = (n + rtk::kTraceBlock - 1) / rtk::kTraceBlock;
+        grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)s->dev->cu_count * 8));
+        const rtk::SceneView v = s->view();
+        const size_t entries = (size_t)std::max(v.stack4_entries - rtk::kStackLds, 1);
+        hipError_t e = fetch.reserve(1);
+        if (e == hipSuccess) e = spill.reserve(entries * (size_t)grid * rtk::kTraceBlock);
+        if (e == hipSuccess) e = hipMemsetAsync(fetch.p, 0, sizeof(uint32_t), st);
+        return e;
+    }
+};
+
+}  // namespace
+
 static rt_status trace_host(rt_scene* s, const rt_ray* rays, int64_t n, void* out, bool any) {
     RT_REQUIRE(s != nullptr, RT_ERROR_INVALID_ARGUMENT, "null scene");
     RT_REQUIRE(n >= 0 && (n == 0 || (rays && out)), RT_ERROR_INVALID_ARGUMENT, "invalid ray arrays");
@@ -618,19 +694,22 @@ static rt_status trace_host(rt_scene* s, const rt_ray* rays, int64_t n, void* ou
     if (n == 0) return RT_OK;
     RT_HIP(hipSetDevice(s->dev->ordinal));
     hipStream_t st = s->dev->stream;
+    RT_REQUIRE(n < (int64_t(1) << 31), RT_ERROR_INVALID_ARGUMENT, "at most 2^31 - 1 rays per call");
     DevBuf<rt_ray> dr;
     RT_HIP(upload(dr, rays, (size_t)n, st));
     const rtk::SceneView v = s->view();
+    TraceScratch ts;
+    RT_HIP(ts.prepare(s, n, st));
     if (any) {
         DevBuf<int32_t> dh;
         RT_HIP(dh.reserve((size_t)n));
-        RT_HIP(rtk::trace_any_launch(v, v.stack_depth, dr.p, n, dh.p, st));
+        RT_HIP(rtk::trace_any_launch(v, ts.grid, dr.p, n, dh.p, ts.fetch.p, ts.spill.p, st));
         RT_HIP(hipMemcpyAsync(out, dh.p, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st));
         RT_HIP(hipStreamSynchronize(st));
     } else {
         DevBuf<rt_hit> dh;
         RT_HIP(dh.reserve((size_t)n));
-        RT_HIP(rtk::trace_closest_launch(v, v.stack_depth, dr.p, n, dh.p, st));
+        RT_HIP(rtk::trace_closest_launch(v, ts.grid, dr.p, n, dh.p, ts.fetch.p, ts.spill.p, st));
         RT_HIP(hipMemcpyAsync(out, dh.p, sizeof(rt_hit) * (size_t)n, hipMemcpyDeviceToHost, st));
         RT_HIP(hipStreamSynchronize(st));
     }
@@ -655,9 +734,12 @@ rt_status rt_trace_closest_device(rt_scene* s, const rt_ray* d_rays, int64_t n, 
     hipEvent_t e0, e1;
     RT_HIP(hipEventCreate(&e0));
     RT_HIP(hipEventCreate(&e1));
+    RT_REQUIRE(n < (int64_t(1) << 31), RT_ERROR_INVALID_ARGUMENT, "at most 2^31 - 1 rays per call");
     const rtk::SceneView v = s->view();
+    TraceScratch ts;
+    RT_HIP(ts.prepare(s, n, st));
     RT_HIP(hipEventRecord(e0, st));
-    RT_HIP(rtk::trace_closest_launch(v, v.stack_depth, d_rays, n, d_hits, st));
+    RT_HIP(rtk::trace_closest_launch(v, ts.grid, d_rays, n, d_hits, ts.fetch.p, ts.spill.p, st));
     RT_HIP(hipEventRecord(e1, st));
     RT_HIP(hipEventSynchronize(e1));
     float ms = 0.0f;
@@ -678,14 +760,14 @@ rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out)
     scene->refs.fetch_add(1);
     r->tuning.wavefront_paths = 1 << 22;
     r->tuning.trace_block = 128;
-    r->tuning.trace_blocks_per_cu = 8;
+    r->tuning.trace_blocks_per_cu = 10;
     r->tuning.use_graph = 0;
     hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&r->host_cams), sizeof(rt_camera) * 8,
         hipHostMallocDefault);
     if (e == hipSuccess) e = r->cams.reserve(8);
     if (e == hipSuccess) e = hipEventCreate(&r->post_ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&r->post_ev[1]);
-    if (e == hipSuccess) e = r->counters.reserve(16);
+    if (e == hipSuccess) e = r->counters.reserve(32);
     if (e != hipSuccess) {
         delete r;
         scene_release(scene);
@@ -840,6 +922,11 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
     const int block = r->tuning.trace_block;
     const rtk::StageLaunch trace_l{r->dev->cu_count * r->tuning.trace_blocks_per_cu, block};
     const rtk::StageLaunch shade_l{r->dev->cu_count * 4, 256};
+    {
+        const size_t entries = (size_t)std::max(sv.stack4_entries - rtk::kStackLds, 1);
+        RT_HIP(r->spill.reserve(entries * (size_t)trace_l.grid * (size_t)block));
+        b.spill = r->spill.p;
+    }
 
     const int chunks = (spp + layers - 1) / layers;
     const size_t stage_events = (size_t)chunks * (size_t)max_b * 6 + 4;
@@ -869,13 +956,13 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
         RT_HIP(rtk::wf_raygen(cv, fp, b, paths, st));
         for (int bounce = 0; bounce < max_b; ++bounce) {
             RT_HIP(mark());
-            RT_HIP(rtk::wf_extend(sv, sv.stack_depth, b, bounce, r->tuning.count_traversal != 0, trace_l, st));
+            RT_HIP(rtk::wf_extend(sv, b, bounce, r->tuning.count_traversal != 0, trace_l, st));
             RT_HIP(mark());
             RT_HIP(rtk::wf_shade(sv, cv, fp, b, bounce, shade_l, st));
             if (fp.restir != 0 && bounce == 0)
-                RT_HIP(rtk::wf_restir(sv, sv.stack_depth, cv, fp, rv, b, trace_l, st));
+                RT_HIP(rtk::wf_restir(sv, cv, fp, rv, b, trace_l, st));
             RT_HIP(mark());
-            RT_HIP(rtk::wf_connect(sv, sv.stack_depth, b, bounce, trace_l, st));
+            RT_HIP(rtk::wf_connect(sv, b, bounce, trace_l, st));
             RT_HIP(mark());
         }
         RT_HIP(rtk::wf_accumulate(fp, b, st));

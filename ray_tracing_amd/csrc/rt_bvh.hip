@@ -282,7 +282,7 @@ __global__ void k_karras(const uint32_t* keys, int n, uint4* nodes_u, uint32_t* 
     }
     nodes_u[4 * i + 3].x = c0;
     nodes_u[4 * i + 3].y = c1;
-    nodes_u[4 * i + 3].w = 0u;
+    nodes_u[4 * i + 3].w = (uint32_t)(last - first + 1);  // leaves under this node
     if (i == 0) nodes_u[3].z = kEmpty;
 }
 
@@ -374,7 +374,7 @@ __global__ void k_single(SceneView s, const uint32_t* sorted, const float4* pbox
     nodes[0] = make_float4(l.x, l.y, l.z, h.x);
     nodes[1] = make_float4(h.y, h.z, inf, inf);
     nodes[2] = make_float4(inf, -inf, -inf, -inf);
-    reinterpret_cast<uint4*>(nodes)[3] = make_uint4(kLeafBit, kEmpty, kEmpty, 0u);
+    reinterpret_cast<uint4*>(nodes)[3] = make_uint4(kLeafBit, kEmpty, kEmpty, 1u);
 }
 
 __global__ void k_depth(int n, const uint32_t* leaf_parent, const float4* nodes, uint32_t* cb) {
@@ -392,9 +392,207 @@ __global__ void k_depth(int n, const uint32_t* leaf_parent, const float4* nodes,
     atomicMax(&cb[6], depth);
 }
 
+// ---- collapse to the 4-wide quantized BVH
+//
+// expand(b): the children of 4-wide node rooted at binary node b.  Start
+// from b's two children and, while fewer than 4, replace the internal child
+// of largest surface area (first one on ties) by its two children, in place.
+// Every step reads boxes stored in binary parents, so the result is a pure
+// function of the binary node array.
+struct Wide {
+    uint32_t ref[kWide];
+    float lo[kWide][3];
+    float hi[kWide][3];
+    int count;
+};
+
+__device__ __forceinline__ float box_area(const float* lo, const float* hi) {
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__device__ void load_binary_children(const float4* nodes, uint32_t b, uint32_t* ref, float (*lo)[3],
+    float (*hi)[3]) {
+    const float* nf = reinterpret_cast<const float*>(nodes + 4u * b);
+    const uint4 refs = reinterpret_cast<const uint4*>(nodes)[4u * b + 3u];
+    ref[0] = refs.x;
+    ref[1] = refs.y;
+    for (int c = 0; c < 2; ++c) {
+        for (int a = 0; a < 3; ++a) {
+            lo[c][a] = nf[c * 6 + a];
+            hi[c][a] = nf[c * 6 + 3 + a];
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t subtree_leaves(const float4* nodes, uint32_t ref) {
+    return (ref & kLeafBit) ? 1u : reinterpret_cast<const uint4*>(nodes)[4u * ref + 3u].w;
+}
+
+// leftmost leaf slot under a binary reference
+__device__ __forceinline__ uint32_t first_leaf(const float4* nodes, uint32_t ref) {
+    while (!(ref & kLeafBit)) ref = reinterpret_cast<const uint4*>(nodes)[4u * ref + 3u].x;
+    return ref & ~kLeafBit;
+}
+
+__device__ void expand_wide(const float4* nodes, uint32_t b, Wide& w) {
+    uint32_t r2[2];
+    float l2[2][3], h2[2][3];
+    load_binary_children(nodes, b, r2, l2, h2);
+    w.count = 0;
+    for (int c = 0; c < 2; ++c) {
+        if (r2[c] == kEmpty) continue;
+        w.ref[w.count] = r2[c];
+        for (int a = 0; a < 3; ++a) { w.lo[w.count][a] = l2[c][a]; w.hi[w.count][a] = h2[c][a]; }
+        ++w.count;
+    }
+    while (w.count < kWide) {
+        int best = -1;
+        float best_area = -1.0f;
+        for (int j = 0; j < w.count; ++j) {
+            if (subtree_leaves(nodes, w.ref[j]) <= kLeafMax) continue;  // becomes a leaf range
+            const float ar = box_area(w.lo[j], w.hi[j]);
+            if (ar > best_area) { best_area = ar; best = j; }
+        }
+        if (best < 0) break;
+        load_binary_children(nodes, w.ref[best], r2, l2, h2);
+        for (int j = w.count; j > best + 1; --j) {
+            w.ref[j] = w.ref[j - 1];
+            for (int a = 0; a < 3; ++a) { w.lo[j][a] = w.lo[j - 1][a]; w.hi[j][a] = w.hi[j - 1][a]; }
+        }
+        for (int c = 0; c < 2; ++c) {
+            w.ref[best + c] = r2[c];
+            for (int a = 0; a < 3; ++a) { w.lo[best + c][a] = l2[c][a]; w.hi[best + c][a] = h2[c][a]; }
+        }
+        ++w.count;
+    }
+}
+
+// one level of the top-down walk: every frontier node marks its internal
+// children as 4-wide nodes and appends them to the next frontier
+__global__ void k_wide_level(const float4* nodes, const uint32_t* front_in, const uint32_t* n_in,
+    uint32_t* front_out, uint32_t* n_out, uint32_t* is_wide) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_in) return;
+    Wide w;
+    expand_wide(nodes, front_in[i], w);
+    for (int k = 0; k < w.count; ++k) {
+        if (subtree_leaves(nodes, w.ref[k]) <= kLeafMax) continue;
+        is_wide[w.ref[k]] = 1u;
+        front_out[atomicAdd(n_out, 1u)] = w.ref[k];
+    }
+}
+
+__global__ void k_wide_root(uint32_t* front, uint32_t* counts, uint32_t* is_wide) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    front[0] = 0u;
+    counts[0] = 1u;
+    counts[1] = 0u;
+    is_wide[0] = 1u;
+}
+
+// quantum exponent: smallest e (from 2^e >= extent / 255, then fixed up) with
+// 255 * 2^e >= extent, all in double so the CPU build gets the same bits
+__device__ int wide_exponent(double extent) {
+    if (!(extent > 0.0)) return -100;
+    int e;
+    (void)frexp(extent / 255.0, &e);
+    while (255.0 * ldexp(1.0, e) < extent) ++e;
+    if (e < -100) e = -100;
+    if (e > 100) e = 100;
+    return e;
+}
+
+__global__ void k_wide_write(const float4* nodes, int nbin, const uint32_t* is_wide,
+    const uint32_t* index, float4* nodes4) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbin || is_wide[b] == 0u) return;
+    Wide w;
+    expand_wide(nodes, (uint32_t)b, w);
+    float org[3];
+    double ext[3];
+    for (int a = 0; a < 3; ++a) {
+        float lo = __builtin_inff(), hi = -__builtin_inff();
+        for (int k = 0; k < w.count; ++k) { lo = sel_min(lo, w.lo[k][a]); hi = sel_max(hi, w.hi[k][a]); }
+        org[a] = lo;
+        ext[a] = (double)hi - (double)lo;
+    }
+    uint32_t meta = (uint32_t)w.count << 24;
+    uint32_t qlo[3] = {0u, 0u, 0u}, qhi[3] = {0u, 0u, 0u};
+    for (int a = 0; a < 3; ++a) {
+        const int e = wide_exponent(ext[a]);
+        meta |= (uint32_t)(e + 127) << (8 * a);
+        const double sc = ldexp(1.0, e), inv = ldexp(1.0, -e);
+        for (int k = 0; k < w.count; ++k) {
+            double ql = floor(((double)w.lo[k][a] - (double)org[a]) * inv);
+            if (ql < 0.0) ql = 0.0;
+            if (ql > 255.0) ql = 255.0;
+            while (ql > 0.0 && (double)org[a] + ql * sc > (double)w.lo[k][a]) ql -= 1.0;
+            double qh = ceil(((double)w.hi[k][a] - (double)org[a]) * inv);
+            if (qh < 0.0) qh = 0.0;
+            if (qh > 255.0) qh = 255.0;
+            while (qh < 255.0 && (double)org[a] + qh * sc < (double)w.hi[k][a]) qh += 1.0;
+            qlo[a] |= (uint32_t)ql << (8 * k);
+            qhi[a] |= (uint32_t)qh << (8 * k);
+        }
+    }
+    uint32_t ref[kWide];
+    for (int k = 0; k < kWide; ++k) {
+        if (k >= w.count) {
+            ref[k] = kEmpty;
+            continue;
+        }
+        const uint32_t nl = subtree_leaves(nodes, w.ref[k]);
+        ref[k] = nl <= kLeafMax ? wide_leaf(first_leaf(nodes, w.ref[k]), nl) : index[w.ref[k]];
+    }
+    float4* out = nodes4 + 4u * index[b];
+    out[0] = make_float4(org[0], org[1], org[2], __uint_as_float(meta));
+    reinterpret_cast<uint4*>(out)[1] = make_uint4(ref[0], ref[1], ref[2], ref[3]);
+    reinterpret_cast<uint4*>(out)[2] = make_uint4(qlo[0], qlo[1], qlo[2], qhi[0]);
+    reinterpret_cast<uint4*>(out)[3] = make_uint4(qhi[1], qhi[2], 0u, 0u);
+}
+
 inline int blocks_for(int n, int b) { return (n + b - 1) / b; }
 
 }  // namespace
+
+hipError_t bvh_collapse_wide(const float4* nodes, int nbin, WideScratch& w, float4* nodes4,
+    int* wide_count, int* wide_levels, hipStream_t st) {
+    *wide_count = 0;
+    *wide_levels = 0;
+    if (nbin <= 0) return hipSuccess;
+    hipError_t e = hipMemsetAsync(w.is_wide, 0, sizeof(uint32_t) * (size_t)nbin, st);
+    if (e != hipSuccess) return e;
+    k_wide_root<<<1, 64, 0, st>>>(w.front[0], w.counts, w.is_wide);
+    int cur = 0, levels = 0;
+    uint32_t host[2] = {1u, 0u};
+    while (true) {
+        ++levels;
+        e = hipMemsetAsync(w.counts + 1 - cur, 0, sizeof(uint32_t), st);
+        if (e != hipSuccess) return e;
+        k_wide_level<<<blocks_for((int)host[cur], 256), 256, 0, st>>>(nodes, w.front[cur],
+            w.counts + cur, w.front[cur ^ 1], w.counts + (cur ^ 1), w.is_wide);
+        e = hipMemcpyAsync(host, w.counts, sizeof(host), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+        cur ^= 1;
+        if (host[cur] == 0u) break;
+    }
+    // compact numbering: exclusive scan of the marks
+    e = hipMemcpyAsync(w.index, w.is_wide, sizeof(uint32_t) * (size_t)nbin, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+    k_scan<<<1, 1024, 0, st>>>(w.index, nbin);
+    uint32_t last[2] = {0u, 0u};
+    e = hipMemcpyAsync(&last[0], w.index + (nbin - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&last[1], w.is_wide + (nbin - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    *wide_count = (int)(last[0] + last[1]);
+    *wide_levels = levels;
+    k_wide_write<<<blocks_for(nbin, 256), 256, 0, st>>>(nodes, nbin, w.is_wide, w.index, nodes4);
+    return hipGetLastError();
+}
 
 hipError_t bvh_build(const SceneView& s, int n, BvhBuildScratch& w, float4* nodes, float4* leaves,
     uint32_t* sorted_out, uint32_t* morton_out, hipStream_t st) {

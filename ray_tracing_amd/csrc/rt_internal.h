@@ -30,10 +30,13 @@ struct LightRec {
 };
 
 struct SceneView {
-    const float4* nodes;
+    const float4* nodes;   // binary LBVH (build intermediate, test downloads)
     const float4* leaves;
     int32_t node_count;
     int32_t prim_count;
+    const float4* nodes4;  // 4-wide quantized BVH traversed by every ray
+    int32_t node4_count;
+    int32_t stack4_entries; // deepest deferred-node stack a 4-wide traversal can need
     const rt_sphere* spheres;
     const rt_quad* quads;
     const rt_triangle* triangles;
@@ -54,6 +57,27 @@ struct SceneView {
     const float* a_sel;   // analytic light selection pdf
     const float* a_cdf;   // analytic light CDF
 };
+
+// 4-wide quantized BVH node, 64 B = 4 x float4, collapsed from the binary
+// LBVH (each node absorbs binary descendants, largest surface area first,
+// until it has 4 children):
+//   v0 = origin.xyz (min corner of the children), meta = ex | ey << 8 |
+//        ez << 16 | count << 24 (per-axis quantum 2^(e - 127))
+//   v1 = child references: a leaf range kLeafBit | (count - 1) << 28 |
+//        first sorted leaf slot (1..kLeafMax primitives), a 4-wide node
+//        index, or kEmpty
+//   v2 = qlo.x, qlo.y, qlo.z, qhi.x   (byte k of each word = child k)
+//   v3 = qhi.y, qhi.z, 0, 0
+// A child's box is origin + [qlo, qhi] * quantum per axis, rounded outward,
+// so it always contains the exact box.
+constexpr int kWide = 4;
+// binary subtrees with at most this many primitives become one leaf range
+constexpr uint32_t kLeafMax = 4;
+__host__ __device__ constexpr uint32_t wide_leaf(uint32_t first, uint32_t count) {
+    return 0x80000000u | ((count - 1u) << 28) | first;
+}
+// LDS part of the per-lane traversal stack; deeper entries spill to HBM
+constexpr int kStackLds = 8;
 
 // RestirReservoir (src/common/restir_di.h:36-48): 84 B per pixel
 struct Reservoir {

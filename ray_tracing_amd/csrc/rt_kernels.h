@@ -21,11 +21,25 @@ struct BvhBuildScratch {
 hipError_t bvh_build(const SceneView& s, int n, BvhBuildScratch& w, float4* nodes, float4* leaves,
     uint32_t* sorted_out, uint32_t* morton_out, hipStream_t st);
 
+struct WideScratch {
+    uint32_t* is_wide;   // per binary node: roots a 4-wide node
+    uint32_t* index;     // exclusive scan of is_wide: 4-wide node index
+    uint32_t* front[2];  // top-down frontiers
+    uint32_t* counts;    // frontier sizes
+};
+// collapse the binary LBVH (nbin nodes, root 0) into the 4-wide quantized
+// BVH; reports its node count and number of levels (synchronises `st`)
+hipError_t bvh_collapse_wide(const float4* nodes, int nbin, WideScratch& w, float4* nodes4,
+    int* wide_count, int* wide_levels, hipStream_t st);
+
 // test-hook traversal over device arrays of rt_ray / rt_hit
-hipError_t trace_closest_launch(const SceneView& s, int stack_depth, const rt_ray* rays,
-    int64_t n, rt_hit* hits, hipStream_t st);
-hipError_t trace_any_launch(const SceneView& s, int stack_depth, const rt_ray* rays, int64_t n,
-    int32_t* occluded, hipStream_t st);
+// (persistent grid of `grid` blocks of kTraceBlock threads; `fetch` is one
+// zeroed counter; `spill` holds (stack entries - kStackLds) x grid x kTraceBlock)
+constexpr int kTraceBlock = 128;
+hipError_t trace_closest_launch(const SceneView& s, int grid, const rt_ray* rays, int64_t n,
+    rt_hit* hits, uint32_t* fetch, uint2* spill, hipStream_t st);
+hipError_t trace_any_launch(const SceneView& s, int grid, const rt_ray* rays, int64_t n,
+    int32_t* occluded, uint32_t* fetch, uint2* spill, hipStream_t st);
 
 // ---- wavefront pipeline
 struct WavefrontBuffers {
@@ -53,7 +67,9 @@ struct WavefrontBuffers {
     float4* restir_o;   // incoming ray origin
     float4* restir_d;   // incoming ray direction, w = hit t
     float4* restir_thr; // throughput before BSDF sampling, w = bits(prim)
-    uint32_t* counters; // [0]/[1] queue counts, [2] shadow count, [4..15] u64 ray / traversal stats
+    uint32_t* counters; // [0]/[1] queue counts, [2] shadow count, [4..15] u64 ray / traversal
+                        // stats, [16..18] ray-fetch cursors of the persistent trace kernels
+    uint2* spill;       // traversal stack spill: (stack entries - kStackLds) x trace threads
     // per pixel (all cameras)
     float4* acc_beauty;
     float4* acc_normal; // xyz sum, w = aux count
@@ -80,13 +96,13 @@ struct StageLaunch {
 hipError_t wf_reset_counters(uint32_t* counters, hipStream_t st);
 hipError_t wf_raygen(const CameraView& cams, const FrameParams& fp, WavefrontBuffers& b,
     int paths, hipStream_t st);
-hipError_t wf_extend(const SceneView& s, int stack_depth, WavefrontBuffers& b, int bounce,
-    bool count, StageLaunch l, hipStream_t st);
+hipError_t wf_extend(const SceneView& s, WavefrontBuffers& b, int bounce, bool count,
+    StageLaunch l, hipStream_t st);
 hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParams& fp,
     WavefrontBuffers& b, int bounce, StageLaunch l, hipStream_t st);
-hipError_t wf_connect(const SceneView& s, int stack_depth, WavefrontBuffers& b, int bounce,
-    StageLaunch l, hipStream_t st);
-hipError_t wf_restir(const SceneView& s, int stack_depth, const CameraView& cams,
+hipError_t wf_connect(const SceneView& s, WavefrontBuffers& b, int bounce, StageLaunch l,
+    hipStream_t st);
+hipError_t wf_restir(const SceneView& s, const CameraView& cams,
     const FrameParams& fp, const RestirView& rv, WavefrontBuffers& b, StageLaunch l,
     hipStream_t st);
 hipError_t wf_accumulate(const FrameParams& fp, WavefrontBuffers& b, hipStream_t st);

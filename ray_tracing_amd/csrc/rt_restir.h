@@ -179,9 +179,9 @@ __device__ inline bool rs_previous_pixel(const RestirView& rv, int cam, int widt
 }
 
 // restir_target_at_previous_surface (programs.cu:2210-2247)
+template <int BLOCK>
 __device__ inline float rs_target_at_source(const SceneView& s, const RestirView& rv, int cam,
-    const Reservoir& src, const Reservoir& sel, bool bsdf_avail, uint32_t* stack_node,
-    float* stack_t, int stride) {
+    const Reservoir& src, const Reservoir& sel, bool bsdf_avail, const TraceStack& st) {
     if (rv.prev_valid[cam] == 0) return 0.0f;
     const rt_camera& pc = rv.prev_cams[cam];
     const f3 sp = mk3(src.position[0], src.position[1], src.position[2]);
@@ -190,8 +190,7 @@ __device__ inline float rs_target_at_source(const SceneView& s, const RestirView
     const float dist = len3(off);
     if (!isfinite(dist) || dist <= kRayEps) return 0.0f;
     const f3 dir = off * (1.0f / dist);
-    const TraceResult tr = traverse<false>(s, origin, dir, kRayEps, dist + 0.05f, stack_node,
-        stack_t, stride);
+    const TraceResult tr = trace_wide<BLOCK, false>(s, origin, dir, kRayEps, dist + 0.05f, st);
     if (!tr.found) return 0.0f;
     HitInfo h;
     make_hit(s, origin, dir, tr.t, (int)tr.prim, h);
@@ -218,10 +217,10 @@ struct RestirOut {
 // accumulate_restir_analytic_direct_light (programs.cu:2315-2449) up to the
 // shadow ray, which the connect stage traces.  `base` is the camera's first
 // pixel in the reservoir buffers, which hold all cameras back to back.
+template <int BLOCK>
 __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& rv,
     const HitInfo& h, f3 ray_d, int cam, int base, int x, int y, int width, int height, uint32_t pixel,
-    uint32_t sample, uint32_t stream, bool bsdf_avail, Reservoir* out, uint32_t* stack_node,
-    float* stack_t, int stride) {
+    uint32_t sample, uint32_t stream, bool bsdf_avail, Reservoir* out, const TraceStack& st) {
     RestirOut ro{};
     ro.shadow = false;
     if (h.mat_type == RT_MAT_DIFFUSE_LIGHT || s.analytic_count <= 0) return ro;
@@ -317,8 +316,8 @@ __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& r
         float sst = r.selected_target;
         float sum = r.selected_target * (float)canonical;
         for (int i = 0; i < nsrc; ++i) {
-            const float t = rs_target_at_source(s, rv, cam, rv.history[src_idx[i]], r, bsdf_avail,
-                stack_node, stack_t, stride);
+            const float t = rs_target_at_source<BLOCK>(s, rv, cam, rv.history[src_idx[i]], r,
+                bsdf_avail, st);
             sum += t * (float)mults[i];
             if (selected_source == i) sst = t;
         }

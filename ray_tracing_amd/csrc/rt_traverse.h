@@ -1,16 +1,17 @@
-// rt_traverse.h -- stack-in-LDS closest-hit and any-hit traversal of the
-// binary LBVH, with sphere / quad / triangle intersection.
+// rt_traverse.h -- closest-hit and any-hit traversal of the 4-wide quantized
+// BVH, with sphere / quad / triangle intersection.
 //
 // Replaces intersect_accelerated_surfaces + try_hit_* of the reference
 // (src/realtime/gpu/programs.cu:1421-1547, 1549-1652), which walks a
 // per-thread 64-entry private stack, tests one box per node fetch, runs
 // material evaluation for every candidate hit and has no any-hit early exit.
-// Here one 64-byte node fetch yields both child boxes; leaves hold one
-// primitive record (48 B) inline in Morton order; the deferred-node stack
-// (node index + entry distance) lives in LDS, lane-strided so that the 64
-// lanes of a wave hit 64 distinct banks; popped entries farther than the
-// current best hit are skipped.  Materials are evaluated once, after the
-// nearest hit, by the shade stage.
+// Here one 64-byte node fetch yields four quantized child boxes; leaves are
+// ranges of up to four 48-byte primitive records in Morton order; the
+// deferred-node stack keeps its top entries in LDS (lane-strided, so the 64
+// lanes of a wave touch distinct banks) and spills deeper ones to HBM;
+// popped entries farther than the current best hit are skipped; lanes that
+// finish refill from the ray queue.  Materials are evaluated once, after
+// the nearest hit, by the shade stage.
 //
 // Closest-hit result: the lexicographic minimum of (t, primitive id) over all
 // primitives with tmin < t < tmax, independent of traversal order.
@@ -21,40 +22,9 @@
 
 namespace rtk {
 
-struct SlabPre {
-    float ix, iy, iz;
-    float ox, oy, oz;  // origin * inverse direction
-};
-
 __device__ __forceinline__ float safe_inv(float d) {
     const float dv = fabsf(d) > 1e-12f ? d : copysignf(1e-12f, d);
     return 1.0f / dv;
-}
-
-__device__ __forceinline__ SlabPre slab_setup(f3 o, f3 d) {
-    SlabPre p;
-    p.ix = safe_inv(d.x);
-    p.iy = safe_inv(d.y);
-    p.iz = safe_inv(d.z);
-    p.ox = o.x * p.ix;
-    p.oy = o.y * p.iy;
-    p.oz = o.z * p.iz;
-    return p;
-}
-
-// box entry/exit by fma slabs; the exit distance is widened by 8e-7
-// relative so a box never culls a primitive hit lying on its surface
-__device__ __forceinline__ bool slab_hit(const SlabPre& p, float lx, float ly, float lz, float hx,
-    float hy, float hz, float tmin, float tbest, float* tnear) {
-    const float t0x = fmaf(lx, p.ix, -p.ox), t1x = fmaf(hx, p.ix, -p.ox);
-    const float t0y = fmaf(ly, p.iy, -p.oy), t1y = fmaf(hy, p.iy, -p.oy);
-    const float t0z = fmaf(lz, p.iz, -p.oz), t1z = fmaf(hz, p.iz, -p.oz);
-    const float nx = fminf(t0x, t1x), ny = fminf(t0y, t1y), nz = fminf(t0z, t1z);
-    const float fx = fmaxf(t0x, t1x), fy = fmaxf(t0y, t1y), fz = fmaxf(t0z, t1z);
-    const float tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
-    const float tf = fminf(fminf(fx, fy), fminf(fz, tbest)) * 1.0000008f;
-    *tnear = tn;
-    return tn <= tf;
 }
 
 // programs.cu:1421-1449
@@ -145,91 +115,268 @@ struct TraceResult {
     bool found;
 };
 
-// node visits and primitive tests of one traversal (algorithmic traffic)
+// box tests and primitive tests of one traversal (algorithmic traffic)
 struct TraverseCounts {
-    uint32_t visits;
+    uint32_t visits;  // child boxes tested (one canonical BVH node each)
     uint32_t tests;
     uint32_t sphere_tests;
 };
 
-// ANY = true: stop at the first hit in (tmin, tmax).
-// COUNT = true: report node visits / primitive tests in *cnt.
-// stack_node / stack_t: this lane's slot 0 in LDS; entry k at [k * stride].
-template <bool ANY, bool COUNT = false>
-__device__ __forceinline__ TraceResult traverse(const SceneView& s, f3 o, f3 d, float tmin,
-    float tmax, uint32_t* stack_node, float* stack_t, int stride, TraverseCounts* cnt = nullptr) {
-    TraceResult r{tmax, 0u, 0.0f, 0.0f, false};
-    uint32_t nv = 0, nt = 0, ns = 0;
-    if (s.prim_count > 0) {
-        const SlabPre p = slab_setup(o, d);
-        int sp = 0;
-        uint32_t node = 0;
-        const float4* nodes = s.nodes;
-        while (true) {
-            const float4 n0 = nodes[4u * node];
-            const float4 n1 = nodes[4u * node + 1u];
-            const float4 n2 = nodes[4u * node + 2u];
-            const uint4 n3 = reinterpret_cast<const uint4*>(nodes)[4u * node + 3u];
-            ++nv;
-            float tn0 = 0.0f, tn1 = 0.0f;
-            const bool h0 = n3.x != kEmpty
-                            && slab_hit(p, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, tmin, r.t, &tn0);
-            const bool h1 = n3.y != kEmpty
-                            && slab_hit(p, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tmin, r.t, &tn1);
-            const bool l0 = h0 && (n3.x & kLeafBit);
-            const bool l1 = h1 && (n3.y & kLeafBit);
-            if (l0) {
+// ---------------------------------------------------------------------------
+// 4-wide quantized BVH traversal.
+//
+// Order (shared with the CPU oracle so node / primitive counts agree): at a
+// node all valid child boxes are tested against the current best distance;
+// leaf ranges that pass are intersected in slot order; internal children
+// that pass are ranked by entry distance (ties: lower slot first), the
+// nearest is visited next and the others are pushed farthest first; popped
+// entries farther than the best hit are skipped.
+//
+// Per-lane deferred-node stack: entries 0..kStackLds-1 in LDS (lane-strided
+// uint2 = node, entry distance), deeper entries in a per-thread HBM spill
+// area.  A 4-wide tree of L levels never holds more than 3 (L - 1) entries.
+
+struct TraceStack {
+    uint2* lds;             // this lane's entry 0; entry k at lds[k * BLOCK]
+    uint2* spill;           // this thread's entry kStackLds; then spill_stride apart
+    uint32_t spill_stride;
+};
+
+struct WideRay {
+    f3 o, d;
+    float ix, iy, iz;
+    float tmin, tmax;
+};
+
+__device__ __forceinline__ WideRay make_wide_ray(f3 o, f3 d, float tmin, float tmax) {
+    WideRay r;
+    r.o = o;
+    r.d = d;
+    r.ix = safe_inv(d.x);
+    r.iy = safe_inv(d.y);
+    r.iz = safe_inv(d.z);
+    r.tmin = tmin;
+    r.tmax = tmax;
+    return r;
+}
+
+struct WideState {
+    uint32_t node;
+    int sp;
+    TraceResult best;
+};
+
+__device__ __forceinline__ void wide_begin(WideState& w, float tmax) {
+    w.node = 0u;
+    w.sp = 0;
+    w.best = TraceResult{tmax, 0u, 0.0f, 0.0f, false};
+}
+
+template <int BLOCK>
+__device__ __forceinline__ void stack_push(const TraceStack& st, int& sp, int limit, uint32_t node,
+    float tn) {
+    if (sp >= limit) return;  // cannot happen for a well-formed tree (3 (L-1) < limit)
+    const uint2 e = make_uint2(node, __float_as_uint(tn));
+    if (sp < kStackLds) st.lds[sp * BLOCK] = e;
+    else st.spill[(uint32_t)(sp - kStackLds) * st.spill_stride] = e;
+    ++sp;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ uint2 stack_pop(const TraceStack& st, int& sp) {
+    --sp;
+    return sp < kStackLds ? st.lds[sp * BLOCK] : st.spill[(uint32_t)(sp - kStackLds) * st.spill_stride];
+}
+
+__device__ __forceinline__ float qbyte(uint32_t word, int k) {
+    return (float)((word >> (8 * k)) & 255u);
+}
+
+__device__ __forceinline__ void consider_hit(TraceResult& b, uint32_t id, float t, float u, float v) {
+    if (t >= 0.0f && (t < b.t || (t == b.t && b.found && id < b.prim))) {
+        b.t = t;
+        b.prim = id;
+        b.u = u;
+        b.v = v;
+        b.found = true;
+    }
+}
+
+// One node visit (then pops until the next node).  Returns true when the
+// traversal is finished.
+template <int BLOCK, bool ANY, bool COUNT>
+__device__ __forceinline__ bool wide_step(const SceneView& s, const WideRay& r, WideState& w,
+    const TraceStack& st, TraverseCounts& cnt) {
+    const float4* np = s.nodes4 + 4u * w.node;
+    const float4 v0 = np[0];
+    const uint4 v1 = reinterpret_cast<const uint4*>(np)[1];
+    const uint4 v2 = reinterpret_cast<const uint4*>(np)[2];
+    const uint4 v3 = reinterpret_cast<const uint4*>(np)[3];
+    const uint32_t meta = __float_as_uint(v0.w);
+    if (COUNT) cnt.visits += meta >> 24;
+    const float sx = r.ix * __uint_as_float((meta & 255u) << 23);
+    const float sy = r.iy * __uint_as_float(((meta >> 8) & 255u) << 23);
+    const float sz = r.iz * __uint_as_float(((meta >> 16) & 255u) << 23);
+    const float ox = (v0.x - r.o.x) * r.ix;
+    const float oy = (v0.y - r.o.y) * r.iy;
+    const float oz = (v0.z - r.o.z) * r.iz;
+    const float slack = 4e-7f * ((fabsf(ox) + fabsf(oy)) + fabsf(oz));
+    const uint32_t ch[4] = {v1.x, v1.y, v1.z, v1.w};
+    float tn[4];
+    bool hit[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float t0x = fmaf(qbyte(v2.x, k), sx, ox), t1x = fmaf(qbyte(v2.w, k), sx, ox);
+        const float t0y = fmaf(qbyte(v2.y, k), sy, oy), t1y = fmaf(qbyte(v3.x, k), sy, oy);
+        const float t0z = fmaf(qbyte(v2.z, k), sz, oz), t1z = fmaf(qbyte(v3.y, k), sz, oz);
+        const float nx = fminf(t0x, t1x), ny = fminf(t0y, t1y), nz = fminf(t0z, t1z);
+        const float fx = fmaxf(t0x, t1x), fy = fmaxf(t0y, t1y), fz = fmaxf(t0z, t1z);
+        const float n = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tmin));
+        const float f = fminf(fminf(fx, fy), fminf(fz, w.best.t)) * 1.0000024f + slack;
+        tn[k] = n;
+        hit[k] = ch[k] != kEmpty && n <= f;
+    }
+    // leaf ranges in slot order
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (hit[k] && (ch[k] & kLeafBit)) {
+            const uint32_t first = ch[k] & 0x0FFFFFFFu;
+            const uint32_t n = ((ch[k] >> 28) & 7u) + 1u;
+            for (uint32_t i = 0; i < n; ++i) {
                 uint32_t id;
                 float u, v;
-                ++nt;
-                const float t = hit_leaf(s.leaves, n3.x & ~kLeafBit, o, d, tmin, tmax, &id, &u, &v);
-                if (COUNT && (int)id < s.sphere_count) ++ns;
-                if (t >= 0.0f && (t < r.t || (t == r.t && r.found && id < r.prim))) {
-                    r.t = t; r.prim = id; r.u = u; r.v = v; r.found = true;
+                const float t = hit_leaf(s.leaves, first + i, r.o, r.d, r.tmin, r.tmax, &id, &u, &v);
+                if (COUNT) {
+                    ++cnt.tests;
+                    if ((int)id < s.sphere_count) ++cnt.sphere_tests;
                 }
-                if (ANY && r.found) break;
+                consider_hit(w.best, id, t, u, v);
+                if (ANY && w.best.found) return true;
             }
-            if (l1) {
-                uint32_t id;
-                float u, v;
-                ++nt;
-                const float t = hit_leaf(s.leaves, n3.y & ~kLeafBit, o, d, tmin, tmax, &id, &u, &v);
-                if (COUNT && (int)id < s.sphere_count) ++ns;
-                if (t >= 0.0f && (t < r.t || (t == r.t && r.found && id < r.prim))) {
-                    r.t = t; r.prim = id; r.u = u; r.v = v; r.found = true;
-                }
-                if (ANY && r.found) break;
-            }
-            const bool i0 = h0 && !(n3.x & kLeafBit);
-            const bool i1 = h1 && !(n3.y & kLeafBit);
-            if (i0 && i1) {
-                const bool near1 = tn1 < tn0;
-                stack_node[sp * stride] = near1 ? n3.x : n3.y;
-                stack_t[sp * stride] = near1 ? tn0 : tn1;
-                ++sp;
-                node = near1 ? n3.y : n3.x;
-                continue;
-            }
-            if (i0) { node = n3.x; continue; }
-            if (i1) { node = n3.y; continue; }
-            bool got = false;
-            while (sp > 0) {
-                --sp;
-                if (stack_t[sp * stride] <= r.t) {
-                    node = stack_node[sp * stride];
-                    got = true;
-                    break;
-                }
-            }
-            if (!got) break;
         }
     }
-    if (COUNT) {
-        cnt->visits = nv;
-        cnt->tests = nt;
-        cnt->sphere_tests = ns;
+    // internal children: stable rank by entry distance
+    bool in[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) in[k] = hit[k] && !(ch[k] & kLeafBit);
+    int rank[4];
+    int m = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int rk = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j == k) continue;
+            rk += (in[j] && (tn[j] < tn[k] || (tn[j] == tn[k] && j < k))) ? 1 : 0;
+        }
+        rank[k] = rk;
+        m += in[k] ? 1 : 0;
     }
-    return r;
+    if (m > 0) {
+#pragma unroll
+        for (int rr = 3; rr >= 1; --rr) {
+            if (rr < m) {
+                uint32_t ref = 0u;
+                float key = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (in[k] && rank[k] == rr) {
+                        ref = ch[k];
+                        key = tn[k];
+                    }
+                }
+                stack_push<BLOCK>(st, w.sp, s.stack4_entries, ref, key);
+            }
+        }
+        uint32_t next = 0u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (in[k] && rank[k] == 0) next = ch[k];
+        }
+        w.node = next;
+        return false;
+    }
+    while (w.sp > 0) {
+        const uint2 e = stack_pop<BLOCK>(st, w.sp);
+        if (__uint_as_float(e.y) <= w.best.t) {
+            w.node = e.x;
+            return false;
+        }
+    }
+    return true;
+}
+
+// one complete traversal by one lane
+template <int BLOCK, bool ANY, bool COUNT = false>
+__device__ __forceinline__ TraceResult trace_wide(const SceneView& s, f3 o, f3 d, float tmin,
+    float tmax, const TraceStack& st, TraverseCounts* cnt = nullptr) {
+    TraverseCounts c{0u, 0u, 0u};
+    WideState w;
+    wide_begin(w, tmax);
+    if (s.prim_count > 0 && s.node4_count > 0) {
+        const WideRay r = make_wide_ray(o, d, tmin, tmax);
+        while (!wide_step<BLOCK, ANY, COUNT>(s, r, w, st, c)) {
+        }
+    }
+    if (COUNT && cnt) *cnt = c;
+    return w.best;
+}
+
+__device__ __forceinline__ uint64_t lanes_below(int lane) {
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Persistent traversal over a queue of `count` rays: a lane whose ray is done
+// takes the next queue entry at once (one atomic per wave for all idle
+// lanes), so lanes never wait for the slowest ray of their wave.
+//   load(q, &ray)         : ray of queue entry q
+//   done(q, result, cnt)  : store the result of queue entry q
+template <int BLOCK, bool ANY, bool COUNT, class Load, class Done>
+__device__ __forceinline__ void trace_queue(const SceneView& s, uint32_t count, uint32_t* fetch,
+    const TraceStack& st, Load load, Done done) {
+    const int lane = (int)(threadIdx.x & 63u);
+    bool active = false;
+    bool exhausted = false;
+    uint32_t q = 0u;
+    WideRay r{};
+    WideState w{};
+    TraverseCounts c{0u, 0u, 0u};
+    const bool has_tree = s.prim_count > 0 && s.node4_count > 0;
+    while (true) {
+        if (!exhausted) {
+            const uint64_t idle = __ballot(!active);
+            if (idle != 0ull) {
+                const int leader = __ffsll((unsigned long long)idle) - 1;
+                const uint32_t want = (uint32_t)__popcll(idle);
+                uint32_t base = 0u;
+                if (lane == leader) base = atomicAdd(fetch, want);
+                base = (uint32_t)__shfl((int)base, leader);
+                if (!active) {
+                    const uint32_t mine = base + (uint32_t)__popcll(idle & lanes_below(lane));
+                    if (mine < count) {
+                        q = mine;
+                        r = load(mine);
+                        wide_begin(w, r.tmax);
+                        c = TraverseCounts{0u, 0u, 0u};
+                        active = true;
+                        if (!has_tree) {
+                            done(q, w.best, c);
+                            active = false;
+                        }
+                    }
+                }
+                if (base + want >= count) exhausted = true;
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active && wide_step<BLOCK, ANY, COUNT>(s, r, w, st, c)) {
+            done(q, w.best, c);
+            active = false;
+        }
+    }
 }
 
 }  // namespace rtk

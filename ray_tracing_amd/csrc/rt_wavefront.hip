@@ -32,7 +32,8 @@ namespace {
 
 enum : uint32_t { kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntShadow2 = 3,
     kCntStatExtend = 4, kCntStatShadow = 6, kCntVisits = 8, kCntTests = 10, kCntSphereTests = 12,
-    kCntRestir = 14, kCounterWords = 16 };
+    kCntRestir = 14, kCntFetch = 16 /* 3 ray-fetch cursors: extend, connect 0, connect 1 */,
+    kCounterWords = 20 };
 
 enum : int32_t { kPrevValid = 1, kPrevDelta = 2 };
 
@@ -96,6 +97,9 @@ __global__ __launch_bounds__(256) void k_raygen(CameraView cams, FrameParams fp,
     if (i == 0) {
         b.counters[kCntQueue0] = (uint32_t)paths;
         b.counters[kCntQueue1] = 0u;
+        b.counters[kCntFetch] = 0u;
+        b.counters[kCntFetch + 1] = 0u;
+        b.counters[kCntFetch + 2] = 0u;
         b.counters[kCntShadow] = 0u;
         b.counters[kCntShadow2] = 0u;
         b.counters[kCntRestir] = 0u;
@@ -128,40 +132,55 @@ __global__ __launch_bounds__(256) void k_raygen(CameraView cams, FrameParams fp,
     b.queue[0][i] = (uint32_t)i;
 }
 
+template <int BLOCK>
+__device__ __forceinline__ TraceStack lane_stack(uint2* lds, uint2* spill) {
+    TraceStack st;
+    st.lds = lds + threadIdx.x;
+    st.spill_stride = gridDim.x * BLOCK;
+    st.spill = spill + (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    return st;
+}
+
 template <int BLOCK, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_extend(SceneView s, WavefrontBuffers b, int cur) {
-    extern __shared__ uint32_t lds[];
+    __shared__ uint2 lds[kStackLds * BLOCK];
     const uint32_t count = b.counters[cur];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntStatExtend),
             (unsigned long long)count);
     }
-    uint32_t* stack_node = lds + threadIdx.x;
-    float* stack_t = reinterpret_cast<float*>(lds + BLOCK * s.stack_depth) + threadIdx.x;
+    const TraceStack st = lane_stack<BLOCK>(lds, b.spill);
     const uint32_t* queue = b.queue[cur];
-    const uint32_t stride = gridDim.x * BLOCK;
-    for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < count; q += stride) {
-        const uint32_t slot = queue[q];
-        const f3 o = xyz(b.ray_o[slot]);
-        const f3 d = xyz(b.ray_d[slot]);
-        TraverseCounts cnt;
-        const TraceResult r = traverse<false, COUNT>(s, o, d, kRayEps, kRayFar, stack_node, stack_t,
-            BLOCK, &cnt);
-        // analytic lights are not in the BVH: nearest front-facing one before
-        // the surface hit (programs.cu:1861-1862)
-        const int ali = s.analytic_count > 0
-                            ? intersect_analytic(s, o, d, kRayEps, r.found ? r.t : kRayFar)
-                            : -1;
-        b.hit[slot] = make_float4(r.t, __int_as_float(r.found ? (int)r.prim : -1),
-            __int_as_float(ali), r.u);
-        if (COUNT) {
-            atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntVisits),
-                (unsigned long long)cnt.visits);
-            atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntTests),
-                (unsigned long long)cnt.tests);
-            atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntSphereTests),
-                (unsigned long long)cnt.sphere_tests);
-        }
+    TraverseCounts tot{0u, 0u, 0u};
+    trace_queue<BLOCK, false, COUNT>(s, count, b.counters + kCntFetch, st,
+        [&](uint32_t q) {
+            const uint32_t slot = queue[q];
+            return make_wide_ray(xyz(b.ray_o[slot]), xyz(b.ray_d[slot]), kRayEps, kRayFar);
+        },
+        [&](uint32_t q, const TraceResult& r, const TraverseCounts& c) {
+            const uint32_t slot = queue[q];
+            // analytic lights are not in the BVH: nearest front-facing one
+            // before the surface hit (programs.cu:1861-1862)
+            int ali = -1;
+            if (s.analytic_count > 0) {
+                ali = intersect_analytic(s, xyz(b.ray_o[slot]), xyz(b.ray_d[slot]), kRayEps,
+                    r.found ? r.t : kRayFar);
+            }
+            b.hit[slot] = make_float4(r.t, __int_as_float(r.found ? (int)r.prim : -1),
+                __int_as_float(ali), r.u);
+            if (COUNT) {
+                tot.visits += c.visits;
+                tot.tests += c.tests;
+                tot.sphere_tests += c.sphere_tests;
+            }
+        });
+    if (COUNT) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntVisits),
+            (unsigned long long)tot.visits);
+        atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntTests),
+            (unsigned long long)tot.tests);
+        atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntSphereTests),
+            (unsigned long long)tot.sphere_tests);
     }
 }
 
@@ -319,10 +338,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, FrameParams fp,
     RestirView rv, WavefrontBuffers b) {
-    extern __shared__ uint32_t lds[];
+    __shared__ uint2 lds[kStackLds * BLOCK];
     const uint32_t count = b.counters[kCntRestir];
-    uint32_t* stack_node = lds + threadIdx.x;
-    float* stack_t = reinterpret_cast<float*>(lds + BLOCK * s.stack_depth) + threadIdx.x;
+    const TraceStack st = lane_stack<BLOCK>(lds, b.spill);
     const uint32_t stride = gridDim.x * BLOCK;
     const uint32_t nq = (count + stride - 1u) / stride * stride;
     for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < nq; q += stride) {
@@ -345,9 +363,9 @@ __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, 
                                  ? rv.current + id.global_pixel
                                  : nullptr;
             const bool bsdf_avail = 1 < fp.max_bounces;
-            const RestirOut ro = restir_shade(s, rv, h, xyz(d4), (int)id.camera,
+            const RestirOut ro = restir_shade<BLOCK>(s, rv, h, xyz(d4), (int)id.camera,
                 cams.pixel_offset[id.camera], x, y, w, hgt,
-                id.pixel, id.sample, fp.stream, bsdf_avail, out, stack_node, stack_t, BLOCK);
+                id.pixel, id.sample, fp.stream, bsdf_avail, out, st);
             if (ro.shadow) {
                 shadow = true;
                 s_o = f4(ro.origin, ro.max_t);
@@ -366,7 +384,7 @@ __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, 
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_connect(SceneView s, WavefrontBuffers b, int which) {
-    extern __shared__ uint32_t lds[];
+    __shared__ uint2 lds[kStackLds * BLOCK];
     const uint32_t count = b.counters[which == 0 ? kCntShadow : kCntShadow2];
     const float4* so = which == 0 ? b.shadow_o : b.shadow2_o;
     const float4* sd = which == 0 ? b.shadow_d : b.shadow2_d;
@@ -375,21 +393,20 @@ __global__ __launch_bounds__(BLOCK) void k_connect(SceneView s, WavefrontBuffers
         atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntStatShadow),
             (unsigned long long)count);
     }
-    uint32_t* stack_node = lds + threadIdx.x;
-    float* stack_t = reinterpret_cast<float*>(lds + BLOCK * s.stack_depth) + threadIdx.x;
-    const uint32_t stride = gridDim.x * BLOCK;
-    for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < count; q += stride) {
-        const float4 o = so[q];
-        const float4 d = sd[q];
-        const TraceResult r = traverse<true>(s, xyz(o), xyz(d), kRayEps, o.w, stack_node, stack_t,
-            BLOCK);
-        if (!r.found) {
-            const uint32_t slot = __float_as_uint(d.w);
-            const float4 c = sc[q];
-            const float4 rad = b.radiance[slot];
-            b.radiance[slot] = f4(xyz(rad) + xyz(c), 0.0f);
-        }
-    }
+    const TraceStack st = lane_stack<BLOCK>(lds, b.spill);
+    trace_queue<BLOCK, true, false>(s, count, b.counters + kCntFetch + 1 + which, st,
+        [&](uint32_t q) {
+            const float4 o = so[q];
+            return make_wide_ray(xyz(o), xyz(sd[q]), kRayEps, o.w);
+        },
+        [&](uint32_t q, const TraceResult& r, const TraverseCounts&) {
+            if (!r.found) {
+                const uint32_t slot = __float_as_uint(sd[q].w);
+                const float4 c = sc[q];
+                const float4 rad = b.radiance[slot];
+                b.radiance[slot] = f4(xyz(rad) + xyz(c), 0.0f);
+            }
+        });
 }
 
 __global__ void k_swap_queues(uint32_t* counters, int cur) {
@@ -399,6 +416,9 @@ __global__ void k_swap_queues(uint32_t* counters, int cur) {
         counters[kCntShadow] = 0u;
         counters[kCntShadow2] = 0u;
         counters[kCntRestir] = 0u;
+        counters[kCntFetch] = 0u;
+        counters[kCntFetch + 1] = 0u;
+        counters[kCntFetch + 2] = 0u;
     }
 }
 
@@ -472,24 +492,18 @@ hipError_t wf_raygen(const CameraView& cams, const FrameParams& fp, WavefrontBuf
 }
 
 template <bool COUNT>
-static void launch_extend(const SceneView& sv, int stack_depth, WavefrontBuffers& b, int cur,
-    StageLaunch l, hipStream_t st) {
-    if (l.block == 64) {
-        k_extend<64, COUNT><<<l.grid, 64, (size_t)stack_depth * 64 * 8, st>>>(sv, b, cur);
-    } else if (l.block == 256) {
-        k_extend<256, COUNT><<<l.grid, 256, (size_t)stack_depth * 256 * 8, st>>>(sv, b, cur);
-    } else {
-        k_extend<128, COUNT><<<l.grid, 128, (size_t)stack_depth * 128 * 8, st>>>(sv, b, cur);
-    }
+static void launch_extend(const SceneView& sv, WavefrontBuffers& b, int cur, StageLaunch l,
+    hipStream_t st) {
+    if (l.block == 64) k_extend<64, COUNT><<<l.grid, 64, 0, st>>>(sv, b, cur);
+    else if (l.block == 256) k_extend<256, COUNT><<<l.grid, 256, 0, st>>>(sv, b, cur);
+    else k_extend<128, COUNT><<<l.grid, 128, 0, st>>>(sv, b, cur);
 }
 
-hipError_t wf_extend(const SceneView& s, int stack_depth, WavefrontBuffers& b, int bounce,
-    bool count, StageLaunch l, hipStream_t st) {
-    SceneView sv = s;
-    sv.stack_depth = stack_depth;
+hipError_t wf_extend(const SceneView& s, WavefrontBuffers& b, int bounce, bool count,
+    StageLaunch l, hipStream_t st) {
     const int cur = bounce & 1;
-    if (count) launch_extend<true>(sv, stack_depth, b, cur, l, st);
-    else launch_extend<false>(sv, stack_depth, b, cur, l, st);
+    if (count) launch_extend<true>(s, b, cur, l, st);
+    else launch_extend<false>(s, b, cur, l, st);
     return hipGetLastError();
 }
 
@@ -499,31 +513,23 @@ hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParam
     return hipGetLastError();
 }
 
-template <int BLOCK>
-static void launch_connect(const SceneView& sv, int stack_depth, WavefrontBuffers& b, int which,
-    StageLaunch l, hipStream_t st) {
-    k_connect<BLOCK><<<l.grid, BLOCK, (size_t)stack_depth * BLOCK * 8, st>>>(sv, b, which);
-}
-
-hipError_t wf_connect(const SceneView& s, int stack_depth, WavefrontBuffers& b, int bounce,
-    StageLaunch l, hipStream_t st) {
-    SceneView sv = s;
-    sv.stack_depth = stack_depth;
+hipError_t wf_connect(const SceneView& s, WavefrontBuffers& b, int bounce, StageLaunch l,
+    hipStream_t st) {
     for (int which = 0; which < 2; ++which) {
-        if (l.block == 64) launch_connect<64>(sv, stack_depth, b, which, l, st);
-        else if (l.block == 256) launch_connect<256>(sv, stack_depth, b, which, l, st);
-        else launch_connect<128>(sv, stack_depth, b, which, l, st);
+        if (l.block == 64) k_connect<64><<<l.grid, 64, 0, st>>>(s, b, which);
+        else if (l.block == 256) k_connect<256><<<l.grid, 256, 0, st>>>(s, b, which);
+        else k_connect<128><<<l.grid, 128, 0, st>>>(s, b, which);
     }
     k_swap_queues<<<1, 64, 0, st>>>(b.counters, bounce & 1);
     return hipGetLastError();
 }
 
-hipError_t wf_restir(const SceneView& s, int stack_depth, const CameraView& cams,
-    const FrameParams& fp, const RestirView& rv, WavefrontBuffers& b, StageLaunch l,
-    hipStream_t st) {
-    SceneView sv = s;
-    sv.stack_depth = stack_depth;
-    k_restir<128><<<l.grid, 128, (size_t)stack_depth * 128 * 8, st>>>(sv, cams, fp, rv, b);
+// the ReSTIR stage always runs 128-thread blocks on the trace grid; its
+// spill area is the traversal one (trace grid x block threads)
+hipError_t wf_restir(const SceneView& s, const CameraView& cams, const FrameParams& fp,
+    const RestirView& rv, WavefrontBuffers& b, StageLaunch l, hipStream_t st) {
+    const int grid = (int)((int64_t)l.grid * l.block / 128);
+    k_restir<128><<<grid > 0 ? grid : 1, 128, 0, st>>>(s, cams, fp, rv, b);
     return hipGetLastError();
 }
 

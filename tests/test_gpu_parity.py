@@ -64,6 +64,9 @@ def compare_bvh(oracle, desc, sc):
     assert np.array_equal(mc, ob.morton()), "Morton codes differ"
     assert np.array_equal(sp, ob.sorted_prims()), "sorted primitive order differs"
     assert nodes.tobytes() == on.tobytes(), "node arrays differ"
+    assert sc.stats.wide_node_count == ob.bvh.wide_count
+    assert sc.stats.wide_levels == ob.bvh.wide_levels
+    assert sc.download_wide().tobytes() == ob.wide().tobytes(), "4-wide node arrays differ"
     return osc, ob
 
 

@@ -64,6 +64,8 @@ def test_instanced_bvh_bitexact(rt, oracle, instanced):
     assert np.array_equal(mc, ob.morton())
     assert np.array_equal(sp, ob.sorted_prims())
     assert nodes.tobytes() == ob.nodes().tobytes()
+    assert sc.stats.wide_node_count == ob.bvh.wide_count
+    assert sc.download_wide().tobytes() == ob.wide().tobytes()
 
 
 def compare_frame(gpu, ref, label, tol_frac=0.995):
