@@ -94,6 +94,8 @@ def main():
                     help="only run a plain 1 GiB device-to-device copy (3 times), the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--refill", type=int, default=0,
+                    help="idle lanes before a traversal wave takes new rays (1..64)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this path tracer shards by camera with no "
@@ -127,6 +129,8 @@ def main():
         tune["trace_block"] = a.trace_block
     if a.blocks_per_cu:
         tune["trace_blocks_per_cu"] = a.blocks_per_cu
+    if a.refill:
+        tune["trace_refill"] = a.refill
     if tune:
         ren.set_tuning(**tune)
     rig = g.rig()

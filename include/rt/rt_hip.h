@@ -291,6 +291,7 @@ typedef struct rt_tuning {
     int32_t trace_blocks_per_cu; /* persistent blocks per CU */
     int32_t use_graph;        /* capture the per-frame stage sequence in a hipGraph */
     int32_t count_traversal;  /* count node visits / primitive tests (slower; for measurement) */
+    int32_t trace_refill;     /* idle lanes (1..64) before a wave takes new rays; 64 = whole batches */
 } rt_tuning;
 rt_status rt_renderer_set_tuning(rt_renderer* r, const rt_tuning* t);
 rt_status rt_renderer_get_tuning(rt_renderer* r, rt_tuning* t);

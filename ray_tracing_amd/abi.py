@@ -114,7 +114,7 @@ class FrameTiming(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("wavefront_paths", C.c_int32), ("trace_block", C.c_int32),
                 ("trace_blocks_per_cu", C.c_int32), ("use_graph", C.c_int32),
-                ("count_traversal", C.c_int32)]
+                ("count_traversal", C.c_int32), ("trace_refill", C.c_int32)]
 
 
 class RestirStats(C.Structure):

@@ -761,6 +761,7 @@ rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out)
     r->tuning.wavefront_paths = 1 << 22;
     r->tuning.trace_block = 128;
     r->tuning.trace_blocks_per_cu = 10;
+    r->tuning.trace_refill = 16;
     r->tuning.use_graph = 0;
     hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&r->host_cams), sizeof(rt_camera) * 8,
         hipHostMallocDefault);
@@ -798,6 +799,10 @@ rt_status rt_renderer_set_tuning(rt_renderer* r, const rt_tuning* t) {
     if (t->trace_blocks_per_cu > 0) r->tuning.trace_blocks_per_cu = t->trace_blocks_per_cu;
     if (t->use_graph >= 0) r->tuning.use_graph = t->use_graph;
     if (t->count_traversal >= 0) r->tuning.count_traversal = t->count_traversal;
+    if (t->trace_refill > 0) {
+        RT_REQUIRE(t->trace_refill <= 64, RT_ERROR_INVALID_ARGUMENT, "trace_refill must be in [1, 64]");
+        r->tuning.trace_refill = t->trace_refill;
+    }
     return RT_OK;
 }
 
@@ -920,7 +925,8 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
     const rtk::SceneView sv = s->view();
     rtk::WavefrontBuffers b = r->buffers();
     const int block = r->tuning.trace_block;
-    const rtk::StageLaunch trace_l{r->dev->cu_count * r->tuning.trace_blocks_per_cu, block};
+    const rtk::StageLaunch trace_l{r->dev->cu_count * r->tuning.trace_blocks_per_cu, block,
+        r->tuning.trace_refill};
     const rtk::StageLaunch shade_l{r->dev->cu_count * 4, 256};
     {
         const size_t entries = (size_t)std::max(sv.stack4_entries - rtk::kStackLds, 1);

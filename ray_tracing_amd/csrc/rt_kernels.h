@@ -91,6 +91,7 @@ struct FrameParams {
 struct StageLaunch {
     int grid;
     int block;
+    int refill_min = 64;  // idle lanes of a wave before it takes new rays (trace stages)
 };
 
 hipError_t wf_reset_counters(uint32_t* counters, hipStream_t st);

@@ -17,7 +17,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneView s, const rt_ray
     st.lds = lds + threadIdx.x;
     st.spill_stride = gridDim.x * kTraceBlock;
     st.spill = spill + (size_t)blockIdx.x * kTraceBlock + threadIdx.x;
-    trace_queue<kTraceBlock, ANY, false>(s, (uint32_t)n, fetch, st,
+    trace_queue<kTraceBlock, ANY, false>(s, (uint32_t)n, fetch, st, 16,
         [&](uint32_t q) {
             const rt_ray r = rays[q];
             return make_wide_ray(ld3(r.o), ld3(r.d), r.tmin, r.tmax);

@@ -326,50 +326,57 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Persistent traversal over a queue of `count` rays: a lane whose ray is done
-// takes the next queue entry at once (one atomic per wave for all idle
-// lanes), so lanes never wait for the slowest ray of their wave.
-//   load(q, &ray)         : ray of queue entry q
+// Persistent traversal over a queue of `count` rays.  Each wave claims the
+// queue in batches of 64 entries (one atomic per batch) and hands them to its
+// idle lanes once at least `refill_min` lanes are idle: refill_min = 64 runs
+// whole coherent batches (classic persistent threads), smaller values let a
+// lane take a new ray as soon as its own is done.
+//   load(q)               : ray of queue entry q
 //   done(q, result, cnt)  : store the result of queue entry q
 template <int BLOCK, bool ANY, bool COUNT, class Load, class Done>
 __device__ __forceinline__ void trace_queue(const SceneView& s, uint32_t count, uint32_t* fetch,
-    const TraceStack& st, Load load, Done done) {
+    const TraceStack& st, int refill_min, Load load, Done done) {
     const int lane = (int)(threadIdx.x & 63u);
     bool active = false;
     bool exhausted = false;
+    uint32_t batch_base = 0u, batch_left = 0u;  // wave-uniform
     uint32_t q = 0u;
     WideRay r{};
     WideState w{};
     TraverseCounts c{0u, 0u, 0u};
     const bool has_tree = s.prim_count > 0 && s.node4_count > 0;
     while (true) {
-        if (!exhausted) {
-            const uint64_t idle = __ballot(!active);
-            if (idle != 0ull) {
-                const int leader = __ffsll((unsigned long long)idle) - 1;
-                const uint32_t want = (uint32_t)__popcll(idle);
+        const uint64_t idle = __ballot(!active);
+        const int nidle = __popcll(idle);
+        if (nidle >= refill_min || nidle == 64) {
+            if (batch_left == 0u && !exhausted) {
                 uint32_t base = 0u;
-                if (lane == leader) base = atomicAdd(fetch, want);
-                base = (uint32_t)__shfl((int)base, leader);
-                if (!active) {
-                    const uint32_t mine = base + (uint32_t)__popcll(idle & lanes_below(lane));
-                    if (mine < count) {
-                        q = mine;
-                        r = load(mine);
-                        wide_begin(w, r.tmax);
-                        c = TraverseCounts{0u, 0u, 0u};
-                        active = true;
-                        if (!has_tree) {
-                            done(q, w.best, c);
-                            active = false;
-                        }
+                if (lane == 0) base = atomicAdd(fetch, 64u);
+                base = (uint32_t)__shfl((int)base, 0);
+                batch_base = base;
+                batch_left = base < count ? (count - base < 64u ? count - base : 64u) : 0u;
+                if (base + 64u >= count) exhausted = true;
+            }
+            const uint32_t take = (uint32_t)nidle < batch_left ? (uint32_t)nidle : batch_left;
+            if (take > 0u) {
+                const uint32_t rank = (uint32_t)__popcll(idle & lanes_below(lane));
+                if (!active && rank < take) {
+                    q = batch_base + rank;
+                    r = load(q);
+                    wide_begin(w, r.tmax);
+                    c = TraverseCounts{0u, 0u, 0u};
+                    active = true;
+                    if (!has_tree) {
+                        done(q, w.best, c);
+                        active = false;
                     }
                 }
-                if (base + want >= count) exhausted = true;
+                batch_base += take;
+                batch_left -= take;
             }
         }
         if (__ballot(active) == 0ull) {
-            if (exhausted) break;
+            if (exhausted && batch_left == 0u) break;
             continue;
         }
         if (active && wide_step<BLOCK, ANY, COUNT>(s, r, w, st, c)) {

@@ -142,7 +142,8 @@ __device__ __forceinline__ TraceStack lane_stack(uint2* lds, uint2* spill) {
 }
 
 template <int BLOCK, bool COUNT>
-__global__ __launch_bounds__(BLOCK) void k_extend(SceneView s, WavefrontBuffers b, int cur) {
+__global__ __launch_bounds__(BLOCK) void k_extend(SceneView s, WavefrontBuffers b, int cur,
+    int refill_min) {
     __shared__ uint2 lds[kStackLds * BLOCK];
     const uint32_t count = b.counters[cur];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneView s, WavefrontBuffers 
     const TraceStack st = lane_stack<BLOCK>(lds, b.spill);
     const uint32_t* queue = b.queue[cur];
     TraverseCounts tot{0u, 0u, 0u};
-    trace_queue<BLOCK, false, COUNT>(s, count, b.counters + kCntFetch, st,
+    trace_queue<BLOCK, false, COUNT>(s, count, b.counters + kCntFetch, st, refill_min,
         [&](uint32_t q) {
             const uint32_t slot = queue[q];
             return make_wide_ray(xyz(b.ray_o[slot]), xyz(b.ray_d[slot]), kRayEps, kRayFar);
@@ -383,7 +384,8 @@ __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, 
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_connect(SceneView s, WavefrontBuffers b, int which) {
+__global__ __launch_bounds__(BLOCK) void k_connect(SceneView s, WavefrontBuffers b, int which,
+    int refill_min) {
     __shared__ uint2 lds[kStackLds * BLOCK];
     const uint32_t count = b.counters[which == 0 ? kCntShadow : kCntShadow2];
     const float4* so = which == 0 ? b.shadow_o : b.shadow2_o;
@@ -394,7 +396,7 @@ __global__ __launch_bounds__(BLOCK) void k_connect(SceneView s, WavefrontBuffers
             (unsigned long long)count);
     }
     const TraceStack st = lane_stack<BLOCK>(lds, b.spill);
-    trace_queue<BLOCK, true, false>(s, count, b.counters + kCntFetch + 1 + which, st,
+    trace_queue<BLOCK, true, false>(s, count, b.counters + kCntFetch + 1 + which, st, refill_min,
         [&](uint32_t q) {
             const float4 o = so[q];
             return make_wide_ray(xyz(o), xyz(sd[q]), kRayEps, o.w);
@@ -494,9 +496,9 @@ hipError_t wf_raygen(const CameraView& cams, const FrameParams& fp, WavefrontBuf
 template <bool COUNT>
 static void launch_extend(const SceneView& sv, WavefrontBuffers& b, int cur, StageLaunch l,
     hipStream_t st) {
-    if (l.block == 64) k_extend<64, COUNT><<<l.grid, 64, 0, st>>>(sv, b, cur);
-    else if (l.block == 256) k_extend<256, COUNT><<<l.grid, 256, 0, st>>>(sv, b, cur);
-    else k_extend<128, COUNT><<<l.grid, 128, 0, st>>>(sv, b, cur);
+    if (l.block == 64) k_extend<64, COUNT><<<l.grid, 64, 0, st>>>(sv, b, cur, l.refill_min);
+    else if (l.block == 256) k_extend<256, COUNT><<<l.grid, 256, 0, st>>>(sv, b, cur, l.refill_min);
+    else k_extend<128, COUNT><<<l.grid, 128, 0, st>>>(sv, b, cur, l.refill_min);
 }
 
 hipError_t wf_extend(const SceneView& s, WavefrontBuffers& b, int bounce, bool count,
@@ -516,9 +518,9 @@ hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParam
 hipError_t wf_connect(const SceneView& s, WavefrontBuffers& b, int bounce, StageLaunch l,
     hipStream_t st) {
     for (int which = 0; which < 2; ++which) {
-        if (l.block == 64) k_connect<64><<<l.grid, 64, 0, st>>>(s, b, which);
-        else if (l.block == 256) k_connect<256><<<l.grid, 256, 0, st>>>(s, b, which);
-        else k_connect<128><<<l.grid, 128, 0, st>>>(s, b, which);
+        if (l.block == 64) k_connect<64><<<l.grid, 64, 0, st>>>(s, b, which, l.refill_min);
+        else if (l.block == 256) k_connect<256><<<l.grid, 256, 0, st>>>(s, b, which, l.refill_min);
+        else k_connect<128><<<l.grid, 128, 0, st>>>(s, b, which, l.refill_min);
     }
     k_swap_queues<<<1, 64, 0, st>>>(b.counters, bounce & 1);
     return hipGetLastError();
