@@ -49,10 +49,11 @@ def lib():
     """Loads librt_hip.so; raises if it has not been built."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() or "
+        path = os.environ.get("RT_HIP_LIBRARY", LIB_PATH)  # a variant build for experiments
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is missing: run __graft_entry__.build() or "
                               "python ray_tracing_amd/build.py")
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         abi.declare(L)
         _lib = L
     return _lib

@@ -39,13 +39,15 @@ def _run(cmd):
     return r.stdout
 
 
-def _compile(src: str) -> str:
+def _compile(src: str, build_dir: str = BUILD, defines=()) -> str:
     path = os.path.join(CSRC, src)
-    obj = os.path.join(BUILD, src + ".o")
+    obj = os.path.join(build_dir, src + ".o")
+    dflags = [f"-D{d}" for d in defines]
     if src.endswith(".hip"):
-        cmd = [HIPCC] + HIP_FLAGS + ["-c", path, "-o", obj]
+        cmd = [HIPCC] + HIP_FLAGS + dflags + ["-c", path, "-o", obj]
     else:
-        cmd = [CLANGXX] + COMMON + ["-I", os.path.join(ROCM, "include"), "-c", path, "-o", obj]
+        cmd = [CLANGXX] + COMMON + dflags + ["-I", os.path.join(ROCM, "include"), "-c", path,
+                                             "-o", obj]
     out = _run(cmd)
     # the host pass does not know the packed-fp32 feature name: not an error
     lines = [l for l in out.splitlines() if "packed-fp32-ops" not in l and l.strip()]
@@ -54,19 +56,22 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int = 4, verbose: bool = True) -> str:
+def build(jobs: int = 4, verbose: bool = True, out: str = LIB, defines=()) -> str:
+    """Compiles and links the library; `defines` (NAME=VALUE) and `out` build a
+    variant (e.g. another LDS stack size) into another file."""
     if not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
-    os.makedirs(BUILD, exist_ok=True)
+    build_dir = BUILD if out == LIB else out + ".objs"
+    os.makedirs(build_dir, exist_ok=True)
     sources = HIP_SOURCES + CXX_SOURCES
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, sources))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda src: _compile(src, build_dir, defines), sources))
+    tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print(f"built {LIB}")
-    return LIB
+        print(f"built {out}")
+    return out
 
 
 def clean():
@@ -79,7 +84,9 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--clean", action="store_true")
     ap.add_argument("-j", type=int, default=4)
+    ap.add_argument("-D", dest="defines", action="append", default=[])
+    ap.add_argument("-o", dest="out", default=LIB)
     a = ap.parse_args()
     if a.clean:
         clean()
-    build(a.j)
+    build(a.j, out=os.path.abspath(a.out), defines=a.defines)

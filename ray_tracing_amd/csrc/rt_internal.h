@@ -72,12 +72,18 @@ struct SceneView {
 // so it always contains the exact box.
 constexpr int kWide = 4;
 // binary subtrees with at most this many primitives become one leaf range
-constexpr uint32_t kLeafMax = 4;
+#ifndef RT_WIDE_LEAF_MAX
+#define RT_WIDE_LEAF_MAX 4
+#endif
+constexpr uint32_t kLeafMax = RT_WIDE_LEAF_MAX;
 __host__ __device__ constexpr uint32_t wide_leaf(uint32_t first, uint32_t count) {
     return 0x80000000u | ((count - 1u) << 28) | first;
 }
 // LDS part of the per-lane traversal stack; deeper entries spill to HBM
-constexpr int kStackLds = 8;
+#ifndef RT_STACK_LDS
+#define RT_STACK_LDS 16
+#endif
+constexpr int kStackLds = RT_STACK_LDS;
 
 // RestirReservoir (src/common/restir_di.h:36-48): 84 B per pixel
 struct Reservoir {
