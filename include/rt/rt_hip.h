@@ -155,7 +155,7 @@ typedef struct rt_bvh_stats {
 
 /* 4-wide quantized BVH node (64 B), collapsed from the binary LBVH: each node
  * absorbs binary descendants, largest surface area first, until it has 4
- * children; binary subtrees of at most 4 primitives become leaf ranges.
+ * children; binary subtrees of at most 2 primitives become leaf ranges.
  * Child k's box is origin + [qlo, qhi] * 2^(e - 127) per axis (e = byte a of
  * meta, qlo/qhi = byte k of the axis word), rounded outward.  Child
  * reference: 0x80000000 | (count - 1) << 28 | first sorted leaf slot for a

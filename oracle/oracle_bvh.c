@@ -305,7 +305,7 @@ static void binary_children(const rto_bvh_node* nodes, uint32_t b, uint32_t ref[
     }
 }
 
-#define WIDE_LEAF_MAX 4u
+#define WIDE_LEAF_MAX 2u
 static uint32_t subtree_leaves(const rto_bvh_node* nodes, uint32_t ref) {
     return (ref & OM_LEAF_BIT) ? 1u : nodes[ref].pad;
 }

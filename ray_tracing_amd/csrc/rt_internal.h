@@ -73,7 +73,7 @@ struct SceneView {
 constexpr int kWide = 4;
 // binary subtrees with at most this many primitives become one leaf range
 #ifndef RT_WIDE_LEAF_MAX
-#define RT_WIDE_LEAF_MAX 4
+#define RT_WIDE_LEAF_MAX 2
 #endif
 constexpr uint32_t kLeafMax = RT_WIDE_LEAF_MAX;
 __host__ __device__ constexpr uint32_t wide_leaf(uint32_t first, uint32_t count) {

@@ -160,15 +160,8 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneView s, WavefrontBuffers 
         },
         [&](uint32_t q, const TraceResult& r, const TraverseCounts& c) {
             const uint32_t slot = queue[q];
-            // analytic lights are not in the BVH: nearest front-facing one
-            // before the surface hit (programs.cu:1861-1862)
-            int ali = -1;
-            if (s.analytic_count > 0) {
-                ali = intersect_analytic(s, xyz(b.ray_o[slot]), xyz(b.ray_d[slot]), kRayEps,
-                    r.found ? r.t : kRayFar);
-            }
             b.hit[slot] = make_float4(r.t, __int_as_float(r.found ? (int)r.prim : -1),
-                __int_as_float(ali), r.u);
+                __int_as_float(-1), r.u);
             if (COUNT) {
                 tot.visits += c.visits;
                 tot.tests += c.tests;
@@ -336,6 +329,24 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
 
 // ReSTIR DI at the first vertex for the deferred paths; its shadow ray joins
 // the second shadow queue
+// analytic lights are not in the BVH: nearest front-facing one before the
+// surface hit (programs.cu:1861-1862), for every path of the queue.  A
+// separate convergent pass: inside the persistent traversal loop the light
+// loop would run once per finishing lane.
+__global__ __launch_bounds__(256) void k_analytic_hits(SceneView s, WavefrontBuffers b, int cur) {
+    const uint32_t count = b.counters[cur];
+    const uint32_t* queue = b.queue[cur];
+    for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < count; q += gridDim.x * 256u) {
+        const uint32_t slot = queue[q];
+        float4 h = b.hit[slot];
+        const bool found = __float_as_int(h.y) >= 0;
+        const int ali = intersect_analytic(s, xyz(b.ray_o[slot]), xyz(b.ray_d[slot]), kRayEps,
+            found ? h.x : kRayFar);
+        h.z = __int_as_float(ali);
+        b.hit[slot] = h;
+    }
+}
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, FrameParams fp,
     RestirView rv, WavefrontBuffers b) {
@@ -506,6 +517,7 @@ hipError_t wf_extend(const SceneView& s, WavefrontBuffers& b, int bounce, bool c
     const int cur = bounce & 1;
     if (count) launch_extend<true>(s, b, cur, l, st);
     else launch_extend<false>(s, b, cur, l, st);
+    if (s.analytic_count > 0) k_analytic_hits<<<l.grid, 256, 0, st>>>(s, b, cur);
     return hipGetLastError();
 }
 
