@@ -284,6 +284,46 @@ rt_status rt_restir_diagnostics(rt_renderer* r, int camera, rt_restir_stats* out
  * OptixRenderer::restir_diagnostics (optix_renderer.cpp:183-215) */
 rt_status rt_restir_download(rt_renderer* r, int camera, void* reservoirs);
 
+/* ------------------------------------------------------ learned denoiser
+ * A small dilated CNN (4 x conv3x3 with dilation 1/2/4/8, 32 channels, 1x1
+ * head predicting log radiance as a residual of log1p(beauty)) run by BF16
+ * MFMA kernels with FP32 accumulation; trained by the same kernels (backward
+ * + Adam).  Replaces the closed OptiX denoiser of OptixDenoiserWrapper::run
+ * (src/realtime/gpu/denoiser.cpp:111-185).  Frames: float4 beauty, 0.5 n +
+ * 0.5 encoded float4 normal, float4 albedo, float depth per pixel; the
+ * images of a batch lie back to back. */
+typedef struct rt_nn rt_nn;
+typedef struct rt_nn_images {
+    int32_t count;        /* 1..16 */
+    int32_t offset[17];   /* first pixel of image i; offset[count] = total */
+    int32_t width[16];
+} rt_nn_images;
+int32_t rt_nn_param_count(void);
+rt_status rt_nn_create(int ordinal, rt_nn** out);   /* parameters start at 0 */
+rt_status rt_nn_destroy(rt_nn* nn);
+/* FP32 parameters: per layer W[co][tap][ci] then b[co]; head W[3][32], b[3].
+ * Setting them resets the optimiser state. */
+rt_status rt_nn_set_params(rt_nn* nn, const float* params, int32_t n);
+rt_status rt_nn_get_params(rt_nn* nn, float* params, int32_t n);
+/* device pointers */
+rt_status rt_nn_infer(rt_nn* nn, const rt_nn_images* images, const float* beauty,
+    const float* normal, const float* albedo, const float* depth, float* out);
+/* one training step on a batch: loss = mean (out_log - log1p(target))^2; the
+ * gradient (before the update, host array of rt_nn_param_count floats) and
+ * the loss are returned when non-NULL; lr = 0 computes them without updating */
+rt_status rt_nn_train_step(rt_nn* nn, const rt_nn_images* images, const float* beauty,
+    const float* normal, const float* albedo, const float* depth, const float* target, float lr,
+    float* loss, float* grad);
+/* the same on host arrays (uploads, runs, downloads) */
+rt_status rt_nn_infer_host(rt_nn* nn, const rt_nn_images* images, const float* beauty,
+    const float* normal, const float* albedo, const float* depth, float* out);
+rt_status rt_nn_train_step_host(rt_nn* nn, const rt_nn_images* images, const float* beauty,
+    const float* normal, const float* albedo, const float* depth, const float* target, float lr,
+    float* loss, float* grad);
+/* denoise with `nn` instead of the a-trous filter when the profile asks for
+ * denoising (NULL detaches); nn must live on the renderer's device */
+rt_status rt_renderer_set_denoiser(rt_renderer* r, rt_nn* nn);
+
 /* tuning knobs chosen by the autotuner; 0 keeps the current value */
 typedef struct rt_tuning {
     int32_t wavefront_paths;  /* paths in flight per wavefront (queue capacity) */

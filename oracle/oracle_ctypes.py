@@ -207,6 +207,12 @@ def lib():
         L.rto_denoise_pass.restype = None
         L.rto_denoise.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 6
         L.rto_denoise.restype = None
+        L.rto_nn_features.argtypes = [C.c_int] + [C.c_void_p] * 5
+        L.rto_nn_features.restype = None
+        L.rto_nn_loss_grad.argtypes = [C.c_void_p] * 7 + [C.c_int]
+        L.rto_nn_loss_grad.restype = C.c_double
+        L.rto_bf16_round.argtypes = [C.c_float]
+        L.rto_bf16_round.restype = C.c_float
         L.rto_primary_rays.argtypes = [P(Camera), C.c_int, C.c_int, C.c_uint32, C.c_int,
                                        C.c_void_p]
         L.rto_build_lights.argtypes = [P(Scene), C.c_void_p, C.c_int]
@@ -483,6 +489,48 @@ def denoise(frame: dict, iterations: int = 4) -> np.ndarray:
                       ptr(frame["normal"]), ptr(frame["albedo"]), ptr(frame["depth"]), ptr(out),
                       ptr(scratch))
     return out
+
+
+class Images(C.Structure):
+    _fields_ = [("count", C.c_int32), ("offset", C.c_int32 * 17), ("width", C.c_int32 * 16)]
+
+
+def images(sizes):
+    """Image table for [(width, height), ...] laid back to back."""
+    im = Images()
+    im.count = len(sizes)
+    o = 0
+    for i, (w, h) in enumerate(sizes):
+        im.offset[i] = o
+        im.width[i] = w
+        o += w * h
+    im.offset[len(sizes)] = o
+    return im
+
+
+def nn_loss_grad(im, params, frame, target=None, bf16=True, want_grad=True):
+    """Oracle denoiser: returns (loss, grad or None, out rgba)."""
+    P = im.offset[im.count]
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    beauty, normal, albedo = f32(frame["beauty"]), f32(frame["normal"]), f32(frame["albedo"])
+    depth = f32(frame["depth"])
+    feat = np.zeros((P, 16), np.float32)
+    lib().rto_nn_features(P, ptr(beauty), ptr(normal), ptr(albedo), ptr(depth), ptr(feat))
+    if not bf16:
+        # the same features without bf16 rounding
+        b = beauty.reshape(P, 4)
+        feat[:, 0:3] = np.log1p(np.maximum(b[:, :3], 0))
+        feat[:, 3:6] = np.clip(albedo.reshape(P, 4)[:, :3], 0, 1)
+        feat[:, 6:9] = normal.reshape(P, 4)[:, :3] * 2 - 1
+        feat[:, 9] = np.log1p(np.maximum(depth.reshape(P), 0)) * 0.125
+    params = f32(params)
+    out = np.zeros((P, 4), np.float32)
+    grad = np.zeros(len(params), np.float32) if (want_grad and target is not None) else None
+    tgt = f32(target) if target is not None else None
+    loss = lib().rto_nn_loss_grad(C.byref(im), ptr(params), ptr(feat), ptr(beauty),
+                                  ptr(tgt) if tgt is not None else None, ptr(out),
+                                  ptr(grad) if grad is not None else None, 1 if bf16 else 0)
+    return loss, grad, out
 
 
 class RestirSequence:

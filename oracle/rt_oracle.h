@@ -194,6 +194,15 @@ void rto_make_equi_lut(rto_camera* cam);
 /* the orbiting rig of the instanced scene (config 2) at frame `frame` of `frames` */
 int rto_generate_rig(int config, uint64_t seed, int frame, int frames, rto_camera* cams, int* count);
 
+/* ---- learned denoiser (oracle_nn.c) */
+typedef struct { int32_t count; int32_t offset[17]; int32_t width[16]; } rto_images;
+int rto_nn_param_count(void);
+float rto_bf16_round(float x);
+void rto_nn_features(int P, const float* beauty, const float* normal, const float* albedo,
+    const float* depth, float* feat);
+double rto_nn_loss_grad(const rto_images* im, const float* params, const float* feat,
+    const float* beauty, const float* target, float* out4, float* grad, int bf16);
+
 /* ---- per-frame post passes (oracle_post.c) */
 /* temporal reprojection + accumulation of one camera image, in place */
 void rto_resolve(const rto_camera* cam, const rto_camera* prev_cam, int history_length,

@@ -32,6 +32,9 @@ EXPORTED = [
     "rt_restir_diagnostics", "rt_renderer_set_tuning", "rt_renderer_get_tuning",
     "rt_scene_generate", "rt_scene_generated_free", "rt_restir_download",
     "rt_scene_generate_rig", "rt_frame_download_guides", "rt_bvh_download_wide",
+    "rt_nn_param_count", "rt_nn_create", "rt_nn_destroy", "rt_nn_set_params", "rt_nn_get_params",
+    "rt_nn_infer", "rt_nn_train_step", "rt_nn_infer_host", "rt_nn_train_step_host",
+    "rt_renderer_set_denoiser",
 ]
 
 
@@ -207,6 +210,10 @@ class Renderer:
                                              ptr(out["resolved"])))
         return out
 
+    def set_denoiser(self, nn: "Denoiser | None"):
+        self._nn = nn  # keeps the network alive while attached
+        check(lib().rt_renderer_set_denoiser(self.h, nn.h if nn is not None else None))
+
     def reservoirs(self, camera: int) -> np.ndarray:
         cam = self.rig.cameras[camera]
         out = np.zeros(cam.width * cam.height, abi.RESERVOIR_DTYPE)
@@ -229,6 +236,62 @@ class Renderer:
         check(lib().rt_frame_download(self.h, camera, ptr(out["beauty"]), ptr(out["normal"]),
                                       ptr(out["albedo"]), ptr(out["depth"])))
         return out
+
+
+class Denoiser:
+    """The learned denoiser (BF16 MFMA CNN); host-array or device-pointer use."""
+
+    def __init__(self, ordinal: int = 0, params: np.ndarray | None = None):
+        self.h = C.c_void_p()
+        check(lib().rt_nn_create(ordinal, C.byref(self.h)))
+        self.n = lib().rt_nn_param_count()
+        if params is not None:
+            self.set_params(params)
+
+    def close(self):
+        if self.h:
+            lib().rt_nn_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == self.n
+        check(lib().rt_nn_set_params(self.h, ptr(p), self.n))
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n, np.float32)
+        check(lib().rt_nn_get_params(self.h, ptr(p), self.n))
+        return p
+
+    @staticmethod
+    def _frame(frame):
+        f32 = lambda a: np.ascontiguousarray(a, np.float32)
+        return (f32(frame["beauty"]), f32(frame["normal"]), f32(frame["albedo"]),
+                f32(frame["depth"]))
+
+    def infer(self, sizes, frame) -> np.ndarray:
+        im = abi.nn_images(sizes)
+        b, n, a, d = self._frame(frame)
+        out = np.zeros((im.offset[im.count], 4), np.float32)
+        check(lib().rt_nn_infer_host(self.h, C.byref(im), ptr(b), ptr(n), ptr(a), ptr(d), ptr(out)))
+        return out
+
+    def train_step(self, sizes, frame, target, lr: float, want_grad: bool = False):
+        im = abi.nn_images(sizes)
+        b, n, a, d = self._frame(frame)
+        t = np.ascontiguousarray(target, np.float32)
+        loss = C.c_float()
+        grad = np.zeros(self.n, np.float32) if want_grad else None
+        check(lib().rt_nn_train_step_host(self.h, C.byref(im), ptr(b), ptr(n), ptr(a), ptr(d),
+                                          ptr(t), lr, C.byref(loss),
+                                          ptr(grad) if grad is not None else None))
+        return loss.value, grad
 
 
 class Generated:
@@ -294,6 +357,6 @@ class Generated:
         }
 
 
-__all__ = ["Device", "Scene", "Renderer", "Generated", "RtError", "lib", "abi", "EXPORTED",
+__all__ = ["Device", "Scene", "Renderer", "Denoiser", "Generated", "RtError", "lib", "abi", "EXPORTED",
            "RAY_DTYPE", "HIT_DTYPE", "NODE_DTYPE", "Camera", "CameraRig", "Profile",
            "FrameTiming", "SceneDesc", "Tuning"]

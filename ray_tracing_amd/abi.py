@@ -122,6 +122,23 @@ class RestirStats(C.Structure):
                 ("spatial_candidates", C.c_uint64), ("bias_correction_sources", C.c_uint64)]
 
 
+class NnImages(C.Structure):
+    _fields_ = [("count", C.c_int32), ("offset", C.c_int32 * 17), ("width", C.c_int32 * 16)]
+
+
+def nn_images(sizes) -> "NnImages":
+    """Image table for [(width, height), ...] laid back to back."""
+    im = NnImages()
+    im.count = len(sizes)
+    o = 0
+    for i, (w, h) in enumerate(sizes):
+        im.offset[i] = o
+        im.width[i] = w
+        o += w * h
+    im.offset[len(sizes)] = o
+    return im
+
+
 class DeviceFrameView(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("beauty", C.c_void_p),
                 ("normal", C.c_void_p), ("albedo", C.c_void_p), ("depth", C.c_void_p),
@@ -181,5 +198,16 @@ def declare(L):
     L.rt_scene_generate.argtypes = [C.c_int, C.c_uint64, P(P(GeneratedScene))]
     L.rt_scene_generated_free.argtypes = [P(GeneratedScene)]
     L.rt_restir_download.argtypes = [vp, C.c_int, vp]
+    L.rt_nn_param_count.argtypes = []
+    L.rt_nn_param_count.restype = C.c_int32
+    L.rt_nn_create.argtypes = [C.c_int, P(vp)]
+    L.rt_nn_destroy.argtypes = [vp]
+    L.rt_nn_set_params.argtypes = [vp, vp, C.c_int32]
+    L.rt_nn_get_params.argtypes = [vp, vp, C.c_int32]
+    L.rt_nn_infer.argtypes = [vp, P(NnImages), vp, vp, vp, vp, vp]
+    L.rt_nn_train_step.argtypes = [vp, P(NnImages), vp, vp, vp, vp, vp, C.c_float, vp, vp]
+    L.rt_nn_infer_host.argtypes = [vp, P(NnImages), vp, vp, vp, vp, vp]
+    L.rt_nn_train_step_host.argtypes = [vp, P(NnImages), vp, vp, vp, vp, vp, C.c_float, vp, vp]
+    L.rt_renderer_set_denoiser.argtypes = [vp, vp]
     L.rt_frame_download_guides.argtypes = [vp, C.c_int, vp, vp, vp]
     L.rt_scene_generate_rig.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, P(CameraRig)]

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "rt_host.h"
+#include "rt_nn_abi.h"
 #include "rt_kernels.h"
 #include "rt_openpbr.h"
 
@@ -26,6 +27,12 @@ rt_status fail(rt_status s, const std::string& msg) {
     g_error = msg;
     return s;
 }
+
+}  // namespace
+
+void rt_set_error(const char* msg) { g_error = msg; }
+
+namespace {
 
 #define RT_HIP(expr)                                                                        \
     do {                                                                                    \
@@ -190,6 +197,7 @@ struct rt_renderer {
     int frame_set = 0;
     int last_set = 0;
     bool last_denoised = false;
+    rt_nn* nn = nullptr;  // learned denoiser (replaces the a-trous filter when set)
     hipEvent_t post_ev[2] = {nullptr, nullptr};
     // wavefront state
     DevBuf<float4> ray_o, ray_d, hit, throughput, radiance, prev_pos, prev_n, aux_normal, aux_albedo;
@@ -989,9 +997,18 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
     const bool denoise = p->enable_denoise != 0;
     if (denoise) {
         if (timed) RT_HIP(hipEventRecord(r->post_ev[0], st));
-        RT_HIP(rtk::wf_denoise_filter(cv, rtk::kDenoiseIterations, r->beauty_set[fs].p,
-            r->normal_set[fs].p, r->albedo.p, r->depth_set[fs].p, r->denoised.p,
-            r->denoise_scratch.p, total, st));
+        if (r->nn != nullptr) {
+            rtk::NnImages im{};
+            im.count = rig->active_count;
+            for (int c = 0; c <= rig->active_count; ++c) im.offset[c] = r->pixel_offset[c];
+            for (int c = 0; c < rig->active_count; ++c) im.width[c] = r->widths[c];
+            RT_HIP(rt_nn_infer_on_stream(r->nn, im, total, r->beauty_set[fs].p, r->normal_set[fs].p,
+                r->albedo.p, r->depth_set[fs].p, r->denoised.p, st));
+        } else {
+            RT_HIP(rtk::wf_denoise_filter(cv, rtk::kDenoiseIterations, r->beauty_set[fs].p,
+                r->normal_set[fs].p, r->albedo.p, r->depth_set[fs].p, r->denoised.p,
+                r->denoise_scratch.p, total, st));
+        }
         if (timed) RT_HIP(hipEventRecord(r->post_ev[1], st));
     }
     if (timed) RT_HIP(hipEventRecord(r->events[1], st));
@@ -1085,6 +1102,14 @@ rt_status rt_frame_device_view(rt_renderer* r, int camera, rt_device_frame_view*
     view->resolved_beauty = reinterpret_cast<const float*>(r->beauty_set[r->last_set].p + off);
     view->flow = reinterpret_cast<const float*>(r->flow.p + off);
     view->trust = r->trust.p + off;
+    return RT_OK;
+}
+
+rt_status rt_renderer_set_denoiser(rt_renderer* r, rt_nn* nn) {
+    RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
+    RT_REQUIRE(nn == nullptr || rt_nn_device(nn) == r->dev->ordinal, RT_ERROR_INVALID_ARGUMENT,
+        "the denoiser lives on another device");
+    r->nn = nn;
     return RT_OK;
 }
 

@@ -1,0 +1,323 @@
+// rt_nn_api.hip -- C ABI of the learned denoiser: parameters, inference,
+// one training step (forward, backward, Adam), all on device buffers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "rt_nn.h"
+#include "rt_nn_abi.h"
+
+namespace {
+
+thread_local std::string g_nn_error;
+
+rt_status nn_fail(rt_status s, const std::string& msg) {
+    g_nn_error = msg;
+    rt_set_error(msg.c_str());
+    return s;
+}
+
+#define NN_HIP(expr)                                                                       \
+    do {                                                                                   \
+        const hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                              \
+            return nn_fail(RT_ERROR_HIP, std::string("HIP runtime failure at ") + #expr + ": " \
+                                             + hipGetErrorString(e_));                     \
+    } while (0)
+
+#define NN_REQUIRE(cond, status, msg) \
+    do {                              \
+        if (!(cond)) return nn_fail(status, msg); \
+    } while (0)
+
+template <typename T>
+struct Buf {
+    T* p = nullptr;
+    size_t n = 0;
+    Buf() = default;
+    Buf(const Buf&) = delete;
+    Buf& operator=(const Buf&) = delete;
+    ~Buf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t reserve(size_t c) {
+        if (c <= n && p) return hipSuccess;
+        release();
+        if (c == 0) c = 1;
+        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * c);
+        if (e == hipSuccess) n = c;
+        return e;
+    }
+};
+
+}  // namespace
+
+struct rt_nn {
+    int ordinal = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    int step = 0;
+    Buf<float> params, grad, m, v, loss;
+    Buf<uint16_t> fwd[rtk::kNnLayers], bwd[rtk::kNnLayers];
+    Buf<float> bias[rtk::kNnLayers];
+    Buf<float> head;
+    // per-batch scratch
+    size_t cap = 0;
+    Buf<uint16_t> feat, act[rtk::kNnLayers], dact[2];
+    Buf<float> head_partial, grad_partial;
+
+    rtk::NnPacked packed() {
+        rtk::NnPacked pk{};
+        for (int l = 0; l < rtk::kNnLayers; ++l) {
+            pk.fwd[l] = fwd[l].p;
+            pk.bwd[l] = bwd[l].p;
+            pk.bias[l] = bias[l].p;
+        }
+        pk.head = head.p;
+        return pk;
+    }
+
+    hipError_t ensure(size_t P) {
+        if (P <= cap) return hipSuccess;
+        hipError_t e = feat.reserve(P * rtk::kNnFeat);
+        for (int l = 0; l < rtk::kNnLayers && e == hipSuccess; ++l) e = act[l].reserve(P * rtk::kNnHidden);
+        if (e == hipSuccess) e = dact[0].reserve(P * rtk::kNnHidden);
+        if (e == hipSuccess) e = dact[1].reserve(P * rtk::kNnHidden);
+        if (e == hipSuccess) e = head_partial.reserve(((P + 255) / 256) * (rtk::kNnHeadParams + 1));
+        if (e == hipSuccess)
+            e = grad_partial.reserve(((P + rtk::kNnGradPixels - 1) / rtk::kNnGradPixels) * 32 * 19 * 16);
+        if (e == hipSuccess) cap = P;
+        return e;
+    }
+
+    rtk::NnScratch scratch() {
+        rtk::NnScratch w{};
+        w.feat = feat.p;
+        for (int l = 0; l < rtk::kNnLayers; ++l) w.act[l] = act[l].p;
+        w.dact[0] = dact[0].p;
+        w.dact[1] = dact[1].p;
+        w.head_partial = head_partial.p;
+        w.grad_partial = grad_partial.p;
+        return w;
+    }
+
+    int conv_grid() const { return std::max(cu_count, 1) * 8; }
+};
+
+namespace {
+
+rt_status check_images(const rt_nn_images* im, int* P) {
+    NN_REQUIRE(im != nullptr, RT_ERROR_INVALID_ARGUMENT, "null image table");
+    NN_REQUIRE(im->count >= 1 && im->count <= rtk::kNnMaxImages, RT_ERROR_INVALID_ARGUMENT,
+        "image count must be in [1, " + std::to_string(rtk::kNnMaxImages) + "]");
+    NN_REQUIRE(im->offset[0] == 0, RT_ERROR_INVALID_ARGUMENT, "images must start at pixel 0");
+    for (int i = 0; i < im->count; ++i) {
+        const int n = im->offset[i + 1] - im->offset[i];
+        NN_REQUIRE(im->width[i] > 0 && n > 0 && n % im->width[i] == 0, RT_ERROR_INVALID_ARGUMENT,
+            "image " + std::to_string(i) + " has an invalid extent");
+    }
+    *P = im->offset[im->count];
+    return RT_OK;
+}
+
+rtk::NnImages to_images(const rt_nn_images* im) {
+    rtk::NnImages r{};
+    r.count = im->count;
+    for (int i = 0; i <= im->count; ++i) r.offset[i] = im->offset[i];
+    for (int i = 0; i < im->count; ++i) r.width[i] = im->width[i];
+    return r;
+}
+
+}  // namespace
+
+hipError_t rt_nn_infer_on_stream(rt_nn* nn, const rtk::NnImages& images, int P,
+    const float4* beauty, const float4* normal, const float4* albedo, const float* depth,
+    float4* out, hipStream_t st) {
+    const hipError_t e = nn->ensure((size_t)P);
+    if (e != hipSuccess) return e;
+    return rtk::nn_forward(images, P, nn->packed(), nn->scratch(), beauty, normal, albedo, depth,
+        nullptr, out, nn->conv_grid(), st);
+}
+
+int rt_nn_device(const rt_nn* nn) { return nn->ordinal; }
+
+extern "C" {
+
+int32_t rt_nn_param_count(void) { return rtk::nn_param_count(); }
+
+rt_status rt_nn_create(int ordinal, rt_nn** out) {
+    NN_REQUIRE(out != nullptr, RT_ERROR_INVALID_ARGUMENT, "null output");
+    int count = 0;
+    NN_HIP(hipGetDeviceCount(&count));
+    NN_REQUIRE(ordinal >= 0 && ordinal < count, RT_ERROR_INVALID_ARGUMENT, "device ordinal out of range");
+    NN_HIP(hipSetDevice(ordinal));
+    auto* nn = new rt_nn();
+    nn->ordinal = ordinal;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, ordinal);
+    if (e == hipSuccess) {
+        nn->cu_count = prop.multiProcessorCount;
+        e = hipStreamCreateWithFlags(&nn->stream, hipStreamNonBlocking);
+    }
+    const size_t np = (size_t)rtk::nn_param_count();
+    for (Buf<float>* b : {&nn->params, &nn->grad, &nn->m, &nn->v}) {
+        if (e == hipSuccess) e = b->reserve(np);
+    }
+    if (e == hipSuccess) e = nn->loss.reserve(1);
+    for (int l = 0; l < rtk::kNnLayers && e == hipSuccess; ++l) {
+        const int cin = l == 0 ? rtk::kNnFeat : rtk::kNnHidden;
+        e = nn->fwd[l].reserve((size_t)32 * (((9 * cin + 31) / 32) * 32));
+        if (e == hipSuccess) e = nn->bwd[l].reserve((size_t)cin * 9 * 32);
+        if (e == hipSuccess) e = nn->bias[l].reserve(32);
+    }
+    if (e == hipSuccess) e = nn->head.reserve(rtk::kNnHeadParams);
+    if (e == hipSuccess) e = hipMemsetAsync(nn->params.p, 0, np * sizeof(float), nn->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(nn->m.p, 0, np * sizeof(float), nn->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(nn->v.p, 0, np * sizeof(float), nn->stream);
+    if (e == hipSuccess) e = rtk::nn_pack(nn->params.p, nn->packed(), nn->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(nn->stream);
+    if (e != hipSuccess) {
+        if (nn->stream) (void)hipStreamDestroy(nn->stream);
+        delete nn;
+        return nn_fail(RT_ERROR_HIP, std::string("HIP runtime failure at denoiser allocation: ")
+                                        + hipGetErrorString(e));
+    }
+    *out = nn;
+    return RT_OK;
+}
+
+rt_status rt_nn_destroy(rt_nn* nn) {
+    if (!nn) return RT_OK;
+    (void)hipSetDevice(nn->ordinal);
+    (void)hipStreamSynchronize(nn->stream);
+    hipStream_t st = nn->stream;
+    delete nn;
+    if (st) (void)hipStreamDestroy(st);
+    return RT_OK;
+}
+
+rt_status rt_nn_set_params(rt_nn* nn, const float* params, int32_t n) {
+    NN_REQUIRE(nn && params, RT_ERROR_INVALID_ARGUMENT, "null network or parameters");
+    NN_REQUIRE(n == rtk::nn_param_count(), RT_ERROR_INVALID_ARGUMENT,
+        "expected " + std::to_string(rtk::nn_param_count()) + " parameters");
+    NN_HIP(hipSetDevice(nn->ordinal));
+    const size_t bytes = sizeof(float) * (size_t)n;
+    NN_HIP(hipMemcpyAsync(nn->params.p, params, bytes, hipMemcpyHostToDevice, nn->stream));
+    NN_HIP(hipMemsetAsync(nn->m.p, 0, bytes, nn->stream));
+    NN_HIP(hipMemsetAsync(nn->v.p, 0, bytes, nn->stream));
+    NN_HIP(rtk::nn_pack(nn->params.p, nn->packed(), nn->stream));
+    NN_HIP(hipStreamSynchronize(nn->stream));
+    nn->step = 0;
+    return RT_OK;
+}
+
+rt_status rt_nn_get_params(rt_nn* nn, float* params, int32_t n) {
+    NN_REQUIRE(nn && params, RT_ERROR_INVALID_ARGUMENT, "null network or parameters");
+    NN_REQUIRE(n == rtk::nn_param_count(), RT_ERROR_INVALID_ARGUMENT,
+        "expected " + std::to_string(rtk::nn_param_count()) + " parameters");
+    NN_HIP(hipSetDevice(nn->ordinal));
+    NN_HIP(hipMemcpyAsync(params, nn->params.p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, nn->stream));
+    NN_HIP(hipStreamSynchronize(nn->stream));
+    return RT_OK;
+}
+
+rt_status rt_nn_infer(rt_nn* nn, const rt_nn_images* images, const float* beauty,
+    const float* normal, const float* albedo, const float* depth, float* out) {
+    NN_REQUIRE(nn != nullptr, RT_ERROR_INVALID_ARGUMENT, "null network");
+    int P = 0;
+    const rt_status s = check_images(images, &P);
+    if (s != RT_OK) return s;
+    NN_REQUIRE(beauty && normal && albedo && depth && out, RT_ERROR_INVALID_ARGUMENT, "null frame buffer");
+    NN_HIP(hipSetDevice(nn->ordinal));
+    NN_HIP(nn->ensure((size_t)P));
+    NN_HIP(rtk::nn_forward(to_images(images), P, nn->packed(), nn->scratch(),
+        reinterpret_cast<const float4*>(beauty), reinterpret_cast<const float4*>(normal),
+        reinterpret_cast<const float4*>(albedo), depth, nullptr, reinterpret_cast<float4*>(out),
+        nn->conv_grid(), nn->stream));
+    NN_HIP(hipStreamSynchronize(nn->stream));
+    return RT_OK;
+}
+
+rt_status rt_nn_train_step(rt_nn* nn, const rt_nn_images* images, const float* beauty,
+    const float* normal, const float* albedo, const float* depth, const float* target, float lr,
+    float* loss, float* grad) {
+    NN_REQUIRE(nn != nullptr, RT_ERROR_INVALID_ARGUMENT, "null network");
+    int P = 0;
+    const rt_status s = check_images(images, &P);
+    if (s != RT_OK) return s;
+    NN_REQUIRE(beauty && normal && albedo && depth && target, RT_ERROR_INVALID_ARGUMENT,
+        "null frame or target buffer");
+    NN_REQUIRE(lr >= 0.0f, RT_ERROR_INVALID_ARGUMENT, "negative learning rate");
+    NN_HIP(hipSetDevice(nn->ordinal));
+    NN_HIP(nn->ensure((size_t)P));
+    const rtk::NnImages im = to_images(images);
+    const rtk::NnPacked pk = nn->packed();
+    const rtk::NnScratch w = nn->scratch();
+    NN_HIP(rtk::nn_forward(im, P, pk, w, reinterpret_cast<const float4*>(beauty),
+        reinterpret_cast<const float4*>(normal), reinterpret_cast<const float4*>(albedo), depth,
+        reinterpret_cast<const float4*>(target), nullptr, nn->conv_grid(), nn->stream));
+    NN_HIP(rtk::nn_backward(im, P, pk, w, nn->grad.p, nn->loss.p, nn->conv_grid(), nn->stream));
+    const int np = rtk::nn_param_count();
+    if (grad) NN_HIP(hipMemcpyAsync(grad, nn->grad.p, sizeof(float) * (size_t)np, hipMemcpyDeviceToHost, nn->stream));
+    if (loss) NN_HIP(hipMemcpyAsync(loss, nn->loss.p, sizeof(float), hipMemcpyDeviceToHost, nn->stream));
+    if (lr > 0.0f) {
+        ++nn->step;
+        NN_HIP(rtk::nn_adam(np, nn->params.p, nn->grad.p, nn->m.p, nn->v.p, lr, nn->step, nn->stream));
+        NN_HIP(rtk::nn_pack(nn->params.p, pk, nn->stream));
+    }
+    NN_HIP(hipStreamSynchronize(nn->stream));
+    return RT_OK;
+}
+
+// host-array forms (tests, small batches): upload, run, download
+rt_status rt_nn_infer_host(rt_nn* nn, const rt_nn_images* images, const float* beauty,
+    const float* normal, const float* albedo, const float* depth, float* out) {
+    int P = 0;
+    const rt_status s = check_images(images, &P);
+    if (s != RT_OK) return s;
+    NN_REQUIRE(nn && beauty && normal && albedo && depth && out, RT_ERROR_INVALID_ARGUMENT, "null argument");
+    NN_HIP(hipSetDevice(nn->ordinal));
+    Buf<float> b, n, a, d, o;
+    NN_HIP(b.reserve((size_t)P * 4));
+    NN_HIP(n.reserve((size_t)P * 4));
+    NN_HIP(a.reserve((size_t)P * 4));
+    NN_HIP(d.reserve((size_t)P));
+    NN_HIP(o.reserve((size_t)P * 4));
+    NN_HIP(hipMemcpyAsync(b.p, beauty, (size_t)P * 16, hipMemcpyHostToDevice, nn->stream));
+    NN_HIP(hipMemcpyAsync(n.p, normal, (size_t)P * 16, hipMemcpyHostToDevice, nn->stream));
+    NN_HIP(hipMemcpyAsync(a.p, albedo, (size_t)P * 16, hipMemcpyHostToDevice, nn->stream));
+    NN_HIP(hipMemcpyAsync(d.p, depth, (size_t)P * 4, hipMemcpyHostToDevice, nn->stream));
+    const rt_status r = rt_nn_infer(nn, images, b.p, n.p, a.p, d.p, o.p);
+    if (r != RT_OK) return r;
+    NN_HIP(hipMemcpy(out, o.p, (size_t)P * 16, hipMemcpyDeviceToHost));
+    return RT_OK;
+}
+
+rt_status rt_nn_train_step_host(rt_nn* nn, const rt_nn_images* images, const float* beauty,
+    const float* normal, const float* albedo, const float* depth, const float* target, float lr,
+    float* loss, float* grad) {
+    int P = 0;
+    const rt_status s = check_images(images, &P);
+    if (s != RT_OK) return s;
+    NN_REQUIRE(nn && beauty && normal && albedo && depth && target, RT_ERROR_INVALID_ARGUMENT, "null argument");
+    NN_HIP(hipSetDevice(nn->ordinal));
+    Buf<float> b, n, a, d, t;
+    NN_HIP(b.reserve((size_t)P * 4));
+    NN_HIP(n.reserve((size_t)P * 4));
+    NN_HIP(a.reserve((size_t)P * 4));
+    NN_HIP(d.reserve((size_t)P));
+    NN_HIP(t.reserve((size_t)P * 4));
+    NN_HIP(hipMemcpyAsync(b.p, beauty, (size_t)P * 16, hipMemcpyHostToDevice, nn->stream));
+    NN_HIP(hipMemcpyAsync(n.p, normal, (size_t)P * 16, hipMemcpyHostToDevice, nn->stream));
+    NN_HIP(hipMemcpyAsync(a.p, albedo, (size_t)P * 16, hipMemcpyHostToDevice, nn->stream));
+    NN_HIP(hipMemcpyAsync(d.p, depth, (size_t)P * 4, hipMemcpyHostToDevice, nn->stream));
+    NN_HIP(hipMemcpyAsync(t.p, target, (size_t)P * 16, hipMemcpyHostToDevice, nn->stream));
+    return rt_nn_train_step(nn, images, b.p, n.p, a.p, d.p, t.p, lr, loss, grad);
+}
+
+}  // extern "C"

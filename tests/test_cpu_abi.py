@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_entry_points():
     src = open(os.path.join(ROOT, "include", "rt", "rt_hip.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:rt_status|int|const char\*)\s+(rt_\w+)\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:rt_status|int|int32_t|const char\*)\s+(rt_\w+)\(", src, re.M)))
 
 
 def test_exports_every_declared_symbol(rt):
@@ -25,7 +25,7 @@ def test_exports_every_declared_symbol(rt):
 
 
 def test_struct_layouts_match_oracle(rt, oracle):
-    pairs = [(rt.abi.Sphere, oracle.Sphere, 20), (rt.abi.Quad, oracle.Quad, 40),
+    pairs = [(rt.abi.NnImages, oracle.Images, 4 + 17 * 4 + 16 * 4), (rt.abi.Sphere, oracle.Sphere, 20), (rt.abi.Quad, oracle.Quad, 40),
              (rt.abi.Triangle, oracle.Triangle, 40), (rt.abi.Material, oracle.Material, 40),
              (rt.abi.OpenPbr, oracle.OpenPbr, 204), (rt.abi.Camera, oracle.Camera, None),
              (rt.abi.Profile, oracle.Profile, None), (rt.abi.SceneDesc, oracle.Scene, None)]

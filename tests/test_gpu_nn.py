@@ -1,0 +1,76 @@
+"""GPU parity of the learned denoiser's BF16 MFMA kernels against the CPU
+oracle: forward output, loss and the full parameter gradient (backward data
+and weight passes), plus a short training run that lowers the loss."""
+import sys
+
+import numpy as np
+import pytest
+
+from test_oracle_nn import random_batch, random_params
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+SIZES = [(40, 24), (17, 33), (64, 16)]
+
+
+def test_mfma_forward_matches_oracle(rt, oracle, device):
+    say("[gpu] nn forward vs oracle")
+    rng = np.random.default_rng(11)
+    im, frame, target = random_batch(rng, SIZES)
+    params = random_params(rng)
+    nn = rt.Denoiser(0, params)
+    out = nn.infer(SIZES, frame)
+    _, _, ref = oracle.nn_loss_grad(oracle.images(SIZES), params, frame, None, bf16=True,
+                                    want_grad=False)
+    a, b = out[:, :3].astype(np.float64), ref[:, :3].astype(np.float64)
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-2)
+    say(f"  max rel {rel.max():.2e}, mean {rel.mean():.2e}")
+    assert np.isfinite(a).all()
+    assert np.mean(rel < 2e-2) > 0.995 and rel.mean() < 2e-3
+
+
+def test_mfma_gradient_matches_oracle(rt, oracle, device):
+    say("[gpu] nn gradient vs oracle")
+    rng = np.random.default_rng(12)
+    im, frame, target = random_batch(rng, SIZES)
+    params = random_params(rng)
+    nn = rt.Denoiser(0, params)
+    loss, grad = nn.train_step(SIZES, frame, target, 0.0, want_grad=True)
+    rloss, rgrad, _ = oracle.nn_loss_grad(oracle.images(SIZES), params, frame, target, bf16=True)
+    say(f"  loss {loss:.6f} vs {rloss:.6f}")
+    assert abs(loss - rloss) <= 1e-3 * abs(rloss)
+    assert np.isfinite(grad).all()
+    err = np.linalg.norm(grad - rgrad) / np.linalg.norm(rgrad)
+    say(f"  gradient relative L2 error {err:.2e}")
+    assert err < 2e-2
+    # every layer's block of the gradient agrees, not just the large ones
+    offs = [0]
+    for cin in (16, 32, 32, 32):
+        offs.append(offs[-1] + 32 * 9 * cin + 32)
+    offs.append(offs[-1] + 99)
+    for i in range(len(offs) - 1):
+        g, r = grad[offs[i]:offs[i + 1]], rgrad[offs[i]:offs[i + 1]]
+        e = np.linalg.norm(g - r) / max(np.linalg.norm(r), 1e-12)
+        say(f"  block {i}: rel L2 {e:.2e}")
+        assert e < 5e-2
+    # the parameters are unchanged by a zero learning rate
+    assert np.array_equal(nn.params(), params)
+
+
+def test_training_lowers_loss(rt, oracle, device):
+    say("[gpu] nn training steps")
+    rng = np.random.default_rng(13)
+    im, frame, target = random_batch(rng, SIZES)
+    nn = rt.Denoiser(0, random_params(rng, 0.05))
+    first, _ = nn.train_step(SIZES, frame, target, 1e-3)
+    last = first
+    for _ in range(60):
+        last, _ = nn.train_step(SIZES, frame, target, 1e-3)
+    say(f"  loss {first:.5f} -> {last:.5f}")
+    assert last < 0.7 * first
