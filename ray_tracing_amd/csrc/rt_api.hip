@@ -172,6 +172,17 @@ struct rt_scene {
     }
 };
 
+// Everything a captured graph of the frame bakes in: the sequence and shape
+// of launches and every buffer pointer they receive.  A frame whose key
+// differs from the graph's is recaptured.
+struct GraphKey {
+    int32_t spp, max_b, rr, jitter, layers, total, count, restir, denoise, block, grid, refill,
+        count_traversal, fs, res_cur;
+    int32_t restir_params[10];
+    int32_t widths[4], heights[4];
+    const void* ptrs[8];
+};
+
 struct rt_renderer {
     rt_device* dev = nullptr;
     rt_scene* scene = nullptr;
@@ -198,6 +209,16 @@ struct rt_renderer {
     int last_set = 0;
     bool last_denoised = false;
     rt_nn* nn = nullptr;  // learned denoiser (replaces the a-trous filter when set)
+    // per-frame device values and captured frame graphs
+    DevBuf<rtk::FrameDyn> dyn;
+    rtk::FrameDyn* host_dyn = nullptr;  // pinned
+    struct Graph {
+        GraphKey key;
+        hipGraphExec_t exec = nullptr;
+    };
+    Graph graphs[4];
+    int graph_next = 0;
+    int graph_captures = 0;
     hipEvent_t post_ev[2] = {nullptr, nullptr};
     // wavefront state
     DevBuf<float4> ray_o, ray_d, hit, throughput, radiance, prev_pos, prev_n, aux_normal, aux_albedo;
@@ -227,6 +248,9 @@ struct rt_renderer {
     bool rendered = false;
 
     ~rt_renderer() {
+        for (Graph& g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (host_dyn) (void)hipHostFree(host_dyn);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         for (hipEvent_t e : post_ev)
             if (e) (void)hipEventDestroy(e);
@@ -774,6 +798,9 @@ rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out)
     hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&r->host_cams), sizeof(rt_camera) * 8,
         hipHostMallocDefault);
     if (e == hipSuccess) e = r->cams.reserve(8);
+    if (e == hipSuccess) e = r->dyn.reserve(1);
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&r->host_dyn), sizeof(rtk::FrameDyn), hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreate(&r->post_ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&r->post_ev[1]);
     if (e == hipSuccess) e = r->counters.reserve(32);
@@ -875,7 +902,7 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
             r->prev_valid[c] = 0;
         }
     }
-    RT_HIP(hipStreamSynchronize(st));  // host_cams may still feed a previous copy
+    RT_HIP(hipStreamSynchronize(st));  // host_cams / host_dyn may still feed a previous copy
     std::memcpy(r->host_cams, rig->cameras, sizeof(rt_camera) * 4);
     std::memcpy(r->host_cams + 4, r->prev_cams, sizeof(rt_camera) * 4);
     RT_HIP(hipMemcpyAsync(r->cams.p, r->host_cams, sizeof(rt_camera) * 8, hipMemcpyHostToDevice, st));
@@ -894,24 +921,36 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
     for (int c = 0; c < 5; ++c) cv.pixel_offset[c] = r->pixel_offset[c];
     for (int c = 0; c < 4; ++c) cv.width[c] = r->widths[c];
 
+    const int min_lights = p->restir_min_analytic_lights > 0 ? p->restir_min_analytic_lights : 1;
+    const int restir_on = p->enable_restir_di != 0 && s->analytic_count >= min_lights ? 1 : 0;
+    if (restir_on && r->reservoir_pixels < P) {
+        RT_HIP(r->reservoirs[0].reserve(P));
+        RT_HIP(r->reservoirs[1].reserve(P));
+        r->reservoir_pixels = P;
+        for (int c = 0; c < 4; ++c) r->history_length[c] = 0;
+    }
+
+    // per-frame values, read by the kernels from device memory
+    rtk::FrameDyn* dyn = r->host_dyn;
+    dyn->stream = sample_stream;
+    dyn->restir_history = r->restir_last ? 1 : 0;
+    for (int c = 0; c < 4; ++c) {
+        dyn->history_length[c] = r->history_length[c];
+        dyn->prev_valid[c] = r->prev_valid[c];
+    }
+    RT_HIP(hipMemcpyAsync(r->dyn.p, dyn, sizeof(rtk::FrameDyn), hipMemcpyHostToDevice, st));
+
     rtk::FrameParams fp{};
     fp.spp = spp;
     fp.max_bounces = max_b;
     fp.rr_start_bounce = p->rr_start_bounce;
     fp.jitter = p->pixel_jitter;
-    fp.stream = sample_stream;
+    fp.dyn = r->dyn.p;
     fp.total_pixels = total;
-    const int min_lights = p->restir_min_analytic_lights > 0 ? p->restir_min_analytic_lights : 1;
     rtk::RestirView rv{};
-    rv.enabled = p->enable_restir_di != 0 && s->analytic_count >= min_lights ? 1 : 0;
+    rv.enabled = restir_on;
     fp.restir = rv.enabled;
     if (rv.enabled) {
-        if (r->reservoir_pixels < P) {
-            RT_HIP(r->reservoirs[0].reserve(P));
-            RT_HIP(r->reservoirs[1].reserve(P));
-            r->reservoir_pixels = P;
-            for (int c = 0; c < 4; ++c) r->history_length[c] = 0;
-        }
         rv.initial_candidates = p->restir_initial_candidates;
         rv.temporal_reuse = p->restir_temporal_reuse;
         rv.max_history_age = p->restir_max_history_age;
@@ -922,12 +961,9 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
         rv.spatial_radius = p->restir_spatial_radius;
         rv.spp = spp;
         rv.current = r->reservoirs[r->res_cur].p;
-        rv.history = r->restir_last ? r->reservoirs[r->res_cur ^ 1].p : nullptr;
+        rv.history = r->reservoirs[r->res_cur ^ 1].p;
         rv.prev_cams = r->cams.p + 4;
-        for (int c = 0; c < 4; ++c) {
-            rv.history_length[c] = r->history_length[c];
-            rv.prev_valid[c] = r->prev_valid[c];
-        }
+        rv.dyn = r->dyn.p;
     }
 
     const rtk::SceneView sv = s->view();
@@ -941,10 +977,14 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
         RT_HIP(r->spill.reserve(entries * (size_t)trace_l.grid * (size_t)block));
         b.spill = r->spill.p;
     }
+    const bool denoise = p->enable_denoise != 0;
+    if (denoise && r->nn != nullptr) RT_HIP(rt_nn_reserve(r->nn, total));
 
     const int chunks = (spp + layers - 1) / layers;
     const size_t stage_events = (size_t)chunks * (size_t)max_b * 6 + 4;
     const bool timed = timing != nullptr;
+    const bool use_graph = r->tuning.use_graph != 0;
+    const bool stage_timing = timed && !use_graph;
     if (timed) {
         while (r->events.size() < stage_events) {
             hipEvent_t e;
@@ -952,64 +992,124 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
             r->events.push_back(e);
         }
     }
-    size_t ev = 2;
-    auto mark = [&](void) -> hipError_t {
-        if (!timed) return hipSuccess;
-        return hipEventRecord(r->events[ev++], st);
-    };
-    if (timed) RT_HIP(hipEventRecord(r->events[0], st));
-    RT_HIP(rtk::wf_reset_counters(b.counters, st));
-    RT_HIP(rtk::wf_clear_accum(total, b, st));
-    // the frame's reservoirs start cleared: pixels without a surface keep an
-    // empty reservoir (optix_renderer.cpp:256-257 clears them per launch)
-    if (rv.enabled) RT_HIP(rtk::wf_zero(rv.current, P * sizeof(rtk::Reservoir), st));
-    for (int s0 = 0; s0 < spp; s0 += layers) {
-        fp.layer_count = std::min(layers, spp - s0);
-        fp.first_sample = s0;
-        const int paths = fp.layer_count * total;
-        RT_HIP(rtk::wf_raygen(cv, fp, b, paths, st));
-        for (int bounce = 0; bounce < max_b; ++bounce) {
-            RT_HIP(mark());
-            RT_HIP(rtk::wf_extend(sv, b, bounce, r->tuning.count_traversal != 0, trace_l, st));
-            RT_HIP(mark());
-            RT_HIP(rtk::wf_shade(sv, cv, fp, b, bounce, shade_l, st));
-            if (fp.restir != 0 && bounce == 0)
-                RT_HIP(rtk::wf_restir(sv, cv, fp, rv, b, trace_l, st));
-            RT_HIP(mark());
-            RT_HIP(rtk::wf_connect(sv, b, bounce, trace_l, st));
-            RT_HIP(mark());
-        }
-        RT_HIP(rtk::wf_accumulate(fp, b, st));
-    }
     const int fs = r->frame_set, hs = fs ^ 1;
-    RT_HIP(rtk::wf_finalize(total, spp, b, r->beauty_set[fs].p, r->normal_set[fs].p, r->albedo.p,
-        r->depth_set[fs].p, st));
-    // launch_resolve_kernel (optix_renderer.cpp:263): cameras without
-    // history only clear their flow / trust
-    rtk::ResolveParams rp{};
-    for (int c = 0; c < 4; ++c) {
-        rp.history_length[c] = r->history_length[c];
-        rp.prev_valid[c] = r->prev_valid[c];
-    }
-    rp.restir_enabled = rv.enabled;
-    RT_HIP(rtk::wf_resolve(cv, rp, r->beauty_set[fs].p, r->normal_set[fs].p, r->depth_set[fs].p,
-        r->beauty_set[hs].p, r->normal_set[hs].p, r->depth_set[hs].p, r->flow.p, r->trust.p, total, st));
-    const bool denoise = p->enable_denoise != 0;
-    if (denoise) {
-        if (timed) RT_HIP(hipEventRecord(r->post_ev[0], st));
-        if (r->nn != nullptr) {
-            rtk::NnImages im{};
-            im.count = rig->active_count;
-            for (int c = 0; c <= rig->active_count; ++c) im.offset[c] = r->pixel_offset[c];
-            for (int c = 0; c < rig->active_count; ++c) im.width[c] = r->widths[c];
-            RT_HIP(rt_nn_infer_on_stream(r->nn, im, total, r->beauty_set[fs].p, r->normal_set[fs].p,
-                r->albedo.p, r->depth_set[fs].p, r->denoised.p, st));
-        } else {
-            RT_HIP(rtk::wf_denoise_filter(cv, rtk::kDenoiseIterations, r->beauty_set[fs].p,
-                r->normal_set[fs].p, r->albedo.p, r->depth_set[fs].p, r->denoised.p,
-                r->denoise_scratch.p, total, st));
+    size_t ev = 2;
+    // every launch of the frame, in order (captured once per key in graph mode)
+    auto enqueue = [&](bool marks) -> hipError_t {
+        hipError_t e;
+        auto mark = [&]() -> hipError_t { return marks ? hipEventRecord(r->events[ev++], st) : hipSuccess; };
+#define ENQ(x) \
+    if ((e = (x)) != hipSuccess) return e
+        ENQ(rtk::wf_reset_counters(b.counters, st));
+        ENQ(rtk::wf_clear_accum(total, b, st));
+        // the frame's reservoirs start cleared: pixels without a surface keep
+        // an empty reservoir (optix_renderer.cpp:256-257 clears them per launch)
+        if (rv.enabled) ENQ(rtk::wf_zero(rv.current, P * sizeof(rtk::Reservoir), st));
+        rtk::FrameParams f = fp;
+        for (int s0 = 0; s0 < spp; s0 += layers) {
+            f.layer_count = std::min(layers, spp - s0);
+            f.first_sample = s0;
+            const int paths = f.layer_count * total;
+            ENQ(rtk::wf_raygen(cv, f, b, paths, st));
+            for (int bounce = 0; bounce < max_b; ++bounce) {
+                ENQ(mark());
+                ENQ(rtk::wf_extend(sv, b, bounce, r->tuning.count_traversal != 0, trace_l, st));
+                ENQ(mark());
+                ENQ(rtk::wf_shade(sv, cv, f, b, bounce, shade_l, st));
+                if (f.restir != 0 && bounce == 0) ENQ(rtk::wf_restir(sv, cv, f, rv, b, trace_l, st));
+                ENQ(mark());
+                ENQ(rtk::wf_connect(sv, b, bounce, trace_l, st));
+                ENQ(mark());
+            }
+            ENQ(rtk::wf_accumulate(f, b, st));
         }
-        if (timed) RT_HIP(hipEventRecord(r->post_ev[1], st));
+        ENQ(rtk::wf_finalize(total, spp, b, r->beauty_set[fs].p, r->normal_set[fs].p, r->albedo.p,
+            r->depth_set[fs].p, st));
+        // launch_resolve_kernel (optix_renderer.cpp:263): cameras without
+        // history only clear their flow / trust
+        rtk::ResolveParams rp{};
+        rp.dyn = r->dyn.p;
+        rp.restir_enabled = rv.enabled;
+        ENQ(rtk::wf_resolve(cv, rp, r->beauty_set[fs].p, r->normal_set[fs].p, r->depth_set[fs].p,
+            r->beauty_set[hs].p, r->normal_set[hs].p, r->depth_set[hs].p, r->flow.p, r->trust.p,
+            total, st));
+        if (denoise) {
+            if (marks) ENQ(hipEventRecord(r->post_ev[0], st));
+            if (r->nn != nullptr) {
+                rtk::NnImages im{};
+                im.count = rig->active_count;
+                for (int c = 0; c <= rig->active_count; ++c) im.offset[c] = r->pixel_offset[c];
+                for (int c = 0; c < rig->active_count; ++c) im.width[c] = r->widths[c];
+                ENQ(rt_nn_infer_on_stream(r->nn, im, total, r->beauty_set[fs].p,
+                    r->normal_set[fs].p, r->albedo.p, r->depth_set[fs].p, r->denoised.p, st));
+            } else {
+                ENQ(rtk::wf_denoise_filter(cv, rtk::kDenoiseIterations, r->beauty_set[fs].p,
+                    r->normal_set[fs].p, r->albedo.p, r->depth_set[fs].p, r->denoised.p,
+                    r->denoise_scratch.p, total, st));
+            }
+            if (marks) ENQ(hipEventRecord(r->post_ev[1], st));
+        }
+#undef ENQ
+        return hipSuccess;
+    };
+
+    if (timed) RT_HIP(hipEventRecord(r->events[0], st));
+    if (use_graph) {
+        GraphKey key;
+        std::memset(&key, 0, sizeof(key));
+        key.spp = spp;
+        key.max_b = max_b;
+        key.rr = fp.rr_start_bounce;
+        key.jitter = fp.jitter;
+        key.layers = layers;
+        key.total = total;
+        key.count = rig->active_count;
+        key.restir = rv.enabled;
+        key.denoise = denoise ? 1 : 0;
+        key.block = block;
+        key.grid = trace_l.grid;
+        key.refill = trace_l.refill_min;
+        key.count_traversal = r->tuning.count_traversal;
+        key.fs = fs;
+        key.res_cur = r->res_cur;
+        const int32_t rp10[10] = {rv.initial_candidates, rv.temporal_reuse, rv.max_history_age,
+            rv.max_temporal_candidates, rv.spatial_neighbors, rv.max_spatial_candidates,
+            rv.bias_correction, rv.spatial_radius, rv.spp, s->analytic_count};
+        std::memcpy(key.restir_params, rp10, sizeof(rp10));
+        for (int c = 0; c < 4; ++c) {
+            key.widths[c] = r->widths[c];
+            key.heights[c] = r->heights[c];
+        }
+        const void* ptrs[8] = {b.ray_o, b.acc_beauty, r->beauty_set[0].p, r->reservoirs[0].p,
+            b.spill, r->nn, sv.nodes4, r->denoised.p};
+        std::memcpy(key.ptrs, ptrs, sizeof(ptrs));
+        rt_renderer::Graph* g = nullptr;
+        for (auto& gr : r->graphs) {
+            if (gr.exec && std::memcmp(&gr.key, &key, sizeof(key)) == 0) g = &gr;
+        }
+        if (g == nullptr) {
+            // evict the oldest of the cached graphs
+            g = &r->graphs[r->graph_next];
+            r->graph_next = (r->graph_next + 1) % (int)(sizeof(r->graphs) / sizeof(r->graphs[0]));
+            if (g->exec) {
+                (void)hipGraphExecDestroy(g->exec);
+                g->exec = nullptr;
+            }
+            hipGraph_t graph = nullptr;
+            RT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const hipError_t ce = enqueue(false);
+            const hipError_t ee = hipStreamEndCapture(st, &graph);
+            RT_HIP(ce);
+            RT_HIP(ee);
+            const hipError_t ie = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            RT_HIP(ie);
+            g->key = key;
+            ++r->graph_captures;
+        }
+        RT_HIP(hipGraphLaunch(g->exec, st));
+    } else {
+        RT_HIP(enqueue(stage_timing));
     }
     if (timed) RT_HIP(hipEventRecord(r->events[1], st));
     uint32_t stats[16];
@@ -1037,7 +1137,7 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
         RT_HIP(hipEventElapsedTime(&ms, r->events[0], r->events[1]));
         timing->render_ms = ms;
         double ext = 0.0, shd = 0.0, con = 0.0;
-        for (size_t i = 2; i + 3 < ev; i += 4) {
+        for (size_t i = 2; stage_timing && i + 3 < ev; i += 4) {
             float a = 0.0f, bb = 0.0f, c = 0.0f;
             RT_HIP(hipEventElapsedTime(&a, r->events[i], r->events[i + 1]));
             RT_HIP(hipEventElapsedTime(&bb, r->events[i + 1], r->events[i + 2]));
@@ -1050,9 +1150,9 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
         timing->shade_ms = shd;
         timing->connect_ms = con;
         float dn = 0.0f;
-        if (denoise) RT_HIP(hipEventElapsedTime(&dn, r->post_ev[0], r->post_ev[1]));
+        if (denoise && stage_timing) RT_HIP(hipEventElapsedTime(&dn, r->post_ev[0], r->post_ev[1]));
         timing->denoise_ms = dn;
-        timing->other_ms = std::max(0.0, timing->render_ms - ext - shd - con - (double)dn);
+        timing->other_ms = stage_timing ? std::max(0.0, timing->render_ms - ext - shd - con - (double)dn) : 0.0;
         uint64_t ex = 0, sh = 0;
         std::memcpy(&ex, stats + 4, 8);
         std::memcpy(&sh, stats + 6, 8);

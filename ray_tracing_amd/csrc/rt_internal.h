@@ -97,6 +97,16 @@ struct Reservoir {
         bias_correction_source_count, age, valid;
 };
 
+// Values that change from frame to frame, read by the kernels from device
+// memory so that a captured graph of the frame's launches stays valid: the
+// host uploads them before every frame (eager or graph replay).
+struct FrameDyn {
+    uint32_t stream;            // sample stream (RNG key)
+    int32_t restir_history;     // the previous frame ran ReSTIR (its reservoirs are history)
+    int32_t history_length[4];  // frames of history per camera (capture_launch_history)
+    int32_t prev_valid[4];      // previous camera valid per camera
+};
+
 // per-frame ReSTIR DI parameters and per-camera history
 struct RestirView {
     int32_t enabled;          // profile + light-count rule of radiance_launch_setup.cpp:101-106
@@ -110,10 +120,9 @@ struct RestirView {
     int32_t spatial_radius;
     int32_t spp;
     Reservoir* current;             // all cameras, indexed by global pixel
-    const Reservoir* history;       // previous frame, same indexing
+    const Reservoir* history;       // previous frame, same indexing (valid if dyn->restir_history)
     const rt_camera* prev_cams;     // previous camera per slot (device)
-    int32_t history_length[4];
-    int32_t prev_valid[4];
+    const FrameDyn* dyn;
 };
 
 // per-camera constants for ray generation (double camera math as the

@@ -81,7 +81,7 @@ struct FrameParams {
     int32_t max_bounces;
     int32_t rr_start_bounce;
     int32_t jitter;
-    uint32_t stream;
+    const FrameDyn* dyn;    // per-frame values in device memory
     int32_t total_pixels;
     int32_t layer_count;    // samples per pixel in this chunk
     int32_t first_sample;   // sample index of the chunk's first layer
@@ -113,8 +113,7 @@ hipError_t wf_zero(void* p, size_t bytes, hipStream_t st);
 // temporal reprojection of every camera with history (per camera: frames of
 // history so far, previous camera valid; previous cameras at cams.cams[4 + c])
 struct ResolveParams {
-    int32_t history_length[4];
-    int32_t prev_valid[4];
+    const FrameDyn* dyn;    // history lengths / previous-camera flags
     int32_t restir_enabled;
 };
 hipError_t wf_resolve(const CameraView& cams, const ResolveParams& rp, float4* beauty,

@@ -146,6 +146,8 @@ hipError_t rt_nn_infer_on_stream(rt_nn* nn, const rtk::NnImages& images, int P,
 
 int rt_nn_device(const rt_nn* nn) { return nn->ordinal; }
 
+hipError_t rt_nn_reserve(rt_nn* nn, int P) { return nn->ensure((size_t)P); }
+
 extern "C" {
 
 int32_t rt_nn_param_count(void) { return rtk::nn_param_count(); }

@@ -46,8 +46,8 @@ __global__ __launch_bounds__(256) void k_resolve(CameraView cams, ResolveParams 
     const PixelId id = pixel_id(cams, p);
     flow[p] = make_float2(0.0f, 0.0f);
     trust[p] = 0.0f;
-    const int hl = rp.history_length[id.camera];
-    if (hl <= 0 || rp.prev_valid[id.camera] == 0) return;
+    const int hl = rp.dyn->history_length[id.camera];
+    if (hl <= 0 || rp.dyn->prev_valid[id.camera] == 0) return;
     const rt_camera& cam = cams.cams[id.camera];
     const rt_camera& pc = cams.cams[4 + id.camera];
     const float pxf = (float)id.x + 0.5f, pyf = (float)id.y + 0.5f;

@@ -161,7 +161,9 @@ __device__ inline f3 rs_numerator(const SceneView& s, const HitInfo& h, f3 ray_d
 __device__ inline bool rs_previous_pixel(const RestirView& rv, int cam, int width, int height,
     f3 pos, int* index) {
     *index = -1;
-    if (rv.history == nullptr || rv.history_length[cam] <= 0 || rv.prev_valid[cam] == 0) return false;
+    if (rv.history == nullptr || rv.dyn->restir_history == 0 || rv.dyn->history_length[cam] <= 0
+        || rv.dyn->prev_valid[cam] == 0)
+        return false;
     const rt_camera& pc = rv.prev_cams[cam];
     const f3 origin = mk3((float)pc.origin[0], (float)pc.origin[1], (float)pc.origin[2]);
     const f3 off = pos - origin;
@@ -182,7 +184,7 @@ __device__ inline bool rs_previous_pixel(const RestirView& rv, int cam, int widt
 template <int BLOCK>
 __device__ inline float rs_target_at_source(const SceneView& s, const RestirView& rv, int cam,
     const Reservoir& src, const Reservoir& sel, bool bsdf_avail, const TraceStack& st) {
-    if (rv.prev_valid[cam] == 0) return 0.0f;
+    if (rv.dyn->prev_valid[cam] == 0) return 0.0f;
     const rt_camera& pc = rv.prev_cams[cam];
     const f3 sp = mk3(src.position[0], src.position[1], src.position[2]);
     const f3 origin = mk3((float)pc.origin[0], (float)pc.origin[1], (float)pc.origin[2]);

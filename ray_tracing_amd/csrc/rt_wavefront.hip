@@ -93,6 +93,7 @@ __global__ void k_zero_words(uint32_t* p, size_t n) {
 
 __global__ __launch_bounds__(256) void k_raygen(CameraView cams, FrameParams fp, WavefrontBuffers b,
     int paths) {
+    const uint32_t fp_stream = fp.dyn->stream;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         b.counters[kCntQueue0] = (uint32_t)paths;
@@ -112,7 +113,7 @@ __global__ __launch_bounds__(256) void k_raygen(CameraView cams, FrameParams fp,
     const int x = (int)id.pixel - y * w;
     double px, py;
     if (fp.jitter) {
-        const float4 r = rng4(id.pixel, id.sample, id.camera, 0u, kGroupCamera, fp.stream);
+        const float4 r = rng4(id.pixel, id.sample, id.camera, 0u, kGroupCamera, fp_stream);
         px = (double)((float)x + r.x);
         py = (double)((float)y + r.y);
     } else {
@@ -180,6 +181,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneView s, WavefrontBuffers 
 
 __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, FrameParams fp,
     WavefrontBuffers b, int bounce, int cur) {
+    const uint32_t fp_stream = fp.dyn->stream;
     const uint32_t count = b.counters[cur];
     const uint32_t* queue = b.queue[cur];
     uint32_t* next_queue = b.queue[cur ^ 1];
@@ -221,9 +223,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                 rad = rad + thr * (background_at(s, rd) * w);
             } else {
                 const float4 rl = rng4(id.pixel, id.sample, id.camera, (uint32_t)bounce,
-                    kGroupLight, fp.stream);
+                    kGroupLight, fp_stream);
                 const float4 rb = rng4(id.pixel, id.sample, id.camera, (uint32_t)bounce,
-                    kGroupBsdf, fp.stream);
+                    kGroupBsdf, fp_stream);
                 HitInfo h;
                 make_hit(s, ro, rd, hv.x, prim, h);
                 if (bounce == 0) {
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                     }
                 } else if (h.mat_type != RT_MAT_DIFFUSE_LIGHT && s.analytic_count > 0) {
                     const float4 ra = rng4(id.pixel, id.sample, id.camera, (uint32_t)bounce,
-                        kGroupAnalytic, fp.stream);
+                        kGroupAnalytic, fp_stream);
                     const int li = pick_analytic(s, ra.x);
                     const LightSample ls = analytic_candidate(s, h.pos, li, ra.y, ra.z);
                     if (ls.valid) {
@@ -350,6 +352,7 @@ __global__ __launch_bounds__(256) void k_analytic_hits(SceneView s, WavefrontBuf
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, FrameParams fp,
     RestirView rv, WavefrontBuffers b) {
+    const uint32_t fp_stream = fp.dyn->stream;
     __shared__ uint2 lds[kStackLds * BLOCK];
     const uint32_t count = b.counters[kCntRestir];
     const TraceStack st = lane_stack<BLOCK>(lds, b.spill);
@@ -377,7 +380,7 @@ __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, 
             const bool bsdf_avail = 1 < fp.max_bounces;
             const RestirOut ro = restir_shade<BLOCK>(s, rv, h, xyz(d4), (int)id.camera,
                 cams.pixel_offset[id.camera], x, y, w, hgt,
-                id.pixel, id.sample, fp.stream, bsdf_avail, out, st);
+                id.pixel, id.sample, fp_stream, bsdf_avail, out, st);
             if (ro.shadow) {
                 shadow = true;
                 s_o = f4(ro.origin, ro.max_t);

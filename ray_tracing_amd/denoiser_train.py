@@ -161,14 +161,14 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "weights", "denoiser.npy"))
     ap.add_argument("--report", default="")
-    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--crop", type=int, default=128)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--target-spp", type=int, default=1024)
     ap.add_argument("--ref-spp", type=int, default=4096)
-    ap.add_argument("--spheres-seeds", type=int, default=6)
-    ap.add_argument("--instanced-seeds", type=int, default=2)
+    ap.add_argument("--spheres-seeds", type=int, default=10)
+    ap.add_argument("--instanced-seeds", type=int, default=4)
     a = ap.parse_args()
     dev = rt.Device(0)
     pairs = make_pairs(dev, "random_spheres", range(a.spheres_seeds), a.target_spp)
