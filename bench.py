@@ -94,6 +94,9 @@ def main():
                     help="only run a plain 1 GiB device-to-device copy (3 times), the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tuning instead of running the autotuner first")
+    ap.add_argument("--graph", type=int, default=-1, help="force hipGraph capture off (0) or on (1)")
     ap.add_argument("--refill", type=int, default=0,
                     help="idle lanes before a traversal wave takes new rays (1..64)")
     a = ap.parse_args()
@@ -131,6 +134,8 @@ def main():
         tune["trace_blocks_per_cu"] = a.blocks_per_cu
     if a.refill:
         tune["trace_refill"] = a.refill
+    if a.graph >= 0:
+        tune["use_graph"] = a.graph
     if tune:
         ren.set_tuning(**tune)
     rig = g.rig()
@@ -150,10 +155,22 @@ def main():
         f"lbvh depth={bst.max_depth} 4-wide nodes={bst.wide_node_count} levels={bst.wide_levels} "
         f"build={bst.build_ms:.2f} ms paths/frame={paths_per_frame}")
 
+    autotune = None
+    if not a.no_autotune:
+        autotune = ren.autotune(rig_for(0), prof, 2)
+        log(f"[bench] autotune: {autotune['trials']} trials, {autotune['baseline_ms']:.3f} -> "
+            f"{autotune['best_ms']:.3f} ms, tuning {autotune['tuning']}")
+    # the counting frame needs per-stage events: it runs without the graph
+    run_tuning = ren.tuning()
+
     # traversal counts for the algorithmic-bytes roofline (separate, untimed frame)
-    ren.set_tuning(count_traversal=1)
+    ren.set_tuning(count_traversal=1, use_graph=0)
     cnt = ren.render(rig, prof, 0)
     ren.set_tuning(count_traversal=0)
+    # stage times (and the roofline's per-launch duration) come from one
+    # eager frame with per-stage events; the timed steps use the chosen tuning
+    stage = ren.render(rig_for(0), prof, 999)
+    ren.set_tuning(use_graph=run_tuning["use_graph"])
     nonsphere = cnt.prim_tests - cnt.sphere_tests
     extend_bytes = (cnt.extend_rays * (RAY_B + HIT_B) + cnt.node_visits * NODE_B
                     + nonsphere * (PRIM_IDX_B + TRI_B) + cnt.sphere_tests * (PRIM_IDX_B + SPHERE_B))
@@ -173,7 +190,7 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / a.steps
     value = paths_per_frame * a.steps / wall / 1e6
-    ext_ms = sum(f.extend_ms for f in frames) / a.steps
+    ext_ms = stage.extend_ms
     launches = frames[0].extend_launches
     avg_launch_ms = ext_ms / max(launches, 1)
     achieved = extend_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
@@ -200,11 +217,12 @@ def main():
                    "max_bounces": prof.max_bounces, "paths_per_frame": paths_per_frame,
                    "gpu_ms_per_frame": round(gpu_ms, 3),
                    "extend_ms_per_frame": round(ext_ms, 3),
-                   "shade_ms_per_frame": round(sum(f.shade_ms for f in frames) / a.steps, 3),
-                   "connect_ms_per_frame": round(sum(f.connect_ms for f in frames) / a.steps, 3),
+                   "shade_ms_per_frame": round(stage.shade_ms, 3),
+                   "connect_ms_per_frame": round(stage.connect_ms, 3),
+                   "stage_times_from": "one eager frame with per-stage events",
                    "extend_rays_per_frame": cnt.extend_rays,
                    "shadow_rays_per_frame": cnt.shadow_rays,
-                   "tuning": ren.tuning()},
+                   "tuning": ren.tuning(), "autotune": autotune},
         "roofline": {"bound": "hbm", "kernel": "k_extend (closest-hit traversal)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

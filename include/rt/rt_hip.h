@@ -336,6 +336,21 @@ typedef struct rt_tuning {
 rt_status rt_renderer_set_tuning(rt_renderer* r, const rt_tuning* t);
 rt_status rt_renderer_get_tuning(rt_renderer* r, rt_tuning* t);
 
+/* Autotuner: coordinate descent over traversal block shape, persistent blocks
+ * per CU, refill threshold, paths in flight and graph capture, each candidate
+ * timed on `frames_per_trial` frames of this rig and profile (median device
+ * time after one warm-up frame).  Applies the best tuning and resets the
+ * accumulation history. */
+typedef struct rt_autotune_result {
+    rt_tuning best;
+    double best_ms;        /* median frame time of the chosen tuning */
+    double baseline_ms;    /* median frame time of the tuning before the search */
+    int32_t trials;
+    int32_t pad;
+} rt_autotune_result;
+rt_status rt_renderer_autotune(rt_renderer* r, const rt_camera_rig* rig, const rt_profile* p,
+    int32_t frames_per_trial, rt_autotune_result* out);
+
 /* ------------------------------------------------- synthetic scenes
  * Seeded generators of the benchmark scenes (this repository's equivalent
  * of build_scene(id) + make_rig, src/scene/shared_scene_builders.h:56-57):

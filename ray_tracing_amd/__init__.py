@@ -34,7 +34,7 @@ EXPORTED = [
     "rt_scene_generate_rig", "rt_frame_download_guides", "rt_bvh_download_wide",
     "rt_nn_param_count", "rt_nn_create", "rt_nn_destroy", "rt_nn_set_params", "rt_nn_get_params",
     "rt_nn_infer", "rt_nn_train_step", "rt_nn_infer_host", "rt_nn_train_step_host",
-    "rt_renderer_set_denoiser",
+    "rt_renderer_set_denoiser", "rt_renderer_autotune",
 ]
 
 
@@ -209,6 +209,13 @@ class Renderer:
         check(lib().rt_frame_download_guides(self.h, camera, ptr(out["flow"]), ptr(out["trust"]),
                                              ptr(out["resolved"])))
         return out
+
+    def autotune(self, rig: CameraRig, profile: Profile, frames_per_trial: int = 2) -> dict:
+        res = abi.AutotuneResult()
+        check(lib().rt_renderer_autotune(self.h, C.byref(rig), C.byref(profile), frames_per_trial,
+                                         C.byref(res)))
+        return {"baseline_ms": res.baseline_ms, "best_ms": res.best_ms, "trials": res.trials,
+                "tuning": {k: getattr(res.best, k) for k, _ in Tuning._fields_}}
 
     def set_denoiser(self, nn: "Denoiser | None"):
         self._nn = nn  # keeps the network alive while attached

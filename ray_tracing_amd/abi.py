@@ -122,6 +122,11 @@ class RestirStats(C.Structure):
                 ("spatial_candidates", C.c_uint64), ("bias_correction_sources", C.c_uint64)]
 
 
+class AutotuneResult(C.Structure):
+    _fields_ = [("best", Tuning), ("best_ms", C.c_double), ("baseline_ms", C.c_double),
+                ("trials", C.c_int32), ("pad", C.c_int32)]
+
+
 class NnImages(C.Structure):
     _fields_ = [("count", C.c_int32), ("offset", C.c_int32 * 17), ("width", C.c_int32 * 16)]
 
@@ -209,5 +214,6 @@ def declare(L):
     L.rt_nn_infer_host.argtypes = [vp, P(NnImages), vp, vp, vp, vp, vp]
     L.rt_nn_train_step_host.argtypes = [vp, P(NnImages), vp, vp, vp, vp, vp, C.c_float, vp, vp]
     L.rt_renderer_set_denoiser.argtypes = [vp, vp]
+    L.rt_renderer_autotune.argtypes = [vp, P(CameraRig), P(Profile), C.c_int32, P(AutotuneResult)]
     L.rt_frame_download_guides.argtypes = [vp, C.c_int, vp, vp, vp]
     L.rt_scene_generate_rig.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, P(CameraRig)]

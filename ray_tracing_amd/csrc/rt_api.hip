@@ -841,6 +841,74 @@ rt_status rt_renderer_set_tuning(rt_renderer* r, const rt_tuning* t) {
     return RT_OK;
 }
 
+rt_status rt_renderer_autotune(rt_renderer* r, const rt_camera_rig* rig, const rt_profile* p,
+    int32_t frames_per_trial, rt_autotune_result* out) {
+    RT_REQUIRE(r && rig && p && out, RT_ERROR_INVALID_ARGUMENT, "null renderer, rig, profile or result");
+    RT_REQUIRE(frames_per_trial >= 1 && frames_per_trial <= 64, RT_ERROR_INVALID_ARGUMENT,
+        "frames_per_trial must be in [1, 64]");
+    std::memset(out, 0, sizeof(*out));
+    int trials = 0;
+    uint32_t stream = 0x7A5E0000u;
+    // median device time of a tuning (one warm-up frame first)
+    auto measure = [&](const rt_tuning& t, double* ms) -> rt_status {
+        r->tuning = t;
+        rt_frame_timing tm{};
+        rt_status st = rt_render_frame(r, rig, p, stream++, nullptr);
+        if (st != RT_OK) return st;
+        std::vector<double> v;
+        for (int i = 0; i < frames_per_trial; ++i) {
+            st = rt_render_frame(r, rig, p, stream++, &tm);
+            if (st != RT_OK) return st;
+            v.push_back(tm.render_ms);
+        }
+        std::sort(v.begin(), v.end());
+        *ms = v[v.size() / 2];
+        ++trials;
+        return RT_OK;
+    };
+    rt_tuning best = r->tuning;
+    best.count_traversal = 0;
+    double best_ms = 0.0;
+    rt_status st = measure(best, &best_ms);
+    if (st != RT_OK) return st;
+    out->baseline_ms = best_ms;
+    // candidates per knob; paths in flight bounded to 1..16 Mi
+    const int blocks[] = {64, 128, 256};
+    const int per_cu[] = {4, 6, 8, 10, 12, 16};
+    const int refill[] = {4, 16, 64};
+    const int paths[] = {1 << 20, 1 << 21, 1 << 22, 1 << 23};
+    const int graph[] = {0, 1};
+    for (int pass = 0; pass < 2; ++pass) {
+        const rt_tuning start = best;
+        auto sweep = [&](const int* vals, int n, int32_t rt_tuning::*field) -> rt_status {
+            for (int i = 0; i < n; ++i) {
+                if (best.*field == vals[i]) continue;
+                rt_tuning t = best;
+                t.*field = vals[i];
+                double ms = 0.0;
+                const rt_status s2 = measure(t, &ms);
+                if (s2 != RT_OK) return s2;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = t;
+                }
+            }
+            return RT_OK;
+        };
+        if ((st = sweep(blocks, 3, &rt_tuning::trace_block)) != RT_OK) return st;
+        if ((st = sweep(per_cu, 6, &rt_tuning::trace_blocks_per_cu)) != RT_OK) return st;
+        if ((st = sweep(refill, 3, &rt_tuning::trace_refill)) != RT_OK) return st;
+        if ((st = sweep(paths, 4, &rt_tuning::wavefront_paths)) != RT_OK) return st;
+        if ((st = sweep(graph, 2, &rt_tuning::use_graph)) != RT_OK) return st;
+        if (std::memcmp(&start, &best, sizeof(best)) == 0) break;
+    }
+    r->tuning = best;
+    out->best = best;
+    out->best_ms = best_ms;
+    out->trials = trials;
+    return rt_reset_accumulation(r);
+}
+
 rt_status rt_renderer_get_tuning(rt_renderer* r, rt_tuning* t) {
     RT_REQUIRE(r && t, RT_ERROR_INVALID_ARGUMENT, "null renderer or tuning");
     *t = r->tuning;
