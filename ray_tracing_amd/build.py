@@ -75,6 +75,26 @@ def build(jobs: int = 4, verbose: bool = True, out: str = LIB, defines=()) -> st
     return out
 
 
+HOST = os.path.join(HERE, "host")
+HOST_TARGETS = {"render_realtime": ["render_realtime.cpp", "rt_host.cpp"],
+                "test_report": ["test_report.cpp", "rt_host.cpp"]}
+
+
+def build_host(verbose: bool = True):
+    """C++ host API, the render_realtime CLI and the report test, linked
+    against librt_hip.so (found next to the executables' parent directory)."""
+    outs = []
+    for name, srcs in HOST_TARGETS.items():
+        out = os.path.join(HERE, name)
+        cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-o", out] + [os.path.join(HOST, s) for s in srcs]
+        cmd += ["-L", HERE, "-lrt_hip", "-Wl,-rpath,$ORIGIN"]
+        _run(cmd)
+        outs.append(out)
+        if verbose:
+            print(f"built {out}")
+    return outs
+
+
 def clean():
     shutil.rmtree(BUILD, ignore_errors=True)
     if os.path.exists(LIB):
@@ -91,3 +111,5 @@ if __name__ == "__main__":
     if a.clean:
         clean()
     build(a.j, out=os.path.abspath(a.out), defines=a.defines)
+    if os.path.abspath(a.out) == LIB:
+        build_host()

@@ -1,0 +1,359 @@
+// render_realtime -- benchmark CLI of the MI355X path-tracing core, with the
+// flags, stdout lines and artifacts of the reference's utils/render_realtime.cpp
+// (benchmark_frames.csv, benchmark_summary.json schema 3, benchmark_manifest.json).
+//
+//   render_realtime --scene cornell_mesh --profile scene --frames 20 --warmup-frames 3
+//                   --camera-count 4 --output-dir out --skip-image-write
+// Extensions: --denoiser-weights <params.npy> (learned denoiser), --graph 0|1,
+// --autotune.
+#include <sys/utsname.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <limits>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "rt_host.hpp"
+
+namespace {
+
+const char* kVersion = "ray_tracing_amd 0.1";
+
+struct Args {
+    int camera_count = 4;
+    int frames = 10;
+    int warmup_frames = 2;
+    unsigned seed = 0;
+    std::string output_dir = "render_realtime_out";
+    std::string scene = "cornell_mesh";
+    std::string profile;
+    bool skip_image_write = false;
+    std::string denoiser_weights;
+    int graph = -1;
+    bool autotune = false;
+};
+
+int scene_config(const std::string& name) {
+    if (name == "random_spheres") return 0;
+    if (name == "cornell_mesh") return 1;
+    if (name == "instanced_mesh_restir") return 2;
+    return -1;
+}
+
+bool parse(int argc, char** argv, Args& a) {
+    std::map<std::string, std::string> kv;
+    for (int i = 1; i < argc; ++i) {
+        const std::string k = argv[i];
+        if (k == "--skip-image-write" || k == "--autotune") {
+            kv[k] = "1";
+            continue;
+        }
+        if (k.rfind("--", 0) != 0 || i + 1 >= argc) {
+            std::fprintf(stderr, "unknown or incomplete argument: %s\n", k.c_str());
+            return false;
+        }
+        kv[k] = argv[++i];
+    }
+    try {
+        for (const auto& [k, v] : kv) {
+            if (k == "--camera-count") a.camera_count = std::stoi(v);
+            else if (k == "--frames") a.frames = std::stoi(v);
+            else if (k == "--warmup-frames") a.warmup_frames = std::stoi(v);
+            else if (k == "--seed") a.seed = (unsigned)std::stoul(v);
+            else if (k == "--output-dir") a.output_dir = v;
+            else if (k == "--scene") a.scene = v;
+            else if (k == "--profile") a.profile = v;
+            else if (k == "--skip-image-write") a.skip_image_write = true;
+            else if (k == "--denoiser-weights") a.denoiser_weights = v;
+            else if (k == "--graph") a.graph = std::stoi(v);
+            else if (k == "--autotune") a.autotune = true;
+            else {
+                std::fprintf(stderr, "unknown argument: %s\n", k.c_str());
+                return false;
+            }
+        }
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        return false;
+    }
+    return true;
+}
+
+// float32 1-D .npy (little endian) as written by numpy.save
+std::vector<float> load_npy(const std::string& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot open " + path);
+    char magic[6];
+    in.read(magic, 6);
+    if (std::memcmp(magic, "\x93NUMPY", 6) != 0) throw std::runtime_error(path + " is not a .npy file");
+    unsigned char ver[2];
+    in.read(reinterpret_cast<char*>(ver), 2);
+    uint32_t hlen = 0;
+    if (ver[0] == 1) {
+        uint16_t h16 = 0;
+        in.read(reinterpret_cast<char*>(&h16), 2);
+        hlen = h16;
+    } else {
+        in.read(reinterpret_cast<char*>(&hlen), 4);
+    }
+    std::string header(hlen, ' ');
+    in.read(header.data(), hlen);
+    if (header.find("'<f4'") == std::string::npos || header.find("'fortran_order': False") == std::string::npos)
+        throw std::runtime_error(path + ": expected a C-ordered float32 array");
+    std::vector<float> v;
+    float x;
+    while (in.read(reinterpret_cast<char*>(&x), 4)) v.push_back(x);
+    return v;
+}
+
+void write_ppm(const std::string& path, const rt_amd::RadianceFrame& f) {
+    std::ofstream out(path, std::ios::binary);
+    out << "P6\n" << f.width << " " << f.height << "\n255\n";
+    for (size_t i = 0; i < (size_t)f.width * (size_t)f.height; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            float v = f.beauty_rgba[4 * i + c];
+            v = v <= 0.0f ? 0.0f : (v >= 1.0f ? 1.0f : v);
+            out.put((char)(unsigned char)(255.0f * std::pow(v, 1.0f / 2.2f) + 0.5f));
+        }
+    }
+}
+
+std::string utc_now() {
+    char buf[64];
+    std::time_t t = std::time(nullptr);
+    std::strftime(buf, sizeof(buf), "%Y-%m-%dT%H:%M:%SZ", std::gmtime(&t));
+    return buf;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Args a;
+    if (!parse(argc, argv, a)) return EXIT_FAILURE;
+    if (a.camera_count < 1 || a.camera_count > 4) {
+        std::fprintf(stderr, "--camera-count must be in [1, 4]\n");
+        return EXIT_FAILURE;
+    }
+    if (a.frames < 1) {
+        std::fprintf(stderr, "--frames must be >= 1\n");
+        return EXIT_FAILURE;
+    }
+    if (a.warmup_frames < 0) {
+        std::fprintf(stderr, "--warmup-frames must be >= 0\n");
+        return EXIT_FAILURE;
+    }
+    if ((uint64_t)a.seed + (uint64_t)a.warmup_frames + (uint64_t)a.frames - 1u
+        > std::numeric_limits<uint32_t>::max()) {
+        std::fprintf(stderr, "--seed + --warmup-frames + --frames exceeds uint32 sample streams\n");
+        return EXIT_FAILURE;
+    }
+    const int config = scene_config(a.scene);
+    if (config < 0) {
+        std::fprintf(stderr, "--scene must reference a registered realtime scene\n");
+        return EXIT_FAILURE;
+    }
+    rt_generated_scene* gen = nullptr;
+    rt_amd::throw_on_error(rt_scene_generate(config, 0, &gen));
+    rt_amd::RenderProfile profile = rt_amd::RenderProfile::realtime_default();
+    std::string profile_name = rt_amd::render_profile_name(profile);
+    rt_profile abi_profile{};
+    bool scene_profile = false;
+    if (!a.profile.empty()) {
+        if (a.profile == "scene") {
+            scene_profile = true;
+            profile_name = "scene";
+        } else {
+            const auto p = rt_amd::render_profile_from_name(a.profile);
+            if (!p) {
+                std::fprintf(stderr, "--profile must be one of: quality, balanced, realtime\n");
+                return EXIT_FAILURE;
+            }
+            profile = *p;
+            profile_name = a.profile;
+        }
+    }
+    abi_profile = scene_profile ? gen->profile : rt_amd::to_abi(profile);
+    std::filesystem::create_directories(a.output_dir);
+    try {
+        rt_amd::Renderer renderer(0);
+        char name[128] = {0};
+        int cus = 0;
+        uint64_t total_b = 0, free_b = 0;
+        rt_amd::throw_on_error(rt_device_info(renderer.device(), name, sizeof(name), &cus, &total_b, &free_b));
+        const uint64_t baseline_used = total_b - free_b;
+        renderer.prepare_scene(gen->scene);
+        renderer.reset_sequence(a.seed);
+        rt_nn* nn = nullptr;
+        if (!a.denoiser_weights.empty()) {
+            const std::vector<float> w = load_npy(a.denoiser_weights);
+            rt_amd::throw_on_error(rt_nn_create(0, &nn));
+            rt_amd::throw_on_error(rt_nn_set_params(nn, w.data(), (int32_t)w.size()));
+            rt_amd::throw_on_error(rt_renderer_set_denoiser(renderer.handle(), nn));
+        }
+        if (a.graph >= 0) {
+            rt_tuning t{};
+            t.use_graph = a.graph;
+            t.count_traversal = -1;
+            rt_amd::throw_on_error(rt_renderer_set_tuning(renderer.handle(), &t));
+        }
+        rt_camera_rig rig{};
+        rig.active_count = std::min(a.camera_count, gen->camera_count);
+        for (int c = 0; c < rig.active_count; ++c) rig.cameras[c] = gen->cameras[c];
+        if (a.autotune) {
+            rt_autotune_result ar{};
+            rt_amd::throw_on_error(rt_renderer_autotune(renderer.handle(), &rig, &abi_profile, 2, &ar));
+            std::printf("autotune trials=%d baseline_ms=%.3f best_ms=%.3f block=%d blocks_per_cu=%d "
+                        "refill=%d paths=%d graph=%d\n",
+                ar.trials, ar.baseline_ms, ar.best_ms, ar.best.trace_block, ar.best.trace_blocks_per_cu,
+                ar.best.trace_refill, ar.best.wavefront_paths, ar.best.use_graph);
+        }
+        rt_device_info(renderer.device(), name, sizeof(name), &cus, &total_b, &free_b);
+        const uint64_t prepared_used = total_b - free_b;
+        uint64_t peak_used = std::max(baseline_used, prepared_used);
+
+        rt_amd::profiling::RunReport report;
+        report.provenance.captured_at_utc = utc_now();
+        report.provenance.project_version = kVersion;
+        report.provenance.source_scope = "ray_tracing_amd,include,oracle";
+        report.provenance.build_configuration = "Release";
+        report.provenance.cxx_compiler = __VERSION__;
+        struct utsname un;
+        if (uname(&un) == 0) {
+            report.environment.operating_system = std::string(un.sysname) + " " + un.release;
+            report.environment.architecture = un.machine;
+        }
+        report.environment.gpu_name = name;
+        report.environment.compute_capability = name;
+        report.gpu_memory.total_bytes = total_b;
+        report.gpu_memory.baseline_used_bytes = baseline_used;
+        report.gpu_memory.prepared_used_bytes = prepared_used;
+        const rt_bvh_stats bs = renderer.acceleration_stats();
+        report.gpu_scheduling.persistent_worker_count = 1;
+        report.gpu_scheduling.worker_start_count = 1;
+        report.gpu_scheduling.acceleration_update_kind = "rebuild";
+        report.gpu_scheduling.acceleration_update_ms = bs.build_ms;
+        report.gpu_scheduling.acceleration_node_count = bs.wide_node_count;
+        report.gpu_scheduling.acceleration_reference_count = bs.prim_count;
+        report.gpu_scheduling.acceleration_generation = 1;
+        report.scene = a.scene;
+        report.profile = profile_name;
+        report.camera_count = rig.active_count;
+        report.width = rig.cameras[0].width;
+        report.height = rig.cameras[0].height;
+        report.frames_requested = a.frames;
+        report.warmup_frames = a.warmup_frames;
+        report.random_seed = a.seed;
+        report.samples_per_pixel = abi_profile.samples_per_pixel;
+        report.max_bounces = abi_profile.max_bounces;
+        report.denoise_enabled = abi_profile.enable_denoise != 0;
+        report.image_write_enabled = !a.skip_image_write;
+
+        rt_amd::RenderProfile dummy;
+        for (int w = 0; w < a.warmup_frames; ++w) {
+            rt_frame_timing t{};
+            rt_amd::throw_on_error(rt_render_frame(renderer.handle(), &rig, &abi_profile, a.seed + w, &t));
+            std::printf("warmup=%d sample_stream=%u cameras=%d pipeline_ms=%.3f\n", w, a.seed + w,
+                rig.active_count, t.render_ms);
+        }
+        for (int f = 0; f < a.frames; ++f) {
+            rt_amd::profiling::FrameStageSample s;
+            s.frame_index = f;
+            s.sample_stream = a.seed + (uint32_t)a.warmup_frames + (uint32_t)f;
+            s.camera_count = rig.active_count;
+            s.profile = profile_name;
+            s.width = report.width;
+            s.height = report.height;
+            s.samples_per_pixel = report.samples_per_pixel;
+            s.max_bounces = report.max_bounces;
+            s.denoise_enabled = report.denoise_enabled;
+            const auto t0 = std::chrono::steady_clock::now();
+            rt_frame_timing t{};
+            rt_amd::throw_on_error(rt_render_frame(renderer.handle(), &rig, &abi_profile, s.sample_stream, &t));
+            const auto t1 = std::chrono::steady_clock::now();
+            double lum_sum = 0.0, download_ms = 0.0;
+            for (int c = 0; c < rig.active_count; ++c) {
+                const auto d0 = std::chrono::steady_clock::now();
+                const rt_amd::RadianceFrame fr = renderer.download(c);
+                const double dms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - d0).count();
+                download_ms += dms;
+                rt_amd::profiling::CameraStageSample cs;
+                cs.camera_index = c;
+                cs.render_ms = t.render_ms - t.denoise_ms;
+                cs.denoise_ms = t.denoise_ms;
+                cs.download_ms = dms;
+                cs.average_luminance = fr.average_luminance;
+                lum_sum += fr.average_luminance;
+                s.cameras.push_back(cs);
+                if (!a.skip_image_write) {
+                    const auto w0 = std::chrono::steady_clock::now();
+                    write_ppm(a.output_dir + "/frame_" + std::to_string(f) + "_cam" + std::to_string(c) + ".ppm", fr);
+                    s.image_write_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+                }
+            }
+            const auto t2 = std::chrono::steady_clock::now();
+            s.render_ms = t.render_ms - t.denoise_ms;  // all cameras in one launch sequence
+            s.denoise_ms = t.denoise_ms;
+            s.download_ms = download_ms;
+            s.render_work_ms = s.render_ms;
+            s.denoise_work_ms = s.denoise_ms;
+            s.download_work_ms = download_ms;
+            s.pipeline_ms = std::chrono::duration<double, std::milli>(t1 - t0).count() + download_ms;
+            s.frame_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+            s.host_overhead_ms = std::max(0.0, s.frame_ms - s.pipeline_ms - s.image_write_ms);
+            s.fps = s.frame_ms > 0.0 ? 1000.0 / s.frame_ms : 0.0;
+            rt_device_info(renderer.device(), name, sizeof(name), &cus, &total_b, &free_b);
+            peak_used = std::max<uint64_t>(peak_used, total_b - free_b);
+            std::printf("frame=%d sample_stream=%u cameras=%d avg_luminance=%.6f pipeline_ms=%.3f "
+                        "render_ms=%.3f denoise_ms=%.3f download_ms=%.3f render_work_ms=%.3f "
+                        "denoise_work_ms=%.3f download_work_ms=%.3f image_write_ms=%.3f "
+                        "host_overhead_ms=%.3f frame_ms=%.3f\n",
+                f, s.sample_stream, rig.active_count, lum_sum / rig.active_count, s.pipeline_ms,
+                s.render_ms, s.denoise_ms, s.download_ms, s.render_work_ms, s.denoise_work_ms,
+                s.download_work_ms, s.image_write_ms, s.host_overhead_ms, s.frame_ms);
+            report.frames.push_back(std::move(s));
+        }
+        rt_device_info(renderer.device(), name, sizeof(name), &cus, &total_b, &free_b);
+        report.gpu_memory.final_used_bytes = total_b - free_b;
+        report.gpu_memory.peak_used_bytes = std::max<uint64_t>(peak_used, report.gpu_memory.final_used_bytes);
+        report.gpu_memory.peak_delta_bytes =
+            report.gpu_memory.peak_used_bytes >= baseline_used ? report.gpu_memory.peak_used_bytes - baseline_used : 0;
+        report.aggregate = rt_amd::profiling::compute_aggregate(report.frames);
+        const std::filesystem::path out = a.output_dir;
+        rt_amd::profiling::write_csv(report, out / "benchmark_frames.csv");
+        rt_amd::profiling::write_json(report, out / "benchmark_summary.json");
+        rt_amd::profiling::write_artifact_manifest(report,
+            {out / "benchmark_frames.csv", out / "benchmark_summary.json"}, out / "benchmark_manifest.json");
+        const auto& ag = report.aggregate;
+        std::printf("summary profile=%s warmup_frames=%d frames=%d seed=%u cameras=%d resolution=%dx%d spp=%d "
+                    "max_bounces=%d denoise=%s avg_frame_ms=%.3f avg_denoise_ms=%.3f p95_frame_ms=%.3f "
+                    "p99_frame_ms=%.3f fps=%.2f peak_gpu_memory_mib=%.2f peak_gpu_memory_delta_mib=%.2f "
+                    "workers=1/1 as_update=rebuild as_update_ms=%.3f as_nodes=%d as_refs=%d "
+                    "artifacts=benchmark_frames.csv,benchmark_summary.json,benchmark_manifest.json "
+                    "output_dir=%s\n",
+            profile_name.c_str(), a.warmup_frames, a.frames, a.seed, rig.active_count, report.width,
+            report.height, report.samples_per_pixel, report.max_bounces,
+            report.denoise_enabled ? "true" : "false", ag.frame_ms.avg, ag.denoise_ms.avg, ag.frame_ms.p95,
+            ag.frame_ms.p99, 1000.0 / ag.frame_ms.avg,
+            (double)report.gpu_memory.peak_used_bytes / (1024.0 * 1024.0),
+            (double)report.gpu_memory.peak_delta_bytes / (1024.0 * 1024.0), bs.build_ms, bs.wide_node_count,
+            bs.prim_count, a.output_dir.c_str());
+        if (nn) {
+            rt_renderer_set_denoiser(renderer.handle(), nullptr);
+            rt_nn_destroy(nn);
+        }
+        (void)dummy;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        rt_scene_generated_free(gen);
+        return EXIT_FAILURE;
+    }
+    rt_scene_generated_free(gen);
+    return EXIT_SUCCESS;
+}

@@ -66,3 +66,28 @@ def test_graph_with_learned_denoiser(rt, oracle, device):
     for fe, fg in zip(eager, graph):
         assert fe[0]["beauty"].tobytes() == fg[0]["beauty"].tobytes()
     assert np.isfinite(graph[-1][0]["beauty"]).all()
+
+
+def test_render_realtime_cli(rt, tmp_path):
+    """The C++ render_realtime CLI renders, reports and writes its artifacts."""
+    import json
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(rt.__file__), "render_realtime")
+    if not os.path.exists(exe):
+        pytest.skip("render_realtime not built")
+    say("[gpu] render_realtime CLI")
+    out = tmp_path / "rr"
+    r = subprocess.run([exe, "--scene", "random_spheres", "--profile", "scene", "--frames", "3",
+                        "--warmup-frames", "1", "--camera-count", "1", "--skip-image-write",
+                        "--output-dir", str(out)], capture_output=True, text=True, timeout=120)
+    say(r.stdout[-600:])
+    assert r.returncode == 0, r.stderr
+    assert "summary profile=scene" in r.stdout and "frame=2 sample_stream=3" in r.stdout
+    s = json.load(open(out / "benchmark_summary.json"))
+    assert s["schema_version"] == 3 and len(s["frames"]) == 3 and len(s["per-camera"]) == 3
+    assert s["aggregate"]["frame_ms"]["avg"] > 0
+    rows = open(out / "benchmark_frames.csv").read().strip().splitlines()
+    assert len(rows) == 4
+    m = json.load(open(out / "benchmark_manifest.json"))
+    assert m["measured_frame_count"] == 3 and len(m["artifacts"]) == 2
