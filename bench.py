@@ -94,6 +94,8 @@ def main():
                     help="only run a plain 1 GiB device-to-device copy (3 times), the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the plain vs learned comparison at equal relMSE")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tuning instead of running the autotuner first")
     ap.add_argument("--graph", type=int, default=-1, help="force hipGraph capture off (0) or on (1)")
@@ -233,6 +235,15 @@ def main():
         threads = os.cpu_count() or 1
         threads = min(threads, 16)
         result["cpu_baseline"] = cpu_baseline(a.config, a.cpu_seconds, threads)
+    if not a.no_learned:
+        # plain vs learned path at equal relMSE on the four-camera mixed-model
+        # scene (untimed references; reported beside the throughput metric)
+        try:
+            from ray_tracing_amd import equal_error
+
+            result["learned_vs_plain"] = equal_error.compare(dev, log=log)
+        except Exception as e:  # the throughput line is still printed
+            result["learned_vs_plain"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(result), flush=True)
 
 
