@@ -33,9 +33,11 @@ def frame_rel_mse(frames, ref) -> float:
     return num / den
 
 
-def timed_sequence(dev, config, seed, spp, restir, denoise, nn, frame, frames):
+def timed_sequence(dev, config, seed, spp, restir, denoise, nn, frame, frames, history=True):
     """Renders orbit frames frame-frames+1 .. frame; returns the last frame's
-    beauty per camera and the median device ms of the timed frames."""
+    beauty per camera and the median device ms of the timed frames.  Without
+    `history` every frame starts from a reset accumulation (no temporal
+    blending, the plain tracer's estimate of that frame alone)."""
     g = rt.Generated(config, seed)
     sc = rt.Scene(dev, g.desc)
     sc.build_bvh()
@@ -48,6 +50,8 @@ def timed_sequence(dev, config, seed, spp, restir, denoise, nn, frame, frames):
     prof.enable_denoise = 1 if denoise else 0
     ms = []
     for f in range(frame - frames + 1, frame + 1):
+        if not history:
+            ren.reset_accumulation()
         tm = ren.render(T.frame_rig(g, f), prof, 31 * spp + f)
         ms.append(tm.render_ms)
     rig = T.frame_rig(g, frame)
@@ -71,7 +75,7 @@ def compare(dev, config="instanced_mesh_restir", seed=1000, ref_spp=4096, frame=
     plain = []
     spp = 1
     while spp <= max_plain_spp:
-        b, ms = timed_sequence(dev, config, seed, spp, False, False, None, frame, 2)
+        b, ms = timed_sequence(dev, config, seed, spp, False, False, None, frame, 2, history=False)
         err = frame_rel_mse(b, ref)
         plain.append({"spp": spp, "relmse": err, "ms_per_frame": ms})
         log(f"[equal-error] plain {spp} spp: relMSE {err:.4f} at {ms:.3f} ms/frame")
