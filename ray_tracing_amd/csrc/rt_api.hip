@@ -1039,7 +1039,10 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
     const int block = r->tuning.trace_block;
     const rtk::StageLaunch trace_l{r->dev->cu_count * r->tuning.trace_blocks_per_cu, block,
         r->tuning.trace_refill};
-    const rtk::StageLaunch shade_l{r->dev->cu_count * 4, 256};
+    const rtk::StageLaunch shade_l{r->dev->cu_count * rtk::wf_shade_blocks_per_cu(), 256};
+    const rtk::StageLaunch restir_l{std::min(trace_l.grid * block / 128,
+                                        r->dev->cu_count * rtk::wf_restir_blocks_per_cu()),
+        128};
     {
         const size_t entries = (size_t)std::max(sv.stack4_entries - rtk::kStackLds, 1);
         RT_HIP(r->spill.reserve(entries * (size_t)trace_l.grid * (size_t)block));
@@ -1084,7 +1087,7 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
                 ENQ(rtk::wf_extend(sv, b, bounce, r->tuning.count_traversal != 0, trace_l, st));
                 ENQ(mark());
                 ENQ(rtk::wf_shade(sv, cv, f, b, bounce, shade_l, st));
-                if (f.restir != 0 && bounce == 0) ENQ(rtk::wf_restir(sv, cv, f, rv, b, trace_l, st));
+                if (f.restir != 0 && bounce == 0) ENQ(rtk::wf_restir(sv, cv, f, rv, b, restir_l, st));
                 ENQ(mark());
                 ENQ(rtk::wf_connect(sv, b, bounce, trace_l, st));
                 ENQ(mark());

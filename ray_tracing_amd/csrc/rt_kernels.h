@@ -99,6 +99,10 @@ hipError_t wf_raygen(const CameraView& cams, const FrameParams& fp, WavefrontBuf
     int paths, hipStream_t st);
 hipError_t wf_extend(const SceneView& s, WavefrontBuffers& b, int bounce, bool count,
     StageLaunch l, hipStream_t st);
+// resident blocks per CU of the shade (256 threads) and ReSTIR (128 threads)
+// kernels: their launch grids per CU
+int wf_shade_blocks_per_cu();
+int wf_restir_blocks_per_cu();
 hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParams& fp,
     WavefrontBuffers& b, int bounce, StageLaunch l, hipStream_t st);
 hipError_t wf_connect(const SceneView& s, WavefrontBuffers& b, int bounce, StageLaunch l,

@@ -6,7 +6,7 @@
 // per-thread 64-entry private stack, tests one box per node fetch, runs
 // material evaluation for every candidate hit and has no any-hit early exit.
 // Here one 64-byte node fetch yields four quantized child boxes; leaves are
-// ranges of up to four 48-byte primitive records in Morton order; the
+// ranges of up to two 48-byte primitive records in Morton order; the
 // deferred-node stack keeps its top entries in LDS (lane-strided, so the 64
 // lanes of a wave touch distinct banks) and spills deeper ones to HBM;
 // popped entries farther than the current best hit are skipped; lanes that
@@ -86,26 +86,31 @@ __device__ __forceinline__ float hit_triangle(f3 p0, f3 e1, f3 e2, f3 o, f3 d, f
     return t;
 }
 
-__device__ __forceinline__ float hit_leaf(const float4* leaves, uint32_t leaf, f3 o, f3 d,
-    float tmin, float tmax, uint32_t* id, float* u, float* v) {
-    const float4 a = leaves[3u * leaf];
-    const uint32_t tag = __float_as_uint(a.w);
+// one primitive record (48 bytes, loaded whole so the type branch does not
+// wait on a second round trip)
+struct PrimRec {
+    float4 a, b, c;
+};
+
+__device__ __forceinline__ PrimRec load_prim(const float4* leaves, uint32_t leaf) {
+    return PrimRec{leaves[3u * leaf], leaves[3u * leaf + 1u], leaves[3u * leaf + 2u]};
+}
+
+__device__ __forceinline__ float hit_prim(const PrimRec& p, f3 o, f3 d, float tmin, float tmax,
+    uint32_t* id, float* u, float* v) {
+    const uint32_t tag = __float_as_uint(p.a.w);
     *id = tag & kIdMask;
     const uint32_t type = tag >> 30;
     *u = 0.0f;
     *v = 0.0f;
-    if (type == kPrimTriangle) {
-        const float4 b = leaves[3u * leaf + 1u];
-        const float4 c = leaves[3u * leaf + 2u];
-        return hit_triangle(xyz(a), xyz(b), xyz(c), o, d, tmin, tmax, u, v);
-    }
-    if (type == kPrimSphere) {
-        const float4 b = leaves[3u * leaf + 1u];
-        return hit_sphere(xyz(a), b.x, o, d, tmin, tmax);
-    }
-    const float4 b = leaves[3u * leaf + 1u];
-    const float4 c = leaves[3u * leaf + 2u];
-    return hit_quad(xyz(a), xyz(b), xyz(c), o, d, tmin, tmax, u, v);
+    if (type == kPrimTriangle) return hit_triangle(xyz(p.a), xyz(p.b), xyz(p.c), o, d, tmin, tmax, u, v);
+    if (type == kPrimSphere) return hit_sphere(xyz(p.a), p.b.x, o, d, tmin, tmax);
+    return hit_quad(xyz(p.a), xyz(p.b), xyz(p.c), o, d, tmin, tmax, u, v);
+}
+
+__device__ __forceinline__ float hit_leaf(const float4* leaves, uint32_t leaf, f3 o, f3 d,
+    float tmin, float tmax, uint32_t* id, float* u, float* v) {
+    return hit_prim(load_prim(leaves, leaf), o, d, tmin, tmax, id, u, v);
 }
 
 struct TraceResult {
@@ -236,24 +241,32 @@ __device__ __forceinline__ bool wide_step(const SceneView& s, const WideRay& r, 
         tn[k] = n;
         hit[k] = ch[k] != kEmpty && n <= f;
     }
-    // leaf ranges in slot order
+    // leaf ranges in slot order, as one per-lane loop over the primitives of
+    // every hit leaf (a wave runs the longest lane's list once instead of
+    // one loop per slot)
+    uint32_t leaves = 0u;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (hit[k] && (ch[k] & kLeafBit)) {
-            const uint32_t first = ch[k] & 0x0FFFFFFFu;
-            const uint32_t n = ((ch[k] >> 28) & 7u) + 1u;
-            for (uint32_t i = 0; i < n; ++i) {
-                uint32_t id;
-                float u, v;
-                const float t = hit_leaf(s.leaves, first + i, r.o, r.d, r.tmin, r.tmax, &id, &u, &v);
-                if (COUNT) {
-                    ++cnt.tests;
-                    if ((int)id < s.sphere_count) ++cnt.sphere_tests;
-                }
-                consider_hit(w.best, id, t, u, v);
-                if (ANY && w.best.found) return true;
-            }
+    for (int k = 0; k < 4; ++k) leaves |= (hit[k] && (ch[k] & kLeafBit)) ? (1u << k) : 0u;
+    uint32_t prim = 0u, left = 0u;
+    while (leaves != 0u || left != 0u) {
+        if (left == 0u) {
+            const int k = __builtin_ctz(leaves);
+            leaves &= leaves - 1u;
+            const uint32_t c = k == 0 ? ch[0] : k == 1 ? ch[1] : k == 2 ? ch[2] : ch[3];
+            prim = c & 0x0FFFFFFFu;
+            left = ((c >> 28) & 7u) + 1u;
         }
+        uint32_t id;
+        float u, v;
+        const float t = hit_leaf(s.leaves, prim, r.o, r.d, r.tmin, r.tmax, &id, &u, &v);
+        if (COUNT) {
+            ++cnt.tests;
+            if ((int)id < s.sphere_count) ++cnt.sphere_tests;
+        }
+        consider_hit(w.best, id, t, u, v);
+        if (ANY && w.best.found) return true;
+        ++prim;
+        --left;
     }
     // internal children: stable rank by entry distance
     bool in[4];
@@ -345,6 +358,7 @@ __device__ __forceinline__ void trace_queue(const SceneView& s, uint32_t count, 
     WideState w{};
     TraverseCounts c{0u, 0u, 0u};
     const bool has_tree = s.prim_count > 0 && s.node4_count > 0;
+    if (count == 0u) return;  // no wave touches the fetch counter of an empty queue
     while (true) {
         const uint64_t idle = __ballot(!active);
         const int nidle = __popcll(idle);

@@ -20,6 +20,8 @@
 // Floating-point operations per path happen in the same order as the
 // single-path integrator, so a path's radiance does not depend on the queue
 // order in which the stages visit it.
+#include <atomic>
+
 #include "rt_camera.h"
 #include "rt_kernels.h"
 #include "rt_restir.h"
@@ -179,7 +181,12 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneView s, WavefrontBuffers 
     }
 }
 
-__global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, FrameParams fp,
+#ifdef RT_SHADE_WAVES
+#define RT_SHADE_ATTR __attribute__((amdgpu_waves_per_eu(RT_SHADE_WAVES)))
+#else
+#define RT_SHADE_ATTR
+#endif
+__global__ __launch_bounds__(256) RT_SHADE_ATTR void k_shade(SceneView s, CameraView cams, FrameParams fp,
     WavefrontBuffers b, int bounce, int cur) {
     const uint32_t fp_stream = fp.dyn->stream;
     const uint32_t count = b.counters[cur];
@@ -530,6 +537,37 @@ hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParam
     return hipGetLastError();
 }
 
+static int resident_blocks(const void* kernel, int block) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block, 0) != hipSuccess || n < 1)
+        n = 1;
+    return n;
+}
+
+int wf_restir_blocks_per_cu() {
+    static std::atomic<int> cached{0};
+    int n = cached.load(std::memory_order_relaxed);
+    if (n == 0) {
+        n = resident_blocks(reinterpret_cast<const void*>(k_restir<128>), 128);
+        cached.store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
+// k_shade and k_restir are grid-stride loops with an equal share per wave, so
+// their grids are exactly what the device keeps resident (a larger grid
+// leaves a second, partial round of waves behind the first)
+
+int wf_shade_blocks_per_cu() {
+    static std::atomic<int> cached{0};
+    int n = cached.load(std::memory_order_relaxed);
+    if (n == 0) {
+        n = resident_blocks(reinterpret_cast<const void*>(k_shade), 256);
+        cached.store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
 hipError_t wf_connect(const SceneView& s, WavefrontBuffers& b, int bounce, StageLaunch l,
     hipStream_t st) {
     for (int which = 0; which < 2; ++which) {
@@ -541,12 +579,11 @@ hipError_t wf_connect(const SceneView& s, WavefrontBuffers& b, int bounce, Stage
     return hipGetLastError();
 }
 
-// the ReSTIR stage always runs 128-thread blocks on the trace grid; its
-// spill area is the traversal one (trace grid x block threads)
+// the ReSTIR stage runs 128-thread blocks on `l.grid` blocks, which the caller
+// keeps within the traversal spill area (trace grid x trace block threads)
 hipError_t wf_restir(const SceneView& s, const CameraView& cams, const FrameParams& fp,
     const RestirView& rv, WavefrontBuffers& b, StageLaunch l, hipStream_t st) {
-    const int grid = (int)((int64_t)l.grid * l.block / 128);
-    k_restir<128><<<grid > 0 ? grid : 1, 128, 0, st>>>(s, cams, fp, rv, b);
+    k_restir<128><<<l.grid > 0 ? l.grid : 1, 128, 0, st>>>(s, cams, fp, rv, b);
     return hipGetLastError();
 }
 
