@@ -76,6 +76,40 @@ def cpu_baseline(config: str, seconds: float, threads: int) -> dict:
                       f"in {dt:.2f} s on {threads} threads"}
 
 
+TRAFFIC_PROFILE = {"cornell_mesh": "k_extend_cornell_mesh_wide4_pmc.md"}
+
+
+def measured_traffic(config: str, tuning: dict, rays_per_launch: float):
+    """HBM bytes per k_extend launch from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE scaled by the copy
+    calibration).  When this run's launch shape differs from the profiled one
+    the figure is scaled by extend rays per launch and says so."""
+    name = TRAFFIC_PROFILE.get(config)
+    if name is None:
+        return None, None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
+    try:
+        text = open(path).read()
+    except OSError:
+        return None, None
+    vals = {}
+    for line in text.splitlines():
+        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch",
+                                                   "extend_rays_per_launch", "tuning"):
+            k, v = line.split(":", 1)
+            vals[k] = v.strip()
+    try:
+        traffic = float(vals["traffic_bytes_per_launch"])
+        rays = float(vals["extend_rays_per_launch"])
+        shape = dict(kv.split("=") for kv in vals["tuning"].split())
+    except (KeyError, ValueError):
+        return None, None
+    same = all(int(shape[k]) == int(tuning.get(k, -1)) for k in shape)
+    if same:
+        return traffic, f"profiles/{name}"
+    return traffic * rays_per_launch / rays, f"profiles/{name}, scaled by extend rays per launch"
+
+
 def cam_desc(cam) -> str:
     return f"{cam.width}x{cam.height} {'equi' if cam.model == 1 else 'pinhole'}"
 
@@ -198,6 +232,8 @@ def main():
     achieved = extend_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
     gpu_ms = sum(f.render_ms for f in frames) / a.steps
 
+    traffic, traffic_source = measured_traffic(a.config, ren.tuning(),
+                                               cnt.extend_rays / max(launches, 1))
     result = {
         "metric": "Mpaths/s & ms/frame 4-cam; plain vs learned path at equal relMSE",
         "value": round(value, 3),
@@ -227,7 +263,9 @@ def main():
                    "tuning": ren.tuning(), "autotune": autotune},
         "roofline": {"bound": "hbm", "kernel": "k_extend (closest-hit traversal)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": int(traffic) if traffic is not None else None,
+                     "traffic_source": traffic_source,
                      "bytes_per_launch": int(extend_bytes / max(launches, 1)),
                      "avg_launch_ms": round(avg_launch_ms, 4), "launches_per_frame": launches},
     }

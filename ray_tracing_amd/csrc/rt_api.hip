@@ -790,10 +790,11 @@ rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out)
     r->dev = dev;
     r->scene = scene;
     scene->refs.fetch_add(1);
-    r->tuning.wavefront_paths = 1 << 22;
-    r->tuning.trace_block = 128;
-    r->tuning.trace_blocks_per_cu = 10;
-    r->tuning.trace_refill = 16;
+    // the autotuner's choice on the 4-camera Cornell mesh workload (MI355X)
+    r->tuning.wavefront_paths = 1 << 23;
+    r->tuning.trace_block = 256;
+    r->tuning.trace_blocks_per_cu = 4;
+    r->tuning.trace_refill = 4;
     r->tuning.use_graph = 0;
     hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&r->host_cams), sizeof(rt_camera) * 8,
         hipHostMallocDefault);

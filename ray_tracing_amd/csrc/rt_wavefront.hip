@@ -181,12 +181,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneView s, WavefrontBuffers 
     }
 }
 
-#ifdef RT_SHADE_WAVES
-#define RT_SHADE_ATTR __attribute__((amdgpu_waves_per_eu(RT_SHADE_WAVES)))
-#else
-#define RT_SHADE_ATTR
-#endif
-__global__ __launch_bounds__(256) RT_SHADE_ATTR void k_shade(SceneView s, CameraView cams, FrameParams fp,
+__global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, FrameParams fp,
     WavefrontBuffers b, int bounce, int cur) {
     const uint32_t fp_stream = fp.dyn->stream;
     const uint32_t count = b.counters[cur];
