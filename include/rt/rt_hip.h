@@ -321,7 +321,9 @@ rt_status rt_nn_train_step_host(rt_nn* nn, const rt_nn_images* images, const flo
     const float* normal, const float* albedo, const float* depth, const float* target, float lr,
     float* loss, float* grad);
 /* denoise with `nn` instead of the a-trous filter when the profile asks for
- * denoising (NULL detaches); nn must live on the renderer's device */
+ * denoising (NULL detaches); nn must live on the renderer's device.  The
+ * renderer holds its own reference, so rt_nn_destroy on the caller's handle
+ * is allowed while the network is attached. */
 rt_status rt_renderer_set_denoiser(rt_renderer* r, rt_nn* nn);
 
 /* tuning knobs chosen by the autotuner; 0 keeps the current value */
@@ -351,10 +353,83 @@ typedef struct rt_autotune_result {
 rt_status rt_renderer_autotune(rt_renderer* r, const rt_camera_rig* rig, const rt_profile* p,
     int32_t frames_per_trial, rt_autotune_result* out);
 
+/* ------------------------------------------------- radiance cache
+ * Online radiance cache (no counterpart in the reference: the north-star
+ * learned part that ends paths early).  A multiresolution hash-grid encoding
+ * of the hit position feeds a 64-wide MLP (four hidden ReLU layers, RGB
+ * output) together with the outgoing direction, shading normal, roughness and
+ * albedo; inference and training run as BF16 MFMA kernels with FP32
+ * accumulation.  Attached to a renderer, the cache is trained every frame on
+ * path suffixes that a random subset of paths (training paths) traces with
+ * the plain integrator from vertex `query_bounce` on, and once it is usable
+ * every other path ends at that vertex with throughput * cache prediction.
+ * The plain tracer is the fallback: the renderer does not end paths with the
+ * cache before `warmup_frames` frames of training or while the running
+ * relative loss on fresh training samples exceeds `max_rel_loss`. */
+typedef struct rt_cache_config {
+    int32_t levels;           /* hash-grid levels, 1..16 */
+    int32_t log2_table;       /* entries per level = 2^log2_table, 8..22 */
+    int32_t base_resolution;  /* grid resolution of the first level (>= 2) */
+    int32_t max_resolution;   /* grid resolution of the last level */
+    int32_t train_samples;    /* training records per frame (capacity), 256..1048576 */
+    int32_t train_steps;      /* Adam steps per frame (the records split into this many batches) */
+    int32_t query_bounce;     /* path vertex where paths end with the cache (>= 1) */
+    int32_t warmup_frames;    /* frames of training before the cache ends paths */
+    float learning_rate;      /* Adam step size for the MLP and the grid */
+    float max_rel_loss;       /* fallback threshold on the running relative loss */
+    float train_fraction;     /* share of pixel samples traced fully as training paths;
+                                 0: 1.5 x train_samples / paths of the frame */
+    uint32_t seed;            /* parameter initialisation */
+} rt_cache_config;
+
+typedef struct rt_cache_stats {
+    float loss_ema;           /* running relative loss on fresh training samples */
+    float last_loss;          /* relative loss of the last frame's first batch, before its step */
+    int32_t frames;           /* frames trained */
+    int32_t usable;           /* the renderer ends paths with the cache */
+    int32_t steps;            /* Adam steps taken */
+    int32_t records;          /* training records of the last frame */
+    int32_t queries;          /* paths ended with the cache in the last frame */
+    int32_t pad;
+} rt_cache_stats;
+
+typedef struct rt_cache rt_cache;
+/* 16 levels, 2^19 entries, resolutions 16..2048, 65536 samples in 4 steps,
+ * vertex 2, 4 warm-up frames, learning rate 1e-2, automatic training share */
+void rt_cache_default_config(rt_cache_config* c);
+int32_t rt_cache_mlp_param_count(void);
+/* MLP parameters then the grid (levels x 2^log2_table x 2) */
+int64_t rt_cache_param_count(const rt_cache_config* c);
+rt_status rt_cache_create(int ordinal, const rt_cache_config* c, rt_cache** out);
+rt_status rt_cache_destroy(rt_cache* cache);
+/* FP32 parameters: W0..W3 [64 out][64 in], W4 [3][64], grid [L][T][2].
+ * Setting them resets the optimiser and the training state. */
+rt_status rt_cache_set_params(rt_cache* cache, const float* params, int64_t n);
+rt_status rt_cache_get_params(rt_cache* cache, float* params, int64_t n);
+/* the box the hash grid spans (the renderer sets its scene's bounds) */
+rt_status rt_cache_set_bounds(rt_cache* cache, const float lo[3], const float hi[3]);
+/* the encoding's exact constants: per-level resolution and dense flag, box
+ * origin and reciprocal extent */
+rt_status rt_cache_grid_info(rt_cache* cache, float res[16], int32_t dense[16], float lo[3],
+    float inv_extent[3]);
+/* inputs: n x 16 floats (pos.xyz, 0, outgoing dir.xyz, roughness,
+ * normal.xyz, 0, albedo.rgb, 0); out: n x 4 floats (RGB, 0) */
+rt_status rt_cache_infer_host(rt_cache* cache, const float* inputs, int32_t n, float* out);
+/* one Adam step on n samples (targets n x 4 floats, RGB used); loss = the
+ * relative loss before the step; grad (optional, rt_cache_param_count
+ * floats) = its gradient */
+rt_status rt_cache_train_step_host(rt_cache* cache, const float* inputs, const float* targets,
+    int32_t n, float* loss, float* grad);
+rt_status rt_cache_get_stats(rt_cache* cache, rt_cache_stats* out);
+/* trains and uses `cache` in this renderer's frames (NULL detaches); the
+ * renderer holds its own reference */
+rt_status rt_renderer_set_radiance_cache(rt_renderer* r, rt_cache* cache);
+
 /* ------------------------------------------------- synthetic scenes
  * Seeded generators of the benchmark scenes (this repository's equivalent
  * of build_scene(id) + make_rig, src/scene/shared_scene_builders.h:56-57):
- *   0 random_spheres, 1 cornell_mesh, 2 instanced_mesh_restir */
+ *   0 random_spheres, 1 cornell_mesh, 2 instanced_mesh_restir,
+ *   3 radiance_cache_realtime */
 typedef struct rt_generated_scene {
     rt_scene_desc scene;
     int32_t camera_count;
@@ -364,7 +439,7 @@ typedef struct rt_generated_scene {
 } rt_generated_scene;
 rt_status rt_scene_generate(int config, uint64_t seed, rt_generated_scene** out);
 rt_status rt_scene_generated_free(rt_generated_scene* g);
-/* camera rig of frame `frame` of a `frames`-frame orbit (config 2; the other
+/* camera rig of frame `frame` of a `frames`-frame orbit (configs 2 and 3; the other
  * configs have static rigs and return their frame-0 rig) */
 rt_status rt_scene_generate_rig(int config, uint64_t seed, int frame, int frames,
     rt_camera_rig* out);

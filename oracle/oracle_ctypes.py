@@ -164,6 +164,60 @@ NODE_DTYPE = np.dtype([("lo0", "<f4", 3), ("hi0", "<f4", 3), ("lo1", "<f4", 3),
                        ("pad", "<u4")])
 
 
+class CacheGrid(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("log2_table", C.c_int32), ("res", C.c_float * 16),
+                ("dense", C.c_int32 * 16), ("lo", C.c_float * 3), ("inv_extent", C.c_float * 3)]
+
+
+def cache_grid(info: dict) -> CacheGrid:
+    """Grid constants (levels, log2_table, res, dense, lo, inv_extent)."""
+    g = CacheGrid()
+    g.levels = int(info["levels"])
+    g.log2_table = int(info["log2_table"])
+    for l in range(16):
+        g.res[l] = float(info["res"][l])
+        g.dense[l] = int(info["dense"][l])
+    for a in range(3):
+        g.lo[a] = float(info["lo"][a])
+        g.inv_extent[a] = float(info["inv_extent"][a])
+    return g
+
+
+def cache_corners(g: CacheGrid, level: int, p):
+    p = np.ascontiguousarray(p, np.float32)
+    idx = np.zeros(8, np.uint32)
+    w = np.zeros(8, np.float32)
+    lib().rto_cache_corners(C.byref(g), level, ptr(p), ptr(idx), ptr(w))
+    return idx, w
+
+
+def cache_features(g: CacheGrid, table, query, bf16=True) -> np.ndarray:
+    t = np.ascontiguousarray(table, np.float32)
+    q = np.ascontiguousarray(query, np.float32)
+    x = np.zeros(64, np.float32)
+    lib().rto_cache_features(C.byref(g), ptr(t), ptr(q), 1 if bf16 else 0, ptr(x))
+    return x
+
+
+def cache_predict(g: CacheGrid, params, inputs, bf16=True) -> np.ndarray:
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 16)
+    out = np.zeros((x.shape[0], 3), np.float64)
+    lib().rto_cache_predict(C.byref(g), ptr(p), x.shape[0], ptr(x), 1 if bf16 else 0, ptr(out))
+    return out
+
+
+def cache_loss_grad(g: CacheGrid, params, inputs, targets, bf16=True, want_grad=True):
+    """Oracle radiance cache: (loss, gradient (float64) or None)."""
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 16)
+    t = np.ascontiguousarray(targets, np.float32).reshape(-1, 4)
+    grad = np.zeros(p.size, np.float64) if want_grad else None
+    loss = lib().rto_cache_loss_grad(C.byref(g), ptr(p), x.shape[0], ptr(x), ptr(t),
+                                     1 if bf16 else 0, ptr(grad) if grad is not None else None)
+    return loss, grad
+
+
 def build() -> str:
     """Compile liboracle.so with the committed Makefile (gcc)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -239,6 +293,17 @@ def lib():
         L.rto_restir_merge.argtypes = [P(Reservoir), P(Reservoir), C.c_float, C.c_int, C.c_float,
                                        C.c_int, P(C.c_int)]
         L.rto_build_analytic_cdf.argtypes = [P(Scene), C.c_void_p, C.c_void_p]
+        L.rto_cache_mlp_param_count.argtypes = []
+        L.rto_cache_mlp_param_count.restype = C.c_int
+        L.rto_cache_corners.argtypes = [P(CacheGrid), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rto_cache_corners.restype = None
+        L.rto_cache_features.argtypes = [P(CacheGrid), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        L.rto_cache_features.restype = None
+        L.rto_cache_predict.argtypes = [P(CacheGrid), C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                        C.c_void_p]
+        L.rto_cache_loss_grad.argtypes = [P(CacheGrid), C.c_void_p, C.c_int, C.c_void_p,
+                                          C.c_void_p, C.c_int, C.c_void_p]
+        L.rto_cache_loss_grad.restype = C.c_double
         _lib = L
     return _lib
 

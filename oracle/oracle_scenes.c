@@ -316,7 +316,8 @@ static double gauss(pcg32_t* r, double sigma) {
 
 /* four cameras on a radius-70 orbit around the cube centre, 90 degrees apart;
  * frame f of `frames` advances the orbit by 2*pi*f/frames */
-static void instanced_rig(uint64_t seed, int frame, int frames, rto_camera* cams) {
+static void instanced_rig_sized(uint64_t seed, int frame, int frames, int wp, int hp, int we,
+    int he, rto_camera* cams) {
     pcg32_t r;
     pcg_seed(&r, seed ^ 0x5bd1e995ULL);
     /* k_i ~ N(0, 0.01), redrawn until theta_d(theta) is monotone on
@@ -340,14 +341,20 @@ static void instanced_rig(uint64_t seed, int frame, int frames, rto_camera* cams
         const double th = 2.0 * 3.14159265358979323846 * (double)frame / (double)(frames > 0 ? frames : 1)
                           + (double)k * 0.5 * 3.14159265358979323846;
         const double from[3] = {center[0] + 70.0 * cos(th), center[1] + 15.0, center[2] + 70.0 * sin(th)};
-        if (k < 2) lookat_pinhole(&cams[k], 640, 480, 60.0, from, center, up);
-        else lookat_equi(&cams[k], 1024, 1024, radial[k - 2], from, center, up);
+        if (k < 2) lookat_pinhole(&cams[k], wp, hp, 60.0, from, center, up);
+        else lookat_equi(&cams[k], we, he, radial[k - 2], from, center, up);
     }
 }
 
+/* config 2: 2 pinhole 640x480 + 2 equi 1024x1024; config 3: all four 1280x720 */
+static void instanced_rig(uint64_t seed, int frame, int frames, rto_camera* cams) {
+    instanced_rig_sized(seed, frame, frames, 640, 480, 1024, 1024, cams);
+}
+
 int rto_generate_rig(int config, uint64_t seed, int frame, int frames, rto_camera* cams, int* count) {
-    if (config != 2) return -1;
-    instanced_rig(seed, frame, frames, cams);
+    if (config == 2) instanced_rig(seed, frame, frames, cams);
+    else if (config == 3) instanced_rig_sized(seed, frame, frames, 1280, 720, 1280, 720, cams);
+    else return -1;
     *count = 4;
     return 0;
 }
@@ -444,6 +451,16 @@ int rto_generate_scene(int config, uint64_t seed, rto_generated* g) {
         case 0: return gen_random_spheres(g, seed);
         case 1: return gen_cornell_mesh(g, seed);
         case 2: return gen_instanced_mesh(g, seed);
+        case 3: {
+            /* the config-2 scene, four 1280x720 cameras, paths of depth 8 with
+             * Russian roulette from bounce 3 (the radiance-cache workload) */
+            const int rc = gen_instanced_mesh(g, seed);
+            if (rc != 0) return rc;
+            instanced_rig_sized(seed, 0, 60, 1280, 720, 1280, 720, g->cameras);
+            g->profile.max_bounces = 8;
+            g->profile.rr_start_bounce = 3;
+            return 0;
+        }
         default: return -1;
     }
 }

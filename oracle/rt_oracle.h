@@ -203,6 +203,27 @@ void rto_nn_features(int P, const float* beauty, const float* normal, const floa
 double rto_nn_loss_grad(const rto_images* im, const float* params, const float* feat,
     const float* beauty, const float* target, float* out4, float* grad, int bf16);
 
+/* ---- radiance cache (oracle_cache.c) */
+typedef struct rto_cache_grid {
+    int32_t levels, log2_table;
+    float res[16];
+    int32_t dense[16];
+    float lo[3], inv_extent[3];
+} rto_cache_grid;
+int rto_cache_mlp_param_count(void);
+/* table indices and trilinear weights of level l around p in [0,1]^3 */
+void rto_cache_corners(const rto_cache_grid* g, int l, const float p[3], uint32_t idx[8], float w[8]);
+/* the 64 network inputs of one 16-float query */
+void rto_cache_features(const rto_cache_grid* g, const float* table, const float* in16, int bf16,
+    float x[64]);
+/* predictions (n x 3, double) for n 16-float queries; params = MLP then grid */
+int rto_cache_predict(const rto_cache_grid* g, const float* params, int n, const float* inputs,
+    int bf16, double* out);
+/* relative-L2 loss of n samples (targets n x 4) and, when grad is set, its
+ * gradient (MLP then grid, double) */
+double rto_cache_loss_grad(const rto_cache_grid* g, const float* params, int n, const float* inputs,
+    const float* targets, int bf16, double* grad);
+
 /* ---- per-frame post passes (oracle_post.c) */
 /* temporal reprojection + accumulation of one camera image, in place */
 void rto_resolve(const rto_camera* cam, const rto_camera* prev_cam, int history_length,

@@ -35,6 +35,10 @@ EXPORTED = [
     "rt_nn_param_count", "rt_nn_create", "rt_nn_destroy", "rt_nn_set_params", "rt_nn_get_params",
     "rt_nn_infer", "rt_nn_train_step", "rt_nn_infer_host", "rt_nn_train_step_host",
     "rt_renderer_set_denoiser", "rt_renderer_autotune",
+    "rt_cache_default_config", "rt_cache_mlp_param_count", "rt_cache_param_count",
+    "rt_cache_create", "rt_cache_destroy", "rt_cache_set_params", "rt_cache_get_params",
+    "rt_cache_set_bounds", "rt_cache_grid_info", "rt_cache_infer_host",
+    "rt_cache_train_step_host", "rt_cache_get_stats", "rt_renderer_set_radiance_cache",
 ]
 
 
@@ -221,6 +225,10 @@ class Renderer:
         self._nn = nn  # keeps the network alive while attached
         check(lib().rt_renderer_set_denoiser(self.h, nn.h if nn is not None else None))
 
+    def set_radiance_cache(self, cache: "RadianceCache | None"):
+        self._cache = cache
+        check(lib().rt_renderer_set_radiance_cache(self.h, cache.h if cache is not None else None))
+
     def reservoirs(self, camera: int) -> np.ndarray:
         cam = self.rig.cameras[camera]
         out = np.zeros(cam.width * cam.height, abi.RESERVOIR_DTYPE)
@@ -301,10 +309,86 @@ class Denoiser:
         return loss.value, grad
 
 
+def cache_config(**overrides) -> "abi.CacheConfig":
+    """The library's default radiance-cache configuration with overrides."""
+    c = abi.CacheConfig()
+    lib().rt_cache_default_config(C.byref(c))
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
+class RadianceCache:
+    """The online radiance cache (hash grid + BF16 MFMA MLP)."""
+
+    def __init__(self, ordinal: int = 0, config=None, **overrides):
+        self.config = config if config is not None else cache_config(**overrides)
+        self.h = C.c_void_p()
+        check(lib().rt_cache_create(ordinal, C.byref(self.config), C.byref(self.h)))
+        self.n = lib().rt_cache_param_count(C.byref(self.config))
+        self.n_mlp = lib().rt_cache_mlp_param_count()
+
+    def close(self):
+        if self.h:
+            lib().rt_cache_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == self.n
+        check(lib().rt_cache_set_params(self.h, ptr(p), self.n))
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n, np.float32)
+        check(lib().rt_cache_get_params(self.h, ptr(p), self.n))
+        return p
+
+    def set_bounds(self, lo, hi):
+        lo = np.ascontiguousarray(lo, np.float32)
+        hi = np.ascontiguousarray(hi, np.float32)
+        check(lib().rt_cache_set_bounds(self.h, ptr(lo), ptr(hi)))
+
+    def grid_info(self) -> dict:
+        res = np.zeros(16, np.float32)
+        dense = np.zeros(16, np.int32)
+        lo = np.zeros(3, np.float32)
+        inv = np.zeros(3, np.float32)
+        check(lib().rt_cache_grid_info(self.h, ptr(res), ptr(dense), ptr(lo), ptr(inv)))
+        return {"levels": self.config.levels, "log2_table": self.config.log2_table, "res": res,
+                "dense": dense, "lo": lo, "inv_extent": inv}
+
+    def infer(self, inputs: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 16)
+        out = np.zeros((x.shape[0], 4), np.float32)
+        check(lib().rt_cache_infer_host(self.h, ptr(x), x.shape[0], ptr(out)))
+        return out
+
+    def train_step(self, inputs: np.ndarray, targets: np.ndarray, want_grad: bool = False):
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 16)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 4)
+        loss = C.c_float()
+        grad = np.zeros(self.n, np.float32) if want_grad else None
+        check(lib().rt_cache_train_step_host(self.h, ptr(x), ptr(t), x.shape[0], C.byref(loss),
+                                             ptr(grad) if grad is not None else None))
+        return loss.value, grad
+
+    def stats(self) -> dict:
+        st = abi.CacheStats()
+        check(lib().rt_cache_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in abi.CacheStats._fields_ if k != "pad"}
+
+
 class Generated:
     """A seeded benchmark scene from the library's generator (rt_scene_generate)."""
 
-    CONFIGS = {"random_spheres": 0, "cornell_mesh": 1, "instanced_mesh_restir": 2}
+    CONFIGS = {"random_spheres": 0, "cornell_mesh": 1, "instanced_mesh_restir": 2,
+               "radiance_cache_realtime": 3}
 
     def __init__(self, config, seed: int = 0):
         if isinstance(config, str):

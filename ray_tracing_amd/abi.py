@@ -144,6 +144,21 @@ def nn_images(sizes) -> "NnImages":
     return im
 
 
+class CacheConfig(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("log2_table", C.c_int32),
+                ("base_resolution", C.c_int32), ("max_resolution", C.c_int32),
+                ("train_samples", C.c_int32), ("train_steps", C.c_int32),
+                ("query_bounce", C.c_int32), ("warmup_frames", C.c_int32),
+                ("learning_rate", C.c_float), ("max_rel_loss", C.c_float),
+                ("train_fraction", C.c_float), ("seed", C.c_uint32)]
+
+
+class CacheStats(C.Structure):
+    _fields_ = [("loss_ema", C.c_float), ("last_loss", C.c_float), ("frames", C.c_int32),
+                ("usable", C.c_int32), ("steps", C.c_int32), ("records", C.c_int32),
+                ("queries", C.c_int32), ("pad", C.c_int32)]
+
+
 class DeviceFrameView(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("beauty", C.c_void_p),
                 ("normal", C.c_void_p), ("albedo", C.c_void_p), ("depth", C.c_void_p),
@@ -217,3 +232,19 @@ def declare(L):
     L.rt_renderer_autotune.argtypes = [vp, P(CameraRig), P(Profile), C.c_int32, P(AutotuneResult)]
     L.rt_frame_download_guides.argtypes = [vp, C.c_int, vp, vp, vp]
     L.rt_scene_generate_rig.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, P(CameraRig)]
+    L.rt_cache_default_config.argtypes = [P(CacheConfig)]
+    L.rt_cache_default_config.restype = None
+    L.rt_cache_mlp_param_count.argtypes = []
+    L.rt_cache_mlp_param_count.restype = C.c_int32
+    L.rt_cache_param_count.argtypes = [P(CacheConfig)]
+    L.rt_cache_param_count.restype = C.c_int64
+    L.rt_cache_create.argtypes = [C.c_int, P(CacheConfig), P(vp)]
+    L.rt_cache_destroy.argtypes = [vp]
+    L.rt_cache_set_params.argtypes = [vp, vp, C.c_int64]
+    L.rt_cache_get_params.argtypes = [vp, vp, C.c_int64]
+    L.rt_cache_set_bounds.argtypes = [vp, vp, vp]
+    L.rt_cache_grid_info.argtypes = [vp, vp, vp, vp, vp]
+    L.rt_cache_infer_host.argtypes = [vp, vp, C.c_int32, vp]
+    L.rt_cache_train_step_host.argtypes = [vp, vp, vp, C.c_int32, vp, vp]
+    L.rt_cache_get_stats.argtypes = [vp, P(CacheStats)]
+    L.rt_renderer_set_radiance_cache.argtypes = [vp, vp]

@@ -23,7 +23,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 CLANGXX = os.path.join(ROCM, "llvm", "bin", "clang++")
 
 HIP_SOURCES = ["rt_bvh.hip", "rt_trace.hip", "rt_wavefront.hip", "rt_post.hip", "rt_nn.hip",
-               "rt_nn_api.hip", "rt_api.hip"]
+               "rt_nn_api.hip", "rt_cache.hip", "rt_cache_api.hip", "rt_api.hip"]
 CXX_SOURCES = ["rt_scenes.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++20", "-ffp-contract=off", "-fno-fast-math", "-Wall",

@@ -14,7 +14,9 @@
 #include <string>
 #include <vector>
 
+#include "rt_alloc.h"
 #include "rt_host.h"
+#include "rt_cache_abi.h"
 #include "rt_nn_abi.h"
 #include "rt_kernels.h"
 #include "rt_openpbr.h"
@@ -31,6 +33,10 @@ rt_status fail(rt_status s, const std::string& msg) {
 }  // namespace
 
 void rt_set_error(const char* msg) { g_error = msg; }
+
+static std::atomic<uint64_t> g_alloc_epoch{0};
+uint64_t rt_alloc_epoch() { return g_alloc_epoch.load(); }
+void rt_alloc_bump() { g_alloc_epoch.fetch_add(1); }
 
 namespace {
 
@@ -66,6 +72,7 @@ struct DevBuf {
         if (count == 0) count = 1;
         const hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count);
         if (e == hipSuccess) n = count;
+        rt_alloc_bump();
         return e;
     }
 };
@@ -123,6 +130,7 @@ struct rt_scene {
     DevBuf<float> a_sel, a_cdf;
     DevBuf<int32_t> light_of_prim;
     int32_t env_light = -1;
+    float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};  // primitive bounds
     // BVH
     bool bvh_ready = false;
     int32_t prim_count = 0;
@@ -180,7 +188,8 @@ struct GraphKey {
         count_traversal, fs, res_cur;
     int32_t restir_params[10];
     int32_t widths[4], heights[4];
-    const void* ptrs[8];
+    const void* ptrs[10];
+    uint64_t alloc_epoch;  // device allocations of the library when captured
 };
 
 struct rt_renderer {
@@ -209,6 +218,9 @@ struct rt_renderer {
     int last_set = 0;
     bool last_denoised = false;
     rt_nn* nn = nullptr;  // learned denoiser (replaces the a-trous filter when set)
+    rt_cache* cache = nullptr;  // radiance cache (trained and queried when set)
+    DevBuf<rtk::CacheInput> cache_queries;   // one per path slot
+    DevBuf<rtk::CacheRecord> cache_records;  // training records of a frame
     // per-frame device values and captured frame graphs
     DevBuf<rtk::FrameDyn> dyn;
     rtk::FrameDyn* host_dyn = nullptr;  // pinned
@@ -512,6 +524,36 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
     RT_HIP(hipSetDevice(dev->ordinal));
     auto* s = new rt_scene();
     s->dev = dev;
+    {
+        // bounding box of the primitives (the radiance cache's grid domain)
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        auto grow = [&](const float* p, double pad) {
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], (double)p[a] - pad);
+                hi[a] = std::max(hi[a], (double)p[a] + pad);
+            }
+        };
+        for (int i = 0; i < d->sphere_count; ++i) grow(d->spheres[i].center, std::fabs((double)d->spheres[i].radius));
+        for (int i = 0; i < d->quad_count; ++i) {
+            const rt_quad& q = d->quads[i];
+            for (int k = 0; k < 4; ++k) {
+                float c[3];
+                for (int a = 0; a < 3; ++a)
+                    c[a] = q.origin[a] + ((k & 1) ? q.edge_u[a] : 0.0f) + ((k & 2) ? q.edge_v[a] : 0.0f);
+                grow(c, 0.0);
+            }
+        }
+        for (int i = 0; i < d->triangle_count; ++i) {
+            grow(d->triangles[i].p0, 0.0);
+            grow(d->triangles[i].p1, 0.0);
+            grow(d->triangles[i].p2, 0.0);
+        }
+        for (int a = 0; a < 3; ++a) {
+            if (lo[a] > hi[a]) lo[a] = hi[a] = 0.0;
+            s->lo[a] = (float)lo[a];
+            s->hi[a] = (float)hi[a];
+        }
+    }
     dev->refs.fetch_add(1);
     s->sphere_count = d->sphere_count;
     s->quad_count = d->quad_count;
@@ -817,9 +859,13 @@ rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out)
 rt_status rt_renderer_destroy(rt_renderer* r) {
     if (!r) return RT_OK;
     rt_scene* scene = r->scene;
+    rt_nn* nn = r->nn;
+    rt_cache* cache = r->cache;
     (void)hipSetDevice(r->dev->ordinal);
     (void)hipStreamSynchronize(r->dev->stream);
     delete r;
+    if (nn) rt_nn_release(nn);
+    if (cache) rt_cache_release(cache);
     scene_release(scene);
     return RT_OK;
 }
@@ -1051,6 +1097,24 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
     }
     const bool denoise = p->enable_denoise != 0;
     if (denoise && r->nn != nullptr) RT_HIP(rt_nn_reserve(r->nn, total));
+    // radiance cache: queries (one per path slot) and the frame's training records
+    rt_cache* cache = r->cache;
+    fp.cache_bounce = -1;
+    if (cache != nullptr) {
+        const rt_cache_config& cc = cache->cfg;
+        RT_HIP(r->cache_queries.reserve(P * (size_t)layers));
+        RT_HIP(r->cache_records.reserve((size_t)cc.train_samples));
+        RT_HIP(cache->ensure(cc.train_samples));
+        fp.cache_bounce = cc.query_bounce;
+        const double paths_per_frame = (double)P * (double)spp;
+        fp.cache_train_fraction = cc.train_fraction > 0.0f
+            ? cc.train_fraction
+            : (float)std::min(1.0, 1.5 * (double)cc.train_samples / std::max(paths_per_frame, 1.0));
+        fp.cache_state = cache->state.p;
+        fp.cache_queries = r->cache_queries.p;
+        fp.cache_records = r->cache_records.p;
+        fp.cache_record_cap = cc.train_samples;
+    }
 
     const int chunks = (spp + layers - 1) / layers;
     const size_t stage_events = (size_t)chunks * (size_t)max_b * 6 + 4;
@@ -1093,6 +1157,10 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
                 ENQ(rtk::wf_connect(sv, b, bounce, trace_l, st));
                 ENQ(mark());
             }
+            if (cache != nullptr) {
+                ENQ(rtk::wf_cache_chunk(cache->grid_view(), cache->packed(), f, b, paths,
+                    cache->grid(), st));
+            }
             ENQ(rtk::wf_accumulate(f, b, st));
         }
         ENQ(rtk::wf_finalize(total, spp, b, r->beauty_set[fs].p, r->normal_set[fs].p, r->albedo.p,
@@ -1120,6 +1188,11 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
                     r->denoise_scratch.p, total, st));
             }
             if (marks) ENQ(hipEventRecord(r->post_ev[1], st));
+        }
+        // the cache learns from this frame's training paths (used from the next frame)
+        if (cache != nullptr) {
+            ENQ(rt_cache_train_frame(cache, r->cache_records.p, rtk::wf_cache_record_count(b),
+                cache->cfg.train_samples, st));
         }
 #undef ENQ
         return hipSuccess;
@@ -1152,9 +1225,10 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
             key.widths[c] = r->widths[c];
             key.heights[c] = r->heights[c];
         }
-        const void* ptrs[8] = {b.ray_o, b.acc_beauty, r->beauty_set[0].p, r->reservoirs[0].p,
-            b.spill, r->nn, sv.nodes4, r->denoised.p};
+        const void* ptrs[10] = {b.ray_o, b.acc_beauty, r->beauty_set[0].p, r->reservoirs[0].p,
+            b.spill, r->nn, sv.nodes4, r->denoised.p, cache, r->cache_queries.p};
         std::memcpy(key.ptrs, ptrs, sizeof(ptrs));
+        key.alloc_epoch = rt_alloc_epoch();
         rt_renderer::Graph* g = nullptr;
         for (auto& gr : r->graphs) {
             if (gr.exec && std::memcmp(&gr.key, &key, sizeof(key)) == 0) g = &gr;
@@ -1184,9 +1258,14 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
         RT_HIP(enqueue(stage_timing));
     }
     if (timed) RT_HIP(hipEventRecord(r->events[1], st));
-    uint32_t stats[16];
+    uint32_t stats[rtk::kStatWords];
     RT_HIP(hipMemcpyAsync(stats, b.counters, sizeof(stats), hipMemcpyDeviceToHost, st));
     RT_HIP(hipStreamSynchronize(st));
+    if (cache != nullptr) {
+        cache->last_records = (int32_t)std::min<uint32_t>(stats[rtk::kStatCacheRecords],
+            (uint32_t)cache->cfg.train_samples);
+        cache->last_queries = (int32_t)stats[rtk::kStatCacheQueries];
+    }
     r->rendered = true;
     // capture_launch_history (radiance_launch_setup.cpp:122-136)
     for (int c = 0; c < rig->active_count; ++c) {
@@ -1281,7 +1360,32 @@ rt_status rt_renderer_set_denoiser(rt_renderer* r, rt_nn* nn) {
     RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
     RT_REQUIRE(nn == nullptr || rt_nn_device(nn) == r->dev->ordinal, RT_ERROR_INVALID_ARGUMENT,
         "the denoiser lives on another device");
+    if (nn) rt_nn_retain(nn);
+    if (r->nn) {
+        (void)hipStreamSynchronize(r->dev->stream);
+        rt_nn_release(r->nn);
+    }
     r->nn = nn;
+    rt_alloc_bump();  // captured frame graphs hold the previous network's buffers
+    return RT_OK;
+}
+
+rt_status rt_renderer_set_radiance_cache(rt_renderer* r, rt_cache* cache) {
+    RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
+    RT_REQUIRE(cache == nullptr || cache->ordinal == r->dev->ordinal, RT_ERROR_INVALID_ARGUMENT,
+        "the radiance cache lives on another device");
+    RT_HIP(hipSetDevice(r->dev->ordinal));
+    if (cache) {
+        const rt_status bs = rt_cache_set_bounds(cache, r->scene->lo, r->scene->hi);
+        if (bs != RT_OK) return bs;
+        rt_cache_retain(cache);
+    }
+    if (r->cache) {
+        (void)hipStreamSynchronize(r->dev->stream);
+        rt_cache_release(r->cache);
+    }
+    r->cache = cache;
+    rt_alloc_bump();  // captured frame graphs hold the previous cache's buffers and bounds
     return RT_OK;
 }
 

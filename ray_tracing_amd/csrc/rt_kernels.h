@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "rt_cache.h"
 #include "rt_internal.h"
 
 namespace rtk {
@@ -86,6 +87,13 @@ struct FrameParams {
     int32_t layer_count;    // samples per pixel in this chunk
     int32_t first_sample;   // sample index of the chunk's first layer
     int32_t restir;         // bounce-0 analytic lighting goes through the ReSTIR stage
+    // radiance cache (cache_bounce = -1 without one)
+    int32_t cache_bounce;            // path vertex of queries and training records
+    float cache_train_fraction;      // share of pixel samples traced fully for training
+    const CacheState* cache_state;   // usable flag (device memory)
+    CacheInput* cache_queries;       // capacity = path capacity
+    CacheRecord* cache_records;
+    int32_t cache_record_cap;
 };
 
 struct StageLaunch {
@@ -111,6 +119,14 @@ hipError_t wf_restir(const SceneView& s, const CameraView& cams,
     const FrameParams& fp, const RestirView& rv, WavefrontBuffers& b, StageLaunch l,
     hipStream_t st);
 hipError_t wf_accumulate(const FrameParams& fp, WavefrontBuffers& b, hipStream_t st);
+// radiance cache, after a chunk's bounces: cache predictions for the paths
+// that ended at the query vertex, targets of the chunk's training records
+hipError_t wf_cache_chunk(const CacheGridView& g, const CachePacked& pk, const FrameParams& fp,
+    WavefrontBuffers& b, int path_cap, int grid, hipStream_t st);
+// device counters of the cache: records written this frame, queries answered
+const uint32_t* wf_cache_record_count(const WavefrontBuffers& b);
+constexpr int kStatWords = 24;  // counter words downloaded after a frame
+constexpr int kStatCacheRecords = 21, kStatCacheQueries = 23;
 hipError_t wf_clear_accum(int pixels, WavefrontBuffers& b, hipStream_t st);
 // zero `bytes` (a multiple of 4) of device memory with a fill kernel
 hipError_t wf_zero(void* p, size_t bytes, hipStream_t st);

@@ -15,5 +15,8 @@ hipError_t rt_nn_infer_on_stream(rt_nn* nn, const rtk::NnImages& images, int P,
     const float4* beauty, const float4* normal, const float4* albedo, const float* depth,
     float4* out, hipStream_t st);
 int rt_nn_device(const rt_nn* nn);
+// reference counting: a renderer keeps the network it uses alive
+void rt_nn_retain(rt_nn* nn);
+void rt_nn_release(rt_nn* nn);
 // allocates the network's scratch for P pixels (before a graph capture)
 hipError_t rt_nn_reserve(rt_nn* nn, int P);

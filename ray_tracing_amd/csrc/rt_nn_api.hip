@@ -3,9 +3,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <string>
 
+#include "rt_alloc.h"
 #include "rt_nn.h"
 #include "rt_nn_abi.h"
 
@@ -51,6 +53,7 @@ struct Buf {
         if (c == 0) c = 1;
         const hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * c);
         if (e == hipSuccess) n = c;
+        rt_alloc_bump();
         return e;
     }
 };
@@ -58,6 +61,7 @@ struct Buf {
 }  // namespace
 
 struct rt_nn {
+    std::atomic<int> refs{1};  // the caller's handle plus each renderer it is attached to
     int ordinal = 0;
     hipStream_t stream = nullptr;
     int cu_count = 0;
@@ -148,6 +152,17 @@ int rt_nn_device(const rt_nn* nn) { return nn->ordinal; }
 
 hipError_t rt_nn_reserve(rt_nn* nn, int P) { return nn->ensure((size_t)P); }
 
+void rt_nn_retain(rt_nn* nn) { nn->refs.fetch_add(1); }
+
+void rt_nn_release(rt_nn* nn) {
+    if (nn->refs.fetch_sub(1) != 1) return;
+    (void)hipSetDevice(nn->ordinal);
+    (void)hipStreamSynchronize(nn->stream);
+    hipStream_t st = nn->stream;
+    delete nn;
+    if (st) (void)hipStreamDestroy(st);
+}
+
 extern "C" {
 
 int32_t rt_nn_param_count(void) { return rtk::nn_param_count(); }
@@ -195,13 +210,10 @@ rt_status rt_nn_create(int ordinal, rt_nn** out) {
 
 rt_status rt_nn_destroy(rt_nn* nn) {
     if (!nn) return RT_OK;
-    (void)hipSetDevice(nn->ordinal);
-    (void)hipStreamSynchronize(nn->stream);
-    hipStream_t st = nn->stream;
-    delete nn;
-    if (st) (void)hipStreamDestroy(st);
+    rt_nn_release(nn);
     return RT_OK;
 }
+
 
 rt_status rt_nn_set_params(rt_nn* nn, const float* params, int32_t n) {
     NN_REQUIRE(nn && params, RT_ERROR_INVALID_ARGUMENT, "null network or parameters");

@@ -12,6 +12,8 @@
 //   2 instanced_mesh_restir : 1000 rigid instances of a 1024-triangle mesh
 //                             in a 100-unit cube, 256 analytic sphere lights;
 //                             2 pinhole32 640x480 + 2 equi62_lut1d 1024x1024
+//   3 radiance_cache_realtime : the config-2 scene with 2 pinhole32 + 2
+//                             equi62_lut1d cameras at 1280x720, depth 8
 // Draws come from PCG32 (O'Neill 2014; pcg32_srandom_r(seed, 54), XSH-RR
 // output); a float draw is (x >> 8) * 2^-24; the draw order is part of the
 // scene definition.
@@ -290,7 +292,8 @@ double gauss(Pcg32& r, double sigma) {
 // four cameras on a radius-70 orbit around the cube centre, 90 degrees apart:
 // two pinhole32 640x480 (vfov 60) and two equi62_lut1d 1024x1024 with
 // distortion k_i ~ N(0, 0.01) redrawn until theta_d is monotone on [0, 1.6]
-void instanced_rig(uint64_t seed, int frame, int frames, rt_camera* cams) {
+void instanced_rig(uint64_t seed, int frame, int frames, rt_camera* cams, int wp = 640,
+    int hp = 480, int we = 1024, int he = 1024) {
     Pcg32 r(seed ^ 0x5bd1e995ULL);
     double radial[2][6];
     for (int k = 0; k < 2; ++k) {
@@ -318,9 +321,9 @@ void instanced_rig(uint64_t seed, int frame, int frames, rt_camera* cams) {
         const std::array<double, 3> from = {center[0] + 70.0 * std::cos(th), center[1] + 15.0,
             center[2] + 70.0 * std::sin(th)};
         if (k < 2) {
-            make_lookat_pinhole(&cams[k], 640, 480, 60.0, from, center, up);
+            make_lookat_pinhole(&cams[k], wp, hp, 60.0, from, center, up);
         } else {
-            make_lookat_equi(&cams[k], 1024, 1024, radial[k - 2], from, center, up);
+            make_lookat_equi(&cams[k], we, he, radial[k - 2], from, center, up);
         }
     }
 }
@@ -482,6 +485,11 @@ bool generate_rig(int config, uint64_t seed, int frame, int frames, rt_camera_ri
         out->active_count = 4;
         return true;
     }
+    if (config == 3) {
+        instanced_rig(seed, frame, frames, out->cameras, 1280, 720, 1280, 720);
+        out->active_count = 4;
+        return true;
+    }
     rt_generated_scene g;
     if (!generate_scene(config, seed, &g)) return false;
     out->active_count = g.camera_count;
@@ -573,6 +581,14 @@ bool generate_scene(int config, uint64_t seed, rt_generated_scene* g) {
         case 0: generate_random_spheres(seed, g); return true;
         case 1: generate_cornell_mesh(seed, g); return true;
         case 2: generate_instanced_mesh(seed, g); return true;
+        case 3:
+            // the config-2 scene, four 1280x720 cameras, paths of depth 8 with
+            // Russian roulette from bounce 3 (the radiance-cache workload)
+            generate_instanced_mesh(seed, g);
+            instanced_rig(seed, 0, 60, g->cameras, 1280, 720, 1280, 720);
+            g->profile.max_bounces = 8;
+            g->profile.rr_start_bounce = 3;
+            return true;
         default: return false;
     }
 }

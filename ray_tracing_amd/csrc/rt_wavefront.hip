@@ -35,7 +35,11 @@ namespace {
 enum : uint32_t { kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntShadow2 = 3,
     kCntStatExtend = 4, kCntStatShadow = 6, kCntVisits = 8, kCntTests = 10, kCntSphereTests = 12,
     kCntRestir = 14, kCntFetch = 16 /* 3 ray-fetch cursors: extend, connect 0, connect 1 */,
-    kCounterWords = 20 };
+    kCntCacheQuery = 20 /* cache queries of the chunk */, kCntCacheRecord = 21 /* training
+    records of the frame */, kCntCacheDone = 22 /* records with targets */,
+    kCntCacheQueries = 23 /* cache queries of the frame */, kCounterWords = 24 };
+static_assert(kCntCacheRecord == kStatCacheRecords && kCntCacheQueries == kStatCacheQueries
+    && kCounterWords == kStatWords, "counter layout");
 
 enum : int32_t { kPrevValid = 1, kPrevDelta = 2 };
 
@@ -106,6 +110,7 @@ __global__ __launch_bounds__(256) void k_raygen(CameraView cams, FrameParams fp,
         b.counters[kCntShadow] = 0u;
         b.counters[kCntShadow2] = 0u;
         b.counters[kCntRestir] = 0u;
+        b.counters[kCntCacheQuery] = 0u;
     }
     if (i >= paths) return;
     const SlotId id = slot_id(cams, fp, (uint32_t)i);
@@ -181,6 +186,24 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneView s, WavefrontBuffers 
     }
 }
 
+// the radiance cache's input at a surface hit: position (w = path slot),
+// outgoing direction and roughness, shading normal, albedo
+__device__ __forceinline__ CacheInput cache_input(const SceneView& s, const HitInfo& h, f3 rd,
+    uint32_t slot) {
+    const f3 wo = -normalize3(rd);
+    float rough = 1.0f;
+    if (h.mat_type == RT_MAT_METAL) rough = h.fuzz;
+    else if (h.mat_type == RT_MAT_DIELECTRIC) rough = 0.0f;
+    else if (h.mat_type == RT_MAT_OPENPBR) rough = s.openpbr[h.openpbr].specular_roughness;
+    CacheInput c;
+    c.a = make_float4(h.pos.x, h.pos.y, h.pos.z, __uint_as_float(slot));
+    c.b = make_float4(wo.x, wo.y, wo.z, rough);
+    c.c = make_float4(h.sn.x, h.sn.y, h.sn.z, 0.0f);
+    c.d = make_float4(fminf(fmaxf(h.base.x, 0.0f), 1.0f), fminf(fmaxf(h.base.y, 0.0f), 1.0f),
+        fminf(fmaxf(h.base.z, 0.0f), 1.0f), 0.0f);
+    return c;
+}
+
 __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, FrameParams fp,
     WavefrontBuffers b, int bounce, int cur) {
     const uint32_t fp_stream = fp.dyn->stream;
@@ -236,9 +259,36 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                 }
                 const float ew = emission_mis(s, pv, rd, &h, false);
                 rad = rad + thr * (h.emission * ew);
-                const bool bsdf_avail = bounce + 1 < fp.max_bounces;
+                // radiance cache at the query vertex: training paths leave a
+                // record and go on with the plain integrator; once the cache
+                // is usable the other paths end here with its prediction
+                bool cache_end = false;
+                if (bounce == fp.cache_bounce && h.mat_type != RT_MAT_DIFFUSE_LIGHT) {
+                    const float4 rc = rng4(id.pixel, id.sample, id.camera, 0u, kGroupCache, fp_stream);
+                    const bool train = rc.x < fp.cache_train_fraction;
+                    const bool query = !train && fp.cache_state->usable != 0;
+                    CacheInput ci{};
+                    if (train || query) {
+                        ci = cache_input(s, h, rd, slot);
+                        if (train) {
+                            const uint32_t k = atomicAdd(&b.counters[kCntCacheRecord], 1u);
+                            if ((int)k < fp.cache_record_cap) {
+                                CacheRecord* R = fp.cache_records + k;
+                                R->in = ci;
+                                R->thr = f4(thr, __uint_as_float(slot));
+                                R->rad = f4(rad, 0.0f);
+                            }
+                        }
+                    }
+                    const uint32_t qpos = wave_append(query, &b.counters[kCntCacheQuery]);
+                    if (query) {
+                        fp.cache_queries[qpos] = ci;
+                        cache_end = true;
+                    }
+                }
+                const bool bsdf_avail = bounce + 1 < fp.max_bounces && !cache_end;
                 // next-event estimation toward emissive geometry (programs.cu:2249-2280)
-                if (h.mat_type != RT_MAT_DIFFUSE_LIGHT && s.light_count > 0) {
+                if (!cache_end && h.mat_type != RT_MAT_DIFFUSE_LIGHT && s.light_count > 0) {
                     const LightSample ls = sample_light(s, h, rl);
                     if (ls.valid) {
                         float bpdf = 0.0f;
@@ -266,7 +316,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                         b.restir_d[slot] = f4(rd, hv.x);
                         b.restir_thr[slot] = f4(thr, __int_as_float(prim));
                     }
-                } else if (h.mat_type != RT_MAT_DIFFUSE_LIGHT && s.analytic_count > 0) {
+                } else if (!cache_end && h.mat_type != RT_MAT_DIFFUSE_LIGHT && s.analytic_count > 0) {
                     const float4 ra = rng4(id.pixel, id.sample, id.camera, (uint32_t)bounce,
                         kGroupAnalytic, fp_stream);
                     const int li = pick_analytic(s, ra.x);
@@ -440,6 +490,27 @@ __global__ void k_swap_queues(uint32_t* counters, int cur) {
     }
 }
 
+// targets of the chunk's training records: (final path radiance - radiance
+// at the query vertex) / throughput there, per channel
+__global__ __launch_bounds__(256) void k_cache_collect(WavefrontBuffers b, CacheRecord* rec, int cap) {
+    const uint32_t done = b.counters[kCntCacheDone];
+    const uint32_t end = min(b.counters[kCntCacheRecord], (uint32_t)cap);
+    for (uint32_t i = done + blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x) {
+        CacheRecord* R = rec + i;
+        const float4 thr = R->thr, rad = R->rad;
+        const float4 fin = b.radiance[__float_as_uint(thr.w)];
+        R->target = make_float4(thr.x > 0.0f ? (fin.x - rad.x) / thr.x : 0.0f,
+            thr.y > 0.0f ? (fin.y - rad.y) / thr.y : 0.0f, thr.z > 0.0f ? (fin.z - rad.z) / thr.z : 0.0f,
+            0.0f);
+    }
+}
+
+__global__ void k_cache_chunk_done(uint32_t* counters, int cap) {
+    if (threadIdx.x != 0) return;
+    counters[kCntCacheDone] = min(counters[kCntCacheRecord], (uint32_t)cap);
+    counters[kCntCacheQueries] += counters[kCntCacheQuery];
+}
+
 __global__ __launch_bounds__(256) void k_accumulate(FrameParams fp, WavefrontBuffers b) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= fp.total_pixels) return;
@@ -581,6 +652,18 @@ hipError_t wf_restir(const SceneView& s, const CameraView& cams, const FramePara
     k_restir<128><<<l.grid > 0 ? l.grid : 1, 128, 0, st>>>(s, cams, fp, rv, b);
     return hipGetLastError();
 }
+
+hipError_t wf_cache_chunk(const CacheGridView& g, const CachePacked& pk, const FrameParams& fp,
+    WavefrontBuffers& b, int path_cap, int grid, hipStream_t st) {
+    hipError_t e = cache_infer_paths(g, pk, fp.cache_queries, b.counters + kCntCacheQuery, path_cap,
+        b.throughput, b.radiance, grid, st);
+    if (e != hipSuccess) return e;
+    k_cache_collect<<<256, 256, 0, st>>>(b, fp.cache_records, fp.cache_record_cap);
+    k_cache_chunk_done<<<1, 64, 0, st>>>(b.counters, fp.cache_record_cap);
+    return hipGetLastError();
+}
+
+const uint32_t* wf_cache_record_count(const WavefrontBuffers& b) { return b.counters + kCntCacheRecord; }
 
 hipError_t wf_accumulate(const FrameParams& fp, WavefrontBuffers& b, hipStream_t st) {
     k_accumulate<<<(fp.total_pixels + 255) / 256, 256, 0, st>>>(fp, b);

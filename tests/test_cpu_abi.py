@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_entry_points():
     src = open(os.path.join(ROOT, "include", "rt", "rt_hip.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:rt_status|int|int32_t|const char\*)\s+(rt_\w+)\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:rt_status|int|int32_t|int64_t|void|const char\*)\s+(rt_\w+)\(", src, re.M)))
 
 
 def test_exports_every_declared_symbol(rt):
@@ -46,7 +46,7 @@ def test_invalid_arguments_report_errors(rt):
     assert b"unknown scene config" in rt.lib().rt_last_error()
 
 
-@pytest.mark.parametrize("config", [0, 1, 2])
+@pytest.mark.parametrize("config", [0, 1, 2, 3])
 def test_generators_match_oracle(rt, oracle, config):
     g = rt.Generated(config, 0)
     og = oracle.GeneratedScene(config, 0)
