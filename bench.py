@@ -282,6 +282,14 @@ def main():
             result["learned_vs_plain"] = equal_error.compare(dev, log=log)
         except Exception as e:  # the throughput line is still printed
             result["learned_vs_plain"] = {"error": f"{type(e).__name__}: {e}"}
+        # radiance cache + learned denoiser on the four-camera 1280x720 depth-8
+        # workload, against the plain tracer at equal relMSE
+        try:
+            from ray_tracing_amd import cache_eval
+
+            result["cache_vs_plain"] = cache_eval.compare(dev, log=log)
+        except Exception as e:
+            result["cache_vs_plain"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(result), flush=True)
 
 

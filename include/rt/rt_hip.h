@@ -376,9 +376,12 @@ typedef struct rt_cache_config {
     int32_t query_bounce;     /* path vertex where paths end with the cache (>= 1) */
     int32_t warmup_frames;    /* frames of training before the cache ends paths */
     float learning_rate;      /* Adam step size for the MLP and the grid */
-    float max_rel_loss;       /* fallback threshold on the running relative loss */
+    float max_rel_loss;       /* fallback threshold on the running relative loss (its
+                                 fresh single-sample targets keep it near 1-4 on a
+                                 working cache; the default 8 catches divergence) */
     float train_fraction;     /* share of pixel samples traced fully as training paths;
-                                 0: 1.5 x train_samples / paths of the frame */
+                                 0: adaptive, from 1.5 x train_samples / paths of the
+                                 frame toward 1.1 x train_samples records a frame */
     uint32_t seed;            /* parameter initialisation */
 } rt_cache_config;
 
@@ -390,7 +393,7 @@ typedef struct rt_cache_stats {
     int32_t steps;            /* Adam steps taken */
     int32_t records;          /* training records of the last frame */
     int32_t queries;          /* paths ended with the cache in the last frame */
-    int32_t pad;
+    float train_fraction;     /* share of pixel samples traced as training paths next */
 } rt_cache_stats;
 
 typedef struct rt_cache rt_cache;

@@ -381,7 +381,7 @@ class RadianceCache:
     def stats(self) -> dict:
         st = abi.CacheStats()
         check(lib().rt_cache_get_stats(self.h, C.byref(st)))
-        return {k: getattr(st, k) for k, _ in abi.CacheStats._fields_ if k != "pad"}
+        return {k: getattr(st, k) for k, _ in abi.CacheStats._fields_}
 
 
 class Generated:

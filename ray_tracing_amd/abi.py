@@ -156,7 +156,7 @@ class CacheConfig(C.Structure):
 class CacheStats(C.Structure):
     _fields_ = [("loss_ema", C.c_float), ("last_loss", C.c_float), ("frames", C.c_int32),
                 ("usable", C.c_int32), ("steps", C.c_int32), ("records", C.c_int32),
-                ("queries", C.c_int32), ("pad", C.c_int32)]
+                ("queries", C.c_int32), ("train_fraction", C.c_float)]
 
 
 class DeviceFrameView(C.Structure):

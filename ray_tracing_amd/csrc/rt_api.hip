@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1106,10 +1107,17 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
         RT_HIP(r->cache_records.reserve((size_t)cc.train_samples));
         RT_HIP(cache->ensure(cc.train_samples));
         fp.cache_bounce = cc.query_bounce;
-        const double paths_per_frame = (double)P * (double)spp;
-        fp.cache_train_fraction = cc.train_fraction > 0.0f
-            ? cc.train_fraction
-            : (float)std::min(1.0, 1.5 * (double)cc.train_samples / std::max(paths_per_frame, 1.0));
+        if (!cache->fraction_set) {
+            // the training share starts at the configured value, or (adaptive)
+            // at 1.5 x train_samples / paths of this frame
+            const double paths_per_frame = (double)P * (double)spp;
+            cache->fraction_seed = cc.train_fraction > 0.0f
+                ? cc.train_fraction
+                : (float)std::min(1.0, 1.5 * (double)cc.train_samples / std::max(paths_per_frame, 1.0));
+            RT_HIP(hipMemcpyAsync(reinterpret_cast<char*>(cache->state.p) + offsetof(rtk::CacheState, train_fraction),
+                &cache->fraction_seed, sizeof(float), hipMemcpyHostToDevice, st));
+            cache->fraction_set = true;
+        }
         fp.cache_state = cache->state.p;
         fp.cache_queries = r->cache_queries.p;
         fp.cache_records = r->cache_records.p;

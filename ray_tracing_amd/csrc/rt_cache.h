@@ -34,7 +34,7 @@ constexpr int kCacheOut = 3;
 constexpr int kCacheMaxLevels = 16;
 constexpr int kCacheFeatures = 2;
 constexpr int kCacheSamplesPerBlock = 256;   // 4 waves x 64 samples
-constexpr int kCacheGradSamples = 1024;      // samples per weight-gradient block
+constexpr int kCacheGradSamples = 128;       // samples per weight-gradient block
 constexpr float kCacheLossEps = 0.01f;
 
 // input channel layout
@@ -88,7 +88,8 @@ struct CacheState {
     int32_t frames;       // frames trained
     int32_t usable;       // the renderer terminates paths with the cache
     int32_t steps;        // Adam steps taken
-    int32_t pad[3];
+    float train_fraction; // share of pixel samples traced as training paths
+    int32_t pad[2];
 };
 
 // One training record per training path, written at the query vertex:
@@ -127,8 +128,10 @@ hipError_t cache_adam(int64_t n, float* params, const float* grad, float* m, flo
 hipError_t cache_fill(float* p, size_t n, float value, hipStream_t st);
 // after a frame's training with *count > 0 records: EMA of the first
 // batch's loss, frames += 1, usable = frames >= warmup and
-// loss_ema <= max_rel_loss
+// loss_ema <= max_rel_loss (a NaN loss never passes); with `adapt` the
+// training share moves toward 1.1 x `target` records a frame (at most 2x
+// per frame, within [1e-5, 1])
 hipError_t cache_update_state(CacheState* state, const float* first_loss, const uint32_t* count,
-    int warmup, float max_rel_loss, hipStream_t st);
+    int warmup, float max_rel_loss, int adapt, int target, hipStream_t st);
 
 }  // namespace rtk

@@ -492,8 +492,16 @@ __global__ void k_cache_step(CacheState* state, const uint32_t* count) {
 }
 
 __global__ void k_cache_update_state(CacheState* state, const float* first_loss,
-    const uint32_t* count, int warmup, float max_rel_loss) {
-    if (threadIdx.x != 0 || *count == 0u) return;
+    const uint32_t* count, int warmup, float max_rel_loss, int adapt, int target) {
+    if (threadIdx.x != 0) return;
+    const uint32_t n = *count;
+    if (adapt) {
+        const float want = 1.1f * (float)target;
+        const float ratio = n > 0u ? want / (float)n : 2.0f;
+        const float f = state->train_fraction * fminf(fmaxf(ratio, 0.5f), 2.0f);
+        state->train_fraction = fminf(fmaxf(f, 1e-5f), 1.0f);
+    }
+    if (n == 0u) return;
     const float l = *first_loss;
     state->last_loss = l;
     state->loss_ema = state->frames == 0 ? l : 0.8f * state->loss_ema + 0.2f * l;
@@ -635,8 +643,9 @@ hipError_t cache_fill(float* p, size_t n, float value, hipStream_t st) {
 }
 
 hipError_t cache_update_state(CacheState* state, const float* first_loss, const uint32_t* count,
-    int warmup, float max_rel_loss, hipStream_t st) {
-    k_cache_update_state<<<1, 64, 0, st>>>(state, first_loss, count, warmup, max_rel_loss);
+    int warmup, float max_rel_loss, int adapt, int target, hipStream_t st) {
+    k_cache_update_state<<<1, 64, 0, st>>>(state, first_loss, count, warmup, max_rel_loss, adapt,
+        target);
     return hipGetLastError();
 }
 

@@ -59,6 +59,8 @@ struct rt_cache {
     float lo[3] = {0.0f, 0.0f, 0.0f}, inv_extent[3] = {1.0f, 1.0f, 1.0f};
     // last frame of the renderer it is attached to
     int32_t last_records = 0, last_queries = 0;
+    bool fraction_set = false;  // CacheState::train_fraction holds a seeded value
+    float fraction_seed = 0.0f; // host source of that seed copy
 
     rtk::CacheGridView grid_view() const {
         rtk::CacheGridView g{};

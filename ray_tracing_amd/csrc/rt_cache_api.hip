@@ -107,6 +107,7 @@ rt_status reset_training(rt_cache* c) {
     CACHE_HIP(hipMemsetAsync(c->v.p, 0, bytes, c->stream));
     CACHE_HIP(hipMemsetAsync(c->grad.p, 0, bytes, c->stream));
     CACHE_HIP(hipMemsetAsync(c->state.p, 0, sizeof(rtk::CacheState), c->stream));
+    c->fraction_set = false;  // the renderer seeds the training share on its next frame
     CACHE_HIP(rtk::cache_pack(c->params.p, c->packed(), c->stream));
     return RT_OK;
 }
@@ -146,7 +147,7 @@ hipError_t rt_cache_train_frame(rt_cache* c, const rtk::CacheRecord* records,
     }
     if (e == hipSuccess)
         e = rtk::cache_update_state(c->state.p, c->loss.p, count, c->cfg.warmup_frames,
-            c->cfg.max_rel_loss, st);
+            c->cfg.max_rel_loss, c->cfg.train_fraction > 0.0f ? 0 : 1, c->cfg.train_samples, st);
     return e;
 }
 
@@ -164,7 +165,7 @@ void rt_cache_default_config(rt_cache_config* c) {
     c->query_bounce = 2;
     c->warmup_frames = 4;
     c->learning_rate = 1e-2f;
-    c->max_rel_loss = 1.0f;
+    c->max_rel_loss = 8.0f;
     c->train_fraction = 0.0f;
     c->seed = 1u;
 }
@@ -349,6 +350,7 @@ rt_status rt_cache_get_stats(rt_cache* c, rt_cache_stats* out) {
     out->steps = st.steps;
     out->records = c->last_records;
     out->queries = c->last_queries;
+    out->train_fraction = st.train_fraction;
     return RT_OK;
 }
 

@@ -89,8 +89,7 @@ struct FrameParams {
     int32_t restir;         // bounce-0 analytic lighting goes through the ReSTIR stage
     // radiance cache (cache_bounce = -1 without one)
     int32_t cache_bounce;            // path vertex of queries and training records
-    float cache_train_fraction;      // share of pixel samples traced fully for training
-    const CacheState* cache_state;   // usable flag (device memory)
+    const CacheState* cache_state;   // usable flag, training share (device memory)
     CacheInput* cache_queries;       // capacity = path capacity
     CacheRecord* cache_records;
     int32_t cache_record_cap;

@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                 bool cache_end = false;
                 if (bounce == fp.cache_bounce && h.mat_type != RT_MAT_DIFFUSE_LIGHT) {
                     const float4 rc = rng4(id.pixel, id.sample, id.camera, 0u, kGroupCache, fp_stream);
-                    const bool train = rc.x < fp.cache_train_fraction;
+                    const bool train = rc.x < fp.cache_state->train_fraction;
                     const bool query = !train && fp.cache_state->usable != 0;
                     CacheInput ci{};
                     if (train || query) {
