@@ -308,7 +308,8 @@ rt_status rt_nn_get_params(rt_nn* nn, float* params, int32_t n);
 /* device pointers */
 rt_status rt_nn_infer(rt_nn* nn, const rt_nn_images* images, const float* beauty,
     const float* normal, const float* albedo, const float* depth, float* out);
-/* one training step on a batch: loss = mean (out_log - log1p(target))^2; the
+/* one training step on a batch: loss = mean over pixels and RGB of
+ * w (out_log - log1p(t))^2, t = max(target, 0), w = (1 + t)^2 / (t^2 + 1e-2); the
  * gradient (before the update, host array of rt_nn_param_count floats) and
  * the loss are returned when non-NULL; lr = 0 computes them without updating */
 rt_status rt_nn_train_step(rt_nn* nn, const rt_nn_images* images, const float* beauty,
@@ -432,7 +433,7 @@ rt_status rt_renderer_set_radiance_cache(rt_renderer* r, rt_cache* cache);
  * Seeded generators of the benchmark scenes (this repository's equivalent
  * of build_scene(id) + make_rig, src/scene/shared_scene_builders.h:56-57):
  *   0 random_spheres, 1 cornell_mesh, 2 instanced_mesh_restir,
- *   3 radiance_cache_realtime */
+ *   3 radiance_cache_realtime, 4 cornell_random (seeded denoiser training scenes) */
 typedef struct rt_generated_scene {
     rt_scene_desc scene;
     int32_t camera_count;

@@ -294,6 +294,83 @@ static int gen_cornell_mesh(rto_generated* g, uint64_t seed) {
     return 0;
 }
 
+/* config 4: Cornell room with 1-8 displaced icospheres (subdivision 4),
+ * one random OpenPBR material each and a random ceiling light (see the
+ * library's generate_cornell_random for the draw order) */
+static int gen_cornell_random(rto_generated* g, uint64_t seed) {
+    pcg32_t r;
+    pcg_seed(&r, seed);
+    const float lw = 80.0f + 80.0f * pcg_u(&r);
+    const float ld = 70.0f + 70.0f * pcg_u(&r);
+    const float lx = 20.0f + (515.0f - lw) * pcg_u(&r);
+    const float lz = 20.0f + (515.0f - ld) * pcg_u(&r);
+    const float radiance = 8.0f + 12.0f * pcg_u(&r);
+    int spheres = 1 + (int)(8.0f * pcg_u(&r));
+    if (spheres > 8) spheres = 8;
+    rto_material* mt = (rto_material*)calloc((size_t)(4 + spheres), sizeof(rto_material));
+    for (int i = 0; i < 4 + spheres; ++i) { mt[i].openpbr_index = -1; mt[i].ior = 1.0f; }
+    set3(mt[0].albedo, 0.65f, 0.05f, 0.05f);
+    set3(mt[1].albedo, 0.73f, 0.73f, 0.73f);
+    set3(mt[2].albedo, 0.12f, 0.45f, 0.15f);
+    mt[3].type = RTO_MAT_DIFFUSE_LIGHT; set3(mt[3].emission, radiance, radiance, radiance);
+    rto_quad* q = (rto_quad*)calloc(6, sizeof(rto_quad));
+    quad_set(&q[0], 555, 0, 0, 0, 555, 0, 0, 0, 555, 2);
+    quad_set(&q[1], 0, 0, 0, 0, 555, 0, 0, 0, 555, 0);
+    quad_set(&q[2], lx + lw, 554, lz + ld, -lw, 0, 0, 0, 0, -ld, 3);
+    quad_set(&q[3], 0, 0, 0, 555, 0, 0, 0, 0, 555, 1);
+    quad_set(&q[4], 555, 555, 555, -555, 0, 0, 0, 0, -555, 1);
+    quad_set(&q[5], 0, 0, 555, 555, 0, 0, 0, 555, 0, 1);
+    double (*verts)[3];
+    int32_t (*faces)[3];
+    int nv, nf;
+    rto_icosphere(4, &verts, &faces, &nv, &nf);
+    rto_openpbr* op = (rto_openpbr*)calloc((size_t)spheres, sizeof(rto_openpbr));
+    rto_triangle* tr = (rto_triangle*)calloc((size_t)nf * (size_t)spheres, sizeof(rto_triangle));
+    float (*pos)[3] = (float (*)[3])malloc(sizeof(float) * 3 * (size_t)nv);
+    for (int m = 0; m < spheres; ++m) {
+        const double radius = 40.0 + 70.0 * (double)pcg_u(&r);
+        double center[3];
+        for (int k = 0; k < 3; ++k) center[k] = radius + (555.0 - 2.0 * radius) * (double)pcg_u(&r);
+        openpbr_defaults(&op[m]);
+        for (int ch = 0; ch < 3; ++ch) op[m].base_color[ch] = pcg_u(&r);
+        op[m].specular_roughness = 0.05f + 0.85f * pcg_u(&r);
+        op[m].base_metalness = pcg_u(&r) < 0.5f ? 0.0f : 1.0f;
+        op[m].specular_ior = 1.5f;
+        op[m].coat_weight = 0.5f * pcg_u(&r);
+        mt[4 + m].type = RTO_MAT_OPENPBR;
+        mt[4 + m].openpbr_index = m;
+        for (int ch = 0; ch < 3; ++ch) mt[4 + m].albedo[ch] = op[m].base_color[ch];
+        for (int i = 0; i < nv; ++i) {
+            const float u = pcg_u(&r);
+            const double rr = radius * (1.0 + 0.05 * (2.0 * (double)u - 1.0));
+            for (int k = 0; k < 3; ++k) pos[i][k] = (float)(center[k] + verts[i][k] * rr);
+        }
+        for (int f = 0; f < nf; ++f) {
+            rto_triangle* t = &tr[(size_t)m * nf + f];
+            memcpy(t->p0, pos[faces[f][0]], 12);
+            memcpy(t->p1, pos[faces[f][1]], 12);
+            memcpy(t->p2, pos[faces[f][2]], 12);
+            t->material = 4 + m;
+        }
+    }
+    free(pos); free(verts); free(faces);
+    g->scene.quad_count = 6; g->scene.quads = q;
+    g->scene.triangle_count = nf * spheres; g->scene.triangles = tr;
+    g->scene.material_count = 4 + spheres; g->scene.materials = mt;
+    g->scene.openpbr_count = spheres; g->scene.openpbr = op;
+    g->scene.background_mode = RTO_BG_CONSTANT;
+    set3(g->scene.background, 0.0f, 0.0f, 0.0f);
+    g->camera_count = 4;
+    const double center[3] = {277.5, 277.5, 277.5}, up[3] = {0, 1, 0};
+    for (int c = 0; c < 4; ++c) {
+        const double th = (double)c * 0.5 * 3.14159265358979323846;
+        const double from[3] = {center[0] + 800.0 * sin(th), center[1], center[2] - 800.0 * cos(th)};
+        lookat_pinhole(&g->cameras[c], 640, 480, 40.0, from, center, up);
+    }
+    profile_defaults(&g->profile, 16, 8);
+    return 0;
+}
+
 /* equi62_lut1d camera with the reference's default 120 degree horizontal
  * field of view (camera_models.cpp:196-226) and a 1024-entry theta LUT */
 static void lookat_equi(rto_camera* c, int w, int h, const double radial[6], const double from[3],
@@ -451,6 +528,7 @@ int rto_generate_scene(int config, uint64_t seed, rto_generated* g) {
         case 0: return gen_random_spheres(g, seed);
         case 1: return gen_cornell_mesh(g, seed);
         case 2: return gen_instanced_mesh(g, seed);
+        case 4: return gen_cornell_random(g, seed);
         case 3: {
             /* the config-2 scene, four 1280x720 cameras, paths of depth 8 with
              * Russian roulette from bounce 3 (the radiance-cache workload) */

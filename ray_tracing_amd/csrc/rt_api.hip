@@ -1097,7 +1097,12 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
         b.spill = r->spill.p;
     }
     const bool denoise = p->enable_denoise != 0;
-    if (denoise && r->nn != nullptr) RT_HIP(rt_nn_reserve(r->nn, total));
+    // the denoiser's image table: the cameras back to back
+    rtk::NnImages nn_images{};
+    nn_images.count = rig->active_count;
+    for (int c = 0; c <= rig->active_count; ++c) nn_images.offset[c] = r->pixel_offset[c];
+    for (int c = 0; c < rig->active_count; ++c) nn_images.width[c] = r->widths[c];
+    if (denoise && r->nn != nullptr) RT_HIP(rt_nn_reserve(r->nn, nn_images));
     // radiance cache: queries (one per path slot) and the frame's training records
     rt_cache* cache = r->cache;
     fp.cache_bounce = -1;
@@ -1184,11 +1189,7 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
         if (denoise) {
             if (marks) ENQ(hipEventRecord(r->post_ev[0], st));
             if (r->nn != nullptr) {
-                rtk::NnImages im{};
-                im.count = rig->active_count;
-                for (int c = 0; c <= rig->active_count; ++c) im.offset[c] = r->pixel_offset[c];
-                for (int c = 0; c < rig->active_count; ++c) im.width[c] = r->widths[c];
-                ENQ(rt_nn_infer_on_stream(r->nn, im, total, r->beauty_set[fs].p,
+                ENQ(rt_nn_infer_on_stream(r->nn, nn_images, total, r->beauty_set[fs].p,
                     r->normal_set[fs].p, r->albedo.p, r->depth_set[fs].p, r->denoised.p, st));
             } else {
                 ENQ(rtk::wf_denoise_filter(cv, rtk::kDenoiseIterations, r->beauty_set[fs].p,

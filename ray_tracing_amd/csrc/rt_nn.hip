@@ -1,24 +1,22 @@
-// rt_nn.hip -- the learned denoiser: BF16 MFMA inference and training
-// kernels with FP32 accumulation (gfx950 v_mfma_f32_16x16x32_bf16).
+// rt_nn.hip -- the learned denoiser (4-level encoder-decoder, see rt_nn.h):
+// BF16 MFMA inference and training kernels with FP32 accumulation (gfx950
+// v_mfma_f32_16x16x32_bf16).
 //
-// Network (per image, zero padding at borders; all images of a batch lie
-// back to back):
-//   f  (16 ch bf16) = log1p(beauty), albedo, normal, log1p(depth)/8, 0...
-//   h0 = relu(conv3x3 dil 1 (f ; 16 -> 32))
-//   h1 = relu(conv3x3 dil 2 (h0; 32 -> 32))
-//   h2 = relu(conv3x3 dil 4 (h1; 32 -> 32))
-//   h3 = relu(conv3x3 dil 8 (h2; 32 -> 32))       receptive field 31 x 31
-//   out_log = log1p(beauty) + W_head h3 + b_head  (1x1, FP32)
-//   loss = mean (out_log - log1p(target))^2
-//
-// Convolutions are implicit GEMMs: M = 16 pixels per MFMA tile, N = 32
-// output channels (two 16-wide tiles), K = 9 taps x input channels in
-// chunks of 32.  Each lane keeps its share of the layer's weights in
-// registers for the whole launch and streams pixel tiles.  The input-gradient
-// pass is the same kernel with transposed, tap-flipped weights and a ReLU
-// mask epilogue; the weight-gradient pass is a GEMM with K = pixels, written
-// as per-block partial sums reduced in a fixed order (deterministic).
+// Every 3x3 convolution is one implicit GEMM: M = 16 pixels per MFMA tile,
+// N = output channels (16-wide tiles), K = 9 taps x input channels in chunks
+// of 32 (channel counts are multiples of 8, so an 8-wide K group never
+// straddles a tap or the boundary between the two inputs of a decoder
+// convolution).  A decoder convolution reads its second input one level
+// coarser at the parent pixel (nearest upsampling folded into the load).
+// The input gradient of a convolution is the same kernel on the output
+// gradient with transposed, tap-flipped weights; it is written raw (bf16),
+// and small kernels apply the ReLU masks, add the skip and pooled paths and
+// sum the upsampled path back to the coarser level.  Weight gradients are
+// GEMMs with K = pixels, written as per-block partial sums reduced in a fixed
+// order (deterministic).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "rt_bf16.h"
 #include "rt_nn.h"
@@ -27,13 +25,14 @@ namespace rtk {
 namespace {
 
 struct Pix {
-    int base, w, h, x, y;
+    int img, base, w, h, x, y;
 };
 
 __device__ __forceinline__ Pix locate(const NnImages& im, int p) {
     int i = 0;
     while (i + 1 < im.count && p >= im.offset[i + 1]) ++i;
     Pix r;
+    r.img = i;
     r.base = im.offset[i];
     r.w = im.width[i];
     r.h = (im.offset[i + 1] - im.offset[i]) / r.w;
@@ -43,10 +42,13 @@ __device__ __forceinline__ Pix locate(const NnImages& im, int p) {
     return r;
 }
 
-__device__ __forceinline__ int tap_pixel(const Pix& q, int t, int d) {
-    const int x = q.x + (t % 3 - 1) * d, y = q.y + (t / 3 - 1) * d;
-    if (x < 0 || x >= q.w || y < 0 || y >= q.h) return -1;
-    return q.base + y * q.w + x;
+__device__ __forceinline__ int image_height(const NnImages& im, int i) {
+    return (im.offset[i + 1] - im.offset[i]) / im.width[i];
+}
+
+// index of the parent (one level coarser) of pixel (x, y) of image i
+__device__ __forceinline__ int parent_index(const NnImages& pim, int i, int x, int y) {
+    return pim.offset[i] + (y >> 1) * pim.width[i] + (x >> 1);
 }
 
 __global__ __launch_bounds__(256) void k_features(int P, const float4* beauty, const float4* normal,
@@ -72,81 +74,162 @@ __global__ __launch_bounds__(256) void k_features(int P, const float4* beauty, c
     dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
 }
 
-// Implicit-GEMM 3x3 convolution, 32 outputs.  MODE 0: out = relu(acc + bias);
-// MODE 1: out = acc where mask > 0 (input gradient through a ReLU).
-// W: bf16 [32][KP] with k = tap * CIN + ci (zero padded to KP).
-template <int CIN, int MODE>
-__global__ __launch_bounds__(256) void k_conv(NnImages im, int P, int dil, const uint16_t* in,
-    const uint16_t* W, const float* bias, const uint16_t* mask, uint16_t* out) {
+struct ConvArgs {
+    NnImages im;          // this level
+    NnImages pim;         // one level coarser (second input)
+    int P;                // pixels at this level
+    const uint16_t* A;    // [P][C0]
+    const uint16_t* B;    // [P coarser][C1]
+    const uint16_t* W;    // bf16 [COUT][kp]
+    int kp;
+    const float* bias;
+    uint16_t* out;        // [P][COUT]
+};
+
+// MODE 0: out = relu(acc + bias); MODE 1: out = acc (raw input gradient)
+template <int C0, int C1, int COUT, int MODE>
+__global__ __launch_bounds__(256) void k_uconv(ConvArgs a) {
+    constexpr int CIN = C0 + C1;
     constexpr int KC = (9 * CIN + 31) / 32;
-    constexpr int KP = KC * 32;
+    constexpr int NT = COUT / 16;
+    static_assert(CIN % 8 == 0 && C0 % 8 == 0 && COUT % 16 == 0, "channel layout");
     const int lane = threadIdx.x & 63;
     const int row = lane & 15;
-    const int kq = lane >> 4;  // which 8-wide k group of the 32-wide chunk
-    bf16x8 bw[KC][2];
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc) {
-#pragma unroll
-        for (int nt = 0; nt < 2; ++nt) {
-            bw[kc][nt] = *reinterpret_cast<const bf16x8*>(W + (size_t)(nt * 16 + row) * KP + kc * 32 + kq * 8);
-        }
-    }
+    const int kq = lane >> 4;
     const int waves = gridDim.x * 4;
-    const int tiles = (P + 15) / 16;
+    const int tiles = (a.P + 15) / 16;
     for (int tile = blockIdx.x * 4 + (threadIdx.x >> 6); tile < tiles; tile += waves) {
         const int p = tile * 16 + row;
-        const bool valid = p < P;
+        const bool valid = p < a.P;
         Pix q{};
-        if (valid) q = locate(im, p);
-        f32x4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (valid) q = locate(a.im, p);
+        f32x4 acc[NT];
 #pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         for (int kc = 0; kc < KC; ++kc) {
             const int k0 = kc * 32 + kq * 8;
-            const int t = k0 / CIN, ci = k0 % CIN;
-            bf16x8 a;
+            const int t = k0 / CIN, ci = k0 - t * CIN;
             uint4 raw = make_uint4(0u, 0u, 0u, 0u);
             if (valid && t < 9) {
-                const int s = tap_pixel(q, t, dil);
-                if (s >= 0) raw = *reinterpret_cast<const uint4*>(in + (size_t)s * CIN + ci);
+                const int sx = q.x + t % 3 - 1, sy = q.y + t / 3 - 1;
+                if (sx >= 0 && sx < q.w && sy >= 0 && sy < q.h) {
+                    if (C1 == 0 || ci < C0) {
+                        raw = *reinterpret_cast<const uint4*>(a.A + (size_t)(q.base + sy * q.w + sx) * C0 + ci);
+                    } else {
+                        raw = *reinterpret_cast<const uint4*>(
+                            a.B + (size_t)parent_index(a.pim, q.img, sx, sy) * C1 + (ci - C0));
+                    }
+                }
             }
-            a = __builtin_bit_cast(bf16x8, raw);
-            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[kc][0], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[kc][1], acc1, 0, 0, 0);
+            const bf16x8 av = __builtin_bit_cast(bf16x8, raw);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(a.W + (size_t)(nt * 16 + row) * a.kp + k0);
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[nt], 0, 0, 0);
+            }
         }
         // D[i][j]: lane = j + 16 (i / 4), element i % 4; i = pixel row, j = channel
 #pragma unroll
-        for (int nt = 0; nt < 2; ++nt) {
-            const f32x4 acc = nt == 0 ? acc0 : acc1;
+        for (int nt = 0; nt < NT; ++nt) {
             const int co = nt * 16 + row;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int pp = tile * 16 + kq * 4 + e;
-                if (pp >= P) continue;
-                float v = acc[e];
-                if (MODE == 0) {
-                    v = fmaxf(v + bias[co], 0.0f);
-                } else {
-                    v = from_bf16(mask[(size_t)pp * kNnHidden + co]) > 0.0f ? v : 0.0f;
-                }
-                out[(size_t)pp * kNnHidden + co] = to_bf16(v);
+                if (pp >= a.P) continue;
+                float v = acc[nt][e];
+                if (MODE == 0) v = fmaxf(v + a.bias[co], 0.0f);
+                a.out[(size_t)pp * COUT + co] = to_bf16(v);
             }
         }
     }
 }
 
+// mean of the pixels of level l that a level-(l+1) pixel covers
+__global__ __launch_bounds__(256) void k_pool(NnImages im, NnImages pim, int Pp, int C,
+    const uint16_t* in, uint16_t* out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= Pp * C) return;
+    const int pp = i / C, c = i - pp * C;
+    const Pix q = locate(pim, pp);
+    const int w = im.width[q.img], h = image_height(im, q.img), base = im.offset[q.img];
+    float s = 0.0f;
+    int n = 0;
+    for (int dy = 0; dy < 2; ++dy) {
+        for (int dx = 0; dx < 2; ++dx) {
+            const int x = 2 * q.x + dx, y = 2 * q.y + dy;
+            if (x < w && y < h) {
+                s += from_bf16(in[(size_t)(base + y * w + x) * C + c]);
+                ++n;
+            }
+        }
+    }
+    out[(size_t)pp * C + c] = to_bf16(s / (float)n);
+}
+
+// dz = r where act > 0
+__global__ __launch_bounds__(256) void k_mask(size_t n, const uint16_t* r, const uint16_t* act,
+    uint16_t* dz) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    dz[i] = from_bf16(act[i]) > 0.0f ? r[i] : (uint16_t)0;
+}
+
+// dz at level l+1 = sum over the covered level-l pixels of r[:, off .. off + C)
+// (the upsampled input's gradient), where act > 0
+__global__ __launch_bounds__(256) void k_up_combine(NnImages im, NnImages pim, int Pp, int C,
+    const uint16_t* r, int cr, int off, const uint16_t* act, uint16_t* dz) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= Pp * C) return;
+    const int pp = i / C, c = i - pp * C;
+    const Pix q = locate(pim, pp);
+    const int w = im.width[q.img], h = image_height(im, q.img), base = im.offset[q.img];
+    float s = 0.0f;
+    for (int dy = 0; dy < 2; ++dy) {
+        for (int dx = 0; dx < 2; ++dx) {
+            const int x = 2 * q.x + dx, y = 2 * q.y + dy;
+            if (x < w && y < h) s += from_bf16(r[(size_t)(base + y * w + x) * cr + off + c]);
+        }
+    }
+    dz[(size_t)pp * C + c] = from_bf16(act[(size_t)pp * C + c]) > 0.0f ? to_bf16(s) : (uint16_t)0;
+}
+
+// dz at level l = (skip gradient rs[:, 0..C) + pooled-path gradient rp at the
+// parent / pixels the parent covers), where act > 0
+__global__ __launch_bounds__(256) void k_skip_combine(NnImages im, NnImages pim, int P, int C,
+    const uint16_t* rs, int cs, const uint16_t* rp, const uint16_t* act, uint16_t* dz) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= P * C) return;
+    const int p = i / C, c = i - p * C;
+    const Pix q = locate(im, p);
+    const int px = q.x >> 1, py = q.y >> 1;
+    const int n = min(2, q.w - 2 * px) * min(2, q.h - 2 * py);
+    const int pp = pim.offset[q.img] + py * pim.width[q.img] + px;
+    const float v = from_bf16(rs[(size_t)p * cs + c]) + from_bf16(rp[(size_t)pp * C + c]) / (float)n;
+    dz[(size_t)p * C + c] = from_bf16(act[(size_t)p * C + c]) > 0.0f ? to_bf16(v) : (uint16_t)0;
+}
+
+// Per-channel weight of the log-space squared error: with e = log1p(y) -
+// log1p(t) small, y - t ~ (1 + t) e, so w e^2 with w = (1 + t)^2 / (t^2 + 1e-2)
+// is the relative squared error (y - t)^2 / (t^2 + 1e-2) to first order while
+// the gradient of a large error (a firefly) stays linear in e.
+__device__ __forceinline__ float loss_weight(float t) {
+    const float a = 1.0f + t;
+    return a * a / (t * t + 1e-2f);
+}
+
 // 1x1 head (FP32) + output; with a target also the loss, the head's
 // gradient partial sums (per block: 96 W + 3 b + loss) and the ReLU-masked
-// gradient of h3.
+// gradient of d0.
 __global__ __launch_bounds__(256) void k_head(int P, const uint16_t* h3, const float* head,
     const float4* beauty, const float4* target, float4* out, uint16_t* dh3, float* partial) {
     __shared__ float red[256];
     const int p = blockIdx.x * 256 + threadIdx.x;
     const bool valid = p < P;
-    float h[kNnHidden];
+    float h[kNnOut];
     float dout[3] = {0.0f, 0.0f, 0.0f};
     float loss = 0.0f;
     if (valid) {
-        const uint4* src = reinterpret_cast<const uint4*>(h3 + (size_t)p * kNnHidden);
+        const uint4* src = reinterpret_cast<const uint4*>(h3 + (size_t)p * kNnOut);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const uint4 u = src[v];
@@ -162,50 +245,53 @@ __global__ __launch_bounds__(256) void k_head(int P, const uint16_t* h3, const f
         float o[3];
         const float scale = 1.0f / (3.0f * (float)P);
         float tl[3] = {0.0f, 0.0f, 0.0f};
+        float wt[3] = {0.0f, 0.0f, 0.0f};
         if (target) {
             const float4 t4 = target[p];
-            tl[0] = log1pf(fmaxf(t4.x, 0.0f));
-            tl[1] = log1pf(fmaxf(t4.y, 0.0f));
-            tl[2] = log1pf(fmaxf(t4.z, 0.0f));
+            const float t[3] = {fmaxf(t4.x, 0.0f), fmaxf(t4.y, 0.0f), fmaxf(t4.z, 0.0f)};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                tl[c] = log1pf(t[c]);
+                wt[c] = loss_weight(t[c]);
+            }
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            float y = head[3 * kNnHidden + c];
+            float y = head[3 * kNnOut + c];
 #pragma unroll
-            for (int k = 0; k < kNnHidden; ++k) y = fmaf(head[c * kNnHidden + k], h[k], y);
+            for (int k = 0; k < kNnOut; ++k) y = fmaf(head[c * kNnOut + k], h[k], y);
             const float pred = base[c] + y;
             o[c] = fmaxf(expm1f(pred), 0.0f);
             if (target) {
                 const float e = pred - tl[c];
-                loss += e * e * scale;
-                dout[c] = 2.0f * e * scale;
+                loss += wt[c] * e * e * scale;
+                dout[c] = 2.0f * wt[c] * e * scale;
             }
         }
         if (out) out[p] = make_float4(o[0], o[1], o[2], 1.0f);
         if (target) {
-            uint32_t packed[kNnHidden / 2];
+            uint32_t packed[kNnOut / 2];
 #pragma unroll
-            for (int k = 0; k < kNnHidden; k += 2) {
-                float d0 = head[k] * dout[0] + head[kNnHidden + k] * dout[1] + head[2 * kNnHidden + k] * dout[2];
-                float d1 = head[k + 1] * dout[0] + head[kNnHidden + k + 1] * dout[1]
-                           + head[2 * kNnHidden + k + 1] * dout[2];
+            for (int k = 0; k < kNnOut; k += 2) {
+                float d0 = head[k] * dout[0] + head[kNnOut + k] * dout[1] + head[2 * kNnOut + k] * dout[2];
+                float d1 = head[k + 1] * dout[0] + head[kNnOut + k + 1] * dout[1]
+                           + head[2 * kNnOut + k + 1] * dout[2];
                 d0 = h[k] > 0.0f ? d0 : 0.0f;
                 d1 = h[k + 1] > 0.0f ? d1 : 0.0f;
                 packed[k / 2] = (uint32_t)to_bf16(d0) | ((uint32_t)to_bf16(d1) << 16);
             }
-            uint4* dst = reinterpret_cast<uint4*>(dh3 + (size_t)p * kNnHidden);
+            uint4* dst = reinterpret_cast<uint4*>(dh3 + (size_t)p * kNnOut);
 #pragma unroll
             for (int v = 0; v < 4; ++v)
                 dst[v] = make_uint4(packed[4 * v], packed[4 * v + 1], packed[4 * v + 2], packed[4 * v + 3]);
         }
     }
     if (!target) return;
-    // block sums of the head gradient: 96 W, 3 b, loss
     for (int i = 0; i < kNnHeadParams + 1; ++i) {
         float v = 0.0f;
         if (valid) {
-            if (i < 3 * kNnHidden) v = dout[i / kNnHidden] * h[i % kNnHidden];
-            else if (i < kNnHeadParams) v = dout[i - 3 * kNnHidden];
+            if (i < 3 * kNnOut) v = dout[i / kNnOut] * h[i % kNnOut];
+            else if (i < kNnHeadParams) v = dout[i - 3 * kNnOut];
             else v = loss;
         }
         red[threadIdx.x] = v;
@@ -219,59 +305,71 @@ __global__ __launch_bounds__(256) void k_head(int P, const uint16_t* h3, const f
     }
 }
 
-// Weight gradient of one conv layer as a GEMM with K = pixels:
-//   G[co][n] = sum_p dpre[p][co] * X[p + off(n / CIN)][n % CIN]   (n < 9 CIN)
-//   G[co][9 CIN] = sum_p dpre[p][co]                              (bias)
-// Block b covers pixels [b * kNnGradPixels, ...); wave w owns output tiles
-// w, w + 4, ...; writes G to partial[b].
-template <int CIN>
-__global__ __launch_bounds__(256) void k_wgrad(NnImages im, int P, int dil, const uint16_t* dpre,
-    const uint16_t* x, float* partial) {
-    constexpr int NT = (9 * CIN) / 16 + 1;  // + bias tile
-    constexpr int TILES = 2 * NT;
-    constexpr int MAXT = (TILES + 3) / 4;
-    constexpr int NW = NT * 16;
+struct WgradArgs {
+    NnImages im, pim;
+    int P, cout;
+    const uint16_t* dz;   // [P][cout]
+    const uint16_t* A;    // [P][C0]
+    const uint16_t* B;    // [P coarser][C1]
+    float* partial;       // [chunks][cout][NW * 16]
+};
+
+// Weight gradient of one convolution as a GEMM with K = pixels:
+//   G[co][n] = sum_p dz[p][co] * X[p + off(n / CIN)][n % CIN]   (n < 9 CIN)
+//   G[co][9 CIN] = sum_p dz[p][co]                              (bias)
+// Block (chunk, group) covers pixels [chunk * kNnGradPixels, ...) and the
+// 16 output tiles [16 group, 16 group + 16) of the cout/16 x NW tile grid;
+// wave w owns tiles 16 group + w + 4 i.
+template <int C0, int C1>
+__global__ __launch_bounds__(256) void k_uwgrad(WgradArgs a) {
+    constexpr int CIN = C0 + C1;
+    constexpr int NW = (9 * CIN) / 16 + 1;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int col = lane & 15;
     const int kq = lane >> 4;
-    f32x4 acc[MAXT];
+    const int tiles = (a.cout / 16) * NW;
+    f32x4 acc[4];
 #pragma unroll
-    for (int i = 0; i < MAXT; ++i) acc[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const int p0 = blockIdx.x * kNnGradPixels;
-    const int p1 = min(P, p0 + kNnGradPixels);
+    const int p1 = min(a.P, p0 + kNnGradPixels);
     for (int c0 = p0; c0 < p1; c0 += 32) {
-        // this lane's 8 pixels of the 32-pixel chunk
         int pix[8];
         Pix q[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             pix[j] = c0 + kq * 8 + j;
-            if (pix[j] < p1) q[j] = locate(im, pix[j]);
+            if (pix[j] < p1) q[j] = locate(a.im, pix[j]);
         }
 #pragma unroll
-        for (int i = 0; i < MAXT; ++i) {
-            const int tile = wave + 4 * i;
-            if (tile >= TILES) break;
-            const int mt = tile / NT, nt = tile % NT;
-            uint16_t av[8], bv[8];
+        for (int i = 0; i < 4; ++i) {
+            const int tile = blockIdx.y * 16 + wave + 4 * i;
+            if (tile >= tiles) break;
+            const int mt = tile / NW, nt = tile - mt * NW;
             const int co = mt * 16 + col;
             const int n = nt * 16 + col;
-            const int t = n / CIN, ci = n % CIN;
+            const int t = n / CIN, ci = n - t * CIN;
+            uint16_t av[8], bv[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 av[j] = 0;
                 bv[j] = 0;
                 if (pix[j] >= p1) continue;
-                av[j] = dpre[(size_t)pix[j] * kNnHidden + co];
+                av[j] = a.dz[(size_t)pix[j] * a.cout + co];
                 if (t < 9) {
-                    const int s = tap_pixel(q[j], t, dil);
-                    if (s >= 0) bv[j] = x[(size_t)s * CIN + ci];
+                    const int sx = q[j].x + t % 3 - 1, sy = q[j].y + t / 3 - 1;
+                    if (sx >= 0 && sx < q[j].w && sy >= 0 && sy < q[j].h) {
+                        if (C1 == 0 || ci < C0) {
+                            bv[j] = a.A[(size_t)(q[j].base + sy * q[j].w + sx) * C0 + ci];
+                        } else {
+                            bv[j] = a.B[(size_t)parent_index(a.pim, q[j].img, sx, sy) * C1 + (ci - C0)];
+                        }
+                    }
                 } else if (ci == 0) {
                     bv[j] = 0x3F80;  // 1.0: the bias column
                 }
             }
-            bf16x8 a, b;
             uint4 ra, rb;
             ra.x = (uint32_t)av[0] | ((uint32_t)av[1] << 16);
             ra.y = (uint32_t)av[2] | ((uint32_t)av[3] << 16);
@@ -281,43 +379,41 @@ __global__ __launch_bounds__(256) void k_wgrad(NnImages im, int P, int dil, cons
             rb.y = (uint32_t)bv[2] | ((uint32_t)bv[3] << 16);
             rb.z = (uint32_t)bv[4] | ((uint32_t)bv[5] << 16);
             rb.w = (uint32_t)bv[6] | ((uint32_t)bv[7] << 16);
-            a = __builtin_bit_cast(bf16x8, ra);
-            b = __builtin_bit_cast(bf16x8, rb);
-            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ra),
+                __builtin_bit_cast(bf16x8, rb), acc[i], 0, 0, 0);
         }
     }
-    float* dst = partial + (size_t)blockIdx.x * 32 * NW;
+    float* dst = a.partial + (size_t)blockIdx.x * a.cout * (NW * 16);
 #pragma unroll
-    for (int i = 0; i < MAXT; ++i) {
-        const int tile = wave + 4 * i;
-        if (tile >= TILES) break;
-        const int mt = tile / NT, nt = tile % NT;
+    for (int i = 0; i < 4; ++i) {
+        const int tile = blockIdx.y * 16 + wave + 4 * i;
+        if (tile >= tiles) break;
+        const int mt = tile / NW, nt = tile - mt * NW;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int co = mt * 16 + kq * 4 + e;  // D row
             const int n = nt * 16 + col;          // D column
-            dst[(size_t)co * NW + n] = acc[i][e];
+            dst[(size_t)co * (NW * 16) + n] = acc[i][e];
         }
     }
 }
 
-// sum the per-block partials of one conv layer into the gradient vector
-// (W[co][t][ci] then b[co]) in block order
-template <int CIN>
-__global__ void k_wgrad_reduce(int blocks, const float* partial, float* grad) {
-    constexpr int NW = ((9 * CIN) / 16 + 1) * 16;
+// sum the chunks' partials of one convolution into the gradient vector
+// (W[co][tap][ci] then b[co]) in chunk order
+__global__ void k_uwgrad_reduce(int chunks, int cout, int cin, const float* partial, float* grad) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= 32 * (9 * CIN + 1)) return;
+    const int nw16 = ((9 * cin) / 16 + 1) * 16;
+    if (i >= cout * (9 * cin + 1)) return;
     int co, n;
-    if (i < 32 * 9 * CIN) {
-        co = i / (9 * CIN);
-        n = i % (9 * CIN);
+    if (i < cout * 9 * cin) {
+        co = i / (9 * cin);
+        n = i - co * 9 * cin;
     } else {
-        co = i - 32 * 9 * CIN;
-        n = 9 * CIN;
+        co = i - cout * 9 * cin;
+        n = 9 * cin;
     }
     float s = 0.0f;
-    for (int b = 0; b < blocks; ++b) s += partial[(size_t)b * 32 * NW + (size_t)co * NW + n];
+    for (int b = 0; b < chunks; ++b) s += partial[((size_t)b * cout + co) * nw16 + n];
     grad[i] = s;
 }
 
@@ -342,77 +438,274 @@ __global__ void k_adam(int n, float* params, const float* grad, float* m, float*
     params[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// pack FP32 parameters into the kernels' layouts: forward bf16 [32][KP]
-// (k = tap * CIN + ci), input-gradient bf16 [CIN][KP'] (k' = tap' * 32 + co,
-// value W[co][8 - tap'][ci]), FP32 bias and head
-__global__ void k_pack(const float* params, NnPacked pk) {
+// pack convolution l: forward bf16 [cout][kp] (k = tap * cin + ci), input-
+// gradient bf16 [cin][kp'] (k' = tap' * cout + co, value W[co][8 - tap'][ci]),
+// FP32 bias
+__global__ void k_pack_conv(const float* W, int cin, int cout, int kpf, int kpb, uint16_t* fwd,
+    uint16_t* bwd, float* bias) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    for (int l = 0; l < kNnLayers; ++l) {
-        const int cin = l == 0 ? kNnFeat : kNnHidden;
-        const int kp = ((9 * cin + 31) / 32) * 32;
-        const float* W = params + nn_layer_offset(l);
-        if (i < 32 * kp) {
-            const int co = i / kp, k = i % kp;
-            const float v = k < 9 * cin ? W[(size_t)co * 9 * cin + k] : 0.0f;
-            pk.fwd[l][i] = to_bf16(v);
-        }
-        if (l > 0 && i < cin * 9 * 32) {
-            const int ci = i / (9 * 32), k = i % (9 * 32);
-            const int tp = k / 32, co = k % 32;
-            pk.bwd[l][i] = to_bf16(W[((size_t)co * 9 + (8 - tp)) * cin + ci]);
-        }
-        if (i < 32) pk.bias[l][i] = W[(size_t)32 * 9 * cin + i];
+    if (i < cout * kpf) {
+        const int co = i / kpf, k = i - co * kpf;
+        fwd[i] = k < 9 * cin ? to_bf16(W[(size_t)co * 9 * cin + k]) : (uint16_t)0;
     }
-    if (i < kNnHeadParams) pk.head[i] = params[nn_head_offset() + i];
+    if (bwd && i < cin * kpb) {
+        const int ci = i / kpb, k = i - ci * kpb;
+        uint16_t v = 0;
+        if (k < 9 * cout) {
+            const int tp = k / cout, co = k - tp * cout;
+            v = to_bf16(W[((size_t)co * 9 + (8 - tp)) * cin + ci]);
+        }
+        bwd[i] = v;
+    }
+    if (i < cout) bias[i] = W[(size_t)cout * 9 * cin + i];
+}
+
+__global__ void k_copy(int n, const float* src, float* dst) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
+inline int level_pixels(const NnImages& im) { return im.offset[im.count]; }
+
+// forward (MODE 0) or input-gradient (MODE 1) launch of convolution l
+template <int C0, int C1, int COUT, int MODE>
+void conv_launch(const ConvArgs& a, int grid, hipStream_t st) {
+    const int blocks = std::max(1, std::min(grid, cdiv(cdiv(a.P, 16), 4)));
+    k_uconv<C0, C1, COUT, MODE><<<blocks, 256, 0, st>>>(a);
+}
+
+void conv_forward(int l, const ConvArgs& a, int grid, hipStream_t st) {
+    switch (l) {
+        case 0: conv_launch<16, 0, 32, 0>(a, grid, st); break;
+        case 1: conv_launch<32, 0, 32, 0>(a, grid, st); break;
+        case 2: conv_launch<32, 0, 48, 0>(a, grid, st); break;
+        case 3: conv_launch<48, 0, 48, 0>(a, grid, st); break;
+        case 4: conv_launch<48, 0, 64, 0>(a, grid, st); break;
+        case 5: conv_launch<64, 0, 64, 0>(a, grid, st); break;
+        case 6: conv_launch<64, 0, 96, 0>(a, grid, st); break;
+        case 7: conv_launch<96, 0, 96, 0>(a, grid, st); break;
+        case 8: conv_launch<96, 0, 96, 0>(a, grid, st); break;
+        case 9: conv_launch<64, 96, 64, 0>(a, grid, st); break;
+        case 10: conv_launch<64, 0, 64, 0>(a, grid, st); break;
+        case 11: conv_launch<48, 64, 48, 0>(a, grid, st); break;
+        case 12: conv_launch<48, 0, 48, 0>(a, grid, st); break;
+        case 13: conv_launch<32, 48, 32, 0>(a, grid, st); break;
+        default: conv_launch<32, 0, 32, 0>(a, grid, st); break;
+    }
+}
+
+// input gradient of convolution l: the output gradient (cout channels) through
+// the transposed weights into cin raw channels
+void conv_input_grad(int l, const ConvArgs& a, int grid, hipStream_t st) {
+    switch (l) {
+        case 1: conv_launch<32, 0, 32, 1>(a, grid, st); break;
+        case 2: conv_launch<48, 0, 32, 1>(a, grid, st); break;
+        case 3: conv_launch<48, 0, 48, 1>(a, grid, st); break;
+        case 4: conv_launch<64, 0, 48, 1>(a, grid, st); break;
+        case 5: conv_launch<64, 0, 64, 1>(a, grid, st); break;
+        case 6: conv_launch<96, 0, 64, 1>(a, grid, st); break;
+        case 7: conv_launch<96, 0, 96, 1>(a, grid, st); break;
+        case 8: conv_launch<96, 0, 96, 1>(a, grid, st); break;
+        case 9: conv_launch<64, 0, 160, 1>(a, grid, st); break;
+        case 10: conv_launch<64, 0, 64, 1>(a, grid, st); break;
+        case 11: conv_launch<48, 0, 112, 1>(a, grid, st); break;
+        case 12: conv_launch<48, 0, 48, 1>(a, grid, st); break;
+        case 13: conv_launch<32, 0, 80, 1>(a, grid, st); break;
+        default: conv_launch<32, 0, 32, 1>(a, grid, st); break;
+    }
+}
+
+template <int C0, int C1>
+void wgrad_launch(const WgradArgs& a, hipStream_t st) {
+    constexpr int NW = (9 * (C0 + C1)) / 16 + 1;
+    const int tiles = (a.cout / 16) * NW;
+    const dim3 grid((unsigned)cdiv(a.P, kNnGradPixels), (unsigned)cdiv(tiles, 16));
+    k_uwgrad<C0, C1><<<grid, 256, 0, st>>>(a);
+}
+
+void conv_wgrad(int l, const WgradArgs& a, hipStream_t st) {
+    switch (l) {
+        case 0: wgrad_launch<16, 0>(a, st); break;
+        case 1: wgrad_launch<32, 0>(a, st); break;
+        case 2: wgrad_launch<32, 0>(a, st); break;
+        case 3: wgrad_launch<48, 0>(a, st); break;
+        case 4: wgrad_launch<48, 0>(a, st); break;
+        case 5: wgrad_launch<64, 0>(a, st); break;
+        case 6: wgrad_launch<64, 0>(a, st); break;
+        case 7: wgrad_launch<96, 0>(a, st); break;
+        case 8: wgrad_launch<96, 0>(a, st); break;
+        case 9: wgrad_launch<64, 96>(a, st); break;
+        case 10: wgrad_launch<64, 0>(a, st); break;
+        case 11: wgrad_launch<48, 64>(a, st); break;
+        case 12: wgrad_launch<48, 0>(a, st); break;
+        case 13: wgrad_launch<32, 48>(a, st); break;
+        default: wgrad_launch<32, 0>(a, st); break;
+    }
+}
+
+// the inputs of convolution l: A (this level) and B (one level coarser)
+void conv_inputs(int l, const NnScratch& w, const uint16_t** A, const uint16_t** B) {
+    *B = nullptr;
+    switch (l) {
+        case 0: *A = w.feat; break;
+        case 2: *A = w.pool[1]; break;
+        case 4: *A = w.pool[2]; break;
+        case 6: *A = w.pool[3]; break;
+        case 9: *A = w.act[5]; *B = w.act[8]; break;
+        case 11: *A = w.act[3]; *B = w.act[10]; break;
+        case 13: *A = w.act[1]; *B = w.act[12]; break;
+        default: *A = w.act[l - 1]; break;
+    }
+}
+
+ConvArgs conv_args(const NnLevels& lv, int l, const NnScratch& w) {
+    const NnConv c = nn_conv(l);
+    ConvArgs a{};
+    a.im = lv.lv[c.level];
+    a.pim = lv.lv[c.level < kNnLevels - 1 ? c.level + 1 : c.level];
+    a.P = level_pixels(a.im);
+    conv_inputs(l, w, &a.A, &a.B);
+    return a;
+}
+
 }  // namespace
 
+NnLevels nn_levels(const NnImages& im) {
+    NnLevels r{};
+    r.lv[0] = im;
+    for (int l = 1; l < kNnLevels; ++l) {
+        const NnImages& u = r.lv[l - 1];
+        NnImages& d = r.lv[l];
+        d.count = u.count;
+        int o = 0;
+        for (int i = 0; i < u.count; ++i) {
+            const int w = u.width[i], h = (u.offset[i + 1] - u.offset[i]) / u.width[i];
+            d.offset[i] = o;
+            d.width[i] = (w + 1) / 2;
+            o += ((w + 1) / 2) * ((h + 1) / 2);
+        }
+        d.offset[u.count] = o;
+    }
+    return r;
+}
+
+size_t nn_grad_partial_floats(const NnLevels& lv) {
+    size_t m = 0;
+    for (int l = 0; l < kNnConvs; ++l) {
+        const NnConv c = nn_conv(l);
+        const int chunks = cdiv(level_pixels(lv.lv[c.level]), kNnGradPixels);
+        const size_t nw16 = (size_t)((9 * nn_cin(l)) / 16 + 1) * 16;
+        m = std::max(m, (size_t)chunks * (size_t)c.cout * nw16);
+    }
+    return m;
+}
+
 hipError_t nn_pack(const float* params, const NnPacked& pk, hipStream_t st) {
-    k_pack<<<cdiv(32 * 288, 256), 256, 0, st>>>(params, pk);
+    for (int l = 0; l < kNnConvs; ++l) {
+        const NnConv c = nn_conv(l);
+        const int cin = nn_cin(l);
+        const int n = std::max(c.cout * nn_kp_fwd(l), cin * nn_kp_bwd(l));
+        k_pack_conv<<<cdiv(n, 256), 256, 0, st>>>(params + nn_layer_offset(l), cin, c.cout,
+            nn_kp_fwd(l), nn_kp_bwd(l), pk.fwd[l], l > 0 ? pk.bwd[l] : nullptr, pk.bias[l]);
+    }
+    k_copy<<<1, 256, 0, st>>>(kNnHeadParams, params + nn_head_offset(), pk.head);
     return hipGetLastError();
 }
 
-hipError_t nn_forward(const NnImages& im, int P, const NnPacked& pk, const NnScratch& w,
+hipError_t nn_forward(const NnLevels& lv, const NnPacked& pk, const NnScratch& w,
     const float4* beauty, const float4* normal, const float4* albedo, const float* depth,
     const float4* target, float4* out, int conv_grid, hipStream_t st) {
+    const int P = level_pixels(lv.lv[0]);
     if (P <= 0) return hipSuccess;
     k_features<<<cdiv(P, 256), 256, 0, st>>>(P, beauty, normal, albedo, depth, w.feat);
-    const int dil[kNnLayers] = {1, 2, 4, 8};
-    k_conv<kNnFeat, 0><<<conv_grid, 256, 0, st>>>(im, P, dil[0], w.feat, pk.fwd[0], pk.bias[0],
-        nullptr, w.act[0]);
-    for (int l = 1; l < kNnLayers; ++l) {
-        k_conv<kNnHidden, 0><<<conv_grid, 256, 0, st>>>(im, P, dil[l], w.act[l - 1], pk.fwd[l],
-            pk.bias[l], nullptr, w.act[l]);
+    for (int l = 0; l < kNnConvs; ++l) {
+        ConvArgs a = conv_args(lv, l, w);
+        a.W = pk.fwd[l];
+        a.kp = nn_kp_fwd(l);
+        a.bias = pk.bias[l];
+        a.out = w.act[l];
+        conv_forward(l, a, conv_grid, st);
+        if (l == 1 || l == 3 || l == 5) {
+            // pooled e0 / e1 / e2 feed the next level
+            const int lvl = nn_conv(l).level;
+            const int C = nn_conv(l).cout;
+            const int Pp = level_pixels(lv.lv[lvl + 1]);
+            k_pool<<<cdiv(Pp * C, 256), 256, 0, st>>>(lv.lv[lvl], lv.lv[lvl + 1], Pp, C, w.act[l],
+                w.pool[lvl + 1]);
+        }
     }
-    k_head<<<cdiv(P, 256), 256, 0, st>>>(P, w.act[kNnLayers - 1], pk.head, beauty, target, out,
-        w.dact[0], w.head_partial);
+    k_head<<<cdiv(P, 256), 256, 0, st>>>(P, w.act[kNnConvs - 1], pk.head, beauty, target, out,
+        w.dz[kNnConvs - 1], w.head_partial);
     return hipGetLastError();
 }
 
-hipError_t nn_backward(const NnImages& im, int P, const NnPacked& pk, const NnScratch& w,
-    float* grad, float* loss, int conv_grid, hipStream_t st) {
+hipError_t nn_backward(const NnLevels& lv, const NnPacked& pk, const NnScratch& w, float* grad,
+    float* loss, int conv_grid, hipStream_t st) {
+    const int P = level_pixels(lv.lv[0]);
     if (P <= 0) return hipSuccess;
-    const int dil[kNnLayers] = {1, 2, 4, 8};
-    const int hblocks = cdiv(P, 256);
-    k_head_reduce<<<1, 128, 0, st>>>(hblocks, w.head_partial, grad + nn_head_offset(), loss);
-    const int gblocks = cdiv(P, kNnGradPixels);
-    int cur = 0;  // dact[cur] holds the pre-activation gradient of layer l
-    for (int l = kNnLayers - 1; l >= 0; --l) {
-        const uint16_t* xin = l == 0 ? w.feat : w.act[l - 1];
-        if (l == 0) {
-            k_wgrad<kNnFeat><<<gblocks, 256, 0, st>>>(im, P, dil[l], w.dact[cur], xin, w.grad_partial);
-            k_wgrad_reduce<kNnFeat><<<cdiv(32 * (9 * kNnFeat + 1), 256), 256, 0, st>>>(gblocks,
-                w.grad_partial, grad + nn_layer_offset(l));
-        } else {
-            k_wgrad<kNnHidden><<<gblocks, 256, 0, st>>>(im, P, dil[l], w.dact[cur], xin, w.grad_partial);
-            k_wgrad_reduce<kNnHidden><<<cdiv(32 * (9 * kNnHidden + 1), 256), 256, 0, st>>>(gblocks,
-                w.grad_partial, grad + nn_layer_offset(l));
-            k_conv<kNnHidden, 1><<<conv_grid, 256, 0, st>>>(im, P, dil[l], w.dact[cur], pk.bwd[l],
-                nullptr, w.act[l - 1], w.dact[cur ^ 1]);
-            cur ^= 1;
+    k_head_reduce<<<1, 128, 0, st>>>(cdiv(P, 256), w.head_partial, grad + nn_head_offset(), loss);
+    for (int l = kNnConvs - 1; l >= 0; --l) {
+        const NnConv c = nn_conv(l);
+        const int cin = nn_cin(l);
+        const NnImages& im = lv.lv[c.level];
+        const int Pl = level_pixels(im);
+        // weight gradient
+        WgradArgs g{};
+        g.im = im;
+        g.pim = lv.lv[c.level < kNnLevels - 1 ? c.level + 1 : c.level];
+        g.P = Pl;
+        g.cout = c.cout;
+        g.dz = w.dz[l];
+        conv_inputs(l, w, &g.A, &g.B);
+        g.partial = w.grad_partial;
+        conv_wgrad(l, g, st);
+        k_uwgrad_reduce<<<cdiv(c.cout * (9 * cin + 1), 256), 256, 0, st>>>(
+            cdiv(Pl, kNnGradPixels), c.cout, cin, w.grad_partial, grad + nn_layer_offset(l));
+        if (l == 0) break;
+        // raw input gradient r[l] (cin channels at this level)
+        ConvArgs a{};
+        a.im = im;
+        a.pim = im;
+        a.P = Pl;
+        a.A = w.dz[l];
+        a.B = nullptr;
+        a.W = pk.bwd[l];
+        a.kp = nn_kp_bwd(l);
+        a.out = w.r[l];
+        conv_input_grad(l, a, conv_grid, st);
+        // route it to the producers of this convolution's input
+        switch (l) {
+            case 13:  // [e0, up(d1)]: d1's gradient from the upsampled part
+                k_up_combine<<<cdiv(level_pixels(lv.lv[1]) * 48, 256), 256, 0, st>>>(lv.lv[0], lv.lv[1],
+                    level_pixels(lv.lv[1]), 48, w.r[13], 80, 32, w.act[12], w.dz[12]);
+                break;
+            case 11:  // [e1, up(d2)]
+                k_up_combine<<<cdiv(level_pixels(lv.lv[2]) * 64, 256), 256, 0, st>>>(lv.lv[1], lv.lv[2],
+                    level_pixels(lv.lv[2]), 64, w.r[11], 112, 48, w.act[10], w.dz[10]);
+                break;
+            case 9:  // [e2, up(b)]
+                k_up_combine<<<cdiv(level_pixels(lv.lv[3]) * 96, 256), 256, 0, st>>>(lv.lv[2], lv.lv[3],
+                    level_pixels(lv.lv[3]), 96, w.r[9], 160, 64, w.act[8], w.dz[8]);
+                break;
+            case 6:  // pool(e2) -> e2, plus the skip into convolution 9
+                k_skip_combine<<<cdiv(level_pixels(lv.lv[2]) * 64, 256), 256, 0, st>>>(lv.lv[2], lv.lv[3],
+                    level_pixels(lv.lv[2]), 64, w.r[9], 160, w.r[6], w.act[5], w.dz[5]);
+                break;
+            case 4:  // pool(e1) -> e1, plus the skip into convolution 11
+                k_skip_combine<<<cdiv(level_pixels(lv.lv[1]) * 48, 256), 256, 0, st>>>(lv.lv[1], lv.lv[2],
+                    level_pixels(lv.lv[1]), 48, w.r[11], 112, w.r[4], w.act[3], w.dz[3]);
+                break;
+            case 2:  // pool(e0) -> e0, plus the skip into convolution 13
+                k_skip_combine<<<cdiv(level_pixels(lv.lv[0]) * 32, 256), 256, 0, st>>>(lv.lv[0], lv.lv[1],
+                    level_pixels(lv.lv[0]), 32, w.r[13], 80, w.r[2], w.act[1], w.dz[1]);
+                break;
+            default: {  // single input from convolution l - 1 at this level
+                const size_t n = (size_t)Pl * cin;
+                k_mask<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, w.r[l], w.act[l - 1], w.dz[l - 1]);
+                break;
+            }
         }
     }
     return hipGetLastError();

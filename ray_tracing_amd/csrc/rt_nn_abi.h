@@ -18,5 +18,5 @@ int rt_nn_device(const rt_nn* nn);
 // reference counting: a renderer keeps the network it uses alive
 void rt_nn_retain(rt_nn* nn);
 void rt_nn_release(rt_nn* nn);
-// allocates the network's scratch for P pixels (before a graph capture)
-hipError_t rt_nn_reserve(rt_nn* nn, int P);
+// allocates the network's inference scratch for these images (before a graph capture)
+hipError_t rt_nn_reserve(rt_nn* nn, const rtk::NnImages& images);

@@ -67,17 +67,18 @@ struct rt_nn {
     int cu_count = 0;
     int step = 0;
     Buf<float> params, grad, m, v, loss;
-    Buf<uint16_t> fwd[rtk::kNnLayers], bwd[rtk::kNnLayers];
-    Buf<float> bias[rtk::kNnLayers];
+    Buf<uint16_t> fwd[rtk::kNnConvs], bwd[rtk::kNnConvs];
+    Buf<float> bias[rtk::kNnConvs];
     Buf<float> head;
-    // per-batch scratch
-    size_t cap = 0;
-    Buf<uint16_t> feat, act[rtk::kNnLayers], dact[2];
+    // per-batch scratch, sized for the largest batch seen at each level
+    size_t lv_cap[rtk::kNnLevels] = {0, 0, 0, 0};
+    size_t lv_cap_train[rtk::kNnLevels] = {0, 0, 0, 0};
+    Buf<uint16_t> feat, act[rtk::kNnConvs], pool[rtk::kNnLevels], dz[rtk::kNnConvs], r[rtk::kNnConvs];
     Buf<float> head_partial, grad_partial;
 
     rtk::NnPacked packed() {
         rtk::NnPacked pk{};
-        for (int l = 0; l < rtk::kNnLayers; ++l) {
+        for (int l = 0; l < rtk::kNnConvs; ++l) {
             pk.fwd[l] = fwd[l].p;
             pk.bwd[l] = bwd[l].p;
             pk.bias[l] = bias[l].p;
@@ -86,31 +87,56 @@ struct rt_nn {
         return pk;
     }
 
-    hipError_t ensure(size_t P) {
-        if (P <= cap) return hipSuccess;
-        hipError_t e = feat.reserve(P * rtk::kNnFeat);
-        for (int l = 0; l < rtk::kNnLayers && e == hipSuccess; ++l) e = act[l].reserve(P * rtk::kNnHidden);
-        if (e == hipSuccess) e = dact[0].reserve(P * rtk::kNnHidden);
-        if (e == hipSuccess) e = dact[1].reserve(P * rtk::kNnHidden);
-        if (e == hipSuccess) e = head_partial.reserve(((P + 255) / 256) * (rtk::kNnHeadParams + 1));
-        if (e == hipSuccess)
-            e = grad_partial.reserve(((P + rtk::kNnGradPixels - 1) / rtk::kNnGradPixels) * 32 * 19 * 16);
-        if (e == hipSuccess) cap = P;
+    // activations for inference; with `train` also the gradient buffers
+    hipError_t ensure(const rtk::NnLevels& lv, bool train) {
+        size_t P[rtk::kNnLevels];
+        bool grow = false, grow_train = false;
+        for (int l = 0; l < rtk::kNnLevels; ++l) {
+            P[l] = (size_t)lv.lv[l].offset[lv.lv[l].count];
+            grow = grow || P[l] > lv_cap[l];
+            grow_train = grow_train || (train && P[l] > lv_cap_train[l]);
+        }
+        hipError_t e = hipSuccess;
+        if (grow) {
+            e = feat.reserve(P[0] * rtk::kNnFeat);
+            for (int l = 0; l < rtk::kNnConvs && e == hipSuccess; ++l) {
+                const rtk::NnConv c = rtk::nn_conv(l);
+                e = act[l].reserve(P[c.level] * (size_t)c.cout);
+            }
+            for (int l = 1; l < rtk::kNnLevels && e == hipSuccess; ++l)
+                e = pool[l].reserve(P[l] * (size_t)rtk::nn_conv(2 * l - 1).cout);
+            if (e == hipSuccess) e = head_partial.reserve(((P[0] + 255) / 256) * (rtk::kNnHeadParams + 1));
+            if (e == hipSuccess)
+                for (int l = 0; l < rtk::kNnLevels; ++l) lv_cap[l] = std::max(lv_cap[l], P[l]);
+        }
+        if (e == hipSuccess && grow_train) {
+            for (int l = 0; l < rtk::kNnConvs && e == hipSuccess; ++l) {
+                const rtk::NnConv c = rtk::nn_conv(l);
+                e = dz[l].reserve(P[c.level] * (size_t)c.cout);
+                if (e == hipSuccess && l > 0) e = r[l].reserve(P[c.level] * (size_t)rtk::nn_cin(l));
+            }
+            if (e == hipSuccess) e = grad_partial.reserve(rtk::nn_grad_partial_floats(lv));
+            if (e == hipSuccess)
+                for (int l = 0; l < rtk::kNnLevels; ++l) lv_cap_train[l] = std::max(lv_cap_train[l], P[l]);
+        }
         return e;
     }
 
     rtk::NnScratch scratch() {
         rtk::NnScratch w{};
         w.feat = feat.p;
-        for (int l = 0; l < rtk::kNnLayers; ++l) w.act[l] = act[l].p;
-        w.dact[0] = dact[0].p;
-        w.dact[1] = dact[1].p;
+        for (int l = 0; l < rtk::kNnConvs; ++l) {
+            w.act[l] = act[l].p;
+            w.dz[l] = dz[l].p;
+            w.r[l] = r[l].p;
+        }
+        for (int l = 0; l < rtk::kNnLevels; ++l) w.pool[l] = pool[l].p;
         w.head_partial = head_partial.p;
         w.grad_partial = grad_partial.p;
         return w;
     }
 
-    int conv_grid() const { return std::max(cu_count, 1) * 8; }
+    int conv_grid() const { return std::max(cu_count, 1) * 16; }
 };
 
 namespace {
@@ -142,15 +168,19 @@ rtk::NnImages to_images(const rt_nn_images* im) {
 hipError_t rt_nn_infer_on_stream(rt_nn* nn, const rtk::NnImages& images, int P,
     const float4* beauty, const float4* normal, const float4* albedo, const float* depth,
     float4* out, hipStream_t st) {
-    const hipError_t e = nn->ensure((size_t)P);
+    (void)P;
+    const rtk::NnLevels lv = rtk::nn_levels(images);
+    const hipError_t e = nn->ensure(lv, false);
     if (e != hipSuccess) return e;
-    return rtk::nn_forward(images, P, nn->packed(), nn->scratch(), beauty, normal, albedo, depth,
-        nullptr, out, nn->conv_grid(), st);
+    return rtk::nn_forward(lv, nn->packed(), nn->scratch(), beauty, normal, albedo, depth, nullptr,
+        out, nn->conv_grid(), st);
 }
 
 int rt_nn_device(const rt_nn* nn) { return nn->ordinal; }
 
-hipError_t rt_nn_reserve(rt_nn* nn, int P) { return nn->ensure((size_t)P); }
+hipError_t rt_nn_reserve(rt_nn* nn, const rtk::NnImages& images) {
+    return nn->ensure(rtk::nn_levels(images), false);
+}
 
 void rt_nn_retain(rt_nn* nn) { nn->refs.fetch_add(1); }
 
@@ -186,11 +216,11 @@ rt_status rt_nn_create(int ordinal, rt_nn** out) {
         if (e == hipSuccess) e = b->reserve(np);
     }
     if (e == hipSuccess) e = nn->loss.reserve(1);
-    for (int l = 0; l < rtk::kNnLayers && e == hipSuccess; ++l) {
-        const int cin = l == 0 ? rtk::kNnFeat : rtk::kNnHidden;
-        e = nn->fwd[l].reserve((size_t)32 * (((9 * cin + 31) / 32) * 32));
-        if (e == hipSuccess) e = nn->bwd[l].reserve((size_t)cin * 9 * 32);
-        if (e == hipSuccess) e = nn->bias[l].reserve(32);
+    for (int l = 0; l < rtk::kNnConvs && e == hipSuccess; ++l) {
+        const rtk::NnConv c = rtk::nn_conv(l);
+        e = nn->fwd[l].reserve((size_t)c.cout * rtk::nn_kp_fwd(l));
+        if (e == hipSuccess) e = nn->bwd[l].reserve((size_t)rtk::nn_cin(l) * rtk::nn_kp_bwd(l));
+        if (e == hipSuccess) e = nn->bias[l].reserve((size_t)c.cout);
     }
     if (e == hipSuccess) e = nn->head.reserve(rtk::kNnHeadParams);
     if (e == hipSuccess) e = hipMemsetAsync(nn->params.p, 0, np * sizeof(float), nn->stream);
@@ -248,11 +278,11 @@ rt_status rt_nn_infer(rt_nn* nn, const rt_nn_images* images, const float* beauty
     if (s != RT_OK) return s;
     NN_REQUIRE(beauty && normal && albedo && depth && out, RT_ERROR_INVALID_ARGUMENT, "null frame buffer");
     NN_HIP(hipSetDevice(nn->ordinal));
-    NN_HIP(nn->ensure((size_t)P));
-    NN_HIP(rtk::nn_forward(to_images(images), P, nn->packed(), nn->scratch(),
-        reinterpret_cast<const float4*>(beauty), reinterpret_cast<const float4*>(normal),
-        reinterpret_cast<const float4*>(albedo), depth, nullptr, reinterpret_cast<float4*>(out),
-        nn->conv_grid(), nn->stream));
+    const rtk::NnLevels lv = rtk::nn_levels(to_images(images));
+    NN_HIP(nn->ensure(lv, false));
+    NN_HIP(rtk::nn_forward(lv, nn->packed(), nn->scratch(), reinterpret_cast<const float4*>(beauty),
+        reinterpret_cast<const float4*>(normal), reinterpret_cast<const float4*>(albedo), depth,
+        nullptr, reinterpret_cast<float4*>(out), nn->conv_grid(), nn->stream));
     NN_HIP(hipStreamSynchronize(nn->stream));
     return RT_OK;
 }
@@ -268,14 +298,14 @@ rt_status rt_nn_train_step(rt_nn* nn, const rt_nn_images* images, const float* b
         "null frame or target buffer");
     NN_REQUIRE(lr >= 0.0f, RT_ERROR_INVALID_ARGUMENT, "negative learning rate");
     NN_HIP(hipSetDevice(nn->ordinal));
-    NN_HIP(nn->ensure((size_t)P));
-    const rtk::NnImages im = to_images(images);
+    const rtk::NnLevels lv = rtk::nn_levels(to_images(images));
+    NN_HIP(nn->ensure(lv, true));
     const rtk::NnPacked pk = nn->packed();
     const rtk::NnScratch w = nn->scratch();
-    NN_HIP(rtk::nn_forward(im, P, pk, w, reinterpret_cast<const float4*>(beauty),
+    NN_HIP(rtk::nn_forward(lv, pk, w, reinterpret_cast<const float4*>(beauty),
         reinterpret_cast<const float4*>(normal), reinterpret_cast<const float4*>(albedo), depth,
         reinterpret_cast<const float4*>(target), nullptr, nn->conv_grid(), nn->stream));
-    NN_HIP(rtk::nn_backward(im, P, pk, w, nn->grad.p, nn->loss.p, nn->conv_grid(), nn->stream));
+    NN_HIP(rtk::nn_backward(lv, pk, w, nn->grad.p, nn->loss.p, nn->conv_grid(), nn->stream));
     const int np = rtk::nn_param_count();
     if (grad) NN_HIP(hipMemcpyAsync(grad, nn->grad.p, sizeof(float) * (size_t)np, hipMemcpyDeviceToHost, nn->stream));
     if (loss) NN_HIP(hipMemcpyAsync(loss, nn->loss.p, sizeof(float), hipMemcpyDeviceToHost, nn->stream));

@@ -14,6 +14,8 @@
 //                             2 pinhole32 640x480 + 2 equi62_lut1d 1024x1024
 //   3 radiance_cache_realtime : the config-2 scene with 2 pinhole32 + 2
 //                             equi62_lut1d cameras at 1280x720, depth 8
+//   4 cornell_random        : seeded Cornell rooms with 1-8 spheres, random
+//                             materials and light (denoiser training scenes)
 // Draws come from PCG32 (O'Neill 2014; pcg32_srandom_r(seed, 54), XSH-RR
 // output); a float draw is (x >> 8) * 2^-24; the draw order is part of the
 // scene definition.
@@ -279,6 +281,88 @@ void generate_cornell_mesh(uint64_t seed, rt_generated_scene* g) {
         make_lookat_pinhole(&g->cameras[c], 640, 480, 40.0,
             {center[0] + 800.0 * std::sin(th), center[1], center[2] - 800.0 * std::cos(th)},
             {center[0], center[1], center[2]}, {0.0, 1.0, 0.0});
+    }
+    g->profile = default_profile(16, 8);
+}
+
+// config 4: the Cornell room of config 1 with 1-8 displaced icospheres (4
+// subdivisions, 5,120 triangles each), one random constant OpenPBR material
+// per sphere (drawn as in config 1) and a random ceiling light.  Draw order:
+// light width, depth, x, z, radiance; sphere count; per sphere radius,
+// centre x, y, z, base colour, roughness, metalness, coat, then one
+// displacement per vertex.
+void generate_cornell_random(uint64_t seed, rt_generated_scene* g) {
+    Pcg32 r(seed);
+    std::vector<rt_material> mt = {lambert(0.65f, 0.05f, 0.05f), lambert(0.73f, 0.73f, 0.73f),
+        lambert(0.12f, 0.45f, 0.15f)};
+    const float lw = 80.0f + 80.0f * r.uniform();
+    const float ld = 70.0f + 70.0f * r.uniform();
+    const float lx = 20.0f + (515.0f - lw) * r.uniform();
+    const float lz = 20.0f + (515.0f - ld) * r.uniform();
+    const float radiance = 8.0f + 12.0f * r.uniform();
+    rt_material light{};
+    light.type = RT_MAT_DIFFUSE_LIGHT;
+    set3(light.emission, radiance, radiance, radiance);
+    light.ior = 1.0f;
+    light.openpbr_index = -1;
+    mt.push_back(light);
+    const std::vector<rt_quad> q = {quad(555, 0, 0, 0, 555, 0, 0, 0, 555, 2),
+        quad(0, 0, 0, 0, 555, 0, 0, 0, 555, 0), quad(lx + lw, 554, lz + ld, -lw, 0, 0, 0, 0, -ld, 3),
+        quad(0, 0, 0, 555, 0, 0, 0, 0, 555, 1), quad(555, 555, 555, -555, 0, 0, 0, 0, -555, 1),
+        quad(0, 0, 555, 555, 0, 0, 0, 555, 0, 1)};
+    const int spheres = 1 + std::min(7, static_cast<int>(8.0f * r.uniform()));
+    const Icosphere ico = make_icosphere(4);
+    std::vector<rt_openpbr_material> ops;
+    std::vector<rt_triangle> tr;
+    for (int m = 0; m < spheres; ++m) {
+        const double radius = 40.0 + 70.0 * static_cast<double>(r.uniform());
+        double center[3];
+        for (int k = 0; k < 3; ++k) center[k] = radius + (555.0 - 2.0 * radius) * static_cast<double>(r.uniform());
+        rt_openpbr_material op = openpbr_defaults();
+        for (int ch = 0; ch < 3; ++ch) op.base_color[ch] = r.uniform();
+        op.specular_roughness = 0.05f + 0.85f * r.uniform();
+        op.base_metalness = r.uniform() < 0.5f ? 0.0f : 1.0f;
+        op.specular_ior = 1.5f;
+        op.coat_weight = 0.5f * r.uniform();
+        rt_material pbr{};
+        pbr.type = RT_MAT_OPENPBR;
+        pbr.openpbr_index = m;
+        pbr.ior = 1.0f;
+        for (int ch = 0; ch < 3; ++ch) pbr.albedo[ch] = op.base_color[ch];
+        mt.push_back(pbr);
+        ops.push_back(op);
+        std::vector<std::array<float, 3>> pos(ico.verts.size());
+        for (size_t i = 0; i < ico.verts.size(); ++i) {
+            const float u = r.uniform();
+            const double rr = radius * (1.0 + 0.05 * (2.0 * static_cast<double>(u) - 1.0));
+            for (int k = 0; k < 3; ++k) pos[i][k] = static_cast<float>(center[k] + ico.verts[i][k] * rr);
+        }
+        for (size_t f = 0; f < ico.faces.size(); ++f) {
+            rt_triangle t{};
+            std::memcpy(t.p0, pos[ico.faces[f][0]].data(), 12);
+            std::memcpy(t.p1, pos[ico.faces[f][1]].data(), 12);
+            std::memcpy(t.p2, pos[ico.faces[f][2]].data(), 12);
+            t.material = 4 + m;
+            tr.push_back(t);
+        }
+    }
+    g->scene.quad_count = static_cast<int32_t>(q.size());
+    g->scene.quads = copy_out(q);
+    g->scene.triangle_count = static_cast<int32_t>(tr.size());
+    g->scene.triangles = copy_out(tr);
+    g->scene.material_count = static_cast<int32_t>(mt.size());
+    g->scene.materials = copy_out(mt);
+    g->scene.openpbr_count = static_cast<int32_t>(ops.size());
+    g->scene.openpbr = copy_out(ops);
+    g->scene.background_mode = RT_BG_CONSTANT;
+    set3(g->scene.background, 0.0f, 0.0f, 0.0f);
+    g->camera_count = 4;
+    const double c0[3] = {277.5, 277.5, 277.5};
+    for (int c = 0; c < 4; ++c) {
+        const double th = static_cast<double>(c) * 0.5 * 3.14159265358979323846;
+        make_lookat_pinhole(&g->cameras[c], 640, 480, 40.0,
+            {c0[0] + 800.0 * std::sin(th), c0[1], c0[2] - 800.0 * std::cos(th)}, {c0[0], c0[1], c0[2]},
+            {0.0, 1.0, 0.0});
     }
     g->profile = default_profile(16, 8);
 }
@@ -589,6 +673,7 @@ bool generate_scene(int config, uint64_t seed, rt_generated_scene* g) {
             g->profile.max_bounces = 8;
             g->profile.rr_start_bounce = 3;
             return true;
+        case 4: generate_cornell_random(seed, g); return true;
         default: return false;
     }
 }

@@ -1,18 +1,28 @@
-"""Offline training and evaluation of the learned denoiser.
+"""Offline training and evaluation of the learned denoiser (the 4-level
+encoder-decoder of rt_nn.h, 565,443 parameters).
 
 Training pairs come from this path tracer on seeded procedural scenes:
+- the Cornell rooms with 1-8 random spheres, materials and light of the
+  scene generator's config 4 (`cornell_random`, seeds 0..N-1), plus a few
+  seeds of the instanced ReSTIR scene and the random-sphere scene the same
+  network denoises in the realtime loop;
 - inputs are 1-spp frames with their albedo / normal / depth buffers, taken
   the way the realtime loop produces them (ReSTIR DI on the instanced scene
   after a few frames of its camera orbit);
 - targets are 1024-spp renders of the same camera poses by the plain tracer.
 
 Training runs on the GPU through the library's own BF16 MFMA forward /
-backward / Adam kernels (rt_nn_train_step) on random crops.  Evaluation
-scores relMSE = mean over pixels and RGB of (x - r)^2 / (r^2 + 1e-2) against
-a 4096-spp plain render of a held-out seed for: the raw 1-spp frame, the
+backward / Adam kernels (rt_nn_train_step): batches of 8 random 256x256
+crops, Adam with lr 1e-3 on a cosine schedule, minimising the log-space
+squared error weighted by (1 + t)^2 / (t^2 + 1e-2) (t the target), which is
+the relMSE below to first order.  Evaluation scores relMSE =
+mean over pixels and RGB of (x - r)^2 / (r^2 + 1e-2) against 4096-spp plain
+renders of held-out seeds (1000, 1001, ...) for: the raw 1-spp frame, the
 a-trous filter, and the network.
 
     python -m ray_tracing_amd.denoiser_train --out ray_tracing_amd/weights/denoiser.npy
+    python -m ray_tracing_amd.denoiser_train --weights ray_tracing_amd/weights/denoiser.npy \
+        --eval-seeds 50 --report eval.json        # evaluation only
 """
 from __future__ import annotations
 
@@ -30,7 +40,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 import ray_tracing_amd as rt  # noqa: E402
 
 EPS_REL = 1e-2
-CONFIGS = {"random_spheres": 0, "instanced_mesh_restir": 2}
+CONFIGS = {"random_spheres": 0, "instanced_mesh_restir": 2, "cornell_random": 4}
 
 
 def rel_mse(x: np.ndarray, ref: np.ndarray) -> float:
@@ -107,16 +117,24 @@ def crop_batch(rng, pairs, batch: int, size: int):
     return sizes, cat, tgt
 
 
+# (input channels from this level, from the coarser level, outputs) of the
+# encoder-decoder's 15 convolutions, in parameter order
+CONVS = [(16, 0, 32), (32, 0, 32), (32, 0, 48), (48, 0, 48), (48, 0, 64), (64, 0, 64),
+         (64, 0, 96), (96, 0, 96), (96, 0, 96), (64, 96, 64), (64, 0, 64), (48, 64, 48),
+         (48, 0, 48), (32, 48, 32), (32, 0, 32)]
+
+
 def init_params(rng) -> np.ndarray:
     """He-style init for the convolutions, a zero head (the untrained network
     returns its input)."""
     n = rt.lib().rt_nn_param_count()
     p = np.zeros(n, np.float32)
     o = 0
-    for cin in (16, 32, 32, 32):
-        nw = 32 * 9 * cin
-        p[o:o + nw] = rng.normal(0, np.sqrt(2.0 / (9 * cin)), nw)
-        o += nw + 32
+    for c0, c1, cout in CONVS:
+        fan = 9 * (c0 + c1)
+        nw = cout * fan
+        p[o:o + nw] = rng.normal(0, np.sqrt(2.0 / fan), nw)
+        o += nw + cout
     return p
 
 
@@ -140,6 +158,7 @@ def evaluate(dev, nn, config: str, seed: int, ref_spp: int, log=print) -> dict:
     restir_scene = config == "instanced_mesh_restir"
     frame = 4 if restir_scene else 0
     warm = 4 if restir_scene else 0
+    t0 = time.time()
     ref, _ = render(dev, config, seed, ref_spp, 3, frame=frame, restir=False)
     raw, tm_raw = render(dev, config, seed, 1, 4, frame=frame, warm_frames=warm, denoise=True)
     res = {"config": config, "seed": seed, "ref_spp": ref_spp, "cameras": []}
@@ -153,7 +172,7 @@ def evaluate(dev, nn, config: str, seed: int, ref_spp: int, log=print) -> dict:
     for k in ("relmse_raw", "relmse_atrous", "relmse_learned"):
         res[k] = float(np.mean([c[k] for c in res["cameras"]]))
     log(f"[eval] {config} seed {seed}: raw {res['relmse_raw']:.4f} a-trous "
-        f"{res['relmse_atrous']:.4f} learned {res['relmse_learned']:.4f}")
+        f"{res['relmse_atrous']:.4f} learned {res['relmse_learned']:.4f} ({time.time() - t0:.1f} s)")
     return res
 
 
@@ -161,32 +180,50 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "weights", "denoiser.npy"))
     ap.add_argument("--report", default="")
-    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--crop", type=int, default=128)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--crop", type=int, default=256)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--target-spp", type=int, default=1024)
     ap.add_argument("--ref-spp", type=int, default=4096)
-    ap.add_argument("--spheres-seeds", type=int, default=10)
+    ap.add_argument("--cornell-seeds", type=int, default=64)
     ap.add_argument("--instanced-seeds", type=int, default=4)
+    ap.add_argument("--spheres-seeds", type=int, default=4)
+    ap.add_argument("--eval-seeds", type=int, default=50, help="held-out cornell_random seeds 1000..")
+    ap.add_argument("--weights", default="", help="evaluate these weights instead of training")
     a = ap.parse_args()
     dev = rt.Device(0)
-    pairs = make_pairs(dev, "random_spheres", range(a.spheres_seeds), a.target_spp)
-    pairs += make_pairs(dev, "instanced_mesh_restir", range(a.instanced_seeds), a.target_spp)
-    nn, hist = train(pairs, a.steps, a.batch, a.crop, a.lr)
-    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
-    np.save(a.out, nn.params())
-    print(f"[train] wrote {a.out}", flush=True)
-    report = {"train": {"steps": a.steps, "batch": a.batch, "crop": a.crop, "lr": a.lr,
-                        "target_spp": a.target_spp, "pairs": len(pairs),
-                        "loss_first": float(np.mean(hist[:50])),
-                        "loss_last": float(np.mean(hist[-50:]))},
-              "eval": [evaluate(dev, nn, "random_spheres", 1000, a.ref_spp),
-                       evaluate(dev, nn, "instanced_mesh_restir", 1000, a.ref_spp)]}
+    report = {}
+    if a.weights:
+        nn = rt.Denoiser(0, np.load(a.weights))
+        report["weights"] = os.path.basename(a.weights)
+    else:
+        pairs = make_pairs(dev, "cornell_random", range(a.cornell_seeds), a.target_spp)
+        pairs += make_pairs(dev, "instanced_mesh_restir", range(a.instanced_seeds), a.target_spp)
+        pairs += make_pairs(dev, "random_spheres", range(a.spheres_seeds), a.target_spp)
+        nn, hist = train(pairs, a.steps, a.batch, a.crop, a.lr)
+        os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+        np.save(a.out, nn.params())
+        print(f"[train] wrote {a.out}", flush=True)
+        report["train"] = {"steps": a.steps, "batch": a.batch, "crop": a.crop, "lr": a.lr,
+                           "target_spp": a.target_spp, "pairs": len(pairs),
+                           "cornell_seeds": a.cornell_seeds, "instanced_seeds": a.instanced_seeds,
+                           "spheres_seeds": a.spheres_seeds,
+                           "loss_first": float(np.mean(hist[:50])),
+                           "loss_last": float(np.mean(hist[-50:]))}
+    evals = [evaluate(dev, nn, "cornell_random", 1000 + i, a.ref_spp) for i in range(a.eval_seeds)]
+    summary = {k: float(np.mean([e[k] for e in evals]))
+               for k in ("relmse_raw", "relmse_atrous", "relmse_learned")}
+    print(f"[eval] cornell_random seeds 1000..{999 + a.eval_seeds}: " +
+          " ".join(f"{k} {v:.4f}" for k, v in summary.items()), flush=True)
+    report["eval_cornell_random"] = {"seeds": [1000 + i for i in range(a.eval_seeds)],
+                                     "ref_spp": a.ref_spp, **summary}
+    report["eval"] = evals + [evaluate(dev, nn, "random_spheres", 1000, a.ref_spp),
+                              evaluate(dev, nn, "instanced_mesh_restir", 1000, a.ref_spp)]
     if a.report:
         with open(a.report, "w") as f:
             json.dump(report, f, indent=1)
-    print(json.dumps(report["eval"][1] | {"cameras": None}), flush=True)
+    print(json.dumps(report["eval_cornell_random"]), flush=True)
 
 
 if __name__ == "__main__":

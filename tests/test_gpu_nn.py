@@ -1,12 +1,14 @@
-"""GPU parity of the learned denoiser's BF16 MFMA kernels against the CPU
-oracle: forward output, loss and the full parameter gradient (backward data
-and weight passes), plus a short training run that lowers the loss."""
+"""GPU parity of the learned denoiser's BF16 MFMA kernels (4-level
+encoder-decoder) against the CPU oracle: forward output, loss and the full
+parameter gradient (backward data and weight passes through the pooling,
+upsampling and skip routes), plus a short training run that lowers the
+loss."""
 import sys
 
 import numpy as np
 import pytest
 
-from test_oracle_nn import random_batch, random_params
+from test_oracle_nn import random_batch, scaled_params
 
 pytestmark = pytest.mark.gpu
 
@@ -16,14 +18,14 @@ def say(msg):
     sys.stdout.flush()
 
 
-SIZES = [(40, 24), (17, 33), (64, 16)]
+SIZES = [(40, 24), (17, 33), (64, 16)]  # odd sizes: partial pools
 
 
 def test_mfma_forward_matches_oracle(rt, oracle, device):
     say("[gpu] nn forward vs oracle")
     rng = np.random.default_rng(11)
     im, frame, target = random_batch(rng, SIZES)
-    params = random_params(rng)
+    params = scaled_params(rng)
     nn = rt.Denoiser(0, params)
     out = nn.infer(SIZES, frame)
     _, _, ref = oracle.nn_loss_grad(oracle.images(SIZES), params, frame, None, bf16=True,
@@ -39,7 +41,7 @@ def test_mfma_gradient_matches_oracle(rt, oracle, device):
     say("[gpu] nn gradient vs oracle")
     rng = np.random.default_rng(12)
     im, frame, target = random_batch(rng, SIZES)
-    params = random_params(rng)
+    params = scaled_params(rng)
     nn = rt.Denoiser(0, params)
     loss, grad = nn.train_step(SIZES, frame, target, 0.0, want_grad=True)
     rloss, rgrad, _ = oracle.nn_loss_grad(oracle.images(SIZES), params, frame, target, bf16=True)
@@ -50,10 +52,8 @@ def test_mfma_gradient_matches_oracle(rt, oracle, device):
     say(f"  gradient relative L2 error {err:.2e}")
     assert err < 2e-2
     # every layer's block of the gradient agrees, not just the large ones
-    offs = [0]
-    for cin in (16, 32, 32, 32):
-        offs.append(offs[-1] + 32 * 9 * cin + 32)
-    offs.append(offs[-1] + 99)
+    from test_oracle_nn import layer_ranges
+    offs = [a for a, _ in layer_ranges()] + [layer_ranges()[-1][1], len(params)]
     for i in range(len(offs) - 1):
         g, r = grad[offs[i]:offs[i + 1]], rgrad[offs[i]:offs[i + 1]]
         e = np.linalg.norm(g - r) / max(np.linalg.norm(r), 1e-12)
@@ -66,11 +66,14 @@ def test_mfma_gradient_matches_oracle(rt, oracle, device):
 def test_training_lowers_loss(rt, oracle, device):
     say("[gpu] nn training steps")
     rng = np.random.default_rng(13)
-    im, frame, target = random_batch(rng, SIZES)
-    nn = rt.Denoiser(0, random_params(rng, 0.05))
+    im, frame, _ = random_batch(rng, SIZES)
+    # a target the network can represent: a function of the albedo buffer
+    target = np.zeros_like(frame["albedo"])
+    target[:, :3] = 0.8 * frame["albedo"][:, :3] + 0.05
+    nn = rt.Denoiser(0, scaled_params(rng, 0.5))
     first, _ = nn.train_step(SIZES, frame, target, 1e-3)
     last = first
-    for _ in range(60):
+    for _ in range(80):
         last, _ = nn.train_step(SIZES, frame, target, 1e-3)
     say(f"  loss {first:.5f} -> {last:.5f}")
-    assert last < 0.7 * first
+    assert last < 0.5 * first
