@@ -295,8 +295,9 @@ static int gen_cornell_mesh(rto_generated* g, uint64_t seed) {
 }
 
 /* config 4: Cornell room with 1-8 displaced icospheres (subdivision 4),
- * one random OpenPBR material each and a random ceiling light (see the
- * library's generate_cornell_random for the draw order) */
+ * one random OpenPBR material each, a random ceiling light (see the
+ * library's generate_cornell_random for the draw order) and four cameras on
+ * an arc in front of the open side */
 static int gen_cornell_random(rto_generated* g, uint64_t seed) {
     pcg32_t r;
     pcg_seed(&r, seed);
@@ -362,9 +363,11 @@ static int gen_cornell_random(rto_generated* g, uint64_t seed) {
     set3(g->scene.background, 0.0f, 0.0f, 0.0f);
     g->camera_count = 4;
     const double center[3] = {277.5, 277.5, 277.5}, up[3] = {0, 1, 0};
+    const double azimuth[4] = {-24.0, -8.0, 8.0, 24.0}, height[4] = {100.0, -60.0, 60.0, -100.0};
     for (int c = 0; c < 4; ++c) {
-        const double th = (double)c * 0.5 * 3.14159265358979323846;
-        const double from[3] = {center[0] + 800.0 * sin(th), center[1], center[2] - 800.0 * cos(th)};
+        const double th = azimuth[c] * (3.14159265358979323846 / 180.0);
+        const double from[3] = {center[0] + 800.0 * sin(th), center[1] + height[c],
+                                center[2] - 800.0 * cos(th)};
         lookat_pinhole(&g->cameras[c], 640, 480, 40.0, from, center, up);
     }
     profile_defaults(&g->profile, 16, 8);

@@ -86,59 +86,95 @@ struct ConvArgs {
     uint16_t* out;        // [P][COUT]
 };
 
-// MODE 0: out = relu(acc + bias); MODE 1: out = acc (raw input gradient)
+// input channels [ci, ci + 8) of tap t at a pixel (zero outside the image or
+// beyond the 9 taps); decoder inputs read channels >= C0 at the parent
+template <int C0, int C1>
+__device__ __forceinline__ uint4 conv_gather(const ConvArgs& a, const Pix& q, bool valid, int t, int ci) {
+    uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+    if (valid && t < 9) {
+        const int sx = q.x + t % 3 - 1, sy = q.y + t / 3 - 1;
+        if (sx >= 0 && sx < q.w && sy >= 0 && sy < q.h) {
+            if (C1 == 0 || ci < C0) {
+                raw = *reinterpret_cast<const uint4*>(a.A + (size_t)(q.base + sy * q.w + sx) * C0 + ci);
+            } else {
+                raw = *reinterpret_cast<const uint4*>(
+                    a.B + (size_t)parent_index(a.pim, q.img, sx, sy) * C1 + (ci - C0));
+            }
+        }
+    }
+    return raw;
+}
+
+// pixel tiles (of 16) per wave: each weight fragment feeds MT MFMAs
+template <int COUT>
+constexpr int conv_mt() { return COUT / 16 <= 4 ? 4 : 2; }
+
+// One wave computes out[16 MT pixels][COUT] as the transposed GEMM
+// out^T = W X^T: the MFMA A operand is the weights (row = output channel,
+// 8 consecutive K per lane, one 16-byte load), the B operand the gathered
+// input (column = pixel, the 8 channels of one tap), so a lane ends with 4
+// consecutive output channels of one pixel and stores them as 8 bytes.
+// MODE 0: out = relu(acc + bias); MODE 1: out = acc (raw input gradient).
 template <int C0, int C1, int COUT, int MODE>
 __global__ __launch_bounds__(256) void k_uconv(ConvArgs a) {
     constexpr int CIN = C0 + C1;
     constexpr int KC = (9 * CIN + 31) / 32;
     constexpr int NT = COUT / 16;
+    constexpr int MT = conv_mt<COUT>();
     static_assert(CIN % 8 == 0 && C0 % 8 == 0 && COUT % 16 == 0, "channel layout");
     const int lane = threadIdx.x & 63;
-    const int row = lane & 15;
+    const int r16 = lane & 15;
     const int kq = lane >> 4;
     const int waves = gridDim.x * 4;
-    const int tiles = (a.P + 15) / 16;
-    for (int tile = blockIdx.x * 4 + (threadIdx.x >> 6); tile < tiles; tile += waves) {
-        const int p = tile * 16 + row;
-        const bool valid = p < a.P;
-        Pix q{};
-        if (valid) q = locate(a.im, p);
-        f32x4 acc[NT];
+    const int groups = (a.P + 16 * MT - 1) / (16 * MT);
+    for (int g = blockIdx.x * 4 + (threadIdx.x >> 6); g < groups; g += waves) {
+        Pix q[MT];
+        bool valid[MT];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int m = 0; m < MT; ++m) {
+            const int p = (g * MT + m) * 16 + r16;
+            valid[m] = p < a.P;
+            q[m] = valid[m] ? locate(a.im, p) : Pix{};
+        }
+        f32x4 acc[MT][NT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[m][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
         for (int kc = 0; kc < KC; ++kc) {
             const int k0 = kc * 32 + kq * 8;
             const int t = k0 / CIN, ci = k0 - t * CIN;
-            uint4 raw = make_uint4(0u, 0u, 0u, 0u);
-            if (valid && t < 9) {
-                const int sx = q.x + t % 3 - 1, sy = q.y + t / 3 - 1;
-                if (sx >= 0 && sx < q.w && sy >= 0 && sy < q.h) {
-                    if (C1 == 0 || ci < C0) {
-                        raw = *reinterpret_cast<const uint4*>(a.A + (size_t)(q.base + sy * q.w + sx) * C0 + ci);
-                    } else {
-                        raw = *reinterpret_cast<const uint4*>(
-                            a.B + (size_t)parent_index(a.pim, q.img, sx, sy) * C1 + (ci - C0));
-                    }
-                }
-            }
-            const bf16x8 av = __builtin_bit_cast(bf16x8, raw);
+            bf16x8 wv[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                wv[nt] = *reinterpret_cast<const bf16x8*>(a.W + (size_t)(nt * 16 + r16) * a.kp + k0);
+            bf16x8 xv[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) xv[m] = __builtin_bit_cast(bf16x8, conv_gather<C0, C1>(a, q[m], valid[m], t, ci));
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[m][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv[nt], xv[m], acc[m][nt], 0, 0, 0);
+        }
+        // D[i][j]: i = output channel 4 kq + e of the tile, j = pixel r16
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int p = (g * MT + m) * 16 + r16;
+            if (p >= a.P) continue;
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
-                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(a.W + (size_t)(nt * 16 + row) * a.kp + k0);
-                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[nt], 0, 0, 0);
-            }
-        }
-        // D[i][j]: lane = j + 16 (i / 4), element i % 4; i = pixel row, j = channel
+                const int co = nt * 16 + kq * 4;
+                float v[4];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            const int co = nt * 16 + row;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int pp = tile * 16 + kq * 4 + e;
-                if (pp >= a.P) continue;
-                float v = acc[nt][e];
-                if (MODE == 0) v = fmaxf(v + a.bias[co], 0.0f);
-                a.out[(size_t)pp * COUT + co] = to_bf16(v);
+                for (int e = 0; e < 4; ++e) {
+                    v[e] = acc[m][nt][e];
+                    if (MODE == 0) v[e] = fmaxf(v[e] + a.bias[co + e], 0.0f);
+                }
+                const uint2 packed = make_uint2((uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16),
+                                                (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16));
+                *reinterpret_cast<uint2*>(a.out + (size_t)p * COUT + co) = packed;
             }
         }
     }
@@ -307,93 +343,127 @@ __global__ __launch_bounds__(256) void k_head(int P, const uint16_t* h3, const f
 
 struct WgradArgs {
     NnImages im, pim;
-    int P, cout;
+    int P;
     const uint16_t* dz;   // [P][cout]
     const uint16_t* A;    // [P][C0]
     const uint16_t* B;    // [P coarser][C1]
     float* partial;       // [chunks][cout][NW * 16]
 };
 
-// Weight gradient of one convolution as a GEMM with K = pixels:
-//   G[co][n] = sum_p dz[p][co] * X[p + off(n / CIN)][n % CIN]   (n < 9 CIN)
-//   G[co][9 CIN] = sum_p dz[p][co]                              (bias)
-// Block (chunk, group) covers pixels [chunk * kNnGradPixels, ...) and the
-// 16 output tiles [16 group, 16 group + 16) of the cout/16 x NW tile grid;
-// wave w owns tiles 16 group + w + 4 i.
-template <int C0, int C1>
-__global__ __launch_bounds__(256) void k_uwgrad(WgradArgs a) {
-    constexpr int CIN = C0 + C1;
-    constexpr int NW = (9 * CIN) / 16 + 1;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int col = lane & 15;
-    const int kq = lane >> 4;
-    const int tiles = (a.cout / 16) * NW;
-    f32x4 acc[4];
+constexpr int kWgStep = 64;            // pixels per LDS stage (two MFMA K slices)
+constexpr int kWgRow = kWgStep + 8;    // LDS row stride in bf16: 144 B, 16-byte aligned rows
+
+// scatter 8 channels of one pixel into an LDS [channel][pixel] tile
+__device__ __forceinline__ void lds_scatter8(uint16_t* s, int c, int px, uint4 raw) {
+    const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
 #pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int j = 0; j < 4; ++j) {
+        s[(c + 2 * j) * kWgRow + px] = (uint16_t)(w[j] & 0xFFFFu);
+        s[(c + 2 * j + 1) * kWgRow + px] = (uint16_t)(w[j] >> 16);
+    }
+}
+
+// Weight gradient of one convolution as GEMMs with K = pixels:
+//   G[co][t CIN + ci] = sum_p dz[p][co] X[p + off(t)][ci]   (block row t < 9)
+//   G[co][9 CIN]      = sum_p dz[p][co]                      (block row t = 9, the bias)
+// Block (chunk, t) covers pixels [chunk * kNnGradPixels, ...).  Per stage of
+// 64 pixels it copies dz and the tap-shifted input into LDS transposed
+// ([channel][pixel]), so that a lane reads the 8 consecutive pixels of one
+// channel an MFMA operand needs with one 16-byte LDS load; wave w owns the
+// (cout / 16) x (CIN / 16) output tiles w, w + 4, ...  Partial sums per chunk
+// are reduced in chunk order afterwards (deterministic).
+template <int C0, int C1, int COUT>
+__global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
+    constexpr int CIN = C0 + C1;
+    constexpr int MT = COUT / 16, NT = CIN / 16;
+    constexpr int TILES = MT * NT;
+    constexpr int TPW = (TILES + 3) / 4;
+    constexpr int NWC = ((9 * CIN) / 16 + 1) * 16;
+    static_assert(CIN % 16 == 0 && C0 % 8 == 0 && COUT % 16 == 0, "channel layout");
+    __shared__ __align__(16) uint16_t sdz[COUT * kWgRow];
+    __shared__ __align__(16) uint16_t sx[CIN * kWgRow];
+    __shared__ int src_a[kWgStep], src_b[kWgStep];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, kq = lane >> 4;
+    const int t = blockIdx.y;
+    const bool bias = t == 9;
+    const int dx = t % 3 - 1, dy = t / 3 - 1;
+    const int tiles = bias ? MT : TILES;
+    f32x4 acc[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) acc[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const int p0 = blockIdx.x * kNnGradPixels;
     const int p1 = min(a.P, p0 + kNnGradPixels);
-    for (int c0 = p0; c0 < p1; c0 += 32) {
-        int pix[8];
-        Pix q[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            pix[j] = c0 + kq * 8 + j;
-            if (pix[j] < p1) q[j] = locate(a.im, pix[j]);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int tile = blockIdx.y * 16 + wave + 4 * i;
-            if (tile >= tiles) break;
-            const int mt = tile / NW, nt = tile - mt * NW;
-            const int co = mt * 16 + col;
-            const int n = nt * 16 + col;
-            const int t = n / CIN, ci = n - t * CIN;
-            uint16_t av[8], bv[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                av[j] = 0;
-                bv[j] = 0;
-                if (pix[j] >= p1) continue;
-                av[j] = a.dz[(size_t)pix[j] * a.cout + co];
-                if (t < 9) {
-                    const int sx = q[j].x + t % 3 - 1, sy = q[j].y + t / 3 - 1;
-                    if (sx >= 0 && sx < q[j].w && sy >= 0 && sy < q[j].h) {
-                        if (C1 == 0 || ci < C0) {
-                            bv[j] = a.A[(size_t)(q[j].base + sy * q[j].w + sx) * C0 + ci];
-                        } else {
-                            bv[j] = a.B[(size_t)parent_index(a.pim, q[j].img, sx, sy) * C1 + (ci - C0)];
-                        }
-                    }
-                } else if (ci == 0) {
-                    bv[j] = 0x3F80;  // 1.0: the bias column
+    const bf16x8 ones = __builtin_bit_cast(bf16x8, make_uint4(0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u));
+    for (int s0 = p0; s0 < p1; s0 += kWgStep) {
+        if (!bias && tid < kWgStep) {
+            const int p = s0 + tid;
+            int ia = -1, ib = -1;
+            if (p < p1) {
+                const Pix q = locate(a.im, p);
+                const int sx_ = q.x + dx, sy_ = q.y + dy;
+                if (sx_ >= 0 && sx_ < q.w && sy_ >= 0 && sy_ < q.h) {
+                    ia = q.base + sy_ * q.w + sx_;
+                    if (C1 > 0) ib = parent_index(a.pim, q.img, sx_, sy_);
                 }
             }
-            uint4 ra, rb;
-            ra.x = (uint32_t)av[0] | ((uint32_t)av[1] << 16);
-            ra.y = (uint32_t)av[2] | ((uint32_t)av[3] << 16);
-            ra.z = (uint32_t)av[4] | ((uint32_t)av[5] << 16);
-            ra.w = (uint32_t)av[6] | ((uint32_t)av[7] << 16);
-            rb.x = (uint32_t)bv[0] | ((uint32_t)bv[1] << 16);
-            rb.y = (uint32_t)bv[2] | ((uint32_t)bv[3] << 16);
-            rb.z = (uint32_t)bv[4] | ((uint32_t)bv[5] << 16);
-            rb.w = (uint32_t)bv[6] | ((uint32_t)bv[7] << 16);
-            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ra),
-                __builtin_bit_cast(bf16x8, rb), acc[i], 0, 0, 0);
+            src_a[tid] = ia;
+            src_b[tid] = ib;
         }
-    }
-    float* dst = a.partial + (size_t)blockIdx.x * a.cout * (NW * 16);
+        for (int u = tid; u < kWgStep * (COUT / 8); u += 256) {
+            const int px = u / (COUT / 8), c = (u - px * (COUT / 8)) * 8;
+            const int p = s0 + px;
+            uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+            if (p < p1) raw = *reinterpret_cast<const uint4*>(a.dz + (size_t)p * COUT + c);
+            lds_scatter8(sdz, c, px, raw);
+        }
+        __syncthreads();
+        if (!bias) {
+            for (int u = tid; u < kWgStep * (CIN / 8); u += 256) {
+                const int px = u / (CIN / 8), c = (u - px * (CIN / 8)) * 8;
+                uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+                if (C1 == 0 || c < C0) {
+                    const int i = src_a[px];
+                    if (i >= 0) raw = *reinterpret_cast<const uint4*>(a.A + (size_t)i * C0 + c);
+                } else {
+                    const int i = src_b[px];
+                    if (i >= 0) raw = *reinterpret_cast<const uint4*>(a.B + (size_t)i * C1 + (c - C0));
+                }
+                lds_scatter8(sx, c, px, raw);
+            }
+        }
+        __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int tile = blockIdx.y * 16 + wave + 4 * i;
-        if (tile >= tiles) break;
-        const int mt = tile / NW, nt = tile - mt * NW;
+        for (int ks = 0; ks < kWgStep / 32; ++ks) {
+#pragma unroll
+            for (int i = 0; i < TPW; ++i) {
+                const int tile = wave + 4 * i;
+                if (tile < tiles) {
+                    const int mt = bias ? tile : tile / NT, nt = bias ? 0 : tile - (tile / NT) * NT;
+                    const bf16x8 av = *reinterpret_cast<const bf16x8*>(sdz + (mt * 16 + r16) * kWgRow + ks * 32 + kq * 8);
+                    const bf16x8 bv = bias ? ones
+                                           : *reinterpret_cast<const bf16x8*>(sx + (nt * 16 + r16) * kWgRow + ks * 32 + kq * 8);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[i], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    float* dst = a.partial + (size_t)blockIdx.x * COUT * NWC;
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+        const int tile = wave + 4 * i;
+        if (tile >= tiles) continue;
+        const int mt = bias ? tile : tile / NT, nt = bias ? 0 : tile - (tile / NT) * NT;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int co = mt * 16 + kq * 4 + e;  // D row
-            const int n = nt * 16 + col;          // D column
-            dst[(size_t)co * (NW * 16) + n] = acc[i][e];
+            const int co = mt * 16 + kq * 4 + e;   // D row
+            if (bias) {
+                if (r16 == 0) dst[(size_t)co * NWC + 9 * CIN] = acc[i][e];
+            } else {
+                dst[(size_t)co * NWC + t * CIN + nt * 16 + r16] = acc[i][e];
+            }
         }
     }
 }
@@ -417,13 +487,23 @@ __global__ void k_uwgrad_reduce(int chunks, int cout, int cin, const float* part
     grad[i] = s;
 }
 
-__global__ void k_head_reduce(int blocks, const float* partial, float* grad, float* loss) {
-    const int i = threadIdx.x;
-    if (i > kNnHeadParams) return;
+// block i sums column i of the per-block head partials (fixed order)
+__global__ __launch_bounds__(256) void k_head_reduce(int blocks, const float* partial, float* grad,
+    float* loss) {
+    __shared__ float red[256];
+    const int i = blockIdx.x;
     float s = 0.0f;
-    for (int b = 0; b < blocks; ++b) s += partial[(size_t)b * (kNnHeadParams + 1) + i];
-    if (i < kNnHeadParams) grad[i] = s;
-    else *loss = s;
+    for (int b = threadIdx.x; b < blocks; b += 256) s += partial[(size_t)b * (kNnHeadParams + 1) + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (i < kNnHeadParams) grad[i] = red[0];
+        else *loss = red[0];
+    }
 }
 
 __global__ void k_adam(int n, float* params, const float* grad, float* m, float* v, float lr,
@@ -472,7 +552,8 @@ inline int level_pixels(const NnImages& im) { return im.offset[im.count]; }
 // forward (MODE 0) or input-gradient (MODE 1) launch of convolution l
 template <int C0, int C1, int COUT, int MODE>
 void conv_launch(const ConvArgs& a, int grid, hipStream_t st) {
-    const int blocks = std::max(1, std::min(grid, cdiv(cdiv(a.P, 16), 4)));
+    const int groups = cdiv(a.P, 16 * conv_mt<COUT>());
+    const int blocks = std::max(1, std::min(grid, cdiv(groups, 4)));
     k_uconv<C0, C1, COUT, MODE><<<blocks, 256, 0, st>>>(a);
 }
 
@@ -517,31 +598,29 @@ void conv_input_grad(int l, const ConvArgs& a, int grid, hipStream_t st) {
     }
 }
 
-template <int C0, int C1>
+template <int C0, int C1, int COUT>
 void wgrad_launch(const WgradArgs& a, hipStream_t st) {
-    constexpr int NW = (9 * (C0 + C1)) / 16 + 1;
-    const int tiles = (a.cout / 16) * NW;
-    const dim3 grid((unsigned)cdiv(a.P, kNnGradPixels), (unsigned)cdiv(tiles, 16));
-    k_uwgrad<C0, C1><<<grid, 256, 0, st>>>(a);
+    const dim3 grid((unsigned)cdiv(a.P, kNnGradPixels), 10u);
+    k_wgrad<C0, C1, COUT><<<grid, 256, 0, st>>>(a);
 }
 
 void conv_wgrad(int l, const WgradArgs& a, hipStream_t st) {
     switch (l) {
-        case 0: wgrad_launch<16, 0>(a, st); break;
-        case 1: wgrad_launch<32, 0>(a, st); break;
-        case 2: wgrad_launch<32, 0>(a, st); break;
-        case 3: wgrad_launch<48, 0>(a, st); break;
-        case 4: wgrad_launch<48, 0>(a, st); break;
-        case 5: wgrad_launch<64, 0>(a, st); break;
-        case 6: wgrad_launch<64, 0>(a, st); break;
-        case 7: wgrad_launch<96, 0>(a, st); break;
-        case 8: wgrad_launch<96, 0>(a, st); break;
-        case 9: wgrad_launch<64, 96>(a, st); break;
-        case 10: wgrad_launch<64, 0>(a, st); break;
-        case 11: wgrad_launch<48, 64>(a, st); break;
-        case 12: wgrad_launch<48, 0>(a, st); break;
-        case 13: wgrad_launch<32, 48>(a, st); break;
-        default: wgrad_launch<32, 0>(a, st); break;
+        case 0: wgrad_launch<16, 0, 32>(a, st); break;
+        case 1: wgrad_launch<32, 0, 32>(a, st); break;
+        case 2: wgrad_launch<32, 0, 48>(a, st); break;
+        case 3: wgrad_launch<48, 0, 48>(a, st); break;
+        case 4: wgrad_launch<48, 0, 64>(a, st); break;
+        case 5: wgrad_launch<64, 0, 64>(a, st); break;
+        case 6: wgrad_launch<64, 0, 96>(a, st); break;
+        case 7: wgrad_launch<96, 0, 96>(a, st); break;
+        case 8: wgrad_launch<96, 0, 96>(a, st); break;
+        case 9: wgrad_launch<64, 96, 64>(a, st); break;
+        case 10: wgrad_launch<64, 0, 64>(a, st); break;
+        case 11: wgrad_launch<48, 64, 48>(a, st); break;
+        case 12: wgrad_launch<48, 0, 48>(a, st); break;
+        case 13: wgrad_launch<32, 48, 32>(a, st); break;
+        default: wgrad_launch<32, 0, 32>(a, st); break;
     }
 }
 
@@ -645,7 +724,7 @@ hipError_t nn_backward(const NnLevels& lv, const NnPacked& pk, const NnScratch& 
     float* loss, int conv_grid, hipStream_t st) {
     const int P = level_pixels(lv.lv[0]);
     if (P <= 0) return hipSuccess;
-    k_head_reduce<<<1, 128, 0, st>>>(cdiv(P, 256), w.head_partial, grad + nn_head_offset(), loss);
+    k_head_reduce<<<kNnHeadParams + 1, 256, 0, st>>>(cdiv(P, 256), w.head_partial, grad + nn_head_offset(), loss);
     for (int l = kNnConvs - 1; l >= 0; --l) {
         const NnConv c = nn_conv(l);
         const int cin = nn_cin(l);
@@ -656,7 +735,6 @@ hipError_t nn_backward(const NnLevels& lv, const NnPacked& pk, const NnScratch& 
         g.im = im;
         g.pim = lv.lv[c.level < kNnLevels - 1 ? c.level + 1 : c.level];
         g.P = Pl;
-        g.cout = c.cout;
         g.dz = w.dz[l];
         conv_inputs(l, w, &g.A, &g.B);
         g.partial = w.grad_partial;

@@ -287,7 +287,8 @@ void generate_cornell_mesh(uint64_t seed, rt_generated_scene* g) {
 
 // config 4: the Cornell room of config 1 with 1-8 displaced icospheres (4
 // subdivisions, 5,120 triangles each), one random constant OpenPBR material
-// per sphere (drawn as in config 1) and a random ceiling light.  Draw order:
+// per sphere (drawn as in config 1), a random ceiling light and four
+// cameras in front of the open side.  Draw order:
 // light width, depth, x, z, radiance; sphere count; per sphere radius,
 // centre x, y, z, base colour, roughness, metalness, coat, then one
 // displacement per vertex.
@@ -356,13 +357,17 @@ void generate_cornell_random(uint64_t seed, rt_generated_scene* g) {
     g->scene.openpbr = copy_out(ops);
     g->scene.background_mode = RT_BG_CONSTANT;
     set3(g->scene.background, 0.0f, 0.0f, 0.0f);
+    // four cameras on an arc in front of the open side (azimuth -24, -8, 8,
+    // 24 degrees at radius 800, heights +100, -60, +60, -100), all aimed at
+    // the room centre: every image sees the lit interior
     g->camera_count = 4;
     const double c0[3] = {277.5, 277.5, 277.5};
+    const double azimuth[4] = {-24.0, -8.0, 8.0, 24.0}, height[4] = {100.0, -60.0, 60.0, -100.0};
     for (int c = 0; c < 4; ++c) {
-        const double th = static_cast<double>(c) * 0.5 * 3.14159265358979323846;
+        const double th = azimuth[c] * (3.14159265358979323846 / 180.0);
         make_lookat_pinhole(&g->cameras[c], 640, 480, 40.0,
-            {c0[0] + 800.0 * std::sin(th), c0[1], c0[2] - 800.0 * std::cos(th)}, {c0[0], c0[1], c0[2]},
-            {0.0, 1.0, 0.0});
+            {c0[0] + 800.0 * std::sin(th), c0[1] + height[c], c0[2] - 800.0 * std::cos(th)},
+            {c0[0], c0[1], c0[2]}, {0.0, 1.0, 0.0});
     }
     g->profile = default_profile(16, 8);
 }

@@ -186,10 +186,11 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--target-spp", type=int, default=1024)
     ap.add_argument("--ref-spp", type=int, default=4096)
-    ap.add_argument("--cornell-seeds", type=int, default=64)
+    ap.add_argument("--cornell-seeds", type=int, default=48)
     ap.add_argument("--instanced-seeds", type=int, default=4)
     ap.add_argument("--spheres-seeds", type=int, default=4)
-    ap.add_argument("--eval-seeds", type=int, default=50, help="held-out cornell_random seeds 1000..")
+    ap.add_argument("--eval-seeds", type=int, default=50, help="held-out cornell_random seeds")
+    ap.add_argument("--eval-start", type=int, default=1000, help="first held-out seed")
     ap.add_argument("--weights", default="", help="evaluate these weights instead of training")
     a = ap.parse_args()
     dev = rt.Device(0)
@@ -211,13 +212,13 @@ def main():
                            "spheres_seeds": a.spheres_seeds,
                            "loss_first": float(np.mean(hist[:50])),
                            "loss_last": float(np.mean(hist[-50:]))}
-    evals = [evaluate(dev, nn, "cornell_random", 1000 + i, a.ref_spp) for i in range(a.eval_seeds)]
+    seeds = [a.eval_start + i for i in range(a.eval_seeds)]
+    evals = [evaluate(dev, nn, "cornell_random", s, a.ref_spp) for s in seeds]
     summary = {k: float(np.mean([e[k] for e in evals]))
                for k in ("relmse_raw", "relmse_atrous", "relmse_learned")}
-    print(f"[eval] cornell_random seeds 1000..{999 + a.eval_seeds}: " +
+    print(f"[eval] cornell_random seeds {seeds[0]}..{seeds[-1]}: " +
           " ".join(f"{k} {v:.4f}" for k, v in summary.items()), flush=True)
-    report["eval_cornell_random"] = {"seeds": [1000 + i for i in range(a.eval_seeds)],
-                                     "ref_spp": a.ref_spp, **summary}
+    report["eval_cornell_random"] = {"seeds": seeds, "ref_spp": a.ref_spp, **summary}
     report["eval"] = evals + [evaluate(dev, nn, "random_spheres", 1000, a.ref_spp),
                               evaluate(dev, nn, "instanced_mesh_restir", 1000, a.ref_spp)]
     if a.report:
