@@ -111,6 +111,12 @@ typedef struct rt_camera {
     double lut[1024];
 } rt_camera;
 
+/* fills lut_step / lut of an equi62_lut1d camera from its size, intrinsics
+ * and radial coefficients (make_equi62_lut1d_params,
+ * src/realtime/camera_models.cpp:228-248): theta for 1024 evenly spaced
+ * distorted radii up to the image corner plus 10 pixels, by Newton inversion */
+rt_status rt_camera_build_lut(rt_camera* camera);
+
 /* PackedCameraRig, src/realtime/camera_rig.h:31-34 */
 typedef struct rt_camera_rig {
     int32_t active_count;

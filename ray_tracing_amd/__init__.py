@@ -39,6 +39,7 @@ EXPORTED = [
     "rt_cache_create", "rt_cache_destroy", "rt_cache_set_params", "rt_cache_get_params",
     "rt_cache_set_bounds", "rt_cache_grid_info", "rt_cache_infer_host",
     "rt_cache_train_step_host", "rt_cache_get_stats", "rt_renderer_set_radiance_cache",
+    "rt_camera_build_lut",
 ]
 
 

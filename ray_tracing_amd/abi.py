@@ -232,6 +232,7 @@ def declare(L):
     L.rt_renderer_autotune.argtypes = [vp, P(CameraRig), P(Profile), C.c_int32, P(AutotuneResult)]
     L.rt_frame_download_guides.argtypes = [vp, C.c_int, vp, vp, vp]
     L.rt_scene_generate_rig.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, P(CameraRig)]
+    L.rt_camera_build_lut.argtypes = [P(Camera)]
     L.rt_cache_default_config.argtypes = [P(CacheConfig)]
     L.rt_cache_default_config.restype = None
     L.rt_cache_mlp_param_count.argtypes = []

@@ -145,11 +145,20 @@ if __name__ == "__main__":
     ap.add_argument("--frames", type=int, default=16)
     ap.add_argument("--max-rel-loss", type=float, default=None)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--profile-frames", type=int, default=0,
+                    help="only render this many frames of cache + denoiser (for a kernel trace)")
     a = ap.parse_args()
     ov = {}
     if a.max_rel_loss is not None:
         ov["max_rel_loss"] = a.max_rel_loss
     d = rt.Device(0)
+    if a.profile_frames > 0:
+        _, ms, _ = run_sequence(d, "radiance_cache_realtime", 1000, a.profile_frames,
+                                a.profile_frames, nn=rt.Denoiser(0, np.load(WEIGHTS)),
+                                cache=rt.RadianceCache(0, **ov), timed=a.profile_frames)
+        print(f"[cache-eval] cache + denoiser: {a.profile_frames} frames, median {ms:.2f} ms/frame",
+              flush=True)
+        raise SystemExit(0)
     r = compare(d, ref_spp=a.ref_spp, frame=a.frame, frames=a.frames, cache_overrides=ov)
     txt = json.dumps(r)
     print(txt, flush=True)

@@ -1488,6 +1488,16 @@ rt_status rt_scene_generate_rig(int config, uint64_t seed, int frame, int frames
     return RT_OK;
 }
 
+rt_status rt_camera_build_lut(rt_camera* c) {
+    RT_REQUIRE(c != nullptr, RT_ERROR_INVALID_ARGUMENT, "null camera");
+    RT_REQUIRE(c->model == RT_CAM_EQUI62_LUT1D, RT_ERROR_INVALID_ARGUMENT,
+        "only equi62_lut1d cameras have a theta LUT");
+    RT_REQUIRE(c->width > 0 && c->height > 0 && c->fx > 0.0 && c->fy > 0.0, RT_ERROR_INVALID_ARGUMENT,
+        "camera needs a positive size and focal lengths");
+    rth::make_equi_lut(c);
+    return RT_OK;
+}
+
 rt_status rt_scene_generated_free(rt_generated_scene* g) {
     if (!g) return RT_OK;
     rth::free_generated(g);

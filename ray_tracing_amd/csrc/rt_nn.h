@@ -56,9 +56,11 @@ __host__ __device__ constexpr int nn_layer_offset(int l) {
 }
 __host__ __device__ constexpr int nn_head_offset() { return nn_layer_offset(kNnConvs); }
 __host__ __device__ constexpr int nn_param_count() { return nn_head_offset() + kNnHeadParams; }
-// padded K of the forward (9 cin) and input-gradient (9 cout) GEMMs
-__host__ __device__ constexpr int nn_kp_fwd(int l) { return (9 * nn_cin(l) + 31) / 32 * 32; }
-__host__ __device__ constexpr int nn_kp_bwd(int l) { return (9 * nn_conv(l).cout + 31) / 32 * 32; }
+// K of a packed convolution GEMM: each of the 9 taps holds its c input
+// channels in a run padded to a multiple of 32 (one tap per MFMA K step)
+__host__ __device__ constexpr int nn_tap_stride(int c) { return (c + 31) / 32 * 32; }
+__host__ __device__ constexpr int nn_kp_fwd(int l) { return 9 * nn_tap_stride(nn_cin(l)); }
+__host__ __device__ constexpr int nn_kp_bwd(int l) { return 9 * nn_tap_stride(nn_conv(l).cout); }
 
 // images of one batch, back to back
 struct NnImages {
@@ -74,8 +76,8 @@ struct NnLevels {
 
 // device weights in the kernels' layouts
 struct NnPacked {
-    uint16_t* fwd[kNnConvs];   // bf16 [cout][kp_fwd], k = tap * cin + ci
-    uint16_t* bwd[kNnConvs];   // bf16 [cin][kp_bwd], k = tap' * cout + co, W[co][8 - tap'][ci]
+    uint16_t* fwd[kNnConvs];   // bf16 [cout][kp_fwd], k = tap * stride(cin) + ci
+    uint16_t* bwd[kNnConvs];   // bf16 [cin][kp_bwd], k = tap' * stride(cout) + co, W[co][8 - tap'][ci]
     float* bias[kNnConvs];
     float* head;               // FP32 W[3][32], b[3]
 };

@@ -78,6 +78,7 @@ struct ConvArgs {
     NnImages im;          // this level
     NnImages pim;         // one level coarser (second input)
     int P;                // pixels at this level
+    int Pp;               // pixels one level coarser
     const uint16_t* A;    // [P][C0]
     const uint16_t* B;    // [P coarser][C1]
     const uint16_t* W;    // bf16 [COUT][kp]
@@ -86,23 +87,18 @@ struct ConvArgs {
     uint16_t* out;        // [P][COUT]
 };
 
-// input channels [ci, ci + 8) of tap t at a pixel (zero outside the image or
-// beyond the 9 taps); decoder inputs read channels >= C0 at the parent
-template <int C0, int C1>
-__device__ __forceinline__ uint4 conv_gather(const ConvArgs& a, const Pix& q, bool valid, int t, int ci) {
-    uint4 raw = make_uint4(0u, 0u, 0u, 0u);
-    if (valid && t < 9) {
-        const int sx = q.x + t % 3 - 1, sy = q.y + t / 3 - 1;
-        if (sx >= 0 && sx < q.w && sy >= 0 && sy < q.h) {
-            if (C1 == 0 || ci < C0) {
-                raw = *reinterpret_cast<const uint4*>(a.A + (size_t)(q.base + sy * q.w + sx) * C0 + ci);
-            } else {
-                raw = *reinterpret_cast<const uint4*>(
-                    a.B + (size_t)parent_index(a.pim, q.img, sx, sy) * C1 + (ci - C0));
-            }
-        }
-    }
-    return raw;
+// Raw buffer resource over `bytes` bytes (gfx9 descriptor word 3: 32-bit
+// data format); a load at an offset past the end returns zeros, which is how
+// taps outside the image and padding channels read as zero without a branch.
+constexpr uint32_t kBufWord3 = 0x00020000u;
+constexpr int kBufOob = 0x7FFFF000;  // beyond every buffer, also plus an immediate offset < 4096 (sizes are checked < 2^31 - 2^12 bytes)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, (int)kBufWord3);
+}
+
+__device__ __forceinline__ bf16x8 buf_load16(__amdgpu_buffer_rsrc_t r, int off) {
+    return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
 
 // pixel tiles (of 16) per wave: each weight fragment feeds MT MFMAs
@@ -111,14 +107,21 @@ constexpr int conv_mt() { return COUT / 16 <= 4 ? 4 : 2; }
 
 // One wave computes out[16 MT pixels][COUT] as the transposed GEMM
 // out^T = W X^T: the MFMA A operand is the weights (row = output channel,
-// 8 consecutive K per lane, one 16-byte load), the B operand the gathered
-// input (column = pixel, the 8 channels of one tap), so a lane ends with 4
-// consecutive output channels of one pixel and stores them as 8 bytes.
-// MODE 0: out = relu(acc + bias); MODE 1: out = acc (raw input gradient).
+// 8 consecutive K per lane, one 16-byte load), the B operand the input
+// (column = pixel, 8 channels of one tap per lane).  K runs tap by tap, each
+// tap's channels padded to a multiple of 32, so every MFMA step has one
+// compile-time tap: a lane's input address is its pixel's precomputed base
+// plus a constant, and the tap's in-image bit selects it or an out-of-range
+// offset (zeros).  Decoder convolutions read channels >= C0 at the parent
+// pixel of the coarser level (nearest upsampling folded into the load).
+// A lane ends with 4 consecutive output channels of one pixel and stores
+// them as 8 bytes.  MODE 0: out = relu(acc + bias); MODE 1: out = acc.
 template <int C0, int C1, int COUT, int MODE>
 __global__ __launch_bounds__(256) void k_uconv(ConvArgs a) {
     constexpr int CIN = C0 + C1;
-    constexpr int KC = (9 * CIN + 31) / 32;
+    constexpr int TS = nn_tap_stride(CIN);
+    constexpr int KT = TS / 32;
+    constexpr int KP = 9 * TS;
     constexpr int NT = COUT / 16;
     constexpr int MT = conv_mt<COUT>();
     static_assert(CIN % 8 == 0 && C0 % 8 == 0 && COUT % 16 == 0, "channel layout");
@@ -127,36 +130,96 @@ __global__ __launch_bounds__(256) void k_uconv(ConvArgs a) {
     const int kq = lane >> 4;
     const int waves = gridDim.x * 4;
     const int groups = (a.P + 16 * MT - 1) / (16 * MT);
+    const __amdgpu_buffer_rsrc_t ra = buf_rsrc(a.A, (uint32_t)a.P * C0 * 2u);
+    const __amdgpu_buffer_rsrc_t rb = buf_rsrc(C1 > 0 ? a.B : a.A, C1 > 0 ? (uint32_t)a.Pp * C1 * 2u : 0u);
+    const uint16_t* wrow = a.W + (size_t)r16 * KP + kq * 8;
     for (int g = blockIdx.x * 4 + (threadIdx.x >> 6); g < groups; g += waves) {
-        Pix q[MT];
-        bool valid[MT];
+        // per pixel tile: byte offset of this lane's pixel and channel octet,
+        // row pitch, in-image bits of the 9 taps; decoder: the coarser image's
+        // base offset and row pitch and the pixel's coordinates
+        int cA[MT], rowA[MT], cB[MT], px_[MT], py_[MT], pwC[MT];
+        uint32_t mask[MT];
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
             const int p = (g * MT + m) * 16 + r16;
-            valid[m] = p < a.P;
-            q[m] = valid[m] ? locate(a.im, p) : Pix{};
+            mask[m] = 0u;
+            cA[m] = rowA[m] = cB[m] = px_[m] = py_[m] = pwC[m] = 0;
+            if (p < a.P) {
+                const Pix q = locate(a.im, p);
+                cA[m] = ((q.base + q.y * q.w + q.x) * C0 + kq * 8) * 2;
+                rowA[m] = q.w * C0 * 2;
+#pragma unroll
+                for (int t = 0; t < 9; ++t) {
+                    const int sx = q.x + t % 3 - 1, sy = q.y + t / 3 - 1;
+                    mask[m] |= (sx >= 0 && sx < q.w && sy >= 0 && sy < q.h) ? (1u << t) : 0u;
+                }
+                if (C1 > 0) {
+                    cB[m] = (a.pim.offset[q.img] * C1 + kq * 8) * 2;
+                    pwC[m] = a.pim.width[q.img] * C1 * 2;
+                    px_[m] = q.x;
+                    py_[m] = q.y;
+                }
+            }
         }
         f32x4 acc[MT][NT];
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) acc[m][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 2
-        for (int kc = 0; kc < KC; ++kc) {
-            const int k0 = kc * 32 + kq * 8;
-            const int t = k0 / CIN, ci = k0 - t * CIN;
-            bf16x8 wv[NT];
+        // per-lane channel validity of each K step (padding past CIN reads
+        // zero) and, for a step straddling the two inputs, which one it reads
+        bool ch_ok[KT], in_a[KT];
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
-                wv[nt] = *reinterpret_cast<const bf16x8*>(a.W + (size_t)(nt * 16 + r16) * a.kp + k0);
-            bf16x8 xv[MT];
+        for (int c = 0; c < KT; ++c) {
+            ch_ok[c] = c * 32 + kq * 8 < CIN;
+            in_a[c] = c * 32 + kq * 8 < C0;
+        }
+#pragma unroll 1
+        for (int t = 0; t < 9; ++t) {
+            const int dy = t / 3 - 1, dx = t % 3 - 1;
+            // this tap's input base offsets (out-of-range when outside the image)
+            int baseA[MT], baseB[MT];
 #pragma unroll
-            for (int m = 0; m < MT; ++m) xv[m] = __builtin_bit_cast(bf16x8, conv_gather<C0, C1>(a, q[m], valid[m], t, ci));
+            for (int m = 0; m < MT; ++m) {
+                const bool tap_ok = ((mask[m] >> t) & 1u) != 0u;
+                const int rowoff = dy < 0 ? -rowA[m] : dy > 0 ? rowA[m] : 0;
+                baseA[m] = tap_ok ? cA[m] + rowoff + dx * (C0 * 2) : kBufOob;
+                baseB[m] = kBufOob;
+                if (C1 > 0) {
+                    // parent of (x + dx, y + dy) at the coarser level
+                    const int prow = ((py_[m] + dy) >> 1) * pwC[m];
+                    const int pcol = ((px_[m] + dx) >> 1) * (C1 * 2);
+                    baseB[m] = tap_ok ? cB[m] + prow + pcol : kBufOob;
+                }
+            }
+            const uint16_t* wt = wrow + t * TS;
 #pragma unroll
-            for (int m = 0; m < MT; ++m)
+            for (int c = 0; c < KT; ++c) {
+                const int cb = c * 32;   // channel of lane group kq = 0
+                bf16x8 wv[NT];
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
-                    acc[m][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv[nt], xv[m], acc[m][nt], 0, 0, 0);
+                    wv[nt] = *reinterpret_cast<const bf16x8*>(wt + (size_t)nt * 16 * KP + cb);
+                const bool full = cb + 32 <= CIN;
+                bf16x8 xv[MT];
+#pragma unroll
+                for (int m = 0; m < MT; ++m) {
+                    if (C1 == 0 || cb + 32 <= C0) {
+                        xv[m] = buf_load16(ra, (full || ch_ok[c] ? baseA[m] : kBufOob) + cb * 2);
+                    } else if (cb >= C0) {
+                        xv[m] = buf_load16(rb, (full || ch_ok[c] ? baseB[m] : kBufOob) + (cb - C0) * 2);
+                    } else {
+                        const bf16x8 xa = buf_load16(ra, (in_a[c] ? baseA[m] : kBufOob) + cb * 2);
+                        const bf16x8 xb = buf_load16(rb, (!in_a[c] && ch_ok[c] ? baseB[m] : kBufOob) + (cb - C0) * 2);
+                        xv[m] = __builtin_bit_cast(bf16x8, __builtin_bit_cast(uint4, xa) | __builtin_bit_cast(uint4, xb));
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt)
+                        acc[m][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv[nt], xv[m], acc[m][nt], 0, 0, 0);
+            }
         }
         // D[i][j]: i = output channel 4 kq + e of the tile, j = pixel r16
 #pragma unroll
@@ -180,26 +243,39 @@ __global__ __launch_bounds__(256) void k_uconv(ConvArgs a) {
     }
 }
 
-// mean of the pixels of level l that a level-(l+1) pixel covers
+// mean of the pixels of level l that a level-(l+1) pixel covers; one thread
+// per (coarse pixel, 8 channels): 16-byte loads and stores
 __global__ __launch_bounds__(256) void k_pool(NnImages im, NnImages pim, int Pp, int C,
     const uint16_t* in, uint16_t* out) {
+    const int c8 = C / 8;
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= Pp * C) return;
-    const int pp = i / C, c = i - pp * C;
+    if (i >= Pp * c8) return;
+    const int pp = i / c8, c = (i - pp * c8) * 8;
     const Pix q = locate(pim, pp);
     const int w = im.width[q.img], h = image_height(im, q.img), base = im.offset[q.img];
-    float s = 0.0f;
+    float s[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     int n = 0;
     for (int dy = 0; dy < 2; ++dy) {
         for (int dx = 0; dx < 2; ++dx) {
             const int x = 2 * q.x + dx, y = 2 * q.y + dy;
             if (x < w && y < h) {
-                s += from_bf16(in[(size_t)(base + y * w + x) * C + c]);
+                const uint4 raw = *reinterpret_cast<const uint4*>(in + (size_t)(base + y * w + x) * C + c);
+                const uint32_t v[4] = {raw.x, raw.y, raw.z, raw.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    s[2 * j] += from_bf16((uint16_t)(v[j] & 0xFFFFu));
+                    s[2 * j + 1] += from_bf16((uint16_t)(v[j] >> 16));
+                }
                 ++n;
             }
         }
     }
-    out[(size_t)pp * C + c] = to_bf16(s / (float)n);
+    const float inv = (float)n;
+    uint32_t o[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        o[j] = (uint32_t)to_bf16(s[2 * j] / inv) | ((uint32_t)to_bf16(s[2 * j + 1] / inv) << 16);
+    *reinterpret_cast<uint4*>(out + (size_t)pp * C + c) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
 // dz = r where act > 0
@@ -366,7 +442,7 @@ __device__ __forceinline__ void lds_scatter8(uint16_t* s, int c, int px, uint4 r
 // Weight gradient of one convolution as GEMMs with K = pixels:
 //   G[co][t CIN + ci] = sum_p dz[p][co] X[p + off(t)][ci]   (block row t < 9)
 //   G[co][9 CIN]      = sum_p dz[p][co]                      (block row t = 9, the bias)
-// Block (chunk, t) covers pixels [chunk * kNnGradPixels, ...).  Per stage of
+// Block chunk * 10 + t covers pixels [chunk * kNnGradPixels, ...).  Per stage of
 // 64 pixels it copies dz and the tap-shifted input into LDS transposed
 // ([channel][pixel]), so that a lane reads the 8 consecutive pixels of one
 // channel an MFMA operand needs with one 16-byte LDS load; wave w owns the
@@ -386,14 +462,14 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int r16 = lane & 15, kq = lane >> 4;
-    const int t = blockIdx.y;
+    const int chunk = (int)blockIdx.x / 10, t = (int)blockIdx.x - chunk * 10;
     const bool bias = t == 9;
     const int dx = t % 3 - 1, dy = t / 3 - 1;
     const int tiles = bias ? MT : TILES;
     f32x4 acc[TPW];
 #pragma unroll
     for (int i = 0; i < TPW; ++i) acc[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const int p0 = blockIdx.x * kNnGradPixels;
+    const int p0 = chunk * kNnGradPixels;
     const int p1 = min(a.P, p0 + kNnGradPixels);
     const bf16x8 ones = __builtin_bit_cast(bf16x8, make_uint4(0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u));
     for (int s0 = p0; s0 < p1; s0 += kWgStep) {
@@ -450,7 +526,7 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
         }
         __syncthreads();
     }
-    float* dst = a.partial + (size_t)blockIdx.x * COUT * NWC;
+    float* dst = a.partial + (size_t)chunk * COUT * NWC;
 #pragma unroll
     for (int i = 0; i < TPW; ++i) {
         const int tile = wave + 4 * i;
@@ -518,24 +594,23 @@ __global__ void k_adam(int n, float* params, const float* grad, float* m, float*
     params[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// pack convolution l: forward bf16 [cout][kp] (k = tap * cin + ci), input-
-// gradient bf16 [cin][kp'] (k' = tap' * cout + co, value W[co][8 - tap'][ci]),
-// FP32 bias
+// pack convolution l: forward bf16 [cout][kp] (k = tap * stride(cin) + ci),
+// input-gradient bf16 [cin][kp'] (k' = tap' * stride(cout) + co, value
+// W[co][8 - tap'][ci]), padding zero; FP32 bias
 __global__ void k_pack_conv(const float* W, int cin, int cout, int kpf, int kpb, uint16_t* fwd,
     uint16_t* bwd, float* bias) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < cout * kpf) {
+        const int ts = kpf / 9;
         const int co = i / kpf, k = i - co * kpf;
-        fwd[i] = k < 9 * cin ? to_bf16(W[(size_t)co * 9 * cin + k]) : (uint16_t)0;
+        const int t = k / ts, ci = k - t * ts;
+        fwd[i] = ci < cin ? to_bf16(W[((size_t)co * 9 + t) * cin + ci]) : (uint16_t)0;
     }
     if (bwd && i < cin * kpb) {
+        const int ts = kpb / 9;
         const int ci = i / kpb, k = i - ci * kpb;
-        uint16_t v = 0;
-        if (k < 9 * cout) {
-            const int tp = k / cout, co = k - tp * cout;
-            v = to_bf16(W[((size_t)co * 9 + (8 - tp)) * cin + ci]);
-        }
-        bwd[i] = v;
+        const int tp = k / ts, co = k - tp * ts;
+        bwd[i] = co < cout ? to_bf16(W[((size_t)co * 9 + (8 - tp)) * cin + ci]) : (uint16_t)0;
     }
     if (i < cout) bias[i] = W[(size_t)cout * 9 * cin + i];
 }
@@ -555,6 +630,17 @@ void conv_launch(const ConvArgs& a, int grid, hipStream_t st) {
     const int groups = cdiv(a.P, 16 * conv_mt<COUT>());
     const int blocks = std::max(1, std::min(grid, cdiv(groups, 4)));
     k_uconv<C0, C1, COUT, MODE><<<blocks, 256, 0, st>>>(a);
+}
+
+// the convolutions address their inputs with 32-bit buffer offsets
+bool conv_inputs_fit(const NnLevels& lv) {
+    constexpr double kLimit = 2147479552.0;  // kBufOob
+    for (int l = 0; l < kNnConvs; ++l) {
+        const NnConv c = nn_conv(l);
+        const double P = (double)lv.lv[c.level].offset[lv.lv[c.level].count];
+        if (P * (double)std::max(nn_cin(l), c.cout) * 2.0 >= kLimit) return false;
+    }
+    return true;
 }
 
 void conv_forward(int l, const ConvArgs& a, int grid, hipStream_t st) {
@@ -600,8 +686,7 @@ void conv_input_grad(int l, const ConvArgs& a, int grid, hipStream_t st) {
 
 template <int C0, int C1, int COUT>
 void wgrad_launch(const WgradArgs& a, hipStream_t st) {
-    const dim3 grid((unsigned)cdiv(a.P, kNnGradPixels), 10u);
-    k_wgrad<C0, C1, COUT><<<grid, 256, 0, st>>>(a);
+    k_wgrad<C0, C1, COUT><<<cdiv(a.P, kNnGradPixels) * 10, 256, 0, st>>>(a);
 }
 
 void conv_wgrad(int l, const WgradArgs& a, hipStream_t st) {
@@ -645,6 +730,7 @@ ConvArgs conv_args(const NnLevels& lv, int l, const NnScratch& w) {
     a.im = lv.lv[c.level];
     a.pim = lv.lv[c.level < kNnLevels - 1 ? c.level + 1 : c.level];
     a.P = level_pixels(a.im);
+    a.Pp = level_pixels(a.pim);
     conv_inputs(l, w, &a.A, &a.B);
     return a;
 }
@@ -698,6 +784,7 @@ hipError_t nn_forward(const NnLevels& lv, const NnPacked& pk, const NnScratch& w
     const float4* target, float4* out, int conv_grid, hipStream_t st) {
     const int P = level_pixels(lv.lv[0]);
     if (P <= 0) return hipSuccess;
+    if (!conv_inputs_fit(lv)) return hipErrorInvalidValue;
     k_features<<<cdiv(P, 256), 256, 0, st>>>(P, beauty, normal, albedo, depth, w.feat);
     for (int l = 0; l < kNnConvs; ++l) {
         ConvArgs a = conv_args(lv, l, w);
@@ -711,7 +798,7 @@ hipError_t nn_forward(const NnLevels& lv, const NnPacked& pk, const NnScratch& w
             const int lvl = nn_conv(l).level;
             const int C = nn_conv(l).cout;
             const int Pp = level_pixels(lv.lv[lvl + 1]);
-            k_pool<<<cdiv(Pp * C, 256), 256, 0, st>>>(lv.lv[lvl], lv.lv[lvl + 1], Pp, C, w.act[l],
+            k_pool<<<cdiv(Pp * (C / 8), 256), 256, 0, st>>>(lv.lv[lvl], lv.lv[lvl + 1], Pp, C, w.act[l],
                 w.pool[lvl + 1]);
         }
     }
@@ -747,6 +834,7 @@ hipError_t nn_backward(const NnLevels& lv, const NnPacked& pk, const NnScratch& 
         a.im = im;
         a.pim = im;
         a.P = Pl;
+        a.Pp = Pl;
         a.A = w.dz[l];
         a.B = nullptr;
         a.W = pk.bwd[l];
