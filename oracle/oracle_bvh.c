@@ -744,14 +744,23 @@ void rto_brute_closest(const rto_scene* s, const rto_ray* rays, int n, rto_hit* 
     }
 }
 
-/* internal entry used by the renderer */
-int om_closest(const rto_scene* s, const rto_bvh* bvh, v3 o, v3 d, float tmin, float tmax,
-    float* t, int* prim, float* u, float* v) {
+/* internal entries used by the renderer; the counted form adds the
+ * traversal's child-box and primitive tests to *visits / *tests */
+int om_closest_counted(const rto_scene* s, const rto_bvh* bvh, v3 o, v3 d, float tmin, float tmax,
+    float* t, int* prim, float* u, float* v, uint64_t* visits, uint64_t* tests) {
     rto_ray r = {{o.x, o.y, o.z}, tmin, {d.x, d.y, d.z}, tmax};
     best_hit b;
-    traverse(s, bvh, &r, 0, &b, NULL, NULL);
+    uint32_t nv = 0, nt = 0;
+    traverse(s, bvh, &r, 0, &b, &nv, &nt);
+    if (visits) *visits += nv;
+    if (tests) *tests += nt;
     *t = b.t; *prim = b.prim; *u = b.u; *v = b.v;
     return b.found;
+}
+
+int om_closest(const rto_scene* s, const rto_bvh* bvh, v3 o, v3 d, float tmin, float tmax,
+    float* t, int* prim, float* u, float* v) {
+    return om_closest_counted(s, bvh, o, d, tmin, tmax, t, prim, u, v, NULL, NULL);
 }
 
 int om_occluded(const rto_scene* s, const rto_bvh* bvh, v3 o, v3 d, float tmin, float tmax) {

@@ -126,7 +126,8 @@ class Generated(C.Structure):
 
 
 class RenderStats(C.Structure):
-    _fields_ = [("paths", C.c_uint64), ("rays_closest", C.c_uint64), ("rays_shadow", C.c_uint64)]
+    _fields_ = [("paths", C.c_uint64), ("rays_closest", C.c_uint64), ("rays_shadow", C.c_uint64),
+                ("node_visits", C.c_uint64), ("prim_tests", C.c_uint64)]
 
 
 class Reservoir(C.Structure):
@@ -521,7 +522,8 @@ def render_camera(scene: Scene, bvh: OracleBvh, cam: Camera, cam_index: int, pro
     if rc != 0:
         raise RuntimeError("oracle render failed")
     return {"beauty": beauty, "normal": normal, "albedo": albedo, "depth": depth,
-            "paths": st.paths, "rays_closest": st.rays_closest, "rays_shadow": st.rays_shadow}
+            "paths": st.paths, "rays_closest": st.rays_closest, "rays_shadow": st.rays_shadow,
+            "node_visits": st.node_visits, "prim_tests": st.prim_tests}
 
 
 def generate_rig(config: int, seed: int, frame: int, frames: int):

@@ -7,6 +7,8 @@
 
 int om_closest(const rto_scene* s, const rto_bvh* bvh, v3 o, v3 d, float tmin, float tmax,
     float* t, int* prim, float* u, float* v);
+int om_closest_counted(const rto_scene* s, const rto_bvh* bvh, v3 o, v3 d, float tmin, float tmax,
+    float* t, int* prim, float* u, float* v, uint64_t* visits, uint64_t* tests);
 int om_occluded(const rto_scene* s, const rto_bvh* bvh, v3 o, v3 d, float tmin, float tmax);
 
 v3 om_openpbr_emission(const rto_openpbr* m);

@@ -1189,7 +1189,8 @@ static void trace_one(const ctx_t* c, const rto_camera* cam, int cam_index, cons
         float t, u, v;
         int prim;
         if (stats) stats->rays_closest++;
-        const int found = om_closest(s, c->bvh, st.o, st.d, OM_RAY_EPS, OM_RAY_FAR, &t, &prim, &u, &v);
+        const int found = om_closest_counted(s, c->bvh, st.o, st.d, OM_RAY_EPS, OM_RAY_FAR, &t, &prim,
+            &u, &v, stats ? &stats->node_visits : NULL, stats ? &stats->prim_tests : NULL);
         int ali = -1;
         if (c->a_count > 0 && intersect_analytic(c, st.o, st.d, OM_RAY_EPS, found ? t : OM_RAY_FAR, &ali)) {
             const float w = analytic_mis(c, &st, ali);
@@ -1260,7 +1261,7 @@ static void* render_worker(void* arg) {
     render_job* j = (render_job*)arg;
     const rto_camera* cam = j->cam;
     const int spp = j->pr->samples_per_pixel > 0 ? j->pr->samples_per_pixel : 1;
-    rto_render_stats local = {0, 0, 0};
+    rto_render_stats local = {0, 0, 0, 0, 0};
     while (1) {
         pthread_mutex_lock(&j->lock);
         const int y = j->next_row++;
@@ -1306,6 +1307,8 @@ static void* render_worker(void* arg) {
     j->stats.paths += local.paths;
     j->stats.rays_closest += local.rays_closest;
     j->stats.rays_shadow += local.rays_shadow;
+    j->stats.node_visits += local.node_visits;
+    j->stats.prim_tests += local.prim_tests;
     pthread_mutex_unlock(&j->lock);
     return NULL;
 }

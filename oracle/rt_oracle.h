@@ -243,7 +243,13 @@ int rto_build_lights(const rto_scene* scene, rto_light* out, int capacity);
 
 /* full render of one camera; outputs are W*H*4 floats (beauty, normal,
  * albedo) and W*H floats (depth).  Any output may be NULL. */
-typedef struct { uint64_t paths; uint64_t rays_closest; uint64_t rays_shadow; } rto_render_stats;
+/* node_visits / prim_tests: child boxes tested and primitives tested by the
+ * path-extension (closest-hit) traversals, the counts the GPU's counting
+ * frame reports */
+typedef struct {
+    uint64_t paths; uint64_t rays_closest; uint64_t rays_shadow;
+    uint64_t node_visits; uint64_t prim_tests;
+} rto_render_stats;
 int rto_render_camera(const rto_scene* scene, const rto_bvh* bvh, const rto_camera* cam,
     int cam_index, const rto_profile* profile, uint32_t stream, float* beauty, float* normal,
     float* albedo, float* depth, int threads, int row_begin, int row_end,

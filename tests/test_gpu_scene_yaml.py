@@ -21,7 +21,7 @@ def say(msg):
     sys.stdout.flush()
 
 
-def render_yaml(rt, oracle, device, name, preset, spp, depth):
+def render_yaml(rt, oracle, device, name, preset, spp, depth, count=False):
     from ray_tracing_amd import scene_yaml as S
     d = S.load_scene_definition(os.path.join(SCENES, name, "scene.yaml"))
     flat = S.flatten(d)
@@ -29,10 +29,13 @@ def render_yaml(rt, oracle, device, name, preset, spp, depth):
     sc = rt.Scene(device, flat.desc, keepalive=flat)
     sc.build_bvh()
     ren = rt.Renderer(device, sc)
+    if count:
+        ren.set_tuning(count_traversal=1, use_graph=0)
     prof = rt.Generated("cornell_mesh").profile
     prof.samples_per_pixel, prof.max_bounces, prof.rr_start_bounce = spp, depth, depth
-    ren.render(rig, prof, 3)
+    tm = ren.render(rig, prof, 3)
     gpu = ren.download(0)
+    gpu["timing"] = tm
     osc = oracle_scene(oracle, flat.desc)
     ob = oracle.OracleBvh(osc)
     ref = oracle.render_camera(osc, ob, oracle_camera(oracle, rig.cameras[0]), 0,
@@ -64,3 +67,20 @@ def test_cornell_box_yaml_full_paths_match_oracle(rt, oracle, device):
     e = rel_mse(a, b)
     say(f"  beauty bit-exact fraction {same:.6f}, relMSE {e:.2e}")
     assert same > 0.99 and e < 1e-4
+
+
+def test_traversal_counts_equal_the_oracles(rt, oracle, device):
+    """The counting frame's child-box and primitive-test totals (the inputs of
+    bench.py's algorithmic bytes) equal the oracle's for the same rays: the
+    render is bit-exact, so every path-extension ray is the same and the
+    traversal order is shared."""
+    say("[gpu] cornell_box.yaml 1280x720, 2 spp, depth 8, counting traversal")
+    gpu, ref = render_yaml(rt, oracle, device, "cornell_box", "default", 2, 8, count=True)
+    tm = gpu["timing"]
+    say(f"  GPU visits {tm.node_visits} tests {tm.prim_tests}; oracle visits {ref['node_visits']} "
+        f"tests {ref['prim_tests']}; extend rays {tm.extend_rays} vs {ref['rays_closest']}")
+    assert np.mean(gpu["beauty"] == ref["beauty"]) == 1.0
+    assert tm.extend_rays == ref["rays_closest"]
+    assert tm.node_visits == ref["node_visits"] and tm.prim_tests == ref["prim_tests"]
+    assert tm.node_visits > 10 * tm.extend_rays   # a real traversal, not a stub
+
