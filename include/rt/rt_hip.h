@@ -439,7 +439,10 @@ rt_status rt_renderer_set_radiance_cache(rt_renderer* r, rt_cache* cache);
  * Seeded generators of the benchmark scenes (this repository's equivalent
  * of build_scene(id) + make_rig, src/scene/shared_scene_builders.h:56-57):
  *   0 random_spheres, 1 cornell_mesh, 2 instanced_mesh_restir,
- *   3 radiance_cache_realtime, 4 cornell_random (seeded denoiser training scenes) */
+ *   3 radiance_cache_realtime, 4 cornell_random (seeded denoiser training scenes),
+ *   5 smoke, 6 final_room (the reference's builtin realtime scenes of those
+ *   names, src/scene/shared_scene_builders.cpp:475-550, with its four-camera
+ *   surround rig, src/realtime/viewer/four_camera_rig.cpp:55-95) */
 typedef struct rt_generated_scene {
     rt_scene_desc scene;
     int32_t camera_count;

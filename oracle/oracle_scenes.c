@@ -525,6 +525,122 @@ static int gen_instanced_mesh(rto_generated* g, uint64_t seed) {
     return 0;
 }
 
+/* four-camera surround rig of a realtime view preset (world z up): the body
+ * camera at yaw offsets 0, 90, -90, 180 degrees
+ * (src/realtime/viewer/four_camera_rig.cpp:55-95, body_pose.cpp:36-85) */
+static void surround_rig(const double pos[3], double yaw_deg, double pitch_deg, double fx, double fy,
+    double cx, double cy, int w, int h, rto_camera* cams) {
+    const double offsets[4] = {0.0, 90.0, -90.0, 180.0};
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < 4; ++i) {
+        rto_camera* c = &cams[i];
+        memset(c, 0, sizeof(*c));
+        c->enabled = 1; c->width = w; c->height = h; c->model = RTO_CAM_PINHOLE32;
+        const double yaw = (yaw_deg + offsets[i]) * pi / 180.0;
+        const double pitch = -pitch_deg * pi / 180.0;
+        /* Rx(yaw) Ry(-pitch) (0, 0, -1), then the body basis columns
+         * (0,0,1), (-1,0,0), (0,-1,0) */
+        const double b[3] = {-sin(pitch), sin(yaw) * cos(pitch), -cos(yaw) * cos(pitch)};
+        double f[3] = {-b[1], -b[2], b[0]};
+        const double fl = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        for (int k = 0; k < 3; ++k) f[k] /= fl;
+        double r[3] = {f[1], -f[0], 0.0};   /* forward x up */
+        const double rl = sqrt(r[0] * r[0] + r[1] * r[1]);
+        if (rl * rl < 1e-12) {
+            r[0] = cos(yaw); r[1] = sin(yaw); r[2] = 0.0;
+        } else {
+            for (int k = 0; k < 3; ++k) r[k] /= rl;
+        }
+        double u[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0]};
+        const double ul = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        for (int k = 0; k < 3; ++k) u[k] /= ul;
+        for (int k = 0; k < 3; ++k) {
+            c->origin[k] = pos[k]; c->basis_x[k] = r[k]; c->basis_y[k] = -u[k]; c->basis_z[k] = f[k];
+        }
+        c->fx = fx; c->fy = fy; c->cx = cx; c->cy = cy;
+    }
+}
+
+/* legacy renderer axes to world: (x, y, z) -> (x, -z, y) */
+static void legacy_quad(rto_quad* q, double ox, double oy, double oz, double ux, double uy, double uz,
+    double vx, double vy, double vz, int mat) {
+    quad_set(q, (float)ox, (float)-oz, (float)oy, (float)ux, (float)-uz, (float)uy, (float)vx,
+        (float)-vz, (float)vy, mat);
+}
+
+static void legacy_sphere(rto_sphere* s, double x, double y, double z, double radius, int mat) {
+    set3(s->center, (float)x, (float)-z, (float)y);
+    s->radius = (float)radius;
+    s->material = mat;
+}
+
+static rto_material* builtin_materials(int n) {
+    rto_material* mt = (rto_material*)calloc((size_t)n, sizeof(rto_material));
+    for (int i = 0; i < n; ++i) { mt[i].openpbr_index = -1; mt[i].ior = 1.0f; }
+    return mt;
+}
+
+static void finish_builtin(rto_generated* g, rto_sphere* sp, int ns, rto_quad* q, int nq,
+    rto_material* mt, int nm) {
+    g->scene.sphere_count = ns; g->scene.spheres = sp;
+    g->scene.quad_count = nq; g->scene.quads = q;
+    g->scene.material_count = nm; g->scene.materials = mt;
+    g->scene.background_mode = RTO_BG_CONSTANT;
+    set3(g->scene.background, 0.0f, 0.0f, 0.0f);
+    g->camera_count = 4;
+    profile_defaults(&g->profile, 16, 8);
+}
+
+/* config 5: the reference's builtin "smoke" scene (shared_scene_builders.cpp:475-487)
+ * and view (body at (0, -2.5, 0.25), yaw 0, fx = fy = 360) */
+static int gen_smoke(rto_generated* g) {
+    rto_material* mt = builtin_materials(2);
+    set3(mt[0].albedo, 0.75f, 0.25f, 0.2f);
+    mt[1].type = RTO_MAT_DIFFUSE_LIGHT; set3(mt[1].emission, 10.0f, 10.0f, 10.0f);
+    rto_sphere* sp = (rto_sphere*)calloc(1, sizeof(rto_sphere));
+    legacy_sphere(&sp[0], 0.0, 0.0, -1.0, 0.5, 0);
+    rto_quad* q = (rto_quad*)calloc(1, sizeof(rto_quad));
+    legacy_quad(&q[0], -0.75, 1.25, -1.5, 1.5, 0.0, 0.0, 0.0, 0.0, 1.5, 1);
+    finish_builtin(g, sp, 1, q, 1, mt, 2);
+    const double pos[3] = {0.0, -2.5, 0.25};
+    surround_rig(pos, 0.0, 0.0, 360.0, 360.0, 320.0, 240.0, 640, 480, g->cameras);
+    return 0;
+}
+
+/* config 6: the reference's builtin "final_room" (shared_scene_builders.cpp:489-550)
+ * and view (default spawn pose (0, -0.8, 0.35), fx = 480, fy = 360) */
+static int gen_final_room(rto_generated* g) {
+    rto_material* mt = builtin_materials(5);
+    set3(mt[0].albedo, 0.73f, 0.73f, 0.73f);
+    set3(mt[1].albedo, 0.30f, 0.70f, 0.35f);
+    set3(mt[2].albedo, 0.72f, 0.25f, 0.22f);
+    set3(mt[3].albedo, 0.25f, 0.35f, 0.75f);
+    mt[4].type = RTO_MAT_DIFFUSE_LIGHT; set3(mt[4].emission, 10.0f, 10.0f, 10.0f);
+    rto_quad* q = (rto_quad*)calloc(9, sizeof(rto_quad));
+    legacy_quad(&q[0], -4.0, -1.0, -4.0, 0.0, 0.0, 8.0, 8.0, 0.0, 0.0, 0);
+    legacy_quad(&q[1], -4.0, 3.5, -4.0, 8.0, 0.0, 0.0, 0.0, 0.0, 8.0, 0);
+    legacy_quad(&q[2], -4.0, -1.0, -4.0, 0.0, 4.5, 0.0, 0.0, 0.0, 8.0, 1);
+    legacy_quad(&q[3], 4.0, -1.0, -4.0, 0.0, 0.0, 8.0, 0.0, 4.5, 0.0, 2);
+    legacy_quad(&q[4], -4.0, -1.0, -4.0, 8.0, 0.0, 0.0, 0.0, 4.5, 0.0, 0);
+    legacy_quad(&q[5], -4.0, -1.0, 4.0, 0.0, 4.5, 0.0, 8.0, 0.0, 0.0, 3);
+    legacy_quad(&q[6], -1.0, 3.15, -1.0, 2.0, 0.0, 0.0, 0.0, 0.0, 2.0, 4);
+    legacy_quad(&q[7], -3.2, -0.25, -3.0, 0.0, 0.0, 1.8, 1.8, 0.0, 0.0, 0);
+    legacy_quad(&q[8], 1.2, 0.15, 1.0, 0.0, 0.0, 1.6, 1.6, 0.0, 0.0, 0);
+    rto_sphere* sp = (rto_sphere*)calloc(8, sizeof(rto_sphere));
+    legacy_sphere(&sp[0], 0.0, 0.25, -1.2, 0.55, 0);
+    legacy_sphere(&sp[1], -1.6, 0.35, 1.7, 0.55, 0);
+    legacy_sphere(&sp[2], -3.1, 1.0, 0.8, 0.55, 0);
+    legacy_sphere(&sp[3], 3.0, 1.35, -0.9, 0.65, 0);
+    legacy_sphere(&sp[4], 1.1, 1.1, -3.0, 0.60, 0);
+    legacy_sphere(&sp[5], -0.8, 2.55, 2.2, 0.45, 0);
+    legacy_sphere(&sp[6], 0.9, 0.55, -0.1, 0.45, 0);
+    legacy_sphere(&sp[7], -0.35, 0.4, -1.15, 0.35, 0);
+    finish_builtin(g, sp, 8, q, 9, mt, 5);
+    const double pos[3] = {0.0, -0.8, 0.35};
+    surround_rig(pos, 0.0, 0.0, 480.0, 360.0, 320.0, 240.0, 640, 480, g->cameras);
+    return 0;
+}
+
 int rto_generate_scene(int config, uint64_t seed, rto_generated* g) {
     memset(g, 0, sizeof(*g));
     switch (config) {
@@ -532,6 +648,8 @@ int rto_generate_scene(int config, uint64_t seed, rto_generated* g) {
         case 1: return gen_cornell_mesh(g, seed);
         case 2: return gen_instanced_mesh(g, seed);
         case 4: return gen_cornell_random(g, seed);
+        case 5: return gen_smoke(g);
+        case 6: return gen_final_room(g);
         case 3: {
             /* the config-2 scene, four 1280x720 cameras, paths of depth 8 with
              * Russian roulette from bounce 3 (the radiance-cache workload) */

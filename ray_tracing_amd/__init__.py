@@ -389,7 +389,7 @@ class Generated:
     """A seeded benchmark scene from the library's generator (rt_scene_generate)."""
 
     CONFIGS = {"random_spheres": 0, "cornell_mesh": 1, "instanced_mesh_restir": 2,
-               "radiance_cache_realtime": 3, "cornell_random": 4}
+               "radiance_cache_realtime": 3, "cornell_random": 4, "smoke": 5, "final_room": 6}
 
     def __init__(self, config, seed: int = 0):
         if isinstance(config, str):

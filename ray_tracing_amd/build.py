@@ -80,13 +80,36 @@ HOST_TARGETS = {"render_realtime": ["render_realtime.cpp", "rt_host.cpp"],
                 "test_report": ["test_report.cpp", "rt_host.cpp"]}
 
 
+def source_provenance():
+    """(revision, SHA-256 of the source state) for the run reports: the git
+    commit when the tree is a checkout, and a digest over the library's,
+    host layer's and public header's source files (path and contents, in
+    sorted order), which also covers uncommitted changes."""
+    import hashlib
+    root = os.path.dirname(HERE)
+    r = subprocess.run(["git", "-C", root, "rev-parse", "HEAD"], stdout=subprocess.PIPE,
+                       stderr=subprocess.DEVNULL, text=True)
+    revision = r.stdout.strip() if r.returncode == 0 and r.stdout.strip() else "unversioned"
+    h = hashlib.sha256()
+    for d in (CSRC, HOST, os.path.join(root, "include", "rt")):
+        for name in sorted(os.listdir(d)):
+            path = os.path.join(d, name)
+            if os.path.isfile(path) and os.path.splitext(name)[1] in (".h", ".hpp", ".hip", ".cpp"):
+                h.update(os.path.relpath(path, root).encode())
+                with open(path, "rb") as f:
+                    h.update(f.read())
+    return revision, h.hexdigest()
+
+
 def build_host(verbose: bool = True):
     """C++ host API, the render_realtime CLI and the report test, linked
     against librt_hip.so (found next to the executables' parent directory)."""
     outs = []
+    revision, state = source_provenance()
     for name, srcs in HOST_TARGETS.items():
         out = os.path.join(HERE, name)
-        cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-o", out] + [os.path.join(HOST, s) for s in srcs]
+        cmd = ["g++", "-std=c++20", "-O2", "-Wall", f'-DRT_SOURCE_REVISION="{revision}"',
+               f'-DRT_SOURCE_STATE_SHA256="{state}"', "-o", out] + [os.path.join(HOST, s) for s in srcs]
         cmd += ["-L", HERE, "-lrt_hip", "-Wl,-rpath,$ORIGIN"]
         _run(cmd)
         outs.append(out)

@@ -16,6 +16,13 @@
 //                             equi62_lut1d cameras at 1280x720, depth 8
 //   4 cornell_random        : seeded Cornell rooms with 1-8 spheres, random
 //                             materials and light (denoiser training scenes)
+//   5 smoke, 6 final_room   : the reference's builtin realtime scenes of the
+//                             same names (make_realtime_smoke_scene /
+//                             make_final_room_scene,
+//                             src/scene/shared_scene_builders.cpp:475-550)
+//                             with its four-camera surround rig
+//                             (make_default_viewer_rig,
+//                             src/realtime/viewer/four_camera_rig.cpp:55-95)
 // Draws come from PCG32 (O'Neill 2014; pcg32_srandom_r(seed, 54), XSH-RR
 // output); a float draw is (x >> 8) * 2^-24; the draw order is part of the
 // scene definition.
@@ -567,6 +574,134 @@ void make_lookat_equi(rt_camera* c, int w, int h, const double radial[6],
     make_equi_lut(c);
 }
 
+// The reference's surround rig for a realtime view preset: the body pose's
+// camera at yaw offsets 0, 90, -90, 180 degrees (kDefaultSurroundYawOffsetsDeg,
+// src/realtime/default_viewer_conventions.h:7); per camera the axes are
+// right, -up, forward of the yawed body (forward_direction / right_direction,
+// src/realtime/viewer/body_pose.cpp:36-85, world z up), at the body position.
+void surround_rig(const double pos[3], double yaw_deg, double pitch_deg, double fx, double fy, double cx,
+    double cy, int w, int h, rt_camera* cams) {
+    const double offsets[4] = {0.0, 90.0, -90.0, 180.0};
+    const double kPi = 3.14159265358979323846;
+    for (int i = 0; i < 4; ++i) {
+        rt_camera* c = &cams[i];
+        std::memset(c, 0, sizeof(*c));
+        c->enabled = 1;
+        c->width = w;
+        c->height = h;
+        c->model = RT_CAM_PINHOLE32;
+        const double yaw = (yaw_deg + offsets[i]) * kPi / 180.0;
+        const double pitch = -pitch_deg * kPi / 180.0;
+        // body rotation Rx(yaw) * Ry(-pitch) applied to (0, 0, -1)
+        const double b[3] = {-std::sin(pitch), std::sin(yaw) * std::cos(pitch),
+            -std::cos(yaw) * std::cos(pitch)};
+        // body basis (world z up): columns up (0,0,1), -right (-1,0,0), -forward (0,-1,0)
+        double f[3] = {-b[1], -b[2], b[0]};
+        const double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        for (double& v : f) v /= fl;
+        // right = forward x up
+        double r[3] = {f[1], -f[0], 0.0};
+        const double rl = std::sqrt(r[0] * r[0] + r[1] * r[1]);
+        if (rl * rl < 1e-12) {
+            // looking straight up or down: the rotated body -y axis, (cos yaw, sin yaw, 0)
+            r[0] = std::cos(yaw);
+            r[1] = std::sin(yaw);
+            r[2] = 0.0;
+        } else {
+            for (double& v : r) v /= rl;
+        }
+        double u[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0]};
+        const double ul = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        for (double& v : u) v /= ul;
+        for (int k = 0; k < 3; ++k) {
+            c->origin[k] = pos[k];
+            c->basis_x[k] = r[k];
+            c->basis_y[k] = -u[k];
+            c->basis_z[k] = f[k];
+        }
+        c->fx = fx;
+        c->fy = fy;
+        c->cx = cx;
+        c->cy = cy;
+    }
+}
+
+// legacy_renderer_to_world (src/realtime/frame_convention.h:35-37): (x, y, z) -> (x, -z, y)
+rt_quad legacy_quad(double ox, double oy, double oz, double ux, double uy, double uz, double vx,
+    double vy, double vz, int mat) {
+    return quad((float)ox, (float)-oz, (float)oy, (float)ux, (float)-uz, (float)uy, (float)vx,
+        (float)-vz, (float)vy, mat);
+}
+
+rt_sphere legacy_sphere(double x, double y, double z, double radius, int mat) {
+    rt_sphere sp{};
+    set3(sp.center, (float)x, (float)-z, (float)y);
+    sp.radius = (float)radius;
+    sp.material = mat;
+    return sp;
+}
+
+rt_material emissive(float r, float g, float b) {
+    rt_material m{};
+    m.type = RT_MAT_DIFFUSE_LIGHT;
+    set3(m.emission, r, g, b);
+    m.ior = 1.0f;
+    m.openpbr_index = -1;
+    return m;
+}
+
+void finish_builtin(rt_generated_scene* g, const std::vector<rt_sphere>& sp,
+    const std::vector<rt_quad>& q, const std::vector<rt_material>& mt) {
+    g->scene.sphere_count = static_cast<int32_t>(sp.size());
+    g->scene.spheres = sp.empty() ? nullptr : copy_out(sp);
+    g->scene.quad_count = static_cast<int32_t>(q.size());
+    g->scene.quads = copy_out(q);
+    g->scene.material_count = static_cast<int32_t>(mt.size());
+    g->scene.materials = copy_out(mt);
+    g->scene.background_mode = RT_BG_CONSTANT;
+    set3(g->scene.background, 0.0f, 0.0f, 0.0f);
+    g->camera_count = 4;
+    g->profile = default_profile(16, 8);
+}
+
+// make_realtime_smoke_scene (shared_scene_builders.cpp:475-487) and its view
+// preset: body at (0, -2.5, 0.25), yaw 0, pinhole 640x480, fx = fy = 360
+void generate_smoke(rt_generated_scene* g) {
+    const std::vector<rt_material> mt = {lambert(0.75f, 0.25f, 0.2f), emissive(10.0f, 10.0f, 10.0f)};
+    const std::vector<rt_sphere> sp = {legacy_sphere(0.0, 0.0, -1.0, 0.5, 0)};
+    const std::vector<rt_quad> q = {legacy_quad(-0.75, 1.25, -1.5, 1.5, 0.0, 0.0, 0.0, 0.0, 1.5, 1)};
+    finish_builtin(g, sp, q, mt);
+    const double pos[3] = {0.0, -2.5, 0.25};
+    surround_rig(pos, 0.0, 0.0, 360.0, 360.0, 320.0, 240.0, 640, 480, g->cameras);
+}
+
+// make_final_room_scene (shared_scene_builders.cpp:489-550) and its view
+// preset: the default spawn pose (0, -0.8, 0.35), yaw 0, pinhole 640x480,
+// fx = 0.75 * 640, fy = 0.75 * 480
+void generate_final_room(rt_generated_scene* g) {
+    const std::vector<rt_material> mt = {lambert(0.73f, 0.73f, 0.73f), lambert(0.30f, 0.70f, 0.35f),
+        lambert(0.72f, 0.25f, 0.22f), lambert(0.25f, 0.35f, 0.75f), emissive(10.0f, 10.0f, 10.0f)};
+    const int white = 0, green = 1, red = 2, blue = 3, light = 4;
+    const std::vector<rt_quad> q = {
+        legacy_quad(-4.0, -1.0, -4.0, 0.0, 0.0, 8.0, 8.0, 0.0, 0.0, white),
+        legacy_quad(-4.0, 3.5, -4.0, 8.0, 0.0, 0.0, 0.0, 0.0, 8.0, white),
+        legacy_quad(-4.0, -1.0, -4.0, 0.0, 4.5, 0.0, 0.0, 0.0, 8.0, green),
+        legacy_quad(4.0, -1.0, -4.0, 0.0, 0.0, 8.0, 0.0, 4.5, 0.0, red),
+        legacy_quad(-4.0, -1.0, -4.0, 8.0, 0.0, 0.0, 0.0, 4.5, 0.0, white),
+        legacy_quad(-4.0, -1.0, 4.0, 0.0, 4.5, 0.0, 8.0, 0.0, 0.0, blue),
+        legacy_quad(-1.0, 3.15, -1.0, 2.0, 0.0, 0.0, 0.0, 0.0, 2.0, light),
+        legacy_quad(-3.2, -0.25, -3.0, 0.0, 0.0, 1.8, 1.8, 0.0, 0.0, white),
+        legacy_quad(1.2, 0.15, 1.0, 0.0, 0.0, 1.6, 1.6, 0.0, 0.0, white)};
+    const std::vector<rt_sphere> sp = {legacy_sphere(0.0, 0.25, -1.2, 0.55, white),
+        legacy_sphere(-1.6, 0.35, 1.7, 0.55, white), legacy_sphere(-3.1, 1.0, 0.8, 0.55, white),
+        legacy_sphere(3.0, 1.35, -0.9, 0.65, white), legacy_sphere(1.1, 1.1, -3.0, 0.60, white),
+        legacy_sphere(-0.8, 2.55, 2.2, 0.45, white), legacy_sphere(0.9, 0.55, -0.1, 0.45, white),
+        legacy_sphere(-0.35, 0.4, -1.15, 0.35, white)};
+    finish_builtin(g, sp, q, mt);
+    const double pos[3] = {0.0, -0.8, 0.35};
+    surround_rig(pos, 0.0, 0.0, 480.0, 360.0, 320.0, 240.0, 640, 480, g->cameras);
+}
+
 bool generate_rig(int config, uint64_t seed, int frame, int frames, rt_camera_rig* out) {
     std::memset(out, 0, sizeof(*out));
     if (config == 2) {
@@ -679,6 +814,8 @@ bool generate_scene(int config, uint64_t seed, rt_generated_scene* g) {
             g->profile.rr_start_bounce = 3;
             return true;
         case 4: generate_cornell_random(seed, g); return true;
+        case 5: generate_smoke(g); return true;
+        case 6: generate_final_room(g); return true;
         default: return false;
     }
 }

@@ -25,6 +25,12 @@
 namespace {
 
 const char* kVersion = "ray_tracing_amd 0.1";
+#ifndef RT_SOURCE_REVISION
+#define RT_SOURCE_REVISION "unversioned"
+#endif
+#ifndef RT_SOURCE_STATE_SHA256
+#define RT_SOURCE_STATE_SHA256 "0"
+#endif
 
 struct Args {
     int camera_count = 4;
@@ -32,7 +38,7 @@ struct Args {
     int warmup_frames = 2;
     unsigned seed = 0;
     std::string output_dir = "render_realtime_out";
-    std::string scene = "cornell_mesh";
+    std::string scene = "smoke";
     std::string profile;
     bool skip_image_write = false;
     std::string denoiser_weights;
@@ -44,6 +50,9 @@ int scene_config(const std::string& name) {
     if (name == "random_spheres") return 0;
     if (name == "cornell_mesh") return 1;
     if (name == "instanced_mesh_restir") return 2;
+    if (name == "radiance_cache_realtime") return 3;
+    if (name == "smoke") return 5;
+    if (name == "final_room") return 6;
     return -1;
 }
 
@@ -113,16 +122,81 @@ std::vector<float> load_npy(const std::string& path) {
     return v;
 }
 
-void write_ppm(const std::string& path, const rt_amd::RadianceFrame& f) {
-    std::ofstream out(path, std::ios::binary);
-    out << "P6\n" << f.width << " " << f.height << "\n255\n";
-    for (size_t i = 0; i < (size_t)f.width * (size_t)f.height; ++i) {
-        for (int c = 0; c < 3; ++c) {
-            float v = f.beauty_rgba[4 * i + c];
-            v = v <= 0.0f ? 0.0f : (v >= 1.0f ? 1.0f : v);
-            out.put((char)(unsigned char)(255.0f * std::pow(v, 1.0f / 2.2f) + 0.5f));
+// PNG (RFC 2083) of the display-encoded beauty image: 8-bit RGB, filter 0
+// per row, zlib stream of stored (uncompressed) deflate blocks, CRC-32 per
+// chunk and Adler-32 over the raw rows.
+uint32_t crc32_update(uint32_t crc, const unsigned char* p, size_t n) {
+    static uint32_t table[256];
+    static bool init = false;
+    if (!init) {
+        for (uint32_t i = 0; i < 256; ++i) {
+            uint32_t c = i;
+            for (int k = 0; k < 8; ++k) c = (c & 1u) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+            table[i] = c;
+        }
+        init = true;
+    }
+    for (size_t i = 0; i < n; ++i) crc = table[(crc ^ p[i]) & 0xFFu] ^ (crc >> 8);
+    return crc;
+}
+
+void put_be32(std::string& s, uint32_t v) {
+    for (int k = 3; k >= 0; --k) s.push_back((char)((v >> (8 * k)) & 0xFFu));
+}
+
+void png_chunk(std::ofstream& out, const char* type, const std::string& data) {
+    std::string len;
+    put_be32(len, (uint32_t)data.size());
+    out << len;
+    std::string body = std::string(type, 4) + data;
+    out << body;
+    std::string crc;
+    put_be32(crc, crc32_update(0xFFFFFFFFu, reinterpret_cast<const unsigned char*>(body.data()), body.size())
+        ^ 0xFFFFFFFFu);
+    out << crc;
+}
+
+void write_png(const std::string& path, const rt_amd::RadianceFrame& f) {
+    std::string raw;
+    raw.reserve((size_t)f.height * (1 + 3 * (size_t)f.width));
+    for (int y = 0; y < f.height; ++y) {
+        raw.push_back(0);
+        for (int x = 0; x < f.width; ++x) {
+            const size_t i = (size_t)y * (size_t)f.width + (size_t)x;
+            for (int c = 0; c < 3; ++c) {
+                float v = f.beauty_rgba[4 * i + c];
+                v = v <= 0.0f ? 0.0f : (v >= 1.0f ? 1.0f : v);
+                raw.push_back((char)(unsigned char)(255.0f * std::pow(v, 1.0f / 2.2f) + 0.5f));
+            }
         }
     }
+    std::string z = {(char)0x78, (char)0x01};
+    uint32_t a = 1, b = 0;
+    for (unsigned char ch : raw) {
+        a = (a + ch) % 65521u;
+        b = (b + a) % 65521u;
+    }
+    for (size_t off = 0; off < raw.size() || off == 0; off += 65535) {
+        const size_t n = std::min<size_t>(65535, raw.size() - off);
+        const bool last = off + n >= raw.size();
+        z.push_back(last ? 1 : 0);
+        z.push_back((char)(n & 0xFFu));
+        z.push_back((char)(n >> 8));
+        z.push_back((char)(~n & 0xFFu));
+        z.push_back((char)((~n >> 8) & 0xFFu));
+        z.append(raw, off, n);
+        if (last) break;
+    }
+    put_be32(z, (b << 16) | a);
+    std::string ihdr;
+    put_be32(ihdr, (uint32_t)f.width);
+    put_be32(ihdr, (uint32_t)f.height);
+    ihdr += std::string("\x08\x02\x00\x00\x00", 5);
+    std::ofstream out(path, std::ios::binary);
+    out << std::string("\x89PNG\r\n\x1a\n", 8);
+    png_chunk(out, "IHDR", ihdr);
+    png_chunk(out, "IDAT", z);
+    png_chunk(out, "IEND", "");
 }
 
 std::string utc_now() {
@@ -222,6 +296,8 @@ int main(int argc, char** argv) {
         report.provenance.captured_at_utc = utc_now();
         report.provenance.project_version = kVersion;
         report.provenance.source_scope = "ray_tracing_amd,include,oracle";
+        report.provenance.source_revision = RT_SOURCE_REVISION;
+        report.provenance.source_state_sha256 = RT_SOURCE_STATE_SHA256;
         report.provenance.build_configuration = "Release";
         report.provenance.cxx_compiler = __VERSION__;
         struct utsname un;
@@ -293,7 +369,7 @@ int main(int argc, char** argv) {
                 s.cameras.push_back(cs);
                 if (!a.skip_image_write) {
                     const auto w0 = std::chrono::steady_clock::now();
-                    write_ppm(a.output_dir + "/frame_" + std::to_string(f) + "_cam" + std::to_string(c) + ".ppm", fr);
+                    write_png(a.output_dir + "/frame_" + std::to_string(f) + "_cam" + std::to_string(c) + ".png", fr);
                     s.image_write_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
                 }
             }

@@ -46,7 +46,7 @@ def test_invalid_arguments_report_errors(rt):
     assert b"unknown scene config" in rt.lib().rt_last_error()
 
 
-@pytest.mark.parametrize("config", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("config", [0, 1, 2, 3, 4, 5, 6])
 def test_generators_match_oracle(rt, oracle, config):
     g = rt.Generated(config, 0)
     og = oracle.GeneratedScene(config, 0)
