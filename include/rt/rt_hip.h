@@ -290,6 +290,40 @@ rt_status rt_restir_diagnostics(rt_renderer* r, int camera, rt_restir_stats* out
  * OptixRenderer::restir_diagnostics (optix_renderer.cpp:183-215) */
 rt_status rt_restir_download(rt_renderer* r, int camera, void* reservoirs);
 
+/* ------------------------------------------------- viewer presentation
+ * replaces CudaGlTexturePresenter (src/realtime/viewer/cuda_gl_presenter.h:
+ * 14-40, cuda_gl_presenter.cu:45-130).  The reference tone maps a device
+ * beauty frame into an OpenGL pixel-unpack buffer and uploads it to a
+ * texture; with no GL interop here the presenter owns a device RGBA8 buffer
+ * of width x height pixels (4 bytes each, alpha 255) that the caller reads
+ * or hands to its own display path.  rt_presenter_present reports the
+ * CudaGlPresentStatus outcome in *present_status: RT_OK with a present
+ * status other than RT_PRESENT_OK is a refused frame, not a failed call. */
+typedef enum rt_present_status {
+    RT_PRESENT_OK = 0,
+    RT_PRESENT_INVALID_DEVICE_VIEW = 1, /* view->beauty is NULL */
+    RT_PRESENT_RESOLUTION_MISMATCH = 2  /* view size differs from the presenter's */
+} rt_present_status;
+typedef struct rt_presenter rt_presenter;
+/* the constructor (cuda_gl_presenter.cu:48-67): width, height > 0 */
+rt_status rt_presenter_create(rt_device* dev, int32_t width, int32_t height, rt_presenter** out);
+rt_status rt_presenter_destroy(rt_presenter* p);
+/* CudaGlTexturePresenter::present (cuda_gl_presenter.cu:86-130): tone maps
+ * view->beauty (device float4 per pixel, e.g. from rt_frame_device_view)
+ * with linear_to_display_u8 into the presenter's buffer on the device's
+ * stream and waits for it */
+rt_status rt_presenter_present(rt_presenter* p, const rt_device_frame_view* view,
+    int32_t* present_status);
+/* device address of the presenter's RGBA8 buffer, and a copy to host memory
+ * (width * height * 4 bytes) */
+rt_status rt_presenter_pixels(rt_presenter* p, uint8_t** d_rgba8);
+rt_status rt_presenter_download(rt_presenter* p, uint8_t* rgba8);
+/* tone_map_to_rgba8 (cuda_gl_presenter.cu:28-38) between caller buffers in
+ * device memory: pixel_count float4 in, pixel_count RGBA8 words out;
+ * enqueued on the device's stream without waiting (rt_device_synchronize) */
+rt_status rt_tone_map_rgba8(rt_device* dev, const float* d_beauty_rgba, uint8_t* d_rgba8,
+    int64_t pixel_count);
+
 /* ------------------------------------------------------ learned denoiser
  * A small dilated CNN (4 x conv3x3 with dilation 1/2/4/8, 32 channels, 1x1
  * head predicting log radiance as a residual of log1p(beauty)) run by BF16

@@ -262,6 +262,10 @@ def lib():
         L.rto_denoise_pass.restype = None
         L.rto_denoise.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 6
         L.rto_denoise.restype = None
+        L.rto_display_u8.argtypes = [C.c_float]
+        L.rto_display_u8.restype = C.c_uint8
+        L.rto_tone_map_rgba8.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+        L.rto_tone_map_rgba8.restype = None
         L.rto_nn_features.argtypes = [C.c_int] + [C.c_void_p] * 5
         L.rto_nn_features.restype = None
         L.rto_nn_loss_grad.argtypes = [C.c_void_p] * 7 + [C.c_int]
@@ -334,6 +338,10 @@ def ref_lib():
         R.ref_restir_update.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_int,
                                         C.c_float, C.POINTER(C.c_int)]
         R.ref_restir_finalize.argtypes = [C.c_void_p]
+        R.ref_linear_to_display.argtypes = [C.c_float]
+        R.ref_linear_to_display.restype = C.c_float
+        R.ref_linear_to_display_u8.argtypes = [C.c_float]
+        R.ref_linear_to_display_u8.restype = C.c_uint8
         _ref = R
     return _ref
 
@@ -560,6 +568,14 @@ def denoise(frame: dict, iterations: int = 4) -> np.ndarray:
 
 class Images(C.Structure):
     _fields_ = [("count", C.c_int32), ("offset", C.c_int32 * 17), ("width", C.c_int32 * 16)]
+
+
+def tone_map_rgba8(rgba: np.ndarray) -> np.ndarray:
+    """float4 pixels (..., 4) -> uint8 RGBA (..., 4), alpha 255."""
+    src = np.ascontiguousarray(rgba, dtype=np.float32)
+    out = np.empty(src.shape, np.uint8)
+    lib().rto_tone_map_rgba8(ptr(src), src.size // 4, ptr(out))
+    return out
 
 
 def images(sizes):

@@ -149,3 +149,25 @@ void rto_denoise(int w, int h, int iterations, const float* beauty, const float*
         for (size_t i = 0; i < n; ++i) out[i] = scratch[i];
     }
 }
+
+/* linear_to_display_u8 of the viewer's tone-map kernel,
+ * src/realtime/viewer/cuda_gl_presenter.cu:23-26: gamma 2 (square root) of
+ * the radiance clamped below at 0 (fmaxf also sends a quiet NaN to 0) and
+ * above at 0.999, scaled by 255 and rounded half away from zero (lroundf).
+ * The host twin rt::linear_to_display_u8 (src/realtime/display_transfer.h:
+ * 9-15) agrees on every non-NaN input. */
+uint8_t rto_display_u8(float v) {
+    const float d = fminf(sqrtf(fmaxf(v, 0.0f)), 0.999f);
+    return (uint8_t)lroundf(d * 255.0f);
+}
+
+/* tone_map_to_rgba8, cuda_gl_presenter.cu:28-38: one float4 in, RGBA8 out,
+ * alpha 255. */
+void rto_tone_map_rgba8(const float* rgba, int64_t n, uint8_t* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        out[4 * i + 0] = rto_display_u8(rgba[4 * i + 0]);
+        out[4 * i + 1] = rto_display_u8(rgba[4 * i + 1]);
+        out[4 * i + 2] = rto_display_u8(rgba[4 * i + 2]);
+        out[4 * i + 3] = 255;
+    }
+}

@@ -5,10 +5,11 @@
 // reservoir math); never linked into the product.
 //
 // Wrapped headers: src/common/openpbr_core.h, src/common/light_sampling.h,
-// src/common/restir_di.h.
+// src/common/restir_di.h, src/realtime/display_transfer.h.
 #include "common/light_sampling.h"
 #include "common/openpbr_core.h"
 #include "common/restir_di.h"
+#include "realtime/display_transfer.h"
 
 #include <cstdint>
 #include <cstring>
@@ -121,5 +122,8 @@ void ref_restir_finalize(float* rs) {
     rs[2] = r.estimator_weight;
     rs[7] = static_cast<float>(r.valid);
 }
+
+float ref_linear_to_display(float v) { return rt::linear_to_display(v); }
+uint8_t ref_linear_to_display_u8(float v) { return rt::linear_to_display_u8(v); }
 
 }  // extern "C"

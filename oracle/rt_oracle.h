@@ -234,6 +234,8 @@ void rto_denoise_pass(int w, int h, int step, const float* in, const float* norm
     const float* albedo, const float* depth, float* out);
 void rto_denoise(int w, int h, int iterations, const float* beauty, const float* normal,
     const float* albedo, const float* depth, float* out, float* scratch);
+uint8_t rto_display_u8(float v);
+void rto_tone_map_rgba8(const float* rgba, int64_t n, uint8_t* out);
 void rto_primary_rays(const rto_camera* cam, int cam_index, int sample, uint32_t stream,
     int jitter, rto_ray* rays);
 

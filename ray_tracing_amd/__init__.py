@@ -39,8 +39,12 @@ EXPORTED = [
     "rt_cache_create", "rt_cache_destroy", "rt_cache_set_params", "rt_cache_get_params",
     "rt_cache_set_bounds", "rt_cache_grid_info", "rt_cache_infer_host",
     "rt_cache_train_step_host", "rt_cache_get_stats", "rt_renderer_set_radiance_cache",
-    "rt_camera_build_lut",
+    "rt_camera_build_lut", "rt_presenter_create", "rt_presenter_destroy", "rt_presenter_present",
+    "rt_presenter_pixels", "rt_presenter_download", "rt_tone_map_rgba8",
 ]
+
+# rt_present_status (CudaGlPresentStatus, src/realtime/viewer/cuda_gl_presenter.h:14-18)
+PRESENT_OK, PRESENT_INVALID_DEVICE_VIEW, PRESENT_RESOLUTION_MISMATCH = 0, 1, 2
 
 
 class RtError(RuntimeError):
@@ -252,6 +256,57 @@ class Renderer:
         check(lib().rt_frame_download(self.h, camera, ptr(out["beauty"]), ptr(out["normal"]),
                                       ptr(out["albedo"]), ptr(out["depth"])))
         return out
+
+    def device_view(self, camera: int) -> "abi.DeviceFrameView":
+        v = abi.DeviceFrameView()
+        check(lib().rt_frame_device_view(self.h, camera, C.byref(v)))
+        return v
+
+
+class Presenter:
+    """CudaGlTexturePresenter (src/realtime/viewer/cuda_gl_presenter.h:20-40)
+    over a device RGBA8 buffer of a fixed resolution."""
+
+    def __init__(self, device: Device, width: int, height: int):
+        self.device = device
+        self.width, self.height = width, height
+        self.h = C.c_void_p()
+        check(lib().rt_presenter_create(device.h, width, height, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().rt_presenter_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def present(self, view) -> int:
+        """Tone maps a DeviceFrameView (or None) and returns the present status."""
+        st = C.c_int32(-1)
+        check(lib().rt_presenter_present(self.h, C.byref(view) if view is not None else None,
+                                         C.byref(st)))
+        return st.value
+
+    def device_pixels(self) -> int:
+        p = C.c_void_p()
+        check(lib().rt_presenter_pixels(self.h, C.byref(p)))
+        return p.value
+
+    def download(self) -> np.ndarray:
+        out = np.empty((self.height, self.width, 4), np.uint8)
+        check(lib().rt_presenter_download(self.h, ptr(out)))
+        return out
+
+
+def tone_map_rgba8(device: Device, d_beauty: int, d_rgba8: int, pixel_count: int):
+    """tone_map_to_rgba8 between device buffers (addresses as ints), on the
+    device's stream; does not wait."""
+    check(lib().rt_tone_map_rgba8(device.h, C.c_void_p(d_beauty), C.c_void_p(d_rgba8),
+                                  pixel_count))
 
 
 class Denoiser:

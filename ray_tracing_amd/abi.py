@@ -210,6 +210,12 @@ def declare(L):
     L.rt_render_frame.argtypes = [vp, P(CameraRig), P(Profile), C.c_uint32, P(FrameTiming)]
     L.rt_frame_download.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     L.rt_frame_device_view.argtypes = [vp, C.c_int, P(DeviceFrameView)]
+    L.rt_presenter_create.argtypes = [vp, C.c_int32, C.c_int32, P(vp)]
+    L.rt_presenter_destroy.argtypes = [vp]
+    L.rt_presenter_present.argtypes = [vp, P(DeviceFrameView), P(C.c_int32)]
+    L.rt_presenter_pixels.argtypes = [vp, P(vp)]
+    L.rt_presenter_download.argtypes = [vp, vp]
+    L.rt_tone_map_rgba8.argtypes = [vp, vp, vp, C.c_int64]
     L.rt_reset_accumulation.argtypes = [vp]
     L.rt_reset_sequence.argtypes = [vp, C.c_uint32]
     L.rt_restir_diagnostics.argtypes = [vp, C.c_int, P(RestirStats)]

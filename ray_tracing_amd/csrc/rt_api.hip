@@ -113,6 +113,14 @@ struct rt_device {
     size_t total_mem = 0;
 };
 
+// the viewer presenter: an RGBA8 pixel buffer of a fixed resolution
+// (CudaGlTexturePresenter's pixel-unpack buffer, cuda_gl_presenter.cu:53-67)
+struct rt_presenter {
+    rt_device* dev = nullptr;
+    int32_t width = 0, height = 0;
+    DevBuf<uint32_t> pixels;
+};
+
 struct rt_scene {
     std::atomic<int> refs{1};
     rt_device* dev = nullptr;
@@ -1502,6 +1510,86 @@ rt_status rt_scene_generated_free(rt_generated_scene* g) {
     if (!g) return RT_OK;
     rth::free_generated(g);
     delete g;
+    return RT_OK;
+}
+
+rt_status rt_presenter_create(rt_device* dev, int32_t width, int32_t height, rt_presenter** out) {
+    RT_REQUIRE(dev != nullptr && out != nullptr, RT_ERROR_INVALID_ARGUMENT, "null device or output handle");
+    RT_REQUIRE(width > 0 && height > 0, RT_ERROR_INVALID_ARGUMENT,
+        "rt_presenter_create requires a positive resolution");
+    RT_HIP(hipSetDevice(dev->ordinal));
+    auto* p = new rt_presenter();
+    p->width = width;
+    p->height = height;
+    const hipError_t e = p->pixels.reserve((size_t)width * (size_t)height);
+    if (e != hipSuccess) {
+        delete p;
+        return fail(e == hipErrorOutOfMemory ? RT_ERROR_OUT_OF_MEMORY : RT_ERROR_HIP,
+            std::string("HIP runtime failure at presenter buffer allocation: ") + hipGetErrorString(e));
+    }
+    dev->refs.fetch_add(1);
+    p->dev = dev;
+    *out = p;
+    return RT_OK;
+}
+
+rt_status rt_presenter_destroy(rt_presenter* p) {
+    if (!p) return RT_OK;
+    rt_device* d = p->dev;
+    (void)hipSetDevice(d->ordinal);
+    (void)hipStreamSynchronize(d->stream);
+    delete p;
+    device_release(d);
+    return RT_OK;
+}
+
+rt_status rt_presenter_present(rt_presenter* p, const rt_device_frame_view* view,
+    int32_t* present_status) {
+    RT_REQUIRE(p != nullptr && present_status != nullptr, RT_ERROR_INVALID_ARGUMENT,
+        "null presenter or status");
+    if (view == nullptr || view->beauty == nullptr) {
+        *present_status = RT_PRESENT_INVALID_DEVICE_VIEW;
+        return RT_OK;
+    }
+    if (view->width != p->width || view->height != p->height) {
+        *present_status = RT_PRESENT_RESOLUTION_MISMATCH;
+        return RT_OK;
+    }
+    RT_HIP(hipSetDevice(p->dev->ordinal));
+    RT_HIP(rtk::wf_tone_map_rgba8(reinterpret_cast<const float4*>(view->beauty), p->pixels.p,
+        (int64_t)p->width * p->height, p->dev->stream));
+    RT_HIP(hipStreamSynchronize(p->dev->stream));
+    *present_status = RT_PRESENT_OK;
+    return RT_OK;
+}
+
+rt_status rt_presenter_pixels(rt_presenter* p, uint8_t** d_rgba8) {
+    RT_REQUIRE(p != nullptr && d_rgba8 != nullptr, RT_ERROR_INVALID_ARGUMENT, "null presenter or output");
+    *d_rgba8 = reinterpret_cast<uint8_t*>(p->pixels.p);
+    return RT_OK;
+}
+
+rt_status rt_presenter_download(rt_presenter* p, uint8_t* rgba8) {
+    RT_REQUIRE(p != nullptr && rgba8 != nullptr, RT_ERROR_INVALID_ARGUMENT, "null presenter or output");
+    RT_HIP(hipSetDevice(p->dev->ordinal));
+    RT_HIP(hipMemcpyAsync(rgba8, p->pixels.p, (size_t)p->width * p->height * 4,
+        hipMemcpyDeviceToHost, p->dev->stream));
+    RT_HIP(hipStreamSynchronize(p->dev->stream));
+    return RT_OK;
+}
+
+rt_status rt_tone_map_rgba8(rt_device* dev, const float* d_beauty_rgba, uint8_t* d_rgba8,
+    int64_t pixel_count) {
+    RT_REQUIRE(dev != nullptr, RT_ERROR_INVALID_ARGUMENT, "null device");
+    RT_REQUIRE(pixel_count >= 0, RT_ERROR_INVALID_ARGUMENT, "negative pixel count");
+    if (pixel_count == 0) return RT_OK;
+    RT_REQUIRE(d_beauty_rgba != nullptr && d_rgba8 != nullptr, RT_ERROR_INVALID_ARGUMENT,
+        "null source or destination");
+    RT_REQUIRE(((uintptr_t)d_beauty_rgba & 15) == 0 && ((uintptr_t)d_rgba8 & 3) == 0,
+        RT_ERROR_INVALID_ARGUMENT, "the source must be 16-byte and the destination 4-byte aligned");
+    RT_HIP(hipSetDevice(dev->ordinal));
+    RT_HIP(rtk::wf_tone_map_rgba8(reinterpret_cast<const float4*>(d_beauty_rgba),
+        reinterpret_cast<uint32_t*>(d_rgba8), pixel_count, dev->stream));
     return RT_OK;
 }
 

@@ -145,6 +145,9 @@ hipError_t wf_denoise_filter(const CameraView& cams, int iterations, const float
     const float4* normal, const float4* albedo, const float* depth, float4* out,
     float4* scratch, int total, hipStream_t st);
 
+// the viewer's tone map: float4 linear radiance -> RGBA8 (alpha 255)
+hipError_t wf_tone_map_rgba8(const float4* src, uint32_t* dst, int64_t n, hipStream_t st);
+
 hipError_t wf_finalize(int pixels, int spp, const WavefrontBuffers& b, float4* beauty,
     float4* normal, float4* albedo, float* depth, hipStream_t st);
 

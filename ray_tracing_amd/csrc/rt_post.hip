@@ -1,9 +1,12 @@
 // rt_post.hip -- per-frame post passes over all cameras' pixels at once:
-// temporal reprojection / accumulation and the edge-aware denoise filter.
+// temporal reprojection / accumulation and the edge-aware denoise filter,
+// and the viewer's tone map to RGBA8.
 //
 // Pixels of all cameras lie back to back (CameraView::pixel_offset); one
-// launch covers the whole rig.  Both passes are bandwidth-bound stencils.
+// launch covers the whole rig.  All passes are bandwidth-bound.
 #include <hip/hip_runtime.h>
+
+#include <cstdint>
 
 #include "rt_camera.h"
 #include "rt_kernels.h"
@@ -154,7 +157,54 @@ __global__ __launch_bounds__(256) void k_denoise_pass(CameraView cams, int step,
     }
 }
 
+// linear_to_display_u8 (src/realtime/viewer/cuda_gl_presenter.cu:23-26):
+// gamma 2 (square root) of the radiance clamped below at 0 (fmaxf also sends
+// NaN to 0) and above at 0.999, times 255, rounded half away from zero.
+// sqrtf is correctly rounded in this build, so the result is bit-exact with
+// any correctly rounded restatement.
+__device__ __forceinline__ uint32_t display_u8(float v) {
+    const float d = fminf(sqrtf(fmaxf(v, 0.0f)), 0.999f);
+    return (uint32_t)roundf(d * 255.0f);   // d * 255 lies in [0, 254.745]
+}
+
+// tone_map_to_rgba8 (cuda_gl_presenter.cu:28-38): one float4 in and one
+// RGBA8 word out per pixel, alpha 255; 20 bytes of HBM traffic per pixel.
+// Each thread handles four pixels 256 apart inside its block's 1024-pixel
+// tile: all four loads are issued before the first store, and both sides
+// are streamed with non-temporal accesses (the frame is read once and the
+// output is not read back by this launch).
+constexpr int kToneMapTile = 1024;
+
+__global__ __launch_bounds__(256) void k_tone_map_rgba8(const float4* __restrict__ src,
+    uint32_t* __restrict__ dst, int64_t n) {
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const int64_t base = (int64_t)blockIdx.x * kToneMapTile + threadIdx.x;
+    f4v c[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int64_t i = base + j * 256;
+        if (i < n) c[j] = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(src + i));
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int64_t i = base + j * 256;
+        if (i < n) {
+            const uint32_t w = display_u8(c[j].x) | (display_u8(c[j].y) << 8)
+                             | (display_u8(c[j].z) << 16) | 0xFF000000u;
+            __builtin_nontemporal_store(w, dst + i);
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t wf_tone_map_rgba8(const float4* src, uint32_t* dst, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = (n + kToneMapTile - 1) / kToneMapTile;
+    if (blocks > INT32_MAX) return hipErrorInvalidValue;
+    k_tone_map_rgba8<<<(unsigned)blocks, 256, 0, st>>>(src, dst, n);
+    return hipGetLastError();
+}
 
 hipError_t wf_resolve(const CameraView& cams, const ResolveParams& rp, float4* beauty,
     float4* normal, float* depth, const float4* hbeauty, const float4* hnormal,
