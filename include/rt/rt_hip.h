@@ -55,6 +55,24 @@ typedef struct rt_material {
     int32_t openpbr_index;
 } rt_material;
 
+/* PackedTexture (src/realtime/gpu/launch_params.h:225-237), same field
+ * order.  Types: 0 constant colour; 1 checker over the world position
+ * (floor(p / scale) parity picks even_texture or odd_texture, chains resolve
+ * at most 8 deep); 2 image (image_width x image_height texels from
+ * image_offset in rt_scene_desc.image_texels; address modes 0 border (black
+ * outside [0, 1]), 1 clamp, 2 repeat, 3 mirror; filter 0 nearest,
+ * 1 bilinear); 3 turbulence noise.  An index outside the table or a chain
+ * deeper than 8 evaluates to magenta (1, 0, 1), as in evaluate_texture
+ * (src/realtime/gpu/programs.cu:268-302). */
+enum { RT_TEX_CONSTANT = 0, RT_TEX_CHECKER = 1, RT_TEX_IMAGE = 2, RT_TEX_NOISE = 3 };
+typedef struct rt_texture {
+    int32_t type, even_texture, odd_texture;
+    int32_t image_offset, image_width, image_height;
+    int32_t u_address_mode, v_address_mode, filter_type;
+    float scale;
+    float color[3];
+} rt_texture;
+
 /* OpenPbrCoreMaterial, src/common/openpbr_core.h:33-77 (same field order) */
 typedef struct rt_openpbr_material {
     float base_weight; float base_color[3]; float base_diffuse_roughness; float base_metalness;
@@ -95,6 +113,16 @@ typedef struct rt_scene_desc {
     const rt_material* materials;
     const rt_openpbr_material* openpbr;
     const rt_analytic_light* analytic_lights;
+    /* textures.  material_albedo_texture[i] >= 0 replaces material i's
+     * constant albedo by that texture evaluated at the hit
+     * (MaterialSample::albedo_texture, launch_params.h:216-218); NULL or -1
+     * keeps the constant.  Texture coordinates: sphere_uv of the outward
+     * normal, the quad's edge coordinates, the triangle's barycentrics
+     * (programs.cu:1447,1485,1528).  Emission stays the constant. */
+    int32_t texture_count, image_texel_count;
+    const rt_texture* textures;
+    const float* image_texels;               /* 3 floats (linear RGB) per texel */
+    const int32_t* material_albedo_texture;  /* material_count entries, or NULL */
 } rt_scene_desc;
 
 /* -------------------------------------------------------------- cameras

@@ -65,6 +65,15 @@ class AnalyticLight(C.Structure):
     _fields_ = [("type", C.c_int32), ("position", f3), ("radius", C.c_float), ("radiance", f3)]
 
 
+class Texture(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("even_texture", C.c_int32), ("odd_texture", C.c_int32),
+        ("image_offset", C.c_int32), ("image_width", C.c_int32), ("image_height", C.c_int32),
+        ("u_address_mode", C.c_int32), ("v_address_mode", C.c_int32), ("filter_type", C.c_int32),
+        ("scale", C.c_float), ("color", f3),
+    ]
+
+
 class Scene(C.Structure):
     _fields_ = [
         ("sphere_count", C.c_int32), ("quad_count", C.c_int32), ("triangle_count", C.c_int32),
@@ -74,6 +83,9 @@ class Scene(C.Structure):
         ("spheres", C.POINTER(Sphere)), ("quads", C.POINTER(Quad)),
         ("triangles", C.POINTER(Triangle)), ("materials", C.POINTER(Material)),
         ("openpbr", C.POINTER(OpenPbr)), ("analytic_lights", C.POINTER(AnalyticLight)),
+        ("texture_count", C.c_int32), ("image_texel_count", C.c_int32),
+        ("textures", C.c_void_p), ("image_texels", C.c_void_p),
+        ("material_albedo_texture", C.c_void_p),
     ]
 
 
@@ -262,6 +274,15 @@ def lib():
         L.rto_denoise_pass.restype = None
         L.rto_denoise.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 6
         L.rto_denoise.restype = None
+        L.rto_noise_sin.argtypes = [C.c_float]
+        L.rto_noise_sin.restype = C.c_float
+        L.rto_turbulence.argtypes = [C.c_void_p]
+        L.rto_turbulence.restype = C.c_float
+        L.rto_eval_texture.argtypes = [P(Scene), C.c_int, C.c_float, C.c_float, C.c_void_p, C.c_void_p]
+        L.rto_eval_texture.restype = None
+        L.rto_surface_uv.argtypes = [P(Scene), C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                     P(C.c_float), P(C.c_float)]
+        L.rto_surface_uv.restype = None
         L.rto_display_u8.argtypes = [C.c_float]
         L.rto_display_u8.restype = C.c_uint8
         L.rto_tone_map_rgba8.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
@@ -568,6 +589,21 @@ def denoise(frame: dict, iterations: int = 4) -> np.ndarray:
 
 class Images(C.Structure):
     _fields_ = [("count", C.c_int32), ("offset", C.c_int32 * 17), ("width", C.c_int32 * 16)]
+
+
+def eval_texture(scene: Scene, index: int, u: float, v: float, p) -> np.ndarray:
+    pp = np.asarray(p, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().rto_eval_texture(C.byref(scene), index, u, v, ptr(pp), ptr(out))
+    return out
+
+
+def surface_uv(scene: Scene, o, d, prim_type: int, prim_index: int, pos):
+    o, d, pos = (np.asarray(x, np.float32) for x in (o, d, pos))
+    u, v = C.c_float(), C.c_float()
+    lib().rto_surface_uv(C.byref(scene), ptr(o), ptr(d), prim_type, prim_index, ptr(pos),
+                         C.byref(u), C.byref(v))
+    return u.value, v.value
 
 
 def tone_map_rgba8(rgba: np.ndarray) -> np.ndarray:

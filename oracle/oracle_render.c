@@ -435,6 +435,15 @@ static void make_hit(const rto_scene* s, v3 o, v3 d, float t, int prim, hit_t* h
     h->fuzz = m->fuzz;
     h->ior = m->ior;
     h->base = ld3(m->albedo);
+    if (s->material_albedo_texture != NULL && s->material_albedo_texture[mat] >= 0) {
+        /* populate_material's albedo texture, programs.cu:978-979 */
+        const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+        const float pf[3] = {h->pos.x, h->pos.y, h->pos.z};
+        float u, v, c[3];
+        rto_surface_uv(s, of, df, h->prim_type, h->prim_index, pf, &u, &v);
+        rto_eval_texture(s, s->material_albedo_texture[mat], u, v, pf, c);
+        h->base = ld3(c);
+    }
     h->emission = ld3(m->emission);
     if (h->mat_type == RTO_MAT_DIELECTRIC) { h->base = mk3(0.92f, 0.95f, 1.0f); h->emission = mk3(0, 0, 0); }
     if (h->mat_type == RTO_MAT_DIFFUSE_LIGHT && !h->front) h->emission = mk3(0, 0, 0);

@@ -84,6 +84,15 @@ typedef struct {
 
 enum { RTO_BG_CONSTANT = 0, RTO_BG_SKY_GRADIENT = 1 };
 
+/* rt_texture (PackedTexture, launch_params.h:225-237) */
+typedef struct {
+    int32_t type, even_texture, odd_texture;
+    int32_t image_offset, image_width, image_height;
+    int32_t u_address_mode, v_address_mode, filter_type;
+    float scale;
+    float color[3];
+} rto_texture;
+
 typedef struct {
     int32_t sphere_count, quad_count, triangle_count, material_count;
     int32_t openpbr_count, analytic_light_count;
@@ -96,6 +105,10 @@ typedef struct {
     const rto_material* materials;
     const rto_openpbr* openpbr;
     const rto_analytic_light* analytic_lights;
+    int32_t texture_count, image_texel_count;
+    const rto_texture* textures;
+    const float* image_texels;
+    const int32_t* material_albedo_texture;
 } rto_scene;
 
 /* ---- cameras / profile ------------------------------------------------ */
@@ -234,6 +247,11 @@ void rto_denoise_pass(int w, int h, int step, const float* in, const float* norm
     const float* albedo, const float* depth, float* out);
 void rto_denoise(int w, int h, int iterations, const float* beauty, const float* normal,
     const float* albedo, const float* depth, float* out, float* scratch);
+float rto_noise_sin(float x);
+float rto_turbulence(const float p[3]);
+void rto_eval_texture(const rto_scene* s, int index, float u, float v, const float p[3], float out[3]);
+void rto_surface_uv(const rto_scene* s, const float o[3], const float d[3], int prim_type,
+    int prim_index, const float pos[3], float* u, float* v);
 uint8_t rto_display_u8(float v);
 void rto_tone_map_rgba8(const float* rgba, int64_t n, uint8_t* out);
 void rto_primary_rays(const rto_camera* cam, int cam_index, int sample, uint32_t stream,

@@ -59,6 +59,18 @@ class SceneDesc(C.Structure):
         ("background", f3), ("background_top", f3),
         ("spheres", C.c_void_p), ("quads", C.c_void_p), ("triangles", C.c_void_p),
         ("materials", C.c_void_p), ("openpbr", C.c_void_p), ("analytic_lights", C.c_void_p),
+        ("texture_count", C.c_int32), ("image_texel_count", C.c_int32),
+        ("textures", C.c_void_p), ("image_texels", C.c_void_p),
+        ("material_albedo_texture", C.c_void_p),
+    ]
+
+
+class Texture(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("even_texture", C.c_int32), ("odd_texture", C.c_int32),
+        ("image_offset", C.c_int32), ("image_width", C.c_int32), ("image_height", C.c_int32),
+        ("u_address_mode", C.c_int32), ("v_address_mode", C.c_int32), ("filter_type", C.c_int32),
+        ("scale", C.c_float), ("color", f3),
     ]
 
 

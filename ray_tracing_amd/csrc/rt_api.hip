@@ -126,6 +126,11 @@ struct rt_scene {
     rt_device* dev = nullptr;
     int32_t sphere_count = 0, quad_count = 0, triangle_count = 0, material_count = 0;
     int32_t openpbr_count = 0, analytic_count = 0;
+    int32_t texture_count = 0, texel_count = 0;
+    bool has_albedo_textures = false;
+    DevBuf<rt_texture> textures;
+    DevBuf<float> texels;
+    DevBuf<int32_t> mat_albedo_tex;
     int32_t background_mode = 0;
     float bg[3] = {0, 0, 0}, bg_top[3] = {0, 0, 0};
     DevBuf<rt_sphere> spheres;
@@ -185,6 +190,11 @@ struct rt_scene {
         v.stack4_entries = std::max(3 * wide_levels, 1);
         v.a_sel = a_sel.p;
         v.a_cdf = a_cdf.p;
+        v.textures = textures.p;
+        v.texels = texels.p;
+        v.mat_albedo_tex = has_albedo_textures ? mat_albedo_tex.p : nullptr;
+        v.texture_count = texture_count;
+        v.texel_count = texel_count;
         return v;
     }
 };
@@ -530,6 +540,12 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
         RT_REQUIRE(d->analytic_lights[i].type == RT_ALIGHT_SPHERE, RT_ERROR_UNSUPPORTED,
             "analytic light " + std::to_string(i) + " is not a sphere light");
     }
+    RT_REQUIRE(d->texture_count >= 0 && d->image_texel_count >= 0, RT_ERROR_INVALID_ARGUMENT,
+        "texture counts must be non-negative");
+    RT_REQUIRE((d->texture_count == 0 || d->textures) && (d->image_texel_count == 0 || d->image_texels),
+        RT_ERROR_INVALID_ARGUMENT, "texture arrays must be non-null when their count is positive");
+    RT_REQUIRE(d->image_texel_count <= (INT32_MAX / 3), RT_ERROR_INVALID_ARGUMENT,
+        "image texel count exceeds 2^31 / 3");
     RT_HIP(hipSetDevice(dev->ordinal));
     auto* s = new rt_scene();
     s->dev = dev;
@@ -601,6 +617,13 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
     build_analytic_cdf(d, asel, acdf);
     if (e == hipSuccess) e = upload(s->a_sel, asel.data(), asel.size(), st);
     if (e == hipSuccess) e = upload(s->a_cdf, acdf.data(), acdf.size(), st);
+    s->texture_count = d->texture_count;
+    s->texel_count = d->image_texel_count;
+    s->has_albedo_textures = d->material_albedo_texture != nullptr && d->material_count > 0;
+    if (e == hipSuccess) e = upload(s->textures, d->textures, (size_t)d->texture_count, st);
+    if (e == hipSuccess) e = upload(s->texels, d->image_texels, (size_t)d->image_texel_count * 3, st);
+    if (e == hipSuccess && s->has_albedo_textures)
+        e = upload(s->mat_albedo_tex, d->material_albedo_texture, (size_t)d->material_count, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         scene_release(s);

@@ -56,6 +56,10 @@ struct SceneView {
     int32_t stack_depth;  // LDS traversal stack entries per lane (>= BVH depth)
     const float* a_sel;   // analytic light selection pdf
     const float* a_cdf;   // analytic light CDF
+    const rt_texture* textures;
+    const float* texels;               // 3 floats per texel
+    const int32_t* mat_albedo_tex;     // nullptr: every albedo is a constant
+    int32_t texture_count, texel_count;
 };
 
 // 4-wide quantized BVH node, 64 B = 4 x float4, collapsed from the binary

@@ -16,6 +16,7 @@
 #include "rt_internal.h"
 #include "rt_math.h"
 #include "rt_openpbr.h"
+#include "rt_texture.h"
 
 namespace rtk {
 
@@ -83,6 +84,15 @@ __device__ inline void make_hit(const SceneView& s, f3 o, f3 d, float t, int pri
     h.fuzz = m.fuzz;
     h.ior = m.ior;
     h.base = ld3(m.albedo);
+    if (s.mat_albedo_tex != nullptr) {
+        // populate_material's albedo texture (programs.cu:978-979)
+        const int ti = s.mat_albedo_tex[mat];
+        if (ti >= 0) {
+            float u, v;
+            surface_uv(s, o, d, h.prim_type, h.prim_index, h.pos, &u, &v);
+            h.base = eval_texture(s, ti, u, v, h.pos);
+        }
+    }
     h.emission = ld3(m.emission);
     if (h.mat_type == RT_MAT_DIELECTRIC) { h.base = mk3(0.92f, 0.95f, 1.0f); h.emission = mk3(0, 0, 0); }
     if (h.mat_type == RT_MAT_DIFFUSE_LIGHT && !h.front) h.emission = mk3(0, 0, 0);

@@ -22,8 +22,9 @@ compile_realtime_camera_transform / compile_cpu_camera_transform
 src/realtime/viewer/body_pose.cpp:36-85; the camera axes are x right,
 y down, z forward, as the renderer's.
 
-This build renders constant textures only: checker, noise and image
-textures, and participating media (isotropic materials, `scene.media`),
+Albedo textures of every type (constant, checker, image, noise) are
+rendered; textured emission and participating media (isotropic materials,
+`scene.media`),
 raise SceneFileError when a scene is flattened.
 
     python -m ray_tracing_amd.scene_yaml path/to/scene.yaml --spp 16 --out frame.npy
@@ -535,35 +536,88 @@ def load_scene_definition(path) -> SceneDefinition:
 
 # --------------------------------------------------------------- flattening
 
-def _color(defn: SceneDefinition, tex: int, path: str) -> np.ndarray:
+TEX_TYPES = {"constant": 0, "checker": 1, "image": 2, "noise": 3}
+
+
+def _emission_color(defn: SceneDefinition, tex: int, path: str) -> np.ndarray:
     t = defn.textures[tex]
     if t["type"] != "constant":
-        raise SceneFileError(f"{path}: texture type {t['type']} is not supported by this build "
-                             "(constant textures only)")
+        raise SceneFileError(f"{path}: textured emission ({t['type']}) is not supported by this "
+                             "build (constant emission only)")
     return np.asarray(t["color"], np.float64)
 
 
-def _material(defn: SceneDefinition, m: dict, path: str) -> abi.Material:
+def _albedo(defn: SceneDefinition, tex: int, out: abi.Material) -> int:
+    """A constant albedo is stored in the material (binding -1); any other
+    texture is bound by its index in the scene's texture table."""
+    t = defn.textures[tex]
+    if t["type"] == "constant":
+        out.albedo = abi.f3(*t["color"])
+        return -1
+    return tex
+
+
+def _material(defn: SceneDefinition, m: dict, path: str):
     out = abi.Material()
     out.openpbr_index = -1
     out.ior = 1.0
+    binding = -1
     if m["type"] == "diffuse":
         out.type = MAT_LAMBERTIAN
-        out.albedo = abi.f3(*_color(defn, m["albedo"], path))
+        binding = _albedo(defn, m["albedo"], out)
     elif m["type"] == "metal":
         out.type = MAT_METAL
-        out.albedo = abi.f3(*_color(defn, m["albedo"], path))
+        binding = _albedo(defn, m["albedo"], out)
         out.fuzz = m["fuzz"]
     elif m["type"] == "dielectric":
         out.type = MAT_DIELECTRIC
         out.ior = m["ior"]
     elif m["type"] == "emissive":
         out.type = MAT_DIFFUSE_LIGHT
-        out.emission = abi.f3(*_color(defn, m["emission"], path))
+        out.emission = abi.f3(*_emission_color(defn, m["emission"], path))
     else:
         raise SceneFileError(f"{path}: material type {m['type']} (participating media) is not "
                              "supported by this build")
-    return out
+    return out, binding
+
+
+def load_image_rgb(path: str) -> np.ndarray:
+    """load_texture_image_rgb32f (packed_scene_preparation.cpp:126-147):
+    8-bit RGB scaled by 1/255, rows top to bottom."""
+    try:
+        from PIL import Image
+        with Image.open(path) as im:
+            return np.asarray(im.convert("RGB"), np.float32) / np.float32(255.0)
+    except (OSError, ValueError) as e:
+        raise SceneFileError(f"{path}: failed to load texture image ({e})") from None
+
+
+def _texture_table(defn: SceneDefinition):
+    """pack_texture (packed_scene_preparation.cpp:149-190): one rt_texture
+    per scene texture, in definition order; image texels back to back."""
+    table, texels, offset = [], [], 0
+    for t in defn.textures:
+        r = abi.Texture()
+        r.type = TEX_TYPES[t["type"]]
+        r.even_texture = r.odd_texture = -1
+        r.u_address_mode = r.v_address_mode = 1      # clamp
+        r.filter_type = 0                            # closest (ImageTextureDesc defaults)
+        r.scale = 1.0
+        if t["type"] == "constant":
+            r.color = abi.f3(*t["color"])
+        elif t["type"] == "checker":
+            r.scale, r.even_texture, r.odd_texture = t["scale"], t["even"], t["odd"]
+        elif t["type"] == "noise":
+            r.scale = t["scale"]
+        else:
+            img = load_image_rgb(t["path"])
+            r.image_offset = offset
+            r.image_height, r.image_width = img.shape[:2]
+            texels.append(img.reshape(-1, 3))
+            offset += img.shape[0] * img.shape[1]
+        table.append(r)
+    flat = np.ascontiguousarray(np.concatenate(texels) if texels else np.zeros((0, 3), np.float32))
+    return table, flat
 
 
 def box_quads(lo, hi, tf: Transform):
@@ -590,6 +644,9 @@ class FlatScene:
     quads: object
     triangles: object
     materials: object
+    textures: object = None
+    texels: object = None
+    albedo_bindings: object = None
 
 
 def flatten(defn: SceneDefinition) -> FlatScene:
@@ -599,7 +656,11 @@ def flatten(defn: SceneDefinition) -> FlatScene:
     path = defn.dependencies[0] if defn.dependencies else defn.id
     if defn.media:
         raise SceneFileError(f"{path}: participating media are not supported by this build")
-    mats = [_material(defn, m, path) for m in defn.materials]
+    mats, bindings = [], []
+    for m in defn.materials:
+        mat, binding = _material(defn, m, path)
+        mats.append(mat)
+        bindings.append(binding)
     spheres, quads, tris = [], [], []
     for shape_i, mat_i, tf in defn.instances:
         s = defn.shapes[shape_i]
@@ -631,6 +692,15 @@ def flatten(defn: SceneDefinition) -> FlatScene:
     d.materials = C.cast(fs.materials, C.c_void_p) if fs.materials is not None else None
     d.background_mode = 0
     d.background = abi.f3(*defn.background)
+    if any(b >= 0 for b in bindings):
+        table, texels = _texture_table(defn)
+        fs.textures = (abi.Texture * len(table))(*table)
+        fs.texels = texels
+        fs.albedo_bindings = np.asarray(bindings, np.int32)
+        d.texture_count, d.image_texel_count = len(table), texels.shape[0]
+        d.textures = C.cast(fs.textures, C.c_void_p)
+        d.image_texels = texels.ctypes.data_as(C.c_void_p) if texels.size else None
+        d.material_albedo_texture = fs.albedo_bindings.ctypes.data_as(C.c_void_p)
     return fs
 
 

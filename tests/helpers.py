@@ -20,6 +20,9 @@ def oracle_scene(oracle, desc):
     s.materials = C.cast(desc.materials, C.POINTER(oracle.Material))
     s.openpbr = C.cast(desc.openpbr, C.POINTER(oracle.OpenPbr))
     s.analytic_lights = C.cast(desc.analytic_lights, C.POINTER(oracle.AnalyticLight))
+    s.texture_count, s.image_texel_count = desc.texture_count, desc.image_texel_count
+    s.textures, s.image_texels = desc.textures, desc.image_texels
+    s.material_albedo_texture = desc.material_albedo_texture
     return s
 
 

@@ -82,10 +82,50 @@ def test_obj_import_adds_one_diffuse_triangle():
     assert any(p.endswith("triangle.mtl") for p in d.dependencies)
 
 
-def test_noise_texture_is_rejected_when_flattened():
-    d = S.load_scene_definition(scene("simple_light"))   # parses
-    with pytest.raises(S.SceneFileError, match="texture type noise is not supported"):
-        S.flatten(d)
+def test_noise_texture_scene_flattens_with_a_texture_table(tmp_path):
+    d = S.load_scene_definition(scene("simple_light"))
+    f = S.flatten(d)
+    assert f.desc.texture_count == 2 and f.desc.image_texel_count == 0
+    assert (f.textures[0].type, f.textures[0].scale) == (S.TEX_TYPES["noise"], 4.0)
+    # the diffuse "perlin" material is bound to texture 0, the light keeps its constant
+    assert f.albedo_bindings.tolist() == [0, -1]
+    assert tuple(f.materials[1].emission) == (4.0, 4.0, 4.0)
+    # textured emission is refused
+    p = str(tmp_path / "scene.yaml")
+    write(p, open(scene("simple_light")).read().replace("emission: light_tex", "emission: perlin_noise"))
+    with pytest.raises(S.SceneFileError, match="textured emission"):
+        S.flatten(S.load_scene_definition(p))
+
+
+def test_image_and_checker_textures_flatten(tmp_path):
+    from PIL import Image
+    img = np.zeros((2, 3, 3), np.uint8)
+    img[0, 0] = (255, 0, 0)
+    img[1, 2] = (0, 0, 51)
+    Image.fromarray(img).save(str(tmp_path / "tex.png"))
+    text = BASIC.replace("""    white:
+      type: constant
+      color: [1.0, 1.0, 1.0]""", """    white:
+      type: constant
+      color: [1.0, 1.0, 1.0]
+    pic:
+      type: image
+      path: tex.png
+    board:
+      type: checker
+      scale: 0.5
+      even: white
+      odd: pic""").replace("albedo: white", "albedo: board")
+    p = str(tmp_path / "scene.yaml")
+    write(p, text)
+    f = S.flatten(S.load_scene_definition(p))
+    t = [f.textures[i] for i in range(f.desc.texture_count)]
+    assert [x.type for x in t] == [0, 2, 1]
+    assert (t[1].image_width, t[1].image_height, t[1].filter_type, t[1].u_address_mode) == (3, 2, 0, 1)
+    assert (t[2].even_texture, t[2].odd_texture, t[2].scale) == (0, 1, 0.5)
+    assert f.texels.shape == (6, 3) and f.texels[0].tolist() == [1.0, 0.0, 0.0]
+    assert f.texels[5].tolist() == pytest.approx([0.0, 0.0, 0.2])
+    assert f.albedo_bindings.tolist() == [2]
 
 
 def test_realtime_camera_axes():
