@@ -123,6 +123,10 @@ typedef struct rt_scene_desc {
     const rt_texture* textures;
     const float* image_texels;               /* 3 floats (linear RGB) per texel */
     const int32_t* material_albedo_texture;  /* material_count entries, or NULL */
+    /* per-triangle texture coordinates uv0, uv1, uv2 (6 floats per
+     * triangle; PackedTriangle::uv0..uv2 with has_texcoords), or NULL.  A
+     * triangle whose first float is NaN has none and uses its barycentrics. */
+    const float* triangle_uvs;
 } rt_scene_desc;
 
 /* -------------------------------------------------------------- cameras

@@ -156,7 +156,18 @@ void rto_surface_uv(const rto_scene* s, const float o_in[3], const float d_in[3]
         const v3 pvec = cross3(d, e2);
         const float inv_det = 1.0f / dot3(e1, pvec);
         const v3 tvec = sub3(o, p0);
-        *u = dot3(tvec, pvec) * inv_det;
-        *v = dot3(d, cross3(tvec, e1)) * inv_det;
+        const float bu = dot3(tvec, pvec) * inv_det;
+        const float bv = dot3(d, cross3(tvec, e1)) * inv_det;
+        *u = bu;
+        *v = bv;
+        if (s->triangle_uvs != NULL) {
+            /* uv0 w + uv1 u + uv2 v, programs.cu:1524-1530 */
+            const float* t = s->triangle_uvs + 6 * (int64_t)prim_index;
+            if (t[0] == t[0]) {
+                const float w = 1.0f - bu - bv;
+                *u = t[0] * w + t[2] * bu + t[4] * bv;
+                *v = t[1] * w + t[3] * bu + t[5] * bv;
+            }
+        }
     }
 }

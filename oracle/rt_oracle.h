@@ -109,6 +109,7 @@ typedef struct {
     const rto_texture* textures;
     const float* image_texels;
     const int32_t* material_albedo_texture;
+    const float* triangle_uvs;
 } rto_scene;
 
 /* ---- cameras / profile ------------------------------------------------ */

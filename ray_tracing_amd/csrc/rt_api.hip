@@ -128,6 +128,8 @@ struct rt_scene {
     int32_t openpbr_count = 0, analytic_count = 0;
     int32_t texture_count = 0, texel_count = 0;
     bool has_albedo_textures = false;
+    bool has_triangle_uvs = false;
+    DevBuf<float> tri_uv;
     DevBuf<rt_texture> textures;
     DevBuf<float> texels;
     DevBuf<int32_t> mat_albedo_tex;
@@ -193,6 +195,7 @@ struct rt_scene {
         v.textures = textures.p;
         v.texels = texels.p;
         v.mat_albedo_tex = has_albedo_textures ? mat_albedo_tex.p : nullptr;
+        v.tri_uv = has_triangle_uvs ? tri_uv.p : nullptr;
         v.texture_count = texture_count;
         v.texel_count = texel_count;
         return v;
@@ -624,6 +627,9 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
     if (e == hipSuccess) e = upload(s->texels, d->image_texels, (size_t)d->image_texel_count * 3, st);
     if (e == hipSuccess && s->has_albedo_textures)
         e = upload(s->mat_albedo_tex, d->material_albedo_texture, (size_t)d->material_count, st);
+    s->has_triangle_uvs = d->triangle_uvs != nullptr && d->triangle_count > 0;
+    if (e == hipSuccess && s->has_triangle_uvs)
+        e = upload(s->tri_uv, d->triangle_uvs, (size_t)d->triangle_count * 6, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         scene_release(s);

@@ -134,8 +134,9 @@ __device__ inline f3 eval_texture(const SceneView& s, int index, float u, float 
 }
 
 // texture coordinates of a hit at pos = o + d * t (programs.cu:1447,1485,
-// 1528): sphere_uv of the outward normal; the quad's edge coordinates and
-// the triangle's barycentrics, recomputed with the intersectors' arithmetic
+// 1524-1530): sphere_uv of the outward normal; the quad's edge coordinates;
+// the triangle's barycentrics (recomputed with the intersectors'
+// arithmetic), interpolating its per-vertex coordinates when it has them
 __device__ inline void surface_uv(const SceneView& s, f3 o, f3 d, int prim_type, int prim_index,
     f3 pos, float* u, float* v) {
     if (prim_type == 0) {
@@ -160,8 +161,18 @@ __device__ inline void surface_uv(const SceneView& s, f3 o, f3 d, int prim_type,
         const f3 pvec = cross3(d, e2);
         const float inv_det = 1.0f / dot3(e1, pvec);
         const f3 tvec = o - p0;
-        *u = dot3(tvec, pvec) * inv_det;
-        *v = dot3(d, cross3(tvec, e1)) * inv_det;
+        const float bu = dot3(tvec, pvec) * inv_det;
+        const float bv = dot3(d, cross3(tvec, e1)) * inv_det;
+        *u = bu;
+        *v = bv;
+        if (s.tri_uv != nullptr) {
+            const float* t = s.tri_uv + 6 * (int64_t)prim_index;
+            if (t[0] == t[0]) {   // not NaN: has_texcoords
+                const float w = 1.0f - bu - bv;
+                *u = t[0] * w + t[2] * bu + t[4] * bv;
+                *v = t[1] * w + t[3] * bu + t[5] * bv;
+            }
+        }
     }
 }
 

@@ -402,10 +402,12 @@ def import_obj_mtl(path, out: SceneDefinition):
     """import_obj_mtl (src/scene/obj_mtl_importer.cpp:118-230): positions and
     polygon faces (fan-triangulated) grouped by material; one diffuse
     material per group (Kd, or an image texture for map_Kd, or white
-    without a material); appended with an identity transform."""
+    without a material); appended with an identity transform.  A group keeps
+    its `vt` texture coordinates only when every corner of every triangle in
+    it has one (obj_mtl_importer.cpp:202-224)."""
     path = os.path.normpath(path)
     root = os.path.dirname(path)
-    verts, groups, order, mats, cur = [], {}, [], {}, -1
+    verts, uvs, groups, order, mats, cur = [], [], {}, [], {}, -1
     with open(path) as f:
         for line in f:
             tok = line.split()
@@ -413,6 +415,8 @@ def import_obj_mtl(path, out: SceneDefinition):
                 continue
             if tok[0] == "v":
                 verts.append([float(v) for v in tok[1:4]])
+            elif tok[0] == "vt":
+                uvs.append([float(v) for v in tok[1:3]])
             elif tok[0] == "mtllib":
                 mp = os.path.normpath(os.path.join(root, " ".join(tok[1:])))
                 mats.update(_parse_mtl(mp))
@@ -422,31 +426,38 @@ def import_obj_mtl(path, out: SceneDefinition):
                 name = " ".join(tok[1:])
                 cur = list(mats).index(name) if name in mats else -1
             elif tok[0] == "f":
-                idx = []
+                idx, tidx = [], []
                 for c in tok[1:]:
-                    i = int(c.split("/")[0])
+                    parts = c.split("/")
+                    i = int(parts[0])
                     if i < 0:
                         raise ValueError("negative obj vertex indices are not supported")
                     idx.append(i - 1)
+                    tidx.append(int(parts[1]) - 1 if len(parts) > 1 and parts[1] else -1)
                 if len(idx) < 3:
                     raise ValueError("only triangulated faces are supported")
                 if cur not in groups:
                     groups[cur] = []
                     order.append(cur)
                 for k in range(1, len(idx) - 1):
-                    groups[cur].append((idx[0], idx[k], idx[k + 1]))
+                    groups[cur].append((idx[0], idx[k], idx[k + 1], tidx[0], tidx[k], tidx[k + 1]))
     V = np.array(verts, np.float64).reshape(-1, 3)
+    UV = np.array(uvs, np.float64).reshape(-1, 2)
     for gi in sorted(order):
-        tris = np.array(groups[gi], np.int64)
+        rows = np.array(groups[gi], np.int64).reshape(-1, 6)
+        tris, tuv = rows[:, :3], rows[:, 3:]
         if tris.size and (tris.max() >= len(V)):
             raise ValueError("obj face index out of range")
+        if tuv.size and tuv.max() >= len(UV):
+            raise ValueError("obj texture coordinate index out of range")
+        texcoords = UV[tuv] if len(UV) and tuv.size and (tuv >= 0).all() else None
         m = mats[list(mats)[gi]] if gi >= 0 else None
         if m is not None and m["map_Kd"] is not None:
             out.textures.append({"type": "image", "path": os.path.normpath(os.path.join(root, m["map_Kd"]))})
         else:
             out.textures.append({"type": "constant", "color": m["Kd"] if m is not None else np.ones(3)})
         out.materials.append({"type": "diffuse", "albedo": len(out.textures) - 1})
-        out.shapes.append({"type": "mesh", "positions": V, "triangles": tris})
+        out.shapes.append({"type": "mesh", "positions": V, "triangles": tris, "texcoords": texcoords})
         out.instances.append((len(out.shapes) - 1, len(out.materials) - 1, Transform()))
 
 
@@ -537,6 +548,7 @@ def load_scene_definition(path) -> SceneDefinition:
 # --------------------------------------------------------------- flattening
 
 TEX_TYPES = {"constant": 0, "checker": 1, "image": 2, "noise": 3}
+NO_UV = np.full(6, np.nan)      # a triangle without texture coordinates
 
 
 def _emission_color(defn: SceneDefinition, tex: int, path: str) -> np.ndarray:
@@ -647,6 +659,7 @@ class FlatScene:
     textures: object = None
     texels: object = None
     albedo_bindings: object = None
+    triangle_uvs: object = None
 
 
 def flatten(defn: SceneDefinition) -> FlatScene:
@@ -661,7 +674,7 @@ def flatten(defn: SceneDefinition) -> FlatScene:
         mat, binding = _material(defn, m, path)
         mats.append(mat)
         bindings.append(binding)
-    spheres, quads, tris = [], [], []
+    spheres, quads, tris, tri_uv = [], [], [], []
     for shape_i, mat_i, tf in defn.instances:
         s = defn.shapes[shape_i]
         if s["type"] == "sphere":
@@ -674,9 +687,11 @@ def flatten(defn: SceneDefinition) -> FlatScene:
                 quads.append(abi.Quad(abi.f3(*o), abi.f3(*u), abi.f3(*v), mat_i))
         else:
             P = s["positions"]
-            for a, b, c in s["triangles"]:
+            tc = s.get("texcoords")
+            for k, (a, b, c) in enumerate(s["triangles"]):
                 tris.append(abi.Triangle(abi.f3(*tf.point(P[a])), abi.f3(*tf.point(P[b])),
                                          abi.f3(*tf.point(P[c])), mat_i))
+                tri_uv.append(tc[k].reshape(6) if tc is not None else NO_UV)
 
     def arr(T, items):
         return (T * len(items))(*items) if items else None
@@ -701,6 +716,9 @@ def flatten(defn: SceneDefinition) -> FlatScene:
         d.textures = C.cast(fs.textures, C.c_void_p)
         d.image_texels = texels.ctypes.data_as(C.c_void_p) if texels.size else None
         d.material_albedo_texture = fs.albedo_bindings.ctypes.data_as(C.c_void_p)
+    if any(not np.isnan(u[0]) for u in tri_uv):
+        fs.triangle_uvs = np.ascontiguousarray(np.array(tri_uv, np.float32).reshape(-1, 6))
+        d.triangle_uvs = fs.triangle_uvs.ctypes.data_as(C.c_void_p)
     return fs
 
 

@@ -23,6 +23,7 @@ def oracle_scene(oracle, desc):
     s.texture_count, s.image_texel_count = desc.texture_count, desc.image_texel_count
     s.textures, s.image_texels = desc.textures, desc.image_texels
     s.material_albedo_texture = desc.material_albedo_texture
+    s.triangle_uvs = desc.triangle_uvs
     return s
 
 

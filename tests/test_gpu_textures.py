@@ -2,7 +2,8 @@
 src/realtime/gpu/programs.cu:183-302) with the oracle: the reference's
 simple_light scene file (turbulence-noise spheres) and the Cornell box
 re-bound to a checker of an image, a repeat / bilinear image and a mirror /
-nearest image, with an image-textured sphere and triangle added.  Noise,
+nearest image, with an image-textured sphere and two triangles added (one
+with per-vertex texture coordinates, one without).  Noise,
 checker, quad and triangle coordinates use only +, -, *, / and floor, so
 they are bit-exact; the sphere's coordinates go through acosf / atan2f,
 whose device and libm results may differ in the last place, so a few
@@ -103,14 +104,20 @@ def textured_cornell(rt):
     bind += [a_near, b_mirror]
     k = len(mats)
     spheres = (abi.Sphere * 1)(abi.Sphere(abi.f3(400.0, 90.0, 120.0), 80.0, k - 2))
-    tris = (abi.Triangle * 1)(abi.Triangle(abi.f3(60.0, 1.0, 60.0), abi.f3(260.0, 1.0, 60.0),
-                                           abi.f3(120.0, 260.0, 200.0), k - 1))
+    tris = (abi.Triangle * 2)(abi.Triangle(abi.f3(60.0, 1.0, 60.0), abi.f3(260.0, 1.0, 60.0),
+                                           abi.f3(120.0, 260.0, 200.0), k - 1),
+                              abi.Triangle(abi.f3(300.0, 300.0, 500.0), abi.f3(500.0, 300.0, 500.0),
+                                           abi.f3(400.0, 500.0, 500.0), k - 1))
+    # the first triangle has per-vertex coordinates (repeating the image 3 x 2
+    # times), the second none (NaN: barycentrics)
+    tri_uv = np.array([[0.0, 0.0, 3.0, 0.0, 1.5, 2.0], [np.nan] * 6], np.float32)
     mat_arr = (abi.Material * k)(*mats)
     tex_arr = (abi.Texture * len(T))(*T)
     bind_arr = np.asarray(bind, np.int32)
     desc = abi.SceneDesc()
     C.memmove(C.byref(desc), C.byref(flat.desc), C.sizeof(desc))
-    desc.sphere_count, desc.triangle_count, desc.material_count = 1, 1, k
+    desc.sphere_count, desc.triangle_count, desc.material_count = 1, 2, k
+    desc.triangle_uvs = tri_uv.ctypes.data_as(C.c_void_p)
     desc.spheres = C.cast(spheres, C.c_void_p)
     desc.triangles = C.cast(tris, C.c_void_p)
     desc.materials = C.cast(mat_arr, C.c_void_p)
@@ -118,7 +125,7 @@ def textured_cornell(rt):
     desc.textures = C.cast(tex_arr, C.c_void_p)
     desc.image_texels = texels.ctypes.data_as(C.c_void_p)
     desc.material_albedo_texture = bind_arr.ctypes.data_as(C.c_void_p)
-    keep = (flat, spheres, tris, mat_arr, tex_arr, bind_arr, texels)
+    keep = (flat, spheres, tris, tri_uv, mat_arr, tex_arr, bind_arr, texels)
     return desc, keep, S.camera_rig(d, "default")
 
 

@@ -126,3 +126,11 @@ def test_surface_uv():
     # triangle: a ray hitting p0 + 0.3 e1 + 0.2 e2 = (0.6, 0.4, 0)
     uv = O.surface_uv(s, (0.6, 0.4, 5), (0, 0, -1), 2, 0, (0.6, 0.4, 0))
     assert uv == pytest.approx((0.3, 0.2), abs=1e-6)
+    # per-vertex coordinates: uv0 w + uv1 u + uv2 v with (w, u, v) = (0.5, 0.3, 0.2)
+    tuv = np.array([[0.1, 0.9, 0.7, 0.5, 0.2, 0.3]], np.float32)
+    s.triangle_uvs = tuv.ctypes.data_as(C.c_void_p)
+    uv = O.surface_uv(s, (0.6, 0.4, 5), (0, 0, -1), 2, 0, (0.6, 0.4, 0))
+    assert uv == pytest.approx((0.1 * 0.5 + 0.7 * 0.3 + 0.2 * 0.2, 0.9 * 0.5 + 0.5 * 0.3 + 0.3 * 0.2), abs=1e-6)
+    tuv[0, 0] = np.nan                     # no coordinates: barycentrics
+    uv = O.surface_uv(s, (0.6, 0.4, 5), (0, 0, -1), 2, 0, (0.6, 0.4, 0))
+    assert uv == pytest.approx((0.3, 0.2), abs=1e-6)

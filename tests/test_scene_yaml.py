@@ -97,6 +97,25 @@ def test_noise_texture_scene_flattens_with_a_texture_table(tmp_path):
         S.flatten(S.load_scene_definition(p))
 
 
+def test_obj_texture_coordinates(tmp_path):
+    from PIL import Image
+    Image.fromarray(np.full((2, 2, 3), 128, np.uint8)).save(str(tmp_path / "kd.png"))
+    write(str(tmp_path / "m.mtl"), "newmtl tex\nKd 1 1 1\nmap_Kd kd.png\nnewmtl plain\nKd 0.5 0.5 0.5\n")
+    write(str(tmp_path / "m.obj"), "mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\n"
+          "vt 0 0\nvt 1 0\nvt 1 1\nvt 0 1\nusemtl tex\nf 1/1 2/2 3/3 4/4\n"
+          "usemtl plain\nf 1/1 3 4/4\n")
+    p = str(tmp_path / "scene.yaml")
+    write(p, BASIC + "imports:\n  mesh:\n    type: obj_mtl\n    obj: m.obj\n")
+    f = S.flatten(S.load_scene_definition(p))
+    assert f.desc.triangle_count == 2 + 1
+    uv = f.triangle_uvs
+    # the quad fans into (1,2,3), (1,3,4) with their vt; the "plain" group
+    # misses a corner's vt, so its triangle has none (NaN)
+    assert uv[0].tolist() == [0, 0, 1, 0, 1, 1] and uv[1].tolist() == [0, 0, 1, 1, 0, 1]
+    assert np.isnan(uv[2][0])
+    assert f.desc.texture_count >= 2 and (f.albedo_bindings >= 0).sum() == 1
+
+
 def test_image_and_checker_textures_flatten(tmp_path):
     from PIL import Image
     img = np.zeros((2, 3, 3), np.uint8)
