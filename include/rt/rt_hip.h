@@ -127,6 +127,10 @@ typedef struct rt_scene_desc {
      * triangle; PackedTriangle::uv0..uv2 with has_texcoords), or NULL.  A
      * triangle whose first float is NaN has none and uses its barycentrics. */
     const float* triangle_uvs;
+    /* per-triangle vertex normals n0, n1, n2 (9 floats per triangle;
+     * PackedTriangle::n0..n2 with has_vertex_normals), or NULL; a NaN first
+     * float means none.  They set the shading normal (programs.cu:1535-1545). */
+    const float* triangle_normals;
 } rt_scene_desc;
 
 /* -------------------------------------------------------------- cameras

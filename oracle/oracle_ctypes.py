@@ -86,6 +86,7 @@ class Scene(C.Structure):
         ("texture_count", C.c_int32), ("image_texel_count", C.c_int32),
         ("textures", C.c_void_p), ("image_texels", C.c_void_p),
         ("material_albedo_texture", C.c_void_p), ("triangle_uvs", C.c_void_p),
+        ("triangle_normals", C.c_void_p),
     ]
 
 

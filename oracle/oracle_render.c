@@ -422,6 +422,23 @@ static void make_hit(const rto_scene* s, v3 o, v3 d, float t, int prim, hit_t* h
         h->pos = add3(o, scl3(d, t));
         set_face(d, n, h);
         mat = tr->material;
+        if (s->triangle_normals != NULL && s->triangle_normals[9 * (int64_t)ti] == s->triangle_normals[9 * (int64_t)ti]) {
+            /* interpolated vertex normals, programs.cu:1535-1545 */
+            const float* vn = s->triangle_normals + 9 * (int64_t)ti;
+            const v3 e1 = sub3(ld3(tr->p1), p0), e2 = sub3(ld3(tr->p2), p0);
+            const v3 pvec = cross3(d, e2);
+            const float inv_det = 1.0f / dot3(e1, pvec);
+            const v3 tvec = sub3(o, p0);
+            const float bu = dot3(tvec, pvec) * inv_det;
+            const float bv = dot3(d, cross3(tvec, e1)) * inv_det;
+            const float bw = 1.0f - bu - bv;
+            v3 sh = add3(scl3(ld3(vn), bw), add3(scl3(ld3(vn + 3), bu), scl3(ld3(vn + 6), bv)));
+            if (!(fabsf(sh.x) < 1e-7f && fabsf(sh.y) < 1e-7f && fabsf(sh.z) < 1e-7f)) {
+                sh = norm3(sh);
+                if (dot3(sh, n) < 0.0f) sh = scl3(sh, -1.0f);
+                h->sn = h->front ? sh : scl3(sh, -1.0f);
+            }
+        }
     }
     /* populate_material, programs.cu:961-1009 */
     if (mat < 0 || mat >= s->material_count) {

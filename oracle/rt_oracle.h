@@ -110,6 +110,7 @@ typedef struct {
     const float* image_texels;
     const int32_t* material_albedo_texture;
     const float* triangle_uvs;
+    const float* triangle_normals;
 } rto_scene;
 
 /* ---- cameras / profile ------------------------------------------------ */

@@ -128,8 +128,8 @@ struct rt_scene {
     int32_t openpbr_count = 0, analytic_count = 0;
     int32_t texture_count = 0, texel_count = 0;
     bool has_albedo_textures = false;
-    bool has_triangle_uvs = false;
-    DevBuf<float> tri_uv;
+    bool has_triangle_uvs = false, has_triangle_normals = false;
+    DevBuf<float> tri_uv, tri_n;
     DevBuf<rt_texture> textures;
     DevBuf<float> texels;
     DevBuf<int32_t> mat_albedo_tex;
@@ -196,6 +196,7 @@ struct rt_scene {
         v.texels = texels.p;
         v.mat_albedo_tex = has_albedo_textures ? mat_albedo_tex.p : nullptr;
         v.tri_uv = has_triangle_uvs ? tri_uv.p : nullptr;
+        v.tri_n = has_triangle_normals ? tri_n.p : nullptr;
         v.texture_count = texture_count;
         v.texel_count = texel_count;
         return v;
@@ -630,6 +631,9 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
     s->has_triangle_uvs = d->triangle_uvs != nullptr && d->triangle_count > 0;
     if (e == hipSuccess && s->has_triangle_uvs)
         e = upload(s->tri_uv, d->triangle_uvs, (size_t)d->triangle_count * 6, st);
+    s->has_triangle_normals = d->triangle_normals != nullptr && d->triangle_count > 0;
+    if (e == hipSuccess && s->has_triangle_normals)
+        e = upload(s->tri_n, d->triangle_normals, (size_t)d->triangle_count * 9, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         scene_release(s);

@@ -60,6 +60,7 @@ struct SceneView {
     const float* texels;               // 3 floats per texel
     const int32_t* mat_albedo_tex;     // nullptr: every albedo is a constant
     const float* tri_uv;               // 6 floats per triangle, or nullptr
+    const float* tri_n;                // 9 floats per triangle, or nullptr
     int32_t texture_count, texel_count;
 };
 

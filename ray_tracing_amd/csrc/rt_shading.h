@@ -70,8 +70,26 @@ __device__ inline void make_hit(const SceneView& s, f3 o, f3 d, float t, int pri
         h.prim_index = ti;
         const f3 p0 = ld3(tr.p0);
         h.pos = o + d * t;
-        set_face(d, normalize3(cross3(ld3(tr.p1) - p0, ld3(tr.p2) - p0)), h);
+        const f3 normal = normalize3(cross3(ld3(tr.p1) - p0, ld3(tr.p2) - p0));
+        set_face(d, normal, h);
         mat = tr.material;
+        if (s.tri_n != nullptr && s.tri_n[9 * (int64_t)ti] == s.tri_n[9 * (int64_t)ti]) {
+            // interpolated vertex normals (programs.cu:1535-1545)
+            const float* vn = s.tri_n + 9 * (int64_t)ti;
+            const f3 e1 = ld3(tr.p1) - p0, e2 = ld3(tr.p2) - p0;
+            const f3 pvec = cross3(d, e2);
+            const float inv_det = 1.0f / dot3(e1, pvec);
+            const f3 tvec = o - p0;
+            const float bu = dot3(tvec, pvec) * inv_det;
+            const float bv = dot3(d, cross3(tvec, e1)) * inv_det;
+            const float bw = 1.0f - bu - bv;
+            f3 sh = ld3(vn) * bw + (ld3(vn + 3) * bu + ld3(vn + 6) * bv);
+            if (!(fabsf(sh.x) < 1e-7f && fabsf(sh.y) < 1e-7f && fabsf(sh.z) < 1e-7f)) {
+                sh = normalize3(sh);
+                if (dot3(sh, normal) < 0.0f) sh = sh * -1.0f;
+                h.sn = h.front ? sh : sh * -1.0f;
+            }
+        }
     }
     if (mat < 0 || mat >= s.material_count) {
         h.base = mk3(0, 0, 0); h.emission = mk3(0, 0, 0); h.mat_type = 0; h.ior = 1.0f;

@@ -407,7 +407,7 @@ def import_obj_mtl(path, out: SceneDefinition):
     it has one (obj_mtl_importer.cpp:202-224)."""
     path = os.path.normpath(path)
     root = os.path.dirname(path)
-    verts, uvs, groups, order, mats, cur = [], [], {}, [], {}, -1
+    verts, uvs, norms, groups, order, mats, cur = [], [], [], {}, [], {}, -1
     with open(path) as f:
         for line in f:
             tok = line.split()
@@ -417,6 +417,8 @@ def import_obj_mtl(path, out: SceneDefinition):
                 verts.append([float(v) for v in tok[1:4]])
             elif tok[0] == "vt":
                 uvs.append([float(v) for v in tok[1:3]])
+            elif tok[0] == "vn":
+                norms.append([float(v) for v in tok[1:4]])
             elif tok[0] == "mtllib":
                 mp = os.path.normpath(os.path.join(root, " ".join(tok[1:])))
                 mats.update(_parse_mtl(mp))
@@ -426,7 +428,7 @@ def import_obj_mtl(path, out: SceneDefinition):
                 name = " ".join(tok[1:])
                 cur = list(mats).index(name) if name in mats else -1
             elif tok[0] == "f":
-                idx, tidx = [], []
+                idx, tidx, nidx = [], [], []
                 for c in tok[1:]:
                     parts = c.split("/")
                     i = int(parts[0])
@@ -434,30 +436,37 @@ def import_obj_mtl(path, out: SceneDefinition):
                         raise ValueError("negative obj vertex indices are not supported")
                     idx.append(i - 1)
                     tidx.append(int(parts[1]) - 1 if len(parts) > 1 and parts[1] else -1)
+                    nidx.append(int(parts[2]) - 1 if len(parts) > 2 and parts[2] else -1)
                 if len(idx) < 3:
                     raise ValueError("only triangulated faces are supported")
                 if cur not in groups:
                     groups[cur] = []
                     order.append(cur)
                 for k in range(1, len(idx) - 1):
-                    groups[cur].append((idx[0], idx[k], idx[k + 1], tidx[0], tidx[k], tidx[k + 1]))
+                    groups[cur].append((idx[0], idx[k], idx[k + 1], tidx[0], tidx[k], tidx[k + 1],
+                                        nidx[0], nidx[k], nidx[k + 1]))
     V = np.array(verts, np.float64).reshape(-1, 3)
     UV = np.array(uvs, np.float64).reshape(-1, 2)
+    N = np.array(norms, np.float64).reshape(-1, 3)
     for gi in sorted(order):
-        rows = np.array(groups[gi], np.int64).reshape(-1, 6)
-        tris, tuv = rows[:, :3], rows[:, 3:]
+        rows = np.array(groups[gi], np.int64).reshape(-1, 9)
+        tris, tuv, tn = rows[:, :3], rows[:, 3:6], rows[:, 6:]
         if tris.size and (tris.max() >= len(V)):
             raise ValueError("obj face index out of range")
         if tuv.size and tuv.max() >= len(UV):
             raise ValueError("obj texture coordinate index out of range")
+        if tn.size and tn.max() >= len(N):
+            raise ValueError("obj normal index out of range")
         texcoords = UV[tuv] if len(UV) and tuv.size and (tuv >= 0).all() else None
+        normals = N[tn] if len(N) and tn.size and (tn >= 0).all() else None
         m = mats[list(mats)[gi]] if gi >= 0 else None
         if m is not None and m["map_Kd"] is not None:
             out.textures.append({"type": "image", "path": os.path.normpath(os.path.join(root, m["map_Kd"]))})
         else:
             out.textures.append({"type": "constant", "color": m["Kd"] if m is not None else np.ones(3)})
         out.materials.append({"type": "diffuse", "albedo": len(out.textures) - 1})
-        out.shapes.append({"type": "mesh", "positions": V, "triangles": tris, "texcoords": texcoords})
+        out.shapes.append({"type": "mesh", "positions": V, "triangles": tris, "texcoords": texcoords,
+                           "normals": normals})
         out.instances.append((len(out.shapes) - 1, len(out.materials) - 1, Transform()))
 
 
@@ -549,6 +558,7 @@ def load_scene_definition(path) -> SceneDefinition:
 
 TEX_TYPES = {"constant": 0, "checker": 1, "image": 2, "noise": 3}
 NO_UV = np.full(6, np.nan)      # a triangle without texture coordinates
+NO_N = np.full(9, np.nan)       # a triangle without vertex normals
 
 
 def _emission_color(defn: SceneDefinition, tex: int, path: str) -> np.ndarray:
@@ -660,6 +670,7 @@ class FlatScene:
     texels: object = None
     albedo_bindings: object = None
     triangle_uvs: object = None
+    triangle_normals: object = None
 
 
 def flatten(defn: SceneDefinition) -> FlatScene:
@@ -674,7 +685,7 @@ def flatten(defn: SceneDefinition) -> FlatScene:
         mat, binding = _material(defn, m, path)
         mats.append(mat)
         bindings.append(binding)
-    spheres, quads, tris, tri_uv = [], [], [], []
+    spheres, quads, tris, tri_uv, tri_n = [], [], [], [], []
     for shape_i, mat_i, tf in defn.instances:
         s = defn.shapes[shape_i]
         if s["type"] == "sphere":
@@ -692,6 +703,8 @@ def flatten(defn: SceneDefinition) -> FlatScene:
                 tris.append(abi.Triangle(abi.f3(*tf.point(P[a])), abi.f3(*tf.point(P[b])),
                                          abi.f3(*tf.point(P[c])), mat_i))
                 tri_uv.append(tc[k].reshape(6) if tc is not None else NO_UV)
+                vn = s.get("normals")
+                tri_n.append(np.concatenate([tf.vector(x) for x in vn[k]]) if vn is not None else NO_N)
 
     def arr(T, items):
         return (T * len(items))(*items) if items else None
@@ -719,6 +732,9 @@ def flatten(defn: SceneDefinition) -> FlatScene:
     if any(not np.isnan(u[0]) for u in tri_uv):
         fs.triangle_uvs = np.ascontiguousarray(np.array(tri_uv, np.float32).reshape(-1, 6))
         d.triangle_uvs = fs.triangle_uvs.ctypes.data_as(C.c_void_p)
+    if any(not np.isnan(n[0]) for n in tri_n):
+        fs.triangle_normals = np.ascontiguousarray(np.array(tri_n, np.float32).reshape(-1, 9))
+        d.triangle_normals = fs.triangle_normals.ctypes.data_as(C.c_void_p)
     return fs
 
 

@@ -24,6 +24,7 @@ def oracle_scene(oracle, desc):
     s.textures, s.image_texels = desc.textures, desc.image_texels
     s.material_albedo_texture = desc.material_albedo_texture
     s.triangle_uvs = desc.triangle_uvs
+    s.triangle_normals = desc.triangle_normals
     return s
 
 

@@ -111,6 +111,8 @@ def textured_cornell(rt):
     # the first triangle has per-vertex coordinates (repeating the image 3 x 2
     # times), the second none (NaN: barycentrics)
     tri_uv = np.array([[0.0, 0.0, 3.0, 0.0, 1.5, 2.0], [np.nan] * 6], np.float32)
+    # vertex normals on the second triangle (tilted, differing per vertex)
+    tri_n = np.array([[np.nan] * 9, [0.3, 0.2, -1.0, -0.2, 0.1, -1.0, 0.0, -0.4, -1.0]], np.float32)
     mat_arr = (abi.Material * k)(*mats)
     tex_arr = (abi.Texture * len(T))(*T)
     bind_arr = np.asarray(bind, np.int32)
@@ -118,6 +120,7 @@ def textured_cornell(rt):
     C.memmove(C.byref(desc), C.byref(flat.desc), C.sizeof(desc))
     desc.sphere_count, desc.triangle_count, desc.material_count = 1, 2, k
     desc.triangle_uvs = tri_uv.ctypes.data_as(C.c_void_p)
+    desc.triangle_normals = tri_n.ctypes.data_as(C.c_void_p)
     desc.spheres = C.cast(spheres, C.c_void_p)
     desc.triangles = C.cast(tris, C.c_void_p)
     desc.materials = C.cast(mat_arr, C.c_void_p)
@@ -125,7 +128,7 @@ def textured_cornell(rt):
     desc.textures = C.cast(tex_arr, C.c_void_p)
     desc.image_texels = texels.ctypes.data_as(C.c_void_p)
     desc.material_albedo_texture = bind_arr.ctypes.data_as(C.c_void_p)
-    keep = (flat, spheres, tris, tri_uv, mat_arr, tex_arr, bind_arr, texels)
+    keep = (flat, spheres, tris, tri_uv, tri_n, mat_arr, tex_arr, bind_arr, texels)
     return desc, keep, S.camera_rig(d, "default")
 
 
@@ -141,6 +144,8 @@ def test_checker_and_image_textures_match_oracle(rt, oracle, device):
         e = rel_mse(a, b)
         say(f"  depth {depth}: beauty bit-exact {same:.6f}, albedo bit-exact {alb:.6f}, relMSE {e:.2e}")
         assert b.mean() > 0.01
-        assert same > 0.99 and alb > 0.995 and e < 1e-3
+        nrm = np.mean(gpu["normal"] == ref["normal"])
+        say(f"  depth {depth}: normal bit-exact {nrm:.6f}")
+        assert same > 0.99 and alb > 0.995 and e < 1e-3 and nrm > 0.995
     # the textures show: the primary albedo takes many values
     assert len(np.unique(ref["albedo"][..., 0])) > 200

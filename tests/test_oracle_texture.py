@@ -134,3 +134,30 @@ def test_surface_uv():
     tuv[0, 0] = np.nan                     # no coordinates: barycentrics
     uv = O.surface_uv(s, (0.6, 0.4, 5), (0, 0, -1), 2, 0, (0.6, 0.4, 0))
     assert uv == pytest.approx((0.3, 0.2), abs=1e-6)
+
+
+def test_vertex_normals_set_the_shading_normal():
+    """programs.cu:1535-1545: with one tilted normal n at every vertex, each
+    primary hit on that triangle gets the shading normal +-normalize(n)
+    (the sign facing the viewer); the normal AOV (0.5 s + 0.5) changes
+    exactly on those pixels."""
+    import ray_tracing_amd as rt
+    from helpers import oracle_camera, oracle_profile, oracle_scene
+    from test_gpu_textures import textured_cornell
+    desc, keep, rig = textured_cornell(rt)
+    prof = rt.Generated("cornell_mesh").profile
+    prof.samples_per_pixel, prof.max_bounces, prof.rr_start_bounce = 1, 1, 1
+    cam = oracle_camera(O, rig.cameras[0])
+    nv = np.array([0.3, 0.2, -1.0], np.float32)
+    out = []
+    for normals in (None, np.array([[np.nan] * 9, list(nv) * 3], np.float32)):
+        desc.triangle_normals = normals.ctypes.data_as(C.c_void_p) if normals is not None else None
+        osc = oracle_scene(O, desc)
+        out.append(O.render_camera(osc, O.OracleBvh(osc), cam, 0, oracle_profile(O, prof), 3,
+                                   threads=8)["normal"][..., :3])
+    changed = np.any(out[0] != out[1], axis=-1)
+    assert changed.sum() > 500
+    s = nv / np.linalg.norm(nv)
+    enc = out[1][changed]
+    ok = np.minimum(np.abs(enc - (0.5 * s + 0.5)).max(-1), np.abs(enc - (-0.5 * s + 0.5)).max(-1))
+    assert ok.max() < 2e-6

@@ -102,7 +102,7 @@ def test_obj_texture_coordinates(tmp_path):
     Image.fromarray(np.full((2, 2, 3), 128, np.uint8)).save(str(tmp_path / "kd.png"))
     write(str(tmp_path / "m.mtl"), "newmtl tex\nKd 1 1 1\nmap_Kd kd.png\nnewmtl plain\nKd 0.5 0.5 0.5\n")
     write(str(tmp_path / "m.obj"), "mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\n"
-          "vt 0 0\nvt 1 0\nvt 1 1\nvt 0 1\nusemtl tex\nf 1/1 2/2 3/3 4/4\n"
+          "vt 0 0\nvt 1 0\nvt 1 1\nvt 0 1\nvn 0 0.6 0.8\nusemtl tex\nf 1/1/1 2/2/1 3/3/1 4/4/1\n"
           "usemtl plain\nf 1/1 3 4/4\n")
     p = str(tmp_path / "scene.yaml")
     write(p, BASIC + "imports:\n  mesh:\n    type: obj_mtl\n    obj: m.obj\n")
@@ -113,6 +113,8 @@ def test_obj_texture_coordinates(tmp_path):
     # misses a corner's vt, so its triangle has none (NaN)
     assert uv[0].tolist() == [0, 0, 1, 0, 1, 1] and uv[1].tolist() == [0, 0, 1, 1, 0, 1]
     assert np.isnan(uv[2][0])
+    n = f.triangle_normals
+    assert n[0].tolist() == pytest.approx([0, 0.6, 0.8] * 3) and np.isnan(n[2][0])
     assert f.desc.texture_count >= 2 and (f.albedo_bindings >= 0).sum() == 1
 
 
