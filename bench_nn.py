@@ -8,7 +8,7 @@ the encoder-decoder (2 * 9 * cin * cout per pixel at each level, a level
 having a quarter of the pixels of the one above) over the measured time, and
 the fraction of MI355X's dense BF16 MFMA peak (2.5 PFLOP/s) that is.
 
-    python -m ray_tracing_amd.nn_timing            # 4x640x480, 4x1280x720, 8x256^2 training
+    python bench_nn.py            # 4x640x480, 4x1280x720, 8x256^2 training
 """
 from __future__ import annotations
 
@@ -22,7 +22,7 @@ import time
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, HERE)
 
 import ray_tracing_amd as rt  # noqa: E402
 from ray_tracing_amd import abi  # noqa: E402
@@ -112,7 +112,7 @@ def main():
     a = ap.parse_args()
     import torch
     if not torch.cuda.is_available():
-        raise SystemExit("nn_timing needs a GPU")
+        raise SystemExit("bench_nn needs a GPU")
     weights = os.path.join(HERE, "weights", "denoiser.npy")
     nn = rt.Denoiser(0, np.load(weights))
     res = []

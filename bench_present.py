@@ -3,7 +3,7 @@ frames already in HBM, RGBA8 out, `--iters` back-to-back launches on the
 device's stream between two synchronisations.  Algorithmic traffic is 20
 bytes per pixel (16 read, 4 written).
 
-    python -m ray_tracing_amd.present_timing [--pixels N ...] [--iters K]
+    python bench_present.py [--pixels N ...] [--iters K]
 """
 import argparse
 import json

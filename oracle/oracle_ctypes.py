@@ -233,8 +233,10 @@ def cache_loss_grad(g: CacheGrid, params, inputs, targets, bf16=True, want_grad=
 
 
 def build() -> str:
-    """Compile liboracle.so with the committed Makefile (gcc)."""
+    """Compile liboracle.so with the committed Makefile (gcc) and, when the
+    reference tree is present, oracle/_ref/libref.so from its headers."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
+    subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
     return LIB_PATH
 
 

@@ -76,8 +76,15 @@ def build(jobs: int = 4, verbose: bool = True, out: str = LIB, defines=()) -> st
 
 
 HOST = os.path.join(HERE, "host")
-HOST_TARGETS = {"render_realtime": ["render_realtime.cpp", "rt_host.cpp"],
-                "test_report": ["test_report.cpp", "rt_host.cpp"]}
+TESTS_HOST = os.path.join(os.path.dirname(HERE), "tests", "host")
+# (output path, sources): the CLI beside the library, the report test binary
+# beside its source under tests/
+HOST_TARGETS = {"render_realtime": (os.path.join(HERE, "render_realtime"),
+                                    [os.path.join(HOST, "render_realtime.cpp"),
+                                     os.path.join(HOST, "rt_host.cpp")]),
+                "test_report": (os.path.join(TESTS_HOST, "test_report"),
+                                [os.path.join(TESTS_HOST, "test_report.cpp"),
+                                 os.path.join(HOST, "rt_host.cpp")])}
 
 
 def source_provenance():
@@ -106,11 +113,11 @@ def build_host(verbose: bool = True):
     against librt_hip.so (found next to the executables' parent directory)."""
     outs = []
     revision, state = source_provenance()
-    for name, srcs in HOST_TARGETS.items():
-        out = os.path.join(HERE, name)
+    for name, (out, srcs) in HOST_TARGETS.items():
+        rpath = os.path.relpath(HERE, os.path.dirname(out))
         cmd = ["g++", "-std=c++20", "-O2", "-Wall", f'-DRT_SOURCE_REVISION="{revision}"',
-               f'-DRT_SOURCE_STATE_SHA256="{state}"', "-o", out] + [os.path.join(HOST, s) for s in srcs]
-        cmd += ["-L", HERE, "-lrt_hip", "-Wl,-rpath,$ORIGIN"]
+               f'-DRT_SOURCE_STATE_SHA256="{state}"', "-I", HOST, "-o", out] + srcs
+        cmd += ["-L", HERE, "-lrt_hip", "-Wl,-rpath,$ORIGIN/" + rpath]
         _run(cmd)
         outs.append(out)
         if verbose:

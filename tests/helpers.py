@@ -114,3 +114,19 @@ def rel_mse(x, ref, eps=1e-2):
     x = np.asarray(x, np.float64)
     ref = np.asarray(ref, np.float64)
     return float(np.mean((x - ref) ** 2 / (ref ** 2 + eps)))
+
+
+def threshold_neighbourhood(ulps: int = 64) -> np.ndarray:
+    """Every float within `ulps` of each input where the output steps."""
+    k = np.arange(255, dtype=np.float64)
+    t = np.float32(((k + 0.5) / 255.0) ** 2)
+    bits = t.view(np.int32)[:, None] + np.arange(-ulps, ulps + 1, dtype=np.int32)[None, :]
+    return bits.ravel().view(np.float32)
+
+
+def sweep() -> np.ndarray:
+    rng = np.random.default_rng(5)
+    dense = np.linspace(-2.0, 8.0, 2_000_001, dtype=np.float32)
+    rand = rng.uniform(0.0, 1.0, 1_000_000).astype(np.float32)
+    special = np.array([0.0, -0.0, 1e-45, 1e-38, 0.998001, 0.999, 1.0, 3.4e38], np.float32)
+    return np.concatenate([threshold_neighbourhood(), dense, rand, special])

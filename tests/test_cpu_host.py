@@ -9,7 +9,7 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-BIN = os.path.join(ROOT, "ray_tracing_amd", "test_report")
+BIN = os.path.join(ROOT, "tests", "host", "test_report")
 
 
 @pytest.fixture(scope="module")

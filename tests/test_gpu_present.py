@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 from hipmem import DeviceArray, sync
-from test_oracle_display import sweep
+from helpers import sweep
 
 pytestmark = pytest.mark.gpu
 

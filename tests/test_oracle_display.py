@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import oracle_ctypes as O
+from helpers import sweep, threshold_neighbourhood  # noqa: F401
 
 needs_ref = pytest.mark.skipif(O.ref_lib() is None,
                                reason="oracle/_ref not built (reference tree absent)")
@@ -34,22 +35,6 @@ def numpy_u8(v: np.ndarray) -> np.ndarray:
     d = np.minimum(d, np.float32(0.999))
     x = (d * np.float32(255.0)).astype(np.float32).astype(np.float64)
     return np.floor(x + 0.5).astype(np.uint8)      # x >= 0: half away from zero
-
-
-def threshold_neighbourhood(ulps: int = 64) -> np.ndarray:
-    """Every float within `ulps` of each input where the output steps."""
-    k = np.arange(255, dtype=np.float64)
-    t = np.float32(((k + 0.5) / 255.0) ** 2)
-    bits = t.view(np.int32)[:, None] + np.arange(-ulps, ulps + 1, dtype=np.int32)[None, :]
-    return bits.ravel().view(np.float32)
-
-
-def sweep() -> np.ndarray:
-    rng = np.random.default_rng(5)
-    dense = np.linspace(-2.0, 8.0, 2_000_001, dtype=np.float32)
-    rand = rng.uniform(0.0, 1.0, 1_000_000).astype(np.float32)
-    special = np.array([0.0, -0.0, 1e-45, 1e-38, 0.998001, 0.999, 1.0, 3.4e38], np.float32)
-    return np.concatenate([threshold_neighbourhood(), dense, rand, special])
 
 
 def test_reference_known_answers():
