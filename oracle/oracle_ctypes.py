@@ -333,6 +333,20 @@ def lib():
         L.rto_cache_loss_grad.argtypes = [P(CacheGrid), C.c_void_p, C.c_int, C.c_void_p,
                                           C.c_void_p, C.c_int, C.c_void_p]
         L.rto_cache_loss_grad.restype = C.c_double
+        for name in ("rto_power_heuristic", "rto_area_pdf", "rto_cone_pdf",
+                     "rto_uniform_sphere_pdf"):
+            getattr(L, name).restype = C.c_float
+        L.rto_power_heuristic.argtypes = [C.c_float, C.c_float]
+        L.rto_cone_pdf.argtypes = [C.c_float]
+        L.rto_area_pdf.argtypes = [C.c_float, C.c_float, C.c_float]
+        L.rto_uniform_sphere_pdf.argtypes = []
+        L.rto_pick_cdf.argtypes = [C.c_void_p, C.c_int, C.c_float]
+        L.rto_triangle_barycentrics.argtypes = [C.c_float, C.c_float, C.c_void_p]
+        L.rto_triangle_barycentrics.restype = None
+        L.rto_sample_light_at.argtypes = [P(Scene), P(Bvh), C.c_int] + [C.c_void_p] * 5
+        L.rto_light_pdf_for_hit.argtypes = [P(Scene), P(Bvh), C.c_int, C.c_int] + [C.c_void_p] * 3
+        L.rto_light_pdf_for_hit.restype = C.c_float
+        L.rto_scatter.argtypes = [P(Scene), C.c_int] + [C.c_void_p] * 5
         _lib = L
     return _lib
 
@@ -724,3 +738,69 @@ def sincos_turn(u: float):
     c = C.c_float()
     lib().rto_sincos_turn(u, C.byref(s), C.byref(c))
     return s.value, c.value
+
+
+def pinhole_camera(fx, fy, cx, cy, k1=0.0, k2=0.0, k3=0.0, p1=0.0, p2=0.0, width=640,
+                   height=480) -> Camera:
+    """A pinhole32 camera at the origin looking down +z (identity basis)."""
+    c = Camera()
+    c.enabled, c.width, c.height, c.model = 1, width, height, 0
+    c.basis_x[:] = (1.0, 0.0, 0.0)
+    c.basis_y[:] = (0.0, 1.0, 0.0)
+    c.basis_z[:] = (0.0, 0.0, 1.0)
+    c.fx, c.fy, c.cx, c.cy = fx, fy, cx, cy
+    c.k1, c.k2, c.k3, c.p1, c.p2 = k1, k2, k3, p1, p2
+    return c
+
+
+def equi_camera(width, height, fx, fy, cx, cy, radial=(0.0,) * 6, tangential=(0.0, 0.0),
+                build_lut=True) -> Camera:
+    """An equi62_lut1d camera (identity basis); the LUT is built by the
+    oracle's restatement of make_equi62_lut1d_params unless build_lut=False."""
+    c = Camera()
+    c.enabled, c.width, c.height, c.model = 1, width, height, 1
+    c.basis_x[:] = (1.0, 0.0, 0.0)
+    c.basis_y[:] = (0.0, 1.0, 0.0)
+    c.basis_z[:] = (0.0, 0.0, 1.0)
+    c.fx, c.fy, c.cx, c.cy = fx, fy, cx, cy
+    c.radial[:] = tuple(radial)
+    c.tangential[:] = tuple(tangential)
+    if build_lut:
+        lib().rto_make_equi_lut(C.byref(c))
+    return c
+
+
+def unproject(cam: Camera, px: float, py: float) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().rto_unproject(C.byref(cam), float(px), float(py), ptr(out))
+    return out
+
+
+def project(cam: Camera, d) -> np.ndarray:
+    d = np.ascontiguousarray(d, np.float32)
+    out = np.zeros(2, np.float32)
+    lib().rto_project(C.byref(cam), ptr(d), ptr(out))
+    return out
+
+
+def sample_light_at(scene: Scene, bvh: "OracleBvh", prim: int, pos, gn, d, r):
+    """NEE sample from a surface point: (light index or -1, dir, emission, dist, pdf)."""
+    a = [np.ascontiguousarray(v, np.float32) for v in (pos, gn, d, r)]
+    out = np.zeros(8, np.float32)
+    li = lib().rto_sample_light_at(C.byref(scene), C.byref(bvh.bvh), prim, *[ptr(v) for v in a],
+                                   ptr(out))
+    return li, out[0:3], out[3:6], float(out[6]), float(out[7])
+
+
+def light_pdf_for_hit(scene: Scene, bvh: "OracleBvh", prim: int, prev_mat: int, pos, sn, d) -> float:
+    a = [np.ascontiguousarray(v, np.float32) for v in (pos, sn, d)]
+    return float(lib().rto_light_pdf_for_hit(C.byref(scene), C.byref(bvh.bvh), prim, prev_mat,
+                                             *[ptr(v) for v in a]))
+
+
+def scatter(scene: Scene, prim: int, pos, gn, d, r):
+    """One BSDF sample: (alive, new direction, throughput factor, pdf, delta)."""
+    a = [np.ascontiguousarray(v, np.float32) for v in (pos, gn, d, r)]
+    out = np.zeros(8, np.float32)
+    alive = lib().rto_scatter(C.byref(scene), prim, *[ptr(v) for v in a], ptr(out))
+    return bool(alive), out[0:3], out[3:6], float(out[6]), bool(out[7])

@@ -507,26 +507,29 @@ static float eff_selection_pdf(const ctx_t* c, int light, int excluded_prim) {
     return elig > 1e-8f ? c->lights[light].selection_pdf / elig : 0.0f;
 }
 
-/* programs.cu:584-647 */
-static float emission_mis(const ctx_t* c, const path_t* st, const hit_t* h, int env) {
-    if (!st->prev_valid || st->prev_delta) return 1.0f;
+/* conditional_light_pdf_for_hit (programs.cu:584-619) times the effective
+ * selection pdf: the solid-angle density with which next-event estimation
+ * from (prev_pos, prev_n, prev_mat, prev_prim) picks the point the ray in
+ * direction `dir` hit (h; NULL with env for a miss) */
+static float light_pdf_for_hit(const ctx_t* c, v3 prev_pos, v3 prev_n, int prev_mat, int prev_prim,
+    v3 dir, const hit_t* h, int env, int* light_out) {
     const int light = env ? c->env_light : c->light_of_prim[h->prim];
-    if (light < 0) return 1.0f;
-    const v3 dir = norm3(st->d);
+    *light_out = light;
+    if (light < 0) return 0.0f;
     float cond;
     if (env) {
-        cond = st->prev_mat == RTO_MAT_ISOTROPIC ? UNIFORM_SPHERE_PDF
-                                                 : fmaxf(dot3(st->prev_n, dir), 0.0f) / OM_PI;
+        cond = prev_mat == RTO_MAT_ISOTROPIC ? UNIFORM_SPHERE_PDF
+                                             : fmaxf(dot3(prev_n, dir), 0.0f) / OM_PI;
     } else {
         const rto_scene* s = c->s;
         if (!h->front) cond = 0.0f;
         else if (h->prim_type == 0) {
             const rto_sphere* sp = &s->spheres[h->prim_index];
-            const float dsq = lensq3(sub3(ld3(sp->center), st->prev_pos));
+            const float dsq = lensq3(sub3(ld3(sp->center), prev_pos));
             const float rsq = sp->radius * sp->radius;
             cond = dsq <= rsq ? 0.0f : cone_pdf(sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq)));
         } else {
-            const float dsq = lensq3(sub3(h->pos, st->prev_pos));
+            const float dsq = lensq3(sub3(h->pos, prev_pos));
             const float ac = fmaxf(dot3(h->gn, scl3(dir, -1.0f)), 0.0f);
             float area;
             if (h->prim_type == 1) {
@@ -539,8 +542,18 @@ static float emission_mis(const ctx_t* c, const path_t* st, const hit_t* h, int 
             cond = area_pdf(area, dsq, ac);
         }
     }
-    const float sel = eff_selection_pdf(c, light, st->prev_prim);
-    return power_heuristic(st->prev_pdf, sel * cond);
+    const float sel = eff_selection_pdf(c, light, prev_prim);
+    return sel * cond;
+}
+
+/* emission_mis_weight (programs.cu:621-647) */
+static float emission_mis(const ctx_t* c, const path_t* st, const hit_t* h, int env) {
+    if (!st->prev_valid || st->prev_delta) return 1.0f;
+    int light;
+    const float lpdf = light_pdf_for_hit(c, st->prev_pos, st->prev_n, st->prev_mat, st->prev_prim,
+        norm3(st->d), h, env, &light);
+    if (light < 0) return 1.0f;
+    return power_heuristic(st->prev_pdf, lpdf);
 }
 
 typedef struct { v3 dir, emission; float dist, pdf; int infinite, valid; } light_sample_t;
@@ -557,6 +570,25 @@ static v3 cone_dir(v3 axis, float cos_max, float u0, float u1) {
     return norm3(add3(scl3(axis, ct), add3(scl3(tangent, st * cp), scl3(bitangent, st * sp))));
 }
 
+/* sample_packed_light / sample_packed_analytic_light (light_sampling.h:87-116):
+ * the first entry whose CDF exceeds u clamped to [0, 1), else the last;
+ * cdf values are `stride` floats apart */
+static int pick_cdf(const float* cdf, int stride, int n, float u) {
+    if (cdf == NULL || n <= 0) return -1;
+    const float s = u < 0.0f ? 0.0f : (u < 1.0f ? u : 0.99999994f);
+    for (int i = 0; i < n; ++i)
+        if (s < cdf[(size_t)i * (size_t)stride]) return i;
+    return n - 1;
+}
+
+/* sample_uniform_triangle (light_sampling.h:118-125) */
+static void triangle_barycentrics(float u0, float u1, float* w0, float* w1, float* w2) {
+    const float su = sqrtf(u0 < 0.0f ? 0.0f : (u0 < 1.0f ? u0 : 1.0f));
+    *w0 = 1.0f - su;
+    *w1 = su * (1.0f - u1);
+    *w2 = su * u1;
+}
+
 /* programs.cu:1057-1232 */
 static light_sample_t sample_light(const ctx_t* c, const hit_t* h, const float r[4]) {
     light_sample_t ls;
@@ -567,11 +599,8 @@ static light_sample_t sample_light(const ctx_t* c, const hit_t* h, const float r
     if (elig <= 1e-8f) return ls;
     int li = -1;
     if (ex < 0) {
-        const float u = r[0] < 0.0f ? 0.0f : (r[0] < 1.0f ? r[0] : 0.99999994f);
-        for (int i = 0; i < c->light_count; ++i) {
-            if (u < c->lights[i].cdf) { li = i; break; }
-        }
-        if (li < 0) li = c->light_count - 1;
+        li = pick_cdf(&c->lights[0].cdf, (int)(sizeof(rto_light) / sizeof(float)), c->light_count,
+            r[0]);
     } else {
         const float target = r[0] * elig;
         float acc = 0.0f;
@@ -643,8 +672,8 @@ static light_sample_t sample_light(const ctx_t* c, const hit_t* h, const float r
     } else {
         const rto_triangle* t = &s->triangles[L->prim];
         const v3 p0 = ld3(t->p0), p1 = ld3(t->p1), p2 = ld3(t->p2);
-        const float su = sqrtf(r[1] < 0.0f ? 0.0f : (r[1] < 1.0f ? r[1] : 1.0f));
-        const float w0 = 1.0f - su, w1 = su * (1.0f - r[2]), w2 = su * r[2];
+        float w0, w1, w2;
+        triangle_barycentrics(r[1], r[2], &w0, &w1, &w2);
         const v3 pos = add3(scl3(p0, w0), add3(scl3(p1, w1), scl3(p2, w2)));
         const v3 an = cross3(sub3(p1, p0), sub3(p2, p0));
         const float da = len3(an);
@@ -761,10 +790,7 @@ static float analytic_mis(const ctx_t* c, const path_t* st, int li) {
 /* sample_packed_analytic_light (light_sampling.h:104-116) */
 static int pick_analytic(const ctx_t* c, float u) {
     if (c->a_count <= 0) return -1;
-    const float s = u < 0.0f ? 0.0f : (u < 1.0f ? u : 0.99999994f);
-    for (int i = 0; i < c->a_count; ++i)
-        if (s < c->a_cdf[i]) return i;
-    return c->a_count - 1;
+    return pick_cdf(c->a_cdf, 1, c->a_count, u);
 }
 
 /* sample_analytic_direct_light_candidate, sphere branch (programs.cu:1234-1300) */
@@ -1372,4 +1398,114 @@ int rto_render_camera_ex(const rto_scene* s, const rto_bvh* bvh, const rto_camer
     ctx_free(&c);
     pthread_mutex_unlock(&once);
     return 0;
+}
+
+/* ------------------------------------------------------ test entry points */
+/* The integrator's light-sampling densities, CDF selection and scattering,
+ * exposed so that tests can pin them against the reference's known answers,
+ * the reference headers compiled in place, and closed forms. */
+float rto_power_heuristic(float a, float b) { return power_heuristic(a, b); }
+float rto_cone_pdf(float cos_max) { return cone_pdf(cos_max); }
+float rto_area_pdf(float area, float dsq, float abs_cos) { return area_pdf(area, dsq, abs_cos); }
+float rto_uniform_sphere_pdf(void) { return UNIFORM_SPHERE_PDF; }
+int rto_pick_cdf(const float* cdf, int n, float u) { return pick_cdf(cdf, 1, n, u); }
+void rto_triangle_barycentrics(float u0, float u1, float w[3]) {
+    triangle_barycentrics(u0, u1, &w[0], &w[1], &w[2]);
+}
+
+static void test_hit(const rto_scene* s, int prim, const float pos[3], const float gn[3],
+    const float d[3], hit_t* h) {
+    memset(h, 0, sizeof(*h));
+    const int ns = s->sphere_count, nq = s->quad_count;
+    h->prim = prim;
+    h->prim_type = prim < ns ? 0 : (prim < ns + nq ? 1 : 2);
+    h->prim_index = prim < ns ? prim : (prim < ns + nq ? prim - ns : prim - ns - nq);
+    h->pos = ld3(pos);
+    set_face(ld3(d), ld3(gn), h);
+    const int mat = h->prim_type == 0 ? s->spheres[h->prim_index].material
+                  : h->prim_type == 1 ? s->quads[h->prim_index].material
+                                      : s->triangles[h->prim_index].material;
+    const rto_material* m = &s->materials[mat];
+    h->mat_type = m->type;
+    h->base = ld3(m->albedo);
+    h->fuzz = m->fuzz;
+    h->ior = m->ior;
+    h->openpbr = m->openpbr_index;
+    if (h->mat_type == RTO_MAT_DIELECTRIC) h->base = mk3(0.92f, 0.95f, 1.0f);
+    if (h->mat_type == RTO_MAT_OPENPBR) h->base = ld3(s->openpbr[m->openpbr_index].base_color);
+}
+
+/* NEE from a point of primitive `prim` (global id) with outward geometric
+ * normal gn, seen along d: out = dir[3], emission[3], dist, pdf; returns
+ * the light index (or -1 when the sample is invalid) */
+int rto_sample_light_at(const rto_scene* s, const rto_bvh* bvh, int prim, const float pos[3],
+    const float gn[3], const float d[3], const float r[4], float out[8]) {
+    ctx_t* c = (ctx_t*)malloc(sizeof(ctx_t));
+    if (c == NULL || ctx_init(c, s, bvh) != 0) { free(c); return -2; }
+    hit_t h;
+    test_hit(s, prim, pos, gn, d, &h);
+    const light_sample_t L = sample_light(c, &h, r);
+    out[0] = L.dir.x; out[1] = L.dir.y; out[2] = L.dir.z;
+    out[3] = L.emission.x; out[4] = L.emission.y; out[5] = L.emission.z;
+    out[6] = L.dist; out[7] = L.pdf;
+    int li = -1;
+    if (L.valid) {
+        /* the light the sample landed on: trace it (-3 when something else,
+         * another light included, is hit first) */
+        float t, u, v;
+        int hp;
+        const int found = om_closest(s, bvh, h.pos, L.dir, OM_RAY_EPS, OM_RAY_FAR, &t, &hp, &u, &v);
+        if (L.infinite) li = found ? -3 : c->env_light;
+        else if (!found || t < L.dist * 0.999f) li = -3;
+        else li = c->light_of_prim[hp];
+    }
+    ctx_free(c);
+    free(c);
+    return L.valid ? (li >= 0 ? li : -3) : -1;
+}
+
+/* the NEE density of the surface point (or environment) hit by the ray
+ * (pos, dir) leaving primitive `prim` (-1: none) of material type prev_mat;
+ * returns -1 when the ray hits no light */
+float rto_light_pdf_for_hit(const rto_scene* s, const rto_bvh* bvh, int prim, int prev_mat,
+    const float pos[3], const float sn[3], const float dir[3]) {
+    ctx_t* c = (ctx_t*)malloc(sizeof(ctx_t));
+    if (c == NULL || ctx_init(c, s, bvh) != 0) { free(c); return -2.0f; }
+    float t, u, v;
+    int hp, light = -1;
+    float pdf;
+    const v3 o = ld3(pos), d = norm3(ld3(dir));
+    if (om_closest(s, bvh, o, d, OM_RAY_EPS, OM_RAY_FAR, &t, &hp, &u, &v)) {
+        hit_t h;
+        make_hit(s, o, d, t, hp, &h);
+        pdf = light_pdf_for_hit(c, o, ld3(sn), prev_mat, prim, d, &h, 0, &light);
+    } else {
+        pdf = light_pdf_for_hit(c, o, ld3(sn), prev_mat, prim, d, NULL, 1, &light);
+    }
+    ctx_free(c);
+    free(c);
+    return light < 0 ? -1.0f : pdf;
+}
+
+/* one BSDF scattering event at a point of primitive `prim` with outward
+ * geometric normal gn, reached along d: out = new direction[3], throughput
+ * factor[3], pdf, delta; returns 1 when the path continues */
+int rto_scatter(const rto_scene* s, int prim, const float pos[3], const float gn[3],
+    const float d[3], const float r[4], float out[8]) {
+    ctx_t c;
+    memset(&c, 0, sizeof(c));
+    c.s = s;
+    hit_t h;
+    test_hit(s, prim, pos, gn, d, &h);
+    path_t st;
+    memset(&st, 0, sizeof(st));
+    st.d = ld3(d);
+    st.throughput = mk3(1.0f, 1.0f, 1.0f);
+    st.alive = 1;
+    sample_bsdf(&c, &h, r, &st);
+    out[0] = st.d.x; out[1] = st.d.y; out[2] = st.d.z;
+    out[3] = st.throughput.x; out[4] = st.throughput.y; out[5] = st.throughput.z;
+    out[6] = st.prev_pdf;
+    out[7] = (float)st.prev_delta;
+    return st.alive;
 }

@@ -22,10 +22,15 @@
  *   LBVH              : structural invariants + brute-force equivalence of
  *                       closest hits (tests/test_oracle_bvh.py)
  *   intersectors      : closed forms (tests/test_oracle_bvh.py)
- *   light sampling    : reference KATs of tests/test_light_sampling.cpp
- *   camera models     : reference KATs of tests/test_camera_models.cpp
- *   integrator        : furnace / analytic-radiance closed forms
- *                       (tests/test_oracle_render.py)
+ *   light sampling    : reference KATs of tests/test_light_sampling.cpp,
+ *                       oracle/_ref light_sampling.h on random inputs,
+ *                       sample pdf == hit pdf (tests/test_oracle_lights.py)
+ *   camera models     : reference KATs of tests/test_camera_models.cpp and
+ *                       the analytic forward models (tests/test_oracle_camera.py)
+ *   BSDFs, integrator : reflection / Snell / Schlick laws, cosine-pdf
+ *                       moments, furnaces and form-factor irradiance
+ *                       closed forms, RR unbiasedness
+ *                       (tests/test_oracle_integrator.py)
  *   OpenPBR subset    : oracle/_ref (reference header compiled here) +
  *                       reference KATs of tests/test_openpbr_core.cpp
  *   ReSTIR reservoir  : reference KATs of tests/test_restir_di.cpp
@@ -314,6 +319,20 @@ int rto_restir_merge(rto_reservoir* cur, const rto_reservoir* prev, float target
 void rto_trace_path(const rto_scene* scene, const rto_bvh* bvh, const rto_camera* cam,
     int cam_index, const rto_profile* profile, uint32_t stream, int x, int y, int sample,
     float radiance[3]);
+
+/* pieces of the integrator, exposed for known-answer and closed-form tests */
+float rto_power_heuristic(float a, float b);                  /* light_sampling.h:77-84 */
+float rto_cone_pdf(float cos_max);                            /* light_sampling.h:64-67 */
+float rto_area_pdf(float area, float dsq, float abs_cos);     /* light_sampling.h:69-75 */
+float rto_uniform_sphere_pdf(void);                           /* light_sampling.h:60-62 */
+int rto_pick_cdf(const float* cdf, int n, float u);           /* light_sampling.h:87-101 */
+void rto_triangle_barycentrics(float u0, float u1, float w[3]); /* light_sampling.h:118-125 */
+int rto_sample_light_at(const rto_scene* s, const rto_bvh* bvh, int prim, const float pos[3],
+    const float gn[3], const float d[3], const float r[4], float out[8]);
+float rto_light_pdf_for_hit(const rto_scene* s, const rto_bvh* bvh, int prim, int prev_mat,
+    const float pos[3], const float sn[3], const float dir[3]);
+int rto_scatter(const rto_scene* s, int prim, const float pos[3], const float gn[3],
+    const float d[3], const float r[4], float out[8]);
 
 /* OpenPBR subset (src/common/openpbr_core.h) */
 typedef struct { float value[3]; float pdf; } rto_openpbr_eval;
