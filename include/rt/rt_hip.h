@@ -361,13 +361,15 @@ rt_status rt_tone_map_rgba8(rt_device* dev, const float* d_beauty_rgba, uint8_t*
     int64_t pixel_count);
 
 /* ------------------------------------------------------ learned denoiser
- * A small dilated CNN (4 x conv3x3 with dilation 1/2/4/8, 32 channels, 1x1
- * head predicting log radiance as a residual of log1p(beauty)) run by BF16
- * MFMA kernels with FP32 accumulation; trained by the same kernels (backward
- * + Adam).  Replaces the closed OptiX denoiser of OptixDenoiserWrapper::run
- * (src/realtime/gpu/denoiser.cpp:111-185).  Frames: float4 beauty, 0.5 n +
- * 0.5 encoded float4 normal, float4 albedo, float depth per pixel; the
- * images of a batch lie back to back. */
+ * A 4-level convolutional encoder-decoder (widths 32 / 48 / 64 / 96, 15 3x3
+ * convolutions with ReLU, mean-pool down, nearest up, skip connections, a
+ * 1x1 FP32 head predicting log radiance as a residual of log1p(beauty);
+ * 565,443 parameters) run by BF16 MFMA kernels with FP32 accumulation and
+ * trained by the same kernels (backward + Adam).  Input features: 1-spp
+ * beauty, albedo, normal and depth.  Replaces the closed OptiX denoiser of
+ * OptixDenoiserWrapper::run (src/realtime/gpu/denoiser.cpp:111-185).
+ * Frames: float4 beauty, 0.5 n + 0.5 encoded float4 normal, float4 albedo,
+ * float depth per pixel; the images of a batch lie back to back. */
 typedef struct rt_nn rt_nn;
 typedef struct rt_nn_images {
     int32_t count;        /* 1..16 */
@@ -377,7 +379,7 @@ typedef struct rt_nn_images {
 int32_t rt_nn_param_count(void);
 rt_status rt_nn_create(int ordinal, rt_nn** out);   /* parameters start at 0 */
 rt_status rt_nn_destroy(rt_nn* nn);
-/* FP32 parameters: per layer W[co][tap][ci] then b[co]; head W[3][32], b[3].
+/* FP32 parameters: per convolution W[cout][tap][cin] then b[cout]; head W[3][32], b[3].
  * Setting them resets the optimiser state. */
 rt_status rt_nn_set_params(rt_nn* nn, const float* params, int32_t n);
 rt_status rt_nn_get_params(rt_nn* nn, float* params, int32_t n);
@@ -402,6 +404,28 @@ rt_status rt_nn_train_step_host(rt_nn* nn, const rt_nn_images* images, const flo
  * renderer holds its own reference, so rt_nn_destroy on the caller's handle
  * is allowed while the network is attached. */
 rt_status rt_renderer_set_denoiser(rt_renderer* r, rt_nn* nn);
+
+/* The learned denoiser's fallback (no counterpart in the reference, whose
+ * OptiX denoiser has none): after the network, every camera's output is
+ * checked on the device over its surface pixels, and a camera whose output
+ * has a non-finite value, moves the mean luminance by more than
+ * max_mean_shift (relative), or keeps more than max_residual_noise of the
+ * input's high-frequency energy (sum of |L(p) - L(right neighbour)|) is
+ * denoised by the a-trous reference filter instead, in the same frame,
+ * without a host round trip (graph replays included).  Defaults 0.5 / 0.9. */
+rt_status rt_renderer_set_denoiser_budget(rt_renderer* r, float max_mean_shift,
+    float max_residual_noise);
+typedef struct rt_denoiser_guard {
+    int32_t fallback[4];      /* 1: the camera's last frame came from the filter */
+    float nonfinite[4];       /* non-finite network outputs (surface pixels) */
+    float mean_shift[4];      /* |mean L(out) - mean L(in)| / mean L(in) */
+    float residual_noise[4];  /* HF(out) / HF(in) */
+    uint32_t frames_checked;  /* frames the guard ran on */
+    uint32_t frames_fallback; /* camera-frames that fell back */
+    float max_mean_shift, max_residual_noise;
+} rt_denoiser_guard;
+/* the guard's state after the last frame (synchronises the device's stream) */
+rt_status rt_renderer_denoiser_guard(rt_renderer* r, rt_denoiser_guard* out);
 
 /* tuning knobs chosen by the autotuner; 0 keeps the current value */
 typedef struct rt_tuning {

@@ -34,7 +34,8 @@ EXPORTED = [
     "rt_scene_generate_rig", "rt_frame_download_guides", "rt_bvh_download_wide",
     "rt_nn_param_count", "rt_nn_create", "rt_nn_destroy", "rt_nn_set_params", "rt_nn_get_params",
     "rt_nn_infer", "rt_nn_train_step", "rt_nn_infer_host", "rt_nn_train_step_host",
-    "rt_renderer_set_denoiser", "rt_renderer_autotune",
+    "rt_renderer_set_denoiser", "rt_renderer_set_denoiser_budget", "rt_renderer_denoiser_guard",
+    "rt_renderer_autotune",
     "rt_cache_default_config", "rt_cache_mlp_param_count", "rt_cache_param_count",
     "rt_cache_create", "rt_cache_destroy", "rt_cache_set_params", "rt_cache_get_params",
     "rt_cache_set_bounds", "rt_cache_grid_info", "rt_cache_infer_host",
@@ -229,6 +230,18 @@ class Renderer:
     def set_denoiser(self, nn: "Denoiser | None"):
         self._nn = nn  # keeps the network alive while attached
         check(lib().rt_renderer_set_denoiser(self.h, nn.h if nn is not None else None))
+
+    def set_denoiser_budget(self, max_mean_shift: float, max_residual_noise: float):
+        check(lib().rt_renderer_set_denoiser_budget(self.h, max_mean_shift, max_residual_noise))
+
+    def denoiser_guard(self) -> dict:
+        g = abi.DenoiserGuard()
+        check(lib().rt_renderer_denoiser_guard(self.h, C.byref(g)))
+        out = {}
+        for k, _ in abi.DenoiserGuard._fields_:
+            v = getattr(g, k)
+            out[k] = list(v) if hasattr(v, "__len__") else v
+        return out
 
     def set_radiance_cache(self, cache: "RadianceCache | None"):
         self._cache = cache

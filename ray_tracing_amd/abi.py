@@ -157,6 +157,13 @@ def nn_images(sizes) -> "NnImages":
     return im
 
 
+class DenoiserGuard(C.Structure):
+    _fields_ = [("fallback", C.c_int32 * 4), ("nonfinite", C.c_float * 4),
+                ("mean_shift", C.c_float * 4), ("residual_noise", C.c_float * 4),
+                ("frames_checked", C.c_uint32), ("frames_fallback", C.c_uint32),
+                ("max_mean_shift", C.c_float), ("max_residual_noise", C.c_float)]
+
+
 class CacheConfig(C.Structure):
     _fields_ = [("levels", C.c_int32), ("log2_table", C.c_int32),
                 ("base_resolution", C.c_int32), ("max_resolution", C.c_int32),
@@ -248,6 +255,8 @@ def declare(L):
     L.rt_nn_infer_host.argtypes = [vp, P(NnImages), vp, vp, vp, vp, vp]
     L.rt_nn_train_step_host.argtypes = [vp, P(NnImages), vp, vp, vp, vp, vp, C.c_float, vp, vp]
     L.rt_renderer_set_denoiser.argtypes = [vp, vp]
+    L.rt_renderer_set_denoiser_budget.argtypes = [vp, C.c_float, C.c_float]
+    L.rt_renderer_denoiser_guard.argtypes = [vp, P(DenoiserGuard)]
     L.rt_renderer_autotune.argtypes = [vp, P(CameraRig), P(Profile), C.c_int32, P(AutotuneResult)]
     L.rt_frame_download_guides.argtypes = [vp, C.c_int, vp, vp, vp]
     L.rt_scene_generate_rig.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, P(CameraRig)]

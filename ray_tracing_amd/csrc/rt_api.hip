@@ -215,6 +215,14 @@ struct GraphKey {
     uint64_t alloc_epoch;  // device allocations of the library when captured
 };
 
+// Budgets of the learned denoiser's guard: the network's output may move
+// the mean luminance of a camera's surface pixels by at most half, and must
+// remove at least a tenth of the input's high-frequency energy (a network
+// reduced to the identity removes none; a trained one removes most of it
+// from a 1-spp frame).
+constexpr float kNnMaxMeanShift = 0.5f;
+constexpr float kNnMaxResidualNoise = 0.9f;
+
 struct rt_renderer {
     rt_device* dev = nullptr;
     rt_scene* scene = nullptr;
@@ -241,6 +249,9 @@ struct rt_renderer {
     int last_set = 0;
     bool last_denoised = false;
     rt_nn* nn = nullptr;  // learned denoiser (replaces the a-trous filter when set)
+    // its guard: per camera, a frame whose network output fails the budgets
+    // is filtered by the a-trous filter instead (decided on the device)
+    DevBuf<rtk::NnGuard> nn_guard;
     rt_cache* cache = nullptr;  // radiance cache (trained and queried when set)
     DevBuf<rtk::CacheInput> cache_queries;   // one per path slot
     DevBuf<rtk::CacheRecord> cache_records;  // training records of a frame
@@ -357,6 +368,12 @@ struct rt_renderer {
 };
 
 namespace {
+
+// device address of the guard's per-camera fallback flags
+const int32_t* nn_fallback_flags(const rt_renderer* r) {
+    return reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(r->nn_guard.p)
+        + offsetof(rtk::NnGuard, fallback));
+}
 
 void device_release(rt_device* d) {
     if (d->refs.fetch_sub(1) != 1) return;
@@ -889,6 +906,13 @@ rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out)
     if (e == hipSuccess) e = hipEventCreate(&r->post_ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&r->post_ev[1]);
     if (e == hipSuccess) e = r->counters.reserve(32);
+    if (e == hipSuccess) e = r->nn_guard.reserve(1);
+    if (e == hipSuccess) {
+        rtk::NnGuard g{};
+        g.max_mean_shift = kNnMaxMeanShift;
+        g.max_residual_noise = kNnMaxResidualNoise;
+        e = hipMemcpy(r->nn_guard.p, &g, sizeof(g), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         delete r;
         scene_release(scene);
@@ -1232,6 +1256,13 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
             if (r->nn != nullptr) {
                 ENQ(rt_nn_infer_on_stream(r->nn, nn_images, total, r->beauty_set[fs].p,
                     r->normal_set[fs].p, r->albedo.p, r->depth_set[fs].p, r->denoised.p, st));
+                // the learned part's fallback: cameras whose network output is
+                // not finite, shifts the mean or keeps the noise are filtered
+                ENQ(rtk::wf_nn_guard(cv, r->beauty_set[fs].p, r->denoised.p, r->depth_set[fs].p,
+                    r->nn_guard.p, st));
+                ENQ(rtk::wf_denoise_filter(cv, rtk::kDenoiseIterations, r->beauty_set[fs].p,
+                    r->normal_set[fs].p, r->albedo.p, r->depth_set[fs].p, r->denoised.p,
+                    r->denoise_scratch.p, total, st, nn_fallback_flags(r)));
             } else {
                 ENQ(rtk::wf_denoise_filter(cv, rtk::kDenoiseIterations, r->beauty_set[fs].p,
                     r->normal_set[fs].p, r->albedo.p, r->depth_set[fs].p, r->denoised.p,
@@ -1417,6 +1448,40 @@ rt_status rt_renderer_set_denoiser(rt_renderer* r, rt_nn* nn) {
     }
     r->nn = nn;
     rt_alloc_bump();  // captured frame graphs hold the previous network's buffers
+    return RT_OK;
+}
+
+rt_status rt_renderer_set_denoiser_budget(rt_renderer* r, float max_mean_shift,
+    float max_residual_noise) {
+    RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
+    RT_REQUIRE(max_mean_shift >= 0.0f && max_residual_noise >= 0.0f, RT_ERROR_INVALID_ARGUMENT,
+        "denoiser budgets must be non-negative");
+    RT_HIP(hipSetDevice(r->dev->ordinal));
+    RT_HIP(hipStreamSynchronize(r->dev->stream));
+    const float b[2] = {max_mean_shift, max_residual_noise};
+    RT_HIP(hipMemcpy(reinterpret_cast<char*>(r->nn_guard.p) + offsetof(rtk::NnGuard, max_mean_shift),
+        b, sizeof(b), hipMemcpyHostToDevice));
+    return RT_OK;
+}
+
+rt_status rt_renderer_denoiser_guard(rt_renderer* r, rt_denoiser_guard* out) {
+    RT_REQUIRE(r != nullptr && out != nullptr, RT_ERROR_INVALID_ARGUMENT,
+        "rt_renderer_denoiser_guard requires a renderer and an output");
+    RT_HIP(hipSetDevice(r->dev->ordinal));
+    RT_HIP(hipStreamSynchronize(r->dev->stream));
+    rtk::NnGuard g{};
+    RT_HIP(hipMemcpy(&g, r->nn_guard.p, sizeof(g), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    for (int c = 0; c < 4; ++c) {
+        out->fallback[c] = g.fallback[c];
+        out->nonfinite[c] = g.metric[c][0];
+        out->mean_shift[c] = g.metric[c][1];
+        out->residual_noise[c] = g.metric[c][2];
+    }
+    out->frames_checked = g.frames_checked;
+    out->frames_fallback = g.frames_fallback;
+    out->max_mean_shift = g.max_mean_shift;
+    out->max_residual_noise = g.max_residual_noise;
     return RT_OK;
 }
 

@@ -139,11 +139,31 @@ hipError_t wf_resolve(const CameraView& cams, const ResolveParams& rp, float4* b
     float4* normal, float* depth, const float4* hbeauty, const float4* hnormal,
     const float* hdepth, float2* flow, float* trust, int total, hipStream_t st);
 
-// the edge-aware a-trous denoise filter, spacing 1, 2, 4, ... per pass
+// the edge-aware a-trous denoise filter, spacing 1, 2, 4, ... per pass;
+// with `gate` set only the cameras c with gate[c] != 0 are filtered (the
+// others keep what `out` holds)
 constexpr int kDenoiseIterations = 4;
 hipError_t wf_denoise_filter(const CameraView& cams, int iterations, const float4* beauty,
     const float4* normal, const float4* albedo, const float* depth, float4* out,
-    float4* scratch, int total, hipStream_t st);
+    float4* scratch, int total, hipStream_t st, const int32_t* gate = nullptr);
+
+// Guard of the learned denoiser, in device memory: per camera, the
+// statistics of the network's output against its 1-spp input over the
+// surface pixels, the decision to fall back to the a-trous filter, and the
+// budgets the decision applies.
+struct NnGuard {
+    float acc[4][6];         // non-finite outputs, sum L(in), sum L(out), HF(in), HF(out),
+                             // pixels; cleared by each decision
+    int32_t fallback[4];     // 1: this frame's camera output comes from the filter
+    float metric[4][3];      // non-finite outputs, mean shift, residual noise (last frame)
+    uint32_t frames_checked;
+    uint32_t frames_fallback;
+    float max_mean_shift;    // |mean L(out) - mean L(in)| / mean L(in)
+    float max_residual_noise;// HF(out) / HF(in); HF = sum of |L(p) - L(right neighbour)|
+};
+// statistics and decision for `cams.count` cameras (no host synchronisation)
+hipError_t wf_nn_guard(const CameraView& cams, const float4* in, const float4* out,
+    const float* depth, NnGuard* guard, hipStream_t st);
 
 // the viewer's tone map: float4 linear radiance -> RGBA8 (alpha 255)
 hipError_t wf_tone_map_rgba8(const float4* src, uint32_t* dst, int64_t n, hipStream_t st);

@@ -101,16 +101,18 @@ __global__ __launch_bounds__(256) void k_resolve(CameraView cams, ResolveParams 
 __constant__ float kB3[3] = {0.375f, 0.25f, 0.0625f};
 
 __global__ __launch_bounds__(256) void k_denoise_pass(CameraView cams, int step, const float4* in,
-    const float4* normal, const float4* albedo, const float* depth, float4* out, int total) {
+    const float4* normal, const float4* albedo, const float* depth, float4* out, int total,
+    const int32_t* gate) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= total) return;
+    const PixelId id = pixel_id(cams, p);
+    if (gate != nullptr && gate[id.camera] == 0) return;
     const float zp = depth[p];
     const float4 c4 = in[p];
     if (!(zp > 0.0f)) {
         out[p] = c4;
         return;
     }
-    const PixelId id = pixel_id(cams, p);
     const int base = cams.pixel_offset[id.camera];
     const f3 cp = xyz(c4);
     const f3 np = decode_normal(normal[p]);
@@ -155,6 +157,81 @@ __global__ __launch_bounds__(256) void k_denoise_pass(CameraView cams, int step,
     } else {
         out[p] = make_float4(cp.x, cp.y, cp.z, 1.0f);
     }
+}
+
+// Guard statistics of the learned denoiser, one block row per camera
+// (blockIdx.y): over the surface pixels, non-finite outputs, the luminance
+// sums of input and output, and their high-frequency energy (absolute
+// luminance steps to the right-hand neighbour).  Block sums in LDS, one
+// atomic per quantity and block.
+__global__ __launch_bounds__(256) void k_nn_guard_stats(CameraView cams, const float4* in,
+    const float4* out, const float* depth, NnGuard* guard) {
+    const int c = blockIdx.y;
+    const int base = cams.pixel_offset[c];
+    const int n = cams.pixel_offset[c + 1] - base;
+    const int w = cams.width[c];
+    float v[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int p = base + i;
+        if (!(depth[p] > 0.0f)) continue;
+        const float4 a = in[p], b = out[p];
+        const bool finite = isfinite(b.x) && isfinite(b.y) && isfinite(b.z);
+        const float la = 0.2126f * a.x + 0.7152f * a.y + 0.0722f * a.z;
+        const float lb = finite ? 0.2126f * b.x + 0.7152f * b.y + 0.0722f * b.z : 0.0f;
+        v[0] += finite ? 0.0f : 1.0f;
+        v[1] += la;
+        v[2] += lb;
+        v[5] += 1.0f;
+        const int x = i % w;
+        if (x + 1 < w && depth[p + 1] > 0.0f) {
+            const float4 a1 = in[p + 1], b1 = out[p + 1];
+            const float la1 = 0.2126f * a1.x + 0.7152f * a1.y + 0.0722f * a1.z;
+            v[3] += fabsf(la - la1);
+            if (finite && isfinite(b1.x) && isfinite(b1.y) && isfinite(b1.z)) {
+                const float lb1 = 0.2126f * b1.x + 0.7152f * b1.y + 0.0722f * b1.z;
+                v[4] += fabsf(lb - lb1);
+            }
+        }
+    }
+    __shared__ float red[6][256];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) atomicAdd(&guard->acc[c][threadIdx.x], red[threadIdx.x][0]);
+}
+
+// Decision per camera: fall back to the filter when any output is not
+// finite, when the output's mean luminance moved by more than the budget
+// from the input's, or when the output kept more than the budgeted share of
+// the input's high-frequency energy (the network did not denoise).  Clears
+// the accumulators for the next frame.
+__global__ void k_nn_guard_decide(NnGuard* guard, int count) {
+    const int c = threadIdx.x;
+    if (c >= 4) return;
+    if (c < count) {
+        const float* a = guard->acc[c];
+        const float shift = a[1] > 0.0f ? fabsf(a[2] - a[1]) / a[1] : (a[2] != 0.0f ? 1.0f : 0.0f);
+        const float noise = a[3] > 0.0f ? a[4] / a[3] : 0.0f;
+        const int fb = (a[0] > 0.0f || !(shift <= guard->max_mean_shift)
+                           || !(noise <= guard->max_residual_noise)) ? 1 : 0;
+        guard->fallback[c] = fb;
+        guard->metric[c][0] = a[0];
+        guard->metric[c][1] = shift;
+        guard->metric[c][2] = noise;
+        if (fb) atomicAdd(&guard->frames_fallback, 1u);
+    } else {
+        guard->fallback[c] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) guard->acc[c][k] = 0.0f;
+    if (c == 0) guard->frames_checked += 1u;
 }
 
 // linear_to_display_u8 (src/realtime/viewer/cuda_gl_presenter.cu:23-26):
@@ -215,19 +292,30 @@ hipError_t wf_resolve(const CameraView& cams, const ResolveParams& rp, float4* b
     return hipGetLastError();
 }
 
+hipError_t wf_nn_guard(const CameraView& cams, const float4* in, const float4* out,
+    const float* depth, NnGuard* guard, hipStream_t st) {
+    if (cams.count <= 0 || cams.count > 4) return hipErrorInvalidValue;
+    k_nn_guard_stats<<<dim3(64, (unsigned)cams.count), 256, 0, st>>>(cams, in, out, depth, guard);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_nn_guard_decide<<<1, 64, 0, st>>>(guard, cams.count);
+    return hipGetLastError();
+}
+
 hipError_t wf_denoise_filter(const CameraView& cams, int iterations, const float4* beauty,
     const float4* normal, const float4* albedo, const float* depth, float4* out,
-    float4* scratch, int total, hipStream_t st) {
+    float4* scratch, int total, hipStream_t st, const int32_t* gate) {
     if (total <= 0) return hipSuccess;
     // passes alternate between `scratch` and `out`, the last one lands in `out`
     const float4* src = beauty;
     for (int it = 0; it < iterations; ++it) {
         float4* dst = ((iterations - 1 - it) & 1) == 0 ? out : scratch;
         k_denoise_pass<<<(total + 255) / 256, 256, 0, st>>>(cams, 1 << it, src, normal, albedo,
-            depth, dst, total);
+            depth, dst, total, gate);
         src = dst;
     }
     if (iterations <= 0) {
+        if (gate != nullptr) return hipErrorInvalidValue;
         return hipMemcpyAsync(out, beauty, (size_t)total * sizeof(float4), hipMemcpyDeviceToDevice, st);
     }
     return hipGetLastError();
