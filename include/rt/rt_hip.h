@@ -408,8 +408,8 @@ rt_status rt_renderer_set_denoiser(rt_renderer* r, rt_nn* nn);
 /* The learned denoiser's fallback (no counterpart in the reference, whose
  * OptiX denoiser has none): after the network, every camera's output is
  * checked on the device over its surface pixels, and a camera whose output
- * has a non-finite value, moves the mean luminance by more than
- * max_mean_shift (relative), or keeps more than max_residual_noise of the
+ * has a non-finite value, moves the mean of log(1 + luminance) by more
+ * than max_mean_shift (relative), or keeps more than max_residual_noise of the
  * input's high-frequency energy (sum of |L(p) - L(right neighbour)|) is
  * denoised by the a-trous reference filter instead, in the same frame,
  * without a host round trip (graph replays included).  Defaults 0.5 / 0.9. */
@@ -418,7 +418,7 @@ rt_status rt_renderer_set_denoiser_budget(rt_renderer* r, float max_mean_shift,
 typedef struct rt_denoiser_guard {
     int32_t fallback[4];      /* 1: the camera's last frame came from the filter */
     float nonfinite[4];       /* non-finite network outputs (surface pixels) */
-    float mean_shift[4];      /* |mean L(out) - mean L(in)| / mean L(in) */
+    float mean_shift[4];      /* |mean log1p L(out) - mean log1p L(in)| / mean log1p L(in) */
     float residual_noise[4];  /* HF(out) / HF(in) */
     uint32_t frames_checked;  /* frames the guard ran on */
     uint32_t frames_fallback; /* camera-frames that fell back */

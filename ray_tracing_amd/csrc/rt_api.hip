@@ -216,7 +216,8 @@ struct GraphKey {
 };
 
 // Budgets of the learned denoiser's guard: the network's output may move
-// the mean luminance of a camera's surface pixels by at most half, and must
+// the mean log(1 + luminance) of a camera's surface pixels by at most half,
+// and must
 // remove at least a tenth of the input's high-frequency energy (a network
 // reduced to the identity removes none; a trained one removes most of it
 // from a 1-spp frame).

@@ -152,13 +152,13 @@ hipError_t wf_denoise_filter(const CameraView& cams, int iterations, const float
 // surface pixels, the decision to fall back to the a-trous filter, and the
 // budgets the decision applies.
 struct NnGuard {
-    float acc[4][6];         // non-finite outputs, sum L(in), sum L(out), HF(in), HF(out),
-                             // pixels; cleared by each decision
+    float acc[4][6];         // non-finite outputs, sum log1p L(in), sum log1p L(out),
+                             // HF(in), HF(out), pixels; cleared by each decision
     int32_t fallback[4];     // 1: this frame's camera output comes from the filter
     float metric[4][3];      // non-finite outputs, mean shift, residual noise (last frame)
     uint32_t frames_checked;
     uint32_t frames_fallback;
-    float max_mean_shift;    // |mean L(out) - mean L(in)| / mean L(in)
+    float max_mean_shift;    // |mean log1p L(out) - mean log1p L(in)| / mean log1p L(in)
     float max_residual_noise;// HF(out) / HF(in); HF = sum of |L(p) - L(right neighbour)|
 };
 // statistics and decision for `cams.count` cameras (no host synchronisation)

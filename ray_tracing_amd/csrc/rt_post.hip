@@ -160,9 +160,10 @@ __global__ __launch_bounds__(256) void k_denoise_pass(CameraView cams, int step,
 }
 
 // Guard statistics of the learned denoiser, one block row per camera
-// (blockIdx.y): over the surface pixels, non-finite outputs, the luminance
-// sums of input and output, and their high-frequency energy (absolute
-// luminance steps to the right-hand neighbour).  Block sums in LDS, one
+// (blockIdx.y): over the surface pixels, non-finite outputs, the sums of
+// log(1 + luminance) of input and output (a mean that single bright samples
+// of a 1-spp input do not dominate), and their high-frequency energy
+// (absolute luminance steps to the right-hand neighbour).  Block sums in LDS, one
 // atomic per quantity and block.
 __global__ __launch_bounds__(256) void k_nn_guard_stats(CameraView cams, const float4* in,
     const float4* out, const float* depth, NnGuard* guard) {
@@ -179,8 +180,8 @@ __global__ __launch_bounds__(256) void k_nn_guard_stats(CameraView cams, const f
         const float la = 0.2126f * a.x + 0.7152f * a.y + 0.0722f * a.z;
         const float lb = finite ? 0.2126f * b.x + 0.7152f * b.y + 0.0722f * b.z : 0.0f;
         v[0] += finite ? 0.0f : 1.0f;
-        v[1] += la;
-        v[2] += lb;
+        v[1] += log1pf(fmaxf(la, 0.0f));
+        v[2] += log1pf(fmaxf(lb, 0.0f));
         v[5] += 1.0f;
         const int x = i % w;
         if (x + 1 < w && depth[p + 1] > 0.0f) {
@@ -208,8 +209,8 @@ __global__ __launch_bounds__(256) void k_nn_guard_stats(CameraView cams, const f
 }
 
 // Decision per camera: fall back to the filter when any output is not
-// finite, when the output's mean luminance moved by more than the budget
-// from the input's, or when the output kept more than the budgeted share of
+// finite, when the output's mean log(1 + luminance) moved by more than the
+// budget (relative) from the input's, or when the output kept more than the budgeted share of
 // the input's high-frequency energy (the network did not denoise).  Clears
 // the accumulators for the next frame.
 __global__ void k_nn_guard_decide(NnGuard* guard, int count) {

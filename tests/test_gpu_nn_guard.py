@@ -57,7 +57,7 @@ def test_broken_network_falls_back_to_the_filter(rt, oracle, device, filtered, k
     say(f"[gpu] denoiser guard: {kind} weights, graph={graph}")
     p = weights(rt).copy()
     if kind == "nan":
-        p[1000] = np.nan
+        p[-3:] = np.nan          # the FP32 head's biases (a NaN inside the ReLU layers is clipped)
     else:
         p[:] = 0.0
     out, guard = frames(rt, oracle, device, rt.Denoiser(0, p), graph=graph, n=3)
@@ -66,7 +66,8 @@ def test_broken_network_falls_back_to_the_filter(rt, oracle, device, filtered, k
     assert guard["fallback"] == [1, 1, 1, 1]
     assert guard["frames_fallback"] == 12
     if kind == "nan":
-        assert all(v > 0 for v in guard["nonfinite"])
+        # NaN outputs, or zeros where the output stage clamps them: either way caught
+        assert all(n > 0 or m > 0.5 for n, m in zip(guard["nonfinite"], guard["mean_shift"]))
     else:
         assert all(v > 0.9 for v in guard["residual_noise"])
     for f in range(3):
