@@ -296,6 +296,18 @@ rt_status rt_trace_any(rt_scene* scene, const rt_ray* rays, int64_t n, int32_t* 
  * device pointers; kernel_ms receives the traversal kernel time */
 rt_status rt_trace_closest_device(rt_scene* scene, const rt_ray* d_rays, int64_t n,
     rt_hit* d_hits, double* kernel_ms);
+/* Allocation-free variant of rt_trace_closest_device, with no counterpart in
+ * the reference: for callers that cannot allocate (code under HIP graph
+ * capture, serving loops).  It calls no hipMalloc / hipFree and no
+ * allocating library routine: its fetch cursor and traversal stack spill
+ * live in a file-scope device array of 16 MiB + 256 B per device, resolved
+ * at rt_device_create; the persistent grid is cut to what that array holds
+ * (a scene whose per-block stack spill exceeds it is refused with
+ * RT_ERROR_INVALID_ARGUMENT).  Enqueued on the device's stream without
+ * waiting (rt_device_synchronize).  Every call on a device shares the one
+ * array: calls that overlap on one device race. */
+rt_status rt_trace_closest_device_noalloc(rt_scene* scene, const rt_ray* d_rays, int64_t n,
+    rt_hit* d_hits);
 
 /* replaces OptixRenderer + RendererPool (src/realtime/gpu/optix_renderer.h:69-128,
  * src/realtime/gpu/renderer_pool.h:30-53): one renderer traces every

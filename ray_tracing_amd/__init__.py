@@ -27,7 +27,7 @@ EXPORTED = [
     "rt_abi_version", "rt_last_error", "rt_device_create", "rt_device_destroy",
     "rt_device_synchronize", "rt_device_info", "rt_scene_upload", "rt_scene_destroy",
     "rt_bvh_build", "rt_bvh_download", "rt_trace_closest", "rt_trace_any",
-    "rt_trace_closest_device", "rt_renderer_create", "rt_renderer_destroy", "rt_render_frame",
+    "rt_trace_closest_device", "rt_trace_closest_device_noalloc", "rt_renderer_create", "rt_renderer_destroy", "rt_render_frame",
     "rt_frame_download", "rt_frame_device_view", "rt_reset_accumulation", "rt_reset_sequence",
     "rt_restir_diagnostics", "rt_renderer_set_tuning", "rt_renderer_get_tuning",
     "rt_scene_generate", "rt_scene_generated_free", "rt_restir_download",

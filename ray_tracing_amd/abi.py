@@ -225,6 +225,7 @@ def declare(L):
     L.rt_trace_closest.argtypes = [vp, vp, C.c_int64, vp]
     L.rt_trace_any.argtypes = [vp, vp, C.c_int64, vp]
     L.rt_trace_closest_device.argtypes = [vp, vp, C.c_int64, vp, P(C.c_double)]
+    L.rt_trace_closest_device_noalloc.argtypes = [vp, vp, C.c_int64, vp]
     L.rt_renderer_create.argtypes = [vp, vp, P(vp)]
     L.rt_renderer_destroy.argtypes = [vp]
     L.rt_render_frame.argtypes = [vp, P(CameraRig), P(Profile), C.c_uint32, P(FrameTiming)]

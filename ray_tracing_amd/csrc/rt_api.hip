@@ -419,6 +419,13 @@ rt_status rt_device_create(int ordinal, rt_device** out) {
         delete d;
         return fail(RT_ERROR_HIP, std::string("HIP runtime failure at hipStreamCreate: ") + hipGetErrorString(e));
     }
+    // the allocation-free trace entry point's scratch on this device
+    const hipError_t se = rtk::trace_scratch_init(ordinal);
+    if (se != hipSuccess) {
+        (void)hipStreamDestroy(d->stream);
+        delete d;
+        return fail(RT_ERROR_HIP, std::string("HIP runtime failure at hipGetSymbolAddress: ") + hipGetErrorString(se));
+    }
     *out = d;
     return RT_OK;
 }
@@ -856,6 +863,26 @@ rt_status rt_trace_closest(rt_scene* s, const rt_ray* rays, int64_t n, rt_hit* h
 
 rt_status rt_trace_any(rt_scene* s, const rt_ray* rays, int64_t n, int32_t* occluded) {
     return trace_host(s, rays, n, occluded, true);
+}
+
+rt_status rt_trace_closest_device_noalloc(rt_scene* s, const rt_ray* d_rays, int64_t n,
+    rt_hit* d_hits) {
+    RT_REQUIRE(s != nullptr, RT_ERROR_INVALID_ARGUMENT, "null scene");
+    RT_REQUIRE(s->bvh_ready, RT_ERROR_NOT_READY, "tracing requires rt_bvh_build() first");
+    RT_REQUIRE(n >= 0 && (n == 0 || (d_rays && d_hits)), RT_ERROR_INVALID_ARGUMENT, "invalid ray arrays");
+    RT_REQUIRE(n < (int64_t(1) << 31), RT_ERROR_INVALID_ARGUMENT, "at most 2^31 - 1 rays per call");
+    if (n == 0) return RT_OK;
+    RT_HIP(hipSetDevice(s->dev->ordinal));
+    hipError_t e = hipSuccess;
+    const int rc = rtk::trace_closest_noalloc(s->view(), s->dev->ordinal, s->dev->cu_count, d_rays,
+        n, d_hits, s->dev->stream, &e);
+    RT_REQUIRE(rc != 1, RT_ERROR_NOT_READY,
+        "rt_trace_closest_device_noalloc: scratch not resolved on device " + std::to_string(s->dev->ordinal));
+    RT_REQUIRE(rc != 2, RT_ERROR_INVALID_ARGUMENT,
+        "rt_trace_closest_device_noalloc: the scene's traversal stack needs more than the " +
+        std::to_string(rtk::kNoAllocScratchBytes - 256u) + "-byte scratch holds for one block");
+    RT_HIP(e);
+    return RT_OK;
 }
 
 rt_status rt_trace_closest_device(rt_scene* s, const rt_ray* d_rays, int64_t n, rt_hit* d_hits,

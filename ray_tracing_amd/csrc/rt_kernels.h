@@ -42,6 +42,17 @@ hipError_t trace_closest_launch(const SceneView& s, int grid, const rt_ray* rays
 hipError_t trace_any_launch(const SceneView& s, int grid, const rt_ray* rays, int64_t n,
     int32_t* occluded, uint32_t* fetch, uint2* spill, hipStream_t st);
 
+// allocation-free closest hit: a file-scope device array (fetch cursor +
+// stack spill) instead of allocated scratch; trace_scratch_init resolves
+// its address once per device (with that device current)
+constexpr size_t kNoAllocScratchBytes = (16u << 20) + 256u;
+constexpr int kMaxDevices = 64;
+hipError_t trace_scratch_init(int ordinal);
+// 0: launched; 1: scratch not resolved for this device; 2: one block's
+// stack spill exceeds the scratch; 3: a launch failed (*err)
+int trace_closest_noalloc(const SceneView& s, int ordinal, int cu_count, const rt_ray* rays,
+    int64_t n, rt_hit* hits, hipStream_t st, hipError_t* err);
+
 // ---- wavefront pipeline
 struct WavefrontBuffers {
     // per path slot (capacity = queue capacity)
