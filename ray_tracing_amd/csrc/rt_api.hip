@@ -919,10 +919,13 @@ rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out)
     r->dev = dev;
     r->scene = scene;
     scene->refs.fetch_add(1);
-    // the autotuner's choice on the 4-camera Cornell mesh workload (MI355X)
-    r->tuning.wavefront_paths = 1 << 23;
+    // the autotuner's choice on the 4-camera Cornell mesh workload (MI355X):
+    // room for every sample of a 4 x 640x480 x 16-spp frame in one wavefront,
+    // so each bounce is one traversal launch (a launch costs about the
+    // traversal latency of its slowest ray, whatever its size)
+    r->tuning.wavefront_paths = 1 << 25;
     r->tuning.trace_block = 256;
-    r->tuning.trace_blocks_per_cu = 4;
+    r->tuning.trace_blocks_per_cu = 8;
     r->tuning.trace_refill = 4;
     r->tuning.use_graph = 0;
     hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&r->host_cams), sizeof(rt_camera) * 8,
@@ -1013,11 +1016,12 @@ rt_status rt_renderer_autotune(rt_renderer* r, const rt_camera_rig* rig, const r
     rt_status st = measure(best, &best_ms);
     if (st != RT_OK) return st;
     out->baseline_ms = best_ms;
-    // candidates per knob; paths in flight bounded to 1..16 Mi
+    // candidates per knob; paths in flight 4..32 Mi (allocated only up to
+    // the frame's pixels x samples)
     const int blocks[] = {64, 128, 256};
     const int per_cu[] = {4, 6, 8, 10, 12, 16};
     const int refill[] = {4, 16, 64};
-    const int paths[] = {1 << 20, 1 << 21, 1 << 22, 1 << 23};
+    const int paths[] = {1 << 22, 1 << 23, 1 << 24, 1 << 25};
     const int graph[] = {0, 1};
     for (int pass = 0; pass < 2; ++pass) {
         const rt_tuning start = best;

@@ -151,6 +151,7 @@ struct WideRay {
     f3 o, d;
     float ix, iy, iz;
     float tmin, tmax;
+    bool nx, ny, nz;   // the inverse direction is negative: the box's upper plane is entered first
 };
 
 __device__ __forceinline__ WideRay make_wide_ray(f3 o, f3 d, float tmin, float tmax) {
@@ -162,6 +163,9 @@ __device__ __forceinline__ WideRay make_wide_ray(f3 o, f3 d, float tmin, float t
     r.iz = safe_inv(d.z);
     r.tmin = tmin;
     r.tmax = tmax;
+    r.nx = r.ix < 0.0f;
+    r.ny = r.iy < 0.0f;
+    r.nz = r.iz < 0.0f;
     return r;
 }
 
@@ -197,14 +201,15 @@ __device__ __forceinline__ float qbyte(uint32_t word, int k) {
     return (float)((word >> (8 * k)) & 255u);
 }
 
+// branch-free: the comparisons combine with bitwise operators and the
+// record updates are selects, so diverging lanes add no exec-mask branches
 __device__ __forceinline__ void consider_hit(TraceResult& b, uint32_t id, float t, float u, float v) {
-    if (t >= 0.0f && (t < b.t || (t == b.t && b.found && id < b.prim))) {
-        b.t = t;
-        b.prim = id;
-        b.u = u;
-        b.v = v;
-        b.found = true;
-    }
+    const bool take = (t >= 0.0f) & ((t < b.t) | ((t == b.t) & b.found & (id < b.prim)));
+    b.t = take ? t : b.t;
+    b.prim = take ? id : b.prim;
+    b.u = take ? u : b.u;
+    b.v = take ? v : b.v;
+    b.found = b.found | take;
 }
 
 // One node visit (then pops until the next node).  Returns true when the
@@ -227,26 +232,31 @@ __device__ __forceinline__ bool wide_step(const SceneView& s, const WideRay& r, 
     const float oz = (v0.z - r.o.z) * r.iz;
     const float slack = 4e-7f * ((fabsf(ox) + fabsf(oy)) + fabsf(oz));
     const uint32_t ch[4] = {v1.x, v1.y, v1.z, v1.w};
+    // per axis, the quantized plane the ray enters first: the lower bound
+    // when the inverse direction is positive, else the upper one.  fma is
+    // monotone in q for a fixed scale, so this selects exactly the
+    // min / max of the two plane distances.
+    const uint32_t lx = r.nx ? v2.w : v2.x, hx = r.nx ? v2.x : v2.w;
+    const uint32_t ly = r.ny ? v3.x : v2.y, hy = r.ny ? v2.y : v3.x;
+    const uint32_t lz = r.nz ? v3.y : v2.z, hz = r.nz ? v2.z : v3.y;
     float tn[4];
     bool hit[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const float t0x = fmaf(qbyte(v2.x, k), sx, ox), t1x = fmaf(qbyte(v2.w, k), sx, ox);
-        const float t0y = fmaf(qbyte(v2.y, k), sy, oy), t1y = fmaf(qbyte(v3.x, k), sy, oy);
-        const float t0z = fmaf(qbyte(v2.z, k), sz, oz), t1z = fmaf(qbyte(v3.y, k), sz, oz);
-        const float nx = fminf(t0x, t1x), ny = fminf(t0y, t1y), nz = fminf(t0z, t1z);
-        const float fx = fmaxf(t0x, t1x), fy = fmaxf(t0y, t1y), fz = fmaxf(t0z, t1z);
+        const float nx = fmaf(qbyte(lx, k), sx, ox), fx = fmaf(qbyte(hx, k), sx, ox);
+        const float ny = fmaf(qbyte(ly, k), sy, oy), fy = fmaf(qbyte(hy, k), sy, oy);
+        const float nz = fmaf(qbyte(lz, k), sz, oz), fz = fmaf(qbyte(hz, k), sz, oz);
         const float n = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tmin));
         const float f = fminf(fminf(fx, fy), fminf(fz, w.best.t)) * 1.0000024f + slack;
         tn[k] = n;
-        hit[k] = ch[k] != kEmpty && n <= f;
+        hit[k] = (ch[k] != kEmpty) & (n <= f);
     }
     // leaf ranges in slot order, as one per-lane loop over the primitives of
     // every hit leaf (a wave runs the longest lane's list once instead of
     // one loop per slot)
     uint32_t leaves = 0u;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) leaves |= (hit[k] && (ch[k] & kLeafBit)) ? (1u << k) : 0u;
+    for (int k = 0; k < 4; ++k) leaves |= (hit[k] & ((ch[k] & kLeafBit) != 0u)) ? (1u << k) : 0u;
     uint32_t prim = 0u, left = 0u;
     while (leaves != 0u || left != 0u) {
         if (left == 0u) {
@@ -268,10 +278,12 @@ __device__ __forceinline__ bool wide_step(const SceneView& s, const WideRay& r, 
         ++prim;
         --left;
     }
-    // internal children: stable rank by entry distance
-    bool in[4];
+    // internal children: stable rank by entry distance (ties: lower slot
+    // first), computed without branches: for j < k child j goes first when
+    // its distance is not larger, for j > k when it is smaller
+    int in[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) in[k] = hit[k] && !(ch[k] & kLeafBit);
+    for (int k = 0; k < 4; ++k) in[k] = (int)(hit[k] & ((ch[k] & kLeafBit) == 0u));
     int rank[4];
     int m = 0;
 #pragma unroll
@@ -280,10 +292,10 @@ __device__ __forceinline__ bool wide_step(const SceneView& s, const WideRay& r, 
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             if (j == k) continue;
-            rk += (in[j] && (tn[j] < tn[k] || (tn[j] == tn[k] && j < k))) ? 1 : 0;
+            rk += in[j] & (int)(j < k ? tn[j] <= tn[k] : tn[j] < tn[k]);
         }
-        rank[k] = rk;
-        m += in[k] ? 1 : 0;
+        rank[k] = in[k] ? rk : 4;
+        m += in[k];
     }
     if (m > 0) {
 #pragma unroll
@@ -293,19 +305,15 @@ __device__ __forceinline__ bool wide_step(const SceneView& s, const WideRay& r, 
                 float key = 0.0f;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    if (in[k] && rank[k] == rr) {
-                        ref = ch[k];
-                        key = tn[k];
-                    }
+                    ref = rank[k] == rr ? ch[k] : ref;
+                    key = rank[k] == rr ? tn[k] : key;
                 }
                 stack_push<BLOCK>(st, w.sp, s.stack4_entries, ref, key);
             }
         }
         uint32_t next = 0u;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (in[k] && rank[k] == 0) next = ch[k];
-        }
+        for (int k = 0; k < 4; ++k) next = rank[k] == 0 ? ch[k] : next;
         w.node = next;
         return false;
     }
