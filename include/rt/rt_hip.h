@@ -220,9 +220,10 @@ typedef struct rt_frame_timing {
     double render_ms;      /* device time of the whole frame (all cameras) */
     double denoise_ms;
     double download_ms;
-    double extend_ms;      /* closest-hit stage, summed over launches */
+    double extend_ms;      /* traversal launches, summed: each bounce's closest-hit rays together
+                              with the previous bounce's shadow (any-hit) rays */
     double shade_ms;
-    double connect_ms;     /* any-hit (shadow) stage */
+    double connect_ms;     /* the last bounce's shadow rays (their own launch) */
     double other_ms;       /* raygen + accumulate + resolve */
     uint64_t paths;        /* camera paths started */
     uint64_t extend_rays;  /* closest-hit rays traced */
@@ -232,6 +233,9 @@ typedef struct rt_frame_timing {
                               (count_traversal only) */
     uint64_t prim_tests;   /* closest-hit primitive tests (count_traversal only) */
     uint64_t sphere_tests; /* of which sphere tests */
+    uint64_t shadow_node_visits; /* any-hit BVH child boxes tested (count_traversal only) */
+    uint64_t shadow_prim_tests;  /* any-hit primitive tests (count_traversal only) */
+    uint64_t shadow_sphere_tests;/* of which sphere tests */
 } rt_frame_timing;
 
 /* DeviceRadianceFrameView, src/realtime/gpu/device_radiance_frame_view.h */
@@ -424,7 +428,7 @@ rt_status rt_renderer_set_denoiser(rt_renderer* r, rt_nn* nn);
  * than max_mean_shift (relative), or keeps more than max_residual_noise of the
  * input's high-frequency energy (sum of |L(p) - L(right neighbour)|) is
  * denoised by the a-trous reference filter instead, in the same frame,
- * without a host round trip (graph replays included).  Defaults 0.5 / 0.9. */
+ * without a host round trip (graph replays included).  Defaults 0.8 / 0.9. */
 rt_status rt_renderer_set_denoiser_budget(rt_renderer* r, float max_mean_shift,
     float max_residual_noise);
 typedef struct rt_denoiser_guard {

@@ -121,7 +121,9 @@ class FrameTiming(C.Structure):
                 ("shade_ms", C.c_double), ("connect_ms", C.c_double), ("other_ms", C.c_double),
                 ("paths", C.c_uint64), ("extend_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("extend_launches", C.c_uint64), ("node_visits", C.c_uint64),
-                ("prim_tests", C.c_uint64), ("sphere_tests", C.c_uint64)]
+                ("prim_tests", C.c_uint64), ("sphere_tests", C.c_uint64),
+                ("shadow_node_visits", C.c_uint64), ("shadow_prim_tests", C.c_uint64),
+                ("shadow_sphere_tests", C.c_uint64)]
 
 
 class Tuning(C.Structure):

@@ -216,12 +216,13 @@ struct GraphKey {
 };
 
 // Budgets of the learned denoiser's guard: the network's output may move
-// the mean log(1 + luminance) of a camera's surface pixels by at most half,
-// and must
+// the mean log(1 + luminance) of a camera's surface pixels by at most 80 %
+// (an output collapsed to zero moves it by 100 %; a 1-spp frame whose few
+// bright samples the network spreads out can move it by half), and must
 // remove at least a tenth of the input's high-frequency energy (a network
 // reduced to the identity removes none; a trained one removes most of it
 // from a 1-spp frame).
-constexpr float kNnMaxMeanShift = 0.5f;
+constexpr float kNnMaxMeanShift = 0.8f;
 constexpr float kNnMaxResidualNoise = 0.9f;
 
 struct rt_renderer {
@@ -1227,7 +1228,7 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
     }
 
     const int chunks = (spp + layers - 1) / layers;
-    const size_t stage_events = (size_t)chunks * (size_t)max_b * 6 + 4;
+    const size_t stage_events = (size_t)chunks * (size_t)(max_b + 1) * 4 + 4;
     const bool timed = timing != nullptr;
     const bool use_graph = r->tuning.use_graph != 0;
     const bool stage_timing = timed && !use_graph;
@@ -1257,16 +1258,25 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
             f.first_sample = s0;
             const int paths = f.layer_count * total;
             ENQ(rtk::wf_raygen(cv, f, b, paths, st));
+            const bool count = r->tuning.count_traversal != 0;
+            // per bounce one traversal launch: its extension rays together
+            // with the shadow rays of the bounce before; the last bounce's
+            // shadow rays get a launch of their own
             for (int bounce = 0; bounce < max_b; ++bounce) {
                 ENQ(mark());
-                ENQ(rtk::wf_extend(sv, b, bounce, r->tuning.count_traversal != 0, trace_l, st));
+                ENQ(rtk::wf_trace(sv, b, bounce, true, count, trace_l, shade_l.grid, st));
                 ENQ(mark());
                 ENQ(rtk::wf_shade(sv, cv, f, b, bounce, shade_l, st));
                 if (f.restir != 0 && bounce == 0) ENQ(rtk::wf_restir(sv, cv, f, rv, b, restir_l, st));
+                ENQ(rtk::wf_end_bounce(b, bounce, st));
                 ENQ(mark());
-                ENQ(rtk::wf_connect(sv, b, bounce, trace_l, st));
                 ENQ(mark());
             }
+            ENQ(mark());
+            ENQ(mark());
+            ENQ(mark());
+            ENQ(rtk::wf_trace(sv, b, max_b, false, count, trace_l, shade_l.grid, st));
+            ENQ(mark());
             if (cache != nullptr) {
                 ENQ(rtk::wf_cache_chunk(cache->grid_view(), cache->packed(), f, b, paths,
                     cache->grid(), st));
@@ -1427,6 +1437,9 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
         std::memcpy(&timing->node_visits, stats + 8, 8);
         std::memcpy(&timing->prim_tests, stats + 10, 8);
         std::memcpy(&timing->sphere_tests, stats + 12, 8);
+        std::memcpy(&timing->shadow_node_visits, stats + 24, 8);
+        std::memcpy(&timing->shadow_prim_tests, stats + 26, 8);
+        std::memcpy(&timing->shadow_sphere_tests, stats + 28, 8);
     }
     return RT_OK;
 }

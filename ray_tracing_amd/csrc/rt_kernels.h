@@ -79,8 +79,9 @@ struct WavefrontBuffers {
     float4* restir_o;   // incoming ray origin
     float4* restir_d;   // incoming ray direction, w = hit t
     float4* restir_thr; // throughput before BSDF sampling, w = bits(prim)
-    uint32_t* counters; // [0]/[1] queue counts, [2] shadow count, [4..15] u64 ray / traversal
-                        // stats, [16..18] ray-fetch cursors of the persistent trace kernels
+    uint32_t* counters; // [0]/[1] queue counts, [2]/[3] shadow counts, [4..15] u64 ray /
+                        // traversal stats, [16] ray-fetch cursor of the traversal launch,
+                        // [20..23] radiance cache, [24..29] u64 any-hit traversal stats
     uint2* spill;       // traversal stack spill: (stack entries - kStackLds) x trace threads
     // per pixel (all cameras)
     float4* acc_beauty;
@@ -115,16 +116,20 @@ struct StageLaunch {
 hipError_t wf_reset_counters(uint32_t* counters, hipStream_t st);
 hipError_t wf_raygen(const CameraView& cams, const FrameParams& fp, WavefrontBuffers& b,
     int paths, hipStream_t st);
-hipError_t wf_extend(const SceneView& s, WavefrontBuffers& b, int bounce, bool count,
-    StageLaunch l, hipStream_t st);
+// one traversal launch: the extension rays of `bounce` (when `extension`)
+// and the shadow rays the previous shade stage queued, then the analytic
+// light hits of the extension rays and the unoccluded shadow contributions
+// added to their paths in queue order (emissive, then analytic / ReSTIR)
+hipError_t wf_trace(const SceneView& s, WavefrontBuffers& b, int bounce, bool extension,
+    bool count, StageLaunch l, int apply_grid, hipStream_t st);
+// after the shade / ReSTIR stages of `bounce`: its path queue has been consumed
+hipError_t wf_end_bounce(WavefrontBuffers& b, int bounce, hipStream_t st);
 // resident blocks per CU of the shade (256 threads) and ReSTIR (128 threads)
 // kernels: their launch grids per CU
 int wf_shade_blocks_per_cu();
 int wf_restir_blocks_per_cu();
 hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParams& fp,
     WavefrontBuffers& b, int bounce, StageLaunch l, hipStream_t st);
-hipError_t wf_connect(const SceneView& s, WavefrontBuffers& b, int bounce, StageLaunch l,
-    hipStream_t st);
 hipError_t wf_restir(const SceneView& s, const CameraView& cams,
     const FrameParams& fp, const RestirView& rv, WavefrontBuffers& b, StageLaunch l,
     hipStream_t st);
@@ -135,7 +140,7 @@ hipError_t wf_cache_chunk(const CacheGridView& g, const CachePacked& pk, const F
     WavefrontBuffers& b, int path_cap, int grid, hipStream_t st);
 // device counters of the cache: records written this frame, queries answered
 const uint32_t* wf_cache_record_count(const WavefrontBuffers& b);
-constexpr int kStatWords = 24;  // counter words downloaded after a frame
+constexpr int kStatWords = 30;  // counter words downloaded after a frame
 constexpr int kStatCacheRecords = 21, kStatCacheQueries = 23;
 hipError_t wf_clear_accum(int pixels, WavefrontBuffers& b, hipStream_t st);
 // zero `bytes` (a multiple of 4) of device memory with a fill kernel

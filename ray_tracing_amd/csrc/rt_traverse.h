@@ -152,9 +152,11 @@ struct WideRay {
     float ix, iy, iz;
     float tmin, tmax;
     bool nx, ny, nz;   // the inverse direction is negative: the box's upper plane is entered first
+    bool any;          // any-hit query: the first hit ends the traversal
 };
 
-__device__ __forceinline__ WideRay make_wide_ray(f3 o, f3 d, float tmin, float tmax) {
+__device__ __forceinline__ WideRay make_wide_ray(f3 o, f3 d, float tmin, float tmax,
+    bool any = false) {
     WideRay r;
     r.o = o;
     r.d = d;
@@ -166,6 +168,7 @@ __device__ __forceinline__ WideRay make_wide_ray(f3 o, f3 d, float tmin, float t
     r.nx = r.ix < 0.0f;
     r.ny = r.iy < 0.0f;
     r.nz = r.iz < 0.0f;
+    r.any = any;
     return r;
 }
 
@@ -274,7 +277,7 @@ __device__ __forceinline__ bool wide_step(const SceneView& s, const WideRay& r, 
             if ((int)id < s.sphere_count) ++cnt.sphere_tests;
         }
         consider_hit(w.best, id, t, u, v);
-        if (ANY && w.best.found) return true;
+        if ((ANY || r.any) && w.best.found) return true;
         ++prim;
         --left;
     }

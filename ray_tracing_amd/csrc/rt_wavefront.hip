@@ -37,7 +37,9 @@ enum : uint32_t { kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntShadow2 = 
     kCntRestir = 14, kCntFetch = 16 /* 3 ray-fetch cursors: extend, connect 0, connect 1 */,
     kCntCacheQuery = 20 /* cache queries of the chunk */, kCntCacheRecord = 21 /* training
     records of the frame */, kCntCacheDone = 22 /* records with targets */,
-    kCntCacheQueries = 23 /* cache queries of the frame */, kCounterWords = 24 };
+    kCntCacheQueries = 23 /* cache queries of the frame */,
+    kCntAnyVisits = 24, kCntAnyTests = 26, kCntAnySphereTests = 28 /* u64 traversal stats of
+    the any-hit rays */, kCounterWords = 30 };
 static_assert(kCntCacheRecord == kStatCacheRecords && kCntCacheQueries == kStatCacheQueries
     && kCounterWords == kStatWords, "counter layout");
 
@@ -149,31 +151,62 @@ __device__ __forceinline__ TraceStack lane_stack(uint2* lds, uint2* spill) {
     return st;
 }
 
+// One traversal launch per bounce: the closest-hit extension rays of this
+// bounce (path queue `cur`, when `ext` is set) and the any-hit shadow rays
+// the previous bounce queued (emissive next-event queue, then the analytic /
+// ReSTIR queue), claimed from one fetch cursor in that order.  A bounce then
+// costs one launch, whose length is set by its slowest ray, instead of
+// three.  Hits go to the path slots; a shadow ray's visibility goes to the w
+// of its contribution, which k_apply_shadows adds in queue order.
 template <int BLOCK, bool COUNT>
-__global__ __launch_bounds__(BLOCK) void k_extend(SceneView s, WavefrontBuffers b, int cur,
-    int refill_min) {
+__global__ __launch_bounds__(BLOCK) void k_trace_fused(SceneView s, WavefrontBuffers b, int cur,
+    int ext, int refill_min) {
     __shared__ uint2 lds[kStackLds * BLOCK];
-    const uint32_t count = b.counters[cur];
+    const uint32_t ne = ext ? b.counters[cur] : 0u;
+    const uint32_t n1 = b.counters[kCntShadow];
+    const uint32_t n2 = b.counters[kCntShadow2];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntStatExtend),
-            (unsigned long long)count);
+            (unsigned long long)ne);
+        atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntStatShadow),
+            (unsigned long long)(n1 + n2));
     }
     const TraceStack st = lane_stack<BLOCK>(lds, b.spill);
     const uint32_t* queue = b.queue[cur];
-    TraverseCounts tot{0u, 0u, 0u};
-    trace_queue<BLOCK, false, COUNT>(s, count, b.counters + kCntFetch, st, refill_min,
+    TraverseCounts tot{0u, 0u, 0u}, tot_any{0u, 0u, 0u};
+    trace_queue<BLOCK, false, COUNT>(s, ne + n1 + n2, b.counters + kCntFetch, st, refill_min,
         [&](uint32_t q) {
-            const uint32_t slot = queue[q];
-            return make_wide_ray(xyz(b.ray_o[slot]), xyz(b.ray_d[slot]), kRayEps, kRayFar);
+            if (q < ne) {
+                const uint32_t slot = queue[q];
+                return make_wide_ray(xyz(b.ray_o[slot]), xyz(b.ray_d[slot]), kRayEps, kRayFar);
+            }
+            q -= ne;
+            const bool second = q >= n1;
+            const uint32_t i = second ? q - n1 : q;
+            const float4 o = (second ? b.shadow2_o : b.shadow_o)[i];
+            const float4 d = (second ? b.shadow2_d : b.shadow_d)[i];
+            return make_wide_ray(xyz(o), xyz(d), kRayEps, o.w, true);
         },
         [&](uint32_t q, const TraceResult& r, const TraverseCounts& c) {
-            const uint32_t slot = queue[q];
-            b.hit[slot] = make_float4(r.t, __int_as_float(r.found ? (int)r.prim : -1),
-                __int_as_float(-1), r.u);
+            if (q < ne) {
+                const uint32_t slot = queue[q];
+                b.hit[slot] = make_float4(r.t, __int_as_float(r.found ? (int)r.prim : -1),
+                    __int_as_float(-1), r.u);
+                if (COUNT) {
+                    tot.visits += c.visits;
+                    tot.tests += c.tests;
+                    tot.sphere_tests += c.sphere_tests;
+                }
+                return;
+            }
+            q -= ne;
+            const bool second = q >= n1;
+            const uint32_t i = second ? q - n1 : q;
+            (second ? b.shadow2_c : b.shadow_c)[i].w = r.found ? 0.0f : 1.0f;
             if (COUNT) {
-                tot.visits += c.visits;
-                tot.tests += c.tests;
-                tot.sphere_tests += c.sphere_tests;
+                tot_any.visits += c.visits;
+                tot_any.tests += c.tests;
+                tot_any.sphere_tests += c.sphere_tests;
             }
         });
     if (COUNT) {
@@ -183,6 +216,48 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneView s, WavefrontBuffers 
             (unsigned long long)tot.tests);
         atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntSphereTests),
             (unsigned long long)tot.sphere_tests);
+        atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntAnyVisits),
+            (unsigned long long)tot_any.visits);
+        atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntAnyTests),
+            (unsigned long long)tot_any.tests);
+        atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntAnySphereTests),
+            (unsigned long long)tot_any.sphere_tests);
+    }
+}
+
+// the unoccluded shadow rays of one queue add their contribution to their
+// path's radiance (one entry per path and queue, so no two lanes share a
+// slot); the emissive queue is applied before the analytic / ReSTIR one
+__global__ __launch_bounds__(256) void k_apply_shadows(WavefrontBuffers b, int which) {
+    const uint32_t count = b.counters[which == 0 ? kCntShadow : kCntShadow2];
+    const float4* sd = which == 0 ? b.shadow_d : b.shadow2_d;
+    const float4* sc = which == 0 ? b.shadow_c : b.shadow2_c;
+    for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < count; q += gridDim.x * 256u) {
+        const float4 c = sc[q];
+        if (c.w != 0.0f) {
+            const uint32_t slot = __float_as_uint(sd[q].w);
+            const float4 rad = b.radiance[slot];
+            b.radiance[slot] = f4(xyz(rad) + xyz(c), 0.0f);
+        }
+    }
+}
+
+// after a traversal launch and its shadow contributions: the shadow queues
+// are empty for the coming shade stage, and the fetch cursor rewinds
+__global__ void k_after_trace(uint32_t* counters) {
+    if (threadIdx.x == 0) {
+        counters[kCntShadow] = 0u;
+        counters[kCntShadow2] = 0u;
+        counters[kCntFetch] = 0u;
+    }
+}
+
+// after a bounce's shade (and ReSTIR) stage: the consumed path queue and the
+// ReSTIR deferral queue are empty
+__global__ void k_after_shade(uint32_t* counters, int cur) {
+    if (threadIdx.x == 0) {
+        counters[cur] = 0u;
+        counters[kCntRestir] = 0u;
     }
 }
 
@@ -449,47 +524,6 @@ __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, 
     }
 }
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_connect(SceneView s, WavefrontBuffers b, int which,
-    int refill_min) {
-    __shared__ uint2 lds[kStackLds * BLOCK];
-    const uint32_t count = b.counters[which == 0 ? kCntShadow : kCntShadow2];
-    const float4* so = which == 0 ? b.shadow_o : b.shadow2_o;
-    const float4* sd = which == 0 ? b.shadow_d : b.shadow2_d;
-    const float4* sc = which == 0 ? b.shadow_c : b.shadow2_c;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kCntStatShadow),
-            (unsigned long long)count);
-    }
-    const TraceStack st = lane_stack<BLOCK>(lds, b.spill);
-    trace_queue<BLOCK, true, false>(s, count, b.counters + kCntFetch + 1 + which, st, refill_min,
-        [&](uint32_t q) {
-            const float4 o = so[q];
-            return make_wide_ray(xyz(o), xyz(sd[q]), kRayEps, o.w);
-        },
-        [&](uint32_t q, const TraceResult& r, const TraverseCounts&) {
-            if (!r.found) {
-                const uint32_t slot = __float_as_uint(sd[q].w);
-                const float4 c = sc[q];
-                const float4 rad = b.radiance[slot];
-                b.radiance[slot] = f4(xyz(rad) + xyz(c), 0.0f);
-            }
-        });
-}
-
-__global__ void k_swap_queues(uint32_t* counters, int cur) {
-    // the consumed queue becomes the next producer target; shadow queues empty
-    if (threadIdx.x == 0) {
-        counters[cur] = 0u;
-        counters[kCntShadow] = 0u;
-        counters[kCntShadow2] = 0u;
-        counters[kCntRestir] = 0u;
-        counters[kCntFetch] = 0u;
-        counters[kCntFetch + 1] = 0u;
-        counters[kCntFetch + 2] = 0u;
-    }
-}
-
 // targets of the chunk's training records: (final path radiance - radiance
 // at the query vertex) / throughput there, per channel
 __global__ __launch_bounds__(256) void k_cache_collect(WavefrontBuffers b, CacheRecord* rec, int cap) {
@@ -581,19 +615,29 @@ hipError_t wf_raygen(const CameraView& cams, const FrameParams& fp, WavefrontBuf
 }
 
 template <bool COUNT>
-static void launch_extend(const SceneView& sv, WavefrontBuffers& b, int cur, StageLaunch l,
+static void launch_fused(const SceneView& sv, WavefrontBuffers& b, int cur, int ext, StageLaunch l,
     hipStream_t st) {
-    if (l.block == 64) k_extend<64, COUNT><<<l.grid, 64, 0, st>>>(sv, b, cur, l.refill_min);
-    else if (l.block == 256) k_extend<256, COUNT><<<l.grid, 256, 0, st>>>(sv, b, cur, l.refill_min);
-    else k_extend<128, COUNT><<<l.grid, 128, 0, st>>>(sv, b, cur, l.refill_min);
+    if (l.block == 64) k_trace_fused<64, COUNT><<<l.grid, 64, 0, st>>>(sv, b, cur, ext, l.refill_min);
+    else if (l.block == 256) k_trace_fused<256, COUNT><<<l.grid, 256, 0, st>>>(sv, b, cur, ext, l.refill_min);
+    else k_trace_fused<128, COUNT><<<l.grid, 128, 0, st>>>(sv, b, cur, ext, l.refill_min);
 }
 
-hipError_t wf_extend(const SceneView& s, WavefrontBuffers& b, int bounce, bool count,
-    StageLaunch l, hipStream_t st) {
+hipError_t wf_trace(const SceneView& s, WavefrontBuffers& b, int bounce, bool extension,
+    bool count, StageLaunch l, int apply_grid, hipStream_t st) {
     const int cur = bounce & 1;
-    if (count) launch_extend<true>(s, b, cur, l, st);
-    else launch_extend<false>(s, b, cur, l, st);
-    if (s.analytic_count > 0) k_analytic_hits<<<l.grid, 256, 0, st>>>(s, b, cur);
+    const int ext = extension ? 1 : 0;
+    if (count) launch_fused<true>(s, b, cur, ext, l, st);
+    else launch_fused<false>(s, b, cur, ext, l, st);
+    if (extension && s.analytic_count > 0) k_analytic_hits<<<l.grid, 256, 0, st>>>(s, b, cur);
+    const unsigned g = (unsigned)(apply_grid > 0 ? apply_grid : 1);
+    k_apply_shadows<<<g, 256, 0, st>>>(b, 0);
+    k_apply_shadows<<<g, 256, 0, st>>>(b, 1);
+    k_after_trace<<<1, 64, 0, st>>>(b.counters);
+    return hipGetLastError();
+}
+
+hipError_t wf_end_bounce(WavefrontBuffers& b, int bounce, hipStream_t st) {
+    k_after_shade<<<1, 64, 0, st>>>(b.counters, bounce & 1);
     return hipGetLastError();
 }
 
@@ -632,17 +676,6 @@ int wf_shade_blocks_per_cu() {
         cached.store(n, std::memory_order_relaxed);
     }
     return n;
-}
-
-hipError_t wf_connect(const SceneView& s, WavefrontBuffers& b, int bounce, StageLaunch l,
-    hipStream_t st) {
-    for (int which = 0; which < 2; ++which) {
-        if (l.block == 64) k_connect<64><<<l.grid, 64, 0, st>>>(s, b, which, l.refill_min);
-        else if (l.block == 256) k_connect<256><<<l.grid, 256, 0, st>>>(s, b, which, l.refill_min);
-        else k_connect<128><<<l.grid, 128, 0, st>>>(s, b, which, l.refill_min);
-    }
-    k_swap_queues<<<1, 64, 0, st>>>(b.counters, bounce & 1);
-    return hipGetLastError();
 }
 
 // the ReSTIR stage runs 128-thread blocks on `l.grid` blocks, which the caller

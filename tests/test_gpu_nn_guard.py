@@ -92,7 +92,7 @@ def test_budgets_are_honoured(rt, oracle, device):
     say("[gpu] denoiser guard budgets")
     out, guard = frames(rt, oracle, device, rt.Denoiser(0, np.zeros_like(weights(rt))),
                         budget=(1e30, 1e30))
-    assert guard["fallback"] == [0, 0, 0, 0] and guard["max_residual_noise"] == 1e30
+    assert guard["fallback"] == [0, 0, 0, 0] and guard["max_residual_noise"] == np.float32(1e30)
     assert all(abs(v - 1.0) < 1e-2 for v in guard["residual_noise"])
     assert all(v < 1e-3 for v in guard["mean_shift"])
     out, guard = frames(rt, oracle, device, rt.Denoiser(0, weights(rt)), budget=(0.5, 0.0))
