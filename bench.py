@@ -85,7 +85,7 @@ def measured_traffic(config: str, tuning: dict, rays_per_launch: float):
     """HBM bytes per k_extend launch from the committed rocprofv3 counter
     summary of this workload (FETCH_SIZE and WRITE_SIZE scaled by the copy
     calibration).  When this run's launch shape differs from the profiled one
-    the figure is scaled by extend rays per launch and says so."""
+    the figure is scaled by traced rays per launch and says so."""
     name = TRAFFIC_PROFILE.get(config)
     if name is None:
         return None, None
@@ -97,19 +97,19 @@ def measured_traffic(config: str, tuning: dict, rays_per_launch: float):
     vals = {}
     for line in text.splitlines():
         if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch",
-                                                   "extend_rays_per_launch", "tuning"):
+                                                   "rays_per_launch", "tuning"):
             k, v = line.split(":", 1)
             vals[k] = v.strip()
     try:
         traffic = float(vals["traffic_bytes_per_launch"])
-        rays = float(vals["extend_rays_per_launch"])
+        rays = float(vals["rays_per_launch"])
         shape = dict(kv.split("=") for kv in vals["tuning"].split())
     except (KeyError, ValueError):
         return None, None
     same = all(int(shape[k]) == int(tuning.get(k, -1)) for k in shape)
     if same:
         return traffic, f"profiles/{name}"
-    return traffic * rays_per_launch / rays, f"profiles/{name}, scaled by extend rays per launch"
+    return traffic * rays_per_launch / rays, f"profiles/{name}, scaled by traced rays per launch"
 
 
 def cam_desc(cam) -> str:
