@@ -195,6 +195,31 @@ typedef struct rt_bvh_stats {
     int32_t pad;
 } rt_bvh_stats;
 
+/* GpuSceneAcceleration::update's classification
+ * (src/realtime/gpu/gpu_scene_acceleration.cpp:247-291): reuse (nothing
+ * changed), update (materials, lights, background, textures or per-vertex
+ * attributes changed; the BVH is kept), refit (primitive positions changed,
+ * counts did not: node bounds recomputed bottom-up over the same Morton order
+ * and topology, then the 4-wide tree re-collapsed), rebuild (primitive counts
+ * changed, or no BVH yet). */
+typedef enum rt_acceleration_update_kind {
+    RT_ACCEL_REUSE = 0,
+    RT_ACCEL_UPDATE = 1,
+    RT_ACCEL_REFIT = 2,
+    RT_ACCEL_REBUILD = 3
+} rt_acceleration_update_kind;
+
+/* AccelerationUpdateStats (gpu_scene_acceleration.h:21-30) */
+typedef struct rt_acceleration_update {
+    int32_t kind;            /* rt_acceleration_update_kind */
+    int32_t node_count;      /* binary LBVH nodes */
+    int32_t primitive_reference_count;
+    int32_t wide_node_count;
+    double elapsed_ms;       /* host time of the whole update (uploads, BVH work, waits) */
+    double bvh_ms;           /* device time of the BVH kernels (0 for reuse / update) */
+    uint64_t generation;     /* incremented by every update, refit and rebuild */
+} rt_acceleration_update;
+
 /* 4-wide quantized BVH node (64 B), collapsed from the binary LBVH: each node
  * absorbs binary descendants, largest surface area first, until it has 4
  * children; binary subtrees of at most 2 primitives become leaf ranges.
@@ -285,6 +310,15 @@ rt_status rt_scene_destroy(rt_scene* scene);
 /* replaces GpuSceneAcceleration::rebuild (the CPU median split of
  * src/realtime/gpu/gpu_scene_acceleration.cpp:145-179,314-346) with a GPU LBVH */
 rt_status rt_bvh_build(rt_scene* scene, rt_bvh_stats* stats);
+/* GpuSceneAcceleration::update (gpu_scene_acceleration.cpp:247-291) and
+ * SharedGpuSceneState::prepare (optix_renderer.h:57-66): replaces the scene's
+ * contents with `desc`, classified as above by FNV-1a 64 signatures of the
+ * primitive arrays and of the whole description, and brings the BVH up to
+ * date (rebuild / refit / kept).  Renderers that share the scene see the new
+ * contents from their next frame. */
+rt_status rt_scene_update(rt_scene* scene, const rt_scene_desc* desc, rt_acceleration_update* out);
+/* the last update's statistics (the initial upload + rt_bvh_build counts as a rebuild) */
+rt_status rt_scene_last_update(rt_scene* scene, rt_acceleration_update* out);
 /* copies node_count nodes, prim_count leaf primitive ids and Morton codes;
  * any pointer may be NULL */
 rt_status rt_bvh_download(rt_scene* scene, rt_bvh_node* nodes, uint32_t* sorted_prims,

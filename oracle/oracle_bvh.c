@@ -282,6 +282,53 @@ int rto_build_lbvh(const rto_scene* s, rto_bvh* out) {
     return 0;
 }
 
+/* Refit (GpuSceneAcceleration::refit, gpu_scene_acceleration.cpp:348-367,
+ * over this LBVH): the binary topology and the Morton-sorted leaf order of the
+ * last build are kept; every child box is recomputed bottom-up from the
+ * primitive bounds of `s` (the same padded bounds and exact min / max as the
+ * build), then the 4-wide tree is collapsed again.  s must have the same
+ * primitive counts as the scene the BVH was built for. */
+int rto_refit_lbvh(const rto_scene* s, rto_bvh* bvh) {
+    const int n = prim_total(s);
+    if (n != bvh->prim_count || n <= 1) return -1;
+    box3* pb = (box3*)malloc(sizeof(box3) * (size_t)n);
+    for (int i = 0; i < n; ++i) prim_bounds(s, i, &pb[i]);
+    const int nodes = bvh->node_count;
+    box3* nb = (box3*)malloc(sizeof(box3) * (size_t)nodes);
+    int* stack = (int*)malloc(sizeof(int) * (size_t)(2 * nodes + 2));
+    unsigned char* done = (unsigned char*)calloc((size_t)nodes, 1);
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const int i = stack[sp - 1];
+        rto_bvh_node* nd = &bvh->nodes[i];
+        if (!done[i]) {
+            done[i] = 1;
+            for (int c = 0; c < 2; ++c)
+                if (!(nd->child[c] & OM_LEAF_BIT)) stack[sp++] = (int)nd->child[c];
+            continue;
+        }
+        --sp;
+        box3 acc;
+        box_empty(&acc);
+        for (int c = 0; c < 2; ++c) {
+            const uint32_t ch = nd->child[c];
+            const box3* b = (ch & OM_LEAF_BIT) ? &pb[bvh->sorted_prims[ch & ~OM_LEAF_BIT]] : &nb[ch];
+            node_set_child_box(nd, c, b);
+            box_union(&acc, b);
+        }
+        nb[i] = acc;
+    }
+    free(done);
+    free(stack);
+    free(nb);
+    free(pb);
+    free(bvh->wide);
+    bvh->wide = NULL;
+    bvh->wide_count = 0;
+    return collapse_wide(bvh);
+}
+
 /* ------------------------------------------------- 4-wide collapse
  * expand(b): start from binary node b's two children; while fewer than 4,
  * replace the internal child with the largest surface area (first on ties)

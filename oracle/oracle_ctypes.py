@@ -254,6 +254,7 @@ def lib():
         L.rto_free_generated.argtypes = [P(Generated)]
         L.rto_build_lbvh.argtypes = [P(Scene), P(Bvh)]
         L.rto_free_lbvh.argtypes = [P(Bvh)]
+        L.rto_refit_lbvh.argtypes = [P(Scene), P(Bvh)]
         L.rto_bvh_validate.argtypes = [P(Scene), P(Bvh)]
         L.rto_trace_closest.argtypes = [P(Scene), P(Bvh), C.c_void_p, C.c_int, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_int]
@@ -494,6 +495,12 @@ class OracleBvh:
             lib().rto_free_lbvh(C.byref(self.bvh))
         except Exception:
             pass
+
+    def refit(self, scene: "Scene"):
+        """Refit to `scene` (same primitive counts), keeping the topology."""
+        if lib().rto_refit_lbvh(C.byref(scene), C.byref(self.bvh)) != 0:
+            raise ValueError("refit needs a scene with the built primitive counts (> 1)")
+        self.scene = scene
 
     def nodes(self) -> np.ndarray:
         n = self.bvh.node_count

@@ -195,6 +195,8 @@ float rto_cbrt01(float x);
 /* scene generators (seeded PCG32) */
 uint32_t rto_pcg32_next(uint64_t* state, uint64_t inc);
 int rto_build_lbvh(const rto_scene* scene, rto_bvh* out);   /* allocates */
+/* new primitive positions over the topology and leaf order of `bvh` */
+int rto_refit_lbvh(const rto_scene* scene, rto_bvh* bvh);
 void rto_free_lbvh(rto_bvh* bvh);
 int rto_bvh_validate(const rto_scene* scene, const rto_bvh* bvh);
 uint32_t rto_expand_bits10(uint32_t v);

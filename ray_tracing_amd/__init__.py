@@ -26,7 +26,7 @@ STATUS_NAMES = {0: "RT_OK", 1: "RT_ERROR_INVALID_ARGUMENT", 2: "RT_ERROR_HIP",
 EXPORTED = [
     "rt_abi_version", "rt_last_error", "rt_device_create", "rt_device_destroy",
     "rt_device_synchronize", "rt_device_info", "rt_scene_upload", "rt_scene_destroy",
-    "rt_bvh_build", "rt_bvh_download", "rt_trace_closest", "rt_trace_any",
+    "rt_bvh_build", "rt_bvh_download", "rt_scene_update", "rt_scene_last_update", "rt_trace_closest", "rt_trace_any",
     "rt_trace_closest_device", "rt_trace_closest_device_noalloc", "rt_renderer_create", "rt_renderer_destroy", "rt_render_frame",
     "rt_frame_download", "rt_frame_device_view", "rt_reset_accumulation", "rt_reset_sequence",
     "rt_restir_diagnostics", "rt_renderer_set_tuning", "rt_renderer_get_tuning",
@@ -140,6 +140,30 @@ class Scene:
         check(lib().rt_bvh_build(self.h, C.byref(st)))
         self.stats = st
         return st
+
+    def update(self, desc: SceneDesc, keepalive=None) -> dict:
+        """Replaces the scene's contents (rt_scene_update): returns the
+        classification (reuse / update / refit / rebuild) and its timings."""
+        u = abi.AccelerationUpdate()
+        check(lib().rt_scene_update(self.h, C.byref(desc), C.byref(u)))
+        self.desc = desc
+        self._keep = keepalive
+        if self.stats is not None:
+            self.stats.prim_count = u.primitive_reference_count
+            self.stats.node_count = u.node_count
+            self.stats.wide_node_count = u.wide_node_count
+        return self._update_dict(u)
+
+    def last_update(self) -> dict:
+        u = abi.AccelerationUpdate()
+        check(lib().rt_scene_last_update(self.h, C.byref(u)))
+        return self._update_dict(u)
+
+    @staticmethod
+    def _update_dict(u) -> dict:
+        d = {k: getattr(u, k) for k, _ in abi.AccelerationUpdate._fields_}
+        d["kind"] = abi.ACCEL_KINDS.get(u.kind, str(u.kind))
+        return d
 
     def download_bvh(self):
         n = self.prim_count

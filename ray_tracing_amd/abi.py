@@ -159,6 +159,15 @@ def nn_images(sizes) -> "NnImages":
     return im
 
 
+class AccelerationUpdate(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("node_count", C.c_int32),
+                ("primitive_reference_count", C.c_int32), ("wide_node_count", C.c_int32),
+                ("elapsed_ms", C.c_double), ("bvh_ms", C.c_double), ("generation", C.c_uint64)]
+
+
+ACCEL_KINDS = {0: "reuse", 1: "update", 2: "refit", 3: "rebuild"}
+
+
 class DenoiserGuard(C.Structure):
     _fields_ = [("fallback", C.c_int32 * 4), ("nonfinite", C.c_float * 4),
                 ("mean_shift", C.c_float * 4), ("residual_noise", C.c_float * 4),
@@ -228,6 +237,8 @@ def declare(L):
     L.rt_trace_any.argtypes = [vp, vp, C.c_int64, vp]
     L.rt_trace_closest_device.argtypes = [vp, vp, C.c_int64, vp, P(C.c_double)]
     L.rt_trace_closest_device_noalloc.argtypes = [vp, vp, C.c_int64, vp]
+    L.rt_scene_update.argtypes = [vp, P(SceneDesc), P(AccelerationUpdate)]
+    L.rt_scene_last_update.argtypes = [vp, P(AccelerationUpdate)]
     L.rt_renderer_create.argtypes = [vp, vp, P(vp)]
     L.rt_renderer_destroy.argtypes = [vp]
     L.rt_render_frame.argtypes = [vp, P(CameraRig), P(Profile), C.c_uint32, P(FrameTiming)]

@@ -11,6 +11,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstddef>
+#include <chrono>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -159,6 +160,11 @@ struct rt_scene {
     DevBuf<float4> leaves;
     DevBuf<uint32_t> sorted;
     DevBuf<uint32_t> morton;
+    DevBuf<uint32_t> leaf_parent;  // binary LBVH parent of each sorted leaf (kept for refits)
+    // acceleration lifecycle (GpuSceneAcceleration::update,
+    // src/realtime/gpu/gpu_scene_acceleration.cpp:247-291)
+    uint64_t geometry_sig = 0, scene_sig = 0, generation = 0;
+    rt_acceleration_update last_update{};
 
     rtk::SceneView view() const {
         rtk::SceneView v{};
@@ -395,70 +401,7 @@ void scene_release(rt_scene* s) {
 
 }  // namespace
 
-extern "C" {
-
-int rt_abi_version(void) { return RT_ABI_VERSION; }
-
-const char* rt_last_error(void) { return g_error.c_str(); }
-
-rt_status rt_device_create(int ordinal, rt_device** out) {
-    RT_REQUIRE(out != nullptr, RT_ERROR_INVALID_ARGUMENT, "rt_device_create requires an output handle");
-    int count = 0;
-    RT_HIP(hipGetDeviceCount(&count));
-    RT_REQUIRE(ordinal >= 0 && ordinal < count, RT_ERROR_INVALID_ARGUMENT,
-        "device ordinal " + std::to_string(ordinal) + " out of range [0, " + std::to_string(count) + ")");
-    RT_HIP(hipSetDevice(ordinal));
-    hipDeviceProp_t prop;
-    RT_HIP(hipGetDeviceProperties(&prop, ordinal));
-    auto* d = new rt_device();
-    d->ordinal = ordinal;
-    d->cu_count = prop.multiProcessorCount;
-    d->name = prop.gcnArchName;
-    d->total_mem = prop.totalGlobalMem;
-    const hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete d;
-        return fail(RT_ERROR_HIP, std::string("HIP runtime failure at hipStreamCreate: ") + hipGetErrorString(e));
-    }
-    // the allocation-free trace entry point's scratch on this device
-    const hipError_t se = rtk::trace_scratch_init(ordinal);
-    if (se != hipSuccess) {
-        (void)hipStreamDestroy(d->stream);
-        delete d;
-        return fail(RT_ERROR_HIP, std::string("HIP runtime failure at hipGetSymbolAddress: ") + hipGetErrorString(se));
-    }
-    *out = d;
-    return RT_OK;
-}
-
-rt_status rt_device_destroy(rt_device* dev) {
-    if (!dev) return RT_OK;
-    device_release(dev);
-    return RT_OK;
-}
-
-rt_status rt_device_synchronize(rt_device* dev) {
-    RT_REQUIRE(dev != nullptr, RT_ERROR_INVALID_ARGUMENT, "null device");
-    RT_HIP(hipStreamSynchronize(dev->stream));
-    return RT_OK;
-}
-
-rt_status rt_device_info(rt_device* dev, char* name, size_t name_len, int* cu_count,
-    uint64_t* total_bytes, uint64_t* free_bytes) {
-    RT_REQUIRE(dev != nullptr, RT_ERROR_INVALID_ARGUMENT, "null device");
-    if (name && name_len > 0) {
-        std::strncpy(name, dev->name.c_str(), name_len - 1);
-        name[name_len - 1] = '\0';
-    }
-    if (cu_count) *cu_count = dev->cu_count;
-    if (total_bytes) *total_bytes = dev->total_mem;
-    if (free_bytes) {
-        size_t f = 0, t = 0;
-        RT_HIP(hipMemGetInfo(&f, &t));
-        *free_bytes = f;
-    }
-    return RT_OK;
-}
+namespace {
 
 // light distribution: power x area CDF over emissive primitives plus the
 // environment (src/realtime/gpu/packed_scene_preparation.cpp:250-302)
@@ -542,8 +485,8 @@ static void build_analytic_cdf(const rt_scene_desc* d, std::vector<float>& sel,
         for (int i = last; i < n; ++i) cdf[(size_t)i] = 1.0f;
 }
 
-rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out) {
-    RT_REQUIRE(dev && d && out, RT_ERROR_INVALID_ARGUMENT, "rt_scene_upload requires device, description and output");
+// scene arrays and counts (rt_scene_upload / rt_scene_update)
+static rt_status scene_validate(const rt_scene_desc* d) {
     RT_REQUIRE(d->sphere_count >= 0 && d->quad_count >= 0 && d->triangle_count >= 0
                    && d->material_count >= 0 && d->openpbr_count >= 0 && d->analytic_light_count >= 0,
         RT_ERROR_INVALID_ARGUMENT, "scene counts must be non-negative");
@@ -576,9 +519,15 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
         RT_ERROR_INVALID_ARGUMENT, "texture arrays must be non-null when their count is positive");
     RT_REQUIRE(d->image_texel_count <= (INT32_MAX / 3), RT_ERROR_INVALID_ARGUMENT,
         "image texel count exceeds 2^31 / 3");
-    RT_HIP(hipSetDevice(dev->ordinal));
-    auto* s = new rt_scene();
-    s->dev = dev;
+    return RT_OK;
+}
+
+// host-side tables and device copies of a scene description (everything but
+// the BVH); the device is current
+static rt_status scene_fill(rt_scene* s, const rt_scene_desc* d) {
+    const int64_t prims = (int64_t)d->sphere_count + d->quad_count + d->triangle_count;
+    s->env_light = -1;
+    s->host_lights.clear();
     {
         // bounding box of the primitives (the radiance cache's grid domain)
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -609,7 +558,6 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
             s->hi[a] = (float)hi[a];
         }
     }
-    dev->refs.fetch_add(1);
     s->sphere_count = d->sphere_count;
     s->quad_count = d->quad_count;
     s->triangle_count = d->triangle_count;
@@ -633,7 +581,7 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
         const int base = l.type == 0 ? 0 : (l.type == 1 ? d->sphere_count : d->sphere_count + d->quad_count);
         lop[static_cast<size_t>(base + l.prim)] = static_cast<int32_t>(i);
     }
-    hipStream_t st = dev->stream;
+    hipStream_t st = s->dev->stream;
     hipError_t e = hipSuccess;
     if (e == hipSuccess) e = upload(s->spheres, d->spheres, (size_t)d->sphere_count, st);
     if (e == hipSuccess) e = upload(s->quads, d->quads, (size_t)d->quad_count, st);
@@ -661,10 +609,147 @@ rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out
     if (e == hipSuccess && s->has_triangle_normals)
         e = upload(s->tri_n, d->triangle_normals, (size_t)d->triangle_count * 9, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) {
-        scene_release(s);
+    if (e != hipSuccess)
         return fail(RT_ERROR_HIP, std::string("HIP runtime failure at scene upload: ") + hipGetErrorString(e));
+    return RT_OK;
+}
+
+// Content signature of a byte range: FNV-1a 64 (the signatures of
+// gpu_scene_acceleration.cpp:93-135) taken over 64-bit words in four
+// interleaved lanes, folded together at the end, so a million-triangle
+// scene hashes in milliseconds; equal contents give equal signatures.
+struct Signature {
+    uint64_t lane[4] = {14695981039346656037ull, 14695981039346656037ull ^ 1ull,
+        14695981039346656037ull ^ 2ull, 14695981039346656037ull ^ 3ull};
+    static constexpr uint64_t kPrime = 1099511628211ull;
+
+    void bytes(const void* p, size_t n) {
+        const auto* b = static_cast<const unsigned char*>(p);
+        size_t i = 0;
+        for (; i + 32 <= n; i += 32) {
+            uint64_t w[4];
+            std::memcpy(w, b + i, 32);
+            for (int k = 0; k < 4; ++k) lane[k] = (lane[k] ^ w[k]) * kPrime;
+        }
+        for (; i < n; ++i) lane[0] = (lane[0] ^ b[i]) * kPrime;
     }
+    template <class T>
+    void array(const T* p, int64_t n) {
+        bytes(&n, sizeof(n));
+        if (n > 0 && p) bytes(p, sizeof(T) * (size_t)n);
+    }
+    uint64_t value() const {
+        uint64_t h = 14695981039346656037ull;
+        for (int k = 0; k < 4; ++k) h = (h ^ lane[k]) * kPrime;
+        return h;
+    }
+};
+
+// the primitive positions the BVH bounds depend on (geometry), then
+// everything else a frame reads: materials, lights, background, textures,
+// per-vertex attributes (scene, which covers the geometry too)
+static void scene_signatures(const rt_scene_desc* d, uint64_t* geometry, uint64_t* scene) {
+    Signature g;
+    g.array(d->spheres, d->sphere_count);
+    g.array(d->quads, d->quad_count);
+    g.array(d->triangles, d->triangle_count);
+    *geometry = g.value();
+    Signature h = g;
+    h.bytes(&d->background_mode, sizeof(d->background_mode));
+    h.bytes(d->background, sizeof(d->background));
+    h.bytes(d->background_top, sizeof(d->background_top));
+    h.array(d->materials, d->material_count);
+    h.array(d->openpbr, d->openpbr_count);
+    h.array(d->analytic_lights, d->analytic_light_count);
+    h.array(d->textures, d->texture_count);
+    h.array(d->image_texels, (int64_t)d->image_texel_count * 3);
+    h.array(d->material_albedo_texture, d->material_albedo_texture ? d->material_count : 0);
+    h.array(d->triangle_uvs, d->triangle_uvs ? (int64_t)d->triangle_count * 6 : 0);
+    h.array(d->triangle_normals, d->triangle_normals ? (int64_t)d->triangle_count * 9 : 0);
+    *scene = h.value();
+}
+
+}  // namespace
+
+extern "C" {
+
+int rt_abi_version(void) { return RT_ABI_VERSION; }
+
+const char* rt_last_error(void) { return g_error.c_str(); }
+
+rt_status rt_device_create(int ordinal, rt_device** out) {
+    RT_REQUIRE(out != nullptr, RT_ERROR_INVALID_ARGUMENT, "rt_device_create requires an output handle");
+    int count = 0;
+    RT_HIP(hipGetDeviceCount(&count));
+    RT_REQUIRE(ordinal >= 0 && ordinal < count, RT_ERROR_INVALID_ARGUMENT,
+        "device ordinal " + std::to_string(ordinal) + " out of range [0, " + std::to_string(count) + ")");
+    RT_HIP(hipSetDevice(ordinal));
+    hipDeviceProp_t prop;
+    RT_HIP(hipGetDeviceProperties(&prop, ordinal));
+    auto* d = new rt_device();
+    d->ordinal = ordinal;
+    d->cu_count = prop.multiProcessorCount;
+    d->name = prop.gcnArchName;
+    d->total_mem = prop.totalGlobalMem;
+    const hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete d;
+        return fail(RT_ERROR_HIP, std::string("HIP runtime failure at hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    // the allocation-free trace entry point's scratch on this device
+    const hipError_t se = rtk::trace_scratch_init(ordinal);
+    if (se != hipSuccess) {
+        (void)hipStreamDestroy(d->stream);
+        delete d;
+        return fail(RT_ERROR_HIP, std::string("HIP runtime failure at hipGetSymbolAddress: ") + hipGetErrorString(se));
+    }
+    *out = d;
+    return RT_OK;
+}
+
+rt_status rt_device_destroy(rt_device* dev) {
+    if (!dev) return RT_OK;
+    device_release(dev);
+    return RT_OK;
+}
+
+rt_status rt_device_synchronize(rt_device* dev) {
+    RT_REQUIRE(dev != nullptr, RT_ERROR_INVALID_ARGUMENT, "null device");
+    RT_HIP(hipStreamSynchronize(dev->stream));
+    return RT_OK;
+}
+
+rt_status rt_device_info(rt_device* dev, char* name, size_t name_len, int* cu_count,
+    uint64_t* total_bytes, uint64_t* free_bytes) {
+    RT_REQUIRE(dev != nullptr, RT_ERROR_INVALID_ARGUMENT, "null device");
+    if (name && name_len > 0) {
+        std::strncpy(name, dev->name.c_str(), name_len - 1);
+        name[name_len - 1] = '\0';
+    }
+    if (cu_count) *cu_count = dev->cu_count;
+    if (total_bytes) *total_bytes = dev->total_mem;
+    if (free_bytes) {
+        size_t f = 0, t = 0;
+        RT_HIP(hipMemGetInfo(&f, &t));
+        *free_bytes = f;
+    }
+    return RT_OK;
+}
+
+rt_status rt_scene_upload(rt_device* dev, const rt_scene_desc* d, rt_scene** out) {
+    RT_REQUIRE(dev && d && out, RT_ERROR_INVALID_ARGUMENT, "rt_scene_upload requires device, description and output");
+    const rt_status vs = scene_validate(d);
+    if (vs != RT_OK) return vs;
+    RT_HIP(hipSetDevice(dev->ordinal));
+    auto* s = new rt_scene();
+    s->dev = dev;
+    dev->refs.fetch_add(1);
+    const rt_status fs = scene_fill(s, d);
+    if (fs != RT_OK) {
+        scene_release(s);
+        return fs;
+    }
+    scene_signatures(d, &s->geometry_sig, &s->scene_sig);
     *out = s;
     return RT_OK;
 }
@@ -675,22 +760,25 @@ rt_status rt_scene_destroy(rt_scene* scene) {
     return RT_OK;
 }
 
-rt_status rt_bvh_build(rt_scene* s, rt_bvh_stats* stats) {
-    RT_REQUIRE(s != nullptr, RT_ERROR_INVALID_ARGUMENT, "null scene");
-    RT_HIP(hipSetDevice(s->dev->ordinal));
+// LBVH build (refit = false) or refit over the last build's topology, then
+// the 4-wide collapse; the device is current
+static rt_status bvh_work(rt_scene* s, bool refit, rt_bvh_stats* stats) {
     const int n = s->prim_count;
     hipStream_t st = s->dev->stream;
     s->bvh_ready = false;
-    s->node_count = n > 1 ? n - 1 : (n == 1 ? 1 : 0);
-    s->max_depth = 0;
+    if (!refit) {
+        s->node_count = n > 1 ? n - 1 : (n == 1 ? 1 : 0);
+        s->max_depth = 0;
+    }
     double build_ms = 0.0;
     if (n > 0) {
         RT_HIP(s->nodes.reserve((size_t)s->node_count * 4));
         RT_HIP(s->leaves.reserve((size_t)n * 3));
         RT_HIP(s->sorted.reserve((size_t)n));
         RT_HIP(s->morton.reserve((size_t)n));
+        RT_HIP(s->leaf_parent.reserve((size_t)n));
         DevBuf<float4> pbox;
-        DevBuf<uint32_t> k0, k1, v0, v1, counts, leaf_parent, flags, cb;
+        DevBuf<uint32_t> k0, k1, v0, v1, counts, flags, cb;
         const size_t nblocks = ((size_t)n + 1023) / 1024;
         RT_HIP(pbox.reserve((size_t)n * 2));
         RT_HIP(k0.reserve((size_t)n));
@@ -698,7 +786,6 @@ rt_status rt_bvh_build(rt_scene* s, rt_bvh_stats* stats) {
         RT_HIP(v0.reserve((size_t)n));
         RT_HIP(v1.reserve((size_t)n));
         RT_HIP(counts.reserve(16 * nblocks));
-        RT_HIP(leaf_parent.reserve((size_t)n));
         RT_HIP(flags.reserve((size_t)n));
         RT_HIP(cb.reserve(8));
         rtk::BvhBuildScratch w{};
@@ -708,7 +795,7 @@ rt_status rt_bvh_build(rt_scene* s, rt_bvh_stats* stats) {
         w.vals[0] = v0.p;
         w.vals[1] = v1.p;
         w.counts = counts.p;
-        w.leaf_parent = leaf_parent.p;
+        w.leaf_parent = s->leaf_parent.p;
         w.flags = flags.p;
         w.cb = cb.p;
         hipEvent_t e0, e1;
@@ -716,7 +803,11 @@ rt_status rt_bvh_build(rt_scene* s, rt_bvh_stats* stats) {
         RT_HIP(hipEventCreate(&e1));
         rtk::SceneView v = s->view();
         RT_HIP(hipEventRecord(e0, st));
-        RT_HIP(rtk::bvh_build(v, n, w, s->nodes.p, s->leaves.p, s->sorted.p, s->morton.p, st));
+        if (refit) {
+            RT_HIP(rtk::bvh_refit(v, n, w, s->nodes.p, s->leaves.p, s->sorted.p, s->leaf_parent.p, st));
+        } else {
+            RT_HIP(rtk::bvh_build(v, n, w, s->nodes.p, s->leaves.p, s->sorted.p, s->morton.p, st));
+        }
         RT_HIP(hipEventRecord(e1, st));
         uint32_t hcb[8] = {0};
         RT_HIP(hipMemcpyAsync(hcb, cb.p, sizeof(hcb), hipMemcpyDeviceToHost, st));
@@ -726,12 +817,13 @@ rt_status rt_bvh_build(rt_scene* s, rt_bvh_stats* stats) {
         build_ms = ms;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        s->max_depth = static_cast<int32_t>(hcb[6]);
+        if (!refit) s->max_depth = static_cast<int32_t>(hcb[6]);
         RT_REQUIRE(s->max_depth >= 1 && s->max_depth <= 64, RT_ERROR_HIP,
             "LBVH build produced an invalid depth " + std::to_string(s->max_depth));
         // collapse to the 4-wide quantized tree every ray traverses
         const int nbin = s->node_count;
-        DevBuf<uint32_t> is_wide, index, f0, f1, fc;
+        DevBuf<uint32_t> is_wide, index, f0, f1, fc, tiles;
+        RT_HIP(tiles.reserve((size_t)rtk::scan_tiles(nbin)));
         RT_HIP(is_wide.reserve((size_t)nbin));
         RT_HIP(index.reserve((size_t)nbin));
         RT_HIP(f0.reserve((size_t)nbin));
@@ -744,6 +836,7 @@ rt_status rt_bvh_build(rt_scene* s, rt_bvh_stats* stats) {
         ws.front[0] = f0.p;
         ws.front[1] = f1.p;
         ws.counts = fc.p;
+        ws.tiles = tiles.p;
         hipEvent_t w0, w1;
         RT_HIP(hipEventCreate(&w0));
         RT_HIP(hipEventCreate(&w1));
@@ -773,6 +866,78 @@ rt_status rt_bvh_build(rt_scene* s, rt_bvh_stats* stats) {
         stats->pad = 0;
         stats->build_ms = build_ms;
     }
+    return RT_OK;
+}
+
+static void record_update(rt_scene* s, int kind, double elapsed_ms, double bvh_ms) {
+    if (kind != RT_ACCEL_REUSE) ++s->generation;
+    rt_acceleration_update& u = s->last_update;
+    u.kind = kind;
+    u.node_count = s->node_count;
+    u.primitive_reference_count = s->prim_count;
+    u.wide_node_count = s->wide_count;
+    u.elapsed_ms = elapsed_ms;
+    u.bvh_ms = bvh_ms;
+    u.generation = s->generation;
+}
+
+rt_status rt_bvh_build(rt_scene* s, rt_bvh_stats* stats) {
+    RT_REQUIRE(s != nullptr, RT_ERROR_INVALID_ARGUMENT, "null scene");
+    RT_HIP(hipSetDevice(s->dev->ordinal));
+    const auto t0 = std::chrono::steady_clock::now();
+    rt_bvh_stats local{};
+    const rt_status st = bvh_work(s, false, &local);
+    if (st != RT_OK) return st;
+    if (stats) *stats = local;
+    record_update(s, RT_ACCEL_REBUILD,
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(),
+        local.build_ms);
+    return RT_OK;
+}
+
+rt_status rt_scene_update(rt_scene* s, const rt_scene_desc* d, rt_acceleration_update* out) {
+    RT_REQUIRE(s != nullptr && d != nullptr, RT_ERROR_INVALID_ARGUMENT,
+        "rt_scene_update requires a scene and a description");
+    const rt_status vs = scene_validate(d);
+    if (vs != RT_OK) return vs;
+    RT_HIP(hipSetDevice(s->dev->ordinal));
+    const auto t0 = std::chrono::steady_clock::now();
+    uint64_t gsig = 0, ssig = 0;
+    scene_signatures(d, &gsig, &ssig);
+    const bool topology_changed = s->sphere_count != d->sphere_count
+                                  || s->quad_count != d->quad_count
+                                  || s->triangle_count != d->triangle_count;
+    int kind = RT_ACCEL_REUSE;
+    if (topology_changed || !s->bvh_ready || s->prim_count <= 1) kind = RT_ACCEL_REBUILD;
+    else if (gsig != s->geometry_sig) kind = RT_ACCEL_REFIT;
+    else if (ssig != s->scene_sig) kind = RT_ACCEL_UPDATE;
+    double bvh_ms = 0.0;
+    if (kind != RT_ACCEL_REUSE) {
+        // frames in flight read the old contents until here
+        RT_HIP(hipStreamSynchronize(s->dev->stream));
+        const rt_status fs = scene_fill(s, d);
+        if (fs != RT_OK) return fs;
+        if (kind == RT_ACCEL_REBUILD || kind == RT_ACCEL_REFIT) {
+            rt_bvh_stats bs{};
+            const rt_status bw = bvh_work(s, kind == RT_ACCEL_REFIT, &bs);
+            if (bw != RT_OK) return bw;
+            bvh_ms = bs.build_ms;
+        }
+        rt_alloc_bump();  // captured frame graphs may hold the replaced scene buffers
+        s->geometry_sig = gsig;
+        s->scene_sig = ssig;
+    }
+    record_update(s, kind,
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(),
+        bvh_ms);
+    if (out) *out = s->last_update;
+    return RT_OK;
+}
+
+rt_status rt_scene_last_update(rt_scene* s, rt_acceleration_update* out) {
+    RT_REQUIRE(s != nullptr && out != nullptr, RT_ERROR_INVALID_ARGUMENT,
+        "rt_scene_last_update requires a scene and an output");
+    *out = s->last_update;
     return RT_OK;
 }
 

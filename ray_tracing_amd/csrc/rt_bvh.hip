@@ -90,33 +90,55 @@ __global__ void k_init_bounds(uint32_t* cb) {
     if (threadIdx.x == 6) cb[6] = 0u;                          // max depth
 }
 
-__global__ void k_prim_bounds(SceneView s, int n, float4* pbox, uint32_t* cb) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// primitive boxes and the bounds of their centroids: a bounded grid strides
+// over the primitives, each block reduces its waves in LDS, and one lane per
+// block merges the result with six atomics (min / max are order-free)
+constexpr int kBoundsBlock = 256;
+constexpr int kBoundsMaxBlocks = 2048;
+
+__global__ __launch_bounds__(kBoundsBlock) void k_prim_bounds(SceneView s, int n, float4* pbox,
+    uint32_t* cb) {
+    __shared__ uint32_t red[kBoundsBlock / 64][6];
     uint32_t kmin[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
     uint32_t kmax[3] = {0u, 0u, 0u};
-    if (i < n) {
+    for (int i = blockIdx.x * kBoundsBlock + (int)threadIdx.x; i < n; i += gridDim.x * kBoundsBlock) {
         const Box b = prim_box(s, i);
         pbox[2 * i] = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.0f);
         pbox[2 * i + 1] = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.0f);
         for (int a = 0; a < 3; ++a) {
             const uint32_t k = ordered((b.lo[a] + b.hi[a]) * 0.5f);
-            kmin[a] = k;
-            kmax[a] = k;
+            kmin[a] = min(kmin[a], k);
+            kmax[a] = max(kmax[a], k);
         }
     }
-    // wave reduction then one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
         for (int a = 0; a < 3; ++a) {
             kmin[a] = min(kmin[a], (uint32_t)__shfl_xor((int)kmin[a], off));
             kmax[a] = max(kmax[a], (uint32_t)__shfl_xor((int)kmax[a], off));
         }
     }
+    const int wave = (int)threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        for (int a = 0; a < 3; ++a) { red[wave][a] = kmin[a]; red[wave][3 + a] = kmax[a]; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBoundsBlock / 64; ++w) {
+            for (int a = 0; a < 3; ++a) {
+                kmin[a] = min(kmin[a], red[w][a]);
+                kmax[a] = max(kmax[a], red[w][3 + a]);
+            }
+        }
         for (int a = 0; a < 3; ++a) {
             atomicMin(&cb[a], kmin[a]);
             atomicMax(&cb[3 + a], kmax[a]);
         }
     }
+}
+
+inline int bounds_blocks(int n) {
+    const int b = (n + kBoundsBlock - 1) / kBoundsBlock;
+    return b < kBoundsMaxBlocks ? b : kBoundsMaxBlocks;
 }
 
 __device__ __forceinline__ uint32_t expand10(uint32_t v) {
@@ -189,6 +211,56 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* counts, int total) {
         __syncthreads();
         if (threadIdx.x == 1023) carry += part[1023];
         __syncthreads();
+    }
+}
+
+// exclusive scan of a long array in three passes: tile sums, one-block
+// scan of the tile sums, then each tile scanned from its offset
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+
+__global__ __launch_bounds__(kScanBlock) void k_tile_sums(const uint32_t* in, int total,
+    uint32_t* sums) {
+    __shared__ uint32_t part[kScanBlock / 64];
+    const int base = blockIdx.x * kScanTile;
+    uint32_t acc = 0u;
+    for (int k = 0; k < kScanItems; ++k) {
+        const int idx = base + k * kScanBlock + (int)threadIdx.x;
+        if (idx < total) acc += in[idx];
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += (uint32_t)__shfl_xor((int)acc, off);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0u;
+        for (int w = 0; w < kScanBlock / 64; ++w) t += part[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_tile_scan(const uint32_t* in, int total,
+    const uint32_t* offsets, uint32_t* out) {
+    __shared__ uint32_t part[kScanBlock / 64];
+    const int base = blockIdx.x * kScanTile + (int)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t t = 0u;
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = base + k < total ? in[base + k] : 0u;
+        t += v[k];
+    }
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+    uint32_t x = t;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) part[wave] = x;
+    __syncthreads();
+    uint32_t run = offsets[blockIdx.x] + x - t;
+    for (int w = 0; w < wave; ++w) run += part[w];
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < total) out[base + k] = run;
+        run += v[k];
     }
 }
 
@@ -286,11 +358,21 @@ __global__ void k_karras(const uint32_t* keys, int n, uint4* nodes_u, uint32_t* 
     if (i == 0) nodes_u[3].z = kEmpty;
 }
 
-__device__ __forceinline__ void store_child_box(float* node_f, int slot, const float lo[3],
+// A child box passes from the thread that finished the child to the one
+// that finishes the parent, which may run on another XCD with its own L2.
+// The box is written with agent-scope atomic stores (through to the
+// coherence point) and read back with agent-scope atomic loads; the explicit
+// vmcnt(0) wait completes the stores before the arrival counter moves.  An
+// agent-scope release / acquire fence would give the same order but writes
+// back and invalidates the whole L2 of the XCD, twice per tree level.
+__device__ __forceinline__ void publish_child_box(float* node_f, int slot, const float lo[3],
     const float hi[3]) {
-    float* p = node_f + slot * 6;
-    p[0] = lo[0]; p[1] = lo[1]; p[2] = lo[2];
-    p[3] = hi[0]; p[4] = hi[1]; p[5] = hi[2];
+    uint32_t* p = reinterpret_cast<uint32_t*>(node_f + slot * 6);
+    for (int a = 0; a < 3; ++a) {
+        __hip_atomic_store(p + a, __float_as_uint(lo[a]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(p + 3 + a, __float_as_uint(hi[a]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 __device__ __forceinline__ float coherent_load(const float* p) {
@@ -342,11 +424,10 @@ __global__ void k_refit(SceneView s, int n, const uint32_t* sorted, const float4
     while (true) {
         const uint4 refs = nodes_u[4 * node + 3];
         const int slot = refs.x == child ? 0 : 1;
-        store_child_box(reinterpret_cast<float*>(nodes + 4 * node), slot, lo, hi);
-        __threadfence();
+        publish_child_box(reinterpret_cast<float*>(nodes + 4 * node), slot, lo, hi);
         const uint32_t old = atomicAdd(&flags[node], 1u);
         if (old == 0u) return;
-        __threadfence();
+        asm volatile("" ::: "memory");   // the sibling's box is read after the counter
         const float* nf = reinterpret_cast<const float*>(nodes + 4 * node);
         float acc_lo[3], acc_hi[3];
         for (int a = 0; a < 3; ++a) { acc_lo[a] = __builtin_inff(); acc_hi[a] = -__builtin_inff(); }
@@ -579,9 +660,10 @@ hipError_t bvh_collapse_wide(const float4* nodes, int nbin, WideScratch& w, floa
         if (host[cur] == 0u) break;
     }
     // compact numbering: exclusive scan of the marks
-    e = hipMemcpyAsync(w.index, w.is_wide, sizeof(uint32_t) * (size_t)nbin, hipMemcpyDeviceToDevice, st);
-    if (e != hipSuccess) return e;
-    k_scan<<<1, 1024, 0, st>>>(w.index, nbin);
+    const int tiles = scan_tiles(nbin);
+    k_tile_sums<<<tiles, kScanBlock, 0, st>>>(w.is_wide, nbin, w.tiles);
+    k_scan<<<1, 1024, 0, st>>>(w.tiles, tiles);
+    k_tile_scan<<<tiles, kScanBlock, 0, st>>>(w.is_wide, nbin, w.tiles, w.index);
     uint32_t last[2] = {0u, 0u};
     e = hipMemcpyAsync(&last[0], w.index + (nbin - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess)
@@ -598,7 +680,7 @@ hipError_t bvh_build(const SceneView& s, int n, BvhBuildScratch& w, float4* node
     uint32_t* sorted_out, uint32_t* morton_out, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     k_init_bounds<<<1, 64, 0, st>>>(w.cb);
-    k_prim_bounds<<<blocks_for(n, 256), 256, 0, st>>>(s, n, w.pbox, w.cb);
+    k_prim_bounds<<<bounds_blocks(n), kBoundsBlock, 0, st>>>(s, n, w.pbox, w.cb);
     k_morton<<<blocks_for(n, 256), 256, 0, st>>>(n, w.pbox, w.cb, w.keys[0], w.vals[0]);
     const int nblocks = blocks_for(n, kSortTile);
     int cur = 0;
@@ -627,6 +709,21 @@ hipError_t bvh_build(const SceneView& s, int n, BvhBuildScratch& w, float4* node
     k_refit<<<blocks_for(n, 256), 256, 0, st>>>(s, n, sorted_out, w.pbox, w.leaf_parent, nodes,
         w.flags, leaves);
     k_depth<<<blocks_for(n, 256), 256, 0, st>>>(n, w.leaf_parent, nodes, w.cb);
+    return hipGetLastError();
+}
+
+hipError_t bvh_refit(const SceneView& s, int n, BvhBuildScratch& w, float4* nodes, float4* leaves,
+    const uint32_t* sorted, const uint32_t* leaf_parent, hipStream_t st) {
+    if (n <= 1) return hipErrorInvalidValue;
+    k_init_bounds<<<1, 64, 0, st>>>(w.cb);
+    k_prim_bounds<<<bounds_blocks(n), kBoundsBlock, 0, st>>>(s, n, w.pbox, w.cb);
+    const hipError_t e = hipMemsetAsync(w.flags, 0, sizeof(uint32_t) * (size_t)(n - 1), st);
+    if (e != hipSuccess) return e;
+    // the build's bottom-up pass over the kept topology: new leaf records,
+    // new child boxes (exact min / max, so order-free and bit-identical to a
+    // sequential refit)
+    k_refit<<<blocks_for(n, 256), 256, 0, st>>>(s, n, sorted, w.pbox, leaf_parent, nodes, w.flags,
+        leaves);
     return hipGetLastError();
 }
 

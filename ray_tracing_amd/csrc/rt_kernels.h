@@ -21,13 +21,20 @@ struct BvhBuildScratch {
 
 hipError_t bvh_build(const SceneView& s, int n, BvhBuildScratch& w, float4* nodes, float4* leaves,
     uint32_t* sorted_out, uint32_t* morton_out, hipStream_t st);
+// node bounds and leaf records of new primitive positions over the topology
+// and Morton order of the last build (leaf_parent from that build)
+hipError_t bvh_refit(const SceneView& s, int n, BvhBuildScratch& w, float4* nodes, float4* leaves,
+    const uint32_t* sorted, const uint32_t* leaf_parent, hipStream_t st);
 
 struct WideScratch {
     uint32_t* is_wide;   // per binary node: roots a 4-wide node
     uint32_t* index;     // exclusive scan of is_wide: 4-wide node index
     uint32_t* front[2];  // top-down frontiers
     uint32_t* counts;    // frontier sizes
+    uint32_t* tiles;     // scan_tiles(nbin) tile sums of the node numbering scan
 };
+constexpr int kScanTile = 4096;
+inline int scan_tiles(int n) { return (n + kScanTile - 1) / kScanTile; }
 // collapse the binary LBVH (nbin nodes, root 0) into the 4-wide quantized
 // BVH; reports its node count and number of levels (synchronises `st`)
 hipError_t bvh_collapse_wide(const float4* nodes, int nbin, WideScratch& w, float4* nodes4,

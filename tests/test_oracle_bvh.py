@@ -2,6 +2,7 @@
 leaf ranges partition the primitives, every decoded child box contains the
 exact bounds of its subtree, and traversal equals brute force."""
 import numpy as np
+import pytest
 
 import oracle_ctypes as O
 from helpers import TRI_DT, random_rays, random_triangles
@@ -111,3 +112,48 @@ def test_wide_traversal_equals_brute_force():
         occ = b.trace_any(short)
         bs = O.brute_closest(scene, short)
         assert np.array_equal(occ, (bs["prim"] >= 0).astype(np.int32))
+
+
+def _soup(seed, n=3000):
+    from helpers import lambert_materials, random_triangles
+    rng = np.random.default_rng(seed)
+    return random_triangles(rng, n), lambert_materials(1)
+
+
+def test_refit_of_unchanged_geometry_is_the_build(oracle):
+    """A refit to the primitives the tree was built from reproduces the
+    build's binary and 4-wide node arrays byte for byte."""
+    t, m = _soup(31)
+    s, keep = oracle.scene_from_arrays(triangles=t, materials=m)
+    a = oracle.OracleBvh(s)
+    b = oracle.OracleBvh(s)
+    b.refit(s)
+    assert b.nodes().tobytes() == a.nodes().tobytes()
+    assert b.wide().tobytes() == a.wide().tobytes()
+
+
+def test_refit_of_moved_geometry_keeps_topology_and_traces_exactly(oracle):
+    """After moving every vertex the refit tree keeps the Morton order and
+    child references, its boxes contain their subtrees, and closest hits
+    equal brute force over the moved primitives."""
+    from helpers import random_rays
+    t, m = _soup(32)
+    s, keep = oracle.scene_from_arrays(triangles=t, materials=m)
+    ob = oracle.OracleBvh(s)
+    before = ob.nodes()
+    rng = np.random.default_rng(5)
+    t2 = t.copy()
+    for k in ("p0", "p1", "p2"):
+        t2[k] += rng.normal(0, 0.3, t2[k].shape).astype(np.float32)
+    s2, keep2 = oracle.scene_from_arrays(triangles=t2, materials=m)
+    ob.refit(s2)
+    after = ob.nodes()
+    assert np.array_equal(before["child"], after["child"]) and np.array_equal(before["parent"], after["parent"])
+    assert ob.validate() == 0
+    rays = random_rays(rng, 4000)
+    assert np.array_equal(ob.trace_closest(rays)["prim"], oracle.brute_closest(s2, rays)["prim"])
+    # a refit is not a rebuild: the fresh tree of the moved soup differs
+    assert oracle.OracleBvh(s2).nodes().tobytes() != after.tobytes()
+    with pytest.raises(ValueError):
+        fewer, k3 = oracle.scene_from_arrays(triangles=t2[:-1], materials=m)
+        ob.refit(fewer)
