@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define RT_ABI_VERSION 1
+#define RT_ABI_VERSION 2
 
 typedef enum rt_status {
     RT_OK = 0,
@@ -275,12 +275,19 @@ typedef struct rt_device_frame_view {
     const float* trust;  /* float: 1 where the history texel was accepted */
 } rt_device_frame_view;
 
-/* RestirDiagnostics (src/realtime/gpu/optix_renderer.cpp:183-215) */
+/* RestirDiagnostics (src/realtime/gpu/optix_renderer.h:40-48, computed as
+ * optix_renderer.cpp:183-215 does): over one camera's reservoirs of the last
+ * frame, the valid ones, those with temporal / spatial candidates or bias
+ * correction sources, and the largest candidate count and age */
 typedef struct rt_restir_stats {
-    uint64_t valid_reservoirs;
-    uint64_t temporal_candidates;
-    uint64_t spatial_candidates;
-    uint64_t bias_correction_sources;
+    int32_t pixel_count;
+    int32_t active_reservoir_count;
+    int32_t temporal_reuse_count;
+    int32_t spatial_reuse_count;
+    int32_t bias_corrected_count;
+    int32_t max_candidate_count;
+    int32_t max_age;
+    int32_t pad;
 } rt_restir_stats;
 
 typedef struct rt_device rt_device;
@@ -298,6 +305,10 @@ rt_status rt_device_create(int ordinal, rt_device** out);
 rt_status rt_device_destroy(rt_device* dev);
 rt_status rt_device_synchronize(rt_device* dev);
 /* name, CU count, HBM bytes; any pointer may be NULL */
+/* the device's stream (a hipStream_t): every call on the device is
+ * enqueued on it; DeviceRadianceFrameView::stream
+ * (device_radiance_frame_view.h:7-12) */
+rt_status rt_device_stream(rt_device* dev, void** stream);
 rt_status rt_device_info(rt_device* dev, char* name, size_t name_len, int* cu_count,
     uint64_t* total_bytes, uint64_t* free_bytes);
 
@@ -356,6 +367,22 @@ rt_status rt_renderer_destroy(rt_renderer* r);
  * (src/realtime/gpu/optix_renderer.cpp:230-277,434-457) */
 rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_profile* profile,
     uint32_t sample_stream, rt_frame_timing* timing);
+/* rt_render_frame with each camera slot's random-number identity given:
+ * slot c draws from sample stream sample_streams[c] (four values) under the
+ * camera key camera_keys[c] (four values in [0, 4); NULL: the slot index).
+ * The reference keeps one renderer, and one launch_sample_stream_ counter,
+ * per camera (optix_renderer.h:127, renderer_pool.cpp:160-185): per-slot
+ * streams carry those counters into one launch, and a camera rendered alone
+ * in slot 0 with key k draws the samples of camera k of a whole-rig frame.
+ * rt_render_frame(..., s, ...) equals streams {s, s, s, s}, keys {0, 1, 2, 3}. */
+rt_status rt_render_frame_streams(rt_renderer* r, const rt_camera_rig* rig,
+    const rt_profile* profile, const uint32_t* sample_streams, const uint32_t* camera_keys,
+    rt_frame_timing* timing);
+/* LaunchParameterDiagnostics (optix_renderer.h:50-53, optix_renderer.cpp:
+ * 137,217-219,257): the per-frame parameter block is allocated once per
+ * renderer (allocation_count 1) and uploaded once per frame launched */
+rt_status rt_renderer_launch_diagnostics(rt_renderer* r, uint64_t* allocation_count,
+    uint64_t* upload_count);
 /* replaces download_radiance_frame_profiled (optix_renderer.cpp:283-326):
  * W*H*4 floats for beauty/normal/albedo, W*H floats for depth; NULL skips */
 rt_status rt_frame_download(rt_renderer* r, int camera, float* beauty_rgba, float* normal_rgba,
@@ -367,7 +394,10 @@ rt_status rt_frame_download_guides(rt_renderer* r, int camera, float* flow_xy, f
     float* resolved_beauty_rgba);
 /* device view valid until the next rt_render_frame on this renderer */
 rt_status rt_frame_device_view(rt_renderer* r, int camera, rt_device_frame_view* view);
-/* OptixRenderer::reset_accumulation / reset_sequence */
+/* OptixRenderer::reset_accumulation / reset_sequence.  The sample stream is
+ * an argument of every rt_render_frame* call here (the C++ Renderer keeps
+ * the reference's per-launch counter); rt_reset_sequence records the seed
+ * and resets accumulation. */
 rt_status rt_reset_accumulation(rt_renderer* r);
 rt_status rt_reset_sequence(rt_renderer* r, uint32_t seed);
 rt_status rt_restir_diagnostics(rt_renderer* r, int camera, rt_restir_stats* out);

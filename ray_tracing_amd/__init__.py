@@ -25,9 +25,10 @@ STATUS_NAMES = {0: "RT_OK", 1: "RT_ERROR_INVALID_ARGUMENT", 2: "RT_ERROR_HIP",
 # every entry point declared in include/rt/rt_hip.h
 EXPORTED = [
     "rt_abi_version", "rt_last_error", "rt_device_create", "rt_device_destroy",
-    "rt_device_synchronize", "rt_device_info", "rt_scene_upload", "rt_scene_destroy",
+    "rt_device_synchronize", "rt_device_info", "rt_device_stream", "rt_scene_upload", "rt_scene_destroy",
     "rt_bvh_build", "rt_bvh_download", "rt_scene_update", "rt_scene_last_update", "rt_trace_closest", "rt_trace_any",
     "rt_trace_closest_device", "rt_trace_closest_device_noalloc", "rt_renderer_create", "rt_renderer_destroy", "rt_render_frame",
+    "rt_render_frame_streams", "rt_renderer_launch_diagnostics",
     "rt_frame_download", "rt_frame_device_view", "rt_reset_accumulation", "rt_reset_sequence",
     "rt_restir_diagnostics", "rt_renderer_set_tuning", "rt_renderer_get_tuning",
     "rt_scene_generate", "rt_scene_generated_free", "rt_restir_download",
@@ -226,12 +227,26 @@ class Renderer:
         return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
     def render(self, rig: CameraRig, profile: Profile, sample_stream: int = 0,
-               timed: bool = True) -> FrameTiming | None:
+               timed: bool = True, streams=None, keys=None) -> FrameTiming | None:
+        """One frame of the rig. `streams` (four sample streams, one per
+        camera slot) and `keys` (four camera keys) give each slot its own
+        random-number identity; by default every slot draws `sample_stream`
+        under its own index."""
         self.rig = rig
         tm = FrameTiming()
-        check(lib().rt_render_frame(self.h, C.byref(rig), C.byref(profile), sample_stream,
-                                    C.byref(tm) if timed else None))
+        out = C.byref(tm) if timed else None
+        if streams is None and keys is None:
+            check(lib().rt_render_frame(self.h, C.byref(rig), C.byref(profile), sample_stream, out))
+        else:
+            st = (C.c_uint32 * 4)(*(streams if streams is not None else [sample_stream] * 4))
+            ky = (C.c_uint32 * 4)(*keys) if keys is not None else None
+            check(lib().rt_render_frame_streams(self.h, C.byref(rig), C.byref(profile), st, ky, out))
         return tm if timed else None
+
+    def launch_diagnostics(self) -> dict:
+        a, u = C.c_uint64(), C.c_uint64()
+        check(lib().rt_renderer_launch_diagnostics(self.h, C.byref(a), C.byref(u)))
+        return {"allocation_count": a.value, "upload_count": u.value}
 
     def guides(self, camera: int) -> dict:
         """Flow (pixels to the previous frame), flow trust and the resolved

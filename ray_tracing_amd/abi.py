@@ -133,8 +133,10 @@ class Tuning(C.Structure):
 
 
 class RestirStats(C.Structure):
-    _fields_ = [("valid_reservoirs", C.c_uint64), ("temporal_candidates", C.c_uint64),
-                ("spatial_candidates", C.c_uint64), ("bias_correction_sources", C.c_uint64)]
+    _fields_ = [("pixel_count", C.c_int32), ("active_reservoir_count", C.c_int32),
+                ("temporal_reuse_count", C.c_int32), ("spatial_reuse_count", C.c_int32),
+                ("bias_corrected_count", C.c_int32), ("max_candidate_count", C.c_int32),
+                ("max_age", C.c_int32), ("pad", C.c_int32)]
 
 
 class AutotuneResult(C.Structure):
@@ -242,6 +244,9 @@ def declare(L):
     L.rt_renderer_create.argtypes = [vp, vp, P(vp)]
     L.rt_renderer_destroy.argtypes = [vp]
     L.rt_render_frame.argtypes = [vp, P(CameraRig), P(Profile), C.c_uint32, P(FrameTiming)]
+    L.rt_render_frame_streams.argtypes = [vp, P(CameraRig), P(Profile), P(C.c_uint32),
+                                          P(C.c_uint32), P(FrameTiming)]
+    L.rt_renderer_launch_diagnostics.argtypes = [vp, P(C.c_uint64), P(C.c_uint64)]
     L.rt_frame_download.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     L.rt_frame_device_view.argtypes = [vp, C.c_int, P(DeviceFrameView)]
     L.rt_presenter_create.argtypes = [vp, C.c_int32, C.c_int32, P(vp)]

@@ -84,7 +84,15 @@ HOST_TARGETS = {"render_realtime": (os.path.join(HERE, "render_realtime"),
                                      os.path.join(HOST, "rt_host.cpp")]),
                 "test_report": (os.path.join(TESTS_HOST, "test_report"),
                                 [os.path.join(TESTS_HOST, "test_report.cpp"),
-                                 os.path.join(HOST, "rt_host.cpp")])}
+                                 os.path.join(HOST, "rt_host.cpp")]),
+                "test_host_api": (os.path.join(TESTS_HOST, "test_host_api"),
+                                  [os.path.join(TESTS_HOST, "test_host_api.cpp"),
+                                   os.path.join(HOST, "rt_host.cpp")]),
+                "test_pool_gpu": (os.path.join(TESTS_HOST, "test_pool_gpu"),
+                                  [os.path.join(TESTS_HOST, "test_pool_gpu.cpp"),
+                                   os.path.join(HOST, "rt_host.cpp")])}
+# targets that call the HIP runtime themselves (pointer attributes)
+HIP_HOST_TARGETS = {"test_pool_gpu"}
 
 
 def source_provenance():
@@ -118,6 +126,9 @@ def build_host(verbose: bool = True):
         cmd = ["g++", "-std=c++20", "-O2", "-Wall", f'-DRT_SOURCE_REVISION="{revision}"',
                f'-DRT_SOURCE_STATE_SHA256="{state}"', "-I", HOST, "-o", out] + srcs
         cmd += ["-L", HERE, "-lrt_hip", "-Wl,-rpath,$ORIGIN/" + rpath]
+        if name in HIP_HOST_TARGETS:
+            cmd += ["-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROCM, "include"),
+                    "-L", os.path.join(ROCM, "lib"), "-lamdhip64"]
         _run(cmd)
         outs.append(out)
         if verbose:

@@ -236,6 +236,7 @@ struct rt_renderer {
     rt_scene* scene = nullptr;
     rt_tuning tuning{};
     uint32_t sequence_seed = 0;
+    uint64_t parameter_uploads = 0;  // frames launched (one FrameDyn upload each)
     // rig
     rt_camera* host_cams = nullptr;  // pinned
     DevBuf<rt_camera> cams;
@@ -716,6 +717,12 @@ rt_status rt_device_destroy(rt_device* dev) {
 rt_status rt_device_synchronize(rt_device* dev) {
     RT_REQUIRE(dev != nullptr, RT_ERROR_INVALID_ARGUMENT, "null device");
     RT_HIP(hipStreamSynchronize(dev->stream));
+    return RT_OK;
+}
+
+rt_status rt_device_stream(rt_device* dev, void** stream) {
+    RT_REQUIRE(dev != nullptr && stream != nullptr, RT_ERROR_INVALID_ARGUMENT, "null device or output");
+    *stream = static_cast<void*>(dev->stream);
     return RT_OK;
 }
 
@@ -1247,8 +1254,10 @@ static rt_status validate_request(const rt_camera_rig* rig, const rt_profile* p)
     return RT_OK;
 }
 
-rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_profile* p,
-    uint32_t sample_stream, rt_frame_timing* timing) {
+// one frame of the rig's active cameras; slot c draws its random numbers
+// under camera key keys[c] from sample stream streams[c]
+static rt_status render_frame_impl(rt_renderer* r, const rt_camera_rig* rig, const rt_profile* p,
+    const uint32_t streams[4], const uint32_t keys[4], rt_frame_timing* timing) {
     RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
     const rt_status vs = validate_request(rig, p);
     if (vs != RT_OK) return vs;
@@ -1311,13 +1320,17 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
 
     // per-frame values, read by the kernels from device memory
     rtk::FrameDyn* dyn = r->host_dyn;
-    dyn->stream = sample_stream;
+    for (int c = 0; c < 4; ++c) {
+        dyn->stream[c] = streams[c];
+        dyn->camera_key[c] = keys[c];
+    }
     dyn->restir_history = r->restir_last ? 1 : 0;
     for (int c = 0; c < 4; ++c) {
         dyn->history_length[c] = r->history_length[c];
         dyn->prev_valid[c] = r->prev_valid[c];
     }
     RT_HIP(hipMemcpyAsync(r->dyn.p, dyn, sizeof(rtk::FrameDyn), hipMemcpyHostToDevice, st));
+    ++r->parameter_uploads;
 
     rtk::FrameParams fp{};
     fp.spp = spp;
@@ -1609,6 +1622,40 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
     return RT_OK;
 }
 
+rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_profile* p,
+    uint32_t sample_stream, rt_frame_timing* timing) {
+    const uint32_t streams[4] = {sample_stream, sample_stream, sample_stream, sample_stream};
+    const uint32_t keys[4] = {0u, 1u, 2u, 3u};
+    return render_frame_impl(r, rig, p, streams, keys, timing);
+}
+
+rt_status rt_render_frame_streams(rt_renderer* r, const rt_camera_rig* rig, const rt_profile* p,
+    const uint32_t* sample_streams, const uint32_t* camera_keys, rt_frame_timing* timing) {
+    RT_REQUIRE(sample_streams != nullptr, RT_ERROR_INVALID_ARGUMENT,
+        "rt_render_frame_streams requires four sample streams");
+    uint32_t keys[4] = {0u, 1u, 2u, 3u};
+    if (camera_keys != nullptr) {
+        for (int c = 0; c < 4; ++c) {
+            RT_REQUIRE(camera_keys[c] < 4u, RT_ERROR_INVALID_ARGUMENT,
+                "camera key " + std::to_string(camera_keys[c]) + " of slot " + std::to_string(c)
+                    + " out of range [0, 4)");
+            keys[c] = camera_keys[c];
+        }
+    }
+    return render_frame_impl(r, rig, p, sample_streams, keys, timing);
+}
+
+// LaunchParameterDiagnostics (optix_renderer.cpp:130-138,217-219,257): the
+// per-frame parameter block is allocated once with the renderer and
+// uploaded once per frame
+rt_status rt_renderer_launch_diagnostics(rt_renderer* r, uint64_t* allocation_count,
+    uint64_t* upload_count) {
+    RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
+    if (allocation_count) *allocation_count = 1u;
+    if (upload_count) *upload_count = r->parameter_uploads;
+    return RT_OK;
+}
+
 rt_status rt_frame_download(rt_renderer* r, int camera, float* beauty, float* normal, float* albedo,
     float* depth) {
     RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
@@ -1763,11 +1810,15 @@ rt_status rt_restir_diagnostics(rt_renderer* r, int camera, rt_restir_stats* out
     RT_HIP(hipMemcpyAsync(host.data(), r->reservoirs[r->last_reservoirs].p + r->pixel_offset[camera],
         n * sizeof(rtk::Reservoir), hipMemcpyDeviceToHost, r->dev->stream));
     RT_HIP(hipStreamSynchronize(r->dev->stream));
+    out->pixel_count = (int32_t)n;
     for (const rtk::Reservoir& v : host) {
-        out->valid_reservoirs += v.valid != 0 ? 1u : 0u;
-        out->temporal_candidates += (uint64_t)std::max(v.temporal_candidate_count, 0);
-        out->spatial_candidates += (uint64_t)std::max(v.spatial_candidate_count, 0);
-        out->bias_correction_sources += (uint64_t)std::max(v.bias_correction_source_count, 0);
+        if (v.valid == 0) continue;
+        ++out->active_reservoir_count;
+        if (v.temporal_candidate_count > 0) ++out->temporal_reuse_count;
+        if (v.spatial_candidate_count > 0) ++out->spatial_reuse_count;
+        if (v.bias_correction_source_count > 0) ++out->bias_corrected_count;
+        out->max_candidate_count = std::max(out->max_candidate_count, v.candidate_count);
+        out->max_age = std::max(out->max_age, v.age);
     }
     return RT_OK;
 }

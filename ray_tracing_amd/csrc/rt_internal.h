@@ -107,7 +107,8 @@ struct Reservoir {
 // memory so that a captured graph of the frame's launches stays valid: the
 // host uploads them before every frame (eager or graph replay).
 struct FrameDyn {
-    uint32_t stream;            // sample stream (RNG key)
+    uint32_t stream[4];         // sample stream of each camera slot (RNG key)
+    uint32_t camera_key[4];     // camera key of each slot (RNG key; the slot index by default)
     int32_t restir_history;     // the previous frame ran ReSTIR (its reservoirs are history)
     int32_t history_length[4];  // frames of history per camera (capture_launch_history)
     int32_t prev_valid[4];      // previous camera valid per camera

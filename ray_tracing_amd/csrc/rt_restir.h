@@ -217,12 +217,14 @@ struct RestirOut {
 };
 
 // accumulate_restir_analytic_direct_light (programs.cu:2315-2449) up to the
-// shadow ray, which the connect stage traces.  `base` is the camera's first
-// pixel in the reservoir buffers, which hold all cameras back to back.
+// shadow ray, which the connect stage traces.  `cam` is the camera's slot and
+// `base` its first pixel in the reservoir buffers, which hold all cameras
+// back to back; `key` is the camera key of the slot's random numbers.
 template <int BLOCK>
 __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& rv,
-    const HitInfo& h, f3 ray_d, int cam, int base, int x, int y, int width, int height, uint32_t pixel,
-    uint32_t sample, uint32_t stream, bool bsdf_avail, Reservoir* out, const TraceStack& st) {
+    const HitInfo& h, f3 ray_d, int cam, uint32_t key, int base, int x, int y, int width, int height,
+    uint32_t pixel, uint32_t sample, uint32_t stream, bool bsdf_avail, Reservoir* out,
+    const TraceStack& st) {
     RestirOut ro{};
     ro.shadow = false;
     if (h.mat_type == RT_MAT_DIFFUSE_LIGHT || s.analytic_count <= 0) return ro;
@@ -231,7 +233,7 @@ __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& r
     const int req = rv.initial_candidates > 0 ? rv.initial_candidates : 1;
     const int ncand = req < 32 ? req : 32;
     for (int k = 0; k < ncand; ++k) {
-        const float4 q = rng4(pixel, sample, (uint32_t)cam, 0u, kGroupRestir0 + (uint32_t)k, stream);
+        const float4 q = rng4(pixel, sample, key, 0u, kGroupRestir0 + (uint32_t)k, stream);
         const int li = pick_analytic(s, q.x);
         const LightSample L = analytic_candidate(s, h.pos, li, q.y, q.z);
         float target = 0.0f;
@@ -242,11 +244,11 @@ __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& r
     const int canonical = r.candidate_count;
     float sel_u[12], off_u[16];
     for (int g = 0; g < 3; ++g) {
-        const float4 q = rng4(pixel, sample, (uint32_t)cam, 0u, kGroupRestir0 + 32u + (uint32_t)g, stream);
+        const float4 q = rng4(pixel, sample, key, 0u, kGroupRestir0 + 32u + (uint32_t)g, stream);
         sel_u[4 * g] = q.x; sel_u[4 * g + 1] = q.y; sel_u[4 * g + 2] = q.z; sel_u[4 * g + 3] = q.w;
     }
     for (int g = 0; g < 4; ++g) {
-        const float4 q = rng4(pixel, sample, (uint32_t)cam, 0u, kGroupRestir0 + 35u + (uint32_t)g, stream);
+        const float4 q = rng4(pixel, sample, key, 0u, kGroupRestir0 + 35u + (uint32_t)g, stream);
         off_u[4 * g] = q.x; off_u[4 * g + 1] = q.y; off_u[4 * g + 2] = q.z; off_u[4 * g + 3] = q.w;
     }
     Reservoir cur;

@@ -48,11 +48,30 @@ enum : int32_t { kPrevValid = 1, kPrevDelta = 2 };
 struct SlotId {
     uint32_t pixel;   // pixel index inside its camera
     uint32_t sample;
-    uint32_t camera;
+    uint32_t camera;        // slot in the rig (pixel layout, history)
     uint32_t global_pixel;
+    uint32_t key;           // the slot's camera key and sample stream (RNG identity)
+    uint32_t stream;
 };
 
-__device__ __forceinline__ SlotId slot_id(const CameraView& cams, const FrameParams& fp, uint32_t slot) {
+// the frame's per-slot RNG identity, read once per wave (uniform loads)
+struct SlotRng {
+    uint4 key;
+    uint4 stream;
+};
+__device__ __forceinline__ SlotRng slot_rng(const FrameParams& fp) {
+    SlotRng r;
+    r.key = make_uint4(fp.dyn->camera_key[0], fp.dyn->camera_key[1], fp.dyn->camera_key[2],
+        fp.dyn->camera_key[3]);
+    r.stream = make_uint4(fp.dyn->stream[0], fp.dyn->stream[1], fp.dyn->stream[2], fp.dyn->stream[3]);
+    return r;
+}
+__device__ __forceinline__ uint32_t pick4(uint4 v, uint32_t c) {
+    return c == 0u ? v.x : (c == 1u ? v.y : (c == 2u ? v.z : v.w));
+}
+
+__device__ __forceinline__ SlotId slot_id(const CameraView& cams, const FrameParams& fp,
+    const SlotRng& rng, uint32_t slot) {
     const uint32_t P = (uint32_t)fp.total_pixels;
     const uint32_t layer = slot / P;
     const uint32_t p = slot - layer * P;
@@ -63,6 +82,8 @@ __device__ __forceinline__ SlotId slot_id(const CameraView& cams, const FramePar
     id.sample = (uint32_t)fp.first_sample + layer;
     id.camera = c;
     id.global_pixel = p;
+    id.key = pick4(rng.key, c);
+    id.stream = pick4(rng.stream, c);
     return id;
 }
 
@@ -101,7 +122,7 @@ __global__ void k_zero_words(uint32_t* p, size_t n) {
 
 __global__ __launch_bounds__(256) void k_raygen(CameraView cams, FrameParams fp, WavefrontBuffers b,
     int paths) {
-    const uint32_t fp_stream = fp.dyn->stream;
+    const SlotRng slot_rng_v = slot_rng(fp);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         b.counters[kCntQueue0] = (uint32_t)paths;
@@ -115,14 +136,14 @@ __global__ __launch_bounds__(256) void k_raygen(CameraView cams, FrameParams fp,
         b.counters[kCntCacheQuery] = 0u;
     }
     if (i >= paths) return;
-    const SlotId id = slot_id(cams, fp, (uint32_t)i);
+    const SlotId id = slot_id(cams, fp, slot_rng_v, (uint32_t)i);
     const rt_camera& cam = cams.cams[id.camera];
     const int w = cams.width[id.camera];
     const int y = (int)id.pixel / w;
     const int x = (int)id.pixel - y * w;
     double px, py;
     if (fp.jitter) {
-        const float4 r = rng4(id.pixel, id.sample, id.camera, 0u, kGroupCamera, fp_stream);
+        const float4 r = rng4(id.pixel, id.sample, id.key, 0u, kGroupCamera, id.stream);
         px = (double)((float)x + r.x);
         py = (double)((float)y + r.y);
     } else {
@@ -281,7 +302,7 @@ __device__ __forceinline__ CacheInput cache_input(const SceneView& s, const HitI
 
 __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, FrameParams fp,
     WavefrontBuffers b, int bounce, int cur) {
-    const uint32_t fp_stream = fp.dyn->stream;
+    const SlotRng slot_rng_v = slot_rng(fp);
     const uint32_t count = b.counters[cur];
     const uint32_t* queue = b.queue[cur];
     uint32_t* next_queue = b.queue[cur ^ 1];
@@ -295,7 +316,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
         float4 s2_o = sh_o, s2_d = sh_o, s2_c = sh_o;
         if (active) {
             slot = queue[q];
-            const SlotId id = slot_id(cams, fp, slot);
+            const SlotId id = slot_id(cams, fp, slot_rng_v, slot);
             const f3 ro = xyz(b.ray_o[slot]);
             const f3 rd = xyz(b.ray_d[slot]);
             const float4 hv = b.hit[slot];
@@ -322,10 +343,10 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                 const float w = emission_mis(s, pv, rd, nullptr, true);
                 rad = rad + thr * (background_at(s, rd) * w);
             } else {
-                const float4 rl = rng4(id.pixel, id.sample, id.camera, (uint32_t)bounce,
-                    kGroupLight, fp_stream);
-                const float4 rb = rng4(id.pixel, id.sample, id.camera, (uint32_t)bounce,
-                    kGroupBsdf, fp_stream);
+                const float4 rl = rng4(id.pixel, id.sample, id.key, (uint32_t)bounce,
+                    kGroupLight, id.stream);
+                const float4 rb = rng4(id.pixel, id.sample, id.key, (uint32_t)bounce,
+                    kGroupBsdf, id.stream);
                 HitInfo h;
                 make_hit(s, ro, rd, hv.x, prim, h);
                 if (bounce == 0) {
@@ -339,7 +360,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                 // is usable the other paths end here with its prediction
                 bool cache_end = false;
                 if (bounce == fp.cache_bounce && h.mat_type != RT_MAT_DIFFUSE_LIGHT) {
-                    const float4 rc = rng4(id.pixel, id.sample, id.camera, 0u, kGroupCache, fp_stream);
+                    const float4 rc = rng4(id.pixel, id.sample, id.key, 0u, kGroupCache, id.stream);
                     const bool train = rc.x < fp.cache_state->train_fraction;
                     const bool query = !train && fp.cache_state->usable != 0;
                     CacheInput ci{};
@@ -392,8 +413,8 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                         b.restir_thr[slot] = f4(thr, __int_as_float(prim));
                     }
                 } else if (!cache_end && h.mat_type != RT_MAT_DIFFUSE_LIGHT && s.analytic_count > 0) {
-                    const float4 ra = rng4(id.pixel, id.sample, id.camera, (uint32_t)bounce,
-                        kGroupAnalytic, fp_stream);
+                    const float4 ra = rng4(id.pixel, id.sample, id.key, (uint32_t)bounce,
+                        kGroupAnalytic, id.stream);
                     const int li = pick_analytic(s, ra.x);
                     const LightSample ls = analytic_candidate(s, h.pos, li, ra.y, ra.z);
                     if (ls.valid) {
@@ -479,7 +500,7 @@ __global__ __launch_bounds__(256) void k_analytic_hits(SceneView s, WavefrontBuf
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, FrameParams fp,
     RestirView rv, WavefrontBuffers b) {
-    const uint32_t fp_stream = fp.dyn->stream;
+    const SlotRng slot_rng_v = slot_rng(fp);
     __shared__ uint2 lds[kStackLds * BLOCK];
     const uint32_t count = b.counters[kCntRestir];
     const TraceStack st = lane_stack<BLOCK>(lds, b.spill);
@@ -491,7 +512,7 @@ __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, 
         float4 s_o = make_float4(0, 0, 0, 0), s_d = s_o, s_c = s_o;
         if (active) {
             const uint32_t slot = b.restir_queue[q];
-            const SlotId id = slot_id(cams, fp, slot);
+            const SlotId id = slot_id(cams, fp, slot_rng_v, slot);
             const float4 o4 = b.restir_o[slot];
             const float4 d4 = b.restir_d[slot];
             const float4 t4 = b.restir_thr[slot];
@@ -505,9 +526,9 @@ __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, 
                                  ? rv.current + id.global_pixel
                                  : nullptr;
             const bool bsdf_avail = 1 < fp.max_bounces;
-            const RestirOut ro = restir_shade<BLOCK>(s, rv, h, xyz(d4), (int)id.camera,
+            const RestirOut ro = restir_shade<BLOCK>(s, rv, h, xyz(d4), (int)id.camera, id.key,
                 cams.pixel_offset[id.camera], x, y, w, hgt,
-                id.pixel, id.sample, fp_stream, bsdf_avail, out, st);
+                id.pixel, id.sample, id.stream, bsdf_avail, out, st);
             if (ro.shadow) {
                 shadow = true;
                 s_o = f4(ro.origin, ro.max_t);

@@ -256,14 +256,20 @@ int main(int argc, char** argv) {
     abi_profile = scene_profile ? gen->profile : rt_amd::to_abi(profile);
     std::filesystem::create_directories(a.output_dir);
     try {
-        rt_amd::Renderer renderer(0);
+        rt_camera_rig rig{};
+        rig.active_count = std::min(a.camera_count, gen->camera_count);
+        for (int c = 0; c < rig.active_count; ++c) rig.cameras[c] = gen->cameras[c];
+        const rt_amd::RenderProfile run_profile = rt_amd::from_abi(abi_profile);
+        // one pool slot per camera, as the reference's CLI sizes its pool
+        // (render_realtime.cpp:289-291)
+        rt_amd::RendererPool pool(rig.active_count, 0);
+        rt_amd::Renderer& renderer = pool.renderer();
         char name[128] = {0};
         int cus = 0;
         uint64_t total_b = 0, free_b = 0;
         rt_amd::throw_on_error(rt_device_info(renderer.device(), name, sizeof(name), &cus, &total_b, &free_b));
         const uint64_t baseline_used = total_b - free_b;
-        renderer.prepare_scene(gen->scene);
-        renderer.reset_sequence(a.seed);
+        pool.prepare_scene(gen->scene);
         rt_nn* nn = nullptr;
         if (!a.denoiser_weights.empty()) {
             const std::vector<float> w = load_npy(a.denoiser_weights);
@@ -277,9 +283,6 @@ int main(int argc, char** argv) {
             t.count_traversal = -1;
             rt_amd::throw_on_error(rt_renderer_set_tuning(renderer.handle(), &t));
         }
-        rt_camera_rig rig{};
-        rig.active_count = std::min(a.camera_count, gen->camera_count);
-        for (int c = 0; c < rig.active_count; ++c) rig.cameras[c] = gen->cameras[c];
         if (a.autotune) {
             rt_autotune_result ar{};
             rt_amd::throw_on_error(rt_renderer_autotune(renderer.handle(), &rig, &abi_profile, 2, &ar));
@@ -288,6 +291,9 @@ int main(int argc, char** argv) {
                 ar.trials, ar.baseline_ms, ar.best_ms, ar.best.trace_block, ar.best.trace_blocks_per_cu,
                 ar.best.trace_refill, ar.best.wavefront_paths, ar.best.use_graph);
         }
+        // the run's frames start at the seed's sample stream (after any
+        // autotuning trial frames)
+        pool.reset_sequence(a.seed);
         rt_device_info(renderer.device(), name, sizeof(name), &cus, &total_b, &free_b);
         const uint64_t prepared_used = total_b - free_b;
         uint64_t peak_used = std::max(baseline_used, prepared_used);
@@ -311,13 +317,6 @@ int main(int argc, char** argv) {
         report.gpu_memory.baseline_used_bytes = baseline_used;
         report.gpu_memory.prepared_used_bytes = prepared_used;
         const rt_bvh_stats bs = renderer.acceleration_stats();
-        report.gpu_scheduling.persistent_worker_count = 1;
-        report.gpu_scheduling.worker_start_count = 1;
-        report.gpu_scheduling.acceleration_update_kind = "rebuild";
-        report.gpu_scheduling.acceleration_update_ms = bs.build_ms;
-        report.gpu_scheduling.acceleration_node_count = bs.wide_node_count;
-        report.gpu_scheduling.acceleration_reference_count = bs.prim_count;
-        report.gpu_scheduling.acceleration_generation = 1;
         report.scene = a.scene;
         report.profile = profile_name;
         report.camera_count = rig.active_count;
@@ -331,12 +330,12 @@ int main(int argc, char** argv) {
         report.denoise_enabled = abi_profile.enable_denoise != 0;
         report.image_write_enabled = !a.skip_image_write;
 
-        rt_amd::RenderProfile dummy;
         for (int w = 0; w < a.warmup_frames; ++w) {
-            rt_frame_timing t{};
-            rt_amd::throw_on_error(rt_render_frame(renderer.handle(), &rig, &abi_profile, a.seed + w, &t));
-            std::printf("warmup=%d sample_stream=%u cameras=%d pipeline_ms=%.3f\n", w, a.seed + w,
-                rig.active_count, t.render_ms);
+            const auto w0 = std::chrono::steady_clock::now();
+            const std::vector<rt_amd::CameraRenderResult> res = pool.render_frame(rig, run_profile, rig.active_count);
+            const double wms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+            std::printf("warmup=%d sample_stream=%u cameras=%zu pipeline_ms=%.3f\n", w, a.seed + w,
+                res.size(), wms);
         }
         for (int f = 0; f < a.frames; ++f) {
             rt_amd::profiling::FrameStageSample s;
@@ -350,37 +349,35 @@ int main(int argc, char** argv) {
             s.max_bounces = report.max_bounces;
             s.denoise_enabled = report.denoise_enabled;
             const auto t0 = std::chrono::steady_clock::now();
-            rt_frame_timing t{};
-            rt_amd::throw_on_error(rt_render_frame(renderer.handle(), &rig, &abi_profile, s.sample_stream, &t));
+            const std::vector<rt_amd::CameraRenderResult> res = pool.render_frame(rig, run_profile, rig.active_count);
             const auto t1 = std::chrono::steady_clock::now();
             double lum_sum = 0.0, download_ms = 0.0;
-            for (int c = 0; c < rig.active_count; ++c) {
-                const auto d0 = std::chrono::steady_clock::now();
-                const rt_amd::RadianceFrame fr = renderer.download(c);
-                const double dms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - d0).count();
-                download_ms += dms;
+            for (const rt_amd::CameraRenderResult& cr : res) {
+                const rt_amd::RadianceFrame& fr = cr.profiled.frame;
+                download_ms += cr.profiled.timing.download_ms;
                 rt_amd::profiling::CameraStageSample cs;
-                cs.camera_index = c;
-                cs.render_ms = t.render_ms - t.denoise_ms;
-                cs.denoise_ms = t.denoise_ms;
-                cs.download_ms = dms;
+                cs.camera_index = cr.camera_index;
+                cs.render_ms = cr.profiled.timing.render_ms;
+                cs.denoise_ms = cr.profiled.timing.denoise_ms;
+                cs.download_ms = cr.profiled.timing.download_ms;
                 cs.average_luminance = fr.average_luminance;
                 lum_sum += fr.average_luminance;
                 s.cameras.push_back(cs);
                 if (!a.skip_image_write) {
                     const auto w0 = std::chrono::steady_clock::now();
-                    write_png(a.output_dir + "/frame_" + std::to_string(f) + "_cam" + std::to_string(c) + ".png", fr);
+                    write_png(a.output_dir + "/frame_" + std::to_string(f) + "_cam" + std::to_string(cr.camera_index) + ".png", fr);
                     s.image_write_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
                 }
             }
             const auto t2 = std::chrono::steady_clock::now();
-            s.render_ms = t.render_ms - t.denoise_ms;  // all cameras in one launch sequence
-            s.denoise_ms = t.denoise_ms;
+            // all cameras in one launch sequence: the shared render / denoise times
+            s.render_ms = res.empty() ? 0.0 : res[0].profiled.timing.render_ms;
+            s.denoise_ms = res.empty() ? 0.0 : res[0].profiled.timing.denoise_ms;
             s.download_ms = download_ms;
             s.render_work_ms = s.render_ms;
             s.denoise_work_ms = s.denoise_ms;
             s.download_work_ms = download_ms;
-            s.pipeline_ms = std::chrono::duration<double, std::milli>(t1 - t0).count() + download_ms;
+            s.pipeline_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
             s.frame_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
             s.host_overhead_ms = std::max(0.0, s.frame_ms - s.pipeline_ms - s.image_write_ms);
             s.fps = s.frame_ms > 0.0 ? 1000.0 / s.frame_ms : 0.0;
@@ -395,6 +392,21 @@ int main(int argc, char** argv) {
                 s.download_work_ms, s.image_write_ms, s.host_overhead_ms, s.frame_ms);
             report.frames.push_back(std::move(s));
         }
+        // GpuSchedulingReport from the pool's diagnostics (render_realtime.cpp:444-460)
+        const rt_amd::RendererPoolDiagnostics pd = pool.diagnostics();
+        report.gpu_scheduling.persistent_worker_count = pd.worker_count;
+        report.gpu_scheduling.worker_start_count = pd.worker_start_count;
+        report.gpu_scheduling.task_submission_count = pd.task_submission_count;
+        report.gpu_scheduling.launch_parameter_allocation_count = pd.launch_parameter_allocation_count;
+        report.gpu_scheduling.launch_parameter_upload_count = pd.launch_parameter_upload_count;
+        report.gpu_scheduling.acceleration_update_kind = rt_amd::acceleration_update_kind_name(pd.acceleration.kind);
+        report.gpu_scheduling.acceleration_update_ms = pd.acceleration.elapsed_ms;
+        report.gpu_scheduling.acceleration_node_count = pd.acceleration.node_count;
+        report.gpu_scheduling.acceleration_reference_count = pd.acceleration.primitive_reference_count;
+        report.gpu_scheduling.acceleration_prototype_count = pd.acceleration.prototype_count;
+        report.gpu_scheduling.acceleration_instance_count = pd.acceleration.instance_count;
+        report.gpu_scheduling.acceleration_instanced_primitive_count = pd.acceleration.instanced_primitive_count;
+        report.gpu_scheduling.acceleration_generation = pd.acceleration.generation;
         rt_device_info(renderer.device(), name, sizeof(name), &cus, &total_b, &free_b);
         report.gpu_memory.final_used_bytes = total_b - free_b;
         report.gpu_memory.peak_used_bytes = std::max<uint64_t>(peak_used, report.gpu_memory.final_used_bytes);
@@ -410,7 +422,9 @@ int main(int argc, char** argv) {
         std::printf("summary profile=%s warmup_frames=%d frames=%d seed=%u cameras=%d resolution=%dx%d spp=%d "
                     "max_bounces=%d denoise=%s avg_frame_ms=%.3f avg_denoise_ms=%.3f p95_frame_ms=%.3f "
                     "p99_frame_ms=%.3f fps=%.2f peak_gpu_memory_mib=%.2f peak_gpu_memory_delta_mib=%.2f "
-                    "workers=1/1 as_update=rebuild as_update_ms=%.3f as_nodes=%d as_refs=%d "
+                    "workers=%d/%llu launch_param_allocations=%llu launch_param_uploads=%llu "
+                    "as_update=%s as_update_ms=%.3f as_nodes=%d as_refs=%d as_prototypes=%d "
+                    "as_instances=%d instanced_primitives=%d "
                     "artifacts=benchmark_frames.csv,benchmark_summary.json,benchmark_manifest.json "
                     "output_dir=%s\n",
             profile_name.c_str(), a.warmup_frames, a.frames, a.seed, rig.active_count, report.width,
@@ -418,13 +432,19 @@ int main(int argc, char** argv) {
             report.denoise_enabled ? "true" : "false", ag.frame_ms.avg, ag.denoise_ms.avg, ag.frame_ms.p95,
             ag.frame_ms.p99, 1000.0 / ag.frame_ms.avg,
             (double)report.gpu_memory.peak_used_bytes / (1024.0 * 1024.0),
-            (double)report.gpu_memory.peak_delta_bytes / (1024.0 * 1024.0), bs.build_ms, bs.wide_node_count,
-            bs.prim_count, a.output_dir.c_str());
+            (double)report.gpu_memory.peak_delta_bytes / (1024.0 * 1024.0), pd.worker_count,
+            (unsigned long long)pd.worker_start_count,
+            (unsigned long long)pd.launch_parameter_allocation_count,
+            (unsigned long long)pd.launch_parameter_upload_count,
+            rt_amd::acceleration_update_kind_name(pd.acceleration.kind).c_str(), pd.acceleration.elapsed_ms,
+            pd.acceleration.node_count, pd.acceleration.primitive_reference_count,
+            pd.acceleration.prototype_count, pd.acceleration.instance_count,
+            pd.acceleration.instanced_primitive_count, a.output_dir.c_str());
+        (void)bs;
         if (nn) {
             rt_renderer_set_denoiser(renderer.handle(), nullptr);
             rt_nn_destroy(nn);
         }
-        (void)dummy;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "%s\n", e.what());
         rt_scene_generated_free(gen);

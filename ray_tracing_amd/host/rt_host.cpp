@@ -63,7 +63,8 @@ bool same(const RenderProfile& a, const RenderProfile& b) {
            && a.restir_spatial_neighbors == b.restir_spatial_neighbors
            && a.restir_max_spatial_candidates == b.restir_max_spatial_candidates
            && a.restir_bias_correction == b.restir_bias_correction
-           && a.restir_min_analytic_lights == b.restir_min_analytic_lights;
+           && a.restir_min_analytic_lights == b.restir_min_analytic_lights
+           && a.pixel_jitter == b.pixel_jitter && a.restir_spatial_radius == b.restir_spatial_radius;
 }
 
 double average_luminance(const std::vector<float>& rgba) {
@@ -109,9 +110,31 @@ rt_profile to_abi(const RenderProfile& p) {
     r.restir_max_spatial_candidates = p.restir_max_spatial_candidates;
     r.restir_bias_correction = static_cast<int32_t>(p.restir_bias_correction);
     r.restir_min_analytic_lights = p.restir_min_analytic_lights;
-    r.pixel_jitter = 0;
-    r.restir_spatial_radius = 0;
+    r.pixel_jitter = p.pixel_jitter;
+    r.restir_spatial_radius = p.restir_spatial_radius;
     return r;
+}
+
+RenderProfile from_abi(const rt_profile& r) {
+    RenderProfile p;
+    p.samples_per_pixel = r.samples_per_pixel;
+    p.max_bounces = r.max_bounces;
+    p.enable_denoise = r.enable_denoise != 0;
+    p.rr_start_bounce = r.rr_start_bounce;
+    p.accumulation_reset_rotation_deg = r.accumulation_reset_rotation_deg;
+    p.accumulation_reset_translation = r.accumulation_reset_translation;
+    p.enable_restir_di = r.enable_restir_di != 0;
+    p.restir_initial_candidates = r.restir_initial_candidates;
+    p.restir_temporal_reuse = r.restir_temporal_reuse != 0;
+    p.restir_max_history_age = r.restir_max_history_age;
+    p.restir_max_temporal_candidates = r.restir_max_temporal_candidates;
+    p.restir_spatial_neighbors = r.restir_spatial_neighbors;
+    p.restir_max_spatial_candidates = r.restir_max_spatial_candidates;
+    p.restir_bias_correction = static_cast<RestirBiasCorrectionMode>(r.restir_bias_correction);
+    p.restir_min_analytic_lights = r.restir_min_analytic_lights;
+    p.pixel_jitter = r.pixel_jitter;
+    p.restir_spatial_radius = r.restir_spatial_radius;
+    return p;
 }
 
 void throw_on_error(rt_status s) {
@@ -120,6 +143,82 @@ void throw_on_error(rt_status s) {
     if (s == RT_ERROR_INVALID_ARGUMENT || s == RT_ERROR_UNSUPPORTED) throw std::invalid_argument(msg);
     throw std::runtime_error(msg);
 }
+
+// ------------------------------------------------------- request validation
+void validate_render_camera_request(const rt_camera_rig& rig, int camera_index,
+    std::string_view caller) {
+    constexpr int kMaxCameraSlots = 4;
+    const std::string name(caller);
+    if (rig.active_count < 1 || rig.active_count > kMaxCameraSlots) {
+        throw std::runtime_error(name + " requires rig.active_count in [1, 4], got "
+            + std::to_string(rig.active_count));
+    }
+    if (camera_index < 0 || camera_index >= rig.active_count) {
+        throw std::runtime_error(name + " camera_index out of range: camera_index="
+            + std::to_string(camera_index) + ", active_count=" + std::to_string(rig.active_count));
+    }
+    const rt_camera& camera = rig.cameras[camera_index];
+    if (camera.enabled == 0) {
+        throw std::runtime_error(name + " camera slot is disabled at index " + std::to_string(camera_index));
+    }
+    if (camera.width <= 0 || camera.height <= 0) {
+        throw std::runtime_error(name + " camera slot has invalid resolution at index "
+            + std::to_string(camera_index) + ": width=" + std::to_string(camera.width)
+            + ", height=" + std::to_string(camera.height));
+    }
+}
+
+void validate_render_pool_request(const rt_camera_rig& rig, int active_cameras, int renderer_count,
+    std::string_view caller) {
+    const std::string name(caller);
+    if (active_cameras < 1 || active_cameras > renderer_count)
+        throw std::runtime_error(name + " active_cameras out of range");
+    if (rig.active_count < 1 || rig.active_count > 4)
+        throw std::runtime_error(name + " rig.active_count out of range");
+    if (active_cameras > rig.active_count)
+        throw std::runtime_error(name + " active_cameras exceeds rig.active_count");
+    for (int c = 0; c < active_cameras; ++c) {
+        const rt_camera& camera = rig.cameras[c];
+        if (camera.enabled == 0)
+            throw std::runtime_error(name + " leading camera slot disabled at index " + std::to_string(c));
+        if (camera.width <= 0 || camera.height <= 0)
+            throw std::runtime_error(name + " leading camera slot has invalid resolution at index "
+                + std::to_string(c));
+    }
+}
+
+std::string acceleration_update_kind_name(AccelerationUpdateKind kind) {
+    switch (kind) {
+        case AccelerationUpdateKind::reuse: return "reuse";
+        case AccelerationUpdateKind::update: return "update";
+        case AccelerationUpdateKind::refit: return "refit";
+        case AccelerationUpdateKind::rebuild: return "rebuild";
+    }
+    return "rebuild";
+}
+
+namespace {
+
+AccelerationUpdateStats to_stats(const rt_acceleration_update& u) {
+    AccelerationUpdateStats a;
+    switch (u.kind) {
+        case RT_ACCEL_REUSE: a.kind = AccelerationUpdateKind::reuse; break;
+        case RT_ACCEL_UPDATE: a.kind = AccelerationUpdateKind::update; break;
+        case RT_ACCEL_REFIT: a.kind = AccelerationUpdateKind::refit; break;
+        default: a.kind = AccelerationUpdateKind::rebuild; break;
+    }
+    a.elapsed_ms = u.elapsed_ms;
+    a.node_count = u.node_count;
+    a.primitive_reference_count = u.primitive_reference_count;
+    a.generation = u.generation;
+    return a;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
 
 // ----------------------------------------------------------------- Renderer
 Renderer::Renderer(int ordinal) { throw_on_error(rt_device_create(ordinal, &device_)); }
@@ -130,40 +229,83 @@ Renderer::~Renderer() {
     if (device_) rt_device_destroy(device_);
 }
 
+// OptixRenderer::upload_scene / SharedGpuSceneState::prepare
+// (optix_renderer.cpp:76-90,154-158): the first scene is uploaded and built,
+// later ones go through the acceleration lifecycle
+void Renderer::upload_scene(const rt_scene_desc& scene) {
+    scene_prepared_ = false;
+    if (scene.sphere_count + scene.quad_count + scene.triangle_count <= 0)
+        throw std::runtime_error("render_radiance requires at least one surface primitive");
+    rt_acceleration_update u{};
+    if (scene_ == nullptr) {
+        throw_on_error(rt_scene_upload(device_, &scene, &scene_));
+        throw_on_error(rt_bvh_build(scene_, &bvh_));
+        throw_on_error(rt_scene_last_update(scene_, &u));
+    } else {
+        throw_on_error(rt_scene_update(scene_, &scene, &u));
+        bvh_.prim_count = u.primitive_reference_count;
+        bvh_.node_count = u.node_count;
+        bvh_.wide_node_count = u.wide_node_count;
+        bvh_.build_ms = u.bvh_ms;
+    }
+    if (renderer_ == nullptr) throw_on_error(rt_renderer_create(device_, scene_, &renderer_));
+    acceleration_ = to_stats(u);
+}
+
 void Renderer::prepare_scene(const rt_scene_desc& scene) {
-    if (renderer_) {
-        rt_renderer_destroy(renderer_);
-        renderer_ = nullptr;
-    }
-    if (scene_) {
-        rt_scene_destroy(scene_);
-        scene_ = nullptr;
-    }
-    throw_on_error(rt_scene_upload(device_, &scene, &scene_));
-    throw_on_error(rt_bvh_build(scene_, &bvh_));
-    throw_on_error(rt_renderer_create(device_, scene_, &renderer_));
+    upload_scene(scene);
+    use_prepared_scene(scene);
+}
+
+void Renderer::use_prepared_scene(const rt_scene_desc&) {
+    if (scene_ == nullptr || renderer_ == nullptr)
+        throw std::runtime_error("use_prepared_scene requires a prepared scene");
+    scene_prepared_ = true;
+    launch_sample_stream_ = 0;
+    reset_accumulation();
 }
 
 void Renderer::reset_accumulation() {
     if (renderer_) throw_on_error(rt_reset_accumulation(renderer_));
 }
 
-void Renderer::reset_sequence(uint32_t seed) {
-    if (renderer_) throw_on_error(rt_reset_sequence(renderer_, seed));
+void Renderer::reset_sequence(std::uint32_t sample_stream) {
+    launch_sample_stream_ = sample_stream;
+    reset_accumulation();
 }
 
-rt_frame_timing Renderer::render_frame(const rt_camera_rig& rig, const RenderProfile& profile,
-    uint32_t sample_stream) {
-    if (!renderer_) throw std::runtime_error("render_prepared_device requires prepare_scene() first");
+rt_frame_timing Renderer::render_rig(const rt_camera_rig& rig, const RenderProfile& profile,
+    const std::uint32_t sample_streams[4], const std::uint32_t* camera_keys) {
+    if (!scene_prepared_) throw std::runtime_error("render_prepared_device requires prepare_scene() first");
     const rt_profile p = to_abi(profile);
     rt_frame_timing t{};
-    throw_on_error(rt_render_frame(renderer_, &rig, &p, sample_stream, &t));
+    throw_on_error(rt_render_frame_streams(renderer_, &rig, &p, sample_streams, camera_keys, &t));
     return t;
 }
 
-RadianceFrame Renderer::download(int camera_index) const {
+// one camera in slot 0 under its own camera key, with the next sample stream
+rt_frame_timing Renderer::launch_camera(const rt_camera_rig& rig, const RenderProfile& profile,
+    int camera_index) {
+    rt_camera_rig one{};
+    one.active_count = 1;
+    one.cameras[0] = rig.cameras[camera_index];
+    const std::uint32_t s = launch_sample_stream_++;
+    const std::uint32_t streams[4] = {s, s, s, s};
+    const std::uint32_t keys[4] = {(std::uint32_t)camera_index, 1u, 2u, 3u};
+    const rt_profile p = to_abi(profile);
+    rt_frame_timing t{};
+    throw_on_error(rt_render_frame_streams(renderer_, &one, &p, streams, keys, &t));
+    last_slot_ = 0;
+    return t;
+}
+
+RadianceFrame Renderer::download(int slot) const { return download_profiled(slot, nullptr); }
+
+RadianceFrame Renderer::download_profiled(int slot, RadianceTiming* timing) const {
+    if (renderer_ == nullptr) throw std::runtime_error("no frame has been rendered");
+    const auto t0 = std::chrono::steady_clock::now();
     rt_device_frame_view v{};
-    throw_on_error(rt_frame_device_view(renderer_, camera_index, &v));
+    throw_on_error(rt_frame_device_view(renderer_, slot, &v));
     RadianceFrame f;
     f.width = v.width;
     f.height = v.height;
@@ -172,47 +314,182 @@ RadianceFrame Renderer::download(int camera_index) const {
     f.normal_rgba.resize(4 * n);
     f.albedo_rgba.resize(4 * n);
     f.depth.resize(n);
-    throw_on_error(rt_frame_download(renderer_, camera_index, f.beauty_rgba.data(),
-        f.normal_rgba.data(), f.albedo_rgba.data(), f.depth.data()));
+    throw_on_error(rt_frame_download(renderer_, slot, f.beauty_rgba.data(), f.normal_rgba.data(),
+        f.albedo_rgba.data(), f.depth.data()));
     f.average_luminance = average_luminance(f.beauty_rgba);
+    if (timing) timing->download_ms = (float)ms_since(t0);
     return f;
 }
 
-RestirDiagnostics Renderer::restir_diagnostics(int camera_index) const {
-    rt_restir_stats s{};
-    throw_on_error(rt_restir_diagnostics(renderer_, camera_index, &s));
-    return RestirDiagnostics{s.valid_reservoirs, s.temporal_candidates, s.spatial_candidates,
-        s.bias_correction_sources};
+DeviceRadianceFrameView Renderer::device_view(int slot) const {
+    if (renderer_ == nullptr) throw std::runtime_error("no frame has been rendered");
+    rt_device_frame_view v{};
+    throw_on_error(rt_frame_device_view(renderer_, slot, &v));
+    DeviceRadianceFrameView out;
+    out.beauty_rgba = v.beauty;
+    out.width = v.width;
+    out.height = v.height;
+    throw_on_error(rt_device_stream(device_, &out.stream));
+    return out;
 }
 
+RestirDiagnostics Renderer::restir_diagnostics(int slot) const {
+    RestirDiagnostics d;
+    if (renderer_ == nullptr) return d;
+    rt_restir_stats s{};
+    throw_on_error(rt_restir_diagnostics(renderer_, slot, &s));
+    d.pixel_count = s.pixel_count;
+    d.active_reservoir_count = s.active_reservoir_count;
+    d.temporal_reuse_count = s.temporal_reuse_count;
+    d.spatial_reuse_count = s.spatial_reuse_count;
+    d.bias_corrected_count = s.bias_corrected_count;
+    d.max_candidate_count = s.max_candidate_count;
+    d.max_age = s.max_age;
+    return d;
+}
+
+RestirDiagnostics Renderer::restir_diagnostics() const { return restir_diagnostics(last_slot_); }
+
+LaunchParameterDiagnostics Renderer::launch_parameter_diagnostics() const {
+    LaunchParameterDiagnostics d;
+    d.allocation_count = 1;  // the frame parameter block lives as long as the renderer
+    if (renderer_) throw_on_error(rt_renderer_launch_diagnostics(renderer_, &d.allocation_count, &d.upload_count));
+    return d;
+}
+
+RadianceFrame Renderer::render_radiance(const rt_scene_desc& scene, const rt_camera_rig& rig,
+    const RenderProfile& profile, int camera_index) {
+    validate_render_camera_request(rig, camera_index, "render_radiance");
+    reset_accumulation();
+    upload_scene(scene);
+    launch_camera(rig, profile, camera_index);
+    return download_profiled(0, nullptr);
+}
+
+ProfiledDeviceRadianceFrame Renderer::render_prepared_device(const rt_camera_rig& rig,
+    const RenderProfile& profile, int camera_index) {
+    validate_render_camera_request(rig, camera_index, "render_radiance");
+    if (!scene_prepared_) throw std::runtime_error("render_prepared_device requires prepare_scene() first");
+    const rt_frame_timing t = launch_camera(rig, profile, camera_index);
+    ProfiledDeviceRadianceFrame out;
+    out.frame = device_view(0);
+    out.timing.render_ms = (float)(t.render_ms - t.denoise_ms);
+    out.timing.denoise_ms = (float)t.denoise_ms;
+    return out;
+}
+
+ProfiledRadianceFrame Renderer::render_prepared_radiance(const rt_camera_rig& rig,
+    const RenderProfile& profile, int camera_index) {
+    const ProfiledDeviceRadianceFrame device = render_prepared_device(rig, profile, camera_index);
+    ProfiledRadianceFrame out;
+    out.timing = device.timing;
+    out.frame = download_profiled(0, &out.timing);
+    return out;
+}
+
+ProfiledRadianceFrame Renderer::render_radiance_profiled(const rt_scene_desc& scene,
+    const rt_camera_rig& rig, const RenderProfile& profile, int camera_index) {
+    validate_render_camera_request(rig, camera_index, "render_radiance");
+    prepare_scene(scene);
+    return render_prepared_radiance(rig, profile, camera_index);
+}
+
+// ------------------------------------------------------------- RendererPool
 namespace {
 int checked_pool_size(int n) {
-    if (n < 1 || n > 4) throw std::invalid_argument("RendererPool requires renderer_count in [1, 4]");
+    if (n < 1 || n > 4) throw std::runtime_error("RendererPool requires renderer_count in [1, 4]");
     return n;
 }
 }  // namespace
 
 // the count is validated before the device is opened
 RendererPool::RendererPool(int renderer_count, int ordinal)
-    : count_(checked_pool_size(renderer_count)), renderer_(ordinal) {}
+    : count_(checked_pool_size(renderer_count)), renderer_(std::make_unique<Renderer>(ordinal)) {}
+
+RendererPool::~RendererPool() = default;
+
+void RendererPool::prepare_scene(const rt_scene_desc& scene) {
+    std::lock_guard<std::mutex> lock(mutex_);
+    renderer_->prepare_scene(scene);
+    for (auto& s : streams_) s = 0u;
+    submissions_ += (std::uint64_t)count_;
+}
+
+void RendererPool::reset_accumulation() {
+    std::lock_guard<std::mutex> lock(mutex_);
+    renderer_->reset_accumulation();
+    submissions_ += (std::uint64_t)count_;
+}
+
+void RendererPool::reset_sequence(std::uint32_t sample_stream) {
+    std::lock_guard<std::mutex> lock(mutex_);
+    renderer_->reset_accumulation();
+    for (auto& s : streams_) s = sample_stream;
+    submissions_ += (std::uint64_t)count_;
+}
+
+// the leading active_cameras cameras in one launch, each from its slot's
+// sample stream; the mutex is held by the caller
+rt_frame_timing RendererPool::launch(const rt_camera_rig& rig, const RenderProfile& profile,
+    int active_cameras) {
+    validate_render_pool_request(rig, active_cameras, count_, "RendererPool");
+    rt_camera_rig lead = rig;
+    lead.active_count = active_cameras;
+    std::uint32_t streams[4];
+    for (int c = 0; c < 4; ++c) streams[c] = streams_[c];
+    const rt_frame_timing t = renderer_->render_rig(lead, profile, streams);
+    for (int c = 0; c < active_cameras; ++c) ++streams_[c];
+    submissions_ += (std::uint64_t)active_cameras;
+    return t;
+}
+
+std::vector<CameraDeviceRenderResult> RendererPool::render_device_frame(const rt_camera_rig& rig,
+    const RenderProfile& profile, int active_cameras) {
+    std::lock_guard<std::mutex> lock(mutex_);
+    const rt_frame_timing t = launch(rig, profile, active_cameras);
+    std::vector<CameraDeviceRenderResult> out;
+    out.reserve((size_t)active_cameras);
+    for (int c = 0; c < active_cameras; ++c) {
+        CameraDeviceRenderResult r;
+        r.camera_index = c;
+        r.profiled.frame = renderer_->device_view(c);
+        r.profiled.timing.render_ms = (float)(t.render_ms - t.denoise_ms);
+        r.profiled.timing.denoise_ms = (float)t.denoise_ms;
+        out.push_back(r);
+    }
+    return out;
+}
 
 std::vector<CameraRenderResult> RendererPool::render_frame(const rt_camera_rig& rig,
-    const RenderProfile& profile, uint32_t sample_stream) {
-    if (rig.active_count > count_)
-        throw std::invalid_argument("camera count exceeds the renderer pool size");
-    const rt_frame_timing t = renderer_.render_frame(rig, profile, sample_stream);
+    const RenderProfile& profile, int active_cameras) {
+    std::lock_guard<std::mutex> lock(mutex_);
+    const rt_frame_timing t = launch(rig, profile, active_cameras);
     std::vector<CameraRenderResult> out;
-    for (int c = 0; c < rig.active_count; ++c) {
+    out.reserve((size_t)active_cameras);
+    for (int c = 0; c < active_cameras; ++c) {
         CameraRenderResult r;
         r.camera_index = c;
+        r.profiled.timing.render_ms = (float)(t.render_ms - t.denoise_ms);
+        r.profiled.timing.denoise_ms = (float)t.denoise_ms;
         const auto d0 = std::chrono::steady_clock::now();
-        r.frame = renderer_.download(c);
-        r.download_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - d0).count();
-        r.render_ms = t.render_ms - t.denoise_ms;
-        r.denoise_ms = t.denoise_ms;
+        r.profiled.frame = renderer_->download(c);
+        r.profiled.timing.download_ms = (float)ms_since(d0);
         out.push_back(std::move(r));
     }
     return out;
+}
+
+RendererPoolDiagnostics RendererPool::diagnostics() const {
+    std::lock_guard<std::mutex> lock(mutex_);
+    RendererPoolDiagnostics d;
+    d.worker_count = count_;
+    d.worker_start_count = (std::uint64_t)count_;
+    d.task_submission_count = submissions_;
+    const LaunchParameterDiagnostics l = renderer_->launch_parameter_diagnostics();
+    d.launch_parameter_allocation_count = l.allocation_count;
+    d.launch_parameter_upload_count = l.upload_count;
+    d.acceleration = renderer_->acceleration_diagnostics();
+    return d;
 }
 
 // ---------------------------------------------------------------- profiling

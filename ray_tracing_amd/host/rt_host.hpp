@@ -4,15 +4,19 @@
 //   Renderer                       (OptixRenderer, src/realtime/gpu/optix_renderer.h:69-128)
 //   RendererPool                   (src/realtime/gpu/renderer_pool.h:30-53)
 //   profiling::RunReport + writers (src/realtime/profiling/benchmark_report.h:10-147)
-// Errors are thrown as std::invalid_argument / std::runtime_error with the
-// library's message, as the reference throws them.
+// Request-validation failures are thrown as std::runtime_error with the
+// reference's messages; library errors as std::invalid_argument (bad
+// arguments) or std::runtime_error with the library's message.
 #pragma once
 
 #include <cstdint>
 #include <filesystem>
+#include <memory>
+#include <mutex>
 #include <optional>
 #include <stdexcept>
 #include <string>
+#include <string_view>
 #include <vector>
 
 #include "../../include/rt/rt_hip.h"
@@ -37,6 +41,10 @@ struct RenderProfile {
     int restir_max_spatial_candidates = 4;
     RestirBiasCorrectionMode restir_bias_correction = RestirBiasCorrectionMode::basic;
     int restir_min_analytic_lights = 16;
+    // extensions of this library (rt_profile): primary-ray jitter inside the
+    // pixel, and the ReSTIR spatial radius in pixels (0: the reference's)
+    int pixel_jitter = 0;
+    int restir_spatial_radius = 0;
 
     static RenderProfile quality();
     static RenderProfile balanced();
@@ -47,8 +55,16 @@ struct RenderProfile {
 std::optional<RenderProfile> render_profile_from_name(const std::string& name);
 std::string render_profile_name(const RenderProfile& profile);
 rt_profile to_abi(const RenderProfile& p);
+RenderProfile from_abi(const rt_profile& p);
 
 void throw_on_error(rt_status s);
+
+// RenderRequestValidation (src/realtime/gpu/render_request_validation.cpp:
+// 15-66): the reference's checks and messages, thrown as std::runtime_error
+void validate_render_camera_request(const rt_camera_rig& rig, int camera_index,
+    std::string_view caller);
+void validate_render_pool_request(const rt_camera_rig& rig, int active_cameras, int renderer_count,
+    std::string_view caller);
 
 // frames of one camera (RadianceFrame, src/realtime/gpu/frame_types.h:14-21)
 struct RadianceFrame {
@@ -61,68 +77,203 @@ struct RadianceFrame {
     double average_luminance = 0.0;
 };
 
-struct CameraRenderResult {
-    int camera_index = 0;
+// RadianceTiming / ProfiledRadianceFrame / ProfiledDeviceRadianceFrame
+// (src/realtime/gpu/optix_renderer.h:26-38)
+struct RadianceTiming {
+    float render_ms = 0.0f;
+    float denoise_ms = 0.0f;
+    float download_ms = 0.0f;
+};
+
+struct ProfiledRadianceFrame {
     RadianceFrame frame;
-    double render_ms = 0.0;
-    double denoise_ms = 0.0;
-    double download_ms = 0.0;
+    RadianceTiming timing;
 };
 
+// DeviceRadianceFrameView (src/realtime/gpu/device_radiance_frame_view.h):
+// beauty_rgba is a device float4 per pixel; stream is the device's
+// hipStream_t, on which the frame was produced
+struct DeviceRadianceFrameView {
+    const float* beauty_rgba = nullptr;
+    int width = 0;
+    int height = 0;
+    void* stream = nullptr;
+};
+
+struct ProfiledDeviceRadianceFrame {
+    DeviceRadianceFrameView frame;
+    RadianceTiming timing;
+};
+
+// RestirDiagnostics (optix_renderer.h:40-48)
 struct RestirDiagnostics {
-    uint64_t valid_reservoirs = 0;
-    uint64_t temporal_candidates = 0;
-    uint64_t spatial_candidates = 0;
-    uint64_t bias_correction_sources = 0;
+    int pixel_count = 0;
+    int active_reservoir_count = 0;
+    int temporal_reuse_count = 0;
+    int spatial_reuse_count = 0;
+    int bias_corrected_count = 0;
+    int max_candidate_count = 0;
+    int max_age = 0;
 };
 
-// One device, one prepared scene, one renderer tracing all cameras of a rig
-// in a single launch sequence.
+// LaunchParameterDiagnostics (optix_renderer.h:50-53)
+struct LaunchParameterDiagnostics {
+    std::uint64_t allocation_count = 0;
+    std::uint64_t upload_count = 0;
+};
+
+// AccelerationUpdateKind / AccelerationUpdateStats
+// (src/realtime/gpu/gpu_scene_acceleration.h:12-30).  Scenes reach the
+// library flattened, so the prototype and instance counts are 0.
+enum class AccelerationUpdateKind { reuse, update, refit, rebuild };
+
+struct AccelerationUpdateStats {
+    AccelerationUpdateKind kind = AccelerationUpdateKind::rebuild;
+    double elapsed_ms = 0.0;
+    int node_count = 0;
+    int primitive_reference_count = 0;
+    int prototype_count = 0;
+    int instance_count = 0;
+    int instanced_primitive_count = 0;
+    std::uint64_t generation = 0;
+};
+
+std::string acceleration_update_kind_name(AccelerationUpdateKind kind);
+
+// OptixRenderer (src/realtime/gpu/optix_renderer.h:69-128) on one device.
+// The reference renders one camera per launch; here a launch traces any
+// number of a rig's cameras, and the single-camera calls render the asked
+// camera in slot 0 under its own camera key, so its samples are those of
+// the same camera in a whole-rig frame (rt_render_frame_streams).  Every
+// launch takes the renderer's sample stream and advances it by one
+// (launch_sample_stream_, optix_renderer.cpp:237); prepare_scene and
+// use_prepared_scene restart it at 0, reset_sequence sets it.
 class Renderer {
 public:
     explicit Renderer(int ordinal = 0);
     ~Renderer();
     Renderer(const Renderer&) = delete;
     Renderer& operator=(const Renderer&) = delete;
+    Renderer(Renderer&&) = delete;
+    Renderer& operator=(Renderer&&) = delete;
 
-    void prepare_scene(const rt_scene_desc& scene);  // upload + BVH build
-    bool has_scene() const { return renderer_ != nullptr; }
+    // upload + acceleration update (reuse / update / refit / rebuild), then
+    // use_prepared_scene
+    void prepare_scene(const rt_scene_desc& scene);
+    // adopts the scene prepared last (optix_renderer.cpp:412-417)
+    void use_prepared_scene(const rt_scene_desc& scene);
     void reset_accumulation();
-    void reset_sequence(uint32_t seed);
-    // renders every active camera of `rig`; returns the device-timed frame
-    rt_frame_timing render_frame(const rt_camera_rig& rig, const RenderProfile& profile,
-        uint32_t sample_stream);
-    RadianceFrame download(int camera_index) const;
-    RestirDiagnostics restir_diagnostics(int camera_index) const;
+    void reset_sequence(std::uint32_t sample_stream);
+    RestirDiagnostics restir_diagnostics() const;
+    LaunchParameterDiagnostics launch_parameter_diagnostics() const;
+    const AccelerationUpdateStats& acceleration_diagnostics() const { return acceleration_; }
+    // uploads `scene` and renders one camera; leaves the renderer unprepared
+    // (optix_renderer.cpp:419-432)
+    RadianceFrame render_radiance(const rt_scene_desc& scene, const rt_camera_rig& rig,
+        const RenderProfile& profile, int camera_index);
+    // the view remains valid until this renderer launches another frame
+    ProfiledDeviceRadianceFrame render_prepared_device(const rt_camera_rig& rig,
+        const RenderProfile& profile, int camera_index);
+    ProfiledRadianceFrame render_prepared_radiance(const rt_camera_rig& rig,
+        const RenderProfile& profile, int camera_index);
+    ProfiledRadianceFrame render_radiance_profiled(const rt_scene_desc& scene,
+        const rt_camera_rig& rig, const RenderProfile& profile, int camera_index);
+
+    // Whole-rig launch (the pool's path): slot c of `rig` draws
+    // sample_streams[c] under camera key camera_keys[c] (nullptr: the slot
+    // index); requires a prepared scene.  Does not touch the renderer's own
+    // sample stream.
+    rt_frame_timing render_rig(const rt_camera_rig& rig, const RenderProfile& profile,
+        const std::uint32_t sample_streams[4], const std::uint32_t* camera_keys = nullptr);
+    RadianceFrame download(int slot) const;
+    DeviceRadianceFrameView device_view(int slot) const;
+    RestirDiagnostics restir_diagnostics(int slot) const;
+    std::uint32_t next_sample_stream() const { return launch_sample_stream_; }
+    bool scene_prepared() const { return scene_prepared_; }
     rt_bvh_stats acceleration_stats() const { return bvh_; }
     rt_renderer* handle() const { return renderer_; }
     rt_device* device() const { return device_; }
 
 private:
+    void upload_scene(const rt_scene_desc& scene);
+    rt_frame_timing launch_camera(const rt_camera_rig& rig, const RenderProfile& profile,
+        int camera_index);
+    RadianceFrame download_profiled(int slot, RadianceTiming* timing) const;
+
     rt_device* device_ = nullptr;
     rt_scene* scene_ = nullptr;
     rt_renderer* renderer_ = nullptr;
     rt_bvh_stats bvh_{};
+    AccelerationUpdateStats acceleration_{};
+    bool scene_prepared_ = false;
+    std::uint32_t launch_sample_stream_ = 0;
+    int last_slot_ = 0;
 };
 
-// RendererPool(int 1..4): the reference keeps one renderer and worker per
-// camera; here one renderer traces the whole rig, and the pool keeps the
-// reference's interface and its camera-count validation.
+// CameraRenderResult / CameraDeviceRenderResult / RendererPoolDiagnostics
+// (src/realtime/gpu/renderer_pool.h:11-28)
+struct CameraRenderResult {
+    int camera_index = 0;
+    ProfiledRadianceFrame profiled;
+};
+
+struct CameraDeviceRenderResult {
+    int camera_index = 0;
+    ProfiledDeviceRadianceFrame profiled;
+};
+
+struct RendererPoolDiagnostics {
+    int worker_count = 0;
+    std::uint64_t worker_start_count = 0;
+    std::uint64_t task_submission_count = 0;
+    std::uint64_t launch_parameter_allocation_count = 0;
+    std::uint64_t launch_parameter_upload_count = 0;
+    AccelerationUpdateStats acceleration;
+};
+
+// RendererPool (src/realtime/gpu/renderer_pool.h:30-53).  The reference
+// keeps one renderer, worker thread and sample-stream counter per camera;
+// here one renderer traces the pool's leading active_cameras cameras in one
+// launch, each camera slot keeping its own sample-stream counter (advanced
+// by every frame that renders it, restarted by prepare_scene, set by
+// reset_sequence), so camera i's frame equals what a lone Renderer makes of
+// camera i.  Frame results carry the shared launch's render and denoise
+// times.  Diagnostics count the reference's units: worker_count and
+// worker_start_count are the pool's camera slots (started once, at
+// construction), task_submission_count counts one task per slot for
+// prepare / reset calls and one per rendered camera, and the launch
+// parameter counts are the one renderer's (one upload per frame).
 class RendererPool {
 public:
     explicit RendererPool(int renderer_count, int ordinal = 0);
-    void prepare_scene(const rt_scene_desc& scene) { renderer_.prepare_scene(scene); }
-    void reset_accumulation() { renderer_.reset_accumulation(); }
-    void reset_sequence(uint32_t seed) { renderer_.reset_sequence(seed); }
-    // results sorted by camera_index; render_ms is the shared frame time
+    ~RendererPool();
+    RendererPool(const RendererPool&) = delete;
+    RendererPool& operator=(const RendererPool&) = delete;
+    RendererPool(RendererPool&&) = delete;
+    RendererPool& operator=(RendererPool&&) = delete;
+
+    void prepare_scene(const rt_scene_desc& scene);
+    void reset_accumulation();
+    void reset_sequence(std::uint32_t sample_stream);
+    // device views are consumed before the next pool operation
+    std::vector<CameraDeviceRenderResult> render_device_frame(const rt_camera_rig& rig,
+        const RenderProfile& profile, int active_cameras);
     std::vector<CameraRenderResult> render_frame(const rt_camera_rig& rig,
-        const RenderProfile& profile, uint32_t sample_stream);
+        const RenderProfile& profile, int active_cameras);
+    RendererPoolDiagnostics diagnostics() const;
+
     int renderer_count() const { return count_; }
-    Renderer& renderer() { return renderer_; }
+    Renderer& renderer() { return *renderer_; }
 
 private:
+    rt_frame_timing launch(const rt_camera_rig& rig, const RenderProfile& profile,
+        int active_cameras);
+
     int count_;
-    Renderer renderer_;
+    std::unique_ptr<Renderer> renderer_;
+    std::uint32_t streams_[4] = {0u, 0u, 0u, 0u};
+    std::uint64_t submissions_ = 0;
+    mutable std::mutex mutex_;
 };
 
 namespace profiling {

@@ -1,0 +1,249 @@
+// test_pool_gpu -- the C++ host API on a GPU, restating the checks of the
+// reference's tests/test_renderer_pool.cpp:80-199 and
+// tests/test_optix_prepared_scene.cpp:72-128 on a like scene (a diffuse
+// sphere under a quad light, 32 x 32 pinhole cameras):
+// - a pool's camera i equals camera i rendered alone by a fresh Renderer
+//   (here bit for bit: one launch for the rig, the same camera key and
+//   sample stream per camera);
+// - device frames stay in device memory and skip the download;
+// - diagnostics: slots started once, one task per rendered camera, the
+//   first preparation rebuilds, an identical re-preparation reuses;
+// - request validation through the pool and the renderer;
+// - prepare_scene restarts the sample stream, so a re-prepared scene and
+//   the profiled wrapper repeat the first frame; the launch parameter
+//   block is allocated once and uploaded once per launch;
+// - the legacy render_radiance leaves the renderer unprepared;
+// - sample streams: after reset_sequence(s) every slot draws stream s, and
+//   each frame advances it.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rt_host.hpp"
+
+using namespace rt_amd;
+
+static int failures = 0;
+static void expect(bool ok, const std::string& what) {
+    if (!ok) {
+        std::printf("FAIL: %s\n", what.c_str());
+        ++failures;
+    }
+}
+
+template <typename Fn>
+static bool throws(Fn&& fn, const std::string& message = "") {
+    try {
+        fn();
+    } catch (const std::exception& e) {
+        return message.empty() || std::string(e.what()).find(message) != std::string::npos;
+    }
+    return false;
+}
+
+static bool same_frame(const RadianceFrame& a, const RadianceFrame& b) {
+    return a.width == b.width && a.height == b.height && a.beauty_rgba == b.beauty_rgba
+           && a.normal_rgba == b.normal_rgba && a.albedo_rgba == b.albedo_rgba && a.depth == b.depth
+           && a.average_luminance == b.average_luminance;
+}
+
+struct SceneData {
+    std::vector<rt_sphere> spheres;
+    std::vector<rt_quad> quads;
+    std::vector<rt_material> materials;
+    rt_scene_desc desc{};
+};
+
+static void make_scene(SceneData& s) {
+    rt_material diffuse{};
+    diffuse.type = RT_MAT_LAMBERTIAN;
+    diffuse.albedo[0] = 0.75f; diffuse.albedo[1] = 0.25f; diffuse.albedo[2] = 0.2f;
+    diffuse.openpbr_index = -1;
+    rt_material light{};
+    light.type = RT_MAT_DIFFUSE_LIGHT;
+    light.emission[0] = light.emission[1] = light.emission[2] = 10.0f;
+    light.openpbr_index = -1;
+    s.materials = {diffuse, light};
+    s.spheres = {rt_sphere{{0.0f, 0.0f, -1.0f}, 0.5f, 0}};
+    s.quads = {rt_quad{{-0.75f, 1.25f, -1.5f}, {1.5f, 0.0f, 0.0f}, {0.0f, 0.0f, 1.5f}, 1}};
+    s.desc = rt_scene_desc{};
+    s.desc.sphere_count = 1;
+    s.desc.quad_count = 1;
+    s.desc.material_count = 2;
+    s.desc.background_mode = RT_BG_CONSTANT;
+    s.desc.spheres = s.spheres.data();
+    s.desc.quads = s.quads.data();
+    s.desc.materials = s.materials.data();
+}
+
+// four pinhole cameras 3 cm apart looking down -z (OpenCV axes: x right, y down)
+static rt_camera_rig make_rig(int count) {
+    rt_camera_rig rig{};
+    rig.active_count = count;
+    for (int i = 0; i < count; ++i) {
+        rt_camera& c = rig.cameras[i];
+        c.enabled = 1;
+        c.width = 32;
+        c.height = 32;
+        c.model = RT_CAM_PINHOLE32;
+        c.origin[0] = 0.03 * i;
+        c.basis_x[0] = 1.0;
+        c.basis_y[1] = -1.0;
+        c.basis_z[2] = -1.0;
+        c.fx = c.fy = 150.0;
+        c.cx = c.cy = 16.0;
+    }
+    return rig;
+}
+
+int main() {
+    SceneData scene;
+    make_scene(scene);
+    const rt_camera_rig rig4 = make_rig(4);
+    const rt_camera_rig rig1 = make_rig(1);
+    const RenderProfile profile = RenderProfile::realtime();
+
+    std::printf("[pool] single-camera pool vs a lone renderer\n");
+    std::fflush(stdout);
+    {
+        RendererPool single(1);
+        single.prepare_scene(scene.desc);
+        const std::vector<CameraRenderResult> r = single.render_frame(rig4, profile, 1);
+        expect(r.size() == 1 && r[0].camera_index == 0, "single-camera pooled count / ordering");
+        Renderer baseline;
+        baseline.prepare_scene(scene.desc);
+        const ProfiledRadianceFrame e = baseline.render_prepared_radiance(rig4, profile, 0);
+        expect(!r.empty() && same_frame(r[0].profiled.frame, e.frame), "single-camera parity");
+        expect(e.frame.average_luminance > 0.0, "lit frame");
+    }
+
+    std::printf("[pool] device frames\n");
+    std::fflush(stdout);
+    {
+        RendererPool device_pool(1);
+        device_pool.prepare_scene(scene.desc);
+        const std::vector<CameraDeviceRenderResult> d = device_pool.render_device_frame(rig4, profile, 1);
+        expect(d.size() == 1 && d[0].camera_index == 0, "device pooled count / ordering");
+        expect(!d.empty() && d[0].profiled.frame.beauty_rgba != nullptr, "device pooled beauty pointer");
+        expect(!d.empty() && d[0].profiled.frame.width == 32 && d[0].profiled.frame.height == 32,
+            "device pooled size");
+        expect(!d.empty() && d[0].profiled.frame.stream != nullptr, "device pooled stream");
+        expect(!d.empty() && d[0].profiled.timing.download_ms == 0.0f, "device path skips the download");
+        hipPointerAttribute_t attr{};
+        const bool got = !d.empty()
+            && hipPointerGetAttributes(&attr, d[0].profiled.frame.beauty_rgba) == hipSuccess;
+        expect(got && attr.type == hipMemoryTypeDevice, "device pooled beauty stays in device memory");
+    }
+
+    std::printf("[pool] four cameras: diagnostics and per-camera parity\n");
+    std::fflush(stdout);
+    {
+        RendererPool four(4);
+        four.prepare_scene(scene.desc);
+        const std::vector<CameraRenderResult> r = four.render_frame(rig4, profile, 4);
+        const RendererPoolDiagnostics d1 = four.diagnostics();
+        expect(d1.worker_count == 4 && d1.worker_start_count == 4, "four slots, each started once");
+        expect(d1.acceleration.kind == AccelerationUpdateKind::rebuild, "first preparation rebuilds");
+        const std::vector<CameraRenderResult> again = four.render_frame(rig4, profile, 4);
+        const RendererPoolDiagnostics d2 = four.diagnostics();
+        expect(d2.worker_start_count == 4, "no new slots on a repeated frame");
+        expect(d2.task_submission_count == d1.task_submission_count + 4, "one task per camera");
+        expect(d2.launch_parameter_upload_count == d1.launch_parameter_upload_count + 1,
+            "one parameter upload per pooled frame");
+        expect(again.size() == 4 && r.size() == 4, "four-camera pooled count");
+        for (int i = 0; i < (int)r.size(); ++i) {
+            expect(r[i].camera_index == i, "four-camera ordering " + std::to_string(i));
+            Renderer baseline;
+            baseline.prepare_scene(scene.desc);
+            const ProfiledRadianceFrame e = baseline.render_prepared_radiance(rig4, profile, i);
+            expect(same_frame(r[i].profiled.frame, e.frame), "four-camera parity " + std::to_string(i));
+        }
+        expect(throws([&] { four.render_frame(rig4, profile, 0); }, "active_cameras out of range"),
+            "reject active_cameras below range");
+        expect(throws([&] { four.render_frame(rig4, profile, 5); }, "active_cameras out of range"),
+            "reject active_cameras above range");
+        expect(throws([&] { four.render_device_frame(rig4, profile, 0); }), "reject device active_cameras 0");
+        expect(throws([&] { four.render_device_frame(rig4, profile, 5); }), "reject device active_cameras 5");
+        expect(throws([&] { four.render_frame(rig1, profile, 2); }, "active_cameras exceeds rig.active_count"),
+            "reject rig active-count mismatch");
+        rt_camera_rig disabled = rig4;
+        disabled.cameras[0].enabled = 0;
+        expect(throws([&] { four.render_frame(disabled, profile, 4); }, "leading camera slot disabled"),
+            "reject disabled leading camera slot");
+
+        // sample streams: after reset_sequence(9) two frames draw streams 9
+        // and 10, the second with the first as history; a lone renderer
+        // running the same sequence for one camera makes the same frames
+        four.reset_sequence(9);
+        const std::vector<CameraRenderResult> s9 = four.render_frame(rig4, profile, 4);
+        const std::vector<CameraRenderResult> s10 = four.render_frame(rig4, profile, 2);
+        Renderer lone3;
+        lone3.prepare_scene(scene.desc);
+        lone3.reset_sequence(9);
+        const ProfiledRadianceFrame l9 = lone3.render_prepared_radiance(rig4, profile, 3);
+        expect(s9.size() == 4 && same_frame(s9[3].profiled.frame, l9.frame), "stream 9 of camera 3");
+        Renderer lone1;
+        lone1.prepare_scene(scene.desc);
+        lone1.reset_sequence(9);
+        (void)lone1.render_prepared_radiance(rig4, profile, 1);
+        const ProfiledRadianceFrame l10 = lone1.render_prepared_radiance(rig4, profile, 1);
+        expect(s10.size() == 2 && same_frame(s10[1].profiled.frame, l10.frame),
+            "stream 10 of camera 1, after stream 9");
+        expect(!same_frame(l9.frame, r[3].profiled.frame), "another stream draws other samples");
+    }
+
+    std::printf("[renderer] prepared scene lifecycle\n");
+    std::fflush(stdout);
+    {
+        Renderer renderer;
+        renderer.prepare_scene(scene.desc);
+        const ProfiledRadianceFrame first = renderer.render_prepared_radiance(rig1, profile, 0);
+        const ProfiledRadianceFrame second = renderer.render_prepared_radiance(rig1, profile, 0);
+        renderer.prepare_scene(scene.desc);
+        const ProfiledRadianceFrame after_reset = renderer.render_prepared_radiance(rig1, profile, 0);
+        const ProfiledRadianceFrame fallback = renderer.render_radiance_profiled(scene.desc, rig1, profile, 0);
+        expect(second.frame.width == first.frame.width && second.frame.beauty_rgba.size() == first.frame.beauty_rgba.size(),
+            "prepared repeat shape");
+        expect(second.frame.average_luminance > 0.0, "prepared repeat remains lit");
+        expect(same_frame(after_reset.frame, first.frame), "prepared reset parity");
+        expect(same_frame(fallback.frame, first.frame), "fallback parity");
+        expect(first.timing.render_ms >= 0.0f && first.timing.download_ms >= 0.0f, "first timings");
+        expect(second.timing.render_ms >= 0.0f && second.timing.download_ms >= 0.0f, "repeat timings");
+        expect(fallback.timing.render_ms >= 0.0f && fallback.timing.download_ms >= 0.0f, "fallback timings");
+        const LaunchParameterDiagnostics ld = renderer.launch_parameter_diagnostics();
+        expect(ld.allocation_count == 1, "launch parameter block allocated once");
+        expect(ld.upload_count == 4, "one parameter upload per launch");
+        expect(renderer.acceleration_diagnostics().kind == AccelerationUpdateKind::reuse,
+            "identical prepared scene reuses the acceleration structure");
+
+        Renderer no_prepare;
+        expect(throws([&] { no_prepare.render_prepared_radiance(rig1, profile, 0); },
+                   "requires prepare_scene() first"),
+            "prepared render requires prepare_scene");
+        Renderer invalid_wrapper;
+        expect(throws([&] { invalid_wrapper.render_radiance_profiled(scene.desc, rig1, profile, 1); },
+                   "camera_index out of range"),
+            "profiled wrapper rejects an invalid camera index");
+        expect(throws([&] { invalid_wrapper.render_prepared_radiance(rig1, profile, 0); }),
+            "a failed profiled wrapper leaves no prepared state");
+        Renderer invalidation;
+        invalidation.prepare_scene(scene.desc);
+        const RadianceFrame legacy = invalidation.render_radiance(scene.desc, rig1, profile, 0);
+        expect(legacy.average_luminance > 0.0, "legacy render is lit");
+        expect(throws([&] { invalidation.render_prepared_radiance(rig1, profile, 0); }),
+            "legacy render invalidates prepared state");
+        SceneData empty;
+        make_scene(empty);
+        empty.desc.sphere_count = 0;
+        empty.desc.quad_count = 0;
+        expect(throws([&] { Renderer x; x.prepare_scene(empty.desc); },
+                   "render_radiance requires at least one surface primitive"),
+            "a scene without surfaces is refused");
+    }
+    std::printf("%s (%d failures)\n", failures ? "FAILED" : "OK", failures);
+    return failures ? 1 : 0;
+}

@@ -108,7 +108,7 @@ int main(int argc, char** argv) {
     threw = false;
     try {
         RendererPool pool(5);
-    } catch (const std::invalid_argument& e) {
+    } catch (const std::runtime_error& e) {
         threw = std::string(e.what()) == "RendererPool requires renderer_count in [1, 4]";
     } catch (...) {}
     expect(threw, "RendererPool(5) throws the reference's message");

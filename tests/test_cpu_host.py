@@ -26,3 +26,19 @@ def test_report_known_answers(report_binary, tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "OK (0 failures)" in r.stdout
     assert (tmp_path / "out" / "benchmark_summary.json").exists()
+
+
+API_BIN = os.path.join(ROOT, "tests", "host", "test_host_api")
+
+
+def test_host_api_validation_and_call_shapes(rt):
+    """Request validation with the reference's messages as std::runtime_error
+    (render_request_validation.cpp:15-66), RendererPool's size check, the
+    reference's RendererPool / OptixRenderer signatures (static_asserts) and
+    the profile's C ABI round trip; no GPU call."""
+    if not os.path.exists(API_BIN):
+        from ray_tracing_amd import build as rt_build
+        rt_build.build_host(verbose=False)
+    r = subprocess.run([API_BIN], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "OK (0 failures)" in r.stdout

@@ -130,9 +130,18 @@ def test_restir_sequence_matches_oracle(rt, oracle, device, instanced, bias):
             compare_frame(ren.download(ci), refs[ci], f"frame{frame} cam{ci}", tol_frac=0.99)
             gr = ren.reservoirs(ci)
             compare_reservoirs(gr, refs[ci]["reservoirs"], f"frame{frame} cam{ci} reservoirs")
+            # RestirDiagnostics counts (optix_renderer.cpp:183-215) over the same reservoirs
             st = ren.restir_stats(ci)
-            assert st["valid_reservoirs"] == int(gr["valid"].sum())
+            valid = gr["valid"] != 0
+            assert st["pixel_count"] == len(gr)
+            assert st["active_reservoir_count"] == int(valid.sum())
+            assert st["temporal_reuse_count"] == int((valid & (gr["temporal_candidate_count"] > 0)).sum())
+            assert st["spatial_reuse_count"] == int((valid & (gr["spatial_candidate_count"] > 0)).sum())
+            assert st["bias_corrected_count"] == int(
+                (valid & (gr["bias_correction_source_count"] > 0)).sum())
+            assert st["max_candidate_count"] == int(gr["candidate_count"][valid].max(initial=0))
+            assert st["max_age"] == int(gr["age"][valid].max(initial=0))
             if frame > 0:
-                assert st["temporal_candidates"] > 0 and st["spatial_candidates"] > 0
+                assert st["temporal_reuse_count"] > 0 and st["spatial_reuse_count"] > 0
             if bias and frame > 0:
-                assert st["bias_correction_sources"] > 0
+                assert st["bias_corrected_count"] > 0
