@@ -62,6 +62,21 @@ class Transform:
     def vector(self, v) -> np.ndarray:
         return self.rotation @ np.asarray(v, np.float64)
 
+    def normal(self, n) -> np.ndarray:
+        """transform_normal (src/scene/realtime_scene_adapter.cpp:112-120): the
+        inverse transpose of the linear part applied to `n`, normalized; a
+        singular transform or a zero-length or non-finite normal is refused."""
+        with np.errstate(all="ignore"):
+            try:
+                m = np.linalg.inv(self.rotation).T
+            except np.linalg.LinAlgError:
+                m = np.full((3, 3), np.nan)
+            t = m @ np.asarray(n, np.float64)
+            length = float(np.sqrt(t @ t))
+        if not np.isfinite(length) or length <= 1e-12:
+            raise ValueError("SceneIR v2 contains a singular or zero-length mesh normal")
+        return t / length
+
 
 @dataclass
 class CameraSpec:
@@ -382,7 +397,12 @@ def _realtime(node, out: SceneDefinition):
 
 
 def _parse_mtl(path):
+    """newmtl / Kd / map_Kd of one MTL file.  A material without Kd is black
+    (tinyobj's InitMaterial zeroes the diffuse colour), and map_Kd is
+    resolved against the directory of the MTL file that declares it
+    (obj_mtl_importer.cpp:94-99)."""
     mats, cur = {}, None
+    mtl_dir = os.path.dirname(path)
     with open(path) as f:
         for line in f:
             tok = line.split()
@@ -390,12 +410,24 @@ def _parse_mtl(path):
                 continue
             if tok[0] == "newmtl":
                 cur = " ".join(tok[1:])
-                mats[cur] = {"Kd": np.ones(3), "map_Kd": None}
+                mats[cur] = {"Kd": np.zeros(3), "map_Kd": None}
             elif cur is not None and tok[0] == "Kd":
                 mats[cur]["Kd"] = np.array([float(v) for v in tok[1:4]])
             elif cur is not None and tok[0] == "map_Kd":
-                mats[cur]["map_Kd"] = tok[-1]
+                mats[cur]["map_Kd"] = os.path.normpath(os.path.join(mtl_dir, tok[-1]))
     return mats
+
+
+def _obj_index(text, count, what):
+    """An OBJ index: 1-based from the start, or negative from the end of the
+    elements read so far (-1 is the last), as tinyobj resolves them; 0 or
+    beyond the elements read is an error."""
+    i = int(text)
+    if i > 0:
+        return i - 1
+    if i < 0 and count + i >= 0:
+        return count + i
+    raise ValueError(f"obj {what} index {i} out of range")
 
 
 def import_obj_mtl(path, out: SceneDefinition):
@@ -403,8 +435,9 @@ def import_obj_mtl(path, out: SceneDefinition):
     polygon faces (fan-triangulated) grouped by material; one diffuse
     material per group (Kd, or an image texture for map_Kd, or white
     without a material); appended with an identity transform.  A group keeps
-    its `vt` texture coordinates only when every corner of every triangle in
-    it has one (obj_mtl_importer.cpp:202-224)."""
+    its `vt` texture coordinates (and `vn` normals) only when every corner
+    of every triangle in it has one (obj_mtl_importer.cpp:202-224).  Normals
+    are normalized when flattened (Transform.normal)."""
     path = os.path.normpath(path)
     root = os.path.dirname(path)
     verts, uvs, norms, groups, order, mats, cur = [], [], [], {}, [], {}, -1
@@ -431,12 +464,11 @@ def import_obj_mtl(path, out: SceneDefinition):
                 idx, tidx, nidx = [], [], []
                 for c in tok[1:]:
                     parts = c.split("/")
-                    i = int(parts[0])
-                    if i < 0:
-                        raise ValueError("negative obj vertex indices are not supported")
-                    idx.append(i - 1)
-                    tidx.append(int(parts[1]) - 1 if len(parts) > 1 and parts[1] else -1)
-                    nidx.append(int(parts[2]) - 1 if len(parts) > 2 and parts[2] else -1)
+                    idx.append(_obj_index(parts[0], len(verts), "vertex"))
+                    tidx.append(_obj_index(parts[1], len(uvs), "texture coordinate")
+                                if len(parts) > 1 and parts[1] else -1)
+                    nidx.append(_obj_index(parts[2], len(norms), "normal")
+                                if len(parts) > 2 and parts[2] else -1)
                 if len(idx) < 3:
                     raise ValueError("only triangulated faces are supported")
                 if cur not in groups:
@@ -461,7 +493,7 @@ def import_obj_mtl(path, out: SceneDefinition):
         normals = N[tn] if len(N) and tn.size and (tn >= 0).all() else None
         m = mats[list(mats)[gi]] if gi >= 0 else None
         if m is not None and m["map_Kd"] is not None:
-            out.textures.append({"type": "image", "path": os.path.normpath(os.path.join(root, m["map_Kd"]))})
+            out.textures.append({"type": "image", "path": m["map_Kd"]})
         else:
             out.textures.append({"type": "constant", "color": m["Kd"] if m is not None else np.ones(3)})
         out.materials.append({"type": "diffuse", "albedo": len(out.textures) - 1})
@@ -704,7 +736,7 @@ def flatten(defn: SceneDefinition) -> FlatScene:
                                          abi.f3(*tf.point(P[c])), mat_i))
                 tri_uv.append(tc[k].reshape(6) if tc is not None else NO_UV)
                 vn = s.get("normals")
-                tri_n.append(np.concatenate([tf.vector(x) for x in vn[k]]) if vn is not None else NO_N)
+                tri_n.append(np.concatenate([tf.normal(x) for x in vn[k]]) if vn is not None else NO_N)
 
     def arr(T, items):
         return (T * len(items))(*items) if items else None

@@ -271,3 +271,70 @@ def test_camera_lut_helper_matches_the_generators_fisheye():
     c.model = S.CAM_PINHOLE32
     with pytest.raises(rt.RtError, match="equi62_lut1d"):
         rt.check(rt.lib().rt_camera_build_lut(C.byref(c)))
+
+
+def test_normals_take_the_inverse_transpose_and_unit_length():
+    """transform_normal (realtime_scene_adapter.cpp:112-120): under a
+    non-uniform scale a normal follows the inverse transpose, then unit
+    length; it stays perpendicular to a transformed tangent."""
+    tf = S.Transform(rotation=np.diag([2.0, 1.0, 1.0]))
+    n = tf.normal([1.0, 1.0, 0.0])
+    assert n.tolist() == pytest.approx((np.array([0.5, 1.0, 0.0]) / math.sqrt(1.25)).tolist())
+    tangent = tf.vector([1.0, -1.0, 0.0])      # perpendicular to (1, 1, 0) before the scale
+    assert abs(float(n @ tangent)) < 1e-12
+    assert S.Transform().normal([0.0, 3.0, 4.0]).tolist() == pytest.approx([0.0, 0.6, 0.8])
+    for bad, tfm in (([0.0, 0.0, 0.0], S.Transform()), ([float("nan"), 0.0, 1.0], S.Transform()),
+                     ([0.0, 0.0, 1.0], S.Transform(rotation=np.diag([1.0, 1.0, 0.0])))):
+        with pytest.raises(ValueError, match="singular or zero-length mesh normal"):
+            tfm.normal(bad)
+
+
+def test_obj_normals_are_normalized_at_ingest(tmp_path):
+    write(str(tmp_path / "m.obj"), "v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 5\nvn 0 3 4\nf 1//1 2//2 3//1\n")
+    p = str(tmp_path / "scene.yaml")
+    write(p, BASIC + "imports:\n  mesh:\n    type: obj_mtl\n    obj: m.obj\n")
+    n = S.flatten(S.load_scene_definition(p)).triangle_normals
+    assert n[0].tolist() == pytest.approx([0, 0, 1, 0, 0.6, 0.8, 0, 0, 1], abs=1e-7)
+    write(str(tmp_path / "z.obj"), "v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 0\nf 1//1 2//1 3//1\n")
+    write(p, BASIC + "imports:\n  mesh:\n    type: obj_mtl\n    obj: z.obj\n")
+    with pytest.raises(ValueError, match="zero-length mesh normal"):
+        S.flatten(S.load_scene_definition(p))
+
+
+def test_mtl_defaults_and_texture_paths(tmp_path):
+    """A material without Kd is black (tinyobj InitMaterial); map_Kd resolves
+    against the MTL file's own directory (obj_mtl_importer.cpp:94-99)."""
+    from PIL import Image
+    os.makedirs(tmp_path / "mat")
+    Image.fromarray(np.full((2, 2, 3), 200, np.uint8)).save(str(tmp_path / "mat" / "kd.png"))
+    write(str(tmp_path / "mat" / "m.mtl"), "newmtl dark\nnewmtl pic\nmap_Kd kd.png\n")
+    write(str(tmp_path / "m.obj"), "mtllib mat/m.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\n"
+          "usemtl dark\nf 1 2 3\nusemtl pic\nf 1 2 3\n")
+    p = str(tmp_path / "scene.yaml")
+    write(p, BASIC + "imports:\n  mesh:\n    type: obj_mtl\n    obj: m.obj\n")
+    d = S.load_scene_definition(p)
+    tex = d.textures[-2:]
+    assert tex[0]["type"] == "constant" and np.asarray(tex[0]["color"]).tolist() == [0.0, 0.0, 0.0]
+    assert tex[1]["type"] == "image" and tex[1]["path"] == os.path.normpath(str(tmp_path / "mat" / "kd.png"))
+    S.flatten(d)   # the image loads from the MTL's directory
+
+
+def test_obj_negative_indices_count_back_from_the_last_element(tmp_path):
+    """Relative OBJ indices: -1 is the last element read so far, as tinyobj
+    resolves them; 0 and indices before the first element are errors."""
+    write(str(tmp_path / "a.obj"), "v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 0 1\n"
+          "vn 0 0 1\nf -3/-3/-1 -2/-2/-1 -1/-1/-1\n")
+    write(str(tmp_path / "b.obj"), "v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 0 1\n"
+          "vn 0 0 1\nf 1/1/1 2/2/1 3/3/1\n")
+    p = str(tmp_path / "scene.yaml")
+    flat = []
+    for name in ("a.obj", "b.obj"):
+        write(p, BASIC + f"imports:\n  mesh:\n    type: obj_mtl\n    obj: {name}\n")
+        f = S.flatten(S.load_scene_definition(p))
+        flat.append((bytes(f.triangles), f.triangle_uvs.tobytes(), f.triangle_normals.tobytes()))
+    assert flat[0] == flat[1]
+    for face in ("f 0 1 2\n", "f -4 -3 -2\n"):
+        write(str(tmp_path / "c.obj"), "v 0 0 0\nv 1 0 0\nv 0 1 0\n" + face)
+        write(p, BASIC + "imports:\n  mesh:\n    type: obj_mtl\n    obj: c.obj\n")
+        with pytest.raises((ValueError, S.SceneFileError)):
+            S.flatten(S.load_scene_definition(p))
