@@ -90,13 +90,38 @@ typedef struct rt_openpbr_material {
     float geometry_opacity; int32_t geometry_thin_walled;
 } rt_openpbr_material;
 
-/* PackedAnalyticLight subset (src/common/light_sampling.h:44-62): sphere lights */
-enum { RT_ALIGHT_SPHERE = 0 };
+/* PackedAnalyticLight (src/common/light_sampling.h:30-62), same field order.
+ * basis_x / basis_y / basis_z are the columns of the light's local-to-world
+ * linear map.  Sphere: radius scaled by |basis_x|; disk: radius in the
+ * basis_x / basis_y plane; rect: width x height along basis_x / basis_y;
+ * cylinder: radius across basis_x / basis_y, length along basis_z; all emit
+ * on the side facing -(basis_x x basis_y) (disk, rect) or outward (sphere,
+ * cylinder).  Distant: a cone of half-angle acos(cos_theta_max) around
+ * basis_z, delta != 0 for a single direction; dome: every direction.
+ * world_area, selection_pdf and cdf are the compiled values the kernels
+ * read as given (rt_analytic_lights_compile computes them); treat_as_point
+ * makes a sphere a point light; treat_as_line drops a finite light from
+ * emission MIS. */
+enum { RT_ALIGHT_SPHERE = 0, RT_ALIGHT_DISK = 1, RT_ALIGHT_RECT = 2, RT_ALIGHT_CYLINDER = 3,
+       RT_ALIGHT_DISTANT = 4, RT_ALIGHT_DOME = 5 };
 typedef struct rt_analytic_light {
     int32_t type;
     float position[3];
-    float radius;
+    float basis_x[3];
+    float basis_y[3];
+    float basis_z[3];
     float radiance[3];
+    float radius;
+    float width;
+    float height;
+    float length;
+    float world_area;
+    float cos_theta_max;
+    float selection_pdf;
+    float cdf;
+    int32_t delta;
+    int32_t treat_as_point;
+    int32_t treat_as_line;
 } rt_analytic_light;
 
 enum { RT_BG_CONSTANT = 0, RT_BG_SKY_GRADIENT = 1 };
@@ -152,6 +177,18 @@ typedef struct rt_camera {
  * src/realtime/camera_models.cpp:228-248): theta for 1024 evenly spaced
  * distorted radii up to the image corner plus 10 pixels, by Newton inversion */
 rt_status rt_camera_build_lut(rt_camera* camera);
+
+/* The analytic light compiler's derived fields, for lights built by hand
+ * (src/scene/analytic_light_compiler.cpp:178-240 and its finalize step,
+ * :15-39), in FP64: world_area (sphere 4 pi (r s)^2 and cylinder
+ * 2 pi (r s)(l s) with s the basis' similarity scale, disk pi r^2 and rect
+ * w h times |basis_x x basis_y|), delta for a distant light whose
+ * cos_theta_max is 1, and selection weights (world area, cone solid angle
+ * (1 for a delta light) or 4 pi, times Rec.709 luminance) normalized into
+ * selection_pdf and a cdf closed at 1 from the last light with positive
+ * probability.  A sphere or cylinder whose basis is not a similarity
+ * transform is refused, as the reference refuses it. */
+rt_status rt_analytic_lights_compile(rt_analytic_light* lights, int32_t count);
 
 /* PackedCameraRig, src/realtime/camera_rig.h:31-34 */
 typedef struct rt_camera_rig {

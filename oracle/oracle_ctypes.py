@@ -62,7 +62,11 @@ class OpenPbr(C.Structure):
 
 
 class AnalyticLight(C.Structure):
-    _fields_ = [("type", C.c_int32), ("position", f3), ("radius", C.c_float), ("radiance", f3)]
+    _fields_ = [("type", C.c_int32), ("position", f3), ("basis_x", f3), ("basis_y", f3),
+                ("basis_z", f3), ("radiance", f3), ("radius", C.c_float), ("width", C.c_float),
+                ("height", C.c_float), ("length", C.c_float), ("world_area", C.c_float),
+                ("cos_theta_max", C.c_float), ("selection_pdf", C.c_float), ("cdf", C.c_float),
+                ("delta", C.c_int32), ("treat_as_point", C.c_int32), ("treat_as_line", C.c_int32)]
 
 
 class Texture(C.Structure):
@@ -322,7 +326,8 @@ def lib():
         L.rto_restir_finalize_basic.argtypes = [P(Reservoir), C.c_float, C.c_float, C.c_int]
         L.rto_restir_merge.argtypes = [P(Reservoir), P(Reservoir), C.c_float, C.c_int, C.c_float,
                                        C.c_int, P(C.c_int)]
-        L.rto_build_analytic_cdf.argtypes = [P(Scene), C.c_void_p, C.c_void_p]
+        L.rto_analytic_lights_compile.argtypes = [C.c_void_p, C.c_int]
+        L.rto_analytic_lights_compile.restype = C.c_int
         L.rto_cache_mlp_param_count.argtypes = []
         L.rto_cache_mlp_param_count.restype = C.c_int
         L.rto_cache_corners.argtypes = [P(CacheGrid), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]

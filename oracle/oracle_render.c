@@ -307,37 +307,78 @@ typedef struct {
     int a_count;
 } ctx_t;
 
-/* analytic light CDF (src/scene/analytic_light_compiler.cpp:15-39,230-237):
- * weight = world area x luminance, accumulated in FP64, closed at the last
- * light with positive probability */
-int rto_build_analytic_cdf(const rto_scene* s, float* sel, float* cdf) {
-    const int n = s->analytic_light_count;
+/* rto_analytic_lights_compile: analytic_light_compiler.cpp:178-240 (world
+ * area, delta, selection weight per type) and :15-39 (normalize, close the
+ * cdf at the last light with positive probability), in FP64 */
+int rto_analytic_lights_compile(rto_analytic_light* lights, int count) {
+    const double pi = 3.14159265358979323846;
+    double* w = (double*)malloc(sizeof(double) * (size_t)(count > 0 ? count : 1));
     double total = 0.0;
-    double* w = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
-    for (int i = 0; i < n; ++i) {
-        const rto_analytic_light* L = &s->analytic_lights[i];
-        const double r = (double)L->radius;
-        const double area = 4.0 * 3.14159265358979323846 * r * r;
-        const double lum = 0.2126 * (double)L->radiance[0] + 0.7152 * (double)L->radiance[1]
-                           + 0.0722 * (double)L->radiance[2];
-        double x = area * lum;
+    for (int i = 0; i < count; ++i) {
+        rto_analytic_light* L = &lights[i];
+        if (L->type < RTO_ALIGHT_SPHERE || L->type > RTO_ALIGHT_DOME) { free(w); return -1; }
+        /* columns of the local-to-world map */
+        const double bx[3] = {L->basis_x[0], L->basis_x[1], L->basis_x[2]};
+        const double by[3] = {L->basis_y[0], L->basis_y[1], L->basis_y[2]};
+        const double bz[3] = {L->basis_z[0], L->basis_z[1], L->basis_z[2]};
+        const double* col[3] = {bx, by, bz};
+        const double cx = bx[1] * by[2] - bx[2] * by[1];
+        const double cy = bx[2] * by[0] - bx[0] * by[2];
+        const double cz = bx[0] * by[1] - bx[1] * by[0];
+        const double cross_norm = sqrt(cx * cx + cy * cy + cz * cz);
+        double scale = 1.0, area = 0.0;
+        if (L->type == RTO_ALIGHT_SPHERE || L->type == RTO_ALIGHT_CYLINDER) {
+            /* require_similarity_scale (:114-128): the Gram matrix is s^2 I */
+            double g[3][3], s2 = 0.0, dev = 0.0;
+            for (int r = 0; r < 3; ++r)
+                for (int q = 0; q < 3; ++q)
+                    g[r][q] = col[r][0] * col[q][0] + col[r][1] * col[q][1] + col[r][2] * col[q][2];
+            s2 = (g[0][0] + g[1][1] + g[2][2]) / 3.0;
+            if (!isfinite(s2) || s2 <= 0.0) { free(w); return -1; }
+            for (int r = 0; r < 3; ++r)
+                for (int q = 0; q < 3; ++q) {
+                    const double e = fabs(g[r][q] - (r == q ? s2 : 0.0));
+                    if (e > dev) dev = e;
+                }
+            if (dev > 1e-6 * (s2 > 1.0 ? s2 : 1.0)) { free(w); return -1; }
+            scale = sqrt(s2);
+        }
+        switch (L->type) {
+            case RTO_ALIGHT_SPHERE: area = 4.0 * pi * pow((double)L->radius * scale, 2.0); break;
+            case RTO_ALIGHT_DISK: area = pi * (double)L->radius * (double)L->radius * cross_norm; break;
+            case RTO_ALIGHT_RECT: area = (double)L->width * (double)L->height * cross_norm; break;
+            case RTO_ALIGHT_CYLINDER:
+                area = 2.0 * pi * ((double)L->radius * scale) * ((double)L->length * scale);
+                break;
+            default: break;
+        }
+        if (L->type <= RTO_ALIGHT_CYLINDER) L->world_area = (float)area;
+        if (L->type == RTO_ALIGHT_DISTANT) L->delta = L->cos_theta_max >= 1.0f ? 1 : 0;
+        double lum = 0.2126 * (double)L->radiance[0] + 0.7152 * (double)L->radiance[1]
+                     + 0.0722 * (double)L->radiance[2];
+        if (lum < 0.0) lum = 0.0;
+        double x;
+        if (L->type == RTO_ALIGHT_DOME) x = 4.0 * pi * lum;
+        else if (L->type == RTO_ALIGHT_DISTANT)
+            x = (L->delta ? 1.0 : 2.0 * pi * (1.0 - (double)L->cos_theta_max)) * lum;
+        else x = area * lum;
         x = isfinite(x) ? (x > 0.0 ? x : 0.0) : 0.0;
         w[i] = x;
         total += x;
     }
     double acc = 0.0;
     int last = -1;
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < count; ++i) {
         const double p = total > 0.0 ? w[i] / total : 0.0;
         acc += p;
-        sel[i] = (float)p;
-        cdf[i] = (float)acc;
+        lights[i].selection_pdf = (float)p;
+        lights[i].cdf = (float)acc;
         if (p > 0.0) last = i;
     }
     if (total > 0.0)
-        for (int i = last; i < n; ++i) cdf[i] = 1.0f;
+        for (int i = last; i < count; ++i) lights[i].cdf = 1.0f;
     free(w);
-    return n;
+    return 0;
 }
 
 static int prim_global(const rto_scene* s, int type, int idx) {
@@ -359,10 +400,14 @@ static int ctx_init(ctx_t* c, const rto_scene* s, const rto_bvh* bvh) {
         if (c->lights[i].type == 3) c->env_light = i;
         else c->light_of_prim[prim_global(s, c->lights[i].type, c->lights[i].prim)] = i;
     }
+    /* the lights' compiled selection pdf / cdf, read as given */
     c->a_count = s->analytic_light_count;
     c->a_sel = (float*)malloc(sizeof(float) * (size_t)(c->a_count > 0 ? c->a_count : 1));
     c->a_cdf = (float*)malloc(sizeof(float) * (size_t)(c->a_count > 0 ? c->a_count : 1));
-    rto_build_analytic_cdf(s, c->a_sel, c->a_cdf);
+    for (int i = 0; i < c->a_count; ++i) {
+        c->a_sel[i] = s->analytic_lights[i].selection_pdf;
+        c->a_cdf[i] = s->analytic_lights[i].cdf;
+    }
     return 0;
 }
 static void ctx_free(ctx_t* c) {
@@ -556,7 +601,7 @@ static float emission_mis(const ctx_t* c, const path_t* st, const hit_t* h, int 
     return power_heuristic(st->prev_pdf, lpdf);
 }
 
-typedef struct { v3 dir, emission; float dist, pdf; int infinite, valid; } light_sample_t;
+typedef struct { v3 dir, emission; float dist, pdf; int infinite, valid, delta; } light_sample_t;
 
 /* programs.cu:539-550 with the own sin/cos */
 static v3 cone_dir(v3 axis, float cos_max, float u0, float u1) {
@@ -737,39 +782,117 @@ static void direct_light(const ctx_t* c, const hit_t* h, const float r[4], int b
 }
 
 /* ------------------------------------------------------ analytic lights */
-/* try_intersect_analytic_light, sphere branch (programs.cu:679-709): the
- * nearer root in (tmin, tmax), else the farther; front faces only */
+static int alight_finite(int type) {
+    return type == RTO_ALIGHT_SPHERE || type == RTO_ALIGHT_DISK || type == RTO_ALIGHT_RECT
+           || type == RTO_ALIGHT_CYLINDER;
+}
+
+/* analytic_light_normal (programs.cu:655-659) */
+static v3 alight_normal(const rto_analytic_light* L) {
+    return norm3(scl3(cross3(ld3(L->basis_x), ld3(L->basis_y)), -1.0f));
+}
+
+/* analytic_plane_coordinates (programs.cu:661-677) */
+static int alight_plane_xy(const rto_analytic_light* L, v3 p, float* lx, float* ly) {
+    const v3 bx = ld3(L->basis_x), by = ld3(L->basis_y);
+    const v3 local = sub3(p, ld3(L->position));
+    const float xx = dot3(bx, bx), xy = dot3(bx, by), yy = dot3(by, by);
+    const float det = xx * yy - xy * xy;
+    if (det <= 1e-16f) return 0;
+    const float px = dot3(local, bx), py = dot3(local, by);
+    *lx = (px * yy - py * xy) / det;
+    *ly = (py * xx - px * xy) / det;
+    return 1;
+}
+
+/* try_intersect_analytic_light (programs.cu:679-793): sphere (radius scaled
+ * by |basis_x|), disk / rect on the plane through the position with normal
+ * -(basis_x x basis_y), and the cylinder's side; front faces only, the
+ * nearer root in (tmin, tmax), else the farther */
 static int alight_hit(const rto_analytic_light* L, v3 o, v3 d, float tmin, float tmax, float* t,
     v3* pos, v3* n) {
     const v3 center = ld3(L->position);
-    const float radius = L->radius;
-    const v3 oc = sub3(o, center);
-    const float a = lensq3(d);
-    const float hb = dot3(oc, d);
-    const float c = lensq3(oc) - radius * radius;
+    if (L->type == RTO_ALIGHT_SPHERE) {
+        const float radius = L->radius * len3(ld3(L->basis_x));
+        const v3 oc = sub3(o, center);
+        const float a = lensq3(d);
+        const float hb = dot3(oc, d);
+        const float c = lensq3(oc) - radius * radius;
+        const float disc = hb * hb - a * c;
+        if (a <= 1e-16f || disc < 0.0f) return 0;
+        const float rt = sqrtf(disc);
+        float root = (-hb - rt) / a;
+        if (root <= tmin || root >= tmax) root = (-hb + rt) / a;
+        if (root <= tmin || root >= tmax) return 0;
+        const v3 p = add3(o, scl3(d, root));
+        const v3 nn = norm3(sub3(p, center));
+        if (dot3(d, nn) >= 0.0f) return 0;
+        *t = root;
+        *pos = p;
+        *n = nn;
+        return 1;
+    }
+    if (L->type == RTO_ALIGHT_DISK || L->type == RTO_ALIGHT_RECT) {
+        const v3 nn = alight_normal(L);
+        const float den = dot3(d, nn);
+        if (den >= -1e-8f) return 0;
+        const float tt = dot3(sub3(center, o), nn) / den;
+        if (tt <= tmin || tt >= tmax) return 0;
+        const v3 p = add3(o, scl3(d, tt));
+        float lx = 0.0f, ly = 0.0f;
+        if (!alight_plane_xy(L, p, &lx, &ly)) return 0;
+        const int inside = L->type == RTO_ALIGHT_DISK
+                               ? lx * lx + ly * ly <= L->radius * L->radius
+                               : fabsf(lx) <= 0.5f * L->width && fabsf(ly) <= 0.5f * L->height;
+        if (!inside) return 0;
+        *t = tt;
+        *pos = p;
+        *n = nn;
+        return 1;
+    }
+    if (L->type != RTO_ALIGHT_CYLINDER) return 0;
+    const v3 bx = ld3(L->basis_x), by = ld3(L->basis_y), bz = ld3(L->basis_z);
+    const float scale = len3(bx);
+    if (scale <= 1e-8f) return 0;
+    const v3 ax = div3(bx, scale), ay = div3(by, scale), az = norm3(bz);
+    const v3 lo = sub3(o, center);
+    const float ox = dot3(lo, ax), oy = dot3(lo, ay), oz = dot3(lo, az);
+    const float dx = dot3(d, ax), dy = dot3(d, ay), dz = dot3(d, az);
+    const float radius = L->radius * scale;
+    const float a = dx * dx + dy * dy;
+    const float hb = ox * dx + oy * dy;
+    const float c = ox * ox + oy * oy - radius * radius;
     const float disc = hb * hb - a * c;
     if (a <= 1e-16f || disc < 0.0f) return 0;
     const float rt = sqrtf(disc);
     float root = (-hb - rt) / a;
-    if (root <= tmin || root >= tmax) root = (-hb + rt) / a;
-    if (root <= tmin || root >= tmax) return 0;
-    const v3 p = add3(o, scl3(d, root));
-    const v3 nn = norm3(sub3(p, center));
+    float z = oz + root * dz;
+    const float half_len = 0.5f * L->length * len3(bz);
+    if (root <= tmin || root >= tmax || fabsf(z) > half_len) {
+        root = (-hb + rt) / a;
+        z = oz + root * dz;
+    }
+    if (root <= tmin || root >= tmax || fabsf(z) > half_len) return 0;
+    const float x = ox + root * dx, y = oy + root * dy;
+    const v3 nn = norm3(add3(scl3(ax, x), scl3(ay, y)));
     if (dot3(d, nn) >= 0.0f) return 0;
     *t = root;
-    *pos = p;
+    *pos = add3(o, scl3(d, root));
     *n = nn;
     return 1;
 }
 
-/* intersect_analytic_lights (programs.cu:795-813) */
+/* intersect_analytic_lights (programs.cu:795-813): finite lights that are
+ * not points */
 static int intersect_analytic(const ctx_t* c, v3 o, v3 d, float tmin, float tmax, int* index) {
     float closest = tmax;
     int found = -1;
     for (int i = 0; i < c->a_count; ++i) {
+        const rto_analytic_light* L = &c->s->analytic_lights[i];
+        if (!alight_finite(L->type) || L->treat_as_point != 0) continue;
         float t;
         v3 p, n;
-        if (!alight_hit(&c->s->analytic_lights[i], o, d, tmin, closest, &t, &p, &n)) continue;
+        if (!alight_hit(L, o, d, tmin, closest, &t, &p, &n)) continue;
         closest = t;
         found = i;
     }
@@ -777,14 +900,62 @@ static int intersect_analytic(const ctx_t* c, v3 o, v3 d, float tmin, float tmax
     return found >= 0;
 }
 
-/* analytic_emission_mis_weight (programs.cu:838-849) */
+/* analytic_light_pdf_for_hit (programs.cu:815-834) */
+static float alight_pdf_for_hit(const rto_analytic_light* L, v3 origin, v3 dir, v3 hit_pos, v3 hit_n) {
+    if (L->delta != 0 || L->treat_as_point != 0 || L->treat_as_line != 0) return 0.0f;
+    if (L->type == RTO_ALIGHT_SPHERE) {
+        const float radius = L->radius * len3(ld3(L->basis_x));
+        const float dsq = lensq3(sub3(ld3(L->position), origin));
+        const float rsq = radius * radius;
+        if (dsq <= rsq) return 0.0f;
+        return cone_pdf(sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq)));
+    }
+    const float dsq = lensq3(sub3(hit_pos, origin));
+    const float ac = fmaxf(dot3(hit_n, scl3(dir, -1.0f)), 0.0f);
+    return area_pdf(L->world_area, dsq, ac);
+}
+
+/* analytic_emission_mis_weight (programs.cu:836-849), with the light's hit
+ * along the path's ray (o, d) */
 static float analytic_mis(const ctx_t* c, const path_t* st, int li) {
     if (!st->prev_valid || st->prev_delta || li < 0) return 1.0f;
     const rto_analytic_light* L = &c->s->analytic_lights[li];
-    const float dsq = lensq3(sub3(ld3(L->position), st->prev_pos));
-    const float rsq = L->radius * L->radius;
-    const float cond = dsq <= rsq ? 0.0f : cone_pdf(sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq)));
-    return power_heuristic(st->prev_pdf, c->a_sel[li] * cond);
+    float t = 0.0f;
+    v3 p = mk3(0, 0, 0), n = mk3(0, 0, 0);
+    if (L->type != RTO_ALIGHT_SPHERE) (void)alight_hit(L, st->o, st->d, OM_RAY_EPS, OM_RAY_FAR, &t, &p, &n);
+    const float lp = L->selection_pdf * alight_pdf_for_hit(L, st->prev_pos, norm3(st->d), p, n);
+    return power_heuristic(st->prev_pdf, lp);
+}
+
+/* analytic_infinite_direction / _pdf / _radiance (programs.cu:851-890) */
+static v3 analytic_infinite(const ctx_t* c, const path_t* st) {
+    v3 sum = mk3(0, 0, 0);
+    const v3 dir = norm3(st->d);
+    for (int i = 0; i < c->a_count; ++i) {
+        const rto_analytic_light* L = &c->s->analytic_lights[i];
+        if (L->type == RTO_ALIGHT_DISTANT) {
+            const v3 axis = norm3(ld3(L->basis_z));
+            const float threshold = L->delta != 0 ? 0.999999f : L->cos_theta_max;
+            if (!(dot3(axis, dir) >= threshold)) continue;
+        } else if (L->type != RTO_ALIGHT_DOME) {
+            continue;
+        }
+        float w = 1.0f;
+        if (st->prev_valid && !st->prev_delta) {
+            const float cond = L->delta != 0 ? 0.0f
+                               : (L->type == RTO_ALIGHT_DOME ? UNIFORM_SPHERE_PDF : cone_pdf(L->cos_theta_max));
+            if (cond <= 0.0f) continue;
+            w = power_heuristic(st->prev_pdf, L->selection_pdf * cond);
+        }
+        sum = add3(sum, scl3(ld3(L->radiance), w));
+    }
+    return sum;
+}
+
+static int has_infinite_light(const ctx_t* c) {
+    for (int i = 0; i < c->a_count; ++i)
+        if (!alight_finite(c->s->analytic_lights[i].type)) return 1;
+    return 0;
 }
 
 /* sample_packed_analytic_light (light_sampling.h:104-116) */
@@ -793,26 +964,90 @@ static int pick_analytic(const ctx_t* c, float u) {
     return pick_cdf(c->a_cdf, 1, c->a_count, u);
 }
 
-/* sample_analytic_direct_light_candidate, sphere branch (programs.cu:1234-1300) */
+/* sample_analytic_direct_light_candidate (programs.cu:1234-1346) */
 static light_sample_t analytic_candidate(const ctx_t* c, v3 sp, int li, float u0, float u1) {
     light_sample_t ls;
     memset(&ls, 0, sizeof(ls));
     if (li < 0 || li >= c->a_count) return ls;
-    if (c->a_sel[li] <= 0.0f) return ls;
     const rto_analytic_light* L = &c->s->analytic_lights[li];
-    ls.emission = ld3(L->radiance);
+    if (L->selection_pdf <= 0.0f) return ls;
+    const v3 emission = ld3(L->radiance);
+    if (L->type == RTO_ALIGHT_DOME) {
+        ls.dir = om_sphere_dir(u0, u1);
+        ls.dist = OM_RAY_FAR;
+        ls.emission = emission;
+        ls.pdf = L->selection_pdf * UNIFORM_SPHERE_PDF;
+        ls.infinite = 1;
+        ls.valid = ls.pdf > 0.0f && maxc3(ls.emission) > 0.0f;
+        return ls;
+    }
+    if (L->type == RTO_ALIGHT_DISTANT) {
+        const v3 axis = norm3(ld3(L->basis_z));
+        ls.dir = L->delta != 0 ? axis : cone_dir(axis, L->cos_theta_max, u0, u1);
+        ls.dist = OM_RAY_FAR;
+        ls.emission = emission;
+        ls.pdf = L->selection_pdf * (L->delta != 0 ? 1.0f : cone_pdf(L->cos_theta_max));
+        ls.infinite = 1;
+        ls.delta = L->delta != 0;
+        ls.valid = ls.pdf > 0.0f && maxc3(ls.emission) > 0.0f;
+        return ls;
+    }
     const v3 center = ld3(L->position);
-    const v3 tc = sub3(center, sp);
-    const float dsq = lensq3(tc);
-    const float rsq = L->radius * L->radius;
-    if (dsq <= rsq * (1.0f + 1e-6f)) { memset(&ls, 0, sizeof(ls)); return ls; }
-    const float cmax = sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq));
-    ls.dir = cone_dir(norm3(tc), cmax, u0, u1);
-    float t;
-    v3 p, n;
-    if (!alight_hit(L, sp, ls.dir, OM_RAY_EPS, OM_RAY_FAR, &t, &p, &n)) { memset(&ls, 0, sizeof(ls)); return ls; }
-    ls.dist = t;
-    ls.pdf = c->a_sel[li] * cone_pdf(cmax);
+    if (L->type == RTO_ALIGHT_SPHERE) {
+        const v3 tc = sub3(center, sp);
+        const float dsq = lensq3(tc);
+        const float radius = L->radius * len3(ld3(L->basis_x));
+        const float rsq = radius * radius;
+        if (dsq <= rsq * (1.0f + 1e-6f)) return ls;
+        if (L->treat_as_point != 0) {
+            ls.dist = sqrtf(dsq);
+            ls.dir = div3(tc, ls.dist);
+            const float scale = L->world_area > 1e-12f ? L->world_area : 1.0f;
+            ls.emission = scl3(emission, scale / dsq);
+            ls.pdf = L->selection_pdf;
+            ls.delta = 1;
+            ls.valid = maxc3(ls.emission) > 0.0f;
+            return ls;
+        }
+        const float cmax = sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq));
+        const v3 dir = cone_dir(norm3(tc), cmax, u0, u1);
+        float t;
+        v3 p, n;
+        if (!alight_hit(L, sp, dir, OM_RAY_EPS, OM_RAY_FAR, &t, &p, &n)) return ls;
+        ls.dir = dir;
+        ls.emission = emission;
+        ls.dist = t;
+        ls.pdf = L->selection_pdf * cone_pdf(cmax);
+        ls.valid = ls.pdf > 0.0f && maxc3(ls.emission) > 0.0f;
+        return ls;
+    }
+    v3 pos, nrm;
+    if (L->type == RTO_ALIGHT_DISK) {
+        const float radial = L->radius * sqrtf(u0);
+        float sphi, cphi;
+        om_sincos_turn(u1, &sphi, &cphi);
+        pos = add3(center, add3(scl3(ld3(L->basis_x), radial * cphi), scl3(ld3(L->basis_y), radial * sphi)));
+        nrm = alight_normal(L);
+    } else if (L->type == RTO_ALIGHT_RECT) {
+        pos = add3(center, add3(scl3(ld3(L->basis_x), (u0 - 0.5f) * L->width),
+                                scl3(ld3(L->basis_y), (u1 - 0.5f) * L->height)));
+        nrm = alight_normal(L);
+    } else {
+        float sphi, cphi;
+        om_sincos_turn(u0, &sphi, &cphi);
+        const float rx = L->radius * cphi, ry = L->radius * sphi;
+        pos = add3(center, add3(add3(scl3(ld3(L->basis_x), rx), scl3(ld3(L->basis_y), ry)),
+                                scl3(ld3(L->basis_z), (u1 - 0.5f) * L->length)));
+        nrm = norm3(add3(scl3(ld3(L->basis_x), rx), scl3(ld3(L->basis_y), ry)));
+    }
+    const v3 tl = sub3(pos, sp);
+    const float dsq = lensq3(tl);
+    if (dsq <= 1e-12f) return ls;
+    ls.dist = sqrtf(dsq);
+    ls.dir = div3(tl, ls.dist);
+    const float ac = dot3(nrm, scl3(ls.dir, -1.0f));
+    ls.emission = emission;
+    ls.pdf = L->selection_pdf * area_pdf(L->world_area, dsq, ac);
     ls.valid = ls.pdf > 0.0f && maxc3(ls.emission) > 0.0f;
     return ls;
 }
@@ -828,9 +1063,9 @@ static void analytic_direct_light(const ctx_t* c, const hit_t* h, const float r[
     const v3 resp = direct_response(c, h, st, L.dir, &bpdf);
     if (maxc3(resp) <= 0.0f) return;
     const v3 so = add3(h->pos, scl3(L.dir, OM_RAY_EPS * 2.0f));
-    const float max_t = L.dist - OM_RAY_EPS * 3.0f;
+    const float max_t = L.infinite ? OM_RAY_FAR : L.dist - OM_RAY_EPS * 3.0f;
     if (max_t <= OM_RAY_EPS || om_occluded(c->s, c->bvh, so, L.dir, OM_RAY_EPS, max_t)) return;
-    const float mis = !bsdf_avail ? 1.0f : power_heuristic(L.pdf, bpdf);
+    const float mis = L.delta || !bsdf_avail ? 1.0f : power_heuristic(L.pdf, bpdf);
     const v3 direct = scl3(mul3(L.emission, resp), mis / L.pdf);
     st->radiance = add3(st->radiance, mul3(st->throughput, direct));
 }
@@ -948,7 +1183,7 @@ static v3 rs_numerator(const ctx_t* c, const hit_t* h, const path_t* st, const l
     float bpdf = 0.0f;
     const v3 resp = direct_response(c, h, st, L->dir, &bpdf);
     if (maxc3(resp) <= 0.0f) return mk3(0, 0, 0);
-    const float mis = !bsdf_avail ? 1.0f : power_heuristic(L->pdf, bpdf);
+    const float mis = L->delta || !bsdf_avail ? 1.0f : power_heuristic(L->pdf, bpdf);
     const v3 num = scl3(mul3(L->emission, resp), mis);
     *target = fmaxf(0.0f, 0.2126f * num.x + 0.7152f * num.y + 0.0722f * num.z);
     return num;
@@ -1120,7 +1355,7 @@ static void restir_direct(const ctx_t* c, const hit_t* h, const rs_ctx* rc, int 
     const v3 num = rs_numerator(c, h, st, &L, bsdf_avail, &target);
     if (!L.valid || target <= 0.0f) return;
     const v3 so = add3(h->pos, scl3(L.dir, OM_RAY_EPS * 2.0f));
-    const float max_t = L.dist - OM_RAY_EPS * 3.0f;
+    const float max_t = L.infinite ? OM_RAY_FAR : L.dist - OM_RAY_EPS * 3.0f;
     if (max_t <= OM_RAY_EPS || om_occluded(c->s, c->bvh, so, L.dir, OM_RAY_EPS, max_t)) return;
     const v3 direct = scl3(num, r.estimator_weight);
     st->radiance = add3(st->radiance, mul3(st->throughput, direct));
@@ -1253,6 +1488,8 @@ static void trace_one(const ctx_t* c, const rto_camera* cam, int cam_index, cons
         if (!found) {
             const float w = emission_mis(c, &st, NULL, 1);
             st.radiance = add3(st.radiance, mul3(st.throughput, scl3(background_at(s, st.d), w)));
+            if (c->a_count > 0 && has_infinite_light(c))
+                st.radiance = add3(st.radiance, mul3(st.throughput, analytic_infinite(c, &st)));
             break;
         }
         hit_t h;

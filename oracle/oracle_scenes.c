@@ -506,7 +506,13 @@ static int gen_instanced_mesh(rto_generated* g, uint64_t seed) {
         al[i].radius = 0.1f + 0.4f * pcg_u(&r);
         const float rad = 1.0f + 19.0f * pcg_u(&r);
         set3(al[i].radiance, rad, rad, rad);
+        set3(al[i].basis_x, 1.0f, 0.0f, 0.0f);
+        set3(al[i].basis_y, 0.0f, 1.0f, 0.0f);
+        set3(al[i].basis_z, 0.0f, 0.0f, 1.0f);
+        al[i].width = al[i].height = al[i].length = 1.0f;
+        al[i].cos_theta_max = 1.0f;
     }
+    (void)rto_analytic_lights_compile(al, NLIGHT);
     g->scene.triangle_count = NINST * nf;
     g->scene.triangles = tr;
     g->scene.material_count = NMAT;

@@ -79,12 +79,22 @@ typedef struct {
     float geometry_opacity; int32_t geometry_thin_walled;
 } rto_openpbr;
 
-enum { RTO_ALIGHT_SPHERE = 0 };
+/* PackedAnalyticLight (src/common/light_sampling.h:30-62), same field order */
+enum { RTO_ALIGHT_SPHERE = 0, RTO_ALIGHT_DISK = 1, RTO_ALIGHT_RECT = 2, RTO_ALIGHT_CYLINDER = 3,
+       RTO_ALIGHT_DISTANT = 4, RTO_ALIGHT_DOME = 5 };
 typedef struct {
     int32_t type;
     float position[3];
-    float radius;
+    float basis_x[3];
+    float basis_y[3];
+    float basis_z[3];
     float radiance[3];
+    float radius, width, height, length;
+    float world_area;
+    float cos_theta_max;
+    float selection_pdf;
+    float cdf;
+    int32_t delta, treat_as_point, treat_as_line;
 } rto_analytic_light;
 
 enum { RTO_BG_CONSTANT = 0, RTO_BG_SKY_GRADIENT = 1 };
@@ -306,7 +316,12 @@ int rto_render_camera_ex(const rto_scene* scene, const rto_bvh* bvh, const rto_c
     int cam_index, const rto_profile* profile, uint32_t stream, const rto_restir_io* restir,
     float* beauty, float* normal, float* albedo, float* depth, int threads, int row_begin,
     int row_end, rto_render_stats* stats);
-int rto_build_analytic_cdf(const rto_scene* scene, float* sel, float* cdf);
+/* the analytic light compiler's derived fields and selection distribution
+ * (src/scene/analytic_light_compiler.cpp:15-39,114-128,178-240): world area
+ * per type, delta distant lights, FP64 selection weights normalized into
+ * selection_pdf / cdf.  Returns 0, or -1 for an unknown type or a sphere /
+ * cylinder whose basis is not a similarity transform. */
+int rto_analytic_lights_compile(rto_analytic_light* lights, int count);
 
 /* reservoir math (src/common/restir_di.h:67-199) for known-answer tests */
 int rto_restir_update(rto_reservoir* r, int light, float u0, float u1, float target, float weight,

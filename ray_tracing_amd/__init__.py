@@ -28,7 +28,7 @@ EXPORTED = [
     "rt_device_synchronize", "rt_device_info", "rt_device_stream", "rt_scene_upload", "rt_scene_destroy",
     "rt_bvh_build", "rt_bvh_download", "rt_scene_update", "rt_scene_last_update", "rt_trace_closest", "rt_trace_any",
     "rt_trace_closest_device", "rt_trace_closest_device_noalloc", "rt_renderer_create", "rt_renderer_destroy", "rt_render_frame",
-    "rt_render_frame_streams", "rt_renderer_launch_diagnostics",
+    "rt_render_frame_streams", "rt_renderer_launch_diagnostics", "rt_analytic_lights_compile",
     "rt_frame_download", "rt_frame_device_view", "rt_reset_accumulation", "rt_reset_sequence",
     "rt_restir_diagnostics", "rt_renderer_set_tuning", "rt_renderer_get_tuning",
     "rt_scene_generate", "rt_scene_generated_free", "rt_restir_download",
@@ -352,6 +352,15 @@ class Presenter:
         out = np.empty((self.height, self.width, 4), np.uint8)
         check(lib().rt_presenter_download(self.h, ptr(out)))
         return out
+
+
+def compile_analytic_lights(lights) -> None:
+    """rt_analytic_lights_compile in place over an array of analytic lights
+    (a numpy structured array with AnalyticLight's layout, or a ctypes
+    array): world areas, delta distant lights, selection pdf and cdf."""
+    n = len(lights)
+    addr = lights.ctypes.data if isinstance(lights, np.ndarray) else C.addressof(lights)
+    check(lib().rt_analytic_lights_compile(C.c_void_p(addr), n))
 
 
 def tone_map_rgba8(device: Device, d_beauty: int, d_rgba8: int, pixel_count: int):

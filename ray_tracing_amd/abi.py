@@ -48,7 +48,11 @@ class OpenPbr(C.Structure):
 
 
 class AnalyticLight(C.Structure):
-    _fields_ = [("type", C.c_int32), ("position", f3), ("radius", C.c_float), ("radiance", f3)]
+    _fields_ = [("type", C.c_int32), ("position", f3), ("basis_x", f3), ("basis_y", f3),
+                ("basis_z", f3), ("radiance", f3), ("radius", C.c_float), ("width", C.c_float),
+                ("height", C.c_float), ("length", C.c_float), ("world_area", C.c_float),
+                ("cos_theta_max", C.c_float), ("selection_pdf", C.c_float), ("cdf", C.c_float),
+                ("delta", C.c_int32), ("treat_as_point", C.c_int32), ("treat_as_line", C.c_int32)]
 
 
 class SceneDesc(C.Structure):
@@ -247,6 +251,7 @@ def declare(L):
     L.rt_render_frame_streams.argtypes = [vp, P(CameraRig), P(Profile), P(C.c_uint32),
                                           P(C.c_uint32), P(FrameTiming)]
     L.rt_renderer_launch_diagnostics.argtypes = [vp, P(C.c_uint64), P(C.c_uint64)]
+    L.rt_analytic_lights_compile.argtypes = [vp, C.c_int32]
     L.rt_frame_download.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     L.rt_frame_device_view.argtypes = [vp, C.c_int, P(DeviceFrameView)]
     L.rt_presenter_create.argtypes = [vp, C.c_int32, C.c_int32, P(vp)]

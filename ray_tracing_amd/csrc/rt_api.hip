@@ -145,6 +145,8 @@ struct rt_scene {
     std::vector<rtk::LightRec> host_lights;
     DevBuf<rtk::LightRec> lights;
     DevBuf<float> a_sel, a_cdf;
+    DevBuf<int32_t> a_infinite;   // dome / distant lights in index order
+    int32_t a_infinite_count = 0;
     DevBuf<int32_t> light_of_prim;
     int32_t env_light = -1;
     float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};  // primitive bounds
@@ -198,6 +200,8 @@ struct rt_scene {
         v.stack4_entries = std::max(3 * wide_levels, 1);
         v.a_sel = a_sel.p;
         v.a_cdf = a_cdf.p;
+        v.a_infinite = a_infinite.p;
+        v.a_infinite_count = a_infinite_count;
         v.textures = textures.p;
         v.texels = texels.p;
         v.mat_albedo_tex = has_albedo_textures ? mat_albedo_tex.p : nullptr;
@@ -453,37 +457,22 @@ static void build_lights(const rt_scene_desc* d, std::vector<rtk::LightRec>& out
     if (!out.empty()) out.back().cdf = 1.0f;
 }
 
-// analytic light CDF (src/scene/analytic_light_compiler.cpp:15-39,230-237):
-// world area x luminance, accumulated in FP64, closed at the last positive light
-static void build_analytic_cdf(const rt_scene_desc* d, std::vector<float>& sel,
-    std::vector<float>& cdf) {
+// the analytic lights' selection tables, read from their compiled fields
+// (sample_packed_analytic_light, light_sampling.h:104-116, walks the cdf),
+// and the dome / distant lights a missed ray collects
+// (analytic_infinite_radiance, programs.cu:871-890), in index order
+static void analytic_tables(const rt_scene_desc* d, std::vector<float>& sel, std::vector<float>& cdf,
+    std::vector<int32_t>& infinite) {
     const int n = d->analytic_light_count;
-    sel.assign((size_t)n, 0.0f);
-    cdf.assign((size_t)n, 0.0f);
-    std::vector<double> w((size_t)n, 0.0);
-    double total = 0.0;
+    sel.resize((size_t)n);
+    cdf.resize((size_t)n);
+    infinite.clear();
     for (int i = 0; i < n; ++i) {
         const rt_analytic_light& L = d->analytic_lights[i];
-        const double r = (double)L.radius;
-        const double area = 4.0 * 3.14159265358979323846 * r * r;
-        const double lum = 0.2126 * (double)L.radiance[0] + 0.7152 * (double)L.radiance[1]
-                           + 0.0722 * (double)L.radiance[2];
-        double x = area * lum;
-        x = std::isfinite(x) ? (x > 0.0 ? x : 0.0) : 0.0;
-        w[(size_t)i] = x;
-        total += x;
+        sel[(size_t)i] = L.selection_pdf;
+        cdf[(size_t)i] = L.cdf;
+        if (L.type == RT_ALIGHT_DISTANT || L.type == RT_ALIGHT_DOME) infinite.push_back(i);
     }
-    double acc = 0.0;
-    int last = -1;
-    for (int i = 0; i < n; ++i) {
-        const double p = total > 0.0 ? w[(size_t)i] / total : 0.0;
-        acc += p;
-        sel[(size_t)i] = (float)p;
-        cdf[(size_t)i] = (float)acc;
-        if (p > 0.0) last = i;
-    }
-    if (total > 0.0)
-        for (int i = last; i < n; ++i) cdf[(size_t)i] = 1.0f;
 }
 
 // scene arrays and counts (rt_scene_upload / rt_scene_update)
@@ -510,9 +499,20 @@ static rt_status scene_validate(const rt_scene_desc* d) {
             "OpenPBR material " + std::to_string(i)
                 + " uses a lobe outside the supported constant-input subset (base, specular, metal, coat, emission)");
     }
+    float prev_cdf = 0.0f;
     for (int i = 0; i < d->analytic_light_count; ++i) {
-        RT_REQUIRE(d->analytic_lights[i].type == RT_ALIGHT_SPHERE, RT_ERROR_UNSUPPORTED,
-            "analytic light " + std::to_string(i) + " is not a sphere light");
+        const rt_analytic_light& L = d->analytic_lights[i];
+        RT_REQUIRE(L.type >= RT_ALIGHT_SPHERE && L.type <= RT_ALIGHT_DOME, RT_ERROR_INVALID_ARGUMENT,
+            "analytic light " + std::to_string(i) + " has an unknown type " + std::to_string(L.type));
+        const float* f = &L.position[0];
+        bool finite = true;
+        for (int k = 0; k < 15 + 8; ++k) finite = finite && std::isfinite(f[k]);
+        RT_REQUIRE(finite, RT_ERROR_INVALID_ARGUMENT,
+            "analytic light " + std::to_string(i) + " has a non-finite field");
+        RT_REQUIRE(L.selection_pdf >= 0.0f && L.cdf >= prev_cdf && L.cdf <= 1.0f, RT_ERROR_INVALID_ARGUMENT,
+            "analytic light " + std::to_string(i)
+                + " has a selection pdf below 0 or a cdf outside [previous cdf, 1] (rt_analytic_lights_compile)");
+        prev_cdf = L.cdf;
     }
     RT_REQUIRE(d->texture_count >= 0 && d->image_texel_count >= 0, RT_ERROR_INVALID_ARGUMENT,
         "texture counts must be non-negative");
@@ -593,9 +593,12 @@ static rt_status scene_fill(rt_scene* s, const rt_scene_desc* d) {
     if (e == hipSuccess) e = upload(s->lights, s->host_lights.data(), s->host_lights.size(), st);
     if (e == hipSuccess) e = upload(s->light_of_prim, lop.data(), lop.size(), st);
     std::vector<float> asel, acdf;
-    build_analytic_cdf(d, asel, acdf);
+    std::vector<int32_t> ainf;
+    analytic_tables(d, asel, acdf, ainf);
     if (e == hipSuccess) e = upload(s->a_sel, asel.data(), asel.size(), st);
     if (e == hipSuccess) e = upload(s->a_cdf, acdf.data(), acdf.size(), st);
+    if (e == hipSuccess) e = upload(s->a_infinite, ainf.data(), ainf.size(), st);
+    s->a_infinite_count = (int32_t)ainf.size();
     s->texture_count = d->texture_count;
     s->texel_count = d->image_texel_count;
     s->has_albedo_textures = d->material_albedo_texture != nullptr && d->material_count > 0;
@@ -723,6 +726,98 @@ rt_status rt_device_synchronize(rt_device* dev) {
 rt_status rt_device_stream(rt_device* dev, void** stream) {
     RT_REQUIRE(dev != nullptr && stream != nullptr, RT_ERROR_INVALID_ARGUMENT, "null device or output");
     *stream = static_cast<void*>(dev->stream);
+    return RT_OK;
+}
+
+// AnalyticLightDesc compilation (analytic_light_compiler.cpp:114-128,
+// 178-240) and finalize_analytic_light_distribution (:15-39), over the
+// packed fields in FP64
+rt_status rt_analytic_lights_compile(rt_analytic_light* lights, int32_t count) {
+    RT_REQUIRE(count >= 0 && (count == 0 || lights != nullptr), RT_ERROR_INVALID_ARGUMENT,
+        "rt_analytic_lights_compile requires count >= 0 and a light array");
+    const double kPiD = 3.14159265358979323846;
+    std::vector<double> weight((size_t)count, 0.0);
+    double total = 0.0;
+    for (int i = 0; i < count; ++i) {
+        rt_analytic_light& L = lights[i];
+        RT_REQUIRE(L.type >= RT_ALIGHT_SPHERE && L.type <= RT_ALIGHT_DOME, RT_ERROR_INVALID_ARGUMENT,
+            "analytic light " + std::to_string(i) + " has an unknown type " + std::to_string(L.type));
+        double B[3][3];   // columns basis_x, basis_y, basis_z
+        for (int r = 0; r < 3; ++r) {
+            B[r][0] = L.basis_x[r];
+            B[r][1] = L.basis_y[r];
+            B[r][2] = L.basis_z[r];
+        }
+        auto col_cross_norm = [&]() {
+            const double cx = B[1][0] * B[2][1] - B[2][0] * B[1][1];
+            const double cy = B[2][0] * B[0][1] - B[0][0] * B[2][1];
+            const double cz = B[0][0] * B[1][1] - B[1][0] * B[0][1];
+            return std::sqrt(cx * cx + cy * cy + cz * cz);
+        };
+        // require_similarity_scale (:114-128): B^T B = s^2 I within 1e-6 max(1, s^2)
+        auto similarity_scale = [&](const char* what, double* scale) -> rt_status {
+            double G[3][3];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) G[a][b] = B[0][a] * B[0][b] + B[1][a] * B[1][b] + B[2][a] * B[2][b];
+            const double s2 = (G[0][0] + G[1][1] + G[2][2]) / 3.0;
+            RT_REQUIRE(std::isfinite(s2) && s2 > 0.0, RT_ERROR_INVALID_ARGUMENT,
+                std::string(what) + " light has a singular transform: analytic light " + std::to_string(i));
+            const double tol = 1e-6 * std::max(1.0, s2);
+            double dev = 0.0;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) dev = std::max(dev, std::fabs(G[a][b] - (a == b ? s2 : 0.0)));
+            RT_REQUIRE(dev <= tol, RT_ERROR_INVALID_ARGUMENT,
+                std::string(what) + " light requires a similarity transform in this backend: analytic light "
+                    + std::to_string(i));
+            *scale = std::sqrt(s2);
+            return RT_OK;
+        };
+        double area = 0.0, scale = 1.0;
+        switch (L.type) {
+            case RT_ALIGHT_SPHERE: {
+                const rt_status st = similarity_scale("sphere", &scale);
+                if (st != RT_OK) return st;
+                area = 4.0 * kPiD * std::pow((double)L.radius * scale, 2.0);
+                break;
+            }
+            case RT_ALIGHT_DISK:
+                area = kPiD * (double)L.radius * (double)L.radius * col_cross_norm();
+                break;
+            case RT_ALIGHT_RECT:
+                area = (double)L.width * (double)L.height * col_cross_norm();
+                break;
+            case RT_ALIGHT_CYLINDER: {
+                const rt_status st = similarity_scale("cylinder", &scale);
+                if (st != RT_OK) return st;
+                area = 2.0 * kPiD * ((double)L.radius * scale) * ((double)L.length * scale);
+                break;
+            }
+            default: break;
+        }
+        if (L.type <= RT_ALIGHT_CYLINDER) L.world_area = (float)area;
+        if (L.type == RT_ALIGHT_DISTANT) L.delta = L.cos_theta_max >= 1.0f ? 1 : 0;
+        const double lum = std::max(0.0, 0.2126 * (double)L.radiance[0] + 0.7152 * (double)L.radiance[1]
+                                             + 0.0722 * (double)L.radiance[2]);
+        double w;
+        if (L.type == RT_ALIGHT_DOME) w = 4.0 * kPiD * lum;
+        else if (L.type == RT_ALIGHT_DISTANT)
+            w = (L.delta ? 1.0 : 2.0 * kPiD * (1.0 - (double)L.cos_theta_max)) * lum;
+        else w = area * lum;
+        w = std::isfinite(w) ? std::max(0.0, w) : 0.0;
+        weight[(size_t)i] = w;
+        total += w;
+    }
+    double acc = 0.0;
+    int last = -1;
+    for (int i = 0; i < count; ++i) {
+        const double p = total > 0.0 ? weight[(size_t)i] / total : 0.0;
+        acc += p;
+        lights[i].selection_pdf = (float)p;
+        lights[i].cdf = (float)acc;
+        if (p > 0.0) last = i;
+    }
+    if (total > 0.0)
+        for (int i = last; i < count; ++i) lights[i].cdf = 1.0f;
     return RT_OK;
 }
 

@@ -56,6 +56,8 @@ struct SceneView {
     int32_t stack_depth;  // LDS traversal stack entries per lane (>= BVH depth)
     const float* a_sel;   // analytic light selection pdf
     const float* a_cdf;   // analytic light CDF
+    const int32_t* a_infinite;  // dome / distant light indices, ascending
+    int32_t a_infinite_count;
     const rt_texture* textures;
     const float* texels;               // 3 floats per texel
     const int32_t* mat_albedo_tex;     // nullptr: every albedo is a constant

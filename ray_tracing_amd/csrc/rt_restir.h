@@ -151,7 +151,7 @@ __device__ inline f3 rs_numerator(const SceneView& s, const HitInfo& h, f3 ray_d
     float bpdf = 0.0f;
     const f3 resp = direct_response(s, h, ray_d, L.dir, &bpdf);
     if (maxc3(resp) <= 0.0f) return mk3(0, 0, 0);
-    const float mis = !bsdf_avail ? 1.0f : power_heuristic(L.pdf, bpdf);
+    const float mis = L.delta || !bsdf_avail ? 1.0f : power_heuristic(L.pdf, bpdf);
     const f3 num = (L.emission * resp) * mis;
     *target = fmaxf(0.0f, 0.2126f * num.x + 0.7152f * num.y + 0.0722f * num.z);
     return num;
@@ -335,7 +335,7 @@ __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& r
     float target = 0.0f;
     const f3 num = rs_numerator(s, h, ray_d, L, bsdf_avail, &target);
     if (!L.valid || target <= 0.0f) return ro;
-    const float max_t = L.dist - kRayEps * 3.0f;
+    const float max_t = L.infinite ? kRayFar : L.dist - kRayEps * 3.0f;
     if (max_t <= kRayEps) return ro;
     ro.shadow = true;
     ro.origin = h.pos + L.dir * (kRayEps * 2.0f);

@@ -493,7 +493,15 @@ void generate_instanced_mesh(uint64_t seed, rt_generated_scene* g) {
         L.radius = 0.1f + 0.4f * r.uniform();
         const float rad = 1.0f + 19.0f * r.uniform();
         set3(L.radiance, rad, rad, rad);
+        set3(L.basis_x, 1.0f, 0.0f, 0.0f);
+        set3(L.basis_y, 0.0f, 1.0f, 0.0f);
+        set3(L.basis_z, 0.0f, 0.0f, 1.0f);
+        L.width = L.height = L.length = 1.0f;
+        L.cos_theta_max = 1.0f;
     }
+    // world areas and the power x area selection distribution (identity
+    // bases are similarity transforms, so the compile cannot refuse them)
+    (void)rt_analytic_lights_compile(al.data(), NLIGHT);
     g->scene.triangle_count = static_cast<int32_t>(tr.size());
     g->scene.triangles = copy_out(tr);
     g->scene.material_count = NMAT;

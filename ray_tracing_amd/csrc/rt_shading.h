@@ -192,7 +192,7 @@ __device__ inline float emission_mis(const SceneView& s, const PathPrev& pv, f3 
 struct LightSample {
     f3 dir, emission;
     float dist, pdf;
-    bool infinite, valid;
+    bool infinite, valid, delta;
 };
 
 __device__ __forceinline__ f3 cone_dir(f3 axis, float cos_max, float u0, float u1) {
@@ -346,54 +346,176 @@ __device__ inline f3 direct_response(const SceneView& s, const HitInfo& h, f3 ra
     return value * cs;
 }
 
-// ---- analytic sphere lights
-// try_intersect_analytic_light, sphere branch (programs.cu:679-709)
-__device__ __forceinline__ bool alight_hit(const rt_analytic_light& L, f3 o, f3 d, float tmin,
-    float tmax, float* t, f3* pos, f3* n) {
+// ---- analytic lights (PackedAnalyticLight, light_sampling.h:30-62)
+__device__ __forceinline__ bool alight_finite(int type) {
+    return type == RT_ALIGHT_SPHERE || type == RT_ALIGHT_DISK || type == RT_ALIGHT_RECT
+           || type == RT_ALIGHT_CYLINDER;
+}
+
+// analytic_light_normal (programs.cu:655-659): -(basis_x x basis_y)
+__device__ __forceinline__ f3 alight_normal(const rt_analytic_light& L) {
+    return normalize3(cross3(ld3(L.basis_x), ld3(L.basis_y)) * -1.0f);
+}
+
+// analytic_plane_coordinates (programs.cu:661-677): the point's coordinates
+// in the (basis_x, basis_y) frame by the normal equations
+__device__ __forceinline__ bool alight_plane_xy(const rt_analytic_light& L, f3 p, float* lx, float* ly) {
+    const f3 bx = ld3(L.basis_x), by = ld3(L.basis_y);
+    const f3 local = p - ld3(L.position);
+    const float xx = dot3(bx, bx), xy = dot3(bx, by), yy = dot3(by, by);
+    const float det = xx * yy - xy * xy;
+    if (det <= 1e-16f) return false;
+    const float px = dot3(local, bx), py = dot3(local, by);
+    *lx = (px * yy - py * xy) / det;
+    *ly = (py * xx - px * xy) / det;
+    return true;
+}
+
+// try_intersect_analytic_light (programs.cu:679-793): front faces of the
+// sphere, disk / rect and the cylinder's side; the nearer root in
+// (tmin, tmax), else the farther
+__device__ inline bool alight_hit(const rt_analytic_light& L, f3 o, f3 d, float tmin, float tmax,
+    float* t, f3* pos, f3* n) {
     const f3 center = ld3(L.position);
-    const float radius = L.radius;
-    const f3 oc = o - center;
-    const float a = lensq3(d);
-    const float hb = dot3(oc, d);
-    const float c = lensq3(oc) - radius * radius;
+    if (L.type == RT_ALIGHT_SPHERE) {
+        const float radius = L.radius * len3(ld3(L.basis_x));
+        const f3 oc = o - center;
+        const float a = lensq3(d);
+        const float hb = dot3(oc, d);
+        const float c = lensq3(oc) - radius * radius;
+        const float disc = hb * hb - a * c;
+        if (a <= 1e-16f || disc < 0.0f) return false;
+        const float rt = sqrtf(disc);
+        float root = (-hb - rt) / a;
+        if (root <= tmin || root >= tmax) root = (-hb + rt) / a;
+        if (root <= tmin || root >= tmax) return false;
+        const f3 p = o + d * root;
+        const f3 nn = normalize3(p - center);
+        if (dot3(d, nn) >= 0.0f) return false;
+        *t = root;
+        *pos = p;
+        *n = nn;
+        return true;
+    }
+    if (L.type == RT_ALIGHT_DISK || L.type == RT_ALIGHT_RECT) {
+        const f3 nn = alight_normal(L);
+        const float den = dot3(d, nn);
+        if (den >= -1e-8f) return false;
+        const float tt = dot3(center - o, nn) / den;
+        if (tt <= tmin || tt >= tmax) return false;
+        const f3 p = o + d * tt;
+        float lx = 0.0f, ly = 0.0f;
+        if (!alight_plane_xy(L, p, &lx, &ly)) return false;
+        const bool inside = L.type == RT_ALIGHT_DISK
+                                ? lx * lx + ly * ly <= L.radius * L.radius
+                                : fabsf(lx) <= 0.5f * L.width && fabsf(ly) <= 0.5f * L.height;
+        if (!inside) return false;
+        *t = tt;
+        *pos = p;
+        *n = nn;
+        return true;
+    }
+    if (L.type != RT_ALIGHT_CYLINDER) return false;
+    const f3 bx = ld3(L.basis_x), by = ld3(L.basis_y), bz = ld3(L.basis_z);
+    const float scale = len3(bx);
+    if (scale <= 1e-8f) return false;
+    const f3 ax = div3(bx, scale), ay = div3(by, scale), az = normalize3(bz);
+    const f3 lo = o - center;
+    const float ox = dot3(lo, ax), oy = dot3(lo, ay), oz = dot3(lo, az);
+    const float dx = dot3(d, ax), dy = dot3(d, ay), dz = dot3(d, az);
+    const float radius = L.radius * scale;
+    const float a = dx * dx + dy * dy;
+    const float hb = ox * dx + oy * dy;
+    const float c = ox * ox + oy * oy - radius * radius;
     const float disc = hb * hb - a * c;
     if (a <= 1e-16f || disc < 0.0f) return false;
     const float rt = sqrtf(disc);
     float root = (-hb - rt) / a;
-    if (root <= tmin || root >= tmax) root = (-hb + rt) / a;
-    if (root <= tmin || root >= tmax) return false;
-    const f3 p = o + d * root;
-    const f3 nn = normalize3(p - center);
+    float z = oz + root * dz;
+    const float half_len = 0.5f * L.length * len3(bz);
+    if (root <= tmin || root >= tmax || fabsf(z) > half_len) {
+        root = (-hb + rt) / a;
+        z = oz + root * dz;
+    }
+    if (root <= tmin || root >= tmax || fabsf(z) > half_len) return false;
+    const float x = ox + root * dx, y = oy + root * dy;
+    const f3 nn = normalize3(ax * x + ay * y);
     if (dot3(d, nn) >= 0.0f) return false;
     *t = root;
-    *pos = p;
+    *pos = o + d * root;
     *n = nn;
     return true;
 }
 
-// intersect_analytic_lights (programs.cu:795-813): nearest front-facing hit,
-// lowest index on equal distance
+// intersect_analytic_lights (programs.cu:795-813): nearest front-facing hit
+// of the finite lights that are not points, lowest index on equal distance
 __device__ inline int intersect_analytic(const SceneView& s, f3 o, f3 d, float tmin, float tmax) {
     float closest = tmax;
     int found = -1;
     for (int i = 0; i < s.analytic_count; ++i) {
+        const rt_analytic_light L = s.analytic[i];
+        if (!alight_finite(L.type) || L.treat_as_point != 0) continue;
         float t;
         f3 p, n;
-        if (!alight_hit(s.analytic[i], o, d, tmin, closest, &t, &p, &n)) continue;
+        if (!alight_hit(L, o, d, tmin, closest, &t, &p, &n)) continue;
         closest = t;
         found = i;
     }
     return found;
 }
 
-// analytic_emission_mis_weight (programs.cu:838-849)
-__device__ __forceinline__ float analytic_mis(const SceneView& s, const PathPrev& pv, int li) {
+// analytic_light_pdf_for_hit (programs.cu:815-834)
+__device__ __forceinline__ float alight_pdf_for_hit(const rt_analytic_light& L, f3 origin, f3 dir,
+    f3 hit_pos, f3 hit_n) {
+    if (L.delta != 0 || L.treat_as_point != 0 || L.treat_as_line != 0) return 0.0f;
+    if (L.type == RT_ALIGHT_SPHERE) {
+        const float radius = L.radius * len3(ld3(L.basis_x));
+        const float dsq = lensq3(ld3(L.position) - origin);
+        const float rsq = radius * radius;
+        if (dsq <= rsq) return 0.0f;
+        return cone_pdf(sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq)));
+    }
+    const float dsq = lensq3(hit_pos - origin);
+    const float ac = fmaxf(dot3(hit_n, dir * -1.0f), 0.0f);
+    return area_pdf(L.world_area, dsq, ac);
+}
+
+// analytic_emission_mis_weight (programs.cu:836-849): the light's hit along
+// the path's ray (ro, rd) is recomputed; the pdf uses the normalized ray
+__device__ __forceinline__ float analytic_mis(const SceneView& s, const PathPrev& pv, int li, f3 ro,
+    f3 rd) {
     if (!pv.valid || pv.delta || li < 0) return 1.0f;
     const rt_analytic_light L = s.analytic[li];
-    const float dsq = lensq3(ld3(L.position) - pv.pos);
-    const float rsq = L.radius * L.radius;
-    const float cond = dsq <= rsq ? 0.0f : cone_pdf(sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq)));
-    return power_heuristic(pv.pdf, s.a_sel[li] * cond);
+    float t = 0.0f;
+    f3 p = mk3(0, 0, 0), n = mk3(0, 0, 0);
+    if (L.type != RT_ALIGHT_SPHERE) (void)alight_hit(L, ro, rd, kRayEps, kRayFar, &t, &p, &n);
+    const float lp = L.selection_pdf * alight_pdf_for_hit(L, pv.pos, normalize3(rd), p, n);
+    return power_heuristic(pv.pdf, lp);
+}
+
+// analytic_infinite_direction / analytic_infinite_pdf /
+// analytic_infinite_radiance (programs.cu:851-890): the dome and distant
+// lights seen along a missed ray, in index order
+__device__ inline f3 analytic_infinite(const SceneView& s, const PathPrev& pv, f3 rd) {
+    f3 sum = mk3(0, 0, 0);
+    const f3 dir = normalize3(rd);
+    for (int k = 0; k < s.a_infinite_count; ++k) {
+        const rt_analytic_light L = s.analytic[s.a_infinite[k]];
+        if (L.type == RT_ALIGHT_DISTANT) {
+            const f3 axis = normalize3(ld3(L.basis_z));
+            const float threshold = L.delta != 0 ? 0.999999f : L.cos_theta_max;
+            if (!(dot3(axis, dir) >= threshold)) continue;
+        }
+        float w = 1.0f;
+        if (pv.valid && !pv.delta) {
+            const float cond = L.delta != 0 ? 0.0f
+                               : (L.type == RT_ALIGHT_DOME ? kUniformSpherePdf : cone_pdf(L.cos_theta_max));
+            if (cond <= 0.0f) continue;
+            w = power_heuristic(pv.pdf, L.selection_pdf * cond);
+        }
+        sum = sum + ld3(L.radiance) * w;
+    }
+    return sum;
 }
 
 // sample_packed_analytic_light (light_sampling.h:104-116) by bisection
@@ -409,29 +531,91 @@ __device__ __forceinline__ int pick_analytic(const SceneView& s, float u) {
     return lo;
 }
 
-// sample_analytic_direct_light_candidate, sphere branch (programs.cu:1234-1300)
+// sample_analytic_direct_light_candidate (programs.cu:1234-1346)
 __device__ inline LightSample analytic_candidate(const SceneView& s, f3 sp, int li, float u0,
     float u1) {
     LightSample ls{};
     ls.valid = false;
     ls.infinite = false;
+    ls.delta = false;
     if (li < 0 || li >= s.analytic_count) return ls;
-    if (s.a_sel[li] <= 0.0f) return ls;
     const rt_analytic_light L = s.analytic[li];
+    if (L.selection_pdf <= 0.0f) return ls;
+    const f3 emission = ld3(L.radiance);
+    if (L.type == RT_ALIGHT_DOME) {
+        ls.dir = sphere_dir(u0, u1);
+        ls.dist = kRayFar;
+        ls.emission = emission;
+        ls.pdf = L.selection_pdf * kUniformSpherePdf;
+        ls.infinite = true;
+        ls.valid = ls.pdf > 0.0f && maxc3(ls.emission) > 0.0f;
+        return ls;
+    }
+    if (L.type == RT_ALIGHT_DISTANT) {
+        const f3 axis = normalize3(ld3(L.basis_z));
+        ls.dir = L.delta != 0 ? axis : cone_dir(axis, L.cos_theta_max, u0, u1);
+        ls.dist = kRayFar;
+        ls.emission = emission;
+        ls.pdf = L.selection_pdf * (L.delta != 0 ? 1.0f : cone_pdf(L.cos_theta_max));
+        ls.infinite = true;
+        ls.delta = L.delta != 0;
+        ls.valid = ls.pdf > 0.0f && maxc3(ls.emission) > 0.0f;
+        return ls;
+    }
     const f3 center = ld3(L.position);
-    const f3 tc = center - sp;
-    const float dsq = lensq3(tc);
-    const float rsq = L.radius * L.radius;
-    if (dsq <= rsq * (1.0f + 1e-6f)) return ls;
-    const float cmax = sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq));
-    const f3 dir = cone_dir(normalize3(tc), cmax, u0, u1);
-    float t;
-    f3 p, n;
-    if (!alight_hit(L, sp, dir, kRayEps, kRayFar, &t, &p, &n)) return ls;
-    ls.dir = dir;
-    ls.emission = ld3(L.radiance);
-    ls.dist = t;
-    ls.pdf = s.a_sel[li] * cone_pdf(cmax);
+    if (L.type == RT_ALIGHT_SPHERE) {
+        const f3 tc = center - sp;
+        const float dsq = lensq3(tc);
+        const float radius = L.radius * len3(ld3(L.basis_x));
+        const float rsq = radius * radius;
+        if (dsq <= rsq * (1.0f + 1e-6f)) return ls;
+        if (L.treat_as_point != 0) {
+            ls.dist = sqrtf(dsq);
+            ls.dir = div3(tc, ls.dist);
+            const float scale = L.world_area > 1e-12f ? L.world_area : 1.0f;
+            ls.emission = emission * (scale / dsq);
+            ls.pdf = L.selection_pdf;
+            ls.delta = true;
+            ls.valid = maxc3(ls.emission) > 0.0f;
+            return ls;
+        }
+        const float cmax = sqrtf(fmaxf(0.0f, 1.0f - rsq / dsq));
+        const f3 dir = cone_dir(normalize3(tc), cmax, u0, u1);
+        float t;
+        f3 p, n;
+        if (!alight_hit(L, sp, dir, kRayEps, kRayFar, &t, &p, &n)) return ls;
+        ls.dir = dir;
+        ls.emission = emission;
+        ls.dist = t;
+        ls.pdf = L.selection_pdf * cone_pdf(cmax);
+        ls.valid = ls.pdf > 0.0f && maxc3(ls.emission) > 0.0f;
+        return ls;
+    }
+    f3 pos, nrm;
+    if (L.type == RT_ALIGHT_DISK) {
+        const float radial = L.radius * sqrtf(u0);
+        float sphi, cphi;
+        sincos_turn(u1, &sphi, &cphi);
+        pos = center + (ld3(L.basis_x) * (radial * cphi) + ld3(L.basis_y) * (radial * sphi));
+        nrm = alight_normal(L);
+    } else if (L.type == RT_ALIGHT_RECT) {
+        pos = center + (ld3(L.basis_x) * ((u0 - 0.5f) * L.width) + ld3(L.basis_y) * ((u1 - 0.5f) * L.height));
+        nrm = alight_normal(L);
+    } else {
+        float sphi, cphi;
+        sincos_turn(u0, &sphi, &cphi);
+        const float rx = L.radius * cphi, ry = L.radius * sphi;
+        pos = center + ((ld3(L.basis_x) * rx + ld3(L.basis_y) * ry) + ld3(L.basis_z) * ((u1 - 0.5f) * L.length));
+        nrm = normalize3(ld3(L.basis_x) * rx + ld3(L.basis_y) * ry);
+    }
+    const f3 tl = pos - sp;
+    const float dsq = lensq3(tl);
+    if (dsq <= 1e-12f) return ls;
+    ls.dist = sqrtf(dsq);
+    ls.dir = div3(tl, ls.dist);
+    const float ac = dot3(nrm, ls.dir * -1.0f);
+    ls.emission = emission;
+    ls.pdf = L.selection_pdf * area_pdf(L.world_area, dsq, ac);
     ls.valid = ls.pdf > 0.0f && maxc3(ls.emission) > 0.0f;
     return ls;
 }

@@ -336,12 +336,14 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
             const int prim = __float_as_int(hv.y);
             const int ali = __float_as_int(hv.z);
             if (ali >= 0) {
-                // path ends on an analytic light (programs.cu:1863-1872)
-                const float w = analytic_mis(s, pv, ali);
+                // path ends on an analytic light (programs.cu:1857-1866)
+                const float w = analytic_mis(s, pv, ali, ro, rd);
                 rad = rad + thr * (ld3(s.analytic[ali].radiance) * w);
             } else if (prim < 0) {
+                // a miss: background, then the dome / distant lights (programs.cu:1867-1875)
                 const float w = emission_mis(s, pv, rd, nullptr, true);
                 rad = rad + thr * (background_at(s, rd) * w);
+                if (s.a_infinite_count > 0) rad = rad + thr * analytic_infinite(s, pv, rd);
             } else {
                 const float4 rl = rng4(id.pixel, id.sample, id.key, (uint32_t)bounce,
                     kGroupLight, id.stream);
@@ -421,9 +423,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                         float bpdf = 0.0f;
                         const f3 resp = direct_response(s, h, rd, ls.dir, &bpdf);
                         if (maxc3(resp) > 0.0f) {
-                            const float max_t = ls.dist - kRayEps * 3.0f;
+                            const float max_t = ls.infinite ? kRayFar : ls.dist - kRayEps * 3.0f;
                             if (max_t > kRayEps) {
-                                const float mis = !bsdf_avail ? 1.0f : power_heuristic(ls.pdf, bpdf);
+                                const float mis = ls.delta || !bsdf_avail ? 1.0f : power_heuristic(ls.pdf, bpdf);
                                 const f3 direct = (ls.emission * resp) * (mis / ls.pdf);
                                 shadow2 = true;
                                 s2_o = f4(h.pos + ls.dir * (kRayEps * 2.0f), max_t);

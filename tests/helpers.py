@@ -73,8 +73,33 @@ QUAD_DT = np.dtype([("origin", "<f4", 3), ("edge_u", "<f4", 3), ("edge_v", "<f4"
 TRI_DT = np.dtype([("p0", "<f4", 3), ("p1", "<f4", 3), ("p2", "<f4", 3), ("material", "<i4")])
 MAT_DT = np.dtype([("type", "<i4"), ("albedo", "<f4", 3), ("emission", "<f4", 3),
                    ("fuzz", "<f4"), ("ior", "<f4"), ("openpbr_index", "<i4")])
-ALIGHT_DT = np.dtype([("type", "<i4"), ("position", "<f4", 3), ("radius", "<f4"),
-                      ("radiance", "<f4", 3)])
+ALIGHT_DT = np.dtype([("type", "<i4"), ("position", "<f4", 3), ("basis_x", "<f4", 3),
+                      ("basis_y", "<f4", 3), ("basis_z", "<f4", 3), ("radiance", "<f4", 3),
+                      ("radius", "<f4"), ("width", "<f4"), ("height", "<f4"), ("length", "<f4"),
+                      ("world_area", "<f4"), ("cos_theta_max", "<f4"), ("selection_pdf", "<f4"),
+                      ("cdf", "<f4"), ("delta", "<i4"), ("treat_as_point", "<i4"),
+                      ("treat_as_line", "<i4")])
+ALIGHT_TYPES = {"sphere": 0, "disk": 1, "rect": 2, "cylinder": 3, "distant": 4, "dome": 5}
+
+
+def analytic_lights(oracle, specs):
+    """Analytic lights from dicts (type name, position, radiance and any of
+    radius / width / height / length / basis (three columns) /
+    cos_theta_max / treat_as_point / treat_as_line), compiled by the
+    oracle's restatement of the light compiler."""
+    a = np.zeros(len(specs), ALIGHT_DT)
+    for i, sp in enumerate(specs):
+        r = a[i]
+        r["type"] = ALIGHT_TYPES[sp["type"]]
+        r["position"] = sp.get("position", (0.0, 0.0, 0.0))
+        bx, by, bz = sp.get("basis", ((1, 0, 0), (0, 1, 0), (0, 0, 1)))
+        r["basis_x"], r["basis_y"], r["basis_z"] = bx, by, bz
+        r["radiance"] = sp["radiance"]
+        for k, d in (("radius", 0.5), ("width", 1.0), ("height", 1.0), ("length", 1.0),
+                     ("cos_theta_max", 1.0), ("treat_as_point", 0), ("treat_as_line", 0)):
+            r[k] = sp.get(k, d)
+    assert oracle.lib().rto_analytic_lights_compile(a.ctypes.data, len(a)) == 0
+    return a
 
 
 def lambert_materials(n, albedo=(0.5, 0.5, 0.5)):

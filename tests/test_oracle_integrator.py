@@ -33,7 +33,7 @@ import numpy as np
 import pytest
 
 import oracle_ctypes as O
-from helpers import ALIGHT_DT, MAT_DT, QUAD_DT, SPHERE_DT, TRI_DT
+from helpers import ALIGHT_DT, MAT_DT, QUAD_DT, SPHERE_DT, TRI_DT, analytic_lights
 
 
 def material(kind, albedo=(0.5, 0.5, 0.5), emission=(0, 0, 0), fuzz=0.0, ior=1.0, openpbr=-1):
@@ -271,8 +271,8 @@ def lit_floor(kind, le=(6.0, 5.0, 4.0), rho=(0.6, 0.5, 0.4)):
             sph = np.zeros(1, SPHERE_DT)
             sph["center"], sph["radius"], sph["material"] = tuple(centre), r, 1
         else:
-            ana = np.zeros(1, ALIGHT_DT)
-            ana["type"], ana["position"], ana["radius"], ana["radiance"] = 0, tuple(centre), r, le
+            ana = analytic_lights(O, [{"type": "sphere", "position": tuple(centre), "radius": r,
+                                       "radiance": le}])
     s, keep = O.scene_from_arrays(spheres=sph, quads=np.concatenate(quads), triangles=tris,
                                   materials=mats, analytic=ana)
     return s, keep, expect

@@ -99,15 +99,18 @@ def test_update_sequences_match_reference():
 
 
 def test_analytic_cdf_closes_and_is_monotone():
+    """The generated 256 sphere lights, compiled by the oracle's light
+    compiler: selection pdf = world area x luminance / total (grey radiance,
+    so luminance = radiance), cdf non-decreasing and closed at 1."""
+    from helpers import ALIGHT_DT
     g = O.GeneratedScene(2, 0)
     n = g.scene.analytic_light_count
-    sel = np.zeros(n, np.float32)
-    cdf = np.zeros(n, np.float32)
-    O.lib().rto_build_analytic_cdf(C.byref(g.scene), sel.ctypes.data, cdf.ctypes.data)
     assert n == 256
-    assert np.all(np.diff(cdf) >= 0) and cdf[-1] == 1.0
     lights = np.ctypeslib.as_array(
-        C.cast(g.scene.analytic_lights, C.POINTER(C.c_float)), (n * 8,)).reshape(n, 8)
-    radius, radiance = lights[:, 4].astype(np.float64), lights[:, 5].astype(np.float64)
-    w = 4 * np.pi * radius ** 2 * radiance  # grey radiance: luminance == radiance
+        C.cast(g.scene.analytic_lights, C.POINTER(C.c_uint8)), (n * ALIGHT_DT.itemsize,)).view(ALIGHT_DT)
+    sel, cdf = lights["selection_pdf"], lights["cdf"]
+    assert np.all(np.diff(cdf) >= 0) and cdf[-1] == 1.0
+    radius, radiance = lights["radius"].astype(np.float64), lights["radiance"][:, 0].astype(np.float64)
+    w = 4 * np.pi * radius ** 2 * radiance
     assert np.allclose(sel, w / w.sum(), rtol=1e-5)
+    assert np.allclose(lights["world_area"], 4 * np.pi * radius ** 2, rtol=1e-6)
