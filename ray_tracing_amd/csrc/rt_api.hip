@@ -13,6 +13,7 @@
 #include <cstddef>
 #include <chrono>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -147,6 +148,9 @@ struct rt_scene {
     DevBuf<float> a_sel, a_cdf;
     DevBuf<int32_t> a_infinite;   // dome / distant lights in index order
     int32_t a_infinite_count = 0;
+    DevBuf<float4> a_nodes;       // BVH over the finite analytic lights
+    DevBuf<int32_t> a_order;
+    int32_t a_node_count = 0;
     DevBuf<int32_t> light_of_prim;
     int32_t env_light = -1;
     float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};  // primitive bounds
@@ -202,6 +206,9 @@ struct rt_scene {
         v.a_cdf = a_cdf.p;
         v.a_infinite = a_infinite.p;
         v.a_infinite_count = a_infinite_count;
+        v.a_nodes = a_nodes.p;
+        v.a_order = a_order.p;
+        v.a_node_count = a_node_count;
         v.textures = textures.p;
         v.texels = texels.p;
         v.mat_albedo_tex = has_albedo_textures ? mat_albedo_tex.p : nullptr;
@@ -475,6 +482,99 @@ static void analytic_tables(const rt_scene_desc* d, std::vector<float>& sel, std
     }
 }
 
+// A binary BVH over the finite analytic lights that rays can hit (sphere,
+// disk, rect, cylinder; not points), so the analytic hit test visits the
+// lights near a ray instead of all of them.  Each light's box is its exact
+// extent (sphere c +- r |bx|; disk r sqrt(bx_a^2 + by_a^2); rect
+// w/2 |bx_a| + h/2 |by_a|; cylinder the disk term plus l/2 |bz_a|), padded
+// so the box test in float never culls a light the exact test would hit.
+// Median splits on the widest centroid axis, at most 4 lights per leaf.
+static void analytic_light_bvh(const rt_scene_desc* d, std::vector<float4>& nodes,
+    std::vector<int32_t>& order) {
+    nodes.clear();
+    order.clear();
+    auto bits = [](int32_t v) {
+        float f;
+        std::memcpy(&f, &v, sizeof(f));
+        return f;
+    };
+    struct Item { double lo[3], hi[3], c[3]; int32_t index; };
+    std::vector<Item> items;
+    for (int i = 0; i < d->analytic_light_count; ++i) {
+        const rt_analytic_light& L = d->analytic_lights[i];
+        if (L.type > RT_ALIGHT_CYLINDER || L.treat_as_point != 0) continue;
+        double ext[3];
+        for (int a = 0; a < 3; ++a) {
+            const double bx = L.basis_x[a], by = L.basis_y[a], bz = L.basis_z[a];
+            const double nx = std::sqrt((double)L.basis_x[0] * L.basis_x[0] + (double)L.basis_x[1] * L.basis_x[1]
+                                        + (double)L.basis_x[2] * L.basis_x[2]);
+            switch (L.type) {
+                case RT_ALIGHT_SPHERE: ext[a] = (double)L.radius * nx; break;
+                case RT_ALIGHT_DISK: ext[a] = (double)L.radius * std::sqrt(bx * bx + by * by); break;
+                case RT_ALIGHT_RECT:
+                    ext[a] = 0.5 * (double)L.width * std::fabs(bx) + 0.5 * (double)L.height * std::fabs(by);
+                    break;
+                default:
+                    ext[a] = (double)L.radius * std::sqrt(bx * bx + by * by) + 0.5 * (double)L.length * std::fabs(bz);
+                    break;
+            }
+        }
+        Item it;
+        for (int a = 0; a < 3; ++a) {
+            const double c = L.position[a];
+            const double pad = 1e-3 * (std::fabs(c) + ext[a]) + 1e-4;
+            it.lo[a] = c - ext[a] - pad;
+            it.hi[a] = c + ext[a] + pad;
+            it.c[a] = c;
+        }
+        it.index = i;
+        items.push_back(it);
+    }
+    if (items.empty()) return;
+    order.resize(items.size());
+    // depth-first emission: a node's left child follows it, its skip index
+    // (lo.w) is the first node after its subtree; hi.w holds a leaf's light
+    // count in the top 4 bits (0 for an inner node) and its first slot in
+    // the low 28
+    std::function<void(int, int)> emit = [&](int begin, int end) {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        double clo[3] = {1e300, 1e300, 1e300}, chi[3] = {-1e300, -1e300, -1e300};
+        for (int k = begin; k < end; ++k)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], items[(size_t)k].lo[a]);
+                hi[a] = std::max(hi[a], items[(size_t)k].hi[a]);
+                clo[a] = std::min(clo[a], items[(size_t)k].c[a]);
+                chi[a] = std::max(chi[a], items[(size_t)k].c[a]);
+            }
+        float flo[3], fhi[3];   // rounded outward
+        for (int a = 0; a < 3; ++a) {
+            flo[a] = std::nextafter((float)lo[a], -INFINITY);
+            fhi[a] = std::nextafter((float)hi[a], INFINITY);
+        }
+        const size_t node = nodes.size() / 2;
+        nodes.push_back(make_float4(flo[0], flo[1], flo[2], 0.0f));
+        nodes.push_back(make_float4(fhi[0], fhi[1], fhi[2], 0.0f));
+        const int n = end - begin;
+        if (n <= 4) {
+            for (int k = begin; k < end; ++k) order[(size_t)k] = items[(size_t)k].index;
+            nodes[2 * node + 1].w = bits((int32_t)(((uint32_t)n << 28) | (uint32_t)begin));
+        } else {
+            int axis = 0;
+            for (int a = 1; a < 3; ++a)
+                if (chi[a] - clo[a] > chi[axis] - clo[axis]) axis = a;
+            const int mid = begin + n / 2;
+            std::nth_element(items.begin() + begin, items.begin() + mid, items.begin() + end,
+                [axis](const Item& x, const Item& y) {
+                    return x.c[axis] < y.c[axis] || (x.c[axis] == y.c[axis] && x.index < y.index);
+                });
+            emit(begin, mid);
+            emit(mid, end);
+        }
+        nodes[2 * node].w = bits((int32_t)(nodes.size() / 2));   // skip: past the subtree
+    };
+    emit(0, (int)items.size());
+}
+
 // scene arrays and counts (rt_scene_upload / rt_scene_update)
 static rt_status scene_validate(const rt_scene_desc* d) {
     RT_REQUIRE(d->sphere_count >= 0 && d->quad_count >= 0 && d->triangle_count >= 0
@@ -599,6 +699,12 @@ static rt_status scene_fill(rt_scene* s, const rt_scene_desc* d) {
     if (e == hipSuccess) e = upload(s->a_cdf, acdf.data(), acdf.size(), st);
     if (e == hipSuccess) e = upload(s->a_infinite, ainf.data(), ainf.size(), st);
     s->a_infinite_count = (int32_t)ainf.size();
+    std::vector<float4> anodes;
+    std::vector<int32_t> aorder;
+    analytic_light_bvh(d, anodes, aorder);
+    if (e == hipSuccess) e = upload(s->a_nodes, anodes.data(), anodes.size(), st);
+    if (e == hipSuccess) e = upload(s->a_order, aorder.data(), aorder.size(), st);
+    s->a_node_count = (int32_t)(anodes.size() / 2);
     s->texture_count = d->texture_count;
     s->texel_count = d->image_texel_count;
     s->has_albedo_textures = d->material_albedo_texture != nullptr && d->material_count > 0;

@@ -58,6 +58,14 @@ struct SceneView {
     const float* a_cdf;   // analytic light CDF
     const int32_t* a_infinite;  // dome / distant light indices, ascending
     int32_t a_infinite_count;
+    // binary BVH over the finite, non-point analytic lights in depth-first
+    // order: node k is a_nodes[2k] = (lo, skip: the first node after its
+    // subtree) and a_nodes[2k + 1] = (hi, light count << 28 | first slot;
+    // count 0 for an inner node, whose left child is node k + 1); leaves
+    // list lights through a_order
+    const float4* a_nodes;
+    const int32_t* a_order;
+    int32_t a_node_count;
     const rt_texture* textures;
     const float* texels;               // 3 floats per texel
     const int32_t* mat_albedo_tex;     // nullptr: every albedo is a constant

@@ -447,19 +447,56 @@ __device__ inline bool alight_hit(const rt_analytic_light& L, f3 o, f3 d, float 
     return true;
 }
 
-// intersect_analytic_lights (programs.cu:795-813): nearest front-facing hit
-// of the finite lights that are not points, lowest index on equal distance
+// intersect_analytic_lights (programs.cu:795-813): the nearest front-facing
+// hit of the finite lights that are not points, lowest index on equal
+// distance.  The reference walks every light with a shrinking range; a
+// light's hit distance does not depend on the range's end as long as it
+// lies inside, so the walk's answer is the smallest (t, index) over the
+// lights hit in (tmin, tmax), which a walk of the light BVH finds in any
+// order (each light tested against the full range, boxes culled only when
+// they start beyond the best hit).
+__device__ __forceinline__ float light_inv(float x) {
+    return 1.0f / (fabsf(x) > 1e-20f ? x : copysignf(1e-20f, x));
+}
+
 __device__ inline int intersect_analytic(const SceneView& s, f3 o, f3 d, float tmin, float tmax) {
-    float closest = tmax;
+    if (s.a_node_count <= 0) return -1;
+    const f3 inv = mk3(light_inv(d.x), light_inv(d.y), light_inv(d.z));
+    float best = tmax;
     int found = -1;
-    for (int i = 0; i < s.analytic_count; ++i) {
-        const rt_analytic_light L = s.analytic[i];
-        if (!alight_finite(L.type) || L.treat_as_point != 0) continue;
-        float t;
-        f3 p, n;
-        if (!alight_hit(L, o, d, tmin, closest, &t, &p, &n)) continue;
-        closest = t;
-        found = i;
+    // stackless depth-first walk: descend into a box the ray meets before
+    // the best hit, otherwise (and after a leaf) jump past the subtree
+    int k = 0;
+    while (k < s.a_node_count) {
+        const float4 lo = s.a_nodes[2 * k], hi = s.a_nodes[2 * k + 1];
+        const float tx0 = (lo.x - o.x) * inv.x, tx1 = (hi.x - o.x) * inv.x;
+        const float ty0 = (lo.y - o.y) * inv.y, ty1 = (hi.y - o.y) * inv.y;
+        const float tz0 = (lo.z - o.z) * inv.z, tz1 = (hi.z - o.z) * inv.z;
+        const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+        const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+        const int skip = __float_as_int(lo.w);
+        if (tnear > tfar || tnear > best) {
+            k = skip;
+            continue;
+        }
+        const uint32_t leaf = __float_as_uint(hi.w);
+        const int count = (int)(leaf >> 28);
+        if (count == 0) {
+            ++k;
+            continue;
+        }
+        const int first = (int)(leaf & 0x0FFFFFFFu);
+        for (int j = first; j < first + count; ++j) {
+            const int i = s.a_order[j];
+            float t;
+            f3 p, n;
+            if (!alight_hit(s.analytic[i], o, d, tmin, tmax, &t, &p, &n)) continue;
+            if (t < best || (t == best && i < found)) {
+                best = t;
+                found = i;
+            }
+        }
+        k = skip;
     }
     return found;
 }
