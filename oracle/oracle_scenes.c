@@ -294,7 +294,7 @@ static int gen_cornell_mesh(rto_generated* g, uint64_t seed) {
     return 0;
 }
 
-/* config 4: Cornell room with 1-8 displaced icospheres (subdivision 4),
+/* cornell_random (id 4): Cornell room with 1-8 displaced icospheres (subdivision 4),
  * one random OpenPBR material each, a random ceiling light (see the
  * library's generate_cornell_random for the draw order) and four cameras on
  * an arc in front of the open side */
@@ -426,7 +426,8 @@ static void instanced_rig_sized(uint64_t seed, int frame, int frames, int wp, in
     }
 }
 
-/* config 2: 2 pinhole 640x480 + 2 equi 1024x1024; config 3: all four 1280x720 */
+/* instanced_mesh_restir (id 2): 2 pinhole 640x480 + 2 equi 1024x1024;
+ * radiance_cache_realtime (id 3): all four 1280x720 */
 static void instanced_rig(uint64_t seed, int frame, int frames, rto_camera* cams) {
     instanced_rig_sized(seed, frame, frames, 640, 480, 1024, 1024, cams);
 }
@@ -597,7 +598,7 @@ static void finish_builtin(rto_generated* g, rto_sphere* sp, int ns, rto_quad* q
     profile_defaults(&g->profile, 16, 8);
 }
 
-/* config 5: the reference's builtin "smoke" scene (shared_scene_builders.cpp:475-487)
+/* smoke (id 5): the reference's builtin "smoke" scene (shared_scene_builders.cpp:475-487)
  * and view (body at (0, -2.5, 0.25), yaw 0, fx = fy = 360) */
 static int gen_smoke(rto_generated* g) {
     rto_material* mt = builtin_materials(2);
@@ -613,7 +614,7 @@ static int gen_smoke(rto_generated* g) {
     return 0;
 }
 
-/* config 6: the reference's builtin "final_room" (shared_scene_builders.cpp:489-550)
+/* final_room (id 6): the reference's builtin "final_room" (shared_scene_builders.cpp:489-550)
  * and view (default spawn pose (0, -0.8, 0.35), fx = 480, fy = 360) */
 static int gen_final_room(rto_generated* g) {
     rto_material* mt = builtin_materials(5);
@@ -657,7 +658,7 @@ int rto_generate_scene(int config, uint64_t seed, rto_generated* g) {
         case 5: return gen_smoke(g);
         case 6: return gen_final_room(g);
         case 3: {
-            /* the config-2 scene, four 1280x720 cameras, paths of depth 8 with
+            /* the instanced_mesh_restir scene, four 1280x720 cameras, paths of depth 8 with
              * Russian roulette from bounce 3 (the radiance-cache workload) */
             const int rc = gen_instanced_mesh(g, seed);
             if (rc != 0) return rc;

@@ -223,7 +223,7 @@ void rto_unproject(const rto_camera* cam, double px, double py, float dir_cam[3]
 void rto_camera_ray(const rto_camera* cam, double px, double py, float origin[3], float dir[3]);
 void rto_project(const rto_camera* cam, const float dir_cam[3], float pixel[2]);
 void rto_make_equi_lut(rto_camera* cam);
-/* the orbiting rig of the instanced scene (config 2) at frame `frame` of `frames` */
+/* the orbiting rig of the instanced scene (generator id 2 or 3) at frame `frame` of `frames` */
 int rto_generate_rig(int config, uint64_t seed, int frame, int frames, rto_camera* cams, int* count);
 
 /* ---- learned denoiser (oracle_nn.c) */

@@ -12,7 +12,7 @@
 //   2 instanced_mesh_restir : 1000 rigid instances of a 1024-triangle mesh
 //                             in a 100-unit cube, 256 analytic sphere lights;
 //                             2 pinhole32 640x480 + 2 equi62_lut1d 1024x1024
-//   3 radiance_cache_realtime : the config-2 scene with 2 pinhole32 + 2
+//   3 radiance_cache_realtime : the instanced_mesh_restir scene with 2 pinhole32 + 2
 //                             equi62_lut1d cameras at 1280x720, depth 8
 //   4 cornell_random        : seeded Cornell rooms with 1-8 spheres, random
 //                             materials and light (denoiser training scenes)
@@ -292,9 +292,9 @@ void generate_cornell_mesh(uint64_t seed, rt_generated_scene* g) {
     g->profile = default_profile(16, 8);
 }
 
-// config 4: the Cornell room of config 1 with 1-8 displaced icospheres (4
+// cornell_random (id 4): the Cornell room of cornell_mesh with 1-8 displaced icospheres (4
 // subdivisions, 5,120 triangles each), one random constant OpenPBR material
-// per sphere (drawn as in config 1), a random ceiling light and four
+// per sphere (drawn as in cornell_mesh), a random ceiling light and four
 // cameras in front of the open side.  Draw order:
 // light width, depth, x, z, radiance; sphere count; per sphere radius,
 // centre x, y, z, base colour, roughness, metalness, coat, then one
@@ -814,7 +814,7 @@ bool generate_scene(int config, uint64_t seed, rt_generated_scene* g) {
         case 1: generate_cornell_mesh(seed, g); return true;
         case 2: generate_instanced_mesh(seed, g); return true;
         case 3:
-            // the config-2 scene, four 1280x720 cameras, paths of depth 8 with
+            // the instanced_mesh_restir scene, four 1280x720 cameras, paths of depth 8 with
             // Russian roulette from bounce 3 (the radiance-cache workload)
             generate_instanced_mesh(seed, g);
             instanced_rig(seed, 0, 60, g->cameras, 1280, 720, 1280, 720);

@@ -3,7 +3,7 @@ encoder-decoder of rt_nn.h, 565,443 parameters).
 
 Training pairs come from this path tracer on seeded procedural scenes:
 - the Cornell rooms with 1-8 random spheres, materials and light of the
-  scene generator's config 4 (`cornell_random`, seeds 0..N-1), plus a few
+  scene generator's `cornell_random` scenes (seeds 0..N-1), plus a few
   seeds of the instanced ReSTIR scene and the random-sphere scene the same
   network denoises in the realtime loop;
 - inputs are 1-spp frames with their albedo / normal / depth buffers, taken
