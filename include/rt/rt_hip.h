@@ -415,6 +415,13 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
 rt_status rt_render_frame_streams(rt_renderer* r, const rt_camera_rig* rig,
     const rt_profile* profile, const uint32_t* sample_streams, const uint32_t* camera_keys,
     rt_frame_timing* timing);
+/* render_direction_debug_frame / OptixRenderer::render_direction_debug
+ * (direction_debug_renderer.cpp:51-66, programs.cu:1807-1826): the world
+ * direction of camera camera_index's ray through each pixel centre, encoded
+ * as RGBA8 (255 * clamp01(0.5 (d + 1)) truncated per axis, alpha 255) into
+ * `rgba` (width * height * 4 bytes of host memory); waits for the device */
+rt_status rt_render_direction_debug(rt_device* dev, const rt_camera_rig* rig, int camera_index,
+    uint8_t* rgba);
 /* LaunchParameterDiagnostics (optix_renderer.h:50-53, optix_renderer.cpp:
  * 137,217-219,257): the per-frame parameter block is allocated once per
  * renderer (allocation_count 1) and uploaded once per frame launched */

@@ -327,6 +327,7 @@ def lib():
         L.rto_restir_merge.argtypes = [P(Reservoir), P(Reservoir), C.c_float, C.c_int, C.c_float,
                                        C.c_int, P(C.c_int)]
         L.rto_analytic_lights_compile.argtypes = [C.c_void_p, C.c_int]
+        L.rto_direction_debug.argtypes = [P(Camera), C.c_void_p]
         L.rto_analytic_lights_compile.restype = C.c_int
         L.rto_cache_mlp_param_count.argtypes = []
         L.rto_cache_mlp_param_count.restype = C.c_int

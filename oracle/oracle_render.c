@@ -136,6 +136,23 @@ void rto_camera_ray(const rto_camera* cam, double px, double py, float origin[3]
     dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
 }
 
+/* direction_debug_kernel / encode_direction_channel (programs.cu:180-182,
+ * 1807-1826): RGBA8 world directions of the pixel centres' rays */
+void rto_direction_debug(const rto_camera* cam, uint8_t* rgba) {
+    for (int y = 0; y < cam->height; ++y)
+        for (int x = 0; x < cam->width; ++x) {
+            float o[3], d[3];
+            rto_camera_ray(cam, (double)((float)x + 0.5f), (double)((float)y + 0.5f), o, d);
+            uint8_t* px = rgba + 4 * ((size_t)y * (size_t)cam->width + (size_t)x);
+            for (int a = 0; a < 3; ++a) {
+                float t = 0.5f * (d[a] + 1.0f);
+                t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+                px[a] = (uint8_t)(255.0f * t);
+            }
+            px[3] = 255;
+        }
+}
+
 /* programs.cu:375-473 (projection, used by reprojection) */
 void rto_project(const rto_camera* c, const float dir_cam[3], float pixel[2]) {
     if (c->model == RTO_CAM_EQUI62_LUT1D) {

@@ -222,6 +222,9 @@ void rto_brute_closest(const rto_scene* scene, const rto_ray* rays, int n, rto_h
 void rto_unproject(const rto_camera* cam, double px, double py, float dir_cam[3]);
 void rto_camera_ray(const rto_camera* cam, double px, double py, float origin[3], float dir[3]);
 void rto_project(const rto_camera* cam, const float dir_cam[3], float pixel[2]);
+/* the camera-model debug image: width * height * 4 bytes, RGBA8 encoded
+ * world directions of the pixel centres' rays (programs.cu:1807-1826) */
+void rto_direction_debug(const rto_camera* cam, uint8_t* rgba);
 void rto_make_equi_lut(rto_camera* cam);
 /* the orbiting rig of the instanced scene (generator id 2 or 3) at frame `frame` of `frames` */
 int rto_generate_rig(int config, uint64_t seed, int frame, int frames, rto_camera* cams, int* count);

@@ -29,6 +29,7 @@ EXPORTED = [
     "rt_bvh_build", "rt_bvh_download", "rt_scene_update", "rt_scene_last_update", "rt_trace_closest", "rt_trace_any",
     "rt_trace_closest_device", "rt_trace_closest_device_noalloc", "rt_renderer_create", "rt_renderer_destroy", "rt_render_frame",
     "rt_render_frame_streams", "rt_renderer_launch_diagnostics", "rt_analytic_lights_compile",
+    "rt_render_direction_debug",
     "rt_frame_download", "rt_frame_device_view", "rt_reset_accumulation", "rt_reset_sequence",
     "rt_restir_diagnostics", "rt_renderer_set_tuning", "rt_renderer_get_tuning",
     "rt_scene_generate", "rt_scene_generated_free", "rt_restir_download",
@@ -352,6 +353,16 @@ class Presenter:
         out = np.empty((self.height, self.width, 4), np.uint8)
         check(lib().rt_presenter_download(self.h, ptr(out)))
         return out
+
+
+def direction_debug(device: "Device", rig: CameraRig, camera: int = 0) -> np.ndarray:
+    """rt_render_direction_debug: (height, width, 4) uint8 encoded world
+    directions of the camera's pixel-centre rays."""
+    cam = rig.cameras[camera]
+    out = np.zeros((max(cam.height, 0), max(cam.width, 0), 4), np.uint8)
+    check(lib().rt_render_direction_debug(device.h, C.byref(rig), camera,
+                                          ptr(out) if out.size else None))
+    return out
 
 
 def compile_analytic_lights(lights) -> None:

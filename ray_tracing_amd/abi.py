@@ -252,6 +252,7 @@ def declare(L):
                                           P(C.c_uint32), P(FrameTiming)]
     L.rt_renderer_launch_diagnostics.argtypes = [vp, P(C.c_uint64), P(C.c_uint64)]
     L.rt_analytic_lights_compile.argtypes = [vp, C.c_int32]
+    L.rt_render_direction_debug.argtypes = [vp, P(CameraRig), C.c_int, vp]
     L.rt_frame_download.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     L.rt_frame_device_view.argtypes = [vp, C.c_int, P(DeviceFrameView)]
     L.rt_presenter_create.argtypes = [vp, C.c_int32, C.c_int32, P(vp)]

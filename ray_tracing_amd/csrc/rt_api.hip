@@ -1823,6 +1823,35 @@ static rt_status render_frame_impl(rt_renderer* r, const rt_camera_rig* rig, con
     return RT_OK;
 }
 
+// render_direction_debug_frame (direction_debug_renderer.cpp:51-66)
+rt_status rt_render_direction_debug(rt_device* dev, const rt_camera_rig* rig, int camera_index,
+    uint8_t* rgba) {
+    RT_REQUIRE(dev && rig && rgba, RT_ERROR_INVALID_ARGUMENT, "null device, rig or output");
+    RT_REQUIRE(rig->active_count >= 1 && rig->active_count <= 4, RT_ERROR_INVALID_ARGUMENT,
+        "rig.active_count must be in [1, 4]");
+    RT_REQUIRE(camera_index >= 0 && camera_index < rig->active_count, RT_ERROR_INVALID_ARGUMENT,
+        "camera_index " + std::to_string(camera_index) + " out of range");
+    const rt_camera& cam = rig->cameras[camera_index];
+    RT_REQUIRE(cam.enabled != 0 && cam.width > 0 && cam.height > 0, RT_ERROR_INVALID_ARGUMENT,
+        "camera " + std::to_string(camera_index) + " is disabled or has no pixels");
+    RT_REQUIRE(cam.model == RT_CAM_PINHOLE32 || cam.model == RT_CAM_EQUI62_LUT1D, RT_ERROR_INVALID_ARGUMENT,
+        "camera " + std::to_string(camera_index) + " has an unknown model");
+    RT_REQUIRE((int64_t)cam.width * cam.height <= (int64_t(1) << 28), RT_ERROR_INVALID_ARGUMENT,
+        "camera " + std::to_string(camera_index) + " exceeds 2^28 pixels");
+    RT_HIP(hipSetDevice(dev->ordinal));
+    hipStream_t st = dev->stream;
+    const size_t n = (size_t)cam.width * (size_t)cam.height;
+    DevBuf<rt_camera> dcam;
+    DevBuf<uint32_t> out;
+    RT_HIP(dcam.reserve(1));
+    RT_HIP(out.reserve(n));
+    RT_HIP(hipMemcpyAsync(dcam.p, &cam, sizeof(rt_camera), hipMemcpyHostToDevice, st));
+    RT_HIP(rtk::wf_direction_debug(dcam.p, out.p, cam.width, cam.height, st));
+    RT_HIP(hipMemcpyAsync(rgba, out.p, n * 4, hipMemcpyDeviceToHost, st));
+    RT_HIP(hipStreamSynchronize(st));
+    return RT_OK;
+}
+
 rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_profile* p,
     uint32_t sample_stream, rt_frame_timing* timing) {
     const uint32_t streams[4] = {sample_stream, sample_stream, sample_stream, sample_stream};

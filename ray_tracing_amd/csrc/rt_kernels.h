@@ -121,6 +121,10 @@ struct StageLaunch {
 };
 
 hipError_t wf_reset_counters(uint32_t* counters, hipStream_t st);
+// the camera-model debug image: RGBA8 encoded world directions of the pixel
+// centres' rays (width x height words; the camera in device memory)
+hipError_t wf_direction_debug(const rt_camera* cam, uint32_t* rgba, int width, int height,
+    hipStream_t st);
 hipError_t wf_raygen(const CameraView& cams, const FrameParams& fp, WavefrontBuffers& b,
     int paths, hipStream_t st);
 // one traversal launch: the extension rays of `bounce` (when `extension`)

@@ -276,6 +276,34 @@ __global__ __launch_bounds__(256) void k_tone_map_rgba8(const float4* __restrict
 
 }  // namespace
 
+namespace {
+
+// direction_debug_kernel (programs.cu:1807-1826): the world direction of each
+// pixel centre's camera ray, one byte per axis, 255 * clamp01(0.5 (v + 1))
+// truncated (encode_direction_channel, programs.cu:180-182), alpha 255
+__global__ void k_direction_debug(const rt_camera* cam, uint32_t* rgba, int width, int height) {
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (x >= width || y >= height) return;
+    f3 o, d;
+    camera_ray(*cam, (double)((float)x + 0.5f), (double)((float)y + 0.5f), &o, &d);
+    auto enc = [](float v) -> uint32_t {
+        const float t = 0.5f * (v + 1.0f);
+        return (uint32_t)(255.0f * (t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t)));
+    };
+    rgba[(size_t)y * (size_t)width + (size_t)x] = enc(d.x) | (enc(d.y) << 8) | (enc(d.z) << 16) | (255u << 24);
+}
+
+}  // namespace
+
+hipError_t wf_direction_debug(const rt_camera* cam, uint32_t* rgba, int width, int height,
+    hipStream_t st) {
+    const dim3 block(16, 16);
+    const dim3 grid((unsigned)((width + 15) / 16), (unsigned)((height + 15) / 16));
+    k_direction_debug<<<grid, block, 0, st>>>(cam, rgba, width, height);
+    return hipGetLastError();
+}
+
 hipError_t wf_tone_map_rgba8(const float4* src, uint32_t* dst, int64_t n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     const int64_t blocks = (n + kToneMapTile - 1) / kToneMapTile;

@@ -229,6 +229,16 @@ Renderer::~Renderer() {
     if (device_) rt_device_destroy(device_);
 }
 
+DirectionDebugFrame Renderer::render_direction_debug(const rt_camera_rig& rig, int camera_index) {
+    validate_render_camera_request(rig, camera_index, "render_radiance");
+    DirectionDebugFrame f;
+    f.width = rig.cameras[camera_index].width;
+    f.height = rig.cameras[camera_index].height;
+    f.rgba.assign((size_t)f.width * (size_t)f.height * 4, 0);
+    throw_on_error(rt_render_direction_debug(device_, &rig, camera_index, f.rgba.data()));
+    return f;
+}
+
 // OptixRenderer::upload_scene / SharedGpuSceneState::prepare
 // (optix_renderer.cpp:76-90,154-158): the first scene is uploaded and built,
 // later ones go through the acceleration lifecycle

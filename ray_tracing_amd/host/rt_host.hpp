@@ -77,6 +77,13 @@ struct RadianceFrame {
     double average_luminance = 0.0;
 };
 
+// DirectionDebugFrame (src/realtime/gpu/frame_types.h:8-12)
+struct DirectionDebugFrame {
+    int width = 0;
+    int height = 0;
+    std::vector<std::uint8_t> rgba;
+};
+
 // RadianceTiming / ProfiledRadianceFrame / ProfiledDeviceRadianceFrame
 // (src/realtime/gpu/optix_renderer.h:26-38)
 struct RadianceTiming {
@@ -157,6 +164,9 @@ public:
     Renderer(Renderer&&) = delete;
     Renderer& operator=(Renderer&&) = delete;
 
+    // the camera-model debug image of one camera (optix_renderer.cpp:146-152);
+    // needs no scene
+    DirectionDebugFrame render_direction_debug(const rt_camera_rig& rig, int camera_index = 0);
     // upload + acceleration update (reuse / update / refit / rebuild), then
     // use_prepared_scene
     void prepare_scene(const rt_scene_desc& scene);

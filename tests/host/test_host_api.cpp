@@ -38,6 +38,8 @@ static_assert(std::is_same_v<decltype(&Renderer::render_radiance_profiled),
     ProfiledRadianceFrame (Renderer::*)(const rt_scene_desc&, const rt_camera_rig&, const RenderProfile&, int)>);
 static_assert(std::is_same_v<decltype(&Renderer::render_radiance),
     RadianceFrame (Renderer::*)(const rt_scene_desc&, const rt_camera_rig&, const RenderProfile&, int)>);
+static_assert(std::is_same_v<decltype(&Renderer::render_direction_debug),
+    DirectionDebugFrame (Renderer::*)(const rt_camera_rig&, int)>);
 static_assert(std::is_same_v<decltype(&Renderer::launch_parameter_diagnostics),
     LaunchParameterDiagnostics (Renderer::*)() const>);
 static_assert(std::is_same_v<decltype(&Renderer::acceleration_diagnostics),

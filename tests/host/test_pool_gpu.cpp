@@ -196,6 +196,22 @@ int main() {
         expect(!same_frame(l9.frame, r[3].profiled.frame), "another stream draws other samples");
     }
 
+    std::printf("[renderer] direction debug\n");
+    std::fflush(stdout);
+    {
+        // tests/test_direction_debug_renderer.cpp: size and alpha; the
+        // directions themselves are checked against the oracle elsewhere
+        Renderer renderer;
+        const DirectionDebugFrame f = renderer.render_direction_debug(rig4, 2);
+        expect(f.width == 32 && f.height == 32 && f.rgba.size() == 32u * 32u * 4u, "direction debug size");
+        expect(f.rgba[3] == 255, "direction debug alpha");
+        // the centre pixel of a camera looking down -z encodes (0, 0, -1)
+        const size_t c = (16u * 32u + 16u) * 4u;
+        expect(f.rgba[c + 2] <= 1, "direction debug centre z");
+        expect(throws([&] { renderer.render_direction_debug(rig1, 1); }, "camera_index out of range"),
+            "direction debug validates the camera index");
+    }
+
     std::printf("[renderer] prepared scene lifecycle\n");
     std::fflush(stdout);
     {

@@ -217,3 +217,49 @@ def test_cache_frames_replay_in_a_graph(rt, device):
             assert rel < 1e-4, (f, rel)
         se, sg = ce.stats(), cg.stats()
         assert se["usable"] == sg["usable"] and se["queries"] == sg["queries"]
+
+
+def test_cache_guard_falls_back_to_the_plain_tracer(rt, device):
+    """The learned part's fallback for the cache: a cache whose running
+    relative loss stays above max_rel_loss never ends a path, so the frames
+    equal the plain tracer's bit for bit; a usable cache given NaN
+    parameters (set_params restarts its training state, so it must earn
+    usability again) reports a NaN loss, never becomes usable under a
+    threshold it would otherwise pass, and the frames are plain."""
+    say("[gpu] cache guard: loss over budget, diverged parameters")
+    g, sc, prof = spheres_scene(rt, device)
+    plain = rt.Renderer(device, sc)
+    strict = rt.Renderer(device, sc)
+    strict.set_radiance_cache(rt.RadianceCache(0, train_samples=8192, warmup_frames=1,
+                                               max_rel_loss=1e-9))
+    for f in range(4):
+        plain.render(g.rig(), prof, 500 + f)
+        strict.render(g.rig(), prof, 500 + f)
+        st = strict._cache.stats()
+        assert st["usable"] == 0 and st["queries"] == 0 and st["frames"] == f + 1
+        assert strict.download(0)["beauty"].tobytes() == plain.download(0)["beauty"].tobytes(), f
+
+    ren = rt.Renderer(device, sc)
+    cache = rt.RadianceCache(0, train_samples=8192, warmup_frames=1, max_rel_loss=1e9)
+    ren.set_radiance_cache(cache)
+    for f in range(3):
+        ren.render(g.rig(), prof, 600 + f)
+    assert cache.stats()["usable"] == 1 and cache.stats()["queries"] > 0
+    p = cache.params()
+    p[:] = np.nan
+    cache.set_params(p)
+    ren.render(g.rig(), prof, 603)            # trains on the NaN cache: a NaN loss
+    st = cache.stats()
+    say(f"  after NaN parameters: {st}")
+    assert st["usable"] == 0 and not np.isfinite(st["last_loss"])
+    # both start a fresh history (the NaN frame is in this renderer's)
+    ren.reset_accumulation()
+    plain.reset_accumulation()
+    for f in range(4, 6):
+        ren.render(g.rig(), prof, 600 + f)
+        plain.render(g.rig(), prof, 600 + f)
+        st = cache.stats()
+        assert st["usable"] == 0 and st["queries"] == 0
+        # the same paths as the plain tracer: training paths go on with the
+        # plain integrator
+        assert ren.download(0)["beauty"].tobytes() == plain.download(0)["beauty"].tobytes(), f

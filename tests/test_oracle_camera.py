@@ -16,6 +16,7 @@ The oracle returns FP32 camera-space directions (as the GPU does), so the
 reference's FP64 tolerances (1e-12) become FP32 ones (about 2e-7 per
 component) wherever the reference's own tolerance is tighter than that.
 """
+import ctypes as C
 import math
 
 import numpy as np
@@ -215,3 +216,70 @@ def test_default_equi_focal_length_of_the_generated_rigs():
             f = 0.5 * width / (120.0 * math.pi / 360.0)
             assert abs(c.fx - f) < 1e-9 and abs(c.fy - f) < 1e-9
             assert c.cx == 0.5 * c.width and c.cy == 0.5 * c.height
+
+
+# ------------------------------------------------------- direction debug image
+def contract_rig():
+    """The reference's camera contract rig (tests/camera_contract_fixtures.h):
+    a distorted pinhole turned 0.35 rad about y at (1, -2, 3.5) and an equi
+    camera turned -0.2 rad about x at (-1.5, 0.75, 2.25), both 64 x 48."""
+    def rot(axis, a):
+        c, s = math.cos(a), math.sin(a)
+        if axis == "y":
+            return np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+        return np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
+    pin = O.pinhole_camera(52.0, 50.0, 31.5, 23.5, 0.02, -0.01, 0.003, 0.001, -0.0015, width=64, height=48)
+    eq = O.equi_camera(64, 48, 28.0, 29.0, 31.5, 23.5,
+                       radial=(0.01, -0.003, 0.0008, -0.0002, 0.00005, -0.00001),
+                       tangential=(0.0007, -0.0005))
+    rots = (rot("y", 0.35), rot("x", -0.2))
+    for cam, R, t in ((pin, rots[0], (1.0, -2.0, 3.5)), (eq, rots[1], (-1.5, 0.75, 2.25))):
+        cam.origin[:] = t
+        cam.basis_x[:], cam.basis_y[:], cam.basis_z[:] = tuple(R[:, 0]), tuple(R[:, 1]), tuple(R[:, 2])
+    return (pin, eq), rots
+
+
+def solve_forward(forward, pixel):
+    """Normalized coordinates (x, y) that the forward model maps to `pixel`."""
+    from scipy.optimize import least_squares
+    r = least_squares(lambda v: forward(v) - np.asarray(pixel), x0=np.zeros(2), xtol=1e-15,
+                      ftol=1e-15, gtol=1e-15)
+    assert np.abs(forward(r.x) - np.asarray(pixel)).max() < 1e-9
+    return r.x
+
+
+def encode(d):
+    return np.floor(255.0 * np.clip(0.5 * (np.asarray(d) + 1.0), 0.0, 1.0)).astype(int)
+
+
+def test_direction_debug_matches_the_camera_contract():
+    """The reference's direction-debug contract (tests/test_optix_direction.cpp:
+    61-125): the image's RGB at sample pixels is the encoded world direction
+    of the forward model's inverse within 2 codes, alpha is 255, and the equi
+    sample differs from the pinhole sample by at least 12 codes."""
+    (pin, eq), (Rp, Re) = contract_rig()
+    imgs = []
+    for cam in (pin, eq):
+        img = np.zeros((cam.height, cam.width, 4), np.uint8)
+        O.lib().rto_direction_debug(C.byref(cam), img.ctypes.data)
+        assert (img[..., 3] == 255).all()
+        imgs.append(img)
+
+    def fwd_pin(v):
+        xd = brown_conrady(np.asarray(v), 0.02, -0.01, 0.003, 0.001, -0.0015)
+        return np.array([52.0 * xd[0] + 31.5, 50.0 * xd[1] + 23.5])
+
+    def fwd_eq(v):
+        return equi_forward(np.array([v[0], v[1], 1.0]), 28.0, 29.0, 31.5, 23.5,
+                            (0.01, -0.003, 0.0008, -0.0002, 0.00005, -0.00001), (0.0007, -0.0005))
+
+    for img, fwd, R, samples in ((imgs[0], fwd_pin, Rp, ((31.5, 23.5), (17.5, 11.5), (55.5, 38.5))),
+                                 (imgs[1], fwd_eq, Re, ((31.5, 23.5), (45.5, 14.5), (58.5, 40.5)))):
+        for px, py in samples:
+            x, y = solve_forward(fwd, (px, py))
+            world = R @ normalized((x, y, 1.0))
+            got = img[int(py), int(px), :3].astype(int)
+            assert np.abs(got - encode(world)).max() <= 2, (px, py, got, encode(world))
+    a = imgs[1][14, 45, :3].astype(int)
+    b = imgs[0][11, 17, :3].astype(int)
+    assert np.abs(a - b).max() >= 12
