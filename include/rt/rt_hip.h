@@ -124,6 +124,30 @@ typedef struct rt_analytic_light {
     int32_t treat_as_line;
 } rt_analytic_light;
 
+/* PackedMedium (src/realtime/gpu/launch_params.h:201-214), same field order:
+ * a homogeneous medium of the given density inside a sphere (boundary_type
+ * 0: local_center_or_min + radius) or a box (1: local_center_or_min /
+ * local_max), in the medium's frame: local = R (p - translation) with R's
+ * rows rotation_row0..2.  A closest-hit ray that crosses it scatters at an
+ * exponentially distributed free-flight distance, with the medium's material
+ * (normally RT_MAT_ISOTROPIC); shadow rays pass through media, as in the
+ * reference (is_occluded, programs.cu:1759-1762).  At most 60 media. */
+enum { RT_MEDIUM_SPHERE = 0, RT_MEDIUM_BOX = 1 };
+typedef struct rt_medium {
+    float local_center_or_min[3];
+    float radius;
+    float local_max[3];
+    float density;
+    float rotation_row0[3];
+    int32_t material_index;
+    float rotation_row1[3];
+    int32_t boundary_type;
+    float rotation_row2[3];
+    float pad0;
+    float translation[3];
+    float pad1;
+} rt_medium;
+
 enum { RT_BG_CONSTANT = 0, RT_BG_SKY_GRADIENT = 1 };
 
 typedef struct rt_scene_desc {
@@ -156,6 +180,10 @@ typedef struct rt_scene_desc {
      * PackedTriangle::n0..n2 with has_vertex_normals), or NULL; a NaN first
      * float means none.  They set the shading normal (programs.cu:1535-1545). */
     const float* triangle_normals;
+    /* homogeneous media (PackedScene::media), or none */
+    int32_t medium_count;
+    int32_t pad_media;
+    const rt_medium* media;
 } rt_scene_desc;
 
 /* -------------------------------------------------------------- cameras

@@ -77,7 +77,8 @@ static inline void om_rng4(uint32_t pixel, uint32_t sample, uint32_t camera, uin
     out[0] = om_u01(v[0]); out[1] = om_u01(v[1]); out[2] = om_u01(v[2]); out[3] = om_u01(v[3]);
 }
 /* RNG groups within one bounce */
-enum { OM_G_LIGHT = 0, OM_G_BSDF = 1, OM_G_ALIGHT = 2, OM_G_RESTIR0 = 8, OM_G_CAMERA = 63 };
+enum { OM_G_LIGHT = 0, OM_G_BSDF = 1, OM_G_ALIGHT = 2, OM_G_RESTIR0 = 8, OM_G_MEDIUM0 = 48,
+       OM_G_CAMERA = 63 };
 
 /* ---- sin/cos of a whole turn, own Taylor polynomials.
  * Returns sin(2*pi*u), cos(2*pi*u).  q = 4u is exact, k = nearest quadrant,
@@ -114,6 +115,40 @@ static inline float om_cbrt01(float x) {
     return y;
 }
 /* uniform direction on the unit sphere from two uniforms */
+/* the fixed-sequence natural logarithm of the GPU (Cephes logf method:
+ * x = m 2^e, m in [sqrt(1/2), sqrt(2)), degree-9 polynomial in m - 1) */
+static inline float om_log_fixed(float x) {
+    uint32_t b;
+    memcpy(&b, &x, sizeof(b));
+    int e = (int)((b >> 23) & 0xffu) - 126;
+    const uint32_t mb = (b & 0x807fffffu) | 0x3f000000u;
+    float m;
+    memcpy(&m, &mb, sizeof(m));
+    if (m < 0.70710678118654752f) {
+        e -= 1;
+        m = m + m - 1.0f;
+    } else {
+        m = m - 1.0f;
+    }
+    const float z = m * m;
+    float y = 7.0376836292e-2f;
+    y = y * m - 1.1514610310e-1f;
+    y = y * m + 1.1676998740e-1f;
+    y = y * m - 1.2420140846e-1f;
+    y = y * m + 1.4249322787e-1f;
+    y = y * m - 1.6668057665e-1f;
+    y = y * m + 2.0000714765e-1f;
+    y = y * m - 2.4999993993e-1f;
+    y = y * m + 3.3333331174e-1f;
+    y = y * m * z;
+    const float fe = (float)e;
+    y = y + -2.12194440e-4f * fe;
+    y = y + -0.5f * z;
+    float r = m + y;
+    r = r + 0.693359375f * fe;
+    return r;
+}
+
 static inline v3 om_sphere_dir(float u0, float u1) {
     const float z = 1.0f - 2.0f * u0;
     const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));

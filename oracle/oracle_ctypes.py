@@ -91,7 +91,15 @@ class Scene(C.Structure):
         ("textures", C.c_void_p), ("image_texels", C.c_void_p),
         ("material_albedo_texture", C.c_void_p), ("triangle_uvs", C.c_void_p),
         ("triangle_normals", C.c_void_p),
+        ("medium_count", C.c_int32), ("pad_media", C.c_int32), ("media", C.c_void_p),
     ]
+
+
+class Medium(C.Structure):
+    _fields_ = [("local_center_or_min", f3), ("radius", C.c_float), ("local_max", f3),
+                ("density", C.c_float), ("rotation_row0", f3), ("material_index", C.c_int32),
+                ("rotation_row1", f3), ("boundary_type", C.c_int32), ("rotation_row2", f3),
+                ("pad0", C.c_float), ("translation", f3), ("pad1", C.c_float)]
 
 
 class Camera(C.Structure):
@@ -328,6 +336,8 @@ def lib():
                                        C.c_int, P(C.c_int)]
         L.rto_analytic_lights_compile.argtypes = [C.c_void_p, C.c_int]
         L.rto_direction_debug.argtypes = [P(Camera), C.c_void_p]
+        L.rto_log_fixed.argtypes = [C.c_float]
+        L.rto_log_fixed.restype = C.c_float
         L.rto_analytic_lights_compile.restype = C.c_int
         L.rto_cache_mlp_param_count.argtypes = []
         L.rto_cache_mlp_param_count.restype = C.c_int
@@ -463,7 +473,7 @@ class GeneratedScene:
 
 
 def scene_from_arrays(spheres=None, quads=None, triangles=None, materials=None, openpbr=None,
-                      analytic=None, background=(0.0, 0.0, 0.0), background_top=None):
+                      analytic=None, background=(0.0, 0.0, 0.0), background_top=None, media=None):
     """Builds an oracle Scene over numpy structured/raw byte arrays; returns (scene, keepalive)."""
     keep = []
     s = Scene()
@@ -484,6 +494,10 @@ def scene_from_arrays(spheres=None, quads=None, triangles=None, materials=None, 
     attach("materials", "material_count", materials, Material)
     attach("openpbr", "openpbr_count", openpbr, OpenPbr)
     attach("analytic_lights", "analytic_light_count", analytic, AnalyticLight)
+    if media is not None and len(media):
+        a = np.ascontiguousarray(media)
+        keep.append(a)
+        s.medium_count, s.media = len(a), a.ctypes.data
     s.background_mode = 1 if background_top is not None else 0
     s.background = f3(*background)
     s.background_top = f3(*(background_top if background_top is not None else (0, 0, 0)))

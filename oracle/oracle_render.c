@@ -456,11 +456,25 @@ static void set_face(v3 d, v3 outward, hit_t* h) {
     h->sn = h->front ? outward : neg3(outward);
 }
 
+/* a hit record's primitive id: >= 0 a surface, -1 none, OM_MEDIUM_HIT - i
+ * (<= -2) a scatter event inside medium i */
+#define OM_MEDIUM_HIT (-2)
+
 static void make_hit(const rto_scene* s, v3 o, v3 d, float t, int prim, hit_t* h) {
     h->t = t;
     h->prim = prim;
     int mat;
-    if (prim < s->sphere_count) {
+    if (prim <= OM_MEDIUM_HIT) {
+        /* the medium candidate of intersect_scene (programs.cu:1736-1748) */
+        const rto_medium* md = &s->media[OM_MEDIUM_HIT - prim];
+        h->prim = -1;
+        h->prim_type = -1; h->prim_index = -1;
+        h->pos = add3(o, scl3(d, t));
+        h->front = 1;
+        h->gn = scl3(norm3(d), -1.0f);
+        h->sn = h->gn;
+        mat = md->material_index;
+    } else if (prim < s->sphere_count) {
         const rto_sphere* sp = &s->spheres[prim];
         h->prim_type = 0; h->prim_index = prim;
         h->pos = add3(o, scl3(d, t));
@@ -518,8 +532,8 @@ static void make_hit(const rto_scene* s, v3 o, v3 d, float t, int prim, hit_t* h
         /* populate_material's albedo texture, programs.cu:978-979 */
         const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
         const float pf[3] = {h->pos.x, h->pos.y, h->pos.z};
-        float u, v, c[3];
-        rto_surface_uv(s, of, df, h->prim_type, h->prim_index, pf, &u, &v);
+        float u = 0.0f, v = 0.0f, c[3];
+        if (h->prim_type >= 0) rto_surface_uv(s, of, df, h->prim_type, h->prim_index, pf, &u, &v);
         rto_eval_texture(s, s->material_albedo_texture[mat], u, v, pf, c);
         h->base = ld3(c);
     }
@@ -611,6 +625,7 @@ static float light_pdf_for_hit(const ctx_t* c, v3 prev_pos, v3 prev_n, int prev_
 /* emission_mis_weight (programs.cu:621-647) */
 static float emission_mis(const ctx_t* c, const path_t* st, const hit_t* h, int env) {
     if (!st->prev_valid || st->prev_delta) return 1.0f;
+    if (!env && h->prim < 0) return 1.0f;   /* a medium event: no primitive, no light */
     int light;
     const float lpdf = light_pdf_for_hit(c, st->prev_pos, st->prev_n, st->prev_mat, st->prev_prim,
         norm3(st->d), h, env, &light);
@@ -656,7 +671,7 @@ static light_sample_t sample_light(const ctx_t* c, const hit_t* h, const float r
     light_sample_t ls;
     memset(&ls, 0, sizeof(ls));
     const rto_scene* s = c->s;
-    const int ex = c->light_of_prim[h->prim];
+    const int ex = h->prim >= 0 ? c->light_of_prim[h->prim] : -1;
     const float elig = ex < 0 ? 1.0f : 1.0f - c->lights[ex].selection_pdf;
     if (elig <= 1e-8f) return ls;
     int li = -1;
@@ -1068,6 +1083,77 @@ static light_sample_t analytic_candidate(const ctx_t* c, v3 sp, int li, float u0
     ls.valid = ls.pdf > 0.0f && maxc3(ls.emission) > 0.0f;
     return ls;
 }
+
+/* ------------------------------------------------------------ media */
+/* hit_medium_boundary (programs.cu:1356-1419) */
+static int medium_boundary(const rto_medium* m, v3 o, v3 d, float tmin, float tmax, float* entry,
+    float* exit_) {
+    const v3 r0 = ld3(m->rotation_row0), r1 = ld3(m->rotation_row1), r2 = ld3(m->rotation_row2);
+    const v3 sh = sub3(o, ld3(m->translation));
+    const v3 lo = mk3(dot3(r0, sh), dot3(r1, sh), dot3(r2, sh));
+    const v3 ldir = mk3(dot3(r0, d), dot3(r1, d), dot3(r2, d));
+    if (m->boundary_type == RTO_MEDIUM_SPHERE) {
+        const v3 oc = sub3(lo, ld3(m->local_center_or_min));
+        const float a = lensq3(ldir);
+        const float hb = dot3(oc, ldir);
+        const float c = lensq3(oc) - m->radius * m->radius;
+        const float disc = hb * hb - a * c;
+        if (disc <= 0.0f) return 0;
+        const float sq = sqrtf(disc);
+        float t0 = (-hb - sq) / a, t1 = (-hb + sq) / a;
+        if (t0 > t1) { const float tt = t0; t0 = t1; t1 = tt; }
+        *entry = fmaxf(t0, tmin);
+        *exit_ = fminf(t1, tmax);
+        return *entry < *exit_;
+    }
+    if (m->boundary_type != RTO_MEDIUM_BOX) return 0;
+    float t0 = tmin, t1 = tmax;
+    const float org[3] = {lo.x, lo.y, lo.z}, dir[3] = {ldir.x, ldir.y, ldir.z};
+    for (int a = 0; a < 3; ++a) {
+        if (fabsf(dir[a]) < 1e-8f) {
+            if (org[a] < m->local_center_or_min[a] || org[a] > m->local_max[a]) return 0;
+            continue;
+        }
+        float a0 = (m->local_center_or_min[a] - org[a]) / dir[a];
+        float a1 = (m->local_max[a] - org[a]) / dir[a];
+        if (a0 > a1) { const float tt = a0; a0 = a1; a1 = tt; }
+        t0 = fmaxf(t0, a0);
+        t1 = fminf(t1, a1);
+        if (t0 >= t1) return 0;
+    }
+    *entry = t0;
+    *exit_ = t1;
+    return 1;
+}
+
+/* the media part of intersect_scene (programs.cu:1711-1753): medium i draws
+ * its uniform from the path's group 48 + i / 4, component i % 4; returns the
+ * scattering medium (*closest moved to the event) or -1 */
+static int sample_media(const rto_scene* s, v3 o, v3 d, float tmin, float* closest, uint32_t pixel,
+    uint32_t sample, uint32_t cam, uint32_t bounce, uint32_t stream) {
+    const float ray_len = len3(d);
+    if (!(ray_len > 1e-8f)) return -1;
+    int event = -1;
+    float u4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < s->medium_count; ++i) {
+        if ((i & 3) == 0) om_rng4(pixel, sample, cam, bounce, OM_G_MEDIUM0 + (uint32_t)(i >> 2), stream, u4);
+        const rto_medium* m = &s->media[i];
+        if (m->density <= 0.0f) continue;
+        float entry = 0.0f, exit_ = 0.0f;
+        if (!medium_boundary(m, o, d, tmin, *closest, &entry, &exit_)) continue;
+        entry = fmaxf(entry, tmin);
+        exit_ = fminf(exit_, *closest);
+        if (entry >= exit_) continue;
+        const float inside = (exit_ - entry) * ray_len;
+        const float dist = -om_log_fixed(fmaxf(u4[i & 3], 1e-6f)) / m->density;
+        if (dist >= inside) continue;
+        *closest = entry + dist / ray_len;
+        event = i;
+    }
+    return event;
+}
+
+float rto_log_fixed(float x) { return om_log_fixed(x); }
 
 /* accumulate_analytic_direct_light (programs.cu:2282-2313) */
 static void analytic_direct_light(const ctx_t* c, const hit_t* h, const float r[4], int bsdf_avail,
@@ -1493,8 +1579,18 @@ static void trace_one(const ctx_t* c, const rto_camera* cam, int cam_index, cons
         float t, u, v;
         int prim;
         if (stats) stats->rays_closest++;
-        const int found = om_closest_counted(s, c->bvh, st.o, st.d, OM_RAY_EPS, OM_RAY_FAR, &t, &prim,
+        int found = om_closest_counted(s, c->bvh, st.o, st.d, OM_RAY_EPS, OM_RAY_FAR, &t, &prim,
             &u, &v, stats ? &stats->node_visits : NULL, stats ? &stats->prim_tests : NULL);
+        if (s->medium_count > 0) {
+            float closest = found ? t : OM_RAY_FAR;
+            const int mi = sample_media(s, st.o, st.d, OM_RAY_EPS, &closest, pixel, (uint32_t)sample,
+                (uint32_t)cam_index, (uint32_t)b, stream);
+            if (mi >= 0) {
+                t = closest;
+                prim = OM_MEDIUM_HIT - mi;
+                found = 1;
+            }
+        }
         int ali = -1;
         if (c->a_count > 0 && intersect_analytic(c, st.o, st.d, OM_RAY_EPS, found ? t : OM_RAY_FAR, &ali)) {
             const float w = analytic_mis(c, &st, ali);

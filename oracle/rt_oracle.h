@@ -97,6 +97,23 @@ typedef struct {
     int32_t delta, treat_as_point, treat_as_line;
 } rto_analytic_light;
 
+/* PackedMedium (launch_params.h:201-214), same field order */
+enum { RTO_MEDIUM_SPHERE = 0, RTO_MEDIUM_BOX = 1 };
+typedef struct {
+    float local_center_or_min[3];
+    float radius;
+    float local_max[3];
+    float density;
+    float rotation_row0[3];
+    int32_t material_index;
+    float rotation_row1[3];
+    int32_t boundary_type;
+    float rotation_row2[3];
+    float pad0;
+    float translation[3];
+    float pad1;
+} rto_medium;
+
 enum { RTO_BG_CONSTANT = 0, RTO_BG_SKY_GRADIENT = 1 };
 
 /* rt_texture (PackedTexture, launch_params.h:225-237) */
@@ -126,6 +143,9 @@ typedef struct {
     const int32_t* material_albedo_texture;
     const float* triangle_uvs;
     const float* triangle_normals;
+    int32_t medium_count;
+    int32_t pad_media;
+    const rto_medium* media;
 } rto_scene;
 
 /* ---- cameras / profile ------------------------------------------------ */
@@ -225,6 +245,9 @@ void rto_project(const rto_camera* cam, const float dir_cam[3], float pixel[2]);
 /* the camera-model debug image: width * height * 4 bytes, RGBA8 encoded
  * world directions of the pixel centres' rays (programs.cu:1807-1826) */
 void rto_direction_debug(const rto_camera* cam, uint8_t* rgba);
+/* the fixed-sequence natural logarithm the GPU uses for free-flight
+ * distances (test hook) */
+float rto_log_fixed(float x);
 void rto_make_equi_lut(rto_camera* cam);
 /* the orbiting rig of the instanced scene (generator id 2 or 3) at frame `frame` of `frames` */
 int rto_generate_rig(int config, uint64_t seed, int frame, int frames, rto_camera* cams, int* count);

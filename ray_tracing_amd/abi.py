@@ -67,7 +67,15 @@ class SceneDesc(C.Structure):
         ("textures", C.c_void_p), ("image_texels", C.c_void_p),
         ("material_albedo_texture", C.c_void_p), ("triangle_uvs", C.c_void_p),
         ("triangle_normals", C.c_void_p),
+        ("medium_count", C.c_int32), ("pad_media", C.c_int32), ("media", C.c_void_p),
     ]
+
+
+class Medium(C.Structure):
+    _fields_ = [("local_center_or_min", f3), ("radius", C.c_float), ("local_max", f3),
+                ("density", C.c_float), ("rotation_row0", f3), ("material_index", C.c_int32),
+                ("rotation_row1", f3), ("boundary_type", C.c_int32), ("rotation_row2", f3),
+                ("pad0", C.c_float), ("translation", f3), ("pad1", C.c_float)]
 
 
 class Texture(C.Structure):

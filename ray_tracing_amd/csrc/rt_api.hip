@@ -148,6 +148,8 @@ struct rt_scene {
     DevBuf<float> a_sel, a_cdf;
     DevBuf<int32_t> a_infinite;   // dome / distant lights in index order
     int32_t a_infinite_count = 0;
+    DevBuf<rt_medium> media;
+    int32_t medium_count = 0;
     DevBuf<float4> a_nodes;       // BVH over the finite analytic lights
     DevBuf<int32_t> a_order;
     int32_t a_node_count = 0;
@@ -209,6 +211,8 @@ struct rt_scene {
         v.a_nodes = a_nodes.p;
         v.a_order = a_order.p;
         v.a_node_count = a_node_count;
+        v.media = media.p;
+        v.medium_count = medium_count;
         v.textures = textures.p;
         v.texels = texels.p;
         v.mat_albedo_tex = has_albedo_textures ? mat_albedo_tex.p : nullptr;
@@ -614,6 +618,24 @@ static rt_status scene_validate(const rt_scene_desc* d) {
                 + " has a selection pdf below 0 or a cdf outside [previous cdf, 1] (rt_analytic_lights_compile)");
         prev_cdf = L.cdf;
     }
+    RT_REQUIRE(d->medium_count >= 0 && d->medium_count <= 60 && (d->medium_count == 0 || d->media),
+        RT_ERROR_INVALID_ARGUMENT, "medium_count must be in [0, 60] with a media array");
+    for (int i = 0; i < d->medium_count; ++i) {
+        const rt_medium& m = d->media[i];
+        RT_REQUIRE(m.boundary_type == RT_MEDIUM_SPHERE || m.boundary_type == RT_MEDIUM_BOX,
+            RT_ERROR_INVALID_ARGUMENT, "medium " + std::to_string(i) + " has an unknown boundary type");
+        RT_REQUIRE(m.material_index >= 0 && m.material_index < d->material_count, RT_ERROR_INVALID_ARGUMENT,
+            "medium " + std::to_string(i) + " names material " + std::to_string(m.material_index)
+                + " outside the material table");
+        const float* f = &m.local_center_or_min[0];
+        bool finite = true;
+        for (int k = 0; k < 23; ++k) {
+            if (k == 11 || k == 15 || k == 19) continue;   // material_index, boundary_type, pad0
+            finite = finite && std::isfinite(f[k]);
+        }
+        RT_REQUIRE(finite && m.density >= 0.0f, RT_ERROR_INVALID_ARGUMENT,
+            "medium " + std::to_string(i) + " has a negative density or a non-finite field");
+    }
     RT_REQUIRE(d->texture_count >= 0 && d->image_texel_count >= 0, RT_ERROR_INVALID_ARGUMENT,
         "texture counts must be non-negative");
     RT_REQUIRE((d->texture_count == 0 || d->textures) && (d->image_texel_count == 0 || d->image_texels),
@@ -690,6 +712,8 @@ static rt_status scene_fill(rt_scene* s, const rt_scene_desc* d) {
     if (e == hipSuccess) e = upload(s->materials, d->materials, (size_t)d->material_count, st);
     if (e == hipSuccess) e = upload(s->openpbr, d->openpbr, (size_t)d->openpbr_count, st);
     if (e == hipSuccess) e = upload(s->analytic, d->analytic_lights, (size_t)d->analytic_light_count, st);
+    s->medium_count = d->medium_count;
+    if (e == hipSuccess) e = upload(s->media, d->media, (size_t)d->medium_count, st);
     if (e == hipSuccess) e = upload(s->lights, s->host_lights.data(), s->host_lights.size(), st);
     if (e == hipSuccess) e = upload(s->light_of_prim, lop.data(), lop.size(), st);
     std::vector<float> asel, acdf;
@@ -776,6 +800,7 @@ static void scene_signatures(const rt_scene_desc* d, uint64_t* geometry, uint64_
     h.array(d->material_albedo_texture, d->material_albedo_texture ? d->material_count : 0);
     h.array(d->triangle_uvs, d->triangle_uvs ? (int64_t)d->triangle_count * 6 : 0);
     h.array(d->triangle_normals, d->triangle_normals ? (int64_t)d->triangle_count * 9 : 0);
+    h.array(d->media, d->medium_count);
     *scene = h.value();
 }
 
@@ -1643,7 +1668,7 @@ static rt_status render_frame_impl(rt_renderer* r, const rt_camera_rig* rig, con
             // shadow rays get a launch of their own
             for (int bounce = 0; bounce < max_b; ++bounce) {
                 ENQ(mark());
-                ENQ(rtk::wf_trace(sv, b, bounce, true, count, trace_l, shade_l.grid, st));
+                ENQ(rtk::wf_trace(sv, cv, f, b, bounce, true, count, trace_l, shade_l.grid, st));
                 ENQ(mark());
                 ENQ(rtk::wf_shade(sv, cv, f, b, bounce, shade_l, st));
                 if (f.restir != 0 && bounce == 0) ENQ(rtk::wf_restir(sv, cv, f, rv, b, restir_l, st));
@@ -1654,7 +1679,7 @@ static rt_status render_frame_impl(rt_renderer* r, const rt_camera_rig* rig, con
             ENQ(mark());
             ENQ(mark());
             ENQ(mark());
-            ENQ(rtk::wf_trace(sv, b, max_b, false, count, trace_l, shade_l.grid, st));
+            ENQ(rtk::wf_trace(sv, cv, f, b, max_b, false, count, trace_l, shade_l.grid, st));
             ENQ(mark());
             if (cache != nullptr) {
                 ENQ(rtk::wf_cache_chunk(cache->grid_view(), cache->packed(), f, b, paths,

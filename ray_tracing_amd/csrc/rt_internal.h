@@ -66,6 +66,8 @@ struct SceneView {
     const float4* a_nodes;
     const int32_t* a_order;
     int32_t a_node_count;
+    const rt_medium* media;   // homogeneous media (free-flight events of closest-hit rays)
+    int32_t medium_count;
     const rt_texture* textures;
     const float* texels;               // 3 floats per texel
     const int32_t* mat_albedo_tex;     // nullptr: every albedo is a constant

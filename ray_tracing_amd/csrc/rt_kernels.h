@@ -131,8 +131,9 @@ hipError_t wf_raygen(const CameraView& cams, const FrameParams& fp, WavefrontBuf
 // and the shadow rays the previous shade stage queued, then the analytic
 // light hits of the extension rays and the unoccluded shadow contributions
 // added to their paths in queue order (emissive, then analytic / ReSTIR)
-hipError_t wf_trace(const SceneView& s, WavefrontBuffers& b, int bounce, bool extension,
-    bool count, StageLaunch l, int apply_grid, hipStream_t st);
+hipError_t wf_trace(const SceneView& s, const CameraView& cams, const FrameParams& fp,
+    WavefrontBuffers& b, int bounce, bool extension, bool count, StageLaunch l, int apply_grid,
+    hipStream_t st);
 // after the shade / ReSTIR stages of `bounce`: its path queue has been consumed
 hipError_t wf_end_bounce(WavefrontBuffers& b, int bounce, hipStream_t st);
 // resident blocks per CU of the shade (256 threads) and ReSTIR (128 threads)

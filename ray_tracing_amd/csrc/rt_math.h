@@ -83,7 +83,7 @@ __device__ __forceinline__ float4 rng4(uint32_t pixel, uint32_t sample, uint32_t
     return make_float4(u01(h.x), u01(h.y), u01(h.z), u01(h.w));
 }
 enum : uint32_t { kGroupLight = 0, kGroupBsdf = 1, kGroupAnalytic = 2, kGroupCache = 3, kGroupRestir0 = 8,
-    kGroupCamera = 63 };
+    kGroupMedium0 = 48, kGroupCamera = 63 };
 
 // sin/cos of a whole turn u (angle 2*pi*u) by quadrant reduction in turns
 // (exact) and Taylor polynomials on [-pi/4, pi/4] (truncation < 3e-11)
@@ -120,6 +120,40 @@ __device__ __forceinline__ f3 sphere_dir(float u0, float u1) {
 }
 
 constexpr float kPi = 3.14159265358979323846f;
+
+// Natural logarithm by a fixed operation sequence (the Cephes logf method:
+// x = m 2^e with m in [sqrt(1/2), sqrt(2)), a degree-9 polynomial in m - 1
+// and the split ln 2 = 0.693359375 - 2.12194440e-4), so the GPU and the CPU
+// restatement round alike; for finite normal x > 0 (free-flight sampling
+// uses x in [1e-6, 1]).
+__device__ __forceinline__ float log_fixed(float x) {
+    const uint32_t b = __float_as_uint(x);
+    int e = (int)((b >> 23) & 0xffu) - 126;
+    float m = __uint_as_float((b & 0x807fffffu) | 0x3f000000u);   // [0.5, 1)
+    if (m < 0.70710678118654752f) {
+        e -= 1;
+        m = m + m - 1.0f;
+    } else {
+        m = m - 1.0f;
+    }
+    const float z = m * m;
+    float y = 7.0376836292e-2f;
+    y = y * m - 1.1514610310e-1f;
+    y = y * m + 1.1676998740e-1f;
+    y = y * m - 1.2420140846e-1f;
+    y = y * m + 1.4249322787e-1f;
+    y = y * m - 1.6668057665e-1f;
+    y = y * m + 2.0000714765e-1f;
+    y = y * m - 2.4999993993e-1f;
+    y = y * m + 3.3333331174e-1f;
+    y = y * m * z;
+    const float fe = (float)e;
+    y = y + -2.12194440e-4f * fe;
+    y = y + -0.5f * z;
+    float r = m + y;
+    r = r + 0.693359375f * fe;
+    return r;
+}
 constexpr float kRayEps = 1e-3f;
 constexpr float kRayFar = 1e30f;
 constexpr uint32_t kLeafBit = 0x80000000u;

@@ -43,11 +43,27 @@ __device__ __forceinline__ void set_face(f3 d, f3 outward, HitInfo& h) {
     h.sn = h.front ? outward : -outward;
 }
 
+// A hit record's primitive id: >= 0 a surface primitive, -1 none, and
+// kMediumHit - i (<= -2) a scatter event inside medium i.
+constexpr int kMediumHit = -2;
+
 __device__ inline void make_hit(const SceneView& s, f3 o, f3 d, float t, int prim, HitInfo& h) {
     h.t = t;
-    h.prim = prim;
     int mat;
-    if (prim < s.sphere_count) {
+    if (prim <= kMediumHit) {
+        // the medium candidate of intersect_scene (programs.cu:1736-1748): no
+        // primitive, front face, normals against the ray, texture coordinates 0
+        const rt_medium md = s.media[kMediumHit - prim];
+        h.prim = -1;
+        h.prim_type = -1;
+        h.prim_index = -1;
+        h.pos = o + d * t;
+        h.front = true;
+        h.gn = normalize3(d) * -1.0f;
+        h.sn = h.gn;
+        mat = md.material_index;
+    } else if (prim < s.sphere_count) {
+        h.prim = prim;
         const rt_sphere sp = s.spheres[prim];
         h.prim_type = 0;
         h.prim_index = prim;
@@ -55,6 +71,7 @@ __device__ inline void make_hit(const SceneView& s, f3 o, f3 d, float t, int pri
         set_face(d, div3(h.pos - ld3(sp.center), sp.radius), h);
         mat = sp.material;
     } else if (prim < s.sphere_count + s.quad_count) {
+        h.prim = prim;
         const int qi = prim - s.sphere_count;
         const rt_quad q = s.quads[qi];
         h.prim_type = 1;
@@ -64,6 +81,7 @@ __device__ inline void make_hit(const SceneView& s, f3 o, f3 d, float t, int pri
         set_face(d, div3(n, sqrtf(lensq3(n))), h);
         mat = q.material;
     } else {
+        h.prim = prim;
         const int ti = prim - s.sphere_count - s.quad_count;
         const rt_triangle tr = s.triangles[ti];
         h.prim_type = 2;
@@ -106,8 +124,8 @@ __device__ inline void make_hit(const SceneView& s, f3 o, f3 d, float t, int pri
         // populate_material's albedo texture (programs.cu:978-979)
         const int ti = s.mat_albedo_tex[mat];
         if (ti >= 0) {
-            float u, v;
-            surface_uv(s, o, d, h.prim_type, h.prim_index, h.pos, &u, &v);
+            float u = 0.0f, v = 0.0f;
+            if (h.prim_type >= 0) surface_uv(s, o, d, h.prim_type, h.prim_index, h.pos, &u, &v);
             h.base = eval_texture(s, ti, u, v, h.pos);
         }
     }
@@ -159,6 +177,7 @@ __device__ __forceinline__ float eff_selection_pdf(const SceneView& s, int light
 __device__ inline float emission_mis(const SceneView& s, const PathPrev& pv, f3 ray_d,
     const HitInfo* h, bool env) {
     if (!pv.valid || pv.delta) return 1.0f;
+    if (!env && h->prim < 0) return 1.0f;   // a medium event: no primitive, no light
     const int light = env ? s.env_light : s.light_of_prim[h->prim];
     if (light < 0) return 1.0f;
     const f3 dir = normalize3(ray_d);
@@ -220,7 +239,7 @@ __device__ __forceinline__ int pick_light(const SceneView& s, float u) {
 __device__ inline LightSample sample_light(const SceneView& s, const HitInfo& h, float4 r) {
     LightSample ls{};
     ls.valid = false;
-    const int ex = s.light_of_prim[h.prim];
+    const int ex = h.prim >= 0 ? s.light_of_prim[h.prim] : -1;
     const float elig = ex < 0 ? 1.0f : 1.0f - s.lights[ex].selection_pdf;
     if (elig <= 1e-8f) return ls;
     int li = -1;
@@ -553,6 +572,88 @@ __device__ inline f3 analytic_infinite(const SceneView& s, const PathPrev& pv, f
         sum = sum + ld3(L.radiance) * w;
     }
     return sum;
+}
+
+// ---- homogeneous media
+// hit_medium_boundary (programs.cu:1356-1419): the ray's interval inside the
+// medium's sphere or box, in the medium's frame, clipped to (tmin, tmax)
+__device__ inline bool medium_boundary(const rt_medium& m, f3 o, f3 d, float tmin, float tmax,
+    float* entry, float* exit) {
+    const f3 r0 = ld3(m.rotation_row0), r1 = ld3(m.rotation_row1), r2 = ld3(m.rotation_row2);
+    const f3 sh = o - ld3(m.translation);
+    const f3 lo = mk3(dot3(r0, sh), dot3(r1, sh), dot3(r2, sh));
+    const f3 ldir = mk3(dot3(r0, d), dot3(r1, d), dot3(r2, d));
+    if (m.boundary_type == RT_MEDIUM_SPHERE) {
+        const f3 oc = lo - ld3(m.local_center_or_min);
+        const float a = lensq3(ldir);
+        const float hb = dot3(oc, ldir);
+        const float c = lensq3(oc) - m.radius * m.radius;
+        const float disc = hb * hb - a * c;
+        if (disc <= 0.0f) return false;
+        const float sq = sqrtf(disc);
+        float t0 = (-hb - sq) / a, t1 = (-hb + sq) / a;
+        if (t0 > t1) {
+            const float tt = t0;
+            t0 = t1;
+            t1 = tt;
+        }
+        *entry = fmaxf(t0, tmin);
+        *exit = fminf(t1, tmax);
+        return *entry < *exit;
+    }
+    if (m.boundary_type != RT_MEDIUM_BOX) return false;
+    float t0 = tmin, t1 = tmax;
+    const float org[3] = {lo.x, lo.y, lo.z}, dir[3] = {ldir.x, ldir.y, ldir.z};
+    for (int a = 0; a < 3; ++a) {
+        if (fabsf(dir[a]) < 1e-8f) {
+            if (org[a] < m.local_center_or_min[a] || org[a] > m.local_max[a]) return false;
+            continue;
+        }
+        float a0 = (m.local_center_or_min[a] - org[a]) / dir[a];
+        float a1 = (m.local_max[a] - org[a]) / dir[a];
+        if (a0 > a1) {
+            const float tt = a0;
+            a0 = a1;
+            a1 = tt;
+        }
+        t0 = fmaxf(t0, a0);
+        t1 = fminf(t1, a1);
+        if (t0 >= t1) return false;
+    }
+    *entry = t0;
+    *exit = t1;
+    return true;
+}
+
+// the media part of intersect_scene (programs.cu:1711-1753): in index
+// order, a medium the ray crosses before `closest` scatters it at distance
+// -log(max(u, 1e-6)) / density into the medium if that falls inside;
+// medium i draws u from the path's medium group 48 + i / 4, component i % 4.
+// Returns the scattering medium (closest updated) or -1.
+__device__ inline int sample_media(const SceneView& s, f3 o, f3 d, float tmin, float* closest,
+    uint32_t pixel, uint32_t sample, uint32_t key, uint32_t bounce, uint32_t stream) {
+    const float ray_len = len3(d);
+    if (!(ray_len > 1e-8f)) return -1;
+    int event = -1;
+    float4 u4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int i = 0; i < s.medium_count; ++i) {
+        if ((i & 3) == 0)
+            u4 = rng4(pixel, sample, key, bounce, kGroupMedium0 + (uint32_t)(i >> 2), stream);
+        const rt_medium m = s.media[i];
+        if (m.density <= 0.0f) continue;
+        float entry = 0.0f, exit = 0.0f;
+        if (!medium_boundary(m, o, d, tmin, *closest, &entry, &exit)) continue;
+        entry = fmaxf(entry, tmin);
+        exit = fminf(exit, *closest);
+        if (entry >= exit) continue;
+        const float inside = (exit - entry) * ray_len;
+        const float u = (i & 3) == 0 ? u4.x : ((i & 3) == 1 ? u4.y : ((i & 3) == 2 ? u4.z : u4.w));
+        const float dist = -log_fixed(fmaxf(u, 1e-6f)) / m.density;
+        if (dist >= inside) continue;
+        *closest = entry + dist / ray_len;
+        event = i;
+    }
+    return event;
 }
 
 // sample_packed_analytic_light (light_sampling.h:104-116) by bisection

@@ -339,7 +339,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                 // path ends on an analytic light (programs.cu:1857-1866)
                 const float w = analytic_mis(s, pv, ali, ro, rd);
                 rad = rad + thr * (ld3(s.analytic[ali].radiance) * w);
-            } else if (prim < 0) {
+            } else if (prim == -1) {
                 // a miss: background, then the dome / distant lights (programs.cu:1867-1875)
                 const float w = emission_mis(s, pv, rd, nullptr, true);
                 rad = rad + thr * (background_at(s, rd) * w);
@@ -485,15 +485,32 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
 // surface hit (programs.cu:1861-1862), for every path of the queue.  A
 // separate convergent pass: inside the persistent traversal loop the light
 // loop would run once per finishing lane.
-__global__ __launch_bounds__(256) void k_analytic_hits(SceneView s, WavefrontBuffers b, int cur) {
+// After the closest-hit traversal of the extension rays: scatter events in
+// homogeneous media before the surface hit (a medium event replaces the
+// hit), then the nearest analytic light before whatever the ray reached
+// (intersect_scene + intersect_analytic_lights, programs.cu:1654-1757,1856-1866)
+__global__ __launch_bounds__(256) void k_analytic_hits(SceneView s, CameraView cams, FrameParams fp,
+    WavefrontBuffers b, int bounce, int cur) {
+    const SlotRng slot_rng_v = slot_rng(fp);
     const uint32_t count = b.counters[cur];
     const uint32_t* queue = b.queue[cur];
     for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < count; q += gridDim.x * 256u) {
         const uint32_t slot = queue[q];
         float4 h = b.hit[slot];
-        const bool found = __float_as_int(h.y) >= 0;
-        const int ali = intersect_analytic(s, xyz(b.ray_o[slot]), xyz(b.ray_d[slot]), kRayEps,
-            found ? h.x : kRayFar);
+        const f3 o = xyz(b.ray_o[slot]), d = xyz(b.ray_d[slot]);
+        bool found = __float_as_int(h.y) >= 0;
+        float closest = found ? h.x : kRayFar;
+        if (s.medium_count > 0) {
+            const SlotId id = slot_id(cams, fp, slot_rng_v, slot);
+            const int mi = sample_media(s, o, d, kRayEps, &closest, id.pixel, id.sample, id.key,
+                (uint32_t)bounce, id.stream);
+            if (mi >= 0) {
+                h.x = closest;
+                h.y = __int_as_float(kMediumHit - mi);
+                found = true;
+            }
+        }
+        const int ali = s.analytic_count > 0 ? intersect_analytic(s, o, d, kRayEps, found ? closest : kRayFar) : -1;
         h.z = __int_as_float(ali);
         b.hit[slot] = h;
     }
@@ -645,13 +662,15 @@ static void launch_fused(const SceneView& sv, WavefrontBuffers& b, int cur, int 
     else k_trace_fused<128, COUNT><<<l.grid, 128, 0, st>>>(sv, b, cur, ext, l.refill_min);
 }
 
-hipError_t wf_trace(const SceneView& s, WavefrontBuffers& b, int bounce, bool extension,
-    bool count, StageLaunch l, int apply_grid, hipStream_t st) {
+hipError_t wf_trace(const SceneView& s, const CameraView& cams, const FrameParams& fp,
+    WavefrontBuffers& b, int bounce, bool extension, bool count, StageLaunch l, int apply_grid,
+    hipStream_t st) {
     const int cur = bounce & 1;
     const int ext = extension ? 1 : 0;
     if (count) launch_fused<true>(s, b, cur, ext, l, st);
     else launch_fused<false>(s, b, cur, ext, l, st);
-    if (extension && s.analytic_count > 0) k_analytic_hits<<<l.grid, 256, 0, st>>>(s, b, cur);
+    if (extension && (s.analytic_count > 0 || s.medium_count > 0))
+        k_analytic_hits<<<l.grid, 256, 0, st>>>(s, cams, fp, b, bounce, cur);
     const unsigned g = (unsigned)(apply_grid > 0 ? apply_grid : 1);
     k_apply_shadows<<<g, 256, 0, st>>>(b, 0);
     k_apply_shadows<<<g, 256, 0, st>>>(b, 1);

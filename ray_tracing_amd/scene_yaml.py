@@ -23,9 +23,9 @@ src/realtime/viewer/body_pose.cpp:36-85; the camera axes are x right,
 y down, z forward, as the renderer's.
 
 Albedo textures of every type (constant, checker, image, noise) are
-rendered; textured emission and participating media (isotropic materials,
-`scene.media`),
-raise SceneFileError when a scene is flattened.
+rendered, and so are participating media (`scene.media` over sphere or box
+shapes, isotropic materials); textured emission and media bounded by quads
+or meshes raise SceneFileError when a scene is flattened.
 
     python -m ray_tracing_amd.scene_yaml path/to/scene.yaml --spp 16 --out frame.npy
 """
@@ -41,7 +41,7 @@ import yaml
 
 from . import abi
 
-MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_DIFFUSE_LIGHT = 0, 1, 2, 3
+MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_DIFFUSE_LIGHT, MAT_ISOTROPIC = 0, 1, 2, 3, 4
 CAM_PINHOLE32, CAM_EQUI62_LUT1D = 0, 1
 
 
@@ -629,9 +629,13 @@ def _material(defn: SceneDefinition, m: dict, path: str):
     elif m["type"] == "emissive":
         out.type = MAT_DIFFUSE_LIGHT
         out.emission = abi.f3(*_emission_color(defn, m["emission"], path))
+    elif m["type"] == "isotropic":
+        # IsotropicVolumeMaterial (realtime_scene_adapter.cpp:67-68): the
+        # phase function's albedo, constant or textured
+        out.type = MAT_ISOTROPIC
+        binding = _albedo(defn, m["albedo"], out)
     else:
-        raise SceneFileError(f"{path}: material type {m['type']} (participating media) is not "
-                             "supported by this build")
+        raise SceneFileError(f"{path}: material type {m['type']} is not supported by this build")
     return out, binding
 
 
@@ -703,6 +707,7 @@ class FlatScene:
     albedo_bindings: object = None
     triangle_uvs: object = None
     triangle_normals: object = None
+    media: object = None
 
 
 def flatten(defn: SceneDefinition) -> FlatScene:
@@ -710,8 +715,7 @@ def flatten(defn: SceneDefinition) -> FlatScene:
     triangles each in their own array), one C ABI material per scene
     material, the scene background as a constant background."""
     path = defn.dependencies[0] if defn.dependencies else defn.id
-    if defn.media:
-        raise SceneFileError(f"{path}: participating media are not supported by this build")
+    media = _media_records(defn, path)
     mats, bindings = [], []
     for m in defn.materials:
         mat, binding = _material(defn, m, path)
@@ -767,7 +771,43 @@ def flatten(defn: SceneDefinition) -> FlatScene:
     if any(not np.isnan(n[0]) for n in tri_n):
         fs.triangle_normals = np.ascontiguousarray(np.array(tri_n, np.float32).reshape(-1, 9))
         d.triangle_normals = fs.triangle_normals.ctypes.data_as(C.c_void_p)
+    if media:
+        fs.media = (abi.Medium * len(media))(*media)
+        d.medium_count = len(media)
+        d.media = C.cast(fs.media, C.c_void_p)
     return fs
+
+
+def _media_records(defn: SceneDefinition, path: str) -> list:
+    """HomogeneousMediumPrimitive / pack_medium (realtime_scene_adapter.cpp:
+    450-481, packed_scene_preparation.cpp:79-92): a sphere or box shape in
+    the medium's own frame, world -> local = the transform's rotation
+    transposed applied to p - translation; quad and mesh boundaries are
+    refused with the reference's messages."""
+    out = []
+    for m in defn.media:
+        shape = defn.shapes[m["shape"]]
+        tf = m["transform"]
+        rec = abi.Medium()
+        rec.material_index = m["material"]
+        rec.density = m["density"]
+        R = np.asarray(tf.rotation, np.float64).T
+        rec.rotation_row0, rec.rotation_row1, rec.rotation_row2 = (abi.f3(*R[k]) for k in range(3))
+        rec.translation = abi.f3(*tf.translation)
+        if shape["type"] == "sphere":
+            rec.boundary_type = 0
+            rec.local_center_or_min = abi.f3(*shape["center"])
+            rec.radius = shape["radius"]
+        elif shape["type"] == "box":
+            rec.boundary_type = 1
+            rec.local_center_or_min = abi.f3(*shape["min"])
+            rec.local_max = abi.f3(*shape["max"])
+        elif shape["type"] == "quad":
+            raise SceneFileError(f"{path}: quad boundaries are unsupported for homogeneous media")
+        else:
+            raise SceneFileError(f"{path}: triangle mesh boundaries are unsupported for homogeneous media")
+        out.append(rec)
+    return out
 
 
 # ------------------------------------------------------------------ cameras

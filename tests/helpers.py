@@ -25,6 +25,7 @@ def oracle_scene(oracle, desc):
     s.material_albedo_texture = desc.material_albedo_texture
     s.triangle_uvs = desc.triangle_uvs
     s.triangle_normals = desc.triangle_normals
+    s.medium_count, s.media = desc.medium_count, desc.media
     return s
 
 
@@ -61,6 +62,7 @@ def product_desc(rt, arrays: dict, background=(0, 0, 0), background_top=None):
     attach("materials", "material_count", arrays.get("materials"))
     attach("openpbr", "openpbr_count", arrays.get("openpbr"))
     attach("analytic_lights", "analytic_light_count", arrays.get("analytic"))
+    attach("media", "medium_count", arrays.get("media"))
     d.background_mode = 1 if background_top is not None else 0
     d.background = (C.c_float * 3)(*background)
     d.background_top = (C.c_float * 3)(*(background_top or (0, 0, 0)))
@@ -80,6 +82,27 @@ ALIGHT_DT = np.dtype([("type", "<i4"), ("position", "<f4", 3), ("basis_x", "<f4"
                       ("cdf", "<f4"), ("delta", "<i4"), ("treat_as_point", "<i4"),
                       ("treat_as_line", "<i4")])
 ALIGHT_TYPES = {"sphere": 0, "disk": 1, "rect": 2, "cylinder": 3, "distant": 4, "dome": 5}
+MEDIUM_DT = np.dtype([("local_center_or_min", "<f4", 3), ("radius", "<f4"), ("local_max", "<f4", 3),
+                      ("density", "<f4"), ("rotation_row0", "<f4", 3), ("material_index", "<i4"),
+                      ("rotation_row1", "<f4", 3), ("boundary_type", "<i4"),
+                      ("rotation_row2", "<f4", 3), ("pad0", "<f4"), ("translation", "<f4", 3),
+                      ("pad1", "<f4")])
+
+
+def medium(kind, material, density, centre=(0, 0, 0), radius=1.0, lo=(-1, -1, -1), hi=(1, 1, 1),
+           rotation=np.eye(3), translation=(0, 0, 0)):
+    """One PackedMedium record: kind "sphere" (local centre, radius) or "box"
+    (local corners); world -> local = rotation (p - translation)."""
+    m = np.zeros(1, MEDIUM_DT)
+    m["boundary_type"] = 0 if kind == "sphere" else 1
+    m["local_center_or_min"] = centre if kind == "sphere" else lo
+    m["radius"] = radius
+    m["local_max"] = hi
+    m["density"], m["material_index"] = density, material
+    R = np.asarray(rotation, np.float64)
+    m["rotation_row0"], m["rotation_row1"], m["rotation_row2"] = R[0], R[1], R[2]
+    m["translation"] = translation
+    return m
 
 
 def analytic_lights(oracle, specs):

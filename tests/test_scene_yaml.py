@@ -338,3 +338,66 @@ def test_obj_negative_indices_count_back_from_the_last_element(tmp_path):
         write(p, BASIC + "imports:\n  mesh:\n    type: obj_mtl\n    obj: c.obj\n")
         with pytest.raises((ValueError, S.SceneFileError)):
             S.flatten(S.load_scene_definition(p))
+
+
+MEDIA = BASIC.replace("""  materials:
+    matte:
+      type: diffuse
+      albedo: white""", """  materials:
+    matte:
+      type: diffuse
+      albedo: white
+    mist:
+      type: isotropic
+      albedo: white""") + """  media:
+    fog:
+      shape: ball
+      material: mist
+      density: 0.75
+      transform:
+        translation: [1.0, 2.0, 3.0]
+        rotation: [[0.0, -1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]]
+"""
+
+
+def test_media_flatten_to_packed_media(tmp_path):
+    """parse_media + the realtime adapter's HomogeneousMediumPrimitive
+    (realtime_scene_adapter.cpp:450-481) + pack_medium: the sphere in its own
+    frame, world -> local rows = the rotation transposed, the translation,
+    density and material; the isotropic material as type 4 with its albedo;
+    box shapes become box boundaries, quads are refused."""
+    p = str(tmp_path / "scene.yaml")
+    write(p, MEDIA)
+    f = S.flatten(S.load_scene_definition(p))
+    assert f.desc.medium_count == 1
+    m = f.media[0]
+    assert m.boundary_type == 0 and m.material_index == 1 and m.density == 0.75
+    assert list(m.local_center_or_min) == [0.0, 0.0, 0.0] and m.radius == 1.0
+    assert [list(m.rotation_row0), list(m.rotation_row1), list(m.rotation_row2)] == \
+        [[0.0, 1.0, 0.0], [-1.0, 0.0, 0.0], [0.0, 0.0, 1.0]]
+    assert list(m.translation) == [1.0, 2.0, 3.0]
+    mats = np.ctypeslib.as_array(f.materials)
+    assert mats[1]["type"] == 4 and list(mats[1]["albedo"]) == [1.0, 1.0, 1.0]
+    box = MEDIA.replace("""    ball:
+      type: sphere
+      center: [0.0, 0.0, 0.0]
+      radius: 1.0""", """    ball:
+      type: sphere
+      center: [0.0, 0.0, 0.0]
+      radius: 1.0
+    crate:
+      type: box
+      min: [-1.0, -2.0, -0.5]
+      max: [1.0, 2.0, 0.5]
+    pane:
+      type: quad
+      origin: [0.0, 0.0, 0.0]
+      edge_u: [1.0, 0.0, 0.0]
+      edge_v: [0.0, 1.0, 0.0]""")
+    write(p, box.replace("shape: ball\n      material: mist", "shape: crate\n      material: mist"))
+    m = S.flatten(S.load_scene_definition(p)).media[0]
+    assert m.boundary_type == 1 and list(m.local_center_or_min) == [-1.0, -2.0, -0.5]
+    assert list(m.local_max) == [1.0, 2.0, 0.5]
+    write(p, box.replace("shape: ball\n      material: mist", "shape: pane\n      material: mist"))
+    with pytest.raises(S.SceneFileError, match="quad boundaries are unsupported for homogeneous media"):
+        S.flatten(S.load_scene_definition(p))
