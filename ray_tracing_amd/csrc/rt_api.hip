@@ -128,6 +128,9 @@ struct rt_scene {
     rt_device* dev = nullptr;
     int32_t sphere_count = 0, quad_count = 0, triangle_count = 0, material_count = 0;
     int32_t openpbr_count = 0, analytic_count = 0;
+    // shade passes: 0 one full pass (OpenPBR only), 1 the legacy-material
+    // pass alone (no OpenPBR), 2 both (legacy pass, then its OpenPBR hits)
+    int32_t shade_mode = 0;
     int32_t texture_count = 0, texel_count = 0;
     bool has_albedo_textures = false;
     bool has_triangle_uvs = false, has_triangle_normals = false;
@@ -294,6 +297,7 @@ struct rt_renderer {
     DevBuf<float4> ray_o, ray_d, hit, throughput, radiance, prev_pos, prev_n, aux_normal, aux_albedo;
     DevBuf<int32_t> prev_flags;
     DevBuf<uint32_t> queue0, queue1;
+    DevBuf<uint32_t> pbr_queue;  // OpenPBR hits handed from the legacy shade pass
     DevBuf<float4> shadow_o, shadow_d, shadow_c, shadow2_o, shadow2_d, shadow2_c;
     DevBuf<uint32_t> restir_queue;
     DevBuf<float4> restir_o, restir_d, restir_thr;
@@ -341,6 +345,7 @@ struct rt_renderer {
         b.aux_albedo = aux_albedo.p;
         b.queue[0] = queue0.p;
         b.queue[1] = queue1.p;
+        b.pbr_queue = pbr_queue.p;
         b.shadow_o = shadow_o.p;
         b.shadow_d = shadow_d.p;
         b.shadow_c = shadow_c.p;
@@ -370,6 +375,7 @@ struct rt_renderer {
         if ((e = prev_flags.reserve(q)) != hipSuccess) return e;
         if ((e = queue0.reserve(q)) != hipSuccess) return e;
         if ((e = queue1.reserve(q)) != hipSuccess) return e;
+        if ((e = pbr_queue.reserve(q)) != hipSuccess) return e;
         if ((e = restir_queue.reserve(q)) != hipSuccess) return e;
         path_capacity = q;
         return hipSuccess;
@@ -687,6 +693,17 @@ static rt_status scene_fill(rt_scene* s, const rt_scene_desc* d) {
     s->material_count = d->material_count;
     s->openpbr_count = d->openpbr_count;
     s->analytic_count = d->analytic_light_count;
+    {
+        bool pbr = false, legacy = false;
+        for (int i = 0; i < d->material_count; ++i) {
+            const rt_material& m = d->materials[i];
+            if (m.type == RT_MAT_OPENPBR && m.openpbr_index >= 0 && m.openpbr_index < d->openpbr_count)
+                pbr = true;
+            else
+                legacy = true;
+        }
+        s->shade_mode = !pbr ? 1 : (legacy ? 2 : 0);
+    }
     s->background_mode = d->background_mode;
     for (int i = 0; i < 3; ++i) {
         s->bg[i] = d->background[i];
@@ -1590,6 +1607,7 @@ static rt_status render_frame_impl(rt_renderer* r, const rt_camera_rig* rig, con
     const rtk::StageLaunch trace_l{r->dev->cu_count * r->tuning.trace_blocks_per_cu, block,
         r->tuning.trace_refill};
     const rtk::StageLaunch shade_l{r->dev->cu_count * rtk::wf_shade_blocks_per_cu(), 256};
+    const rtk::StageLaunch shade_lite{r->dev->cu_count * rtk::wf_shade_blocks_per_cu(true), 256};
     const rtk::StageLaunch restir_l{std::min(trace_l.grid * block / 128,
                                         r->dev->cu_count * rtk::wf_restir_blocks_per_cu()),
         128};
@@ -1670,7 +1688,7 @@ static rt_status render_frame_impl(rt_renderer* r, const rt_camera_rig* rig, con
                 ENQ(mark());
                 ENQ(rtk::wf_trace(sv, cv, f, b, bounce, true, count, trace_l, shade_l.grid, st));
                 ENQ(mark());
-                ENQ(rtk::wf_shade(sv, cv, f, b, bounce, shade_l, st));
+                ENQ(rtk::wf_shade(sv, cv, f, b, bounce, shade_l, shade_lite, s->shade_mode, st));
                 if (f.restir != 0 && bounce == 0) ENQ(rtk::wf_restir(sv, cv, f, rv, b, restir_l, st));
                 ENQ(rtk::wf_end_bounce(b, bounce, st));
                 ENQ(mark());

@@ -74,6 +74,7 @@ struct WavefrontBuffers {
     float4* aux_normal; // bounce-0 shading normal, w = 1 if captured
     float4* aux_albedo; // bounce-0 base colour, w = depth
     uint32_t* queue[2]; // active path slots (ping-pong)
+    uint32_t* pbr_queue;// OpenPBR hits handed from the legacy-material shade pass
     float4* shadow_o;   // origin.xyz, max_t
     float4* shadow_d;   // direction.xyz, bits(path slot)
     float4* shadow_c;   // contribution.xyz
@@ -136,12 +137,16 @@ hipError_t wf_trace(const SceneView& s, const CameraView& cams, const FrameParam
     hipStream_t st);
 // after the shade / ReSTIR stages of `bounce`: its path queue has been consumed
 hipError_t wf_end_bounce(WavefrontBuffers& b, int bounce, hipStream_t st);
-// resident blocks per CU of the shade (256 threads) and ReSTIR (128 threads)
-// kernels: their launch grids per CU
-int wf_shade_blocks_per_cu();
+// resident blocks per CU of the shade (256 threads; `lite`: its plain
+// legacy-material variant) and ReSTIR (128 threads) kernels: their launch
+// grids per CU
+int wf_shade_blocks_per_cu(bool lite = false);
 int wf_restir_blocks_per_cu();
+// mode 0: one full pass; 1: the legacy-material pass alone (the scene has
+// no OpenPBR material); 2: the legacy pass, then the full pass over the
+// OpenPBR hits it handed over
 hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParams& fp,
-    WavefrontBuffers& b, int bounce, StageLaunch l, hipStream_t st);
+    WavefrontBuffers& b, int bounce, StageLaunch l, StageLaunch lite, int mode, hipStream_t st);
 hipError_t wf_restir(const SceneView& s, const CameraView& cams,
     const FrameParams& fp, const RestirView& rv, WavefrontBuffers& b, StageLaunch l,
     hipStream_t st);
@@ -152,7 +157,7 @@ hipError_t wf_cache_chunk(const CacheGridView& g, const CachePacked& pk, const F
     WavefrontBuffers& b, int path_cap, int grid, hipStream_t st);
 // device counters of the cache: records written this frame, queries answered
 const uint32_t* wf_cache_record_count(const WavefrontBuffers& b);
-constexpr int kStatWords = 30;  // counter words downloaded after a frame
+constexpr int kStatWords = 31;  // counter words downloaded after a frame
 constexpr int kStatCacheRecords = 21, kStatCacheQueries = 23;
 hipError_t wf_clear_accum(int pixels, WavefrontBuffers& b, hipStream_t st);
 // zero `bytes` (a multiple of 4) of device memory with a fill kernel

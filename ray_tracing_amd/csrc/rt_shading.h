@@ -47,6 +47,8 @@ __device__ __forceinline__ void set_face(f3 d, f3 outward, HitInfo& h) {
 // kMediumHit - i (<= -2) a scatter event inside medium i.
 constexpr int kMediumHit = -2;
 
+// TEX = false leaves the albedo textures out (a scene without them)
+template <bool TEX = true>
 __device__ inline void make_hit(const SceneView& s, f3 o, f3 d, float t, int prim, HitInfo& h) {
     h.t = t;
     int mat;
@@ -120,7 +122,7 @@ __device__ inline void make_hit(const SceneView& s, f3 o, f3 d, float t, int pri
     h.fuzz = m.fuzz;
     h.ior = m.ior;
     h.base = ld3(m.albedo);
-    if (s.mat_albedo_tex != nullptr) {
+    if (TEX && s.mat_albedo_tex != nullptr) {
         // populate_material's albedo texture (programs.cu:978-979)
         const int ti = s.mat_albedo_tex[mat];
         if (ti >= 0) {
@@ -346,6 +348,9 @@ __device__ inline LightSample sample_light(const SceneView& s, const HitInfo& h,
     return ls;
 }
 
+// PBR = false leaves the OpenPBR lobes out (a response of 0): the shade
+// stage's legacy-material pass, which hands OpenPBR hits to a second pass
+template <bool PBR = true>
 __device__ inline f3 direct_response(const SceneView& s, const HitInfo& h, f3 ray_d, f3 ldir,
     float* pdf) {
     *pdf = 0.0f;
@@ -358,11 +363,15 @@ __device__ inline f3 direct_response(const SceneView& s, const HitInfo& h, f3 ra
         *pdf = kUniformSpherePdf;
         return h.base * *pdf;
     }
-    if (h.mat_type != RT_MAT_OPENPBR) return mk3(0, 0, 0);
-    f3 value;
-    openpbr_eval(s.openpbr[h.openpbr], h.gn, -normalize3(ray_d), ldir, &value, pdf);
-    const float cs = fabsf(dot3(h.sn, ldir));
-    return value * cs;
+    if constexpr (!PBR) {
+        return mk3(0, 0, 0);
+    } else {
+        if (h.mat_type != RT_MAT_OPENPBR) return mk3(0, 0, 0);
+        f3 value;
+        openpbr_eval(s.openpbr[h.openpbr], h.gn, -normalize3(ray_d), ldir, &value, pdf);
+        const float cs = fabsf(dot3(h.sn, ldir));
+        return value * cs;
+    }
 }
 
 // ---- analytic lights (PackedAnalyticLight, light_sampling.h:30-62)
@@ -772,6 +781,7 @@ struct BsdfResult {
     bool delta, alive;
 };
 
+template <bool PBR = true>
 __device__ inline BsdfResult sample_bsdf(const SceneView& s, const HitInfo& h, f3 ray_d, float4 r) {
     BsdfResult b{};
     b.alive = false;
@@ -818,7 +828,7 @@ __device__ inline BsdfResult sample_bsdf(const SceneView& s, const HitInfo& h, f
         b.alive = true;
         return b;
     }
-    if (h.mat_type == RT_MAT_OPENPBR) {
+    if (PBR && h.mat_type == RT_MAT_OPENPBR) {
         OpenPbrSampleResult smp;
         openpbr_sample(s.openpbr[h.openpbr], h.gn, -normalize3(ray_d), r.x, r.y, r.z, &smp);
         if (!smp.valid) return b;

@@ -351,10 +351,14 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) {
 }
 
 // Persistent traversal over a queue of `count` rays.  Each wave claims the
-// queue in batches of 64 entries (one atomic per batch) and hands them to its
-// idle lanes once at least `refill_min` lanes are idle: refill_min = 64 runs
-// whole coherent batches (classic persistent threads), smaller values let a
-// lane take a new ray as soon as its own is done.
+// queue in batches (one atomic per batch) and hands them to its idle lanes
+// once at least `refill_min` lanes are idle: refill_min = 64 runs whole
+// coherent batches (classic persistent threads), smaller values let a lane
+// take a new ray as soon as its own is done.  A batch is 64 entries, or 128
+// when the queue holds at least 32 batches of 64 per wave of the grid: the
+// claims are same-address atomics, which serialise at one memory channel and
+// limit the large launches, while a larger batch would leave a longer tail
+// of unclaimed work at the end of the small ones.
 //   load(q)               : ray of queue entry q
 //   done(q, result, cnt)  : store the result of queue entry q
 template <int BLOCK, bool ANY, bool COUNT, class Load, class Done>
@@ -370,17 +374,19 @@ __device__ __forceinline__ void trace_queue(const SceneView& s, uint32_t count, 
     TraverseCounts c{0u, 0u, 0u};
     const bool has_tree = s.prim_count > 0 && s.node4_count > 0;
     if (count == 0u) return;  // no wave touches the fetch counter of an empty queue
+    const uint32_t waves = gridDim.x * (uint32_t)(BLOCK / 64);
+    const uint32_t batch = count / 64u / waves >= 32u ? 128u : 64u;
     while (true) {
         const uint64_t idle = __ballot(!active);
         const int nidle = __popcll(idle);
         if (nidle >= refill_min || nidle == 64) {
             if (batch_left == 0u && !exhausted) {
                 uint32_t base = 0u;
-                if (lane == 0) base = atomicAdd(fetch, 64u);
+                if (lane == 0) base = atomicAdd(fetch, batch);
                 base = (uint32_t)__shfl((int)base, 0);
                 batch_base = base;
-                batch_left = base < count ? (count - base < 64u ? count - base : 64u) : 0u;
-                if (base + 64u >= count) exhausted = true;
+                batch_left = base < count ? (count - base < batch ? count - base : batch) : 0u;
+                if (base + batch >= count) exhausted = true;
             }
             const uint32_t take = (uint32_t)nidle < batch_left ? (uint32_t)nidle : batch_left;
             if (take > 0u) {

@@ -39,11 +39,16 @@ enum : uint32_t { kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntShadow2 = 
     records of the frame */, kCntCacheDone = 22 /* records with targets */,
     kCntCacheQueries = 23 /* cache queries of the frame */,
     kCntAnyVisits = 24, kCntAnyTests = 26, kCntAnySphereTests = 28 /* u64 traversal stats of
-    the any-hit rays */, kCounterWords = 30 };
+    the any-hit rays */, kCntPbr = 30 /* OpenPBR hits handed from the legacy-material shade
+    pass to the OpenPBR pass */, kCounterWords = 31 };
 static_assert(kCntCacheRecord == kStatCacheRecords && kCntCacheQueries == kStatCacheQueries
     && kCounterWords == kStatWords, "counter layout");
 
 enum : int32_t { kPrevValid = 1, kPrevDelta = 2 };
+
+// feature bits of the shade kernel's variants
+enum : uint32_t { kShadePbr = 1, kShadeAnalytic = 2, kShadeCache = 4, kShadeTexture = 8,
+    kShadeAll = 15 };
 
 struct SlotId {
     uint32_t pixel;   // pixel index inside its camera
@@ -98,6 +103,45 @@ __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
     base = (uint32_t)__shfl((int)base, leader);
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     return base + (uint32_t)__popcll(mask & lt);
+}
+
+// Queue appends of a 256-thread block for N queues at once: one atomic per
+// queue per block, all N in flight together, instead of one per queue per
+// wave in sequence (the shade stage's appends are same-address atomics that
+// serialise at one memory channel). Lanes keep their order inside a wave and
+// waves their order inside the block. Every thread of the block must call it
+// (the grid-stride loops give every wave the same trip count); `lds` holds
+// four rows of N words.
+template <int N>
+__device__ __forceinline__ void block_append(const bool (&pred)[N], uint32_t* counters,
+    const uint32_t (&counter)[N], uint32_t (&pos)[N], uint32_t (*lds)[N]) {
+    const int lane = (int)(threadIdx.x & 63u);
+    const int w = (int)(threadIdx.x >> 6);
+    uint64_t m[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        m[i] = __ballot(pred[i]);
+        if (lane == 0) lds[w][i] = (uint32_t)__popcll(m[i]);
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)N) {
+        const int i = (int)threadIdx.x;
+        uint32_t ci = 0u;
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            if (j == i) ci = counter[j];
+        const uint32_t c0 = lds[0][i], c1 = lds[1][i], c2 = lds[2][i], c3 = lds[3][i];
+        const uint32_t total = c0 + c1 + c2 + c3;
+        const uint32_t base = total != 0u ? atomicAdd(counters + ci, total) : 0u;
+        lds[0][i] = base;
+        lds[1][i] = base + c0;
+        lds[2][i] = base + c0 + c1;
+        lds[3][i] = base + c0 + c1 + c2;
+    }
+    __syncthreads();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int i = 0; i < N; ++i) pos[i] = lds[w][i] + (uint32_t)__popcll(m[i] & lt);
 }
 
 __global__ void k_reset_counters(uint32_t* counters) {
@@ -279,6 +323,7 @@ __global__ void k_after_shade(uint32_t* counters, int cur) {
     if (threadIdx.x == 0) {
         counters[cur] = 0u;
         counters[kCntRestir] = 0u;
+        counters[kCntPbr] = 0u;
     }
 }
 
@@ -300,17 +345,29 @@ __device__ __forceinline__ CacheInput cache_input(const SceneView& s, const HitI
     return c;
 }
 
-__global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, FrameParams fp,
-    WavefrontBuffers b, int bounce, int cur) {
+// One shade pass over a queue of paths, compiled for a feature set F
+// (kShade* bits). Leaving features out shrinks the register budget, so more
+// waves are in flight: the kernel is bound by the latency of its gathers.
+// Without kShadePbr (LITE) the pass hands paths whose hit is OpenPBR to the
+// pbr queue, untouched, and the full pass shades them next. Each path's
+// result depends only on its own slot, so the split changes queue order,
+// never values.
+template <uint32_t F>
+__device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView& cams,
+    const FrameParams& fp, const WavefrontBuffers& b, int bounce, int cur, const uint32_t* queue,
+    const uint32_t* count_ptr) {
+    constexpr bool LITE = (F & kShadePbr) == 0;
+    constexpr bool ANALYTIC = (F & kShadeAnalytic) != 0;
+    constexpr bool CACHE = (F & kShadeCache) != 0;
     const SlotRng slot_rng_v = slot_rng(fp);
-    const uint32_t count = b.counters[cur];
-    const uint32_t* queue = b.queue[cur];
+    const uint32_t count = *count_ptr;
     uint32_t* next_queue = b.queue[cur ^ 1];
+    __shared__ uint32_t append_lds[4][5];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t nq = (count + stride - 1u) / stride * stride;  // uniform trip count per wave
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
         const bool active = q < count;
-        bool shadow = false, shadow2 = false, survive = false, defer = false;
+        bool shadow = false, shadow2 = false, survive = false, defer = false, pbr = false;
         uint32_t slot = 0u;
         float4 sh_o = make_float4(0, 0, 0, 0), sh_d = sh_o, sh_c = sh_o;
         float4 s2_o = sh_o, s2_d = sh_o, s2_c = sh_o;
@@ -335,7 +392,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
             pv.prim = __float_as_int(pn.w);
             const int prim = __float_as_int(hv.y);
             const int ali = __float_as_int(hv.z);
-            if (ali >= 0) {
+            if (ANALYTIC && ali >= 0) {
                 // path ends on an analytic light (programs.cu:1857-1866)
                 const float w = analytic_mis(s, pv, ali, ro, rd);
                 rad = rad + thr * (ld3(s.analytic[ali].radiance) * w);
@@ -343,14 +400,18 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                 // a miss: background, then the dome / distant lights (programs.cu:1867-1875)
                 const float w = emission_mis(s, pv, rd, nullptr, true);
                 rad = rad + thr * (background_at(s, rd) * w);
-                if (s.a_infinite_count > 0) rad = rad + thr * analytic_infinite(s, pv, rd);
+                if (ANALYTIC && s.a_infinite_count > 0) rad = rad + thr * analytic_infinite(s, pv, rd);
             } else {
                 const float4 rl = rng4(id.pixel, id.sample, id.key, (uint32_t)bounce,
                     kGroupLight, id.stream);
                 const float4 rb = rng4(id.pixel, id.sample, id.key, (uint32_t)bounce,
                     kGroupBsdf, id.stream);
                 HitInfo h;
-                make_hit(s, ro, rd, hv.x, prim, h);
+                make_hit<(F & kShadeTexture) != 0>(s, ro, rd, hv.x, prim, h);
+                if (LITE && h.mat_type == RT_MAT_OPENPBR) {
+                    pbr = true;
+                    goto handed_over;
+                }
                 if (bounce == 0) {
                     b.aux_normal[slot] = f4(h.sn, 1.0f);
                     b.aux_albedo[slot] = f4(h.base, h.t);
@@ -361,7 +422,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                 // record and go on with the plain integrator; once the cache
                 // is usable the other paths end here with its prediction
                 bool cache_end = false;
-                if (bounce == fp.cache_bounce && h.mat_type != RT_MAT_DIFFUSE_LIGHT) {
+                if (CACHE && bounce == fp.cache_bounce && h.mat_type != RT_MAT_DIFFUSE_LIGHT) {
                     const float4 rc = rng4(id.pixel, id.sample, id.key, 0u, kGroupCache, id.stream);
                     const bool train = rc.x < fp.cache_state->train_fraction;
                     const bool query = !train && fp.cache_state->usable != 0;
@@ -390,7 +451,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                     const LightSample ls = sample_light(s, h, rl);
                     if (ls.valid) {
                         float bpdf = 0.0f;
-                        const f3 resp = direct_response(s, h, rd, ls.dir, &bpdf);
+                        const f3 resp = direct_response<!LITE>(s, h, rd, ls.dir, &bpdf);
                         if (maxc3(resp) > 0.0f) {
                             const f3 so = h.pos + ls.dir * (kRayEps * 2.0f);
                             const float max_t = ls.infinite ? kRayFar : ls.dist - kRayEps * 3.0f;
@@ -407,7 +468,8 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                 }
                 // analytic lights: ReSTIR stage at the first vertex, otherwise one
                 // light sample (programs.cu:2282-2313)
-                if (fp.restir != 0 && bounce == 0) {
+                if (!ANALYTIC) {
+                } else if (fp.restir != 0 && bounce == 0) {
                     if (h.mat_type != RT_MAT_DIFFUSE_LIGHT && s.analytic_count > 0) {
                         defer = true;
                         b.restir_o[slot] = f4(ro, 0.0f);
@@ -421,7 +483,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                     const LightSample ls = analytic_candidate(s, h.pos, li, ra.y, ra.z);
                     if (ls.valid) {
                         float bpdf = 0.0f;
-                        const f3 resp = direct_response(s, h, rd, ls.dir, &bpdf);
+                        const f3 resp = direct_response<!LITE>(s, h, rd, ls.dir, &bpdf);
                         if (maxc3(resp) > 0.0f) {
                             const float max_t = ls.infinite ? kRayFar : ls.dist - kRayEps * 3.0f;
                             if (max_t > kRayEps) {
@@ -436,7 +498,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
                     }
                 }
                 if (bsdf_avail) {
-                    const BsdfResult bs = sample_bsdf(s, h, rd, rb);
+                    const BsdfResult bs = sample_bsdf<!LITE>(s, h, rd, rb);
                     if (bs.alive) {
                         thr = thr * bs.weight;
                         bool alive = true;
@@ -460,23 +522,39 @@ __global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, Fra
             }
             b.radiance[slot] = f4(rad, 0.0f);
         }
-        const uint32_t spos = wave_append(shadow, &b.counters[kCntShadow]);
+    handed_over:
+        const bool pred[5] = {survive, shadow, shadow2, defer, pbr};
+        const uint32_t counter[5] = {(uint32_t)(cur ^ 1), kCntShadow, kCntShadow2, kCntRestir,
+            kCntPbr};
+        uint32_t pos[5];
+        block_append<5>(pred, b.counters, counter, pos, append_lds);
+        if (survive) next_queue[pos[0]] = slot;
         if (shadow) {
-            b.shadow_o[spos] = sh_o;
-            b.shadow_d[spos] = sh_d;
-            b.shadow_c[spos] = sh_c;
+            b.shadow_o[pos[1]] = sh_o;
+            b.shadow_d[pos[1]] = sh_d;
+            b.shadow_c[pos[1]] = sh_c;
         }
-        const uint32_t s2pos = wave_append(shadow2, &b.counters[kCntShadow2]);
         if (shadow2) {
-            b.shadow2_o[s2pos] = s2_o;
-            b.shadow2_d[s2pos] = s2_d;
-            b.shadow2_c[s2pos] = s2_c;
+            b.shadow2_o[pos[2]] = s2_o;
+            b.shadow2_d[pos[2]] = s2_d;
+            b.shadow2_c[pos[2]] = s2_c;
         }
-        const uint32_t rpos = wave_append(defer, &b.counters[kCntRestir]);
-        if (defer) b.restir_queue[rpos] = slot;
-        const uint32_t npos = wave_append(survive, &b.counters[cur ^ 1]);
-        if (survive) next_queue[npos] = slot;
+        if (defer) b.restir_queue[pos[3]] = slot;
+        if (pbr) b.pbr_queue[pos[4]] = slot;
     }
+}
+
+template <uint32_t F>
+__global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, FrameParams fp,
+    WavefrontBuffers b, int bounce, int cur, const uint32_t* queue, const uint32_t* count_ptr) {
+    shade_pass<F>(s, cams, fp, b, bounce, cur, queue, count_ptr);
+}
+
+// the plain legacy-material pass fits 96 registers: five waves per SIMD
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_shade_plain(
+    SceneView s, CameraView cams, FrameParams fp, WavefrontBuffers b, int bounce, int cur,
+    const uint32_t* queue, const uint32_t* count_ptr) {
+    shade_pass<0>(s, cams, fp, b, bounce, cur, queue, count_ptr);
 }
 
 // ReSTIR DI at the first vertex for the deferred paths; its shadow ray joins
@@ -684,8 +762,26 @@ hipError_t wf_end_bounce(WavefrontBuffers& b, int bounce, hipStream_t st) {
 }
 
 hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParams& fp,
-    WavefrontBuffers& b, int bounce, StageLaunch l, hipStream_t st) {
-    k_shade<<<l.grid, 256, 0, st>>>(s, cams, fp, b, bounce, bounce & 1);
+    WavefrontBuffers& b, int bounce, StageLaunch l, StageLaunch lite, int mode, hipStream_t st) {
+    const int cur = bounce & 1;
+    if (mode == 0) {
+        k_shade<kShadeAll><<<l.grid, 256, 0, st>>>(s, cams, fp, b, bounce, cur, b.queue[cur],
+            b.counters + cur);
+        return hipGetLastError();
+    }
+    // the legacy-material pass: its plain variant when the scene and frame
+    // use no analytic light, radiance cache or albedo texture
+    const bool plain = s.analytic_count == 0 && s.a_infinite_count == 0 && fp.cache_bounce < 0
+                       && s.mat_albedo_tex == nullptr;
+    if (plain)
+        k_shade_plain<<<lite.grid, 256, 0, st>>>(s, cams, fp, b, bounce, cur, b.queue[cur],
+            b.counters + cur);
+    else
+        k_shade<kShadeAll & ~kShadePbr><<<l.grid, 256, 0, st>>>(s, cams, fp, b, bounce, cur,
+            b.queue[cur], b.counters + cur);
+    if (mode == 2)
+        k_shade<kShadeAll><<<l.grid, 256, 0, st>>>(s, cams, fp, b, bounce, cur, b.pbr_queue,
+            b.counters + kCntPbr);
     return hipGetLastError();
 }
 
@@ -710,12 +806,13 @@ int wf_restir_blocks_per_cu() {
 // their grids are exactly what the device keeps resident (a larger grid
 // leaves a second, partial round of waves behind the first)
 
-int wf_shade_blocks_per_cu() {
-    static std::atomic<int> cached{0};
-    int n = cached.load(std::memory_order_relaxed);
+int wf_shade_blocks_per_cu(bool lite) {
+    static std::atomic<int> cached[2]{{0}, {0}};
+    int n = cached[lite].load(std::memory_order_relaxed);
     if (n == 0) {
-        n = resident_blocks(reinterpret_cast<const void*>(k_shade), 256);
-        cached.store(n, std::memory_order_relaxed);
+        n = resident_blocks(lite ? reinterpret_cast<const void*>(k_shade_plain)
+                                 : reinterpret_cast<const void*>(k_shade<kShadeAll>), 256);
+        cached[lite].store(n, std::memory_order_relaxed);
     }
     return n;
 }
