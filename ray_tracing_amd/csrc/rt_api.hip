@@ -1352,7 +1352,7 @@ rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out)
         e = hipHostMalloc(reinterpret_cast<void**>(&r->host_dyn), sizeof(rtk::FrameDyn), hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreate(&r->post_ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&r->post_ev[1]);
-    if (e == hipSuccess) e = r->counters.reserve(32);
+    if (e == hipSuccess) e = r->counters.reserve(rtk::kCounterAlloc);
     if (e == hipSuccess) e = r->nn_guard.reserve(1);
     if (e == hipSuccess) {
         rtk::NnGuard g{};

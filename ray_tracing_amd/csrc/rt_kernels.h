@@ -158,6 +158,7 @@ hipError_t wf_cache_chunk(const CacheGridView& g, const CachePacked& pk, const F
 // device counters of the cache: records written this frame, queries answered
 const uint32_t* wf_cache_record_count(const WavefrontBuffers& b);
 constexpr int kStatWords = 31;  // counter words downloaded after a frame
+constexpr int kCounterAlloc = 1024;  // counter words allocated (queue counts, stats, cursors)
 constexpr int kStatCacheRecords = 21, kStatCacheQueries = 23;
 hipError_t wf_clear_accum(int pixels, WavefrontBuffers& b, hipStream_t st);
 // zero `bytes` (a multiple of 4) of device memory with a fill kernel

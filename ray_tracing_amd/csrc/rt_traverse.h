@@ -359,9 +359,16 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) {
 // claims are same-address atomics, which serialise at one memory channel and
 // limit the large launches, while a larger batch would leave a longer tail
 // of unclaimed work at the end of the small ones.
+// PARTS > 1 splits the queue into PARTS contiguous partitions with a cursor
+// each (fetch[p * kCursorStride], one cache line apart): a wave claims from
+// partition blockIdx % PARTS, then moves on to the next ones as they drain.
+// Workgroups are dispatched to the XCDs round robin, so with PARTS = 8 each
+// XCD starts on its own eighth of the queue: neighbouring rays (pixels) meet
+// the same BVH nodes in one L2, and the claims spread over eight addresses.
 //   load(q)               : ray of queue entry q
 //   done(q, result, cnt)  : store the result of queue entry q
-template <int BLOCK, bool ANY, bool COUNT, class Load, class Done>
+constexpr uint32_t kCursorStride = 64;  // words between partition cursors
+template <int BLOCK, bool ANY, bool COUNT, int PARTS = 1, class Load, class Done>
 __device__ __forceinline__ void trace_queue(const SceneView& s, uint32_t count, uint32_t* fetch,
     const TraceStack& st, int refill_min, Load load, Done done) {
     const int lane = (int)(threadIdx.x & 63u);
@@ -376,17 +383,26 @@ __device__ __forceinline__ void trace_queue(const SceneView& s, uint32_t count, 
     if (count == 0u) return;  // no wave touches the fetch counter of an empty queue
     const uint32_t waves = gridDim.x * (uint32_t)(BLOCK / 64);
     const uint32_t batch = count / 64u / waves >= 32u ? 128u : 64u;
+    uint32_t part = PARTS > 1 ? blockIdx.x % (uint32_t)PARTS : 0u;  // wave-uniform
+    uint32_t parts_left = (uint32_t)PARTS;
     while (true) {
         const uint64_t idle = __ballot(!active);
         const int nidle = __popcll(idle);
         if (nidle >= refill_min || nidle == 64) {
-            if (batch_left == 0u && !exhausted) {
+            while (batch_left == 0u && !exhausted) {
+                // claim a batch of the current partition; a partition whose
+                // cursor has passed its end sends the wave on to the next one
+                const uint32_t lo = (uint32_t)((uint64_t)count * part / (uint32_t)PARTS);
+                const uint32_t n = (uint32_t)((uint64_t)count * (part + 1u) / (uint32_t)PARTS) - lo;
                 uint32_t base = 0u;
-                if (lane == 0) base = atomicAdd(fetch, batch);
+                if (lane == 0) base = atomicAdd(fetch + part * kCursorStride, batch);
                 base = (uint32_t)__shfl((int)base, 0);
-                batch_base = base;
-                batch_left = base < count ? (count - base < batch ? count - base : batch) : 0u;
-                if (base + batch >= count) exhausted = true;
+                batch_base = lo + base;
+                batch_left = base < n ? (n - base < batch ? n - base : batch) : 0u;
+                if (base + batch >= n) {
+                    part = part + 1u == (uint32_t)PARTS ? 0u : part + 1u;
+                    if (--parts_left == 0u) exhausted = true;
+                }
             }
             const uint32_t take = (uint32_t)nidle < batch_left ? (uint32_t)nidle : batch_left;
             if (take > 0u) {

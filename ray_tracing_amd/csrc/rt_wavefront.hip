@@ -34,13 +34,17 @@ namespace {
 
 enum : uint32_t { kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntShadow2 = 3,
     kCntStatExtend = 4, kCntStatShadow = 6, kCntVisits = 8, kCntTests = 10, kCntSphereTests = 12,
-    kCntRestir = 14, kCntFetch = 16 /* 3 ray-fetch cursors: extend, connect 0, connect 1 */,
+    kCntRestir = 14,
     kCntCacheQuery = 20 /* cache queries of the chunk */, kCntCacheRecord = 21 /* training
     records of the frame */, kCntCacheDone = 22 /* records with targets */,
     kCntCacheQueries = 23 /* cache queries of the frame */,
     kCntAnyVisits = 24, kCntAnyTests = 26, kCntAnySphereTests = 28 /* u64 traversal stats of
     the any-hit rays */, kCntPbr = 30 /* OpenPBR hits handed from the legacy-material shade
-    pass to the OpenPBR pass */, kCounterWords = 31 };
+    pass to the OpenPBR pass */, kCounterWords = 31,
+    kCntCursor = 64 /* kTraceParts ray-fetch cursors of the traversal launch, kCursorStride
+    words apart */ };
+constexpr int kTraceParts = 8;
+static_assert(kCntCursor + kTraceParts * kCursorStride <= kCounterAlloc, "cursor layout");
 static_assert(kCntCacheRecord == kStatCacheRecords && kCntCacheQueries == kStatCacheQueries
     && kCounterWords == kStatWords, "counter layout");
 
@@ -146,6 +150,7 @@ __device__ __forceinline__ void block_append(const bool (&pred)[N], uint32_t* co
 
 __global__ void k_reset_counters(uint32_t* counters) {
     if (threadIdx.x < kCounterWords) counters[threadIdx.x] = 0u;
+    if (threadIdx.x < kTraceParts) counters[kCntCursor + threadIdx.x * kCursorStride] = 0u;
 }
 
 __global__ void k_clear_accum(int pixels, float4* a, float4* b, float4* c) {
@@ -168,12 +173,10 @@ __global__ __launch_bounds__(256) void k_raygen(CameraView cams, FrameParams fp,
     int paths) {
     const SlotRng slot_rng_v = slot_rng(fp);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kTraceParts) b.counters[kCntCursor + i * kCursorStride] = 0u;
     if (i == 0) {
         b.counters[kCntQueue0] = (uint32_t)paths;
         b.counters[kCntQueue1] = 0u;
-        b.counters[kCntFetch] = 0u;
-        b.counters[kCntFetch + 1] = 0u;
-        b.counters[kCntFetch + 2] = 0u;
         b.counters[kCntShadow] = 0u;
         b.counters[kCntShadow2] = 0u;
         b.counters[kCntRestir] = 0u;
@@ -239,7 +242,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace_fused(SceneView s, WavefrontBuf
     const TraceStack st = lane_stack<BLOCK>(lds, b.spill);
     const uint32_t* queue = b.queue[cur];
     TraverseCounts tot{0u, 0u, 0u}, tot_any{0u, 0u, 0u};
-    trace_queue<BLOCK, false, COUNT>(s, ne + n1 + n2, b.counters + kCntFetch, st, refill_min,
+    trace_queue<BLOCK, false, COUNT, kTraceParts>(s, ne + n1 + n2, b.counters + kCntCursor, st,
+        refill_min,
         [&](uint32_t q) {
             if (q < ne) {
                 const uint32_t slot = queue[q];
@@ -313,8 +317,8 @@ __global__ void k_after_trace(uint32_t* counters) {
     if (threadIdx.x == 0) {
         counters[kCntShadow] = 0u;
         counters[kCntShadow2] = 0u;
-        counters[kCntFetch] = 0u;
     }
+    if (threadIdx.x < kTraceParts) counters[kCntCursor + threadIdx.x * kCursorStride] = 0u;
 }
 
 // after a bounce's shade (and ReSTIR) stage: the consumed path queue and the
