@@ -322,6 +322,13 @@ def lib():
         L.rto_openpbr_sample_fn.argtypes = [P(OpenPbr), C.c_void_p, C.c_void_p, C.c_float,
                                             C.c_float, C.c_float, C.c_void_p]
         L.rto_openpbr_emission.argtypes = [P(OpenPbr), C.c_void_p]
+        L.rto_openpbr_evaluate_ctx.argtypes = [P(OpenPbr)] + [C.c_void_p] * 5
+        L.rto_openpbr_sample_ctx.argtypes = [P(OpenPbr), C.c_void_p, C.c_void_p, C.c_float,
+                                             C.c_float, C.c_float, C.c_void_p, C.c_void_p]
+        L.rto_openpbr_subsurface_medium.argtypes = [P(OpenPbr), C.c_void_p]
+        L.rto_openpbr_subsurface_segment.argtypes = [P(OpenPbr), C.c_float, C.c_float,
+                                                     C.c_void_p, C.c_void_p]
+        L.rto_openpbr_hg.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_void_p]
         L.rto_generate_rig.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, P(Camera),
                                        P(C.c_int)]
         L.rto_render_camera_ex.argtypes = [P(Scene), P(Bvh), P(Camera), C.c_int, P(Profile),
@@ -381,6 +388,13 @@ def ref_lib():
         R.ref_openpbr_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
                                          C.c_float, C.c_float, C.c_void_p]
         R.ref_openpbr_emission.argtypes = [C.c_void_p, C.c_void_p]
+        R.ref_openpbr_evaluate_ctx.argtypes = [C.c_void_p] * 6
+        R.ref_openpbr_sample_ctx.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
+                                             C.c_float, C.c_float, C.c_void_p, C.c_void_p]
+        R.ref_openpbr_subsurface_medium.argtypes = [C.c_void_p, C.c_void_p]
+        R.ref_openpbr_subsurface_segment.argtypes = [C.c_void_p, C.c_float, C.c_float,
+                                                     C.c_void_p, C.c_void_p]
+        R.ref_openpbr_hg.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_void_p]
         R.ref_openpbr_material_floats.restype = C.c_int
         for name in ("ref_light_uniform_cone_pdf", "ref_light_power_heuristic"):
             getattr(R, name).restype = C.c_float
@@ -421,20 +435,31 @@ def openpbr_material(**kw) -> "OpenPbr":
     return m
 
 
-def openpbr_evaluate(m, n, wo, wi):
+def openpbr_evaluate(m, n, wo, wi, thr=None):
+    """evaluate_openpbr_core; thr = the transport context's path throughput."""
     out = np.zeros(4, np.float32)
     a = [np.asarray(x, np.float32) for x in (n, wo, wi)]
-    lib().rto_openpbr_evaluate(C.byref(m), ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(out))
+    if thr is None:
+        lib().rto_openpbr_evaluate(C.byref(m), ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(out))
+    else:
+        t = np.asarray(thr, np.float32)
+        lib().rto_openpbr_evaluate_ctx(C.byref(m), ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(t),
+                                       ptr(out))
     return out[:3].copy(), float(out[3])
 
 
-def openpbr_sample(m, n, wo, u_lobe, u1, u2):
+def openpbr_sample(m, n, wo, u_lobe, u1, u2, thr=None):
     class _S(C.Structure):
         _fields_ = [("wi", f3), ("weight", f3), ("pdf", C.c_float), ("delta", C.c_int32),
                     ("valid", C.c_int32), ("event", C.c_int32)]
     s = _S()
     a = [np.asarray(x, np.float32) for x in (n, wo)]
-    lib().rto_openpbr_sample_fn(C.byref(m), ptr(a[0]), ptr(a[1]), u_lobe, u1, u2, C.byref(s))
+    if thr is None:
+        lib().rto_openpbr_sample_fn(C.byref(m), ptr(a[0]), ptr(a[1]), u_lobe, u1, u2, C.byref(s))
+    else:
+        t = np.asarray(thr, np.float32)
+        lib().rto_openpbr_sample_ctx(C.byref(m), ptr(a[0]), ptr(a[1]), u_lobe, u1, u2, ptr(t),
+                                     C.byref(s))
     return {"wi": np.array(s.wi[:], np.float32), "weight": np.array(s.weight[:], np.float32),
             "pdf": s.pdf, "delta": s.delta, "valid": s.valid, "event": s.event}
 
@@ -442,6 +467,28 @@ def openpbr_sample(m, n, wo, u_lobe, u1, u2):
 def openpbr_emission(m):
     out = np.zeros(3, np.float32)
     lib().rto_openpbr_emission(C.byref(m), ptr(out))
+    return out
+
+
+def openpbr_subsurface_medium(m):
+    """(extinction[3], albedo[3], anisotropy, active) of openpbr_subsurface_medium."""
+    out = np.zeros(8, np.float32)
+    lib().rto_openpbr_subsurface_medium(C.byref(m), ptr(out))
+    return out
+
+
+def openpbr_subsurface_segment(m, boundary, u, thr):
+    """(weight[3], distance, scattered) of one random-walk segment."""
+    out = np.zeros(5, np.float32)
+    t = np.asarray(thr, np.float32)
+    lib().rto_openpbr_subsurface_segment(C.byref(m), boundary, u, ptr(t), ptr(out))
+    return out
+
+
+def openpbr_hg(incident, g, u1, u2):
+    out = np.zeros(3, np.float32)
+    a = np.asarray(incident, np.float32)
+    lib().rto_openpbr_hg(ptr(a), g, u1, u2, ptr(out))
     return out
 
 

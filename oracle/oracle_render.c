@@ -792,7 +792,7 @@ static v3 direct_response(const ctx_t* c, const hit_t* h, const path_t* st, v3 l
     if (h->mat_type != RTO_MAT_OPENPBR) return mk3(0, 0, 0);
     const rto_openpbr* p = &c->s->openpbr[h->openpbr];
     v3 value;
-    om_openpbr_eval(p, h->gn, neg3(norm3(st->d)), ldir, &value, pdf);
+    om_openpbr_eval(p, h->gn, neg3(norm3(st->d)), ldir, st->throughput, &value, pdf);
     const float cs = fabsf(dot3(h->sn, ldir));
     return scl3(value, cs);
 }
@@ -1525,7 +1525,7 @@ static void sample_bsdf(const ctx_t* c, const hit_t* h, const float r[4], path_t
     if (h->mat_type == RTO_MAT_OPENPBR) {
         const rto_openpbr* p = &c->s->openpbr[h->openpbr];
         rto_openpbr_sample smp;
-        om_openpbr_sample(p, h->gn, neg3(norm3(st->d)), r[0], r[1], r[2], &smp);
+        om_openpbr_sample(p, h->gn, neg3(norm3(st->d)), r[0], r[1], r[2], st->throughput, &smp);
         if (!smp.valid) { st->alive = 0; return; }
         const v3 dir = ld3(smp.wi);
         st->o = add3(h->pos, scl3(dir, OM_RAY_EPS));

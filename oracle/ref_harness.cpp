@@ -75,6 +75,62 @@ void ref_openpbr_sample(const float* material, const float* n, const float* wo, 
     out[9] = static_cast<float>(static_cast<int>(s.event));
 }
 
+// the same with the transport context's path throughput
+void ref_openpbr_evaluate_ctx(const float* material, const float* n, const float* wo,
+    const float* wi, const float* thr, float* value_pdf) {
+    const rt::OpenPbrCoreMaterial m = to_ref(material);
+    const rt::OpenPbrFrame frame = rt::make_openpbr_frame(vec(n), rt::OpenPbrVec3{});
+    rt::OpenPbrTransportContext ctx{};
+    ctx.path_throughput = vec(thr);
+    const rt::OpenPbrEvaluation e = rt::evaluate_openpbr_core(m, frame, vec(wo), vec(wi), ctx);
+    value_pdf[0] = e.value.x;
+    value_pdf[1] = e.value.y;
+    value_pdf[2] = e.value.z;
+    value_pdf[3] = e.pdf;
+}
+
+void ref_openpbr_sample_ctx(const float* material, const float* n, const float* wo, float u_lobe,
+    float u1, float u2, const float* thr, float* out) {
+    const rt::OpenPbrCoreMaterial m = to_ref(material);
+    const rt::OpenPbrFrame frame = rt::make_openpbr_frame(vec(n), rt::OpenPbrVec3{});
+    rt::OpenPbrTransportContext ctx{};
+    ctx.path_throughput = vec(thr);
+    const rt::OpenPbrSample s = rt::sample_openpbr_core(m, frame, vec(wo), u_lobe, u1, u2, ctx);
+    out[0] = s.wi.x; out[1] = s.wi.y; out[2] = s.wi.z;
+    out[3] = s.weight.x; out[4] = s.weight.y; out[5] = s.weight.z;
+    out[6] = s.pdf;
+    out[7] = static_cast<float>(s.delta);
+    out[8] = static_cast<float>(s.valid);
+    out[9] = static_cast<float>(static_cast<int>(s.event));
+}
+
+// openpbr_subsurface_medium: extinction[3], albedo[3], anisotropy, active
+void ref_openpbr_subsurface_medium(const float* material, float* out) {
+    const rt::OpenPbrSubsurfaceMedium md = rt::openpbr_subsurface_medium(to_ref(material));
+    out[0] = md.extinction.x; out[1] = md.extinction.y; out[2] = md.extinction.z;
+    out[3] = md.single_scattering_albedo.x; out[4] = md.single_scattering_albedo.y;
+    out[5] = md.single_scattering_albedo.z;
+    out[6] = md.anisotropy;
+    out[7] = static_cast<float>(md.active);
+}
+
+// openpbr_sample_subsurface_segment over the material's medium: weight[3], distance, scattered
+void ref_openpbr_subsurface_segment(const float* material, float boundary, float u,
+    const float* thr, float* out) {
+    const rt::OpenPbrSubsurfaceMedium md = rt::openpbr_subsurface_medium(to_ref(material));
+    rt::OpenPbrTransportContext ctx{};
+    ctx.path_throughput = vec(thr);
+    const rt::OpenPbrSubsurfaceSegment sg = rt::openpbr_sample_subsurface_segment(md, boundary, u, ctx);
+    out[0] = sg.weight.x; out[1] = sg.weight.y; out[2] = sg.weight.z;
+    out[3] = sg.distance;
+    out[4] = static_cast<float>(sg.scattered);
+}
+
+void ref_openpbr_hg(const float* incident, float g, float u1, float u2, float* out) {
+    const rt::OpenPbrVec3 d = rt::openpbr_sample_henyey_greenstein(vec(incident), g, u1, u2);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
 void ref_openpbr_emission(const float* material, float* out) {
     const rt::OpenPbrVec3 e = rt::emission_openpbr_core(to_ref(material));
     out[0] = e.x; out[1] = e.y; out[2] = e.z;

@@ -377,7 +377,8 @@ float rto_light_pdf_for_hit(const rto_scene* s, const rto_bvh* bvh, int prim, in
 int rto_scatter(const rto_scene* s, int prim, const float pos[3], const float gn[3],
     const float d[3], const float r[4], float out[8]);
 
-/* OpenPBR subset (src/common/openpbr_core.h) */
+/* OpenPBR (src/common/openpbr_core.h); the _ctx forms take the path
+ * throughput of the transport context, the others use (1, 1, 1) */
 typedef struct { float value[3]; float pdf; } rto_openpbr_eval;
 typedef struct { float wi[3]; float weight[3]; float pdf; int32_t delta; int32_t valid; int32_t event; } rto_openpbr_sample;
 void rto_openpbr_evaluate(const rto_openpbr* m, const float n[3], const float wo[3],
@@ -385,6 +386,17 @@ void rto_openpbr_evaluate(const rto_openpbr* m, const float n[3], const float wo
 void rto_openpbr_emission(const rto_openpbr* m, float out[3]);
 void rto_openpbr_sample_fn(const rto_openpbr* m, const float n[3], const float wo[3],
     float u_lobe, float u1, float u2, rto_openpbr_sample* out);
+void rto_openpbr_evaluate_ctx(const rto_openpbr* m, const float n[3], const float wo[3],
+    const float wi[3], const float thr[3], rto_openpbr_eval* out);
+void rto_openpbr_sample_ctx(const rto_openpbr* m, const float n[3], const float wo[3],
+    float u_lobe, float u1, float u2, const float thr[3], rto_openpbr_sample* out);
+/* openpbr_subsurface_medium: extinction[3], albedo[3], anisotropy, active */
+void rto_openpbr_subsurface_medium(const rto_openpbr* m, float out[8]);
+/* openpbr_sample_subsurface_segment: weight[3], distance, scattered */
+void rto_openpbr_subsurface_segment(const rto_openpbr* m, float boundary, float u,
+    const float thr[3], float out[5]);
+/* openpbr_sample_henyey_greenstein */
+void rto_openpbr_hg(const float incident[3], float g, float u1, float u2, float out[3]);
 
 #ifdef __cplusplus
 }
