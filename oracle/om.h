@@ -77,7 +77,7 @@ static inline void om_rng4(uint32_t pixel, uint32_t sample, uint32_t camera, uin
     out[0] = om_u01(v[0]); out[1] = om_u01(v[1]); out[2] = om_u01(v[2]); out[3] = om_u01(v[3]);
 }
 /* RNG groups within one bounce */
-enum { OM_G_LIGHT = 0, OM_G_BSDF = 1, OM_G_ALIGHT = 2, OM_G_RESTIR0 = 8, OM_G_MEDIUM0 = 48,
+enum { OM_G_LIGHT = 0, OM_G_BSDF = 1, OM_G_ALIGHT = 2, OM_G_SUBSURFACE = 4, OM_G_RESTIR0 = 8, OM_G_MEDIUM0 = 48,
        OM_G_CAMERA = 63 };
 
 /* ---- sin/cos of a whole turn, own Taylor polynomials.

@@ -558,6 +558,7 @@ typedef struct {
     v3 prev_pos, prev_n;
     float prev_pdf;
     int prev_mat, prev_prim, prev_valid, prev_delta, alive;
+    int inside;   /* OpenPBR index of the subsurface medium the path is in, -1 none */
 } path_t;
 
 static float power_heuristic(float a, float b) {
@@ -1325,6 +1326,8 @@ static float rs_target_at_source(const ctx_t* c, const rto_restir_io* io, const 
     if (!isfinite(dist) || dist <= OM_RAY_EPS) return 0.0f;
     path_t ss;
     memset(&ss, 0, sizeof(ss));
+    ss.throughput = mk3(1.0f, 1.0f, 1.0f);   /* a fresh PathState */
+    ss.inside = -1;
     ss.o = origin;
     ss.d = scl3(off, 1.0f / dist);
     float t, u, v;
@@ -1532,6 +1535,9 @@ static void sample_bsdf(const ctx_t* c, const hit_t* h, const float r[4], path_t
         st->d = norm3(dir);
         st->throughput = mul3(st->throughput, ld3(smp.weight));
         record_scatter(st, h, smp.pdf, smp.delta != 0);
+        /* subsurface_entry / subsurface_exit (programs.cu:2545-2551) */
+        if (smp.event == 8) st->inside = h->openpbr;
+        else if (smp.event == 9) st->inside = -1;
         return;
     }
     if (h->mat_type == RTO_MAT_ISOTROPIC) {
@@ -1567,6 +1573,7 @@ static void trace_one(const ctx_t* c, const rto_camera* cam, int cam_index, cons
     st.prev_prim = -1;
     st.prev_mat = -1;
     st.alive = 1;
+    st.inside = -1;
     out->aux = 0;
     out->normal = mk3(0, 0, 1);
     out->albedo = mk3(0, 0, 0);
@@ -1607,6 +1614,25 @@ static void trace_one(const ctx_t* c, const rto_camera* cam, int cam_index, cons
         }
         hit_t h;
         make_hit(s, st.o, st.d, t, prim, &h);
+        if (st.inside >= 0) {
+            /* the OpenPBR subsurface random walk (programs.cu:1878-1901): only
+             * the medium's own back face continues the path */
+            if (h.mat_type != RTO_MAT_OPENPBR || h.openpbr != st.inside || h.front) break;
+            float rw[4], seg[5];
+            om_rng4(pixel, (uint32_t)sample, (uint32_t)cam_index, (uint32_t)b, OM_G_SUBSURFACE, stream, rw);
+            const rto_openpbr* pm = &s->openpbr[st.inside];
+            const float rlen = sqrtf(lensq3(st.d));
+            om_openpbr_segment(pm, t * rlen, rw[0], st.throughput, seg);
+            st.throughput = mul3(st.throughput, ld3(seg));
+            if (seg[4] != 0.0f) {
+                const v3 dir = om_openpbr_hg(pm, norm3(st.d), rw[1], rw[2]);
+                st.o = add3(st.o, scl3(st.d, seg[3] / fmaxf(rlen, 1e-8f)));
+                st.d = dir;
+                st.o = add3(st.o, scl3(st.d, OM_RAY_EPS));
+                st.prev_valid = 0;
+                continue;
+            }
+        }
         if (b == 0) {
             out->normal = h.sn;
             out->albedo = h.base;

@@ -131,6 +131,7 @@ struct rt_scene {
     // shade passes: 0 one full pass (OpenPBR only), 1 the legacy-material
     // pass alone (no OpenPBR), 2 both (legacy pass, then its OpenPBR hits)
     int32_t shade_mode = 0;
+    bool pbr_full = false;   // an OpenPBR material uses lobes beyond the opaque core
     int32_t texture_count = 0, texel_count = 0;
     bool has_albedo_textures = false;
     bool has_triangle_uvs = false, has_triangle_normals = false;
@@ -193,6 +194,7 @@ struct rt_scene {
         v.materials = materials.p;
         v.openpbr = openpbr.p;
         v.openpbr_count = openpbr_count;
+        v.pbr_full = pbr_full ? 1 : 0;
         v.light_count = static_cast<int32_t>(host_lights.size());
         v.lights = lights.p;
         v.light_of_prim = light_of_prim.p;
@@ -599,16 +601,10 @@ static rt_status scene_validate(const rt_scene_desc* d) {
     const int64_t prims = (int64_t)d->sphere_count + d->quad_count + d->triangle_count;
     // leaf ranges address 28-bit sorted slots
     RT_REQUIRE(prims < (int64_t(1) << 28), RT_ERROR_INVALID_ARGUMENT, "scene primitive count exceeds 2^28");
-    for (int i = 0; i < d->openpbr_count; ++i) {
-        const rt_openpbr_material& m = d->openpbr[i];
-        RT_REQUIRE(m.transmission_weight <= 0.0f && m.subsurface_weight <= 0.0f && m.fuzz_weight <= 0.0f
-                       && m.thin_film_weight <= 0.0f && m.transmission_dispersion_scale <= 0.0f
-                       && m.specular_roughness_anisotropy == 0.0f
-                       && m.coat_roughness_anisotropy == 0.0f && m.geometry_thin_walled == 0,
-            RT_ERROR_UNSUPPORTED,
-            "OpenPBR material " + std::to_string(i)
-                + " uses a lobe outside the supported constant-input subset (base, specular, metal, coat, emission)");
-    }
+    // a path carries the OpenPBR index of the subsurface medium it is inside in
+    // 19 bits of its vertex flags
+    RT_REQUIRE(d->openpbr_count < (1 << 19) - 1, RT_ERROR_INVALID_ARGUMENT,
+        "scene OpenPBR material count exceeds 2^19 - 2");
     float prev_cdf = 0.0f;
     for (int i = 0; i < d->analytic_light_count; ++i) {
         const rt_analytic_light& L = d->analytic_lights[i];
@@ -703,6 +699,8 @@ static rt_status scene_fill(rt_scene* s, const rt_scene_desc* d) {
                 legacy = true;
         }
         s->shade_mode = !pbr ? 1 : (legacy ? 2 : 0);
+        s->pbr_full = false;
+        for (int i = 0; i < d->openpbr_count; ++i) s->pbr_full = s->pbr_full || rtk::opbr::needs_full(d->openpbr[i]);
     }
     s->background_mode = d->background_mode;
     for (int i = 0; i < 3; ++i) {

@@ -44,6 +44,7 @@ struct SceneView {
     const rt_material* materials;
     const rt_openpbr_material* openpbr;
     int32_t openpbr_count;
+    int32_t pbr_full;   // a material uses OpenPBR lobes beyond the opaque core
     int32_t light_count;
     const LightRec* lights;
     const int32_t* light_of_prim;

@@ -82,7 +82,7 @@ __device__ __forceinline__ float4 rng4(uint32_t pixel, uint32_t sample, uint32_t
     const uint4 h = pcg4d(make_uint4(pixel, sample, (camera << 16) | (bounce << 6) | group, stream));
     return make_float4(u01(h.x), u01(h.y), u01(h.z), u01(h.w));
 }
-enum : uint32_t { kGroupLight = 0, kGroupBsdf = 1, kGroupAnalytic = 2, kGroupCache = 3, kGroupRestir0 = 8,
+enum : uint32_t { kGroupLight = 0, kGroupBsdf = 1, kGroupAnalytic = 2, kGroupCache = 3, kGroupSubsurface = 4, kGroupRestir0 = 8,
     kGroupMedium0 = 48, kGroupCamera = 63 };
 
 // sin/cos of a whole turn u (angle 2*pi*u) by quadrant reduction in turns

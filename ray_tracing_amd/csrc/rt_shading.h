@@ -348,11 +348,15 @@ __device__ inline LightSample sample_light(const SceneView& s, const HitInfo& h,
     return ls;
 }
 
-// PBR = false leaves the OpenPBR lobes out (a response of 0): the shade
-// stage's legacy-material pass, which hands OpenPBR hits to a second pass
-template <bool PBR = true>
+// PBR selects the OpenPBR code: kPbrOff leaves it out (a response of 0: the
+// shade stage's legacy-material pass, which hands OpenPBR hits to a second
+// pass), kPbrCore the opaque core (scenes whose materials use no other
+// lobe), kPbrFull every lobe.  thr: the path throughput, the transport
+// context of the OpenPBR lobes.
+enum : int { kPbrOff = 0, kPbrCore = 1, kPbrFull = 2 };
+template <int PBR = kPbrFull>
 __device__ inline f3 direct_response(const SceneView& s, const HitInfo& h, f3 ray_d, f3 ldir,
-    float* pdf) {
+    f3 thr, float* pdf) {
     *pdf = 0.0f;
     if (h.mat_type == RT_MAT_LAMBERTIAN) {
         const float cs = fmaxf(dot3(h.sn, ldir), 0.0f);
@@ -363,12 +367,16 @@ __device__ inline f3 direct_response(const SceneView& s, const HitInfo& h, f3 ra
         *pdf = kUniformSpherePdf;
         return h.base * *pdf;
     }
-    if constexpr (!PBR) {
+    if constexpr (PBR == kPbrOff) {
         return mk3(0, 0, 0);
     } else {
         if (h.mat_type != RT_MAT_OPENPBR) return mk3(0, 0, 0);
         f3 value;
-        openpbr_eval(s.openpbr[h.openpbr], h.gn, -normalize3(ray_d), ldir, &value, pdf);
+        const rt_openpbr_material* pm = s.openpbr + h.openpbr;
+        if constexpr (PBR == kPbrCore)
+            openpbr_eval_v(opbr::MatCore(pm), h.gn, -normalize3(ray_d), ldir, thr, &value, pdf);
+        else
+            openpbr_eval_v(opbr::MatMem{pm}, h.gn, -normalize3(ray_d), ldir, thr, &value, pdf);
         const float cs = fabsf(dot3(h.sn, ldir));
         return value * cs;
     }
@@ -779,12 +787,15 @@ struct BsdfResult {
     f3 o, d, weight;
     float pdf;
     bool delta, alive;
+    int event;   // OpenPBR scatter event (kPbr*), 0 for the other materials
 };
 
-template <bool PBR = true>
-__device__ inline BsdfResult sample_bsdf(const SceneView& s, const HitInfo& h, f3 ray_d, float4 r) {
+template <int PBR = kPbrFull>
+__device__ inline BsdfResult sample_bsdf(const SceneView& s, const HitInfo& h, f3 ray_d, float4 r,
+    f3 thr) {
     BsdfResult b{};
     b.alive = false;
+    b.event = 0;
     if (h.mat_type == RT_MAT_DIFFUSE_LIGHT) return b;
     if (h.mat_type == RT_MAT_LAMBERTIAN) {
         f3 sc = h.sn + sphere_dir(r.x, r.y);
@@ -828,10 +839,15 @@ __device__ inline BsdfResult sample_bsdf(const SceneView& s, const HitInfo& h, f
         b.alive = true;
         return b;
     }
-    if (PBR && h.mat_type == RT_MAT_OPENPBR) {
+    if (PBR != kPbrOff && h.mat_type == RT_MAT_OPENPBR) {
         OpenPbrSampleResult smp;
-        openpbr_sample(s.openpbr[h.openpbr], h.gn, -normalize3(ray_d), r.x, r.y, r.z, &smp);
+        const rt_openpbr_material* pm = s.openpbr + h.openpbr;
+        if constexpr (PBR == kPbrCore)
+            openpbr_sample_v(opbr::MatCore(pm), h.gn, -normalize3(ray_d), r.x, r.y, r.z, thr, &smp);
+        else
+            openpbr_sample_v(opbr::MatMem{pm}, h.gn, -normalize3(ray_d), r.x, r.y, r.z, thr, &smp);
         if (!smp.valid) return b;
+        b.event = smp.event;
         b.o = h.pos + smp.wi * kRayEps;
         b.d = normalize3(smp.wi);
         b.weight = smp.weight;

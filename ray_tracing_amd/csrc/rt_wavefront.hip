@@ -48,11 +48,15 @@ static_assert(kCntCursor + kTraceParts * kCursorStride <= kCounterAlloc, "cursor
 static_assert(kCntCacheRecord == kStatCacheRecords && kCntCacheQueries == kStatCacheQueries
     && kCounterWords == kStatWords, "counter layout");
 
-enum : int32_t { kPrevValid = 1, kPrevDelta = 2 };
+// prev_flags: bit 0 valid, bit 1 delta, bits 8-11 material type + 1 of the
+// previous vertex, bits 12-30 the OpenPBR index + 1 of the subsurface medium
+// the path is inside (0: none)
+enum : int32_t { kPrevValid = 1, kPrevDelta = 2, kPrevMedium = 12 };
 
 // feature bits of the shade kernel's variants
+// (kShadePbrFull: every OpenPBR lobe, not only the opaque core)
 enum : uint32_t { kShadePbr = 1, kShadeAnalytic = 2, kShadeCache = 4, kShadeTexture = 8,
-    kShadeAll = 15 };
+    kShadeAll = 15, kShadePbrFull = 16 };
 
 struct SlotId {
     uint32_t pixel;   // pixel index inside its camera
@@ -361,6 +365,7 @@ __device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView&
     const FrameParams& fp, const WavefrontBuffers& b, int bounce, int cur, const uint32_t* queue,
     const uint32_t* count_ptr) {
     constexpr bool LITE = (F & kShadePbr) == 0;
+    constexpr int PBR = LITE ? kPbrOff : ((F & kShadePbrFull) != 0 ? kPbrFull : kPbrCore);
     constexpr bool ANALYTIC = (F & kShadeAnalytic) != 0;
     constexpr bool CACHE = (F & kShadeCache) != 0;
     const SlotRng slot_rng_v = slot_rng(fp);
@@ -389,7 +394,8 @@ __device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView&
             PathPrev pv;
             pv.valid = (flags & kPrevValid) != 0;
             pv.delta = (flags & kPrevDelta) != 0;
-            pv.mat = (flags >> 8) - 1;
+            pv.mat = ((flags >> 8) & 15) - 1;
+            const int inside = (flags >> kPrevMedium) - 1;   // subsurface medium, -1 none
             pv.pos = xyz(ppos);
             pv.pdf = ppos.w;
             pv.n = xyz(pn);
@@ -416,6 +422,36 @@ __device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView&
                     pbr = true;
                     goto handed_over;
                 }
+                // inside an OpenPBR subsurface medium (programs.cu:1878-1901): a
+                // hit on anything but the medium's own back face ends the path;
+                // otherwise a free-flight segment either scatters inside (a new
+                // ray along the phase function, with no vertex for MIS) or
+                // reaches the boundary, which is shaded as usual
+                bool walk_ended = false;
+                if (inside >= 0) {
+                    walk_ended = true;
+                    if (PBR == kPbrFull && h.mat_type == RT_MAT_OPENPBR && h.openpbr == inside
+                        && !h.front) {
+                        const float4 rw = rng4(id.pixel, id.sample, id.key, (uint32_t)bounce,
+                            kGroupSubsurface, id.stream);
+                        const rt_openpbr_material& pm = s.openpbr[inside];
+                        const float rl = len3(rd);
+                        const OpenPbrSegment sg = openpbr_segment(pm, hv.x * rl, rw.x, thr);
+                        thr = thr * sg.weight;
+                        walk_ended = sg.scattered;
+                        if (sg.scattered && bounce + 1 < fp.max_bounces) {
+                            const f3 dir = openpbr_hg(pm, normalize3(rd), rw.y, rw.z);
+                            const f3 o = (ro + rd * (sg.distance / fmaxf(rl, 1e-8f))) + dir * kRayEps;
+                            survive = true;
+                            b.ray_o[slot] = f4(o, 0.0f);
+                            b.ray_d[slot] = f4(dir, 0.0f);
+                            b.throughput[slot] = f4(thr, 0.0f);
+                            b.prev_flags[slot] = (inside + 1) << kPrevMedium;
+                        }
+                    }
+                }
+                if (walk_ended) {
+                } else {
                 if (bounce == 0) {
                     b.aux_normal[slot] = f4(h.sn, 1.0f);
                     b.aux_albedo[slot] = f4(h.base, h.t);
@@ -455,7 +491,7 @@ __device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView&
                     const LightSample ls = sample_light(s, h, rl);
                     if (ls.valid) {
                         float bpdf = 0.0f;
-                        const f3 resp = direct_response<!LITE>(s, h, rd, ls.dir, &bpdf);
+                        const f3 resp = direct_response<PBR>(s, h, rd, ls.dir, thr, &bpdf);
                         if (maxc3(resp) > 0.0f) {
                             const f3 so = h.pos + ls.dir * (kRayEps * 2.0f);
                             const float max_t = ls.infinite ? kRayFar : ls.dist - kRayEps * 3.0f;
@@ -487,7 +523,7 @@ __device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView&
                     const LightSample ls = analytic_candidate(s, h.pos, li, ra.y, ra.z);
                     if (ls.valid) {
                         float bpdf = 0.0f;
-                        const f3 resp = direct_response<!LITE>(s, h, rd, ls.dir, &bpdf);
+                        const f3 resp = direct_response<PBR>(s, h, rd, ls.dir, thr, &bpdf);
                         if (maxc3(resp) > 0.0f) {
                             const float max_t = ls.infinite ? kRayFar : ls.dist - kRayEps * 3.0f;
                             if (max_t > kRayEps) {
@@ -502,7 +538,7 @@ __device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView&
                     }
                 }
                 if (bsdf_avail) {
-                    const BsdfResult bs = sample_bsdf<!LITE>(s, h, rd, rb);
+                    const BsdfResult bs = sample_bsdf<PBR>(s, h, rd, rb, thr);
                     if (bs.alive) {
                         thr = thr * bs.weight;
                         bool alive = true;
@@ -518,10 +554,15 @@ __device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView&
                             b.throughput[slot] = f4(thr, 0.0f);
                             b.prev_pos[slot] = f4(h.pos, bs.pdf);
                             b.prev_n[slot] = f4(h.sn, __int_as_float(h.prim));
+                            // subsurface_entry / subsurface_exit (programs.cu:2545-2551)
+                            const int next_inside = bs.event == kPbrSubsurfaceEntry ? h.openpbr
+                                                    : (bs.event == kPbrSubsurfaceExit ? -1 : inside);
                             b.prev_flags[slot] = kPrevValid | (bs.delta ? kPrevDelta : 0)
-                                                 | ((h.mat_type + 1) << 8);
+                                                 | ((h.mat_type + 1) << 8)
+                                                 | ((next_inside + 1) << kPrevMedium);
                         }
                     }
+                }
                 }
             }
             b.radiance[slot] = f4(rad, 0.0f);
@@ -548,8 +589,11 @@ __device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView&
     }
 }
 
+// three waves per SIMD for the core variants (their natural budget is a few
+// registers above it); the variant with every OpenPBR lobe takes what it needs
 template <uint32_t F>
-__global__ __launch_bounds__(256) void k_shade(SceneView s, CameraView cams, FrameParams fp,
+__global__ __launch_bounds__(256, (F & kShadePbrFull) != 0 ? 1 : 3) void k_shade(SceneView s,
+    CameraView cams, FrameParams fp,
     WavefrontBuffers b, int bounce, int cur, const uint32_t* queue, const uint32_t* count_ptr) {
     shade_pass<F>(s, cams, fp, b, bounce, cur, queue, count_ptr);
 }
@@ -598,7 +642,7 @@ __global__ __launch_bounds__(256) void k_analytic_hits(SceneView s, CameraView c
     }
 }
 
-template <int BLOCK>
+template <int BLOCK, int PBR>
 __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, FrameParams fp,
     RestirView rv, WavefrontBuffers b) {
     const SlotRng slot_rng_v = slot_rng(fp);
@@ -627,7 +671,7 @@ __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, 
                                  ? rv.current + id.global_pixel
                                  : nullptr;
             const bool bsdf_avail = 1 < fp.max_bounces;
-            const RestirOut ro = restir_shade<BLOCK>(s, rv, h, xyz(d4), (int)id.camera, id.key,
+            const RestirOut ro = restir_shade<BLOCK, PBR>(s, rv, h, xyz(d4), xyz(t4), (int)id.camera, id.key,
                 cams.pixel_offset[id.camera], x, y, w, hgt,
                 id.pixel, id.sample, id.stream, bsdf_avail, out, st);
             if (ro.shadow) {
@@ -768,9 +812,17 @@ hipError_t wf_end_bounce(WavefrontBuffers& b, int bounce, hipStream_t st) {
 hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParams& fp,
     WavefrontBuffers& b, int bounce, StageLaunch l, StageLaunch lite, int mode, hipStream_t st) {
     const int cur = bounce & 1;
+    // the full pass: every OpenPBR lobe only when a material uses more than
+    // the opaque core
+    auto full = [&](const uint32_t* queue, const uint32_t* count) {
+        if (s.pbr_full != 0)
+            k_shade<kShadeAll | kShadePbrFull><<<l.grid, 256, 0, st>>>(s, cams, fp, b, bounce, cur,
+                queue, count);
+        else
+            k_shade<kShadeAll><<<l.grid, 256, 0, st>>>(s, cams, fp, b, bounce, cur, queue, count);
+    };
     if (mode == 0) {
-        k_shade<kShadeAll><<<l.grid, 256, 0, st>>>(s, cams, fp, b, bounce, cur, b.queue[cur],
-            b.counters + cur);
+        full(b.queue[cur], b.counters + cur);
         return hipGetLastError();
     }
     // the legacy-material pass: its plain variant when the scene and frame
@@ -783,9 +835,7 @@ hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParam
     else
         k_shade<kShadeAll & ~kShadePbr><<<l.grid, 256, 0, st>>>(s, cams, fp, b, bounce, cur,
             b.queue[cur], b.counters + cur);
-    if (mode == 2)
-        k_shade<kShadeAll><<<l.grid, 256, 0, st>>>(s, cams, fp, b, bounce, cur, b.pbr_queue,
-            b.counters + kCntPbr);
+    if (mode == 2) full(b.pbr_queue, b.counters + kCntPbr);
     return hipGetLastError();
 }
 
@@ -800,7 +850,7 @@ int wf_restir_blocks_per_cu() {
     static std::atomic<int> cached{0};
     int n = cached.load(std::memory_order_relaxed);
     if (n == 0) {
-        n = resident_blocks(reinterpret_cast<const void*>(k_restir<128>), 128);
+        n = resident_blocks(reinterpret_cast<const void*>(k_restir<128, kPbrCore>), 128);
         cached.store(n, std::memory_order_relaxed);
     }
     return n;
@@ -825,7 +875,9 @@ int wf_shade_blocks_per_cu(bool lite) {
 // keeps within the traversal spill area (trace grid x trace block threads)
 hipError_t wf_restir(const SceneView& s, const CameraView& cams, const FrameParams& fp,
     const RestirView& rv, WavefrontBuffers& b, StageLaunch l, hipStream_t st) {
-    k_restir<128><<<l.grid > 0 ? l.grid : 1, 128, 0, st>>>(s, cams, fp, rv, b);
+    const unsigned g = (unsigned)(l.grid > 0 ? l.grid : 1);
+    if (s.pbr_full != 0) k_restir<128, kPbrFull><<<g, 128, 0, st>>>(s, cams, fp, rv, b);
+    else k_restir<128, kPbrCore><<<g, 128, 0, st>>>(s, cams, fp, rv, b);
     return hipGetLastError();
 }
 
