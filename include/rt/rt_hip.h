@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define RT_ABI_VERSION 2
+#define RT_ABI_VERSION 3
 
 typedef enum rt_status {
     RT_OK = 0,
@@ -89,6 +89,26 @@ typedef struct rt_openpbr_material {
     float emission_luminance; float emission_color[3];
     float geometry_opacity; int32_t geometry_thin_walled;
 } rt_openpbr_material;
+
+/* The texture bindings of OpenPbrCompiledMaterial (src/common/openpbr_core.h:
+ * 97-122), one record per OpenPBR material.  Each field is a texture index
+ * (-1: the material's constant input).  A colour binding carries the colour
+ * space of its texture: RT_COLOR_SRGB_TEXTURE values are decoded with the
+ * sRGB transfer function, the others are taken as linear
+ * (openpbr_source_to_linear, openpbr_core.h:322-333).  base_color and
+ * emission_color are evaluated at every hit (populate_material,
+ * programs.cu:995-1008) and at light samples; specular_color,
+ * transmission_color, base_metalness and specular_roughness when the surface
+ * scatters (evaluate_openpbr_scattering_material, programs.cu:944-959).  A
+ * scalar binding reads the texture's first channel. */
+enum { RT_COLOR_RAW = 0, RT_COLOR_LINEAR_SRGB = 1, RT_COLOR_SRGB_TEXTURE = 2 };
+typedef struct rt_openpbr_textures {
+    int32_t base_color, base_color_space;
+    int32_t specular_color, specular_color_space;
+    int32_t transmission_color, transmission_color_space;
+    int32_t emission_color, emission_color_space;
+    int32_t base_metalness, specular_roughness;
+} rt_openpbr_textures;
 
 /* PackedAnalyticLight (src/common/light_sampling.h:30-62), same field order.
  * basis_x / basis_y / basis_z are the columns of the light's local-to-world
@@ -167,7 +187,7 @@ typedef struct rt_scene_desc {
      * (MaterialSample::albedo_texture, launch_params.h:216-218); NULL or -1
      * keeps the constant.  Texture coordinates: sphere_uv of the outward
      * normal, the quad's edge coordinates, the triangle's barycentrics
-     * (programs.cu:1447,1485,1528).  Emission stays the constant. */
+     * (programs.cu:1447,1485,1528). */
     int32_t texture_count, image_texel_count;
     const rt_texture* textures;
     const float* image_texels;               /* 3 floats (linear RGB) per texel */
@@ -184,6 +204,15 @@ typedef struct rt_scene_desc {
     int32_t medium_count;
     int32_t pad_media;
     const rt_medium* media;
+    /* emission textures.  material_emission_texture[i] >= 0 replaces
+     * material i's constant emission by that texture, evaluated at the hit
+     * and at the sampled point of a light (MaterialSample::emission_texture,
+     * programs.cu:980-981,1225-1226); the light-selection power takes the
+     * texture's average (packed_scene_preparation.cpp:195-224).  Only diffuse
+     * lights emit.  NULL or -1 keeps the constant. */
+    const int32_t* material_emission_texture;  /* material_count entries, or NULL */
+    /* OpenPBR texture bindings, openpbr_count records, or NULL (none bound) */
+    const rt_openpbr_textures* openpbr_textures;
 } rt_scene_desc;
 
 /* -------------------------------------------------------------- cameras

@@ -92,7 +92,15 @@ class Scene(C.Structure):
         ("material_albedo_texture", C.c_void_p), ("triangle_uvs", C.c_void_p),
         ("triangle_normals", C.c_void_p),
         ("medium_count", C.c_int32), ("pad_media", C.c_int32), ("media", C.c_void_p),
+        ("material_emission_texture", C.c_void_p), ("openpbr_textures", C.c_void_p),
     ]
+
+
+class OpenPbrTextures(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "base_color", "base_color_space", "specular_color", "specular_color_space",
+        "transmission_color", "transmission_color_space", "emission_color", "emission_color_space",
+        "base_metalness", "specular_roughness")]
 
 
 class Medium(C.Structure):
@@ -312,6 +320,8 @@ def lib():
         L.rto_primary_rays.argtypes = [P(Camera), C.c_int, C.c_int, C.c_uint32, C.c_int,
                                        C.c_void_p]
         L.rto_build_lights.argtypes = [P(Scene), C.c_void_p, C.c_int]
+        L.rto_hit_material.argtypes = [P(Scene), C.c_void_p, C.c_void_p, C.c_float, C.c_int, C.c_void_p]
+        L.rto_hit_material.restype = None
         L.rto_render_camera.argtypes = [P(Scene), P(Bvh), P(Camera), C.c_int, P(Profile),
                                         C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_int, C.c_int, C.c_int, P(RenderStats)]
@@ -864,6 +874,28 @@ def sample_light_at(scene: Scene, bvh: "OracleBvh", prim: int, pos, gn, d, r):
     li = lib().rto_sample_light_at(C.byref(scene), C.byref(bvh.bvh), prim, *[ptr(v) for v in a],
                                    ptr(out))
     return li, out[0:3], out[3:6], float(out[6]), float(out[7])
+
+
+def hit_material(scene: Scene, o, d, t: float, prim: int):
+    """make_hit at o + t d on global primitive prim: (base colour, emission, (u, v))."""
+    a = [np.ascontiguousarray(v, np.float32) for v in (o, d)]
+    out = np.zeros(8, np.float32)
+    lib().rto_hit_material(C.byref(scene), ptr(a[0]), ptr(a[1]), float(t), int(prim), ptr(out))
+    return out[0:3], out[3:6], out[6:8]
+
+
+class LightRec(C.Structure):
+    _fields_ = [("type", C.c_int32), ("prim", C.c_int32), ("selection_pdf", C.c_float),
+                ("cdf", C.c_float)]
+
+
+def build_lights(scene: Scene):
+    """The emissive light table: [(type, primitive, selection pdf, cdf)], type 0 sphere,
+    1 quad, 2 triangle, 3 environment."""
+    n = lib().rto_build_lights(C.byref(scene), None, 0)
+    arr = (LightRec * max(n, 1))()
+    lib().rto_build_lights(C.byref(scene), C.cast(arr, C.c_void_p), n)
+    return [(r.type, r.prim, r.selection_pdf, r.cdf) for r in arr[:n]]
 
 
 def light_pdf_for_hit(scene: Scene, bvh: "OracleBvh", prim: int, prev_mat: int, pos, sn, d) -> float:

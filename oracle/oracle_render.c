@@ -255,13 +255,63 @@ static float lum_clamped(v3 c) {
     return fmaxf(0.0f, 0.2126f * m.x + 0.7152f * m.y + 0.0722f * m.z);
 }
 
+/* average_texture (packed_scene_preparation.cpp:195-222) */
+static v3 average_texture(const rto_scene* s, int index, int depth) {
+    if (index < 0 || index >= s->texture_count || depth >= 8) return mk3(0, 0, 0);
+    const rto_texture* t = &s->textures[index];
+    if (t->type == 0) return mk3(fmaxf(t->color[0], 0.0f), fmaxf(t->color[1], 0.0f), fmaxf(t->color[2], 0.0f));
+    if (t->type == 1) {
+        const v3 a = average_texture(s, t->even_texture, depth + 1);
+        const v3 b = average_texture(s, t->odd_texture, depth + 1);
+        return mk3(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z));
+    }
+    if (t->type == 2 && t->image_width > 0 && t->image_height > 0) {
+        const int64_t count = (int64_t)t->image_width * t->image_height;
+        if (t->image_offset < 0 || t->image_offset + count > s->image_texel_count) return mk3(0, 0, 0);
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        for (int64_t i = 0; i < count; ++i) {
+            const float* px = s->image_texels + 3 * ((int64_t)t->image_offset + i);
+            sx += px[0];
+            sy += px[1];
+            sz += px[2];
+        }
+        const float n = (float)count;
+        return mk3(fmaxf(sx / n, 0.0f), fmaxf(sy / n, 0.0f), fmaxf(sz / n, 0.0f));
+    }
+    if (t->type == 3) return mk3(0.5f, 0.5f, 0.5f);
+    return mk3(0, 0, 0);
+}
+
+/* openpbr_source_to_linear (openpbr_core.h:322-333) */
+static float srgb_channel_to_linear(float v) {
+    return v <= 0.04045f ? v / 12.92f : powf((v + 0.055f) / 1.055f, 2.4f);
+}
+static v3 source_to_linear(v3 c, int space) {
+    if (space != 2) return c;
+    return mk3(srgb_channel_to_linear(c.x), srgb_channel_to_linear(c.y), srgb_channel_to_linear(c.z));
+}
+
+/* material_emission_luminance (packed_scene_preparation.cpp:224-248): a
+ * textured emission counts with its texture's average */
 static float material_emission_lum(const rto_scene* s, int m) {
     if (m < 0 || m >= s->material_count) return 0.0f;
     const rto_material* mt = &s->materials[m];
-    if (mt->type == RTO_MAT_DIFFUSE_LIGHT) return lum_clamped(ld3(mt->emission));
+    if (mt->type == RTO_MAT_DIFFUSE_LIGHT) {
+        if (s->material_emission_texture != NULL && s->material_emission_texture[m] >= 0)
+            return lum_clamped(average_texture(s, s->material_emission_texture[m], 0));
+        return lum_clamped(ld3(mt->emission));
+    }
     if (mt->type != RTO_MAT_OPENPBR || mt->openpbr_index < 0 || mt->openpbr_index >= s->openpbr_count)
         return 0.0f;
-    const v3 e = om_openpbr_emission(&s->openpbr[mt->openpbr_index]);
+    rto_openpbr p = s->openpbr[mt->openpbr_index];
+    if (s->openpbr_textures != NULL && s->openpbr_textures[mt->openpbr_index].emission_color >= 0) {
+        const rto_openpbr_textures* tb = &s->openpbr_textures[mt->openpbr_index];
+        const v3 c = source_to_linear(average_texture(s, tb->emission_color, 0), tb->emission_color_space);
+        p.emission_color[0] = c.x;
+        p.emission_color[1] = c.y;
+        p.emission_color[2] = c.z;
+    }
+    const v3 e = om_openpbr_emission(&p);
     return 0.2126f * fmaxf(0.0f, e.x) + 0.7152f * fmaxf(0.0f, e.y) + 0.0722f * fmaxf(0.0f, e.z);
 }
 
@@ -447,8 +497,31 @@ typedef struct {
     float t;
     v3 pos, gn, sn, base, emission;
     float fuzz, ior;
+    float tex_u, tex_v;
     int mat_type, openpbr, prim, prim_type, prim_index, front;
 } hit_t;
+
+/* a texture at a point, with the texture coordinates u, v */
+static v3 texture_at(const rto_scene* s, int index, float u, float v, v3 p) {
+    const float pf[3] = {p.x, p.y, p.z};
+    float c[3];
+    rto_eval_texture(s, index, u, v, pf, c);
+    return ld3(c);
+}
+
+/* openpbr_apply_color_input / openpbr_apply_scalar_input
+ * (openpbr_core.h:335-353) from a texture at the hit */
+static void apply_color_binding(const rto_scene* s, float dst[3], int tex, int space, const hit_t* h) {
+    if (tex < 0) return;
+    const v3 c = source_to_linear(texture_at(s, tex, h->tex_u, h->tex_v, h->pos), space);
+    dst[0] = c.x;
+    dst[1] = c.y;
+    dst[2] = c.z;
+}
+static void apply_scalar_binding(const rto_scene* s, float* dst, int tex, const hit_t* h) {
+    if (tex < 0) return;
+    *dst = texture_at(s, tex, h->tex_u, h->tex_v, h->pos).x;
+}
 
 static void set_face(v3 d, v3 outward, hit_t* h) {
     h->front = dot3(d, outward) < 0.0f;
@@ -516,6 +589,15 @@ static void make_hit(const rto_scene* s, v3 o, v3 d, float t, int prim, hit_t* h
             }
         }
     }
+    /* the intersectors' texture coordinates (programs.cu:1447,1485,1524-1530);
+     * 0 for a medium event */
+    h->tex_u = 0.0f;
+    h->tex_v = 0.0f;
+    if (h->prim_type >= 0) {
+        const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+        const float pf[3] = {h->pos.x, h->pos.y, h->pos.z};
+        rto_surface_uv(s, of, df, h->prim_type, h->prim_index, pf, &h->tex_u, &h->tex_v);
+    }
     /* populate_material, programs.cu:961-1009 */
     if (mat < 0 || mat >= s->material_count) {
         h->base = mk3(0, 0, 0); h->emission = mk3(0, 0, 0); h->mat_type = 0; h->ior = 1.0f;
@@ -527,17 +609,13 @@ static void make_hit(const rto_scene* s, v3 o, v3 d, float t, int prim, hit_t* h
     h->mat_type = m->type;
     h->fuzz = m->fuzz;
     h->ior = m->ior;
+    /* the albedo and emission textures (programs.cu:978-981); -1 keeps the constant */
     h->base = ld3(m->albedo);
-    if (s->material_albedo_texture != NULL && s->material_albedo_texture[mat] >= 0) {
-        /* populate_material's albedo texture, programs.cu:978-979 */
-        const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
-        const float pf[3] = {h->pos.x, h->pos.y, h->pos.z};
-        float u = 0.0f, v = 0.0f, c[3];
-        if (h->prim_type >= 0) rto_surface_uv(s, of, df, h->prim_type, h->prim_index, pf, &u, &v);
-        rto_eval_texture(s, s->material_albedo_texture[mat], u, v, pf, c);
-        h->base = ld3(c);
-    }
+    if (s->material_albedo_texture != NULL && s->material_albedo_texture[mat] >= 0)
+        h->base = texture_at(s, s->material_albedo_texture[mat], h->tex_u, h->tex_v, h->pos);
     h->emission = ld3(m->emission);
+    if (s->material_emission_texture != NULL && s->material_emission_texture[mat] >= 0)
+        h->emission = texture_at(s, s->material_emission_texture[mat], h->tex_u, h->tex_v, h->pos);
     if (h->mat_type == RTO_MAT_DIELECTRIC) { h->base = mk3(0.92f, 0.95f, 1.0f); h->emission = mk3(0, 0, 0); }
     if (h->mat_type == RTO_MAT_DIFFUSE_LIGHT && !h->front) h->emission = mk3(0, 0, 0);
     if (h->mat_type != RTO_MAT_DIFFUSE_LIGHT) h->emission = mk3(0, 0, 0);
@@ -546,10 +624,36 @@ static void make_hit(const rto_scene* s, v3 o, v3 d, float t, int prim, hit_t* h
             h->mat_type = -1; h->base = mk3(0, 0, 0); h->emission = mk3(0, 0, 0);
             return;
         }
-        const rto_openpbr* p = &s->openpbr[m->openpbr_index];
-        h->base = ld3(p->base_color);
-        h->emission = om_openpbr_emission(p);
+        /* the base and emission colour bindings (programs.cu:1001-1008) */
+        rto_openpbr p = s->openpbr[m->openpbr_index];
+        if (s->openpbr_textures != NULL) {
+            const rto_openpbr_textures* tb = &s->openpbr_textures[m->openpbr_index];
+            apply_color_binding(s, p.base_color, tb->base_color, tb->base_color_space, h);
+            apply_color_binding(s, p.emission_color, tb->emission_color, tb->emission_color_space, h);
+        }
+        h->base = ld3(p.base_color);
+        h->emission = om_openpbr_emission(&p);
     }
+}
+
+/* evaluate_openpbr_scattering_material (programs.cu:944-959): the record
+ * with the hit's base colour and the bound specular / transmission colours,
+ * metalness and roughness at the hit */
+static rto_openpbr scattering_material(const rto_scene* s, const hit_t* h) {
+    rto_openpbr p = s->openpbr[h->openpbr];
+    if (s->openpbr_textures != NULL) {
+        const rto_openpbr_textures* tb = &s->openpbr_textures[h->openpbr];
+        if (tb->base_color >= 0) {
+            p.base_color[0] = h->base.x;
+            p.base_color[1] = h->base.y;
+            p.base_color[2] = h->base.z;
+        }
+        apply_color_binding(s, p.specular_color, tb->specular_color, tb->specular_color_space, h);
+        apply_color_binding(s, p.transmission_color, tb->transmission_color, tb->transmission_color_space, h);
+        apply_scalar_binding(s, &p.base_metalness, tb->base_metalness, h);
+        apply_scalar_binding(s, &p.specular_roughness, tb->specular_roughness, h);
+    }
+    return p;
 }
 
 /* ------------------------------------------------------------- path state */
@@ -559,6 +663,8 @@ typedef struct {
     float prev_pdf;
     int prev_mat, prev_prim, prev_valid, prev_delta, alive;
     int inside;   /* OpenPBR index of the subsurface medium the path is in, -1 none */
+    rto_openpbr inside_mat;   /* that material as it was where the walk began
+                                 (PathState::subsurface_medium, programs.cu:2545-2548) */
 } path_t;
 
 static float power_heuristic(float a, float b) {
@@ -715,6 +821,9 @@ static light_sample_t sample_light(const ctx_t* c, const hit_t* h, const float r
     }
     float cond = 0.0f;
     int mat = -1;
+    /* the sampled point and its texture coordinates (programs.cu:1154,1166-1167,1193-1199) */
+    v3 lpos = mk3(0, 0, 0);
+    float tu = 0.0f, tv = 0.0f;
     if (L->type == 0) {
         const rto_sphere* sph = &s->spheres[L->prim];
         const v3 center = ld3(sph->center);
@@ -732,6 +841,11 @@ static light_sample_t sample_light(const ctx_t* c, const hit_t* h, const float r
         if (ls.dist <= OM_RAY_EPS) return ls;
         mat = sph->material;
         cond = cone_pdf(cmax);
+        lpos = add3(sp, scl3(ls.dir, ls.dist));
+        const v3 n = norm3(sub3(lpos, center));
+        /* sphere_uv (programs.cu:188-193) */
+        tu = (atan2f(-n.z, n.x) + OM_PI) / (2.0f * OM_PI);
+        tv = acosf(fminf(fmaxf(-n.y, -1.0f), 1.0f)) / OM_PI;
     } else if (L->type == 1) {
         const rto_quad* q = &s->quads[L->prim];
         const v3 eu = ld3(q->edge_u), ev = ld3(q->edge_v);
@@ -741,6 +855,9 @@ static light_sample_t sample_light(const ctx_t* c, const hit_t* h, const float r
         if (area <= 1e-12f) return ls;
         const v3 normal = div3(an, area);
         mat = q->material;
+        lpos = pos;
+        tu = r[1];
+        tv = r[2];
         const v3 tl = sub3(pos, sp);
         const float dsq = lensq3(tl);
         if (dsq <= 1e-12f) { memset(&ls, 0, sizeof(ls)); return ls; }
@@ -758,6 +875,14 @@ static light_sample_t sample_light(const ctx_t* c, const hit_t* h, const float r
         if (da <= 1e-12f) return ls;
         const v3 normal = div3(an, da);
         mat = t->material;
+        lpos = pos;
+        tu = w1;
+        tv = w2;
+        if (s->triangle_uvs != NULL && s->triangle_uvs[6 * (int64_t)L->prim] == s->triangle_uvs[6 * (int64_t)L->prim]) {
+            const float* uv = s->triangle_uvs + 6 * (int64_t)L->prim;
+            tu = uv[0] * w0 + uv[2] * w1 + uv[4] * w2;
+            tv = uv[1] * w0 + uv[3] * w1 + uv[5] * w2;
+        }
         const v3 tl = sub3(pos, sp);
         const float dsq = lensq3(tl);
         if (dsq <= 1e-12f) { memset(&ls, 0, sizeof(ls)); return ls; }
@@ -767,9 +892,24 @@ static light_sample_t sample_light(const ctx_t* c, const hit_t* h, const float r
     }
     if (cond <= 0.0f || mat < 0 || mat >= s->material_count) { memset(&ls, 0, sizeof(ls)); return ls; }
     const rto_material* m = &s->materials[mat];
+    /* evaluate_material_emission (programs.cu:1013-1024) at the sampled point */
     if (m->type == RTO_MAT_OPENPBR) {
-        ls.emission = (m->openpbr_index >= 0 && m->openpbr_index < s->openpbr_count)
-                          ? om_openpbr_emission(&s->openpbr[m->openpbr_index]) : mk3(0, 0, 0);
+        if (m->openpbr_index >= 0 && m->openpbr_index < s->openpbr_count) {
+            rto_openpbr p = s->openpbr[m->openpbr_index];
+            if (s->openpbr_textures != NULL && s->openpbr_textures[m->openpbr_index].emission_color >= 0) {
+                const rto_openpbr_textures* tb = &s->openpbr_textures[m->openpbr_index];
+                const v3 c = source_to_linear(texture_at(s, tb->emission_color, tu, tv, lpos),
+                    tb->emission_color_space);
+                p.emission_color[0] = c.x;
+                p.emission_color[1] = c.y;
+                p.emission_color[2] = c.z;
+            }
+            ls.emission = om_openpbr_emission(&p);
+        } else {
+            ls.emission = mk3(0, 0, 0);
+        }
+    } else if (s->material_emission_texture != NULL && s->material_emission_texture[mat] >= 0) {
+        ls.emission = texture_at(s, s->material_emission_texture[mat], tu, tv, lpos);
     } else {
         ls.emission = ld3(m->emission);
     }
@@ -791,9 +931,9 @@ static v3 direct_response(const ctx_t* c, const hit_t* h, const path_t* st, v3 l
         return scl3(h->base, *pdf);
     }
     if (h->mat_type != RTO_MAT_OPENPBR) return mk3(0, 0, 0);
-    const rto_openpbr* p = &c->s->openpbr[h->openpbr];
+    const rto_openpbr p = scattering_material(c->s, h);
     v3 value;
-    om_openpbr_eval(p, h->gn, neg3(norm3(st->d)), ldir, st->throughput, &value, pdf);
+    om_openpbr_eval(&p, h->gn, neg3(norm3(st->d)), ldir, st->throughput, &value, pdf);
     const float cs = fabsf(dot3(h->sn, ldir));
     return scl3(value, cs);
 }
@@ -1526,9 +1666,9 @@ static void sample_bsdf(const ctx_t* c, const hit_t* h, const float r[4], path_t
         return;
     }
     if (h->mat_type == RTO_MAT_OPENPBR) {
-        const rto_openpbr* p = &c->s->openpbr[h->openpbr];
+        const rto_openpbr p = scattering_material(c->s, h);
         rto_openpbr_sample smp;
-        om_openpbr_sample(p, h->gn, neg3(norm3(st->d)), r[0], r[1], r[2], st->throughput, &smp);
+        om_openpbr_sample(&p, h->gn, neg3(norm3(st->d)), r[0], r[1], r[2], st->throughput, &smp);
         if (!smp.valid) { st->alive = 0; return; }
         const v3 dir = ld3(smp.wi);
         st->o = add3(h->pos, scl3(dir, OM_RAY_EPS));
@@ -1536,8 +1676,12 @@ static void sample_bsdf(const ctx_t* c, const hit_t* h, const float r[4], path_t
         st->throughput = mul3(st->throughput, ld3(smp.weight));
         record_scatter(st, h, smp.pdf, smp.delta != 0);
         /* subsurface_entry / subsurface_exit (programs.cu:2545-2551) */
-        if (smp.event == 8) st->inside = h->openpbr;
-        else if (smp.event == 9) st->inside = -1;
+        if (smp.event == 8) {
+            st->inside = h->openpbr;
+            st->inside_mat = p;
+        } else if (smp.event == 9) {
+            st->inside = -1;
+        }
         return;
     }
     if (h->mat_type == RTO_MAT_ISOTROPIC) {
@@ -1620,7 +1764,7 @@ static void trace_one(const ctx_t* c, const rto_camera* cam, int cam_index, cons
             if (h.mat_type != RTO_MAT_OPENPBR || h.openpbr != st.inside || h.front) break;
             float rw[4], seg[5];
             om_rng4(pixel, (uint32_t)sample, (uint32_t)cam_index, (uint32_t)b, OM_G_SUBSURFACE, stream, rw);
-            const rto_openpbr* pm = &s->openpbr[st.inside];
+            const rto_openpbr* pm = &st.inside_mat;
             const float rlen = sqrtf(lensq3(st.d));
             om_openpbr_segment(pm, t * rlen, rw[0], st.throughput, seg);
             st.throughput = mul3(st.throughput, ld3(seg));
@@ -1838,6 +1982,17 @@ int rto_sample_light_at(const rto_scene* s, const rto_bvh* bvh, int prim, const 
     ctx_free(c);
     free(c);
     return L.valid ? (li >= 0 ? li : -3) : -1;
+}
+
+/* the hit record of the ray o + t d on global primitive prim (populate_material,
+ * textures included): out = base colour[3], emission[3], texture u, v */
+void rto_hit_material(const rto_scene* s, const float o[3], const float d[3], float t, int prim,
+    float out[8]) {
+    hit_t h;
+    make_hit(s, ld3(o), ld3(d), t, prim, &h);
+    out[0] = h.base.x; out[1] = h.base.y; out[2] = h.base.z;
+    out[3] = h.emission.x; out[4] = h.emission.y; out[5] = h.emission.z;
+    out[6] = h.tex_u; out[7] = h.tex_v;
 }
 
 /* the NEE density of the surface point (or environment) hit by the ray

@@ -116,6 +116,17 @@ typedef struct {
 
 enum { RTO_BG_CONSTANT = 0, RTO_BG_SKY_GRADIENT = 1 };
 
+/* OpenPbrCompiledMaterial's texture bindings (openpbr_core.h:97-122):
+ * texture index (-1 none) and, for colours, the source colour space
+ * (OpenPbrSourceColorSpace: 0 raw, 1 linear sRGB, 2 sRGB texture) */
+typedef struct {
+    int32_t base_color, base_color_space;
+    int32_t specular_color, specular_color_space;
+    int32_t transmission_color, transmission_color_space;
+    int32_t emission_color, emission_color_space;
+    int32_t base_metalness, specular_roughness;
+} rto_openpbr_textures;
+
 /* rt_texture (PackedTexture, launch_params.h:225-237) */
 typedef struct {
     int32_t type, even_texture, odd_texture;
@@ -146,6 +157,8 @@ typedef struct {
     int32_t medium_count;
     int32_t pad_media;
     const rto_medium* media;
+    const int32_t* material_emission_texture;       /* MaterialSample::emission_texture, or NULL */
+    const rto_openpbr_textures* openpbr_textures;   /* OpenPbrCompiledMaterial bindings, or NULL */
 } rto_scene;
 
 /* ---- cameras / profile ------------------------------------------------ */
@@ -370,6 +383,10 @@ float rto_area_pdf(float area, float dsq, float abs_cos);     /* light_sampling.
 float rto_uniform_sphere_pdf(void);                           /* light_sampling.h:60-62 */
 int rto_pick_cdf(const float* cdf, int n, float u);           /* light_sampling.h:87-101 */
 void rto_triangle_barycentrics(float u0, float u1, float w[3]); /* light_sampling.h:118-125 */
+/* make_hit's base colour, emission and texture coordinates for the ray
+ * o + t d on global primitive prim (test hook) */
+void rto_hit_material(const rto_scene* s, const float o[3], const float d[3], float t, int prim,
+    float out[8]);
 int rto_sample_light_at(const rto_scene* s, const rto_bvh* bvh, int prim, const float pos[3],
     const float gn[3], const float d[3], const float r[4], float out[8]);
 float rto_light_pdf_for_hit(const rto_scene* s, const rto_bvh* bvh, int prim, int prev_mat,

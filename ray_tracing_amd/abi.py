@@ -68,7 +68,17 @@ class SceneDesc(C.Structure):
         ("material_albedo_texture", C.c_void_p), ("triangle_uvs", C.c_void_p),
         ("triangle_normals", C.c_void_p),
         ("medium_count", C.c_int32), ("pad_media", C.c_int32), ("media", C.c_void_p),
+        ("material_emission_texture", C.c_void_p), ("openpbr_textures", C.c_void_p),
     ]
+
+
+class OpenPbrTextures(C.Structure):
+    """rt_openpbr_textures: texture index per bound OpenPBR input (-1 none) and the
+    colour space of each colour binding (RT_COLOR_*)."""
+    _fields_ = [(n, C.c_int32) for n in (
+        "base_color", "base_color_space", "specular_color", "specular_color_space",
+        "transmission_color", "transmission_color_space", "emission_color", "emission_color_space",
+        "base_metalness", "specular_roughness")]
 
 
 class Medium(C.Structure):

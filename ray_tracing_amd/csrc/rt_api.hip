@@ -139,6 +139,9 @@ struct rt_scene {
     DevBuf<rt_texture> textures;
     DevBuf<float> texels;
     DevBuf<int32_t> mat_albedo_tex;
+    bool has_emission_textures = false, has_pbr_textures = false;
+    DevBuf<int32_t> mat_emission_tex;
+    DevBuf<rt_openpbr_textures> pbr_tex;
     int32_t background_mode = 0;
     float bg[3] = {0, 0, 0}, bg_top[3] = {0, 0, 0};
     DevBuf<rt_sphere> spheres;
@@ -225,6 +228,8 @@ struct rt_scene {
         v.tri_n = has_triangle_normals ? tri_n.p : nullptr;
         v.texture_count = texture_count;
         v.texel_count = texel_count;
+        v.mat_emission_tex = has_emission_textures ? mat_emission_tex.p : nullptr;
+        v.pbr_tex = has_pbr_textures ? pbr_tex.p : nullptr;
         return v;
     }
 };
@@ -429,16 +434,62 @@ namespace {
 
 // light distribution: power x area CDF over emissive primitives plus the
 // environment (src/realtime/gpu/packed_scene_preparation.cpp:250-302)
+// average_texture (packed_scene_preparation.cpp:195-222): a texture's mean
+// colour, the emission a textured light is selected by. Checkers average
+// their two children (8 levels at most), images their texels, noise is 0.5,
+// a bad index or a missing image 0.
+static void average_texture(const rt_scene_desc* d, int index, int depth, float out[3]) {
+    out[0] = out[1] = out[2] = 0.0f;
+    if (index < 0 || index >= d->texture_count || d->textures == nullptr || depth >= 8) return;
+    const rt_texture& t = d->textures[index];
+    if (t.type == RT_TEX_CONSTANT) {
+        for (int k = 0; k < 3; ++k) out[k] = std::max(t.color[k], 0.0f);
+    } else if (t.type == RT_TEX_CHECKER) {
+        float a[3], b[3];
+        average_texture(d, t.even_texture, depth + 1, a);
+        average_texture(d, t.odd_texture, depth + 1, b);
+        for (int k = 0; k < 3; ++k) out[k] = 0.5f * (a[k] + b[k]);
+    } else if (t.type == RT_TEX_IMAGE && t.image_width > 0 && t.image_height > 0) {
+        const int64_t count = (int64_t)t.image_width * t.image_height;
+        if (t.image_offset < 0 || t.image_offset + count > d->image_texel_count) return;
+        float sum[3] = {0.0f, 0.0f, 0.0f};
+        const float* px = d->image_texels + 3 * (int64_t)t.image_offset;
+        for (int64_t i = 0; i < count; ++i)
+            for (int k = 0; k < 3; ++k) sum[k] += px[3 * i + k];
+        for (int k = 0; k < 3; ++k) out[k] = std::max(sum[k] / (float)count, 0.0f);
+    } else if (t.type == RT_TEX_NOISE) {
+        out[0] = out[1] = out[2] = 0.5f;
+    }
+}
+
 static void build_lights(const rt_scene_desc* d, std::vector<rtk::LightRec>& out) {
     const float kPi = 3.14159265358979323846f;
     out.clear();
+    // material_emission_luminance (packed_scene_preparation.cpp:224-248)
     auto emission_lum = [&](int m) -> float {
         if (m < 0 || m >= d->material_count) return 0.0f;
         const rt_material& mt = d->materials[m];
-        if (mt.type == RT_MAT_DIFFUSE_LIGHT) return lum_clamped(mt.emission);
+        if (mt.type == RT_MAT_DIFFUSE_LIGHT) {
+            if (d->material_emission_texture != nullptr && d->material_emission_texture[m] >= 0) {
+                float avg[3];
+                average_texture(d, d->material_emission_texture[m], 0, avg);
+                return lum_clamped(avg);
+            }
+            return lum_clamped(mt.emission);
+        }
         if (mt.type != RT_MAT_OPENPBR || mt.openpbr_index < 0 || mt.openpbr_index >= d->openpbr_count)
             return 0.0f;
-        const rtk::f3 e = rtk::openpbr_emission(d->openpbr[mt.openpbr_index]);
+        rt_openpbr_material pm = d->openpbr[mt.openpbr_index];
+        if (d->openpbr_textures != nullptr && d->openpbr_textures[mt.openpbr_index].emission_color >= 0) {
+            const rt_openpbr_textures& tb = d->openpbr_textures[mt.openpbr_index];
+            float avg[3];
+            average_texture(d, tb.emission_color, 0, avg);
+            const rtk::f3 c = rtk::source_to_linear(rtk::f3{avg[0], avg[1], avg[2]}, tb.emission_color_space);
+            pm.emission_color[0] = c.x;
+            pm.emission_color[1] = c.y;
+            pm.emission_color[2] = c.z;
+        }
+        const rtk::f3 e = rtk::openpbr_emission(pm);
         return 0.2126f * std::max(0.0f, e.x) + 0.7152f * std::max(0.0f, e.y)
                + 0.0722f * std::max(0.0f, e.z);
     };
@@ -751,6 +802,23 @@ static rt_status scene_fill(rt_scene* s, const rt_scene_desc* d) {
     if (e == hipSuccess) e = upload(s->texels, d->image_texels, (size_t)d->image_texel_count * 3, st);
     if (e == hipSuccess && s->has_albedo_textures)
         e = upload(s->mat_albedo_tex, d->material_albedo_texture, (size_t)d->material_count, st);
+    // bindings that are all -1 leave the scene on the untextured kernels
+    s->has_emission_textures = false;
+    if (d->material_emission_texture != nullptr)
+        for (int i = 0; i < d->material_count; ++i)
+            s->has_emission_textures = s->has_emission_textures || d->material_emission_texture[i] >= 0;
+    if (e == hipSuccess && s->has_emission_textures)
+        e = upload(s->mat_emission_tex, d->material_emission_texture, (size_t)d->material_count, st);
+    s->has_pbr_textures = false;
+    if (d->openpbr_textures != nullptr)
+        for (int i = 0; i < d->openpbr_count; ++i) {
+            const rt_openpbr_textures& t = d->openpbr_textures[i];
+            s->has_pbr_textures = s->has_pbr_textures || t.base_color >= 0 || t.specular_color >= 0
+                                  || t.transmission_color >= 0 || t.emission_color >= 0
+                                  || t.base_metalness >= 0 || t.specular_roughness >= 0;
+        }
+    if (e == hipSuccess && s->has_pbr_textures)
+        e = upload(s->pbr_tex, d->openpbr_textures, (size_t)d->openpbr_count, st);
     s->has_triangle_uvs = d->triangle_uvs != nullptr && d->triangle_count > 0;
     if (e == hipSuccess && s->has_triangle_uvs)
         e = upload(s->tri_uv, d->triangle_uvs, (size_t)d->triangle_count * 6, st);
@@ -813,6 +881,8 @@ static void scene_signatures(const rt_scene_desc* d, uint64_t* geometry, uint64_
     h.array(d->textures, d->texture_count);
     h.array(d->image_texels, (int64_t)d->image_texel_count * 3);
     h.array(d->material_albedo_texture, d->material_albedo_texture ? d->material_count : 0);
+    h.array(d->material_emission_texture, d->material_emission_texture ? d->material_count : 0);
+    h.array(d->openpbr_textures, d->openpbr_textures ? d->openpbr_count : 0);
     h.array(d->triangle_uvs, d->triangle_uvs ? (int64_t)d->triangle_count * 6 : 0);
     h.array(d->triangle_normals, d->triangle_normals ? (int64_t)d->triangle_count * 9 : 0);
     h.array(d->media, d->medium_count);

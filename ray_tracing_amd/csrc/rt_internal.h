@@ -75,6 +75,8 @@ struct SceneView {
     const float* tri_uv;               // 6 floats per triangle, or nullptr
     const float* tri_n;                // 9 floats per triangle, or nullptr
     int32_t texture_count, texel_count;
+    const int32_t* mat_emission_tex;   // nullptr: every emission is a constant
+    const rt_openpbr_textures* pbr_tex;  // nullptr: no OpenPBR input is bound to a texture
 };
 
 // 4-wide quantized BVH node, 64 B = 4 x float4, collapsed from the binary

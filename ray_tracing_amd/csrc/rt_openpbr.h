@@ -48,6 +48,17 @@ enum : int { kPbrNone = 0, kPbrDiffuse = 1, kPbrGlossyReflection = 2, kPbrGlossy
     kPbrThinTransmission = 4, kPbrPassthrough = 5, kPbrCoat = 6, kPbrFuzz = 7,
     kPbrSubsurfaceEntry = 8, kPbrSubsurfaceExit = 9 };
 
+// openpbr_srgb_channel_to_linear / openpbr_source_to_linear
+// (openpbr_core.h:322-333): a texture value in the sRGB-encoded colour space
+// is decoded, the other spaces are taken as linear
+RTK_HD float srgb_channel_to_linear(float v) {
+    return v <= 0.04045f ? v / 12.92f : powf((v + 0.055f) / 1.055f, 2.4f);
+}
+RTK_HD f3 source_to_linear(f3 c, int space) {
+    if (space != RT_COLOR_SRGB_TEXTURE) return c;
+    return f3{srgb_channel_to_linear(c.x), srgb_channel_to_linear(c.y), srgb_channel_to_linear(c.z)};
+}
+
 namespace opbr {
 
 constexpr float kPiF = 3.14159265358979323846f;
@@ -170,6 +181,36 @@ RTK_HD bool needs_full(const rt_openpbr_material& m) {
            || m.specular_roughness_anisotropy != 0.0f || m.coat_roughness_anisotropy != 0.0f
            || m.geometry_thin_walled != 0;
 }
+
+// a material with the inputs that textures may drive (base, specular and
+// transmission colour, metalness, roughness, emission colour) held as values:
+// the record's own, or a texture's at the hit (the per-hit parameter copy of
+// evaluate_openpbr_scattering_material / populate_material,
+// programs.cu:944-959,995-1008)
+template <class M>
+struct MatTex : M {
+    f3 bc, sc, tc, ec;
+    float metal, rough;
+    RTK_HD explicit MatTex(const M& m)
+        : M(m), bc(m.base_color()), sc(m.specular_color()), tc(m.transmission_color()),
+          ec(m.emission_color()), metal(m.base_metalness()), rough(m.specular_roughness()) {}
+    RTK_HD f3 base_color() const { return bc; }
+    RTK_HD f3 specular_color() const { return sc; }
+    RTK_HD f3 transmission_color() const { return tc; }
+    RTK_HD f3 emission_color() const { return ec; }
+    RTK_HD float base_metalness() const { return metal; }
+    RTK_HD float specular_roughness() const { return rough; }
+};
+
+// the medium of a path inside a subsurface random walk: the walk started
+// only where the metalness at the entry point was below one, so the
+// metalness gate of ss_random_walk holds inside whatever a texture gave there
+// (the medium's other inputs are never texture-driven)
+template <class M>
+struct MatInside : M {
+    RTK_HD explicit MatInside(const M& m) : M(m) {}
+    RTK_HD float base_metalness() const { return 0.0f; }
+};
 
 // openpbr_subsurface_boundary_material (openpbr_core.h:1891-1905): a clear
 // dielectric with the material's IOR, roughness and thin film
@@ -1427,10 +1468,10 @@ RTK_HD bool openpbr_random_walk(const rt_openpbr_material& m) {
     return opbr::ss_random_walk(opbr::MatMem{&m});
 }
 RTK_HD OpenPbrSegment openpbr_segment(const rt_openpbr_material& m, float boundary, float u, f3 thr) {
-    return opbr::ss_segment(opbr::ss_medium(opbr::MatMem{&m}), boundary, u, thr);
+    return opbr::ss_segment(opbr::ss_medium(opbr::MatInside<opbr::MatMem>(opbr::MatMem{&m})), boundary, u, thr);
 }
 RTK_HD f3 openpbr_hg(const rt_openpbr_material& m, f3 incident, float u1, float u2) {
-    return opbr::sample_hg(incident, opbr::ss_medium(opbr::MatMem{&m}).g, u1, u2);
+    return opbr::sample_hg(incident, opbr::ss_medium(opbr::MatInside<opbr::MatMem>(opbr::MatMem{&m})).g, u1, u2);
 }
 
 #undef RTK_HD

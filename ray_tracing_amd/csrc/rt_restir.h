@@ -144,13 +144,13 @@ __device__ __forceinline__ void rs_surface(const HitInfo& h, Reservoir& r) {
 }
 
 // restir_unshadowed_numerator (programs.cu:2123-2145)
-template <int PBR>
+template <int PBR, bool PTEX>
 __device__ inline f3 rs_numerator(const SceneView& s, const HitInfo& h, f3 ray_d, f3 thr,
     const LightSample& L, bool bsdf_avail, float* target) {
     *target = 0.0f;
     if (!L.valid || L.pdf <= 0.0f) return mk3(0, 0, 0);
     float bpdf = 0.0f;
-    const f3 resp = direct_response<PBR>(s, h, ray_d, L.dir, thr, &bpdf);
+    const f3 resp = direct_response<PBR, PTEX>(s, h, ray_d, L.dir, thr, &bpdf);
     if (maxc3(resp) <= 0.0f) return mk3(0, 0, 0);
     const float mis = L.delta || !bsdf_avail ? 1.0f : power_heuristic(L.pdf, bpdf);
     const f3 num = (L.emission * resp) * mis;
@@ -182,7 +182,7 @@ __device__ inline bool rs_previous_pixel(const RestirView& rv, int cam, int widt
 }
 
 // restir_target_at_previous_surface (programs.cu:2210-2247)
-template <int BLOCK, int PBR>
+template <int BLOCK, int PBR, bool PTEX>
 __device__ inline float rs_target_at_source(const SceneView& s, const RestirView& rv, int cam,
     const Reservoir& src, const Reservoir& sel, bool bsdf_avail, const TraceStack& st) {
     if (rv.dyn->prev_valid[cam] == 0) return 0.0f;
@@ -207,7 +207,7 @@ __device__ inline float rs_target_at_source(const SceneView& s, const RestirView
     const LightSample L = analytic_candidate(s, h.pos, sel.light_index, sel.sample_u0, sel.sample_u1);
     float target = 0.0f;
     // a fresh path state at the previous surface: unit throughput
-    rs_numerator<PBR>(s, h, dir, mk3(1.0f, 1.0f, 1.0f), L, bsdf_avail, &target);
+    rs_numerator<PBR, PTEX>(s, h, dir, mk3(1.0f, 1.0f, 1.0f), L, bsdf_avail, &target);
     return target;
 }
 
@@ -222,8 +222,9 @@ struct RestirOut {
 // shadow ray, which the connect stage traces.  `cam` is the camera's slot and
 // `base` its first pixel in the reservoir buffers, which hold all cameras
 // back to back; `key` is the camera key of the slot's random numbers and
-// `thr` the path throughput at the vertex.
-template <int BLOCK, int PBR>
+// `thr` the path throughput at the vertex.  PTEX: the scene binds OpenPBR
+// inputs to textures.
+template <int BLOCK, int PBR, bool PTEX>
 __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& rv,
     const HitInfo& h, f3 ray_d, f3 thr, int cam, uint32_t key, int base, int x, int y, int width, int height,
     uint32_t pixel, uint32_t sample, uint32_t stream, bool bsdf_avail, Reservoir* out,
@@ -240,7 +241,7 @@ __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& r
         const int li = pick_analytic(s, q.x);
         const LightSample L = analytic_candidate(s, h.pos, li, q.y, q.z);
         float target = 0.0f;
-        rs_numerator<PBR>(s, h, ray_d, thr, L, bsdf_avail, &target);
+        rs_numerator<PBR, PTEX>(s, h, ray_d, thr, L, bsdf_avail, &target);
         const float w = L.pdf > 0.0f ? target / L.pdf : 0.0f;
         rs_update(r, li, q.y, q.z, target, w, 1, q.w);
     }
@@ -268,7 +269,7 @@ __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& r
             if (rs_surface_valid(cur, prev, fmaxf(0.005f, 0.01f * h.t), 0.95f, rv.max_history_age)) {
                 const LightSample L = analytic_candidate(s, h.pos, prev.light_index, prev.sample_u0, prev.sample_u1);
                 float target = 0.0f;
-                rs_numerator<PBR>(s, h, ray_d, thr, L, bsdf_avail, &target);
+                rs_numerator<PBR, PTEX>(s, h, ray_d, thr, L, bsdf_avail, &target);
                 bool sel = false;
                 const int mult = rs_merge(r, prev, target, rv.max_temporal_candidates, sel_u[0], true, &sel);
                 if (mult > 0) {
@@ -306,7 +307,7 @@ __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& r
                 if (!rs_surface_valid(cur, prev, fmaxf(0.01f, 0.02f * h.t), 0.95f, rv.max_history_age)) continue;
                 const LightSample L = analytic_candidate(s, h.pos, prev.light_index, prev.sample_u0, prev.sample_u1);
                 float target = 0.0f;
-                rs_numerator<PBR>(s, h, ray_d, thr, L, bsdf_avail, &target);
+                rs_numerator<PBR, PTEX>(s, h, ray_d, thr, L, bsdf_avail, &target);
                 bool sel = false;
                 const int mult = rs_merge(r, prev, target, rv.max_spatial_candidates, sel_u[1 + j], false, &sel);
                 if (mult > 0 && nsrc < 9) {
@@ -323,7 +324,7 @@ __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& r
         float sst = r.selected_target;
         float sum = r.selected_target * (float)canonical;
         for (int i = 0; i < nsrc; ++i) {
-            const float t = rs_target_at_source<BLOCK, PBR>(s, rv, cam, rv.history[src_idx[i]], r,
+            const float t = rs_target_at_source<BLOCK, PBR, PTEX>(s, rv, cam, rv.history[src_idx[i]], r,
                 bsdf_avail, st);
             sum += t * (float)mults[i];
             if (selected_source == i) sst = t;
@@ -336,7 +337,7 @@ __device__ inline RestirOut restir_shade(const SceneView& s, const RestirView& r
     if (r.valid == 0) return ro;
     const LightSample L = analytic_candidate(s, h.pos, r.light_index, r.sample_u0, r.sample_u1);
     float target = 0.0f;
-    const f3 num = rs_numerator<PBR>(s, h, ray_d, thr, L, bsdf_avail, &target);
+    const f3 num = rs_numerator<PBR, PTEX>(s, h, ray_d, thr, L, bsdf_avail, &target);
     if (!L.valid || target <= 0.0f) return ro;
     const float max_t = L.infinite ? kRayFar : L.dist - kRayEps * 3.0f;
     if (max_t <= kRayEps) return ro;

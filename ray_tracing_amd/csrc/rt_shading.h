@@ -24,6 +24,7 @@ struct HitInfo {
     float t;
     f3 pos, gn, sn, base, emission;
     float fuzz, ior;
+    float tu, tv;   // texture coordinates (set where a texture is bound)
     int mat_type, openpbr, prim, prim_type, prim_index;
     bool front;
 };
@@ -47,10 +48,13 @@ __device__ __forceinline__ void set_face(f3 d, f3 outward, HitInfo& h) {
 // kMediumHit - i (<= -2) a scatter event inside medium i.
 constexpr int kMediumHit = -2;
 
-// TEX = false leaves the albedo textures out (a scene without them)
+// TEX = false leaves textures out (a scene without them): albedo and
+// emission textures and the OpenPBR base and emission colour bindings
 template <bool TEX = true>
 __device__ inline void make_hit(const SceneView& s, f3 o, f3 d, float t, int prim, HitInfo& h) {
     h.t = t;
+    h.tu = 0.0f;
+    h.tv = 0.0f;
     int mat;
     if (prim <= kMediumHit) {
         // the medium candidate of intersect_scene (programs.cu:1736-1748): no
@@ -122,28 +126,63 @@ __device__ inline void make_hit(const SceneView& s, f3 o, f3 d, float t, int pri
     h.fuzz = m.fuzz;
     h.ior = m.ior;
     h.base = ld3(m.albedo);
-    if (TEX && s.mat_albedo_tex != nullptr) {
-        // populate_material's albedo texture (programs.cu:978-979)
-        const int ti = s.mat_albedo_tex[mat];
-        if (ti >= 0) {
-            float u = 0.0f, v = 0.0f;
-            if (h.prim_type >= 0) surface_uv(s, o, d, h.prim_type, h.prim_index, h.pos, &u, &v);
-            h.base = eval_texture(s, ti, u, v, h.pos);
-        }
-    }
     h.emission = ld3(m.emission);
+    const bool pbr_ok = m.type == RT_MAT_OPENPBR && m.openpbr_index >= 0 && m.openpbr_index < s.openpbr_count;
+    if (TEX) {
+        // populate_material's albedo and emission textures (programs.cu:978-981;
+        // an emission texture only shows on the front of a diffuse light) and
+        // the OpenPBR bindings read at every hit (:995-1008)
+        const int at = s.mat_albedo_tex != nullptr ? s.mat_albedo_tex[mat] : -1;
+        const int et = s.mat_emission_tex != nullptr && m.type == RT_MAT_DIFFUSE_LIGHT && h.front
+                           ? s.mat_emission_tex[mat] : -1;
+        const bool pt = s.pbr_tex != nullptr && pbr_ok;
+        if ((at >= 0 || et >= 0 || pt) && h.prim_type >= 0)
+            surface_uv(s, o, d, h.prim_type, h.prim_index, h.pos, &h.tu, &h.tv);
+        if (at >= 0) h.base = eval_texture(s, at, h.tu, h.tv, h.pos);
+        if (et >= 0) h.emission = eval_texture(s, et, h.tu, h.tv, h.pos);
+    }
     if (h.mat_type == RT_MAT_DIELECTRIC) { h.base = mk3(0.92f, 0.95f, 1.0f); h.emission = mk3(0, 0, 0); }
     if (h.mat_type == RT_MAT_DIFFUSE_LIGHT && !h.front) h.emission = mk3(0, 0, 0);
     if (h.mat_type != RT_MAT_DIFFUSE_LIGHT) h.emission = mk3(0, 0, 0);
     if (h.mat_type == RT_MAT_OPENPBR) {
-        if (m.openpbr_index < 0 || m.openpbr_index >= s.openpbr_count) {
+        if (!pbr_ok) {
             h.mat_type = -1; h.base = mk3(0, 0, 0); h.emission = mk3(0, 0, 0);
             return;
         }
         const rt_openpbr_material& p = s.openpbr[m.openpbr_index];
-        h.base = ld3(p.base_color);
-        h.emission = openpbr_emission(p);
+        if (TEX && s.pbr_tex != nullptr) {
+            const rt_openpbr_textures tb = s.pbr_tex[m.openpbr_index];
+            opbr::MatTex<opbr::MatMem> mt(opbr::MatMem{&p});
+            if (tb.base_color >= 0)
+                mt.bc = source_to_linear(eval_texture(s, tb.base_color, h.tu, h.tv, h.pos), tb.base_color_space);
+            if (tb.emission_color >= 0)
+                mt.ec = source_to_linear(eval_texture(s, tb.emission_color, h.tu, h.tv, h.pos),
+                    tb.emission_color_space);
+            h.base = mt.bc;
+            h.emission = openpbr_emission_v(mt);
+        } else {
+            h.base = ld3(p.base_color);
+            h.emission = openpbr_emission(p);
+        }
     }
+}
+
+// evaluate_openpbr_scattering_material (programs.cu:944-959): the record with
+// its texture-bound inputs replaced by the textures' values at the hit (the
+// base colour is the hit's, already textured)
+template <class M>
+__device__ inline opbr::MatTex<M> pbr_scattering(const SceneView& s, const M& m, const HitInfo& h) {
+    opbr::MatTex<M> mt(m);
+    const rt_openpbr_textures tb = s.pbr_tex[h.openpbr];
+    mt.bc = h.base;
+    if (tb.specular_color >= 0)
+        mt.sc = source_to_linear(eval_texture(s, tb.specular_color, h.tu, h.tv, h.pos), tb.specular_color_space);
+    if (tb.transmission_color >= 0)
+        mt.tc = source_to_linear(eval_texture(s, tb.transmission_color, h.tu, h.tv, h.pos),
+            tb.transmission_color_space);
+    if (tb.base_metalness >= 0) mt.metal = eval_texture(s, tb.base_metalness, h.tu, h.tv, h.pos).x;
+    if (tb.specular_roughness >= 0) mt.rough = eval_texture(s, tb.specular_roughness, h.tu, h.tv, h.pos).x;
+    return mt;
 }
 
 // path-vertex state carried between bounces
@@ -238,6 +277,9 @@ __device__ __forceinline__ int pick_light(const SceneView& s, float u) {
     return lo;
 }
 
+// TEX = false: every emission is the material's constant (a scene without
+// emission textures or OpenPBR emission bindings)
+template <bool TEX = true>
 __device__ inline LightSample sample_light(const SceneView& s, const HitInfo& h, float4 r) {
     LightSample ls{};
     ls.valid = false;
@@ -285,6 +327,9 @@ __device__ inline LightSample sample_light(const SceneView& s, const HitInfo& h,
     ls.infinite = false;
     float cond = 0.0f;
     int mat = -1;
+    // the sampled point and its texture coordinates (programs.cu:1154,1166-1167,1193-1199)
+    f3 lpos = mk3(0, 0, 0);
+    float tu = 0.0f, tv = 0.0f;
     if (L.type == 0) {
         const rt_sphere sph = s.spheres[L.prim];
         const f3 center = ld3(sph.center);
@@ -302,6 +347,10 @@ __device__ inline LightSample sample_light(const SceneView& s, const HitInfo& h,
         if (ls.dist <= kRayEps) return ls;
         mat = sph.material;
         cond = cone_pdf(cmax);
+        if (TEX) {
+            lpos = sp + ls.dir * ls.dist;
+            sphere_uv(normalize3(lpos - center), &tu, &tv);
+        }
     } else if (L.type == 1) {
         const rt_quad q = s.quads[L.prim];
         const f3 eu = ld3(q.edge_u), ev = ld3(q.edge_v);
@@ -311,6 +360,9 @@ __device__ inline LightSample sample_light(const SceneView& s, const HitInfo& h,
         if (area <= 1e-12f) return ls;
         const f3 normal = div3(an, area);
         mat = q.material;
+        lpos = pos;
+        tu = r.y;
+        tv = r.z;
         const f3 tl = pos - sp;
         const float dsq = lensq3(tl);
         if (dsq <= 1e-12f) return ls;
@@ -328,6 +380,16 @@ __device__ inline LightSample sample_light(const SceneView& s, const HitInfo& h,
         if (da <= 1e-12f) return ls;
         const f3 normal = div3(an, da);
         mat = t.material;
+        lpos = pos;
+        tu = w1;
+        tv = w2;
+        if (TEX && s.tri_uv != nullptr) {
+            const float* tc = s.tri_uv + 6 * (int64_t)L.prim;
+            if (tc[0] == tc[0]) {   // not NaN: has_texcoords
+                tu = tc[0] * w0 + tc[2] * w1 + tc[4] * w2;
+                tv = tc[1] * w0 + tc[3] * w1 + tc[5] * w2;
+            }
+        }
         const f3 tl = pos - sp;
         const float dsq = lensq3(tl);
         if (dsq <= 1e-12f) return ls;
@@ -337,9 +399,28 @@ __device__ inline LightSample sample_light(const SceneView& s, const HitInfo& h,
     }
     if (cond <= 0.0f || mat < 0 || mat >= s.material_count) return ls;
     const rt_material m = s.materials[mat];
+    // evaluate_material_emission (programs.cu:1013-1024) at the sampled point
+    int et = -1;
+    if (TEX) {
+        if (m.type == RT_MAT_OPENPBR && s.pbr_tex != nullptr && m.openpbr_index >= 0
+            && m.openpbr_index < s.openpbr_count)
+            et = s.pbr_tex[m.openpbr_index].emission_color;
+        else if (m.type != RT_MAT_OPENPBR && s.mat_emission_tex != nullptr)
+            et = s.mat_emission_tex[mat];
+    }
     if (m.type == RT_MAT_OPENPBR) {
-        ls.emission = (m.openpbr_index >= 0 && m.openpbr_index < s.openpbr_count)
-                          ? openpbr_emission(s.openpbr[m.openpbr_index]) : mk3(0, 0, 0);
+        if (m.openpbr_index < 0 || m.openpbr_index >= s.openpbr_count) {
+            ls.emission = mk3(0, 0, 0);
+        } else if (TEX && et >= 0) {
+            const rt_openpbr_material& p = s.openpbr[m.openpbr_index];
+            opbr::MatTex<opbr::MatMem> mt(opbr::MatMem{&p});
+            mt.ec = source_to_linear(eval_texture(s, et, tu, tv, lpos), s.pbr_tex[m.openpbr_index].emission_color_space);
+            ls.emission = openpbr_emission_v(mt);
+        } else {
+            ls.emission = openpbr_emission(s.openpbr[m.openpbr_index]);
+        }
+    } else if (TEX && et >= 0) {
+        ls.emission = eval_texture(s, et, tu, tv, lpos);
     } else {
         ls.emission = ld3(m.emission);
     }
@@ -354,7 +435,22 @@ __device__ inline LightSample sample_light(const SceneView& s, const HitInfo& h,
 // lobe), kPbrFull every lobe.  thr: the path throughput, the transport
 // context of the OpenPBR lobes.
 enum : int { kPbrOff = 0, kPbrCore = 1, kPbrFull = 2 };
-template <int PBR = kPbrFull>
+
+// an OpenPBR material as the scattering code reads it: the record itself, or
+// with PTEX (a scene that binds OpenPBR inputs to textures) the record with
+// the bound inputs evaluated at the hit
+template <bool PTEX, class M, class F>
+__device__ __forceinline__ void with_pbr_material(const SceneView& s, const M& m, const HitInfo& h, F&& f) {
+    if constexpr (PTEX) {
+        if (s.pbr_tex != nullptr) {
+            f(pbr_scattering(s, m, h));
+            return;
+        }
+    }
+    f(m);
+}
+
+template <int PBR = kPbrFull, bool PTEX = false>
 __device__ inline f3 direct_response(const SceneView& s, const HitInfo& h, f3 ray_d, f3 ldir,
     f3 thr, float* pdf) {
     *pdf = 0.0f;
@@ -373,10 +469,9 @@ __device__ inline f3 direct_response(const SceneView& s, const HitInfo& h, f3 ra
         if (h.mat_type != RT_MAT_OPENPBR) return mk3(0, 0, 0);
         f3 value;
         const rt_openpbr_material* pm = s.openpbr + h.openpbr;
-        if constexpr (PBR == kPbrCore)
-            openpbr_eval_v(opbr::MatCore(pm), h.gn, -normalize3(ray_d), ldir, thr, &value, pdf);
-        else
-            openpbr_eval_v(opbr::MatMem{pm}, h.gn, -normalize3(ray_d), ldir, thr, &value, pdf);
+        auto eval = [&](const auto& m) { openpbr_eval_v(m, h.gn, -normalize3(ray_d), ldir, thr, &value, pdf); };
+        if constexpr (PBR == kPbrCore) with_pbr_material<PTEX>(s, opbr::MatCore(pm), h, eval);
+        else with_pbr_material<PTEX>(s, opbr::MatMem{pm}, h, eval);
         const float cs = fabsf(dot3(h.sn, ldir));
         return value * cs;
     }
@@ -790,7 +885,7 @@ struct BsdfResult {
     int event;   // OpenPBR scatter event (kPbr*), 0 for the other materials
 };
 
-template <int PBR = kPbrFull>
+template <int PBR = kPbrFull, bool PTEX = false>
 __device__ inline BsdfResult sample_bsdf(const SceneView& s, const HitInfo& h, f3 ray_d, float4 r,
     f3 thr) {
     BsdfResult b{};
@@ -842,10 +937,11 @@ __device__ inline BsdfResult sample_bsdf(const SceneView& s, const HitInfo& h, f
     if (PBR != kPbrOff && h.mat_type == RT_MAT_OPENPBR) {
         OpenPbrSampleResult smp;
         const rt_openpbr_material* pm = s.openpbr + h.openpbr;
-        if constexpr (PBR == kPbrCore)
-            openpbr_sample_v(opbr::MatCore(pm), h.gn, -normalize3(ray_d), r.x, r.y, r.z, thr, &smp);
-        else
-            openpbr_sample_v(opbr::MatMem{pm}, h.gn, -normalize3(ray_d), r.x, r.y, r.z, thr, &smp);
+        auto sample = [&](const auto& m) {
+            openpbr_sample_v(m, h.gn, -normalize3(ray_d), r.x, r.y, r.z, thr, &smp);
+        };
+        if constexpr (PBR == kPbrCore) with_pbr_material<PTEX>(s, opbr::MatCore(pm), h, sample);
+        else with_pbr_material<PTEX>(s, opbr::MatMem{pm}, h, sample);
         if (!smp.valid) return b;
         b.event = smp.event;
         b.o = h.pos + smp.wi * kRayEps;

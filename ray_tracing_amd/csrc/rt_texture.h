@@ -133,6 +133,14 @@ __device__ inline f3 eval_texture(const SceneView& s, int index, float u, float 
     return mk3(1.0f, 0.0f, 1.0f);
 }
 
+// sphere_uv (programs.cu:188-193) of a unit outward normal
+__device__ __forceinline__ void sphere_uv(f3 n, float* u, float* v) {
+    const float theta = acosf(fminf(fmaxf(-n.y, -1.0f), 1.0f));
+    const float phi = atan2f(-n.z, n.x) + kTexPi;
+    *u = phi / (2.0f * kTexPi);
+    *v = theta / kTexPi;
+}
+
 // texture coordinates of a hit at pos = o + d * t (programs.cu:1447,1485,
 // 1524-1530): sphere_uv of the outward normal; the quad's edge coordinates;
 // the triangle's barycentrics (recomputed with the intersectors'
@@ -141,11 +149,7 @@ __device__ inline void surface_uv(const SceneView& s, f3 o, f3 d, int prim_type,
     f3 pos, float* u, float* v) {
     if (prim_type == 0) {
         const rt_sphere sp = s.spheres[prim_index];
-        const f3 n = div3(pos - ld3(sp.center), sp.radius);
-        const float theta = acosf(fminf(fmaxf(-n.y, -1.0f), 1.0f));
-        const float phi = atan2f(-n.z, n.x) + kTexPi;
-        *u = phi / (2.0f * kTexPi);
-        *v = theta / kTexPi;
+        sphere_uv(div3(pos - ld3(sp.center), sp.radius), u, v);
     } else if (prim_type == 1) {
         const rt_quad q = s.quads[prim_index];
         const f3 eu = ld3(q.edge_u), ev = ld3(q.edge_v);

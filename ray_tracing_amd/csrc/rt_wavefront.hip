@@ -54,9 +54,10 @@ static_assert(kCntCacheRecord == kStatCacheRecords && kCntCacheQueries == kStatC
 enum : int32_t { kPrevValid = 1, kPrevDelta = 2, kPrevMedium = 12 };
 
 // feature bits of the shade kernel's variants
-// (kShadePbrFull: every OpenPBR lobe, not only the opaque core)
+// (kShadePbrFull: every OpenPBR lobe, not only the opaque core;
+// kShadePbrTex: OpenPBR inputs bound to textures)
 enum : uint32_t { kShadePbr = 1, kShadeAnalytic = 2, kShadeCache = 4, kShadeTexture = 8,
-    kShadeAll = 15, kShadePbrFull = 16 };
+    kShadeAll = 15, kShadePbrFull = 16, kShadePbrTex = 32 };
 
 struct SlotId {
     uint32_t pixel;   // pixel index inside its camera
@@ -337,13 +338,18 @@ __global__ void k_after_shade(uint32_t* counters, int cur) {
 
 // the radiance cache's input at a surface hit: position (w = path slot),
 // outgoing direction and roughness, shading normal, albedo
+template <bool PTEX>
 __device__ __forceinline__ CacheInput cache_input(const SceneView& s, const HitInfo& h, f3 rd,
     uint32_t slot) {
     const f3 wo = -normalize3(rd);
     float rough = 1.0f;
     if (h.mat_type == RT_MAT_METAL) rough = h.fuzz;
     else if (h.mat_type == RT_MAT_DIELECTRIC) rough = 0.0f;
-    else if (h.mat_type == RT_MAT_OPENPBR) rough = s.openpbr[h.openpbr].specular_roughness;
+    else if (h.mat_type == RT_MAT_OPENPBR) {
+        rough = s.openpbr[h.openpbr].specular_roughness;
+        if (PTEX && s.pbr_tex != nullptr && s.pbr_tex[h.openpbr].specular_roughness >= 0)
+            rough = eval_texture(s, s.pbr_tex[h.openpbr].specular_roughness, h.tu, h.tv, h.pos).x;
+    }
     CacheInput c;
     c.a = make_float4(h.pos.x, h.pos.y, h.pos.z, __uint_as_float(slot));
     c.b = make_float4(wo.x, wo.y, wo.z, rough);
@@ -368,6 +374,8 @@ __device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView&
     constexpr int PBR = LITE ? kPbrOff : ((F & kShadePbrFull) != 0 ? kPbrFull : kPbrCore);
     constexpr bool ANALYTIC = (F & kShadeAnalytic) != 0;
     constexpr bool CACHE = (F & kShadeCache) != 0;
+    constexpr bool TEX = (F & kShadeTexture) != 0;
+    constexpr bool PTEX = (F & kShadePbrTex) != 0;
     const SlotRng slot_rng_v = slot_rng(fp);
     const uint32_t count = *count_ptr;
     uint32_t* next_queue = b.queue[cur ^ 1];
@@ -417,7 +425,7 @@ __device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView&
                 const float4 rb = rng4(id.pixel, id.sample, id.key, (uint32_t)bounce,
                     kGroupBsdf, id.stream);
                 HitInfo h;
-                make_hit<(F & kShadeTexture) != 0>(s, ro, rd, hv.x, prim, h);
+                make_hit<TEX>(s, ro, rd, hv.x, prim, h);
                 if (LITE && h.mat_type == RT_MAT_OPENPBR) {
                     pbr = true;
                     goto handed_over;
@@ -468,7 +476,7 @@ __device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView&
                     const bool query = !train && fp.cache_state->usable != 0;
                     CacheInput ci{};
                     if (train || query) {
-                        ci = cache_input(s, h, rd, slot);
+                        ci = cache_input<PTEX>(s, h, rd, slot);
                         if (train) {
                             const uint32_t k = atomicAdd(&b.counters[kCntCacheRecord], 1u);
                             if ((int)k < fp.cache_record_cap) {
@@ -488,10 +496,10 @@ __device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView&
                 const bool bsdf_avail = bounce + 1 < fp.max_bounces && !cache_end;
                 // next-event estimation toward emissive geometry (programs.cu:2249-2280)
                 if (!cache_end && h.mat_type != RT_MAT_DIFFUSE_LIGHT && s.light_count > 0) {
-                    const LightSample ls = sample_light(s, h, rl);
+                    const LightSample ls = sample_light<TEX>(s, h, rl);
                     if (ls.valid) {
                         float bpdf = 0.0f;
-                        const f3 resp = direct_response<PBR>(s, h, rd, ls.dir, thr, &bpdf);
+                        const f3 resp = direct_response<PBR, PTEX>(s, h, rd, ls.dir, thr, &bpdf);
                         if (maxc3(resp) > 0.0f) {
                             const f3 so = h.pos + ls.dir * (kRayEps * 2.0f);
                             const float max_t = ls.infinite ? kRayFar : ls.dist - kRayEps * 3.0f;
@@ -523,7 +531,7 @@ __device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView&
                     const LightSample ls = analytic_candidate(s, h.pos, li, ra.y, ra.z);
                     if (ls.valid) {
                         float bpdf = 0.0f;
-                        const f3 resp = direct_response<PBR>(s, h, rd, ls.dir, thr, &bpdf);
+                        const f3 resp = direct_response<PBR, PTEX>(s, h, rd, ls.dir, thr, &bpdf);
                         if (maxc3(resp) > 0.0f) {
                             const float max_t = ls.infinite ? kRayFar : ls.dist - kRayEps * 3.0f;
                             if (max_t > kRayEps) {
@@ -538,7 +546,7 @@ __device__ __forceinline__ void shade_pass(const SceneView& s, const CameraView&
                     }
                 }
                 if (bsdf_avail) {
-                    const BsdfResult bs = sample_bsdf<PBR>(s, h, rd, rb, thr);
+                    const BsdfResult bs = sample_bsdf<PBR, PTEX>(s, h, rd, rb, thr);
                     if (bs.alive) {
                         thr = thr * bs.weight;
                         bool alive = true;
@@ -642,7 +650,7 @@ __global__ __launch_bounds__(256) void k_analytic_hits(SceneView s, CameraView c
     }
 }
 
-template <int BLOCK, int PBR>
+template <int BLOCK, int PBR, bool PTEX>
 __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, FrameParams fp,
     RestirView rv, WavefrontBuffers b) {
     const SlotRng slot_rng_v = slot_rng(fp);
@@ -671,7 +679,7 @@ __global__ __launch_bounds__(BLOCK) void k_restir(SceneView s, CameraView cams, 
                                  ? rv.current + id.global_pixel
                                  : nullptr;
             const bool bsdf_avail = 1 < fp.max_bounces;
-            const RestirOut ro = restir_shade<BLOCK, PBR>(s, rv, h, xyz(d4), xyz(t4), (int)id.camera, id.key,
+            const RestirOut ro = restir_shade<BLOCK, PBR, PTEX>(s, rv, h, xyz(d4), xyz(t4), (int)id.camera, id.key,
                 cams.pixel_offset[id.camera], x, y, w, hgt,
                 id.pixel, id.sample, id.stream, bsdf_avail, out, st);
             if (ro.shadow) {
@@ -815,20 +823,29 @@ hipError_t wf_shade(const SceneView& s, const CameraView& cams, const FrameParam
     // the full pass: every OpenPBR lobe only when a material uses more than
     // the opaque core
     auto full = [&](const uint32_t* queue, const uint32_t* count) {
-        if (s.pbr_full != 0)
+        if (s.pbr_tex != nullptr) {
+            if (s.pbr_full != 0)
+                k_shade<kShadeAll | kShadePbrFull | kShadePbrTex><<<l.grid, 256, 0, st>>>(s, cams, fp, b,
+                    bounce, cur, queue, count);
+            else
+                k_shade<kShadeAll | kShadePbrTex><<<l.grid, 256, 0, st>>>(s, cams, fp, b, bounce, cur,
+                    queue, count);
+        } else if (s.pbr_full != 0) {
             k_shade<kShadeAll | kShadePbrFull><<<l.grid, 256, 0, st>>>(s, cams, fp, b, bounce, cur,
                 queue, count);
-        else
+        } else {
             k_shade<kShadeAll><<<l.grid, 256, 0, st>>>(s, cams, fp, b, bounce, cur, queue, count);
+        }
     };
     if (mode == 0) {
         full(b.queue[cur], b.counters + cur);
         return hipGetLastError();
     }
     // the legacy-material pass: its plain variant when the scene and frame
-    // use no analytic light, radiance cache or albedo texture
+    // use no analytic light, radiance cache, albedo or emission texture
     const bool plain = s.analytic_count == 0 && s.a_infinite_count == 0 && fp.cache_bounce < 0
-                       && s.mat_albedo_tex == nullptr;
+                       && s.mat_albedo_tex == nullptr && s.mat_emission_tex == nullptr
+                       && s.pbr_tex == nullptr;
     if (plain)
         k_shade_plain<<<lite.grid, 256, 0, st>>>(s, cams, fp, b, bounce, cur, b.queue[cur],
             b.counters + cur);
@@ -850,7 +867,7 @@ int wf_restir_blocks_per_cu() {
     static std::atomic<int> cached{0};
     int n = cached.load(std::memory_order_relaxed);
     if (n == 0) {
-        n = resident_blocks(reinterpret_cast<const void*>(k_restir<128, kPbrCore>), 128);
+        n = resident_blocks(reinterpret_cast<const void*>(k_restir<128, kPbrCore, false>), 128);
         cached.store(n, std::memory_order_relaxed);
     }
     return n;
@@ -876,8 +893,14 @@ int wf_shade_blocks_per_cu(bool lite) {
 hipError_t wf_restir(const SceneView& s, const CameraView& cams, const FrameParams& fp,
     const RestirView& rv, WavefrontBuffers& b, StageLaunch l, hipStream_t st) {
     const unsigned g = (unsigned)(l.grid > 0 ? l.grid : 1);
-    if (s.pbr_full != 0) k_restir<128, kPbrFull><<<g, 128, 0, st>>>(s, cams, fp, rv, b);
-    else k_restir<128, kPbrCore><<<g, 128, 0, st>>>(s, cams, fp, rv, b);
+    if (s.pbr_tex != nullptr) {
+        if (s.pbr_full != 0) k_restir<128, kPbrFull, true><<<g, 128, 0, st>>>(s, cams, fp, rv, b);
+        else k_restir<128, kPbrCore, true><<<g, 128, 0, st>>>(s, cams, fp, rv, b);
+    } else if (s.pbr_full != 0) {
+        k_restir<128, kPbrFull, false><<<g, 128, 0, st>>>(s, cams, fp, rv, b);
+    } else {
+        k_restir<128, kPbrCore, false><<<g, 128, 0, st>>>(s, cams, fp, rv, b);
+    }
     return hipGetLastError();
 }
 

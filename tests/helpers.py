@@ -26,6 +26,8 @@ def oracle_scene(oracle, desc):
     s.triangle_uvs = desc.triangle_uvs
     s.triangle_normals = desc.triangle_normals
     s.medium_count, s.media = desc.medium_count, desc.media
+    s.material_emission_texture = desc.material_emission_texture
+    s.openpbr_textures = desc.openpbr_textures
     return s
 
 
@@ -178,3 +180,66 @@ def sweep() -> np.ndarray:
     rand = rng.uniform(0.0, 1.0, 1_000_000).astype(np.float32)
     special = np.array([0.0, -0.0, 1e-45, 1e-38, 0.998001, 0.999, 1.0, 3.4e38], np.float32)
     return np.concatenate([threshold_neighbourhood(), dense, rand, special])
+
+
+TEX_DT = np.dtype([("type", "<i4"), ("even_texture", "<i4"), ("odd_texture", "<i4"),
+                   ("image_offset", "<i4"), ("image_width", "<i4"), ("image_height", "<i4"),
+                   ("u_address_mode", "<i4"), ("v_address_mode", "<i4"), ("filter_type", "<i4"),
+                   ("scale", "<f4"), ("color", "<f4", 3)])
+PBR_TEX_DT = np.dtype([(n, "<i4") for n in (
+    "base_color", "base_color_space", "specular_color", "specular_color_space",
+    "transmission_color", "transmission_color_space", "emission_color", "emission_color_space",
+    "base_metalness", "specular_roughness")])
+
+
+def texture(kind, color=(0, 0, 0), even=-1, odd=-1, scale=1.0, offset=0, w=0, h=0, um=1, vm=1,
+            filt=0):
+    """One PackedTexture record: kind "constant", "checker", "image" or "noise"."""
+    t = np.zeros(1, TEX_DT)
+    t["type"] = {"constant": 0, "checker": 1, "image": 2, "noise": 3}[kind]
+    t["color"], t["even_texture"], t["odd_texture"], t["scale"] = color, even, odd, scale
+    t["image_offset"], t["image_width"], t["image_height"] = offset, w, h
+    t["u_address_mode"], t["v_address_mode"], t["filter_type"] = um, vm, filt
+    return t
+
+
+def pbr_bindings(n, **kw):
+    """n OpenPBR texture-binding records, all unbound (-1, linear sRGB) but for
+    the per-material overrides kw[name] = [(material, texture[, space]), ...]."""
+    b = np.zeros(n, PBR_TEX_DT)
+    for name in ("base_color", "specular_color", "transmission_color", "emission_color",
+                 "base_metalness", "specular_roughness"):
+        b[name] = -1
+    for name in ("base_color_space", "specular_color_space", "transmission_color_space",
+                 "emission_color_space"):
+        b[name] = 1
+    for name, items in kw.items():
+        for it in items:
+            b[name][it[0]] = it[1]
+            if len(it) > 2:
+                b[name + "_space"][it[0]] = it[2]
+    return b
+
+
+def attach_textures(scene, keep, textures=None, texels=None, albedo=None, emission=None,
+                    pbr=None, triangle_uvs=None):
+    """Sets the texture fields of an oracle Scene or a product SceneDesc (same
+    layout) over numpy arrays, which `keep` keeps alive."""
+    def ref(a, dtype=None):
+        a = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype))
+        keep.append(a)
+        return a.ctypes.data
+    if textures is not None:
+        t = np.concatenate(textures) if isinstance(textures, list) else textures
+        scene.texture_count, scene.textures = len(t), ref(t)
+    if texels is not None:
+        px = np.asarray(texels, np.float32).reshape(-1, 3)
+        scene.image_texel_count, scene.image_texels = len(px), ref(px)
+    if albedo is not None:
+        scene.material_albedo_texture = ref(albedo, np.int32)
+    if emission is not None:
+        scene.material_emission_texture = ref(emission, np.int32)
+    if pbr is not None:
+        scene.openpbr_textures = ref(pbr)
+    if triangle_uvs is not None:
+        scene.triangle_uvs = ref(triangle_uvs, np.float32)

@@ -21,7 +21,7 @@ def test_exports_every_declared_symbol(rt):
     assert sorted(rt.EXPORTED) == names
     for n in names:
         assert hasattr(rt.lib(), n), f"{n} not exported"
-    assert rt.lib().rt_abi_version() == 2
+    assert rt.lib().rt_abi_version() == 3
 
 
 def test_struct_layouts_match_oracle(rt, oracle):

@@ -90,11 +90,13 @@ def test_noise_texture_scene_flattens_with_a_texture_table(tmp_path):
     # the diffuse "perlin" material is bound to texture 0, the light keeps its constant
     assert f.albedo_bindings.tolist() == [0, -1]
     assert tuple(f.materials[1].emission) == (4.0, 4.0, 4.0)
-    # textured emission is refused
+    assert f.emission_bindings is None and not f.desc.material_emission_texture
+    # a textured emission binds the light's emission texture
     p = str(tmp_path / "scene.yaml")
     write(p, open(scene("simple_light")).read().replace("emission: light_tex", "emission: perlin_noise"))
-    with pytest.raises(S.SceneFileError, match="textured emission"):
-        S.flatten(S.load_scene_definition(p))
+    g = S.flatten(S.load_scene_definition(p))
+    assert g.emission_bindings.tolist() == [-1, 0]
+    assert g.desc.material_emission_texture
 
 
 def test_obj_texture_coordinates(tmp_path):
