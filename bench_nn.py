@@ -113,7 +113,7 @@ def main():
     import torch
     if not torch.cuda.is_available():
         raise SystemExit("bench_nn needs a GPU")
-    weights = os.path.join(HERE, "weights", "denoiser.npy")
+    weights = os.path.join(HERE, "ray_tracing_amd", "weights", "denoiser.npy")
     nn = rt.Denoiser(0, np.load(weights))
     res = []
     for name, sizes in (("infer 4x640x480", [(640, 480)] * 4),

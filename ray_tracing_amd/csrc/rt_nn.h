@@ -29,7 +29,6 @@ constexpr int kNnConvs = 15;
 constexpr int kNnOut = 32;        // channels of d0 (the head's input)
 constexpr int kNnHeadParams = 3 * kNnOut + 3;
 constexpr int kNnMaxImages = 16;
-constexpr int kNnGradPixels = 2048;  // pixels per weight-gradient block
 
 // convolution l: c0 channels from tensor A at its level, c1 channels from
 // tensor B one level coarser (upsampled), cout outputs

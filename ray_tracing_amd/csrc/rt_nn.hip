@@ -87,35 +87,51 @@ struct ConvArgs {
     uint16_t* out;        // [P][COUT]
 };
 
-// Raw buffer resource over `bytes` bytes (gfx9 descriptor word 3: 32-bit
-// data format); a load at an offset past the end returns zeros, which is how
-// taps outside the image and padding channels read as zero without a branch.
-constexpr uint32_t kBufWord3 = 0x00020000u;
-constexpr int kBufOob = 0x7FFFF000;  // beyond every buffer, also plus an immediate offset < 4096 (sizes are checked < 2^31 - 2^12 bytes)
+// Output tiles of the convolution kernel: kCvW pixels of kCvRows rows of one
+// image, one row per wave (four 16-pixel MFMA tiles).  The input rows
+// y0 - 1 .. y0 + kCvRows, pixels x0 - 1 .. x0 + kCvW, are staged in LDS; a
+// decoder's coarser input is staged at its own resolution (the parents of
+// those pixels: kCvCoarseRows x kCvCoarseRun).
+constexpr int kCvW = 64;
+constexpr int kCvRows = 4;
+constexpr int kCvRun = kCvW + 2;
+constexpr int kCvInRows = kCvRows + 2;
+constexpr int kCvCoarseRun = kCvW / 2 + 2;
+constexpr int kCvCoarseRows = kCvRows / 2 + 2;
 
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, (int)kBufWord3);
+// LDS row stride (bf16) of a staged pixel of c channels: at least c, and an
+// odd number of 16-byte units, so that the 8 pixels one quarter-wave of
+// 16-byte reads touches fall on distinct bank groups
+__host__ __device__ constexpr int cv_row(int c) { return (c / 8) % 2 == 1 ? c : c + 8; }
+
+// tiles of kCvW x kCvRows pixels of a level: per image, rows of tiles
+__device__ __forceinline__ void tile_locate(const NnImages& im, int b, int* img, int* x0, int* y0) {
+    int i = 0;
+    for (;;) {
+        const int w = im.width[i], h = (im.offset[i + 1] - im.offset[i]) / w;
+        const int tx = (w + kCvW - 1) / kCvW, n = tx * ((h + kCvRows - 1) / kCvRows);
+        if (b < n || i + 1 >= im.count) {
+            *img = i;
+            *y0 = (b / tx) * kCvRows;
+            *x0 = (b - (b / tx) * tx) * kCvW;
+            return;
+        }
+        b -= n;
+        ++i;
+    }
 }
 
-__device__ __forceinline__ bf16x8 buf_load16(__amdgpu_buffer_rsrc_t r, int off) {
-    return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-}
-
-// pixel tiles (of 16) per wave: each weight fragment feeds MT MFMAs
-template <int COUT>
-constexpr int conv_mt() { return COUT / 16 <= 4 ? 4 : 2; }
-
-// One wave computes out[16 MT pixels][COUT] as the transposed GEMM
-// out^T = W X^T: the MFMA A operand is the weights (row = output channel,
-// 8 consecutive K per lane, one 16-byte load), the B operand the input
-// (column = pixel, 8 channels of one tap per lane).  K runs tap by tap, each
-// tap's channels padded to a multiple of 32, so every MFMA step has one
-// compile-time tap: a lane's input address is its pixel's precomputed base
-// plus a constant, and the tap's in-image bit selects it or an out-of-range
-// offset (zeros).  Decoder convolutions read channels >= C0 at the parent
-// pixel of the coarser level (nearest upsampling folded into the load).
-// A lane ends with 4 consecutive output channels of one pixel and stores
-// them as 8 bytes.  MODE 0: out = relu(acc + bias); MODE 1: out = acc.
+// One block computes out[tile][COUT] for a kCvW x kCvRows tile as the
+// transposed GEMM out^T = W X^T: the MFMA A operand is the weights (row =
+// output channel, 8 consecutive K per lane, one 16-byte load from the
+// L1-resident weight matrix, fetched one K step ahead), the B operand the
+// input (column = pixel, 8 channels of one tap per lane), read from the
+// staged tile with one 16-byte LDS load.  K runs tap by tap, each tap's
+// channels padded to a multiple of 32 (padding lanes read zero).  The fine
+// tile holds zeros outside the image; a decoder's coarse channels are read
+// at the parent of the tap's pixel and zeroed where that pixel is outside
+// the image.  A lane ends with 4 consecutive output channels of one pixel and
+// stores them as 8 bytes.  MODE 0: out = relu(acc + bias); MODE 1: out = acc.
 template <int C0, int C1, int COUT, int MODE>
 __global__ __launch_bounds__(256) void k_uconv(ConvArgs a) {
     constexpr int CIN = C0 + C1;
@@ -123,122 +139,118 @@ __global__ __launch_bounds__(256) void k_uconv(ConvArgs a) {
     constexpr int KT = TS / 32;
     constexpr int KP = 9 * TS;
     constexpr int NT = COUT / 16;
-    constexpr int MT = conv_mt<COUT>();
+    constexpr int MT = kCvW / 16;
+    constexpr int RA = cv_row(C0);
+    constexpr int RB = C1 > 0 ? cv_row(C1) : 8;
+    constexpr int STEPS = 9 * KT;
     static_assert(CIN % 8 == 0 && C0 % 8 == 0 && COUT % 16 == 0, "channel layout");
-    const int lane = threadIdx.x & 63;
-    const int r16 = lane & 15;
-    const int kq = lane >> 4;
-    const int waves = gridDim.x * 4;
-    const int groups = (a.P + 16 * MT - 1) / (16 * MT);
-    const __amdgpu_buffer_rsrc_t ra = buf_rsrc(a.A, (uint32_t)a.P * C0 * 2u);
-    const __amdgpu_buffer_rsrc_t rb = buf_rsrc(C1 > 0 ? a.B : a.A, C1 > 0 ? (uint32_t)a.Pp * C1 * 2u : 0u);
-    const uint16_t* wrow = a.W + (size_t)r16 * KP + kq * 8;
-    for (int g = blockIdx.x * 4 + (threadIdx.x >> 6); g < groups; g += waves) {
-        // per pixel tile: byte offset of this lane's pixel and channel octet,
-        // row pitch, in-image bits of the 9 taps; decoder: the coarser image's
-        // base offset and row pitch and the pixel's coordinates
-        int cA[MT], rowA[MT], cB[MT], px_[MT], py_[MT], pwC[MT];
-        uint32_t mask[MT];
-#pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            const int p = (g * MT + m) * 16 + r16;
-            mask[m] = 0u;
-            cA[m] = rowA[m] = cB[m] = px_[m] = py_[m] = pwC[m] = 0;
-            if (p < a.P) {
-                const Pix q = locate(a.im, p);
-                cA[m] = ((q.base + q.y * q.w + q.x) * C0 + kq * 8) * 2;
-                rowA[m] = q.w * C0 * 2;
-#pragma unroll
-                for (int t = 0; t < 9; ++t) {
-                    const int sx = q.x + t % 3 - 1, sy = q.y + t / 3 - 1;
-                    mask[m] |= (sx >= 0 && sx < q.w && sy >= 0 && sy < q.h) ? (1u << t) : 0u;
-                }
-                if (C1 > 0) {
-                    cB[m] = (a.pim.offset[q.img] * C1 + kq * 8) * 2;
-                    pwC[m] = a.pim.width[q.img] * C1 * 2;
-                    px_[m] = q.x;
-                    py_[m] = q.y;
-                }
-            }
+    __shared__ __align__(16) uint16_t sa[kCvInRows * kCvRun * RA];
+    __shared__ __align__(16) uint16_t sb[C1 > 0 ? kCvCoarseRows * kCvCoarseRun * RB : 8];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, kq = lane >> 4;
+    int img, x0, y0;
+    tile_locate(a.im, blockIdx.x, &img, &x0, &y0);
+    const int w = a.im.width[img], h = (a.im.offset[img + 1] - a.im.offset[img]) / w;
+    const int base = a.im.offset[img];
+    for (int u = tid; u < kCvInRows * kCvRun * (C0 / 8); u += 256) {
+        const int rest = u / (C0 / 8), c = (u - rest * (C0 / 8)) * 8;
+        const int r = rest / kCvRun, q = rest - r * kCvRun;
+        const int x = x0 - 1 + q, y = y0 - 1 + r;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (x >= 0 && x < w && y >= 0 && y < h)
+            v = *reinterpret_cast<const uint4*>(a.A + (size_t)(base + y * w + x) * C0 + c);
+        *reinterpret_cast<uint4*>(sa + rest * RA + c) = v;
+    }
+    const int px0 = (x0 - 1) >> 1, py0 = (y0 - 1) >> 1;
+    if (C1 > 0) {
+        const int pw = a.pim.width[img];
+        const int ph = (a.pim.offset[img + 1] - a.pim.offset[img]) / pw;
+        for (int u = tid; u < kCvCoarseRows * kCvCoarseRun * (C1 / 8); u += 256) {
+            const int rest = u / (C1 / 8), c = (u - rest * (C1 / 8)) * 8;
+            const int r = rest / kCvCoarseRun, q = rest - r * kCvCoarseRun;
+            const int x = px0 + q, y = py0 + r;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (x >= 0 && x < pw && y >= 0 && y < ph)
+                v = *reinterpret_cast<const uint4*>(a.B + (size_t)(a.pim.offset[img] + y * pw + x) * C1 + c);
+            *reinterpret_cast<uint4*>(sb + rest * RB + c) = v;
         }
-        f32x4 acc[MT][NT];
+    }
+    __syncthreads();
+    f32x4 acc[MT][NT];
 #pragma unroll
-        for (int m = 0; m < MT; ++m)
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[m][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        // per-lane channel validity of each K step (padding past CIN reads
-        // zero) and, for a step straddling the two inputs, which one it reads
-        bool ch_ok[KT], in_a[KT];
+        for (int nt = 0; nt < NT; ++nt) acc[m][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const uint16_t* wrow = a.W + (size_t)r16 * KP + kq * 8;
+    bf16x8 wv[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) wv[nt] = *reinterpret_cast<const bf16x8*>(wrow + (size_t)nt * 16 * KP);
+    const bf16x8 zero8 = __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u));
+    const int y = y0 + wave;
+#pragma unroll 1
+    for (int t = 0; t < 9; ++t) {
+        const int dy = t / 3 - 1, dx = t % 3 - 1;
+        // this lane's pixel in tile m is (x0 + 16 m + r16, y); the tap reads
+        // fine slot (wave + dy + 1, 16 m + r16 + dx + 1)
+        const uint16_t* xa = sa + ((wave + dy + 1) * kCvRun + r16 + dx + 1) * RA;
 #pragma unroll
         for (int c = 0; c < KT; ++c) {
-            ch_ok[c] = c * 32 + kq * 8 < CIN;
-            in_a[c] = c * 32 + kq * 8 < C0;
-        }
-#pragma unroll 1
-        for (int t = 0; t < 9; ++t) {
-            const int dy = t / 3 - 1, dx = t % 3 - 1;
-            // this tap's input base offsets (out-of-range when outside the image)
-            int baseA[MT], baseB[MT];
-#pragma unroll
-            for (int m = 0; m < MT; ++m) {
-                const bool tap_ok = ((mask[m] >> t) & 1u) != 0u;
-                const int rowoff = dy < 0 ? -rowA[m] : dy > 0 ? rowA[m] : 0;
-                baseA[m] = tap_ok ? cA[m] + rowoff + dx * (C0 * 2) : kBufOob;
-                baseB[m] = kBufOob;
-                if (C1 > 0) {
-                    // parent of (x + dx, y + dy) at the coarser level
-                    const int prow = ((py_[m] + dy) >> 1) * pwC[m];
-                    const int pcol = ((px_[m] + dx) >> 1) * (C1 * 2);
-                    baseB[m] = tap_ok ? cB[m] + prow + pcol : kBufOob;
-                }
-            }
-            const uint16_t* wt = wrow + t * TS;
-#pragma unroll
-            for (int c = 0; c < KT; ++c) {
-                const int cb = c * 32;   // channel of lane group kq = 0
-                bf16x8 wv[NT];
+            const int step = t * KT + c;
+            // the next step's weights, fetched while this step's MFMAs run
+            bf16x8 wn[NT];
+            if (step + 1 < STEPS) {
+                const int tn = (step + 1) / KT, cn = (step + 1) - tn * KT;
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
-                    wv[nt] = *reinterpret_cast<const bf16x8*>(wt + (size_t)nt * 16 * KP + cb);
-                const bool full = cb + 32 <= CIN;
-                bf16x8 xv[MT];
+                    wn[nt] = *reinterpret_cast<const bf16x8*>(wrow + (size_t)nt * 16 * KP + tn * TS + cn * 32);
+            }
+            const int ch = c * 32 + kq * 8;    // this lane's first channel
+            bf16x8 xv[MT];
 #pragma unroll
-                for (int m = 0; m < MT; ++m) {
-                    if (C1 == 0 || cb + 32 <= C0) {
-                        xv[m] = buf_load16(ra, (full || ch_ok[c] ? baseA[m] : kBufOob) + cb * 2);
-                    } else if (cb >= C0) {
-                        xv[m] = buf_load16(rb, (full || ch_ok[c] ? baseB[m] : kBufOob) + (cb - C0) * 2);
-                    } else {
-                        const bf16x8 xa = buf_load16(ra, (in_a[c] ? baseA[m] : kBufOob) + cb * 2);
-                        const bf16x8 xb = buf_load16(rb, (!in_a[c] && ch_ok[c] ? baseB[m] : kBufOob) + (cb - C0) * 2);
-                        xv[m] = __builtin_bit_cast(bf16x8, __builtin_bit_cast(uint4, xa) | __builtin_bit_cast(uint4, xb));
-                    }
+            for (int m = 0; m < MT; ++m) {
+                if (C1 == 0 || c * 32 + 32 <= C0) {
+                    xv[m] = *reinterpret_cast<const bf16x8*>(xa + m * 16 * RA + (ch < CIN ? ch : 0));
+                    if (c * 32 + 32 > CIN && ch >= CIN) xv[m] = zero8;
+                } else {
+                    const int xs = x0 + m * 16 + r16 + dx, ys = y + dy;
+                    const bool in = xs >= 0 && xs < w && ys >= 0 && ys < h;
+                    const uint16_t* pb = sb + (((ys >> 1) - py0) * kCvCoarseRun + ((xs >> 1) - px0)) * RB;
+                    const uint16_t* src = ch < C0 ? xa + m * 16 * RA + ch : pb + (ch < CIN ? ch - C0 : 0);
+                    xv[m] = *reinterpret_cast<const bf16x8*>(src);
+                    if (ch >= CIN || (ch >= C0 && !in)) xv[m] = zero8;
                 }
+            }
 #pragma unroll
-                for (int m = 0; m < MT; ++m)
+            for (int m = 0; m < MT; ++m)
 #pragma unroll
-                    for (int nt = 0; nt < NT; ++nt)
-                        acc[m][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv[nt], xv[m], acc[m][nt], 0, 0, 0);
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[m][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv[nt], xv[m], acc[m][nt], 0, 0, 0);
+            if (step + 1 < STEPS) {
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) wv[nt] = wn[nt];
             }
         }
-        // D[i][j]: i = output channel 4 kq + e of the tile, j = pixel r16
+    }
+    // D[i][j]: i = output channel 4 kq + e of the tile, j = pixel r16
+    if (y >= h) return;
 #pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            const int p = (g * MT + m) * 16 + r16;
-            if (p >= a.P) continue;
+    for (int m = 0; m < MT; ++m) {
+        const int x = x0 + m * 16 + r16;
+        if (x >= w) continue;
+        const size_t p = (size_t)(base + y * w + x);
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                const int co = nt * 16 + kq * 4;
-                float v[4];
+        for (int nt = 0; nt < NT; ++nt) {
+            const int co = nt * 16 + kq * 4;
+            float v[4];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    v[e] = acc[m][nt][e];
-                    if (MODE == 0) v[e] = fmaxf(v[e] + a.bias[co + e], 0.0f);
-                }
-                const uint2 packed = make_uint2((uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16),
-                                                (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16));
-                *reinterpret_cast<uint2*>(a.out + (size_t)p * COUT + co) = packed;
+            for (int e = 0; e < 4; ++e) {
+                v[e] = acc[m][nt][e];
+                if (MODE == 0) v[e] = fmaxf(v[e] + a.bias[co + e], 0.0f);
             }
+            const uint2 packed = make_uint2((uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16),
+                                            (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16));
+            *reinterpret_cast<uint2*>(a.out + p * COUT + co) = packed;
         }
     }
 }
@@ -278,46 +290,88 @@ __global__ __launch_bounds__(256) void k_pool(NnImages im, NnImages pim, int Pp,
     *reinterpret_cast<uint4*>(out + (size_t)pp * C + c) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
-// dz = r where act > 0
-__global__ __launch_bounds__(256) void k_mask(size_t n, const uint16_t* r, const uint16_t* act,
+// 8 bf16 values of one 16-byte load as floats, and back
+__device__ __forceinline__ void unpack8(uint4 u, float f[8]) {
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        f[2 * k] = from_bf16((uint16_t)(w[k] & 0xFFFFu));
+        f[2 * k + 1] = from_bf16((uint16_t)(w[k] >> 16));
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float f[8]) {
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = (uint32_t)to_bf16(f[2 * k]) | ((uint32_t)to_bf16(f[2 * k + 1]) << 16);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+// the bf16 words of v where the matching words of act are > 0, else 0
+__device__ __forceinline__ uint4 relu_mask8(uint4 v, uint4 act) {
+    const uint32_t a[4] = {act.x, act.y, act.z, act.w};
+    uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t lo = from_bf16((uint16_t)(a[k] & 0xFFFFu)) > 0.0f ? 0x0000FFFFu : 0u;
+        const uint32_t hi = from_bf16((uint16_t)(a[k] >> 16)) > 0.0f ? 0xFFFF0000u : 0u;
+        w[k] &= lo | hi;
+    }
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// dz = r where act > 0; 8 channels per thread
+__global__ __launch_bounds__(256) void k_mask(size_t n8, const uint16_t* r, const uint16_t* act,
     uint16_t* dz) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    dz[i] = from_bf16(act[i]) > 0.0f ? r[i] : (uint16_t)0;
+    if (i >= n8) return;
+    reinterpret_cast<uint4*>(dz)[i] = relu_mask8(reinterpret_cast<const uint4*>(r)[i],
+        reinterpret_cast<const uint4*>(act)[i]);
 }
 
 // dz at level l+1 = sum over the covered level-l pixels of r[:, off .. off + C)
-// (the upsampled input's gradient), where act > 0
+// (the upsampled input's gradient), where act > 0; 8 channels per thread
 __global__ __launch_bounds__(256) void k_up_combine(NnImages im, NnImages pim, int Pp, int C,
     const uint16_t* r, int cr, int off, const uint16_t* act, uint16_t* dz) {
+    const int c8 = C / 8;
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= Pp * C) return;
-    const int pp = i / C, c = i - pp * C;
+    if (i >= Pp * c8) return;
+    const int pp = i / c8, c = (i - pp * c8) * 8;
     const Pix q = locate(pim, pp);
     const int w = im.width[q.img], h = image_height(im, q.img), base = im.offset[q.img];
-    float s = 0.0f;
+    float s[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     for (int dy = 0; dy < 2; ++dy) {
         for (int dx = 0; dx < 2; ++dx) {
             const int x = 2 * q.x + dx, y = 2 * q.y + dy;
-            if (x < w && y < h) s += from_bf16(r[(size_t)(base + y * w + x) * cr + off + c]);
+            if (x < w && y < h) {
+                float f[8];
+                unpack8(*reinterpret_cast<const uint4*>(r + (size_t)(base + y * w + x) * cr + off + c), f);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) s[k] += f[k];
+            }
         }
     }
-    dz[(size_t)pp * C + c] = from_bf16(act[(size_t)pp * C + c]) > 0.0f ? to_bf16(s) : (uint16_t)0;
+    const size_t o = (size_t)pp * C + c;
+    *reinterpret_cast<uint4*>(dz + o) = relu_mask8(pack8(s), *reinterpret_cast<const uint4*>(act + o));
 }
 
 // dz at level l = (skip gradient rs[:, 0..C) + pooled-path gradient rp at the
-// parent / pixels the parent covers), where act > 0
+// parent / pixels the parent covers), where act > 0; 8 channels per thread
 __global__ __launch_bounds__(256) void k_skip_combine(NnImages im, NnImages pim, int P, int C,
     const uint16_t* rs, int cs, const uint16_t* rp, const uint16_t* act, uint16_t* dz) {
+    const int c8 = C / 8;
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= P * C) return;
-    const int p = i / C, c = i - p * C;
+    if (i >= P * c8) return;
+    const int p = i / c8, c = (i - p * c8) * 8;
     const Pix q = locate(im, p);
     const int px = q.x >> 1, py = q.y >> 1;
     const int n = min(2, q.w - 2 * px) * min(2, q.h - 2 * py);
     const int pp = pim.offset[q.img] + py * pim.width[q.img] + px;
-    const float v = from_bf16(rs[(size_t)p * cs + c]) + from_bf16(rp[(size_t)pp * C + c]) / (float)n;
-    dz[(size_t)p * C + c] = from_bf16(act[(size_t)p * C + c]) > 0.0f ? to_bf16(v) : (uint16_t)0;
+    float a[8], b[8];
+    unpack8(*reinterpret_cast<const uint4*>(rs + (size_t)p * cs + c), a);
+    unpack8(*reinterpret_cast<const uint4*>(rp + (size_t)pp * C + c), b);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = a[k] + b[k] / (float)n;
+    const size_t o = (size_t)p * C + c;
+    *reinterpret_cast<uint4*>(dz + o) = relu_mask8(pack8(a), *reinterpret_cast<const uint4*>(act + o));
 }
 
 // Per-channel weight of the log-space squared error: with e = log1p(y) -
@@ -334,7 +388,7 @@ __device__ __forceinline__ float loss_weight(float t) {
 // gradient of d0.
 __global__ __launch_bounds__(256) void k_head(int P, const uint16_t* h3, const float* head,
     const float4* beauty, const float4* target, float4* out, uint16_t* dh3, float* partial) {
-    __shared__ float red[256];
+    __shared__ float red[4 * (kNnHeadParams + 1)];
     const int p = blockIdx.x * 256 + threadIdx.x;
     const bool valid = p < P;
     float h[kNnOut];
@@ -399,6 +453,9 @@ __global__ __launch_bounds__(256) void k_head(int P, const uint16_t* h3, const f
         }
     }
     if (!target) return;
+    // per-block sums of the head gradient terms and the loss: a butterfly
+    // within each wave, then the four wave sums in wave order (fixed order)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int i = 0; i < kNnHeadParams + 1; ++i) {
         float v = 0.0f;
         if (valid) {
@@ -406,161 +463,266 @@ __global__ __launch_bounds__(256) void k_head(int P, const uint16_t* h3, const f
             else if (i < kNnHeadParams) v = dout[i - 3 * kNnOut];
             else v = loss;
         }
-        red[threadIdx.x] = v;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) partial[(size_t)blockIdx.x * (kNnHeadParams + 1) + i] = red[0];
-        __syncthreads();
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) red[wave * (kNnHeadParams + 1) + i] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kNnHeadParams + 1; i += 256) {
+        const int n = kNnHeadParams + 1;
+        partial[(size_t)blockIdx.x * n + i] = ((red[i] + red[n + i]) + red[2 * n + i]) + red[3 * n + i];
     }
 }
 
 struct WgradArgs {
     NnImages im, pim;
-    int P;
+    int segs;             // row segments of kWgSeg pixels at this level
+    int chunk_segs;       // segments per block
     const uint16_t* dz;   // [P][cout]
     const uint16_t* A;    // [P][C0]
     const uint16_t* B;    // [P coarser][C1]
-    float* partial;       // [chunks][cout][NW * 16]
+    float* partial;       // [chunks][cout][nw16]
 };
 
-constexpr int kWgStep = 64;            // pixels per LDS stage (two MFMA K slices)
-constexpr int kWgRow = kWgStep + 8;    // LDS row stride in bf16: 144 B, 16-byte aligned rows
+constexpr int kWgSeg = 64;   // pixels per stage: one segment of one image row (two MFMA K steps)
 
-// scatter 8 channels of one pixel into an LDS [channel][pixel] tile
-__device__ __forceinline__ void lds_scatter8(uint16_t* s, int c, int px, uint4 raw) {
-    const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        s[(c + 2 * j) * kWgRow + px] = (uint16_t)(w[j] & 0xFFFFu);
-        s[(c + 2 * j + 1) * kWgRow + px] = (uint16_t)(w[j] >> 16);
+// LDS row stride (bf16 elements) of a [pixel][channel] tile: at least c, and
+// 2 * stride = 32 mod 128 bytes, so that the four rows one 16-lane
+// transposed read touches fall on distinct banks
+__host__ __device__ constexpr int wg_row(int c) { return c + ((16 - c % 64) + 64) % 64; }
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+
+// ds_read_b64_tr_b16: within each group of 16 lanes, lane j reads 4 bf16 at
+// its address; lane i then holds element (i % 4) of lanes 4 k + i / 4,
+// k = 0..3.  With lane j addressing row r0 + j / 4, column c0 + 4 (j % 4) of a
+// row-major [row][column] tile, lane i holds column c0 + i of rows r0..r0+3:
+// a K-contiguous MFMA operand read straight from a pixel-major tile.  Two
+// reads, 4 rows apart, give the 8 rows of a 16x16x32 operand.
+__device__ __forceinline__ bf16x8 tr_operand(const uint16_t* tile, int row_stride, int r0, int c0,
+    int j16) {
+    const uint16_t* p = tile + (r0 + (j16 >> 2)) * row_stride + c0 + 4 * (j16 & 3);
+    const s16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) s16x4*)p);
+    const s16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+        (__attribute__((address_space(3))) s16x4*)(p + 4 * row_stride));
+    return __builtin_bit_cast(bf16x8, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+}
+
+struct Seg {
+    int img, base, w, h, y, x0;
+};
+
+// segment s: image, row and first pixel (segments run along each row, rows
+// in order, images back to back)
+__device__ __forceinline__ Seg seg_locate(const NnImages& im, int s) {
+    int i = 0;
+    Seg r;
+    for (;;) {
+        r.w = im.width[i];
+        r.h = (im.offset[i + 1] - im.offset[i]) / r.w;
+        const int n = r.h * ((r.w + kWgSeg - 1) / kWgSeg);
+        if (s < n || i + 1 >= im.count) break;
+        s -= n;
+        ++i;
     }
+    const int per_row = (r.w + kWgSeg - 1) / kWgSeg;
+    r.img = i;
+    r.base = im.offset[i];
+    r.y = s / per_row;
+    r.x0 = (s - r.y * per_row) * kWgSeg;
+    return r;
 }
 
 // Weight gradient of one convolution as GEMMs with K = pixels:
-//   G[co][t CIN + ci] = sum_p dz[p][co] X[p + off(t)][ci]   (block row t < 9)
-//   G[co][9 CIN]      = sum_p dz[p][co]                      (block row t = 9, the bias)
-// Block chunk * 10 + t covers pixels [chunk * kNnGradPixels, ...).  Per stage of
-// 64 pixels it copies dz and the tap-shifted input into LDS transposed
-// ([channel][pixel]), so that a lane reads the 8 consecutive pixels of one
-// channel an MFMA operand needs with one 16-byte LDS load; wave w owns the
-// (cout / 16) x (CIN / 16) output tiles w, w + 4, ...  Partial sums per chunk
-// are reduced in chunk order afterwards (deterministic).
-template <int C0, int C1, int COUT>
+//   G[co][t CIN + ci] = sum_p dz[p][co] X[p + off(t)][ci]   (taps t)
+//   G[co][9 CIN]      = sum_p dz[p][co]                      (the bias)
+// Block chunk * (9 / TPB) + g covers the segments of one chunk and the taps
+// g TPB .. g TPB + TPB - 1.  Per segment it stages dz and the input rows the
+// taps read (pixels x0 - 1 .. x0 + kWgSeg, zero outside the image) in LDS as
+// [pixel][channel] with 16-byte stores; both MFMA operands (8 consecutive
+// pixels of one channel per lane) come from transposed LDS reads.  Wave w
+// owns the (tap, 16-channel input) tiles w, w + 4, ... of the block with all
+// output tiles, so each operand fragment it reads feeds several MFMAs.
+// Partial sums per chunk are reduced in chunk order afterwards
+// (deterministic).
+template <int C0, int C1, int COUT, int TPB>
 __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
     constexpr int CIN = C0 + C1;
-    constexpr int MT = COUT / 16, NT = CIN / 16;
-    constexpr int TILES = MT * NT;
-    constexpr int TPW = (TILES + 3) / 4;
+    constexpr int MT = COUT / 16, CT = CIN / 16;
+    constexpr int TN = TPB * CT;
+    // the four waves as a WN x WM grid over the (tap, input) x output tiles
+    constexpr int WN = TN >= 4 ? 4 : (TN >= 2 ? 2 : 1);
+    constexpr int WM = 4 / WN;
+    constexpr int NPW = (TN + WN - 1) / WN, MPW = (MT + WM - 1) / WM;
+    constexpr int ROWS = TPB == 9 ? 3 : 1;
+    constexpr int GROUPS = 9 / TPB;
+    constexpr int RZ = wg_row(COUT), RX = wg_row(CIN);
+    constexpr int RUN = kWgSeg + 2;
     constexpr int NWC = ((9 * CIN) / 16 + 1) * 16;
-    static_assert(CIN % 16 == 0 && C0 % 8 == 0 && COUT % 16 == 0, "channel layout");
-    __shared__ __align__(16) uint16_t sdz[COUT * kWgRow];
-    __shared__ __align__(16) uint16_t sx[CIN * kWgRow];
-    __shared__ int src_a[kWgStep], src_b[kWgStep];
+    static_assert(CIN % 16 == 0 && C0 % 8 == 0 && COUT % 16 == 0 && 9 % TPB == 0, "layout");
+    __shared__ __align__(16) uint16_t sdz[kWgSeg * RZ];
+    __shared__ __align__(16) uint16_t sx[ROWS * RUN * RX];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int r16 = lane & 15, kq = lane >> 4;
-    const int chunk = (int)blockIdx.x / 10, t = (int)blockIdx.x - chunk * 10;
-    const bool bias = t == 9;
-    const int dx = t % 3 - 1, dy = t / 3 - 1;
-    const int tiles = bias ? MT : TILES;
-    f32x4 acc[TPW];
+    const int wn = wave % WN, wm = wave / WN;
+    const int j16 = lane & 15, g4 = lane >> 4;
+    const int chunk = (int)blockIdx.x / GROUPS, grp = (int)blockIdx.x - chunk * GROUPS;
+    const int t0 = grp * TPB;
+    const bool bias = grp == 0 && wn == 0;   // the wn = 0 waves cover every output tile once
+    f32x4 acc[MPW][NPW];
+    f32x4 accb[MPW];
 #pragma unroll
-    for (int i = 0; i < TPW; ++i) acc[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const int p0 = chunk * kNnGradPixels;
-    const int p1 = min(a.P, p0 + kNnGradPixels);
+    for (int m = 0; m < MPW; ++m) {
+        accb[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int n = 0; n < NPW; ++n) acc[m][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
     const bf16x8 ones = __builtin_bit_cast(bf16x8, make_uint4(0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u));
-    for (int s0 = p0; s0 < p1; s0 += kWgStep) {
-        if (!bias && tid < kWgStep) {
-            const int p = s0 + tid;
-            int ia = -1, ib = -1;
-            if (p < p1) {
-                const Pix q = locate(a.im, p);
-                const int sx_ = q.x + dx, sy_ = q.y + dy;
-                if (sx_ >= 0 && sx_ < q.w && sy_ >= 0 && sy_ < q.h) {
-                    ia = q.base + sy_ * q.w + sx_;
-                    if (C1 > 0) ib = parent_index(a.pim, q.img, sx_, sy_);
-                }
-            }
-            src_a[tid] = ia;
-            src_b[tid] = ib;
+    const int s_begin = chunk * a.chunk_segs;
+    const int s_end = min(a.segs, s_begin + a.chunk_segs);
+    // software pipeline: the 16-byte pieces of segment s + 1 are fetched into
+    // registers while the MFMAs of segment s run
+    constexpr int NZ = (kWgSeg * (COUT / 8) + 255) / 256;
+    constexpr int NX = (ROWS * RUN * (CIN / 8) + 255) / 256;
+    uint4 rz[NZ], rx[NX];
+    auto fetch = [&](int seg) {
+        const Seg sg = seg_locate(a.im, seg);
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+            const int u = tid + 256 * k;
+            const int q = u / (COUT / 8), c = (u - q * (COUT / 8)) * 8;
+            rz[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (u < kWgSeg * (COUT / 8) && sg.x0 + q < sg.w)
+                rz[k] = *reinterpret_cast<const uint4*>(a.dz + (size_t)(sg.base + sg.y * sg.w + sg.x0 + q) * COUT + c);
         }
-        for (int u = tid; u < kWgStep * (COUT / 8); u += 256) {
-            const int px = u / (COUT / 8), c = (u - px * (COUT / 8)) * 8;
-            const int p = s0 + px;
-            uint4 raw = make_uint4(0u, 0u, 0u, 0u);
-            if (p < p1) raw = *reinterpret_cast<const uint4*>(a.dz + (size_t)p * COUT + c);
-            lds_scatter8(sdz, c, px, raw);
-        }
-        __syncthreads();
-        if (!bias) {
-            for (int u = tid; u < kWgStep * (CIN / 8); u += 256) {
-                const int px = u / (CIN / 8), c = (u - px * (CIN / 8)) * 8;
-                uint4 raw = make_uint4(0u, 0u, 0u, 0u);
-                if (C1 == 0 || c < C0) {
-                    const int i = src_a[px];
-                    if (i >= 0) raw = *reinterpret_cast<const uint4*>(a.A + (size_t)i * C0 + c);
-                } else {
-                    const int i = src_b[px];
-                    if (i >= 0) raw = *reinterpret_cast<const uint4*>(a.B + (size_t)i * C1 + (c - C0));
-                }
-                lds_scatter8(sx, c, px, raw);
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            const int u = tid + 256 * k;
+            const int rest = u / (CIN / 8), c = (u - rest * (CIN / 8)) * 8;
+            const int r = rest / RUN, q = rest - r * RUN;
+            const int dy = ROWS == 3 ? r - 1 : t0 / 3 - 1;
+            const int x = sg.x0 - 1 + q, y = sg.y + dy;
+            rx[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (u < ROWS * RUN * (CIN / 8) && x >= 0 && x < sg.w && y >= 0 && y < sg.h) {
+                if (C1 == 0 || c < C0)
+                    rx[k] = *reinterpret_cast<const uint4*>(a.A + (size_t)(sg.base + y * sg.w + x) * C0 + c);
+                else
+                    rx[k] = *reinterpret_cast<const uint4*>(a.B + (size_t)parent_index(a.pim, sg.img, x, y) * C1 + (c - C0));
             }
         }
+    };
+    if (s_begin < s_end) fetch(s_begin);
+    for (int s = s_begin; s < s_end; ++s) {
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+            const int u = tid + 256 * k;
+            const int q = u / (COUT / 8), c = (u - q * (COUT / 8)) * 8;
+            if (u < kWgSeg * (COUT / 8)) *reinterpret_cast<uint4*>(sdz + q * RZ + c) = rz[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            const int u = tid + 256 * k;
+            const int rest = u / (CIN / 8), c = (u - rest * (CIN / 8)) * 8;
+            if (u < ROWS * RUN * (CIN / 8)) *reinterpret_cast<uint4*>(sx + rest * RX + c) = rx[k];
+        }
         __syncthreads();
+        if (s + 1 < s_end) fetch(s + 1);
 #pragma unroll
-        for (int ks = 0; ks < kWgStep / 32; ++ks) {
+        for (int ks = 0; ks < kWgSeg / 32; ++ks) {
+            const int r0 = ks * 32 + g4 * 8;
+            bf16x8 af[MPW];
 #pragma unroll
-            for (int i = 0; i < TPW; ++i) {
-                const int tile = wave + 4 * i;
-                if (tile < tiles) {
-                    const int mt = bias ? tile : tile / NT, nt = bias ? 0 : tile - (tile / NT) * NT;
-                    const bf16x8 av = *reinterpret_cast<const bf16x8*>(sdz + (mt * 16 + r16) * kWgRow + ks * 32 + kq * 8);
-                    const bf16x8 bv = bias ? ones
-                                           : *reinterpret_cast<const bf16x8*>(sx + (nt * 16 + r16) * kWgRow + ks * 32 + kq * 8);
-                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[i], 0, 0, 0);
+            for (int m = 0; m < MPW; ++m) {
+                const int mt = wm + WM * m;
+                if (mt < MT) af[m] = tr_operand(sdz, RZ, r0, mt * 16, j16);
+            }
+            // B fragments in batches of four (a tile past the last reads the
+            // last one and is not used), so the reads of a batch overlap and
+            // its MFMAs follow without waiting on one read at a time
+            constexpr int NB = NPW < 4 ? NPW : 4;
+#pragma unroll
+            for (int n0 = 0; n0 < NPW; n0 += NB) {
+                bf16x8 bfr[NB];
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    const int nt = min(wn + WN * min(n0 + b, NPW - 1), TN - 1);
+                    const int t = t0 + nt / CT, ct = nt - (nt / CT) * CT;
+                    const int dx = t % 3 - 1;
+                    const int row = ROWS == 3 ? t / 3 : 0;
+                    bfr[b] = tr_operand(sx + row * RUN * RX, RX, r0 + dx + 1, ct * 16, j16);
                 }
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    const int n = n0 + b;
+                    if (n < NPW && wn + WN * n < TN) {
+#pragma unroll
+                        for (int m = 0; m < MPW; ++m)
+                            if (wm + WM * m < MT)
+                                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], bfr[b], acc[m][n], 0, 0, 0);
+                    }
+                }
+            }
+            if (bias) {
+#pragma unroll
+                for (int m = 0; m < MPW; ++m)
+                    if (wm + WM * m < MT)
+                        accb[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], ones, accb[m], 0, 0, 0);
             }
         }
         __syncthreads();
     }
+    // D[i][j] of a tile: i = output channel 4 g4 + e, j = input channel j16
     float* dst = a.partial + (size_t)chunk * COUT * NWC;
 #pragma unroll
-    for (int i = 0; i < TPW; ++i) {
-        const int tile = wave + 4 * i;
-        if (tile >= tiles) continue;
-        const int mt = bias ? tile : tile / NT, nt = bias ? 0 : tile - (tile / NT) * NT;
+    for (int m = 0; m < MPW; ++m) {
+        const int mt = wm + WM * m;
+        if (mt >= MT) continue;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int co = mt * 16 + kq * 4 + e;   // D row
-            if (bias) {
-                if (r16 == 0) dst[(size_t)co * NWC + 9 * CIN] = acc[i][e];
-            } else {
-                dst[(size_t)co * NWC + t * CIN + nt * 16 + r16] = acc[i][e];
-            }
+        for (int n = 0; n < NPW; ++n) {
+            const int nt = wn + WN * n;
+            if (nt >= TN) continue;
+            const int t = t0 + nt / CT, ct = nt - (nt / CT) * CT;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                dst[(size_t)(mt * 16 + g4 * 4 + e) * NWC + t * CIN + ct * 16 + j16] = acc[m][n][e];
+        }
+        if (bias && j16 == 0) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) dst[(size_t)(mt * 16 + g4 * 4 + e) * NWC + 9 * CIN] = accb[m][e];
         }
     }
 }
 
 // sum the chunks' partials of one convolution into the gradient vector
-// (W[co][tap][ci] then b[co]) in chunk order
-__global__ void k_uwgrad_reduce(int chunks, int cout, int cin, const float* partial, float* grad) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+// (W[co][tap][ci] then b[co]): block of 16 outputs x 16 chunk lanes, lane c
+// summing chunks c, c + 16, ... in order, then the 16 lane sums in order
+// (a fixed order: deterministic)
+__global__ __launch_bounds__(256) void k_uwgrad_reduce(int chunks, int cout, int cin,
+    const float* partial, float* grad) {
+    __shared__ float part[16][17];
+    const int o = threadIdx.x & 15, c = threadIdx.x >> 4;
+    const int i = blockIdx.x * 16 + o;
+    const int total = cout * (9 * cin + 1);
     const int nw16 = ((9 * cin) / 16 + 1) * 16;
-    if (i >= cout * (9 * cin + 1)) return;
-    int co, n;
-    if (i < cout * 9 * cin) {
-        co = i / (9 * cin);
-        n = i - co * 9 * cin;
-    } else {
-        co = i - cout * 9 * cin;
-        n = 9 * cin;
-    }
     float s = 0.0f;
-    for (int b = 0; b < chunks; ++b) s += partial[((size_t)b * cout + co) * nw16 + n];
-    grad[i] = s;
+    if (i < total) {
+        int co, n;
+        if (i < cout * 9 * cin) {
+            co = i / (9 * cin);
+            n = i - co * 9 * cin;
+        } else {
+            co = i - cout * 9 * cin;
+            n = 9 * cin;
+        }
+        for (int b = c; b < chunks; b += 16) s += partial[((size_t)b * cout + co) * nw16 + n];
+    }
+    part[c][o] = s;
+    __syncthreads();
+    if (c == 0 && i < total) {
+        float t = 0.0f;
+        for (int k = 0; k < 16; ++k) t += part[k][o];
+        grad[i] = t;
+    }
 }
 
 // block i sums column i of the per-block head partials (fixed order)
@@ -624,23 +786,23 @@ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 inline int level_pixels(const NnImages& im) { return im.offset[im.count]; }
 
-// forward (MODE 0) or input-gradient (MODE 1) launch of convolution l
-template <int C0, int C1, int COUT, int MODE>
-void conv_launch(const ConvArgs& a, int grid, hipStream_t st) {
-    const int groups = cdiv(a.P, 16 * conv_mt<COUT>());
-    const int blocks = std::max(1, std::min(grid, cdiv(groups, 4)));
-    k_uconv<C0, C1, COUT, MODE><<<blocks, 256, 0, st>>>(a);
+// tiles of kCvW x kCvRows pixels at a level
+int level_tiles(const NnImages& im) {
+    int n = 0;
+    for (int i = 0; i < im.count; ++i) {
+        const int w = im.width[i], h = (im.offset[i + 1] - im.offset[i]) / w;
+        n += cdiv(w, kCvW) * cdiv(h, kCvRows);
+    }
+    return n;
 }
 
-// the convolutions address their inputs with 32-bit buffer offsets
-bool conv_inputs_fit(const NnLevels& lv) {
-    constexpr double kLimit = 2147479552.0;  // kBufOob
-    for (int l = 0; l < kNnConvs; ++l) {
-        const NnConv c = nn_conv(l);
-        const double P = (double)lv.lv[c.level].offset[lv.lv[c.level].count];
-        if (P * (double)std::max(nn_cin(l), c.cout) * 2.0 >= kLimit) return false;
-    }
-    return true;
+// forward (MODE 0) or input-gradient (MODE 1) launch of convolution l: one
+// block per output tile
+template <int C0, int C1, int COUT, int MODE>
+void conv_launch(const ConvArgs& a, int grid, hipStream_t st) {
+    (void)grid;
+    const int tiles = level_tiles(a.im);
+    if (tiles > 0) k_uconv<C0, C1, COUT, MODE><<<tiles, 256, 0, st>>>(a);
 }
 
 void conv_forward(int l, const ConvArgs& a, int grid, hipStream_t st) {
@@ -684,28 +846,57 @@ void conv_input_grad(int l, const ConvArgs& a, int grid, hipStream_t st) {
     }
 }
 
-template <int C0, int C1, int COUT>
-void wgrad_launch(const WgradArgs& a, hipStream_t st) {
-    k_wgrad<C0, C1, COUT><<<cdiv(a.P, kNnGradPixels) * 10, 256, 0, st>>>(a);
+// row segments of kWgSeg pixels at a level
+int level_segments(const NnImages& im) {
+    int n = 0;
+    for (int i = 0; i < im.count; ++i) {
+        const int w = im.width[i], h = (im.offset[i + 1] - im.offset[i]) / w;
+        n += h * cdiv(w, kWgSeg);
+    }
+    return n;
 }
 
-void conv_wgrad(int l, const WgradArgs& a, hipStream_t st) {
+// taps per weight-gradient block: all nine at full size with at most 32
+// input channels (most pixels, fewest channels: the most MFMAs per staged
+// byte), otherwise a row of three at the two finer levels and one below
+// (more blocks, fewer accumulators per wave: at most 16 tiles per block)
+constexpr int wgrad_tpb(int l) {
+    return nn_conv(l).level == 0 ? (nn_cin(l) <= 32 ? 9 : 3) : (nn_conv(l).level == 1 ? 3 : 1);
+}
+
+// segments per weight-gradient block: the partial sums (chunks x weights)
+// are written and read once more, so chunks are bounded: 384 at full size,
+// 192 at half size, 64 below (384 to 576 blocks per convolution)
+int wgrad_chunk_segs(int l, int segs) {
+    const int level = nn_conv(l).level;
+    const int chunks = std::max(1, std::min(segs, level == 0 ? 384 : (level == 1 ? 192 : 64)));
+    return std::max(1, cdiv(segs, chunks));
+}
+
+template <int L>
+void wgrad_launch(const WgradArgs& a, int chunks, hipStream_t st) {
+    constexpr NnConv c = nn_conv(L);
+    constexpr int TPB = wgrad_tpb(L);
+    if (chunks > 0) k_wgrad<c.c0, c.c1, c.cout, TPB><<<chunks * (9 / TPB), 256, 0, st>>>(a);
+}
+
+void conv_wgrad(int l, const WgradArgs& a, int chunks, hipStream_t st) {
     switch (l) {
-        case 0: wgrad_launch<16, 0, 32>(a, st); break;
-        case 1: wgrad_launch<32, 0, 32>(a, st); break;
-        case 2: wgrad_launch<32, 0, 48>(a, st); break;
-        case 3: wgrad_launch<48, 0, 48>(a, st); break;
-        case 4: wgrad_launch<48, 0, 64>(a, st); break;
-        case 5: wgrad_launch<64, 0, 64>(a, st); break;
-        case 6: wgrad_launch<64, 0, 96>(a, st); break;
-        case 7: wgrad_launch<96, 0, 96>(a, st); break;
-        case 8: wgrad_launch<96, 0, 96>(a, st); break;
-        case 9: wgrad_launch<64, 96, 64>(a, st); break;
-        case 10: wgrad_launch<64, 0, 64>(a, st); break;
-        case 11: wgrad_launch<48, 64, 48>(a, st); break;
-        case 12: wgrad_launch<48, 0, 48>(a, st); break;
-        case 13: wgrad_launch<32, 48, 32>(a, st); break;
-        default: wgrad_launch<32, 0, 32>(a, st); break;
+        case 0: wgrad_launch<0>(a, chunks, st); break;
+        case 1: wgrad_launch<1>(a, chunks, st); break;
+        case 2: wgrad_launch<2>(a, chunks, st); break;
+        case 3: wgrad_launch<3>(a, chunks, st); break;
+        case 4: wgrad_launch<4>(a, chunks, st); break;
+        case 5: wgrad_launch<5>(a, chunks, st); break;
+        case 6: wgrad_launch<6>(a, chunks, st); break;
+        case 7: wgrad_launch<7>(a, chunks, st); break;
+        case 8: wgrad_launch<8>(a, chunks, st); break;
+        case 9: wgrad_launch<9>(a, chunks, st); break;
+        case 10: wgrad_launch<10>(a, chunks, st); break;
+        case 11: wgrad_launch<11>(a, chunks, st); break;
+        case 12: wgrad_launch<12>(a, chunks, st); break;
+        case 13: wgrad_launch<13>(a, chunks, st); break;
+        default: wgrad_launch<14>(a, chunks, st); break;
     }
 }
 
@@ -760,7 +951,8 @@ size_t nn_grad_partial_floats(const NnLevels& lv) {
     size_t m = 0;
     for (int l = 0; l < kNnConvs; ++l) {
         const NnConv c = nn_conv(l);
-        const int chunks = cdiv(level_pixels(lv.lv[c.level]), kNnGradPixels);
+        const int segs = level_segments(lv.lv[c.level]);
+        const int chunks = cdiv(segs, wgrad_chunk_segs(l, segs));
         const size_t nw16 = (size_t)((9 * nn_cin(l)) / 16 + 1) * 16;
         m = std::max(m, (size_t)chunks * (size_t)c.cout * nw16);
     }
@@ -784,7 +976,6 @@ hipError_t nn_forward(const NnLevels& lv, const NnPacked& pk, const NnScratch& w
     const float4* target, float4* out, int conv_grid, hipStream_t st) {
     const int P = level_pixels(lv.lv[0]);
     if (P <= 0) return hipSuccess;
-    if (!conv_inputs_fit(lv)) return hipErrorInvalidValue;
     k_features<<<cdiv(P, 256), 256, 0, st>>>(P, beauty, normal, albedo, depth, w.feat);
     for (int l = 0; l < kNnConvs; ++l) {
         ConvArgs a = conv_args(lv, l, w);
@@ -821,13 +1012,15 @@ hipError_t nn_backward(const NnLevels& lv, const NnPacked& pk, const NnScratch& 
         WgradArgs g{};
         g.im = im;
         g.pim = lv.lv[c.level < kNnLevels - 1 ? c.level + 1 : c.level];
-        g.P = Pl;
+        g.segs = level_segments(im);
+        g.chunk_segs = wgrad_chunk_segs(l, g.segs);
         g.dz = w.dz[l];
         conv_inputs(l, w, &g.A, &g.B);
         g.partial = w.grad_partial;
-        conv_wgrad(l, g, st);
-        k_uwgrad_reduce<<<cdiv(c.cout * (9 * cin + 1), 256), 256, 0, st>>>(
-            cdiv(Pl, kNnGradPixels), c.cout, cin, w.grad_partial, grad + nn_layer_offset(l));
+        const int chunks = cdiv(g.segs, g.chunk_segs);
+        conv_wgrad(l, g, chunks, st);
+        k_uwgrad_reduce<<<cdiv(c.cout * (9 * cin + 1), 16), 256, 0, st>>>(
+            chunks, c.cout, cin, w.grad_partial, grad + nn_layer_offset(l));
         if (l == 0) break;
         // raw input gradient r[l] (cin channels at this level)
         ConvArgs a{};
@@ -844,32 +1037,32 @@ hipError_t nn_backward(const NnLevels& lv, const NnPacked& pk, const NnScratch& 
         // route it to the producers of this convolution's input
         switch (l) {
             case 13:  // [e0, up(d1)]: d1's gradient from the upsampled part
-                k_up_combine<<<cdiv(level_pixels(lv.lv[1]) * 48, 256), 256, 0, st>>>(lv.lv[0], lv.lv[1],
+                k_up_combine<<<cdiv(level_pixels(lv.lv[1]) * 48 / 8, 256), 256, 0, st>>>(lv.lv[0], lv.lv[1],
                     level_pixels(lv.lv[1]), 48, w.r[13], 80, 32, w.act[12], w.dz[12]);
                 break;
             case 11:  // [e1, up(d2)]
-                k_up_combine<<<cdiv(level_pixels(lv.lv[2]) * 64, 256), 256, 0, st>>>(lv.lv[1], lv.lv[2],
+                k_up_combine<<<cdiv(level_pixels(lv.lv[2]) * 64 / 8, 256), 256, 0, st>>>(lv.lv[1], lv.lv[2],
                     level_pixels(lv.lv[2]), 64, w.r[11], 112, 48, w.act[10], w.dz[10]);
                 break;
             case 9:  // [e2, up(b)]
-                k_up_combine<<<cdiv(level_pixels(lv.lv[3]) * 96, 256), 256, 0, st>>>(lv.lv[2], lv.lv[3],
+                k_up_combine<<<cdiv(level_pixels(lv.lv[3]) * 96 / 8, 256), 256, 0, st>>>(lv.lv[2], lv.lv[3],
                     level_pixels(lv.lv[3]), 96, w.r[9], 160, 64, w.act[8], w.dz[8]);
                 break;
             case 6:  // pool(e2) -> e2, plus the skip into convolution 9
-                k_skip_combine<<<cdiv(level_pixels(lv.lv[2]) * 64, 256), 256, 0, st>>>(lv.lv[2], lv.lv[3],
+                k_skip_combine<<<cdiv(level_pixels(lv.lv[2]) * 64 / 8, 256), 256, 0, st>>>(lv.lv[2], lv.lv[3],
                     level_pixels(lv.lv[2]), 64, w.r[9], 160, w.r[6], w.act[5], w.dz[5]);
                 break;
             case 4:  // pool(e1) -> e1, plus the skip into convolution 11
-                k_skip_combine<<<cdiv(level_pixels(lv.lv[1]) * 48, 256), 256, 0, st>>>(lv.lv[1], lv.lv[2],
+                k_skip_combine<<<cdiv(level_pixels(lv.lv[1]) * 48 / 8, 256), 256, 0, st>>>(lv.lv[1], lv.lv[2],
                     level_pixels(lv.lv[1]), 48, w.r[11], 112, w.r[4], w.act[3], w.dz[3]);
                 break;
             case 2:  // pool(e0) -> e0, plus the skip into convolution 13
-                k_skip_combine<<<cdiv(level_pixels(lv.lv[0]) * 32, 256), 256, 0, st>>>(lv.lv[0], lv.lv[1],
+                k_skip_combine<<<cdiv(level_pixels(lv.lv[0]) * 32 / 8, 256), 256, 0, st>>>(lv.lv[0], lv.lv[1],
                     level_pixels(lv.lv[0]), 32, w.r[13], 80, w.r[2], w.act[1], w.dz[1]);
                 break;
             default: {  // single input from convolution l - 1 at this level
-                const size_t n = (size_t)Pl * cin;
-                k_mask<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, w.r[l], w.act[l - 1], w.dz[l - 1]);
+                const size_t n8 = (size_t)Pl * cin / 8;
+                k_mask<<<(unsigned)((n8 + 255) / 256), 256, 0, st>>>(n8, w.r[l], w.act[l - 1], w.dz[l - 1]);
                 break;
             }
         }
