@@ -118,19 +118,23 @@ def main():
     res = []
     for name, sizes in (("infer 4x640x480", [(640, 480)] * 4),
                         ("infer 4x1280x720", [(1280, 720)] * 4)):
+        # the convolution tile shapes autotuned for this batch (rt_nn_autotune)
+        tuned = nn.autotune(sizes, reps=5)
         ms = time_infer(nn, torch, sizes, a.reps, a.warmup)
         fl = forward_flops(sizes)
         res.append({"case": name, "ms": ms, "gflop": fl / 1e9, "tflops": fl / ms / 1e9,
-                    "frac_bf16_peak": fl / ms / 1e9 / BF16_PEAK_TFLOPS})
+                    "frac_bf16_peak": fl / ms / 1e9 / BF16_PEAK_TFLOPS, "conv_rows": tuned["conv_rows"]})
     sizes = [(256, 256)] * 8
+    tuned = nn.autotune(sizes, reps=5)
     ms = time_train(nn, torch, sizes, a.reps, a.warmup)
     fl = train_flops(sizes)
     res.append({"case": "train step 8x256^2 (forward, backward, Adam)", "ms": ms,
                 "gflop": fl / 1e9, "tflops": fl / ms / 1e9,
-                "frac_bf16_peak": fl / ms / 1e9 / BF16_PEAK_TFLOPS})
+                "frac_bf16_peak": fl / ms / 1e9 / BF16_PEAK_TFLOPS, "conv_rows": tuned["conv_rows"]})
     for r in res:
         print(f"[nn] {r['case']}: {r['ms']:.3f} ms, {r['gflop']:.1f} GFLOP, "
-              f"{r['tflops']:.1f} TFLOP/s ({100 * r['frac_bf16_peak']:.2f} % of BF16 peak)", flush=True)
+              f"{r['tflops']:.1f} TFLOP/s ({100 * r['frac_bf16_peak']:.2f} % of BF16 peak), "
+              f"tile rows per level {r['conv_rows']}", flush=True)
     if a.json:
         with open(a.json, "w") as f:
             json.dump(res, f, indent=1)

@@ -580,6 +580,26 @@ rt_status rt_nn_infer_host(rt_nn* nn, const rt_nn_images* images, const float* b
 rt_status rt_nn_train_step_host(rt_nn* nn, const rt_nn_images* images, const float* beauty,
     const float* normal, const float* albedo, const float* depth, const float* target, float lr,
     float* loss, float* grad);
+/* Tile shapes of the network's convolution kernels (no counterpart in the
+ * reference): rows of output pixels per block at each of the 4 levels, 4 or
+ * 8 (8 re-reads fewer halo rows, 4 needs less LDS per block); 0 picks 8 when
+ * the level's widest image is at least 160 pixels.  rt_nn_autotune times
+ * `reps` inferences of the image batch per candidate (coordinate descent
+ * over the levels, median after a warm-up) and applies the fastest. */
+typedef struct rt_nn_tiling {
+    int32_t conv_rows[4];
+} rt_nn_tiling;
+typedef struct rt_nn_autotune_result {
+    rt_nn_tiling best;
+    double best_ms;        /* median inference time of the chosen tiling */
+    double baseline_ms;    /* median inference time before the search */
+    int32_t trials;
+    int32_t pad;
+} rt_nn_autotune_result;
+rt_status rt_nn_set_tiling(rt_nn* nn, const rt_nn_tiling* t);
+rt_status rt_nn_get_tiling(rt_nn* nn, rt_nn_tiling* t);
+rt_status rt_nn_autotune(rt_nn* nn, const rt_nn_images* images, int32_t reps,
+    rt_nn_autotune_result* out);
 /* denoise with `nn` instead of the a-trous filter when the profile asks for
  * denoising (NULL detaches); nn must live on the renderer's device.  The
  * renderer holds its own reference, so rt_nn_destroy on the caller's handle
@@ -623,14 +643,17 @@ rt_status rt_renderer_get_tuning(rt_renderer* r, rt_tuning* t);
 /* Autotuner: coordinate descent over traversal block shape, persistent blocks
  * per CU, refill threshold, paths in flight and graph capture, each candidate
  * timed on `frames_per_trial` frames of this rig and profile (median device
- * time after one warm-up frame).  Applies the best tuning and resets the
- * accumulation history. */
+ * time after one warm-up frame); with a learned denoiser attached and
+ * denoising on, then the network's convolution tile shapes for the rig's
+ * camera images (rt_nn_autotune, `denoiser`).  Applies the best tuning and
+ * resets the accumulation history. */
 typedef struct rt_autotune_result {
     rt_tuning best;
     double best_ms;        /* median frame time of the chosen tuning */
     double baseline_ms;    /* median frame time of the tuning before the search */
     int32_t trials;
-    int32_t pad;
+    int32_t denoiser_tuned; /* 1: `denoiser` holds the network's search */
+    rt_nn_autotune_result denoiser;
 } rt_autotune_result;
 rt_status rt_renderer_autotune(rt_renderer* r, const rt_camera_rig* rig, const rt_profile* p,
     int32_t frames_per_trial, rt_autotune_result* out);

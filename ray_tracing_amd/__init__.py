@@ -36,6 +36,7 @@ EXPORTED = [
     "rt_scene_generate_rig", "rt_frame_download_guides", "rt_bvh_download_wide",
     "rt_nn_param_count", "rt_nn_create", "rt_nn_destroy", "rt_nn_set_params", "rt_nn_get_params",
     "rt_nn_infer", "rt_nn_train_step", "rt_nn_infer_host", "rt_nn_train_step_host",
+    "rt_nn_set_tiling", "rt_nn_get_tiling", "rt_nn_autotune",
     "rt_renderer_set_denoiser", "rt_renderer_set_denoiser_budget", "rt_renderer_denoiser_guard",
     "rt_renderer_autotune",
     "rt_cache_default_config", "rt_cache_mlp_param_count", "rt_cache_param_count",
@@ -264,8 +265,13 @@ class Renderer:
         res = abi.AutotuneResult()
         check(lib().rt_renderer_autotune(self.h, C.byref(rig), C.byref(profile), frames_per_trial,
                                          C.byref(res)))
-        return {"baseline_ms": res.baseline_ms, "best_ms": res.best_ms, "trials": res.trials,
-                "tuning": {k: getattr(res.best, k) for k, _ in Tuning._fields_}}
+        out = {"baseline_ms": res.baseline_ms, "best_ms": res.best_ms, "trials": res.trials,
+               "tuning": {k: getattr(res.best, k) for k, _ in Tuning._fields_}}
+        if res.denoiser_tuned:
+            d = res.denoiser
+            out["denoiser"] = {"conv_rows": list(d.best.conv_rows), "best_ms": d.best_ms,
+                               "baseline_ms": d.baseline_ms, "trials": d.trials}
+        return out
 
     def set_denoiser(self, nn: "Denoiser | None"):
         self._nn = nn  # keeps the network alive while attached
@@ -401,6 +407,25 @@ class Denoiser:
             self.close()
         except Exception:
             pass
+
+    def set_tiling(self, conv_rows):
+        t = abi.NnTiling()
+        t.conv_rows[:] = list(conv_rows)
+        check(lib().rt_nn_set_tiling(self.h, C.byref(t)))
+
+    def tiling(self) -> list:
+        t = abi.NnTiling()
+        check(lib().rt_nn_get_tiling(self.h, C.byref(t)))
+        return list(t.conv_rows)
+
+    def autotune(self, sizes, reps: int = 5) -> dict:
+        """Searches the convolution tile rows per level for images of these
+        [(width, height), ...] and applies the fastest."""
+        im = abi.nn_images(sizes)
+        res = abi.NnAutotuneResult()
+        check(lib().rt_nn_autotune(self.h, C.byref(im), reps, C.byref(res)))
+        return {"conv_rows": list(res.best.conv_rows), "best_ms": res.best_ms,
+                "baseline_ms": res.baseline_ms, "trials": res.trials}
 
     def set_params(self, params: np.ndarray):
         p = np.ascontiguousarray(params, np.float32)

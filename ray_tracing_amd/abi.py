@@ -161,9 +161,19 @@ class RestirStats(C.Structure):
                 ("max_age", C.c_int32), ("pad", C.c_int32)]
 
 
+class NnTiling(C.Structure):
+    _fields_ = [("conv_rows", C.c_int32 * 4)]
+
+
+class NnAutotuneResult(C.Structure):
+    _fields_ = [("best", NnTiling), ("best_ms", C.c_double), ("baseline_ms", C.c_double),
+                ("trials", C.c_int32), ("pad", C.c_int32)]
+
+
 class AutotuneResult(C.Structure):
     _fields_ = [("best", Tuning), ("best_ms", C.c_double), ("baseline_ms", C.c_double),
-                ("trials", C.c_int32), ("pad", C.c_int32)]
+                ("trials", C.c_int32), ("denoiser_tuned", C.c_int32),
+                ("denoiser", NnAutotuneResult)]
 
 
 class NnImages(C.Structure):
@@ -297,6 +307,9 @@ def declare(L):
     L.rt_nn_train_step.argtypes = [vp, P(NnImages), vp, vp, vp, vp, vp, C.c_float, vp, vp]
     L.rt_nn_infer_host.argtypes = [vp, P(NnImages), vp, vp, vp, vp, vp]
     L.rt_nn_train_step_host.argtypes = [vp, P(NnImages), vp, vp, vp, vp, vp, C.c_float, vp, vp]
+    L.rt_nn_set_tiling.argtypes = [vp, P(NnTiling)]
+    L.rt_nn_get_tiling.argtypes = [vp, P(NnTiling)]
+    L.rt_nn_autotune.argtypes = [vp, P(NnImages), C.c_int32, P(NnAutotuneResult)]
     L.rt_renderer_set_denoiser.argtypes = [vp, vp]
     L.rt_renderer_set_denoiser_budget.argtypes = [vp, C.c_float, C.c_float]
     L.rt_renderer_denoiser_guard.argtypes = [vp, P(DenoiserGuard)]

@@ -1535,6 +1535,20 @@ rt_status rt_renderer_autotune(rt_renderer* r, const rt_camera_rig* rig, const r
     out->best = best;
     out->best_ms = best_ms;
     out->trials = trials;
+    if (r->nn != nullptr && p->enable_denoise != 0) {
+        // the network's tile shapes for this rig's camera images, back to back
+        rt_nn_images im{};
+        im.count = rig->active_count;
+        int off = 0;
+        for (int c = 0; c < rig->active_count; ++c) {
+            im.offset[c] = off;
+            im.width[c] = rig->cameras[c].width;
+            off += rig->cameras[c].width * rig->cameras[c].height;
+        }
+        im.offset[rig->active_count] = off;
+        if ((st = rt_nn_autotune(r->nn, &im, frames_per_trial, &out->denoiser)) != RT_OK) return st;
+        out->denoiser_tuned = 1;
+    }
     return rt_reset_accumulation(r);
 }
 

@@ -92,16 +92,25 @@ struct NnScratch {
     float* grad_partial;             // weight-gradient partial sums
 };
 
+// rows per convolution tile at each level: 4 or 8 (more rows re-read fewer
+// halo rows, fewer rows need less LDS per block); 0 chooses 8 when the
+// level's widest image is at least 160 pixels, else 4
+struct NnTiling {
+    int32_t rows[kNnLevels];
+};
+
 NnLevels nn_levels(const NnImages& im);
+// the tile rows convolutions at `level` (images `im`) run with
+int nn_tile_rows(const NnTiling& t, const NnImages& im, int level);
 hipError_t nn_pack(const float* params, const NnPacked& pk, hipStream_t st);
 // features, the 15 convolutions and the head; with `target` also the loss
 // terms and the head-level gradients needed by nn_backward
 hipError_t nn_forward(const NnLevels& lv, const NnPacked& pk, const NnScratch& w,
     const float4* beauty, const float4* normal, const float4* albedo, const float* depth,
-    const float4* target, float4* out, int conv_grid, hipStream_t st);
+    const float4* target, float4* out, const NnTiling& tiling, hipStream_t st);
 // parameter gradient (FP32, parameter layout) and loss of the last forward
 hipError_t nn_backward(const NnLevels& lv, const NnPacked& pk, const NnScratch& w, float* grad,
-    float* loss, int conv_grid, hipStream_t st);
+    float* loss, const NnTiling& tiling, hipStream_t st);
 hipError_t nn_adam(int n, float* params, const float* grad, float* m, float* v, float lr, int step,
     hipStream_t st);
 // floats of weight-gradient partial sums needed for a batch

@@ -87,32 +87,31 @@ struct ConvArgs {
     uint16_t* out;        // [P][COUT]
 };
 
-// Output tiles of the convolution kernel: kCvW pixels of kCvRows rows of one
-// image, one row per wave (four 16-pixel MFMA tiles).  The input rows
-// y0 - 1 .. y0 + kCvRows, pixels x0 - 1 .. x0 + kCvW, are staged in LDS; a
-// decoder's coarser input is staged at its own resolution (the parents of
-// those pixels: kCvCoarseRows x kCvCoarseRun).
+// Output tiles of the convolution kernel: kCvW pixels of ROWS rows of one
+// image, one row per wave (four 16-pixel MFMA tiles); 8 rows at full size
+// (fewer re-read halo rows), 4 below (less LDS per block: more blocks per
+// CU for the wider layers).  The input rows y0 - 1 .. y0 + ROWS, pixels
+// x0 - 1 .. x0 + kCvW, are staged in LDS; a decoder's coarser input is
+// staged at its own resolution (the parents of those pixels: ROWS / 2 + 2
+// rows of kCvCoarseRun).
 constexpr int kCvW = 64;
-constexpr int kCvRows = 4;
 constexpr int kCvRun = kCvW + 2;
-constexpr int kCvInRows = kCvRows + 2;
 constexpr int kCvCoarseRun = kCvW / 2 + 2;
-constexpr int kCvCoarseRows = kCvRows / 2 + 2;
 
 // LDS row stride (bf16) of a staged pixel of c channels: at least c, and an
 // odd number of 16-byte units, so that the 8 pixels one quarter-wave of
 // 16-byte reads touches fall on distinct bank groups
 __host__ __device__ constexpr int cv_row(int c) { return (c / 8) % 2 == 1 ? c : c + 8; }
 
-// tiles of kCvW x kCvRows pixels of a level: per image, rows of tiles
-__device__ __forceinline__ void tile_locate(const NnImages& im, int b, int* img, int* x0, int* y0) {
+// tiles of kCvW x rows pixels of a level: per image, rows of tiles
+__device__ __forceinline__ void tile_locate(const NnImages& im, int rows, int b, int* img, int* x0, int* y0) {
     int i = 0;
     for (;;) {
         const int w = im.width[i], h = (im.offset[i + 1] - im.offset[i]) / w;
-        const int tx = (w + kCvW - 1) / kCvW, n = tx * ((h + kCvRows - 1) / kCvRows);
+        const int tx = (w + kCvW - 1) / kCvW, n = tx * ((h + rows - 1) / rows);
         if (b < n || i + 1 >= im.count) {
             *img = i;
-            *y0 = (b / tx) * kCvRows;
+            *y0 = (b / tx) * rows;
             *x0 = (b - (b / tx) * tx) * kCvW;
             return;
         }
@@ -121,7 +120,7 @@ __device__ __forceinline__ void tile_locate(const NnImages& im, int b, int* img,
     }
 }
 
-// One block computes out[tile][COUT] for a kCvW x kCvRows tile as the
+// One block computes out[tile][COUT] for a kCvW x ROWS tile as the
 // transposed GEMM out^T = W X^T: the MFMA A operand is the weights (row =
 // output channel, 8 consecutive K per lane, one 16-byte load from the
 // L1-resident weight matrix, fetched one K step ahead), the B operand the
@@ -132,8 +131,11 @@ __device__ __forceinline__ void tile_locate(const NnImages& im, int b, int* img,
 // at the parent of the tap's pixel and zeroed where that pixel is outside
 // the image.  A lane ends with 4 consecutive output channels of one pixel and
 // stores them as 8 bytes.  MODE 0: out = relu(acc + bias); MODE 1: out = acc.
-template <int C0, int C1, int COUT, int MODE>
-__global__ __launch_bounds__(256) void k_uconv(ConvArgs a) {
+template <int C0, int C1, int COUT, int MODE, int ROWS>
+__global__ __launch_bounds__(64 * ROWS) void k_uconv(ConvArgs a) {
+    constexpr int kCvThreads = 64 * ROWS;
+    constexpr int kCvInRows = ROWS + 2;
+    constexpr int kCvCoarseRows = ROWS / 2 + 2;
     constexpr int CIN = C0 + C1;
     constexpr int TS = nn_tap_stride(CIN);
     constexpr int KT = TS / 32;
@@ -150,10 +152,10 @@ __global__ __launch_bounds__(256) void k_uconv(ConvArgs a) {
     const int lane = tid & 63, wave = tid >> 6;
     const int r16 = lane & 15, kq = lane >> 4;
     int img, x0, y0;
-    tile_locate(a.im, blockIdx.x, &img, &x0, &y0);
+    tile_locate(a.im, ROWS, blockIdx.x, &img, &x0, &y0);
     const int w = a.im.width[img], h = (a.im.offset[img + 1] - a.im.offset[img]) / w;
     const int base = a.im.offset[img];
-    for (int u = tid; u < kCvInRows * kCvRun * (C0 / 8); u += 256) {
+    for (int u = tid; u < kCvInRows * kCvRun * (C0 / 8); u += kCvThreads) {
         const int rest = u / (C0 / 8), c = (u - rest * (C0 / 8)) * 8;
         const int r = rest / kCvRun, q = rest - r * kCvRun;
         const int x = x0 - 1 + q, y = y0 - 1 + r;
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(256) void k_uconv(ConvArgs a) {
     if (C1 > 0) {
         const int pw = a.pim.width[img];
         const int ph = (a.pim.offset[img + 1] - a.pim.offset[img]) / pw;
-        for (int u = tid; u < kCvCoarseRows * kCvCoarseRun * (C1 / 8); u += 256) {
+        for (int u = tid; u < kCvCoarseRows * kCvCoarseRun * (C1 / 8); u += kCvThreads) {
             const int rest = u / (C1 / 8), c = (u - rest * (C1 / 8)) * 8;
             const int r = rest / kCvCoarseRun, q = rest - r * kCvCoarseRun;
             const int x = px0 + q, y = py0 + r;
@@ -786,63 +788,76 @@ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 inline int level_pixels(const NnImages& im) { return im.offset[im.count]; }
 
-// tiles of kCvW x kCvRows pixels at a level
-int level_tiles(const NnImages& im) {
+// tiles of kCvW x rows pixels at a level
+int level_tiles(const NnImages& im, int rows) {
     int n = 0;
     for (int i = 0; i < im.count; ++i) {
         const int w = im.width[i], h = (im.offset[i + 1] - im.offset[i]) / w;
-        n += cdiv(w, kCvW) * cdiv(h, kCvRows);
+        n += cdiv(w, kCvW) * cdiv(h, rows);
     }
     return n;
 }
 
-// forward (MODE 0) or input-gradient (MODE 1) launch of convolution l: one
-// block per output tile
+// forward (MODE 0) or input-gradient (MODE 1) launch of a convolution: one
+// block per output tile of `rows` rows (4 or 8)
 template <int C0, int C1, int COUT, int MODE>
-void conv_launch(const ConvArgs& a, int grid, hipStream_t st) {
-    (void)grid;
-    const int tiles = level_tiles(a.im);
-    if (tiles > 0) k_uconv<C0, C1, COUT, MODE><<<tiles, 256, 0, st>>>(a);
+void conv_launch(const ConvArgs& a, int rows, hipStream_t st) {
+    const int tiles = level_tiles(a.im, rows);
+    if (tiles <= 0) return;
+    if (rows == 8) k_uconv<C0, C1, COUT, MODE, 8><<<tiles, 512, 0, st>>>(a);
+    else k_uconv<C0, C1, COUT, MODE, 4><<<tiles, 256, 0, st>>>(a);
 }
 
-void conv_forward(int l, const ConvArgs& a, int grid, hipStream_t st) {
+}  // namespace
+
+// the tiling's rows at a level (0: 8 when the widest image is >= 160 pixels)
+int nn_tile_rows(const NnTiling& t, const NnImages& im, int level) {
+    if (t.rows[level] == 4 || t.rows[level] == 8) return t.rows[level];
+    int wmax = 0;
+    for (int i = 0; i < im.count; ++i) wmax = std::max(wmax, im.width[i]);
+    return wmax >= 160 ? 8 : 4;
+}
+
+namespace {
+
+void conv_forward(int l, const ConvArgs& a, int rows, hipStream_t st) {
     switch (l) {
-        case 0: conv_launch<16, 0, 32, 0>(a, grid, st); break;
-        case 1: conv_launch<32, 0, 32, 0>(a, grid, st); break;
-        case 2: conv_launch<32, 0, 48, 0>(a, grid, st); break;
-        case 3: conv_launch<48, 0, 48, 0>(a, grid, st); break;
-        case 4: conv_launch<48, 0, 64, 0>(a, grid, st); break;
-        case 5: conv_launch<64, 0, 64, 0>(a, grid, st); break;
-        case 6: conv_launch<64, 0, 96, 0>(a, grid, st); break;
-        case 7: conv_launch<96, 0, 96, 0>(a, grid, st); break;
-        case 8: conv_launch<96, 0, 96, 0>(a, grid, st); break;
-        case 9: conv_launch<64, 96, 64, 0>(a, grid, st); break;
-        case 10: conv_launch<64, 0, 64, 0>(a, grid, st); break;
-        case 11: conv_launch<48, 64, 48, 0>(a, grid, st); break;
-        case 12: conv_launch<48, 0, 48, 0>(a, grid, st); break;
-        case 13: conv_launch<32, 48, 32, 0>(a, grid, st); break;
-        default: conv_launch<32, 0, 32, 0>(a, grid, st); break;
+        case 0: conv_launch<16, 0, 32, 0>(a, rows, st); break;
+        case 1: conv_launch<32, 0, 32, 0>(a, rows, st); break;
+        case 2: conv_launch<32, 0, 48, 0>(a, rows, st); break;
+        case 3: conv_launch<48, 0, 48, 0>(a, rows, st); break;
+        case 4: conv_launch<48, 0, 64, 0>(a, rows, st); break;
+        case 5: conv_launch<64, 0, 64, 0>(a, rows, st); break;
+        case 6: conv_launch<64, 0, 96, 0>(a, rows, st); break;
+        case 7: conv_launch<96, 0, 96, 0>(a, rows, st); break;
+        case 8: conv_launch<96, 0, 96, 0>(a, rows, st); break;
+        case 9: conv_launch<64, 96, 64, 0>(a, rows, st); break;
+        case 10: conv_launch<64, 0, 64, 0>(a, rows, st); break;
+        case 11: conv_launch<48, 64, 48, 0>(a, rows, st); break;
+        case 12: conv_launch<48, 0, 48, 0>(a, rows, st); break;
+        case 13: conv_launch<32, 48, 32, 0>(a, rows, st); break;
+        default: conv_launch<32, 0, 32, 0>(a, rows, st); break;
     }
 }
 
 // input gradient of convolution l: the output gradient (cout channels) through
 // the transposed weights into cin raw channels
-void conv_input_grad(int l, const ConvArgs& a, int grid, hipStream_t st) {
+void conv_input_grad(int l, const ConvArgs& a, int rows, hipStream_t st) {
     switch (l) {
-        case 1: conv_launch<32, 0, 32, 1>(a, grid, st); break;
-        case 2: conv_launch<48, 0, 32, 1>(a, grid, st); break;
-        case 3: conv_launch<48, 0, 48, 1>(a, grid, st); break;
-        case 4: conv_launch<64, 0, 48, 1>(a, grid, st); break;
-        case 5: conv_launch<64, 0, 64, 1>(a, grid, st); break;
-        case 6: conv_launch<96, 0, 64, 1>(a, grid, st); break;
-        case 7: conv_launch<96, 0, 96, 1>(a, grid, st); break;
-        case 8: conv_launch<96, 0, 96, 1>(a, grid, st); break;
-        case 9: conv_launch<64, 0, 160, 1>(a, grid, st); break;
-        case 10: conv_launch<64, 0, 64, 1>(a, grid, st); break;
-        case 11: conv_launch<48, 0, 112, 1>(a, grid, st); break;
-        case 12: conv_launch<48, 0, 48, 1>(a, grid, st); break;
-        case 13: conv_launch<32, 0, 80, 1>(a, grid, st); break;
-        default: conv_launch<32, 0, 32, 1>(a, grid, st); break;
+        case 1: conv_launch<32, 0, 32, 1>(a, rows, st); break;
+        case 2: conv_launch<48, 0, 32, 1>(a, rows, st); break;
+        case 3: conv_launch<48, 0, 48, 1>(a, rows, st); break;
+        case 4: conv_launch<64, 0, 48, 1>(a, rows, st); break;
+        case 5: conv_launch<64, 0, 64, 1>(a, rows, st); break;
+        case 6: conv_launch<96, 0, 64, 1>(a, rows, st); break;
+        case 7: conv_launch<96, 0, 96, 1>(a, rows, st); break;
+        case 8: conv_launch<96, 0, 96, 1>(a, rows, st); break;
+        case 9: conv_launch<64, 0, 160, 1>(a, rows, st); break;
+        case 10: conv_launch<64, 0, 64, 1>(a, rows, st); break;
+        case 11: conv_launch<48, 0, 112, 1>(a, rows, st); break;
+        case 12: conv_launch<48, 0, 48, 1>(a, rows, st); break;
+        case 13: conv_launch<32, 0, 80, 1>(a, rows, st); break;
+        default: conv_launch<32, 0, 32, 1>(a, rows, st); break;
     }
 }
 
@@ -973,7 +988,7 @@ hipError_t nn_pack(const float* params, const NnPacked& pk, hipStream_t st) {
 
 hipError_t nn_forward(const NnLevels& lv, const NnPacked& pk, const NnScratch& w,
     const float4* beauty, const float4* normal, const float4* albedo, const float* depth,
-    const float4* target, float4* out, int conv_grid, hipStream_t st) {
+    const float4* target, float4* out, const NnTiling& tiling, hipStream_t st) {
     const int P = level_pixels(lv.lv[0]);
     if (P <= 0) return hipSuccess;
     k_features<<<cdiv(P, 256), 256, 0, st>>>(P, beauty, normal, albedo, depth, w.feat);
@@ -983,7 +998,7 @@ hipError_t nn_forward(const NnLevels& lv, const NnPacked& pk, const NnScratch& w
         a.kp = nn_kp_fwd(l);
         a.bias = pk.bias[l];
         a.out = w.act[l];
-        conv_forward(l, a, conv_grid, st);
+        conv_forward(l, a, nn_tile_rows(tiling, a.im, nn_conv(l).level), st);
         if (l == 1 || l == 3 || l == 5) {
             // pooled e0 / e1 / e2 feed the next level
             const int lvl = nn_conv(l).level;
@@ -999,7 +1014,7 @@ hipError_t nn_forward(const NnLevels& lv, const NnPacked& pk, const NnScratch& w
 }
 
 hipError_t nn_backward(const NnLevels& lv, const NnPacked& pk, const NnScratch& w, float* grad,
-    float* loss, int conv_grid, hipStream_t st) {
+    float* loss, const NnTiling& tiling, hipStream_t st) {
     const int P = level_pixels(lv.lv[0]);
     if (P <= 0) return hipSuccess;
     k_head_reduce<<<kNnHeadParams + 1, 256, 0, st>>>(cdiv(P, 256), w.head_partial, grad + nn_head_offset(), loss);
@@ -1033,7 +1048,7 @@ hipError_t nn_backward(const NnLevels& lv, const NnPacked& pk, const NnScratch& 
         a.W = pk.bwd[l];
         a.kp = nn_kp_bwd(l);
         a.out = w.r[l];
-        conv_input_grad(l, a, conv_grid, st);
+        conv_input_grad(l, a, nn_tile_rows(tiling, im, c.level), st);
         // route it to the producers of this convolution's input
         switch (l) {
             case 13:  // [e0, up(d1)]: d1's gradient from the upsampled part

@@ -6,6 +6,7 @@
 #include <atomic>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "rt_alloc.h"
 #include "rt_nn.h"
@@ -136,7 +137,7 @@ struct rt_nn {
         return w;
     }
 
-    int conv_grid() const { return std::max(cu_count, 1) * 16; }
+    rtk::NnTiling tiling{};   // convolution tile rows per level (0: by image width)
 };
 
 namespace {
@@ -173,7 +174,7 @@ hipError_t rt_nn_infer_on_stream(rt_nn* nn, const rtk::NnImages& images, int P,
     const hipError_t e = nn->ensure(lv, false);
     if (e != hipSuccess) return e;
     return rtk::nn_forward(lv, nn->packed(), nn->scratch(), beauty, normal, albedo, depth, nullptr,
-        out, nn->conv_grid(), st);
+        out, nn->tiling, st);
 }
 
 int rt_nn_device(const rt_nn* nn) { return nn->ordinal; }
@@ -282,7 +283,7 @@ rt_status rt_nn_infer(rt_nn* nn, const rt_nn_images* images, const float* beauty
     NN_HIP(nn->ensure(lv, false));
     NN_HIP(rtk::nn_forward(lv, nn->packed(), nn->scratch(), reinterpret_cast<const float4*>(beauty),
         reinterpret_cast<const float4*>(normal), reinterpret_cast<const float4*>(albedo), depth,
-        nullptr, reinterpret_cast<float4*>(out), nn->conv_grid(), nn->stream));
+        nullptr, reinterpret_cast<float4*>(out), nn->tiling, nn->stream));
     NN_HIP(hipStreamSynchronize(nn->stream));
     return RT_OK;
 }
@@ -304,8 +305,8 @@ rt_status rt_nn_train_step(rt_nn* nn, const rt_nn_images* images, const float* b
     const rtk::NnScratch w = nn->scratch();
     NN_HIP(rtk::nn_forward(lv, pk, w, reinterpret_cast<const float4*>(beauty),
         reinterpret_cast<const float4*>(normal), reinterpret_cast<const float4*>(albedo), depth,
-        reinterpret_cast<const float4*>(target), nullptr, nn->conv_grid(), nn->stream));
-    NN_HIP(rtk::nn_backward(lv, pk, w, nn->grad.p, nn->loss.p, nn->conv_grid(), nn->stream));
+        reinterpret_cast<const float4*>(target), nullptr, nn->tiling, nn->stream));
+    NN_HIP(rtk::nn_backward(lv, pk, w, nn->grad.p, nn->loss.p, nn->tiling, nn->stream));
     const int np = rtk::nn_param_count();
     if (grad) NN_HIP(hipMemcpyAsync(grad, nn->grad.p, sizeof(float) * (size_t)np, hipMemcpyDeviceToHost, nn->stream));
     if (loss) NN_HIP(hipMemcpyAsync(loss, nn->loss.p, sizeof(float), hipMemcpyDeviceToHost, nn->stream));
@@ -315,6 +316,101 @@ rt_status rt_nn_train_step(rt_nn* nn, const rt_nn_images* images, const float* b
         NN_HIP(rtk::nn_pack(nn->params.p, pk, nn->stream));
     }
     NN_HIP(hipStreamSynchronize(nn->stream));
+    return RT_OK;
+}
+
+rt_status rt_nn_set_tiling(rt_nn* nn, const rt_nn_tiling* t) {
+    NN_REQUIRE(nn && t, RT_ERROR_INVALID_ARGUMENT, "null network or tiling");
+    for (int l = 0; l < rtk::kNnLevels; ++l)
+        NN_REQUIRE(t->conv_rows[l] == 0 || t->conv_rows[l] == 4 || t->conv_rows[l] == 8,
+            RT_ERROR_INVALID_ARGUMENT, "convolution tile rows must be 0, 4 or 8");
+    for (int l = 0; l < rtk::kNnLevels; ++l) nn->tiling.rows[l] = t->conv_rows[l];
+    rt_alloc_bump();   // frame graphs captured with the old launch shapes are recaptured
+    return RT_OK;
+}
+
+rt_status rt_nn_get_tiling(rt_nn* nn, rt_nn_tiling* t) {
+    NN_REQUIRE(nn && t, RT_ERROR_INVALID_ARGUMENT, "null network or tiling");
+    for (int l = 0; l < rtk::kNnLevels; ++l) t->conv_rows[l] = nn->tiling.rows[l];
+    return RT_OK;
+}
+
+// Coordinate descent over the convolution tile rows of each level (4, 8),
+// each candidate timed by `reps` inferences of this image batch on device
+// buffers (median, after one warm-up); applies the fastest tiling.
+rt_status rt_nn_autotune(rt_nn* nn, const rt_nn_images* images, int32_t reps,
+    rt_nn_autotune_result* out) {
+    NN_REQUIRE(nn && out, RT_ERROR_INVALID_ARGUMENT, "null network or result");
+    NN_REQUIRE(reps >= 1 && reps <= 64, RT_ERROR_INVALID_ARGUMENT, "reps must be in [1, 64]");
+    int P = 0;
+    const rt_status s = check_images(images, &P);
+    if (s != RT_OK) return s;
+    std::memset(out, 0, sizeof(*out));
+    NN_HIP(hipSetDevice(nn->ordinal));
+    const rtk::NnLevels lv = rtk::nn_levels(to_images(images));
+    NN_HIP(nn->ensure(lv, false));
+    Buf<float4> b, n, a, o;
+    Buf<float> d;
+    NN_HIP(b.reserve((size_t)P));
+    NN_HIP(n.reserve((size_t)P));
+    NN_HIP(a.reserve((size_t)P));
+    NN_HIP(o.reserve((size_t)P));
+    NN_HIP(d.reserve((size_t)P));
+    NN_HIP(hipMemsetAsync(b.p, 0, (size_t)P * 16, nn->stream));
+    NN_HIP(hipMemsetAsync(n.p, 0, (size_t)P * 16, nn->stream));
+    NN_HIP(hipMemsetAsync(a.p, 0, (size_t)P * 16, nn->stream));
+    NN_HIP(hipMemsetAsync(d.p, 0, (size_t)P * 4, nn->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    NN_HIP(hipEventCreate(&e0));
+    hipError_t herr = hipEventCreate(&e1);
+    if (herr != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        NN_HIP(herr);
+    }
+    int trials = 0;
+    auto measure = [&](const rtk::NnTiling& t, double* ms) -> hipError_t {
+        std::vector<double> v;
+        for (int i = 0; i <= reps; ++i) {
+            hipError_t e = hipEventRecord(e0, nn->stream);
+            if (e == hipSuccess)
+                e = rtk::nn_forward(lv, nn->packed(), nn->scratch(), b.p, n.p, a.p, d.p, nullptr, o.p, t,
+                    nn->stream);
+            if (e == hipSuccess) e = hipEventRecord(e1, nn->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float f = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+            if (e != hipSuccess) return e;
+            if (i > 0) v.push_back(f);
+        }
+        std::sort(v.begin(), v.end());
+        *ms = v[v.size() / 2];
+        ++trials;
+        return hipSuccess;
+    };
+    // resolve the default rule into explicit rows, then try the other value per level
+    rtk::NnTiling best{};
+    for (int l = 0; l < rtk::kNnLevels; ++l) best.rows[l] = rtk::nn_tile_rows(nn->tiling, lv.lv[l], l);
+    double best_ms = 0.0;
+    herr = measure(best, &best_ms);
+    out->baseline_ms = best_ms;
+    for (int l = 0; l < rtk::kNnLevels && herr == hipSuccess; ++l) {
+        rtk::NnTiling t = best;
+        t.rows[l] = best.rows[l] == 8 ? 4 : 8;
+        double ms = 0.0;
+        herr = measure(t, &ms);
+        if (herr == hipSuccess && ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    NN_HIP(herr);
+    nn->tiling = best;
+    rt_alloc_bump();
+    for (int l = 0; l < rtk::kNnLevels; ++l) out->best.conv_rows[l] = best.rows[l];
+    out->best_ms = best_ms;
+    out->trials = trials;
     return RT_OK;
 }
 

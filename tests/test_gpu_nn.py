@@ -77,3 +77,32 @@ def test_training_lowers_loss(rt, oracle, device):
         last, _ = nn.train_step(SIZES, frame, target, 1e-3)
     say(f"  loss {first:.5f} -> {last:.5f}")
     assert last < 0.5 * first
+
+
+def test_tile_shapes_give_identical_outputs_and_autotune(rt, oracle, device):
+    """The convolution tile rows change the launch shape, not the arithmetic:
+    every output accumulates its taps and channel chunks in the same order,
+    so tilings of 4 and 8 rows agree bit for bit (forward and gradient), and
+    the autotuner applies one of them."""
+    say("[gpu] nn tile shapes 4 / 8 rows, autotune")
+    rng = np.random.default_rng(12)
+    sizes = [(200, 70), (90, 41)]            # widths on both sides of the 160-pixel rule
+    im, frame, target = random_batch(rng, sizes)
+    params = scaled_params(rng)
+    nn = rt.Denoiser(0, params)
+    outs, grads = [], []
+    for rows in ([4, 4, 4, 4], [8, 8, 8, 8], [8, 4, 8, 4], [0, 0, 0, 0]):
+        nn.set_tiling(rows)
+        assert nn.tiling() == rows
+        outs.append(nn.infer(sizes, frame))
+        loss, g = nn.train_step(sizes, frame, target, lr=0.0, want_grad=True)
+        grads.append(g)
+    for o, g in zip(outs[1:], grads[1:]):
+        assert np.array_equal(o, outs[0])
+        assert np.array_equal(g, grads[0])
+    res = nn.autotune(sizes, reps=2)
+    say(f"  autotune {res}")
+    assert all(r in (4, 8) for r in res["conv_rows"]) and res["trials"] == 5
+    assert nn.tiling() == res["conv_rows"]
+    assert res["best_ms"] <= res["baseline_ms"]
+    assert np.array_equal(nn.infer(sizes, frame), outs[0])
