@@ -55,11 +55,12 @@ __host__ __device__ constexpr int nn_layer_offset(int l) {
 }
 __host__ __device__ constexpr int nn_head_offset() { return nn_layer_offset(kNnConvs); }
 __host__ __device__ constexpr int nn_param_count() { return nn_head_offset() + kNnHeadParams; }
-// K of a packed convolution GEMM: each of the 9 taps holds its c input
-// channels in a run padded to a multiple of 32 (one tap per MFMA K step)
-__host__ __device__ constexpr int nn_tap_stride(int c) { return (c + 31) / 32 * 32; }
-__host__ __device__ constexpr int nn_kp_fwd(int l) { return 9 * nn_tap_stride(nn_cin(l)); }
-__host__ __device__ constexpr int nn_kp_bwd(int l) { return 9 * nn_tap_stride(nn_conv(l).cout); }
+// K of a packed convolution GEMM over c input channels: k = tap * c + channel,
+// the 9 c values padded once to a multiple of 32 (one MFMA K step); an
+// 8-wide K group never straddles two taps since c is a multiple of 8
+__host__ __device__ constexpr int nn_kp(int c) { return (9 * c + 31) / 32 * 32; }
+__host__ __device__ constexpr int nn_kp_fwd(int l) { return nn_kp(nn_cin(l)); }
+__host__ __device__ constexpr int nn_kp_bwd(int l) { return nn_kp(nn_conv(l).cout); }
 
 // images of one batch, back to back
 struct NnImages {
@@ -75,8 +76,8 @@ struct NnLevels {
 
 // device weights in the kernels' layouts
 struct NnPacked {
-    uint16_t* fwd[kNnConvs];   // bf16 [cout][kp_fwd], k = tap * stride(cin) + ci
-    uint16_t* bwd[kNnConvs];   // bf16 [cin][kp_bwd], k = tap' * stride(cout) + co, W[co][8 - tap'][ci]
+    uint16_t* fwd[kNnConvs];   // bf16 [cout][kp_fwd], k = tap * cin + ci
+    uint16_t* bwd[kNnConvs];   // bf16 [cin][kp_bwd], k = tap' * cout + co, W[co][8 - tap'][ci]
     float* bias[kNnConvs];
     float* head;               // FP32 W[3][32], b[3]
 };

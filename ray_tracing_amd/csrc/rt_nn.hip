@@ -125,11 +125,12 @@ __device__ __forceinline__ void tile_locate(const NnImages& im, int rows, int b,
 // output channel, 8 consecutive K per lane, one 16-byte load from the
 // L1-resident weight matrix, fetched one K step ahead), the B operand the
 // input (column = pixel, 8 channels of one tap per lane), read from the
-// staged tile with one 16-byte LDS load.  K runs tap by tap, each tap's
-// channels padded to a multiple of 32 (padding lanes read zero).  The fine
-// tile holds zeros outside the image; a decoder's coarse channels are read
-// at the parent of the tap's pixel and zeroed where that pixel is outside
-// the image.  A lane ends with 4 consecutive output channels of one pixel and
+// staged tile with one 16-byte LDS load at the lane's own tap.  K runs over
+// (tap, channel) pairs without per-tap padding (k = tap * CIN + channel,
+// padded once to a multiple of 32; padding lanes read zero).  The fine tile
+// holds zeros outside the image; a decoder's coarse channels are read at
+// the parent of the tap's pixel and zeroed where that pixel is outside the
+// image.  A lane ends with 4 consecutive output channels of one pixel and
 // stores them as 8 bytes.  MODE 0: out = relu(acc + bias); MODE 1: out = acc.
 template <int C0, int C1, int COUT, int MODE, int ROWS>
 __global__ __launch_bounds__(64 * ROWS) void k_uconv(ConvArgs a) {
@@ -137,14 +138,12 @@ __global__ __launch_bounds__(64 * ROWS) void k_uconv(ConvArgs a) {
     constexpr int kCvInRows = ROWS + 2;
     constexpr int kCvCoarseRows = ROWS / 2 + 2;
     constexpr int CIN = C0 + C1;
-    constexpr int TS = nn_tap_stride(CIN);
-    constexpr int KT = TS / 32;
-    constexpr int KP = 9 * TS;
+    constexpr int KP = nn_kp(CIN);
+    constexpr int STEPS = KP / 32;
     constexpr int NT = COUT / 16;
     constexpr int MT = kCvW / 16;
     constexpr int RA = cv_row(C0);
     constexpr int RB = C1 > 0 ? cv_row(C1) : 8;
-    constexpr int STEPS = 9 * KT;
     static_assert(CIN % 8 == 0 && C0 % 8 == 0 && COUT % 16 == 0, "channel layout");
     __shared__ __align__(16) uint16_t sa[kCvInRows * kCvRun * RA];
     __shared__ __align__(16) uint16_t sb[C1 > 0 ? kCvCoarseRows * kCvCoarseRun * RB : 8];
@@ -190,48 +189,47 @@ __global__ __launch_bounds__(64 * ROWS) void k_uconv(ConvArgs a) {
     for (int nt = 0; nt < NT; ++nt) wv[nt] = *reinterpret_cast<const bf16x8*>(wrow + (size_t)nt * 16 * KP);
     const bf16x8 zero8 = __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u));
     const int y = y0 + wave;
-#pragma unroll 1
-    for (int t = 0; t < 9; ++t) {
-        const int dy = t / 3 - 1, dx = t % 3 - 1;
+#pragma unroll 2
+    for (int step = 0; step < STEPS; ++step) {
+        // the next step's weights, fetched while this step's MFMAs run
+        bf16x8 wn[NT];
+        if (step + 1 < STEPS) {
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                wn[nt] = *reinterpret_cast<const bf16x8*>(wrow + (size_t)nt * 16 * KP + (step + 1) * 32);
+        }
+        // this lane's 8 K values: channels ch .. ch + 7 of tap t
+        const int k0 = step * 32 + kq * 8;
+        const bool kin = k0 < 9 * CIN;
+        const int t = kin ? k0 / CIN : 0;
+        const int ch = k0 - t * CIN;
+        const int dy = t / 3 - 1, dx = t - (t / 3) * 3 - 1;
         // this lane's pixel in tile m is (x0 + 16 m + r16, y); the tap reads
         // fine slot (wave + dy + 1, 16 m + r16 + dx + 1)
         const uint16_t* xa = sa + ((wave + dy + 1) * kCvRun + r16 + dx + 1) * RA;
+        bf16x8 xv[MT];
 #pragma unroll
-        for (int c = 0; c < KT; ++c) {
-            const int step = t * KT + c;
-            // the next step's weights, fetched while this step's MFMAs run
-            bf16x8 wn[NT];
-            if (step + 1 < STEPS) {
-                const int tn = (step + 1) / KT, cn = (step + 1) - tn * KT;
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    wn[nt] = *reinterpret_cast<const bf16x8*>(wrow + (size_t)nt * 16 * KP + tn * TS + cn * 32);
+        for (int m = 0; m < MT; ++m) {
+            if (C1 == 0) {
+                xv[m] = *reinterpret_cast<const bf16x8*>(xa + m * 16 * RA + ch);
+            } else {
+                const int xs = x0 + m * 16 + r16 + dx, ys = y + dy;
+                const bool in = xs >= 0 && xs < w && ys >= 0 && ys < h;
+                const uint16_t* pb = sb + (((ys >> 1) - py0) * kCvCoarseRun + ((xs >> 1) - px0)) * RB;
+                const uint16_t* src = ch < C0 ? xa + m * 16 * RA + ch : pb + (ch - C0);
+                xv[m] = *reinterpret_cast<const bf16x8*>(src);
+                if (ch >= C0 && !in) xv[m] = zero8;
             }
-            const int ch = c * 32 + kq * 8;    // this lane's first channel
-            bf16x8 xv[MT];
+            if (!kin) xv[m] = zero8;
+        }
 #pragma unroll
-            for (int m = 0; m < MT; ++m) {
-                if (C1 == 0 || c * 32 + 32 <= C0) {
-                    xv[m] = *reinterpret_cast<const bf16x8*>(xa + m * 16 * RA + (ch < CIN ? ch : 0));
-                    if (c * 32 + 32 > CIN && ch >= CIN) xv[m] = zero8;
-                } else {
-                    const int xs = x0 + m * 16 + r16 + dx, ys = y + dy;
-                    const bool in = xs >= 0 && xs < w && ys >= 0 && ys < h;
-                    const uint16_t* pb = sb + (((ys >> 1) - py0) * kCvCoarseRun + ((xs >> 1) - px0)) * RB;
-                    const uint16_t* src = ch < C0 ? xa + m * 16 * RA + ch : pb + (ch < CIN ? ch - C0 : 0);
-                    xv[m] = *reinterpret_cast<const bf16x8*>(src);
-                    if (ch >= CIN || (ch >= C0 && !in)) xv[m] = zero8;
-                }
-            }
+        for (int m = 0; m < MT; ++m)
 #pragma unroll
-            for (int m = 0; m < MT; ++m)
+            for (int nt = 0; nt < NT; ++nt)
+                acc[m][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv[nt], xv[m], acc[m][nt], 0, 0, 0);
+        if (step + 1 < STEPS) {
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    acc[m][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv[nt], xv[m], acc[m][nt], 0, 0, 0);
-            if (step + 1 < STEPS) {
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) wv[nt] = wn[nt];
-            }
+            for (int nt = 0; nt < NT; ++nt) wv[nt] = wn[nt];
         }
     }
     // D[i][j]: i = output channel 4 kq + e of the tile, j = pixel r16
@@ -765,16 +763,14 @@ __global__ void k_pack_conv(const float* W, int cin, int cout, int kpf, int kpb,
     uint16_t* bwd, float* bias) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < cout * kpf) {
-        const int ts = kpf / 9;
         const int co = i / kpf, k = i - co * kpf;
-        const int t = k / ts, ci = k - t * ts;
-        fwd[i] = ci < cin ? to_bf16(W[((size_t)co * 9 + t) * cin + ci]) : (uint16_t)0;
+        const int t = k / cin, ci = k - t * cin;
+        fwd[i] = k < 9 * cin ? to_bf16(W[((size_t)co * 9 + t) * cin + ci]) : (uint16_t)0;
     }
     if (bwd && i < cin * kpb) {
-        const int ts = kpb / 9;
         const int ci = i / kpb, k = i - ci * kpb;
-        const int tp = k / ts, co = k - tp * ts;
-        bwd[i] = co < cout ? to_bf16(W[((size_t)co * 9 + (8 - tp)) * cin + ci]) : (uint16_t)0;
+        const int tp = k / cout, co = k - tp * cout;
+        bwd[i] = k < 9 * cout ? to_bf16(W[((size_t)co * 9 + (8 - tp)) * cin + ci]) : (uint16_t)0;
     }
     if (i < cout) bias[i] = W[(size_t)cout * 9 * cin + i];
 }
