@@ -51,6 +51,7 @@ __host__ __device__ constexpr int cache_mlp_params() {
 struct CacheGridView {
     const float* table;   // [L][T][2]
     float* grad;          // same layout (training only)
+    uint8_t* touched;     // [L][T]: 1 where a sample of this step wrote a gradient (training only)
     int32_t levels;
     uint32_t tmask;       // T - 1
     int32_t log2_table;
@@ -121,10 +122,14 @@ hipError_t cache_train_pass(const CacheGridView& g, const CachePacked& pk,
     const CacheTrainScratch& w, const float4* in, int in_stride, const float4* target,
     int t_stride, const uint32_t* count, int cap, int batch, int batches, float* grad, float* loss,
     int grid, hipStream_t st);
-// Adam over n parameters with the step number state->steps + 1 (device
-// memory), then state->steps += 1; nothing when `count` (optional) is zero
-hipError_t cache_adam(int64_t n, float* params, const float* grad, float* m, float* v, float lr,
-    CacheState* state, const uint32_t* count, hipStream_t st);
+// Adam with the step number state->steps + 1 (device memory), then
+// state->steps += 1; nothing when `count` (optional) is zero.  The n_mlp
+// MLP parameters are updated densely; the grid entries only where
+// g.touched is set (lazy, sparse Adam: the moments of untouched entries are
+// left as they are), and those entries' gradients and flags are cleared, so
+// the grid gradient is zero again for the next step.
+hipError_t cache_adam(int64_t n_mlp, float* params, const float* grad, float* m, float* v, float lr,
+    const CacheGridView& g, CacheState* state, const uint32_t* count, hipStream_t st);
 hipError_t cache_fill(float* p, size_t n, float value, hipStream_t st);
 // after a frame's training with *count > 0 records: EMA of the first
 // batch's loss, frames += 1, usable = frames >= warmup and

@@ -12,7 +12,8 @@
 // activations (bf16) for the backward pass, which runs the same layout with
 // the transposed weights; weight gradients are GEMMs over samples written as
 // per-block partial sums reduced in a fixed order; the grid gradient is
-// scattered with FP32 atomics.
+// scattered with FP32 atomics and the cells it reaches are flagged, so that
+// Adam visits only those (sparse, lazy Adam) and leaves the gradient zero.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -380,10 +381,12 @@ __global__ __launch_bounds__(256) void k_cache_backward(BwdArgs a) {
                         grid_corners(a.g, lv, p[0], p[1], p[2], idx, w);
                         float* gt = a.g.grad + ((size_t)lv << a.g.log2_table) * 2;
                         const float d0 = acc[2 * half], d1 = acc[2 * half + 1];
+                        uint8_t* tl = a.g.touched + ((size_t)lv << a.g.log2_table);
 #pragma unroll
                         for (int k = 0; k < 8; ++k) {
                             atomicAdd(gt + (size_t)idx[k] * 2, w[k] * d0);
                             atomicAdd(gt + (size_t)idx[k] * 2 + 1, w[k] * d1);
+                            tl[idx[k]] = 1;
                         }
                     }
                 }
@@ -485,6 +488,35 @@ __global__ void k_cache_adam(int64_t n, float* params, const float* grad, float*
     m[i] = mi;
     v[i] = vi;
     params[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+}
+
+// sparse Adam over the grid: 4 cells per thread, only the touched ones
+// (their 2 features), whose gradients and flags are cleared afterwards
+__global__ void k_cache_adam_grid(int64_t cells, float* params, float* grad, float* m, float* v,
+    uint8_t* touched, float lr, const CacheState* state, const uint32_t* count) {
+    const int64_t c0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    if (c0 >= cells || (count && *count == 0u)) return;
+    const uint32_t flags = *reinterpret_cast<const uint32_t*>(touched + c0);
+    if (flags == 0u) return;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float t = (float)(state->steps + 1);
+    const float c1 = 1.0f - powf(b1, t), c2 = 1.0f - powf(b2, t);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (((flags >> (8 * k)) & 0xFFu) == 0u) continue;
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            const int64_t i = (c0 + k) * 2 + f;
+            const float gi = grad[i];
+            const float mi = b1 * m[i] + (1.0f - b1) * gi;
+            const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+            m[i] = mi;
+            v[i] = vi;
+            params[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+            grad[i] = 0.0f;
+        }
+    }
+    *reinterpret_cast<uint32_t*>(touched + c0) = 0u;
 }
 
 __global__ void k_cache_step(CacheState* state, const uint32_t* count) {
@@ -628,10 +660,15 @@ hipError_t cache_train_pass(const CacheGridView& g, const CachePacked& pk,
     return hipGetLastError();
 }
 
-hipError_t cache_adam(int64_t n, float* params, const float* grad, float* m, float* v, float lr,
-    CacheState* state, const uint32_t* count, hipStream_t st) {
-    if (n <= 0) return hipSuccess;
-    k_cache_adam<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, params, grad, m, v, lr, state, count);
+hipError_t cache_adam(int64_t n_mlp, float* params, const float* grad, float* m, float* v, float lr,
+    const CacheGridView& g, CacheState* state, const uint32_t* count, hipStream_t st) {
+    if (n_mlp > 0)
+        k_cache_adam<<<(unsigned)((n_mlp + 255) / 256), 256, 0, st>>>(n_mlp, params, grad, m, v, lr,
+            state, count);
+    const int64_t cells = (int64_t)g.levels << g.log2_table;   // a multiple of 4 (log2_table >= 8)
+    if (cells > 0)
+        k_cache_adam_grid<<<(unsigned)((cells / 4 + 255) / 256), 256, 0, st>>>(cells, params + n_mlp,
+            g.grad, m + n_mlp, v + n_mlp, g.touched, lr, state, count);
     k_cache_step<<<1, 64, 0, st>>>(state, count);
     return hipGetLastError();
 }

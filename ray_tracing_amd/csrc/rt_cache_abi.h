@@ -42,6 +42,7 @@ struct rt_cache {
     rt_cache_config cfg{};
     int64_t n_mlp = 0, n_grid = 0, n_total = 0;
     CacheBuf<float> params, grad, m, v;    // MLP then grid
+    CacheBuf<uint8_t> touched;             // grid cells with a gradient in the current step
     CacheBuf<uint16_t> fwd[rtk::kCacheHiddenLayers + 1], bwd[rtk::kCacheHiddenLayers + 1];
     CacheBuf<rtk::CacheState> state;
     CacheBuf<float> loss;                  // one per training batch of a frame
@@ -66,6 +67,7 @@ struct rt_cache {
         rtk::CacheGridView g{};
         g.table = params.p + n_mlp;
         g.grad = grad.p + n_mlp;
+        g.touched = touched.p;
         g.levels = cfg.levels;
         g.tmask = (1u << cfg.log2_table) - 1u;
         g.log2_table = cfg.log2_table;

@@ -106,6 +106,7 @@ rt_status reset_training(rt_cache* c) {
     CACHE_HIP(hipMemsetAsync(c->m.p, 0, bytes, c->stream));
     CACHE_HIP(hipMemsetAsync(c->v.p, 0, bytes, c->stream));
     CACHE_HIP(hipMemsetAsync(c->grad.p, 0, bytes, c->stream));
+    CACHE_HIP(hipMemsetAsync(c->touched.p, 0, (size_t)(c->n_grid / 2), c->stream));
     CACHE_HIP(hipMemsetAsync(c->state.p, 0, sizeof(rtk::CacheState), c->stream));
     c->fraction_set = false;  // the renderer seeds the training share on its next frame
     CACHE_HIP(rtk::cache_pack(c->params.p, c->packed(), c->stream));
@@ -135,14 +136,14 @@ hipError_t rt_cache_train_frame(rt_cache* c, const rtk::CacheRecord* records,
     const float4* tgt = &records[0].target;
     constexpr int stride = (int)(sizeof(rtk::CacheRecord) / sizeof(float4));
     hipError_t e = hipSuccess;
+    // the grid gradient is zero on entry to each step: the sparse Adam of the
+    // previous step cleared the cells it touched
     for (int k = 0; k < steps && e == hipSuccess; ++k) {
-        e = rtk::cache_fill(c->grad.p + c->n_mlp, (size_t)c->n_grid, 0.0f, st);
+        e = rtk::cache_train_pass(g, pk, w, in, stride, tgt, stride, count, cap, k, steps,
+            c->grad.p, c->loss.p + k, c->grid(), st);
         if (e == hipSuccess)
-            e = rtk::cache_train_pass(g, pk, w, in, stride, tgt, stride, count, cap, k, steps,
-                c->grad.p, c->loss.p + k, c->grid(), st);
-        if (e == hipSuccess)
-            e = rtk::cache_adam(c->n_total, c->params.p, c->grad.p, c->m.p, c->v.p,
-                c->cfg.learning_rate, c->state.p, count, st);
+            e = rtk::cache_adam(c->n_mlp, c->params.p, c->grad.p, c->m.p, c->v.p,
+                c->cfg.learning_rate, g, c->state.p, count, st);
         if (e == hipSuccess) e = rtk::cache_pack(c->params.p, pk, st);
     }
     if (e == hipSuccess)
@@ -198,6 +199,7 @@ rt_status rt_cache_create(int ordinal, const rt_cache_config* cfg, rt_cache** ou
         c->cu_count = prop.multiProcessorCount;
         e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     }
+    if (e == hipSuccess) e = c->touched.reserve((size_t)(c->n_grid / 2));
     for (CacheBuf<float>* b : {&c->params, &c->grad, &c->m, &c->v}) {
         if (e == hipSuccess) e = b->reserve((size_t)c->n_total);
     }
@@ -327,8 +329,8 @@ rt_status rt_cache_train_step_host(rt_cache* c, const float* inputs, const float
         CACHE_HIP(hipMemcpyAsync(grad, c->grad.p, sizeof(float) * (size_t)c->n_total,
             hipMemcpyDeviceToHost, c->stream));
     }
-    CACHE_HIP(rtk::cache_adam(c->n_total, c->params.p, c->grad.p, c->m.p, c->v.p,
-        c->cfg.learning_rate, c->state.p, c->stage_count.p, c->stream));
+    CACHE_HIP(rtk::cache_adam(c->n_mlp, c->params.p, c->grad.p, c->m.p, c->v.p,
+        c->cfg.learning_rate, c->grid_view(), c->state.p, c->stage_count.p, c->stream));
     CACHE_HIP(rtk::cache_pack(c->params.p, c->packed(), c->stream));
     float l = 0.0f;
     CACHE_HIP(hipMemcpyAsync(&l, c->loss.p, sizeof(float), hipMemcpyDeviceToHost, c->stream));
