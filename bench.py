@@ -177,14 +177,21 @@ def measure(rt, dev, info, config, steps, warmup, tune, autotune_first, per_came
 
     for i in range(warmup):
         ren.render(rig_for(i), prof, 1000 + i)
+    # device time per frame from timed frames (each waits for its frame)
+    frames = [ren.render(rig_for(i), prof, 2000 + i) for i in range(max(3, min(steps, 10)))]
+    log(f"[bench] {config} device ms per frame: {[round(f.render_ms, 3) for f in frames]}")
+    # the timed region: untimed asynchronous frames, each returning once
+    # enqueued, so the host stages frame k + 1 while frame k runs
+    ren.set_async(True)
     dev.synchronize()
-    frames = []
     t0 = time.perf_counter()
     for i in range(steps):
-        frames.append(ren.render(rig_for(warmup + i), prof, i))
-        log(f"[bench] {config} step {i} render_ms={frames[-1].render_ms:.3f}")
+        ren.render(rig_for(warmup + i), prof, i, timed=False)
+        log(f"[bench] {config} step {i} enqueued at {(time.perf_counter() - t0) * 1e3:.1f} ms")
+    ren.synchronize()
     dev.synchronize()
     wall = time.perf_counter() - t0
+    ren.set_async(False)
     ms_per_step = wall * 1e3 / steps
     value = paths_per_frame * steps / wall / 1e6
     # every traversal launch of the frame: one per bounce (extension rays with
@@ -194,7 +201,7 @@ def measure(rt, dev, info, config, steps, warmup, tune, autotune_first, per_came
     launches = stage.extend_launches + stage.extend_launches // max(prof.max_bounces, 1)
     avg_launch_ms = trace_ms / max(launches, 1)
     achieved = trace_bytes / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
-    gpu_ms = sum(f.render_ms for f in frames) / steps
+    gpu_ms = sorted(f.render_ms for f in frames)[len(frames) // 2]
 
     traffic, traffic_source = measured_traffic(config, ren.tuning(),
                                                (cnt.extend_rays + cnt.shadow_rays) / max(launches, 1))

@@ -472,6 +472,20 @@ rt_status rt_render_frame(rt_renderer* r, const rt_camera_rig* rig, const rt_pro
 rt_status rt_render_frame_streams(rt_renderer* r, const rt_camera_rig* rig,
     const rt_profile* profile, const uint32_t* sample_streams, const uint32_t* camera_keys,
     rt_frame_timing* timing);
+/* No counterpart in the reference (whose render_prepared_device waits for
+ * its launch): with async on, an rt_render_frame* call whose timing is NULL
+ * returns once the frame is enqueued on the device's stream, so the host
+ * prepares frame k + 1 while frame k runs.  The renderer's host staging is
+ * double-buffered: a call waits only for the frame two calls back.  Its
+ * outputs are complete in stream order (every download here waits for the
+ * stream; a device view is read on the device's stream or after
+ * rt_renderer_synchronize).  A call with timing always waits for its frame.
+ * The radiance cache's record / query counts of frames not yet waited on
+ * are folded in by rt_renderer_synchronize (or a later frame).  Off by
+ * default; turning it off waits for pending frames. */
+rt_status rt_renderer_set_async(rt_renderer* r, int32_t enable);
+/* waits for every frame of this renderer enqueued on the device's stream */
+rt_status rt_renderer_synchronize(rt_renderer* r);
 /* render_direction_debug_frame / OptixRenderer::render_direction_debug
  * (direction_debug_renderer.cpp:51-66, programs.cu:1807-1826): the world
  * direction of camera camera_index's ray through each pixel centre, encoded

@@ -28,7 +28,7 @@ EXPORTED = [
     "rt_device_synchronize", "rt_device_info", "rt_device_stream", "rt_scene_upload", "rt_scene_destroy",
     "rt_bvh_build", "rt_bvh_download", "rt_scene_update", "rt_scene_last_update", "rt_trace_closest", "rt_trace_any",
     "rt_trace_closest_device", "rt_trace_closest_device_noalloc", "rt_renderer_create", "rt_renderer_destroy", "rt_render_frame",
-    "rt_render_frame_streams", "rt_renderer_launch_diagnostics", "rt_analytic_lights_compile",
+    "rt_render_frame_streams", "rt_renderer_set_async", "rt_renderer_synchronize", "rt_renderer_launch_diagnostics", "rt_analytic_lights_compile",
     "rt_render_direction_debug",
     "rt_frame_download", "rt_frame_device_view", "rt_reset_accumulation", "rt_reset_sequence",
     "rt_restir_diagnostics", "rt_renderer_set_tuning", "rt_renderer_get_tuning",
@@ -244,6 +244,14 @@ class Renderer:
             ky = (C.c_uint32 * 4)(*keys) if keys is not None else None
             check(lib().rt_render_frame_streams(self.h, C.byref(rig), C.byref(profile), st, ky, out))
         return tm if timed else None
+
+    def set_async(self, enable: bool):
+        """Untimed frames return once enqueued (host staging double-buffered);
+        downloads still see complete frames."""
+        check(lib().rt_renderer_set_async(self.h, 1 if enable else 0))
+
+    def synchronize(self):
+        check(lib().rt_renderer_synchronize(self.h))
 
     def launch_diagnostics(self) -> dict:
         a, u = C.c_uint64(), C.c_uint64()

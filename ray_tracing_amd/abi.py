@@ -279,6 +279,8 @@ def declare(L):
     L.rt_render_frame_streams.argtypes = [vp, P(CameraRig), P(Profile), P(C.c_uint32),
                                           P(C.c_uint32), P(FrameTiming)]
     L.rt_renderer_launch_diagnostics.argtypes = [vp, P(C.c_uint64), P(C.c_uint64)]
+    L.rt_renderer_set_async.argtypes = [vp, C.c_int32]
+    L.rt_renderer_synchronize.argtypes = [vp]
     L.rt_analytic_lights_compile.argtypes = [vp, C.c_int32]
     L.rt_render_direction_debug.argtypes = [vp, P(CameraRig), C.c_int, vp]
     L.rt_frame_download.argtypes = [vp, C.c_int, vp, vp, vp, vp]

@@ -263,7 +263,7 @@ struct rt_renderer {
     uint32_t sequence_seed = 0;
     uint64_t parameter_uploads = 0;  // frames launched (one FrameDyn upload each)
     // rig
-    rt_camera* host_cams = nullptr;  // pinned
+    rt_camera* host_cams = nullptr;  // pinned, two frame slots of 8 cameras
     DevBuf<rt_camera> cams;
     int camera_count = 0;
     int pixel_offset[5] = {0, 0, 0, 0, 0};
@@ -291,7 +291,39 @@ struct rt_renderer {
     DevBuf<rtk::CacheRecord> cache_records;  // training records of a frame
     // per-frame device values and captured frame graphs
     DevBuf<rtk::FrameDyn> dyn;
-    rtk::FrameDyn* host_dyn = nullptr;  // pinned
+    rtk::FrameDyn* host_dyn = nullptr;  // pinned, two frame slots
+    // Host staging is double-buffered so that an asynchronous frame returns
+    // once enqueued: a frame writes slot `slot` and a call waits only for the
+    // frame two calls back (slot_ev) before it rewrites that slot.  The
+    // frame's statistics words land in host_stats[slot] and are folded into
+    // the radiance cache's record / query counts once the slot is waited on.
+    uint32_t* host_stats = nullptr;  // pinned, two slots of kStatWords
+    hipEvent_t slot_ev[2] = {nullptr, nullptr};
+    bool slot_pending[2] = {false, false};
+    rt_cache* slot_cache[2] = {nullptr, nullptr};
+    int slot = 0;
+    bool async_frames = false;
+
+    void absorb_stats(int k) {
+        rt_cache* c = slot_cache[k];
+        const uint32_t* w = host_stats + (size_t)k * rtk::kStatWords;
+        if (c != nullptr) {
+            c->last_records = (int32_t)std::min<uint32_t>(w[rtk::kStatCacheRecords], (uint32_t)c->cfg.train_samples);
+            c->last_queries = (int32_t)w[rtk::kStatCacheQueries];
+        }
+        slot_pending[k] = false;
+        slot_cache[k] = nullptr;
+    }
+    // waits for every frame enqueued on the stream and folds in their statistics
+    hipError_t drain() {
+        const hipError_t e = hipStreamSynchronize(dev->stream);
+        if (e != hipSuccess) return e;
+        for (int j = 1; j <= 2; ++j) {  // older slot first
+            const int k = (slot + j) & 1;
+            if (slot_pending[k]) absorb_stats(k);
+        }
+        return hipSuccess;
+    }
     struct Graph {
         GraphKey key;
         hipGraphExec_t exec = nullptr;
@@ -332,6 +364,9 @@ struct rt_renderer {
         for (Graph& g : graphs)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (host_dyn) (void)hipHostFree(host_dyn);
+        if (host_stats) (void)hipHostFree(host_stats);
+        for (hipEvent_t e : slot_ev)
+            if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         for (hipEvent_t e : post_ev)
             if (e) (void)hipEventDestroy(e);
@@ -1412,12 +1447,17 @@ rt_status rt_renderer_create(rt_device* dev, rt_scene* scene, rt_renderer** out)
     r->tuning.trace_blocks_per_cu = 8;
     r->tuning.trace_refill = 4;
     r->tuning.use_graph = 0;
-    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&r->host_cams), sizeof(rt_camera) * 8,
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&r->host_cams), sizeof(rt_camera) * 16,
         hipHostMallocDefault);
     if (e == hipSuccess) e = r->cams.reserve(8);
     if (e == hipSuccess) e = r->dyn.reserve(1);
     if (e == hipSuccess)
-        e = hipHostMalloc(reinterpret_cast<void**>(&r->host_dyn), sizeof(rtk::FrameDyn), hipHostMallocDefault);
+        e = hipHostMalloc(reinterpret_cast<void**>(&r->host_dyn), sizeof(rtk::FrameDyn) * 2, hipHostMallocDefault);
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&r->host_stats), sizeof(uint32_t) * rtk::kStatWords * 2,
+            hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&r->slot_ev[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&r->slot_ev[1], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreate(&r->post_ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&r->post_ev[1]);
     if (e == hipSuccess) e = r->counters.reserve(rtk::kCounterAlloc);
@@ -1615,10 +1655,16 @@ static rt_status render_frame_impl(rt_renderer* r, const rt_camera_rig* rig, con
             r->prev_valid[c] = 0;
         }
     }
-    RT_HIP(hipStreamSynchronize(st));  // host_cams / host_dyn may still feed a previous copy
-    std::memcpy(r->host_cams, rig->cameras, sizeof(rt_camera) * 4);
-    std::memcpy(r->host_cams + 4, r->prev_cams, sizeof(rt_camera) * 4);
-    RT_HIP(hipMemcpyAsync(r->cams.p, r->host_cams, sizeof(rt_camera) * 8, hipMemcpyHostToDevice, st));
+    // this slot's staging may still feed the copies of the frame two calls back
+    const int slot = r->slot;
+    if (r->slot_pending[slot]) {
+        RT_HIP(hipEventSynchronize(r->slot_ev[slot]));
+        r->absorb_stats(slot);
+    }
+    rt_camera* host_cams = r->host_cams + 8 * slot;
+    std::memcpy(host_cams, rig->cameras, sizeof(rt_camera) * 4);
+    std::memcpy(host_cams + 4, r->prev_cams, sizeof(rt_camera) * 4);
+    RT_HIP(hipMemcpyAsync(r->cams.p, host_cams, sizeof(rt_camera) * 8, hipMemcpyHostToDevice, st));
 
     const int spp = p->samples_per_pixel > 0 ? p->samples_per_pixel : 1;
     const int max_b = p->max_bounces > 0 ? p->max_bounces : 1;
@@ -1644,7 +1690,7 @@ static rt_status render_frame_impl(rt_renderer* r, const rt_camera_rig* rig, con
     }
 
     // per-frame values, read by the kernels from device memory
-    rtk::FrameDyn* dyn = r->host_dyn;
+    rtk::FrameDyn* dyn = r->host_dyn + slot;
     for (int c = 0; c < 4; ++c) {
         dyn->stream[c] = streams[c];
         dyn->camera_key[c] = keys[c];
@@ -1885,13 +1931,17 @@ static rt_status render_frame_impl(rt_renderer* r, const rt_camera_rig* rig, con
         RT_HIP(enqueue(stage_timing));
     }
     if (timed) RT_HIP(hipEventRecord(r->events[1], st));
-    uint32_t stats[rtk::kStatWords];
-    RT_HIP(hipMemcpyAsync(stats, b.counters, sizeof(stats), hipMemcpyDeviceToHost, st));
-    RT_HIP(hipStreamSynchronize(st));
-    if (cache != nullptr) {
-        cache->last_records = (int32_t)std::min<uint32_t>(stats[rtk::kStatCacheRecords],
-            (uint32_t)cache->cfg.train_samples);
-        cache->last_queries = (int32_t)stats[rtk::kStatCacheQueries];
+    uint32_t* stats = r->host_stats + (size_t)slot * rtk::kStatWords;
+    RT_HIP(hipMemcpyAsync(stats, b.counters, sizeof(uint32_t) * rtk::kStatWords, hipMemcpyDeviceToHost, st));
+    RT_HIP(hipEventRecord(r->slot_ev[slot], st));
+    r->slot_pending[slot] = true;
+    r->slot_cache[slot] = cache;
+    r->slot = slot ^ 1;
+    if (timed || !r->async_frames) {
+        // timing reads the frame's events and counters; a synchronous frame
+        // returns with its outputs complete
+        RT_HIP(hipEventSynchronize(r->slot_ev[slot]));
+        r->absorb_stats(slot);
     }
     r->rendered = true;
     // capture_launch_history (radiance_launch_setup.cpp:122-136)
@@ -1998,6 +2048,21 @@ rt_status rt_render_frame_streams(rt_renderer* r, const rt_camera_rig* rig, cons
         }
     }
     return render_frame_impl(r, rig, p, sample_streams, keys, timing);
+}
+
+rt_status rt_renderer_set_async(rt_renderer* r, int32_t enable) {
+    RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
+    RT_HIP(hipSetDevice(r->dev->ordinal));
+    if (!enable) RT_HIP(r->drain());
+    r->async_frames = enable != 0;
+    return RT_OK;
+}
+
+rt_status rt_renderer_synchronize(rt_renderer* r) {
+    RT_REQUIRE(r != nullptr, RT_ERROR_INVALID_ARGUMENT, "null renderer");
+    RT_HIP(hipSetDevice(r->dev->ordinal));
+    RT_HIP(r->drain());
+    return RT_OK;
 }
 
 // LaunchParameterDiagnostics (optix_renderer.cpp:130-138,217-219,257): the
@@ -2108,7 +2173,7 @@ rt_status rt_renderer_set_radiance_cache(rt_renderer* r, rt_cache* cache) {
         rt_cache_retain(cache);
     }
     if (r->cache) {
-        (void)hipStreamSynchronize(r->dev->stream);
+        RT_HIP(r->drain());
         rt_cache_release(r->cache);
     }
     r->cache = cache;
