@@ -1181,7 +1181,7 @@ static rt_status bvh_work(rt_scene* s, bool refit, rt_bvh_stats* stats) {
         RT_HIP(index.reserve((size_t)nbin));
         RT_HIP(f0.reserve((size_t)nbin));
         RT_HIP(f1.reserve((size_t)nbin));
-        RT_HIP(fc.reserve(2));
+        RT_HIP(fc.reserve(3));
         RT_HIP(s->nodes4.reserve((size_t)nbin * 4));
         rtk::WideScratch ws{};
         ws.is_wide = is_wide.p;

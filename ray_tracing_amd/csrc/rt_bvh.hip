@@ -551,16 +551,22 @@ __device__ void expand_wide(const float4* nodes, uint32_t b, Wide& w) {
 
 // one level of the top-down walk: every frontier node marks its internal
 // children as 4-wide nodes and appends them to the next frontier
+// one level of the top-down collapse; the frontier size comes from device
+// memory (grid-stride over it), so levels are enqueued without the host
+// reading each size, and a launch over an empty frontier returns at once.
+// *levels counts the non-empty frontiers.
 __global__ void k_wide_level(const float4* nodes, const uint32_t* front_in, const uint32_t* n_in,
-    uint32_t* front_out, uint32_t* n_out, uint32_t* is_wide) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *n_in) return;
-    Wide w;
-    expand_wide(nodes, front_in[i], w);
-    for (int k = 0; k < w.count; ++k) {
-        if (subtree_leaves(nodes, w.ref[k]) <= kLeafMax) continue;
-        is_wide[w.ref[k]] = 1u;
-        front_out[atomicAdd(n_out, 1u)] = w.ref[k];
+    uint32_t* front_out, uint32_t* n_out, uint32_t* is_wide, uint32_t* levels) {
+    const uint32_t n = *n_in;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0u) *levels += 1u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        Wide w;
+        expand_wide(nodes, front_in[i], w);
+        for (int k = 0; k < w.count; ++k) {
+            if (subtree_leaves(nodes, w.ref[k]) <= kLeafMax) continue;
+            is_wide[w.ref[k]] = 1u;
+            front_out[atomicAdd(n_out, 1u)] = w.ref[k];
+        }
     }
 }
 
@@ -569,6 +575,7 @@ __global__ void k_wide_root(uint32_t* front, uint32_t* counts, uint32_t* is_wide
     front[0] = 0u;
     counts[0] = 1u;
     counts[1] = 0u;
+    counts[2] = 0u;
     is_wide[0] = 1u;
 }
 
@@ -645,20 +652,26 @@ hipError_t bvh_collapse_wide(const float4* nodes, int nbin, WideScratch& w, floa
     hipError_t e = hipMemsetAsync(w.is_wide, 0, sizeof(uint32_t) * (size_t)nbin, st);
     if (e != hipSuccess) return e;
     k_wide_root<<<1, 64, 0, st>>>(w.front[0], w.counts, w.is_wide);
-    int cur = 0, levels = 0;
-    uint32_t host[2] = {1u, 0u};
+    // levels go out in batches with one host check per batch: a frontier
+    // holds at most nbin nodes, and the grid-stride launch covers any size
+    constexpr int kLevelBatch = 8;
+    const int grid = blocks_for(nbin, 256) < 2048 ? blocks_for(nbin, 256) : 2048;
+    int cur = 0;
+    uint32_t host[3] = {1u, 0u, 0u};
     while (true) {
-        ++levels;
-        e = hipMemsetAsync(w.counts + 1 - cur, 0, sizeof(uint32_t), st);
-        if (e != hipSuccess) return e;
-        k_wide_level<<<blocks_for((int)host[cur], 256), 256, 0, st>>>(nodes, w.front[cur],
-            w.counts + cur, w.front[cur ^ 1], w.counts + (cur ^ 1), w.is_wide);
+        for (int j = 0; j < kLevelBatch; ++j) {
+            e = hipMemsetAsync(w.counts + (cur ^ 1), 0, sizeof(uint32_t), st);
+            if (e != hipSuccess) return e;
+            k_wide_level<<<grid, 256, 0, st>>>(nodes, w.front[cur], w.counts + cur, w.front[cur ^ 1],
+                w.counts + (cur ^ 1), w.is_wide, w.counts + 2);
+            cur ^= 1;
+        }
         e = hipMemcpyAsync(host, w.counts, sizeof(host), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return e;
-        cur ^= 1;
         if (host[cur] == 0u) break;
     }
+    const int levels = (int)host[2];
     // compact numbering: exclusive scan of the marks
     const int tiles = scan_tiles(nbin);
     k_tile_sums<<<tiles, kScanBlock, 0, st>>>(w.is_wide, nbin, w.tiles);

@@ -30,7 +30,7 @@ struct WideScratch {
     uint32_t* is_wide;   // per binary node: roots a 4-wide node
     uint32_t* index;     // exclusive scan of is_wide: 4-wide node index
     uint32_t* front[2];  // top-down frontiers
-    uint32_t* counts;    // frontier sizes
+    uint32_t* counts;    // two frontier sizes, then the level count (3 words)
     uint32_t* tiles;     // scan_tiles(nbin) tile sums of the node numbering scan
 };
 constexpr int kScanTile = 4096;
