@@ -5,8 +5,16 @@
 //   render_realtime --scene cornell_mesh --profile scene --frames 20 --warmup-frames 3
 //                   --camera-count 4 --output-dir out --skip-image-write
 // Extensions: --denoiser-weights <params.npy> (learned denoiser), --graph 0|1,
-// --autotune.
+// --autotune.  Scenes: the built-in ids below, or a YAML scene file: --scene
+// <file.yaml>, --scene <id> with --scene-dir <dir> (or RT_SCENE_DIR) holding
+// <id>/scene.yaml or <id>.yaml, or --scene-bundle <file> (written by
+// `python -m ray_tracing_amd.scene_yaml <file.yaml> --bundle <file>`).  A YAML
+// file is flattened by that same ingest, run as a child process before the
+// device is touched, into a temporary bundle.
+#include <spawn.h>
 #include <sys/utsname.h>
+#include <sys/wait.h>
+#include <unistd.h>
 
 #include <chrono>
 #include <cmath>
@@ -44,7 +52,49 @@ struct Args {
     std::string denoiser_weights;
     int graph = -1;
     bool autotune = false;
+    std::string scene_dir;
+    std::string scene_bundle;
+    bool print_scene = false;  // describe the resolved scene and exit before the device is used
 };
+
+// the YAML scene file `--scene` names: a path to a .yaml / .yml file, or an
+// id found in the scene directory; empty when there is none
+std::filesystem::path scene_yaml_path(const Args& a) {
+    namespace fs = std::filesystem;
+    const fs::path direct(a.scene);
+    const std::string ext = direct.extension().string();
+    if ((ext == ".yaml" || ext == ".yml") && fs::is_regular_file(direct)) return direct;
+    std::string dir = a.scene_dir;
+    if (dir.empty()) {
+        const char* env = std::getenv("RT_SCENE_DIR");
+        if (env != nullptr) dir = env;
+    }
+    if (dir.empty() || a.scene.empty() || a.scene.find('/') != std::string::npos) return {};
+    for (const fs::path& p : {fs::path(dir) / a.scene / "scene.yaml", fs::path(dir) / (a.scene + ".yaml")})
+        if (fs::is_regular_file(p)) return p;
+    return {};
+}
+
+// runs the package's YAML ingest (python3 -m ray_tracing_amd.scene_yaml) as a
+// child process that writes a scene bundle; the package root is the parent of
+// the directory that holds this executable
+void convert_yaml_scene(const std::filesystem::path& yaml, const std::filesystem::path& bundle) {
+    namespace fs = std::filesystem;
+    const fs::path root = fs::read_symlink("/proc/self/exe").parent_path().parent_path();
+    if (!fs::is_regular_file(root / "ray_tracing_amd" / "scene_yaml.py"))
+        throw std::runtime_error("the YAML scene ingest (ray_tracing_amd/scene_yaml.py) is not beside " + root.string());
+    const char* old = std::getenv("PYTHONPATH");
+    const std::string pp = root.string() + (old != nullptr && *old ? ":" + std::string(old) : "");
+    setenv("PYTHONPATH", pp.c_str(), 1);
+    const std::string ys = yaml.string(), bs = bundle.string();
+    const char* argv[] = {"python3", "-m", "ray_tracing_amd.scene_yaml", ys.c_str(), "--bundle", bs.c_str(), nullptr};
+    pid_t pid = 0;
+    const int rc = posix_spawnp(&pid, "python3", nullptr, nullptr, const_cast<char* const*>(argv), environ);
+    if (rc != 0) throw std::runtime_error("cannot start python3 for the YAML scene ingest: " + std::string(std::strerror(rc)));
+    int status = 0;
+    if (waitpid(pid, &status, 0) != pid || !WIFEXITED(status) || WEXITSTATUS(status) != 0)
+        throw std::runtime_error("the YAML scene ingest failed on " + ys);
+}
 
 int scene_config(const std::string& name) {
     if (name == "random_spheres") return 0;
@@ -60,7 +110,7 @@ bool parse(int argc, char** argv, Args& a) {
     std::map<std::string, std::string> kv;
     for (int i = 1; i < argc; ++i) {
         const std::string k = argv[i];
-        if (k == "--skip-image-write" || k == "--autotune") {
+        if (k == "--skip-image-write" || k == "--autotune" || k == "--print-scene") {
             kv[k] = "1";
             continue;
         }
@@ -78,11 +128,14 @@ bool parse(int argc, char** argv, Args& a) {
             else if (k == "--seed") a.seed = (unsigned)std::stoul(v);
             else if (k == "--output-dir") a.output_dir = v;
             else if (k == "--scene") a.scene = v;
+            else if (k == "--scene-dir") a.scene_dir = v;
+            else if (k == "--scene-bundle") a.scene_bundle = v;
             else if (k == "--profile") a.profile = v;
             else if (k == "--skip-image-write") a.skip_image_write = true;
             else if (k == "--denoiser-weights") a.denoiser_weights = v;
             else if (k == "--graph") a.graph = std::stoi(v);
             else if (k == "--autotune") a.autotune = true;
+            else if (k == "--print-scene") a.print_scene = true;
             else {
                 std::fprintf(stderr, "unknown argument: %s\n", k.c_str());
                 return false;
@@ -228,13 +281,63 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "--seed + --warmup-frames + --frames exceeds uint32 sample streams\n");
         return EXIT_FAILURE;
     }
-    const int config = scene_config(a.scene);
-    if (config < 0) {
-        std::fprintf(stderr, "--scene must reference a registered realtime scene\n");
-        return EXIT_FAILURE;
-    }
+    const int config = a.scene_bundle.empty() ? scene_config(a.scene) : -1;
     rt_generated_scene* gen = nullptr;
-    rt_amd::throw_on_error(rt_scene_generate(config, 0, &gen));
+    rt_amd::SceneBundle bundle;
+    if (config >= 0) {
+        rt_amd::throw_on_error(rt_scene_generate(config, 0, &gen));
+    } else {
+        std::filesystem::path bundle_path = a.scene_bundle;
+        bool temporary = false;
+        if (bundle_path.empty()) {
+            const std::filesystem::path yaml = scene_yaml_path(a);
+            if (yaml.empty()) {
+                std::fprintf(stderr, "--scene must reference a registered realtime scene\n");
+                return EXIT_FAILURE;
+            }
+            bundle_path = std::filesystem::temp_directory_path()
+                / ("rt_scene_" + std::to_string((long)getpid()) + ".rtsb");
+            temporary = true;
+            try {
+                convert_yaml_scene(yaml, bundle_path);
+            } catch (const std::exception& e) {
+                std::fprintf(stderr, "%s\n", e.what());
+                return EXIT_FAILURE;
+            }
+        }
+        try {
+            bundle = rt_amd::load_scene_bundle(bundle_path);
+        } catch (const std::exception& e) {
+            if (temporary) std::filesystem::remove(bundle_path);
+            std::fprintf(stderr, "%s\n", e.what());
+            return EXIT_FAILURE;
+        }
+        if (temporary) std::filesystem::remove(bundle_path);
+        if (!bundle.name.empty()) a.scene = bundle.name;  // the scene id, not the file path
+        // the generated-scene shape the rest of the run reads
+        gen = new rt_generated_scene{};
+        gen->scene = bundle.desc;
+        gen->camera_count = bundle.rig.active_count;
+        for (int c = 0; c < bundle.rig.active_count; ++c) gen->cameras[c] = bundle.rig.cameras[c];
+        gen->profile = bundle.profile;
+    }
+    auto free_scene = [&]() {
+        if (config >= 0) rt_scene_generated_free(gen);
+        else delete gen;
+    };
+    if (a.print_scene) {
+        const rt_scene_desc& d = gen->scene;
+        std::printf("scene %s: %d spheres, %d quads, %d triangles, %d materials, %d openpbr, "
+                    "%d analytic lights, %d textures, %d texels, %d media, %d cameras",
+            a.scene.c_str(), d.sphere_count, d.quad_count, d.triangle_count, d.material_count,
+            d.openpbr_count, d.analytic_light_count, d.texture_count, d.image_texel_count, d.medium_count,
+            gen->camera_count);
+        for (int c = 0; c < gen->camera_count; ++c)
+            std::printf("%s%dx%d", c ? ", " : " ", gen->cameras[c].width, gen->cameras[c].height);
+        std::printf("; spp %d, depth %d\n", gen->profile.samples_per_pixel, gen->profile.max_bounces);
+        free_scene();
+        return EXIT_SUCCESS;
+    }
     rt_amd::RenderProfile profile = rt_amd::RenderProfile::realtime_default();
     std::string profile_name = rt_amd::render_profile_name(profile);
     rt_profile abi_profile{};
@@ -447,9 +550,9 @@ int main(int argc, char** argv) {
         }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "%s\n", e.what());
-        rt_scene_generated_free(gen);
+        free_scene();
         return EXIT_FAILURE;
     }
-    rt_scene_generated_free(gen);
+    free_scene();
     return EXIT_SUCCESS;
 }

@@ -503,6 +503,120 @@ RendererPoolDiagnostics RendererPool::diagnostics() const {
 }
 
 // ---------------------------------------------------------------- profiling
+namespace {
+
+enum BundleTag : std::uint32_t {
+    kBundleBackground = 1, kBundleSpheres, kBundleQuads, kBundleTriangles, kBundleMaterials,
+    kBundleOpenpbr, kBundleAnalytic, kBundleTextures, kBundleTexels, kBundleAlbedoTex,
+    kBundleTriUvs, kBundleTriNormals, kBundleMedia, kBundleEmissionTex, kBundlePbrTex,
+    kBundleCameras, kBundleProfile, kBundleName
+};
+
+}  // namespace
+
+SceneBundle load_scene_bundle(const std::filesystem::path& path) {
+    const std::string where = "scene bundle " + path.string();
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error(where + ": cannot open");
+    std::vector<unsigned char> data((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    auto rd32 = [&](size_t off) {
+        std::uint32_t v;
+        std::memcpy(&v, data.data() + off, 4);
+        return v;
+    };
+    if (data.size() < 12 || std::memcmp(data.data(), "RTSB", 4) != 0)
+        throw std::runtime_error(where + ": not a scene bundle");
+    if (rd32(4) != 1u) throw std::runtime_error(where + ": unsupported version " + std::to_string(rd32(4)));
+    const std::uint32_t n = rd32(8);
+    SceneBundle b;
+    struct Sec {
+        std::uint32_t size = 0;
+        std::uint64_t count = 0;
+        const unsigned char* bytes = nullptr;
+    };
+    Sec sec[kBundleName + 1];
+    size_t off = 12;
+    for (std::uint32_t i = 0; i < n; ++i) {
+        if (off + 16 > data.size()) throw std::runtime_error(where + ": truncated section header");
+        const std::uint32_t tag = rd32(off), size = rd32(off + 4);
+        std::uint64_t count;
+        std::memcpy(&count, data.data() + off + 8, 8);
+        off += 16;
+        if (size == 0 || count > (data.size() - off) / size)
+            throw std::runtime_error(where + ": truncated section " + std::to_string(tag));
+        if (tag < kBundleBackground || tag > kBundleName)
+            throw std::runtime_error(where + ": unknown section " + std::to_string(tag));
+        sec[tag] = Sec{size, count, data.data() + off};
+        off += (size_t)(size * count);
+    }
+    if (off != data.size()) throw std::runtime_error(where + ": trailing bytes");
+    // copies section `tag` into owned storage after checking its element size
+    auto take = [&](std::uint32_t tag, size_t elem, const char* what) -> const void* {
+        const Sec& s = sec[tag];
+        if (s.bytes == nullptr) return nullptr;
+        if (s.size != elem)
+            throw std::runtime_error(where + ": " + what + " elements are " + std::to_string(s.size)
+                + " bytes, this build's are " + std::to_string(elem));
+        b.storage.emplace_back(s.bytes, s.bytes + (size_t)s.size * s.count);
+        return b.storage.back().data();
+    };
+    auto count_of = [&](std::uint32_t tag) { return (int32_t)sec[tag].count; };
+    if (sec[kBundleBackground].bytes == nullptr || sec[kBundleBackground].size != 28)
+        throw std::runtime_error(where + ": missing background");
+    std::memcpy(&b.desc.background_mode, sec[kBundleBackground].bytes, 4);
+    std::memcpy(&b.desc.background, sec[kBundleBackground].bytes + 4, 12);
+    std::memcpy(&b.desc.background_top, sec[kBundleBackground].bytes + 16, 12);
+    rt_scene_desc& d = b.desc;
+    d.spheres = static_cast<const rt_sphere*>(take(kBundleSpheres, sizeof(rt_sphere), "sphere"));
+    d.sphere_count = d.spheres ? count_of(kBundleSpheres) : 0;
+    d.quads = static_cast<const rt_quad*>(take(kBundleQuads, sizeof(rt_quad), "quad"));
+    d.quad_count = d.quads ? count_of(kBundleQuads) : 0;
+    d.triangles = static_cast<const rt_triangle*>(take(kBundleTriangles, sizeof(rt_triangle), "triangle"));
+    d.triangle_count = d.triangles ? count_of(kBundleTriangles) : 0;
+    d.materials = static_cast<const rt_material*>(take(kBundleMaterials, sizeof(rt_material), "material"));
+    d.material_count = d.materials ? count_of(kBundleMaterials) : 0;
+    d.openpbr = static_cast<const rt_openpbr_material*>(
+        take(kBundleOpenpbr, sizeof(rt_openpbr_material), "OpenPBR material"));
+    d.openpbr_count = d.openpbr ? count_of(kBundleOpenpbr) : 0;
+    d.analytic_lights = static_cast<const rt_analytic_light*>(
+        take(kBundleAnalytic, sizeof(rt_analytic_light), "analytic light"));
+    d.analytic_light_count = d.analytic_lights ? count_of(kBundleAnalytic) : 0;
+    d.textures = static_cast<const rt_texture*>(take(kBundleTextures, sizeof(rt_texture), "texture"));
+    d.texture_count = d.textures ? count_of(kBundleTextures) : 0;
+    d.image_texels = static_cast<const float*>(take(kBundleTexels, 3 * sizeof(float), "texel"));
+    d.image_texel_count = d.image_texels ? count_of(kBundleTexels) : 0;
+    d.media = static_cast<const rt_medium*>(take(kBundleMedia, sizeof(rt_medium), "medium"));
+    d.medium_count = d.media ? count_of(kBundleMedia) : 0;
+    // per-material, per-triangle and per-record side arrays must match their owners' counts
+    auto side = [&](std::uint32_t tag, size_t elem, int32_t owner, const char* what) -> const void* {
+        if (sec[tag].bytes != nullptr && (int64_t)sec[tag].count != (int64_t)owner)
+            throw std::runtime_error(where + ": " + what + " has " + std::to_string(sec[tag].count)
+                + " entries for " + std::to_string(owner));
+        return take(tag, elem, what);
+    };
+    d.material_albedo_texture = static_cast<const int32_t*>(
+        side(kBundleAlbedoTex, 4, d.material_count, "material albedo textures"));
+    d.material_emission_texture = static_cast<const int32_t*>(
+        side(kBundleEmissionTex, 4, d.material_count, "material emission textures"));
+    d.triangle_uvs = static_cast<const float*>(side(kBundleTriUvs, 24, d.triangle_count, "triangle uvs"));
+    d.triangle_normals = static_cast<const float*>(
+        side(kBundleTriNormals, 36, d.triangle_count, "triangle normals"));
+    d.openpbr_textures = static_cast<const rt_openpbr_textures*>(
+        side(kBundlePbrTex, sizeof(rt_openpbr_textures), d.openpbr_count, "OpenPBR texture bindings"));
+    const Sec& cams = sec[kBundleCameras];
+    if (cams.bytes == nullptr || cams.size != sizeof(rt_camera) || cams.count < 1 || cams.count > 4)
+        throw std::runtime_error(where + ": needs 1-4 cameras of " + std::to_string(sizeof(rt_camera)) + " bytes");
+    b.rig.active_count = (int32_t)cams.count;
+    std::memcpy(b.rig.cameras, cams.bytes, sizeof(rt_camera) * cams.count);
+    const Sec& prof = sec[kBundleProfile];
+    if (prof.bytes == nullptr || prof.size != sizeof(rt_profile) || prof.count != 1)
+        throw std::runtime_error(where + ": needs one profile of " + std::to_string(sizeof(rt_profile)) + " bytes");
+    std::memcpy(&b.profile, prof.bytes, sizeof(rt_profile));
+    if (sec[kBundleName].bytes != nullptr)
+        b.name.assign(reinterpret_cast<const char*>(sec[kBundleName].bytes), (size_t)sec[kBundleName].count);
+    return b;
+}
+
 namespace profiling {
 namespace {
 

@@ -286,6 +286,22 @@ private:
     mutable std::mutex mutex_;
 };
 
+// A scene bundle written by ray_tracing_amd.scene_bundle (the package's
+// YAML ingest flattened into every array of an rt_scene_desc, the scene's
+// camera rig and a render profile).  Layout: magic "RTSB", u32 version 1,
+// u32 section count, then per section u32 tag, u32 element size, u64
+// element count and the elements; element sizes must equal this build's
+// struct sizes.  `desc` points into the bundle's own storage.
+struct SceneBundle {
+    std::string name;
+    rt_scene_desc desc{};
+    rt_camera_rig rig{};
+    rt_profile profile{};
+    std::vector<std::vector<unsigned char>> storage;
+};
+// throws std::runtime_error naming the file and the defect
+SceneBundle load_scene_bundle(const std::filesystem::path& path);
+
 namespace profiling {
 
 struct CameraStageSample {

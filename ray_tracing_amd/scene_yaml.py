@@ -34,6 +34,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+import sys
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -939,16 +940,24 @@ def main():
     ap.add_argument("--spp", type=int, default=16)
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--out", default="")
+    ap.add_argument("--bundle", default="",
+                    help="write the flattened scene, rig and profile to this scene bundle and exit "
+                         "(no device is used)")
     a = ap.parse_args()
     defn = load_scene_definition(a.scene)
     flat = flatten(defn)
     rig = camera_rig(defn, a.preset)
+    prof = rt.Generated("cornell_mesh").profile   # the plain tracer's settings
+    prof.samples_per_pixel, prof.max_bounces = a.spp, a.depth
+    if a.bundle:
+        from . import scene_bundle
+        scene_bundle.write(a.bundle, flat.desc, rig, prof, defn.id)
+        print(f"[scene] {defn.id}: bundle {a.bundle}", file=sys.stderr)
+        return
     dev = rt.Device(0)
     sc = rt.Scene(dev, flat.desc, keepalive=flat)
     sc.build_bvh()
     ren = rt.Renderer(dev, sc)
-    prof = rt.Generated("cornell_mesh").profile   # the plain tracer's settings
-    prof.samples_per_pixel, prof.max_bounces = a.spp, a.depth
     t0 = time.time()
     tm = ren.render(rig, prof, 0)
     img = ren.download(0)["beauty"]

@@ -139,3 +139,16 @@ def test_four_camera_quality_cli(rt, tmp_path):
 def test_final_room_quality_cli(rt, tmp_path):
     say("[gpu] render_realtime final_room, 4 cameras, quality profile")
     four_camera(rt, tmp_path / "fr", "final_room", "quality", 1, False, min_spread_milli=10)
+
+
+def test_cli_renders_yaml_scene(rt, tmp_path):
+    """render_realtime --scene <id> --scene-dir <dir>: a YAML scene file
+    flattened by the package's ingest and rendered through the pool."""
+    say("[gpu] render_realtime on the cornell_box YAML scene")
+    scenes = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes")
+    out = tmp_path / "yaml"
+    run_cli(rt, out, "--scene", "cornell_box", "--scene-dir", scenes, "--profile", "scene",
+            "--camera-count", "1", "--frames", "2", "--warmup-frames", "1")
+    summary = json.load(open(out / "benchmark_summary.json"))
+    assert summary["metadata"]["scene"] == "cornell_box"
+    assert len(pngs(out)) >= 1
