@@ -177,14 +177,15 @@ def measure(rt, dev, info, config, steps, warmup, tune, autotune_first, per_came
 
     for i in range(warmup):
         ren.render(rig_for(i), prof, 1000 + i)
-    # device time per frame from timed frames (each waits for its frame)
-    frames = [ren.render(rig_for(i), prof, 2000 + i) for i in range(max(3, min(steps, 10)))]
-    log(f"[bench] {config} device ms per frame: {[round(f.render_ms, 3) for f in frames]}")
-    # the timed region: untimed asynchronous frames, each returning once
-    # enqueued, so the host stages frame k + 1 while frame k runs
     # the orbit poses' rigs (equirect lookup tables included) are inputs,
     # built before the timed region like the scene
     rigs = [rig_for(warmup + i) for i in range(steps)]
+    # device time per frame from timed frames (each waits for its frame) of
+    # the same poses as the timed steps
+    frames = [ren.render(rigs[i % steps], prof, 2000 + i) for i in range(max(3, min(steps, 10)))]
+    log(f"[bench] {config} device ms per frame: {[round(f.render_ms, 3) for f in frames]}")
+    # the timed region: untimed asynchronous frames, each returning once
+    # enqueued, so the host stages frame k + 1 while frame k runs
     ren.set_async(True)
     dev.synchronize()
     t0 = time.perf_counter()
